@@ -1,0 +1,303 @@
+/*
+ * radiantium_hip.h — C ABI of the MI355X (gfx950) wavefront path tracer.
+ *
+ * This is the drop-in boundary below the reference's plugin surface.  The
+ * reference (C#, ksgfk/Radiantium) reaches its hot path through
+ *   MonteCarloIntegrator.Li(Ray3F, Scene, Random)      Radiantium.Offline/IIntegrator.cs:24-34
+ *   Renderer.{RenderTarget,Start,Wait,Stop,IsSuccess}  Radiantium.Offline/Renderer.cs:5-52
+ * and calls Li once per pixel sample from BlockBasedRenderer.RenderTask
+ * (Radiantium.Offline/Renderers/BlockBasedRenderer.cs:107-179).  A GPU
+ * replacement therefore sits at the Renderer level: a C# WavefrontRenderer
+ * calls rd_render() on a worker task and exposes the result as the
+ * reference's column-major ColorBuffer (Radiantium.Core/ColorBuffer.cs:25-42).
+ *
+ * Conventions
+ *  - Every call returns an rd_status; on failure rd_last_error() returns a
+ *    thread-local message (the reference throws ArgumentException from
+ *    RendererBuilder.Build, Config/RendererBuilder.cs:433-470, and turns
+ *    render-time exceptions into IsSuccess=false, BlockBasedRenderer.cs:161-166).
+ *  - Host buffers passed in are copied; the caller keeps ownership.  No
+ *    callbacks into the caller.  One context is used from one host thread
+ *    at a time (an internal mutex rejects concurrent use).
+ *  - Image output is float[W*H*3] addressed (x*H + y)*3, as ColorBuffer.
+ *  - Plain C types only: pointers, sizes, POD structs.
+ */
+#ifndef RADIANTIUM_HIP_H
+#define RADIANTIUM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RD_ABI_VERSION 1
+
+typedef enum rd_status {
+    RD_OK = 0,
+    RD_ERR_INVALID = 1,     /* bad argument or scene description            */
+    RD_ERR_DEVICE = 2,      /* HIP runtime error                            */
+    RD_ERR_OOM = 3,         /* device allocation failed                     */
+    RD_ERR_CANCELLED = 4,   /* *cancel became non-zero during rd_render     */
+    RD_ERR_STACK = 5,       /* traversal stack overflow (Bvh.StackSize=64)  */
+    RD_ERR_UNSUPPORTED = 6, /* feature not built for this path              */
+    RD_ERR_BUSY = 7         /* context used from two threads at once        */
+} rd_status;
+
+/* ---- scene description (mirrors RendererBuilder.Build's products) ------ */
+
+/* Material kinds: the registry names of Config/RendererBuilder.cs:86-213. */
+typedef enum rd_material_type {
+    RD_MAT_DIFFUSE = 0,        /* "diffuse"        Materials/Diffuse.cs        */
+    RD_MAT_ROUGH_METAL = 1,    /* "rough_metal"    Materials/RoughMetal.cs     */
+    RD_MAT_ROUGH_GLASS = 2,    /* "rough_glass"    Materials/RoughGlass.cs     */
+    RD_MAT_ROUGH_PLASTIC = 3,  /* "rough_plastic"  Materials/RoughPlastic.cs   */
+    RD_MAT_PERFECT_GLASS = 4,  /* "perfect_glass"  Materials/PerfectGlass.cs   */
+    RD_MAT_PERFECT_MIRROR = 5, /* "perfect_mirror" Materials/PerfectMirror.cs  */
+    RD_MAT_DISNEY = 6          /* "disney"         Materials/Disney.cs         */
+} rd_material_type;
+
+/* Texture slot meaning per material (index into rd_scene_desc.textures):
+ *   DIFFUSE        tex[0]=kd
+ *   ROUGH_METAL    tex[0]=roughness tex[1]=anisotropic          (eta,k)
+ *   ROUGH_GLASS    tex[0]=r tex[1]=t tex[2]=roughness tex[3]=anisotropic (eta_a,eta_b)
+ *   ROUGH_PLASTIC  tex[0]=r tex[1]=roughness tex[2]=anisotropic (kd,ks,eta_a,eta_b)
+ *   PERFECT_GLASS  tex[0]=r tex[1]=t                            (eta_a,eta_b)
+ *   PERFECT_MIRROR tex[0]=r
+ *   DISNEY         tex[0..14] = base_color metallic roughness eta specular
+ *                  specular_tint anisotropic sheen sheen_tint clearcoat
+ *                  clearcoat_gloss scatting_distance transmission
+ *                  transmission_roughness flatness                (thin)   */
+typedef struct rd_material_desc {
+    int32_t type;       /* rd_material_type                                 */
+    int32_t two_sided;  /* "is_two_side"                                    */
+    int32_t dist;       /* 0 = Beckmann, 1 = GGX (Bxdf/Microfacet.cs:9-13)  */
+    int32_t thin;       /* Disney "is_thin"                                 */
+    int32_t tex[16];
+    float eta_a, eta_b; /* dielectric IORs (defaults 1.000277 / 1.5046)     */
+    float kd, ks;       /* plastic lobe weights                             */
+    float eta[3];       /* conductor eta (default Cu, Spectrum.cs:64)       */
+    float k[3];         /* conductor k                                      */
+} rd_material_desc;
+
+/* Texture2D: constant (width==0) or image.  Images are 3-channel float,
+ * column-major (x*height + y)*3 like ColorBuffer; sampled by
+ * TextureSampler.Sample (TextureSampler.cs:28-72). */
+typedef struct rd_texture_desc {
+    int32_t width, height; /* 0,0 => constant texture                       */
+    int32_t wrap;          /* 0 = Clamp, 1 = Repeat                         */
+    int32_t filter;        /* 0 = Nearest, 1 = Linear                       */
+    float value[3];        /* constant colour (ConstColorTexture2D)         */
+    int32_t _pad;
+    const float* data;     /* width*height*3 floats, or NULL for constants  */
+} rd_texture_desc;
+
+typedef enum rd_light_type {
+    RD_LIGHT_AREA = 0,    /* "diffuse_area"  Lights/DiffuseAreaLight.cs     */
+    RD_LIGHT_INFINITE = 1 /* "infinite"      Lights/InfiniteAreaLight.cs    */
+} rd_light_type;
+
+typedef struct rd_light_desc {
+    int32_t type;       /* rd_light_type                                     */
+    int32_t prim;       /* AREA: primitive index of the emitting shape       */
+    float le[3];        /* AREA: emitted radiance "le"                       */
+    int32_t tex;        /* INFINITE: texture index of the radiance map       */
+    float to_world[16]; /* INFINITE: model-to-world, row-major M11..M44      */
+} rd_light_desc;
+
+/* Homogeneous medium (Mediums/Homogeneous.cs). */
+typedef struct rd_medium_desc {
+    float sigma_a[3];
+    float sigma_s[3];
+    float g;
+    float scale;
+} rd_medium_desc;
+
+typedef enum rd_prim_kind { RD_PRIM_TRIANGLE = 0, RD_PRIM_SPHERE = 1 } rd_prim_kind;
+
+/* One GeometricPrimitive (Primitive.cs:44-78), in the builder's order
+ * (entity order, then shapes, then model triangles; RendererBuilder.cs:940-999).
+ * The position of a primitive in this array is the "primitive index" that
+ * rd_trace_closest reports. */
+typedef struct rd_prim_desc {
+    int32_t kind;           /* rd_prim_kind                                   */
+    int32_t index;          /* triangle or sphere index                       */
+    int32_t material;       /* -1: no surface (a medium boundary)             */
+    int32_t light;          /* -1: not emissive                               */
+    int32_t medium_inside;  /* -1: vacuum                                     */
+    int32_t medium_outside; /* -1: vacuum                                     */
+} rd_prim_desc;
+
+typedef struct rd_camera_desc {
+    float fov, near_z, far_z;     /* degrees; defaults 60, 0.001, 1000        */
+    int32_t width, height;        /* "screen"                                 */
+    int32_t use_matrix;           /* 0: origin/target/up, 1: to_world         */
+    float origin[3], target[3], up[3];
+    float to_world[16];           /* row-major M11..M44 (after ReadMat4)      */
+} rd_camera_desc;
+
+typedef enum rd_integrator_type { RD_INTEGRATOR_PATH = 0, RD_INTEGRATOR_VOL_PATH = 1 } rd_integrator_type;
+typedef enum rd_path_method { RD_METHOD_BXDF = 0, RD_METHOD_NEE = 1, RD_METHOD_MIS = 2 } rd_path_method;
+typedef enum rd_light_strategy { RD_STRATEGY_UNIFORM = 0, RD_STRATEGY_ALL = 1 } rd_light_strategy;
+
+typedef struct rd_integrator_desc {
+    int32_t type;         /* rd_integrator_type ("path" / "vol_path")        */
+    int32_t max_depth;    /* default -1                                      */
+    int32_t min_depth;    /* default 3                                       */
+    float rr_threshold;   /* default 0.99                                    */
+    int32_t method;       /* rd_path_method, default Mis                     */
+    int32_t strategy;     /* rd_light_strategy, default Uniform              */
+} rd_integrator_desc;
+
+typedef struct rd_scene_desc {
+    int32_t num_vertices;
+    const float* positions;      /* [num_vertices][3], world space           */
+    const float* normals;        /* [num_vertices][3] or NULL                */
+    const float* uvs;            /* [num_vertices][2] or NULL                */
+    int32_t num_triangles;
+    const int32_t* indices;      /* [num_triangles][3]                       */
+    const uint8_t* tri_flags;    /* per triangle: bit0 has normal, bit1 has uv; NULL = all 0 */
+    int32_t num_spheres;
+    const float* spheres;        /* [num_spheres][4] center.xyz, radius      */
+    int32_t num_prims;
+    const rd_prim_desc* prims;
+    int32_t num_materials;
+    const rd_material_desc* materials;
+    int32_t num_textures;
+    const rd_texture_desc* textures;
+    int32_t num_lights;
+    const rd_light_desc* lights; /* Scene.Lights order (RendererBuilder.cs:945-993) */
+    int32_t num_media;
+    const rd_medium_desc* media;
+    int32_t global_medium;       /* -1: none                                 */
+    int32_t bvh_max_prim;        /* accel "max_prim", default 1              */
+    rd_camera_desc camera;
+    rd_integrator_desc integrator;
+} rd_scene_desc;
+
+/* ---- rays for the traversal oracle hooks -------------------------------- */
+
+/* Ray3F (Radiantium.Core/Ray3F.cs:5-20); InvD is recomputed as 1/d. */
+typedef struct rd_ray {
+    float o[3];
+    float tmin;
+    float d[3];
+    float tmax;
+} rd_ray;
+
+/* Closest hit: t, primitive index (rd_scene_desc.prims order, -1 on miss),
+ * barycentric u, v (SurfacePoint, Intersection.cs:6-18). */
+typedef struct rd_hit {
+    float t;
+    int32_t prim;
+    float u, v;
+} rd_hit;
+
+typedef struct rd_render_params {
+    int32_t spp;            /* renderer "spp", default 256                  */
+    uint32_t seed;          /* counter-based RNG seed                       */
+    int32_t x0, y0, x1, y1; /* pixel crop [x0,x1) x [y0,y1); all 0 = full   */
+    int32_t wave_paths;     /* paths in flight per wave; 0 = autotuned      */
+    int32_t flags;          /* RD_RENDER_* bits                              */
+} rd_render_params;
+
+#define RD_RENDER_NO_GRAPH 1 /* launch the bounce loop kernel by kernel     */
+
+typedef struct rd_render_stats {
+    uint64_t rays_closest;   /* closest-hit BVH queries                      */
+    uint64_t rays_any;       /* any-hit BVH queries (shadow rays)            */
+    uint64_t paths;          /* camera samples traced                        */
+    uint64_t vertices;       /* surface / medium vertices shaded             */
+    uint64_t invalid;        /* samples replaced by 0 (Color3F.IsValid)      */
+    double render_ms;        /* device time of the last rd_render            */
+    int32_t waves;
+    int32_t bounces;         /* deepest bounce processed                     */
+    /* traversal work per query class: [0] extension (closest), [1] MIS
+     * (closest), [2] shadow (any-hit): rays, nodes visited, primitives tested */
+    uint64_t class_rays[3];
+    uint64_t class_nodes[3];
+    uint64_t class_prims[3];
+} rd_render_stats;
+
+typedef struct rd_scene_info {
+    int32_t num_nodes;
+    int32_t num_leaves;
+    int32_t max_depth;      /* deepest leaf (root = 0)                        */
+    int32_t num_prims;
+    double build_ms;
+    uint64_t device_bytes;
+} rd_scene_info;
+
+typedef struct rd_ctx rd_ctx;
+typedef struct rd_scene rd_scene;
+
+/* ---- context ------------------------------------------------------------- */
+int rd_abi_version(void);
+const char* rd_last_error(void);
+/* Creates a context on HIP device `device`.  NULL on failure. */
+rd_ctx* rd_create(int device);
+void rd_destroy(rd_ctx* ctx);
+
+/* ---- scene: replaces RendererBuilder.Build + new Bvh(...) --------------- */
+/* Builds the SAH BVH (Accelerators/Bvh.cs:89-124) on the host and uploads the
+ * flattened scene to HBM.  NULL on failure (see rd_last_error). */
+rd_scene* rd_scene_create(rd_ctx* ctx, const rd_scene_desc* desc);
+void rd_scene_destroy(rd_scene* scene);
+int rd_scene_get_info(const rd_scene* scene, rd_scene_info* info);
+/* Copies the flattened BVH: nodes as 32-byte LinearBVHNode records
+ * (Bvh.cs:18-26) and the BVH-ordered primitive indices. */
+int rd_scene_export_bvh(const rd_scene* scene, void* nodes32, int32_t max_nodes,
+                        int32_t* prim_order, int32_t max_prims);
+
+/* Host-only BVH build over primitive bounds (bmin/bmax: [n][3]), the same
+ * builder rd_scene_create uses (Bvh.RecursiveBuild + FlattenBVHTree,
+ * Bvh.cs:234-442).  Needs no device.  nodes32 receives num_nodes 32-byte
+ * records (capacity max_nodes; 2n-1 always suffices), prim_order n indices. */
+int rd_bvh_build_host(const float* bmin, const float* bmax, int32_t n, int32_t max_prim, void* nodes32,
+                      int32_t max_nodes, int32_t* prim_order, int32_t* num_nodes, int32_t* max_depth);
+
+/* ---- traversal oracle hooks: Scene.Intersect / IsOccluded ---------------- */
+/* Bvh.Intersect(ray, out inct) (Bvh.cs:174-232) for n rays (host buffers). */
+int rd_trace_closest(rd_ctx* ctx, rd_scene* scene, const rd_ray* rays, int32_t n, rd_hit* hits);
+/* Bvh.Intersect(ray) (Bvh.cs:126-172): occluded[i] = 1 if any hit. */
+int rd_trace_any(rd_ctx* ctx, rd_scene* scene, const rd_ray* rays, int32_t n, uint8_t* occluded);
+/* Device-pointer variants: rays/hits already in HBM; stream may be NULL.
+ * counters (device, may be NULL) receives {nodes visited, prims tested}
+ * summed over the batch as two uint64. */
+int rd_trace_closest_device(rd_ctx* ctx, rd_scene* scene, const rd_ray* d_rays, int32_t n,
+                            rd_hit* d_hits, uint64_t* d_counters, void* stream);
+int rd_trace_any_device(rd_ctx* ctx, rd_scene* scene, const rd_ray* d_rays, int32_t n,
+                        uint8_t* d_occluded, uint64_t* d_counters, void* stream);
+
+/* ---- rendering: replaces BlockBasedRenderer + PathTracer/VolumetricPathTracer */
+/* Renders the scene's camera with its integrator.  out_rgb: host float[W*H*3]
+ * column-major, the per-pixel mean (BlockBasedRenderer.cs:167-178).
+ * out_sum2 (optional): per-pixel sum of squared samples, same layout.
+ * cancel (optional): polled between waves; non-zero => RD_ERR_CANCELLED
+ * (Renderer.Stop, BlockBasedRenderer.cs:85-88).  progress (optional):
+ * completed waves (Renderer.CompletedTaskCount). */
+int rd_render(rd_ctx* ctx, rd_scene* scene, const rd_render_params* params, float* out_rgb,
+              float* out_sum2, volatile int32_t* cancel, volatile int32_t* progress);
+/* Same, with out_rgb / out_sum2 in device memory (no host copies). */
+int rd_render_device(rd_ctx* ctx, rd_scene* scene, const rd_render_params* params,
+                     float* d_out_rgb, float* d_out_sum2, volatile int32_t* cancel,
+                     volatile int32_t* progress);
+int rd_get_render_stats(const rd_ctx* ctx, rd_render_stats* stats);
+
+/* ---- device memory helpers (so hosts need no GPU framework) ------------- */
+void* rd_device_alloc(rd_ctx* ctx, uint64_t bytes);
+int rd_device_free(rd_ctx* ctx, void* ptr);
+int rd_memcpy_h2d(rd_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int rd_memcpy_d2h(rd_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int rd_synchronize(rd_ctx* ctx);
+/* Average device time (ms) of the most recent launches of the named kernel
+ * stage ("trace_closest", "trace_any", "shade", ...), measured with HIP events
+ * on the stream the kernel ran on; count receives the number of launches. */
+int rd_kernel_timing(const rd_ctx* ctx, const char* stage, double* avg_ms, int64_t* count);
+int rd_reset_kernel_timing(rd_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RADIANTIUM_HIP_H */

@@ -1,0 +1,1496 @@
+// cpu_ref.cpp — TEST INFRASTRUCTURE: the oracle.  A scalar C++ restatement
+// of the reference's path / vol_path hot path (ksgfk/Radiantium, C#/.NET 6),
+// used only as the checker in tests/, by __graft_entry__.smoke() and as the
+// CPU baseline in bench.py.  The product never links or calls it.
+//
+// Each function cites the reference file:line it restates (paths relative to
+// the reference root).  Build: -O2 -ffp-contract=off, no fast-math, so float
+// expressions round like .NET's (which does not contract to FMA).
+#include "cpu_ref.h"
+#include "cpu_ref_math.h"
+
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+using namespace cref;
+
+static thread_local std::string g_err;
+static void set_err(const std::string& s) { g_err = s; }
+
+// ---------------------------------------------------------------- AABB / ray
+// BoundingBox3F (Radiantium.Core/BoundingBox3F.cs)
+struct BBox {
+    V3 mn, mx;
+    BBox() : mn(FLT_MAX), mx(-FLT_MAX) {}                       // :37-41
+    explicit BBox(V3 p) : mn(p), mx(p) {}                       // :43
+    void expand(V3 p) { mn = vmin(mn, p); mx = vmax(mx, p); }   // ExpendBy :101-105
+    void unite(const BBox& b) { mn = vmin(mn, b.mn); mx = vmax(mx, b.mx); }  // Union :107-111
+    V3 diag() const { return mx - mn; }
+    float area() const {                                        // SurfaceArea :24-31
+        V3 d = diag();
+        return 2 * (d.x * d.y + d.x * d.z + d.y * d.z);
+    }
+    int max_extent() const {                                    // MaximumExtent :216-225
+        V3 d = diag();
+        if (d.x > d.y && d.x > d.z) return 0;
+        else if (d.y > d.z) return 1;
+        return 2;
+    }
+    V3 offset(V3 p) const {                                     // Offset :140-147
+        V3 o = p - mn;
+        if (mx.x > mn.x) o.x /= mx.x - mn.x;
+        if (mx.y > mn.y) o.y /= mx.y - mn.y;
+        if (mx.z > mn.z) o.z /= mx.z - mn.z;
+        return o;
+    }
+    V3 center() const { return (mx + mn) * 0.5f; }
+};
+
+// Ray3F (Radiantium.Core/Ray3F.cs:5-20)
+struct Ray {
+    V3 o, d;
+    float tmin, tmax;
+    V3 inv;
+    Ray() : tmin(0), tmax(0) {}
+    Ray(V3 o_, V3 d_, float mn = EPS_F, float mxv = FLT_MAX) : o(o_), d(d_), tmin(mn), tmax(mxv) {
+        inv = V3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    }
+    V3 at(float t) const { return o + t * d; }
+};
+
+// BoundingBox3F.Intersect(Ray3F) (BoundingBox3F.cs:155-176).  The factor
+// 1+2*Gamma(3) is exactly 1.0f because Gamma uses float.Epsilon (:149-153),
+// so it is omitted.
+static inline bool box_hit(const BBox& b, const Ray& r) {
+    bool nx = r.inv.x < 0, ny = r.inv.y < 0, nz = r.inv.z < 0;
+    float tmin = ((nx ? b.mx : b.mn).x - r.o.x) * r.inv.x;
+    float tmax = ((nx ? b.mn : b.mx).x - r.o.x) * r.inv.x;
+    float tymin = ((ny ? b.mx : b.mn).y - r.o.y) * r.inv.y;
+    float tymax = ((ny ? b.mn : b.mx).y - r.o.y) * r.inv.y;
+    if (tmin > tymax || tymin > tmax) return false;
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    float tzmin = ((nz ? b.mx : b.mn).z - r.o.z) * r.inv.z;
+    float tzmax = ((nz ? b.mn : b.mx).z - r.o.z) * r.inv.z;
+    if (tmin > tzmax || tzmin > tmax) return false;
+    if (tzmin > tmin) tmin = tzmin;
+    if (tzmax < tmax) tmax = tzmax;
+    return (tmin < r.tmax) && (tmax > 0);
+}
+
+// Coordinate (Radiantium.Offline/Coordinate.cs:14-36): Duff et al. ONB.
+struct Frame {
+    V3 X, Y, Z;
+    Frame() {}
+    explicit Frame(V3 n) {
+        float sign = std::copysign(1.0f, n.z);
+        float a = -1 / (sign + n.z);
+        float b = n.x * n.y * a;
+        X = V3(mulsign(sqr(n.x) * a, n.z) + 1.0f, mulsign(b, n.z), mulsign(n.x, -n.z));
+        Y = V3(b, sign + sqr(n.y) * a, -n.y);
+        Z = n;
+    }
+    V3 to_local(V3 v) const { return V3(dot(v, X), dot(v, Y), dot(v, Z)); }   // :70-73
+    V3 to_world(V3 v) const { return X * v.x + Y * v.y + Z * v.z; }            // :75-78
+};
+// Coordinate static helpers (:80-122).  MathExt.Fma is a*b+c (not fused).
+static inline float cos_theta(V3 v) { return v.z; }
+static inline float abs_cos_theta(V3 v) { return std::fabs(v.z); }
+static inline float cos2_theta(V3 v) { return sqr(v.z); }
+static inline float sin2_theta(V3 v) { return v.x * v.x + sqr(v.y); }
+static inline float tan_theta(V3 v) { return safe_sqrt(-v.z * v.z + 1.0f) / v.z; }
+static inline float tan2_theta(V3 v) { return nmax(-v.z * v.z + 1.0f, 0.0f) / sqr(v.z); }
+static inline float cos2_phi(V3 v) {
+    float s2 = sin2_theta(v);
+    return std::fabs(s2) <= 4.0f * EPS_F ? 1.0f : nclamp(sqr(v.x) / s2, -1.0f, 1.0f);
+}
+static inline float sin2_phi(V3 v) {
+    float s2 = sin2_theta(v);
+    return std::fabs(s2) <= 4.0f * EPS_F ? 0.0f : nclamp(sqr(v.y) / s2, -1.0f, 1.0f);
+}
+static inline bool same_hemisphere(V3 a, V3 b) { return a.z * b.z > 0; }
+static inline V3 faceforward(V3 v, V3 v2) { return dot(v, v2) < 0.0f ? -v : v; }
+
+// Intersection (Radiantium.Offline/Intersection.cs:20-72)
+struct Inct {
+    V3 P;
+    V2 UV;
+    float T;
+    int prim;   // primitive index (input order)
+    Frame sh;
+    V3 wr;
+    V3 N() const { return sh.Z; }
+};
+
+// ---------------------------------------------------------------- scene data
+struct Tex {
+    int w = 0, h = 0, wrap = 0, filter = 0;
+    C3 value;
+    std::vector<float> data;
+    // ColorBuffer.GetRGB (ColorBuffer.cs:44-60), 3 channels.
+    C3 read(int x, int y) const {
+        if (w == 0) return value;
+        size_t i = ((size_t)x * h + y) * 3;
+        return C3(data[i], data[i + 1], data[i + 2]);
+    }
+    // TextureSampler.Sample (TextureSampler.cs:28-72)
+    C3 sample(V2 uv) const {
+        if (w == 0) return value;   // ConstColorTexture2D.Read ignores (x, y)
+        auto wrapf = [&](float a) {
+            if (wrap == 0) return nclamp(a, 0.0f, 1.0f);
+            return nclamp(a - std::floor(a), 0.0f, 1.0f);
+        };
+        float u = wrapf(uv.x), v = wrapf(uv.y);
+        float fu = u * (w - 1), fv = v * (h - 1);
+        int pu = (int)fu, pv = (int)fv;
+        if (filter == 0) return read(pu, pv);
+        int dpu = (fu > pu + 0.5f) ? 1 : -1;
+        int dpv = (fv > pv + 0.5f) ? 1 : -1;
+        int apu = iclamp(pu + dpu, 0, w - 1);
+        int apv = iclamp(pv + dpv, 0, h - 1);
+        float du = nmin(std::fabs(fu - pu - 0.5f), 1.0f);
+        float dv = nmin(std::fabs(fv - pv - 0.5f), 1.0f);
+        C3 u0v0 = read(pu, pv), u1v0 = read(apu, pv), u0v1 = read(pu, apv), u1v1 = read(apu, apv);
+        return (u0v0 * (1 - du) + u1v0 * du) * (1 - dv) + (u0v1 * (1 - du) + u1v1 * du) * dv;
+    }
+};
+
+struct Camera {
+    // PerspectiveCamera (Radiantium.Offline/Camera.cs:101-191)
+    float near_z, far_z;
+    float inv_res_x, inv_res_y;
+    M4 clip_to_camera, camera_to_world;
+    int W, H;
+    bool init(const rd_camera_desc& c) {
+        W = c.width; H = c.height;
+        near_z = c.near_z; far_z = c.far_z;
+        M4 c2w;
+        if (c.use_matrix) {
+            for (int i = 0; i < 16; i++) c2w.m[i / 4][i % 4] = c.to_world[i];
+        } else {
+            // MathExt.LookAtLeftHand (MathExt.cs:149-161)
+            V3 origin(c.origin[0], c.origin[1], c.origin[2]);
+            V3 target(c.target[0], c.target[1], c.target[2]);
+            V3 up(c.up[0], c.up[1], c.up[2]);
+            V3 dir = normalize(target - origin);
+            V3 left = normalize(cross(normalize(up), dir));
+            V3 real_up = normalize(cross(dir, left));
+            c2w = m4_identity();
+            c2w.m[0][0] = left.x; c2w.m[0][1] = left.y; c2w.m[0][2] = left.z; c2w.m[0][3] = 0;
+            c2w.m[1][0] = real_up.x; c2w.m[1][1] = real_up.y; c2w.m[1][2] = real_up.z; c2w.m[1][3] = 0;
+            c2w.m[2][0] = dir.x; c2w.m[2][1] = dir.y; c2w.m[2][2] = dir.z; c2w.m[2][3] = 0;
+            c2w.m[3][0] = origin.x; c2w.m[3][1] = origin.y; c2w.m[3][2] = origin.z; c2w.m[3][3] = 1;
+        }
+        // Update (Camera.cs:128-180)
+        inv_res_x = 1.0f / (float)W;
+        inv_res_y = 1.0f / (float)H;
+        float aspect = W / (float)H;
+        float recip = 1.0f / (far_z - near_z);
+        float cot = 1.0f / std::tan(radian(c.fov / 2.0f));
+        M4 persp{}; persp.m[0][0] = cot; persp.m[1][1] = cot; persp.m[2][2] = far_z * recip;
+        persp.m[2][3] = 1; persp.m[3][2] = -near_z * far_z * recip; persp.m[3][3] = 0;
+        M4 scale{}; scale.m[0][0] = -0.5f; scale.m[1][1] = -0.5f * aspect; scale.m[2][2] = 1; scale.m[3][3] = 1;
+        M4 trans = m4_identity(); trans.m[3][0] = -1; trans.m[3][1] = -1 / aspect;
+        M4 p = m4_mul(m4_mul(persp, trans), scale);
+        if (!m4_invert(p, &clip_to_camera)) return false;
+        camera_to_world = c2w;
+        M4 w2c;
+        if (!m4_invert(c2w, &w2c)) return false;
+        return true;
+    }
+    // SampleRay (Camera.cs:182-191)
+    Ray sample_ray(float sx, float sy) const {
+        V3 ndc(sx * inv_res_x, sy * inv_res_y, 0.0f);
+        V3 nearp = xform_point(ndc, clip_to_camera);
+        V3 dir = normalize(nearp);
+        float inv_z = 1.0f / dir.z;
+        V3 o = xform_point(V3(0.0f), camera_to_world);
+        V3 d = normalize(xform_vector(dir, camera_to_world));
+        return Ray(o, d, near_z * inv_z, far_z * inv_z);
+    }
+};
+
+struct Node {
+    BBox b;
+    int offset;          // PrimitivesIndex / SecondChildIndex
+    uint16_t count;
+    uint8_t axis;
+};
+
+struct cr_scene {
+    std::vector<V3> pos, nrm;
+    std::vector<V2> uvs;
+    std::vector<int32_t> idx;
+    std::vector<uint8_t> tflags;
+    std::vector<float> sph;
+    std::vector<rd_prim_desc> prims;
+    std::vector<float> prim_area;
+    std::vector<Tex> tex;
+    std::vector<rd_material_desc> mats;
+    std::vector<rd_light_desc> lights;
+    std::vector<rd_medium_desc> media;
+    std::vector<int> infinite;
+    int global_medium = -1;
+    int max_prim = 1;
+    std::vector<Node> nodes;
+    std::vector<int> order;
+    int leaf_count = 0;
+    int max_depth = 0;
+    Camera cam;
+    rd_integrator_desc integ;
+
+    // Triangle accessors (Shapes/Triangle.cs:71-81)
+    V3 pa(int t) const { return pos[idx[3 * t]]; }
+    V3 pb(int t) const { return pos[idx[3 * t + 1]]; }
+    V3 pc(int t) const { return pos[idx[3 * t + 2]]; }
+    bool has_n(int t) const { return (tflags[t] & 1) != 0; }
+    bool has_uv(int t) const { return (tflags[t] & 2) != 0; }
+};
+
+// ---------------------------------------------------------------- shapes
+struct SurfPt { float u, v, t; };
+
+// Triangle.Intersect(ray, out SurfacePoint) (Shapes/Triangle.cs:149-177)
+static inline bool tri_hit(const cr_scene& s, int t, const Ray& r, SurfPt* sp) {
+    V3 A = s.pa(t), B = s.pb(t), C = s.pc(t);
+    V3 e1 = B - A, e2 = C - A;
+    V3 pvec = cross(r.d, e2);
+    float det = dot(e1, pvec);
+    if (std::fabs(det) < EPS_F) return false;
+    float inv_det = 1.0f / det;
+    V3 tvec = r.o - A;
+    float u = dot(tvec, pvec) * inv_det;
+    if (u < 0.0f || u > 1.0f) return false;
+    V3 qvec = cross(tvec, e1);
+    float v = dot(r.d, qvec) * inv_det;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    float tt = dot(e2, qvec) * inv_det;
+    bool hit = tt >= r.tmin && tt <= r.tmax;
+    if (hit) { sp->u = u; sp->v = v; sp->t = tt; }
+    return hit;
+}
+
+// Triangle.GetIntersection (Triangle.cs:96-120)
+static Inct tri_inct(const cr_scene& s, int t, const Ray& r, const SurfPt& sp, int prim) {
+    V3 bary(1 - (sp.u + sp.v), sp.u, sp.v);
+    V3 A = s.pa(t), B = s.pb(t), C = s.pc(t);
+    Inct it;
+    it.P = bary.x * A + bary.y * B + bary.z * C;
+    V3 n;
+    if (s.has_n(t)) {
+        n = normalize(bary.x * s.nrm[s.idx[3 * t]] + bary.y * s.nrm[s.idx[3 * t + 1]] + bary.z * s.nrm[s.idx[3 * t + 2]]);
+    } else {
+        n = normalize(cross(B - A, C - A));
+    }
+    if (s.has_uv(t)) {
+        it.UV = bary.x * s.uvs[s.idx[3 * t]] + bary.y * s.uvs[s.idx[3 * t + 1]] + bary.z * s.uvs[s.idx[3 * t + 2]];
+    } else {
+        it.UV = bary.x * V2(0, 0) + bary.y * V2(1, 0) + bary.z * V2(1, 1);
+    }
+    it.sh = Frame(n);
+    it.T = sp.t;
+    it.wr = -r.d;
+    it.prim = prim;
+    return it;
+}
+
+// Math.Min / Math.Max on doubles (NaN propagates, -0 < +0).
+static inline double dmin_net(double a, double b) {
+    if (a != b) { if (!std::isnan(a)) return a < b ? a : b; return a; }
+    return std::signbit(a) ? a : b;
+}
+static inline double dmax_net(double a, double b) {
+    if (a != b) { if (!std::isnan(a)) return b < a ? a : b; return a; }
+    return std::signbit(b) ? a : b;
+}
+
+// MathExt.SolveQuadratic (MathExt.cs:119-138), double precision
+static bool solve_quadratic(double a, double b, double c, double* x0o, double* x1o) {
+    bool linear = a == 0.0;
+    bool valid_linear = linear && b != 0.0;
+    double x0 = -c / b, x1 = -c / b;
+    double disc = b * b - 4.0 * a * c;
+    bool valid_quad = !linear && (disc >= 0.0);
+    if (valid_quad) {
+        double rd = std::sqrt(disc);
+        double temp = -0.5 * (b + std::copysign(rd, b));
+        double x0p = temp / a;
+        double x1p = c / temp;
+        double x0m = dmin_net(x0p, x1p);
+        double x1m = dmax_net(x0p, x1p);
+        x0 = linear ? x0 : x0m;
+        x1 = linear ? x0 : x1m;
+    }
+    *x0o = x0; *x1o = x1;
+    return valid_linear || valid_quad;
+}
+
+// Sphere.Intersect(ray, out SurfacePoint) (Shapes/Sphere.cs:65-80)
+static inline bool sphere_hit(const cr_scene& s, int k, const Ray& r, SurfPt* sp) {
+    const float* S = &s.sph[4 * k];
+    double mint = r.tmin, maxt = r.tmax;
+    double ox = (double)r.o.x - (double)S[0], oy = (double)r.o.y - (double)S[1], oz = (double)r.o.z - (double)S[2];
+    double dx = r.d.x, dy = r.d.y, dz = r.d.z;
+    double a = dx * dx + dy * dy + dz * dz;
+    double b = 2 * (ox * dx + oy * dy + oz * dz);
+    double c = (ox * ox + oy * oy + oz * oz) - (double)sqr(S[3]);
+    double nt, ft;
+    bool found = solve_quadratic(a, b, c, &nt, &ft);
+    bool out_b = !(nt <= maxt && ft >= mint);
+    bool in_b = nt < mint && ft > maxt;
+    bool hit = found && !out_b && !in_b;
+    if (hit) { sp->u = 0; sp->v = 0; sp->t = nt < mint ? (float)ft : (float)nt; }
+    return hit;
+}
+
+// Sphere.GetIntersection (Sphere.cs:34-48); sphere shapes carry an identity
+// WorldToModel here (the build bakes transforms into the centre).
+static Inct sphere_inct(const cr_scene& s, int k, const Ray& r, const SurfPt& sp, int prim) {
+    const float* S = &s.sph[4 * k];
+    V3 c(S[0], S[1], S[2]);
+    Inct it;
+    float t = sp.t;
+    V3 n = normalize(r.at(t) - c);
+    it.P = n * S[3] + c;
+    it.sh = Frame(n);
+    V3 local = n;
+    float theta = std::acos(-local.y);
+    float phi = std::atan2(-local.z, local.x) + PI_F;
+    it.UV = V2(phi / (2 * PI_F), theta / PI_F);
+    it.T = t;
+    it.wr = -r.d;
+    it.prim = prim;
+    return it;
+}
+
+static inline bool prim_hit(const cr_scene& s, int p, const Ray& r, SurfPt* sp) {
+    const rd_prim_desc& pd = s.prims[p];
+    return pd.kind == RD_PRIM_TRIANGLE ? tri_hit(s, pd.index, r, sp) : sphere_hit(s, pd.index, r, sp);
+}
+static inline Inct prim_inct(const cr_scene& s, int p, const Ray& r, const SurfPt& sp) {
+    const rd_prim_desc& pd = s.prims[p];
+    return pd.kind == RD_PRIM_TRIANGLE ? tri_inct(s, pd.index, r, sp, p) : sphere_inct(s, pd.index, r, sp, p);
+}
+
+static BBox prim_bound(const cr_scene& s, int p) {
+    const rd_prim_desc& pd = s.prims[p];
+    if (pd.kind == RD_PRIM_TRIANGLE) {                     // Triangle ctor (Triangle.cs:88-91)
+        BBox b(s.pa(pd.index));
+        b.expand(s.pb(pd.index));
+        b.expand(s.pc(pd.index));
+        return b;
+    }
+    const float* S = &s.sph[4 * pd.index];                 // Sphere ctor (Sphere.cs:30-31)
+    V3 c(S[0], S[1], S[2]);
+    BBox b; b.mn = c - V3(S[3]); b.mx = c + V3(S[3]);
+    return b;
+}
+
+// ---------------------------------------------------------------- BVH build
+// Accelerators/Bvh.cs:234-442 (SplitMethod.SAH).
+struct PInfo { int num; BBox b; V3 c; };
+struct BuildNode { BBox b; int left = -1, right = -1, axis = 0, first = 0, count = 0; };
+
+// MathExt.Partition (MathExt.cs:170-201); `end` is an inclusive index.
+template <class F>
+static int partition_ref(std::vector<PInfo>& list, int begin, int end, F pred) {
+    int first = begin, last = end;
+    while (true) {
+        while (true) {
+            if (first == last) return first;
+            if (!pred(list[first])) break;
+            ++first;
+        }
+        while (!pred(list[last])) {
+            if (first == last) return first;
+            last--;
+        }
+        std::swap(list[first], list[last]);
+        first++;
+    }
+}
+
+struct Builder {
+    const cr_scene& s;
+    int max_prims;
+    std::vector<PInfo> info;
+    std::vector<BuildNode> bn;
+    std::vector<int> ordered;
+    int total = 0, leaves = 0;
+    Builder(const cr_scene& sc, int mp) : s(sc), max_prims(mp) {}
+
+    int make_leaf(int start, int end, const BBox& bounds) {
+        BuildNode n; n.b = bounds; n.first = (int)ordered.size(); n.count = end - start;
+        for (int i = start; i < end; i++) ordered.push_back(info[i].num);
+        leaves++;
+        bn.push_back(n);
+        return (int)bn.size() - 1;
+    }
+    int build(int start, int end, int depth, int* max_depth) {
+        total++;
+        if (depth > *max_depth) *max_depth = depth;
+        BBox bounds;
+        for (int i = start; i < end; i++) bounds.unite(info[i].b);
+        int n = end - start;
+        if (n == 1) return make_leaf(start, end, bounds);
+        BBox cb;
+        for (int i = start; i < end; i++) cb.expand(info[i].c);
+        int dim = cb.max_extent();
+        int mid = (start + end) / 2;
+        if (cb.mx[dim] == cb.mn[dim]) return make_leaf(start, end, bounds);
+        if (n <= 2) {
+            mid = (start + end) / 2;
+            // List.Sort of two elements: swap when the comparer says l > r.
+            float lv = info[start].c[dim], rv = info[start + 1].c[dim];
+            if (!(lv == rv) && !(lv < rv)) std::swap(info[start], info[start + 1]);
+        } else {
+            const int nb = 12;
+            int cnt[nb] = {0};
+            BBox bb[nb];
+            for (int i = start; i < end; i++) {
+                V3 off = cb.offset(info[i].c);
+                int b = (int)(nb * off[dim]);
+                if (b == nb) b = nb - 1;
+                cnt[b]++;
+                bb[b].unite(info[i].b);
+            }
+            float cost[nb - 1];
+            for (int i = 0; i < nb - 1; i++) {
+                BBox b0, b1;
+                int c0 = 0, c1 = 0;
+                for (int j = 0; j <= i; j++) { b0.unite(bb[j]); c0 += cnt[j]; }
+                for (int j = i + 1; j < nb; j++) { b1.unite(bb[j]); c1 += cnt[j]; }
+                cost[i] = 1 + (c0 * b0.area() + c1 * b1.area()) / bounds.area();
+            }
+            float min_cost = cost[0];
+            int split = 0;
+            for (int i = 1; i < nb - 1; i++) if (cost[i] < min_cost) { min_cost = cost[i]; split = i; }
+            float leaf_cost = (float)n;
+            if (n > max_prims || min_cost < leaf_cost) {
+                mid = partition_ref(info, start, end - 1, [&](const PInfo& pi) {
+                    V3 off = cb.offset(pi.c);
+                    int b = (int)(nb * off[dim]);
+                    return b <= split;
+                });
+            } else {
+                return make_leaf(start, end, bounds);
+            }
+        }
+        int l = build(start, mid, depth + 1, max_depth);
+        int r = build(mid, end, depth + 1, max_depth);
+        BuildNode node;
+        node.left = l; node.right = r; node.axis = dim;
+        node.b = bn[l].b; node.b.unite(bn[r].b);           // BVHBuildNode(axis, l, r) :46-53
+        bn.push_back(node);
+        return (int)bn.size() - 1;
+    }
+    // FlattenBVHTree (Bvh.cs:417-442): depth-first, left child adjacent.
+    int flatten(int id, std::vector<Node>& out, int* offset) {
+        const BuildNode& n = bn[id];
+        Node& ln = out[*offset];
+        ln.b = n.b;
+        int my = (*offset)++;
+        if (n.count > 0) {
+            out[my].offset = n.first;
+            out[my].count = (uint16_t)n.count;
+            out[my].axis = 0;
+        } else {
+            out[my].axis = (uint8_t)n.axis;
+            out[my].count = 0;
+            flatten(n.left, out, offset);
+            out[my].offset = flatten(n.right, out, offset);
+        }
+        return my;
+    }
+};
+
+static void build_bvh(cr_scene& s) {
+    int np = (int)s.prims.size();
+    if (np == 0) return;
+    Builder B(s, s.max_prim);
+    B.info.resize(np);
+    for (int i = 0; i < np; i++) {
+        B.info[i].num = i;
+        B.info[i].b = prim_bound(s, i);
+        B.info[i].c = 0.5f * B.info[i].b.mn + 0.5f * B.info[i].b.mx;   // BVHPrimitiveInfo :61-66
+    }
+    B.bn.reserve(2 * np);
+    B.ordered.reserve(np);
+    int maxd = 0;
+    int root = B.build(0, np, 0, &maxd);
+    s.nodes.assign(B.total, Node());
+    int off = 0;
+    B.flatten(root, s.nodes, &off);
+    s.order = B.ordered;
+    s.leaf_count = B.leaves;
+    s.max_depth = maxd;
+}
+
+// ---------------------------------------------------------------- traversal
+// Bvh.Intersect(ray, out inct) (Bvh.cs:174-232).  The full Intersection is
+// built for the final hit only: candidate Intersections differ from it in
+// nothing the loop reads except T, which equals SurfacePoint.T.
+static bool bvh_closest(const cr_scene& s, Ray ray, int* prim_out, SurfPt* sp_out, uint64_t* cnt) {
+    if (s.nodes.empty()) return false;
+    bool neg[3] = {ray.inv.x < 0, ray.inv.y < 0, ray.inv.z < 0};
+    int stack[64];
+    int sp = 0, cur = 0;
+    float best = FLT_MAX;
+    bool any = false;
+    uint64_t nv = 0, nt = 0;
+    while (true) {
+        const Node& node = s.nodes[cur];
+        nv++;
+        if (box_hit(node.b, ray)) {
+            if (node.count > 0) {
+                for (int i = 0; i < node.count; ++i) {
+                    int p = s.order[node.offset + i];
+                    SurfPt pt;
+                    nt++;
+                    if (prim_hit(s, p, ray, &pt)) {
+                        any = true;
+                        if (pt.t < best) { best = pt.t; *prim_out = p; *sp_out = pt; ray.tmax = pt.t; }
+                    }
+                }
+                if (sp == 0) break;
+                cur = stack[--sp];
+            } else {
+                if (sp >= 64) { set_err("BVH stack overflow"); return false; }
+                if (neg[node.axis]) { stack[sp++] = cur + 1; cur = node.offset; }
+                else { stack[sp++] = node.offset; cur = cur + 1; }
+            }
+        } else {
+            if (sp == 0) break;
+            cur = stack[--sp];
+        }
+    }
+    if (cnt) { cnt[0] += nv; cnt[1] += nt; }
+    return any;
+}
+
+// Bvh.Intersect(ray) (Bvh.cs:126-172)
+static bool bvh_any(const cr_scene& s, const Ray& ray, uint64_t* cnt) {
+    if (s.nodes.empty()) return false;
+    bool neg[3] = {ray.inv.x < 0, ray.inv.y < 0, ray.inv.z < 0};
+    int stack[64];
+    int sp = 0, cur = 0;
+    uint64_t nv = 0, nt = 0;
+    bool hit = false;
+    while (true) {
+        const Node& node = s.nodes[cur];
+        nv++;
+        if (box_hit(node.b, ray)) {
+            if (node.count > 0) {
+                for (int i = 0; i < node.count; ++i) {
+                    int p = s.order[node.offset + i];
+                    SurfPt pt;
+                    nt++;
+                    if (prim_hit(s, p, ray, &pt)) { hit = true; break; }
+                }
+                if (hit) break;
+                if (sp == 0) break;
+                cur = stack[--sp];
+            } else {
+                if (neg[node.axis]) { stack[sp++] = cur + 1; cur = node.offset; }
+                else { stack[sp++] = node.offset; cur = cur + 1; }
+            }
+        } else {
+            if (sp == 0) break;
+            cur = stack[--sp];
+        }
+    }
+    if (cnt) { cnt[0] += nv; cnt[1] += nt; }
+    return hit;
+}
+
+// Scene.Intersect(ray, out inct) (Scene.cs:32-35)
+static bool scene_intersect(const cr_scene& s, const Ray& r, Inct* it, uint64_t* st) {
+    int p = -1;
+    SurfPt sp;
+    if (st) st[0]++;
+    if (!bvh_closest(s, r, &p, &sp, nullptr)) return false;
+    *it = prim_inct(s, p, r, sp);
+    return true;
+}
+static bool scene_occluded_ray(const cr_scene& s, const Ray& r, uint64_t* st) {
+    if (st) st[1]++;
+    return bvh_any(s, r, nullptr);
+}
+// Scene.IsOccluded (Scene.cs:58-70)
+static bool scene_is_occluded(const cr_scene& s, V3 from, V3 to, uint64_t* st) {
+    V3 vec = to - from;
+    float dis = length(vec);
+    if (dis < 0.001f) return true;
+    V3 dir = normalize(vec);
+    Ray sr(from, dir, 0.0001f, dis - 0.0001f);
+    return scene_occluded_ray(s, sr, st);
+}
+
+// ---------------------------------------------------------------- sampling
+static inline V3 square_to_cosine_hemisphere(float ux, float uy) {
+    // Probability.SquareToUniformDisk + SquareToCosineHemisphere (Probability.cs:53-58,94-100)
+    float radius = std::sqrt(ux);
+    float angle = uy * PI_F * 2;
+    float x = radius * std::cos(angle), y = radius * std::sin(angle);
+    return V3(x, y, std::sqrt(1 - x * x - y * y));
+}
+static inline V3 spherical_direction(float st, float ct, float phi) {   // MathExt.cs:111-115
+    float sp = std::sin(phi), cp = std::cos(phi);
+    return V3(st * cp, st * sp, ct);
+}
+
+// ---------------------------------------------------------------- microfacet
+// Bxdf/Microfacet.cs
+struct MfDist {
+    int kind;   // 0 Beckmann 1 GGX
+    float ax, ay;
+    MfDist(int k, float roughness, float aniso) : kind(k) {          // RoughnessToAlpha :177-183
+        ax = nmax(roughness * roughness * (1 + aniso), 0.001f);
+        ay = nmax(roughness * roughness * (1 - aniso), 0.001f);
+    }
+    float D(V3 wh) const {
+        float t2 = tan2_theta(wh);
+        if (std::isinf(t2)) return 0.0f;
+        float c4 = cos2_theta(wh) * cos2_theta(wh);
+        if (kind == 0)                                               // :117-123
+            return std::exp(-t2 * (cos2_phi(wh) / (ax * ax) + sin2_phi(wh) / (ay * ay))) / (PI_F * ax * ay * c4);
+        float e = (cos2_phi(wh) / (ax * ax) + sin2_phi(wh) / (ay * ay)) * t2;   // :125-132
+        return 1 / (PI_F * ax * ay * c4 * (1 + e) * (1 + e));
+    }
+    float lambda(V3 w) const {
+        float at = std::fabs(tan_theta(w));
+        if (std::isinf(at)) return 0.0f;
+        float alpha = std::sqrt(cos2_phi(w) * ax * ax + sin2_phi(w) * ay * ay);
+        if (kind == 0) {                                             // :134-142
+            float a = 1 / (alpha * at);
+            if (a >= 1.6f) return 0;
+            return (1 - 1.259f * a + 0.396f * a * a) / (3.535f * a + 2.181f * a * a);
+        }
+        float a2t2 = (alpha * at) * (alpha * at);                    // :144-151
+        return (-1 + std::sqrt(1.0f + a2t2)) / 2;
+    }
+    float G(V3 wo, V3 wi) const { return 1 / (1 + lambda(wo) + lambda(wi)); }
+    float pdf(V3 wo, V3 wh) const { return D(wh) * abs_cos_theta(wh); }
+    V3 sample_wh(V3 wo, Rng& rng) const {
+        float ux = rng.next_float(), uy = rng.next_float();
+        if (kind == 0) {                                             // Beckmann.SampleWh :60-86
+            float t2, phi;
+            if (ax == ay) {
+                float ls = std::log(1 - ux);
+                t2 = -ax * ax * ls;
+                phi = uy * 2 * PI_F;
+            } else {
+                float ls = std::log(1 - ux);
+                phi = std::atan(ay / ax * std::tan(2 * PI_F * uy + 0.5f * PI_F));
+                if (uy > 0.5f) phi += PI_F;
+                float sp = std::sin(phi), cp = std::cos(phi);
+                float ax2 = ax * ax, ay2 = ay * ay;
+                t2 = -ls / (cp * cp / ax2 + sp * sp / ay2);
+            }
+            float ct = 1 / std::sqrt(1 + t2);
+            float st = std::sqrt(nmax(0, 1 - ct * ct));
+            V3 wh = spherical_direction(st, ct, phi);
+            if (!same_hemisphere(wo, wh)) wh = -wh;
+            return wh;
+        }
+        float phi = (2 * PI_F) * uy;                                 // SampleWhGGX :185-209
+        float ct;
+        if (ax == ay) {
+            float t2 = ax * ax * ux / (1.0f - ux);
+            ct = 1 / std::sqrt(1 + t2);
+        } else {
+            phi = std::atan(ay / ax * std::tan(2 * PI_F * uy + .5f * PI_F));
+            if (uy > 0.5f) phi += PI_F;
+            float sp = std::sin(phi), cp = std::cos(phi);
+            float ax2 = ax * ax, ay2 = ay * ay;
+            float a2 = 1 / (cp * cp / ax2 + sp * sp / ay2);
+            float t2 = a2 * ux / (1 - ux);
+            ct = 1 / std::sqrt(1 + t2);
+        }
+        float st = std::sqrt(nmax(0.0f, 1.0f - ct * ct));
+        V3 wh = spherical_direction(st, ct, phi);
+        if (!same_hemisphere(wo, wh)) wh = -wh;
+        return wh;
+    }
+};
+
+// Fresnel.DielectricFunc (Bxdf/Fresnel.cs:50-68)
+static float fresnel_dielectric(float ci, float ei, float et) {
+    ci = nclamp(ci, -1, 1);
+    bool entering = ci > 0.0f;
+    if (!entering) { std::swap(ei, et); ci = std::fabs(ci); }
+    float si = std::sqrt(nmax(0, 1 - ci * ci));
+    float st = ei / et * si;
+    if (st >= 1) return 1;
+    float ct = std::sqrt(nmax(0, 1 - st * st));
+    float rparl = ((et * ci) - (ei * ct)) / ((et * ci) + (ei * ct));
+    float rperp = ((ei * ci) - (et * ct)) / ((ei * ci) + (et * ct));
+    return (rparl * rparl + rperp * rperp) / 2;
+}
+// Fresnel.ConductorFunc (Bxdf/Fresnel.cs:72-95)
+static C3 fresnel_conductor(float ci, C3 etai, C3 etat, C3 k) {
+    ci = nclamp(ci, -1, 1);
+    C3 eta = etat / etai;
+    C3 etak = k / etai;
+    float ci2 = ci * ci;
+    float si2 = 1.0f - ci2;
+    C3 eta2 = eta * eta;
+    C3 etak2 = etak * etak;
+    C3 t0 = eta2 - etak2 - si2;
+    C3 a2pb2 = csqrt(t0 * t0 + 4 * eta2 * etak2);
+    C3 t1 = a2pb2 + ci2;
+    C3 a = csqrt(0.5f * (a2pb2 + t0));
+    C3 t2 = 2 * ci * a;
+    C3 Rs = (t1 - t2) / (t1 + t2);
+    C3 t3 = ci2 * a2pb2 + si2 * si2;
+    C3 t4 = t2 * si2;
+    C3 Rp = Rs * (t3 - t4) / (t3 + t4);
+    return 0.5f * (Rp + Rs);
+}
+// MathExt.Refract (MathExt.cs:83-92)
+static bool refract(V3 wi, V3 n, float eta, V3* wt) {
+    float ci = dot(n, wi);
+    float s2i = nmax(0, 1 - ci * ci);
+    float s2t = eta * eta * s2i;
+    if (s2t >= 1) return false;
+    float ct = std::sqrt(1 - s2t);
+    *wt = normalize(eta * -wi + (eta * ci - ct) * n);
+    return true;
+}
+
+// ---------------------------------------------------------------- materials
+enum { T_REFLECTION = 1, T_TRANSMISSION = 2, T_DIFFUSE = 4, T_GLOSSY = 8, T_SPECULAR = 16 };  // IBxdf.cs:7-15
+struct BSample { V3 wi; C3 fr; float pdf = 0; int type = 0; };
+
+static inline C3 tex_at(const cr_scene& s, int t, V2 uv) { return s.tex[t].sample(uv); }
+
+// MicrofacetReflectionBrdf<Conductor, D>.Fr (Bxdf/MicrofacetReflectionBrdf.cs:26-41), R = 1
+static C3 mf_reflect_fr(const MfDist& d, C3 eta, C3 k, V3 wo, V3 wi) {
+    if (!same_hemisphere(wo, wi)) return C3(0.0f);
+    float cto = abs_cos_theta(wo), cti = abs_cos_theta(wi);
+    V3 wh = wi + wo;
+    if (cti == 0 || cto == 0) return C3(0);
+    if (wh.x == 0 && wh.y == 0 && wh.z == 0) return C3(0);
+    wh = normalize(wh);
+    C3 f = fresnel_conductor(dot(wi, faceforward(wh, V3(0, 0, 1))), C3(1.0f), eta, k);
+    float D = d.D(wh);
+    float G = d.G(wo, wi);
+    return C3(1.0f) * cabs((D * G * f) / (4 * cti * cto));
+}
+static float mf_reflect_pdf(const MfDist& d, V3 wo, V3 wi) {        // :43-48
+    if (!same_hemisphere(wo, wi)) return 0;
+    V3 wh = normalize(wo + wi);
+    return d.pdf(wo, wh) / (4 * dot(wo, wh));
+}
+
+static C3 material_fr(const cr_scene& s, const rd_material_desc& m, V3 wo, V3 wi, V2 uv) {
+    switch (m.type) {
+    case RD_MAT_DIFFUSE: {                                           // Materials/Diffuse.cs:21-27
+        if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!m.two_sided) return C3(0.0f); }
+        C3 R = tex_at(s, m.tex[0], uv);                              // LambertianReflectionBrdf.Fr :16-19
+        return same_hemisphere(wo, wi) ? R / PI_F : C3(0.0f);
+    }
+    case RD_MAT_ROUGH_METAL: {                                       // Materials/RoughMetal.cs:33-58
+        float rough = tex_at(s, m.tex[0], uv).r, an = tex_at(s, m.tex[1], uv).r;
+        MfDist d(m.dist, rough, an);
+        if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!m.two_sided) return C3(0.0f); }
+        return mf_reflect_fr(d, C3(m.eta[0], m.eta[1], m.eta[2]), C3(m.k[0], m.k[1], m.k[2]), wo, wi);
+    }
+    case RD_MAT_ROUGH_GLASS: {                                       // Materials/RoughGlass.cs:151-195
+        float rough = tex_at(s, m.tex[2], uv).r, an = tex_at(s, m.tex[3], uv).r;
+        MfDist d(m.dist, rough, an);
+        C3 r = tex_at(s, m.tex[0], uv), t = tex_at(s, m.tex[1], uv);
+        bool is_reflect = same_hemisphere(wo, wi);
+        bool entering = cos_theta(wo) > 0;
+        float ei = entering ? m.eta_a : m.eta_b;
+        float et = entering ? m.eta_b : m.eta_a;
+        float eta = is_reflect ? 1 : et / ei;
+        V3 wh = normalize(wo + wi * eta);
+        if (cos_theta(wh) < 0) wh = -wh;
+        float sqrt_denom = dot(wo, wh) + eta * dot(wi, wh);
+        float D = d.D(wh);
+        float G = d.G(wo, wi);
+        float F = fresnel_dielectric(dot(wo, wh), ei, et);
+        C3 bsdf;
+        if (is_reflect) {
+            bsdf = r * (D * G * F / (4.0f * cos_theta(wi) * cos_theta(wo)));
+        } else {
+            if (dot(wo, wh) * dot(wi, wh) > 0) return C3();
+            float factor = 1 / eta;   // TransportMode.Radiance
+            bsdf = (1 - F) * t *
+                   std::fabs(D * G * eta * eta * abs_dot(wi, wh) * abs_dot(wo, wh) * factor * factor /
+                             (cos_theta(wo) * cos_theta(wi) * sqrt_denom * sqrt_denom));
+        }
+        return bsdf.valid() ? bsdf : C3(0.0f);
+    }
+    default:
+        return C3(0.0f);
+    }
+}
+
+static float material_pdf(const cr_scene& s, const rd_material_desc& m, V3 wo, V3 wi, V2 uv) {
+    switch (m.type) {
+    case RD_MAT_DIFFUSE: {                                           // Diffuse.cs:29-37
+        if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!m.two_sided) return 0.0f; }
+        return same_hemisphere(wo, wi) ? abs_cos_theta(wi) / PI_F : 0.0f;   // Lambert.Pdf :21-24
+    }
+    case RD_MAT_ROUGH_METAL: {                                       // RoughMetal.cs:60-85
+        float rough = tex_at(s, m.tex[0], uv).r, an = tex_at(s, m.tex[1], uv).r;
+        MfDist d(m.dist, rough, an);
+        if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!m.two_sided) return 0.0f; }
+        return mf_reflect_pdf(d, wo, wi);
+    }
+    case RD_MAT_ROUGH_GLASS: {                                       // RoughGlass.cs:197-238
+        float rough = tex_at(s, m.tex[2], uv).r, an = tex_at(s, m.tex[3], uv).r;
+        MfDist d(m.dist, rough, an);
+        bool is_reflect = same_hemisphere(wo, wi);
+        bool entering = cos_theta(wo) > 0;
+        float ei = entering ? m.eta_a : m.eta_b;
+        float et = entering ? m.eta_b : m.eta_a;
+        float pdf;
+        if (is_reflect) {
+            V3 wh = normalize(wo + wi);
+            if (cos_theta(wh) < 0) wh = -wh;
+            if (dot(wo, wh) * cos_theta(wo) <= 0 || dot(wi, wh) * cos_theta(wi) <= 0) return 0.0f;
+            float jac = 1.0f / (4.0f * abs_dot(wh, wi));
+            float F = fresnel_dielectric(dot(wo, wh), ei, et);
+            pdf = d.pdf(wo, wh) * jac * F;
+        } else {
+            float eta = et / ei;
+            V3 wh = normalize(wo + wi * eta);
+            if (cos_theta(wh) < 0) wh = -wh;
+            if (dot(wo, wh) * dot(wi, wh) > 0) return 0.0f;
+            if (dot(wo, wh) * cos_theta(wo) <= 0 || dot(wi, wh) * cos_theta(wi) <= 0) return 0.0f;
+            float sd = dot(wo, wh) + eta * dot(wi, wh);
+            float dwh = std::fabs((eta * eta * dot(wi, wh)) / (sd * sd));
+            float F = fresnel_dielectric(dot(wo, wh), ei, et);
+            pdf = d.pdf(wo, wh) * dwh * (1 - F);
+        }
+        return std::isinf(pdf) || std::isnan(pdf) ? 0.0f : pdf;
+    }
+    default:
+        return 0.0f;
+    }
+}
+
+static BSample material_sample(const cr_scene& s, const rd_material_desc& m, V3 wo, V2 uv, Rng& rng) {
+    BSample out;
+    switch (m.type) {
+    case RD_MAT_DIFFUSE: {                                           // Diffuse.cs:39-49
+        if (cos_theta(wo) <= 0) { if (!m.two_sided) return out; }
+        C3 R = tex_at(s, m.tex[0], uv);
+        float ux = rng.next_float(), uy = rng.next_float();         // Lambert.Sample :26-36
+        V3 wi = normalize(square_to_cosine_hemisphere(ux, uy));
+        if (wo.z < 0) wi.z *= -1;
+        if (!same_hemisphere(wo, wi)) return out;
+        out.wi = wi;
+        out.pdf = same_hemisphere(wo, wi) ? abs_cos_theta(wi) / PI_F : 0.0f;
+        out.fr = R / PI_F;
+        out.type = T_REFLECTION | T_DIFFUSE;
+        return out;
+    }
+    case RD_MAT_ROUGH_METAL: {                                       // RoughMetal.cs:87-112
+        float rough = tex_at(s, m.tex[0], uv).r, an = tex_at(s, m.tex[1], uv).r;
+        MfDist d(m.dist, rough, an);
+        if (cos_theta(wo) <= 0) { if (!m.two_sided) return out; }
+        // MicrofacetReflectionBrdf.Sample (MicrofacetReflectionBrdf.cs:50-59)
+        if (cos_theta(wo) == 0) return out;
+        V3 wh = d.sample_wh(wo, rng);
+        if (dot(wo, wh) < 0) return out;
+        V3 wi = reflect(-wo, wh);
+        if (!same_hemisphere(wo, wi)) return out;
+        out.pdf = d.pdf(wo, wh) / (4 * dot(wi, wh));
+        out.fr = mf_reflect_fr(d, C3(m.eta[0], m.eta[1], m.eta[2]), C3(m.k[0], m.k[1], m.k[2]), wo, wi);
+        out.wi = wi;
+        out.type = T_REFLECTION | T_GLOSSY;
+        return out;
+    }
+    case RD_MAT_ROUGH_GLASS: {                                       // RoughGlass.cs:240-297
+        float rough = tex_at(s, m.tex[2], uv).r, an = tex_at(s, m.tex[3], uv).r;
+        MfDist d(m.dist, rough, an);
+        C3 r = tex_at(s, m.tex[0], uv), t = tex_at(s, m.tex[1], uv);
+        V3 wh = d.sample_wh(wo, rng);
+        float F = fresnel_dielectric(dot(wo, wh), m.eta_a, m.eta_b);
+        if (rng.next_float() < F) {
+            V3 wi = reflect(-wo, wh);
+            if (!same_hemisphere(wo, wi)) return out;
+            float D = d.D(wh);
+            float G = d.G(wo, wi);
+            C3 brdf = r * (D * G * F / (4.0f * cos_theta(wi) * cos_theta(wo)));
+            float jac = 1.0f / (4.0f * abs_dot(wh, wi));
+            float pdf = d.pdf(wo, wh) * jac * F;
+            if (std::isinf(pdf) || std::isnan(pdf) || !brdf.valid()) return out;
+            out.wi = wi; out.fr = brdf; out.pdf = pdf; out.type = T_REFLECTION | T_GLOSSY;
+            return out;
+        }
+        bool entering = cos_theta(wo) > 0;
+        float ei = entering ? m.eta_a : m.eta_b;
+        float et = entering ? m.eta_b : m.eta_a;
+        float eta = et / ei;
+        V3 wi;
+        if (refract(wo, wh, ei / et, &wi)) {
+            if (dot(wo, wh) * dot(wi, wh) > 0) return out;
+        } else {
+            wi = reflect(-wo, wh);
+            eta = 1;
+        }
+        float sd = dot(wo, wh) + eta * dot(wi, wh);
+        float factor = 1 / eta;
+        float D = d.D(wh);
+        float G = d.G(wo, wi);
+        C3 btdf = (1 - F) * t *
+                  std::fabs(D * G * eta * eta * abs_dot(wi, wh) * abs_dot(wo, wh) * factor * factor /
+                            (cos_theta(wo) * cos_theta(wi) * sd * sd));
+        float dwh = std::fabs((eta * eta * dot(wi, wh)) / (sd * sd));
+        float pdf = d.pdf(wo, wh) * dwh * (1 - F);
+        if (std::isinf(pdf) || std::isnan(pdf) || !btdf.valid()) return out;
+        out.wi = wi; out.fr = btdf; out.pdf = pdf; out.type = T_TRANSMISSION | T_GLOSSY;
+        return out;
+    }
+    default:
+        return out;
+    }
+}
+
+// ---------------------------------------------------------------- lights
+struct LSample { V3 p, wi; float pdf = 0; C3 li; };
+
+// Shape.Sample for the emitting primitive: Triangle.Sample (Triangle.cs:184-212),
+// Sphere.Sample (Sphere.cs:87-102).
+static void shape_sample(const cr_scene& s, int prim, Rng& rng, V3* P, V3* N, float* pdf) {
+    const rd_prim_desc& pd = s.prims[prim];
+    float ux = rng.next_float(), uy = rng.next_float();
+    if (pd.kind == RD_PRIM_TRIANGLE) {
+        int t = pd.index;
+        float alpha = 1 - std::sqrt(1 - ux);
+        float beta = uy * std::sqrt(1 - ux);
+        V3 bary(alpha, beta, (1 - alpha - beta));
+        V3 A = s.pa(t), B = s.pb(t), C = s.pc(t);
+        *P = bary.x * A + bary.y * B + bary.z * C;
+        if (s.has_n(t)) *N = normalize(bary.x * s.nrm[s.idx[3 * t]] + bary.y * s.nrm[s.idx[3 * t + 1]] + bary.z * s.nrm[s.idx[3 * t + 2]]);
+        else *N = normalize(cross(B - A, C - A));
+        *pdf = 1.0f / s.prim_area[prim];
+    } else {
+        const float* S = &s.sph[4 * pd.index];
+        // Probability.SquareToUniformSphere (Probability.cs:62-71)
+        float phi = ux * PI_F * 2;
+        float theta = std::acos(1 - 2 * uy);
+        V3 v(std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi), std::cos(theta));
+        V3 n = normalize(v);
+        *P = n * S[3] + V3(S[0], S[1], S[2]);
+        *N = n;
+        *pdf = 1 / s.prim_area[prim];
+    }
+}
+
+// DiffuseAreaLight.SampleLi (Lights/DiffuseAreaLight.cs:48-68)
+static LSample light_sample_li(const cr_scene& s, const rd_light_desc& L, V3 refp, Rng& rng) {
+    LSample out;
+    V3 P, N;
+    float spdf;
+    shape_sample(s, L.prim, rng, &P, &N, &spdf);
+    V3 wi = P - refp;
+    C3 le(L.le[0], L.le[1], L.le[2]);
+    out.li = dot(N, -wi) > 0 ? le : C3(0.0f);
+    if (length_sq(wi) == 0) {
+        out.pdf = 0.0f;
+    } else {
+        wi = normalize(wi);
+        out.pdf = spdf * length_sq(refp - P) / abs_dot(N, -wi);
+        if (std::isinf(out.pdf)) out.pdf = 0.0f;
+    }
+    out.p = P;
+    out.wi = wi;
+    return out;
+}
+// DiffuseAreaLight.PdfLi (DiffuseAreaLight.cs:31-46): intersects the light's
+// own shape only.
+static float light_pdf_li(const cr_scene& s, const rd_light_desc& L, V3 refp, V3 wi) {
+    Ray ray(refp, wi);
+    SurfPt sp;
+    if (!prim_hit(s, L.prim, ray, &sp)) return 0.0f;
+    Inct hit = prim_inct(s, L.prim, ray, sp);
+    float pdf = length_sq(refp - hit.P) / (abs_dot(hit.N(), -wi) * s.prim_area[L.prim]);
+    if (std::isinf(pdf)) pdf = 0.0f;
+    return pdf;
+}
+// Intersection.Le -> DiffuseAreaLight.L (DiffuseAreaLight.cs:26-29)
+static C3 inct_le(const cr_scene& s, const Inct& it, V3 w) {
+    int l = s.prims[it.prim].light;
+    if (l < 0) return C3(0.0f);
+    const rd_light_desc& L = s.lights[l];
+    return dot(it.N(), w) > 0 ? C3(L.le[0], L.le[1], L.le[2]) : C3(0.0f);
+}
+// Scene.EvalAllInfiniteLights (Scene.cs:37-45).  Infinite lights are not
+// part of this restatement yet; scenes with them are rejected at creation.
+static C3 eval_infinite(const cr_scene&, const Ray&) { return C3(0.0f); }
+
+// PathUtility.PowerHeuristic (Integrators/PathUtility.cs:5-9), in double
+static inline float power_heuristic(int nf, float fp, int ng, float gp) {
+    double f = nf * (double)fp, g = ng * (double)gp;
+    return (float)((f * f) / (f * f + g * g));
+}
+
+// ---------------------------------------------------------------- path
+static inline Ray spawn_ray(const Inct& it, V3 d) { return Ray(it.P, d, 0.0001f); }  // Intersection.cs:48-51
+
+// PathTracer.Mis.EstimateDirect (Integrators/PathTracer.cs:309-399)
+static C3 estimate_direct(const cr_scene& s, Rng& rng, int light, const Inct& it, uint64_t* st) {
+    const rd_light_desc& L = s.lights[light];
+    const rd_material_desc& m = s.mats[s.prims[it.prim].material];
+    V3 wo = it.wr;
+    C3 le(0.0f);
+    LSample ls = light_sample_li(s, L, it.P, rng);
+    if (ls.pdf > 0.0f && !ls.li.black()) {
+        C3 fr = material_fr(s, m, it.sh.to_local(wo), it.sh.to_local(ls.wi), it.UV);
+        float spdf = material_pdf(s, m, it.sh.to_local(wo), it.sh.to_local(ls.wi), it.UV);
+        if (!fr.black()) {
+            if (!scene_is_occluded(s, it.P, ls.p, st)) {
+                fr = fr * abs_cos_theta(it.sh.to_local(ls.wi));
+                float w = power_heuristic(1, ls.pdf, 1, spdf);        // area lights are not delta
+                le = le + fr * ls.li * w / ls.pdf;
+            }
+        }
+    }
+    BSample bs = material_sample(s, m, it.sh.to_local(wo), it.UV, rng);
+    C3 fr = bs.fr * abs_cos_theta(bs.wi);
+    float spdf = bs.pdf;
+    bool sampled_specular = (bs.type & T_SPECULAR) != 0;
+    if (!fr.black() && spdf > 0) {
+        float w = 1;
+        if (!sampled_specular) {
+            float lpdf = light_pdf_li(s, L, it.P, it.sh.to_world(bs.wi));
+            if (lpdf == 0.0f) return le;
+            w = power_heuristic(1, spdf, 1, lpdf);
+        }
+        Ray to_light = spawn_ray(it, it.sh.to_world(bs.wi));
+        Inct li_it;
+        bool hit = scene_intersect(s, to_light, &li_it, st);
+        C3 li(0);
+        if (hit) {
+            if (s.prims[li_it.prim].light == light) li = inct_le(s, li_it, it.sh.to_world(-bs.wi));
+        } else {
+            li = eval_infinite(s, to_light);
+        }
+        if (!li.black()) le = le + fr * li * w / spdf;
+    }
+    return le;
+}
+
+// PathTracer.Mis.SampleLightToEstimateDirect (PathTracer.cs:273-307)
+static C3 sample_light_direct(const cr_scene& s, Rng& rng, const Inct& it, uint64_t* st) {
+    C3 result;
+    if (s.integ.strategy == RD_STRATEGY_UNIFORM) {
+        if (s.lights.empty()) return result;                       // Scene.SampleLight (Scene.cs:47-57)
+        int l = rng.next_int((int)s.lights.size());
+        float lpdf = 1.0f / s.lights.size();
+        if (lpdf > 0.0f) result = result + estimate_direct(s, rng, l, it, st) / lpdf;
+    } else {
+        for (int l = 0; l < (int)s.lights.size(); l++) result = result + estimate_direct(s, rng, l, it, st);
+    }
+    return result;
+}
+
+// PathTracer.Mis (PathTracer.cs:192-270)
+static C3 li_mis(const cr_scene& s, Ray ray, Rng& rng, uint64_t* st) {
+    C3 radiance(0.0f), coeff(1.0f);
+    bool specular_path = true;
+    const rd_integrator_desc& I = s.integ;
+    for (int bounces = 0;; bounces++) {
+        Inct it;
+        bool hit = scene_intersect(s, ray, &it, st);
+        V3 wo = -ray.d;
+        if (bounces == 0 || specular_path) {
+            if (hit) radiance = radiance + coeff * inct_le(s, it, wo);
+            else radiance = radiance + coeff * eval_infinite(s, ray);
+        }
+        if (!hit) break;
+        if (I.max_depth != -1 && bounces >= I.max_depth) break;
+        int mat = s.prims[it.prim].material;
+        if (mat < 0) {
+            ray = Ray(it.P, ray.d, ray.tmin);
+            bounces--;
+            continue;
+        }
+        if (st) st[3]++;
+        radiance = radiance + coeff * sample_light_direct(s, rng, it, st);
+        BSample bs = material_sample(s, s.mats[mat], it.sh.to_local(wo), it.UV, rng);
+        specular_path = (bs.type & T_SPECULAR) != 0;
+        if (bs.pdf > 0.0f) coeff = coeff * (bs.fr * abs_cos_theta(bs.wi) / bs.pdf);
+        else break;
+        ray = spawn_ray(it, it.sh.to_world(bs.wi));
+        if (bounces > I.min_depth) {
+            float q = nmin(coeff.max_elem(), I.rr_threshold);
+            if (rng.next_float() > q) break;
+            coeff = coeff / q;
+        }
+    }
+    return radiance;
+}
+
+// PathTracer.NextEventEstimation (PathTracer.cs:95-190)
+static C3 li_nee(const cr_scene& s, Ray ray, Rng& rng, uint64_t* st) {
+    C3 radiance(0.0f), coeff(1.0f);
+    bool specular_path = true;
+    const rd_integrator_desc& I = s.integ;
+    for (int bounces = 0;; bounces++) {
+        Inct it;
+        bool hit = scene_intersect(s, ray, &it, st);
+        V3 wo = -ray.d;
+        if (bounces == 0 || specular_path) {
+            if (hit) radiance = radiance + coeff * inct_le(s, it, wo);
+            else radiance = radiance + coeff * eval_infinite(s, ray);
+        }
+        if (!hit) break;
+        if (I.max_depth != -1 && bounces >= I.max_depth) break;
+        int mat = s.prims[it.prim].material;
+        if (mat < 0) { ray = Ray(it.P, ray.d, ray.tmin); bounces--; continue; }
+        if (st) st[3]++;
+        const rd_material_desc& m = s.mats[mat];
+        auto estimate_light = [&](int l) -> C3 {                   // EstimateLight :117-129
+            LSample ls = light_sample_li(s, s.lights[l], it.P, rng);
+            if (ls.pdf <= 0.0f) return C3(0);
+            if (scene_is_occluded(s, it.P, ls.p, st)) return C3(0.0f);
+            C3 fr = material_fr(s, m, it.sh.to_local(wo), it.sh.to_local(ls.wi), it.UV);
+            return fr * ls.li * abs_cos_theta(it.sh.to_local(ls.wi)) / ls.pdf;
+        };
+        if (I.strategy == RD_STRATEGY_UNIFORM) {
+            if (!s.lights.empty()) {
+                int l = rng.next_int((int)s.lights.size());
+                float lpdf = 1.0f / s.lights.size();
+                if (lpdf > 0.0f) radiance = radiance + coeff * estimate_light(l) / lpdf;
+            }
+        } else {
+            for (int l = 0; l < (int)s.lights.size(); l++) radiance = radiance + coeff * estimate_light(l);
+        }
+        BSample bs = material_sample(s, m, it.sh.to_local(wo), it.UV, rng);
+        specular_path = (bs.type & T_SPECULAR) != 0;
+        if (bs.pdf > 0.0f) coeff = coeff * (bs.fr * abs_cos_theta(bs.wi) / bs.pdf);
+        else break;
+        ray = spawn_ray(it, it.sh.to_world(bs.wi));
+        if (bounces > I.min_depth) {
+            float q = nmin(coeff.max_elem(), I.rr_threshold);
+            if (rng.next_float() > q) break;
+            coeff = coeff / q;
+        }
+    }
+    return radiance;
+}
+
+// PathTracer.OnlySampleBxdf (PathTracer.cs:43-93)
+static C3 li_bxdf(const cr_scene& s, Ray ray, Rng& rng, uint64_t* st) {
+    C3 radiance(0.0f), coeff(1.0f);
+    const rd_integrator_desc& I = s.integ;
+    for (int bounces = 0;; bounces++) {
+        Inct it;
+        if (!scene_intersect(s, ray, &it, st)) {
+            radiance = radiance + coeff * eval_infinite(s, ray);
+            break;
+        }
+        if (I.max_depth != -1 && bounces >= I.max_depth) break;
+        int mat = s.prims[it.prim].material;
+        if (mat < 0) { ray = Ray(it.P, ray.d, ray.tmin); bounces--; continue; }
+        if (st) st[3]++;
+        V3 wo = -ray.d;
+        if (s.prims[it.prim].light >= 0) radiance = radiance + coeff * inct_le(s, it, wo);
+        BSample bs = material_sample(s, s.mats[mat], it.sh.to_local(-ray.d), it.UV, rng);
+        if (bs.pdf > 0.0f) coeff = coeff * (bs.fr * abs_cos_theta(bs.wi) / bs.pdf);
+        else break;
+        ray = spawn_ray(it, it.sh.to_world(bs.wi));
+        if (bounces > I.min_depth) {
+            float q = nmin(coeff.max_elem(), I.rr_threshold);
+            if (rng.next_float() > q) break;
+            coeff = coeff / q;
+        }
+    }
+    return radiance;
+}
+
+static C3 li(const cr_scene& s, const Ray& ray, Rng& rng, uint64_t* st) {
+    switch (s.integ.method) {
+    case RD_METHOD_BXDF: return li_bxdf(s, ray, rng, st);
+    case RD_METHOD_NEE: return li_nee(s, ray, rng, st);
+    default: return li_mis(s, ray, rng, st);
+    }
+}
+
+// One pixel sample: BlockBasedRenderer.RenderTask body (BlockBasedRenderer.cs:136-146).
+static C3 pixel_sample(const cr_scene& s, uint32_t seed, int x, int y, int t, uint64_t* st) {
+    Rng rng;
+    rng.key = path_key(seed, (uint32_t)(y * s.cam.W + x), (uint32_t)t);
+    rng.n = 0;
+    float ox = rng.next_float(), oy = rng.next_float();
+    Ray ray = s.cam.sample_ray((float)x + ox, (float)y + oy);
+    if (st) st[2]++;
+    return li(s, ray, rng, st);
+}
+
+// ---------------------------------------------------------------- C API
+extern "C" {
+
+const char* cr_last_error(void) { return g_err.c_str(); }
+
+cr_scene* cr_scene_create(const rd_scene_desc* d) {
+    if (!d) { set_err("null desc"); return nullptr; }
+    cr_scene* s = new cr_scene();
+    s->pos.resize(d->num_vertices);
+    for (int i = 0; i < d->num_vertices; i++) s->pos[i] = V3(d->positions[3 * i], d->positions[3 * i + 1], d->positions[3 * i + 2]);
+    if (d->normals) {
+        s->nrm.resize(d->num_vertices);
+        for (int i = 0; i < d->num_vertices; i++) s->nrm[i] = V3(d->normals[3 * i], d->normals[3 * i + 1], d->normals[3 * i + 2]);
+    }
+    if (d->uvs) {
+        s->uvs.resize(d->num_vertices);
+        for (int i = 0; i < d->num_vertices; i++) s->uvs[i] = V2(d->uvs[2 * i], d->uvs[2 * i + 1]);
+    }
+    s->idx.assign(d->indices, d->indices + 3 * (size_t)d->num_triangles);
+    s->tflags.assign(d->num_triangles, 0);
+    if (d->tri_flags) s->tflags.assign(d->tri_flags, d->tri_flags + d->num_triangles);
+    for (int t = 0; t < d->num_triangles; t++) {
+        if ((s->tflags[t] & 1) && s->nrm.empty()) { set_err("triangle flags normals but no normals"); delete s; return nullptr; }
+        if ((s->tflags[t] & 2) && s->uvs.empty()) { set_err("triangle flags uvs but no uvs"); delete s; return nullptr; }
+        for (int k = 0; k < 3; k++) {
+            int v = s->idx[3 * t + k];
+            if (v < 0 || v >= d->num_vertices) { set_err("index out of range"); delete s; return nullptr; }
+        }
+    }
+    if (d->num_spheres) s->sph.assign(d->spheres, d->spheres + 4 * (size_t)d->num_spheres);
+    s->prims.assign(d->prims, d->prims + d->num_prims);
+    s->mats.assign(d->materials, d->materials + d->num_materials);
+    s->lights.assign(d->lights, d->lights + d->num_lights);
+    if (d->num_media) s->media.assign(d->media, d->media + d->num_media);
+    s->global_medium = d->global_medium;
+    s->max_prim = d->bvh_max_prim > 0 ? d->bvh_max_prim : 1;
+    s->integ = d->integrator;
+    s->tex.resize(d->num_textures);
+    for (int i = 0; i < d->num_textures; i++) {
+        const rd_texture_desc& td = d->textures[i];
+        Tex& t = s->tex[i];
+        t.w = td.width; t.h = td.height; t.wrap = td.wrap; t.filter = td.filter;
+        t.value = C3(td.value[0], td.value[1], td.value[2]);
+        if (t.w > 0) {
+            if (!td.data || t.h <= 0) { set_err("image texture without data"); delete s; return nullptr; }
+            t.data.assign(td.data, td.data + (size_t)t.w * t.h * 3);
+        }
+    }
+    s->prim_area.resize(s->prims.size());
+    for (size_t p = 0; p < s->prims.size(); p++) {
+        const rd_prim_desc& pd = s->prims[p];
+        if (pd.kind == RD_PRIM_TRIANGLE) {
+            if (pd.index < 0 || pd.index >= d->num_triangles) { set_err("prim triangle index"); delete s; return nullptr; }
+            int t = pd.index;
+            s->prim_area[p] = 0.5f * length(cross(s->pb(t) - s->pa(t), s->pc(t) - s->pa(t)));   // Triangle.cs:93
+        } else if (pd.kind == RD_PRIM_SPHERE) {
+            if (pd.index < 0 || pd.index >= d->num_spheres) { set_err("prim sphere index"); delete s; return nullptr; }
+            float r = s->sph[4 * pd.index + 3];
+            s->prim_area[p] = 4 * r * r * PI_F;                                                   // Sphere.cs:18
+        } else { set_err("prim kind"); delete s; return nullptr; }
+        if (pd.material >= d->num_materials || pd.light >= d->num_lights) { set_err("prim material/light index"); delete s; return nullptr; }
+    }
+    for (int i = 0; i < d->num_materials; i++) {
+        int ty = s->mats[i].type;
+        if (ty != RD_MAT_DIFFUSE && ty != RD_MAT_ROUGH_METAL && ty != RD_MAT_ROUGH_GLASS) {
+            set_err("material type not in this restatement yet"); delete s; return nullptr;
+        }
+    }
+    for (int i = 0; i < d->num_lights; i++) {
+        if (s->lights[i].type != RD_LIGHT_AREA) { set_err("infinite lights not in this restatement yet"); delete s; return nullptr; }
+        int p = s->lights[i].prim;
+        if (p < 0 || p >= d->num_prims) { set_err("light prim index"); delete s; return nullptr; }
+    }
+    if (d->integrator.type != RD_INTEGRATOR_PATH) { set_err("vol_path not in this restatement yet"); delete s; return nullptr; }
+    if (!s->cam.init(d->camera)) { set_err("camera matrix not invertible"); delete s; return nullptr; }
+    build_bvh(*s);
+    return s;
+}
+
+void cr_scene_destroy(cr_scene* s) { delete s; }
+int32_t cr_scene_num_nodes(const cr_scene* s) { return s ? (int32_t)s->nodes.size() : -1; }
+int32_t cr_scene_max_depth(const cr_scene* s) { return s ? s->max_depth : -1; }
+
+int cr_scene_export_bvh(const cr_scene* s, void* nodes32, int32_t* prim_order) {
+    if (!s) return RD_ERR_INVALID;
+    unsigned char* out = (unsigned char*)nodes32;
+    for (size_t i = 0; i < s->nodes.size(); i++) {
+        const Node& n = s->nodes[i];
+        float b[6] = {n.b.mn.x, n.b.mn.y, n.b.mn.z, n.b.mx.x, n.b.mx.y, n.b.mx.z};
+        unsigned char rec[32];
+        memset(rec, 0, 32);
+        memcpy(rec, b, 24);
+        memcpy(rec + 24, &n.offset, 4);
+        memcpy(rec + 28, &n.count, 2);
+        memcpy(rec + 30, &n.axis, 1);
+        memcpy(out + 32 * i, rec, 32);
+    }
+    if (prim_order) memcpy(prim_order, s->order.data(), s->order.size() * sizeof(int32_t));
+    return RD_OK;
+}
+
+static inline Ray to_ray(const rd_ray& r) {
+    return Ray(V3(r.o[0], r.o[1], r.o[2]), V3(r.d[0], r.d[1], r.d[2]), r.tmin, r.tmax);
+}
+
+int cr_trace_closest(const cr_scene* s, const rd_ray* rays, int32_t n, rd_hit* hits, uint64_t* counters) {
+    if (!s) return RD_ERR_INVALID;
+    for (int i = 0; i < n; i++) {
+        Ray r = to_ray(rays[i]);
+        int p = -1;
+        SurfPt sp{0, 0, 0};
+        if (bvh_closest(*s, r, &p, &sp, counters)) { hits[i].t = sp.t; hits[i].prim = p; hits[i].u = sp.u; hits[i].v = sp.v; }
+        else { hits[i].t = 0; hits[i].prim = -1; hits[i].u = 0; hits[i].v = 0; }
+    }
+    return RD_OK;
+}
+
+int cr_trace_any(const cr_scene* s, const rd_ray* rays, int32_t n, uint8_t* occ, uint64_t* counters) {
+    if (!s) return RD_ERR_INVALID;
+    for (int i = 0; i < n; i++) occ[i] = bvh_any(*s, to_ray(rays[i]), counters) ? 1 : 0;
+    return RD_OK;
+}
+
+int cr_trace_brute(const cr_scene* s, const rd_ray* rays, int32_t n, rd_hit* hits) {
+    if (!s) return RD_ERR_INVALID;
+    for (int i = 0; i < n; i++) {
+        Ray r = to_ray(rays[i]);
+        hits[i].prim = -1; hits[i].t = 0; hits[i].u = 0; hits[i].v = 0;
+        float best = FLT_MAX;
+        for (int p = 0; p < (int)s->prims.size(); p++) {
+            SurfPt sp;
+            if (prim_hit(*s, p, r, &sp) && sp.t < best) {
+                best = sp.t; hits[i].t = sp.t; hits[i].prim = p; hits[i].u = sp.u; hits[i].v = sp.v;
+            }
+        }
+    }
+    return RD_OK;
+}
+
+int cr_camera_rays(const cr_scene* s, const float* xy, int32_t n, rd_ray* rays) {
+    if (!s) return RD_ERR_INVALID;
+    for (int i = 0; i < n; i++) {
+        Ray r = s->cam.sample_ray(xy[2 * i], xy[2 * i + 1]);
+        rays[i].o[0] = r.o.x; rays[i].o[1] = r.o.y; rays[i].o[2] = r.o.z;
+        rays[i].d[0] = r.d.x; rays[i].d[1] = r.d.y; rays[i].d[2] = r.d.z;
+        rays[i].tmin = r.tmin; rays[i].tmax = r.tmax;
+    }
+    return RD_OK;
+}
+
+int cr_render(const cr_scene* s, const rd_render_params* p, int32_t threads, float* out_rgb,
+              float* out_sum2, uint64_t* stats, double* seconds) {
+    if (!s || !p || !out_rgb) return RD_ERR_INVALID;
+    int W = s->cam.W, H = s->cam.H;
+    int x0 = p->x0, y0 = p->y0, x1 = p->x1, y1 = p->y1;
+    if (x0 == 0 && y0 == 0 && x1 == 0 && y1 == 0) { x1 = W; y1 = H; }
+    if (x0 < 0 || y0 < 0 || x1 > W || y1 > H || x0 >= x1 || y0 >= y1 || p->spp <= 0) { set_err("bad crop/spp"); return RD_ERR_INVALID; }
+    // 32x32 blocks over the crop (RenderBlockGenerator.cs; scheduling only:
+    // results do not depend on block order because the RNG is keyed per sample).
+    const int bs = 32;
+    int nbx = (x1 - x0 + bs - 1) / bs, nby = (y1 - y0 + bs - 1) / bs;
+    int nblocks = nbx * nby;
+    std::atomic<int> next(0);
+    std::mutex mu;
+    uint64_t tot[5] = {0, 0, 0, 0, 0};
+    if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
+    auto t0 = std::chrono::steady_clock::now();
+    auto worker = [&]() {
+        uint64_t st[5] = {0, 0, 0, 0, 0};
+        while (true) {
+            int b = next.fetch_add(1);
+            if (b >= nblocks) break;
+            int bx = x0 + (b % nbx) * bs, by = y0 + (b / nbx) * bs;
+            int ex = std::min(bx + bs, x1), ey = std::min(by + bs, y1);
+            for (int x = bx; x < ex; x++) {
+                for (int y = by; y < ey; y++) {
+                    C3 sum(0.0f), sum2(0.0f);
+                    for (int t = 0; t < p->spp; t++) {
+                        C3 c = pixel_sample(*s, p->seed, x, y, t, st);
+                        if (!c.valid()) { c = C3(0.0f); st[4]++; }
+                        sum = sum + c;
+                        sum2 = sum2 + c * c;
+                    }
+                    size_t o = ((size_t)x * H + y) * 3;
+                    C3 mean = sum / (float)p->spp;
+                    out_rgb[o] = mean.r; out_rgb[o + 1] = mean.g; out_rgb[o + 2] = mean.b;
+                    if (out_sum2) { out_sum2[o] = sum2.r; out_sum2[o + 1] = sum2.g; out_sum2[o + 2] = sum2.b; }
+                }
+            }
+        }
+        std::lock_guard<std::mutex> g(mu);
+        for (int i = 0; i < 5; i++) tot[i] += st[i];
+    };
+    std::vector<std::thread> pool;
+    for (int i = 0; i < threads; i++) pool.emplace_back(worker);
+    for (auto& th : pool) th.join();
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (stats) for (int i = 0; i < 5; i++) stats[i] = tot[i];
+    return RD_OK;
+}
+
+int cr_sample_li(const cr_scene* s, uint32_t seed, int32_t x, int32_t y, int32_t sample, float* rgb) {
+    if (!s) return RD_ERR_INVALID;
+    C3 c = pixel_sample(*s, seed, x, y, sample, nullptr);
+    rgb[0] = c.r; rgb[1] = c.g; rgb[2] = c.b;
+    return RD_OK;
+}
+
+int cr_material_eval(const cr_scene* s, int32_t material, const float* wo, const float* wi, const float* uv,
+                     float* fr_rgb, float* pdf) {
+    if (!s || material < 0 || material >= (int)s->mats.size()) return RD_ERR_INVALID;
+    const rd_material_desc& m = s->mats[material];
+    V3 o(wo[0], wo[1], wo[2]), i(wi[0], wi[1], wi[2]);
+    V2 u(uv[0], uv[1]);
+    C3 f = material_fr(*s, m, o, i, u);
+    fr_rgb[0] = f.r; fr_rgb[1] = f.g; fr_rgb[2] = f.b;
+    *pdf = material_pdf(*s, m, o, i, u);
+    return RD_OK;
+}
+
+int cr_material_sample(const cr_scene* s, int32_t material, const float* wo, const float* uv, uint64_t key,
+                       float* wi, float* fr_rgb, float* pdf, int32_t* type) {
+    if (!s || material < 0 || material >= (int)s->mats.size()) return RD_ERR_INVALID;
+    Rng rng; rng.key = key; rng.n = 0;
+    BSample b = material_sample(*s, s->mats[material], V3(wo[0], wo[1], wo[2]), V2(uv[0], uv[1]), rng);
+    wi[0] = b.wi.x; wi[1] = b.wi.y; wi[2] = b.wi.z;
+    fr_rgb[0] = b.fr.r; fr_rgb[1] = b.fr.g; fr_rgb[2] = b.fr.b;
+    *pdf = b.pdf;
+    *type = b.type;
+    return RD_OK;
+}
+
+float cr_fresnel_dielectric(float ci, float ei, float et) { return fresnel_dielectric(ci, ei, et); }
+
+void cr_fresnel_conductor(float ci, const float* eta, const float* k, float* out) {
+    C3 f = fresnel_conductor(ci, C3(1.0f), C3(eta[0], eta[1], eta[2]), C3(k[0], k[1], k[2]));
+    out[0] = f.r; out[1] = f.g; out[2] = f.b;
+}
+
+void cr_mf_eval(int32_t kind, float roughness, float aniso, const float* wh, const float* wo, const float* wi, float* out) {
+    MfDist d(kind, roughness, aniso);
+    V3 h(wh[0], wh[1], wh[2]), o(wo[0], wo[1], wo[2]), i(wi[0], wi[1], wi[2]);
+    out[0] = d.D(h);
+    out[1] = d.G(o, i);
+    out[2] = d.pdf(o, h);
+}
+
+void cr_mf_sample(int32_t kind, float roughness, float aniso, const float* wo, uint64_t key, float* wh) {
+    MfDist d(kind, roughness, aniso);
+    Rng rng; rng.key = key; rng.n = 0;
+    V3 h = d.sample_wh(V3(wo[0], wo[1], wo[2]), rng);
+    wh[0] = h.x; wh[1] = h.y; wh[2] = h.z;
+}
+
+void cr_rng_floats(uint64_t key, int32_t n, float* out) {
+    Rng rng; rng.key = key; rng.n = 0;
+    for (int i = 0; i < n; i++) out[i] = rng.next_float();
+}
+
+uint64_t cr_path_key(uint32_t seed, uint32_t pixel, uint32_t sample) { return path_key(seed, pixel, sample); }
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+/* cpu_ref.h — TEST INFRASTRUCTURE: C API of the oracle ("cpu_ref"), a scalar
+ * C++ restatement of the reference's path / vol_path hot path
+ * (ksgfk/Radiantium, C#/.NET 6).  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it; the product never links it.
+ * It consumes the same rd_scene_desc as the product's C ABI. */
+#ifndef CPU_REF_H
+#define CPU_REF_H
+#include <stdint.h>
+#include "../include/radiantium_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cr_scene cr_scene;
+
+const char* cr_last_error(void);
+cr_scene* cr_scene_create(const rd_scene_desc* desc);
+void cr_scene_destroy(cr_scene* s);
+/* nodes32: num_nodes*32 bytes in LinearBVHNode layout; prim_order: BVH order
+ * of primitive indices.  Returns the node count (or -1). */
+int32_t cr_scene_num_nodes(const cr_scene* s);
+int32_t cr_scene_max_depth(const cr_scene* s);
+int cr_scene_export_bvh(const cr_scene* s, void* nodes32, int32_t* prim_order);
+/* counters (may be NULL): [0] nodes visited (AABB tests), [1] primitives tested */
+int cr_trace_closest(const cr_scene* s, const rd_ray* rays, int32_t n, rd_hit* hits, uint64_t* counters);
+int cr_trace_any(const cr_scene* s, const rd_ray* rays, int32_t n, uint8_t* occluded, uint64_t* counters);
+/* Brute force over all primitives in input order (the Naive aggregate,
+ * Accelerators/Naive.cs): smallest t, first index on ties. */
+int cr_trace_brute(const cr_scene* s, const rd_ray* rays, int32_t n, rd_hit* hits);
+/* Camera.SampleRay for pixel sample positions (x,y) (already offset by U^2). */
+int cr_camera_rays(const cr_scene* s, const float* xy, int32_t n, rd_ray* rays);
+/* Renders like BlockBasedRenderer with `threads` workers over 32x32 blocks
+ * of the crop in params.  out_rgb / out_sum2: full W*H*3 column-major
+ * (pixels outside the crop untouched).  stats: [0] closest queries,
+ * [1] any-hit queries, [2] paths, [3] vertices, [4] invalid samples. */
+int cr_render(const cr_scene* s, const rd_render_params* params, int32_t threads,
+              float* out_rgb, float* out_sum2, uint64_t* stats, double* seconds);
+/* Single-sample radiance for (pixel x, y, sample index): returns the Li
+ * value before the IsValid replacement; for debugging / per-sample tests. */
+int cr_sample_li(const cr_scene* s, uint32_t seed, int32_t x, int32_t y, int32_t sample, float* rgb);
+
+/* BSDF probes in the local shading frame (uv given): value, pdf, sampled
+ * direction, with an explicit RNG key so sampling is reproducible. */
+int cr_material_eval(const cr_scene* s, int32_t material, const float* wo, const float* wi,
+                     const float* uv, float* fr_rgb, float* pdf);
+int cr_material_sample(const cr_scene* s, int32_t material, const float* wo, const float* uv,
+                       uint64_t key, float* wi, float* fr_rgb, float* pdf, int32_t* type);
+
+/* Closed-form probes: Fresnel.DielectricFunc / ConductorFunc, and the
+ * microfacet distribution (kind 0 Beckmann, 1 GGX): out = {D(wh), G(wo,wi),
+ * Pdf(wo,wh)}; sample_wh draws with the given RNG key. */
+float cr_fresnel_dielectric(float cos_i, float eta_i, float eta_t);
+void cr_fresnel_conductor(float cos_i, const float* eta, const float* k, float* out_rgb);
+void cr_mf_eval(int32_t kind, float roughness, float aniso, const float* wh, const float* wo, const float* wi, float* out3);
+void cr_mf_sample(int32_t kind, float roughness, float aniso, const float* wo, uint64_t key, float* wh);
+/* Pixel-sample camera ray for an explicit RNG-free sample position is
+ * cr_camera_rays; this returns the oracle RNG's first n floats for a key. */
+void cr_rng_floats(uint64_t key, int32_t n, float* out);
+uint64_t cr_path_key(uint32_t seed, uint32_t pixel, uint32_t sample);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,742 @@
+// rd_api.cpp — the C ABI (include/radiantium_hip.h): context, scene upload,
+// traversal hooks and the wavefront render loop that replaces
+// BlockBasedRenderer (Radiantium.Offline/Renderers/BlockBasedRenderer.cs).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/radiantium_hip.h"
+#include "rd_bvh_build.h"
+#include "rd_camera.h"
+#include "rd_kernels.h"
+
+using namespace rd;
+
+static thread_local std::string g_err;
+static int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail(RD_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+struct StageTiming {
+    double total_ms = 0;
+    int64_t count = 0;
+};
+
+// Device buffers of the wavefront, reused across renders and grown on demand.
+struct WaveBuffers {
+    size_t cap = 0;      // paths
+    size_t pix_cap = 0;  // accumulation pixels
+    std::vector<void*> allocs;
+    RayQueue qa{}, qb{}, qshadow{}, qmis{};
+    float4* hits = nullptr;
+    PathState ps{};
+    float4* sum = nullptr;
+    float4* sum2 = nullptr;
+    void release() {
+        for (void* p : allocs) (void)hipFree(p);
+        allocs.clear();
+        cap = pix_cap = 0;
+    }
+};
+
+}  // namespace
+
+struct rd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    int* counts = nullptr;        // device counters (kCntSlots ints)
+    int* host_counts = nullptr;   // pinned mirror
+    WaveBuffers wb;
+    rd_render_stats stats{};
+    std::map<std::string, StageTiming> timing;
+    bool timing_enabled = true;
+};
+
+struct rd_scene {
+    rd_ctx* ctx = nullptr;
+    DevScene dev{};
+    DevIntegrator integ{};
+    std::vector<void*> allocs;
+    std::vector<NodeRec> nodes;
+    std::vector<int32_t> order;
+    rd_scene_info info{};
+    int width = 0, height = 0;
+    ~rd_scene() {
+        for (void* p : allocs) (void)hipFree(p);
+    }
+};
+
+namespace {
+
+// RAII "one host thread at a time" guard (the reference's renderer is driven
+// from one thread; concurrent use of a context is rejected, not serialised).
+struct CtxLock {
+    std::unique_lock<std::mutex> lk;
+    explicit CtxLock(rd_ctx* c) : lk(c->mu, std::try_to_lock) {}
+    bool ok() const { return lk.owns_lock(); }
+};
+
+template <class T>
+int upload(rd_scene* s, const T* host, size_t count, const T** dst) {
+    if (count == 0) { *dst = nullptr; return RD_OK; }
+    void* d = nullptr;
+    hipError_t e = hipMalloc(&d, count * sizeof(T));
+    if (e != hipSuccess) return fail(RD_ERR_OOM, "hipMalloc failed for scene buffer");
+    s->allocs.push_back(d);
+    HIP_TRY(hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice));
+    *dst = (const T*)d;
+    return RD_OK;
+}
+
+// HIP-event timing of one launch on the context stream.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    std::string stage;
+};
+
+struct Timer {
+    rd_ctx* ctx;
+    std::vector<EventPair> pairs;
+    std::vector<EventPair> pool;
+    explicit Timer(rd_ctx* c) : ctx(c) {}
+    ~Timer() {
+        for (auto& p : pairs) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+        for (auto& p : pool) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    }
+    EventPair get() {
+        if (!pool.empty()) { EventPair p = pool.back(); pool.pop_back(); return p; }
+        EventPair p;
+        (void)hipEventCreate(&p.a);
+        (void)hipEventCreate(&p.b);
+        return p;
+    }
+    template <class F>
+    hipError_t run(const char* stage, F&& launch) {
+        if (!ctx->timing_enabled) return launch();
+        EventPair p = get();
+        p.stage = stage;
+        (void)hipEventRecord(p.a, ctx->stream);
+        hipError_t e = launch();
+        (void)hipEventRecord(p.b, ctx->stream);
+        pairs.push_back(p);
+        return e;
+    }
+    // Called after a stream synchronize: folds elapsed times into ctx->timing.
+    void collect() {
+        for (auto& p : pairs) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+                StageTiming& t = ctx->timing[p.stage];
+                t.total_ms += ms;
+                t.count++;
+            }
+            pool.push_back(p);
+        }
+        pairs.clear();
+    }
+};
+
+int ensure_wave_buffers(rd_ctx* c, size_t paths, size_t pixels) {
+    WaveBuffers& w = c->wb;
+    if (paths <= w.cap && pixels <= w.pix_cap) return RD_OK;
+    w.release();
+    size_t cap = std::max(paths, (size_t)1024);
+    size_t pcap = std::max(pixels, (size_t)1024);
+    auto alloc = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        w.allocs.push_back(p);
+        return p;
+    };
+    auto mkq = [&](RayQueue& q, int slot) -> bool {
+        q.count = c->counts + slot;
+        q.pid = (int*)alloc(cap * sizeof(int));
+        q.org = (float4*)alloc(cap * sizeof(float4));
+        q.dir = (float4*)alloc(cap * sizeof(float4));
+        return q.pid && q.org && q.dir;
+    };
+    bool ok = mkq(w.qa, kCntExtA) && mkq(w.qb, kCntExtB) && mkq(w.qshadow, kCntShadow) && mkq(w.qmis, kCntMis);
+    w.hits = (float4*)alloc(cap * sizeof(float4));
+    PathState& ps = w.ps;
+    ps.L = (float4*)alloc(cap * sizeof(float4));
+    ps.beta = (float4*)alloc(cap * sizeof(float4));
+    ps.nee_beta = (float4*)alloc(cap * sizeof(float4));
+    ps.nee_le = (float4*)alloc(cap * sizeof(float4));
+    ps.mis_f = (float4*)alloc(cap * sizeof(float4));
+    ps.mis_li = (float4*)alloc(cap * sizeof(float4));
+    ps.flags = (int*)alloc(cap * sizeof(int));
+    ps.sel_light = (int*)alloc(cap * sizeof(int));
+    ps.rng_n = (uint32_t*)alloc(cap * sizeof(uint32_t));
+    ps.depth = (int*)alloc(cap * sizeof(int));
+    ps.vis = (uint8_t*)alloc(cap);
+    w.sum = (float4*)alloc(pcap * sizeof(float4));
+    w.sum2 = (float4*)alloc(pcap * sizeof(float4));
+    ok = ok && w.hits && ps.L && ps.beta && ps.nee_beta && ps.nee_le && ps.mis_f && ps.mis_li && ps.flags &&
+         ps.sel_light && ps.rng_n && ps.depth && ps.vis && w.sum && w.sum2;
+    if (!ok) {
+        w.release();
+        return fail(RD_ERR_OOM, "hipMalloc failed for wavefront buffers");
+    }
+    w.cap = cap;
+    w.pix_cap = pcap;
+    return RD_OK;
+}
+
+int check_stack(const rd_scene* s) {
+    if (s->dev.stack_needed > 64)
+        return fail(RD_ERR_STACK, "BVH deeper than the 64-entry traversal stack (Bvh.StackSize)");
+    return RD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rd_abi_version(void) { return RD_ABI_VERSION; }
+const char* rd_last_error(void) { return g_err.c_str(); }
+
+rd_ctx* rd_create(int device) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) { g_err = "no HIP device"; return nullptr; }
+    if (device < 0 || device >= n) { g_err = "device index out of range"; return nullptr; }
+    if (hipSetDevice(device) != hipSuccess) { g_err = "hipSetDevice failed"; return nullptr; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) { g_err = "hipGetDeviceProperties failed"; return nullptr; }
+    rd_ctx* c = new rd_ctx();
+    c->device = device;
+    set_device_grid(prop.multiProcessorCount);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_err = "hipStreamCreate failed"; delete c; return nullptr;
+    }
+    if (hipMalloc(&c->counts, kCntSlots * sizeof(int)) != hipSuccess ||
+        hipHostMalloc(&c->host_counts, kCntSlots * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        g_err = "counter allocation failed";
+        if (c->counts) (void)hipFree(c->counts);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return nullptr;
+    }
+    (void)hipMemset(c->counts, 0, kCntSlots * sizeof(int));
+    return c;
+}
+
+void rd_destroy(rd_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->wb.release();
+    if (c->counts) (void)hipFree(c->counts);
+    if (c->host_counts) (void)hipHostFree(c->host_counts);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
+    if (!c || !d) { g_err = "null argument"; return nullptr; }
+    CtxLock lock(c);
+    if (!lock.ok()) { g_err = "context in use by another thread"; return nullptr; }
+    (void)hipSetDevice(c->device);
+    auto bad = [](const char* m) -> rd_scene* { g_err = m; return nullptr; };
+    if (d->num_vertices < 0 || d->num_triangles < 0 || d->num_spheres < 0 || d->num_prims < 0) return bad("negative count");
+    if (d->num_triangles > 0 && (!d->positions || !d->indices)) return bad("triangles without positions/indices");
+    if (d->num_spheres > 0 && !d->spheres) return bad("spheres without data");
+    if (d->num_prims > 0 && !d->prims) return bad("prims missing");
+    if (d->camera.width <= 0 || d->camera.height <= 0) return bad("camera screen must be positive");
+    if (d->integrator.type != RD_INTEGRATOR_PATH) return bad("integrator type not built for this path yet (vol_path)");
+    if (d->integrator.strategy != RD_STRATEGY_UNIFORM) return bad("light strategy All is not built for the GPU path yet");
+    for (int t = 0; t < d->num_triangles; t++) {
+        uint8_t fl = d->tri_flags ? d->tri_flags[t] : 0;
+        if ((fl & 1) && !d->normals) return bad("triangle flags normals but none given");
+        if ((fl & 2) && !d->uvs) return bad("triangle flags uvs but none given");
+        for (int k = 0; k < 3; k++) {
+            int v = d->indices[3 * t + k];
+            if (v < 0 || v >= d->num_vertices) return bad("triangle index out of range");
+        }
+    }
+    for (int i = 0; i < d->num_materials; i++) {
+        int ty = d->materials[i].type;
+        if (ty != RD_MAT_DIFFUSE && ty != RD_MAT_ROUGH_METAL && ty != RD_MAT_ROUGH_GLASS)
+            return bad("material type not built for the GPU path yet");
+        for (int k = 0; k < 16; k++) {
+            int tx = d->materials[i].tex[k];
+            if (tx < 0 || (d->num_textures > 0 && tx >= d->num_textures)) return bad("material texture index out of range");
+        }
+        if (d->num_textures == 0) return bad("materials need textures (constants count)");
+    }
+    for (int i = 0; i < d->num_lights; i++) {
+        if (d->lights[i].type != RD_LIGHT_AREA) return bad("infinite lights not built for the GPU path yet");
+        if (d->lights[i].prim < 0 || d->lights[i].prim >= d->num_prims) return bad("light primitive out of range");
+    }
+
+    rd_scene* s = new rd_scene();
+    s->ctx = c;
+    s->width = d->camera.width;
+    s->height = d->camera.height;
+    auto t0 = std::chrono::steady_clock::now();
+    const int np = d->num_prims;
+    std::vector<float> bmin(3 * (size_t)np), bmax(3 * (size_t)np);
+    std::vector<DevPrim> info(np);
+    auto P = [&](int v, int k) { return d->positions[3 * (size_t)v + k]; };
+    for (int p = 0; p < np; p++) {
+        const rd_prim_desc& pd = d->prims[p];
+        DevPrim& di = info[p];
+        di.kind = pd.kind; di.index = pd.index; di.material = pd.material; di.light = pd.light;
+        di.med_in = pd.medium_inside; di.med_out = pd.medium_outside; di.flags = 0;
+        if (pd.material >= d->num_materials || pd.light >= d->num_lights) { delete s; return bad("primitive material/light out of range"); }
+        if (pd.kind == RD_PRIM_TRIANGLE) {
+            if (pd.index < 0 || pd.index >= d->num_triangles) { delete s; return bad("primitive triangle index out of range"); }
+            const int* ix = d->indices + 3 * (size_t)pd.index;
+            f3 A = mk(P(ix[0], 0), P(ix[0], 1), P(ix[0], 2));
+            f3 B = mk(P(ix[1], 0), P(ix[1], 1), P(ix[1], 2));
+            f3 C = mk(P(ix[2], 0), P(ix[2], 1), P(ix[2], 2));
+            // Triangle ctor: bounds of the three vertices, area = |cross|/2 (Triangle.cs:83-94)
+            for (int k = 0; k < 3; k++) {
+                float a = comp(A, k), b = comp(B, k), cc = comp(C, k);
+                float lo = a, hi = a;
+                lo = lo < b ? lo : b; hi = hi > b ? hi : b;
+                lo = lo < cc ? lo : cc; hi = hi > cc ? hi : cc;
+                bmin[3 * p + k] = lo;
+                bmax[3 * p + k] = hi;
+            }
+            di.area = 0.5f * length(cross(B - A, C - A));
+            di.flags = d->tri_flags ? d->tri_flags[pd.index] : 0;
+        } else if (pd.kind == RD_PRIM_SPHERE) {
+            if (pd.index < 0 || pd.index >= d->num_spheres) { delete s; return bad("primitive sphere index out of range"); }
+            const float* S = d->spheres + 4 * (size_t)pd.index;
+            for (int k = 0; k < 3; k++) { bmin[3 * p + k] = S[k] - S[3]; bmax[3 * p + k] = S[k] + S[3]; }
+            di.area = 4 * S[3] * S[3] * kPi;   // Sphere.SurfaceArea (Sphere.cs:18)
+        } else {
+            delete s;
+            return bad("unknown primitive kind");
+        }
+    }
+    BvhResult bvh;
+    std::string err;
+    if (!build_bvh(np, bmin.data(), bmax.data(), d->bvh_max_prim > 0 ? d->bvh_max_prim : 1, &bvh, &err)) {
+        delete s;
+        g_err = err;
+        return nullptr;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    s->nodes = bvh.nodes;
+    s->order = bvh.order;
+
+    // node and ordered-primitive records
+    std::vector<float4> nodes(2 * bvh.nodes.size());
+    for (size_t i = 0; i < bvh.nodes.size(); i++) {
+        const NodeRec& r = bvh.nodes[i];
+        int packed = (int)r.count | ((int)r.axis << 16);
+        float offf, packf;
+        std::memcpy(&offf, &r.offset, 4);
+        std::memcpy(&packf, &packed, 4);
+        nodes[2 * i] = make_float4(r.mn[0], r.mn[1], r.mn[2], r.mx[0]);
+        nodes[2 * i + 1] = make_float4(r.mx[1], r.mx[2], offf, packf);
+    }
+    std::vector<float4> prims(3 * (size_t)np);
+    for (int i = 0; i < np; i++) {
+        int orig = bvh.order[i];
+        const rd_prim_desc& pd = d->prims[orig];
+        float origf, kindf;
+        int kind = pd.kind;
+        std::memcpy(&origf, &orig, 4);
+        std::memcpy(&kindf, &kind, 4);
+        if (pd.kind == RD_PRIM_TRIANGLE) {
+            const int* ix = d->indices + 3 * (size_t)pd.index;
+            prims[3 * i] = make_float4(P(ix[0], 0), P(ix[0], 1), P(ix[0], 2), origf);
+            prims[3 * i + 1] = make_float4(P(ix[1], 0), P(ix[1], 1), P(ix[1], 2), kindf);
+            prims[3 * i + 2] = make_float4(P(ix[2], 0), P(ix[2], 1), P(ix[2], 2), 0.0f);
+        } else {
+            const float* S = d->spheres + 4 * (size_t)pd.index;
+            prims[3 * i] = make_float4(S[0], S[1], S[2], origf);
+            prims[3 * i + 1] = make_float4(S[3], 0.f, 0.f, kindf);
+            prims[3 * i + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+    std::vector<float4> spheres(d->num_spheres);
+    for (int i = 0; i < d->num_spheres; i++)
+        spheres[i] = make_float4(d->spheres[4 * i], d->spheres[4 * i + 1], d->spheres[4 * i + 2], d->spheres[4 * i + 3]);
+    std::vector<DevMaterial> mats(d->num_materials);
+    for (int i = 0; i < d->num_materials; i++) {
+        const rd_material_desc& m = d->materials[i];
+        DevMaterial& o = mats[i];
+        std::memset(&o, 0, sizeof(o));
+        o.type = m.type; o.two_sided = m.two_sided; o.dist = m.dist; o.thin = m.thin;
+        for (int k = 0; k < 16; k++) o.tex[k] = m.tex[k];
+        o.eta_a = m.eta_a; o.eta_b = m.eta_b; o.kd = m.kd; o.ks = m.ks;
+        for (int k = 0; k < 3; k++) { o.eta[k] = m.eta[k]; o.k[k] = m.k[k]; }
+    }
+    std::vector<DevTexture> texs(d->num_textures);
+    std::vector<float> texels;
+    for (int i = 0; i < d->num_textures; i++) {
+        const rd_texture_desc& t = d->textures[i];
+        DevTexture& o = texs[i];
+        std::memset(&o, 0, sizeof(o));
+        o.w = t.width; o.h = t.height; o.wrap = t.wrap; o.filter = t.filter;
+        for (int k = 0; k < 3; k++) o.v[k] = t.value[k];
+        o.offset = (int64_t)texels.size();
+        if (t.width > 0) {
+            if (t.height <= 0 || !t.data) { delete s; return bad("image texture without data"); }
+            texels.insert(texels.end(), t.data, t.data + (size_t)t.width * t.height * 3);
+        }
+    }
+    std::vector<DevLight> lights(d->num_lights);
+    for (int i = 0; i < d->num_lights; i++) {
+        const rd_light_desc& l = d->lights[i];
+        DevLight& o = lights[i];
+        std::memset(&o, 0, sizeof(o));
+        o.type = l.type; o.prim = l.prim; o.tex = l.tex;
+        for (int k = 0; k < 3; k++) o.le[k] = l.le[k];
+        o.area = info[l.prim].area;
+    }
+    std::vector<DevMedium> media(d->num_media);
+    for (int i = 0; i < d->num_media; i++) {
+        const rd_medium_desc& m = d->media[i];
+        for (int k = 0; k < 3; k++) { media[i].sigma_a[k] = m.sigma_a[k]; media[i].sigma_s[k] = m.sigma_s[k]; }
+        media[i].g = m.g;
+        media[i].scale = m.scale;
+    }
+
+    DevScene& ds = s->dev;
+    if (!setup_camera(d->camera, &ds.cam)) { delete s; return bad("camera matrix not invertible"); }
+    ds.num_nodes = (int)bvh.nodes.size();
+    ds.num_prims = np;
+    ds.stack_needed = bvh.max_depth;
+    ds.num_lights = d->num_lights;
+    ds.global_medium = d->global_medium;
+    int rc = RD_OK;
+    if (!rc) rc = upload(s, nodes.data(), nodes.size(), &ds.nodes);
+    if (!rc) rc = upload(s, prims.data(), prims.size(), &ds.prims);
+    if (!rc) rc = upload(s, info.data(), info.size(), &ds.prim_info);
+    if (!rc) rc = upload(s, d->indices, 3 * (size_t)d->num_triangles, &ds.tri_idx);
+    if (!rc) rc = upload(s, d->positions, 3 * (size_t)d->num_vertices, &ds.pos);
+    if (!rc && d->normals) rc = upload(s, d->normals, 3 * (size_t)d->num_vertices, &ds.nrm);
+    if (!rc && d->uvs) rc = upload(s, d->uvs, 2 * (size_t)d->num_vertices, &ds.uv);
+    if (!rc) rc = upload(s, spheres.data(), spheres.size(), &ds.spheres);
+    if (!rc) rc = upload(s, mats.data(), mats.size(), &ds.mats);
+    if (!rc) rc = upload(s, texs.data(), texs.size(), &ds.texs);
+    if (!rc) rc = upload(s, texels.data(), texels.size(), &ds.texels);
+    if (!rc) rc = upload(s, lights.data(), lights.size(), &ds.lights);
+    if (!rc) rc = upload(s, media.data(), media.size(), &ds.media);
+    if (rc) { delete s; return nullptr; }
+
+    const rd_integrator_desc& I = d->integrator;
+    s->integ.type = I.type;
+    s->integ.max_depth = I.max_depth;
+    s->integ.min_depth = I.min_depth;
+    s->integ.method = I.method;
+    s->integ.strategy = I.strategy;
+    s->integ.rr_threshold = I.rr_threshold;
+
+    s->info.num_nodes = ds.num_nodes;
+    s->info.num_leaves = bvh.leaves;
+    s->info.max_depth = bvh.max_depth;
+    s->info.num_prims = np;
+    s->info.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    uint64_t bytes = nodes.size() * 16 + prims.size() * 16 + info.size() * sizeof(DevPrim) +
+                     12ull * d->num_triangles + 12ull * d->num_vertices + texels.size() * 4;
+    s->info.device_bytes = bytes;
+    return s;
+}
+
+void rd_scene_destroy(rd_scene* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    delete s;
+}
+
+int rd_scene_get_info(const rd_scene* s, rd_scene_info* info) {
+    if (!s || !info) return fail(RD_ERR_INVALID, "null argument");
+    *info = s->info;
+    return RD_OK;
+}
+
+int rd_scene_export_bvh(const rd_scene* s, void* nodes32, int32_t max_nodes, int32_t* prim_order, int32_t max_prims) {
+    if (!s) return fail(RD_ERR_INVALID, "null scene");
+    if (nodes32) {
+        if (max_nodes < (int32_t)s->nodes.size()) return fail(RD_ERR_INVALID, "node buffer too small");
+        std::memcpy(nodes32, s->nodes.data(), s->nodes.size() * sizeof(NodeRec));
+    }
+    if (prim_order) {
+        if (max_prims < (int32_t)s->order.size()) return fail(RD_ERR_INVALID, "order buffer too small");
+        std::memcpy(prim_order, s->order.data(), s->order.size() * sizeof(int32_t));
+    }
+    return RD_OK;
+}
+
+int rd_bvh_build_host(const float* bmin, const float* bmax, int32_t n, int32_t max_prim, void* nodes32,
+                      int32_t max_nodes, int32_t* prim_order, int32_t* num_nodes, int32_t* max_depth) {
+    if (n < 0 || (n > 0 && (!bmin || !bmax))) return fail(RD_ERR_INVALID, "bad argument");
+    BvhResult r;
+    std::string err;
+    if (!build_bvh(n, bmin, bmax, max_prim, &r, &err)) return fail(RD_ERR_INVALID, err);
+    if (num_nodes) *num_nodes = (int32_t)r.nodes.size();
+    if (max_depth) *max_depth = r.max_depth;
+    if (nodes32) {
+        if (max_nodes < (int32_t)r.nodes.size()) return fail(RD_ERR_INVALID, "node buffer too small");
+        std::memcpy(nodes32, r.nodes.data(), r.nodes.size() * sizeof(NodeRec));
+    }
+    if (prim_order) std::memcpy(prim_order, r.order.data(), r.order.size() * sizeof(int32_t));
+    return RD_OK;
+}
+
+int rd_trace_closest_device(rd_ctx* c, rd_scene* s, const rd_ray* d_rays, int32_t n, rd_hit* d_hits,
+                            uint64_t* d_counters, void* stream) {
+    if (!c || !s || n < 0 || (n > 0 && (!d_rays || !d_hits))) return fail(RD_ERR_INVALID, "bad argument");
+    if (int rc = check_stack(s)) return rc;
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    Timer tm(c);
+    HIP_TRY(tm.run("trace_rays_closest", [&] {
+        return launch_trace_rays(s->dev, (const float4*)d_rays, n, (float4*)d_hits, nullptr,
+                                 (unsigned long long*)d_counters, false, st);
+    }));
+    if (!stream) {
+        HIP_TRY(hipStreamSynchronize(st));
+        tm.collect();
+    }
+    return RD_OK;
+}
+
+int rd_trace_any_device(rd_ctx* c, rd_scene* s, const rd_ray* d_rays, int32_t n, uint8_t* d_occ,
+                        uint64_t* d_counters, void* stream) {
+    if (!c || !s || n < 0 || (n > 0 && (!d_rays || !d_occ))) return fail(RD_ERR_INVALID, "bad argument");
+    if (int rc = check_stack(s)) return rc;
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    Timer tm(c);
+    HIP_TRY(tm.run("trace_rays_any", [&] {
+        return launch_trace_rays(s->dev, (const float4*)d_rays, n, nullptr, d_occ, (unsigned long long*)d_counters,
+                                 true, st);
+    }));
+    if (!stream) {
+        HIP_TRY(hipStreamSynchronize(st));
+        tm.collect();
+    }
+    return RD_OK;
+}
+
+int rd_trace_closest(rd_ctx* c, rd_scene* s, const rd_ray* rays, int32_t n, rd_hit* hits) {
+    if (!c || !s || n < 0 || (n > 0 && (!rays || !hits))) return fail(RD_ERR_INVALID, "bad argument");
+    if (n == 0) return RD_OK;
+    CtxLock lock(c);
+    if (!lock.ok()) return fail(RD_ERR_BUSY, "context in use by another thread");
+    (void)hipSetDevice(c->device);
+    void *dr = nullptr, *dh = nullptr;
+    if (hipMalloc(&dr, sizeof(rd_ray) * (size_t)n) != hipSuccess) return fail(RD_ERR_OOM, "hipMalloc rays");
+    if (hipMalloc(&dh, sizeof(rd_hit) * (size_t)n) != hipSuccess) { (void)hipFree(dr); return fail(RD_ERR_OOM, "hipMalloc hits"); }
+    int rc = RD_OK;
+    if (hipMemcpyAsync(dr, rays, sizeof(rd_ray) * (size_t)n, hipMemcpyHostToDevice, c->stream) != hipSuccess) rc = fail(RD_ERR_DEVICE, "copy rays");
+    if (!rc) rc = rd_trace_closest_device(c, s, (const rd_ray*)dr, n, (rd_hit*)dh, nullptr, nullptr);
+    if (!rc && hipMemcpy(hits, dh, sizeof(rd_hit) * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(RD_ERR_DEVICE, "copy hits");
+    (void)hipFree(dr);
+    (void)hipFree(dh);
+    return rc;
+}
+
+int rd_trace_any(rd_ctx* c, rd_scene* s, const rd_ray* rays, int32_t n, uint8_t* occ) {
+    if (!c || !s || n < 0 || (n > 0 && (!rays || !occ))) return fail(RD_ERR_INVALID, "bad argument");
+    if (n == 0) return RD_OK;
+    CtxLock lock(c);
+    if (!lock.ok()) return fail(RD_ERR_BUSY, "context in use by another thread");
+    (void)hipSetDevice(c->device);
+    void *dr = nullptr, *dh = nullptr;
+    if (hipMalloc(&dr, sizeof(rd_ray) * (size_t)n) != hipSuccess) return fail(RD_ERR_OOM, "hipMalloc rays");
+    if (hipMalloc(&dh, (size_t)n) != hipSuccess) { (void)hipFree(dr); return fail(RD_ERR_OOM, "hipMalloc occluded"); }
+    int rc = RD_OK;
+    if (hipMemcpyAsync(dr, rays, sizeof(rd_ray) * (size_t)n, hipMemcpyHostToDevice, c->stream) != hipSuccess) rc = fail(RD_ERR_DEVICE, "copy rays");
+    if (!rc) rc = rd_trace_any_device(c, s, (const rd_ray*)dr, n, (uint8_t*)dh, nullptr, nullptr);
+    if (!rc && hipMemcpy(occ, dh, (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(RD_ERR_DEVICE, "copy occluded");
+    (void)hipFree(dr);
+    (void)hipFree(dh);
+    return rc;
+}
+
+int rd_render_device(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d_out_rgb, float* d_out_sum2,
+                     volatile int32_t* cancel, volatile int32_t* progress) {
+    if (!c || !s || !p || !d_out_rgb) return fail(RD_ERR_INVALID, "null argument");
+    if (p->spp <= 0) return fail(RD_ERR_INVALID, "spp must be positive");
+    if (int rc = check_stack(s)) return rc;
+    const int W = s->width, H = s->height;
+    int x0 = p->x0, y0 = p->y0, x1 = p->x1, y1 = p->y1;
+    if (x0 == 0 && y0 == 0 && x1 == 0 && y1 == 0) { x1 = W; y1 = H; }
+    if (x0 < 0 || y0 < 0 || x1 > W || y1 > H || x0 >= x1 || y0 >= y1) return fail(RD_ERR_INVALID, "bad crop");
+    const int cw = x1 - x0, ch = y1 - y0;
+    const size_t npix = (size_t)cw * ch;
+    // Paths in flight per wave: whole samples of every crop pixel.
+    size_t target = p->wave_paths > 0 ? (size_t)p->wave_paths : ((size_t)1 << 22);
+    int spw = (int)std::max<size_t>(1, std::min<size_t>((size_t)p->spp, target / std::max<size_t>(npix, 1)));
+    size_t cap = npix * (size_t)spw;
+    if (cap > (size_t)INT32_MAX / 2) return fail(RD_ERR_INVALID, "wave too large");
+    if (int rc = ensure_wave_buffers(c, cap, npix)) return rc;
+    WaveBuffers& wb = c->wb;
+    hipStream_t st = c->stream;
+    Timer tm(c);
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    HIP_TRY(launch_zero_counts(c->counts, kCntSlots, st));
+    HIP_TRY(launch_zero(wb.sum, (int)npix, st));
+    HIP_TRY(launch_zero(wb.sum2, (int)npix, st));
+    hipEvent_t ev0, ev1;
+    HIP_TRY(hipEventCreate(&ev0));
+    HIP_TRY(hipEventCreate(&ev1));
+    HIP_TRY(hipEventRecord(ev0, st));
+    WaveParams wp;
+    wp.x0 = x0; wp.y0 = y0; wp.crop_w = cw; wp.crop_h = ch;
+    wp.npix = (int)npix;
+    wp.width = W;
+    wp.seed = p->seed;
+    int waves = 0, max_bounce = 0;
+    int rc = RD_OK;
+    for (int sb = 0; sb < p->spp && !rc; sb += spw) {
+        if (cancel && *cancel) { rc = fail(RD_ERR_CANCELLED, "render cancelled"); break; }
+        wp.sample_base = sb;
+        wp.nsamples = std::min(spw, p->spp - sb);
+        int npaths = (int)(npix * wp.nsamples);
+        RayQueue qin = wb.qa, qout = wb.qb;
+        int cin = kCntExtA, cout = kCntExtB;
+        HIP_TRY(tm.run("raygen", [&] { return launch_raygen(s->dev, wp, npaths, qin, wb.ps, c->counts, st); }));
+        for (int bounce = 0;; bounce++) {
+            HIP_TRY(tm.run("trace_closest", [&] {
+                return launch_trace_ext(s->dev, qin, wb.hits, c->counts, cin, cout, kCntShadow, kCntMis, st);
+            }));
+            HIP_TRY(tm.run("shade", [&] {
+                return launch_shade(s->dev, s->integ, wp, qin, wb.hits, qout, wb.qshadow, wb.qmis, wb.ps, c->counts,
+                                    cin, cout, st);
+            }));
+            if (s->integ.method != RD_METHOD_BXDF) {
+                HIP_TRY(tm.run("trace_any", [&] { return launch_trace_shadow(s->dev, wb.qshadow, wb.ps, c->counts, st); }));
+                if (s->integ.method == RD_METHOD_MIS)
+                    HIP_TRY(tm.run("trace_mis", [&] { return launch_trace_mis(s->dev, wb.qmis, wb.ps, c->counts, st); }));
+                HIP_TRY(tm.run("resolve", [&] { return launch_resolve(s->integ, qin, wb.ps, c->counts, cin, st); }));
+            }
+            HIP_TRY(hipMemcpyAsync(c->host_counts + cout, c->counts + cout, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            if (bounce > max_bounce) max_bounce = bounce;
+            if (c->host_counts[cout] == 0) break;
+            if (cancel && *cancel) { rc = fail(RD_ERR_CANCELLED, "render cancelled"); break; }
+            std::swap(qin, qout);
+            std::swap(cin, cout);
+        }
+        if (rc) break;
+        HIP_TRY(tm.run("accumulate", [&] { return launch_accumulate(wp, wb.ps, wb.sum, wb.sum2, c->counts, st); }));
+        waves++;
+        if (progress) *progress = waves;
+    }
+    if (!rc) {
+        wp.sample_base = 0;
+        HIP_TRY(tm.run("finalize", [&] { return launch_finalize(wp, H, p->spp, wb.sum, wb.sum2, d_out_rgb, d_out_sum2, st); }));
+    }
+    HIP_TRY(hipEventRecord(ev1, st));
+    HIP_TRY(hipMemcpyAsync(c->host_counts, c->counts, kCntSlots * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    tm.collect();
+    auto u64 = [&](int slot) { uint64_t v; std::memcpy(&v, c->host_counts + slot, 8); return v; };
+    c->stats.rays_closest = u64(kCntRaysClosest);
+    c->stats.rays_any = u64(kCntRaysAny);
+    c->stats.vertices = u64(kCntVertices);
+    c->stats.invalid = u64(kCntInvalid);
+    c->stats.paths = (uint64_t)npix * (uint64_t)p->spp;
+    c->stats.render_ms = ms;
+    c->stats.waves = waves;
+    c->stats.bounces = max_bounce;
+    for (int k = 0; k < 3; k++) {
+        c->stats.class_rays[k] = u64(kCntClassRays + 2 * k);
+        c->stats.class_nodes[k] = u64(kCntClassNodes + 2 * k);
+        c->stats.class_prims[k] = u64(kCntClassPrims + 2 * k);
+    }
+    return rc;
+}
+
+int rd_render(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* out_rgb, float* out_sum2,
+              volatile int32_t* cancel, volatile int32_t* progress) {
+    if (!c || !s || !p || !out_rgb) return fail(RD_ERR_INVALID, "null argument");
+    CtxLock lock(c);
+    if (!lock.ok()) return fail(RD_ERR_BUSY, "context in use by another thread");
+    (void)hipSetDevice(c->device);
+    size_t n = (size_t)s->width * s->height * 3;
+    float *d1 = nullptr, *d2 = nullptr;
+    if (hipMalloc(&d1, n * 4) != hipSuccess) return fail(RD_ERR_OOM, "hipMalloc output");
+    if (out_sum2 && hipMalloc(&d2, n * 4) != hipSuccess) { (void)hipFree(d1); return fail(RD_ERR_OOM, "hipMalloc output"); }
+    int rc = RD_OK;
+    // pixels outside a crop keep the caller's values
+    if (hipMemcpy(d1, out_rgb, n * 4, hipMemcpyHostToDevice) != hipSuccess) rc = fail(RD_ERR_DEVICE, "copy output");
+    if (!rc && d2 && hipMemcpy(d2, out_sum2, n * 4, hipMemcpyHostToDevice) != hipSuccess) rc = fail(RD_ERR_DEVICE, "copy output");
+    if (!rc) rc = rd_render_device(c, s, p, d1, d2, cancel, progress);
+    if (!rc && hipMemcpy(out_rgb, d1, n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(RD_ERR_DEVICE, "copy output");
+    if (!rc && d2 && hipMemcpy(out_sum2, d2, n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(RD_ERR_DEVICE, "copy output");
+    (void)hipFree(d1);
+    if (d2) (void)hipFree(d2);
+    return rc;
+}
+
+int rd_get_render_stats(const rd_ctx* c, rd_render_stats* st) {
+    if (!c || !st) return fail(RD_ERR_INVALID, "null argument");
+    *st = c->stats;
+    return RD_OK;
+}
+
+void* rd_device_alloc(rd_ctx* c, uint64_t bytes) {
+    if (!c) return nullptr;
+    (void)hipSetDevice(c->device);
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) { g_err = "hipMalloc failed"; return nullptr; }
+    return p;
+}
+int rd_device_free(rd_ctx* c, void* ptr) {
+    if (!c) return fail(RD_ERR_INVALID, "null context");
+    HIP_TRY(hipFree(ptr));
+    return RD_OK;
+}
+int rd_memcpy_h2d(rd_ctx* c, void* dst, const void* src, uint64_t bytes) {
+    if (!c) return fail(RD_ERR_INVALID, "null context");
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return RD_OK;
+}
+int rd_memcpy_d2h(rd_ctx* c, void* dst, const void* src, uint64_t bytes) {
+    if (!c) return fail(RD_ERR_INVALID, "null context");
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return RD_OK;
+}
+int rd_synchronize(rd_ctx* c) {
+    if (!c) return fail(RD_ERR_INVALID, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipDeviceSynchronize());
+    return RD_OK;
+}
+int rd_kernel_timing(const rd_ctx* c, const char* stage, double* avg_ms, int64_t* count) {
+    if (!c || !stage) return fail(RD_ERR_INVALID, "null argument");
+    auto it = c->timing.find(stage);
+    if (it == c->timing.end()) {
+        if (avg_ms) *avg_ms = 0;
+        if (count) *count = 0;
+        return RD_OK;
+    }
+    if (avg_ms) *avg_ms = it->second.count ? it->second.total_ms / it->second.count : 0;
+    if (count) *count = it->second.count;
+    return RD_OK;
+}
+int rd_reset_kernel_timing(rd_ctx* c) {
+    if (!c) return fail(RD_ERR_INVALID, "null context");
+    c->timing.clear();
+    return RD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+// rd_kernels.h — host-side launchers for the gfx950 kernels in rd_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "rd_types.h"
+
+namespace rd {
+
+// Ray queue, structure of arrays.  org = {o.xyz, tmin}, dir = {d.xyz, tmax}.
+struct RayQueue {
+    int* count;
+    int* pid;
+    float4* org;
+    float4* dir;
+};
+
+// Per-path wavefront state (capacity = paths per wave).
+struct PathState {
+    float4* L;          // radiance
+    float4* beta;       // path throughput (PathTracer "coeff")
+    float4* nee_beta;   // throughput at the NEE vertex; w = light selection pdf
+    float4* nee_le;     // light-sample contribution, gated by the shadow ray
+    float4* mis_f;      // BSDF-sample f*|cos|; w = MIS weight
+    float4* mis_li;     // radiance found by the MIS ray; w = BSDF-sample pdf
+    int* flags;
+    int* sel_light;
+    uint32_t* rng_n;
+    int* depth;
+    uint8_t* vis;
+};
+
+struct WaveParams {
+    int x0, y0, crop_w, crop_h;
+    int npix;           // crop_w * crop_h
+    int sample_base;    // first sample index of this wave
+    int nsamples;       // samples per pixel in this wave
+    int width;          // full image width (pixel id = y * width + x)
+    uint32_t seed;
+};
+
+// Device counters used by the bounce loop.
+// 64-bit statistics occupy two int slots at an even index.
+enum : int {
+    kCntExtA = 0, kCntExtB = 1, kCntShadow = 2, kCntMis = 3,
+    kCntRaysClosest = 4, kCntRaysAny = 6, kCntVertices = 8, kCntInvalid = 10,
+    kCntWork = 12,        // work-queue cursors (4 slots)
+    kCntClassRays = 16,   // 3 x u64: extension, MIS, shadow
+    kCntClassNodes = 22,  // 3 x u64
+    kCntClassPrims = 28,  // 3 x u64
+    kCntSlots = 34
+};
+
+struct Timing;  // per-stage HIP event timing (rd_api.cpp)
+
+// Traversal over caller-provided ray arrays (rd_ray / rd_hit layouts).
+hipError_t launch_trace_rays(const DevScene& sc, const float4* rays, int n, float4* hits, uint8_t* occ,
+                             unsigned long long* counters, bool any, hipStream_t s);
+
+// Wavefront path tracing stages.
+hipError_t launch_raygen(const DevScene& sc, const WaveParams& wp, int npaths, RayQueue q, PathState ps,
+                         int* counts, hipStream_t s);
+hipError_t launch_trace_ext(const DevScene& sc, RayQueue q, float4* hits, int* counts, int cnt_slot,
+                            int zero_a, int zero_b, int zero_c, hipStream_t s);
+hipError_t launch_shade(const DevScene& sc, const DevIntegrator& in, const WaveParams& wp, RayQueue qin,
+                        const float4* hits, RayQueue qout, RayQueue qshadow, RayQueue qmis, PathState ps,
+                        int* counts, int cnt_in, int cnt_out, hipStream_t s);
+hipError_t launch_trace_shadow(const DevScene& sc, RayQueue q, PathState ps, int* counts, hipStream_t s);
+hipError_t launch_trace_mis(const DevScene& sc, RayQueue q, PathState ps, int* counts, hipStream_t s);
+hipError_t launch_resolve(const DevIntegrator& in, RayQueue qin, PathState ps, int* counts, int cnt_in,
+                          hipStream_t s);
+hipError_t launch_accumulate(const WaveParams& wp, PathState ps, float4* sum, float4* sum2, int* counts,
+                             hipStream_t s);
+hipError_t launch_finalize(const WaveParams& wp, int height, int spp, const float4* sum, const float4* sum2,
+                           float* out_rgb, float* out_sum2, hipStream_t s);
+hipError_t launch_zero(float4* buf, int n, hipStream_t s);
+hipError_t launch_zero_counts(int* counts, int n, hipStream_t s);
+
+// Occupancy-sized persistent grid for the device.
+void set_device_grid(int num_cus);
+int persistent_blocks();
+
+}  // namespace rd

@@ -1,0 +1,680 @@
+// rd_kernels.hip — gfx950 kernels: BVH traversal and the wavefront stages of
+// PathTracer.Mis / NextEventEstimation / OnlySampleBxdf
+// (Radiantium.Offline/Integrators/PathTracer.cs:43-270).
+//
+// Wavefront mapping of one reference bounce (PathTracer.cs:197-269):
+//   trace_ext     Scene.Intersect of the extension rays (closest hit)
+//   shade         emission, light selection + SampleLi, the MIS BSDF sample,
+//                 the continuation BSDF sample and Russian roulette — every
+//                 random draw of the bounce, in the reference's order — and
+//                 appends shadow, MIS and next-extension rays to queues
+//   trace_shadow  Scene.IsOccluded (any hit) of the light-sample rays
+//   trace_mis     Scene.Intersect of the BSDF-sample rays + Le of the hit
+//   resolve       radiance += coeff * (le_light + le_bsdf) / lightPdf
+// The random stream is keyed by (seed, pixel, sample) and a per-path draw
+// counter, so the kernels reproduce the CPU restatement's draws exactly.
+#include "rd_kernels.h"
+#include "rd_shading.h"
+
+namespace rd {
+
+static int g_num_cus = 256;
+void set_device_grid(int num_cus) { g_num_cus = num_cus > 0 ? num_cus : 256; }
+int persistent_blocks() { return g_num_cus * 8; }
+
+enum : int {
+    kFlagSpecular = 1,
+    kFlagNeePending = 2,
+    kFlagShadowIssued = 4,
+    kFlagMisIssued = 8,
+};
+
+// ---------------------------------------------------------------- wave helpers
+// Appends one slot per lane with `pred` set; one atomic per wavefront.
+// Must be reached by every lane of the wavefront (converged).
+__device__ __forceinline__ int wave_append(int* counter, bool pred) {
+    uint64_t mask = __ballot(pred);
+    if (mask == 0) return -1;
+    int lane = __lane_id();
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+    uint64_t below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+    return pred ? base + __popcll(below) : -1;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__device__ __forceinline__ void add_counter(unsigned long long* c, uint64_t v) {
+    uint64_t s = wave_sum_u64(v);
+    if (__lane_id() == 0 && s) atomicAdd(c, (unsigned long long)s);
+}
+
+// Per query class (0 extension, 1 MIS, 2 shadow): rays, node visits and
+// primitive tests, for the algorithmic-bytes accounting of the trace kernels.
+__device__ __forceinline__ void add_class_counters(int* counts, int cls, uint64_t rays, uint64_t nv, uint64_t nt) {
+    add_counter((unsigned long long*)(counts + kCntClassRays) + cls, rays);
+    add_counter((unsigned long long*)(counts + kCntClassNodes) + cls, nv);
+    add_counter((unsigned long long*)(counts + kCntClassPrims) + cls, nt);
+}
+
+// ---------------------------------------------------------------- traversal
+// BoundingBox3F.Intersect (BoundingBox3F.cs:155-176) on a node stored as
+// lo = {min.xyz, max.x}, hi = {max.y, max.z, ...}.  The reference's factor
+// 1 + 2*Gamma(3) is exactly 1.0f and is omitted.
+__device__ __forceinline__ bool box_hit(float4 lo, float4 hi, f3 o, f3 inv, bool nx, bool ny, bool nz, float ray_tmax) {
+    float tmin = ((nx ? lo.w : lo.x) - o.x) * inv.x;
+    float tmax = ((nx ? lo.x : lo.w) - o.x) * inv.x;
+    float tymin = ((ny ? hi.x : lo.y) - o.y) * inv.y;
+    float tymax = ((ny ? lo.y : hi.x) - o.y) * inv.y;
+    if (tmin > tymax || tymin > tmax) return false;
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    float tzmin = ((nz ? hi.y : lo.z) - o.z) * inv.z;
+    float tzmax = ((nz ? lo.z : hi.y) - o.z) * inv.z;
+    if (tmin > tzmax || tzmin > tmax) return false;
+    if (tzmin > tmin) tmin = tzmin;
+    if (tzmax < tmax) tmax = tzmax;
+    return (tmin < ray_tmax) && (tmax > 0);
+}
+
+struct Hit {
+    float t, u, v;
+    int prim;   // original primitive index, -1 = miss
+};
+
+// Bvh.Intersect (Accelerators/Bvh.cs:126-232): depth-first traversal, near
+// child chosen by the sign of InvD along the node's split axis, first-found
+// hit kept on ties (strict '<'), ray.MaxT shrunk to the closest hit.  The
+// stack lives in LDS, one column per lane (stride kBlock) so that lanes of a
+// wavefront hit distinct banks.
+template <bool kAny, int kBlock>
+__device__ __forceinline__ bool traverse(const DevScene& sc, f3 o, f3 d, float tmin, float tmax, int* stk,
+                                         Hit* out, uint32_t& nv, uint32_t& nt) {
+    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
+    int sp = 0, cur = 0;
+    bool found = false;
+    float best = kFloatMax;
+    out->prim = -1;
+    const float4* __restrict__ nodes = sc.nodes;
+    const float4* __restrict__ prims = sc.prims;
+    while (true) {
+        float4 lo = nodes[2 * cur], hi = nodes[2 * cur + 1];
+        nv++;
+        if (box_hit(lo, hi, o, inv, nx, ny, nz, tmax)) {
+            int off = __float_as_int(hi.z);
+            int ca = __float_as_int(hi.w);
+            int cnt = ca & 0xffff;
+            if (cnt > 0) {
+                for (int i = 0; i < cnt; i++) {
+                    const float4* p = prims + 3 * (off + i);
+                    float4 p0 = p[0], p1 = p[1];
+                    nt++;
+                    float tu = 0, tv = 0, tt;
+                    bool h;
+                    if (__float_as_int(p1.w) == kTri) {
+                        float4 p2 = p[2];
+                        h = tri_test(mk(p0.x, p0.y, p0.z), mk(p1.x, p1.y, p1.z), mk(p2.x, p2.y, p2.z), o, d, tmin, tmax,
+                                     &tu, &tv, &tt);
+                    } else {
+                        h = sphere_test(make_float4(p0.x, p0.y, p0.z, p1.x), o, d, tmin, tmax, &tt);
+                    }
+                    if (h) {
+                        if (kAny) { out->prim = __float_as_int(p0.w); return true; }
+                        found = true;
+                        if (tt < best) {
+                            best = tt;
+                            out->t = tt; out->u = tu; out->v = tv; out->prim = __float_as_int(p0.w);
+                            tmax = tt;
+                        }
+                    }
+                }
+                if (sp == 0) break;
+                cur = stk[(--sp) * kBlock];
+            } else {
+                int axis = ca >> 16;
+                bool neg = axis == 0 ? nx : (axis == 1 ? ny : nz);
+                if (neg) { stk[(sp++) * kBlock] = cur + 1; cur = off; }
+                else { stk[(sp++) * kBlock] = off; cur = cur + 1; }
+            }
+        } else {
+            if (sp == 0) break;
+            cur = stk[(--sp) * kBlock];
+        }
+    }
+    return found;
+}
+
+// Stack depth in LDS per lane.  The DFS stack never holds more entries than
+// the deepest leaf's depth; scenes deeper than this use the 64-entry kernel
+// (the reference's Bvh.StackSize).
+constexpr int kStackShort = 40;
+constexpr int kStackLong = 64;
+
+// ---------------------------------------------------------------- ray-array traversal
+template <bool kAny, int kStack, int kBlock>
+__global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene sc, const float4* __restrict__ rays, int n,
+                                                       float4* __restrict__ hits, uint8_t* __restrict__ occ,
+                                                       unsigned long long* counters) {
+    __shared__ int lds[kStack * kBlock];
+    int* stk = lds + threadIdx.x;
+    uint32_t nv = 0, nt = 0;
+    for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        int i = base + threadIdx.x;
+        if (i < n) {
+            float4 a = rays[2 * i], b = rays[2 * i + 1];
+            Hit h;
+            bool r = traverse<kAny, kBlock>(sc, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, stk, &h, nv, nt);
+            if (kAny) occ[i] = r ? 1 : 0;
+            else hits[i] = r ? make_float4(h.t, __int_as_float(h.prim), h.u, h.v) : make_float4(0.f, __int_as_float(-1), 0.f, 0.f);
+        }
+    }
+    if (counters) {
+        add_counter(counters, nv);
+        add_counter(counters + 1, nt);
+    }
+}
+
+hipError_t launch_trace_rays(const DevScene& sc, const float4* rays, int n, float4* hits, uint8_t* occ,
+                             unsigned long long* counters, bool any, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (sc.stack_needed <= kStackShort) {
+        constexpr int B = 256;
+        int blocks = (n + B - 1) / B;
+        if (blocks > persistent_blocks()) blocks = persistent_blocks();
+        if (any) hipLaunchKernelGGL((k_trace_rays<true, kStackShort, B>), dim3(blocks), dim3(B), 0, s, sc, rays, n, hits, occ, counters);
+        else hipLaunchKernelGGL((k_trace_rays<false, kStackShort, B>), dim3(blocks), dim3(B), 0, s, sc, rays, n, hits, occ, counters);
+    } else {
+        constexpr int B = 128;
+        int blocks = (n + B - 1) / B;
+        if (blocks > 2 * persistent_blocks()) blocks = 2 * persistent_blocks();
+        if (any) hipLaunchKernelGGL((k_trace_rays<true, kStackLong, B>), dim3(blocks), dim3(B), 0, s, sc, rays, n, hits, occ, counters);
+        else hipLaunchKernelGGL((k_trace_rays<false, kStackLong, B>), dim3(blocks), dim3(B), 0, s, sc, rays, n, hits, occ, counters);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- wavefront
+__device__ __forceinline__ void wave_pixel(const WaveParams& wp, int p, int* x, int* y, int* sample) {
+    int s = p / wp.npix;
+    int q = p - s * wp.npix;
+    int yy = q / wp.crop_w;
+    *x = wp.x0 + (q - yy * wp.crop_w);
+    *y = wp.y0 + yy;
+    *sample = wp.sample_base + s;
+}
+__device__ __forceinline__ uint64_t wave_key(const WaveParams& wp, int p) {
+    int x, y, s;
+    wave_pixel(wp, p, &x, &y, &s);
+    return path_key(wp.seed, (uint32_t)(y * wp.width + x), (uint32_t)s);
+}
+__device__ __forceinline__ f3 ld3(const float4* a, int i) { float4 v = a[i]; return mk(v.x, v.y, v.z); }
+__device__ __forceinline__ void st3(float4* a, int i, f3 v, float w = 0.0f) { a[i] = make_float4(v.x, v.y, v.z, w); }
+
+// Vector3.Transform / TransformNormal with a row-major 4x4 (row-vector convention).
+__device__ __forceinline__ f3 xform_point(const float* M, f3 v) {
+    return mk(v.x * M[0] + v.y * M[4] + v.z * M[8] + M[12], v.x * M[1] + v.y * M[5] + v.z * M[9] + M[13],
+              v.x * M[2] + v.y * M[6] + v.z * M[10] + M[14]);
+}
+__device__ __forceinline__ f3 xform_vector(const float* M, f3 v) {
+    return mk(v.x * M[0] + v.y * M[4] + v.z * M[8], v.x * M[1] + v.y * M[5] + v.z * M[9],
+              v.x * M[2] + v.y * M[6] + v.z * M[10]);
+}
+
+// Pixel sample (x,y)+U^2 and PerspectiveCamera.SampleRay (Camera.cs:182-191);
+// BlockBasedRenderer.RenderTask (BlockBasedRenderer.cs:136-139).
+__global__ __launch_bounds__(256) void k_raygen(DevScene sc, WaveParams wp, int npaths, RayQueue q, PathState ps,
+                                                int* counts) {
+    for (int p = blockIdx.x * 256 + threadIdx.x; p < npaths; p += gridDim.x * 256) {
+        int x, y, s;
+        wave_pixel(wp, p, &x, &y, &s);
+        Rng rng{path_key(wp.seed, (uint32_t)(y * wp.width + x), (uint32_t)s), 0u};
+        float ox = rng.next_float(), oy = rng.next_float();
+        float sx = (float)x + ox, sy = (float)y + oy;
+        const DevCamera& c = sc.cam;
+        f3 ndc = mk(sx * c.inv_res_x, sy * c.inv_res_y, 0.0f);
+        f3 nearp = xform_point(c.clip_to_camera, ndc);
+        f3 dir = normalize(nearp);
+        float inv_z = 1.0f / dir.z;
+        f3 o = xform_point(c.camera_to_world, mk(0.0f));
+        f3 d = normalize(xform_vector(c.camera_to_world, dir));
+        q.pid[p] = p;
+        q.org[p] = make_float4(o.x, o.y, o.z, c.near_z * inv_z);
+        q.dir[p] = make_float4(d.x, d.y, d.z, c.far_z * inv_z);
+        ps.L[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+        ps.beta[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+        ps.flags[p] = kFlagSpecular;
+        ps.rng_n[p] = rng.n;
+        ps.depth[p] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) counts[kCntExtA] = npaths;
+}
+
+hipError_t launch_raygen(const DevScene& sc, const WaveParams& wp, int npaths, RayQueue q, PathState ps,
+                         int* counts, hipStream_t s) {
+    int blocks = (npaths + 255) / 256;
+    if (blocks > persistent_blocks()) blocks = persistent_blocks();
+    hipLaunchKernelGGL(k_raygen, dim3(blocks), dim3(256), 0, s, sc, wp, npaths, q, ps, counts);
+    return hipGetLastError();
+}
+
+// Closest-hit over the extension queue.  Block 0 also clears the counters of
+// the queues the following shade stage appends to.
+template <int kStack, int kBlock>
+__global__ __launch_bounds__(kBlock) void k_trace_ext(DevScene sc, RayQueue q, float4* __restrict__ hits, int* counts,
+                                                      int cnt_slot, int za, int zb, int zc) {
+    __shared__ int lds[kStack * kBlock];
+    int* stk = lds + threadIdx.x;
+    int n = counts[cnt_slot];
+    if (blockIdx.x == 0 && threadIdx.x == 0) { counts[za] = 0; counts[zb] = 0; counts[zc] = 0; }
+    uint32_t nv = 0, nt = 0;
+    uint64_t nrays = 0;
+    for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        int i = base + threadIdx.x;
+        if (i < n) {
+            float4 a = q.org[i], b = q.dir[i];
+            Hit h;
+            bool r = traverse<false, kBlock>(sc, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, stk, &h, nv, nt);
+            hits[i] = r ? make_float4(h.t, __int_as_float(h.prim), h.u, h.v) : make_float4(0.f, __int_as_float(-1), 0.f, 0.f);
+            nrays++;
+        }
+    }
+    add_class_counters(counts, 0, nrays, nv, nt);
+    add_counter((unsigned long long*)(counts + kCntRaysClosest), nrays);
+}
+
+hipError_t launch_trace_ext(const DevScene& sc, RayQueue q, float4* hits, int* counts, int cnt_slot, int za,
+                            int zb, int zc, hipStream_t s) {
+    if (sc.stack_needed <= kStackShort)
+        hipLaunchKernelGGL((k_trace_ext<kStackShort, 256>), dim3(persistent_blocks()), dim3(256), 0, s, sc, q, hits,
+                           counts, cnt_slot, za, zb, zc);
+    else
+        hipLaunchKernelGGL((k_trace_ext<kStackLong, 128>), dim3(2 * persistent_blocks()), dim3(128), 0, s, sc, q,
+                           hits, counts, cnt_slot, za, zb, zc);
+    return hipGetLastError();
+}
+
+// Any-hit over the shadow queue: Scene.IsOccluded (Scene.cs:58-70).
+template <int kStack, int kBlock>
+__global__ __launch_bounds__(kBlock) void k_trace_shadow(DevScene sc, RayQueue q, PathState ps, int* counts) {
+    __shared__ int lds[kStack * kBlock];
+    int* stk = lds + threadIdx.x;
+    int n = counts[kCntShadow];
+    uint32_t nv = 0, nt = 0;
+    uint64_t nrays = 0;
+    for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        int i = base + threadIdx.x;
+        if (i < n) {
+            float4 a = q.org[i], b = q.dir[i];
+            Hit h;
+            bool occluded = traverse<true, kBlock>(sc, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, stk, &h, nv, nt);
+            ps.vis[q.pid[i]] = occluded ? 0 : 1;
+            nrays++;
+        }
+    }
+    add_class_counters(counts, 2, nrays, nv, nt);
+    add_counter((unsigned long long*)(counts + kCntRaysAny), nrays);
+}
+
+hipError_t launch_trace_shadow(const DevScene& sc, RayQueue q, PathState ps, int* counts, hipStream_t s) {
+    if (sc.stack_needed <= kStackShort)
+        hipLaunchKernelGGL((k_trace_shadow<kStackShort, 256>), dim3(persistent_blocks()), dim3(256), 0, s, sc, q, ps, counts);
+    else
+        hipLaunchKernelGGL((k_trace_shadow<kStackLong, 128>), dim3(2 * persistent_blocks()), dim3(128), 0, s, sc, q, ps, counts);
+    return hipGetLastError();
+}
+
+// Closest-hit of the BSDF-sampled MIS rays, then the radiance they carry:
+// Le of the hit when it is the selected light (PathTracer.cs:379-387), the
+// infinite lights on a miss (:388-391; none are modelled, so 0).
+template <int kStack, int kBlock>
+__global__ __launch_bounds__(kBlock) void k_trace_mis(DevScene sc, RayQueue q, PathState ps, int* counts) {
+    __shared__ int lds[kStack * kBlock];
+    int* stk = lds + threadIdx.x;
+    int n = counts[kCntMis];
+    uint32_t nv = 0, nt = 0;
+    uint64_t nrays = 0;
+    for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        int i = base + threadIdx.x;
+        if (i < n) {
+            float4 a = q.org[i], b = q.dir[i];
+            f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
+            Hit h;
+            bool hit = traverse<false, kBlock>(sc, o, d, a.w, b.w, stk, &h, nv, nt);
+            int pid = q.pid[i];
+            f3 li = mk(0.0f);
+            if (hit) {
+                const DevPrim pr = sc.prim_info[h.prim];
+                int sel = ps.sel_light[pid];
+                if (pr.light >= 0 && pr.light == sel) {
+                    f3 n = hit_normal(sc, pr, o, d, h.u, h.v, h.t);
+                    const DevLight& L = sc.lights[sel];
+                    if (dot(n, -d) > 0) li = mk(L.le[0], L.le[1], L.le[2]);
+                }
+            }
+            float4 prev = ps.mis_li[pid];
+            ps.mis_li[pid] = make_float4(li.x, li.y, li.z, prev.w);
+            nrays++;
+        }
+    }
+    add_class_counters(counts, 1, nrays, nv, nt);
+    add_counter((unsigned long long*)(counts + kCntRaysClosest), nrays);
+}
+
+hipError_t launch_trace_mis(const DevScene& sc, RayQueue q, PathState ps, int* counts, hipStream_t s) {
+    if (sc.stack_needed <= kStackShort)
+        hipLaunchKernelGGL((k_trace_mis<kStackShort, 256>), dim3(persistent_blocks()), dim3(256), 0, s, sc, q, ps, counts);
+    else
+        hipLaunchKernelGGL((k_trace_mis<kStackLong, 128>), dim3(2 * persistent_blocks()), dim3(128), 0, s, sc, q, ps, counts);
+    return hipGetLastError();
+}
+
+// One bounce of PathTracer.Mis / NextEventEstimation / OnlySampleBxdf for
+// every path in the extension queue (PathTracer.cs:43-270).
+__global__ __launch_bounds__(256) void k_shade(DevScene sc, DevIntegrator in, WaveParams wp, RayQueue qin,
+                                               const float4* __restrict__ hits, RayQueue qout, RayQueue qshadow,
+                                               RayQueue qmis, PathState ps, int* counts, int cnt_in, int cnt_out) {
+    int n = counts[cnt_in];
+    uint64_t nvert = 0;
+    for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        int k = base + threadIdx.x;
+        bool want_ext = false, want_shadow = false, want_mis = false;
+        f3 eo = mk(0.f), ed = mk(0.f), so = mk(0.f), sd = mk(0.f), mo = mk(0.f), md = mk(0.f);
+        float etmin = 0.f, etmax = 0.f, stmax = 0.f;
+        int pid = 0;
+        if (k < n) {
+            pid = qin.pid[k];
+            float4 a = qin.org[k], b = qin.dir[k];
+            f3 ro = mk(a.x, a.y, a.z), rd = mk(b.x, b.y, b.z);
+            float rtmin = a.w;
+            float4 h = hits[k];
+            int prim = __float_as_int(h.y);
+            bool hit = prim >= 0;
+            Rng rng{wave_key(wp, pid), ps.rng_n[pid]};
+            f3 beta = ld3(ps.beta, pid);
+            f3 L = ld3(ps.L, pid);
+            int flags = ps.flags[pid];
+            bool spec = (flags & kFlagSpecular) != 0;
+            int depth = ps.depth[pid];
+            int nflags = 0;
+            f3 wo = -rd;
+            DevPrim pr;
+            Inct it;
+            if (hit) {
+                pr = sc.prim_info[prim];
+                it = make_inct(sc, pr, ro, rd, h.z, h.w, h.x);
+            }
+            auto inct_le = [&](f3 w) -> f3 {   // Intersection.Le -> DiffuseAreaLight.L
+                if (pr.light < 0) return mk(0.0f);
+                const DevLight& Lt = sc.lights[pr.light];
+                return dot(it.sh.Z, w) > 0 ? mk(Lt.le[0], Lt.le[1], Lt.le[2]) : mk(0.0f);
+            };
+            bool go = true;
+            if (in.method == 0) {
+                // OnlySampleBxdf (PathTracer.cs:43-93)
+                if (!hit) { L = L + beta * mk(0.0f); go = false; }   // EvalAllInfiniteLights: none
+                else if (in.max_depth != -1 && depth >= in.max_depth) go = false;
+            } else {
+                if (depth == 0 || spec) {
+                    if (hit) L = L + beta * inct_le(wo);
+                    else L = L + beta * mk(0.0f);
+                }
+                if (!hit) go = false;
+                else if (in.max_depth != -1 && depth >= in.max_depth) go = false;
+            }
+            if (go && pr.material < 0) {
+                // medium boundary without a surface: continue straight on
+                want_ext = true;
+                eo = it.P; ed = rd; etmin = rtmin; etmax = kFloatMax;
+                nflags = flags & kFlagSpecular;
+                go = false;
+            } else if (go) {
+                nvert++;
+                MatCtx m = load_material(sc, pr.material, it.u, it.v);
+                f3 wo_l = to_local(it.sh, wo);
+                if (in.method == 0) {
+                    if (pr.light >= 0) L = L + beta * inct_le(wo);
+                } else if (sc.num_lights > 0) {
+                    int l = rng.next_int(sc.num_lights);
+                    float selpdf = 1.0f / (float)sc.num_lights;
+                    const DevLight Lt = sc.lights[l];
+                    LSample ls = light_sample(sc, Lt, it.P, rng);
+                    f3 nee = mk(0.0f);
+                    if (in.method == 2) {
+                        // EstimateDirect, light-sampling half (PathTracer.cs:317-350)
+                        if (ls.pdf > 0.0f && !is_black(ls.li)) {
+                            f3 wi_l = to_local(it.sh, ls.wi);
+                            f3 fr = mat_fr(m, wo_l, wi_l);
+                            float spdf = mat_pdf(m, wo_l, wi_l);
+                            if (!is_black(fr)) {
+                                f3 vec = ls.p - it.P;
+                                float dis = length(vec);
+                                if (!(dis < 0.001f)) {
+                                    want_shadow = true;
+                                    so = it.P; sd = normalize(vec); stmax = dis - 0.0001f;
+                                    fr = fr * abs_cos_theta(to_local(it.sh, ls.wi));
+                                    float w = power_heuristic(ls.pdf, spdf);
+                                    nee = fr * ls.li * w / ls.pdf;
+                                }
+                            }
+                        }
+                        // EstimateDirect, BSDF-sampling half (PathTracer.cs:351-397)
+                        BSample bs = mat_sample(m, wo_l, rng);
+                        f3 fr2 = bs.fr * abs_cos_theta(bs.wi);
+                        float sp2 = bs.pdf;
+                        if (!is_black(fr2) && sp2 > 0) {
+                            float w2 = 1;
+                            bool ok = true;
+                            f3 wi_w = to_world(it.sh, bs.wi);
+                            if (!(bs.type & kSpecular)) {
+                                float lp = light_pdf(sc, Lt, it.P, wi_w);
+                                if (lp == 0.0f) ok = false;
+                                else w2 = power_heuristic(sp2, lp);
+                            }
+                            if (ok) {
+                                want_mis = true;
+                                mo = it.P; md = wi_w;
+                                st3(ps.mis_f, pid, fr2, w2);
+                                ps.mis_li[pid] = make_float4(0.f, 0.f, 0.f, sp2);
+                                ps.sel_light[pid] = l;
+                            }
+                        }
+                    } else {
+                        // NextEventEstimation.EstimateLight (PathTracer.cs:175-189)
+                        if (ls.pdf > 0.0f) {
+                            f3 vec = ls.p - it.P;
+                            float dis = length(vec);
+                            if (!(dis < 0.001f)) {
+                                want_shadow = true;
+                                so = it.P; sd = normalize(vec); stmax = dis - 0.0001f;
+                                f3 wi_l = to_local(it.sh, ls.wi);
+                                f3 fr = mat_fr(m, wo_l, wi_l);
+                                nee = fr * ls.li * abs_cos_theta(wi_l) / ls.pdf;
+                            }
+                        }
+                    }
+                    nflags |= kFlagNeePending;
+                    st3(ps.nee_beta, pid, beta, selpdf);
+                    st3(ps.nee_le, pid, nee);
+                } else if (in.method == 2) {
+                    // SampleLight returned pdf 0: the vertex still adds coeff * 0
+                    nflags |= kFlagNeePending;
+                    st3(ps.nee_beta, pid, beta, 1.0f);
+                    st3(ps.nee_le, pid, mk(0.0f));
+                }
+                if (want_shadow) nflags |= kFlagShadowIssued;
+                if (want_mis) nflags |= kFlagMisIssued;
+                // continuation (PathTracer.cs:227-268)
+                BSample bs = mat_sample(m, wo_l, rng);
+                bool sspec = (bs.type & kSpecular) != 0;
+                if (bs.pdf > 0.0f) {
+                    beta = beta * (bs.fr * abs_cos_theta(bs.wi) / bs.pdf);
+                    ed = to_world(it.sh, bs.wi);
+                    eo = it.P; etmin = 0.0001f; etmax = kFloatMax;
+                    bool alive = true;
+                    if (depth > in.min_depth) {
+                        float qq = nmin(max_elem(beta), in.rr_threshold);
+                        if (rng.next_float() > qq) alive = false;
+                        else beta = beta / qq;
+                    }
+                    if (alive) { want_ext = true; depth++; }
+                }
+                if (sspec) nflags |= kFlagSpecular;
+            }
+            st3(ps.L, pid, L);
+            st3(ps.beta, pid, beta);
+            ps.flags[pid] = nflags;
+            ps.rng_n[pid] = rng.n;
+            ps.depth[pid] = depth;
+        }
+        int slot = wave_append(&counts[cnt_out], want_ext);
+        if (want_ext) {
+            qout.pid[slot] = pid;
+            qout.org[slot] = make_float4(eo.x, eo.y, eo.z, etmin);
+            qout.dir[slot] = make_float4(ed.x, ed.y, ed.z, etmax);
+        }
+        slot = wave_append(&counts[kCntShadow], want_shadow);
+        if (want_shadow) {
+            qshadow.pid[slot] = pid;
+            qshadow.org[slot] = make_float4(so.x, so.y, so.z, 0.0001f);
+            qshadow.dir[slot] = make_float4(sd.x, sd.y, sd.z, stmax);
+        }
+        slot = wave_append(&counts[kCntMis], want_mis);
+        if (want_mis) {
+            qmis.pid[slot] = pid;
+            qmis.org[slot] = make_float4(mo.x, mo.y, mo.z, 0.0001f);
+            qmis.dir[slot] = make_float4(md.x, md.y, md.z, kFloatMax);
+        }
+    }
+    add_counter((unsigned long long*)(counts + kCntVertices), nvert);
+}
+
+hipError_t launch_shade(const DevScene& sc, const DevIntegrator& in, const WaveParams& wp, RayQueue qin,
+                        const float4* hits, RayQueue qout, RayQueue qshadow, RayQueue qmis, PathState ps,
+                        int* counts, int cnt_in, int cnt_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_shade, dim3(persistent_blocks()), dim3(256), 0, s, sc, in, wp, qin, hits, qout, qshadow,
+                       qmis, ps, counts, cnt_in, cnt_out);
+    return hipGetLastError();
+}
+
+// radiance += coeff * SampleLightToEstimateDirect (PathTracer.cs:226, :273-307)
+// for Mis; radiance += coeff * EstimateLight / lightPdf (:81) for Nee.
+__global__ __launch_bounds__(256) void k_resolve(DevIntegrator in, RayQueue qin, PathState ps, int* counts,
+                                                 int cnt_in) {
+    int n = counts[cnt_in];
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+        int pid = qin.pid[k];
+        int flags = ps.flags[pid];
+        if (!(flags & kFlagNeePending)) continue;
+        float4 nb = ps.nee_beta[pid];
+        f3 beta = mk(nb.x, nb.y, nb.z);
+        float selpdf = nb.w;
+        bool vis = (flags & kFlagShadowIssued) && ps.vis[pid];
+        f3 L = ld3(ps.L, pid);
+        if (in.method == 2) {
+            f3 le = mk(0.0f);
+            if (vis) le = le + ld3(ps.nee_le, pid);
+            if (flags & kFlagMisIssued) {
+                float4 li4 = ps.mis_li[pid];
+                f3 li = mk(li4.x, li4.y, li4.z);
+                if (!is_black(li)) {
+                    float4 f4 = ps.mis_f[pid];
+                    le = le + mk(f4.x, f4.y, f4.z) * li * f4.w / li4.w;
+                }
+            }
+            f3 result = mk(0.0f) + le / selpdf;
+            L = L + beta * result;
+        } else {
+            f3 e = vis ? ld3(ps.nee_le, pid) : mk(0.0f);
+            L = L + beta * e / selpdf;
+        }
+        st3(ps.L, pid, L);
+    }
+}
+
+hipError_t launch_resolve(const DevIntegrator& in, RayQueue qin, PathState ps, int* counts, int cnt_in, hipStream_t s) {
+    hipLaunchKernelGGL(k_resolve, dim3(persistent_blocks()), dim3(256), 0, s, in, qin, ps, counts, cnt_in);
+    return hipGetLastError();
+}
+
+// Per-pixel running sums over this wave's samples, in sample order
+// (BlockBasedRenderer.cs:140-146: invalid samples count as 0).
+__global__ __launch_bounds__(256) void k_accumulate(WaveParams wp, PathState ps, float4* sum, float4* sum2,
+                                                    int* counts) {
+    uint64_t invalid = 0;
+    for (int base = blockIdx.x * 256; base < wp.npix; base += gridDim.x * 256) {
+        int q = base + threadIdx.x;
+        if (q < wp.npix) {
+            float4 s4 = sum[q], q4 = sum2[q];
+            f3 s = mk(s4.x, s4.y, s4.z), s2 = mk(q4.x, q4.y, q4.z);
+            for (int j = 0; j < wp.nsamples; j++) {
+                f3 c = ld3(ps.L, j * wp.npix + q);
+                if (!valid_color(c)) { c = mk(0.0f); invalid++; }
+                s = s + c;
+                s2 = s2 + c * c;
+            }
+            sum[q] = make_float4(s.x, s.y, s.z, 0.f);
+            sum2[q] = make_float4(s2.x, s2.y, s2.z, 0.f);
+        }
+    }
+    add_counter((unsigned long long*)(counts + kCntInvalid), invalid);
+}
+
+hipError_t launch_accumulate(const WaveParams& wp, PathState ps, float4* sum, float4* sum2, int* counts, hipStream_t s) {
+    int blocks = (wp.npix + 255) / 256;
+    if (blocks > persistent_blocks()) blocks = persistent_blocks();
+    hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(256), 0, s, wp, ps, sum, sum2, counts);
+    return hipGetLastError();
+}
+
+// sum / spp into the column-major ColorBuffer layout (x*H + y)*3
+// (ColorBuffer.cs:35, BlockBasedRenderer.cs:167-178).
+__global__ __launch_bounds__(256) void k_finalize(WaveParams wp, int height, int spp, const float4* sum,
+                                                  const float4* sum2, float* out, float* out2) {
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < wp.npix; q += gridDim.x * 256) {
+        int yy = q / wp.crop_w;
+        int x = wp.x0 + (q - yy * wp.crop_w), y = wp.y0 + yy;
+        size_t o = ((size_t)x * height + y) * 3;
+        float4 s = sum[q];
+        float fs = (float)spp;
+        out[o] = s.x / fs; out[o + 1] = s.y / fs; out[o + 2] = s.z / fs;
+        if (out2) {
+            float4 s2 = sum2[q];
+            out2[o] = s2.x; out2[o + 1] = s2.y; out2[o + 2] = s2.z;
+        }
+    }
+}
+
+hipError_t launch_finalize(const WaveParams& wp, int height, int spp, const float4* sum, const float4* sum2,
+                           float* out_rgb, float* out_sum2, hipStream_t s) {
+    int blocks = (wp.npix + 255) / 256;
+    if (blocks > persistent_blocks()) blocks = persistent_blocks();
+    hipLaunchKernelGGL(k_finalize, dim3(blocks), dim3(256), 0, s, wp, height, spp, sum, sum2, out_rgb, out_sum2);
+    return hipGetLastError();
+}
+
+__global__ void k_zero4(float4* b, int n) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) b[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+hipError_t launch_zero(float4* buf, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int blocks = (n + 255) / 256;
+    if (blocks > persistent_blocks()) blocks = persistent_blocks();
+    hipLaunchKernelGGL(k_zero4, dim3(blocks), dim3(256), 0, s, buf, n);
+    return hipGetLastError();
+}
+__global__ void k_zero_counts(int* c, int n) {
+    int i = threadIdx.x;
+    if (i < n) c[i] = 0;
+}
+hipError_t launch_zero_counts(int* counts, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, counts, n);
+    return hipGetLastError();
+}
+
+}  // namespace rd

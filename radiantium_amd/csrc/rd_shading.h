@@ -1,0 +1,609 @@
+// rd_shading.h — gfx950 device functions for intersection records, textures,
+// BSDFs and area lights.  Each function restates the reference function it
+// cites; texture parameters are fetched once per shading point into a
+// MatCtx (the reference re-samples them in every Fr/Pdf/Sample call with the
+// same uv, which gives the same values).
+#pragma once
+#include "rd_types.h"
+
+namespace rd {
+
+// ---------------------------------------------------------------- frames
+// Coordinate (Radiantium.Offline/Coordinate.cs:14-36), Duff et al. basis.
+struct Frame {
+    f3 X, Y, Z;
+};
+__device__ __forceinline__ Frame make_frame(f3 n) {
+    float sign = copysignf(1.0f, n.z);
+    float a = -1 / (sign + n.z);
+    float b = n.x * n.y * a;
+    Frame f;
+    f.X = mk(mulsign(sqr(n.x) * a, n.z) + 1.0f, mulsign(b, n.z), mulsign(n.x, -n.z));
+    f.Y = mk(b, sign + sqr(n.y) * a, -n.y);
+    f.Z = n;
+    return f;
+}
+__device__ __forceinline__ f3 to_local(const Frame& f, f3 v) { return mk(dot(v, f.X), dot(v, f.Y), dot(v, f.Z)); }
+__device__ __forceinline__ f3 to_world(const Frame& f, f3 v) { return f.X * v.x + f.Y * v.y + f.Z * v.z; }
+
+// Coordinate helpers (Coordinate.cs:80-122); MathExt.Fma is a*b+c unfused.
+__device__ __forceinline__ float cos_theta(f3 v) { return v.z; }
+__device__ __forceinline__ float abs_cos_theta(f3 v) { return fabsf(v.z); }
+__device__ __forceinline__ float cos2_theta(f3 v) { return sqr(v.z); }
+__device__ __forceinline__ float sin2_theta(f3 v) { return v.x * v.x + sqr(v.y); }
+__device__ __forceinline__ float tan_theta(f3 v) { return safe_sqrt(-v.z * v.z + 1.0f) / v.z; }
+__device__ __forceinline__ float tan2_theta(f3 v) { return nmax(-v.z * v.z + 1.0f, 0.0f) / sqr(v.z); }
+__device__ __forceinline__ float cos2_phi(f3 v) {
+    float s2 = sin2_theta(v);
+    return fabsf(s2) <= 4.0f * kFloatEps ? 1.0f : nclamp(sqr(v.x) / s2, -1.0f, 1.0f);
+}
+__device__ __forceinline__ float sin2_phi(f3 v) {
+    float s2 = sin2_theta(v);
+    return fabsf(s2) <= 4.0f * kFloatEps ? 0.0f : nclamp(sqr(v.y) / s2, -1.0f, 1.0f);
+}
+__device__ __forceinline__ bool same_hemisphere(f3 a, f3 b) { return a.z * b.z > 0; }
+__device__ __forceinline__ f3 faceforward(f3 v, f3 v2) { return dot(v, v2) < 0.0f ? -v : v; }
+
+// ---------------------------------------------------------------- geometry access
+__device__ __forceinline__ f3 load_f3(const float* p, int i) { return mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+
+struct Inct {
+    f3 P;
+    float u, v;   // texture coordinates (Intersection.UV)
+    float T;
+    Frame sh;
+};
+
+// Triangle.GetIntersection (Shapes/Triangle.cs:96-120) / Sphere.GetIntersection
+// (Shapes/Sphere.cs:34-48) for a hit (bu, bv, t) on original primitive `prim`.
+__device__ __forceinline__ Inct make_inct(const DevScene& sc, const DevPrim& pr, f3 ro, f3 rd, float bu, float bv, float t) {
+    Inct it;
+    it.T = t;
+    if (pr.kind == kTri) {
+        int i0 = sc.tri_idx[3 * pr.index], i1 = sc.tri_idx[3 * pr.index + 1], i2 = sc.tri_idx[3 * pr.index + 2];
+        f3 A = load_f3(sc.pos, i0), B = load_f3(sc.pos, i1), C = load_f3(sc.pos, i2);
+        float bx = 1 - (bu + bv), by = bu, bz = bv;
+        it.P = bx * A + by * B + bz * C;
+        f3 n;
+        if (pr.flags & 1) n = normalize(bx * load_f3(sc.nrm, i0) + by * load_f3(sc.nrm, i1) + bz * load_f3(sc.nrm, i2));
+        else n = normalize(cross(B - A, C - A));
+        if (pr.flags & 2) {
+            it.u = bx * sc.uv[2 * i0] + by * sc.uv[2 * i1] + bz * sc.uv[2 * i2];
+            it.v = bx * sc.uv[2 * i0 + 1] + by * sc.uv[2 * i1 + 1] + bz * sc.uv[2 * i2 + 1];
+        } else {
+            it.u = bx * 0.0f + by * 1.0f + bz * 1.0f;
+            it.v = bx * 0.0f + by * 0.0f + bz * 1.0f;
+        }
+        it.sh = make_frame(n);
+    } else {
+        float4 s = sc.spheres[pr.index];
+        f3 c = mk(s.x, s.y, s.z);
+        f3 n = normalize((ro + t * rd) - c);
+        it.P = n * s.w + c;
+        it.sh = make_frame(n);
+        float theta = acosf(-n.y);
+        float phi = atan2f(-n.z, n.x) + kPi;
+        it.u = phi / (2 * kPi);
+        it.v = theta / kPi;
+    }
+    return it;
+}
+
+// Geometric/shading normal only (for Le checks at a hit).
+__device__ __forceinline__ f3 hit_normal(const DevScene& sc, const DevPrim& pr, f3 ro, f3 rd, float bu, float bv, float t) {
+    if (pr.kind == kTri) {
+        int i0 = sc.tri_idx[3 * pr.index], i1 = sc.tri_idx[3 * pr.index + 1], i2 = sc.tri_idx[3 * pr.index + 2];
+        if (pr.flags & 1) {
+            float bx = 1 - (bu + bv), by = bu, bz = bv;
+            return normalize(bx * load_f3(sc.nrm, i0) + by * load_f3(sc.nrm, i1) + bz * load_f3(sc.nrm, i2));
+        }
+        f3 A = load_f3(sc.pos, i0), B = load_f3(sc.pos, i1), C = load_f3(sc.pos, i2);
+        return normalize(cross(B - A, C - A));
+    }
+    float4 s = sc.spheres[pr.index];
+    return normalize((ro + t * rd) - mk(s.x, s.y, s.z));
+}
+
+// Triangle.Intersect(ray, out SurfacePoint) (Triangle.cs:149-177) on vertices.
+__device__ __forceinline__ bool tri_test(f3 A, f3 B, f3 C, f3 o, f3 d, float tmin, float tmax, float* tu, float* tv, float* tt) {
+    f3 e1 = B - A, e2 = C - A;
+    f3 pvec = cross(d, e2);
+    float det = dot(e1, pvec);
+    if (fabsf(det) < kFloatEps) return false;
+    float inv_det = 1.0f / det;
+    f3 tvec = o - A;
+    float u = dot(tvec, pvec) * inv_det;
+    if (u < 0.0f || u > 1.0f) return false;
+    f3 qvec = cross(tvec, e1);
+    float v = dot(d, qvec) * inv_det;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    float t = dot(e2, qvec) * inv_det;
+    if (!(t >= tmin && t <= tmax)) return false;
+    *tu = u; *tv = v; *tt = t;
+    return true;
+}
+
+// Math.Min / Math.Max on doubles (NaN propagates, -0 < +0).
+__device__ __forceinline__ double dmin_net(double a, double b) {
+    if (a != b) { if (!isnan(a)) return a < b ? a : b; return a; }
+    return signbit(a) ? a : b;
+}
+__device__ __forceinline__ double dmax_net(double a, double b) {
+    if (a != b) { if (!isnan(a)) return b < a ? a : b; return a; }
+    return signbit(b) ? a : b;
+}
+
+// Sphere.Intersect(ray, out SurfacePoint) (Sphere.cs:65-80) with
+// MathExt.SolveQuadratic (MathExt.cs:119-138), double precision.
+__device__ __forceinline__ bool sphere_test(float4 s, f3 o, f3 d, float tmin, float tmax, float* tt) {
+    double mint = tmin, maxt = tmax;
+    double ox = (double)o.x - (double)s.x, oy = (double)o.y - (double)s.y, oz = (double)o.z - (double)s.z;
+    double dx = d.x, dy = d.y, dz = d.z;
+    double a = dx * dx + dy * dy + dz * dz;
+    double b = 2 * (ox * dx + oy * dy + oz * dz);
+    double c = (ox * ox + oy * oy + oz * oz) - (double)(s.w * s.w);
+    bool linear = a == 0.0;
+    bool valid_linear = linear && b != 0.0;
+    double x0 = -c / b, x1 = -c / b;
+    double disc = b * b - 4.0 * a * c;
+    bool valid_quad = !linear && (disc >= 0.0);
+    if (valid_quad) {
+        double rdd = sqrt(disc);
+        double temp = -0.5 * (b + copysign(rdd, b));
+        double x0p = temp / a, x1p = c / temp;
+        double x0m = dmin_net(x0p, x1p);
+        double x1m = dmax_net(x0p, x1p);
+        x0 = x0m;
+        x1 = x1m;
+    }
+    bool found = valid_linear || valid_quad;
+    bool out_b = !(x0 <= maxt && x1 >= mint);
+    bool in_b = x0 < mint && x1 > maxt;
+    if (!(found && !out_b && !in_b)) return false;
+    *tt = x0 < mint ? (float)x1 : (float)x0;
+    return true;
+}
+
+// Intersects original primitive `prim` alone (Shape.Intersect).
+__device__ __forceinline__ bool prim_test(const DevScene& sc, const DevPrim& pr, f3 o, f3 d, float tmin, float tmax,
+                                          float* tu, float* tv, float* tt) {
+    if (pr.kind == kTri) {
+        int i0 = sc.tri_idx[3 * pr.index], i1 = sc.tri_idx[3 * pr.index + 1], i2 = sc.tri_idx[3 * pr.index + 2];
+        return tri_test(load_f3(sc.pos, i0), load_f3(sc.pos, i1), load_f3(sc.pos, i2), o, d, tmin, tmax, tu, tv, tt);
+    }
+    *tu = 0; *tv = 0;
+    return sphere_test(sc.spheres[pr.index], o, d, tmin, tmax, tt);
+}
+
+// ---------------------------------------------------------------- textures
+// TextureSampler.Sample (TextureSampler.cs:28-72) over 3-channel column-major
+// texels; constant textures return their colour.
+__device__ __forceinline__ f3 tex_read(const DevScene& sc, const DevTexture& t, int x, int y) {
+    const float* p = sc.texels + t.offset + ((int64_t)x * t.h + y) * 3;
+    return mk(p[0], p[1], p[2]);
+}
+__device__ __forceinline__ f3 tex_sample(const DevScene& sc, int id, float su, float sv) {
+    const DevTexture t = sc.texs[id];
+    if (t.w == 0) return mk(t.v[0], t.v[1], t.v[2]);
+    float u, v;
+    if (t.wrap == 0) { u = nclamp(su, 0.0f, 1.0f); v = nclamp(sv, 0.0f, 1.0f); }
+    else { u = nclamp(su - floorf(su), 0.0f, 1.0f); v = nclamp(sv - floorf(sv), 0.0f, 1.0f); }
+    float fu = u * (t.w - 1), fv = v * (t.h - 1);
+    int pu = (int)fu, pv = (int)fv;
+    if (t.filter == 0) return tex_read(sc, t, pu, pv);
+    int dpu = (fu > pu + 0.5f) ? 1 : -1;
+    int dpv = (fv > pv + 0.5f) ? 1 : -1;
+    int apu = iclamp(pu + dpu, 0, t.w - 1);
+    int apv = iclamp(pv + dpv, 0, t.h - 1);
+    float du = nmin(fabsf(fu - pu - 0.5f), 1.0f);
+    float dv = nmin(fabsf(fv - pv - 0.5f), 1.0f);
+    f3 a = tex_read(sc, t, pu, pv), b = tex_read(sc, t, apu, pv), c = tex_read(sc, t, pu, apv), e = tex_read(sc, t, apu, apv);
+    return (a * (1 - du) + b * du) * (1 - dv) + (c * (1 - du) + e * du) * dv;
+}
+
+// ---------------------------------------------------------------- microfacets
+// Bxdf/Microfacet.cs: Beckmann (kind 0) and GGX (kind 1).
+struct Mf {
+    int kind;
+    float ax, ay;
+};
+__device__ __forceinline__ Mf make_mf(int kind, float roughness, float aniso) {   // RoughnessToAlpha :177-183
+    Mf m;
+    m.kind = kind;
+    m.ax = nmax(roughness * roughness * (1 + aniso), 0.001f);
+    m.ay = nmax(roughness * roughness * (1 - aniso), 0.001f);
+    return m;
+}
+__device__ __forceinline__ float mf_D(const Mf& m, f3 wh) {
+    float t2 = tan2_theta(wh);
+    if (isinf(t2)) return 0.0f;
+    float c4 = cos2_theta(wh) * cos2_theta(wh);
+    if (m.kind == 0)
+        return expf(-t2 * (cos2_phi(wh) / (m.ax * m.ax) + sin2_phi(wh) / (m.ay * m.ay))) / (kPi * m.ax * m.ay * c4);
+    float e = (cos2_phi(wh) / (m.ax * m.ax) + sin2_phi(wh) / (m.ay * m.ay)) * t2;
+    return 1 / (kPi * m.ax * m.ay * c4 * (1 + e) * (1 + e));
+}
+__device__ __forceinline__ float mf_lambda(const Mf& m, f3 w) {
+    float at = fabsf(tan_theta(w));
+    if (isinf(at)) return 0.0f;
+    float alpha = sqrtf(cos2_phi(w) * m.ax * m.ax + sin2_phi(w) * m.ay * m.ay);
+    if (m.kind == 0) {
+        float a = 1 / (alpha * at);
+        if (a >= 1.6f) return 0;
+        return (1 - 1.259f * a + 0.396f * a * a) / (3.535f * a + 2.181f * a * a);
+    }
+    float a2t2 = (alpha * at) * (alpha * at);
+    return (-1 + sqrtf(1.0f + a2t2)) / 2;
+}
+__device__ __forceinline__ float mf_G(const Mf& m, f3 wo, f3 wi) { return 1 / (1 + mf_lambda(m, wo) + mf_lambda(m, wi)); }
+__device__ __forceinline__ float mf_pdf(const Mf& m, f3 wo, f3 wh) { return mf_D(m, wh) * abs_cos_theta(wh); }
+__device__ __forceinline__ f3 spherical_dir(float st, float ct, float phi) {
+    float sp = sinf(phi), cp = cosf(phi);
+    return mk(st * cp, st * sp, ct);
+}
+__device__ __forceinline__ f3 mf_sample_wh(const Mf& m, f3 wo, Rng& rng) {
+    float ux = rng.next_float(), uy = rng.next_float();
+    float phi, ct;
+    if (m.kind == 0) {   // Beckmann.SampleWh (Microfacet.cs:60-86)
+        float t2;
+        if (m.ax == m.ay) {
+            float ls = logf(1 - ux);
+            t2 = -m.ax * m.ax * ls;
+            phi = uy * 2 * kPi;
+        } else {
+            float ls = logf(1 - ux);
+            phi = atanf(m.ay / m.ax * tanf(2 * kPi * uy + 0.5f * kPi));
+            if (uy > 0.5f) phi += kPi;
+            float sp = sinf(phi), cp = cosf(phi);
+            float ax2 = m.ax * m.ax, ay2 = m.ay * m.ay;
+            t2 = -ls / (cp * cp / ax2 + sp * sp / ay2);
+        }
+        ct = 1 / sqrtf(1 + t2);
+        float st = sqrtf(nmax(0.0f, 1 - ct * ct));
+        f3 wh = spherical_dir(st, ct, phi);
+        if (!same_hemisphere(wo, wh)) wh = -wh;
+        return wh;
+    }
+    phi = (2 * kPi) * uy;   // SampleWhGGX (Microfacet.cs:185-209)
+    if (m.ax == m.ay) {
+        float t2 = m.ax * m.ax * ux / (1.0f - ux);
+        ct = 1 / sqrtf(1 + t2);
+    } else {
+        phi = atanf(m.ay / m.ax * tanf(2 * kPi * uy + .5f * kPi));
+        if (uy > 0.5f) phi += kPi;
+        float sp = sinf(phi), cp = cosf(phi);
+        float ax2 = m.ax * m.ax, ay2 = m.ay * m.ay;
+        float a2 = 1 / (cp * cp / ax2 + sp * sp / ay2);
+        float t2 = a2 * ux / (1 - ux);
+        ct = 1 / sqrtf(1 + t2);
+    }
+    float st = sqrtf(nmax(0.0f, 1.0f - ct * ct));
+    f3 wh = spherical_dir(st, ct, phi);
+    if (!same_hemisphere(wo, wh)) wh = -wh;
+    return wh;
+}
+
+// Fresnel.DielectricFunc (Bxdf/Fresnel.cs:50-68)
+__device__ __forceinline__ float fr_dielectric(float ci, float ei, float et) {
+    ci = nclamp(ci, -1, 1);
+    if (!(ci > 0.0f)) { float t = ei; ei = et; et = t; ci = fabsf(ci); }
+    float si = sqrtf(nmax(0, 1 - ci * ci));
+    float st = ei / et * si;
+    if (st >= 1) return 1;
+    float ct = sqrtf(nmax(0, 1 - st * st));
+    float rparl = ((et * ci) - (ei * ct)) / ((et * ci) + (ei * ct));
+    float rperp = ((ei * ci) - (et * ct)) / ((ei * ci) + (et * ct));
+    return (rparl * rparl + rperp * rperp) / 2;
+}
+// Fresnel.ConductorFunc (Bxdf/Fresnel.cs:72-95), etai = 1
+__device__ __forceinline__ f3 fr_conductor(float ci, f3 etai, f3 etat, f3 k) {
+    ci = nclamp(ci, -1, 1);
+    f3 eta = etat / etai;
+    f3 etak = k / etai;
+    float ci2 = ci * ci;
+    float si2 = 1.0f - ci2;
+    f3 eta2 = eta * eta;
+    f3 etak2 = etak * etak;
+    f3 t0 = eta2 - etak2 - si2;
+    f3 a2pb2 = sqrt3(t0 * t0 + 4 * eta2 * etak2);
+    f3 t1 = a2pb2 + ci2;
+    f3 a = sqrt3(0.5f * (a2pb2 + t0));
+    f3 t2 = 2 * ci * a;
+    f3 Rs = (t1 - t2) / (t1 + t2);
+    f3 t3 = ci2 * a2pb2 + si2 * si2;
+    f3 t4 = t2 * si2;
+    f3 Rp = Rs * (t3 - t4) / (t3 + t4);
+    return 0.5f * (Rp + Rs);
+}
+// MathExt.Refract (MathExt.cs:83-92)
+__device__ __forceinline__ bool refract(f3 wi, f3 n, float eta, f3* wt) {
+    float ci = dot(n, wi);
+    float s2i = nmax(0, 1 - ci * ci);
+    float s2t = eta * eta * s2i;
+    if (s2t >= 1) return false;
+    float ct = sqrtf(1 - s2t);
+    *wt = normalize(eta * -wi + (eta * ci - ct) * n);
+    return true;
+}
+
+// ---------------------------------------------------------------- materials
+struct MatCtx {
+    int type, two_sided;
+    Mf mf;
+    f3 r, t;        // diffuse kd / glass r,t
+    f3 eta, k;      // conductor
+    float eta_a, eta_b;
+};
+
+__device__ __forceinline__ MatCtx load_material(const DevScene& sc, int id, float su, float sv) {
+    const DevMaterial& m = sc.mats[id];
+    MatCtx c;
+    c.type = m.type;
+    c.two_sided = m.two_sided;
+    c.eta_a = m.eta_a;
+    c.eta_b = m.eta_b;
+    c.r = mk(0.0f); c.t = mk(0.0f); c.eta = mk(0.0f); c.k = mk(0.0f);
+    c.mf.kind = 0; c.mf.ax = c.mf.ay = 0.001f;
+    switch (m.type) {
+    case kMatDiffuse:
+        c.r = tex_sample(sc, m.tex[0], su, sv);
+        break;
+    case kMatRoughMetal:   // RoughMetal.GetParam (RoughMetal.cs:28-31)
+        c.mf = make_mf(m.dist, tex_sample(sc, m.tex[0], su, sv).x, tex_sample(sc, m.tex[1], su, sv).x);
+        c.eta = mk(m.eta[0], m.eta[1], m.eta[2]);
+        c.k = mk(m.k[0], m.k[1], m.k[2]);
+        break;
+    case kMatRoughGlass:   // RoughGlass.GetParam (RoughGlass.cs:146-149)
+        c.mf = make_mf(m.dist, tex_sample(sc, m.tex[2], su, sv).x, tex_sample(sc, m.tex[3], su, sv).x);
+        c.r = tex_sample(sc, m.tex[0], su, sv);
+        c.t = tex_sample(sc, m.tex[1], su, sv);
+        break;
+    default:
+        break;
+    }
+    return c;
+}
+
+// MicrofacetReflectionBrdf<Conductor,D>.Fr with R = 1 (MicrofacetReflectionBrdf.cs:26-41)
+__device__ __forceinline__ f3 conductor_fr(const MatCtx& c, f3 wo, f3 wi) {
+    if (!same_hemisphere(wo, wi)) return mk(0.0f);
+    float cto = abs_cos_theta(wo), cti = abs_cos_theta(wi);
+    f3 wh = wi + wo;
+    if (cti == 0 || cto == 0) return mk(0.0f);
+    if (wh.x == 0 && wh.y == 0 && wh.z == 0) return mk(0.0f);
+    wh = normalize(wh);
+    f3 F = fr_conductor(dot(wi, faceforward(wh, mk(0, 0, 1))), mk(1.0f), c.eta, c.k);
+    float D = mf_D(c.mf, wh);
+    float G = mf_G(c.mf, wo, wi);
+    return mk(1.0f) * fabs3((D * G * F) / (4 * cti * cto));
+}
+
+// Material.Fr (Materials/Diffuse.cs:21-27, RoughMetal.cs:33-58, RoughGlass.cs:151-195)
+__device__ __forceinline__ f3 mat_fr(const MatCtx& c, f3 wo, f3 wi) {
+    switch (c.type) {
+    case kMatDiffuse:
+        if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!c.two_sided) return mk(0.0f); }
+        return same_hemisphere(wo, wi) ? c.r / kPi : mk(0.0f);
+    case kMatRoughMetal:
+        if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!c.two_sided) return mk(0.0f); }
+        return conductor_fr(c, wo, wi);
+    case kMatRoughGlass: {
+        bool is_reflect = same_hemisphere(wo, wi);
+        bool entering = cos_theta(wo) > 0;
+        float ei = entering ? c.eta_a : c.eta_b;
+        float et = entering ? c.eta_b : c.eta_a;
+        float eta = is_reflect ? 1 : et / ei;
+        f3 wh = normalize(wo + wi * eta);
+        if (cos_theta(wh) < 0) wh = -wh;
+        float sd = dot(wo, wh) + eta * dot(wi, wh);
+        float D = mf_D(c.mf, wh);
+        float G = mf_G(c.mf, wo, wi);
+        float F = fr_dielectric(dot(wo, wh), ei, et);
+        f3 bsdf;
+        if (is_reflect) {
+            bsdf = c.r * (D * G * F / (4.0f * cos_theta(wi) * cos_theta(wo)));
+        } else {
+            if (dot(wo, wh) * dot(wi, wh) > 0) return mk(0.0f);
+            float factor = 1 / eta;
+            bsdf = (1 - F) * c.t *
+                   fabsf(D * G * eta * eta * abs_dot(wi, wh) * abs_dot(wo, wh) * factor * factor /
+                         (cos_theta(wo) * cos_theta(wi) * sd * sd));
+        }
+        return valid_color(bsdf) ? bsdf : mk(0.0f);
+    }
+    default:
+        return mk(0.0f);
+    }
+}
+
+// Material.Pdf (Diffuse.cs:29-37, RoughMetal.cs:60-85, RoughGlass.cs:197-238)
+__device__ __forceinline__ float mat_pdf(const MatCtx& c, f3 wo, f3 wi) {
+    switch (c.type) {
+    case kMatDiffuse:
+        if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!c.two_sided) return 0.0f; }
+        return same_hemisphere(wo, wi) ? abs_cos_theta(wi) / kPi : 0.0f;
+    case kMatRoughMetal: {
+        if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!c.two_sided) return 0.0f; }
+        if (!same_hemisphere(wo, wi)) return 0;
+        f3 wh = normalize(wo + wi);
+        return mf_pdf(c.mf, wo, wh) / (4 * dot(wo, wh));
+    }
+    case kMatRoughGlass: {
+        bool is_reflect = same_hemisphere(wo, wi);
+        bool entering = cos_theta(wo) > 0;
+        float ei = entering ? c.eta_a : c.eta_b;
+        float et = entering ? c.eta_b : c.eta_a;
+        float pdf;
+        if (is_reflect) {
+            f3 wh = normalize(wo + wi);
+            if (cos_theta(wh) < 0) wh = -wh;
+            if (dot(wo, wh) * cos_theta(wo) <= 0 || dot(wi, wh) * cos_theta(wi) <= 0) return 0.0f;
+            float jac = 1.0f / (4.0f * abs_dot(wh, wi));
+            float F = fr_dielectric(dot(wo, wh), ei, et);
+            pdf = mf_pdf(c.mf, wo, wh) * jac * F;
+        } else {
+            float eta = et / ei;
+            f3 wh = normalize(wo + wi * eta);
+            if (cos_theta(wh) < 0) wh = -wh;
+            if (dot(wo, wh) * dot(wi, wh) > 0) return 0.0f;
+            if (dot(wo, wh) * cos_theta(wo) <= 0 || dot(wi, wh) * cos_theta(wi) <= 0) return 0.0f;
+            float sd = dot(wo, wh) + eta * dot(wi, wh);
+            float dwh = fabsf((eta * eta * dot(wi, wh)) / (sd * sd));
+            float F = fr_dielectric(dot(wo, wh), ei, et);
+            pdf = mf_pdf(c.mf, wo, wh) * dwh * (1 - F);
+        }
+        return isinf(pdf) || isnan(pdf) ? 0.0f : pdf;
+    }
+    default:
+        return 0.0f;
+    }
+}
+
+struct BSample {
+    f3 wi, fr;
+    float pdf;
+    int type;
+};
+
+// Material.Sample (Diffuse.cs:39-49 + LambertianReflectionBrdf.cs:26-36,
+// RoughMetal.cs:87-112 + MicrofacetReflectionBrdf.cs:50-59, RoughGlass.cs:240-297)
+__device__ __forceinline__ BSample mat_sample(const MatCtx& c, f3 wo, Rng& rng) {
+    BSample s;
+    s.wi = mk(0.0f); s.fr = mk(0.0f); s.pdf = 0.0f; s.type = 0;
+    switch (c.type) {
+    case kMatDiffuse: {
+        if (cos_theta(wo) <= 0) { if (!c.two_sided) return s; }
+        float ux = rng.next_float(), uy = rng.next_float();
+        float radius = sqrtf(ux);
+        float angle = uy * kPi * 2;
+        float x = radius * cosf(angle), y = radius * sinf(angle);
+        f3 wi = normalize(mk(x, y, sqrtf(1 - x * x - y * y)));
+        if (wo.z < 0) wi.z *= -1;
+        if (!same_hemisphere(wo, wi)) return s;
+        s.wi = wi;
+        s.pdf = same_hemisphere(wo, wi) ? abs_cos_theta(wi) / kPi : 0.0f;
+        s.fr = c.r / kPi;
+        s.type = kReflection | kDiffuse;
+        return s;
+    }
+    case kMatRoughMetal: {
+        if (cos_theta(wo) <= 0) { if (!c.two_sided) return s; }
+        if (cos_theta(wo) == 0) return s;
+        f3 wh = mf_sample_wh(c.mf, wo, rng);
+        if (dot(wo, wh) < 0) return s;
+        f3 wi = reflect(-wo, wh);
+        if (!same_hemisphere(wo, wi)) return s;
+        s.pdf = mf_pdf(c.mf, wo, wh) / (4 * dot(wi, wh));
+        s.fr = conductor_fr(c, wo, wi);
+        s.wi = wi;
+        s.type = kReflection | kGlossy;
+        return s;
+    }
+    case kMatRoughGlass: {
+        f3 wh = mf_sample_wh(c.mf, wo, rng);
+        float F = fr_dielectric(dot(wo, wh), c.eta_a, c.eta_b);
+        if (rng.next_float() < F) {
+            f3 wi = reflect(-wo, wh);
+            if (!same_hemisphere(wo, wi)) return s;
+            float D = mf_D(c.mf, wh);
+            float G = mf_G(c.mf, wo, wi);
+            f3 brdf = c.r * (D * G * F / (4.0f * cos_theta(wi) * cos_theta(wo)));
+            float jac = 1.0f / (4.0f * abs_dot(wh, wi));
+            float pdf = mf_pdf(c.mf, wo, wh) * jac * F;
+            if (isinf(pdf) || isnan(pdf) || !valid_color(brdf)) return s;
+            s.wi = wi; s.fr = brdf; s.pdf = pdf; s.type = kReflection | kGlossy;
+            return s;
+        }
+        bool entering = cos_theta(wo) > 0;
+        float ei = entering ? c.eta_a : c.eta_b;
+        float et = entering ? c.eta_b : c.eta_a;
+        float eta = et / ei;
+        f3 wi;
+        if (refract(wo, wh, ei / et, &wi)) {
+            if (dot(wo, wh) * dot(wi, wh) > 0) return s;
+        } else {
+            wi = reflect(-wo, wh);
+            eta = 1;
+        }
+        float sd = dot(wo, wh) + eta * dot(wi, wh);
+        float factor = 1 / eta;
+        float D = mf_D(c.mf, wh);
+        float G = mf_G(c.mf, wo, wi);
+        f3 btdf = (1 - F) * c.t *
+                  fabsf(D * G * eta * eta * abs_dot(wi, wh) * abs_dot(wo, wh) * factor * factor /
+                        (cos_theta(wo) * cos_theta(wi) * sd * sd));
+        float dwh = fabsf((eta * eta * dot(wi, wh)) / (sd * sd));
+        float pdf = mf_pdf(c.mf, wo, wh) * dwh * (1 - F);
+        if (isinf(pdf) || isnan(pdf) || !valid_color(btdf)) return s;
+        s.wi = wi; s.fr = btdf; s.pdf = pdf; s.type = kTransmission | kGlossy;
+        return s;
+    }
+    default:
+        return s;
+    }
+}
+
+// ---------------------------------------------------------------- area lights
+struct LSample {
+    f3 p, wi, li;
+    float pdf;
+};
+
+// DiffuseAreaLight.SampleLi (Lights/DiffuseAreaLight.cs:48-68) with
+// Triangle.Sample (Triangle.cs:184-212) / Sphere.Sample (Sphere.cs:87-102).
+__device__ __forceinline__ LSample light_sample(const DevScene& sc, const DevLight& L, f3 refp, Rng& rng) {
+    const DevPrim pr = sc.prim_info[L.prim];
+    float ux = rng.next_float(), uy = rng.next_float();
+    f3 P, N;
+    if (pr.kind == kTri) {
+        int i0 = sc.tri_idx[3 * pr.index], i1 = sc.tri_idx[3 * pr.index + 1], i2 = sc.tri_idx[3 * pr.index + 2];
+        float alpha = 1 - sqrtf(1 - ux);
+        float beta = uy * sqrtf(1 - ux);
+        float gamma = (1 - alpha - beta);
+        f3 A = load_f3(sc.pos, i0), B = load_f3(sc.pos, i1), C = load_f3(sc.pos, i2);
+        P = alpha * A + beta * B + gamma * C;
+        if (pr.flags & 1) N = normalize(alpha * load_f3(sc.nrm, i0) + beta * load_f3(sc.nrm, i1) + gamma * load_f3(sc.nrm, i2));
+        else N = normalize(cross(B - A, C - A));
+    } else {
+        float4 s = sc.spheres[pr.index];
+        float phi = ux * kPi * 2;   // Probability.SquareToUniformSphere (Probability.cs:62-71)
+        float theta = acosf(1 - 2 * uy);
+        f3 v = mk(sinf(theta) * cosf(phi), sinf(theta) * sinf(phi), cosf(theta));
+        N = normalize(v);
+        P = N * s.w + mk(s.x, s.y, s.z);
+    }
+    float spdf = 1.0f / L.area;
+    LSample out;
+    f3 wi = P - refp;
+    f3 le = mk(L.le[0], L.le[1], L.le[2]);
+    out.li = dot(N, -wi) > 0 ? le : mk(0.0f);
+    if (length_sq(wi) == 0) {
+        out.pdf = 0.0f;
+    } else {
+        wi = normalize(wi);
+        out.pdf = spdf * length_sq(refp - P) / abs_dot(N, -wi);
+        if (isinf(out.pdf)) out.pdf = 0.0f;
+    }
+    out.p = P;
+    out.wi = wi;
+    return out;
+}
+
+// DiffuseAreaLight.PdfLi (DiffuseAreaLight.cs:31-46): tests the emitter alone.
+__device__ __forceinline__ float light_pdf(const DevScene& sc, const DevLight& L, f3 refp, f3 wi) {
+    const DevPrim pr = sc.prim_info[L.prim];
+    float tu, tv, tt;
+    if (!prim_test(sc, pr, refp, wi, kFloatEps, kFloatMax, &tu, &tv, &tt)) return 0.0f;
+    Inct h = make_inct(sc, pr, refp, wi, tu, tv, tt);
+    float pdf = length_sq(refp - h.P) / (abs_dot(h.sh.Z, -wi) * L.area);
+    if (isinf(pdf)) pdf = 0.0f;
+    return pdf;
+}
+
+// PathUtility.PowerHeuristic (Integrators/PathUtility.cs:5-9), in double.
+__device__ __forceinline__ float power_heuristic(float fp, float gp) {
+    double f = (double)fp, g = (double)gp;
+    return (float)((f * f) / (f * f + g * g));
+}
+
+}  // namespace rd

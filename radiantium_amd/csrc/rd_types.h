@@ -1,0 +1,102 @@
+// rd_types.h — the scene and wavefront layouts held in HBM.
+//
+// BVH nodes keep the reference's 32-byte LinearBVHNode (Accelerators/Bvh.cs:18-26)
+// so one node is two 16-byte loads:
+//   float4 lo = {min.x, min.y, min.z, max.x}
+//   float4 hi = {max.y, max.z, offset (int bits), count | axis << 16 (int bits)}
+// offset is PrimitivesIndex for leaves and SecondChildIndex for interior nodes.
+// Primitives are stored in BVH leaf order, 48 bytes each (three float4):
+//   triangle: {a.xyz, orig}, {b.xyz, kind=0}, {c.xyz, 0}
+//   sphere:   {center.xyz, orig}, {radius, 0, 0, kind=1}, {0,0,0,0}
+// so a leaf's primitives are contiguous and each test reads 36 (triangle) or
+// 16 (sphere) payload bytes.
+#pragma once
+#include <stdint.h>
+#include "rd_math.h"
+
+namespace rd {
+
+enum : int { kTri = 0, kSphere = 1 };
+enum : int {
+    kMatDiffuse = 0, kMatRoughMetal = 1, kMatRoughGlass = 2, kMatRoughPlastic = 3,
+    kMatPerfectGlass = 4, kMatPerfectMirror = 5, kMatDisney = 6
+};
+// BxdfType flags (IBxdf.cs:7-15)
+enum : int { kReflection = 1, kTransmission = 2, kDiffuse = 4, kGlossy = 8, kSpecular = 16 };
+
+struct DevTexture {
+    int w, h, wrap, filter;
+    float v[3];
+    int _pad;
+    int64_t offset;  // float offset into texel pool (3 floats per texel)
+};
+
+struct DevMaterial {
+    int type, two_sided, dist, thin;
+    int tex[16];
+    float eta_a, eta_b, kd, ks;
+    float eta[3], k[3];
+    float _pad[2];
+};
+
+struct DevLight {
+    int type;     // 0 area, 1 infinite
+    int prim;     // original primitive index of the emitter shape
+    float le[3];
+    int tex;
+    float area;   // Shape.SurfaceArea of the emitter
+    int _pad;
+};
+
+struct DevMedium {
+    float sigma_a[3], sigma_s[3], g, scale;
+};
+
+// Per original primitive (rd_prim_desc order): shading record.
+struct DevPrim {
+    int kind;      // kTri / kSphere
+    int index;     // triangle or sphere index
+    int material;  // -1: no surface
+    int light;     // -1: not emissive
+    int med_in, med_out;
+    float area;
+    int flags;     // triangles: bit0 has normal, bit1 has uv
+};
+
+struct DevCamera {
+    float clip_to_camera[16];   // row-major M11..M44
+    float camera_to_world[16];
+    float inv_res_x, inv_res_y, near_z, far_z;
+    int width, height;
+};
+
+struct DevScene {
+    const float4* nodes;   // 2 per node
+    const float4* prims;   // 3 per ordered primitive
+    int num_nodes;
+    int num_prims;
+    int stack_needed;      // deepest interior path (entries a DFS stack can hold)
+    int num_lights;
+    const DevPrim* prim_info;
+    const int* tri_idx;
+    const float* pos;
+    const float* nrm;
+    const float* uv;
+    const float4* spheres;
+    const DevMaterial* mats;
+    const DevTexture* texs;
+    const float* texels;
+    const DevLight* lights;
+    const DevMedium* media;
+    int global_medium;
+    int _pad;
+    DevCamera cam;
+};
+
+// Integrator parameters (PathTracer ctor, Integrators/PathTracer.cs:34-41)
+struct DevIntegrator {
+    int type, max_depth, min_depth, method, strategy;
+    float rr_threshold;
+};
+
+}  // namespace rd

@@ -1,0 +1,220 @@
+"""ctypes binding of libradiantium_hip.so (the C ABI of include/radiantium_hip.h).
+
+The library is the only compute path: there is no CPU fallback, and every
+entry point raises if the library or a HIP device is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libradiantium_hip.so")
+_lib = None
+
+EXPORTS = [
+    "rd_abi_version", "rd_last_error", "rd_create", "rd_destroy", "rd_scene_create", "rd_scene_destroy",
+    "rd_scene_get_info", "rd_scene_export_bvh", "rd_trace_closest", "rd_trace_any",
+    "rd_trace_closest_device", "rd_trace_any_device", "rd_render", "rd_render_device",
+    "rd_get_render_stats", "rd_bvh_build_host", "rd_device_alloc", "rd_device_free", "rd_memcpy_h2d", "rd_memcpy_d2h",
+    "rd_synchronize", "rd_kernel_timing", "rd_reset_kernel_timing",
+]
+
+
+class RadiantiumError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{abi.STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: build it with `python -m radiantium_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, u64 = C.c_void_p, C.c_int32, C.c_uint64
+        L.rd_abi_version.restype = C.c_int
+        L.rd_last_error.restype = C.c_char_p
+        L.rd_create.restype = vp
+        L.rd_create.argtypes = [C.c_int]
+        L.rd_destroy.argtypes = [vp]
+        L.rd_scene_create.restype = vp
+        L.rd_scene_create.argtypes = [vp, C.POINTER(abi.rd_scene_desc)]
+        L.rd_scene_destroy.argtypes = [vp]
+        L.rd_scene_get_info.argtypes = [vp, C.POINTER(abi.rd_scene_info)]
+        L.rd_scene_export_bvh.argtypes = [vp, vp, i32, vp, i32]
+        L.rd_trace_closest.argtypes = [vp, vp, vp, i32, vp]
+        L.rd_trace_any.argtypes = [vp, vp, vp, i32, vp]
+        L.rd_trace_closest_device.argtypes = [vp, vp, vp, i32, vp, vp, vp]
+        L.rd_trace_any_device.argtypes = [vp, vp, vp, i32, vp, vp, vp]
+        L.rd_render.argtypes = [vp, vp, C.POINTER(abi.rd_render_params), vp, vp, vp, vp]
+        L.rd_render_device.argtypes = [vp, vp, C.POINTER(abi.rd_render_params), vp, vp, vp, vp]
+        L.rd_get_render_stats.argtypes = [vp, C.POINTER(abi.rd_render_stats)]
+        L.rd_device_alloc.restype = vp
+        L.rd_device_alloc.argtypes = [vp, u64]
+        L.rd_device_free.argtypes = [vp, vp]
+        L.rd_memcpy_h2d.argtypes = [vp, vp, vp, u64]
+        L.rd_memcpy_d2h.argtypes = [vp, vp, vp, u64]
+        L.rd_synchronize.argtypes = [vp]
+        L.rd_kernel_timing.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.rd_reset_kernel_timing.argtypes = [vp]
+        L.rd_bvh_build_host.argtypes = [vp, vp, i32, i32, vp, i32, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise RadiantiumError(rc, lib().rd_last_error().decode())
+    return rc
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def bvh_build_host(bmin, bmax, max_prim):
+    """Host SAH build (no device needed): returns (nodes [N,32] uint8, order, max_depth)."""
+    bmin = np.ascontiguousarray(bmin, np.float32)
+    bmax = np.ascontiguousarray(bmax, np.float32)
+    n = bmin.shape[0]
+    cap = max(2 * n - 1, 1)
+    nodes = np.zeros(cap * 32, np.uint8)
+    order = np.zeros(max(n, 1), np.int32)
+    nn = C.c_int32(0)
+    md = C.c_int32(0)
+    check(lib().rd_bvh_build_host(_ptr(bmin), _ptr(bmax), n, max_prim, _ptr(nodes), cap, _ptr(order),
+                                  C.byref(nn), C.byref(md)))
+    return nodes[: nn.value * 32].reshape(-1, 32), order[:n], md.value
+
+
+class Context:
+    """One HIP device (rd_create)."""
+
+    def __init__(self, device=0):
+        self.h = lib().rd_create(device)
+        if not self.h:
+            raise RadiantiumError(abi.RD_ERR_DEVICE, lib().rd_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def alloc(self, nbytes):
+        p = lib().rd_device_alloc(self.h, nbytes)
+        if not p:
+            raise RadiantiumError(abi.RD_ERR_OOM, lib().rd_last_error().decode())
+        return p
+
+    def free(self, p):
+        check(lib().rd_device_free(self.h, p))
+
+    def h2d(self, dst, arr):
+        arr = np.ascontiguousarray(arr)
+        check(lib().rd_memcpy_h2d(self.h, dst, _ptr(arr), arr.nbytes))
+
+    def d2h(self, arr, src):
+        check(lib().rd_memcpy_d2h(self.h, _ptr(arr), src, arr.nbytes))
+        return arr
+
+    def synchronize(self):
+        check(lib().rd_synchronize(self.h))
+
+    def kernel_timing(self, stage):
+        avg = C.c_double(0)
+        cnt = C.c_int64(0)
+        check(lib().rd_kernel_timing(self.h, stage.encode(), C.byref(avg), C.byref(cnt)))
+        return avg.value, cnt.value
+
+    def reset_kernel_timing(self):
+        check(lib().rd_reset_kernel_timing(self.h))
+
+    def render_stats(self):
+        st = abi.rd_render_stats()
+        check(lib().rd_get_render_stats(self.h, C.byref(st)))
+        out = {}
+        for f, _ in abi.rd_render_stats._fields_:
+            v = getattr(st, f)
+            out[f] = list(v) if f.startswith("class_") else v
+        return out
+
+
+class Scene:
+    """A scene resident in HBM (rd_scene_create: host SAH build + upload)."""
+
+    def __init__(self, ctx, scene_desc):
+        self.ctx = ctx
+        self.desc = scene_desc
+        self._c = scene_desc.to_c()
+        self.h = lib().rd_scene_create(ctx.h, C.byref(self._c))
+        if not self.h:
+            raise RadiantiumError(abi.RD_ERR_INVALID, lib().rd_last_error().decode())
+        self.width = scene_desc.camera["width"]
+        self.height = scene_desc.camera["height"]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rd_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def info(self):
+        i = abi.rd_scene_info()
+        check(lib().rd_scene_get_info(self.h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in abi.rd_scene_info._fields_}
+
+    def export_bvh(self):
+        inf = self.info()
+        nodes = np.zeros(inf["num_nodes"] * 32, np.uint8)
+        order = np.zeros(inf["num_prims"], np.int32)
+        check(lib().rd_scene_export_bvh(self.h, _ptr(nodes), inf["num_nodes"], _ptr(order), inf["num_prims"]))
+        return nodes.reshape(-1, 32), order
+
+    def trace_closest(self, rays):
+        rays = np.ascontiguousarray(rays, abi.ray_dtype())
+        hits = np.zeros(rays.shape[0], abi.hit_dtype())
+        check(lib().rd_trace_closest(self.ctx.h, self.h, _ptr(rays), rays.shape[0], _ptr(hits)))
+        return hits
+
+    def trace_any(self, rays):
+        rays = np.ascontiguousarray(rays, abi.ray_dtype())
+        occ = np.zeros(rays.shape[0], np.uint8)
+        check(lib().rd_trace_any(self.ctx.h, self.h, _ptr(rays), rays.shape[0], _ptr(occ)))
+        return occ
+
+    def trace_closest_device(self, d_rays, n, d_hits, d_counters=None):
+        check(lib().rd_trace_closest_device(self.ctx.h, self.h, d_rays, n, d_hits, d_counters, None))
+
+    def trace_any_device(self, d_rays, n, d_occ, d_counters=None):
+        check(lib().rd_trace_any_device(self.ctx.h, self.h, d_rays, n, d_occ, d_counters, None))
+
+    def render(self, spp, seed=0, crop=None, wave_paths=0, with_sum2=True, cancel=None, progress=None):
+        """Returns (mean rgb [W*H*3] column-major, sum2 or None)."""
+        p = abi.rd_render_params()
+        p.spp, p.seed, p.wave_paths = spp, seed, wave_paths
+        if crop is not None:
+            p.x0, p.y0, p.x1, p.y1 = crop
+        n = self.width * self.height * 3
+        rgb = np.zeros(n, np.float32)
+        s2 = np.zeros(n, np.float32) if with_sum2 else None
+        check(lib().rd_render(self.ctx.h, self.h, C.byref(p), _ptr(rgb), _ptr(s2) if s2 is not None else None,
+                              C.cast(cancel, C.c_void_p) if cancel is not None else None,
+                              C.cast(progress, C.c_void_p) if progress is not None else None))
+        return rgb, s2
+
+    def render_device(self, spp, d_rgb, d_sum2=None, seed=0, crop=None, wave_paths=0):
+        p = abi.rd_render_params()
+        p.spp, p.seed, p.wave_paths = spp, seed, wave_paths
+        if crop is not None:
+            p.x0, p.y0, p.x1, p.y1 = crop
+        check(lib().rd_render_device(self.ctx.h, self.h, C.byref(p), d_rgb, d_sum2, None, None))

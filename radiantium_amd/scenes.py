@@ -1,0 +1,212 @@
+"""Deterministic procedural scenes for the benchmark workloads.
+
+Every generator is pure numpy with explicit seeds, so the product and the
+oracle receive bit-identical float32 vertices.  Geometry is baked to world
+space with identity entity transforms, so TriangleMesh keeps positions
+verbatim (Radiantium.Offline/Shapes/Triangle.cs:15-29).
+
+Conventions follow the reference's Cornell examples (Example/cbox/*.json):
+y up, camera on +z looking toward -z, area lights facing -y, one-sided
+diffuse walls whose geometric normal (cross(Pb-Pa, Pc-Pa)) faces the room.
+"""
+import math
+
+import numpy as np
+
+from .scene import SceneDesc
+
+
+def _quad(p0, p1, p2, p3):
+    """Two triangles (p0,p1,p2), (p0,p2,p3); normal = cross(p1-p0, p2-p0)."""
+    pos = np.array([p0, p1, p2, p3], np.float32)
+    idx = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    return pos, idx
+
+
+def _box(size, center, angle_deg):
+    """Axis-aligned box of extent `size` (sx, sy, sz) resting on y=center[1],
+    rotated about y by angle_deg, outward-facing normals, 12 triangles."""
+    sx, sy, sz = size
+    hx, hz = sx / 2, sz / 2
+    c = [(-hx, 0, -hz), (hx, 0, -hz), (hx, 0, hz), (-hx, 0, hz),
+         (-hx, sy, -hz), (hx, sy, -hz), (hx, sy, hz), (-hx, sy, hz)]
+    a = math.radians(angle_deg)
+    ca, sa = math.cos(a), math.sin(a)
+    pts = []
+    for x, y, z in c:
+        rx = ca * x + sa * z
+        rz = -sa * x + ca * z
+        pts.append((rx + center[0], y + center[1], rz + center[2]))
+    pos = np.array(pts, np.float32)
+    # faces wound so cross(b-a, c-a) points outward
+    quads = [(4, 7, 6, 5),   # top (+y)
+             (0, 1, 2, 3),   # bottom (-y)
+             (3, 2, 6, 7),   # +z
+             (1, 0, 4, 5),   # -z
+             (2, 1, 5, 6),   # +x
+             (0, 3, 7, 4)]   # -x
+    idx = []
+    for q in quads:
+        idx.append([q[0], q[1], q[2]])
+        idx.append([q[0], q[2], q[3]])
+    return pos, np.array(idx, np.int32)
+
+
+def cornell_box(width=256, height=256, max_depth=8, method="Mis"):
+    """Plain path integrator Cornell box: five unit wall quads (grey 0.73,
+    red 0.65/0.05/0.05, green 0.12/0.45/0.15), boxes of size 0.3 and 0.6
+    (0.3 x 0.6 x 0.3) rotated +-17 degrees about y, a 0.25 x 0.25 ceiling
+    quad light of radiance (17,12,4); all Lambertian, one-sided.
+    Room: x in [-0.5,0.5], y in [0,1], z in [-0.5,0.5], open at +z."""
+    s = SceneDesc()
+    grey = s.add_material("diffuse", kd=(0.73, 0.73, 0.73))
+    red = s.add_material("diffuse", kd=(0.65, 0.05, 0.05))
+    green = s.add_material("diffuse", kd=(0.12, 0.45, 0.15))
+    light_mat = s.add_material("diffuse", kd=(0.0, 0.0, 0.0))
+    L, H = -0.5, 0.5
+    # floor (normal +y)
+    s.add_mesh(*_quad((L, 0, L), (L, 0, H), (H, 0, H), (H, 0, L)), material=grey)
+    # ceiling (normal -y)
+    s.add_mesh(*_quad((L, 1, L), (H, 1, L), (H, 1, H), (L, 1, H)), material=grey)
+    # back wall z=-0.5 (normal +z)
+    s.add_mesh(*_quad((L, 0, L), (H, 0, L), (H, 1, L), (L, 1, L)), material=grey)
+    # left wall x=-0.5, red (normal +x)
+    s.add_mesh(*_quad((L, 0, L), (L, 1, L), (L, 1, H), (L, 0, H)), material=red)
+    # right wall x=+0.5, green (normal -x)
+    s.add_mesh(*_quad((H, 0, L), (H, 0, H), (H, 1, H), (H, 1, L)), material=green)
+    # short box (0.3 cube) rotated +17, tall box (0.3 x 0.6 x 0.3) rotated -17
+    s.add_mesh(*_box((0.3, 0.3, 0.3), (0.17, 0.0, 0.12), 17.0), material=grey)
+    s.add_mesh(*_box((0.3, 0.6, 0.3), (-0.17, 0.0, -0.15), -17.0), material=grey)
+    # ceiling light 0.25 x 0.25 just below the ceiling, facing -y
+    q = 0.125
+    y = 1.0 - 1e-3
+    s.add_mesh(*_quad((-q, y, -q), (q, y, -q), (q, y, q), (-q, y, q)), material=light_mat, le=(17.0, 12.0, 4.0))
+    s.set_camera(fov=40.0, width=width, height=height, origin=(0.0, 0.5, 1.85), target=(0.0, 0.5, 0.0),
+                 up=(0.0, 1.0, 0.0))
+    s.set_integrator(type="path", max_depth=max_depth, method=method)
+    s.bvh_max_prim = 1
+    return s
+
+
+# ---------------------------------------------------------------- icosphere
+def icosphere(level):
+    """Unit icosphere: 20*4^level faces, midpoint subdivision with shared
+    edge vertices, projected to the sphere after each level."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0],
+                  [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11],
+                  [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8],
+                  [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+                  [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    for _ in range(level):
+        nv = v.shape[0]
+        nf = f.shape[0]
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        es = np.sort(e, axis=1)
+        key = es[:, 0] * nv + es[:, 1]
+        uniq, inv = np.unique(key, return_inverse=True)
+        a = uniq // nv
+        b = uniq % nv
+        mid = v[a] + v[b]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        v = np.concatenate([v, mid])
+        m01 = inv[0:nf] + nv
+        m12 = inv[nf:2 * nf] + nv
+        m20 = inv[2 * nf:3 * nf] + nv
+        c0 = np.stack([f[:, 0], m01, m20], 1)
+        c1 = np.stack([f[:, 1], m12, m01], 1)
+        c2 = np.stack([f[:, 2], m20, m12], 1)
+        c3 = np.stack([m01, m12, m20], 1)
+        f = np.stack([c0, c1, c2, c3], 1).reshape(-1, 3)
+    return v, f
+
+
+def _hash3(ix, iy, iz, seed):
+    """Integer lattice hash -> [0,1) (value-noise lattice values)."""
+    h = (ix.astype(np.uint64) * np.uint64(0x8DA6B343) ^ iy.astype(np.uint64) * np.uint64(0xD8163841)
+         ^ iz.astype(np.uint64) * np.uint64(0xCB1AB31F) ^ np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15))
+    h ^= h >> np.uint64(33)
+    h *= np.uint64(0xFF51AFD7ED558CCD)
+    h ^= h >> np.uint64(33)
+    h *= np.uint64(0xC4CEB9FE1A85EC53)
+    h ^= h >> np.uint64(33)
+    return (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def value_noise(p, seed, octaves=3, base_freq=4.0):
+    """Seeded multi-octave value noise in [-1,1]: trilinear interpolation of
+    lattice values with smoothstep weights, octave k at base_freq*2^k with
+    amplitude 2^-k, normalised by the amplitude sum."""
+    total = np.zeros(p.shape[0])
+    amp_sum = 0.0
+    with np.errstate(over="ignore"):
+        for k in range(octaves):
+            freq = base_freq * (2.0 ** k)
+            amp = 0.5 ** k
+            q = p * freq + 1000.0
+            i0 = np.floor(q).astype(np.int64)
+            fr = q - i0
+            w = fr * fr * (3.0 - 2.0 * fr)
+            acc = np.zeros(p.shape[0])
+            for dx in (0, 1):
+                for dy in (0, 1):
+                    for dz in (0, 1):
+                        val = _hash3(i0[:, 0] + dx, i0[:, 1] + dy, i0[:, 2] + dz, seed + k)
+                        wx = w[:, 0] if dx else 1 - w[:, 0]
+                        wy = w[:, 1] if dy else 1 - w[:, 1]
+                        wz = w[:, 2] if dz else 1 - w[:, 2]
+                        acc += val * wx * wy * wz
+            total += amp * (2.0 * acc - 1.0)
+            amp_sum += amp
+    return total / amp_sum
+
+
+def displaced_icosphere_scene(level=8, width=1920, height=1080, max_depth=12, seed=0, groups=64,
+                              with_light=True):
+    """1M-triangle workload: icosphere subdivided to `level` (level 8 =
+    1,310,720 triangles), radially displaced by seeded 3-octave value noise
+    of amplitude 0.05, over a ground quad.  The mesh is split into `groups`
+    face groups (8 latitude x 8 longitude bins of the face centroid), each a
+    GGX rough_metal (default Cu) with alpha ~ U[0.1,0.5] stored as
+    roughness = sqrt(alpha) (Microfacet.RoughnessToAlpha squares it).  The
+    ground is a rough_glass with etaB = 1.5 (roughness: reference default
+    0.3).  The workload names no emitter, so a 2 x 2 quad area light of
+    radiance (8,8,8) faces down from y = 4 (one light per triangle, as the
+    reference builds emissive meshes).  BVH max_prim 4."""
+    rng = np.random.default_rng(seed)
+    v, f = icosphere(level)
+    disp = value_noise(v, seed, octaves=3)
+    center = np.array([0.0, 1.1, 0.0])
+    pos = (v * (1.0 + 0.05 * disp)[:, None] + center).astype(np.float32)
+    s = SceneDesc()
+    alphas = rng.uniform(0.1, 0.5, size=groups)
+    mats = [s.add_material("rough_metal", roughness=float(np.sqrt(np.float32(a))), dist="GGX") for a in alphas]
+    cen = v[f].mean(axis=1)
+    lat = np.clip(((np.arcsin(np.clip(cen[:, 1] / np.linalg.norm(cen, axis=1), -1, 1)) / np.pi + 0.5) * 8).astype(int), 0, 7)
+    lon = np.clip(((np.arctan2(cen[:, 2], cen[:, 0]) / (2 * np.pi) + 0.5) * 8).astype(int), 0, 7)
+    gid = (lat * 8 + lon) % groups
+    order = np.argsort(gid, kind="stable")
+    fs = f[order]
+    gs = gid[order]
+    bounds = np.searchsorted(gs, np.arange(groups + 1))
+    for g in range(groups):
+        sub = fs[bounds[g]:bounds[g + 1]]
+        if sub.shape[0] == 0:
+            continue
+        used, remap = np.unique(sub.reshape(-1), return_inverse=True)
+        s.add_mesh(pos[used], remap.reshape(-1, 3).astype(np.int32), material=mats[g])
+    glass = s.add_material("rough_glass", etaB=1.5)
+    G = 10.0
+    s.add_mesh(*_quad((-G, 0, -G), (-G, 0, G), (G, 0, G), (G, 0, -G)), material=glass)
+    if with_light:
+        lm = s.add_material("diffuse", kd=(0.0, 0.0, 0.0))
+        y = 4.0
+        s.add_mesh(*_quad((-1, y, -1), (1, y, -1), (1, y, 1), (-1, y, 1)), material=lm, le=(8.0, 8.0, 8.0))
+    s.set_camera(fov=40.0, width=width, height=height, origin=(0.0, 1.7, 4.6), target=(0.0, 1.0, 0.0),
+                 up=(0.0, 1.0, 0.0))
+    s.set_integrator(type="path", max_depth=max_depth, method="Mis")
+    s.bvh_max_prim = 4
+    return s
