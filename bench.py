@@ -57,18 +57,37 @@ def cpu_baseline(desc, args, gpu_rate):
         x0, y0 = (W - cw) // 2, (H - ch) // 2
         return (x0, y0, x0 + cw, y0 + ch)
 
-    cw, ch = 64, 36
+    # calibrate on a centred crop, then time the whole frame when it fits the
+    # budget (the same workload the GPU renders), else the crop that does
+    cw, ch = 192, 108
     _, _, st, secs = o.render(args.spp, seed=1, crop=crop(cw, ch), threads=threads, with_sum2=False)
-    rate = (st["rays_closest"] + st["rays_any"]) / max(secs, 1e-9)
-    scale = min(max((args.cpu_seconds / max(secs, 1e-3)) ** 0.5, 1.0), 8.0)
-    cw, ch = min(W, int(cw * scale) // 4 * 4), min(H, int(ch * scale) // 4 * 4)
+    full = secs * (W * H) / (cw * ch)
+    log(f"[cpu] calibration {cw}x{ch}: {secs:.2f}s, full frame projected {full:.1f}s")
+    if full <= 3 * args.cpu_seconds:
+        cw, ch = W, H
+    else:
+        scale = (args.cpu_seconds / max(secs, 1e-3)) ** 0.5
+        cw, ch = min(W, max(16, int(cw * scale) // 4 * 4)), min(H, max(16, int(ch * scale) // 4 * 4))
     _, _, st, secs = o.render(args.spp, seed=1, crop=crop(cw, ch), threads=threads, with_sum2=False)
     rays = st["rays_closest"] + st["rays_any"]
     rate = rays / secs / 1e6
     log(f"[cpu] crop {cw}x{ch} spp {args.spp}: {rays} rays in {secs:.2f}s = {rate:.3f} Mrays/s")
+    what = "the whole frame" if (cw, ch) == (W, H) else f"a centred {cw}x{ch} crop of the frame"
     return {"value": round(rate, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"centred {cw}x{ch} crop of the same frame at {args.spp} spp, seed 1, "
-                      f"{rays} BVH queries in {secs:.1f} s"}
+            "sample": f"{what} at {args.spp} spp, seed 1: {rays} BVH queries in {secs:.1f} s"}
+
+
+def copy_calibration():
+    """A plain device copy of known size, profiled beside the path kernels so
+    FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
+    import torch
+    n = 256 * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 2.0, out=b)   # one elementwise kernel: reads n*4 bytes, writes n*4 bytes
+    torch.cuda.synchronize()
+    log(f"[bench] copy calibration: 3 x MulFunctor over {n * 4} bytes read + written")
 
 
 def main():
@@ -84,6 +103,8 @@ def main():
     ap.add_argument("--wave-paths", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 256 MiB device copy (scale for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: the path renders one frame per GPU and no multi-GPU "
@@ -92,6 +113,8 @@ def main():
 
     from radiantium_amd import native
 
+    if args.copy_calibration:
+        copy_calibration()
     t0 = time.perf_counter()
     desc = make_scene(args)
     ntri = desc.arrays()["indices"].shape[0]

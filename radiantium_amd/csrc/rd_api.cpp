@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -221,6 +222,11 @@ rd_ctx* rd_create(int device) {
     rd_ctx* c = new rd_ctx();
     c->device = device;
     set_device_grid(prop.multiProcessorCount);
+    if (const char* env = getenv("RD_TRACE_SCHEDULE")) {   // "ext,shadow,mis", e.g. "0,1,0"
+        int v[3] = {0, 0, 0};
+        if (sscanf(env, "%d,%d,%d", &v[0], &v[1], &v[2]) == 3)
+            for (int k = 0; k < 3; k++) set_trace_schedule(k, v[k]);
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         g_err = "hipStreamCreate failed"; delete c; return nullptr;
     }
@@ -504,7 +510,7 @@ int rd_trace_closest_device(rd_ctx* c, rd_scene* s, const rd_ray* d_rays, int32_
     Timer tm(c);
     HIP_TRY(tm.run("trace_rays_closest", [&] {
         return launch_trace_rays(s->dev, (const float4*)d_rays, n, (float4*)d_hits, nullptr,
-                                 (unsigned long long*)d_counters, false, st);
+                                 (unsigned long long*)d_counters, false, c->counts, st);
     }));
     if (!stream) {
         HIP_TRY(hipStreamSynchronize(st));
@@ -521,7 +527,7 @@ int rd_trace_any_device(rd_ctx* c, rd_scene* s, const rd_ray* d_rays, int32_t n,
     Timer tm(c);
     HIP_TRY(tm.run("trace_rays_any", [&] {
         return launch_trace_rays(s->dev, (const float4*)d_rays, n, nullptr, d_occ, (unsigned long long*)d_counters,
-                                 true, st);
+                                 true, c->counts, st);
     }));
     if (!stream) {
         HIP_TRY(hipStreamSynchronize(st));

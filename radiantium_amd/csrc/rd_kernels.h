@@ -54,7 +54,7 @@ struct Timing;  // per-stage HIP event timing (rd_api.cpp)
 
 // Traversal over caller-provided ray arrays (rd_ray / rd_hit layouts).
 hipError_t launch_trace_rays(const DevScene& sc, const float4* rays, int n, float4* hits, uint8_t* occ,
-                             unsigned long long* counters, bool any, hipStream_t s);
+                             unsigned long long* counters, bool any, int* counts, hipStream_t s);
 
 // Wavefront path tracing stages.
 hipError_t launch_raygen(const DevScene& sc, const WaveParams& wp, int npaths, RayQueue q, PathState ps,
@@ -74,6 +74,12 @@ hipError_t launch_finalize(const WaveParams& wp, int height, int spp, const floa
                            float* out_rgb, float* out_sum2, hipStream_t s);
 hipError_t launch_zero(float4* buf, int n, hipStream_t s);
 hipError_t launch_zero_counts(int* counts, int n, hipStream_t s);
+
+// Traversal schedule per query class (0 extension, 1 shadow, 2 MIS):
+// 0 = straight (grid-stride, one ray per lane to completion),
+// 1 = refill (idle lanes refilled from an atomic cursor).
+void set_trace_schedule(int cls, int schedule);
+int get_trace_schedule(int cls);
 
 // Occupancy-sized persistent grid for the device.
 void set_device_grid(int num_cus);

@@ -87,118 +87,292 @@ struct Hit {
     int prim;   // original primitive index, -1 = miss
 };
 
-// Bvh.Intersect (Accelerators/Bvh.cs:126-232): depth-first traversal, near
-// child chosen by the sign of InvD along the node's split axis, first-found
-// hit kept on ties (strict '<'), ray.MaxT shrunk to the closest hit.  The
-// stack lives in LDS, one column per lane (stride kBlock) so that lanes of a
-// wavefront hit distinct banks.
+// ---------------------------------------------------------------- traversal kernels
+// One kernel template per (query class, LDS stack depth, schedule).  Every
+// ray walks the reference's depth-first order (Bvh.cs:126-232) one node per
+// step, which keeps hits and visit counts identical to the oracle; the
+// schedules differ only in how rays are assigned to lanes.
+enum : int { kModeRaysClosest = 0, kModeRaysAny = 1, kModeExt = 2, kModeShadow = 3, kModeMis = 4 };
+constexpr int kRefill = 16;
+
+struct TraceArgs {
+    const float4* rays;       // ray-array modes: 2 float4 per ray (rd_ray)
+    RayQueue q;               // queue modes
+    float4* hits;             // closest modes: {t, prim bits, u, v} per ray / slot
+    uint8_t* occ;             // ray-array any-hit output
+    PathState ps;             // shadow / MIS outputs
+    int* counts;
+    unsigned long long* counters;   // ray-array modes: {nodes, prims} (may be null)
+    int n_fixed;              // ray-array modes: ray count
+    int cnt_slot;             // queue modes: count slot
+    int cursor_slot;          // work cursor, zero on entry
+    int zero[3];              // count slots block 0 clears for the next stage (-1: none)
+};
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    int lane = __lane_id();
+    return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+}
+
+// Per-lane traversal state of one ray.
+struct Trav {
+    f3 o, d, inv;
+    float tmin, tmax;
+    bool nx, ny, nz;
+    int sp, cur;
+    Hit best;
+};
+
+__device__ __forceinline__ void trav_begin(Trav& t, float4 ra, float4 rb) {
+    t.o = mk(ra.x, ra.y, ra.z);
+    t.d = mk(rb.x, rb.y, rb.z);
+    t.tmin = ra.w;
+    t.tmax = rb.w;
+    t.inv = mk(1.0f / t.d.x, 1.0f / t.d.y, 1.0f / t.d.z);
+    t.nx = t.inv.x < 0; t.ny = t.inv.y < 0; t.nz = t.inv.z < 0;
+    t.sp = 0; t.cur = 0;
+    t.best.prim = -1; t.best.t = t.best.u = t.best.v = 0.f;
+}
+
+// One iteration of Bvh.Intersect's loop (Bvh.cs:134-170 / 187-229): test the
+// current node; a leaf tests its primitives (any-hit stops at the first, closest
+// hit keeps the first strictly closer one and shrinks MaxT); an interior node
+// descends to the child on the near side of its split axis and pushes the
+// other.  Returns true when the ray is finished.
 template <bool kAny, int kBlock>
-__device__ __forceinline__ bool traverse(const DevScene& sc, f3 o, f3 d, float tmin, float tmax, int* stk,
-                                         Hit* out, uint32_t& nv, uint32_t& nt) {
-    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
-    int sp = 0, cur = 0;
-    bool found = false;
-    float best = kFloatMax;
-    out->prim = -1;
+__device__ __forceinline__ bool trav_step(const DevScene& sc, Trav& t, int* stk, uint32_t& nv, uint32_t& nt) {
     const float4* __restrict__ nodes = sc.nodes;
     const float4* __restrict__ prims = sc.prims;
-    while (true) {
-        float4 lo = nodes[2 * cur], hi = nodes[2 * cur + 1];
-        nv++;
-        if (box_hit(lo, hi, o, inv, nx, ny, nz, tmax)) {
-            int off = __float_as_int(hi.z);
-            int ca = __float_as_int(hi.w);
-            int cnt = ca & 0xffff;
-            if (cnt > 0) {
-                for (int i = 0; i < cnt; i++) {
-                    const float4* p = prims + 3 * (off + i);
-                    float4 p0 = p[0], p1 = p[1];
-                    nt++;
-                    float tu = 0, tv = 0, tt;
-                    bool h;
-                    if (__float_as_int(p1.w) == kTri) {
-                        float4 p2 = p[2];
-                        h = tri_test(mk(p0.x, p0.y, p0.z), mk(p1.x, p1.y, p1.z), mk(p2.x, p2.y, p2.z), o, d, tmin, tmax,
-                                     &tu, &tv, &tt);
-                    } else {
-                        h = sphere_test(make_float4(p0.x, p0.y, p0.z, p1.x), o, d, tmin, tmax, &tt);
-                    }
-                    if (h) {
-                        if (kAny) { out->prim = __float_as_int(p0.w); return true; }
-                        found = true;
-                        if (tt < best) {
-                            best = tt;
-                            out->t = tt; out->u = tu; out->v = tv; out->prim = __float_as_int(p0.w);
-                            tmax = tt;
-                        }
+    float4 lo = nodes[2 * t.cur], hi = nodes[2 * t.cur + 1];
+    nv++;
+    if (box_hit(lo, hi, t.o, t.inv, t.nx, t.ny, t.nz, t.tmax)) {
+        int off = __float_as_int(hi.z);
+        int ca = __float_as_int(hi.w);
+        int cnt = ca & 0xffff;
+        if (cnt > 0) {
+            for (int i = 0; i < cnt; i++) {
+                const float4* p = prims + 3 * (off + i);
+                float4 p0 = p[0], p1 = p[1];
+                nt++;
+                float tu = 0.f, tv = 0.f, tt = 0.f;
+                bool h;
+                if (__float_as_int(p1.w) == kTri) {
+                    float4 p2 = p[2];
+                    h = tri_test(mk(p0.x, p0.y, p0.z), mk(p1.x, p1.y, p1.z), mk(p2.x, p2.y, p2.z), t.o, t.d, t.tmin,
+                                 t.tmax, &tu, &tv, &tt);
+                } else {
+                    h = sphere_test(make_float4(p0.x, p0.y, p0.z, p1.x), t.o, t.d, t.tmin, t.tmax, &tt);
+                }
+                if (h) {
+                    if (kAny) { t.best.prim = __float_as_int(p0.w); return true; }
+                    if (tt < (t.best.prim >= 0 ? t.best.t : kFloatMax)) {
+                        t.best.t = tt; t.best.u = tu; t.best.v = tv; t.best.prim = __float_as_int(p0.w);
+                        t.tmax = tt;
                     }
                 }
-                if (sp == 0) break;
-                cur = stk[(--sp) * kBlock];
-            } else {
-                int axis = ca >> 16;
-                bool neg = axis == 0 ? nx : (axis == 1 ? ny : nz);
-                if (neg) { stk[(sp++) * kBlock] = cur + 1; cur = off; }
-                else { stk[(sp++) * kBlock] = off; cur = cur + 1; }
             }
+            if (t.sp == 0) return true;
+            t.cur = stk[(--t.sp) * kBlock];
         } else {
-            if (sp == 0) break;
-            cur = stk[(--sp) * kBlock];
+            int axis = ca >> 16;
+            bool neg = axis == 0 ? t.nx : (axis == 1 ? t.ny : t.nz);
+            if (neg) { stk[(t.sp++) * kBlock] = t.cur + 1; t.cur = off; }
+            else { stk[(t.sp++) * kBlock] = off; t.cur = t.cur + 1; }
         }
+    } else {
+        if (t.sp == 0) return true;
+        t.cur = stk[(--t.sp) * kBlock];
     }
-    return found;
+    return false;
 }
 
-// Stack depth in LDS per lane.  The DFS stack never holds more entries than
-// the deepest leaf's depth; scenes deeper than this use the 64-entry kernel
-// (the reference's Bvh.StackSize).
-constexpr int kStackShort = 40;
-constexpr int kStackLong = 64;
+template <int kMode>
+__device__ __forceinline__ void trav_load(const TraceArgs& a, int r, float4* ra, float4* rb) {
+    if (kMode <= kModeRaysAny) { *ra = a.rays[2 * r]; *rb = a.rays[2 * r + 1]; }
+    else { *ra = a.q.org[r]; *rb = a.q.dir[r]; }
+}
 
-// ---------------------------------------------------------------- ray-array traversal
-template <bool kAny, int kStack, int kBlock>
-__global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene sc, const float4* __restrict__ rays, int n,
-                                                       float4* __restrict__ hits, uint8_t* __restrict__ occ,
-                                                       unsigned long long* counters) {
+// Writes the query result of ray / slot r.
+template <int kMode>
+__device__ __forceinline__ void trav_finish(const DevScene& sc, const TraceArgs& a, int r, const Trav& t) {
+    const bool found = t.best.prim >= 0;
+    if (kMode == kModeRaysClosest || kMode == kModeExt) {
+        a.hits[r] = found ? make_float4(t.best.t, __int_as_float(t.best.prim), t.best.u, t.best.v)
+                          : make_float4(0.f, __int_as_float(-1), 0.f, 0.f);
+    } else if (kMode == kModeRaysAny) {
+        a.occ[r] = found ? 1 : 0;
+    } else if (kMode == kModeShadow) {
+        a.ps.vis[a.q.pid[r]] = found ? 0 : 1;
+    } else {
+        // BSDF-sampled MIS ray: Le of the hit when it is the selected light
+        // (PathTracer.cs:379-387); infinite lights on a miss (none modelled: 0)
+        int pid = a.q.pid[r];
+        f3 li = mk(0.0f);
+        if (found) {
+            const DevPrim pr = sc.prim_info[t.best.prim];
+            int sel = a.ps.sel_light[pid];
+            if (pr.light >= 0 && pr.light == sel) {
+                f3 nrm = hit_normal(sc, pr, t.o, t.d, t.best.u, t.best.v, t.best.t);
+                const DevLight& L = sc.lights[sel];
+                if (dot(nrm, -t.d) > 0) li = mk(L.le[0], L.le[1], L.le[2]);
+            }
+        }
+        float4 prev = a.ps.mis_li[pid];
+        a.ps.mis_li[pid] = make_float4(li.x, li.y, li.z, prev.w);
+    }
+}
+
+template <int kMode>
+__device__ __forceinline__ void trav_counters(const TraceArgs& a, uint32_t nrays, uint32_t nv, uint32_t nt) {
+    constexpr bool kAny = (kMode == kModeRaysAny || kMode == kModeShadow);
+    if (kMode <= kModeRaysAny) {
+        if (a.counters) {
+            add_counter(a.counters, nv);
+            add_counter(a.counters + 1, nt);
+        }
+    } else {
+        constexpr int cls = kMode == kModeExt ? 0 : (kMode == kModeMis ? 1 : 2);
+        add_class_counters(a.counts, cls, nrays, nv, nt);
+        add_counter((unsigned long long*)(a.counts + (kAny ? kCntRaysAny : kCntRaysClosest)), nrays);
+    }
+}
+
+__device__ __forceinline__ int trav_count(const TraceArgs& a, bool fixed) {
+    return fixed ? a.n_fixed : a.counts[a.cnt_slot];
+}
+
+// Schedule 0 ("straight"): each lane runs its ray to completion, rays are
+// taken grid-stride.
+template <int kMode, int kStack, int kBlock>
+__global__ __launch_bounds__(kBlock) void k_trace_straight(DevScene sc, TraceArgs a) {
+    constexpr bool kAny = (kMode == kModeRaysAny || kMode == kModeShadow);
     __shared__ int lds[kStack * kBlock];
     int* stk = lds + threadIdx.x;
-    uint32_t nv = 0, nt = 0;
-    for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
-        int i = base + threadIdx.x;
-        if (i < n) {
-            float4 a = rays[2 * i], b = rays[2 * i + 1];
-            Hit h;
-            bool r = traverse<kAny, kBlock>(sc, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, stk, &h, nv, nt);
-            if (kAny) occ[i] = r ? 1 : 0;
-            else hits[i] = r ? make_float4(h.t, __int_as_float(h.prim), h.u, h.v) : make_float4(0.f, __int_as_float(-1), 0.f, 0.f);
-        }
+    const int n = trav_count(a, kMode <= kModeRaysAny);
+    if (blockIdx.x == 0 && threadIdx.x < 3 && a.zero[threadIdx.x] >= 0) a.counts[a.zero[threadIdx.x]] = 0;
+    uint32_t nv = 0, nt = 0, nrays = 0;
+    for (int r = blockIdx.x * kBlock + threadIdx.x; r < n; r += gridDim.x * kBlock) {
+        float4 ra, rb;
+        trav_load<kMode>(a, r, &ra, &rb);
+        Trav t;
+        trav_begin(t, ra, rb);
+        while (!trav_step<kAny, kBlock>(sc, t, stk, nv, nt)) {}
+        trav_finish<kMode>(sc, a, r, t);
+        nrays++;
     }
-    if (counters) {
-        add_counter(counters, nv);
-        add_counter(counters + 1, nt);
-    }
+    trav_counters<kMode>(a, nrays, nv, nt);
 }
 
-hipError_t launch_trace_rays(const DevScene& sc, const float4* rays, int n, float4* hits, uint8_t* occ,
-                             unsigned long long* counters, bool any, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    if (sc.stack_needed <= kStackShort) {
-        constexpr int B = 256;
-        int blocks = (n + B - 1) / B;
-        if (blocks > persistent_blocks()) blocks = persistent_blocks();
-        if (any) hipLaunchKernelGGL((k_trace_rays<true, kStackShort, B>), dim3(blocks), dim3(B), 0, s, sc, rays, n, hits, occ, counters);
-        else hipLaunchKernelGGL((k_trace_rays<false, kStackShort, B>), dim3(blocks), dim3(B), 0, s, sc, rays, n, hits, occ, counters);
+// Schedule 1 ("refill"): lanes that finish are refilled from an atomic ray
+// cursor once at least kRefill lanes of the wavefront are idle.
+template <int kMode, int kStack, int kBlock>
+__global__ __launch_bounds__(kBlock) void k_trace_refill(DevScene sc, TraceArgs a) {
+    constexpr bool kAny = (kMode == kModeRaysAny || kMode == kModeShadow);
+    __shared__ int lds[kStack * kBlock];
+    int* stk = lds + threadIdx.x;
+    const int n = trav_count(a, kMode <= kModeRaysAny);
+    if (blockIdx.x == 0 && threadIdx.x < 3 && a.zero[threadIdx.x] >= 0) a.counts[a.zero[threadIdx.x]] = 0;
+    int* cursor = a.counts + a.cursor_slot;
+    uint32_t nv = 0, nt = 0, nrays = 0;
+    bool active = false, drained = false;
+    int r = 0;
+    Trav t;
+    trav_begin(t, make_float4(0.f, 0.f, 0.f, 0.f), make_float4(1.f, 1.f, 1.f, 0.f));
+    while (true) {
+        if (!drained) {
+            uint64_t idle = __ballot(!active);
+            int nidle = __popcll(idle);
+            if (nidle >= kRefill) {
+                int leader = __ffsll((unsigned long long)idle) - 1;
+                int base = 0;
+                if (__lane_id() == leader) base = atomicAdd(cursor, nidle);
+                base = __shfl(base, leader);
+                if (base + nidle >= n) drained = true;
+                if (!active) {
+                    int rr = base + __popcll(idle & lanemask_lt());
+                    if (rr < n) {
+                        r = rr;
+                        float4 ra, rb;
+                        trav_load<kMode>(a, r, &ra, &rb);
+                        trav_begin(t, ra, rb);
+                        active = true;
+                        nrays++;
+                    }
+                }
+            }
+        }
+        if (__ballot(active) == 0) break;
+        if (!active) continue;
+        // a few steps between refill checks
+        bool done = false;
+        for (int k = 0; k < 4 && !done; k++) done = trav_step<kAny, kBlock>(sc, t, stk, nv, nt);
+        if (!done) continue;
+        active = false;
+        trav_finish<kMode>(sc, a, r, t);
+    }
+    trav_counters<kMode>(a, nrays, nv, nt);
+}
+
+// Traversal schedule per query class (0 straight, 1 refill): extension,
+// shadow and MIS, chosen by the autotuner.
+static int g_schedule[3] = {0, 0, 0};
+void set_trace_schedule(int cls, int schedule) {
+    if (cls >= 0 && cls < 3) g_schedule[cls] = schedule ? 1 : 0;
+}
+int get_trace_schedule(int cls) { return (cls >= 0 && cls < 3) ? g_schedule[cls] : 0; }
+
+// LDS stack variants: the DFS stack never holds more entries than the
+// deepest leaf's depth, so the smallest variant that covers the tree is
+// launched (64 = the reference's Bvh.StackSize).
+template <int kMode, int kStack, int kBlock>
+static void launch_trace_variant(const DevScene& sc, const TraceArgs& a, int blocks, int schedule, hipStream_t s) {
+    if (schedule) hipLaunchKernelGGL((k_trace_refill<kMode, kStack, kBlock>), dim3(blocks), dim3(kBlock), 0, s, sc, a);
+    else hipLaunchKernelGGL((k_trace_straight<kMode, kStack, kBlock>), dim3(blocks), dim3(kBlock), 0, s, sc, a);
+}
+
+template <int kMode>
+static hipError_t launch_trace_mode(const DevScene& sc, const TraceArgs& a, int blocks_hint, hipStream_t s) {
+    int need = sc.stack_needed;
+    int pb = persistent_blocks();
+    int cls = kMode == kModeExt ? 0 : (kMode == kModeShadow ? 1 : (kMode == kModeMis ? 2 : 0));
+    int sched = kMode <= kModeRaysAny ? 0 : g_schedule[cls];
+    if (need <= 24) {
+        int b = blocks_hint > 0 ? std::min(blocks_hint, pb) : pb;
+        launch_trace_variant<kMode, 24, 256>(sc, a, b, sched, s);
+    } else if (need <= 32) {
+        int b = blocks_hint > 0 ? std::min(blocks_hint, pb) : pb;
+        launch_trace_variant<kMode, 32, 256>(sc, a, b, sched, s);
+    } else if (need <= 48) {
+        int b = blocks_hint > 0 ? std::min(2 * blocks_hint, 2 * pb) : 2 * pb;
+        launch_trace_variant<kMode, 48, 128>(sc, a, b, sched, s);
     } else {
-        constexpr int B = 128;
-        int blocks = (n + B - 1) / B;
-        if (blocks > 2 * persistent_blocks()) blocks = 2 * persistent_blocks();
-        if (any) hipLaunchKernelGGL((k_trace_rays<true, kStackLong, B>), dim3(blocks), dim3(B), 0, s, sc, rays, n, hits, occ, counters);
-        else hipLaunchKernelGGL((k_trace_rays<false, kStackLong, B>), dim3(blocks), dim3(B), 0, s, sc, rays, n, hits, occ, counters);
+        int b = blocks_hint > 0 ? std::min(2 * blocks_hint, 2 * pb) : 2 * pb;
+        launch_trace_variant<kMode, 64, 128>(sc, a, b, sched, s);
     }
     return hipGetLastError();
 }
 
+static TraceArgs trace_args(int* counts) {
+    TraceArgs a{};
+    a.counts = counts;
+    a.zero[0] = a.zero[1] = a.zero[2] = -1;
+    return a;
+}
+
+__global__ void k_zero_slot(int* c, int slot) {
+    if (threadIdx.x == 0) c[slot] = 0;
+}
+
+hipError_t launch_trace_rays(const DevScene& sc, const float4* rays, int n, float4* hits, uint8_t* occ,
+                             unsigned long long* counters, bool any, int* counts, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_slot, dim3(1), dim3(64), 0, s, counts, kCntWork + 3);
+    TraceArgs a = trace_args(counts);
+    a.rays = rays; a.hits = hits; a.occ = occ; a.counters = counters; a.n_fixed = n;
+    a.cursor_slot = kCntWork + 3;
+    int blocks = (n + 255) / 256;
+    return any ? launch_trace_mode<kModeRaysAny>(sc, a, blocks, s) : launch_trace_mode<kModeRaysClosest>(sc, a, blocks, s);
+}
 // ---------------------------------------------------------------- wavefront
 __device__ __forceinline__ void wave_pixel(const WaveParams& wp, int p, int* x, int* y, int* sample) {
     int s = p / wp.npix;
@@ -263,123 +437,37 @@ hipError_t launch_raygen(const DevScene& sc, const WaveParams& wp, int npaths, R
     return hipGetLastError();
 }
 
-// Closest-hit over the extension queue.  Block 0 also clears the counters of
-// the queues the following shade stage appends to.
-template <int kStack, int kBlock>
-__global__ __launch_bounds__(kBlock) void k_trace_ext(DevScene sc, RayQueue q, float4* __restrict__ hits, int* counts,
-                                                      int cnt_slot, int za, int zb, int zc) {
-    __shared__ int lds[kStack * kBlock];
-    int* stk = lds + threadIdx.x;
-    int n = counts[cnt_slot];
-    if (blockIdx.x == 0 && threadIdx.x == 0) { counts[za] = 0; counts[zb] = 0; counts[zc] = 0; }
-    uint32_t nv = 0, nt = 0;
-    uint64_t nrays = 0;
-    for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
-        int i = base + threadIdx.x;
-        if (i < n) {
-            float4 a = q.org[i], b = q.dir[i];
-            Hit h;
-            bool r = traverse<false, kBlock>(sc, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, stk, &h, nv, nt);
-            hits[i] = r ? make_float4(h.t, __int_as_float(h.prim), h.u, h.v) : make_float4(0.f, __int_as_float(-1), 0.f, 0.f);
-            nrays++;
-        }
-    }
-    add_class_counters(counts, 0, nrays, nv, nt);
-    add_counter((unsigned long long*)(counts + kCntRaysClosest), nrays);
-}
-
+// Closest hit of the extension queue.  It also clears the queue counters and
+// work cursors the following shade / shadow / MIS stages start from.
 hipError_t launch_trace_ext(const DevScene& sc, RayQueue q, float4* hits, int* counts, int cnt_slot, int za,
                             int zb, int zc, hipStream_t s) {
-    if (sc.stack_needed <= kStackShort)
-        hipLaunchKernelGGL((k_trace_ext<kStackShort, 256>), dim3(persistent_blocks()), dim3(256), 0, s, sc, q, hits,
-                           counts, cnt_slot, za, zb, zc);
-    else
-        hipLaunchKernelGGL((k_trace_ext<kStackLong, 128>), dim3(2 * persistent_blocks()), dim3(128), 0, s, sc, q,
-                           hits, counts, cnt_slot, za, zb, zc);
-    return hipGetLastError();
+    TraceArgs a = trace_args(counts);
+    a.q = q; a.hits = hits; a.cnt_slot = cnt_slot; a.cursor_slot = kCntWork + 0;
+    a.zero[0] = za; a.zero[1] = zb; a.zero[2] = zc;
+    return launch_trace_mode<kModeExt>(sc, a, 0, s);
 }
 
-// Any-hit over the shadow queue: Scene.IsOccluded (Scene.cs:58-70).
-template <int kStack, int kBlock>
-__global__ __launch_bounds__(kBlock) void k_trace_shadow(DevScene sc, RayQueue q, PathState ps, int* counts) {
-    __shared__ int lds[kStack * kBlock];
-    int* stk = lds + threadIdx.x;
-    int n = counts[kCntShadow];
-    uint32_t nv = 0, nt = 0;
-    uint64_t nrays = 0;
-    for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
-        int i = base + threadIdx.x;
-        if (i < n) {
-            float4 a = q.org[i], b = q.dir[i];
-            Hit h;
-            bool occluded = traverse<true, kBlock>(sc, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, stk, &h, nv, nt);
-            ps.vis[q.pid[i]] = occluded ? 0 : 1;
-            nrays++;
-        }
-    }
-    add_class_counters(counts, 2, nrays, nv, nt);
-    add_counter((unsigned long long*)(counts + kCntRaysAny), nrays);
-}
-
+// Any hit of the shadow queue: Scene.IsOccluded (Scene.cs:58-70).
 hipError_t launch_trace_shadow(const DevScene& sc, RayQueue q, PathState ps, int* counts, hipStream_t s) {
-    if (sc.stack_needed <= kStackShort)
-        hipLaunchKernelGGL((k_trace_shadow<kStackShort, 256>), dim3(persistent_blocks()), dim3(256), 0, s, sc, q, ps, counts);
-    else
-        hipLaunchKernelGGL((k_trace_shadow<kStackLong, 128>), dim3(2 * persistent_blocks()), dim3(128), 0, s, sc, q, ps, counts);
-    return hipGetLastError();
+    TraceArgs a = trace_args(counts);
+    a.q = q; a.ps = ps; a.cnt_slot = kCntShadow; a.cursor_slot = kCntWork + 1;
+    return launch_trace_mode<kModeShadow>(sc, a, 0, s);
 }
 
-// Closest-hit of the BSDF-sampled MIS rays, then the radiance they carry:
-// Le of the hit when it is the selected light (PathTracer.cs:379-387), the
-// infinite lights on a miss (:388-391; none are modelled, so 0).
-template <int kStack, int kBlock>
-__global__ __launch_bounds__(kBlock) void k_trace_mis(DevScene sc, RayQueue q, PathState ps, int* counts) {
-    __shared__ int lds[kStack * kBlock];
-    int* stk = lds + threadIdx.x;
-    int n = counts[kCntMis];
-    uint32_t nv = 0, nt = 0;
-    uint64_t nrays = 0;
-    for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
-        int i = base + threadIdx.x;
-        if (i < n) {
-            float4 a = q.org[i], b = q.dir[i];
-            f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
-            Hit h;
-            bool hit = traverse<false, kBlock>(sc, o, d, a.w, b.w, stk, &h, nv, nt);
-            int pid = q.pid[i];
-            f3 li = mk(0.0f);
-            if (hit) {
-                const DevPrim pr = sc.prim_info[h.prim];
-                int sel = ps.sel_light[pid];
-                if (pr.light >= 0 && pr.light == sel) {
-                    f3 n = hit_normal(sc, pr, o, d, h.u, h.v, h.t);
-                    const DevLight& L = sc.lights[sel];
-                    if (dot(n, -d) > 0) li = mk(L.le[0], L.le[1], L.le[2]);
-                }
-            }
-            float4 prev = ps.mis_li[pid];
-            ps.mis_li[pid] = make_float4(li.x, li.y, li.z, prev.w);
-            nrays++;
-        }
-    }
-    add_class_counters(counts, 1, nrays, nv, nt);
-    add_counter((unsigned long long*)(counts + kCntRaysClosest), nrays);
-}
-
+// Closest hit of the BSDF-sampled MIS rays and the radiance they carry.
 hipError_t launch_trace_mis(const DevScene& sc, RayQueue q, PathState ps, int* counts, hipStream_t s) {
-    if (sc.stack_needed <= kStackShort)
-        hipLaunchKernelGGL((k_trace_mis<kStackShort, 256>), dim3(persistent_blocks()), dim3(256), 0, s, sc, q, ps, counts);
-    else
-        hipLaunchKernelGGL((k_trace_mis<kStackLong, 128>), dim3(2 * persistent_blocks()), dim3(128), 0, s, sc, q, ps, counts);
-    return hipGetLastError();
+    TraceArgs a = trace_args(counts);
+    a.q = q; a.ps = ps; a.cnt_slot = kCntMis; a.cursor_slot = kCntWork + 2;
+    return launch_trace_mode<kModeMis>(sc, a, 0, s);
 }
-
 // One bounce of PathTracer.Mis / NextEventEstimation / OnlySampleBxdf for
 // every path in the extension queue (PathTracer.cs:43-270).
 __global__ __launch_bounds__(256) void k_shade(DevScene sc, DevIntegrator in, WaveParams wp, RayQueue qin,
                                                const float4* __restrict__ hits, RayQueue qout, RayQueue qshadow,
                                                RayQueue qmis, PathState ps, int* counts, int cnt_in, int cnt_out) {
     int n = counts[cnt_in];
+    // the traversal work cursors restart at zero for the next trace stages
+    if (blockIdx.x == 0 && threadIdx.x < 3) counts[kCntWork + threadIdx.x] = 0;
     uint64_t nvert = 0;
     for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
         int k = base + threadIdx.x;

@@ -171,7 +171,7 @@ class SceneDesc:
         self._indices.append(idx + self._nverts)
         self._tflags.append(np.full(idx.shape[0], flags, dtype=np.uint8))
         self._nverts += nv
-        first_prim = len(self.prims)
+        first_prim = self.num_prims()
         t0 = self._ntri
         nt = idx.shape[0]
         self._ntri += nt

@@ -205,7 +205,7 @@ def displaced_icosphere_scene(level=8, width=1920, height=1080, max_depth=12, se
         lm = s.add_material("diffuse", kd=(0.0, 0.0, 0.0))
         y = 4.0
         s.add_mesh(*_quad((-1, y, -1), (1, y, -1), (1, y, 1), (-1, y, 1)), material=lm, le=(8.0, 8.0, 8.0))
-    s.set_camera(fov=40.0, width=width, height=height, origin=(0.0, 1.7, 4.6), target=(0.0, 1.0, 0.0),
+    s.set_camera(fov=40.0, width=width, height=height, origin=(0.0, 2.2, 7.0), target=(0.0, 0.9, 0.0),
                  up=(0.0, 1.0, 0.0))
     s.set_integrator(type="path", max_depth=max_depth, method="Mis")
     s.bvh_max_prim = 4
