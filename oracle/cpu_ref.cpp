@@ -789,8 +789,268 @@ static float mf_reflect_pdf(const MfDist& d, V3 wo, V3 wi) {        // :43-48
     return d.pdf(wo, wh) / (4 * dot(wo, wh));
 }
 
+// ---------------------------------------------------------------- plastic
+// RoughPlastic (Materials/RoughPlastic.cs:37-166): kd*Lambert + ks*microfacet
+// reflection with dielectric Fresnel and tint R; lobe probability ks/(kd+ks).
+static C3 plastic_fr(const MfDist& d, C3 r, float kd, float ks, float ei, float et, bool two, V3 wo, V3 wi) {
+    if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!two) return C3(0.0f); }
+    C3 diffuse = same_hemisphere(wo, wi) ? r / PI_F : C3(0.0f);
+    C3 glossy(0.0f);
+    // MicrofacetReflectionBrdf<Fresnel.Dielectric, D>.Fr with R = r
+    if (same_hemisphere(wo, wi)) {
+        float cto = abs_cos_theta(wo), cti = abs_cos_theta(wi);
+        V3 wh = wi + wo;
+        if (!(cti == 0 || cto == 0) && !(wh.x == 0 && wh.y == 0 && wh.z == 0)) {
+            wh = normalize(wh);
+            C3 f(fresnel_dielectric(dot(wi, faceforward(wh, V3(0, 0, 1))), ei, et));
+            float D = d.D(wh), G = d.G(wo, wi);
+            glossy = r * cabs((D * G * f) / (4 * cti * cto));
+        }
+    }
+    return kd * diffuse + ks * glossy;
+}
+static float plastic_pdf(const MfDist& d, float kd, float ks, bool two, V3 wo, V3 wi) {
+    if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!two) return 0.0f; }
+    float p = ks / (kd + ks);
+    float diffuse = same_hemisphere(wo, wi) ? abs_cos_theta(wi) / PI_F : 0.0f;
+    float glossy = mf_reflect_pdf(d, wo, wi);
+    return (1 - p) * diffuse + p * glossy;
+}
+
+// ---------------------------------------------------------------- Disney
+// DisneyBsdf and its lobes (Bxdf/DisneyBsdf.cs:13-512).  The reference
+// author marks it abandoned (:11); this restates that code as written.
+struct Disney {
+    C3 color, sdist;
+    float metallic, eta, roughness, spec_tint, aniso, sheen, sheen_tint, clearcoat, cc_gloss, spec_scale;
+    float transmission, trans_rough, flatness;
+    bool thin;
+};
+static inline float lerpf(float t, float a, float b) { return (1 - t) * a + t * b; }        // MathExt.Lerp
+static inline C3 lerpc(float w, C3 a, C3 b) { return (1 - w) * a + w * b; }                // Color3F.Lerp
+static inline float pow5(float a) { return (a * a) * (a * a) * a; }
+static inline float schlick_weight(float c) { return pow5(nclamp(1 - c, 0, 1)); }
+static inline C3 schlick_c(C3 r0, float c) { return lerpc(schlick_weight(c), r0, C3(1.0f)); }
+static inline float schlick_f(float r0, float c) { return r0 + (1.0f - r0) * schlick_weight(c); }
+static inline float gtr1(float c, float a) {
+    float a2 = a * a;
+    return (a2 - 1) / (PI_F * std::log(a2) * (1 + (a2 - 1) * c * c));
+}
+static inline float smith_g1_ggx(float c, float a) {
+    float a2 = a * a, c2 = c * c;
+    return 2 / (1 + std::sqrt(1 + a2 * (1 - c2) / c2));
+}
+static inline C3 color_tint(const Disney& b) {
+    float l = b.color.lum();
+    return l > 0 ? b.color / l : C3(1.0f);
+}
+static inline void aniso_alpha(const Disney& b, float rough, float* ax, float* ay) {
+    float aspect = std::sqrt(1 - b.aniso * 0.9f);
+    *ax = nmax(0.001f, sqr(rough) / aspect);
+    *ay = nmax(0.001f, sqr(rough) * aspect);
+}
+// DisneyDistributionGtr2: GGX D, separable G = G1(wo) G1(wi)
+struct Gtr2 {
+    MfDist m;
+    explicit Gtr2(float ax, float ay) : m(1, 0.0f, 0.0f) { m.ax = ax; m.ay = ay; }
+    float D(V3 wh) const { return m.D(wh); }
+    float G(V3 wo, V3 wi) const { return (1 / (1 + m.lambda(wo))) * (1 / (1 + m.lambda(wi))); }
+    float pdf(V3 wo, V3 wh) const { return D(wh) * abs_cos_theta(wh); }
+};
+static C3 disney_fresnel(const Disney& b, float ci) {                                     // DisneyFresnel.Eval
+    C3 r0 = color_tint(b) * (sqr(b.eta - 1) / sqr(b.eta + 1));
+    C3 fd = lerpc(b.spec_tint, C3(fresnel_dielectric(ci, 1, b.eta)), schlick_c(r0, ci));
+    return lerpc(b.metallic, fd * b.spec_scale, schlick_c(b.color, ci));
+}
+// MicrofacetReflectionBrdf with R, a Fresnel functor and Gtr2
+template <class F>
+static C3 mf_reflect_generic(const Gtr2& d, C3 R, F fres, V3 wo, V3 wi) {
+    if (!same_hemisphere(wo, wi)) return C3(0.0f);
+    float cto = abs_cos_theta(wo), cti = abs_cos_theta(wi);
+    V3 wh = wi + wo;
+    if (cti == 0 || cto == 0) return C3(0);
+    if (wh.x == 0 && wh.y == 0 && wh.z == 0) return C3(0);
+    wh = normalize(wh);
+    C3 f = fres(dot(wi, faceforward(wh, V3(0, 0, 1))));
+    float D = d.D(wh), G = d.G(wo, wi);
+    return R * cabs((D * G * f) / (4 * cti * cto));
+}
+static float mf_reflect_pdf_g(const Gtr2& d, V3 wo, V3 wi) {
+    if (!same_hemisphere(wo, wi)) return 0;
+    V3 wh = normalize(wo + wi);
+    return d.pdf(wo, wh) / (4 * dot(wo, wh));
+}
+static Gtr2 spec_dist(const Disney& b) { float ax, ay; aniso_alpha(b, b.roughness, &ax, &ay); return Gtr2(ax, ay); }
+static Gtr2 trans_dist(const Disney& b) { float ax, ay; aniso_alpha(b, b.trans_rough, &ax, &ay); return Gtr2(ax, ay); }
+static C3 disney_spec_brdf(const Disney& b, V3 wo, V3 wi) {
+    return mf_reflect_generic(spec_dist(b), C3(1.0f), [&](float c) { return disney_fresnel(b, c); }, wo, wi);
+}
+static C3 disney_inner_brdf(const Disney& b, V3 wo, V3 wi) {
+    return mf_reflect_generic(trans_dist(b), b.transmission * b.color,
+                              [&](float c) { return C3(fresnel_dielectric(c, 1, b.eta)); }, wo, wi);
+}
+// MicrofacetTransmissionBtdf<Gtr2> (Bxdf/MicrofacetTransmissionBtdf.cs:25-77), etaA = 1, etaB = Eta
+static C3 disney_btdf_fr(const Disney& b, V3 wo, V3 wi) {
+    if (same_hemisphere(wo, wi)) return C3();
+    float cto = cos_theta(wo), cti = cos_theta(wi);
+    if (cti == 0 || cto == 0) return C3();
+    Gtr2 d = trans_dist(b);
+    bool entering = cos_theta(wo) > 0;
+    float ei = entering ? 1.0f : b.eta, et = entering ? b.eta : 1.0f;
+    float eta = et / ei;
+    V3 wh = normalize(wo + wi * eta);
+    if (cos_theta(wh) < 0) wh = -wh;
+    if (dot(wo, wh) * dot(wi, wh) > 0) return C3();
+    C3 f(fresnel_dielectric(dot(wo, wh), 1.0f, b.eta));
+    float sd = dot(wo, wh) + eta * dot(wi, wh);
+    float factor = 1 / eta;
+    float D = d.D(wh), G = d.G(wo, wi);
+    C3 T = (1 - b.metallic) * b.transmission * csqrt(b.color);
+    return (1 - f) * T * std::fabs(D * G * eta * eta * abs_dot(wi, wh) * abs_dot(wo, wh) * factor * factor /
+                                   (cti * cto * sd * sd));
+}
+static float disney_btdf_pdf(const Disney& b, V3 wo, V3 wi) {
+    if (same_hemisphere(wo, wi)) return 0.0f;
+    Gtr2 d = trans_dist(b);
+    bool entering = cos_theta(wo) > 0;
+    float ei = entering ? 1.0f : b.eta, et = entering ? b.eta : 1.0f;
+    float eta = et / ei;
+    V3 wh = normalize(wo + wi * eta);
+    if (dot(wo, wh) * dot(wi, wh) > 0) return 0;
+    float sd = dot(wo, wh) + eta * dot(wi, wh);
+    float dwh = std::fabs((eta * eta * dot(wi, wh)) / (sd * sd));
+    return d.pdf(wo, wh) * dwh;
+}
+static C3 disney_clearcoat_fr(const Disney& b, V3 wo, V3 wi) {                          // DisneyClearcoatBrdf
+    float gloss = lerpf(b.cc_gloss, 0.1f, 0.001f);
+    V3 wh = wo + wi;
+    if (wh.x == 0 && wh.y == 0 && wh.z == 0) return C3(0);
+    wh = normalize(wh);
+    float D = gtr1(abs_cos_theta(wh), gloss);
+    float F = schlick_f(0.04f, dot(wo, wh));
+    float G = smith_g1_ggx(abs_cos_theta(wo), 0.25f) * smith_g1_ggx(abs_cos_theta(wi), 0.25f);
+    return C3(b.clearcoat * D * F * G / 4);
+}
+static float disney_clearcoat_pdf(const Disney& b, V3 wo, V3 wi) {
+    float gloss = lerpf(b.cc_gloss, 0.1f, 0.001f);
+    if (!same_hemisphere(wo, wi)) return 0;
+    V3 wh = wo + wi;
+    if (wh.x == 0 && wh.y == 0 && wh.z == 0) return 0;
+    wh = normalize(wh);
+    float D = gtr1(abs_cos_theta(wh), gloss);
+    return D * abs_cos_theta(wh) / (4 * dot(wo, wh));
+}
+static C3 disney_fr(const Disney& b, V3 wo, V3 wi) {                                      // DisneyBsdf.FrImpl
+    if (!same_hemisphere(wo, wi)) {
+        if (b.transmission == 0) return C3(0.0f);
+        return disney_btdf_fr(b, wo, wi);
+    }
+    if (cos_theta(wo) < 0 && cos_theta(wi) < 0) return disney_inner_brdf(b, wo, wi);
+    C3 diffuse(0.0f);
+    float dw = 1 - b.metallic;
+    if (dw > 0) {
+        float fo = schlick_weight(abs_cos_theta(wo)), fi = schlick_weight(abs_cos_theta(wi));
+        C3 R = dw * b.color;
+        C3 ddiff(0.0f);
+        C3 lambert = R * (1 / PI_F) * (1 - fo / 2) * (1 - fi / 2);                      // DisneyDiffuseBrdf
+        if (b.thin) {
+            C3 fake(0.0f);                                                               // DisneyFakeSubsurfaceBrdf
+            V3 wh = wo + wi;
+            if (!(wh.x == 0 && wh.y == 0 && wh.z == 0)) {
+                wh = normalize(wh);
+                float cd = dot(wi, wh);
+                float fss90 = cd * cd * b.roughness;
+                float fss = lerpf(fo, 1.0f, fss90) * lerpf(fi, 1.0f, fss90);
+                float ss = 1.25f * (fss * (1 / (abs_cos_theta(wo) + abs_cos_theta(wi)) - 0.5f) + 0.5f);
+                fake = R * (1 / PI_F) * ss;
+            }
+            ddiff = lambert * (1 - b.flatness) + fake * b.flatness;
+        } else if (b.sdist.black()) {
+            ddiff = lambert;
+        }
+        C3 retro(0.0f);                                                                  // DisneyRetroBrdf
+        {
+            V3 wh = wo + wi;
+            if (!(wh.x == 0 && wh.y == 0 && wh.z == 0)) {
+                wh = normalize(wh);
+                float cd = dot(wi, wh);
+                float rr = 2 * b.roughness * cd * cd;
+                retro = R * (1 / PI_F) * rr * (fo + fi + fo * fi * (rr - 1));
+            }
+        }
+        C3 sheen(0.0f);
+        if (b.sheen > 0) {                                                               // DisneySheenBrdf
+            C3 sc = lerpc(b.sheen_tint, C3(1.0f), color_tint(b));
+            V3 wh = wo + wi;
+            if (!(wh.x == 0 && wh.y == 0 && wh.z == 0)) {
+                wh = normalize(wh);
+                sheen = (dw * b.sheen * sc) * schlick_weight(dot(wi, wh));
+            }
+        }
+        diffuse = (retro + sheen + ddiff) * (1 - b.transmission);
+    }
+    C3 spec = disney_spec_brdf(b, wo, wi);
+    C3 cc(0.0f);
+    if (b.clearcoat > 0) cc = disney_clearcoat_fr(b, wo, wi);
+    return diffuse + spec + cc;
+}
+static void disney_weights(const Disney& b, float* dw, float* tw, float* sw, float* cw) {     // GetSampleWeight
+    float a = nclamp(b.color.lum() * (1 - b.metallic), 0.3f, 0.7f);
+    float bb = 1 - a;
+    *dw = a * (1 - b.transmission);
+    *tw = a * b.transmission;
+    *sw = bb * 2 / (2 + b.clearcoat);
+    *cw = bb * b.clearcoat / (2 + b.clearcoat);
+}
+static float disney_pdf(const Disney& b, V3 wo, V3 wi) {                                  // DisneyBsdf.PdfImpl
+    float dw, tw, sw, cw;
+    disney_weights(b, &dw, &tw, &sw, &cw);
+    if (cos_theta(wo) < 0) {
+        if (b.transmission == 0) return 0;
+        float fr = nclamp(fresnel_dielectric(cos_theta(wo), 1, b.eta), 0.1f, 0.9f);
+        if (cos_theta(wi) > 0) return disney_btdf_pdf(b, wo, wi) * (1 - fr);
+        return mf_reflect_pdf_g(trans_dist(b), wo, wi) * fr;
+    }
+    if (cos_theta(wi) < 0) return disney_btdf_pdf(b, wo, wi) * tw;
+    float dp = 0;
+    if (b.thin || b.sdist.black()) dp = same_hemisphere(wo, wi) ? abs_cos_theta(wi) / PI_F : 0.0f;
+    float sp = mf_reflect_pdf_g(spec_dist(b), wo, wi);
+    float cp = disney_clearcoat_pdf(b, wo, wi);
+    return dp * dw + sp * sw + cp * cw;
+}
+
+static Disney load_disney(const cr_scene& s, const rd_material_desc& m, V2 uv) {         // Disney.CreateBsdf
+    Disney b;
+    b.color = tex_at(s, m.tex[0], uv);
+    b.metallic = tex_at(s, m.tex[1], uv).r;
+    b.roughness = tex_at(s, m.tex[2], uv).r;
+    b.eta = tex_at(s, m.tex[3], uv).r;
+    b.spec_scale = tex_at(s, m.tex[4], uv).r;
+    b.spec_tint = tex_at(s, m.tex[5], uv).r;
+    b.aniso = tex_at(s, m.tex[6], uv).r;
+    b.sheen = tex_at(s, m.tex[7], uv).r;
+    b.sheen_tint = tex_at(s, m.tex[8], uv).r;
+    b.clearcoat = tex_at(s, m.tex[9], uv).r;
+    b.cc_gloss = tex_at(s, m.tex[10], uv).r;
+    b.sdist = tex_at(s, m.tex[11], uv);
+    b.transmission = tex_at(s, m.tex[12], uv).r;
+    b.trans_rough = tex_at(s, m.tex[13], uv).r;
+    b.flatness = tex_at(s, m.tex[14], uv).r;
+    b.thin = m.thin != 0;
+    return b;
+}
+
 static C3 material_fr(const cr_scene& s, const rd_material_desc& m, V3 wo, V3 wi, V2 uv) {
     switch (m.type) {
+    case RD_MAT_ROUGH_PLASTIC: {
+        MfDist d(m.dist, tex_at(s, m.tex[1], uv).r, tex_at(s, m.tex[2], uv).r);
+        return plastic_fr(d, tex_at(s, m.tex[0], uv), m.kd, m.ks, m.eta_a, m.eta_b, m.two_sided != 0, wo, wi);
+    }
+    case RD_MAT_PERFECT_GLASS:     // FresnelSpecularBsdf.Fr (FresnelSpecularBsdf.cs:30-33)
+    case RD_MAT_PERFECT_MIRROR:    // SpecularReflectionBrdf.Fr (SpecularReflectionBrdf.cs:17-20)
+        return C3(0.0f);
+    case RD_MAT_DISNEY:
+        return disney_fr(load_disney(s, m, uv), wo, wi);
     case RD_MAT_DIFFUSE: {                                           // Materials/Diffuse.cs:21-27
         if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!m.two_sided) return C3(0.0f); }
         C3 R = tex_at(s, m.tex[0], uv);                              // LambertianReflectionBrdf.Fr :16-19
@@ -836,6 +1096,15 @@ static C3 material_fr(const cr_scene& s, const rd_material_desc& m, V3 wo, V3 wi
 
 static float material_pdf(const cr_scene& s, const rd_material_desc& m, V3 wo, V3 wi, V2 uv) {
     switch (m.type) {
+    case RD_MAT_ROUGH_PLASTIC: {
+        MfDist d(m.dist, tex_at(s, m.tex[1], uv).r, tex_at(s, m.tex[2], uv).r);
+        return plastic_pdf(d, m.kd, m.ks, m.two_sided != 0, wo, wi);
+    }
+    case RD_MAT_PERFECT_GLASS:
+    case RD_MAT_PERFECT_MIRROR:
+        return 0.0f;
+    case RD_MAT_DISNEY:
+        return disney_pdf(load_disney(s, m, uv), wo, wi);
     case RD_MAT_DIFFUSE: {                                           // Diffuse.cs:29-37
         if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!m.two_sided) return 0.0f; }
         return same_hemisphere(wo, wi) ? abs_cos_theta(wi) / PI_F : 0.0f;   // Lambert.Pdf :21-24
@@ -879,9 +1148,165 @@ static float material_pdf(const cr_scene& s, const rd_material_desc& m, V3 wo, V
     }
 }
 
+// Cosine-hemisphere sample with the z-flip of LambertianReflectionBrdf.Sample
+// (LambertianReflectionBrdf.cs:26-36) / DisneyBsdf.DiffuseSample.
+static bool cosine_sample(V3 wo, Rng& rng, V3* wi_out) {
+    float ux = rng.next_float(), uy = rng.next_float();
+    V3 wi = normalize(square_to_cosine_hemisphere(ux, uy));
+    if (wo.z < 0) wi.z *= -1;
+    if (!same_hemisphere(wo, wi)) return false;
+    *wi_out = wi;
+    return true;
+}
+
+// MicrofacetReflectionBrdf.Sample's direction part (MicrofacetReflectionBrdf.cs:50-59)
+template <class SampleWh>
+static bool mf_reflect_sample_dir(V3 wo, Rng& rng, SampleWh swh, V3* wi_out, V3* wh_out) {
+    if (cos_theta(wo) == 0) return false;
+    V3 wh = swh(wo, rng);
+    if (dot(wo, wh) < 0) return false;
+    V3 wi = reflect(-wo, wh);
+    if (!same_hemisphere(wo, wi)) return false;
+    *wi_out = wi;
+    *wh_out = wh;
+    return true;
+}
+
+static BSample disney_sample(const Disney& b, V3 wo, Rng& rng) {                         // DisneyBsdf.Sample
+    BSample out;
+    Gtr2 sd = spec_dist(b), td = trans_dist(b);
+    auto wh_spec = [&](V3 w, Rng& r) { return sd.m.sample_wh(w, r); };
+    auto wh_trans = [&](V3 w, Rng& r) { return td.m.sample_wh(w, r); };
+    auto finish = [&](V3 wi, int type) {
+        out.wi = wi;
+        out.fr = disney_fr(b, wo, wi);
+        out.pdf = disney_pdf(b, wo, wi);
+        out.type = type;
+        return out;
+    };
+    // MicrofacetTransmissionBtdf.Sample (MicrofacetTransmissionBtdf.cs:59-75), etaA = 1, etaB = Eta
+    auto btdf_sample = [&](V3* wi_out) -> bool {
+        if (wo.z == 0) return false;
+        V3 wh = td.m.sample_wh(wo, rng);
+        if (dot(wo, wh) < 0) return false;
+        bool entering = cos_theta(wo) > 0;
+        float ei = entering ? 1.0f : b.eta, et = entering ? b.eta : 1.0f;
+        V3 wi;
+        if (!refract(wo, wh, ei / et, &wi)) return false;
+        C3 f = disney_btdf_fr(b, wo, wi);
+        float p = disney_btdf_pdf(b, wo, wi);
+        if (!f.valid() || std::isinf(p)) return false;
+        if (p == 0) return false;
+        *wi_out = wi;
+        return true;
+    };
+    V3 wi, wh;
+    if (cos_theta(wo) < 0) {
+        if (b.transmission == 0) return out;
+        float fr = nclamp(fresnel_dielectric(cos_theta(wo), 1, b.eta), 0.1f, 0.9f);
+        if (rng.next_float() >= fr) {
+            if (!btdf_sample(&wi)) return BSample();
+            return finish(wi, T_GLOSSY | T_TRANSMISSION);
+        }
+        // inner MicrofacetReflectionBrdf<Dielectric, Gtr2>: pdf 0 when its sample fails
+        if (!mf_reflect_sample_dir(wo, rng, wh_trans, &wi, &wh)) return BSample();
+        if (td.pdf(wo, wh) / (4 * dot(wi, wh)) == 0) return BSample();
+        return finish(wi, T_REFLECTION | T_GLOSSY);
+    }
+    float dw, tw, sw, cw;
+    disney_weights(b, &dw, &tw, &sw, &cw);
+    float u = rng.next_float();
+    if (u < dw) {
+        if (b.thin || b.sdist.black()) {
+            if (!cosine_sample(wo, rng, &wi)) return BSample();
+            if ((same_hemisphere(wo, wi) ? abs_cos_theta(wi) / PI_F : 0.0f) == 0) return BSample();
+            return finish(wi, T_REFLECTION | T_DIFFUSE);
+        }
+        // scatting_distance > 0 selects the BSSRDF branch, rejected at scene creation
+        return BSample();
+    }
+    u -= dw;
+    if (u < tw) {
+        if (!btdf_sample(&wi)) return BSample();
+        return finish(wi, T_GLOSSY | T_TRANSMISSION);
+    }
+    u -= tw;
+    if (u < sw) {
+        if (!mf_reflect_sample_dir(wo, rng, wh_spec, &wi, &wh)) return BSample();
+        if (sd.pdf(wo, wh) / (4 * dot(wi, wh)) == 0) return BSample();
+        return finish(wi, T_REFLECTION | T_GLOSSY);
+    }
+    // DisneyClearcoatBrdf.Sample
+    if (cos_theta(wo) == 0) return BSample();
+    float gloss = lerpf(b.cc_gloss, 0.1f, 0.001f);
+    float ux = rng.next_float(), uy = rng.next_float();
+    float a2 = sqr(gloss);
+    float ct = std::sqrt(nmax(0, (1 - std::pow(a2, 1 - ux)) / (1 - a2)));
+    float st = std::sqrt(nmax(0, 1 - ct * ct));
+    float phi = 2 * PI_F * uy;
+    wh = spherical_direction(st, ct, phi);
+    if (!same_hemisphere(wo, wh)) wh = -wh;
+    wi = reflect(-wo, wh);
+    if (!same_hemisphere(wo, wi)) return BSample();
+    if (disney_clearcoat_pdf(b, wo, wi) == 0) return BSample();
+    return finish(wi, T_REFLECTION | T_GLOSSY);
+}
+
 static BSample material_sample(const cr_scene& s, const rd_material_desc& m, V3 wo, V2 uv, Rng& rng) {
     BSample out;
     switch (m.type) {
+    case RD_MAT_ROUGH_PLASTIC: {                                     // RoughPlastic.SampleImpl (:134-151)
+        if (cos_theta(wo) <= 0) { if (!m.two_sided) return out; }
+        MfDist d(m.dist, tex_at(s, m.tex[1], uv).r, tex_at(s, m.tex[2], uv).r);
+        C3 r = tex_at(s, m.tex[0], uv);
+        float p = m.ks / (m.kd + m.ks);
+        V3 gwi(0.0f), gwh, dwi(0.0f);
+        int gty = 0, dty = 0;
+        auto swh = [&](V3 w, Rng& rr) { return d.sample_wh(w, rr); };
+        if (mf_reflect_sample_dir(wo, rng, swh, &gwi, &gwh)) gty = T_REFLECTION | T_GLOSSY;
+        else gwi = V3(0.0f);
+        if (cosine_sample(wo, rng, &dwi)) dty = T_REFLECTION | T_DIFFUSE;
+        else dwi = V3(0.0f);
+        V3 wi;
+        int ty;
+        if (rng.next_float() < p) { wi = gwi; ty = gty; }
+        else { wi = dwi; ty = dty; }
+        out.wi = wi;
+        out.pdf = plastic_pdf(d, m.kd, m.ks, m.two_sided != 0, wo, wi);
+        out.fr = plastic_fr(d, r, m.kd, m.ks, m.eta_a, m.eta_b, m.two_sided != 0, wo, wi);
+        out.type = ty;
+        return out;
+    }
+    case RD_MAT_PERFECT_GLASS: {                                     // FresnelSpecularBsdf.Sample (:34-65)
+        C3 R = tex_at(s, m.tex[0], uv), T = tex_at(s, m.tex[1], uv);
+        float f = fresnel_dielectric(cos_theta(wo), m.eta_a, m.eta_b);
+        if (rng.next_float() < f) {
+            if (cos_theta(wo) == 0) return out;
+            V3 wi(-wo.x, -wo.y, wo.z);
+            out.wi = wi; out.fr = f * R / abs_cos_theta(wi); out.pdf = f; out.type = T_REFLECTION | T_SPECULAR;
+            return out;
+        }
+        bool entering = cos_theta(wo) > 0;
+        float ei = entering ? m.eta_a : m.eta_b, et = entering ? m.eta_b : m.eta_a;
+        V3 n = entering ? V3(0, 0, 1) : V3(0, 0, -1);
+        V3 wi;
+        if (!refract(wo, n, ei / et, &wi) || cos_theta(wi) == 0.0f) return out;
+        C3 ft = T * (1.0f - f);
+        ft = ft * ((ei * ei) / (et * et));   // TransportMode.Radiance
+        ft = ft / abs_cos_theta(wi);
+        out.wi = wi; out.fr = ft; out.pdf = 1 - f; out.type = T_TRANSMISSION | T_SPECULAR;
+        return out;
+    }
+    case RD_MAT_PERFECT_MIRROR: {                                    // PerfectMirror.Sample + SpecularReflectionBrdf
+        if (cos_theta(wo) <= 0) { if (!m.two_sided) return out; }
+        if (abs_cos_theta(wo) == 0) return out;
+        V3 wi(-wo.x, -wo.y, wo.z);
+        out.wi = wi; out.fr = tex_at(s, m.tex[0], uv) / abs_cos_theta(wi); out.pdf = 1.0f;
+        out.type = T_REFLECTION | T_SPECULAR;
+        return out;
+    }
+    case RD_MAT_DISNEY:
+        return disney_sample(load_disney(s, m, uv), wo, rng);
     case RD_MAT_DIFFUSE: {                                           // Diffuse.cs:39-49
         if (cos_theta(wo) <= 0) { if (!m.two_sided) return out; }
         C3 R = tex_at(s, m.tex[0], uv);
@@ -1295,8 +1720,14 @@ cr_scene* cr_scene_create(const rd_scene_desc* d) {
     }
     for (int i = 0; i < d->num_materials; i++) {
         int ty = s->mats[i].type;
-        if (ty != RD_MAT_DIFFUSE && ty != RD_MAT_ROUGH_METAL && ty != RD_MAT_ROUGH_GLASS) {
-            set_err("material type not in this restatement yet"); delete s; return nullptr;
+        if (ty < RD_MAT_DIFFUSE || ty > RD_MAT_DISNEY) { set_err("unknown material type"); delete s; return nullptr; }
+        if (ty == RD_MAT_DISNEY) {
+            // scatting_distance > 0 takes the reference's unfinished BSSRDF branch
+            // (PathTracer.cs:244-259, "TODO"); it is outside this path
+            const Tex& sd = s->tex[s->mats[i].tex[11]];
+            bool black = sd.w == 0 ? sd.value.black() : true;
+            for (size_t k = 0; sd.w > 0 && k < sd.data.size(); k++) if (sd.data[k] != 0.0f) black = false;
+            if (!black && !s->mats[i].thin) { set_err("Disney scatting_distance > 0 (BSSRDF) is not on this path"); delete s; return nullptr; }
         }
     }
     for (int i = 0; i < d->num_lights; i++) {

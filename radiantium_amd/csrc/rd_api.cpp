@@ -277,13 +277,21 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     }
     for (int i = 0; i < d->num_materials; i++) {
         int ty = d->materials[i].type;
-        if (ty != RD_MAT_DIFFUSE && ty != RD_MAT_ROUGH_METAL && ty != RD_MAT_ROUGH_GLASS)
-            return bad("material type not built for the GPU path yet");
+        if (ty < RD_MAT_DIFFUSE || ty > RD_MAT_DISNEY) return bad("unknown material type");
+        if (d->num_textures == 0) return bad("materials need textures (constants count)");
         for (int k = 0; k < 16; k++) {
             int tx = d->materials[i].tex[k];
-            if (tx < 0 || (d->num_textures > 0 && tx >= d->num_textures)) return bad("material texture index out of range");
+            if (tx < 0 || tx >= d->num_textures) return bad("material texture index out of range");
         }
-        if (d->num_textures == 0) return bad("materials need textures (constants count)");
+        if (ty == RD_MAT_DISNEY && !d->materials[i].thin) {
+            // scatting_distance > 0 selects the reference's unfinished BSSRDF
+            // branch (PathTracer.cs:244-259), which is not on this path
+            const rd_texture_desc& sd = d->textures[d->materials[i].tex[11]];
+            bool black = sd.width == 0 ? (sd.value[0] == 0 && sd.value[1] == 0 && sd.value[2] == 0) : true;
+            for (size_t k = 0; sd.width > 0 && k < (size_t)sd.width * sd.height * 3; k++)
+                if (sd.data[k] != 0.0f) black = false;
+            if (!black) return bad("Disney scatting_distance > 0 (BSSRDF) is not on this path");
+        }
     }
     for (int i = 0; i < d->num_lights; i++) {
         if (d->lights[i].type != RD_LIGHT_AREA) return bad("infinite lights not built for the GPU path yet");

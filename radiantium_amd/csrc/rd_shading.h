@@ -327,12 +327,21 @@ __device__ __forceinline__ bool refract(f3 wi, f3 n, float eta, f3* wt) {
 }
 
 // ---------------------------------------------------------------- materials
+// Disney parameters (Materials/Disney.cs:65-85 CreateBsdf).
+struct DisneyP {
+    f3 color;
+    float metallic, eta, roughness, spec_tint, aniso, sheen, sheen_tint, clearcoat, cc_gloss, spec_scale;
+    float transmission, trans_rough, flatness;
+    bool thin, sdist_black;
+};
+
 struct MatCtx {
     int type, two_sided;
     Mf mf;
-    f3 r, t;        // diffuse kd / glass r,t
+    f3 r, t;        // diffuse kd / glass, plastic and mirror colours
     f3 eta, k;      // conductor
-    float eta_a, eta_b;
+    float eta_a, eta_b, kd, ks;
+    DisneyP dp;
 };
 
 __device__ __forceinline__ MatCtx load_material(const DevScene& sc, int id, float su, float sv) {
@@ -342,6 +351,8 @@ __device__ __forceinline__ MatCtx load_material(const DevScene& sc, int id, floa
     c.two_sided = m.two_sided;
     c.eta_a = m.eta_a;
     c.eta_b = m.eta_b;
+    c.kd = m.kd;
+    c.ks = m.ks;
     c.r = mk(0.0f); c.t = mk(0.0f); c.eta = mk(0.0f); c.k = mk(0.0f);
     c.mf.kind = 0; c.mf.ax = c.mf.ay = 0.001f;
     switch (m.type) {
@@ -358,6 +369,37 @@ __device__ __forceinline__ MatCtx load_material(const DevScene& sc, int id, floa
         c.r = tex_sample(sc, m.tex[0], su, sv);
         c.t = tex_sample(sc, m.tex[1], su, sv);
         break;
+    case kMatRoughPlastic: // RoughPlastic.SampleParams / SampleRoughness (RoughPlastic.cs:37-45)
+        c.mf = make_mf(m.dist, tex_sample(sc, m.tex[1], su, sv).x, tex_sample(sc, m.tex[2], su, sv).x);
+        c.r = tex_sample(sc, m.tex[0], su, sv);
+        break;
+    case kMatPerfectGlass:
+        c.r = tex_sample(sc, m.tex[0], su, sv);
+        c.t = tex_sample(sc, m.tex[1], su, sv);
+        break;
+    case kMatPerfectMirror:
+        c.r = tex_sample(sc, m.tex[0], su, sv);
+        break;
+    case kMatDisney: {
+        DisneyP& b = c.dp;
+        b.color = tex_sample(sc, m.tex[0], su, sv);
+        b.metallic = tex_sample(sc, m.tex[1], su, sv).x;
+        b.roughness = tex_sample(sc, m.tex[2], su, sv).x;
+        b.eta = tex_sample(sc, m.tex[3], su, sv).x;
+        b.spec_scale = tex_sample(sc, m.tex[4], su, sv).x;
+        b.spec_tint = tex_sample(sc, m.tex[5], su, sv).x;
+        b.aniso = tex_sample(sc, m.tex[6], su, sv).x;
+        b.sheen = tex_sample(sc, m.tex[7], su, sv).x;
+        b.sheen_tint = tex_sample(sc, m.tex[8], su, sv).x;
+        b.clearcoat = tex_sample(sc, m.tex[9], su, sv).x;
+        b.cc_gloss = tex_sample(sc, m.tex[10], su, sv).x;
+        b.sdist_black = is_black(tex_sample(sc, m.tex[11], su, sv));
+        b.transmission = tex_sample(sc, m.tex[12], su, sv).x;
+        b.trans_rough = tex_sample(sc, m.tex[13], su, sv).x;
+        b.flatness = tex_sample(sc, m.tex[14], su, sv).x;
+        b.thin = m.thin != 0;
+        break;
+    }
     default:
         break;
     }
@@ -378,7 +420,217 @@ __device__ __forceinline__ f3 conductor_fr(const MatCtx& c, f3 wo, f3 wi) {
     return mk(1.0f) * fabs3((D * G * F) / (4 * cti * cto));
 }
 
-// Material.Fr (Materials/Diffuse.cs:21-27, RoughMetal.cs:33-58, RoughGlass.cs:151-195)
+__device__ __forceinline__ float mf_reflect_pdf(const Mf& m, f3 wo, f3 wi) {   // MicrofacetReflectionBrdf.Pdf
+    if (!same_hemisphere(wo, wi)) return 0;
+    f3 wh = normalize(wo + wi);
+    return mf_pdf(m, wo, wh) / (4 * dot(wo, wh));
+}
+
+// RoughPlastic FrImpl / PdfImpl (RoughPlastic.cs:47-110)
+__device__ __forceinline__ f3 plastic_fr(const MatCtx& c, f3 wo, f3 wi) {
+    if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!c.two_sided) return mk(0.0f); }
+    f3 diffuse = same_hemisphere(wo, wi) ? c.r / kPi : mk(0.0f);
+    f3 glossy = mk(0.0f);
+    if (same_hemisphere(wo, wi)) {
+        float cto = abs_cos_theta(wo), cti = abs_cos_theta(wi);
+        f3 wh = wi + wo;
+        if (!(cti == 0 || cto == 0) && !(wh.x == 0 && wh.y == 0 && wh.z == 0)) {
+            wh = normalize(wh);
+            f3 F = mk(fr_dielectric(dot(wi, faceforward(wh, mk(0, 0, 1))), c.eta_a, c.eta_b));
+            float D = mf_D(c.mf, wh), G = mf_G(c.mf, wo, wi);
+            glossy = c.r * fabs3((D * G * F) / (4 * cti * cto));
+        }
+    }
+    return c.kd * diffuse + c.ks * glossy;
+}
+__device__ __forceinline__ float plastic_pdf(const MatCtx& c, f3 wo, f3 wi) {
+    if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!c.two_sided) return 0.0f; }
+    float p = c.ks / (c.kd + c.ks);
+    float diffuse = same_hemisphere(wo, wi) ? abs_cos_theta(wi) / kPi : 0.0f;
+    float glossy = mf_reflect_pdf(c.mf, wo, wi);
+    return (1 - p) * diffuse + p * glossy;
+}
+
+// ---- Disney (Bxdf/DisneyBsdf.cs:13-512)
+__device__ __forceinline__ float lerpf(float t, float a, float b) { return (1 - t) * a + t * b; }
+__device__ __forceinline__ f3 lerp3(float w, f3 a, f3 b) { return (1 - w) * a + w * b; }
+__device__ __forceinline__ float pow5f(float a) { return (a * a) * (a * a) * a; }
+__device__ __forceinline__ float schlick_weight(float c) { return pow5f(nclamp(1 - c, 0, 1)); }
+__device__ __forceinline__ f3 schlick3(f3 r0, float c) { return lerp3(schlick_weight(c), r0, mk(1.0f)); }
+__device__ __forceinline__ float schlick1(float r0, float c) { return r0 + (1.0f - r0) * schlick_weight(c); }
+__device__ __forceinline__ float luminance(f3 c) { return c.x * 0.212671f + c.y * 0.715160f + c.z * 0.072169f; }
+__device__ __forceinline__ f3 color_tint(const DisneyP& b) {
+    float l = luminance(b.color);
+    return l > 0 ? b.color / l : mk(1.0f);
+}
+__device__ __forceinline__ Mf gtr2_dist(const DisneyP& b, float rough) {   // AnisAlpha
+    float aspect = sqrtf(1 - b.aniso * 0.9f);
+    Mf m;
+    m.kind = 1;
+    m.ax = nmax(0.001f, sqr(rough) / aspect);
+    m.ay = nmax(0.001f, sqr(rough) * aspect);
+    return m;
+}
+// DisneyDistributionGtr2.G: separable G1(wo) G1(wi)
+__device__ __forceinline__ float gtr2_G(const Mf& m, f3 wo, f3 wi) {
+    return (1 / (1 + mf_lambda(m, wo))) * (1 / (1 + mf_lambda(m, wi)));
+}
+__device__ __forceinline__ float gtr2_reflect_pdf(const Mf& m, f3 wo, f3 wi) {
+    if (!same_hemisphere(wo, wi)) return 0;
+    f3 wh = normalize(wo + wi);
+    return mf_pdf(m, wo, wh) / (4 * dot(wo, wh));
+}
+__device__ __forceinline__ f3 disney_fresnel(const DisneyP& b, float ci) {   // DisneyFresnel.Eval
+    f3 r0 = color_tint(b) * (sqr(b.eta - 1) / sqr(b.eta + 1));
+    f3 fd = lerp3(b.spec_tint, mk(fr_dielectric(ci, 1, b.eta)), schlick3(r0, ci));
+    return lerp3(b.metallic, fd * b.spec_scale, schlick3(b.color, ci));
+}
+// MicrofacetReflectionBrdf<F, Gtr2>.Fr: spec (DisneyFresnel, R=1) or inner (dielectric F, R=T*color)
+__device__ __forceinline__ f3 gtr2_reflect_fr(const DisneyP& b, const Mf& m, bool inner, f3 wo, f3 wi) {
+    if (!same_hemisphere(wo, wi)) return mk(0.0f);
+    float cto = abs_cos_theta(wo), cti = abs_cos_theta(wi);
+    f3 wh = wi + wo;
+    if (cti == 0 || cto == 0) return mk(0.0f);
+    if (wh.x == 0 && wh.y == 0 && wh.z == 0) return mk(0.0f);
+    wh = normalize(wh);
+    float ci = dot(wi, faceforward(wh, mk(0, 0, 1)));
+    f3 F = inner ? mk(fr_dielectric(ci, 1, b.eta)) : disney_fresnel(b, ci);
+    f3 R = inner ? b.transmission * b.color : mk(1.0f);
+    float D = mf_D(m, wh), G = gtr2_G(m, wo, wi);
+    return R * fabs3((D * G * F) / (4 * cti * cto));
+}
+// MicrofacetTransmissionBtdf<Gtr2> Fr / Pdf (MicrofacetTransmissionBtdf.cs:25-57), etaA = 1, etaB = Eta
+__device__ __forceinline__ f3 disney_btdf_fr(const DisneyP& b, f3 wo, f3 wi) {
+    if (same_hemisphere(wo, wi)) return mk(0.0f);
+    float cto = cos_theta(wo), cti = cos_theta(wi);
+    if (cti == 0 || cto == 0) return mk(0.0f);
+    Mf m = gtr2_dist(b, b.trans_rough);
+    bool entering = cos_theta(wo) > 0;
+    float ei = entering ? 1.0f : b.eta, et = entering ? b.eta : 1.0f;
+    float eta = et / ei;
+    f3 wh = normalize(wo + wi * eta);
+    if (cos_theta(wh) < 0) wh = -wh;
+    if (dot(wo, wh) * dot(wi, wh) > 0) return mk(0.0f);
+    f3 F = mk(fr_dielectric(dot(wo, wh), 1.0f, b.eta));
+    float sd = dot(wo, wh) + eta * dot(wi, wh);
+    float factor = 1 / eta;
+    float D = mf_D(m, wh), G = gtr2_G(m, wo, wi);
+    f3 T = (1 - b.metallic) * b.transmission * sqrt3(b.color);
+    return (1 - F) * T * fabsf(D * G * eta * eta * abs_dot(wi, wh) * abs_dot(wo, wh) * factor * factor /
+                              (cti * cto * sd * sd));
+}
+__device__ __forceinline__ float disney_btdf_pdf(const DisneyP& b, f3 wo, f3 wi) {
+    if (same_hemisphere(wo, wi)) return 0.0f;
+    Mf m = gtr2_dist(b, b.trans_rough);
+    bool entering = cos_theta(wo) > 0;
+    float ei = entering ? 1.0f : b.eta, et = entering ? b.eta : 1.0f;
+    float eta = et / ei;
+    f3 wh = normalize(wo + wi * eta);
+    if (dot(wo, wh) * dot(wi, wh) > 0) return 0;
+    float sd = dot(wo, wh) + eta * dot(wi, wh);
+    float dwh = fabsf((eta * eta * dot(wi, wh)) / (sd * sd));
+    return mf_pdf(m, wo, wh) * dwh;
+}
+__device__ __forceinline__ float gtr1(float c, float a) {
+    float a2 = a * a;
+    return (a2 - 1) / (kPi * logf(a2) * (1 + (a2 - 1) * c * c));
+}
+__device__ __forceinline__ float smith_g1_ggx(float c, float a) {
+    float a2 = a * a, c2 = c * c;
+    return 2 / (1 + sqrtf(1 + a2 * (1 - c2) / c2));
+}
+__device__ __forceinline__ f3 clearcoat_fr(const DisneyP& b, f3 wo, f3 wi) {   // DisneyClearcoatBrdf.Fr
+    float gloss = lerpf(b.cc_gloss, 0.1f, 0.001f);
+    f3 wh = wo + wi;
+    if (wh.x == 0 && wh.y == 0 && wh.z == 0) return mk(0.0f);
+    wh = normalize(wh);
+    float D = gtr1(abs_cos_theta(wh), gloss);
+    float F = schlick1(0.04f, dot(wo, wh));
+    float G = smith_g1_ggx(abs_cos_theta(wo), 0.25f) * smith_g1_ggx(abs_cos_theta(wi), 0.25f);
+    return mk(b.clearcoat * D * F * G / 4);
+}
+__device__ __forceinline__ float clearcoat_pdf(const DisneyP& b, f3 wo, f3 wi) {
+    float gloss = lerpf(b.cc_gloss, 0.1f, 0.001f);
+    if (!same_hemisphere(wo, wi)) return 0;
+    f3 wh = wo + wi;
+    if (wh.x == 0 && wh.y == 0 && wh.z == 0) return 0;
+    wh = normalize(wh);
+    return gtr1(abs_cos_theta(wh), gloss) * abs_cos_theta(wh) / (4 * dot(wo, wh));
+}
+__device__ __forceinline__ f3 disney_fr(const DisneyP& b, f3 wo, f3 wi) {     // DisneyBsdf.FrImpl
+    if (!same_hemisphere(wo, wi)) {
+        if (b.transmission == 0) return mk(0.0f);
+        return disney_btdf_fr(b, wo, wi);
+    }
+    if (cos_theta(wo) < 0 && cos_theta(wi) < 0) return gtr2_reflect_fr(b, gtr2_dist(b, b.trans_rough), true, wo, wi);
+    f3 diffuse = mk(0.0f);
+    float dw = 1 - b.metallic;
+    if (dw > 0) {
+        float fo = schlick_weight(abs_cos_theta(wo)), fi = schlick_weight(abs_cos_theta(wi));
+        f3 R = dw * b.color;
+        f3 ddiff = mk(0.0f);
+        f3 lambert = R * (1 / kPi) * (1 - fo / 2) * (1 - fi / 2);
+        f3 wh = wo + wi;
+        bool whz = wh.x == 0 && wh.y == 0 && wh.z == 0;
+        f3 whn = whz ? wh : normalize(wh);
+        if (b.thin) {
+            f3 fake = mk(0.0f);
+            if (!whz) {
+                float cd = dot(wi, whn);
+                float fss90 = cd * cd * b.roughness;
+                float fss = lerpf(fo, 1.0f, fss90) * lerpf(fi, 1.0f, fss90);
+                float ss = 1.25f * (fss * (1 / (abs_cos_theta(wo) + abs_cos_theta(wi)) - 0.5f) + 0.5f);
+                fake = R * (1 / kPi) * ss;
+            }
+            ddiff = lambert * (1 - b.flatness) + fake * b.flatness;
+        } else if (b.sdist_black) {
+            ddiff = lambert;
+        }
+        f3 retro = mk(0.0f);
+        if (!whz) {
+            float cd = dot(wi, whn);
+            float rr = 2 * b.roughness * cd * cd;
+            retro = R * (1 / kPi) * rr * (fo + fi + fo * fi * (rr - 1));
+        }
+        f3 sheen = mk(0.0f);
+        if (b.sheen > 0) {
+            f3 scol = lerp3(b.sheen_tint, mk(1.0f), color_tint(b));
+            if (!whz) sheen = (dw * b.sheen * scol) * schlick_weight(dot(wi, whn));
+        }
+        diffuse = (retro + sheen + ddiff) * (1 - b.transmission);
+    }
+    f3 spec = gtr2_reflect_fr(b, gtr2_dist(b, b.roughness), false, wo, wi);
+    f3 cc = mk(0.0f);
+    if (b.clearcoat > 0) cc = clearcoat_fr(b, wo, wi);
+    return diffuse + spec + cc;
+}
+__device__ __forceinline__ void disney_weights(const DisneyP& b, float* dw, float* tw, float* sw, float* cw) {
+    float a = nclamp(luminance(b.color) * (1 - b.metallic), 0.3f, 0.7f);
+    float bb = 1 - a;
+    *dw = a * (1 - b.transmission);
+    *tw = a * b.transmission;
+    *sw = bb * 2 / (2 + b.clearcoat);
+    *cw = bb * b.clearcoat / (2 + b.clearcoat);
+}
+__device__ __forceinline__ float disney_pdf(const DisneyP& b, f3 wo, f3 wi) {  // DisneyBsdf.PdfImpl
+    float dw, tw, sw, cw;
+    disney_weights(b, &dw, &tw, &sw, &cw);
+    if (cos_theta(wo) < 0) {
+        if (b.transmission == 0) return 0;
+        float fr = nclamp(fr_dielectric(cos_theta(wo), 1, b.eta), 0.1f, 0.9f);
+        if (cos_theta(wi) > 0) return disney_btdf_pdf(b, wo, wi) * (1 - fr);
+        return gtr2_reflect_pdf(gtr2_dist(b, b.trans_rough), wo, wi) * fr;
+    }
+    if (cos_theta(wi) < 0) return disney_btdf_pdf(b, wo, wi) * tw;
+    float dp = 0;
+    if (b.thin || b.sdist_black) dp = same_hemisphere(wo, wi) ? abs_cos_theta(wi) / kPi : 0.0f;
+    float sp = gtr2_reflect_pdf(gtr2_dist(b, b.roughness), wo, wi);
+    float cp = clearcoat_pdf(b, wo, wi);
+    return dp * dw + sp * sw + cp * cw;
+}
+
+// Material.Fr (Diffuse.cs:21-27, RoughMetal.cs:33-58, RoughGlass.cs:151-195,
+// RoughPlastic.cs:65-74, FresnelSpecularBsdf.cs:30, SpecularReflectionBrdf.cs:17, Disney.cs:87-90)
 __device__ __forceinline__ f3 mat_fr(const MatCtx& c, f3 wo, f3 wi) {
     switch (c.type) {
     case kMatDiffuse:
@@ -411,12 +663,16 @@ __device__ __forceinline__ f3 mat_fr(const MatCtx& c, f3 wo, f3 wi) {
         }
         return valid_color(bsdf) ? bsdf : mk(0.0f);
     }
-    default:
+    case kMatRoughPlastic:
+        return plastic_fr(c, wo, wi);
+    case kMatDisney:
+        return disney_fr(c.dp, wo, wi);
+    default:   // perfect glass / mirror: delta lobes
         return mk(0.0f);
     }
 }
 
-// Material.Pdf (Diffuse.cs:29-37, RoughMetal.cs:60-85, RoughGlass.cs:197-238)
+// Material.Pdf
 __device__ __forceinline__ float mat_pdf(const MatCtx& c, f3 wo, f3 wi) {
     switch (c.type) {
     case kMatDiffuse:
@@ -424,9 +680,7 @@ __device__ __forceinline__ float mat_pdf(const MatCtx& c, f3 wo, f3 wi) {
         return same_hemisphere(wo, wi) ? abs_cos_theta(wi) / kPi : 0.0f;
     case kMatRoughMetal: {
         if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!c.two_sided) return 0.0f; }
-        if (!same_hemisphere(wo, wi)) return 0;
-        f3 wh = normalize(wo + wi);
-        return mf_pdf(c.mf, wo, wh) / (4 * dot(wo, wh));
+        return mf_reflect_pdf(c.mf, wo, wi);
     }
     case kMatRoughGlass: {
         bool is_reflect = same_hemisphere(wo, wi);
@@ -454,6 +708,10 @@ __device__ __forceinline__ float mat_pdf(const MatCtx& c, f3 wo, f3 wi) {
         }
         return isinf(pdf) || isnan(pdf) ? 0.0f : pdf;
     }
+    case kMatRoughPlastic:
+        return plastic_pdf(c, wo, wi);
+    case kMatDisney:
+        return disney_pdf(c.dp, wo, wi);
     default:
         return 0.0f;
     }
@@ -465,21 +723,119 @@ struct BSample {
     int type;
 };
 
-// Material.Sample (Diffuse.cs:39-49 + LambertianReflectionBrdf.cs:26-36,
-// RoughMetal.cs:87-112 + MicrofacetReflectionBrdf.cs:50-59, RoughGlass.cs:240-297)
+// Cosine-hemisphere direction with LambertianReflectionBrdf.Sample's z flip
+// (LambertianReflectionBrdf.cs:26-36, DisneyBsdf.DiffuseSample).
+__device__ __forceinline__ bool cosine_dir(f3 wo, Rng& rng, f3* out) {
+    float ux = rng.next_float(), uy = rng.next_float();
+    float radius = sqrtf(ux);
+    float angle = uy * kPi * 2;
+    float x = radius * cosf(angle), y = radius * sinf(angle);
+    f3 wi = normalize(mk(x, y, sqrtf(1 - x * x - y * y)));
+    if (wo.z < 0) wi.z *= -1;
+    if (!same_hemisphere(wo, wi)) return false;
+    *out = wi;
+    return true;
+}
+// Direction part of MicrofacetReflectionBrdf.Sample (MicrofacetReflectionBrdf.cs:50-59).
+__device__ __forceinline__ bool mf_reflect_dir(const Mf& m, f3 wo, Rng& rng, f3* wi_out, f3* wh_out) {
+    if (cos_theta(wo) == 0) return false;
+    f3 wh = mf_sample_wh(m, wo, rng);
+    if (dot(wo, wh) < 0) return false;
+    f3 wi = reflect(-wo, wh);
+    if (!same_hemisphere(wo, wi)) return false;
+    *wi_out = wi;
+    *wh_out = wh;
+    return true;
+}
+
+__device__ __forceinline__ BSample disney_sample(const DisneyP& b, f3 wo, Rng& rng) {           // DisneyBsdf.Sample
+    BSample s;
+    s.wi = mk(0.0f); s.fr = mk(0.0f); s.pdf = 0.0f; s.type = 0;
+    Mf td = gtr2_dist(b, b.trans_rough);
+    f3 wi, wh;
+    int type = 0;
+    bool ok = false;
+    // MicrofacetTransmissionBtdf.Sample (MicrofacetTransmissionBtdf.cs:59-75)
+    auto btdf = [&](f3* out) -> bool {
+        if (wo.z == 0) return false;
+        f3 h = mf_sample_wh(td, wo, rng);
+        if (dot(wo, h) < 0) return false;
+        bool entering = cos_theta(wo) > 0;
+        float ei = entering ? 1.0f : b.eta, et = entering ? b.eta : 1.0f;
+        f3 w;
+        if (!refract(wo, h, ei / et, &w)) return false;
+        f3 f = disney_btdf_fr(b, wo, w);
+        float p = disney_btdf_pdf(b, wo, w);
+        if (!valid_color(f) || isinf(p) || p == 0) return false;
+        *out = w;
+        return true;
+    };
+    if (cos_theta(wo) < 0) {
+        if (b.transmission == 0) return s;
+        float fr = nclamp(fr_dielectric(cos_theta(wo), 1, b.eta), 0.1f, 0.9f);
+        if (rng.next_float() >= fr) {
+            ok = btdf(&wi);
+            type = kGlossy | kTransmission;
+        } else {
+            ok = mf_reflect_dir(td, wo, rng, &wi, &wh) && (mf_pdf(td, wo, wh) / (4 * dot(wi, wh)) != 0);
+            type = kReflection | kGlossy;
+        }
+    } else {
+        float dw, tw, sw, cw;
+        disney_weights(b, &dw, &tw, &sw, &cw);
+        float u = rng.next_float();
+        if (u < dw) {
+            // scatting_distance > 0 (the BSSRDF branch) is rejected at scene creation
+            if (b.thin || b.sdist_black) {
+                ok = cosine_dir(wo, rng, &wi) && ((same_hemisphere(wo, wi) ? abs_cos_theta(wi) / kPi : 0.0f) != 0);
+                type = kReflection | kDiffuse;
+            }
+        } else {
+            u -= dw;
+            if (u < tw) {
+                ok = btdf(&wi);
+                type = kGlossy | kTransmission;
+            } else {
+                u -= tw;
+                if (u < sw) {
+                    Mf sd = gtr2_dist(b, b.roughness);
+                    ok = mf_reflect_dir(sd, wo, rng, &wi, &wh) && (mf_pdf(sd, wo, wh) / (4 * dot(wi, wh)) != 0);
+                    type = kReflection | kGlossy;
+                } else if (cos_theta(wo) != 0) {
+                    // DisneyClearcoatBrdf.Sample
+                    float gloss = lerpf(b.cc_gloss, 0.1f, 0.001f);
+                    float ux = rng.next_float(), uy = rng.next_float();
+                    float a2 = sqr(gloss);
+                    float ct = sqrtf(nmax(0, (1 - powf(a2, 1 - ux)) / (1 - a2)));
+                    float st = sqrtf(nmax(0, 1 - ct * ct));
+                    float phi = 2 * kPi * uy;
+                    wh = spherical_dir(st, ct, phi);
+                    if (!same_hemisphere(wo, wh)) wh = -wh;
+                    wi = reflect(-wo, wh);
+                    ok = same_hemisphere(wo, wi) && clearcoat_pdf(b, wo, wi) != 0;
+                    type = kReflection | kGlossy;
+                }
+            }
+        }
+    }
+    if (!ok) return s;
+    s.wi = wi;
+    s.fr = disney_fr(b, wo, wi);
+    s.pdf = disney_pdf(b, wo, wi);
+    s.type = type;
+    return s;
+}
+
+// Material.Sample (Diffuse.cs:39-49, RoughMetal.cs:87-112, RoughGlass.cs:240-297,
+// RoughPlastic.cs:112-166, FresnelSpecularBsdf.cs:34-65, PerfectMirror.cs:29-39, Disney.cs:97-100)
 __device__ __forceinline__ BSample mat_sample(const MatCtx& c, f3 wo, Rng& rng) {
     BSample s;
     s.wi = mk(0.0f); s.fr = mk(0.0f); s.pdf = 0.0f; s.type = 0;
     switch (c.type) {
     case kMatDiffuse: {
         if (cos_theta(wo) <= 0) { if (!c.two_sided) return s; }
-        float ux = rng.next_float(), uy = rng.next_float();
-        float radius = sqrtf(ux);
-        float angle = uy * kPi * 2;
-        float x = radius * cosf(angle), y = radius * sinf(angle);
-        f3 wi = normalize(mk(x, y, sqrtf(1 - x * x - y * y)));
-        if (wo.z < 0) wi.z *= -1;
-        if (!same_hemisphere(wo, wi)) return s;
+        f3 wi;
+        if (!cosine_dir(wo, rng, &wi)) return s;
         s.wi = wi;
         s.pdf = same_hemisphere(wo, wi) ? abs_cos_theta(wi) / kPi : 0.0f;
         s.fr = c.r / kPi;
@@ -488,11 +844,8 @@ __device__ __forceinline__ BSample mat_sample(const MatCtx& c, f3 wo, Rng& rng) 
     }
     case kMatRoughMetal: {
         if (cos_theta(wo) <= 0) { if (!c.two_sided) return s; }
-        if (cos_theta(wo) == 0) return s;
-        f3 wh = mf_sample_wh(c.mf, wo, rng);
-        if (dot(wo, wh) < 0) return s;
-        f3 wi = reflect(-wo, wh);
-        if (!same_hemisphere(wo, wi)) return s;
+        f3 wi, wh;
+        if (!mf_reflect_dir(c.mf, wo, rng, &wi, &wh)) return s;
         s.pdf = mf_pdf(c.mf, wo, wh) / (4 * dot(wi, wh));
         s.fr = conductor_fr(c, wo, wi);
         s.wi = wi;
@@ -538,6 +891,51 @@ __device__ __forceinline__ BSample mat_sample(const MatCtx& c, f3 wo, Rng& rng) 
         s.wi = wi; s.fr = btdf; s.pdf = pdf; s.type = kTransmission | kGlossy;
         return s;
     }
+    case kMatRoughPlastic: {
+        if (cos_theta(wo) <= 0) { if (!c.two_sided) return s; }
+        float p = c.ks / (c.kd + c.ks);
+        f3 gwi = mk(0.0f), gwh, dwi = mk(0.0f);
+        int gty = 0, dty = 0;
+        if (mf_reflect_dir(c.mf, wo, rng, &gwi, &gwh)) gty = kReflection | kGlossy;
+        else gwi = mk(0.0f);
+        if (cosine_dir(wo, rng, &dwi)) dty = kReflection | kDiffuse;
+        else dwi = mk(0.0f);
+        bool glossy = rng.next_float() < p;
+        f3 wi = glossy ? gwi : dwi;
+        s.wi = wi;
+        s.type = glossy ? gty : dty;
+        s.pdf = plastic_pdf(c, wo, wi);
+        s.fr = plastic_fr(c, wo, wi);
+        return s;
+    }
+    case kMatPerfectGlass: {
+        float f = fr_dielectric(cos_theta(wo), c.eta_a, c.eta_b);
+        if (rng.next_float() < f) {
+            if (cos_theta(wo) == 0) return s;
+            f3 wi = mk(-wo.x, -wo.y, wo.z);
+            s.wi = wi; s.fr = f * c.r / abs_cos_theta(wi); s.pdf = f; s.type = kReflection | kSpecular;
+            return s;
+        }
+        bool entering = cos_theta(wo) > 0;
+        float ei = entering ? c.eta_a : c.eta_b, et = entering ? c.eta_b : c.eta_a;
+        f3 n = entering ? mk(0, 0, 1) : mk(0, 0, -1);
+        f3 wi;
+        if (!refract(wo, n, ei / et, &wi) || cos_theta(wi) == 0.0f) return s;
+        f3 ft = c.t * (1.0f - f);
+        ft = ft * ((ei * ei) / (et * et));
+        ft = ft / abs_cos_theta(wi);
+        s.wi = wi; s.fr = ft; s.pdf = 1 - f; s.type = kTransmission | kSpecular;
+        return s;
+    }
+    case kMatPerfectMirror: {
+        if (cos_theta(wo) <= 0) { if (!c.two_sided) return s; }
+        if (abs_cos_theta(wo) == 0) return s;
+        f3 wi = mk(-wo.x, -wo.y, wo.z);
+        s.wi = wi; s.fr = c.r / abs_cos_theta(wi); s.pdf = 1.0f; s.type = kReflection | kSpecular;
+        return s;
+    }
+    case kMatDisney:
+        return disney_sample(c.dp, wo, rng);
     default:
         return s;
     }

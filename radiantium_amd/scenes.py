@@ -88,6 +88,60 @@ def cornell_box(width=256, height=256, max_depth=8, method="Mis"):
     return s
 
 
+def checker_texture(n=64, c0=(0.8, 0.8, 0.8), c1=(0.2, 0.3, 0.7), cells=8):
+    """n x n checkerboard as a 3-channel column-major image (x*H + y)."""
+    x = np.arange(n)[:, None]
+    y = np.arange(n)[None, :]
+    mask = ((x * cells // n) + (y * cells // n)) % 2 == 0
+    img = np.where(mask[..., None], np.array(c0, np.float32), np.array(c1, np.float32))
+    return img.astype(np.float32)
+
+
+def material_box(width=64, height=64, max_depth=8, method="Mis"):
+    """Cornell box exercising every material of the registry and the image
+    texture sampler: a textured rough_plastic left wall (Linear filter, Repeat
+    wrap, UVs tiled twice), GGX plastic floor, diffuse green right wall,
+    a Disney short box (sheen + clearcoat), a perfect_mirror tall box, a
+    perfect_glass sphere, an anisotropic rough_metal sphere and a rough_glass
+    sphere, lit by the ceiling quad light and a small emissive sphere."""
+    from .scene import Texture
+    s = SceneDesc()
+    grey = s.add_material("diffuse", kd=(0.73, 0.73, 0.73))
+    green = s.add_material("diffuse", kd=(0.12, 0.45, 0.15))
+    tex = Texture(image=checker_texture(), wrap="Repeat", filter="Linear")
+    plastic_tex = s.add_material("rough_plastic", r=tex, roughness=0.35, dist="Beckmann", kd=0.6, ks=0.4)
+    plastic_floor = s.add_material("rough_plastic", r=(0.6, 0.5, 0.4), roughness=0.25, dist="GGX")
+    disney = s.add_material("disney", base_color=(0.8, 0.3, 0.2), roughness=0.45, sheen=0.6, sheen_tint=0.5,
+                            clearcoat=0.7, clearcoat_gloss=0.6, specular_tint=0.3, metallic=0.1)
+    mirror = s.add_material("perfect_mirror", r=(0.9, 0.9, 0.9))
+    glass = s.add_material("perfect_glass")
+    metal = s.add_material("rough_metal", roughness=0.4, anisotropic=0.5)
+    rglass = s.add_material("rough_glass", roughness=0.2)
+    light_mat = s.add_material("diffuse", kd=(0.0, 0.0, 0.0))
+    L, H = -0.5, 0.5
+    uv = np.array([[0, 0], [2, 0], [2, 2], [0, 2]], np.float32)
+    s.add_mesh(*_quad((L, 0, L), (L, 0, H), (H, 0, H), (H, 0, L)), material=plastic_floor)
+    s.add_mesh(*_quad((L, 1, L), (H, 1, L), (H, 1, H), (L, 1, H)), material=grey)
+    s.add_mesh(*_quad((L, 0, L), (H, 0, L), (H, 1, L), (L, 1, L)), material=grey)
+    pos, idx = _quad((L, 0, L), (L, 1, L), (L, 1, H), (L, 0, H))
+    s.add_mesh(pos, idx, material=plastic_tex, uvs=uv)
+    s.add_mesh(*_quad((H, 0, L), (H, 0, H), (H, 1, H), (H, 1, L)), material=green)
+    s.add_mesh(*_box((0.3, 0.3, 0.3), (0.17, 0.0, 0.12), 17.0), material=disney)
+    s.add_mesh(*_box((0.3, 0.6, 0.3), (-0.17, 0.0, -0.15), -17.0), material=mirror)
+    s.add_sphere((0.18, 0.45, 0.15), 0.12, material=glass)
+    s.add_sphere((-0.28, 0.1, 0.25), 0.1, material=metal)
+    s.add_sphere((0.3, 0.08, -0.3), 0.08, material=rglass)
+    s.add_sphere((-0.3, 0.85, 0.3), 0.04, material=light_mat, le=(6.0, 6.0, 6.0))
+    q = 0.125
+    y = 1.0 - 1e-3
+    s.add_mesh(*_quad((-q, y, -q), (q, y, -q), (q, y, q), (-q, y, q)), material=light_mat, le=(17.0, 12.0, 4.0))
+    s.set_camera(fov=40.0, width=width, height=height, origin=(0.0, 0.5, 1.85), target=(0.0, 0.5, 0.0),
+                 up=(0.0, 1.0, 0.0))
+    s.set_integrator(type="path", max_depth=max_depth, method=method)
+    s.bvh_max_prim = 1
+    return s
+
+
 # ---------------------------------------------------------------- icosphere
 def icosphere(level):
     """Unit icosphere: 20*4^level faces, midpoint subdivision with shared

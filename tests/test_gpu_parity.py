@@ -114,6 +114,22 @@ def test_render_twin_cornell(ctx, oracle, method):
     assert rel_mse(g, o) < 1e-3
 
 
+@pytest.mark.parametrize("method", ["Mis", "Nee", "Bxdf"])
+def test_render_twin_material_box(ctx, oracle, method):
+    """Every material (plastic with a Linear/Repeat image texture, Disney,
+    mirror, perfect and rough glass, anisotropic metal) and sphere emitters."""
+    desc = scenes.material_box(48, 48, method=method)
+    print(f"[render twin] material_box 48x48 spp=8 method={method}", flush=True)
+    g, g2, gst, o, o2, ost = _render_pair(ctx, oracle, desc, 8, seed=13)
+    assert np.isfinite(g).all()
+    for k in ("rays_closest", "rays_any", "vertices"):
+        assert abs(gst[k] - ost[k]) <= max(16, 0.02 * ost[k]), (k, gst[k], ost[k])
+    close = np.isclose(g, o, rtol=1e-3, atol=1e-5)
+    assert close.mean() > 0.95, close.mean()
+    z = z_test(g, g2, 8, o, o2, 8)
+    assert np.mean(np.abs(z) < 4) > 0.99
+
+
 def test_render_twin_icosphere(ctx, oracle):
     """GGX conductor + rough dielectric (the 1M-triangle workload's materials) at level 5."""
     desc = scenes.displaced_icosphere_scene(level=5, width=96, height=54)
