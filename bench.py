@@ -77,6 +77,23 @@ def cpu_baseline(desc, args, gpu_rate):
             "sample": f"{what} at {args.spp} spp, seed 1: {rays} BVH queries in {secs:.1f} s"}
 
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "traffic_icosphere_l8_1080p_64spp.json")
+
+
+def pmc_traffic(stage, args):
+    """HBM bytes per launch of `stage` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs (scaled by a plain copy kernel's counters), when this run is
+    the profiled workload; None otherwise."""
+    if (args.width, args.height, args.spp, args.level, args.max_depth) != (1920, 1080, 64, 8, 12):
+        return None
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)["stages"][stage]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def copy_calibration():
     """A plain device copy of known size, profiled beside the path kernels so
     FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
@@ -108,7 +125,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: the path renders one frame per GPU and no multi-GPU "
-              "figure is produced (see DESIGN.md for how it would shard).", file=sys.stderr)
+              "figure is produced (the frame would shard over samples, with one all-reduce of the "
+              "sum and sum2 buffers).", file=sys.stderr)
         sys.exit(2)
 
     from radiantium_amd import native
@@ -162,7 +180,8 @@ def main():
         per_launch = total_bytes / cnt
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
         roofline = {"kernel": trace_dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": pmc_traffic(trace_dom, args),
                     "bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg_ms, 4),
                     "nodes_per_ray": round(agg["class_nodes"][c] / max(agg["class_rays"][c], 1), 2),
                     "prims_per_ray": round(agg["class_prims"][c] / max(agg["class_rays"][c], 1), 2)}

@@ -3,7 +3,7 @@
  *
  * This is the drop-in boundary below the reference's plugin surface.  The
  * reference (C#, ksgfk/Radiantium) reaches its hot path through
- *   MonteCarloIntegrator.Li(Ray3F, Scene, Random)      Radiantium.Offline/IIntegrator.cs:24-34
+ *   MonteCarloIntegrator.Li(Ray3F, Scene, Random)      Radiantium.Offline/IIntegrator.cs:13-24
  *   Renderer.{RenderTarget,Start,Wait,Stop,IsSuccess}  Radiantium.Offline/Renderer.cs:5-52
  * and calls Li once per pixel sample from BlockBasedRenderer.RenderTask
  * (Radiantium.Offline/Renderers/BlockBasedRenderer.cs:107-179).  A GPU

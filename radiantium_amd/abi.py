@@ -5,6 +5,7 @@ checks sizes and offsets against the compiled library.
 """
 import ctypes as C
 
+ABI_VERSION = 1   # RD_ABI_VERSION
 RD_OK = 0
 RD_ERR_INVALID = 1
 RD_ERR_DEVICE = 2

@@ -89,6 +89,43 @@ def test_trace_parity_full_icosphere(ctx, oracle):
     assert gc.tolist() == oc.tolist() and gac.tolist() == oac.tolist()
 
 
+def _fixture_cases():
+    """The hand-built traversal cases of the oracle's own tests: a shared
+    quad edge, a parallel ray, t == MinT / MaxT boundaries, the NaN slab of a
+    grazing ray, sphere roots, and a tree deeper than 24 levels."""
+    from tests.test_oracle_traversal import _rays, _scene
+    one_up = np.nextafter(np.float32(1), np.float32(2))
+    tri = [[(0, 0, 0), (1, 0, 0), (0, 1, 0)]]
+    quad = [[(0, 0, 0), (1, 0, 0), (1, 1, 0)], [(0, 0, 0), (1, 1, 0), (0, 1, 0)]]
+    bnd = np.concatenate([_rays([(0.25, 0.25, 1.0)], [(0, 0, -1)], tmin, tmax)
+                          for tmin, tmax in [(1.0, 10.0), (one_up, 10.0), (0.0, one_up), (0.0, 1.0)]])
+    xs = [0.8 ** i for i in range(200)]
+    deep = [[(x, 0, 0), (x * 1.5, 0, 0.01), (x, 0.01, 0)] for x in xs]
+    return [
+        ("shared_edge", _scene(quad, max_prim=4), _rays([(0.5, 0.5, 1.0)], [(0, 0, -1)])),
+        ("shared_edge_mp1", _scene(quad, max_prim=1), _rays([(0.5, 0.5, 1.0)], [(0, 0, -1)])),
+        ("parallel", _scene(tri), _rays([(-1, 0.25, 0)], [(1, 0, 0)])),
+        ("boundaries", _scene(tri), bnd),
+        ("nan_slab", _scene([[(0, 0, 0), (0, 1, 0), (1, 0.5, 1)]]),
+         _rays([(0.0, 0.5, -1.0), (1e-3, 0.5, -1.0)], [(0, 0, 1), (0, 0, 1)])),
+        ("sphere", _scene(spheres=[((0, 0, 0), 0.5)]), _rays([(0, 0, -2), (0, 0, 0)], [(0, 0, 1), (0, 0, 1)])),
+        ("deep", _scene(deep), _rays([(1.1 * x, 0.002, -1) for x in xs], [(0, 0, 1)] * len(xs))),
+    ]
+
+
+def test_trace_fixtures(ctx, oracle):
+    for name, desc, rays in _fixture_cases():
+        print(f"[trace fixture] {name} rays={rays.shape[0]}", flush=True)
+        gs = Scene(ctx, desc)
+        os_ = oracle.OracleScene(desc)
+        g, o = gs.trace_closest(rays), os_.trace_closest(rays)
+        assert np.array_equal(g["prim"], o["prim"]), name
+        hit = o["prim"] >= 0
+        for k in ("t", "u", "v"):
+            assert np.array_equal(g[k][hit].view(np.uint32), o[k][hit].view(np.uint32)), (name, k)
+        assert np.array_equal(gs.trace_any(rays), os_.trace_any(rays)), name
+
+
 def _render_pair(ctx, oracle, desc, spp, seed):
     gs = Scene(ctx, desc)
     os_ = oracle.OracleScene(desc)

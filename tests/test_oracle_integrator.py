@@ -1,0 +1,130 @@
+"""Pins the oracle's path integrator (Integrators/PathTracer.cs:43-411) with
+answers that the rendering equation fixes in closed form, for each of the
+three estimators (OnlySampleBxdf, NextEventEstimation, Mis).  CPU only."""
+import numpy as np
+import pytest
+
+from radiantium_amd import scenes
+from radiantium_amd.scene import SceneDesc
+from radiantium_amd.scenes import _quad
+
+
+def _furnace(method, rho, le, level=2, res=8):
+    """Camera at the centre of a closed icosphere whose triangles all face
+    inward, are diffuse with albedo rho and emit le: every ray sees
+    L = le + rho * L, so L = le / (1 - rho) exactly (unbiased with Russian
+    roulette and unlimited depth)."""
+    v, f = scenes.icosphere(level)
+    n = np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]])
+    outward = np.einsum("ij,ij->i", n, v[f].mean(axis=1)) > 0
+    f = np.where(outward[:, None], f[:, ::-1], f)     # inward winding
+    s = SceneDesc()
+    m = s.add_material("diffuse", kd=(rho, rho, rho))
+    s.add_mesh(v.astype(np.float32), f, material=m, le=(le, le, le))
+    s.set_camera(fov=60.0, width=res, height=res, origin=(0, 0, 0), target=(0, 0, 1), up=(0, 1, 0))
+    s.set_integrator(type="path", max_depth=-1, method=method)
+    s.bvh_max_prim = 1
+    return s
+
+
+@pytest.mark.parametrize("method", ["Mis", "Nee", "Bxdf"])
+def test_furnace_closed_diffuse_sphere(oracle, method):
+    rho, le, spp = 0.5, 1.0, 64
+    o = oracle.OracleScene(_furnace(method, rho, le))
+    rgb, s2, st, _ = o.render(spp, seed=3, threads=0)
+    n = rgb.size // 3 * spp
+    mean = float(rgb.mean())
+    var = float(s2.sum() / 3 / n - mean ** 2)
+    se = (max(var, 1e-12) / n) ** 0.5
+    expect = le / (1 - rho)
+    assert abs(mean - expect) < 4 * se + 1e-4, (mean, expect, se)
+    assert st["invalid"] == 0
+
+
+def _direct_scene(method, res=16, rho=0.5, le=4.0, half=0.25, h=1.0):
+    """A large diffuse ground at y=0 lit by one downward-facing square light
+    of side 2*half at height h (its own material black), seen by a camera
+    that cannot see the light.  Only direct light reaches the camera."""
+    s = SceneDesc()
+    g = s.add_material("diffuse", kd=(rho, rho, rho))
+    black = s.add_material("diffuse", kd=(0.0, 0.0, 0.0))
+    W = 5.0
+    s.add_mesh(*_quad((-W, 0, -W), (-W, 0, W), (W, 0, W), (W, 0, -W)), material=g)
+    s.add_mesh(*_quad((-half, h, -half), (half, h, -half), (half, h, half), (-half, h, half)),
+               material=black, le=(le, le, le))
+    s.set_camera(fov=30.0, width=res, height=res, origin=(0, 0.6, 1.6), target=(0, 0, 0), up=(0, 1, 0))
+    s.set_integrator(type="path", max_depth=8, method=method)
+    s.bvh_max_prim = 1
+    return s
+
+
+def _irradiance(p, half, h, le, n=160):
+    """E(p) = le * sum over light cells of cos_p cos_l / r^2 dA (midpoint rule)."""
+    g = (np.arange(n) + 0.5) / n * 2 * half - half
+    lx, lz = np.meshgrid(g, g, indexing="ij")
+    dA = (2 * half / n) ** 2
+    out = np.zeros(p.shape[0])
+    for i, q in enumerate(p):
+        dx, dz = lx - q[0], lz - q[2]
+        r2 = dx * dx + h * h + dz * dz
+        out[i] = le * np.sum(h * h / (r2 * r2)) * dA
+    return out
+
+
+@pytest.mark.parametrize("method", ["Mis", "Nee", "Bxdf"])
+def test_direct_light_from_quad(oracle, method):
+    res, rho, le, half, h = 16, 0.5, 4.0, 0.25, 1.0
+    desc = _direct_scene(method, res, rho, le, half, h)
+    o = oracle.OracleScene(desc)
+    spp = 64 if method != "Bxdf" else 512
+    rgb, s2, _, _ = o.render(spp, seed=5, threads=0)
+    img = rgb.reshape(res, res, 3)[:, :, 0]          # [x][y], channel r
+    sq = s2.reshape(res, res, 3)[:, :, 0]
+    # expected pixel value: mean of rho/pi * E over a 4x4 grid of sub-pixel positions
+    sub = (np.arange(4) + 0.5) / 4
+    expect = np.zeros((res, res))
+    for sx in sub:
+        for sy in sub:
+            xy = np.array([(x + sx, y + sy) for x in range(res) for y in range(res)], np.float32)
+            rays = o.camera_rays(xy)
+            d = rays["d"].astype(np.float64)
+            t = -rays["o"][:, 1].astype(np.float64) / d[:, 1]
+            p = rays["o"].astype(np.float64) + t[:, None] * d
+            expect += (rho / np.pi * _irradiance(p, half, h, le)).reshape(res, res) / 16
+    var = np.maximum(sq / spp - img.astype(np.float64) ** 2, 0) / spp
+    # whole-image mean: tight; 4x4 pixel blocks: a 4-sigma z-test.  Blocks
+    # pool enough samples that the sample variance is trustworthy even for
+    # BSDF-only sampling, where a pixel sees the light in a few % of samples.
+    # The sub-pixel quadrature error is far below the Monte Carlo error.
+    se_all = np.sqrt(var.sum()) / img.size
+    assert abs(img.mean() - expect.mean()) < 4 * se_all + 2e-4 * expect.mean(), (img.mean(), expect.mean())
+    blk = lambda a: a.reshape(res // 4, 4, res // 4, 4).sum(axis=(1, 3))
+    z = (blk(img) - blk(expect)) / np.sqrt(blk(var) + 1e-12)
+    assert np.all(np.abs(z) < 4), z
+
+
+def test_estimators_agree_on_cornell(oracle):
+    """OnlySampleBxdf, NextEventEstimation and Mis estimate the same image
+    (the reference's Veach variants): a z-test between each pair on 4x4
+    pixel blocks (BSDF-only sampling sees the small light rarely, so single
+    pixels have heavy-tailed sample variances)."""
+    imgs = {}
+    for method in ("Mis", "Nee", "Bxdf"):
+        o = oracle.OracleScene(scenes.cornell_box(24, 24, method=method))
+        spp = 256 if method != "Bxdf" else 1024
+        rgb, s2, _, _ = o.render(spp, seed=17, threads=0)
+        imgs[method] = (rgb.astype(np.float64), s2.astype(np.float64), spp)
+
+    def blk(a):
+        return a.reshape(6, 4, 6, 4, 3).sum(axis=(1, 3))
+
+    def z(a, b):
+        (ma, sa, na), (mb, sb, nb) = imgs[a], imgs[b]
+        va = np.maximum(sa / na - ma ** 2, 0) / na
+        vb = np.maximum(sb / nb - mb ** 2, 0) / nb
+        return (blk(ma) - blk(mb)) / np.sqrt(blk(va) + blk(vb) + 1e-8)
+
+    for a, b in (("Mis", "Nee"), ("Mis", "Bxdf"), ("Nee", "Bxdf")):
+        zz = z(a, b)
+        assert np.all(np.abs(zz) < 4.5), (a, b, np.abs(zz).max())
+        assert abs(np.mean(zz)) < 0.5, (a, b, np.mean(zz))
