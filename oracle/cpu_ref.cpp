@@ -223,6 +223,78 @@ struct Node {
     uint8_t axis;
 };
 
+// Distribution1D (Radiantium.Offline/Distribution1D.cs:19-111): pbrt's
+// piecewise-constant 1D distribution with the reference's float operation order.
+struct Dist1D {
+    std::vector<float> func, cdf;
+    float integ = 0;
+    void build(const float* f, int n) {
+        func.assign(f, f + n);
+        cdf.assign(n + 1, 0.0f);
+        for (int i = 1; i < n + 1; ++i) cdf[i] = cdf[i - 1] + func[i - 1] / n;
+        integ = cdf[n];
+        if (integ == 0) {
+            for (int i = 1; i < n + 1; i++) cdf[i] = i / (float)n;
+        } else {
+            for (int i = 1; i < n + 1; i++) cdf[i] /= integ;
+        }
+    }
+    int count() const { return (int)func.size(); }
+    int find_interval(float u) const {                     // :52-71
+        int first = 0, len = (int)cdf.size();
+        while (len > 0) {
+            int half = len >> 1, middle = first + half;
+            if (cdf[middle] <= u) { first = middle + 1; len -= half + 1; }
+            else len = half;
+        }
+        return iclamp(first - 1, 0, (int)cdf.size() - 2);
+    }
+    float continuous_pdf(int off) const { return integ > 0 ? func[off] / integ : 0; }   // :91-94
+    float discrete_pdf(int i) const { return integ > 0 ? func[i] / (integ * count()) : 0; }  // :108-111
+    float sample_continuous(float u, float* pdf, int* off) const {                     // :73-84
+        *off = find_interval(u);
+        float du = u - cdf[*off];
+        if ((cdf[*off + 1] - cdf[*off]) > 0) du /= cdf[*off + 1] - cdf[*off];
+        *pdf = continuous_pdf(*off);
+        return (*off + du) / count();
+    }
+};
+
+// Distribution2D (Radiantium.Offline/Distribution2D.cs:14-60).
+struct Dist2D {
+    std::vector<Dist1D> cond;
+    Dist1D marg;
+    void build(const float* data, int nu, int nv) {
+        cond.resize(nv);
+        std::vector<float> m(nv);
+        for (int v = 0; v < nv; v++) { cond[v].build(data + (size_t)v * nu, nu); m[v] = cond[v].integ; }
+        marg.build(m.data(), nv);
+    }
+    // SampleContinuous(u, out pdf, out (vj, vi)) (:33-40)
+    V2 sample_continuous(float u0, float u1, float* pdf, int* vj, int* vi) const {
+        float pdf1, pdf0;
+        float d1 = marg.sample_continuous(u1, &pdf1, vj);
+        float d0 = cond[*vj].sample_continuous(u0, &pdf0, vi);
+        *pdf = pdf0 * pdf1;
+        return V2(d0, d1);
+    }
+    float continuous_pdf(float u, float v) const {                                     // :55-60
+        int iu = iclamp((int)(u * cond[0].count()), 0, cond[0].count() - 1);
+        int iv = iclamp((int)(v * marg.count()), 0, marg.count() - 1);
+        return cond[iv].func[iu] / marg.integ;
+    }
+};
+
+// InfiniteAreaLight (Lights/InfiniteAreaLight.cs:21-119) after
+// InfiniteLight.Preprocess (Light.cs:181-187).
+struct EnvLight {
+    int tex = -1, w = 1, h = 1;
+    M4 m2w, w2m;
+    Dist2D dist;
+    V3 center;
+    float radius = 0;
+};
+
 struct cr_scene {
     std::vector<V3> pos, nrm;
     std::vector<V2> uvs;
@@ -235,7 +307,9 @@ struct cr_scene {
     std::vector<rd_material_desc> mats;
     std::vector<rd_light_desc> lights;
     std::vector<rd_medium_desc> media;
-    std::vector<int> infinite;
+    std::vector<int> infinite;          // light index of each InfiniteLight, Scene.InfiniteLights order
+    std::vector<int> env_of_light;      // light index -> EnvLight index (-1 for area lights)
+    std::vector<EnvLight> env;
     int global_medium = -1;
     int max_prim = 1;
     std::vector<Node> nodes;
@@ -1414,8 +1488,78 @@ static void shape_sample(const cr_scene& s, int prim, Rng& rng, V3* P, V3* N, fl
     }
 }
 
+// ---- InfiniteAreaLight (Lights/InfiniteAreaLight.cs)
+// Le (:56-69): the world direction in model space, then equirectangular
+// (phi = atan2(x, -z), theta = acos y) through the map's sampler.
+static C3 env_le(const cr_scene& s, const EnvLight& e, V3 d) {
+    V3 dir = normalize(xform_vector(d, e.w2m));
+    float phi = std::atan2(dir.x, -dir.z);
+    phi = (phi < 0) ? (phi + 2 * PI_F) : phi;
+    float theta = std::acos(nclamp(dir.y, -1.0f, 1.0f));
+    float u = phi / (2 * PI_F);
+    float v = theta / PI_F;
+    return s.tex[e.tex].sample(V2(u, v));
+}
+// PdfDir (:71-85)
+static float env_pdf_dir(const EnvLight& e, V3 wi) {
+    V3 dir = normalize(xform_vector(wi, e.w2m));
+    float phi = std::atan2(dir.x, -dir.z);
+    phi = (phi < 0) ? (phi + 2 * PI_F) : phi;
+    float theta = std::acos(nclamp(dir.y, -1.0f, 1.0f));
+    float sin_theta = std::sin(theta);
+    if (sin_theta == 0.0f) return 0.0f;
+    float u = phi / (2 * PI_F);
+    float v = theta / PI_F;
+    return e.dist.continuous_pdf(u, v) / (2 * PI_F * PI_F * sin_theta);
+}
+// SampleLightDir (:92-109): the direction is that of the sampled cell's
+// corner (offset / size), the pdf the continuous one of the cell.
+static bool env_sample_dir(const EnvLight& e, float u0, float u1, V3* wdir, float* pdf_out, V2* uv) {
+    float pdf;
+    int vj, vi;
+    e.dist.sample_continuous(u0, u1, &pdf, &vj, &vi);
+    if (pdf <= 0) return false;
+    float y = vj / (float)e.h;
+    float x = vi / (float)e.w;
+    *uv = V2(x, y);
+    float theta = y * PI_F;
+    float phi = x * 2 * PI_F;
+    float sin_theta = std::sin(theta);
+    if (sin_theta == 0) return false;
+    pdf /= (2 * PI_F * PI_F * sin_theta);
+    // MathExt.SphericalCoordinates (MathExt.cs:94-102)
+    float st = std::sin(theta), ct = std::cos(theta), sp = std::sin(phi), cp = std::cos(phi);
+    V3 sph(cp * st, sp * st, ct);
+    V3 dir(sph.y, sph.z, -sph.x);
+    *wdir = normalize(xform_vector(dir, e.m2w));
+    *pdf_out = pdf;
+    return true;
+}
+// SampleLi (:111-119): the shadow target sits 20 world radii out.
+static LSample env_sample_li(const cr_scene& s, const EnvLight& e, V3 refp, Rng& rng) {
+    LSample out;
+    float u0 = rng.next_float(), u1 = rng.next_float();     // Random.NextVec2: X then Y
+    V3 wdir;
+    float pdf;
+    V2 uv;
+    if (!env_sample_dir(e, u0, u1, &wdir, &pdf, &uv)) return out;
+    out.li = s.tex[e.tex].sample(uv);
+    out.p = wdir * e.radius * 20.0f + e.center;
+    out.wi = normalize(out.p - refp);
+    out.pdf = pdf;
+    return out;
+}
+
+// Light.SampleLi dispatch
+static LSample area_sample_li(const cr_scene& s, const rd_light_desc& L, V3 refp, Rng& rng);
+static LSample light_sample_li(const cr_scene& s, int light, V3 refp, Rng& rng) {
+    const rd_light_desc& L = s.lights[light];
+    if (L.type == RD_LIGHT_INFINITE) return env_sample_li(s, s.env[s.env_of_light[light]], refp, rng);
+    return area_sample_li(s, L, refp, rng);
+}
+
 // DiffuseAreaLight.SampleLi (Lights/DiffuseAreaLight.cs:48-68)
-static LSample light_sample_li(const cr_scene& s, const rd_light_desc& L, V3 refp, Rng& rng) {
+static LSample area_sample_li(const cr_scene& s, const rd_light_desc& L, V3 refp, Rng& rng) {
     LSample out;
     V3 P, N;
     float spdf;
@@ -1436,7 +1580,9 @@ static LSample light_sample_li(const cr_scene& s, const rd_light_desc& L, V3 ref
 }
 // DiffuseAreaLight.PdfLi (DiffuseAreaLight.cs:31-46): intersects the light's
 // own shape only.
-static float light_pdf_li(const cr_scene& s, const rd_light_desc& L, V3 refp, V3 wi) {
+static float light_pdf_li(const cr_scene& s, int light, V3 refp, V3 wi) {
+    const rd_light_desc& L = s.lights[light];
+    if (L.type == RD_LIGHT_INFINITE) return env_pdf_dir(s.env[s.env_of_light[light]], wi);   // PdfLi :87-90
     Ray ray(refp, wi);
     SurfPt sp;
     if (!prim_hit(s, L.prim, ray, &sp)) return 0.0f;
@@ -1452,9 +1598,12 @@ static C3 inct_le(const cr_scene& s, const Inct& it, V3 w) {
     const rd_light_desc& L = s.lights[l];
     return dot(it.N(), w) > 0 ? C3(L.le[0], L.le[1], L.le[2]) : C3(0.0f);
 }
-// Scene.EvalAllInfiniteLights (Scene.cs:37-45).  Infinite lights are not
-// part of this restatement yet; scenes with them are rejected at creation.
-static C3 eval_infinite(const cr_scene&, const Ray&) { return C3(0.0f); }
+// Scene.EvalAllInfiniteLights (Scene.cs:37-45)
+static C3 eval_infinite(const cr_scene& s, const Ray& r) {
+    C3 l(0.0f);
+    for (size_t i = 0; i < s.env.size(); i++) l = l + env_le(s, s.env[i], r.d);
+    return l;
+}
 
 // PathUtility.PowerHeuristic (Integrators/PathUtility.cs:5-9), in double
 static inline float power_heuristic(int nf, float fp, int ng, float gp) {
@@ -1467,11 +1616,10 @@ static inline Ray spawn_ray(const Inct& it, V3 d) { return Ray(it.P, d, 0.0001f)
 
 // PathTracer.Mis.EstimateDirect (Integrators/PathTracer.cs:309-399)
 static C3 estimate_direct(const cr_scene& s, Rng& rng, int light, const Inct& it, uint64_t* st) {
-    const rd_light_desc& L = s.lights[light];
     const rd_material_desc& m = s.mats[s.prims[it.prim].material];
     V3 wo = it.wr;
     C3 le(0.0f);
-    LSample ls = light_sample_li(s, L, it.P, rng);
+    LSample ls = light_sample_li(s, light, it.P, rng);
     if (ls.pdf > 0.0f && !ls.li.black()) {
         C3 fr = material_fr(s, m, it.sh.to_local(wo), it.sh.to_local(ls.wi), it.UV);
         float spdf = material_pdf(s, m, it.sh.to_local(wo), it.sh.to_local(ls.wi), it.UV);
@@ -1490,7 +1638,7 @@ static C3 estimate_direct(const cr_scene& s, Rng& rng, int light, const Inct& it
     if (!fr.black() && spdf > 0) {
         float w = 1;
         if (!sampled_specular) {
-            float lpdf = light_pdf_li(s, L, it.P, it.sh.to_world(bs.wi));
+            float lpdf = light_pdf_li(s, light, it.P, it.sh.to_world(bs.wi));
             if (lpdf == 0.0f) return le;
             w = power_heuristic(1, spdf, 1, lpdf);
         }
@@ -1579,7 +1727,7 @@ static C3 li_nee(const cr_scene& s, Ray ray, Rng& rng, uint64_t* st) {
         if (st) st[3]++;
         const rd_material_desc& m = s.mats[mat];
         auto estimate_light = [&](int l) -> C3 {                   // EstimateLight :117-129
-            LSample ls = light_sample_li(s, s.lights[l], it.P, rng);
+            LSample ls = light_sample_li(s, l, it.P, rng);
             if (ls.pdf <= 0.0f) return C3(0);
             if (scene_is_occluded(s, it.P, ls.p, st)) return C3(0.0f);
             C3 fr = material_fr(s, m, it.sh.to_local(wo), it.sh.to_local(ls.wi), it.UV);
@@ -1730,14 +1878,50 @@ cr_scene* cr_scene_create(const rd_scene_desc* d) {
             if (!black && !s->mats[i].thin) { set_err("Disney scatting_distance > 0 (BSSRDF) is not on this path"); delete s; return nullptr; }
         }
     }
+    s->env_of_light.assign(d->num_lights, -1);
     for (int i = 0; i < d->num_lights; i++) {
-        if (s->lights[i].type != RD_LIGHT_AREA) { set_err("infinite lights not in this restatement yet"); delete s; return nullptr; }
-        int p = s->lights[i].prim;
+        const rd_light_desc& L = s->lights[i];
+        if (L.type == RD_LIGHT_INFINITE) {
+            if (L.tex < 0 || L.tex >= d->num_textures) { set_err("infinite light texture index"); delete s; return nullptr; }
+            EnvLight e;
+            e.tex = L.tex;
+            for (int k = 0; k < 16; k++) e.m2w.m[k / 4][k % 4] = L.to_world[k];
+            if (!m4_invert(e.m2w, &e.w2m)) { set_err("infinite light: invalid matrix"); delete s; return nullptr; }
+            s->env_of_light[i] = (int)s->env.size();
+            s->infinite.push_back(i);
+            s->env.push_back(e);
+            continue;
+        }
+        if (L.type != RD_LIGHT_AREA) { set_err("unknown light type"); delete s; return nullptr; }
+        int p = L.prim;
         if (p < 0 || p >= d->num_prims) { set_err("light prim index"); delete s; return nullptr; }
     }
     if (d->integrator.type != RD_INTEGRATOR_PATH) { set_err("vol_path not in this restatement yet"); delete s; return nullptr; }
     if (!s->cam.init(d->camera)) { set_err("camera matrix not invertible"); delete s; return nullptr; }
     build_bvh(*s);
+    for (EnvLight& e : s->env) {
+        // InfiniteAreaLight ctor (InfiniteAreaLight.cs:31-53): luminance * sin(theta)
+        // of the map sampled at texel corners (x/W, y/H), rows of the 2D distribution
+        const Tex& t = s->tex[e.tex];
+        e.w = t.w > 0 ? t.w : 1;     // ConstColorTexture2D is 1x1
+        e.h = t.w > 0 ? t.h : 1;
+        std::vector<float> lum((size_t)e.w * e.h);
+        for (int y = 0; y < e.h; y++) {
+            float sin_theta = std::sin((y + 0.5f) / e.h * PI_F);
+            float v = y / (float)e.h;
+            for (int x = 0; x < e.w; x++) {
+                float u = x / (float)e.w;
+                C3 c = t.sample(V2(u, v));
+                lum[(size_t)y * e.w + x] = (c.r * 0.212671f + c.g * 0.715160f + c.b * 0.072169f) * sin_theta;
+            }
+        }
+        e.dist.build(lum.data(), e.w, e.h);
+        // InfiniteLight.Preprocess (Light.cs:181-187): Aggregate.WorldBound = root node bounds
+        BBox b = s->nodes.empty() ? BBox() : s->nodes[0].b;
+        if (s->nodes.empty()) { b.mn = V3(0.0f); b.mx = V3(0.0f); }
+        e.center = (b.mx + b.mn) * 0.5f;
+        e.radius = length(b.mx - b.mn) / 2.0f;
+    }
     return s;
 }
 
@@ -1868,6 +2052,59 @@ int cr_sample_li(const cr_scene* s, uint32_t seed, int32_t x, int32_t y, int32_t
     if (!s) return RD_ERR_INVALID;
     C3 c = pixel_sample(*s, seed, x, y, sample, nullptr);
     rgb[0] = c.r; rgb[1] = c.g; rgb[2] = c.b;
+    return RD_OK;
+}
+
+void cr_dist1d_sample(const float* f, int32_t n, float u, float* out3, int32_t* offset) {
+    Dist1D d;
+    d.build(f, n);
+    float pdf;
+    int off;
+    out3[0] = d.sample_continuous(u, &pdf, &off);
+    out3[1] = pdf;
+    out3[2] = d.discrete_pdf(off);
+    *offset = off;
+}
+void cr_dist2d_sample(const float* f2, int32_t nu, int32_t nv, float u0, float u1, float* out3, int32_t* vj,
+                      int32_t* vi) {
+    Dist2D d;
+    d.build(f2, nu, nv);
+    float pdf;
+    int j, i;
+    V2 r = d.sample_continuous(u0, u1, &pdf, &j, &i);
+    out3[0] = r.x; out3[1] = r.y; out3[2] = pdf;
+    *vj = j; *vi = i;
+}
+float cr_dist2d_pdf(const float* f2, int32_t nu, int32_t nv, float u, float v) {
+    Dist2D d;
+    d.build(f2, nu, nv);
+    return d.continuous_pdf(u, v);
+}
+int cr_env_le(const cr_scene* s, int32_t env, const float* dir, float* rgb) {
+    if (!s || env < 0 || env >= (int)s->env.size()) return RD_ERR_INVALID;
+    C3 c = env_le(*s, s->env[env], V3(dir[0], dir[1], dir[2]));
+    rgb[0] = c.r; rgb[1] = c.g; rgb[2] = c.b;
+    return RD_OK;
+}
+float cr_env_pdf(const cr_scene* s, int32_t env, const float* wi) {
+    if (!s || env < 0 || env >= (int)s->env.size()) return -1.0f;
+    return env_pdf_dir(s->env[env], V3(wi[0], wi[1], wi[2]));
+}
+int cr_env_sample(const cr_scene* s, int32_t env, float u0, float u1, float* wdir, float* pdf, float* uv) {
+    if (!s || env < 0 || env >= (int)s->env.size()) return -1;
+    V3 w;
+    V2 t;
+    float p = 0;
+    if (!env_sample_dir(s->env[env], u0, u1, &w, &p, &t)) { *pdf = 0; return 0; }
+    wdir[0] = w.x; wdir[1] = w.y; wdir[2] = w.z;
+    *pdf = p; uv[0] = t.x; uv[1] = t.y;
+    return 1;
+}
+int cr_env_bounds(const cr_scene* s, int32_t env, float* center, float* radius) {
+    if (!s || env < 0 || env >= (int)s->env.size()) return RD_ERR_INVALID;
+    const EnvLight& e = s->env[env];
+    center[0] = e.center.x; center[1] = e.center.y; center[2] = e.center.z;
+    *radius = e.radius;
     return RD_OK;
 }
 

@@ -47,6 +47,22 @@ int cr_material_eval(const cr_scene* s, int32_t material, const float* wo, const
 int cr_material_sample(const cr_scene* s, int32_t material, const float* wo, const float* uv,
                        uint64_t key, float* wi, float* fr_rgb, float* pdf, int32_t* type);
 
+/* Distribution1D / Distribution2D probes on a caller's function values.
+ * dist1d: out = {continuous sample, pdf, discrete pdf of the sampled
+ * interval} and *offset; dist2d_sample: out = {d0, d1, pdf}, offsets (vj, vi);
+ * dist2d_pdf: ContinuousPdf(u, v).  f2 is nv rows of nu values. */
+void cr_dist1d_sample(const float* f, int32_t n, float u, float* out3, int32_t* offset);
+void cr_dist2d_sample(const float* f2, int32_t nu, int32_t nv, float u0, float u1, float* out3, int32_t* vj,
+                      int32_t* vi);
+float cr_dist2d_pdf(const float* f2, int32_t nu, int32_t nv, float u, float v);
+/* InfiniteAreaLight probes for the scene's env-th infinite light: Le of a
+ * world direction, PdfDir, SampleLightDir from two uniforms (returns 0 when
+ * the reference returns pdf 0), world centre / radius after Preprocess. */
+int cr_env_le(const cr_scene* s, int32_t env, const float* dir, float* rgb);
+float cr_env_pdf(const cr_scene* s, int32_t env, const float* wi);
+int cr_env_sample(const cr_scene* s, int32_t env, float u0, float u1, float* wdir, float* pdf, float* uv);
+int cr_env_bounds(const cr_scene* s, int32_t env, float* center, float* radius);
+
 /* Closed-form probes: Fresnel.DielectricFunc / ConductorFunc, and the
  * microfacet distribution (kind 0 Beckmann, 1 GGX): out = {D(wh), G(wo,wi),
  * Pdf(wo,wh)}; sample_wh draws with the given RNG key. */

@@ -50,6 +50,16 @@ def lib():
         L.cr_mf_eval.argtypes = [C.c_int32, C.c_float, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.cr_mf_sample.argtypes = [C.c_int32, C.c_float, C.c_float, C.c_void_p, C.c_uint64, C.c_void_p]
         L.cr_rng_floats.argtypes = [C.c_uint64, C.c_int32, C.c_void_p]
+        L.cr_dist1d_sample.argtypes = [C.c_void_p, C.c_int32, C.c_float, C.c_void_p, C.c_void_p]
+        L.cr_dist2d_sample.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_void_p,
+                                       C.c_void_p, C.c_void_p]
+        L.cr_dist2d_pdf.restype = C.c_float
+        L.cr_dist2d_pdf.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_float]
+        L.cr_env_le.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+        L.cr_env_pdf.restype = C.c_float
+        L.cr_env_pdf.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+        L.cr_env_sample.argtypes = [C.c_void_p, C.c_int32, C.c_float, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.cr_env_bounds.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
         L.cr_path_key.restype = C.c_uint64
         L.cr_path_key.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
         _lib = L
@@ -79,6 +89,30 @@ def mf_sample(kind, roughness, aniso, wo, key):
     wh = np.zeros(3, np.float32)
     lib().cr_mf_sample(kind, roughness, aniso, _ptr(wo), C.c_uint64(key), _ptr(wh))
     return wh
+
+
+def dist1d_sample(f, u):
+    """Distribution1D over f at uniform u: (continuous sample, pdf, discrete pdf, offset)."""
+    f = np.ascontiguousarray(f, np.float32)
+    out = np.zeros(3, np.float32)
+    off = C.c_int32(0)
+    lib().cr_dist1d_sample(_ptr(f), f.size, C.c_float(u), _ptr(out), C.byref(off))
+    return float(out[0]), float(out[1]), float(out[2]), off.value
+
+
+def dist2d_sample(f2, u0, u1):
+    """Distribution2D over f2 [nv][nu]: ((d0, d1), pdf, (vj, vi))."""
+    f2 = np.ascontiguousarray(f2, np.float32)
+    out = np.zeros(3, np.float32)
+    vj, vi = C.c_int32(0), C.c_int32(0)
+    lib().cr_dist2d_sample(_ptr(f2), f2.shape[1], f2.shape[0], C.c_float(u0), C.c_float(u1), _ptr(out),
+                           C.byref(vj), C.byref(vi))
+    return (float(out[0]), float(out[1])), float(out[2]), (vj.value, vi.value)
+
+
+def dist2d_pdf(f2, u, v):
+    f2 = np.ascontiguousarray(f2, np.float32)
+    return float(lib().cr_dist2d_pdf(_ptr(f2), f2.shape[1], f2.shape[0], C.c_float(u), C.c_float(v)))
 
 
 def rng_floats(key, n):
@@ -177,6 +211,27 @@ class OracleScene:
         out = np.zeros(3, np.float32)
         lib().cr_sample_li(self.h, seed, x, y, sample, _ptr(out))
         return out
+
+    def env_le(self, d, env=0):
+        d = np.asarray(d, np.float32); out = np.zeros(3, np.float32)
+        if lib().cr_env_le(self.h, env, _ptr(d), _ptr(out)) != 0:
+            raise RuntimeError("env_le")
+        return out
+
+    def env_pdf(self, wi, env=0):
+        wi = np.asarray(wi, np.float32)
+        return float(lib().cr_env_pdf(self.h, env, _ptr(wi)))
+
+    def env_sample(self, u0, u1, env=0):
+        """SampleLightDir: (world direction, pdf, corner uv) or None when pdf is 0."""
+        w = np.zeros(3, np.float32); uv = np.zeros(2, np.float32); pdf = C.c_float(0)
+        ok = lib().cr_env_sample(self.h, env, C.c_float(u0), C.c_float(u1), _ptr(w), C.byref(pdf), _ptr(uv))
+        return (w, pdf.value, uv) if ok == 1 else None
+
+    def env_bounds(self, env=0):
+        c = np.zeros(3, np.float32); r = C.c_float(0)
+        lib().cr_env_bounds(self.h, env, _ptr(c), C.byref(r))
+        return c, r.value
 
     def material_eval(self, mat, wo, wi, uv=(0.0, 0.0)):
         wo = np.asarray(wo, np.float32); wi = np.asarray(wi, np.float32); uv = np.asarray(uv, np.float32)

@@ -16,6 +16,7 @@
 #include "../../include/radiantium_hip.h"
 #include "rd_bvh_build.h"
 #include "rd_camera.h"
+#include "rd_envmap.h"
 #include "rd_kernels.h"
 
 using namespace rd;
@@ -294,8 +295,15 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
         }
     }
     for (int i = 0; i < d->num_lights; i++) {
-        if (d->lights[i].type != RD_LIGHT_AREA) return bad("infinite lights not built for the GPU path yet");
-        if (d->lights[i].prim < 0 || d->lights[i].prim >= d->num_prims) return bad("light primitive out of range");
+        const rd_light_desc& l = d->lights[i];
+        if (l.type == RD_LIGHT_INFINITE) {
+            if (l.tex < 0 || l.tex >= d->num_textures) return bad("infinite light texture out of range");
+            float inv[16];
+            if (!invert4(l.to_world, inv)) return bad("infinite light: invalid matrix");
+            continue;
+        }
+        if (l.type != RD_LIGHT_AREA) return bad("unknown light type");
+        if (l.prim < 0 || l.prim >= d->num_prims) return bad("light primitive out of range");
     }
 
     rd_scene* s = new rd_scene();
@@ -410,11 +418,39 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
         }
     }
     std::vector<DevLight> lights(d->num_lights);
+    std::vector<DevEnv> envs;
+    std::vector<EnvTables> env_tables;
     for (int i = 0; i < d->num_lights; i++) {
         const rd_light_desc& l = d->lights[i];
         DevLight& o = lights[i];
         std::memset(&o, 0, sizeof(o));
-        o.type = l.type; o.prim = l.prim; o.tex = l.tex;
+        o.type = l.type;
+        if (l.type == RD_LIGHT_INFINITE) {
+            // InfiniteAreaLight ctor + InfiniteLight.Preprocess (InfiniteAreaLight.cs:21-54,
+            // Light.cs:181-187): tables on the host, bounding sphere of the BVH root
+            o.prim = -1;
+            o.tex = (int)envs.size();
+            DevEnv e;
+            std::memset(&e, 0, sizeof(e));
+            e.tex = l.tex;
+            float inv[16];
+            invert4(l.to_world, inv);
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) { e.m2w[3 * r + c] = l.to_world[4 * r + c]; e.w2m[3 * r + c] = inv[4 * r + c]; }
+            const NodeRec& root = bvh.nodes[0];
+            f3 mn = mk(root.mn[0], root.mn[1], root.mn[2]), mx = mk(root.mx[0], root.mx[1], root.mx[2]);
+            f3 c = (mx + mn) * 0.5f;
+            e.center[0] = c.x; e.center[1] = c.y; e.center[2] = c.z;
+            e.radius = length(mx - mn) / 2.0f;
+            env_tables.emplace_back();
+            build_env_tables(d->textures[l.tex], &env_tables.back());
+            e.w = env_tables.back().w;
+            e.h = env_tables.back().h;
+            e.marg_int = env_tables.back().marg_int;
+            envs.push_back(e);
+            continue;
+        }
+        o.prim = l.prim; o.tex = l.tex;
         for (int k = 0; k < 3; k++) o.le[k] = l.le[k];
         o.area = info[l.prim].area;
     }
@@ -447,6 +483,15 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     if (!rc) rc = upload(s, texels.data(), texels.size(), &ds.texels);
     if (!rc) rc = upload(s, lights.data(), lights.size(), &ds.lights);
     if (!rc) rc = upload(s, media.data(), media.size(), &ds.media);
+    for (size_t i = 0; i < envs.size() && !rc; i++) {
+        const EnvTables& t = env_tables[i];
+        rc = upload(s, t.cond_func.data(), t.cond_func.size(), &envs[i].cond_func);
+        if (!rc) rc = upload(s, t.cond_cdf.data(), t.cond_cdf.size(), &envs[i].cond_cdf);
+        if (!rc) rc = upload(s, t.marg_func.data(), t.marg_func.size(), &envs[i].marg_func);
+        if (!rc) rc = upload(s, t.marg_cdf.data(), t.marg_cdf.size(), &envs[i].marg_cdf);
+    }
+    ds.num_envs = (int)envs.size();
+    if (!rc) rc = upload(s, envs.data(), envs.size(), &ds.envs);
     if (rc) { delete s; return nullptr; }
 
     const rd_integrator_desc& I = d->integrator;

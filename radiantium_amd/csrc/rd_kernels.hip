@@ -204,11 +204,13 @@ __device__ __forceinline__ void trav_finish(const DevScene& sc, const TraceArgs&
     } else if (kMode == kModeShadow) {
         a.ps.vis[a.q.pid[r]] = found ? 0 : 1;
     } else {
-        // BSDF-sampled MIS ray: Le of the hit when it is the selected light
-        // (PathTracer.cs:379-387); infinite lights on a miss (none modelled: 0)
+        // BSDF-sampled MIS ray: Le of the hit when it is the selected light,
+        // every infinite light on a miss (PathTracer.cs:377-390)
         int pid = a.q.pid[r];
         f3 li = mk(0.0f);
-        if (found) {
+        if (!found) {
+            li = eval_infinite(sc, t.d);
+        } else {
             const DevPrim pr = sc.prim_info[t.best.prim];
             int sel = a.ps.sel_light[pid];
             if (pr.light >= 0 && pr.light == sel) {
@@ -505,12 +507,12 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, DevIntegrator in, Wa
             bool go = true;
             if (in.method == 0) {
                 // OnlySampleBxdf (PathTracer.cs:43-93)
-                if (!hit) { L = L + beta * mk(0.0f); go = false; }   // EvalAllInfiniteLights: none
+                if (!hit) { L = L + beta * eval_infinite(sc, rd); go = false; }
                 else if (in.max_depth != -1 && depth >= in.max_depth) go = false;
             } else {
                 if (depth == 0 || spec) {
                     if (hit) L = L + beta * inct_le(wo);
-                    else L = L + beta * mk(0.0f);
+                    else L = L + beta * eval_infinite(sc, rd);
                 }
                 if (!hit) go = false;
                 else if (in.max_depth != -1 && depth >= in.max_depth) go = false;

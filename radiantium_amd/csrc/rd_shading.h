@@ -942,14 +942,104 @@ __device__ __forceinline__ BSample mat_sample(const MatCtx& c, f3 wo, Rng& rng) 
 }
 
 // ---------------------------------------------------------------- area lights
+// ---------------------------------------------------------------- environment light
+// Vector3.TransformNormal with a 3x3 row-major matrix: x*M11 + y*M21 + z*M31, ...
+__device__ __forceinline__ f3 xform_normal3(const float* M, f3 v) {
+    return mk(v.x * M[0] + v.y * M[3] + v.z * M[6],
+              v.x * M[1] + v.y * M[4] + v.z * M[7],
+              v.x * M[2] + v.y * M[5] + v.z * M[8]);
+}
+
+// Distribution1D.FindInterval (Distribution1D.cs:52-71) over cdf[0..n].
+__device__ __forceinline__ int dist_find(const float* cdf, int n, float u) {
+    int first = 0, len = n + 1;
+    while (len > 0) {
+        int half = len >> 1, middle = first + half;
+        if (cdf[middle] <= u) { first = middle + 1; len -= half + 1; }
+        else len = half;
+    }
+    return iclamp(first - 1, 0, n - 1);
+}
+
+// Equirectangular (u, v) of a world direction: phi = atan2(x, -z) in [0, 2 pi),
+// theta = acos(y) (InfiniteAreaLight.Le / PdfDir, :56-85).
+__device__ __forceinline__ void env_uv(const DevEnv& e, f3 d, float* u, float* v, float* theta_out) {
+    f3 dir = normalize(xform_normal3(e.w2m, d));
+    float phi = atan2f(dir.x, -dir.z);
+    phi = (phi < 0) ? (phi + 2 * kPi) : phi;
+    float theta = acosf(nclamp(dir.y, -1.0f, 1.0f));
+    *u = phi / (2 * kPi);
+    *v = theta / kPi;
+    *theta_out = theta;
+}
+
+// InfiniteAreaLight.Le (:56-69)
+__device__ __forceinline__ f3 env_le(const DevScene& sc, const DevEnv& e, f3 d) {
+    float u, v, th;
+    env_uv(e, d, &u, &v, &th);
+    return tex_sample(sc, e.tex, u, v);
+}
+
+// Scene.EvalAllInfiniteLights (Scene.cs:37-45)
+__device__ __forceinline__ f3 eval_infinite(const DevScene& sc, f3 d) {
+    f3 l = mk(0.0f);
+    for (int i = 0; i < sc.num_envs; i++) l = l + env_le(sc, sc.envs[i], d);
+    return l;
+}
+
+// InfiniteAreaLight.PdfDir (:71-85) with Distribution2D.ContinuousPdf (Distribution2D.cs:55-60)
+__device__ __forceinline__ float env_pdf_dir(const DevEnv& e, f3 wi) {
+    float u, v, theta;
+    env_uv(e, wi, &u, &v, &theta);
+    float sin_theta = sinf(theta);
+    if (sin_theta == 0.0f) return 0.0f;
+    int iu = iclamp((int)(u * e.w), 0, e.w - 1);
+    int iv = iclamp((int)(v * e.h), 0, e.h - 1);
+    float pdf = e.cond_func[(size_t)iv * e.w + iu] / e.marg_int;
+    return pdf / (2 * kPi * kPi * sin_theta);
+}
+
 struct LSample {
     f3 p, wi, li;
     float pdf;
 };
 
+// InfiniteAreaLight.SampleLi (:111-119) via SampleLightDir (:92-109): the
+// Distribution2D picks a cell (marginal on u.Y, conditional on u.X); the
+// direction is the cell's corner, the pdf the cell's continuous density over
+// 2 pi^2 sin(theta); the shadow target lies 20 world radii out.
+__device__ __forceinline__ LSample env_sample_li(const DevScene& sc, const DevEnv& e, f3 refp, Rng& rng) {
+    LSample out;
+    out.p = mk(0.0f); out.wi = mk(0.0f); out.li = mk(0.0f); out.pdf = 0.0f;
+    float u0 = rng.next_float(), u1 = rng.next_float();
+    int vj = dist_find(e.marg_cdf, e.h, u1);
+    float pdf1 = e.marg_int > 0 ? e.marg_func[vj] / e.marg_int : 0;
+    const float rint = e.marg_func[vj];
+    int vi = dist_find(e.cond_cdf + (size_t)vj * (e.w + 1), e.w, u0);
+    float pdf0 = rint > 0 ? e.cond_func[(size_t)vj * e.w + vi] / rint : 0;
+    float pdf = pdf0 * pdf1;
+    if (pdf <= 0) return out;
+    float y = vj / (float)e.h;
+    float x = vi / (float)e.w;
+    float theta = y * kPi;
+    float phi = x * 2 * kPi;
+    float sin_theta = sinf(theta);
+    if (sin_theta == 0) return out;
+    pdf /= (2 * kPi * kPi * sin_theta);
+    float st = sinf(theta), ct = cosf(theta), sp = sinf(phi), cp = cosf(phi);   // MathExt.SphericalCoordinates
+    f3 sph = mk(cp * st, sp * st, ct);
+    f3 wdir = normalize(xform_normal3(e.m2w, mk(sph.y, sph.z, -sph.x)));
+    out.li = tex_sample(sc, e.tex, x, y);
+    out.p = wdir * e.radius * 20.0f + mk(e.center[0], e.center[1], e.center[2]);
+    out.wi = normalize(out.p - refp);
+    out.pdf = pdf;
+    return out;
+}
+
 // DiffuseAreaLight.SampleLi (Lights/DiffuseAreaLight.cs:48-68) with
 // Triangle.Sample (Triangle.cs:184-212) / Sphere.Sample (Sphere.cs:87-102).
 __device__ __forceinline__ LSample light_sample(const DevScene& sc, const DevLight& L, f3 refp, Rng& rng) {
+    if (L.type == 1) return env_sample_li(sc, sc.envs[L.tex], refp, rng);
     const DevPrim pr = sc.prim_info[L.prim];
     float ux = rng.next_float(), uy = rng.next_float();
     f3 P, N;
@@ -989,6 +1079,7 @@ __device__ __forceinline__ LSample light_sample(const DevScene& sc, const DevLig
 
 // DiffuseAreaLight.PdfLi (DiffuseAreaLight.cs:31-46): tests the emitter alone.
 __device__ __forceinline__ float light_pdf(const DevScene& sc, const DevLight& L, f3 refp, f3 wi) {
+    if (L.type == 1) return env_pdf_dir(sc.envs[L.tex], wi);   // InfiniteAreaLight.PdfLi (:87-90)
     const DevPrim pr = sc.prim_info[L.prim];
     float tu, tv, tt;
     if (!prim_test(sc, pr, refp, wi, kFloatEps, kFloatMax, &tu, &tv, &tt)) return 0.0f;

@@ -41,11 +41,27 @@ struct DevMaterial {
 
 struct DevLight {
     int type;     // 0 area, 1 infinite
-    int prim;     // original primitive index of the emitter shape
+    int prim;     // area: original primitive index of the emitter shape
     float le[3];
-    int tex;
+    int tex;      // infinite: index into DevScene::envs
     float area;   // Shape.SurfaceArea of the emitter
     int _pad;
+};
+
+// InfiniteAreaLight (Lights/InfiniteAreaLight.cs) after Preprocess
+// (Light.cs:181-187): the radiance map, the linear part of the model/world
+// transforms (row-major M11..M33, row vectors as Vector3.TransformNormal),
+// the scene's bounding sphere, and the Distribution2D tables (host-built).
+struct DevEnv {
+    int tex, w, h, _pad;
+    float m2w[9], w2m[9];
+    float center[3], radius;
+    const float* cond_func;  // [h][w]
+    const float* cond_cdf;   // [h][w+1]
+    const float* marg_func;  // [h]
+    const float* marg_cdf;   // [h+1]
+    float marg_int;
+    int _pad2;
 };
 
 struct DevMedium {
@@ -88,8 +104,9 @@ struct DevScene {
     const float* texels;
     const DevLight* lights;
     const DevMedium* media;
+    const DevEnv* envs;    // Scene.InfiniteLights order
+    int num_envs;
     int global_medium;
-    int _pad;
     DevCamera cam;
 };
 

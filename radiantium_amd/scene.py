@@ -200,6 +200,17 @@ class SceneDesc:
         self.prims.append(np.array([[abi.PRIM_SPHERE, k, material, light, medium_inside, medium_outside]], np.int32))
         return p
 
+    def add_infinite_light(self, le, to_world=None):
+        """An "infinite" light entity (RendererBuilder.cs:221-225, InfiniteAreaLight):
+        `le` is a Texture (or colour) holding the equirectangular radiance map,
+        to_world the entity's model-to-world matrix (row-major, row vectors).
+        It joins Scene.Lights at this point of the entity order.  Returns its
+        light index."""
+        t = self.add_texture(_as_tex(le, (0.0, 0.0, 0.0)))
+        tw = None if to_world is None else np.asarray(to_world, np.float32).reshape(4, 4)
+        self.lights.append(dict(type=abi.LIGHT_INFINITE, prim=-1, le=None, tex=t, to_world=tw))
+        return len(self.lights) - 1
+
     def num_prims(self):
         return int(sum(b.shape[0] for b in self.prims))
 

@@ -142,6 +142,65 @@ def material_box(width=64, height=64, max_depth=8, method="Mis"):
     return s
 
 
+def hdr_sky(width, height, seed=0, sigma=1.5, sun_dir=(0.4, 0.75, 0.3), sun_radiance=400.0, sun_deg=2.5):
+    """Equirectangular HDR radiance map [width][height][3] (column-major like
+    ColorBuffer): per-texel log-normal radiance (mu 0, sigma) with a slight
+    colour spread, plus one bright sun disk baked into the map (the reference
+    has no sun-light type).  Texel (x, y) covers phi = 2 pi (x+0.5)/W,
+    theta = pi (y+0.5)/H; direction (sin phi sin theta, cos theta, -cos phi sin theta)
+    as InfiniteAreaLight.Le's inverse mapping."""
+    rng = np.random.default_rng(seed)
+    lum = np.exp(rng.normal(0.0, sigma, (width, height)))
+    tint = rng.uniform(0.8, 1.2, (width, height, 3))
+    img = lum[..., None] * tint
+    x, y = np.meshgrid((np.arange(width) + 0.5) / width, (np.arange(height) + 0.5) / height, indexing="ij")
+    phi, theta = 2 * np.pi * x, np.pi * y
+    d = np.stack([np.sin(phi) * np.sin(theta), np.cos(theta), -np.cos(phi) * np.sin(theta)], -1)
+    sd = np.asarray(sun_dir, np.float64)
+    sd /= np.linalg.norm(sd)
+    sun = (d @ sd) > np.cos(np.deg2rad(sun_deg))
+    img[sun] = sun_radiance * np.array([1.0, 0.95, 0.85])
+    return img.astype(np.float32)
+
+
+def env_showcase(width=64, height=64, max_depth=8, method="Mis", map_w=64, map_h=32, seed=0,
+                 rotate_deg=30.0, area_light=True):
+    """Objects under an HDR environment map (rotated about y by `rotate_deg`
+    through the light's to_world): Disney, rough plastic and rough metal
+    spheres, a perfect-glass sphere, a rough-glass slab on a diffuse ground,
+    and (optionally) one small emissive quad, so light selection mixes area
+    and infinite lights."""
+    from .scene import Texture
+    s = SceneDesc()
+    ground = s.add_material("diffuse", kd=(0.5, 0.5, 0.5))
+    disney = s.add_material("disney", base_color=(0.7, 0.4, 0.2), roughness=0.4, metallic=0.3, clearcoat=0.5)
+    plastic = s.add_material("rough_plastic", r=(0.2, 0.4, 0.7), roughness=0.3, dist="GGX")
+    metal = s.add_material("rough_metal", roughness=0.3)
+    glass = s.add_material("perfect_glass")
+    rglass = s.add_material("rough_glass", roughness=0.25)
+    black = s.add_material("diffuse", kd=(0.0, 0.0, 0.0))
+    W = 4.0
+    s.add_mesh(*_quad((-W, 0, -W), (-W, 0, W), (W, 0, W), (W, 0, -W)), material=ground)
+    s.add_sphere((-0.9, 0.4, 0.0), 0.4, material=disney)
+    s.add_sphere((0.0, 0.4, -0.4), 0.4, material=plastic)
+    s.add_sphere((0.9, 0.4, 0.0), 0.4, material=metal)
+    s.add_sphere((0.3, 0.25, 0.7), 0.25, material=glass)
+    s.add_mesh(*_box((0.5, 0.1, 0.5), (-0.35, 0.0, 0.75), 20.0), material=rglass)
+    a = np.deg2rad(rotate_deg)
+    R = np.array([[np.cos(a), 0, -np.sin(a), 0], [0, 1, 0, 0], [np.sin(a), 0, np.cos(a), 0], [0, 0, 0, 1]],
+                 np.float32)
+    s.add_infinite_light(Texture(image=hdr_sky(map_w, map_h, seed)), to_world=R)
+    if area_light:
+        q = 0.2
+        s.add_mesh(*_quad((-q, 2.0, -q), (q, 2.0, -q), (q, 2.0, q), (-q, 2.0, q)), material=black,
+                   le=(20.0, 20.0, 20.0))
+    s.set_camera(fov=45.0, width=width, height=height, origin=(0.0, 1.2, 3.2), target=(0.0, 0.35, 0.0),
+                 up=(0.0, 1.0, 0.0))
+    s.set_integrator(type="path", max_depth=max_depth, method=method)
+    s.bvh_max_prim = 1
+    return s
+
+
 # ---------------------------------------------------------------- icosphere
 def icosphere(level):
     """Unit icosphere: 20*4^level faces, midpoint subdivision with shared

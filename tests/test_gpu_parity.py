@@ -167,6 +167,34 @@ def test_render_twin_material_box(ctx, oracle, method):
     assert np.mean(np.abs(z) < 4) > 0.99
 
 
+@pytest.mark.parametrize("method", ["Mis", "Nee", "Bxdf"])
+def test_render_twin_env(ctx, oracle, method):
+    """InfiniteAreaLight (rotated HDR map with a sun disk, importance-sampled
+    through Distribution2D) mixed with an area light, over Disney, plastic,
+    metal, perfect and rough glass."""
+    desc = scenes.env_showcase(48, 48, method=method, map_w=64, map_h=32)
+    print(f"[render twin] env_showcase 48x48 spp=8 method={method}", flush=True)
+    g, g2, gst, o, o2, ost = _render_pair(ctx, oracle, desc, 8, seed=13)
+    assert np.isfinite(g).all()
+    for k in ("rays_closest", "rays_any", "vertices"):
+        assert abs(gst[k] - ost[k]) <= max(16, 0.02 * ost[k]), (k, gst[k], ost[k])
+    close = np.isclose(g, o, rtol=1e-3, atol=1e-5)
+    assert close.mean() > 0.95, close.mean()
+    z = z_test(g, g2, 8, o, o2, 8)
+    assert np.mean(np.abs(z) < 4) > 0.99
+
+
+def test_render_env_statistical(ctx, oracle):
+    """Independent seeds on a larger map (256x128): per-pixel z-test."""
+    desc = scenes.env_showcase(32, 32, method="Mis", map_w=256, map_h=128)
+    gs = Scene(ctx, desc)
+    os_ = oracle.OracleScene(desc)
+    g, g2 = gs.render(64, seed=100)
+    o, o2, _, _ = os_.render(64, seed=200)
+    z = z_test(g, g2, 64, o, o2, 64)
+    assert np.mean(np.abs(z) > 3) < 0.02
+
+
 def test_render_twin_icosphere(ctx, oracle):
     """GGX conductor + rough dielectric (the 1M-triangle workload's materials) at level 5."""
     desc = scenes.displaced_icosphere_scene(level=5, width=96, height=54)
