@@ -285,6 +285,12 @@ struct Dist2D {
     }
 };
 
+// Homogeneous (Mediums/Homogeneous.cs:14-20): coefficients scaled at construction.
+struct Homo {
+    C3 sa, ss, st;
+    float g;
+};
+
 // InfiniteAreaLight (Lights/InfiniteAreaLight.cs:21-119) after
 // InfiniteLight.Preprocess (Light.cs:181-187).
 struct EnvLight {
@@ -307,6 +313,7 @@ struct cr_scene {
     std::vector<rd_material_desc> mats;
     std::vector<rd_light_desc> lights;
     std::vector<rd_medium_desc> media;
+    std::vector<Homo> homo;
     std::vector<int> infinite;          // light index of each InfiniteLight, Scene.InfiniteLights order
     std::vector<int> env_of_light;      // light index -> EnvLight index (-1 for area lights)
     std::vector<EnvLight> env;
@@ -1785,7 +1792,295 @@ static C3 li_bxdf(const cr_scene& s, Ray ray, Rng& rng, uint64_t* st) {
     return radiance;
 }
 
+// ---------------------------------------------------------------- media
+static Homo make_homo(const rd_medium_desc& m) {
+    Homo h;
+    h.sa = C3(m.sigma_a[0], m.sigma_a[1], m.sigma_a[2]) * m.scale;
+    h.ss = C3(m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]) * m.scale;
+    h.st = h.sa + h.ss;
+    h.g = m.g;
+    return h;
+}
+// Homogeneous.Tr (:22-30): exp(sigma_t * (MinT - MaxT)) per channel
+static C3 medium_tr(const Homo& m, float tmin, float tmax) {
+    float neg = tmin - tmax;
+    C3 tr;
+    tr.r = m.st.r == 0 ? 1.0f : std::exp(m.st.r * neg);
+    tr.g = m.st.g == 0 ? 1.0f : std::exp(m.st.g * neg);
+    tr.b = m.st.b == 0 ? 1.0f : std::exp(m.st.b * neg);
+    return tr;
+}
+struct MSample {
+    bool medium = false;
+    V3 p, wo;
+    C3 tr;
+    float t = 0;
+};
+// Homogeneous.Sample (:32-61): spectral free flight, channel chosen uniformly
+static MSample medium_sample(const Homo& m, const Ray& r, Rng& rng) {
+    MSample out;
+    int ch = rng.next_int(3);
+    float dens = ch == 0 ? m.st.r : (ch == 1 ? m.st.g : m.st.b);
+    float sd = -std::log(1 - rng.next_float()) / dens;
+    float dist = r.tmax - r.tmin;
+    if (sd < dist) {
+        float t = r.tmin + sd;
+        out.p = r.o + t * r.d;
+        C3 tr(std::exp(-m.st.r * sd), std::exp(-m.st.g * sd), std::exp(-m.st.b * sd));
+        float pdf = (tr.r * m.st.r + tr.g * m.st.g + tr.b * m.st.b) / 3;
+        if (pdf == 0) pdf = 0.0001f;
+        out.medium = true;
+        out.wo = -r.d;
+        out.tr = tr * m.ss / pdf;
+        out.t = t;
+    } else {
+        C3 tr(std::exp(-m.st.r * dist), std::exp(-m.st.g * dist), std::exp(-m.st.b * dist));
+        float pdf = (tr.r + tr.g + tr.b) / 3;
+        if (pdf == 0) pdf = 0.0001f;
+        out.tr = tr / pdf;
+    }
+    return out;
+}
+// PhaseFunctionUtility.HenyeyGreenstein (IPhaseFunction.cs:39-44)
+static inline float hg_eval(float cos_t, float g) {
+    float denom = 1 + g * g + 2 * g * cos_t;
+    return (1 / (4 * PI_F)) * (1 - g * g) / (denom * std::sqrt(denom));
+}
+static inline float hg_g(const Homo& m) { return nclamp(m.g, -0.999f, 0.999f); }     // :51-54
+static inline float hg_p(const Homo& m, V3 wo, V3 wi) { return hg_eval(dot(wo, wi), hg_g(m)); }
+struct PSample { float p; V3 wi; };
+// HenyeyGreenstein.SampleWi (:62-81)
+static PSample hg_sample(const Homo& m, V3 wo, Rng& rng) {
+    float G = hg_g(m);
+    float x = rng.next_float(), y = rng.next_float();
+    float cos_t;
+    if (std::fabs(G) < 0.001f) {
+        cos_t = 1 - 2 * x;
+    } else {
+        float sqr = (1 - G * G) / (1 + G - 2 * G * x);
+        cos_t = -(1 + G * G - sqr * sqr) / (2 * G);
+    }
+    float sin_t = std::sqrt(nmax(0.0f, 1 - cos_t * cos_t));
+    float phi = 2 * PI_F * y;
+    Frame f(wo);
+    float sp = std::sin(phi), cp = std::cos(phi);
+    PSample out;
+    out.wi = sin_t * cp * f.X + sin_t * sp * f.Y + cos_t * f.Z;     // MathExt.SphericalDirection
+    out.p = hg_eval(cos_t, G);
+    return out;
+}
+// Intersection.GetMedium (Intersection.cs:68-71)
+static inline int inct_medium(const cr_scene& s, const Inct& it, V3 w) {
+    const rd_prim_desc& pd = s.prims[it.prim];
+    return dot(w, it.N()) > 0 ? pd.medium_outside : pd.medium_inside;
+}
+// Scene.Transmittance (Scene.cs:72-103): closest-hit segments toward `to`;
+// a surface blocks, medium boundaries are crossed.  Tr covers each whole
+// segment [1e-4, dis - 1e-4].
+static C3 scene_transmittance(const cr_scene& s, V3 from, V3 to, int env, uint64_t* st) {
+    C3 tr(1.0f);
+    V3 start = from;
+    V3 dir = normalize(to - start);
+    while (true) {
+        V3 vec = to - start;
+        float dis = length(vec);
+        if (dis < 0.0001f) break;
+        Ray ray(start, dir, 0.0001f, dis - 0.0001f);
+        Inct it;
+        bool any = scene_intersect(s, ray, &it, st);
+        if (any && s.prims[it.prim].material >= 0) return C3(0.0f);
+        if (env >= 0) tr = tr * medium_tr(s.homo[env], ray.tmin, ray.tmax);
+        if (!any) break;
+        start = it.P;
+        env = inct_medium(s, it, dir);
+    }
+    return tr;
+}
+// Scene.IntersectTr (Scene.cs:105-131).  The medium segment before a hit
+// ends at |ray.At(t)| (the length of the hit's position vector), as written.
+static bool scene_intersect_tr(const cr_scene& s, Ray ray, int env, Inct* out, C3* tr_out, uint64_t* st) {
+    C3 tr(1.0f);
+    while (true) {
+        bool any = scene_intersect(s, ray, out, st);
+        if (!any) {
+            if (env >= 0) tr = tr * medium_tr(s.homo[env], ray.tmin, ray.tmax);
+            *tr_out = tr;
+            return false;
+        }
+        if (env >= 0) tr = tr * medium_tr(s.homo[env], ray.tmin, length(ray.o + out->T * ray.d));
+        if (s.prims[out->prim].material >= 0) {
+            *tr_out = tr;
+            return true;
+        }
+        ray = Ray(out->P, ray.d, 0.001f);
+        env = inct_medium(s, *out, ray.d);
+    }
+}
+
+// VolumetricPathTracer.EstimateDirectSurface (Integrators/VolumetricPathTracer.cs:151-219)
+static C3 estimate_direct_surface(const cr_scene& s, Rng& rng, int light, const Inct& it, int env, uint64_t* st) {
+    const rd_material_desc& m = s.mats[s.prims[it.prim].material];
+    C3 le(0.0f);
+    V3 wo = it.wr;
+    LSample ls = light_sample_li(s, light, it.P, rng);
+    if (ls.pdf > 0.0f && !ls.li.black()) {
+        C3 fr = material_fr(s, m, it.sh.to_local(wo), it.sh.to_local(ls.wi), it.UV);
+        float spdf = material_pdf(s, m, it.sh.to_local(wo), it.sh.to_local(ls.wi), it.UV);
+        if (!fr.black()) {
+            C3 li = ls.li * scene_transmittance(s, it.P, ls.p, env, st);
+            if (!li.black()) {
+                fr = fr * abs_cos_theta(it.sh.to_local(ls.wi));
+                float w = power_heuristic(1, ls.pdf, 1, spdf);
+                le = le + fr * li * w / ls.pdf;
+            }
+        }
+    }
+    BSample bs = material_sample(s, m, it.sh.to_local(wo), it.UV, rng);
+    C3 fr = bs.fr * abs_cos_theta(bs.wi);
+    float spdf = bs.pdf;
+    bool sampled_specular = (bs.type & T_SPECULAR) != 0;
+    if (!fr.black() && spdf > 0) {
+        float w = 1;
+        if (!sampled_specular) {
+            float lpdf = light_pdf_li(s, light, it.P, it.sh.to_world(bs.wi));
+            if (lpdf == 0.0f) return le;
+            w = power_heuristic(1, spdf, 1, lpdf);
+        }
+        Ray to_light = spawn_ray(it, it.sh.to_world(bs.wi));
+        Inct li_it;
+        C3 tr;
+        bool hit = scene_intersect_tr(s, to_light, env, &li_it, &tr, st);
+        C3 li(0);
+        if (hit) {
+            if (s.prims[li_it.prim].light == light) li = inct_le(s, li_it, it.sh.to_world(-bs.wi));
+        } else {
+            li = eval_infinite(s, to_light);
+        }
+        if (!li.black()) le = le + fr * li * tr * w / spdf;
+    }
+    return le;
+}
+
+// VolumetricPathTracer.EstimateDirectMedium (VolumetricPathTracer.cs:252-314)
+static C3 estimate_direct_medium(const cr_scene& s, Rng& rng, int light, const MSample& ms, int env, uint64_t* st) {
+    const Homo& md = s.homo[env];
+    C3 le(0.0f);
+    LSample ls = light_sample_li(s, light, ms.p, rng);
+    if (ls.pdf > 0.0f && !ls.li.black()) {
+        float p = hg_p(md, ms.wo, ls.wi);
+        float spdf = p;
+        if (p != 0.0f) {
+            C3 li = ls.li * scene_transmittance(s, ms.p, ls.p, env, st);
+            C3 f(p);
+            if (!li.black() && !f.black()) {
+                float w = power_heuristic(1, ls.pdf, 1, spdf);
+                le = le + f * li * w / ls.pdf;
+            }
+        }
+    }
+    PSample ps = hg_sample(md, ms.wo, rng);
+    C3 f(ps.p);
+    float spdf = ps.p;
+    if (spdf > 0) {
+        float lpdf = light_pdf_li(s, light, ms.p, ps.wi);
+        if (lpdf == 0.0f) return le;
+        float w = power_heuristic(1, spdf, 1, lpdf);
+        Ray to_light(ms.p, ps.wi);                       // Ray3F default MinT = float.Epsilon
+        Inct li_it;
+        C3 tr;
+        bool hit = scene_intersect_tr(s, to_light, env, &li_it, &tr, st);
+        C3 li(0);
+        if (hit) {
+            if (s.prims[li_it.prim].light == light) li = inct_le(s, li_it, -ps.wi);
+        } else {
+            li = eval_infinite(s, to_light);
+        }
+        if (!li.black()) le = le + f * li * tr * w / spdf;
+    }
+    return le;
+}
+
+// SampleLightToEstimateDirectSurface / ...Medium (VolumetricPathTracer.cs:121-147, 221-247)
+template <class F>
+static C3 sample_light_vol(const cr_scene& s, Rng& rng, F&& estimate) {
+    C3 result(0.0f);
+    if (s.integ.strategy == RD_STRATEGY_UNIFORM) {
+        if (s.lights.empty()) return result;
+        int l = rng.next_int((int)s.lights.size());
+        float lpdf = 1.0f / s.lights.size();
+        if (lpdf > 0.0f) result = result + estimate(l) / lpdf;
+    } else {
+        for (int l = 0; l < (int)s.lights.size(); l++) result = result + estimate(l);
+    }
+    return result;
+}
+
+// VolumetricPathTracer.Li (VolumetricPathTracer.cs:28-119)
+static C3 li_vol(const cr_scene& s, Ray ray, Rng& rng, uint64_t* st) {
+    C3 radiance(0.0f), coeff(1.0f);
+    bool specular_path = true;
+    int env = s.global_medium;
+    const rd_integrator_desc& I = s.integ;
+    for (int bounces = 0;; bounces++) {
+        V3 wo = -ray.d;
+        if (ray.tmax <= ray.tmin) break;
+        Inct it;
+        bool hit = scene_intersect(s, ray, &it, st);
+        MSample ms;
+        if (env >= 0) {
+            Ray real(ray.o, ray.d, ray.tmin, hit ? it.T : FLT_MAX);
+            ms = medium_sample(s.homo[env], real, rng);
+            coeff = coeff * ms.tr;
+        }
+        if (coeff.black()) break;
+        if (ms.medium) {
+            if (bounces >= I.max_depth) break;          // as written: also when max_depth is -1
+            if (st) st[3]++;
+            int menv = env;
+            radiance = radiance + coeff * sample_light_vol(s, rng, [&](int l) {
+                return estimate_direct_medium(s, rng, l, ms, menv, st);
+            });
+            PSample ps = hg_sample(s.homo[menv], wo, rng);
+            ray = Ray(ms.p, ps.wi, ray.tmin);
+            specular_path = false;
+        } else {
+            if (bounces == 0 || specular_path) {
+                if (hit) radiance = radiance + coeff * inct_le(s, it, wo);
+                else radiance = radiance + coeff * eval_infinite(s, ray);
+            }
+            if (!hit) break;
+            if (I.max_depth != -1 && bounces >= I.max_depth) break;
+            int mat = s.prims[it.prim].material;
+            if (mat < 0) {
+                ray = Ray(it.P, ray.d, ray.tmin);
+                env = inct_medium(s, it, ray.d);
+                bounces--;
+                continue;
+            }
+            if (st) st[3]++;
+            int senv = env;
+            radiance = radiance + coeff * sample_light_vol(s, rng, [&](int l) {
+                return estimate_direct_surface(s, rng, l, it, senv, st);
+            });
+            BSample bs = material_sample(s, s.mats[mat], it.sh.to_local(wo), it.UV, rng);
+            specular_path = (bs.type & T_SPECULAR) != 0;
+            if (bs.pdf > 0.0f) coeff = coeff * (bs.fr * abs_cos_theta(bs.wi) / bs.pdf);
+            else break;
+            V3 next = it.sh.to_world(bs.wi);
+            ray = Ray(it.P, next, ray.tmin);
+            env = inct_medium(s, it, next);
+        }
+        if (bounces > I.min_depth) {
+            float q = nmin(coeff.max_elem(), I.rr_threshold);
+            if (rng.next_float() > q) break;
+            coeff = coeff / q;
+        }
+    }
+    return radiance;
+}
+
 static C3 li(const cr_scene& s, const Ray& ray, Rng& rng, uint64_t* st) {
+    if (s.integ.type == RD_INTEGRATOR_VOL_PATH) return li_vol(s, ray, rng, st);
     switch (s.integ.method) {
     case RD_METHOD_BXDF: return li_bxdf(s, ray, rng, st);
     case RD_METHOD_NEE: return li_nee(s, ray, rng, st);
@@ -1896,7 +2191,14 @@ cr_scene* cr_scene_create(const rd_scene_desc* d) {
         int p = L.prim;
         if (p < 0 || p >= d->num_prims) { set_err("light prim index"); delete s; return nullptr; }
     }
-    if (d->integrator.type != RD_INTEGRATOR_PATH) { set_err("vol_path not in this restatement yet"); delete s; return nullptr; }
+    if (d->integrator.type != RD_INTEGRATOR_PATH && d->integrator.type != RD_INTEGRATOR_VOL_PATH) {
+        set_err("unknown integrator type"); delete s; return nullptr;
+    }
+    for (int i = 0; i < d->num_media; i++) s->homo.push_back(make_homo(d->media[i]));
+    if (s->global_medium < -1 || s->global_medium >= d->num_media) { set_err("global medium index"); delete s; return nullptr; }
+    for (const rd_prim_desc& pd : s->prims)
+        if (pd.medium_inside < -1 || pd.medium_inside >= d->num_media || pd.medium_outside < -1 ||
+            pd.medium_outside >= d->num_media) { set_err("prim medium index"); delete s; return nullptr; }
     if (!s->cam.init(d->camera)) { set_err("camera matrix not invertible"); delete s; return nullptr; }
     build_bvh(*s);
     for (EnvLight& e : s->env) {
@@ -2106,6 +2408,38 @@ int cr_env_bounds(const cr_scene* s, int32_t env, float* center, float* radius) 
     center[0] = e.center.x; center[1] = e.center.y; center[2] = e.center.z;
     *radius = e.radius;
     return RD_OK;
+}
+
+void cr_medium_tr(const rd_medium_desc* m, float tmin, float tmax, float* rgb) {
+    C3 t = medium_tr(make_homo(*m), tmin, tmax);
+    rgb[0] = t.r; rgb[1] = t.g; rgb[2] = t.b;
+}
+int cr_medium_sample(const rd_medium_desc* m, const float* o, const float* d, float tmin, float tmax, uint64_t key,
+                     float* p, float* tr, float* t) {
+    Rng rng;
+    rng.key = key;
+    rng.n = 0;
+    Ray r(V3(o[0], o[1], o[2]), V3(d[0], d[1], d[2]), tmin, tmax);
+    MSample ms = medium_sample(make_homo(*m), r, rng);
+    p[0] = ms.p.x; p[1] = ms.p.y; p[2] = ms.p.z;
+    tr[0] = ms.tr.r; tr[1] = ms.tr.g; tr[2] = ms.tr.b;
+    *t = ms.t;
+    return ms.medium ? 1 : 0;
+}
+float cr_hg_p(float g, const float* wo, const float* wi) {
+    Homo h{};
+    h.g = g;
+    return hg_p(h, V3(wo[0], wo[1], wo[2]), V3(wi[0], wi[1], wi[2]));
+}
+float cr_hg_sample(float g, const float* wo, uint64_t key, float* wi) {
+    Homo h{};
+    h.g = g;
+    Rng rng;
+    rng.key = key;
+    rng.n = 0;
+    PSample ps = hg_sample(h, V3(wo[0], wo[1], wo[2]), rng);
+    wi[0] = ps.wi.x; wi[1] = ps.wi.y; wi[2] = ps.wi.z;
+    return ps.p;
 }
 
 int cr_material_eval(const cr_scene* s, int32_t material, const float* wo, const float* wi, const float* uv,

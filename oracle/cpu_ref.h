@@ -63,6 +63,16 @@ float cr_env_pdf(const cr_scene* s, int32_t env, const float* wi);
 int cr_env_sample(const cr_scene* s, int32_t env, float u0, float u1, float* wdir, float* pdf, float* uv);
 int cr_env_bounds(const cr_scene* s, int32_t env, float* center, float* radius);
 
+/* Homogeneous medium and Henyey-Greenstein probes (Mediums/Homogeneous.cs,
+ * IPhaseFunction.cs): Tr over [tmin, tmax]; free-flight Sample with an RNG
+ * key (returns 1 for a medium vertex: p, weight tr, t; 0 when the segment
+ * is passed: weight tr); HG value and SampleWi (returns p). */
+void cr_medium_tr(const rd_medium_desc* m, float tmin, float tmax, float* rgb);
+int cr_medium_sample(const rd_medium_desc* m, const float* o, const float* d, float tmin, float tmax, uint64_t key,
+                     float* p, float* tr, float* t);
+float cr_hg_p(float g, const float* wo, const float* wi);
+float cr_hg_sample(float g, const float* wo, uint64_t key, float* wi);
+
 /* Closed-form probes: Fresnel.DielectricFunc / ConductorFunc, and the
  * microfacet distribution (kind 0 Beckmann, 1 GGX): out = {D(wh), G(wo,wi),
  * Pdf(wo,wh)}; sample_wh draws with the given RNG key. */

@@ -60,6 +60,13 @@ def lib():
         L.cr_env_pdf.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
         L.cr_env_sample.argtypes = [C.c_void_p, C.c_int32, C.c_float, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]
         L.cr_env_bounds.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+        L.cr_medium_tr.argtypes = [C.POINTER(abi.rd_medium_desc), C.c_float, C.c_float, C.c_void_p]
+        L.cr_medium_sample.argtypes = [C.POINTER(abi.rd_medium_desc), C.c_void_p, C.c_void_p, C.c_float, C.c_float,
+                                       C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.cr_hg_p.restype = C.c_float
+        L.cr_hg_p.argtypes = [C.c_float, C.c_void_p, C.c_void_p]
+        L.cr_hg_sample.restype = C.c_float
+        L.cr_hg_sample.argtypes = [C.c_float, C.c_void_p, C.c_uint64, C.c_void_p]
         L.cr_path_key.restype = C.c_uint64
         L.cr_path_key.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
         _lib = L
@@ -113,6 +120,41 @@ def dist2d_sample(f2, u0, u1):
 def dist2d_pdf(f2, u, v):
     f2 = np.ascontiguousarray(f2, np.float32)
     return float(lib().cr_dist2d_pdf(_ptr(f2), f2.shape[1], f2.shape[0], C.c_float(u), C.c_float(v)))
+
+
+def _medium(sigma_a, sigma_s, g=0.0, scale=1.0):
+    m = abi.rd_medium_desc()
+    for i in range(3):
+        m.sigma_a[i] = sigma_a[i]
+        m.sigma_s[i] = sigma_s[i]
+    m.g, m.scale = g, scale
+    return m
+
+
+def medium_tr(sigma_a, sigma_s, tmin, tmax, scale=1.0):
+    out = np.zeros(3, np.float32)
+    lib().cr_medium_tr(C.byref(_medium(sigma_a, sigma_s, 0.0, scale)), C.c_float(tmin), C.c_float(tmax), _ptr(out))
+    return out
+
+
+def medium_sample(sigma_a, sigma_s, o, d, tmin, tmax, key, scale=1.0):
+    """Homogeneous.Sample: (is_medium_vertex, p, weight, t)."""
+    o = np.asarray(o, np.float32); d = np.asarray(d, np.float32)
+    p = np.zeros(3, np.float32); tr = np.zeros(3, np.float32); t = C.c_float(0)
+    r = lib().cr_medium_sample(C.byref(_medium(sigma_a, sigma_s, 0.0, scale)), _ptr(o), _ptr(d), C.c_float(tmin),
+                               C.c_float(tmax), C.c_uint64(key), _ptr(p), _ptr(tr), C.byref(t))
+    return bool(r), p, tr, t.value
+
+
+def hg_p(g, wo, wi):
+    wo = np.asarray(wo, np.float32); wi = np.asarray(wi, np.float32)
+    return float(lib().cr_hg_p(C.c_float(g), _ptr(wo), _ptr(wi)))
+
+
+def hg_sample(g, wo, key):
+    wo = np.asarray(wo, np.float32); wi = np.zeros(3, np.float32)
+    p = lib().cr_hg_sample(C.c_float(g), _ptr(wo), C.c_uint64(key), _ptr(wi))
+    return wi, float(p)
 
 
 def rng_floats(key, n):
