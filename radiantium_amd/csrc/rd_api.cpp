@@ -45,7 +45,10 @@ struct WaveBuffers {
     size_t pix_cap = 0;  // accumulation pixels
     std::vector<void*> allocs;
     RayQueue qa{}, qb{}, qshadow{}, qmis{};
+    RayQueue qshadow2{}, qmis2{};   // vol_path chain queues (second buffers)
     float4* hits = nullptr;
+    float4* hits2 = nullptr;        // vol_path: IntersectTr chain hits
+    bool vol = false;
     PathState ps{};
     float4* sum = nullptr;
     float4* sum2 = nullptr;
@@ -53,6 +56,7 @@ struct WaveBuffers {
         for (void* p : allocs) (void)hipFree(p);
         allocs.clear();
         cap = pix_cap = 0;
+        vol = false;
     }
 };
 
@@ -154,9 +158,13 @@ struct Timer {
     }
 };
 
-int ensure_wave_buffers(rd_ctx* c, size_t paths, size_t pixels) {
+// A Transmittance / IntersectTr chain takes one step per medium boundary it
+// crosses; this bounds the host loop for pathological geometry.
+constexpr int kMaxChainSteps = 1 << 16;
+
+int ensure_wave_buffers(rd_ctx* c, size_t paths, size_t pixels, bool vol) {
     WaveBuffers& w = c->wb;
-    if (paths <= w.cap && pixels <= w.pix_cap) return RD_OK;
+    if (paths <= w.cap && pixels <= w.pix_cap && (w.vol || !vol)) return RD_OK;
     w.release();
     size_t cap = std::max(paths, (size_t)1024);
     size_t pcap = std::max(pixels, (size_t)1024);
@@ -191,6 +199,18 @@ int ensure_wave_buffers(rd_ctx* c, size_t paths, size_t pixels) {
     w.sum2 = (float4*)alloc(pcap * sizeof(float4));
     ok = ok && w.hits && ps.L && ps.beta && ps.nee_beta && ps.nee_le && ps.mis_f && ps.mis_li && ps.flags &&
          ps.sel_light && ps.rng_n && ps.depth && ps.vis && w.sum && w.sum2;
+    ps.env = nullptr; ps.nee_li = ps.sh_to = ps.sh_tr = ps.mis_tr = nullptr;
+    if (ok && vol) {
+        ok = mkq(w.qshadow2, kCntShadowB) && mkq(w.qmis2, kCntMisB);
+        w.hits2 = (float4*)alloc(cap * sizeof(float4));
+        ps.env = (int*)alloc(cap * sizeof(int));
+        ps.nee_li = (float4*)alloc(cap * sizeof(float4));
+        ps.sh_to = (float4*)alloc(cap * sizeof(float4));
+        ps.sh_tr = (float4*)alloc(cap * sizeof(float4));
+        ps.mis_tr = (float4*)alloc(cap * sizeof(float4));
+        ok = ok && w.hits2 && ps.env && ps.nee_li && ps.sh_to && ps.sh_tr && ps.mis_tr;
+        w.vol = ok;
+    }
     if (!ok) {
         w.release();
         return fail(RD_ERR_OOM, "hipMalloc failed for wavefront buffers");
@@ -265,7 +285,15 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     if (d->num_spheres > 0 && !d->spheres) return bad("spheres without data");
     if (d->num_prims > 0 && !d->prims) return bad("prims missing");
     if (d->camera.width <= 0 || d->camera.height <= 0) return bad("camera screen must be positive");
-    if (d->integrator.type != RD_INTEGRATOR_PATH) return bad("integrator type not built for this path yet (vol_path)");
+    if (d->integrator.type != RD_INTEGRATOR_PATH && d->integrator.type != RD_INTEGRATOR_VOL_PATH)
+        return bad("unknown integrator type");
+    if (d->global_medium < -1 || d->global_medium >= d->num_media) return bad("global medium out of range");
+    for (int i = 0; i < d->num_prims; i++) {
+        const rd_prim_desc& pd = d->prims[i];
+        if (pd.medium_inside < -1 || pd.medium_inside >= d->num_media || pd.medium_outside < -1 ||
+            pd.medium_outside >= d->num_media)
+            return bad("primitive medium out of range");
+    }
     if (d->integrator.strategy != RD_STRATEGY_UNIFORM) return bad("light strategy All is not built for the GPU path yet");
     for (int t = 0; t < d->num_triangles; t++) {
         uint8_t fl = d->tri_flags ? d->tri_flags[t] : 0;
@@ -457,9 +485,12 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     std::vector<DevMedium> media(d->num_media);
     for (int i = 0; i < d->num_media; i++) {
         const rd_medium_desc& m = d->media[i];
-        for (int k = 0; k < 3; k++) { media[i].sigma_a[k] = m.sigma_a[k]; media[i].sigma_s[k] = m.sigma_s[k]; }
+        for (int k = 0; k < 3; k++) {
+            media[i].sa[k] = m.sigma_a[k] * m.scale;
+            media[i].ss[k] = m.sigma_s[k] * m.scale;
+            media[i].st[k] = media[i].sa[k] + media[i].ss[k];
+        }
         media[i].g = m.g;
-        media[i].scale = m.scale;
     }
 
     DevScene& ds = s->dev;
@@ -641,7 +672,8 @@ int rd_render_device(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d
     int spw = (int)std::max<size_t>(1, std::min<size_t>((size_t)p->spp, target / std::max<size_t>(npix, 1)));
     size_t cap = npix * (size_t)spw;
     if (cap > (size_t)INT32_MAX / 2) return fail(RD_ERR_INVALID, "wave too large");
-    if (int rc = ensure_wave_buffers(c, cap, npix)) return rc;
+    const bool vol = s->integ.type == RD_INTEGRATOR_VOL_PATH;
+    if (int rc = ensure_wave_buffers(c, cap, npix, vol)) return rc;
     WaveBuffers& wb = c->wb;
     hipStream_t st = c->stream;
     Timer tm(c);
@@ -668,7 +700,50 @@ int rd_render_device(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d
         RayQueue qin = wb.qa, qout = wb.qb;
         int cin = kCntExtA, cout = kCntExtB;
         HIP_TRY(tm.run("raygen", [&] { return launch_raygen(s->dev, wp, npaths, qin, wb.ps, c->counts, st); }));
-        for (int bounce = 0;; bounce++) {
+        for (int bounce = 0; vol; bounce++) {
+            // VolumetricPathTracer.Li, one bounce: extension query, shade, then the
+            // Transmittance / IntersectTr chains step by step until both queues drain.
+            HIP_TRY(tm.run("trace_closest", [&] {
+                return launch_trace_ext(s->dev, qin, wb.hits, c->counts, cin, cout, kCntShadow, kCntMis, st);
+            }));
+            HIP_TRY(tm.run("shade", [&] {
+                return launch_shade_vol(s->dev, s->integ, wp, qin, wb.hits, qout, wb.qshadow, wb.qmis, wb.ps,
+                                        c->counts, cin, cout, st);
+            }));
+            RayQueue sa = wb.qshadow, sb = wb.qshadow2, ma = wb.qmis, mb = wb.qmis2;
+            int csa = kCntShadow, csb = kCntShadowB, cma = kCntMis, cmb = kCntMisB;
+            for (int step = 0;; step++) {
+                if (step >= kMaxChainSteps) { rc = fail(RD_ERR_DEVICE, "medium chain did not terminate"); break; }
+                HIP_TRY(tm.run("trace_chain", [&] {
+                    return launch_trace_chain(s->dev, sa, wb.hits, false, c->counts, csa, csb, st);
+                }));
+                HIP_TRY(tm.run("trace_chain", [&] {
+                    return launch_trace_chain(s->dev, ma, wb.hits2, true, c->counts, cma, cmb, st);
+                }));
+                HIP_TRY(tm.run("chain_advance", [&] {
+                    return launch_chain_advance(s->dev, sa, wb.hits, sb, false, wb.ps, c->counts, csa, csb, st);
+                }));
+                HIP_TRY(tm.run("chain_advance", [&] {
+                    return launch_chain_advance(s->dev, ma, wb.hits2, mb, true, wb.ps, c->counts, cma, cmb, st);
+                }));
+                HIP_TRY(hipMemcpyAsync(c->host_counts + kCntShadowB, c->counts + kCntShadowB, 2 * sizeof(int),
+                                       hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipMemcpyAsync(c->host_counts + cout, c->counts + cout, sizeof(int), hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipStreamSynchronize(st));
+                int nsb = c->host_counts[csb], nmb = c->host_counts[cmb];
+                if (nsb == 0 && nmb == 0) break;
+                std::swap(sa, sb); std::swap(csa, csb);
+                std::swap(ma, mb); std::swap(cma, cmb);
+            }
+            if (rc) break;
+            HIP_TRY(tm.run("resolve", [&] { return launch_resolve_vol(qin, wb.ps, c->counts, cin, st); }));
+            if (bounce > max_bounce) max_bounce = bounce;
+            if (c->host_counts[cout] == 0) break;
+            if (cancel && *cancel) { rc = fail(RD_ERR_CANCELLED, "render cancelled"); break; }
+            std::swap(qin, qout);
+            std::swap(cin, cout);
+        }
+        for (int bounce = 0; !vol && !rc; bounce++) {
             HIP_TRY(tm.run("trace_closest", [&] {
                 return launch_trace_ext(s->dev, qin, wb.hits, c->counts, cin, cout, kCntShadow, kCntMis, st);
             }));

@@ -27,6 +27,12 @@ struct PathState {
     uint32_t* rng_n;
     int* depth;
     uint8_t* vis;
+    // vol_path only (VolumetricPathTracer.cs:28-314)
+    int* env;           // medium of the path's current ray (-1: vacuum)
+    float4* nee_li;     // light sample Li; w = light pdf (nee_le then holds f*|cos|, w = MIS weight)
+    float4* sh_to;      // Transmittance chain: light point; w = current medium (int bits)
+    float4* sh_tr;      // Transmittance chain: accumulated Tr
+    float4* mis_tr;     // IntersectTr chain: accumulated Tr; w = current medium (int bits)
 };
 
 struct WaveParams {
@@ -47,7 +53,8 @@ enum : int {
     kCntClassRays = 16,   // 3 x u64: extension, MIS, shadow
     kCntClassNodes = 22,  // 3 x u64
     kCntClassPrims = 28,  // 3 x u64
-    kCntSlots = 34
+    kCntShadowB = 34, kCntMisB = 35,   // vol_path chain queues, second buffers
+    kCntSlots = 36
 };
 
 struct Timing;  // per-stage HIP event timing (rd_api.cpp)
@@ -68,6 +75,20 @@ hipError_t launch_trace_shadow(const DevScene& sc, RayQueue q, PathState ps, int
 hipError_t launch_trace_mis(const DevScene& sc, RayQueue q, PathState ps, int* counts, hipStream_t s);
 hipError_t launch_resolve(const DevIntegrator& in, RayQueue qin, PathState ps, int* counts, int cnt_in,
                           hipStream_t s);
+// vol_path stages (VolumetricPathTracer.Li and its Scene.Transmittance /
+// Scene.IntersectTr chains): shade_vol draws every random number of the
+// bounce and starts one Transmittance chain (light sample) and one
+// IntersectTr chain (BSDF / phase sample) per vertex; each chain step is a
+// closest-hit query (trace_chain) followed by chain_advance, which either
+// finishes the chain into the path state or appends its next segment.
+hipError_t launch_shade_vol(const DevScene& sc, const DevIntegrator& in, const WaveParams& wp, RayQueue qin,
+                            const float4* hits, RayQueue qout, RayQueue qshadow, RayQueue qmis, PathState ps,
+                            int* counts, int cnt_in, int cnt_out, hipStream_t s);
+hipError_t launch_trace_chain(const DevScene& sc, RayQueue q, float4* hits, bool mis, int* counts, int cnt_slot,
+                              int zero_slot, hipStream_t s);
+hipError_t launch_chain_advance(const DevScene& sc, RayQueue qin, const float4* hits, RayQueue qout, bool mis,
+                                PathState ps, int* counts, int cnt_in, int cnt_out, hipStream_t s);
+hipError_t launch_resolve_vol(RayQueue qin, PathState ps, int* counts, int cnt_in, hipStream_t s);
 hipError_t launch_accumulate(const WaveParams& wp, PathState ps, float4* sum, float4* sum2, int* counts,
                              hipStream_t s);
 hipError_t launch_finalize(const WaveParams& wp, int height, int spp, const float4* sum, const float4* sum2,

@@ -15,52 +15,13 @@
 // counter, so the kernels reproduce the CPU restatement's draws exactly.
 #include "rd_kernels.h"
 #include "rd_shading.h"
+#include "rd_wave.h"
 
 namespace rd {
 
 static int g_num_cus = 256;
 void set_device_grid(int num_cus) { g_num_cus = num_cus > 0 ? num_cus : 256; }
 int persistent_blocks() { return g_num_cus * 8; }
-
-enum : int {
-    kFlagSpecular = 1,
-    kFlagNeePending = 2,
-    kFlagShadowIssued = 4,
-    kFlagMisIssued = 8,
-};
-
-// ---------------------------------------------------------------- wave helpers
-// Appends one slot per lane with `pred` set; one atomic per wavefront.
-// Must be reached by every lane of the wavefront (converged).
-__device__ __forceinline__ int wave_append(int* counter, bool pred) {
-    uint64_t mask = __ballot(pred);
-    if (mask == 0) return -1;
-    int lane = __lane_id();
-    int leader = __ffsll((unsigned long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
-    base = __shfl(base, leader);
-    uint64_t below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
-    return pred ? base + __popcll(below) : -1;
-}
-
-__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
-}
-
-__device__ __forceinline__ void add_counter(unsigned long long* c, uint64_t v) {
-    uint64_t s = wave_sum_u64(v);
-    if (__lane_id() == 0 && s) atomicAdd(c, (unsigned long long)s);
-}
-
-// Per query class (0 extension, 1 MIS, 2 shadow): rays, node visits and
-// primitive tests, for the algorithmic-bytes accounting of the trace kernels.
-__device__ __forceinline__ void add_class_counters(int* counts, int cls, uint64_t rays, uint64_t nv, uint64_t nt) {
-    add_counter((unsigned long long*)(counts + kCntClassRays) + cls, rays);
-    add_counter((unsigned long long*)(counts + kCntClassNodes) + cls, nv);
-    add_counter((unsigned long long*)(counts + kCntClassPrims) + cls, nt);
-}
 
 // ---------------------------------------------------------------- traversal
 // BoundingBox3F.Intersect (BoundingBox3F.cs:155-176) on a node stored as
@@ -92,7 +53,11 @@ struct Hit {
 // ray walks the reference's depth-first order (Bvh.cs:126-232) one node per
 // step, which keeps hits and visit counts identical to the oracle; the
 // schedules differ only in how rays are assigned to lanes.
-enum : int { kModeRaysClosest = 0, kModeRaysAny = 1, kModeExt = 2, kModeShadow = 3, kModeMis = 4 };
+enum : int {
+    kModeRaysClosest = 0, kModeRaysAny = 1, kModeExt = 2, kModeShadow = 3, kModeMis = 4,
+    kModeChainShadow = 5,   // vol_path Transmittance segment: closest hit into hits[]
+    kModeChainMis = 6       // vol_path IntersectTr segment: closest hit into hits[]
+};
 constexpr int kRefill = 16;
 
 struct TraceArgs {
@@ -196,7 +161,7 @@ __device__ __forceinline__ void trav_load(const TraceArgs& a, int r, float4* ra,
 template <int kMode>
 __device__ __forceinline__ void trav_finish(const DevScene& sc, const TraceArgs& a, int r, const Trav& t) {
     const bool found = t.best.prim >= 0;
-    if (kMode == kModeRaysClosest || kMode == kModeExt) {
+    if (kMode == kModeRaysClosest || kMode == kModeExt || kMode == kModeChainShadow || kMode == kModeChainMis) {
         a.hits[r] = found ? make_float4(t.best.t, __int_as_float(t.best.prim), t.best.u, t.best.v)
                           : make_float4(0.f, __int_as_float(-1), 0.f, 0.f);
     } else if (kMode == kModeRaysAny) {
@@ -233,7 +198,7 @@ __device__ __forceinline__ void trav_counters(const TraceArgs& a, uint32_t nrays
             add_counter(a.counters + 1, nt);
         }
     } else {
-        constexpr int cls = kMode == kModeExt ? 0 : (kMode == kModeMis ? 1 : 2);
+        constexpr int cls = kMode == kModeExt ? 0 : ((kMode == kModeMis || kMode == kModeChainMis) ? 1 : 2);
         add_class_counters(a.counts, cls, nrays, nv, nt);
         add_counter((unsigned long long*)(a.counts + (kAny ? kCntRaysAny : kCntRaysClosest)), nrays);
     }
@@ -336,7 +301,9 @@ template <int kMode>
 static hipError_t launch_trace_mode(const DevScene& sc, const TraceArgs& a, int blocks_hint, hipStream_t s) {
     int need = sc.stack_needed;
     int pb = persistent_blocks();
-    int cls = kMode == kModeExt ? 0 : (kMode == kModeShadow ? 1 : (kMode == kModeMis ? 2 : 0));
+    int cls = kMode == kModeExt ? 0
+              : ((kMode == kModeShadow || kMode == kModeChainShadow) ? 1
+                 : ((kMode == kModeMis || kMode == kModeChainMis) ? 2 : 0));
     int sched = kMode <= kModeRaysAny ? 0 : g_schedule[cls];
     if (need <= 24) {
         int b = blocks_hint > 0 ? std::min(blocks_hint, pb) : pb;
@@ -376,22 +343,6 @@ hipError_t launch_trace_rays(const DevScene& sc, const float4* rays, int n, floa
     return any ? launch_trace_mode<kModeRaysAny>(sc, a, blocks, s) : launch_trace_mode<kModeRaysClosest>(sc, a, blocks, s);
 }
 // ---------------------------------------------------------------- wavefront
-__device__ __forceinline__ void wave_pixel(const WaveParams& wp, int p, int* x, int* y, int* sample) {
-    int s = p / wp.npix;
-    int q = p - s * wp.npix;
-    int yy = q / wp.crop_w;
-    *x = wp.x0 + (q - yy * wp.crop_w);
-    *y = wp.y0 + yy;
-    *sample = wp.sample_base + s;
-}
-__device__ __forceinline__ uint64_t wave_key(const WaveParams& wp, int p) {
-    int x, y, s;
-    wave_pixel(wp, p, &x, &y, &s);
-    return path_key(wp.seed, (uint32_t)(y * wp.width + x), (uint32_t)s);
-}
-__device__ __forceinline__ f3 ld3(const float4* a, int i) { float4 v = a[i]; return mk(v.x, v.y, v.z); }
-__device__ __forceinline__ void st3(float4* a, int i, f3 v, float w = 0.0f) { a[i] = make_float4(v.x, v.y, v.z, w); }
-
 // Vector3.Transform / TransformNormal with a row-major 4x4 (row-vector convention).
 __device__ __forceinline__ f3 xform_point(const float* M, f3 v) {
     return mk(v.x * M[0] + v.y * M[4] + v.z * M[8] + M[12], v.x * M[1] + v.y * M[5] + v.z * M[9] + M[13],
@@ -427,6 +378,7 @@ __global__ __launch_bounds__(256) void k_raygen(DevScene sc, WaveParams wp, int 
         ps.flags[p] = kFlagSpecular;
         ps.rng_n[p] = rng.n;
         ps.depth[p] = 0;
+        if (ps.env) ps.env[p] = sc.global_medium;   // vol_path: rayEnv = Scene.GlobalMedium
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) counts[kCntExtA] = npaths;
 }
@@ -462,6 +414,16 @@ hipError_t launch_trace_mis(const DevScene& sc, RayQueue q, PathState ps, int* c
     a.q = q; a.ps = ps; a.cnt_slot = kCntMis; a.cursor_slot = kCntWork + 2;
     return launch_trace_mode<kModeMis>(sc, a, 0, s);
 }
+// One closest-hit step of the vol_path Transmittance (mis = false) or
+// IntersectTr (mis = true) chains; clears the next chain queue's counter.
+hipError_t launch_trace_chain(const DevScene& sc, RayQueue q, float4* hits, bool mis, int* counts, int cnt_slot,
+                              int zero_slot, hipStream_t s) {
+    TraceArgs a = trace_args(counts);
+    a.q = q; a.hits = hits; a.cnt_slot = cnt_slot; a.cursor_slot = kCntWork + (mis ? 2 : 1);
+    a.zero[0] = zero_slot;
+    return mis ? launch_trace_mode<kModeChainMis>(sc, a, 0, s) : launch_trace_mode<kModeChainShadow>(sc, a, 0, s);
+}
+
 // One bounce of PathTracer.Mis / NextEventEstimation / OnlySampleBxdf for
 // every path in the extension queue (PathTracer.cs:43-270).
 __global__ __launch_bounds__(256) void k_shade(DevScene sc, DevIntegrator in, WaveParams wp, RayQueue qin,

@@ -1089,6 +1089,86 @@ __device__ __forceinline__ float light_pdf(const DevScene& sc, const DevLight& L
     return pdf;
 }
 
+// ---------------------------------------------------------------- media
+// Homogeneous.Tr (Mediums/Homogeneous.cs:22-30): exp(sigma_t * (MinT - MaxT)).
+__device__ __forceinline__ f3 medium_tr(const DevMedium& m, float tmin, float tmax) {
+    float neg = tmin - tmax;
+    return mk(m.st[0] == 0 ? 1.0f : expf(m.st[0] * neg),
+              m.st[1] == 0 ? 1.0f : expf(m.st[1] * neg),
+              m.st[2] == 0 ? 1.0f : expf(m.st[2] * neg));
+}
+
+struct MSample {
+    bool medium;
+    f3 p, tr;
+    float t;
+};
+// Homogeneous.Sample (:32-61): free flight on a uniformly chosen channel;
+// a medium vertex carries tr * sigma_s / pdf, a pass tr / pdf.
+__device__ __forceinline__ MSample medium_sample(const DevMedium& m, f3 o, f3 d, float tmin, float tmax, Rng& rng) {
+    MSample out;
+    int ch = rng.next_int(3);
+    float dens = ch == 0 ? m.st[0] : (ch == 1 ? m.st[1] : m.st[2]);
+    float sd = -logf(1 - rng.next_float()) / dens;
+    float dist = tmax - tmin;
+    if (sd < dist) {
+        float t = tmin + sd;
+        f3 tr = mk(expf(-m.st[0] * sd), expf(-m.st[1] * sd), expf(-m.st[2] * sd));
+        float pdf = (tr.x * m.st[0] + tr.y * m.st[1] + tr.z * m.st[2]) / 3;
+        if (pdf == 0) pdf = 0.0001f;
+        out.medium = true;
+        out.p = o + t * d;
+        out.tr = tr * mk(m.ss[0], m.ss[1], m.ss[2]) / pdf;
+        out.t = t;
+    } else {
+        f3 tr = mk(expf(-m.st[0] * dist), expf(-m.st[1] * dist), expf(-m.st[2] * dist));
+        float pdf = (tr.x + tr.y + tr.z) / 3;
+        if (pdf == 0) pdf = 0.0001f;
+        out.medium = false;
+        out.p = mk(0.0f);
+        out.tr = tr / pdf;
+        out.t = 0.0f;
+    }
+    return out;
+}
+
+// PhaseFunctionUtility.HenyeyGreenstein (IPhaseFunction.cs:39-44)
+__device__ __forceinline__ float hg_eval(float cos_t, float g) {
+    float denom = 1 + g * g + 2 * g * cos_t;
+    return (1 / (4 * kPi)) * (1 - g * g) / (denom * sqrtf(denom));
+}
+__device__ __forceinline__ float hg_g(const DevMedium& m) { return nclamp(m.g, -0.999f, 0.999f); }   // :51-54
+
+struct PSample {
+    float p;
+    f3 wi;
+};
+// HenyeyGreenstein.SampleWi (:62-81), directions around wo's Duff frame.
+__device__ __forceinline__ PSample hg_sample(const DevMedium& m, f3 wo, Rng& rng) {
+    float G = hg_g(m);
+    float x = rng.next_float(), y = rng.next_float();
+    float cos_t;
+    if (fabsf(G) < 0.001f) {
+        cos_t = 1 - 2 * x;
+    } else {
+        float sqr = (1 - G * G) / (1 + G - 2 * G * x);
+        cos_t = -(1 + G * G - sqr * sqr) / (2 * G);
+    }
+    float sin_t = sqrtf(nmax(0.0f, 1 - cos_t * cos_t));
+    float phi = 2 * kPi * y;
+    Frame f = make_frame(wo);
+    float sp = sinf(phi), cp = cosf(phi);
+    PSample out;
+    out.wi = sin_t * cp * f.X + sin_t * sp * f.Y + cos_t * f.Z;   // MathExt.SphericalDirection
+    out.p = hg_eval(cos_t, G);
+    return out;
+}
+
+// Intersection.GetMedium (Intersection.cs:68-71): the side of the shading normal.
+__device__ __forceinline__ int get_medium(const DevPrim& pr, f3 n, f3 w) {
+    return dot(w, n) > 0 ? pr.med_out : pr.med_in;
+}
+
 // PathUtility.PowerHeuristic (Integrators/PathUtility.cs:5-9), in double.
 __device__ __forceinline__ float power_heuristic(float fp, float gp) {
     double f = (double)fp, g = (double)gp;

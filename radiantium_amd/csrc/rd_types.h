@@ -64,8 +64,10 @@ struct DevEnv {
     int _pad2;
 };
 
+// Homogeneous (Mediums/Homogeneous.cs:14-20), coefficients already scaled:
+// sa = sigma_a * scale, ss = sigma_s * scale, st = sa + ss.
 struct DevMedium {
-    float sigma_a[3], sigma_s[3], g, scale;
+    float sa[3], ss[3], st[3], g;
 };
 
 // Per original primitive (rd_prim_desc order): shading record.

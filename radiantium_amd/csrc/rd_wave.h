@@ -1,0 +1,66 @@
+// rd_wave.h — device helpers shared by the wavefront kernels: path flags,
+// wavefront-aggregated queue appends and counters, and the wave's
+// path-index -> (pixel, sample) mapping.
+#pragma once
+#include "rd_kernels.h"
+#include "rd_math.h"
+
+namespace rd {
+
+enum : int {
+    kFlagSpecular = 1,
+    kFlagNeePending = 2,
+    kFlagShadowIssued = 4,
+    kFlagMisIssued = 8,
+};
+
+// ---------------------------------------------------------------- wave helpers
+// Appends one slot per lane with `pred` set; one atomic per wavefront.
+// Must be reached by every lane of the wavefront (converged).
+__device__ __forceinline__ int wave_append(int* counter, bool pred) {
+    uint64_t mask = __ballot(pred);
+    if (mask == 0) return -1;
+    int lane = __lane_id();
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+    uint64_t below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+    return pred ? base + __popcll(below) : -1;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__device__ __forceinline__ void add_counter(unsigned long long* c, uint64_t v) {
+    uint64_t s = wave_sum_u64(v);
+    if (__lane_id() == 0 && s) atomicAdd(c, (unsigned long long)s);
+}
+
+// Per query class (0 extension, 1 MIS, 2 shadow): rays, node visits and
+// primitive tests, for the algorithmic-bytes accounting of the trace kernels.
+__device__ __forceinline__ void add_class_counters(int* counts, int cls, uint64_t rays, uint64_t nv, uint64_t nt) {
+    add_counter((unsigned long long*)(counts + kCntClassRays) + cls, rays);
+    add_counter((unsigned long long*)(counts + kCntClassNodes) + cls, nv);
+    add_counter((unsigned long long*)(counts + kCntClassPrims) + cls, nt);
+}
+
+__device__ __forceinline__ void wave_pixel(const WaveParams& wp, int p, int* x, int* y, int* sample) {
+    int s = p / wp.npix;
+    int q = p - s * wp.npix;
+    int yy = q / wp.crop_w;
+    *x = wp.x0 + (q - yy * wp.crop_w);
+    *y = wp.y0 + yy;
+    *sample = wp.sample_base + s;
+}
+__device__ __forceinline__ uint64_t wave_key(const WaveParams& wp, int p) {
+    int x, y, s;
+    wave_pixel(wp, p, &x, &y, &s);
+    return path_key(wp.seed, (uint32_t)(y * wp.width + x), (uint32_t)s);
+}
+__device__ __forceinline__ f3 ld3(const float4* a, int i) { float4 v = a[i]; return mk(v.x, v.y, v.z); }
+__device__ __forceinline__ void st3(float4* a, int i, f3 v, float w = 0.0f) { a[i] = make_float4(v.x, v.y, v.z, w); }
+
+}  // namespace rd

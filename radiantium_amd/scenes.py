@@ -201,6 +201,49 @@ def env_showcase(width=64, height=64, max_depth=8, method="Mis", map_w=64, map_h
     return s
 
 
+def vol_cornell(width=64, height=64, max_depth=64, sigma_a=0.05, sigma_s=2.0, g=0.5, radius=0.22):
+    """Volumetric Cornell box: the Cornell box of cornell_box() with a
+    perfect_glass sphere whose inside holds a homogeneous medium (grey
+    sigma_a, sigma_s, Henyey-Greenstein g), rendered by vol_path."""
+    s = cornell_box(width, height, max_depth=max_depth)
+    glass = s.add_material("perfect_glass")
+    m = s.add_medium(sigma_a=(sigma_a,) * 3, sigma_s=(sigma_s,) * 3, g=g)
+    s.add_sphere((0.05, 0.55, 0.1), radius, material=glass, medium_inside=m)
+    s.set_integrator(type="vol_path", max_depth=max_depth)
+    return s
+
+
+def vol_fog(width=64, height=64, max_depth=32, sigma_a=(0.01, 0.015, 0.02), sigma_s=(0.1, 0.08, 0.06), g=0.3):
+    """Spectral fog everywhere (global medium) over material_box-like objects,
+    lit by an HDR sky and a quad light, plus a medium slab bounded by a
+    surface-less box (a medium boundary the chains must cross)."""
+    from .scene import Texture
+    s = SceneDesc()
+    ground = s.add_material("diffuse", kd=(0.6, 0.6, 0.6))
+    plastic = s.add_material("rough_plastic", r=(0.7, 0.3, 0.2), roughness=0.3, dist="GGX")
+    metal = s.add_material("rough_metal", roughness=0.25)
+    black = s.add_material("diffuse", kd=(0.0, 0.0, 0.0))
+    fog = s.add_medium(sigma_a=sigma_a, sigma_s=sigma_s, g=g)
+    slab = s.add_medium(sigma_a=(0.4, 0.2, 0.1), sigma_s=(1.0, 1.5, 2.0), g=-0.2)
+    W = 4.0
+    s.add_mesh(*_quad((-W, 0, -W), (-W, 0, W), (W, 0, W), (W, 0, -W)), material=ground,
+               medium_outside=fog, medium_inside=fog)
+    s.add_sphere((-0.6, 0.4, 0.0), 0.4, material=plastic, medium_outside=fog)
+    s.add_sphere((0.6, 0.4, -0.2), 0.4, material=metal, medium_outside=fog)
+    # medium slab: triangles without a surface, slab medium inside, fog outside
+    s.add_mesh(*_box((0.6, 0.5, 0.3), (0.0, 0.0, 0.8), 10.0), material=-1, medium_inside=slab, medium_outside=fog)
+    q = 0.3
+    s.add_mesh(*_quad((-q, 2.2, -q), (q, 2.2, -q), (q, 2.2, q), (-q, 2.2, q)), material=black, le=(40.0, 36.0, 30.0),
+               medium_outside=fog, medium_inside=fog)
+    s.add_infinite_light(Texture(image=hdr_sky(32, 16, seed=3, sigma=0.8, sun_radiance=50.0)))
+    s.global_medium = fog
+    s.set_camera(fov=45.0, width=width, height=height, origin=(0.0, 1.2, 3.2), target=(0.0, 0.35, 0.0),
+                 up=(0.0, 1.0, 0.0))
+    s.set_integrator(type="vol_path", max_depth=max_depth)
+    s.bvh_max_prim = 1
+    return s
+
+
 # ---------------------------------------------------------------- icosphere
 def icosphere(level):
     """Unit icosphere: 20*4^level faces, midpoint subdivision with shared

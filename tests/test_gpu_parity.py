@@ -195,6 +195,36 @@ def test_render_env_statistical(ctx, oracle):
     assert np.mean(np.abs(z) > 3) < 0.02
 
 
+@pytest.mark.parametrize("which", ["vol_cornell", "vol_fog"])
+def test_render_twin_vol(ctx, oracle, which):
+    """vol_path: a perfect-glass sphere holding a homogeneous HG medium in the
+    Cornell box; spectral global fog with a surface-less medium slab, an HDR
+    sky and a quad light (Transmittance / IntersectTr chains across
+    boundaries)."""
+    desc = scenes.vol_cornell(32, 32) if which == "vol_cornell" else scenes.vol_fog(32, 32)
+    print(f"[render twin] {which} 32x32 spp=8", flush=True)
+    g, g2, gst, o, o2, ost = _render_pair(ctx, oracle, desc, 8, seed=21)
+    assert np.isfinite(g).all()
+    for k in ("rays_closest", "vertices"):
+        assert abs(gst[k] - ost[k]) <= max(16, 0.02 * ost[k]), (k, gst[k], ost[k])
+    assert gst["rays_any"] == 0 and ost["rays_any"] == 0
+    close = np.isclose(g, o, rtol=1e-3, atol=1e-5)
+    assert close.mean() > 0.95, close.mean()
+    z = z_test(g, g2, 8, o, o2, 8)
+    assert np.mean(np.abs(z) < 4) > 0.99
+
+
+def test_render_vol_statistical(ctx, oracle):
+    """vol_path with independent seeds: per-pixel z-test at 3 sigma."""
+    desc = scenes.vol_cornell(24, 24)
+    gs = Scene(ctx, desc)
+    os_ = oracle.OracleScene(desc)
+    g, g2 = gs.render(128, seed=100)
+    o, o2, _, _ = os_.render(128, seed=200)
+    z = z_test(g, g2, 128, o, o2, 128)
+    assert np.mean(np.abs(z) > 3) < 0.02
+
+
 def test_render_twin_icosphere(ctx, oracle):
     """GGX conductor + rough dielectric (the 1M-triangle workload's materials) at level 5."""
     desc = scenes.displaced_icosphere_scene(level=5, width=96, height=54)
