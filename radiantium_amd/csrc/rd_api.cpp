@@ -72,6 +72,11 @@ struct rd_ctx {
     rd_render_stats stats{};
     std::map<std::string, StageTiming> timing;
     bool timing_enabled = true;
+    // scenes created on this context and not yet destroyed; rd_destroy with
+    // live scenes releases the wavefront buffers and leaves the rest to the
+    // last rd_scene_destroy, so destruction order does not matter
+    int live_scenes = 0;
+    bool closed = false;
 };
 
 struct rd_scene {
@@ -85,6 +90,7 @@ struct rd_scene {
     int width = 0, height = 0;
     ~rd_scene() {
         for (void* p : allocs) (void)hipFree(p);
+        if (ctx) ctx->live_scenes--;
     }
 };
 
@@ -263,15 +269,20 @@ rd_ctx* rd_create(int device) {
     return c;
 }
 
+static void free_ctx(rd_ctx* c) {
+    if (c->counts) (void)hipFree(c->counts);
+    if (c->host_counts) (void)hipHostFree(c->host_counts);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
 void rd_destroy(rd_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     c->wb.release();
-    if (c->counts) (void)hipFree(c->counts);
-    if (c->host_counts) (void)hipHostFree(c->host_counts);
-    (void)hipStreamDestroy(c->stream);
-    delete c;
+    c->closed = true;
+    if (c->live_scenes == 0) free_ctx(c);
 }
 
 rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
@@ -334,8 +345,10 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
         if (l.prim < 0 || l.prim >= d->num_prims) return bad("light primitive out of range");
     }
 
+    if (c->closed) { g_err = "context destroyed"; return nullptr; }
     rd_scene* s = new rd_scene();
     s->ctx = c;
+    c->live_scenes++;
     s->width = d->camera.width;
     s->height = d->camera.height;
     auto t0 = std::chrono::steady_clock::now();
@@ -546,9 +559,11 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
 
 void rd_scene_destroy(rd_scene* s) {
     if (!s) return;
-    (void)hipSetDevice(s->ctx->device);
-    (void)hipStreamSynchronize(s->ctx->stream);
+    rd_ctx* c = s->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
     delete s;
+    if (c->live_scenes == 0 && c->closed) free_ctx(c);
 }
 
 int rd_scene_get_info(const rd_scene* s, rd_scene_info* info) {
