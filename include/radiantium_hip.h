@@ -284,6 +284,30 @@ int rd_render_device(rd_ctx* ctx, rd_scene* scene, const rd_render_params* param
                      volatile int32_t* progress);
 int rd_get_render_stats(const rd_ctx* ctx, rd_render_stats* stats);
 
+/* ---- batched rendering: many scenes / sample counts in shared waves ----- */
+/* One request: a scene of this context, its render parameters (spp, seed,
+ * crop; wave_paths and flags are ignored) and its output buffers (layout as
+ * rd_render; out_sum2 may be NULL).  The reference renders one scene per
+ * BlockBasedRenderer (Renderers/BlockBasedRenderer.cs:56-83); a batch is
+ * the per-request Start() of many renderers fused into one launch sequence. */
+typedef struct rd_request {
+    rd_scene* scene;
+    rd_render_params params;
+    float* out_rgb;
+    float* out_sum2;
+} rd_request;
+
+/* Renders every request: paths of all requests share waves of wave_paths
+ * paths (0 = default), each path traversing and shading its own request's
+ * scene.  Each request's image equals what rd_render gives for it alone
+ * (same random stream, same per-pixel sample order).  Requests are grouped by
+ * integrator kind (path method, vol_path).  Host buffers; rd_render_batch_device
+ * takes device buffers.  stats: sums over the batch. */
+int rd_render_batch(rd_ctx* ctx, int32_t n, const rd_request* reqs, int32_t wave_paths,
+                    volatile int32_t* cancel, volatile int32_t* progress);
+int rd_render_batch_device(rd_ctx* ctx, int32_t n, const rd_request* reqs, int32_t wave_paths,
+                           volatile int32_t* cancel, volatile int32_t* progress);
+
 /* ---- device memory helpers (so hosts need no GPU framework) ------------- */
 void* rd_device_alloc(rd_ctx* ctx, uint64_t bytes);
 int rd_device_free(rd_ctx* ctx, void* ptr);

@@ -131,6 +131,13 @@ class rd_render_params(C.Structure):
     ]
 
 
+class rd_request(C.Structure):
+    _fields_ = [
+        ("scene", C.c_void_p), ("params", rd_render_params),
+        ("out_rgb", C.c_void_p), ("out_sum2", C.c_void_p),
+    ]
+
+
 class rd_render_stats(C.Structure):
     _fields_ = [
         ("rays_closest", C.c_uint64), ("rays_any", C.c_uint64), ("paths", C.c_uint64),

@@ -198,13 +198,15 @@ int ensure_wave_buffers(rd_ctx* c, size_t paths, size_t pixels, bool vol) {
     ps.mis_li = (float4*)alloc(cap * sizeof(float4));
     ps.flags = (int*)alloc(cap * sizeof(int));
     ps.sel_light = (int*)alloc(cap * sizeof(int));
+    ps.key = (uint64_t*)alloc(cap * sizeof(uint64_t));
+    ps.req = (int*)alloc(cap * sizeof(int));
     ps.rng_n = (uint32_t*)alloc(cap * sizeof(uint32_t));
     ps.depth = (int*)alloc(cap * sizeof(int));
     ps.vis = (uint8_t*)alloc(cap);
     w.sum = (float4*)alloc(pcap * sizeof(float4));
     w.sum2 = (float4*)alloc(pcap * sizeof(float4));
     ok = ok && w.hits && ps.L && ps.beta && ps.nee_beta && ps.nee_le && ps.mis_f && ps.mis_li && ps.flags &&
-         ps.sel_light && ps.rng_n && ps.depth && ps.vis && w.sum && w.sum2;
+         ps.sel_light && ps.key && ps.req && ps.rng_n && ps.depth && ps.vis && w.sum && w.sum2;
     ps.env = nullptr; ps.nee_li = ps.sh_to = ps.sh_tr = ps.mis_tr = nullptr;
     if (ok && vol) {
         ok = mkq(w.qshadow2, kCntShadowB) && mkq(w.qmis2, kCntMisB);
@@ -539,6 +541,12 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     if (rc) { delete s; return nullptr; }
 
     const rd_integrator_desc& I = d->integrator;
+    ds.integ.type = I.type;
+    ds.integ.max_depth = I.max_depth;
+    ds.integ.min_depth = I.min_depth;
+    ds.integ.method = I.method;
+    ds.integ.strategy = I.strategy;
+    ds.integ.rr_threshold = I.rr_threshold;
     s->integ.type = I.type;
     s->integ.max_depth = I.max_depth;
     s->integ.min_depth = I.min_depth;
@@ -671,16 +679,132 @@ int rd_trace_any(rd_ctx* c, rd_scene* s, const rd_ray* rays, int32_t n, uint8_t*
     return rc;
 }
 
+namespace {
+
+// What one wave's bounce loop launches: a single scene (kernel argument) or a
+// batch (device array of scenes, the path's request selects one); `sc` is
+// also the traversal variant's reference for the LDS stack depth.
+struct WaveLaunch {
+    DevScene sc;
+    const DevScene* scenes;
+    int method;
+    bool vol;
+};
+
+// The bounce loop of one wave, after ray generation: per bounce one sync
+// reads the next queue size (and, for vol_path, the chain queue sizes).
+int run_bounces(rd_ctx* c, Timer& tm, const WaveLaunch& L, int* max_bounce, volatile int32_t* cancel) {
+    WaveBuffers& wb = c->wb;
+    hipStream_t st = c->stream;
+    RayQueue qin = wb.qa, qout = wb.qb;
+    int cin = kCntExtA, cout = kCntExtB;
+    int rc = RD_OK;
+    for (int bounce = 0; !rc; bounce++) {
+        HIP_TRY(tm.run("trace_closest", [&] {
+            return launch_trace_ext(L.sc, L.scenes, qin, wb.ps, wb.hits, c->counts, cin, cout, kCntShadow, kCntMis, st);
+        }));
+        if (L.vol) {
+            // VolumetricPathTracer.Li: shade, then the Transmittance / IntersectTr
+            // chains step by step until both chain queues drain
+            HIP_TRY(tm.run("shade", [&] {
+                return launch_shade_vol(L.sc, L.scenes, qin, wb.hits, qout, wb.qshadow, wb.qmis, wb.ps, c->counts,
+                                        cin, cout, st);
+            }));
+            RayQueue sa = wb.qshadow, sb = wb.qshadow2, ma = wb.qmis, mb = wb.qmis2;
+            int csa = kCntShadow, csb = kCntShadowB, cma = kCntMis, cmb = kCntMisB;
+            for (int step = 0;; step++) {
+                if (step >= kMaxChainSteps) return fail(RD_ERR_DEVICE, "medium chain did not terminate");
+                HIP_TRY(tm.run("trace_chain", [&] {
+                    return launch_trace_chain(L.sc, L.scenes, sa, wb.ps, wb.hits, false, c->counts, csa, csb, st);
+                }));
+                HIP_TRY(tm.run("trace_chain", [&] {
+                    return launch_trace_chain(L.sc, L.scenes, ma, wb.ps, wb.hits2, true, c->counts, cma, cmb, st);
+                }));
+                HIP_TRY(tm.run("chain_advance", [&] {
+                    return launch_chain_advance(L.sc, L.scenes, sa, wb.hits, sb, false, wb.ps, c->counts, csa, csb, st);
+                }));
+                HIP_TRY(tm.run("chain_advance", [&] {
+                    return launch_chain_advance(L.sc, L.scenes, ma, wb.hits2, mb, true, wb.ps, c->counts, cma, cmb, st);
+                }));
+                HIP_TRY(hipMemcpyAsync(c->host_counts + kCntShadowB, c->counts + kCntShadowB, 2 * sizeof(int),
+                                       hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipMemcpyAsync(c->host_counts + kCntShadow, c->counts + kCntShadow, 2 * sizeof(int),
+                                       hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipMemcpyAsync(c->host_counts + cout, c->counts + cout, sizeof(int), hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipStreamSynchronize(st));
+                if (c->host_counts[csb] == 0 && c->host_counts[cmb] == 0) break;
+                std::swap(sa, sb); std::swap(csa, csb);
+                std::swap(ma, mb); std::swap(cma, cmb);
+            }
+            HIP_TRY(tm.run("resolve", [&] { return launch_resolve_vol(qin, wb.ps, c->counts, cin, st); }));
+        } else {
+            HIP_TRY(tm.run("shade", [&] {
+                return launch_shade(L.sc, L.scenes, qin, wb.hits, qout, wb.qshadow, wb.qmis, wb.ps, c->counts, cin,
+                                    cout, st);
+            }));
+            if (L.method != RD_METHOD_BXDF) {
+                HIP_TRY(tm.run("trace_any", [&] {
+                    return launch_trace_shadow(L.sc, L.scenes, wb.qshadow, wb.ps, c->counts, st);
+                }));
+                if (L.method == RD_METHOD_MIS)
+                    HIP_TRY(tm.run("trace_mis", [&] {
+                        return launch_trace_mis(L.sc, L.scenes, wb.qmis, wb.ps, c->counts, st);
+                    }));
+                HIP_TRY(tm.run("resolve", [&] { return launch_resolve(L.method, qin, wb.ps, c->counts, cin, st); }));
+            }
+            HIP_TRY(hipMemcpyAsync(c->host_counts + cout, c->counts + cout, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+        }
+        if (bounce > *max_bounce) *max_bounce = bounce;
+        if (c->host_counts[cout] == 0) break;
+        if (cancel && *cancel) rc = fail(RD_ERR_CANCELLED, "render cancelled");
+        std::swap(qin, qout);
+        std::swap(cin, cout);
+    }
+    return rc;
+}
+
+// Device counters -> rd_render_stats after the render's final synchronize.
+void collect_stats(rd_ctx* c, uint64_t paths, float ms, int waves, int max_bounce) {
+    auto u64 = [&](int slot) { uint64_t v; std::memcpy(&v, c->host_counts + slot, 8); return v; };
+    c->stats.rays_closest = u64(kCntRaysClosest);
+    c->stats.rays_any = u64(kCntRaysAny);
+    c->stats.vertices = u64(kCntVertices);
+    c->stats.invalid = u64(kCntInvalid);
+    c->stats.paths = paths;
+    c->stats.render_ms = ms;
+    c->stats.waves = waves;
+    c->stats.bounces = max_bounce;
+    for (int k = 0; k < 3; k++) {
+        c->stats.class_rays[k] = u64(kCntClassRays + 2 * k);
+        c->stats.class_nodes[k] = u64(kCntClassNodes + 2 * k);
+        c->stats.class_prims[k] = u64(kCntClassPrims + 2 * k);
+    }
+}
+
+struct Crop {
+    int x0, y0, x1, y1;
+};
+int resolve_crop(const rd_scene* s, const rd_render_params* p, Crop* out) {
+    int x0 = p->x0, y0 = p->y0, x1 = p->x1, y1 = p->y1;
+    if (x0 == 0 && y0 == 0 && x1 == 0 && y1 == 0) { x1 = s->width; y1 = s->height; }
+    if (x0 < 0 || y0 < 0 || x1 > s->width || y1 > s->height || x0 >= x1 || y0 >= y1)
+        return fail(RD_ERR_INVALID, "bad crop");
+    *out = Crop{x0, y0, x1, y1};
+    return RD_OK;
+}
+
+}  // namespace
+
 int rd_render_device(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d_out_rgb, float* d_out_sum2,
                      volatile int32_t* cancel, volatile int32_t* progress) {
     if (!c || !s || !p || !d_out_rgb) return fail(RD_ERR_INVALID, "null argument");
     if (p->spp <= 0) return fail(RD_ERR_INVALID, "spp must be positive");
     if (int rc = check_stack(s)) return rc;
-    const int W = s->width, H = s->height;
-    int x0 = p->x0, y0 = p->y0, x1 = p->x1, y1 = p->y1;
-    if (x0 == 0 && y0 == 0 && x1 == 0 && y1 == 0) { x1 = W; y1 = H; }
-    if (x0 < 0 || y0 < 0 || x1 > W || y1 > H || x0 >= x1 || y0 >= y1) return fail(RD_ERR_INVALID, "bad crop");
-    const int cw = x1 - x0, ch = y1 - y0;
+    const int H = s->height;
+    Crop cr;
+    if (int rc = resolve_crop(s, p, &cr)) return rc;
+    const int cw = cr.x1 - cr.x0, ch = cr.y1 - cr.y0;
     const size_t npix = (size_t)cw * ch;
     // Paths in flight per wave: whole samples of every crop pixel.
     size_t target = p->wave_paths > 0 ? (size_t)p->wave_paths : ((size_t)1 << 22);
@@ -701,10 +825,11 @@ int rd_render_device(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d
     HIP_TRY(hipEventCreate(&ev1));
     HIP_TRY(hipEventRecord(ev0, st));
     WaveParams wp;
-    wp.x0 = x0; wp.y0 = y0; wp.crop_w = cw; wp.crop_h = ch;
+    wp.x0 = cr.x0; wp.y0 = cr.y0; wp.crop_w = cw; wp.crop_h = ch;
     wp.npix = (int)npix;
-    wp.width = W;
+    wp.width = s->width;
     wp.seed = p->seed;
+    WaveLaunch L{s->dev, nullptr, s->integ.method, vol};
     int waves = 0, max_bounce = 0;
     int rc = RD_OK;
     for (int sb = 0; sb < p->spp && !rc; sb += spw) {
@@ -712,74 +837,8 @@ int rd_render_device(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d
         wp.sample_base = sb;
         wp.nsamples = std::min(spw, p->spp - sb);
         int npaths = (int)(npix * wp.nsamples);
-        RayQueue qin = wb.qa, qout = wb.qb;
-        int cin = kCntExtA, cout = kCntExtB;
-        HIP_TRY(tm.run("raygen", [&] { return launch_raygen(s->dev, wp, npaths, qin, wb.ps, c->counts, st); }));
-        for (int bounce = 0; vol; bounce++) {
-            // VolumetricPathTracer.Li, one bounce: extension query, shade, then the
-            // Transmittance / IntersectTr chains step by step until both queues drain.
-            HIP_TRY(tm.run("trace_closest", [&] {
-                return launch_trace_ext(s->dev, qin, wb.hits, c->counts, cin, cout, kCntShadow, kCntMis, st);
-            }));
-            HIP_TRY(tm.run("shade", [&] {
-                return launch_shade_vol(s->dev, s->integ, wp, qin, wb.hits, qout, wb.qshadow, wb.qmis, wb.ps,
-                                        c->counts, cin, cout, st);
-            }));
-            RayQueue sa = wb.qshadow, sb = wb.qshadow2, ma = wb.qmis, mb = wb.qmis2;
-            int csa = kCntShadow, csb = kCntShadowB, cma = kCntMis, cmb = kCntMisB;
-            for (int step = 0;; step++) {
-                if (step >= kMaxChainSteps) { rc = fail(RD_ERR_DEVICE, "medium chain did not terminate"); break; }
-                HIP_TRY(tm.run("trace_chain", [&] {
-                    return launch_trace_chain(s->dev, sa, wb.hits, false, c->counts, csa, csb, st);
-                }));
-                HIP_TRY(tm.run("trace_chain", [&] {
-                    return launch_trace_chain(s->dev, ma, wb.hits2, true, c->counts, cma, cmb, st);
-                }));
-                HIP_TRY(tm.run("chain_advance", [&] {
-                    return launch_chain_advance(s->dev, sa, wb.hits, sb, false, wb.ps, c->counts, csa, csb, st);
-                }));
-                HIP_TRY(tm.run("chain_advance", [&] {
-                    return launch_chain_advance(s->dev, ma, wb.hits2, mb, true, wb.ps, c->counts, cma, cmb, st);
-                }));
-                HIP_TRY(hipMemcpyAsync(c->host_counts + kCntShadowB, c->counts + kCntShadowB, 2 * sizeof(int),
-                                       hipMemcpyDeviceToHost, st));
-                HIP_TRY(hipMemcpyAsync(c->host_counts + cout, c->counts + cout, sizeof(int), hipMemcpyDeviceToHost, st));
-                HIP_TRY(hipStreamSynchronize(st));
-                int nsb = c->host_counts[csb], nmb = c->host_counts[cmb];
-                if (nsb == 0 && nmb == 0) break;
-                std::swap(sa, sb); std::swap(csa, csb);
-                std::swap(ma, mb); std::swap(cma, cmb);
-            }
-            if (rc) break;
-            HIP_TRY(tm.run("resolve", [&] { return launch_resolve_vol(qin, wb.ps, c->counts, cin, st); }));
-            if (bounce > max_bounce) max_bounce = bounce;
-            if (c->host_counts[cout] == 0) break;
-            if (cancel && *cancel) { rc = fail(RD_ERR_CANCELLED, "render cancelled"); break; }
-            std::swap(qin, qout);
-            std::swap(cin, cout);
-        }
-        for (int bounce = 0; !vol && !rc; bounce++) {
-            HIP_TRY(tm.run("trace_closest", [&] {
-                return launch_trace_ext(s->dev, qin, wb.hits, c->counts, cin, cout, kCntShadow, kCntMis, st);
-            }));
-            HIP_TRY(tm.run("shade", [&] {
-                return launch_shade(s->dev, s->integ, wp, qin, wb.hits, qout, wb.qshadow, wb.qmis, wb.ps, c->counts,
-                                    cin, cout, st);
-            }));
-            if (s->integ.method != RD_METHOD_BXDF) {
-                HIP_TRY(tm.run("trace_any", [&] { return launch_trace_shadow(s->dev, wb.qshadow, wb.ps, c->counts, st); }));
-                if (s->integ.method == RD_METHOD_MIS)
-                    HIP_TRY(tm.run("trace_mis", [&] { return launch_trace_mis(s->dev, wb.qmis, wb.ps, c->counts, st); }));
-                HIP_TRY(tm.run("resolve", [&] { return launch_resolve(s->integ, qin, wb.ps, c->counts, cin, st); }));
-            }
-            HIP_TRY(hipMemcpyAsync(c->host_counts + cout, c->counts + cout, sizeof(int), hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipStreamSynchronize(st));
-            if (bounce > max_bounce) max_bounce = bounce;
-            if (c->host_counts[cout] == 0) break;
-            if (cancel && *cancel) { rc = fail(RD_ERR_CANCELLED, "render cancelled"); break; }
-            std::swap(qin, qout);
-            std::swap(cin, cout);
-        }
+        HIP_TRY(tm.run("raygen", [&] { return launch_raygen(s->dev, wp, npaths, wb.qa, wb.ps, c->counts, st); }));
+        rc = run_bounces(c, tm, L, &max_bounce, cancel);
         if (rc) break;
         HIP_TRY(tm.run("accumulate", [&] { return launch_accumulate(wp, wb.ps, wb.sum, wb.sum2, c->counts, st); }));
         waves++;
@@ -797,20 +856,212 @@ int rd_render_device(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
     tm.collect();
-    auto u64 = [&](int slot) { uint64_t v; std::memcpy(&v, c->host_counts + slot, 8); return v; };
-    c->stats.rays_closest = u64(kCntRaysClosest);
-    c->stats.rays_any = u64(kCntRaysAny);
-    c->stats.vertices = u64(kCntVertices);
-    c->stats.invalid = u64(kCntInvalid);
-    c->stats.paths = (uint64_t)npix * (uint64_t)p->spp;
-    c->stats.render_ms = ms;
-    c->stats.waves = waves;
-    c->stats.bounces = max_bounce;
-    for (int k = 0; k < 3; k++) {
-        c->stats.class_rays[k] = u64(kCntClassRays + 2 * k);
-        c->stats.class_nodes[k] = u64(kCntClassNodes + 2 * k);
-        c->stats.class_prims[k] = u64(kCntClassPrims + 2 * k);
+    collect_stats(c, (uint64_t)npix * (uint64_t)p->spp, ms, waves, max_bounce);
+    return rc;
+}
+
+// Batched rendering: the requests' paths share waves.  The host packs each
+// request's samples into waves in request order (a wave holds whole samples
+// of a request's crop; one request may span several waves and one wave
+// several requests), then every wave runs the same bounce loop with each path
+// reading its request's scene.  Requests are grouped by integrator kind
+// (path method / vol_path); each group is one sequence of waves.
+int rd_render_batch_device(rd_ctx* c, int32_t n, const rd_request* reqs, int32_t wave_paths,
+                           volatile int32_t* cancel, volatile int32_t* progress) {
+    if (!c || n < 0 || (n > 0 && !reqs)) return fail(RD_ERR_INVALID, "bad argument");
+    if (n == 0) return RD_OK;
+    std::vector<Crop> crops(n);
+    for (int i = 0; i < n; i++) {
+        const rd_request& r = reqs[i];
+        if (!r.scene || !r.out_rgb) return fail(RD_ERR_INVALID, "request without scene or output");
+        if (r.scene->ctx != c) return fail(RD_ERR_INVALID, "request scene belongs to another context");
+        if (r.params.spp <= 0) return fail(RD_ERR_INVALID, "spp must be positive");
+        if (int rc = check_stack(r.scene)) return rc;
+        if (int rc = resolve_crop(r.scene, &r.params, &crops[i])) return rc;
     }
+    // groups by (integrator type, method)
+    std::vector<std::vector<int>> groups;
+    std::vector<std::pair<int, int>> kinds;
+    for (int i = 0; i < n; i++) {
+        std::pair<int, int> k{reqs[i].scene->integ.type,
+                              reqs[i].scene->integ.type == RD_INTEGRATOR_VOL_PATH ? 0 : reqs[i].scene->integ.method};
+        size_t g = 0;
+        while (g < kinds.size() && kinds[g] != k) g++;
+        if (g == kinds.size()) { kinds.push_back(k); groups.emplace_back(); }
+        groups[g].push_back(i);
+    }
+    const size_t target = wave_paths > 0 ? (size_t)wave_paths : ((size_t)1 << 22);
+    size_t max_npix = 0, total_pix = 0;
+    for (int i = 0; i < n; i++) {
+        size_t np = (size_t)(crops[i].x1 - crops[i].x0) * (crops[i].y1 - crops[i].y0);
+        max_npix = std::max(max_npix, np);
+        total_pix += np;
+    }
+    const size_t cap = std::max(target, max_npix);
+    if (cap > (size_t)INT32_MAX / 2 || total_pix > (size_t)INT32_MAX / 2) return fail(RD_ERR_INVALID, "batch too large");
+    bool any_vol = false;
+    for (auto& k : kinds) any_vol = any_vol || k.first == RD_INTEGRATOR_VOL_PATH;
+    if (int rc = ensure_wave_buffers(c, cap, 1, any_vol)) return rc;
+    WaveBuffers& wb = c->wb;
+    hipStream_t st = c->stream;
+    // per-batch device memory: scenes, segments, sums (freed at the end)
+    std::vector<void*> tmp;
+    auto dalloc = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        tmp.push_back(p);
+        return p;
+    };
+    auto cleanup = [&] { for (void* p : tmp) (void)hipFree(p); tmp.clear(); };
+    std::vector<DevScene> dscenes(n);
+    for (int i = 0; i < n; i++) dscenes[i] = reqs[i].scene->dev;
+    DevScene* d_scenes = (DevScene*)dalloc(sizeof(DevScene) * n);
+    float4* sums = (float4*)dalloc(sizeof(float4) * total_pix);
+    float4* sums2 = (float4*)dalloc(sizeof(float4) * total_pix);
+    const int kMaxSegs = 1024;
+    BatchSeg* d_segs = (BatchSeg*)dalloc(sizeof(BatchSeg) * kMaxSegs);
+    if (!d_scenes || !sums || !sums2 || !d_segs) { cleanup(); return fail(RD_ERR_OOM, "hipMalloc failed for batch"); }
+    std::vector<size_t> sum_off(n);
+    {
+        size_t o = 0;
+        for (int i = 0; i < n; i++) { sum_off[i] = o; o += (size_t)(crops[i].x1 - crops[i].x0) * (crops[i].y1 - crops[i].y0); }
+    }
+    Timer tm(c);
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    int rc = RD_OK;
+    auto chk = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && !rc) rc = fail(RD_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    chk(hipMemcpy(d_scenes, dscenes.data(), sizeof(DevScene) * n, hipMemcpyHostToDevice), "upload scenes");
+    chk(launch_zero_counts(c->counts, kCntSlots, st), "zero counts");
+    chk(launch_zero(sums, (int)total_pix, st), "zero sums");
+    chk(launch_zero(sums2, (int)total_pix, st), "zero sums");
+    hipEvent_t ev0, ev1;
+    chk(hipEventCreate(&ev0), "event");
+    chk(hipEventCreate(&ev1), "event");
+    chk(hipEventRecord(ev0, st), "event");
+    int waves = 0, max_bounce = 0;
+    uint64_t paths = 0;
+    std::vector<BatchSeg> segs;
+    for (size_t g = 0; g < groups.size() && !rc; g++) {
+        const std::vector<int>& G = groups[g];
+        // the traversal variant must cover the deepest tree of the group
+        DevScene rep = dscenes[G[0]];
+        for (int i : G) rep.stack_needed = std::max(rep.stack_needed, dscenes[i].stack_needed);
+        WaveLaunch L{rep, d_scenes, kinds[g].second, kinds[g].first == RD_INTEGRATOR_VOL_PATH};
+        std::vector<int> next_sample(n, 0);
+        size_t gi = 0;   // first request of the group with samples left
+        while (gi < G.size() && !rc) {
+            if (cancel && *cancel) { rc = fail(RD_ERR_CANCELLED, "render cancelled"); break; }
+            // pack one wave
+            segs.clear();
+            size_t used = 0;
+            int wave_max_npix = 0;
+            for (size_t j = gi; j < G.size() && (int)segs.size() < kMaxSegs; j++) {
+                int i = G[j];
+                const Crop& cr = crops[i];
+                int cw = cr.x1 - cr.x0;
+                size_t np = (size_t)cw * (cr.y1 - cr.y0);
+                int left = reqs[i].params.spp - next_sample[i];
+                if (left <= 0) continue;
+                int k = (int)std::min<size_t>((size_t)left, (cap - used) / np);
+                if (k <= 0) break;
+                BatchSeg sg{};
+                sg.req = i;
+                sg.path_offset = (int)used;
+                sg.sample_base = next_sample[i];
+                sg.nsamples = k;
+                sg.x0 = cr.x0; sg.y0 = cr.y0; sg.crop_w = cw; sg.npix = (int)np;
+                sg.width = reqs[i].scene->width;
+                sg.seed = reqs[i].params.seed;
+                sg.sum_offset = (int)sum_off[i];
+                segs.push_back(sg);
+                used += np * (size_t)k;
+                next_sample[i] += k;
+                wave_max_npix = std::max(wave_max_npix, (int)np);
+                if (k < left) break;   // this request fills the wave
+            }
+            while (gi < G.size() && next_sample[G[gi]] >= reqs[G[gi]].params.spp) gi++;
+            if (segs.empty()) break;
+            chk(hipMemcpyAsync(d_segs, segs.data(), sizeof(BatchSeg) * segs.size(), hipMemcpyHostToDevice, st), "segments");
+            int npaths = (int)used;
+            paths += used;
+            chk(tm.run("raygen", [&] {
+                return launch_raygen_batch(d_scenes, d_segs, (int)segs.size(), npaths, wb.qa, wb.ps, c->counts, st);
+            }), "raygen");
+            if (rc) break;
+            rc = run_bounces(c, tm, L, &max_bounce, cancel);
+            if (rc) break;
+            chk(tm.run("accumulate", [&] {
+                return launch_accumulate_batch(d_segs, (int)segs.size(), wave_max_npix, wb.ps, sums, sums2, c->counts, st);
+            }), "accumulate");
+            // the segment table is reused by the next wave: wait for this one's kernels
+            chk(hipStreamSynchronize(st), "sync");
+            waves++;
+            if (progress) *progress = waves;
+        }
+    }
+    for (int i = 0; i < n && !rc; i++) {
+        const Crop& cr = crops[i];
+        WaveParams wp{};
+        wp.x0 = cr.x0; wp.y0 = cr.y0; wp.crop_w = cr.x1 - cr.x0; wp.crop_h = cr.y1 - cr.y0;
+        wp.npix = wp.crop_w * wp.crop_h;
+        wp.width = reqs[i].scene->width;
+        chk(tm.run("finalize", [&] {
+            return launch_finalize(wp, reqs[i].scene->height, reqs[i].params.spp, sums + sum_off[i],
+                                   sums2 + sum_off[i], reqs[i].out_rgb, reqs[i].out_sum2, st);
+        }), "finalize");
+    }
+    chk(hipEventRecord(ev1, st), "event");
+    chk(hipMemcpyAsync(c->host_counts, c->counts, kCntSlots * sizeof(int), hipMemcpyDeviceToHost, st), "counts");
+    chk(hipStreamSynchronize(st), "sync");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    tm.collect();
+    collect_stats(c, paths, ms, waves, max_bounce);
+    cleanup();
+    return rc;
+}
+
+int rd_render_batch(rd_ctx* c, int32_t n, const rd_request* reqs, int32_t wave_paths, volatile int32_t* cancel,
+                    volatile int32_t* progress) {
+    if (!c || n < 0 || (n > 0 && !reqs)) return fail(RD_ERR_INVALID, "bad argument");
+    CtxLock lock(c);
+    if (!lock.ok()) return fail(RD_ERR_BUSY, "context in use by another thread");
+    (void)hipSetDevice(c->device);
+    // device outputs for every request; pixels outside a crop keep the caller's values
+    std::vector<rd_request> dreq(reqs, reqs + n);
+    std::vector<void*> bufs;
+    int rc = RD_OK;
+    for (int i = 0; i < n && !rc; i++) {
+        if (!reqs[i].scene || !reqs[i].out_rgb) { rc = fail(RD_ERR_INVALID, "request without scene or output"); break; }
+        size_t bytes = (size_t)reqs[i].scene->width * reqs[i].scene->height * 3 * 4;
+        void* d1 = nullptr;
+        void* d2 = nullptr;
+        if (hipMalloc(&d1, bytes) != hipSuccess) { rc = fail(RD_ERR_OOM, "hipMalloc output"); break; }
+        bufs.push_back(d1);
+        if (reqs[i].out_sum2) {
+            if (hipMalloc(&d2, bytes) != hipSuccess) { rc = fail(RD_ERR_OOM, "hipMalloc output"); break; }
+            bufs.push_back(d2);
+        }
+        if (hipMemcpy(d1, reqs[i].out_rgb, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = fail(RD_ERR_DEVICE, "copy output");
+        if (!rc && d2 && hipMemcpy(d2, reqs[i].out_sum2, bytes, hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(RD_ERR_DEVICE, "copy output");
+        dreq[i].out_rgb = (float*)d1;
+        dreq[i].out_sum2 = (float*)d2;
+    }
+    if (!rc) rc = rd_render_batch_device(c, n, dreq.data(), wave_paths, cancel, progress);
+    for (int i = 0; i < n && !rc; i++) {
+        size_t bytes = (size_t)reqs[i].scene->width * reqs[i].scene->height * 3 * 4;
+        if (hipMemcpy(reqs[i].out_rgb, dreq[i].out_rgb, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(RD_ERR_DEVICE, "copy output");
+        if (!rc && reqs[i].out_sum2 &&
+            hipMemcpy(reqs[i].out_sum2, dreq[i].out_sum2, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(RD_ERR_DEVICE, "copy output");
+    }
+    for (void* p : bufs) (void)hipFree(p);
     return rc;
 }
 

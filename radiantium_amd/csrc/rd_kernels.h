@@ -24,6 +24,8 @@ struct PathState {
     float4* mis_li;     // radiance found by the MIS ray; w = BSDF-sample pdf
     int* flags;
     int* sel_light;
+    uint64_t* key;      // RNG key of the path: path_key(seed, pixel, sample)
+    int* req;           // batched renders: request (scene) index of the path
     uint32_t* rng_n;
     int* depth;
     uint8_t* vis;
@@ -57,6 +59,20 @@ enum : int {
     kCntSlots = 36
 };
 
+// One segment of a batched wave: samples [sample_base, sample_base + nsamples)
+// of every pixel of request `req`'s crop, stored at paths [path_offset, ...)
+// sample-major, and accumulated into that request's sums at sum_offset.
+struct BatchSeg {
+    int req;
+    int path_offset;
+    int sample_base, nsamples;
+    int x0, y0, crop_w, npix;
+    int width;
+    uint32_t seed;
+    int sum_offset;
+    int _pad;
+};
+
 struct Timing;  // per-stage HIP event timing (rd_api.cpp)
 
 // Traversal over caller-provided ray arrays (rd_ray / rd_hit layouts).
@@ -66,28 +82,36 @@ hipError_t launch_trace_rays(const DevScene& sc, const float4* rays, int n, floa
 // Wavefront path tracing stages.
 hipError_t launch_raygen(const DevScene& sc, const WaveParams& wp, int npaths, RayQueue q, PathState ps,
                          int* counts, hipStream_t s);
-hipError_t launch_trace_ext(const DevScene& sc, RayQueue q, float4* hits, int* counts, int cnt_slot,
-                            int zero_a, int zero_b, int zero_c, hipStream_t s);
-hipError_t launch_shade(const DevScene& sc, const DevIntegrator& in, const WaveParams& wp, RayQueue qin,
-                        const float4* hits, RayQueue qout, RayQueue qshadow, RayQueue qmis, PathState ps,
-                        int* counts, int cnt_in, int cnt_out, hipStream_t s);
-hipError_t launch_trace_shadow(const DevScene& sc, RayQueue q, PathState ps, int* counts, hipStream_t s);
-hipError_t launch_trace_mis(const DevScene& sc, RayQueue q, PathState ps, int* counts, hipStream_t s);
-hipError_t launch_resolve(const DevIntegrator& in, RayQueue qin, PathState ps, int* counts, int cnt_in,
-                          hipStream_t s);
+hipError_t launch_trace_ext(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, float4* hits,
+                            int* counts, int cnt_slot, int zero_a, int zero_b, int zero_c, hipStream_t s);
+// `scenes` (device array, may be null): batched mode, the path's scene is
+// scenes[ps.req[pid]]; otherwise every path uses `sc`.
+hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
+                        RayQueue qout, RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in,
+                        int cnt_out, hipStream_t s);
+hipError_t launch_raygen_batch(const DevScene* scenes, const BatchSeg* segs, int nseg, int npaths, RayQueue q,
+                               PathState ps, int* counts, hipStream_t s);
+hipError_t launch_accumulate_batch(const BatchSeg* segs, int nseg, int max_npix, PathState ps, float4* sum,
+                                   float4* sum2, int* counts, hipStream_t s);
+hipError_t launch_trace_shadow(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,
+                               hipStream_t s);
+hipError_t launch_trace_mis(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,
+                            hipStream_t s);
+hipError_t launch_resolve(int method, RayQueue qin, PathState ps, int* counts, int cnt_in, hipStream_t s);
 // vol_path stages (VolumetricPathTracer.Li and its Scene.Transmittance /
 // Scene.IntersectTr chains): shade_vol draws every random number of the
 // bounce and starts one Transmittance chain (light sample) and one
 // IntersectTr chain (BSDF / phase sample) per vertex; each chain step is a
 // closest-hit query (trace_chain) followed by chain_advance, which either
 // finishes the chain into the path state or appends its next segment.
-hipError_t launch_shade_vol(const DevScene& sc, const DevIntegrator& in, const WaveParams& wp, RayQueue qin,
-                            const float4* hits, RayQueue qout, RayQueue qshadow, RayQueue qmis, PathState ps,
-                            int* counts, int cnt_in, int cnt_out, hipStream_t s);
-hipError_t launch_trace_chain(const DevScene& sc, RayQueue q, float4* hits, bool mis, int* counts, int cnt_slot,
-                              int zero_slot, hipStream_t s);
-hipError_t launch_chain_advance(const DevScene& sc, RayQueue qin, const float4* hits, RayQueue qout, bool mis,
-                                PathState ps, int* counts, int cnt_in, int cnt_out, hipStream_t s);
+hipError_t launch_shade_vol(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
+                            RayQueue qout, RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in,
+                            int cnt_out, hipStream_t s);
+hipError_t launch_trace_chain(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, float4* hits,
+                              bool mis, int* counts, int cnt_slot, int zero_slot, hipStream_t s);
+hipError_t launch_chain_advance(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
+                                RayQueue qout, bool mis, PathState ps, int* counts, int cnt_in, int cnt_out,
+                                hipStream_t s);
 hipError_t launch_resolve_vol(RayQueue qin, PathState ps, int* counts, int cnt_in, hipStream_t s);
 hipError_t launch_accumulate(const WaveParams& wp, PathState ps, float4* sum, float4* sum2, int* counts,
                              hipStream_t s);

@@ -72,6 +72,7 @@ struct TraceArgs {
     int cnt_slot;             // queue modes: count slot
     int cursor_slot;          // work cursor, zero on entry
     int zero[3];              // count slots block 0 clears for the next stage (-1: none)
+    const DevScene* scenes;   // batched queue modes: the ray's scene is scenes[ps.req[pid]]
 };
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
@@ -105,9 +106,8 @@ __device__ __forceinline__ void trav_begin(Trav& t, float4 ra, float4 rb) {
 // descends to the child on the near side of its split axis and pushes the
 // other.  Returns true when the ray is finished.
 template <bool kAny, int kBlock>
-__device__ __forceinline__ bool trav_step(const DevScene& sc, Trav& t, int* stk, uint32_t& nv, uint32_t& nt) {
-    const float4* __restrict__ nodes = sc.nodes;
-    const float4* __restrict__ prims = sc.prims;
+__device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes, const float4* __restrict__ prims, Trav& t,
+                                          int* stk, uint32_t& nv, uint32_t& nt) {
     float4 lo = nodes[2 * t.cur], hi = nodes[2 * t.cur + 1];
     nv++;
     if (box_hit(lo, hi, t.o, t.inv, t.nx, t.ny, t.nz, t.tmax)) {
@@ -209,8 +209,8 @@ __device__ __forceinline__ int trav_count(const TraceArgs& a, bool fixed) {
 }
 
 // Schedule 0 ("straight"): each lane runs its ray to completion, rays are
-// taken grid-stride.
-template <int kMode, int kStack, int kBlock>
+// taken grid-stride.  kBatch: each ray traverses its own request's scene.
+template <int kMode, int kStack, int kBlock, bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_trace_straight(DevScene sc, TraceArgs a) {
     constexpr bool kAny = (kMode == kModeRaysAny || kMode == kModeShadow);
     __shared__ int lds[kStack * kBlock];
@@ -223,8 +223,12 @@ __global__ __launch_bounds__(kBlock) void k_trace_straight(DevScene sc, TraceArg
         trav_load<kMode>(a, r, &ra, &rb);
         Trav t;
         trav_begin(t, ra, rb);
-        while (!trav_step<kAny, kBlock>(sc, t, stk, nv, nt)) {}
-        trav_finish<kMode>(sc, a, r, t);
+        auto run = [&](const DevScene& s2) {
+            while (!trav_step<kAny, kBlock>(s2.nodes, s2.prims, t, stk, nv, nt)) {}
+            trav_finish<kMode>(s2, a, r, t);
+        };
+        if constexpr (kBatch) run(a.scenes[a.ps.req[a.q.pid[r]]]);
+        else run(sc);
         nrays++;
     }
     trav_counters<kMode>(a, nrays, nv, nt);
@@ -272,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(DevScene sc, TraceArgs 
         if (!active) continue;
         // a few steps between refill checks
         bool done = false;
-        for (int k = 0; k < 4 && !done; k++) done = trav_step<kAny, kBlock>(sc, t, stk, nv, nt);
+        for (int k = 0; k < 4 && !done; k++) done = trav_step<kAny, kBlock>(sc.nodes, sc.prims, t, stk, nv, nt);
         if (!done) continue;
         active = false;
         trav_finish<kMode>(sc, a, r, t);
@@ -293,8 +297,14 @@ int get_trace_schedule(int cls) { return (cls >= 0 && cls < 3) ? g_schedule[cls]
 // launched (64 = the reference's Bvh.StackSize).
 template <int kMode, int kStack, int kBlock>
 static void launch_trace_variant(const DevScene& sc, const TraceArgs& a, int blocks, int schedule, hipStream_t s) {
+    if constexpr (kMode >= kModeExt) {
+        if (a.scenes) {   // batched: straight schedule, per-ray scene
+            hipLaunchKernelGGL((k_trace_straight<kMode, kStack, kBlock, true>), dim3(blocks), dim3(kBlock), 0, s, sc, a);
+            return;
+        }
+    }
     if (schedule) hipLaunchKernelGGL((k_trace_refill<kMode, kStack, kBlock>), dim3(blocks), dim3(kBlock), 0, s, sc, a);
-    else hipLaunchKernelGGL((k_trace_straight<kMode, kStack, kBlock>), dim3(blocks), dim3(kBlock), 0, s, sc, a);
+    else hipLaunchKernelGGL((k_trace_straight<kMode, kStack, kBlock, false>), dim3(blocks), dim3(kBlock), 0, s, sc, a);
 }
 
 template <int kMode>
@@ -325,6 +335,7 @@ static TraceArgs trace_args(int* counts) {
     TraceArgs a{};
     a.counts = counts;
     a.zero[0] = a.zero[1] = a.zero[2] = -1;
+    a.scenes = nullptr;
     return a;
 }
 
@@ -360,7 +371,8 @@ __global__ __launch_bounds__(256) void k_raygen(DevScene sc, WaveParams wp, int 
     for (int p = blockIdx.x * 256 + threadIdx.x; p < npaths; p += gridDim.x * 256) {
         int x, y, s;
         wave_pixel(wp, p, &x, &y, &s);
-        Rng rng{path_key(wp.seed, (uint32_t)(y * wp.width + x), (uint32_t)s), 0u};
+        const uint64_t key = path_key(wp.seed, (uint32_t)(y * wp.width + x), (uint32_t)s);
+        Rng rng{key, 0u};
         float ox = rng.next_float(), oy = rng.next_float();
         float sx = (float)x + ox, sy = (float)y + oy;
         const DevCamera& c = sc.cam;
@@ -377,10 +389,68 @@ __global__ __launch_bounds__(256) void k_raygen(DevScene sc, WaveParams wp, int 
         ps.beta[p] = make_float4(1.f, 1.f, 1.f, 0.f);
         ps.flags[p] = kFlagSpecular;
         ps.rng_n[p] = rng.n;
+        ps.key[p] = key;
         ps.depth[p] = 0;
         if (ps.env) ps.env[p] = sc.global_medium;   // vol_path: rayEnv = Scene.GlobalMedium
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) counts[kCntExtA] = npaths;
+}
+
+// Batched ray generation: path p of the wave belongs to the segment whose
+// [path_offset, path_offset + nsamples * npix) holds it; within a segment
+// paths are sample-major like a single render's wave.
+__device__ __forceinline__ int find_seg(const BatchSeg* segs, int nseg, int p) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].path_offset <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_raygen_batch(const DevScene* __restrict__ scenes, const BatchSeg* segs,
+                                                      int nseg, int npaths, RayQueue q, PathState ps, int* counts) {
+    for (int p = blockIdx.x * 256 + threadIdx.x; p < npaths; p += gridDim.x * 256) {
+        const BatchSeg sg = segs[find_seg(segs, nseg, p)];
+        int local = p - sg.path_offset;
+        int si = local / sg.npix;
+        int qq = local - si * sg.npix;
+        int yy = qq / sg.crop_w;
+        int x = sg.x0 + (qq - yy * sg.crop_w), y = sg.y0 + yy;
+        const DevScene& sc = scenes[sg.req];
+        const uint64_t key = path_key(sg.seed, (uint32_t)(y * sg.width + x), (uint32_t)(sg.sample_base + si));
+        Rng rng{key, 0u};
+        float ox = rng.next_float(), oy = rng.next_float();
+        float sx = (float)x + ox, sy = (float)y + oy;
+        const DevCamera& c = sc.cam;
+        f3 ndc = mk(sx * c.inv_res_x, sy * c.inv_res_y, 0.0f);
+        f3 nearp = xform_point(c.clip_to_camera, ndc);
+        f3 dir = normalize(nearp);
+        float inv_z = 1.0f / dir.z;
+        f3 o = xform_point(c.camera_to_world, mk(0.0f));
+        f3 d = normalize(xform_vector(c.camera_to_world, dir));
+        q.pid[p] = p;
+        q.org[p] = make_float4(o.x, o.y, o.z, c.near_z * inv_z);
+        q.dir[p] = make_float4(d.x, d.y, d.z, c.far_z * inv_z);
+        ps.L[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+        ps.beta[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+        ps.flags[p] = kFlagSpecular;
+        ps.rng_n[p] = rng.n;
+        ps.key[p] = key;
+        ps.req[p] = sg.req;
+        ps.depth[p] = 0;
+        if (ps.env) ps.env[p] = sc.global_medium;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) counts[kCntExtA] = npaths;
+}
+
+hipError_t launch_raygen_batch(const DevScene* scenes, const BatchSeg* segs, int nseg, int npaths, RayQueue q,
+                               PathState ps, int* counts, hipStream_t s) {
+    int blocks = (npaths + 255) / 256;
+    if (blocks > persistent_blocks()) blocks = persistent_blocks();
+    hipLaunchKernelGGL(k_raygen_batch, dim3(blocks), dim3(256), 0, s, scenes, segs, nseg, npaths, q, ps, counts);
+    return hipGetLastError();
 }
 
 hipError_t launch_raygen(const DevScene& sc, const WaveParams& wp, int npaths, RayQueue q, PathState ps,
@@ -393,40 +463,44 @@ hipError_t launch_raygen(const DevScene& sc, const WaveParams& wp, int npaths, R
 
 // Closest hit of the extension queue.  It also clears the queue counters and
 // work cursors the following shade / shadow / MIS stages start from.
-hipError_t launch_trace_ext(const DevScene& sc, RayQueue q, float4* hits, int* counts, int cnt_slot, int za,
-                            int zb, int zc, hipStream_t s) {
+hipError_t launch_trace_ext(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, float4* hits,
+                            int* counts, int cnt_slot, int za, int zb, int zc, hipStream_t s) {
     TraceArgs a = trace_args(counts);
-    a.q = q; a.hits = hits; a.cnt_slot = cnt_slot; a.cursor_slot = kCntWork + 0;
+    a.q = q; a.ps = ps; a.hits = hits; a.cnt_slot = cnt_slot; a.cursor_slot = kCntWork + 0; a.scenes = scenes;
     a.zero[0] = za; a.zero[1] = zb; a.zero[2] = zc;
     return launch_trace_mode<kModeExt>(sc, a, 0, s);
 }
 
 // Any hit of the shadow queue: Scene.IsOccluded (Scene.cs:58-70).
-hipError_t launch_trace_shadow(const DevScene& sc, RayQueue q, PathState ps, int* counts, hipStream_t s) {
+hipError_t launch_trace_shadow(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,
+                               hipStream_t s) {
     TraceArgs a = trace_args(counts);
-    a.q = q; a.ps = ps; a.cnt_slot = kCntShadow; a.cursor_slot = kCntWork + 1;
+    a.q = q; a.ps = ps; a.cnt_slot = kCntShadow; a.cursor_slot = kCntWork + 1; a.scenes = scenes;
     return launch_trace_mode<kModeShadow>(sc, a, 0, s);
 }
 
 // Closest hit of the BSDF-sampled MIS rays and the radiance they carry.
-hipError_t launch_trace_mis(const DevScene& sc, RayQueue q, PathState ps, int* counts, hipStream_t s) {
+hipError_t launch_trace_mis(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,
+                            hipStream_t s) {
     TraceArgs a = trace_args(counts);
-    a.q = q; a.ps = ps; a.cnt_slot = kCntMis; a.cursor_slot = kCntWork + 2;
+    a.q = q; a.ps = ps; a.cnt_slot = kCntMis; a.cursor_slot = kCntWork + 2; a.scenes = scenes;
     return launch_trace_mode<kModeMis>(sc, a, 0, s);
 }
 // One closest-hit step of the vol_path Transmittance (mis = false) or
 // IntersectTr (mis = true) chains; clears the next chain queue's counter.
-hipError_t launch_trace_chain(const DevScene& sc, RayQueue q, float4* hits, bool mis, int* counts, int cnt_slot,
-                              int zero_slot, hipStream_t s) {
+hipError_t launch_trace_chain(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, float4* hits,
+                              bool mis, int* counts, int cnt_slot, int zero_slot, hipStream_t s) {
     TraceArgs a = trace_args(counts);
-    a.q = q; a.hits = hits; a.cnt_slot = cnt_slot; a.cursor_slot = kCntWork + (mis ? 2 : 1);
+    a.q = q; a.ps = ps; a.hits = hits; a.cnt_slot = cnt_slot; a.cursor_slot = kCntWork + (mis ? 2 : 1);
+    a.scenes = scenes;
     a.zero[0] = zero_slot;
     return mis ? launch_trace_mode<kModeChainMis>(sc, a, 0, s) : launch_trace_mode<kModeChainShadow>(sc, a, 0, s);
 }
 
 // One bounce of PathTracer.Mis / NextEventEstimation / OnlySampleBxdf for
 // every path in the extension queue (PathTracer.cs:43-270).
-__global__ __launch_bounds__(256) void k_shade(DevScene sc, DevIntegrator in, WaveParams wp, RayQueue qin,
+template <bool kBatch>
+__global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __restrict__ scenes, RayQueue qin,
                                                const float4* __restrict__ hits, RayQueue qout, RayQueue qshadow,
                                                RayQueue qmis, PathState ps, int* counts, int cnt_in, int cnt_out) {
     int n = counts[cnt_in];
@@ -441,13 +515,15 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, DevIntegrator in, Wa
         int pid = 0;
         if (k < n) {
             pid = qin.pid[k];
+            auto run = [&](const DevScene& sc) {
+            const DevIntegrator& in = sc.integ;
             float4 a = qin.org[k], b = qin.dir[k];
             f3 ro = mk(a.x, a.y, a.z), rd = mk(b.x, b.y, b.z);
             float rtmin = a.w;
             float4 h = hits[k];
             int prim = __float_as_int(h.y);
             bool hit = prim >= 0;
-            Rng rng{wave_key(wp, pid), ps.rng_n[pid]};
+            Rng rng{ps.key[pid], ps.rng_n[pid]};
             f3 beta = ld3(ps.beta, pid);
             f3 L = ld3(ps.L, pid);
             int flags = ps.flags[pid];
@@ -583,6 +659,9 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, DevIntegrator in, Wa
             ps.flags[pid] = nflags;
             ps.rng_n[pid] = rng.n;
             ps.depth[pid] = depth;
+            };
+            if constexpr (kBatch) run(scenes[ps.req[pid]]);
+            else run(sc0);
         }
         int slot = wave_append(&counts[cnt_out], want_ext);
         if (want_ext) {
@@ -606,18 +685,21 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, DevIntegrator in, Wa
     add_counter((unsigned long long*)(counts + kCntVertices), nvert);
 }
 
-hipError_t launch_shade(const DevScene& sc, const DevIntegrator& in, const WaveParams& wp, RayQueue qin,
-                        const float4* hits, RayQueue qout, RayQueue qshadow, RayQueue qmis, PathState ps,
-                        int* counts, int cnt_in, int cnt_out, hipStream_t s) {
-    hipLaunchKernelGGL(k_shade, dim3(persistent_blocks()), dim3(256), 0, s, sc, in, wp, qin, hits, qout, qshadow,
-                       qmis, ps, counts, cnt_in, cnt_out);
+hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
+                        RayQueue qout, RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in,
+                        int cnt_out, hipStream_t s) {
+    if (scenes)
+        hipLaunchKernelGGL(k_shade<true>, dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits, qout,
+                           qshadow, qmis, ps, counts, cnt_in, cnt_out);
+    else
+        hipLaunchKernelGGL(k_shade<false>, dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits, qout,
+                           qshadow, qmis, ps, counts, cnt_in, cnt_out);
     return hipGetLastError();
 }
 
 // radiance += coeff * SampleLightToEstimateDirect (PathTracer.cs:226, :273-307)
 // for Mis; radiance += coeff * EstimateLight / lightPdf (:81) for Nee.
-__global__ __launch_bounds__(256) void k_resolve(DevIntegrator in, RayQueue qin, PathState ps, int* counts,
-                                                 int cnt_in) {
+__global__ __launch_bounds__(256) void k_resolve(int method, RayQueue qin, PathState ps, int* counts, int cnt_in) {
     int n = counts[cnt_in];
     for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
         int pid = qin.pid[k];
@@ -628,7 +710,7 @@ __global__ __launch_bounds__(256) void k_resolve(DevIntegrator in, RayQueue qin,
         float selpdf = nb.w;
         bool vis = (flags & kFlagShadowIssued) && ps.vis[pid];
         f3 L = ld3(ps.L, pid);
-        if (in.method == 2) {
+        if (method == 2) {
             f3 le = mk(0.0f);
             if (vis) le = le + ld3(ps.nee_le, pid);
             if (flags & kFlagMisIssued) {
@@ -649,8 +731,8 @@ __global__ __launch_bounds__(256) void k_resolve(DevIntegrator in, RayQueue qin,
     }
 }
 
-hipError_t launch_resolve(const DevIntegrator& in, RayQueue qin, PathState ps, int* counts, int cnt_in, hipStream_t s) {
-    hipLaunchKernelGGL(k_resolve, dim3(persistent_blocks()), dim3(256), 0, s, in, qin, ps, counts, cnt_in);
+hipError_t launch_resolve(int method, RayQueue qin, PathState ps, int* counts, int cnt_in, hipStream_t s) {
+    hipLaunchKernelGGL(k_resolve, dim3(persistent_blocks()), dim3(256), 0, s, method, qin, ps, counts, cnt_in);
     return hipGetLastError();
 }
 
@@ -681,6 +763,39 @@ hipError_t launch_accumulate(const WaveParams& wp, PathState ps, float4* sum, fl
     int blocks = (wp.npix + 255) / 256;
     if (blocks > persistent_blocks()) blocks = persistent_blocks();
     hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(256), 0, s, wp, ps, sum, sum2, counts);
+    return hipGetLastError();
+}
+
+// Batched accumulation: for every segment of the wave, each of its pixels
+// adds the segment's samples in sample order to the request's sums.
+__global__ __launch_bounds__(256) void k_accumulate_batch(const BatchSeg* segs, int nseg, PathState ps, float4* sum,
+                                                          float4* sum2, int* counts) {
+    uint64_t invalid = 0;
+    for (int g = blockIdx.y; g < nseg; g += gridDim.y) {
+        const BatchSeg sg = segs[g];
+        for (int q = blockIdx.x * 256 + threadIdx.x; q < sg.npix; q += gridDim.x * 256) {
+            int o = sg.sum_offset + q;
+            float4 s4 = sum[o], q4 = sum2[o];
+            f3 s = mk(s4.x, s4.y, s4.z), s2 = mk(q4.x, q4.y, q4.z);
+            for (int j = 0; j < sg.nsamples; j++) {
+                f3 c = ld3(ps.L, sg.path_offset + j * sg.npix + q);
+                if (!valid_color(c)) { c = mk(0.0f); invalid++; }
+                s = s + c;
+                s2 = s2 + c * c;
+            }
+            sum[o] = make_float4(s.x, s.y, s.z, 0.f);
+            sum2[o] = make_float4(s2.x, s2.y, s2.z, 0.f);
+        }
+    }
+    add_counter((unsigned long long*)(counts + kCntInvalid), invalid);
+}
+
+hipError_t launch_accumulate_batch(const BatchSeg* segs, int nseg, int max_npix, PathState ps, float4* sum,
+                                   float4* sum2, int* counts, hipStream_t s) {
+    int bx = (max_npix + 255) / 256;
+    if (bx > 1024) bx = 1024;
+    int by = nseg < 64 ? nseg : 64;
+    hipLaunchKernelGGL(k_accumulate_batch, dim3(bx, by), dim3(256), 0, s, segs, nseg, ps, sum, sum2, counts);
     return hipGetLastError();
 }
 

@@ -88,6 +88,12 @@ struct DevCamera {
     int width, height;
 };
 
+// Integrator parameters (PathTracer ctor, Integrators/PathTracer.cs:34-41)
+struct DevIntegrator {
+    int type, max_depth, min_depth, method, strategy;
+    float rr_threshold;
+};
+
 struct DevScene {
     const float4* nodes;   // 2 per node
     const float4* prims;   // 3 per ordered primitive
@@ -109,13 +115,9 @@ struct DevScene {
     const DevEnv* envs;    // Scene.InfiniteLights order
     int num_envs;
     int global_medium;
+    DevIntegrator integ;   // the scene's integrator (a batch may mix parameters)
     DevCamera cam;
 };
 
-// Integrator parameters (PathTracer ctor, Integrators/PathTracer.cs:34-41)
-struct DevIntegrator {
-    int type, max_depth, min_depth, method, strategy;
-    float rr_threshold;
-};
 
 }  // namespace rd

@@ -24,7 +24,8 @@
 
 namespace rd {
 
-__global__ __launch_bounds__(256) void k_shade_vol(DevScene sc, DevIntegrator in, WaveParams wp, RayQueue qin,
+template <bool kBatch>
+__global__ __launch_bounds__(256) void k_shade_vol(DevScene sc0, const DevScene* __restrict__ scenes, RayQueue qin,
                                                    const float4* __restrict__ hits, RayQueue qout, RayQueue qshadow,
                                                    RayQueue qmis, PathState ps, int* counts, int cnt_in,
                                                    int cnt_out) {
@@ -39,13 +40,15 @@ __global__ __launch_bounds__(256) void k_shade_vol(DevScene sc, DevIntegrator in
         int pid = 0;
         if (k < n) {
             pid = qin.pid[k];
+            auto run = [&](const DevScene& sc) {
+            const DevIntegrator& in = sc.integ;
             float4 a = qin.org[k], b = qin.dir[k];
             f3 ro = mk(a.x, a.y, a.z), rd = mk(b.x, b.y, b.z);
             float rtmin = a.w;
             float4 h = hits[k];
             int prim = __float_as_int(h.y);
             bool hit = prim >= 0;
-            Rng rng{wave_key(wp, pid), ps.rng_n[pid]};
+            Rng rng{ps.key[pid], ps.rng_n[pid]};
             f3 beta = ld3(ps.beta, pid);
             f3 L = ld3(ps.L, pid);
             int flags = ps.flags[pid];
@@ -222,6 +225,9 @@ __global__ __launch_bounds__(256) void k_shade_vol(DevScene sc, DevIntegrator in
             ps.rng_n[pid] = rng.n;
             ps.depth[pid] = depth;
             ps.env[pid] = env;
+            };
+            if constexpr (kBatch) run(scenes[ps.req[pid]]);
+            else run(sc0);
         }
         int slot = wave_append(&counts[cnt_out], want_ext);
         if (want_ext) {
@@ -245,11 +251,15 @@ __global__ __launch_bounds__(256) void k_shade_vol(DevScene sc, DevIntegrator in
     add_counter((unsigned long long*)(counts + kCntVertices), nvert);
 }
 
-hipError_t launch_shade_vol(const DevScene& sc, const DevIntegrator& in, const WaveParams& wp, RayQueue qin,
-                            const float4* hits, RayQueue qout, RayQueue qshadow, RayQueue qmis, PathState ps,
-                            int* counts, int cnt_in, int cnt_out, hipStream_t s) {
-    hipLaunchKernelGGL(k_shade_vol, dim3(persistent_blocks()), dim3(256), 0, s, sc, in, wp, qin, hits, qout, qshadow,
-                       qmis, ps, counts, cnt_in, cnt_out);
+hipError_t launch_shade_vol(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
+                            RayQueue qout, RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in,
+                            int cnt_out, hipStream_t s) {
+    if (scenes)
+        hipLaunchKernelGGL(k_shade_vol<true>, dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
+                           qout, qshadow, qmis, ps, counts, cnt_in, cnt_out);
+    else
+        hipLaunchKernelGGL(k_shade_vol<false>, dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
+                           qout, qshadow, qmis, ps, counts, cnt_in, cnt_out);
     return hipGetLastError();
 }
 
@@ -261,9 +271,10 @@ hipError_t launch_shade_vol(const DevScene& sc, const DevIntegrator& in, const W
 //   takes the infinite lights; a hit multiplies Tr up to |ray.At(t)| (the
 //   length of the hit's position vector, as written); a surface ends the chain
 //   with the selected light's Le; a boundary continues with MinT 0.001.
-__global__ __launch_bounds__(256) void k_chain_advance(DevScene sc, RayQueue qin, const float4* __restrict__ hits,
-                                                       RayQueue qout, int mis, PathState ps, int* counts,
-                                                       int cnt_in, int cnt_out) {
+template <bool kBatch>
+__global__ __launch_bounds__(256) void k_chain_advance(DevScene sc0, const DevScene* __restrict__ scenes, RayQueue qin,
+                                                       const float4* __restrict__ hits, RayQueue qout, int mis,
+                                                       PathState ps, int* counts, int cnt_in, int cnt_out) {
     int n = counts[cnt_in];
     // the traversal work cursor of this chain restarts at zero for its next step
     if (blockIdx.x == 0 && threadIdx.x == 0) counts[kCntWork + (mis ? 2 : 1)] = 0;
@@ -275,6 +286,7 @@ __global__ __launch_bounds__(256) void k_chain_advance(DevScene sc, RayQueue qin
         int pid = 0;
         if (k < n) {
             pid = qin.pid[k];
+            auto run = [&](const DevScene& sc) {
             float4 a = qin.org[k], b = qin.dir[k];
             f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
             float tmin = a.w, tmax = b.w;
@@ -333,6 +345,9 @@ __global__ __launch_bounds__(256) void k_chain_advance(DevScene sc, RayQueue qin
                 }
                 ps.mis_tr[pid] = make_float4(tr.x, tr.y, tr.z, __int_as_float(env));
             }
+            };
+            if constexpr (kBatch) run(scenes[ps.req[pid]]);
+            else run(sc0);
         }
         int slot = wave_append(&counts[cnt_out], want);
         if (want) {
@@ -343,10 +358,15 @@ __global__ __launch_bounds__(256) void k_chain_advance(DevScene sc, RayQueue qin
     }
 }
 
-hipError_t launch_chain_advance(const DevScene& sc, RayQueue qin, const float4* hits, RayQueue qout, bool mis,
-                                PathState ps, int* counts, int cnt_in, int cnt_out, hipStream_t s) {
-    hipLaunchKernelGGL(k_chain_advance, dim3(persistent_blocks()), dim3(256), 0, s, sc, qin, hits, qout,
-                       mis ? 1 : 0, ps, counts, cnt_in, cnt_out);
+hipError_t launch_chain_advance(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
+                                RayQueue qout, bool mis, PathState ps, int* counts, int cnt_in, int cnt_out,
+                                hipStream_t s) {
+    if (scenes)
+        hipLaunchKernelGGL(k_chain_advance<true>, dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
+                           qout, mis ? 1 : 0, ps, counts, cnt_in, cnt_out);
+    else
+        hipLaunchKernelGGL(k_chain_advance<false>, dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
+                           qout, mis ? 1 : 0, ps, counts, cnt_in, cnt_out);
     return hipGetLastError();
 }
 

@@ -19,7 +19,7 @@ EXPORTS = [
     "rd_scene_get_info", "rd_scene_export_bvh", "rd_trace_closest", "rd_trace_any",
     "rd_trace_closest_device", "rd_trace_any_device", "rd_render", "rd_render_device",
     "rd_get_render_stats", "rd_bvh_build_host", "rd_device_alloc", "rd_device_free", "rd_memcpy_h2d", "rd_memcpy_d2h",
-    "rd_synchronize", "rd_kernel_timing", "rd_reset_kernel_timing",
+    "rd_synchronize", "rd_kernel_timing", "rd_reset_kernel_timing", "rd_render_batch", "rd_render_batch_device",
 ]
 
 
@@ -42,6 +42,8 @@ def lib():
         L.rd_create.restype = vp
         L.rd_create.argtypes = [C.c_int]
         L.rd_destroy.argtypes = [vp]
+        L.rd_render_batch.argtypes = [vp, i32, vp, i32, vp, vp]
+        L.rd_render_batch_device.argtypes = [vp, i32, vp, i32, vp, vp]
         L.rd_scene_create.restype = vp
         L.rd_scene_create.argtypes = [vp, C.POINTER(abi.rd_scene_desc)]
         L.rd_scene_destroy.argtypes = [vp]
@@ -107,6 +109,40 @@ class Context:
 
     def __del__(self):
         self.close()
+
+    def render_batch(self, requests, wave_paths=0, with_sum2=True):
+        """Batched render of [(scene, spp, seed[, crop])]: returns [(rgb, sum2)]
+        (layouts as Scene.render)."""
+        reqs = (abi.rd_request * len(requests))()
+        outs = []
+        for i, r in enumerate(requests):
+            scene, spp, seed = r[0], r[1], r[2]
+            crop = r[3] if len(r) > 3 else None
+            n = scene.width * scene.height * 3
+            rgb = np.zeros(n, np.float32)
+            s2 = np.zeros(n, np.float32) if with_sum2 else None
+            reqs[i].scene = scene.h
+            reqs[i].params.spp, reqs[i].params.seed = spp, seed
+            if crop is not None:
+                reqs[i].params.x0, reqs[i].params.y0, reqs[i].params.x1, reqs[i].params.y1 = crop
+            reqs[i].out_rgb = rgb.ctypes.data
+            reqs[i].out_sum2 = s2.ctypes.data if s2 is not None else None
+            outs.append((rgb, s2))
+        check(lib().rd_render_batch(self.h, len(requests), reqs, wave_paths, None, None))
+        return outs
+
+    def render_batch_device(self, requests, wave_paths=0):
+        """requests: [(scene, spp, seed, d_rgb, d_sum2 or None[, crop])] with device outputs."""
+        reqs = (abi.rd_request * len(requests))()
+        for i, r in enumerate(requests):
+            scene, spp, seed, d_rgb, d_s2 = r[:5]
+            reqs[i].scene = scene.h
+            reqs[i].params.spp, reqs[i].params.seed = spp, seed
+            if len(r) > 5 and r[5] is not None:
+                reqs[i].params.x0, reqs[i].params.y0, reqs[i].params.x1, reqs[i].params.y1 = r[5]
+            reqs[i].out_rgb = d_rgb
+            reqs[i].out_sum2 = d_s2
+        check(lib().rd_render_batch_device(self.h, len(requests), reqs, wave_paths, None, None))
 
     def alloc(self, nbytes):
         p = lib().rd_device_alloc(self.h, nbytes)

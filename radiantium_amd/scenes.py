@@ -244,6 +244,25 @@ def vol_fog(width=64, height=64, max_depth=32, sigma_a=(0.01, 0.015, 0.02), sigm
     return s
 
 
+def batch_requests(seed=0, n=16, resolutions=(128, 256, 512, 1024), spps=(16, 64, 256), max_depth=64):
+    """The batched workload: `n` vol_path Cornell boxes, each with a
+    perfect_glass sphere holding a homogeneous medium whose grey sigma_a ~
+    U[0.01, 0.2], sigma_s ~ U[0.5, 5] and g ~ U[-0.3, 0.9]; square resolution
+    drawn from `resolutions`, spp from `spps`, max depth 64.  Returns a list of
+    (SceneDesc, spp, seed)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        res = int(rng.choice(resolutions))
+        spp = int(rng.choice(spps))
+        sa = float(rng.uniform(0.01, 0.2))
+        ss = float(rng.uniform(0.5, 5.0))
+        g = float(rng.uniform(-0.3, 0.9))
+        d = vol_cornell(res, res, max_depth=max_depth, sigma_a=sa, sigma_s=ss, g=g)
+        out.append((d, spp, 1000 + i))
+    return out
+
+
 # ---------------------------------------------------------------- icosphere
 def icosphere(level):
     """Unit icosphere: 20*4^level faces, midpoint subdivision with shared

@@ -277,6 +277,35 @@ def test_renderer_api(ctx):
     assert not r2.IsSuccess and r2.Error.code == abi.RD_ERR_CANCELLED
 
 
+def test_batch_equals_single_renders(ctx):
+    """rd_render_batch: requests of different scenes, integrators,
+    resolutions, spp and crops share waves (a small wave size forces requests
+    to split across waves and waves to hold several requests); every image is
+    bit-identical to the request rendered alone."""
+    descs = [
+        (scenes.cornell_box(24, 20, method="Mis"), 5, 3, None),
+        (scenes.cornell_box(16, 16, method="Mis"), 9, 4, (2, 3, 13, 11)),
+        (scenes.env_showcase(20, 16, method="Mis", map_w=32, map_h=16), 4, 5, None),
+        (scenes.vol_cornell(18, 18), 6, 6, None),
+        (scenes.cornell_box(12, 12, method="Nee"), 7, 7, None),
+        (scenes.vol_fog(16, 12), 3, 8, None),
+    ]
+    sc = [Scene(ctx, d) for d, _, _, _ in descs]
+    singles, rays = [], 0
+    for s_, (d, spp, seed, crop) in zip(sc, descs):
+        singles.append(s_.render(spp, seed=seed, crop=crop))
+        rays += ctx.render_stats()["rays_closest"] + ctx.render_stats()["rays_any"]
+    for wave in (1500, 0):
+        print(f"[batch] {len(descs)} requests, wave_paths={wave}", flush=True)
+        outs = ctx.render_batch([(s_, spp, seed, crop) for s_, (d, spp, seed, crop) in zip(sc, descs)],
+                                wave_paths=wave)
+        st = ctx.render_stats()
+        assert st["rays_closest"] + st["rays_any"] == rays
+        for (g, g2), (o, o2) in zip(outs, singles):
+            assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+            assert np.array_equal(g2.view(np.uint32), o2.view(np.uint32))
+
+
 def test_scene_may_outlive_its_context():
     """rd_destroy with live scenes defers the context's release to the last
     rd_scene_destroy; a closed context accepts no new scene."""
