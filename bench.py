@@ -94,6 +94,47 @@ def pmc_traffic(stage, args):
         return None
 
 
+def batched_speedup(ctx, args):
+    """Batched workload: 16 vol_path Cornell boxes with a glass sphere of
+    homogeneous medium each (sigma_a ~ U[0.01,0.2], sigma_s ~ U[0.5,5],
+    g ~ U[-0.3,0.9]), resolutions from {128^2..1024^2}, spp from {16,64,256},
+    max depth 64.  Sequential = one rd_render_device per request; batched =
+    one rd_render_batch_device over all of them.  Same outputs (bit-identical)."""
+    from radiantium_amd import native, scenes
+    reqs = scenes.batch_requests(args.batch_seed)
+    sc = [native.Scene(ctx, d) for d, _, _ in reqs]
+    outs = [ctx.alloc(d.camera["width"] * d.camera["height"] * 12) for d, _, _ in reqs]
+    paths = sum(d.camera["width"] * d.camera["height"] * spp for d, spp, _ in reqs)
+    # warm both paths once at low spp
+    for s_, o in zip(sc, outs):
+        s_.render_device(1, o, seed=7)
+    ctx.render_batch_device([(s_, 1, 7, o, None) for s_, o in zip(sc, outs)])
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    rays_seq = 0
+    for s_, (d, spp, seed), o in zip(sc, reqs, outs):
+        s_.render_device(spp, o, seed=seed)
+        st = ctx.render_stats()
+        rays_seq += st["rays_closest"] + st["rays_any"]
+    ctx.synchronize()
+    seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ctx.render_batch_device([(s_, spp, seed, o, None) for s_, (d, spp, seed), o in zip(sc, reqs, outs)])
+    ctx.synchronize()
+    bat = time.perf_counter() - t0
+    st = ctx.render_stats()
+    rays_bat = st["rays_closest"] + st["rays_any"]
+    for o in outs:
+        ctx.free(o)
+    log(f"[bench] batched: {len(reqs)} requests, {paths / 1e6:.0f} M paths; sequential {seq * 1000:.0f} ms, "
+        f"batched {bat * 1000:.0f} ms")
+    return {"speedup": round(seq / bat, 3), "sequential_ms": round(seq * 1000, 1), "batched_ms": round(bat * 1000, 1),
+            "requests": len(reqs), "paths": paths, "rays": rays_bat, "rays_equal": rays_seq == rays_bat,
+            "batched_mrays_s": round(rays_bat / bat / 1e6, 1),
+            "workload": f"16 vol_path glass-medium Cornell boxes, seed {args.batch_seed}, res 128^2-1024^2, "
+                        f"spp 16-256, depth 64"}
+
+
 def copy_calibration():
     """A plain device copy of known size, profiled beside the path kernels so
     FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
@@ -120,6 +161,8 @@ def main():
     ap.add_argument("--wave-paths", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-seed", type=int, default=0, help="seed of the batched workload (16 vol_path scenes)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-sequential measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (scale for PMC byte counters)")
     args = ap.parse_args()
@@ -188,6 +231,8 @@ def main():
     if dom != trace_dom and roofline is not None:
         roofline["note"] = f"longest stage is {dom}; roofline reported for the longest traversal kernel"
 
+    batched = None if args.no_batch else batched_speedup(ctx, args)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(desc, args, value)
@@ -201,7 +246,7 @@ def main():
                                f"{args.max_depth}, bvh max_prim 4",
                    "width": args.width, "height": args.height, "spp": args.spp, "max_depth": args.max_depth,
                    "triangles": ntri, "parallelism": "1 GPU"},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "batched": batched,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

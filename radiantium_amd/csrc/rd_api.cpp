@@ -40,22 +40,41 @@ struct StageTiming {
 };
 
 // Device buffers of the wavefront, reused across renders and grown on demand.
+// Most segments one pass holds (the host-built segment table).
+constexpr int kMaxSegs = 1024;
+
 struct WaveBuffers {
-    size_t cap = 0;      // paths
-    size_t pix_cap = 0;  // accumulation pixels
-    std::vector<void*> allocs;
+    size_t cap = 0;        // path pool slots
+    size_t res_cap = 0;    // per-sample results of one pass
+    size_t pix_cap = 0;    // accumulation pixels (all requests of a render)
+    size_t scene_cap = 0;  // batched: device scene table entries
+    bool vol = false;
+    std::vector<void*> pool_allocs, res_allocs, pix_allocs, scene_allocs, seg_allocs;
     RayQueue qa{}, qb{}, qshadow{}, qmis{};
     RayQueue qshadow2{}, qmis2{};   // vol_path chain queues (second buffers)
     float4* hits = nullptr;
     float4* hits2 = nullptr;        // vol_path: IntersectTr chain hits
-    bool vol = false;
     PathState ps{};
     float4* sum = nullptr;
     float4* sum2 = nullptr;
+    DevScene* scenes = nullptr;
+    BatchSeg* segs = nullptr;       // device segment table
+    BatchSeg* host_segs = nullptr;  // pinned staging for it
+    static void free_list(std::vector<void*>& v) {
+        for (void* p : v) (void)hipFree(p);
+        v.clear();
+    }
     void release() {
-        for (void* p : allocs) (void)hipFree(p);
-        allocs.clear();
-        cap = pix_cap = 0;
+        free_list(pool_allocs);
+        free_list(res_allocs);
+        free_list(pix_allocs);
+        free_list(scene_allocs);
+        free_list(seg_allocs);
+        if (host_segs) (void)hipHostFree(host_segs);
+        host_segs = nullptr;
+        segs = nullptr;
+        scenes = nullptr;
+        cap = res_cap = pix_cap = scene_cap = 0;
         vol = false;
     }
 };
@@ -168,63 +187,99 @@ struct Timer {
 // crosses; this bounds the host loop for pathological geometry.
 constexpr int kMaxChainSteps = 1 << 16;
 
-int ensure_wave_buffers(rd_ctx* c, size_t paths, size_t pixels, bool vol) {
+// Grows the render buffers to hold `pool` path slots (+ the vol_path arrays),
+// `results` per-sample results and `pixels` accumulation pixels.
+int ensure_wave_buffers(rd_ctx* c, size_t pool, size_t results, size_t pixels, bool vol) {
     WaveBuffers& w = c->wb;
-    if (paths <= w.cap && pixels <= w.pix_cap && (w.vol || !vol)) return RD_OK;
-    w.release();
-    size_t cap = std::max(paths, (size_t)1024);
-    size_t pcap = std::max(pixels, (size_t)1024);
-    auto alloc = [&](size_t bytes) -> void* {
+    auto alloc_in = [&](std::vector<void*>& list, size_t bytes) -> void* {
         void* p = nullptr;
         if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
-        w.allocs.push_back(p);
+        list.push_back(p);
         return p;
     };
-    auto mkq = [&](RayQueue& q, int slot) -> bool {
-        q.count = c->counts + slot;
-        q.pid = (int*)alloc(cap * sizeof(int));
-        q.org = (float4*)alloc(cap * sizeof(float4));
-        q.dir = (float4*)alloc(cap * sizeof(float4));
-        return q.pid && q.org && q.dir;
-    };
-    bool ok = mkq(w.qa, kCntExtA) && mkq(w.qb, kCntExtB) && mkq(w.qshadow, kCntShadow) && mkq(w.qmis, kCntMis);
-    w.hits = (float4*)alloc(cap * sizeof(float4));
-    PathState& ps = w.ps;
-    ps.L = (float4*)alloc(cap * sizeof(float4));
-    ps.beta = (float4*)alloc(cap * sizeof(float4));
-    ps.nee_beta = (float4*)alloc(cap * sizeof(float4));
-    ps.nee_le = (float4*)alloc(cap * sizeof(float4));
-    ps.mis_f = (float4*)alloc(cap * sizeof(float4));
-    ps.mis_li = (float4*)alloc(cap * sizeof(float4));
-    ps.flags = (int*)alloc(cap * sizeof(int));
-    ps.sel_light = (int*)alloc(cap * sizeof(int));
-    ps.key = (uint64_t*)alloc(cap * sizeof(uint64_t));
-    ps.req = (int*)alloc(cap * sizeof(int));
-    ps.rng_n = (uint32_t*)alloc(cap * sizeof(uint32_t));
-    ps.depth = (int*)alloc(cap * sizeof(int));
-    ps.vis = (uint8_t*)alloc(cap);
-    w.sum = (float4*)alloc(pcap * sizeof(float4));
-    w.sum2 = (float4*)alloc(pcap * sizeof(float4));
-    ok = ok && w.hits && ps.L && ps.beta && ps.nee_beta && ps.nee_le && ps.mis_f && ps.mis_li && ps.flags &&
-         ps.sel_light && ps.key && ps.req && ps.rng_n && ps.depth && ps.vis && w.sum && w.sum2;
-    ps.env = nullptr; ps.nee_li = ps.sh_to = ps.sh_tr = ps.mis_tr = nullptr;
-    if (ok && vol) {
-        ok = mkq(w.qshadow2, kCntShadowB) && mkq(w.qmis2, kCntMisB);
-        w.hits2 = (float4*)alloc(cap * sizeof(float4));
-        ps.env = (int*)alloc(cap * sizeof(int));
-        ps.nee_li = (float4*)alloc(cap * sizeof(float4));
-        ps.sh_to = (float4*)alloc(cap * sizeof(float4));
-        ps.sh_tr = (float4*)alloc(cap * sizeof(float4));
-        ps.mis_tr = (float4*)alloc(cap * sizeof(float4));
-        ok = ok && w.hits2 && ps.env && ps.nee_li && ps.sh_to && ps.sh_tr && ps.mis_tr;
-        w.vol = ok;
+    if (!w.segs) {
+        w.segs = (BatchSeg*)alloc_in(w.seg_allocs, sizeof(BatchSeg) * kMaxSegs);
+        if (hipHostMalloc((void**)&w.host_segs, sizeof(BatchSeg) * kMaxSegs, hipHostMallocDefault) != hipSuccess)
+            w.host_segs = nullptr;
+        if (!w.segs || !w.host_segs) { w.release(); return fail(RD_ERR_OOM, "segment table allocation failed"); }
     }
-    if (!ok) {
-        w.release();
-        return fail(RD_ERR_OOM, "hipMalloc failed for wavefront buffers");
+    if (pool > w.cap || (vol && !w.vol)) {
+        WaveBuffers::free_list(w.pool_allocs);
+        w.cap = 0;
+        w.vol = false;
+        size_t cap = std::max(pool, (size_t)1024);
+        auto alloc = [&](size_t bytes) { return alloc_in(w.pool_allocs, bytes); };
+        auto mkq = [&](RayQueue& q, int slot) -> bool {
+            q.count = c->counts + slot;
+            q.pid = (int*)alloc(cap * sizeof(int));
+            q.org = (float4*)alloc(cap * sizeof(float4));
+            q.dir = (float4*)alloc(cap * sizeof(float4));
+            return q.pid && q.org && q.dir;
+        };
+        bool ok = mkq(w.qa, kCntExtA) && mkq(w.qb, kCntExtB) && mkq(w.qshadow, kCntShadow) && mkq(w.qmis, kCntMis);
+        w.hits = (float4*)alloc(cap * sizeof(float4));
+        PathState& ps = w.ps;
+        ps.L = (float4*)alloc(cap * sizeof(float4));
+        ps.beta = (float4*)alloc(cap * sizeof(float4));
+        ps.nee_beta = (float4*)alloc(cap * sizeof(float4));
+        ps.nee_le = (float4*)alloc(cap * sizeof(float4));
+        ps.mis_f = (float4*)alloc(cap * sizeof(float4));
+        ps.mis_li = (float4*)alloc(cap * sizeof(float4));
+        ps.flags = (int*)alloc(cap * sizeof(int));
+        ps.sel_light = (int*)alloc(cap * sizeof(int));
+        ps.key = (uint64_t*)alloc(cap * sizeof(uint64_t));
+        ps.req = (int*)alloc(cap * sizeof(int));
+        ps.gid = (int*)alloc(cap * sizeof(int));
+        ps.free_slots = (int*)alloc(cap * sizeof(int));
+        ps.rng_n = (uint32_t*)alloc(cap * sizeof(uint32_t));
+        ps.depth = (int*)alloc(cap * sizeof(int));
+        ps.vis = (uint8_t*)alloc(cap);
+        ok = ok && w.hits && ps.L && ps.beta && ps.nee_beta && ps.nee_le && ps.mis_f && ps.mis_li && ps.flags &&
+             ps.sel_light && ps.key && ps.req && ps.gid && ps.free_slots && ps.rng_n && ps.depth && ps.vis;
+        ps.env = nullptr; ps.nee_li = ps.sh_to = ps.sh_tr = ps.mis_tr = nullptr;
+        if (ok && vol) {
+            ok = mkq(w.qshadow2, kCntShadowB) && mkq(w.qmis2, kCntMisB);
+            w.hits2 = (float4*)alloc(cap * sizeof(float4));
+            ps.env = (int*)alloc(cap * sizeof(int));
+            ps.nee_li = (float4*)alloc(cap * sizeof(float4));
+            ps.sh_to = (float4*)alloc(cap * sizeof(float4));
+            ps.sh_tr = (float4*)alloc(cap * sizeof(float4));
+            ps.mis_tr = (float4*)alloc(cap * sizeof(float4));
+            ok = ok && w.hits2 && ps.env && ps.nee_li && ps.sh_to && ps.sh_tr && ps.mis_tr;
+        }
+        if (!ok) { w.release(); return fail(RD_ERR_OOM, "hipMalloc failed for the path pool"); }
+        w.cap = cap;
+        w.vol = vol;
     }
-    w.cap = cap;
-    w.pix_cap = pcap;
+    if (results > w.res_cap) {
+        WaveBuffers::free_list(w.res_allocs);
+        w.res_cap = 0;
+        w.ps.res = (float4*)alloc_in(w.res_allocs, results * sizeof(float4));
+        if (!w.ps.res) { w.release(); return fail(RD_ERR_OOM, "hipMalloc failed for per-sample results"); }
+        w.res_cap = results;
+    }
+    if (pixels > w.pix_cap) {
+        WaveBuffers::free_list(w.pix_allocs);
+        w.pix_cap = 0;
+        size_t pcap = std::max(pixels, (size_t)1024);
+        w.sum = (float4*)alloc_in(w.pix_allocs, pcap * sizeof(float4));
+        w.sum2 = (float4*)alloc_in(w.pix_allocs, pcap * sizeof(float4));
+        if (!w.sum || !w.sum2) { w.release(); return fail(RD_ERR_OOM, "hipMalloc failed for accumulation"); }
+        w.pix_cap = pcap;
+    }
+    return RD_OK;
+}
+
+int ensure_batch_buffers(rd_ctx* c, size_t nscenes) {
+    WaveBuffers& w = c->wb;
+    if (nscenes <= w.scene_cap) return RD_OK;
+    WaveBuffers::free_list(w.scene_allocs);
+    w.scene_cap = 0;
+    void* p = nullptr;
+    if (hipMalloc(&p, sizeof(DevScene) * nscenes) != hipSuccess) return fail(RD_ERR_OOM, "hipMalloc scene table");
+    w.scene_allocs.push_back(p);
+    w.scenes = (DevScene*)p;
+    w.scene_cap = nscenes;
     return RD_OK;
 }
 
@@ -681,23 +736,33 @@ int rd_trace_any(rd_ctx* c, rd_scene* s, const rd_ray* rays, int32_t n, uint8_t*
 
 namespace {
 
-// What one wave's bounce loop launches: a single scene (kernel argument) or a
+// What one pass's bounce loop launches: a single scene (kernel argument) or a
 // batch (device array of scenes, the path's request selects one); `sc` is
 // also the traversal variant's reference for the LDS stack depth.
-struct WaveLaunch {
+struct PassLaunch {
     DevScene sc;
     const DevScene* scenes;
     int method;
     bool vol;
+    const BatchSeg* segs;
+    int nseg;
+    int npass;     // paths in the pass
+    int pool;      // path slots in flight
 };
 
-// The bounce loop of one wave, after ray generation: per bounce one sync
-// reads the next queue size (and, for vol_path, the chain queue sizes).
-int run_bounces(rd_ctx* c, Timer& tm, const WaveLaunch& L, int* max_bounce, volatile int32_t* cancel) {
+// The bounce loop of one pass: every bounce traces and shades the queue,
+// retires finished paths and refills their slots with the pass's next paths;
+// one sync per bounce reads the next queue size (for vol_path also the chain
+// queue sizes).  Ends when the queue is empty (the pass has run dry).
+int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatile int32_t* cancel) {
     WaveBuffers& wb = c->wb;
     hipStream_t st = c->stream;
     RayQueue qin = wb.qa, qout = wb.qb;
     int cin = kCntExtA, cout = kCntExtB;
+    HIP_TRY(launch_init_pool(wb.ps, L.pool, c->counts, cin, st));
+    HIP_TRY(tm.run("raygen", [&] {
+        return launch_refill(L.sc, L.scenes, L.segs, L.nseg, L.npass, L.pool, qin, cin, wb.ps, c->counts, st);
+    }));
     int rc = RD_OK;
     for (int bounce = 0; !rc; bounce++) {
         HIP_TRY(tm.run("trace_closest", [&] {
@@ -730,7 +795,6 @@ int run_bounces(rd_ctx* c, Timer& tm, const WaveLaunch& L, int* max_bounce, vola
                                        hipMemcpyDeviceToHost, st));
                 HIP_TRY(hipMemcpyAsync(c->host_counts + kCntShadow, c->counts + kCntShadow, 2 * sizeof(int),
                                        hipMemcpyDeviceToHost, st));
-                HIP_TRY(hipMemcpyAsync(c->host_counts + cout, c->counts + cout, sizeof(int), hipMemcpyDeviceToHost, st));
                 HIP_TRY(hipStreamSynchronize(st));
                 if (c->host_counts[csb] == 0 && c->host_counts[cmb] == 0) break;
                 std::swap(sa, sb); std::swap(csa, csb);
@@ -750,11 +814,14 @@ int run_bounces(rd_ctx* c, Timer& tm, const WaveLaunch& L, int* max_bounce, vola
                     HIP_TRY(tm.run("trace_mis", [&] {
                         return launch_trace_mis(L.sc, L.scenes, wb.qmis, wb.ps, c->counts, st);
                     }));
-                HIP_TRY(tm.run("resolve", [&] { return launch_resolve(L.method, qin, wb.ps, c->counts, cin, st); }));
             }
-            HIP_TRY(hipMemcpyAsync(c->host_counts + cout, c->counts + cout, sizeof(int), hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipStreamSynchronize(st));
+            HIP_TRY(tm.run("resolve", [&] { return launch_resolve(L.method, qin, wb.ps, c->counts, cin, st); }));
         }
+        HIP_TRY(tm.run("raygen", [&] {
+            return launch_refill(L.sc, L.scenes, L.segs, L.nseg, L.npass, L.pool, qout, cout, wb.ps, c->counts, st);
+        }));
+        HIP_TRY(hipMemcpyAsync(c->host_counts + cout, c->counts + cout, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
         if (bounce > *max_bounce) *max_bounce = bounce;
         if (c->host_counts[cout] == 0) break;
         if (cancel && *cancel) rc = fail(RD_ERR_CANCELLED, "render cancelled");
@@ -765,7 +832,7 @@ int run_bounces(rd_ctx* c, Timer& tm, const WaveLaunch& L, int* max_bounce, vola
 }
 
 // Device counters -> rd_render_stats after the render's final synchronize.
-void collect_stats(rd_ctx* c, uint64_t paths, float ms, int waves, int max_bounce) {
+void collect_stats(rd_ctx* c, uint64_t paths, float ms, int passes, int max_bounce) {
     auto u64 = [&](int slot) { uint64_t v; std::memcpy(&v, c->host_counts + slot, 8); return v; };
     c->stats.rays_closest = u64(kCntRaysClosest);
     c->stats.rays_any = u64(kCntRaysAny);
@@ -773,7 +840,7 @@ void collect_stats(rd_ctx* c, uint64_t paths, float ms, int waves, int max_bounc
     c->stats.invalid = u64(kCntInvalid);
     c->stats.paths = paths;
     c->stats.render_ms = ms;
-    c->stats.waves = waves;
+    c->stats.waves = passes;
     c->stats.bounces = max_bounce;
     for (int k = 0; k < 3; k++) {
         c->stats.class_rays[k] = u64(kCntClassRays + 2 * k);
@@ -794,82 +861,19 @@ int resolve_crop(const rd_scene* s, const rd_render_params* p, Crop* out) {
     return RD_OK;
 }
 
-}  // namespace
+// Paths of one pass (their per-sample results stay in HBM until the pass's
+// accumulation): 2^27 paths = 2 GiB of results.
+constexpr size_t kPassPaths = (size_t)1 << 27;
+constexpr size_t kDefaultPool = (size_t)1 << 22;
 
-int rd_render_device(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d_out_rgb, float* d_out_sum2,
-                     volatile int32_t* cancel, volatile int32_t* progress) {
-    if (!c || !s || !p || !d_out_rgb) return fail(RD_ERR_INVALID, "null argument");
-    if (p->spp <= 0) return fail(RD_ERR_INVALID, "spp must be positive");
-    if (int rc = check_stack(s)) return rc;
-    const int H = s->height;
-    Crop cr;
-    if (int rc = resolve_crop(s, p, &cr)) return rc;
-    const int cw = cr.x1 - cr.x0, ch = cr.y1 - cr.y0;
-    const size_t npix = (size_t)cw * ch;
-    // Paths in flight per wave: whole samples of every crop pixel.
-    size_t target = p->wave_paths > 0 ? (size_t)p->wave_paths : ((size_t)1 << 22);
-    int spw = (int)std::max<size_t>(1, std::min<size_t>((size_t)p->spp, target / std::max<size_t>(npix, 1)));
-    size_t cap = npix * (size_t)spw;
-    if (cap > (size_t)INT32_MAX / 2) return fail(RD_ERR_INVALID, "wave too large");
-    const bool vol = s->integ.type == RD_INTEGRATOR_VOL_PATH;
-    if (int rc = ensure_wave_buffers(c, cap, npix, vol)) return rc;
-    WaveBuffers& wb = c->wb;
-    hipStream_t st = c->stream;
-    Timer tm(c);
-    std::memset(&c->stats, 0, sizeof(c->stats));
-    HIP_TRY(launch_zero_counts(c->counts, kCntSlots, st));
-    HIP_TRY(launch_zero(wb.sum, (int)npix, st));
-    HIP_TRY(launch_zero(wb.sum2, (int)npix, st));
-    hipEvent_t ev0, ev1;
-    HIP_TRY(hipEventCreate(&ev0));
-    HIP_TRY(hipEventCreate(&ev1));
-    HIP_TRY(hipEventRecord(ev0, st));
-    WaveParams wp;
-    wp.x0 = cr.x0; wp.y0 = cr.y0; wp.crop_w = cw; wp.crop_h = ch;
-    wp.npix = (int)npix;
-    wp.width = s->width;
-    wp.seed = p->seed;
-    WaveLaunch L{s->dev, nullptr, s->integ.method, vol};
-    int waves = 0, max_bounce = 0;
-    int rc = RD_OK;
-    for (int sb = 0; sb < p->spp && !rc; sb += spw) {
-        if (cancel && *cancel) { rc = fail(RD_ERR_CANCELLED, "render cancelled"); break; }
-        wp.sample_base = sb;
-        wp.nsamples = std::min(spw, p->spp - sb);
-        int npaths = (int)(npix * wp.nsamples);
-        HIP_TRY(tm.run("raygen", [&] { return launch_raygen(s->dev, wp, npaths, wb.qa, wb.ps, c->counts, st); }));
-        rc = run_bounces(c, tm, L, &max_bounce, cancel);
-        if (rc) break;
-        HIP_TRY(tm.run("accumulate", [&] { return launch_accumulate(wp, wb.ps, wb.sum, wb.sum2, c->counts, st); }));
-        waves++;
-        if (progress) *progress = waves;
-    }
-    if (!rc) {
-        wp.sample_base = 0;
-        HIP_TRY(tm.run("finalize", [&] { return launch_finalize(wp, H, p->spp, wb.sum, wb.sum2, d_out_rgb, d_out_sum2, st); }));
-    }
-    HIP_TRY(hipEventRecord(ev1, st));
-    HIP_TRY(hipMemcpyAsync(c->host_counts, c->counts, kCntSlots * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, ev0, ev1);
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
-    tm.collect();
-    collect_stats(c, (uint64_t)npix * (uint64_t)p->spp, ms, waves, max_bounce);
-    return rc;
-}
-
-// Batched rendering: the requests' paths share waves.  The host packs each
-// request's samples into waves in request order (a wave holds whole samples
-// of a request's crop; one request may span several waves and one wave
-// several requests), then every wave runs the same bounce loop with each path
-// reading its request's scene.  Requests are grouped by integrator kind
-// (path method / vol_path); each group is one sequence of waves.
-int rd_render_batch_device(rd_ctx* c, int32_t n, const rd_request* reqs, int32_t wave_paths,
-                           volatile int32_t* cancel, volatile int32_t* progress) {
-    if (!c || n < 0 || (n > 0 && !reqs)) return fail(RD_ERR_INVALID, "bad argument");
-    if (n == 0) return RD_OK;
+// Renders requests with device outputs.  single = true: one request rendered
+// with the scene as a kernel argument; otherwise requests are grouped by
+// integrator kind and their paths share passes, each path reading its
+// request's scene.  Passes pack whole samples of requests in request order
+// (a pass may hold several requests, a request may span several passes);
+// within a pass paths stream through the slot pool.
+int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, int32_t pool_paths,
+                    volatile int32_t* cancel, volatile int32_t* progress) {
     std::vector<Crop> crops(n);
     for (int i = 0; i < n; i++) {
         const rd_request& r = reqs[i];
@@ -879,48 +883,39 @@ int rd_render_batch_device(rd_ctx* c, int32_t n, const rd_request* reqs, int32_t
         if (int rc = check_stack(r.scene)) return rc;
         if (int rc = resolve_crop(r.scene, &r.params, &crops[i])) return rc;
     }
-    // groups by (integrator type, method)
     std::vector<std::vector<int>> groups;
     std::vector<std::pair<int, int>> kinds;
     for (int i = 0; i < n; i++) {
-        std::pair<int, int> k{reqs[i].scene->integ.type,
-                              reqs[i].scene->integ.type == RD_INTEGRATOR_VOL_PATH ? 0 : reqs[i].scene->integ.method};
+        const DevIntegrator& I = reqs[i].scene->dev.integ;
+        std::pair<int, int> k{I.type, I.type == RD_INTEGRATOR_VOL_PATH ? 0 : I.method};
         size_t g = 0;
         while (g < kinds.size() && kinds[g] != k) g++;
         if (g == kinds.size()) { kinds.push_back(k); groups.emplace_back(); }
         groups[g].push_back(i);
     }
-    const size_t target = wave_paths > 0 ? (size_t)wave_paths : ((size_t)1 << 22);
-    size_t max_npix = 0, total_pix = 0;
+    size_t max_npix = 0, total_pix = 0, total_paths = 0;
     for (int i = 0; i < n; i++) {
         size_t np = (size_t)(crops[i].x1 - crops[i].x0) * (crops[i].y1 - crops[i].y0);
         max_npix = std::max(max_npix, np);
         total_pix += np;
+        total_paths += np * (size_t)reqs[i].params.spp;
     }
-    const size_t cap = std::max(target, max_npix);
-    if (cap > (size_t)INT32_MAX / 2 || total_pix > (size_t)INT32_MAX / 2) return fail(RD_ERR_INVALID, "batch too large");
+    const size_t pool = std::min(pool_paths > 0 ? (size_t)pool_paths : kDefaultPool, std::max<size_t>(total_paths, 1));
+    const size_t pass_cap = std::max(std::min(kPassPaths, total_paths), max_npix);
+    if (pass_cap > (size_t)INT32_MAX / 2 || total_pix > (size_t)INT32_MAX / 2) return fail(RD_ERR_INVALID, "render too large");
     bool any_vol = false;
     for (auto& k : kinds) any_vol = any_vol || k.first == RD_INTEGRATOR_VOL_PATH;
-    if (int rc = ensure_wave_buffers(c, cap, 1, any_vol)) return rc;
+    if (int rc = ensure_wave_buffers(c, pool, pass_cap, total_pix, any_vol)) return rc;
     WaveBuffers& wb = c->wb;
     hipStream_t st = c->stream;
-    // per-batch device memory: scenes, segments, sums (freed at the end)
-    std::vector<void*> tmp;
-    auto dalloc = [&](size_t bytes) -> void* {
-        void* p = nullptr;
-        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
-        tmp.push_back(p);
-        return p;
-    };
-    auto cleanup = [&] { for (void* p : tmp) (void)hipFree(p); tmp.clear(); };
     std::vector<DevScene> dscenes(n);
     for (int i = 0; i < n; i++) dscenes[i] = reqs[i].scene->dev;
-    DevScene* d_scenes = (DevScene*)dalloc(sizeof(DevScene) * n);
-    float4* sums = (float4*)dalloc(sizeof(float4) * total_pix);
-    float4* sums2 = (float4*)dalloc(sizeof(float4) * total_pix);
-    const int kMaxSegs = 1024;
-    BatchSeg* d_segs = (BatchSeg*)dalloc(sizeof(BatchSeg) * kMaxSegs);
-    if (!d_scenes || !sums || !sums2 || !d_segs) { cleanup(); return fail(RD_ERR_OOM, "hipMalloc failed for batch"); }
+    const DevScene* d_scenes = nullptr;
+    if (!single) {
+        if (int rc = ensure_batch_buffers(c, (size_t)n)) return rc;
+        HIP_TRY(hipMemcpyAsync(wb.scenes, dscenes.data(), sizeof(DevScene) * n, hipMemcpyHostToDevice, st));
+        d_scenes = wb.scenes;
+    }
     std::vector<size_t> sum_off(n);
     {
         size_t o = 0;
@@ -928,35 +923,27 @@ int rd_render_batch_device(rd_ctx* c, int32_t n, const rd_request* reqs, int32_t
     }
     Timer tm(c);
     std::memset(&c->stats, 0, sizeof(c->stats));
-    int rc = RD_OK;
-    auto chk = [&](hipError_t e, const char* what) {
-        if (e != hipSuccess && !rc) rc = fail(RD_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
-    };
-    chk(hipMemcpy(d_scenes, dscenes.data(), sizeof(DevScene) * n, hipMemcpyHostToDevice), "upload scenes");
-    chk(launch_zero_counts(c->counts, kCntSlots, st), "zero counts");
-    chk(launch_zero(sums, (int)total_pix, st), "zero sums");
-    chk(launch_zero(sums2, (int)total_pix, st), "zero sums");
+    HIP_TRY(launch_zero_counts(c->counts, kCntSlots, st));
+    HIP_TRY(launch_zero(wb.sum, (int)total_pix, st));
+    HIP_TRY(launch_zero(wb.sum2, (int)total_pix, st));
     hipEvent_t ev0, ev1;
-    chk(hipEventCreate(&ev0), "event");
-    chk(hipEventCreate(&ev1), "event");
-    chk(hipEventRecord(ev0, st), "event");
-    int waves = 0, max_bounce = 0;
-    uint64_t paths = 0;
+    HIP_TRY(hipEventCreate(&ev0));
+    HIP_TRY(hipEventCreate(&ev1));
+    HIP_TRY(hipEventRecord(ev0, st));
+    int passes = 0, max_bounce = 0;
+    int rc = RD_OK;
     std::vector<BatchSeg> segs;
     for (size_t g = 0; g < groups.size() && !rc; g++) {
         const std::vector<int>& G = groups[g];
-        // the traversal variant must cover the deepest tree of the group
-        DevScene rep = dscenes[G[0]];
+        DevScene rep = dscenes[G[0]];   // the traversal variant must cover the deepest tree of the group
         for (int i : G) rep.stack_needed = std::max(rep.stack_needed, dscenes[i].stack_needed);
-        WaveLaunch L{rep, d_scenes, kinds[g].second, kinds[g].first == RD_INTEGRATOR_VOL_PATH};
         std::vector<int> next_sample(n, 0);
-        size_t gi = 0;   // first request of the group with samples left
+        size_t gi = 0;
         while (gi < G.size() && !rc) {
             if (cancel && *cancel) { rc = fail(RD_ERR_CANCELLED, "render cancelled"); break; }
-            // pack one wave
             segs.clear();
             size_t used = 0;
-            int wave_max_npix = 0;
+            int pass_max_npix = 0;
             for (size_t j = gi; j < G.size() && (int)segs.size() < kMaxSegs; j++) {
                 int i = G[j];
                 const Crop& cr = crops[i];
@@ -964,7 +951,7 @@ int rd_render_batch_device(rd_ctx* c, int32_t n, const rd_request* reqs, int32_t
                 size_t np = (size_t)cw * (cr.y1 - cr.y0);
                 int left = reqs[i].params.spp - next_sample[i];
                 if (left <= 0) continue;
-                int k = (int)std::min<size_t>((size_t)left, (cap - used) / np);
+                int k = (int)std::min<size_t>((size_t)left, (pass_cap - used) / np);
                 if (k <= 0) break;
                 BatchSeg sg{};
                 sg.req = i;
@@ -978,27 +965,25 @@ int rd_render_batch_device(rd_ctx* c, int32_t n, const rd_request* reqs, int32_t
                 segs.push_back(sg);
                 used += np * (size_t)k;
                 next_sample[i] += k;
-                wave_max_npix = std::max(wave_max_npix, (int)np);
-                if (k < left) break;   // this request fills the wave
+                pass_max_npix = std::max(pass_max_npix, (int)np);
+                if (k < left) break;   // this request fills the pass
             }
             while (gi < G.size() && next_sample[G[gi]] >= reqs[G[gi]].params.spp) gi++;
             if (segs.empty()) break;
-            chk(hipMemcpyAsync(d_segs, segs.data(), sizeof(BatchSeg) * segs.size(), hipMemcpyHostToDevice, st), "segments");
-            int npaths = (int)used;
-            paths += used;
-            chk(tm.run("raygen", [&] {
-                return launch_raygen_batch(d_scenes, d_segs, (int)segs.size(), npaths, wb.qa, wb.ps, c->counts, st);
-            }), "raygen");
+            // the segment table of the previous pass may still be read by its kernels
+            HIP_TRY(hipStreamSynchronize(st));
+            std::memcpy(wb.host_segs, segs.data(), sizeof(BatchSeg) * segs.size());
+            HIP_TRY(hipMemcpyAsync(wb.segs, wb.host_segs, sizeof(BatchSeg) * segs.size(), hipMemcpyHostToDevice, st));
+            PassLaunch L{rep, d_scenes, kinds[g].second, kinds[g].first == RD_INTEGRATOR_VOL_PATH, wb.segs,
+                         (int)segs.size(), (int)used, (int)std::min(pool, used)};
+            rc = run_pass(c, tm, L, &max_bounce, cancel);
             if (rc) break;
-            rc = run_bounces(c, tm, L, &max_bounce, cancel);
-            if (rc) break;
-            chk(tm.run("accumulate", [&] {
-                return launch_accumulate_batch(d_segs, (int)segs.size(), wave_max_npix, wb.ps, sums, sums2, c->counts, st);
-            }), "accumulate");
-            // the segment table is reused by the next wave: wait for this one's kernels
-            chk(hipStreamSynchronize(st), "sync");
-            waves++;
-            if (progress) *progress = waves;
+            HIP_TRY(tm.run("accumulate", [&] {
+                return launch_accumulate_batch(wb.segs, (int)segs.size(), pass_max_npix, wb.ps, wb.sum, wb.sum2,
+                                               c->counts, st);
+            }));
+            passes++;
+            if (progress) *progress = passes;
         }
     }
     for (int i = 0; i < n && !rc; i++) {
@@ -1007,22 +992,43 @@ int rd_render_batch_device(rd_ctx* c, int32_t n, const rd_request* reqs, int32_t
         wp.x0 = cr.x0; wp.y0 = cr.y0; wp.crop_w = cr.x1 - cr.x0; wp.crop_h = cr.y1 - cr.y0;
         wp.npix = wp.crop_w * wp.crop_h;
         wp.width = reqs[i].scene->width;
-        chk(tm.run("finalize", [&] {
-            return launch_finalize(wp, reqs[i].scene->height, reqs[i].params.spp, sums + sum_off[i],
-                                   sums2 + sum_off[i], reqs[i].out_rgb, reqs[i].out_sum2, st);
-        }), "finalize");
+        HIP_TRY(tm.run("finalize", [&] {
+            return launch_finalize(wp, reqs[i].scene->height, reqs[i].params.spp, wb.sum + sum_off[i],
+                                   wb.sum2 + sum_off[i], reqs[i].out_rgb, reqs[i].out_sum2, st);
+        }));
     }
-    chk(hipEventRecord(ev1, st), "event");
-    chk(hipMemcpyAsync(c->host_counts, c->counts, kCntSlots * sizeof(int), hipMemcpyDeviceToHost, st), "counts");
-    chk(hipStreamSynchronize(st), "sync");
+    HIP_TRY(hipEventRecord(ev1, st));
+    HIP_TRY(hipMemcpyAsync(c->host_counts, c->counts, kCntSlots * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
     float ms = 0;
     (void)hipEventElapsedTime(&ms, ev0, ev1);
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
     tm.collect();
-    collect_stats(c, paths, ms, waves, max_bounce);
-    cleanup();
+    collect_stats(c, total_paths, ms, passes, max_bounce);
     return rc;
+}
+
+}  // namespace
+
+int rd_render_device(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d_out_rgb, float* d_out_sum2,
+                     volatile int32_t* cancel, volatile int32_t* progress) {
+    if (!c || !s || !p || !d_out_rgb) return fail(RD_ERR_INVALID, "null argument");
+    rd_request r{};
+    r.scene = s;
+    r.params = *p;
+    r.out_rgb = d_out_rgb;
+    r.out_sum2 = d_out_sum2;
+    return render_requests(c, 1, &r, true, p->wave_paths, cancel, progress);
+}
+
+// Batched rendering: the requests' paths share passes and the slot pool; see
+// render_requests.
+int rd_render_batch_device(rd_ctx* c, int32_t n, const rd_request* reqs, int32_t wave_paths,
+                           volatile int32_t* cancel, volatile int32_t* progress) {
+    if (!c || n < 0 || (n > 0 && !reqs)) return fail(RD_ERR_INVALID, "bad argument");
+    if (n == 0) return RD_OK;
+    return render_requests(c, n, reqs, false, wave_paths, cancel, progress);
 }
 
 int rd_render_batch(rd_ctx* c, int32_t n, const rd_request* reqs, int32_t wave_paths, volatile int32_t* cancel,

@@ -25,7 +25,10 @@ struct PathState {
     int* flags;
     int* sel_light;
     uint64_t* key;      // RNG key of the path: path_key(seed, pixel, sample)
-    int* req;           // batched renders: request (scene) index of the path
+    int* req;           // request (scene) index of the path
+    int* gid;           // the path's index in the current pass
+    int* free_slots;    // free-slot list of the pool (count at kCntFree)
+    float4* res;        // per pass path: final radiance (read in sample order by accumulate)
     uint32_t* rng_n;
     int* depth;
     uint8_t* vis;
@@ -56,7 +59,9 @@ enum : int {
     kCntClassNodes = 22,  // 3 x u64
     kCntClassPrims = 28,  // 3 x u64
     kCntShadowB = 34, kCntMisB = 35,   // vol_path chain queues, second buffers
-    kCntSlots = 36
+    kCntFree = 36,        // free slots of the path pool
+    kCntNext = 37,        // next pass path to start
+    kCntSlots = 38
 };
 
 // One segment of a batched wave: samples [sample_base, sample_base + nsamples)
@@ -79,9 +84,13 @@ struct Timing;  // per-stage HIP event timing (rd_api.cpp)
 hipError_t launch_trace_rays(const DevScene& sc, const float4* rays, int n, float4* hits, uint8_t* occ,
                              unsigned long long* counters, bool any, int* counts, hipStream_t s);
 
-// Wavefront path tracing stages.
-hipError_t launch_raygen(const DevScene& sc, const WaveParams& wp, int npaths, RayQueue q, PathState ps,
-                         int* counts, hipStream_t s);
+// Wavefront path tracing stages.  A pass is a list of segments (BatchSeg);
+// its paths stream through a pool of `pool` slots: init_pool frees every
+// slot, refill starts new paths in free slots (appending them to a queue),
+// resolve retires finished paths into ps.res and the free list.
+hipError_t launch_init_pool(PathState ps, int pool, int* counts, int cnt_q, hipStream_t s);
+hipError_t launch_refill(const DevScene& sc, const DevScene* scenes, const BatchSeg* segs, int nseg, int npass,
+                         int pool, RayQueue q, int cnt_q, PathState ps, int* counts, hipStream_t s);
 hipError_t launch_trace_ext(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, float4* hits,
                             int* counts, int cnt_slot, int zero_a, int zero_b, int zero_c, hipStream_t s);
 // `scenes` (device array, may be null): batched mode, the path's scene is
@@ -89,8 +98,6 @@ hipError_t launch_trace_ext(const DevScene& sc, const DevScene* scenes, RayQueue
 hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
                         RayQueue qout, RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in,
                         int cnt_out, hipStream_t s);
-hipError_t launch_raygen_batch(const DevScene* scenes, const BatchSeg* segs, int nseg, int npaths, RayQueue q,
-                               PathState ps, int* counts, hipStream_t s);
 hipError_t launch_accumulate_batch(const BatchSeg* segs, int nseg, int max_npix, PathState ps, float4* sum,
                                    float4* sum2, int* counts, hipStream_t s);
 hipError_t launch_trace_shadow(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,
@@ -113,8 +120,6 @@ hipError_t launch_chain_advance(const DevScene& sc, const DevScene* scenes, RayQ
                                 RayQueue qout, bool mis, PathState ps, int* counts, int cnt_in, int cnt_out,
                                 hipStream_t s);
 hipError_t launch_resolve_vol(RayQueue qin, PathState ps, int* counts, int cnt_in, hipStream_t s);
-hipError_t launch_accumulate(const WaveParams& wp, PathState ps, float4* sum, float4* sum2, int* counts,
-                             hipStream_t s);
 hipError_t launch_finalize(const WaveParams& wp, int height, int spp, const float4* sum, const float4* sum2,
                            float* out_rgb, float* out_sum2, hipStream_t s);
 hipError_t launch_zero(float4* buf, int n, hipStream_t s);

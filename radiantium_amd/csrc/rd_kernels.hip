@@ -223,12 +223,16 @@ __global__ __launch_bounds__(kBlock) void k_trace_straight(DevScene sc, TraceArg
         trav_load<kMode>(a, r, &ra, &rb);
         Trav t;
         trav_begin(t, ra, rb);
-        auto run = [&](const DevScene& s2) {
-            while (!trav_step<kAny, kBlock>(s2.nodes, s2.prims, t, stk, nv, nt)) {}
+        if constexpr (kBatch) {
+            const DevScene& s2 = a.scenes[a.ps.req[a.q.pid[r]]];
+            const float4* nodes = s2.nodes;   // per-ray scene: pointers loaded once per ray
+            const float4* prims = s2.prims;
+            while (!trav_step<kAny, kBlock>(nodes, prims, t, stk, nv, nt)) {}
             trav_finish<kMode>(s2, a, r, t);
-        };
-        if constexpr (kBatch) run(a.scenes[a.ps.req[a.q.pid[r]]]);
-        else run(sc);
+        } else {
+            while (!trav_step<kAny, kBlock>(sc.nodes, sc.prims, t, stk, nv, nt)) {}
+            trav_finish<kMode>(sc, a, r, t);
+        }
         nrays++;
     }
     trav_counters<kMode>(a, nrays, nv, nt);
@@ -364,41 +368,12 @@ __device__ __forceinline__ f3 xform_vector(const float* M, f3 v) {
               v.x * M[2] + v.y * M[6] + v.z * M[10]);
 }
 
-// Pixel sample (x,y)+U^2 and PerspectiveCamera.SampleRay (Camera.cs:182-191);
-// BlockBasedRenderer.RenderTask (BlockBasedRenderer.cs:136-139).
-__global__ __launch_bounds__(256) void k_raygen(DevScene sc, WaveParams wp, int npaths, RayQueue q, PathState ps,
-                                                int* counts) {
-    for (int p = blockIdx.x * 256 + threadIdx.x; p < npaths; p += gridDim.x * 256) {
-        int x, y, s;
-        wave_pixel(wp, p, &x, &y, &s);
-        const uint64_t key = path_key(wp.seed, (uint32_t)(y * wp.width + x), (uint32_t)s);
-        Rng rng{key, 0u};
-        float ox = rng.next_float(), oy = rng.next_float();
-        float sx = (float)x + ox, sy = (float)y + oy;
-        const DevCamera& c = sc.cam;
-        f3 ndc = mk(sx * c.inv_res_x, sy * c.inv_res_y, 0.0f);
-        f3 nearp = xform_point(c.clip_to_camera, ndc);
-        f3 dir = normalize(nearp);
-        float inv_z = 1.0f / dir.z;
-        f3 o = xform_point(c.camera_to_world, mk(0.0f));
-        f3 d = normalize(xform_vector(c.camera_to_world, dir));
-        q.pid[p] = p;
-        q.org[p] = make_float4(o.x, o.y, o.z, c.near_z * inv_z);
-        q.dir[p] = make_float4(d.x, d.y, d.z, c.far_z * inv_z);
-        ps.L[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-        ps.beta[p] = make_float4(1.f, 1.f, 1.f, 0.f);
-        ps.flags[p] = kFlagSpecular;
-        ps.rng_n[p] = rng.n;
-        ps.key[p] = key;
-        ps.depth[p] = 0;
-        if (ps.env) ps.env[p] = sc.global_medium;   // vol_path: rayEnv = Scene.GlobalMedium
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) counts[kCntExtA] = npaths;
-}
-
-// Batched ray generation: path p of the wave belongs to the segment whose
-// [path_offset, path_offset + nsamples * npix) holds it; within a segment
-// paths are sample-major like a single render's wave.
+// ---------------------------------------------------------------- path pool
+// Paths stream through a fixed pool of slots: every bounce, the slots of the
+// paths that finished are refilled with the next samples of the pass, so the
+// bounce kernels see a full queue until the pass runs dry.  Pass path g lives
+// in the segment whose [path_offset, path_offset + nsamples * npix) holds it
+// (sample-major within a segment, as a single render's sample loop).
 __device__ __forceinline__ int find_seg(const BatchSeg* segs, int nseg, int p) {
     int lo = 0, hi = nseg - 1;
     while (lo < hi) {
@@ -409,55 +384,93 @@ __device__ __forceinline__ int find_seg(const BatchSeg* segs, int nseg, int p) {
     return lo;
 }
 
-__global__ __launch_bounds__(256) void k_raygen_batch(const DevScene* __restrict__ scenes, const BatchSeg* segs,
-                                                      int nseg, int npaths, RayQueue q, PathState ps, int* counts) {
-    for (int p = blockIdx.x * 256 + threadIdx.x; p < npaths; p += gridDim.x * 256) {
-        const BatchSeg sg = segs[find_seg(segs, nseg, p)];
-        int local = p - sg.path_offset;
-        int si = local / sg.npix;
-        int qq = local - si * sg.npix;
-        int yy = qq / sg.crop_w;
-        int x = sg.x0 + (qq - yy * sg.crop_w), y = sg.y0 + yy;
-        const DevScene& sc = scenes[sg.req];
-        const uint64_t key = path_key(sg.seed, (uint32_t)(y * sg.width + x), (uint32_t)(sg.sample_base + si));
-        Rng rng{key, 0u};
-        float ox = rng.next_float(), oy = rng.next_float();
-        float sx = (float)x + ox, sy = (float)y + oy;
-        const DevCamera& c = sc.cam;
-        f3 ndc = mk(sx * c.inv_res_x, sy * c.inv_res_y, 0.0f);
-        f3 nearp = xform_point(c.clip_to_camera, ndc);
-        f3 dir = normalize(nearp);
-        float inv_z = 1.0f / dir.z;
-        f3 o = xform_point(c.camera_to_world, mk(0.0f));
-        f3 d = normalize(xform_vector(c.camera_to_world, dir));
-        q.pid[p] = p;
-        q.org[p] = make_float4(o.x, o.y, o.z, c.near_z * inv_z);
-        q.dir[p] = make_float4(d.x, d.y, d.z, c.far_z * inv_z);
-        ps.L[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-        ps.beta[p] = make_float4(1.f, 1.f, 1.f, 0.f);
-        ps.flags[p] = kFlagSpecular;
-        ps.rng_n[p] = rng.n;
-        ps.key[p] = key;
-        ps.req[p] = sg.req;
-        ps.depth[p] = 0;
-        if (ps.env) ps.env[p] = sc.global_medium;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) counts[kCntExtA] = npaths;
+// Pixel sample (x,y)+U^2 and PerspectiveCamera.SampleRay (Camera.cs:182-191),
+// BlockBasedRenderer.RenderTask (BlockBasedRenderer.cs:136-139), into slot `slot`.
+__device__ __forceinline__ void start_path(const DevScene& sc, const BatchSeg& sg, int g, int slot, int qi, RayQueue q,
+                                           PathState ps) {
+    int local = g - sg.path_offset;
+    int si = local / sg.npix;
+    int qq = local - si * sg.npix;
+    int yy = qq / sg.crop_w;
+    int x = sg.x0 + (qq - yy * sg.crop_w), y = sg.y0 + yy;
+    const uint64_t key = path_key(sg.seed, (uint32_t)(y * sg.width + x), (uint32_t)(sg.sample_base + si));
+    Rng rng{key, 0u};
+    float ox = rng.next_float(), oy = rng.next_float();
+    float sx = (float)x + ox, sy = (float)y + oy;
+    const DevCamera& c = sc.cam;
+    f3 ndc = mk(sx * c.inv_res_x, sy * c.inv_res_y, 0.0f);
+    f3 nearp = xform_point(c.clip_to_camera, ndc);
+    f3 dir = normalize(nearp);
+    float inv_z = 1.0f / dir.z;
+    f3 o = xform_point(c.camera_to_world, mk(0.0f));
+    f3 d = normalize(xform_vector(c.camera_to_world, dir));
+    q.pid[qi] = slot;
+    q.org[qi] = make_float4(o.x, o.y, o.z, c.near_z * inv_z);
+    q.dir[qi] = make_float4(d.x, d.y, d.z, c.far_z * inv_z);
+    ps.L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    ps.beta[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+    ps.flags[slot] = kFlagSpecular;
+    ps.rng_n[slot] = rng.n;
+    ps.key[slot] = key;
+    ps.gid[slot] = g;
+    ps.req[slot] = sg.req;
+    ps.depth[slot] = 0;
+    if (ps.env) ps.env[slot] = sc.global_medium;   // vol_path: rayEnv = Scene.GlobalMedium
 }
 
-hipError_t launch_raygen_batch(const DevScene* scenes, const BatchSeg* segs, int nseg, int npaths, RayQueue q,
-                               PathState ps, int* counts, hipStream_t s) {
-    int blocks = (npaths + 255) / 256;
+// Starts min(free slots, paths left) new paths, appended to queue q after the
+// paths already in it; k_refill_commit then advances the counters.
+template <bool kBatch>
+__global__ __launch_bounds__(256) void k_refill(DevScene sc0, const DevScene* __restrict__ scenes, const BatchSeg* segs,
+                                                int nseg, int npass, RayQueue q, int cnt_q, PathState ps, int* counts) {
+    const int n_free = counts[kCntFree], next = counts[kCntNext], base = counts[cnt_q];
+    const int n_new = min(n_free, npass - next);
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n_new; i += gridDim.x * 256) {
+        int g = next + i;
+        const BatchSeg sg = segs[find_seg(segs, nseg, g)];
+        if constexpr (kBatch) start_path(scenes[sg.req], sg, g, ps.free_slots[i], base + i, q, ps);
+        else start_path(sc0, sg, g, ps.free_slots[i], base + i, q, ps);
+    }
+}
+
+__global__ void k_refill_commit(int npass, int cnt_q, int* counts) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const int n_free = counts[kCntFree], next = counts[kCntNext];
+        const int n_new = min(n_free, npass - next);
+        counts[cnt_q] += n_new;
+        counts[kCntNext] = next + n_new;
+        counts[kCntFree] = 0;   // unused free slots only remain once the pass has run dry
+    }
+}
+
+hipError_t launch_refill(const DevScene& sc, const DevScene* scenes, const BatchSeg* segs, int nseg, int npass,
+                         int pool, RayQueue q, int cnt_q, PathState ps, int* counts, hipStream_t s) {
+    int blocks = (pool + 255) / 256;
     if (blocks > persistent_blocks()) blocks = persistent_blocks();
-    hipLaunchKernelGGL(k_raygen_batch, dim3(blocks), dim3(256), 0, s, scenes, segs, nseg, npaths, q, ps, counts);
+    if (scenes)
+        hipLaunchKernelGGL(k_refill<true>, dim3(blocks), dim3(256), 0, s, sc, scenes, segs, nseg, npass, q, cnt_q, ps,
+                           counts);
+    else
+        hipLaunchKernelGGL(k_refill<false>, dim3(blocks), dim3(256), 0, s, sc, scenes, segs, nseg, npass, q, cnt_q, ps,
+                           counts);
+    hipLaunchKernelGGL(k_refill_commit, dim3(1), dim3(64), 0, s, npass, cnt_q, counts);
     return hipGetLastError();
 }
 
-hipError_t launch_raygen(const DevScene& sc, const WaveParams& wp, int npaths, RayQueue q, PathState ps,
-                         int* counts, hipStream_t s) {
-    int blocks = (npaths + 255) / 256;
+// Every slot free, the pass cursor at zero, queue `cnt_q` empty.
+__global__ void k_init_pool(PathState ps, int pool, int* counts, int cnt_q) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < pool; i += gridDim.x * 256) ps.free_slots[i] = i;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counts[kCntFree] = pool;
+        counts[kCntNext] = 0;
+        counts[cnt_q] = 0;
+    }
+}
+
+hipError_t launch_init_pool(PathState ps, int pool, int* counts, int cnt_q, hipStream_t s) {
+    int blocks = (pool + 255) / 256;
     if (blocks > persistent_blocks()) blocks = persistent_blocks();
-    hipLaunchKernelGGL(k_raygen, dim3(blocks), dim3(256), 0, s, sc, wp, npaths, q, ps, counts);
+    hipLaunchKernelGGL(k_init_pool, dim3(blocks), dim3(256), 0, s, ps, pool, counts, cnt_q);
     return hipGetLastError();
 }
 
@@ -656,6 +669,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __r
             }
             st3(ps.L, pid, L);
             st3(ps.beta, pid, beta);
+            if (!want_ext) nflags |= kFlagDone;
             ps.flags[pid] = nflags;
             ps.rng_n[pid] = rng.n;
             ps.depth[pid] = depth;
@@ -698,36 +712,50 @@ hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin
 }
 
 // radiance += coeff * SampleLightToEstimateDirect (PathTracer.cs:226, :273-307)
-// for Mis; radiance += coeff * EstimateLight / lightPdf (:81) for Nee.
+// for Mis; radiance += coeff * EstimateLight / lightPdf (:81) for Nee.  A
+// path that ended this bounce then retires: its radiance goes to the pass's
+// per-sample result and its slot to the free list.
 __global__ __launch_bounds__(256) void k_resolve(int method, RayQueue qin, PathState ps, int* counts, int cnt_in) {
     int n = counts[cnt_in];
-    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
-        int pid = qin.pid[k];
-        int flags = ps.flags[pid];
-        if (!(flags & kFlagNeePending)) continue;
-        float4 nb = ps.nee_beta[pid];
-        f3 beta = mk(nb.x, nb.y, nb.z);
-        float selpdf = nb.w;
-        bool vis = (flags & kFlagShadowIssued) && ps.vis[pid];
-        f3 L = ld3(ps.L, pid);
-        if (method == 2) {
-            f3 le = mk(0.0f);
-            if (vis) le = le + ld3(ps.nee_le, pid);
-            if (flags & kFlagMisIssued) {
-                float4 li4 = ps.mis_li[pid];
-                f3 li = mk(li4.x, li4.y, li4.z);
-                if (!is_black(li)) {
-                    float4 f4 = ps.mis_f[pid];
-                    le = le + mk(f4.x, f4.y, f4.z) * li * f4.w / li4.w;
+    for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        int k = base + threadIdx.x;
+        bool done = false;
+        int pid = 0;
+        if (k < n) {
+            pid = qin.pid[k];
+            int flags = ps.flags[pid];
+            done = (flags & kFlagDone) != 0;
+            if (flags & kFlagNeePending) {
+                float4 nb = ps.nee_beta[pid];
+                f3 beta = mk(nb.x, nb.y, nb.z);
+                float selpdf = nb.w;
+                bool vis = (flags & kFlagShadowIssued) && ps.vis[pid];
+                f3 L = ld3(ps.L, pid);
+                if (method == 2) {
+                    f3 le = mk(0.0f);
+                    if (vis) le = le + ld3(ps.nee_le, pid);
+                    if (flags & kFlagMisIssued) {
+                        float4 li4 = ps.mis_li[pid];
+                        f3 li = mk(li4.x, li4.y, li4.z);
+                        if (!is_black(li)) {
+                            float4 f4 = ps.mis_f[pid];
+                            le = le + mk(f4.x, f4.y, f4.z) * li * f4.w / li4.w;
+                        }
+                    }
+                    f3 result = mk(0.0f) + le / selpdf;
+                    L = L + beta * result;
+                } else {
+                    f3 e = vis ? ld3(ps.nee_le, pid) : mk(0.0f);
+                    L = L + beta * e / selpdf;
                 }
+                st3(ps.L, pid, L);
+                if (done) ps.res[ps.gid[pid]] = make_float4(L.x, L.y, L.z, 0.f);
+            } else if (done) {
+                ps.res[ps.gid[pid]] = ps.L[pid];
             }
-            f3 result = mk(0.0f) + le / selpdf;
-            L = L + beta * result;
-        } else {
-            f3 e = vis ? ld3(ps.nee_le, pid) : mk(0.0f);
-            L = L + beta * e / selpdf;
         }
-        st3(ps.L, pid, L);
+        int slot = wave_append(&counts[kCntFree], done);
+        if (done) ps.free_slots[slot] = pid;
     }
 }
 
@@ -736,38 +764,9 @@ hipError_t launch_resolve(int method, RayQueue qin, PathState ps, int* counts, i
     return hipGetLastError();
 }
 
-// Per-pixel running sums over this wave's samples, in sample order
+// Pass accumulation: for every segment of the pass, each of its pixels adds
+// the segment's samples in sample order to its request's sums
 // (BlockBasedRenderer.cs:140-146: invalid samples count as 0).
-__global__ __launch_bounds__(256) void k_accumulate(WaveParams wp, PathState ps, float4* sum, float4* sum2,
-                                                    int* counts) {
-    uint64_t invalid = 0;
-    for (int base = blockIdx.x * 256; base < wp.npix; base += gridDim.x * 256) {
-        int q = base + threadIdx.x;
-        if (q < wp.npix) {
-            float4 s4 = sum[q], q4 = sum2[q];
-            f3 s = mk(s4.x, s4.y, s4.z), s2 = mk(q4.x, q4.y, q4.z);
-            for (int j = 0; j < wp.nsamples; j++) {
-                f3 c = ld3(ps.L, j * wp.npix + q);
-                if (!valid_color(c)) { c = mk(0.0f); invalid++; }
-                s = s + c;
-                s2 = s2 + c * c;
-            }
-            sum[q] = make_float4(s.x, s.y, s.z, 0.f);
-            sum2[q] = make_float4(s2.x, s2.y, s2.z, 0.f);
-        }
-    }
-    add_counter((unsigned long long*)(counts + kCntInvalid), invalid);
-}
-
-hipError_t launch_accumulate(const WaveParams& wp, PathState ps, float4* sum, float4* sum2, int* counts, hipStream_t s) {
-    int blocks = (wp.npix + 255) / 256;
-    if (blocks > persistent_blocks()) blocks = persistent_blocks();
-    hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(256), 0, s, wp, ps, sum, sum2, counts);
-    return hipGetLastError();
-}
-
-// Batched accumulation: for every segment of the wave, each of its pixels
-// adds the segment's samples in sample order to the request's sums.
 __global__ __launch_bounds__(256) void k_accumulate_batch(const BatchSeg* segs, int nseg, PathState ps, float4* sum,
                                                           float4* sum2, int* counts) {
     uint64_t invalid = 0;
@@ -778,7 +777,7 @@ __global__ __launch_bounds__(256) void k_accumulate_batch(const BatchSeg* segs, 
             float4 s4 = sum[o], q4 = sum2[o];
             f3 s = mk(s4.x, s4.y, s4.z), s2 = mk(q4.x, q4.y, q4.z);
             for (int j = 0; j < sg.nsamples; j++) {
-                f3 c = ld3(ps.L, sg.path_offset + j * sg.npix + q);
+                f3 c = ld3(ps.res, sg.path_offset + j * sg.npix + q);
                 if (!valid_color(c)) { c = mk(0.0f); invalid++; }
                 s = s + c;
                 s2 = s2 + c * c;

@@ -219,6 +219,7 @@ __global__ __launch_bounds__(256) void k_shade_vol(DevScene sc0, const DevScene*
             // the next iteration starts with `if (ray.MaxT <= ray.MinT) break;` (:37-40)
             want_ext = alive && etmax > etmin;
             if (spec) nflags |= kFlagSpecular;
+            if (!want_ext) nflags |= kFlagDone;
             st3(ps.L, pid, L);
             st3(ps.beta, pid, beta);
             ps.flags[pid] = nflags;
@@ -371,37 +372,48 @@ hipError_t launch_chain_advance(const DevScene& sc, const DevScene* scenes, RayQ
 }
 
 // SampleLightToEstimateDirect{Surface,Medium} result for every path of the
-// bounce (VolumetricPathTracer.cs:121-147, 164-176, 199-215, 221-247, 264-311).
+// bounce (VolumetricPathTracer.cs:121-147, 164-176, 199-215, 221-247, 264-311);
+// a path that ended retires into the pass's per-sample result.
 __global__ __launch_bounds__(256) void k_resolve_vol(RayQueue qin, PathState ps, int* counts, int cnt_in) {
     int n = counts[cnt_in];
-    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
-        int pid = qin.pid[k];
-        int flags = ps.flags[pid];
-        if (!(flags & kFlagNeePending)) continue;
-        float4 nb = ps.nee_beta[pid];
-        f3 beta = mk(nb.x, nb.y, nb.z);
-        float selpdf = nb.w;
-        f3 le = mk(0.0f);
-        if (flags & kFlagShadowIssued) {
-            float4 li4 = ps.nee_li[pid];
-            f3 li = mk(li4.x, li4.y, li4.z) * ld3(ps.sh_tr, pid);
-            if (!is_black(li)) {
-                float4 f4 = ps.nee_le[pid];
-                le = le + mk(f4.x, f4.y, f4.z) * li * f4.w / li4.w;
+    for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        int k = base + threadIdx.x;
+        bool done = false;
+        int pid = 0;
+        if (k < n) {
+            pid = qin.pid[k];
+            int flags = ps.flags[pid];
+            done = (flags & kFlagDone) != 0;
+            f3 L = ld3(ps.L, pid);
+            if (flags & kFlagNeePending) {
+                float4 nb = ps.nee_beta[pid];
+                f3 beta = mk(nb.x, nb.y, nb.z);
+                float selpdf = nb.w;
+                f3 le = mk(0.0f);
+                if (flags & kFlagShadowIssued) {
+                    float4 li4 = ps.nee_li[pid];
+                    f3 li = mk(li4.x, li4.y, li4.z) * ld3(ps.sh_tr, pid);
+                    if (!is_black(li)) {
+                        float4 f4 = ps.nee_le[pid];
+                        le = le + mk(f4.x, f4.y, f4.z) * li * f4.w / li4.w;
+                    }
+                }
+                if (flags & kFlagMisIssued) {
+                    float4 li4 = ps.mis_li[pid];
+                    f3 li = mk(li4.x, li4.y, li4.z);
+                    if (!is_black(li)) {
+                        float4 f4 = ps.mis_f[pid];
+                        le = le + mk(f4.x, f4.y, f4.z) * li * ld3(ps.mis_tr, pid) * f4.w / li4.w;
+                    }
+                }
+                f3 result = mk(0.0f) + le / selpdf;
+                L = L + beta * result;
+                st3(ps.L, pid, L);
             }
+            if (done) ps.res[ps.gid[pid]] = make_float4(L.x, L.y, L.z, 0.f);
         }
-        if (flags & kFlagMisIssued) {
-            float4 li4 = ps.mis_li[pid];
-            f3 li = mk(li4.x, li4.y, li4.z);
-            if (!is_black(li)) {
-                float4 f4 = ps.mis_f[pid];
-                le = le + mk(f4.x, f4.y, f4.z) * li * ld3(ps.mis_tr, pid) * f4.w / li4.w;
-            }
-        }
-        f3 result = mk(0.0f) + le / selpdf;
-        f3 L = ld3(ps.L, pid);
-        L = L + beta * result;
-        st3(ps.L, pid, L);
+        int slot = wave_append(&counts[kCntFree], done);
+        if (done) ps.free_slots[slot] = pid;
     }
 }
 

@@ -12,6 +12,7 @@ enum : int {
     kFlagNeePending = 2,
     kFlagShadowIssued = 4,
     kFlagMisIssued = 8,
+    kFlagDone = 16,        // the path ended this bounce (retired after resolve)
 };
 
 // ---------------------------------------------------------------- wave helpers
@@ -47,19 +48,6 @@ __device__ __forceinline__ void add_class_counters(int* counts, int cls, uint64_
     add_counter((unsigned long long*)(counts + kCntClassPrims) + cls, nt);
 }
 
-__device__ __forceinline__ void wave_pixel(const WaveParams& wp, int p, int* x, int* y, int* sample) {
-    int s = p / wp.npix;
-    int q = p - s * wp.npix;
-    int yy = q / wp.crop_w;
-    *x = wp.x0 + (q - yy * wp.crop_w);
-    *y = wp.y0 + yy;
-    *sample = wp.sample_base + s;
-}
-__device__ __forceinline__ uint64_t wave_key(const WaveParams& wp, int p) {
-    int x, y, s;
-    wave_pixel(wp, p, &x, &y, &s);
-    return path_key(wp.seed, (uint32_t)(y * wp.width + x), (uint32_t)s);
-}
 __device__ __forceinline__ f3 ld3(const float4* a, int i) { float4 v = a[i]; return mk(v.x, v.y, v.z); }
 __device__ __forceinline__ void st3(float4* a, int i, f3 v, float w = 0.0f) { a[i] = make_float4(v.x, v.y, v.z, w); }
 

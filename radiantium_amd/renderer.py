@@ -88,10 +88,11 @@ class WavefrontRenderer:
         self._error = None
         self._t0 = None
         self._t1 = None
+        # one task per pass: a pass holds whole samples, up to 2^27 paths
         npix = scene.width * scene.height
-        target = wave_paths if wave_paths > 0 else (1 << 22)
-        spw = max(1, min(spp, target // max(npix, 1)))
-        self.AllTaskCount = (spp + spw - 1) // spw
+        pass_cap = max(min(1 << 27, npix * spp), npix)
+        per_pass = max(1, pass_cap // max(npix, 1))
+        self.AllTaskCount = (spp + per_pass - 1) // per_pass
 
     @property
     def IsCompleted(self):
