@@ -94,14 +94,19 @@ def pmc_traffic(stage, args):
         return None
 
 
-def batched_speedup(ctx, args):
+def batched_speedup(ctx, args, small=False):
     """Batched workload: 16 vol_path Cornell boxes with a glass sphere of
     homogeneous medium each (sigma_a ~ U[0.01,0.2], sigma_s ~ U[0.5,5],
     g ~ U[-0.3,0.9]), resolutions from {128^2..1024^2}, spp from {16,64,256},
     max depth 64.  Sequential = one rd_render_device per request; batched =
     one rd_render_batch_device over all of them.  Same outputs (bit-identical)."""
     from radiantium_amd import native, scenes
-    reqs = scenes.batch_requests(args.batch_seed)
+    if small:
+        # the same scene family with the small resolutions and spp only: the
+        # serving regime where one request alone cannot fill the GPU
+        reqs = scenes.batch_requests(args.batch_seed, resolutions=(128, 256), spps=(16, 64))
+    else:
+        reqs = scenes.batch_requests(args.batch_seed)
     sc = [native.Scene(ctx, d) for d, _, _ in reqs]
     outs = [ctx.alloc(d.camera["width"] * d.camera["height"] * 12) for d, _, _ in reqs]
     paths = sum(d.camera["width"] * d.camera["height"] * spp for d, spp, _ in reqs)
@@ -131,8 +136,8 @@ def batched_speedup(ctx, args):
     return {"speedup": round(seq / bat, 3), "sequential_ms": round(seq * 1000, 1), "batched_ms": round(bat * 1000, 1),
             "requests": len(reqs), "paths": paths, "rays": rays_bat, "rays_equal": rays_seq == rays_bat,
             "batched_mrays_s": round(rays_bat / bat / 1e6, 1),
-            "workload": f"16 vol_path glass-medium Cornell boxes, seed {args.batch_seed}, res 128^2-1024^2, "
-                        f"spp 16-256, depth 64"}
+            "workload": f"16 vol_path glass-medium Cornell boxes, seed {args.batch_seed}, "
+                        + ("res {128,256}^2, spp {16,64}" if small else "res 128^2-1024^2, spp 16-256") + ", depth 64"}
 
 
 def copy_calibration():
@@ -231,7 +236,10 @@ def main():
     if dom != trace_dom and roofline is not None:
         roofline["note"] = f"longest stage is {dom}; roofline reported for the longest traversal kernel"
 
-    batched = None if args.no_batch else batched_speedup(ctx, args)
+    batched = None
+    if not args.no_batch:
+        batched = batched_speedup(ctx, args)
+        batched["small_requests"] = batched_speedup(ctx, args, small=True)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -60,6 +60,7 @@ struct WaveBuffers {
     DevScene* scenes = nullptr;
     BatchSeg* segs = nullptr;       // device segment table
     BatchSeg* host_segs = nullptr;  // pinned staging for it
+    int2* blk = nullptr;            // pool compaction scratch
     static void free_list(std::vector<void*>& v) {
         for (void* p : v) (void)hipFree(p);
         v.clear();
@@ -230,12 +231,16 @@ int ensure_wave_buffers(rd_ctx* c, size_t pool, size_t results, size_t pixels, b
         ps.key = (uint64_t*)alloc(cap * sizeof(uint64_t));
         ps.req = (int*)alloc(cap * sizeof(int));
         ps.gid = (int*)alloc(cap * sizeof(int));
-        ps.free_slots = (int*)alloc(cap * sizeof(int));
+        ps.alive = (int*)alloc(cap * sizeof(int));
+        ps.ray_o = (float4*)alloc(cap * sizeof(float4));
+        ps.ray_d = (float4*)alloc(cap * sizeof(float4));
+        w.blk = (int2*)alloc(2 * ((cap + 1023) / 1024) * sizeof(int2));
         ps.rng_n = (uint32_t*)alloc(cap * sizeof(uint32_t));
         ps.depth = (int*)alloc(cap * sizeof(int));
         ps.vis = (uint8_t*)alloc(cap);
         ok = ok && w.hits && ps.L && ps.beta && ps.nee_beta && ps.nee_le && ps.mis_f && ps.mis_li && ps.flags &&
-             ps.sel_light && ps.key && ps.req && ps.gid && ps.free_slots && ps.rng_n && ps.depth && ps.vis;
+             ps.sel_light && ps.key && ps.req && ps.gid && ps.alive && ps.ray_o && ps.ray_d && w.blk && ps.rng_n &&
+             ps.depth && ps.vis;
         ps.env = nullptr; ps.nee_li = ps.sh_to = ps.sh_tr = ps.mis_tr = nullptr;
         if (ok && vol) {
             ok = mkq(w.qshadow2, kCntShadowB) && mkq(w.qmis2, kCntMisB);
@@ -751,7 +756,8 @@ struct PassLaunch {
 };
 
 // The bounce loop of one pass: every bounce traces and shades the queue,
-// retires finished paths and refills their slots with the pass's next paths;
+// retires finished paths, then compacts the pool into the next queue (slot
+// order) while starting the pass's next paths in the free slots;
 // one sync per bounce reads the next queue size (for vol_path also the chain
 // queue sizes).  Ends when the queue is empty (the pass has run dry).
 int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatile int32_t* cancel) {
@@ -759,9 +765,9 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
     hipStream_t st = c->stream;
     RayQueue qin = wb.qa, qout = wb.qb;
     int cin = kCntExtA, cout = kCntExtB;
-    HIP_TRY(launch_init_pool(wb.ps, L.pool, c->counts, cin, st));
+    HIP_TRY(launch_init_pool(wb.ps, L.pool, c->counts, st));
     HIP_TRY(tm.run("raygen", [&] {
-        return launch_refill(L.sc, L.scenes, L.segs, L.nseg, L.npass, L.pool, qin, cin, wb.ps, c->counts, st);
+        return launch_compact(L.sc, L.scenes, L.segs, L.nseg, L.npass, L.pool, wb.blk, qin, cin, wb.ps, c->counts, st);
     }));
     int rc = RD_OK;
     for (int bounce = 0; !rc; bounce++) {
@@ -772,8 +778,7 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
             // VolumetricPathTracer.Li: shade, then the Transmittance / IntersectTr
             // chains step by step until both chain queues drain
             HIP_TRY(tm.run("shade", [&] {
-                return launch_shade_vol(L.sc, L.scenes, qin, wb.hits, qout, wb.qshadow, wb.qmis, wb.ps, c->counts,
-                                        cin, cout, st);
+                return launch_shade_vol(L.sc, L.scenes, qin, wb.hits, wb.qshadow, wb.qmis, wb.ps, c->counts, cin, st);
             }));
             RayQueue sa = wb.qshadow, sb = wb.qshadow2, ma = wb.qmis, mb = wb.qmis2;
             int csa = kCntShadow, csb = kCntShadowB, cma = kCntMis, cmb = kCntMisB;
@@ -803,8 +808,7 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
             HIP_TRY(tm.run("resolve", [&] { return launch_resolve_vol(qin, wb.ps, c->counts, cin, st); }));
         } else {
             HIP_TRY(tm.run("shade", [&] {
-                return launch_shade(L.sc, L.scenes, qin, wb.hits, qout, wb.qshadow, wb.qmis, wb.ps, c->counts, cin,
-                                    cout, st);
+                return launch_shade(L.sc, L.scenes, qin, wb.hits, wb.qshadow, wb.qmis, wb.ps, c->counts, cin, st);
             }));
             if (L.method != RD_METHOD_BXDF) {
                 HIP_TRY(tm.run("trace_any", [&] {
@@ -818,7 +822,8 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
             HIP_TRY(tm.run("resolve", [&] { return launch_resolve(L.method, qin, wb.ps, c->counts, cin, st); }));
         }
         HIP_TRY(tm.run("raygen", [&] {
-            return launch_refill(L.sc, L.scenes, L.segs, L.nseg, L.npass, L.pool, qout, cout, wb.ps, c->counts, st);
+            return launch_compact(L.sc, L.scenes, L.segs, L.nseg, L.npass, L.pool, wb.blk, qout, cout, wb.ps,
+                                  c->counts, st);
         }));
         HIP_TRY(hipMemcpyAsync(c->host_counts + cout, c->counts + cout, sizeof(int), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));

@@ -27,7 +27,9 @@ struct PathState {
     uint64_t* key;      // RNG key of the path: path_key(seed, pixel, sample)
     int* req;           // request (scene) index of the path
     int* gid;           // the path's index in the current pass
-    int* free_slots;    // free-slot list of the pool (count at kCntFree)
+    int* alive;         // 1: the slot's path continues with ray_o / ray_d next bounce
+    float4* ray_o;      // next ray of the slot's path: {o.xyz, tmin}
+    float4* ray_d;      //                              {d.xyz, tmax}
     float4* res;        // per pass path: final radiance (read in sample order by accumulate)
     uint32_t* rng_n;
     int* depth;
@@ -59,7 +61,7 @@ enum : int {
     kCntClassNodes = 22,  // 3 x u64
     kCntClassPrims = 28,  // 3 x u64
     kCntShadowB = 34, kCntMisB = 35,   // vol_path chain queues, second buffers
-    kCntFree = 36,        // free slots of the path pool
+    kCntFree = 36,        // pass path index of the first path the last compaction started
     kCntNext = 37,        // next pass path to start
     kCntSlots = 38
 };
@@ -85,19 +87,20 @@ hipError_t launch_trace_rays(const DevScene& sc, const float4* rays, int n, floa
                              unsigned long long* counters, bool any, int* counts, hipStream_t s);
 
 // Wavefront path tracing stages.  A pass is a list of segments (BatchSeg);
-// its paths stream through a pool of `pool` slots: init_pool frees every
-// slot, refill starts new paths in free slots (appending them to a queue),
-// resolve retires finished paths into ps.res and the free list.
-hipError_t launch_init_pool(PathState ps, int pool, int* counts, int cnt_q, hipStream_t s);
-hipError_t launch_refill(const DevScene& sc, const DevScene* scenes, const BatchSeg* segs, int nseg, int npass,
-                         int pool, RayQueue q, int cnt_q, PathState ps, int* counts, hipStream_t s);
+// its paths stream through a pool of `pool` slots.  init_pool frees every
+// slot; compact builds the next extension queue in slot order from the slots
+// whose path continues and starts the pass's next paths in the free slots
+// (slot order, consecutive pass indices); resolve retires finished paths into
+// ps.res.  blk: 2 * ceil(pool / 1024) int2 of scratch.
+hipError_t launch_init_pool(PathState ps, int pool, int* counts, hipStream_t s);
+hipError_t launch_compact(const DevScene& sc, const DevScene* scenes, const BatchSeg* segs, int nseg, int npass,
+                          int pool, int2* blk, RayQueue q, int cnt_q, PathState ps, int* counts, hipStream_t s);
 hipError_t launch_trace_ext(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, float4* hits,
                             int* counts, int cnt_slot, int zero_a, int zero_b, int zero_c, hipStream_t s);
 // `scenes` (device array, may be null): batched mode, the path's scene is
 // scenes[ps.req[pid]]; otherwise every path uses `sc`.
 hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
-                        RayQueue qout, RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in,
-                        int cnt_out, hipStream_t s);
+                        RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in, hipStream_t s);
 hipError_t launch_accumulate_batch(const BatchSeg* segs, int nseg, int max_npix, PathState ps, float4* sum,
                                    float4* sum2, int* counts, hipStream_t s);
 hipError_t launch_trace_shadow(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,
@@ -112,8 +115,7 @@ hipError_t launch_resolve(int method, RayQueue qin, PathState ps, int* counts, i
 // closest-hit query (trace_chain) followed by chain_advance, which either
 // finishes the chain into the path state or appends its next segment.
 hipError_t launch_shade_vol(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
-                            RayQueue qout, RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in,
-                            int cnt_out, hipStream_t s);
+                            RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in, hipStream_t s);
 hipError_t launch_trace_chain(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, float4* hits,
                               bool mis, int* counts, int cnt_slot, int zero_slot, hipStream_t s);
 hipError_t launch_chain_advance(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,

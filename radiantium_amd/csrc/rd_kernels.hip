@@ -418,59 +418,164 @@ __device__ __forceinline__ void start_path(const DevScene& sc, const BatchSeg& s
     if (ps.env) ps.env[slot] = sc.global_medium;   // vol_path: rayEnv = Scene.GlobalMedium
 }
 
-// Starts min(free slots, paths left) new paths, appended to queue q after the
-// paths already in it; k_refill_commit then advances the counters.
+// ---- ordered compaction of the pool
+// Blocks of kCompactSlots slots (256 threads x 4 slots).  count: per block the
+// slots whose path continues and the free slots; scan: block offsets of the
+// free slots and of the queue entries (continuing slots + free slots that get
+// a new path, i.e. whose pool-wide free rank is below the paths left);
+// scatter: queue entries in slot order, new paths started in place.
+constexpr int kCompactSlots = 1024;
+
+// Exclusive scan of one int per thread over a block of kThreads threads;
+// *total receives the block sum.  tmp: kThreads / 64 ints of LDS.
+template <int kThreads>
+__device__ __forceinline__ int block_excl_scan(int v, int* tmp, int* total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) tmp[wid] = x;
+    __syncthreads();
+    int before = 0, tot = 0;
+    for (int w = 0; w < kThreads / 64; w++) {
+        int t = tmp[w];
+        if (w < wid) before += t;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return before + x - v;
+}
+
+__global__ __launch_bounds__(256) void k_pool_count(const int* __restrict__ alive, int pool, int2* blk) {
+    __shared__ int tmp[4];
+    int s0 = blockIdx.x * kCompactSlots + threadIdx.x * 4;
+    int na = 0, nf = 0;
+    for (int i = 0; i < 4; i++) {
+        int slot = s0 + i;
+        if (slot < pool) {
+            if (alive[slot]) na++;
+            else nf++;
+        }
+    }
+    int ta, tf;
+    block_excl_scan<256>(na, tmp, &ta);
+    block_excl_scan<256>(nf, tmp, &tf);
+    if (threadIdx.x == 0) blk[blockIdx.x] = make_int2(ta, tf);
+}
+
+__global__ __launch_bounds__(1024) void k_pool_scan(const int2* __restrict__ blk, int nblk, int npass, int* counts,
+                                                    int cnt_q, int2* off) {
+    __shared__ int tmp[16];
+    const int next = counts[kCntNext];
+    const int left = max(npass - next, 0);
+    const int per = (nblk + 1023) / 1024;
+    const int b0 = threadIdx.x * per;
+    int fsum = 0;
+    for (int i = 0; i < per; i++)
+        if (b0 + i < nblk) fsum += blk[b0 + i].y;
+    int ftot;
+    const int fpre = block_excl_scan<1024>(fsum, tmp, &ftot);
+    int ssum = 0, run = fpre;
+    for (int i = 0; i < per; i++) {
+        if (b0 + i >= nblk) break;
+        int2 c = blk[b0 + i];
+        ssum += c.x + min(max(left - run, 0), c.y);
+        run += c.y;
+    }
+    int stot;
+    const int spre = block_excl_scan<1024>(ssum, tmp, &stot);
+    run = fpre;
+    int srun = spre;
+    for (int i = 0; i < per; i++) {
+        if (b0 + i >= nblk) break;
+        int2 c = blk[b0 + i];
+        off[b0 + i] = make_int2(run, srun);
+        srun += c.x + min(max(left - run, 0), c.y);
+        run += c.y;
+    }
+    if (threadIdx.x == 0) {
+        counts[cnt_q] = stot;
+        counts[kCntFree] = next;
+        counts[kCntNext] = next + min(left, ftot);
+    }
+}
+
 template <bool kBatch>
-__global__ __launch_bounds__(256) void k_refill(DevScene sc0, const DevScene* __restrict__ scenes, const BatchSeg* segs,
-                                                int nseg, int npass, RayQueue q, int cnt_q, PathState ps, int* counts) {
-    const int n_free = counts[kCntFree], next = counts[kCntNext], base = counts[cnt_q];
-    const int n_new = min(n_free, npass - next);
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n_new; i += gridDim.x * 256) {
-        int g = next + i;
-        const BatchSeg sg = segs[find_seg(segs, nseg, g)];
-        if constexpr (kBatch) start_path(scenes[sg.req], sg, g, ps.free_slots[i], base + i, q, ps);
-        else start_path(sc0, sg, g, ps.free_slots[i], base + i, q, ps);
+__global__ __launch_bounds__(256) void k_pool_scatter(DevScene sc0, const DevScene* __restrict__ scenes,
+                                                      const BatchSeg* segs, int nseg, int npass, PathState ps,
+                                                      int pool, const int2* __restrict__ off, const int* counts,
+                                                      RayQueue q) {
+    __shared__ int tmp[4];
+    const int s0 = blockIdx.x * kCompactSlots + threadIdx.x * 4;
+    const int first = counts[kCntFree];
+    const int left = max(npass - first, 0);
+    const int2 o = off[blockIdx.x];
+    int st[4];
+    int nf = 0;
+    for (int i = 0; i < 4; i++) {
+        int slot = s0 + i;
+        st[i] = slot < pool ? ps.alive[slot] : -1;
+        if (st[i] == 0) nf++;
+    }
+    int tf;
+    const int fpre = block_excl_scan<256>(nf, tmp, &tf);
+    int frank = o.x + fpre, ns = 0;
+    for (int i = 0; i < 4; i++) {
+        if (st[i] == 1) ns++;
+        else if (st[i] == 0) { if (frank < left) ns++; frank++; }
+    }
+    int ts;
+    int pos = o.y + block_excl_scan<256>(ns, tmp, &ts);
+    frank = o.x + fpre;
+    for (int i = 0; i < 4; i++) {
+        const int slot = s0 + i;
+        if (st[i] == 1) {
+            q.pid[pos] = slot;
+            q.org[pos] = ps.ray_o[slot];
+            q.dir[pos] = ps.ray_d[slot];
+            pos++;
+        } else if (st[i] == 0) {
+            if (frank < left) {
+                const int g = first + frank;
+                const BatchSeg sg = segs[find_seg(segs, nseg, g)];
+                if constexpr (kBatch) start_path(scenes[sg.req], sg, g, slot, pos, q, ps);
+                else start_path(sc0, sg, g, slot, pos, q, ps);
+                ps.alive[slot] = 1;
+                pos++;
+            }
+            frank++;
+        }
     }
 }
 
-__global__ void k_refill_commit(int npass, int cnt_q, int* counts) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const int n_free = counts[kCntFree], next = counts[kCntNext];
-        const int n_new = min(n_free, npass - next);
-        counts[cnt_q] += n_new;
-        counts[kCntNext] = next + n_new;
-        counts[kCntFree] = 0;   // unused free slots only remain once the pass has run dry
-    }
-}
-
-hipError_t launch_refill(const DevScene& sc, const DevScene* scenes, const BatchSeg* segs, int nseg, int npass,
-                         int pool, RayQueue q, int cnt_q, PathState ps, int* counts, hipStream_t s) {
-    int blocks = (pool + 255) / 256;
-    if (blocks > persistent_blocks()) blocks = persistent_blocks();
+hipError_t launch_compact(const DevScene& sc, const DevScene* scenes, const BatchSeg* segs, int nseg, int npass,
+                          int pool, int2* blk, RayQueue q, int cnt_q, PathState ps, int* counts, hipStream_t s) {
+    const int nblk = (pool + kCompactSlots - 1) / kCompactSlots;
+    int2* off = blk + nblk;
+    hipLaunchKernelGGL(k_pool_count, dim3(nblk), dim3(256), 0, s, ps.alive, pool, blk);
+    hipLaunchKernelGGL(k_pool_scan, dim3(1), dim3(1024), 0, s, blk, nblk, npass, counts, cnt_q, off);
     if (scenes)
-        hipLaunchKernelGGL(k_refill<true>, dim3(blocks), dim3(256), 0, s, sc, scenes, segs, nseg, npass, q, cnt_q, ps,
-                           counts);
+        hipLaunchKernelGGL(k_pool_scatter<true>, dim3(nblk), dim3(256), 0, s, sc, scenes, segs, nseg, npass, ps, pool,
+                           off, counts, q);
     else
-        hipLaunchKernelGGL(k_refill<false>, dim3(blocks), dim3(256), 0, s, sc, scenes, segs, nseg, npass, q, cnt_q, ps,
-                           counts);
-    hipLaunchKernelGGL(k_refill_commit, dim3(1), dim3(64), 0, s, npass, cnt_q, counts);
+        hipLaunchKernelGGL(k_pool_scatter<false>, dim3(nblk), dim3(256), 0, s, sc, scenes, segs, nseg, npass, ps, pool,
+                           off, counts, q);
     return hipGetLastError();
 }
 
-// Every slot free, the pass cursor at zero, queue `cnt_q` empty.
-__global__ void k_init_pool(PathState ps, int pool, int* counts, int cnt_q) {
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < pool; i += gridDim.x * 256) ps.free_slots[i] = i;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        counts[kCntFree] = pool;
-        counts[kCntNext] = 0;
-        counts[cnt_q] = 0;
-    }
+// Every slot free and the pass cursor at zero.
+__global__ void k_init_pool(PathState ps, int pool, int* counts) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < pool; i += gridDim.x * 256) ps.alive[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) counts[kCntNext] = 0;
 }
 
-hipError_t launch_init_pool(PathState ps, int pool, int* counts, int cnt_q, hipStream_t s) {
+hipError_t launch_init_pool(PathState ps, int pool, int* counts, hipStream_t s) {
     int blocks = (pool + 255) / 256;
     if (blocks > persistent_blocks()) blocks = persistent_blocks();
-    hipLaunchKernelGGL(k_init_pool, dim3(blocks), dim3(256), 0, s, ps, pool, counts, cnt_q);
+    hipLaunchKernelGGL(k_init_pool, dim3(blocks), dim3(256), 0, s, ps, pool, counts);
     return hipGetLastError();
 }
 
@@ -514,8 +619,8 @@ hipError_t launch_trace_chain(const DevScene& sc, const DevScene* scenes, RayQue
 // every path in the extension queue (PathTracer.cs:43-270).
 template <bool kBatch>
 __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __restrict__ scenes, RayQueue qin,
-                                               const float4* __restrict__ hits, RayQueue qout, RayQueue qshadow,
-                                               RayQueue qmis, PathState ps, int* counts, int cnt_in, int cnt_out) {
+                                               const float4* __restrict__ hits, RayQueue qshadow, RayQueue qmis,
+                                               PathState ps, int* counts, int cnt_in) {
     int n = counts[cnt_in];
     // the traversal work cursors restart at zero for the next trace stages
     if (blockIdx.x == 0 && threadIdx.x < 3) counts[kCntWork + threadIdx.x] = 0;
@@ -676,14 +781,14 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __r
             };
             if constexpr (kBatch) run(scenes[ps.req[pid]]);
             else run(sc0);
+            // the continuation ray waits in the slot for the next compaction
+            ps.alive[pid] = want_ext ? 1 : 0;
+            if (want_ext) {
+                ps.ray_o[pid] = make_float4(eo.x, eo.y, eo.z, etmin);
+                ps.ray_d[pid] = make_float4(ed.x, ed.y, ed.z, etmax);
+            }
         }
-        int slot = wave_append(&counts[cnt_out], want_ext);
-        if (want_ext) {
-            qout.pid[slot] = pid;
-            qout.org[slot] = make_float4(eo.x, eo.y, eo.z, etmin);
-            qout.dir[slot] = make_float4(ed.x, ed.y, ed.z, etmax);
-        }
-        slot = wave_append(&counts[kCntShadow], want_shadow);
+        int slot = wave_append(&counts[kCntShadow], want_shadow);
         if (want_shadow) {
             qshadow.pid[slot] = pid;
             qshadow.org[slot] = make_float4(so.x, so.y, so.z, 0.0001f);
@@ -700,21 +805,20 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __r
 }
 
 hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
-                        RayQueue qout, RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in,
-                        int cnt_out, hipStream_t s) {
+                        RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in, hipStream_t s) {
     if (scenes)
-        hipLaunchKernelGGL(k_shade<true>, dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits, qout,
-                           qshadow, qmis, ps, counts, cnt_in, cnt_out);
+        hipLaunchKernelGGL(k_shade<true>, dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits, qshadow,
+                           qmis, ps, counts, cnt_in);
     else
-        hipLaunchKernelGGL(k_shade<false>, dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits, qout,
-                           qshadow, qmis, ps, counts, cnt_in, cnt_out);
+        hipLaunchKernelGGL(k_shade<false>, dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits, qshadow,
+                           qmis, ps, counts, cnt_in);
     return hipGetLastError();
 }
 
 // radiance += coeff * SampleLightToEstimateDirect (PathTracer.cs:226, :273-307)
 // for Mis; radiance += coeff * EstimateLight / lightPdf (:81) for Nee.  A
-// path that ended this bounce then retires: its radiance goes to the pass's
-// per-sample result and its slot to the free list.
+// path that ended this bounce then retires its radiance into the pass's
+// per-sample result (its slot is free for the next compaction).
 __global__ __launch_bounds__(256) void k_resolve(int method, RayQueue qin, PathState ps, int* counts, int cnt_in) {
     int n = counts[cnt_in];
     for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
@@ -754,8 +858,6 @@ __global__ __launch_bounds__(256) void k_resolve(int method, RayQueue qin, PathS
                 ps.res[ps.gid[pid]] = ps.L[pid];
             }
         }
-        int slot = wave_append(&counts[kCntFree], done);
-        if (done) ps.free_slots[slot] = pid;
     }
 }
 

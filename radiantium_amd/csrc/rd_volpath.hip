@@ -26,9 +26,8 @@ namespace rd {
 
 template <bool kBatch>
 __global__ __launch_bounds__(256) void k_shade_vol(DevScene sc0, const DevScene* __restrict__ scenes, RayQueue qin,
-                                                   const float4* __restrict__ hits, RayQueue qout, RayQueue qshadow,
-                                                   RayQueue qmis, PathState ps, int* counts, int cnt_in,
-                                                   int cnt_out) {
+                                                   const float4* __restrict__ hits, RayQueue qshadow, RayQueue qmis,
+                                                   PathState ps, int* counts, int cnt_in) {
     int n = counts[cnt_in];
     if (blockIdx.x == 0 && threadIdx.x < 3) counts[kCntWork + threadIdx.x] = 0;
     uint64_t nvert = 0;
@@ -229,14 +228,14 @@ __global__ __launch_bounds__(256) void k_shade_vol(DevScene sc0, const DevScene*
             };
             if constexpr (kBatch) run(scenes[ps.req[pid]]);
             else run(sc0);
+            // the continuation ray waits in the slot for the next compaction
+            ps.alive[pid] = want_ext ? 1 : 0;
+            if (want_ext) {
+                ps.ray_o[pid] = make_float4(eo.x, eo.y, eo.z, etmin);
+                ps.ray_d[pid] = make_float4(ed.x, ed.y, ed.z, etmax);
+            }
         }
-        int slot = wave_append(&counts[cnt_out], want_ext);
-        if (want_ext) {
-            qout.pid[slot] = pid;
-            qout.org[slot] = make_float4(eo.x, eo.y, eo.z, etmin);
-            qout.dir[slot] = make_float4(ed.x, ed.y, ed.z, etmax);
-        }
-        slot = wave_append(&counts[kCntShadow], want_shadow);
+        int slot = wave_append(&counts[kCntShadow], want_shadow);
         if (want_shadow) {
             qshadow.pid[slot] = pid;
             qshadow.org[slot] = make_float4(so.x, so.y, so.z, 0.0001f);
@@ -253,14 +252,13 @@ __global__ __launch_bounds__(256) void k_shade_vol(DevScene sc0, const DevScene*
 }
 
 hipError_t launch_shade_vol(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
-                            RayQueue qout, RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in,
-                            int cnt_out, hipStream_t s) {
+                            RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in, hipStream_t s) {
     if (scenes)
         hipLaunchKernelGGL(k_shade_vol<true>, dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
-                           qout, qshadow, qmis, ps, counts, cnt_in, cnt_out);
+                           qshadow, qmis, ps, counts, cnt_in);
     else
         hipLaunchKernelGGL(k_shade_vol<false>, dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
-                           qout, qshadow, qmis, ps, counts, cnt_in, cnt_out);
+                           qshadow, qmis, ps, counts, cnt_in);
     return hipGetLastError();
 }
 
@@ -412,8 +410,6 @@ __global__ __launch_bounds__(256) void k_resolve_vol(RayQueue qin, PathState ps,
             }
             if (done) ps.res[ps.gid[pid]] = make_float4(L.x, L.y, L.z, 0.f);
         }
-        int slot = wave_append(&counts[kCntFree], done);
-        if (done) ps.free_slots[slot] = pid;
     }
 }
 
