@@ -308,6 +308,31 @@ int rd_render_batch(rd_ctx* ctx, int32_t n, const rd_request* reqs, int32_t wave
 int rd_render_batch_device(rd_ctx* ctx, int32_t n, const rd_request* reqs, int32_t wave_paths,
                            volatile int32_t* cancel, volatile int32_t* progress);
 
+/* ---- differentiation: path-replay backpropagation ------------------------ */
+/* Gradient of an image loss with respect to the scene's texture parameters
+ * (no counterpart in the reference, which renders only).  Given
+ * dl_dimage = dLoss/dImage (float[W*H*3], layout as rd_render's output, for
+ * the image rd_render produces with the same params), adds dLoss/dtexel to
+ * tex_grads[t] (one buffer per scene texture, NULL = no gradient; w*h*3
+ * floats laid out as the texture, 3 for a constant).  Differentiable
+ * parameters: the albedo texture of "diffuse" (kd) and "rough_plastic" (r),
+ * and the roughness texture of "rough_plastic" (its first channel).  Path
+ * integrator only.  Each sample's path is replayed twice with the render's
+ * random numbers: the primal replay gives the path's radiance, the adjoint
+ * replay propagates dLoss/dradiance back through the BSDF factors
+ * (Vicini, Speierer, Jakob 2021).  Sampling, MIS weights and Russian
+ * roulette are held fixed (detached).  rd_adjoint: host buffers;
+ * rd_adjoint_device: device buffers (tex_grads is a host array of device
+ * pointers). */
+int rd_adjoint(rd_ctx* ctx, rd_scene* scene, const rd_render_params* params, const float* dl_dimage,
+               float* const* tex_grads);
+int rd_adjoint_device(rd_ctx* ctx, rd_scene* scene, const rd_render_params* params, const float* d_dl_dimage,
+                      float* const* d_tex_grads);
+/* Replaces texture `tex`'s texels (w*h*3 floats, or 3 for a constant), e.g.
+ * after an optimiser step.  Environment-map textures cannot be replaced. */
+int rd_scene_set_texture(rd_ctx* ctx, rd_scene* scene, int32_t tex, const float* data);
+int rd_scene_set_texture_device(rd_ctx* ctx, rd_scene* scene, int32_t tex, const float* d_data);
+
 /* ---- device memory helpers (so hosts need no GPU framework) ------------- */
 void* rd_device_alloc(rd_ctx* ctx, uint64_t bytes);
 int rd_device_free(rd_ctx* ctx, void* ptr);

@@ -2079,6 +2079,198 @@ static C3 li_vol(const cr_scene& s, Ray ray, Rng& rng, uint64_t* st) {
     return radiance;
 }
 
+// ---------------------------------------------------------------- adjoint
+// CPU reference for rd_adjoint: path-replay backpropagation (Vicini, Speierer,
+// Jakob 2021) of dLoss/dradiance to the albedo texture of "diffuse" /
+// "rough_plastic" and the roughness texture of "rough_plastic", with sampling,
+// MIS weights and Russian roulette held fixed.  Diffuse and plastic values are
+// linear in R per channel, so a term C with f(R) has dC_c/dR_c = C_c / R_c;
+// plastic roughness rho scales both alphas by s = rho^2 and only the glossy
+// lobe's D * G depends on it: dC/drho = C * (glossy / f) * s dln(DG)/ds * 2 / rho.
+static float sdlog_D(const MfDist& m, V3 wh) {                 // s dln(D)/ds
+    float t2 = tan2_theta(wh);
+    if (std::isinf(t2)) return 0.0f;
+    float e = (cos2_phi(wh) / (m.ax * m.ax) + sin2_phi(wh) / (m.ay * m.ay)) * t2;
+    if (m.kind == 0) return 2 * e - 2;                          // D ~ exp(-e) / (ax ay)
+    return -2 * (1 - e) / (1 + e);                              // D ~ 1 / (ax ay (1 + e)^2)
+}
+static float sdlambda(const MfDist& m, V3 w) {                  // s dLambda/ds
+    float at = std::fabs(tan_theta(w));
+    if (std::isinf(at)) return 0.0f;
+    float alpha = std::sqrt(cos2_phi(w) * m.ax * m.ax + sin2_phi(w) * m.ay * m.ay);
+    if (m.kind == 0) {
+        float a = 1 / (alpha * at);
+        if (a >= 1.6f) return 0.0f;
+        float num = 1 - 1.259f * a + 0.396f * a * a, den = 3.535f * a + 2.181f * a * a;
+        float dnum = -1.259f + 0.792f * a, dden = 3.535f + 4.362f * a;
+        return -a * (dnum * den - num * dden) / (den * den);   // a ~ 1/s
+    }
+    float k = (alpha * at) * (alpha * at);                      // k ~ s^2
+    return k / (2 * std::sqrt(1 + k));
+}
+// dC/drho / C for a term with f(wo, wi) of material m at uv (0 unless rough_plastic).
+static float rho_ratio(const cr_scene& s, const rd_material_desc& m, V2 uv, V3 wo, V3 wi) {
+    if (m.type != RD_MAT_ROUGH_PLASTIC) return 0.0f;
+    float rho = tex_at(s, m.tex[1], uv).r;
+    if (!(rho > 0)) return 0.0f;
+    bool two = m.two_sided != 0;
+    if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!two) return 0.0f; }
+    if (!same_hemisphere(wo, wi)) return 0.0f;
+    MfDist d(m.dist, rho, tex_at(s, m.tex[2], uv).r);
+    float dw = m.kd / PI_F, gw = 0.0f, sd = 0.0f;
+    float cto = abs_cos_theta(wo), cti = abs_cos_theta(wi);
+    V3 wh = wi + wo;
+    if (!(cti == 0 || cto == 0) && !(wh.x == 0 && wh.y == 0 && wh.z == 0)) {
+        wh = normalize(wh);
+        float F = fresnel_dielectric(dot(wi, faceforward(wh, V3(0, 0, 1))), m.eta_a, m.eta_b);
+        float D = d.D(wh), G = d.G(wo, wi);
+        gw = m.ks * std::fabs((D * G * F) / (4 * cti * cto));
+        sd = sdlog_D(d, wh) - G * (sdlambda(d, wo) + sdlambda(d, wi));
+    }
+    float f = dw + gw;
+    if (!(f > 0)) return 0.0f;
+    return (gw / f) * sd * 2 / rho;
+}
+
+struct AdjGrads {
+    std::vector<std::vector<double>> g;   // per texture: w*h*3 (3 for a constant); empty = not requested
+};
+// Adds dLoss/dR (RGB) and dLoss/drho at material `mi`'s lookups at uv.
+static void adj_scatter(const cr_scene& s, int mi, V2 uv, C3 gr, float grho, AdjGrads& G) {
+    const rd_material_desc& m = s.mats[mi];
+    auto add = [&](int t, int ch, double v) {
+        if (t < 0 || G.g[t].empty()) return;
+        const Tex& tx = s.tex[t];
+        if (tx.w == 0) { G.g[t][ch] += v; return; }
+        // TextureSampler.Sample's texel reads (TextureSampler.cs:28-72)
+        auto wrapf = [&](float a) { return tx.wrap == 0 ? nclamp(a, 0.0f, 1.0f) : nclamp(a - std::floor(a), 0.0f, 1.0f); };
+        float u = wrapf(uv.x), v2 = wrapf(uv.y);
+        float fu = u * (tx.w - 1), fv = v2 * (tx.h - 1);
+        int pu = (int)fu, pv = (int)fv;
+        if (tx.filter == 0) { G.g[t][((size_t)pu * tx.h + pv) * 3 + ch] += v; return; }
+        int apu = iclamp(pu + ((fu > pu + 0.5f) ? 1 : -1), 0, tx.w - 1);
+        int apv = iclamp(pv + ((fv > pv + 0.5f) ? 1 : -1), 0, tx.h - 1);
+        float du = nmin(std::fabs(fu - pu - 0.5f), 1.0f), dv = nmin(std::fabs(fv - pv - 0.5f), 1.0f);
+        G.g[t][((size_t)pu * tx.h + pv) * 3 + ch] += v * ((1 - du) * (1 - dv));
+        G.g[t][((size_t)apu * tx.h + pv) * 3 + ch] += v * (du * (1 - dv));
+        G.g[t][((size_t)pu * tx.h + apv) * 3 + ch] += v * ((1 - du) * dv);
+        G.g[t][((size_t)apu * tx.h + apv) * 3 + ch] += v * (du * dv);
+    };
+    add(m.tex[0], 0, gr.r);
+    add(m.tex[0], 1, gr.g);
+    add(m.tex[0], 2, gr.b);
+    if (m.type == RD_MAT_ROUGH_PLASTIC && grho != 0.0f) add(m.tex[1], 0, grho);
+}
+
+// One replay of PathTracer.Li (PathTracer.cs:43-270) with the same random
+// numbers as pixel_sample, propagating delta = dLoss/dLi to the textures.
+static void li_adjoint(const cr_scene& s, Ray ray, Rng& rng, C3 ltot, C3 delta, AdjGrads& G) {
+    C3 radiance(0.0f), coeff(1.0f);
+    bool specular_path = true;
+    const rd_integrator_desc& I = s.integ;
+    const int method = I.method;
+    for (int bounces = 0;; bounces++) {
+        Inct it;
+        bool hit = scene_intersect(s, ray, &it, nullptr);
+        V3 wo = -ray.d;
+        if (method == RD_METHOD_BXDF) {
+            if (!hit) { radiance = radiance + coeff * eval_infinite(s, ray); break; }
+        } else {
+            if (bounces == 0 || specular_path) {
+                if (hit) radiance = radiance + coeff * inct_le(s, it, wo);
+                else radiance = radiance + coeff * eval_infinite(s, ray);
+            }
+            if (!hit) break;
+        }
+        if (I.max_depth != -1 && bounces >= I.max_depth) break;
+        int mat = s.prims[it.prim].material;
+        if (mat < 0) { ray = Ray(it.P, ray.d, ray.tmin); bounces--; continue; }
+        const rd_material_desc& m = s.mats[mat];
+        V3 wo_l = it.sh.to_local(wo);
+        C3 c_nee(0.0f), c_mis(0.0f);
+        float r_nee = 0, r_mis = 0, r_cont = 0;
+        if (method == RD_METHOD_BXDF) {
+            if (s.prims[it.prim].light >= 0) radiance = radiance + coeff * inct_le(s, it, wo);
+        } else if (!s.lights.empty()) {
+            int l = rng.next_int((int)s.lights.size());
+            float lpdf = 1.0f / s.lights.size();
+            LSample ls = light_sample_li(s, l, it.P, rng);
+            C3 le_nee(0.0f), le_mis(0.0f);
+            if (method == RD_METHOD_MIS) {                          // EstimateDirect (PathTracer.cs:309-399)
+                if (ls.pdf > 0.0f && !ls.li.black()) {
+                    C3 fr = material_fr(s, m, wo_l, it.sh.to_local(ls.wi), it.UV);
+                    float spdf = material_pdf(s, m, wo_l, it.sh.to_local(ls.wi), it.UV);
+                    if (!fr.black() && !scene_is_occluded(s, it.P, ls.p, nullptr)) {
+                        fr = fr * abs_cos_theta(it.sh.to_local(ls.wi));
+                        le_nee = fr * ls.li * power_heuristic(1, ls.pdf, 1, spdf) / ls.pdf;
+                        r_nee = rho_ratio(s, m, it.UV, wo_l, it.sh.to_local(ls.wi));
+                    }
+                }
+                BSample bs = material_sample(s, m, wo_l, it.UV, rng);
+                C3 fr = bs.fr * abs_cos_theta(bs.wi);
+                if (!fr.black() && bs.pdf > 0) {
+                    float w = 1;
+                    bool ok = true;
+                    if (!(bs.type & T_SPECULAR)) {
+                        float lp = light_pdf_li(s, l, it.P, it.sh.to_world(bs.wi));
+                        if (lp == 0.0f) ok = false;
+                        else w = power_heuristic(1, bs.pdf, 1, lp);
+                    }
+                    if (ok) {
+                        Ray to_light = spawn_ray(it, it.sh.to_world(bs.wi));
+                        Inct li_it;
+                        C3 li(0);
+                        if (scene_intersect(s, to_light, &li_it, nullptr)) {
+                            if (s.prims[li_it.prim].light == l) li = inct_le(s, li_it, it.sh.to_world(-bs.wi));
+                        } else {
+                            li = eval_infinite(s, to_light);
+                        }
+                        if (!li.black()) {
+                            le_mis = fr * li * w / bs.pdf;
+                            r_mis = rho_ratio(s, m, it.UV, wo_l, bs.wi);
+                        }
+                    }
+                }
+                C3 result = C3(0.0f) + (le_nee + le_mis) / lpdf;
+                radiance = radiance + coeff * result;
+            } else {                                                // NextEventEstimation.EstimateLight
+                if (ls.pdf > 0.0f && !scene_is_occluded(s, it.P, ls.p, nullptr)) {
+                    C3 fr = material_fr(s, m, wo_l, it.sh.to_local(ls.wi), it.UV);
+                    le_nee = fr * ls.li * abs_cos_theta(it.sh.to_local(ls.wi)) / ls.pdf;
+                    r_nee = rho_ratio(s, m, it.UV, wo_l, it.sh.to_local(ls.wi));
+                }
+                radiance = radiance + coeff * le_nee / lpdf;
+            }
+            c_nee = coeff * (le_nee / lpdf);
+            c_mis = coeff * (le_mis / lpdf);
+        }
+        BSample bs = material_sample(s, m, wo_l, it.UV, rng);
+        specular_path = (bs.type & T_SPECULAR) != 0;
+        bool cont = bs.pdf > 0.0f;
+        if (cont) {
+            r_cont = rho_ratio(s, m, it.UV, wo_l, bs.wi);
+            coeff = coeff * (bs.fr * abs_cos_theta(bs.wi) / bs.pdf);
+            ray = spawn_ray(it, it.sh.to_world(bs.wi));
+            if (bounces > I.min_depth) {
+                float q = nmin(coeff.max_elem(), I.rr_threshold);
+                if (rng.next_float() > q) cont = false;
+                else coeff = coeff / q;
+            }
+        }
+        if (m.type == RD_MAT_DIFFUSE || m.type == RD_MAT_ROUGH_PLASTIC) {
+            C3 R = tex_at(s, m.tex[0], it.UV);
+            C3 lr = ltot - radiance;                                // what the rest of the path adds
+            C3 t = c_nee + c_mis + lr;
+            C3 gr(R.r != 0 ? delta.r * t.r / R.r : 0.0f, R.g != 0 ? delta.g * t.g / R.g : 0.0f,
+                  R.b != 0 ? delta.b * t.b / R.b : 0.0f);
+            C3 q3 = c_nee * r_nee + c_mis * r_mis + lr * r_cont;
+            float grho = delta.r * q3.r + delta.g * q3.g + delta.b * q3.b;
+            adj_scatter(s, mat, it.UV, gr, grho, G);
+        }
+        if (!cont) break;
+    }
+}
+
 static C3 li(const cr_scene& s, const Ray& ray, Rng& rng, uint64_t* st) {
     if (s.integ.type == RD_INTEGRATOR_VOL_PATH) return li_vol(s, ray, rng, st);
     switch (s.integ.method) {
@@ -2348,6 +2540,59 @@ int cr_render(const cr_scene* s, const rd_render_params* p, int32_t threads, flo
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
     if (stats) for (int i = 0; i < 5; i++) stats[i] = tot[i];
     return RD_OK;
+}
+
+int cr_adjoint(const cr_scene* s, const rd_render_params* p, int32_t threads, const float* dl_dimage,
+               float* const* tex_grads) {
+    if (!s || !p || !dl_dimage || !tex_grads || p->spp <= 0) return RD_ERR_INVALID;
+    if (s->integ.type != RD_INTEGRATOR_PATH) { set_err("the adjoint covers the path integrator"); return RD_ERR_UNSUPPORTED; }
+    int W = s->cam.W, H = s->cam.H;
+    int x0 = p->x0, y0 = p->y0, x1 = p->x1, y1 = p->y1;
+    if (x0 == 0 && y0 == 0 && x1 == 0 && y1 == 0) { x1 = W; y1 = H; }
+    if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
+    std::atomic<int> next(x0);
+    std::mutex mu;
+    AdjGrads total;
+    total.g.resize(s->tex.size());
+    for (size_t t = 0; t < s->tex.size(); t++)
+        if (tex_grads[t]) total.g[t].assign(s->tex[t].w > 0 ? (size_t)s->tex[t].w * s->tex[t].h * 3 : 3, 0.0);
+    auto worker = [&]() {
+        AdjGrads G;
+        G.g.resize(total.g.size());
+        for (size_t t = 0; t < total.g.size(); t++) if (!total.g[t].empty()) G.g[t].assign(total.g[t].size(), 0.0);
+        while (true) {
+            int x = next.fetch_add(1);
+            if (x >= x1) break;
+            for (int y = y0; y < y1; y++) {
+                size_t o = ((size_t)x * H + y) * 3;
+                C3 delta = C3(dl_dimage[o], dl_dimage[o + 1], dl_dimage[o + 2]) / (float)p->spp;
+                for (int t = 0; t < p->spp; t++) {
+                    C3 ltot = pixel_sample(*s, p->seed, x, y, t, nullptr);   // the primal replay
+                    if (!ltot.valid()) continue;
+                    Rng rng;
+                    rng.key = path_key(p->seed, (uint32_t)(y * s->cam.W + x), (uint32_t)t);
+                    rng.n = 0;
+                    float ox = rng.next_float(), oy = rng.next_float();
+                    Ray ray = s->cam.sample_ray((float)x + ox, (float)y + oy);
+                    li_adjoint(*s, ray, rng, ltot, delta, G);
+                }
+            }
+        }
+        std::lock_guard<std::mutex> g(mu);
+        for (size_t t = 0; t < G.g.size(); t++)
+            for (size_t i = 0; i < G.g[t].size(); i++) total.g[t][i] += G.g[t][i];
+    };
+    std::vector<std::thread> pool;
+    for (int i = 0; i < threads; i++) pool.emplace_back(worker);
+    for (auto& th : pool) th.join();
+    for (size_t t = 0; t < total.g.size(); t++)
+        for (size_t i = 0; i < total.g[t].size(); i++) tex_grads[t][i] += (float)total.g[t][i];
+    return RD_OK;
+}
+
+float cr_rho_ratio(const cr_scene* s, int32_t material, const float* uv, const float* wo, const float* wi) {
+    if (!s || material < 0 || material >= (int)s->mats.size()) return 0.0f;
+    return rho_ratio(*s, s->mats[material], V2(uv[0], uv[1]), V3(wo[0], wo[1], wo[2]), V3(wi[0], wi[1], wi[2]));
 }
 
 int cr_sample_li(const cr_scene* s, uint32_t seed, int32_t x, int32_t y, int32_t sample, float* rgb) {

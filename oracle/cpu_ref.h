@@ -73,6 +73,14 @@ int cr_medium_sample(const rd_medium_desc* m, const float* o, const float* d, fl
 float cr_hg_p(float g, const float* wo, const float* wi);
 float cr_hg_sample(float g, const float* wo, uint64_t key, float* wi);
 
+/* Adjoint reference (the test reference for rd_adjoint): adds dLoss/dtexel
+ * to tex_grads[t] (NULL = skip) for dl_dimage = dLoss/dImage of the render
+ * with the same params; cr_rho_ratio probes dC/drho / C of a rough_plastic
+ * term f(wo, wi) at uv. */
+int cr_adjoint(const cr_scene* s, const rd_render_params* params, int32_t threads, const float* dl_dimage,
+               float* const* tex_grads);
+float cr_rho_ratio(const cr_scene* s, int32_t material, const float* uv, const float* wo, const float* wi);
+
 /* Closed-form probes: Fresnel.DielectricFunc / ConductorFunc, and the
  * microfacet distribution (kind 0 Beckmann, 1 GGX): out = {D(wh), G(wo,wi),
  * Pdf(wo,wh)}; sample_wh draws with the given RNG key. */

@@ -67,6 +67,9 @@ def lib():
         L.cr_hg_p.argtypes = [C.c_float, C.c_void_p, C.c_void_p]
         L.cr_hg_sample.restype = C.c_float
         L.cr_hg_sample.argtypes = [C.c_float, C.c_void_p, C.c_uint64, C.c_void_p]
+        L.cr_adjoint.argtypes = [C.c_void_p, C.POINTER(abi.rd_render_params), C.c_int32, C.c_void_p, C.c_void_p]
+        L.cr_rho_ratio.restype = C.c_float
+        L.cr_rho_ratio.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.cr_path_key.restype = C.c_uint64
         L.cr_path_key.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
         _lib = L
@@ -253,6 +256,30 @@ class OracleScene:
         out = np.zeros(3, np.float32)
         lib().cr_sample_li(self.h, seed, x, y, sample, _ptr(out))
         return out
+
+    def adjoint(self, spp, dl_dimage, textures, seed=0, crop=None, threads=0):
+        """dLoss/dtexel for the listed texture indices: {tex: float32 [n*3]}."""
+        p = abi.rd_render_params()
+        p.spp, p.seed = spp, seed
+        if crop is not None:
+            p.x0, p.y0, p.x1, p.y1 = crop
+        dl = np.ascontiguousarray(dl_dimage, np.float32)
+        ntex = len(self.desc.textures)
+        out = {}
+        ptrs = (C.c_void_p * max(ntex, 1))()
+        for t in textures:
+            tx = self.desc.textures[t]
+            n = tx.image.shape[0] * tx.image.shape[1] if tx.image is not None else 1
+            out[t] = np.zeros(n * 3, np.float32)
+            ptrs[t] = out[t].ctypes.data
+        rc = lib().cr_adjoint(self.h, C.byref(p), threads, _ptr(dl), ptrs)
+        if rc != 0:
+            raise RuntimeError("oracle adjoint: " + lib().cr_last_error().decode())
+        return out
+
+    def rho_ratio(self, mat, wo, wi, uv=(0.0, 0.0)):
+        wo = np.asarray(wo, np.float32); wi = np.asarray(wi, np.float32); uv = np.asarray(uv, np.float32)
+        return float(lib().cr_rho_ratio(self.h, mat, _ptr(uv), _ptr(wo), _ptr(wi)))
 
     def env_le(self, d, env=0):
         d = np.asarray(d, np.float32); out = np.zeros(3, np.float32)

@@ -61,6 +61,10 @@ struct WaveBuffers {
     BatchSeg* segs = nullptr;       // device segment table
     BatchSeg* host_segs = nullptr;  // pinned staging for it
     int2* blk = nullptr;            // pool compaction scratch
+    size_t adj_cap = 0;             // adjoint per-slot arrays
+    std::vector<void*> adj_allocs;
+    void** grad_ptrs = nullptr;     // adjoint: device table of gradient accumulators
+    size_t grad_cap = 0;
     static void free_list(std::vector<void*>& v) {
         for (void* p : v) (void)hipFree(p);
         v.clear();
@@ -71,6 +75,9 @@ struct WaveBuffers {
         free_list(pix_allocs);
         free_list(scene_allocs);
         free_list(seg_allocs);
+        free_list(adj_allocs);
+        adj_cap = grad_cap = 0;
+        grad_ptrs = nullptr;
         if (host_segs) (void)hipHostFree(host_segs);
         host_segs = nullptr;
         segs = nullptr;
@@ -108,6 +115,10 @@ struct rd_scene {
     std::vector<int32_t> order;
     rd_scene_info info{};
     int width = 0, height = 0;
+    int num_textures = 0;
+    std::vector<DevTexture> texs;       // host copy of the texture table
+    std::vector<size_t> tex_elems;      // texels per texture (1 for constants)
+    std::vector<int> env_textures;      // textures an environment light samples
     ~rd_scene() {
         for (void* p : allocs) (void)hipFree(p);
         if (ctx) ctx->live_scenes--;
@@ -231,6 +242,7 @@ int ensure_wave_buffers(rd_ctx* c, size_t pool, size_t results, size_t pixels, b
         ps.key = (uint64_t*)alloc(cap * sizeof(uint64_t));
         ps.req = (int*)alloc(cap * sizeof(int));
         ps.gid = (int*)alloc(cap * sizeof(int));
+        ps.pix = (int*)alloc(cap * sizeof(int));
         ps.alive = (int*)alloc(cap * sizeof(int));
         ps.ray_o = (float4*)alloc(cap * sizeof(float4));
         ps.ray_d = (float4*)alloc(cap * sizeof(float4));
@@ -239,7 +251,7 @@ int ensure_wave_buffers(rd_ctx* c, size_t pool, size_t results, size_t pixels, b
         ps.depth = (int*)alloc(cap * sizeof(int));
         ps.vis = (uint8_t*)alloc(cap);
         ok = ok && w.hits && ps.L && ps.beta && ps.nee_beta && ps.nee_le && ps.mis_f && ps.mis_li && ps.flags &&
-             ps.sel_light && ps.key && ps.req && ps.gid && ps.alive && ps.ray_o && ps.ray_d && w.blk && ps.rng_n &&
+             ps.sel_light && ps.key && ps.req && ps.gid && ps.pix && ps.alive && ps.ray_o && ps.ray_d && w.blk && ps.rng_n &&
              ps.depth && ps.vis;
         ps.env = nullptr; ps.nee_li = ps.sh_to = ps.sh_tr = ps.mis_tr = nullptr;
         if (ok && vol) {
@@ -252,9 +264,11 @@ int ensure_wave_buffers(rd_ctx* c, size_t pool, size_t results, size_t pixels, b
             ps.mis_tr = (float4*)alloc(cap * sizeof(float4));
             ok = ok && w.hits2 && ps.env && ps.nee_li && ps.sh_to && ps.sh_tr && ps.mis_tr;
         }
+        ps.adj0 = ps.adj1 = ps.adj2 = nullptr;
         if (!ok) { w.release(); return fail(RD_ERR_OOM, "hipMalloc failed for the path pool"); }
         w.cap = cap;
         w.vol = vol;
+        w.adj_cap = 0;
     }
     if (results > w.res_cap) {
         WaveBuffers::free_list(w.res_allocs);
@@ -271,6 +285,32 @@ int ensure_wave_buffers(rd_ctx* c, size_t pool, size_t results, size_t pixels, b
         w.sum2 = (float4*)alloc_in(w.pix_allocs, pcap * sizeof(float4));
         if (!w.sum || !w.sum2) { w.release(); return fail(RD_ERR_OOM, "hipMalloc failed for accumulation"); }
         w.pix_cap = pcap;
+    }
+    return RD_OK;
+}
+
+// Adjoint replay: per-slot derivative records and a table of `ntex` gradient pointers.
+int ensure_adjoint_buffers(rd_ctx* c, size_t ntex) {
+    WaveBuffers& w = c->wb;
+    if (w.adj_cap < w.cap || w.grad_cap < ntex) {
+        WaveBuffers::free_list(w.adj_allocs);
+        w.adj_cap = w.grad_cap = 0;
+        auto alloc = [&](size_t bytes) -> void* {
+            void* p = nullptr;
+            if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+            w.adj_allocs.push_back(p);
+            return p;
+        };
+        w.ps.adj0 = (float4*)alloc(w.cap * sizeof(float4));
+        w.ps.adj1 = (float4*)alloc(w.cap * sizeof(float4));
+        w.ps.adj2 = (float4*)alloc(w.cap * sizeof(float4));
+        w.grad_ptrs = (void**)alloc(std::max<size_t>(ntex, 1) * sizeof(void*));
+        if (!w.ps.adj0 || !w.ps.adj1 || !w.ps.adj2 || !w.grad_ptrs) {
+            WaveBuffers::free_list(w.adj_allocs);
+            return fail(RD_ERR_OOM, "hipMalloc failed for adjoint buffers");
+        }
+        w.adj_cap = w.cap;
+        w.grad_cap = std::max<size_t>(ntex, 1);
     }
     return RD_OK;
 }
@@ -520,6 +560,12 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
             texels.insert(texels.end(), t.data, t.data + (size_t)t.width * t.height * 3);
         }
     }
+    s->num_textures = d->num_textures;
+    s->texs = texs;
+    for (int i = 0; i < d->num_textures; i++)
+        s->tex_elems.push_back(texs[i].w > 0 ? (size_t)texs[i].w * texs[i].h : 1);
+    for (int i = 0; i < d->num_lights; i++)
+        if (d->lights[i].type == RD_LIGHT_INFINITE) s->env_textures.push_back(d->lights[i].tex);
     std::vector<DevLight> lights(d->num_lights);
     std::vector<DevEnv> envs;
     std::vector<EnvTables> env_tables;
@@ -747,6 +793,7 @@ namespace {
 struct PassLaunch {
     DevScene sc;
     const DevScene* scenes;
+    const AdjArgs* adj;   // adjoint replay (single scene, path integrator), else null
     int method;
     bool vol;
     const BatchSeg* segs;
@@ -808,7 +855,8 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
             HIP_TRY(tm.run("resolve", [&] { return launch_resolve_vol(qin, wb.ps, c->counts, cin, st); }));
         } else {
             HIP_TRY(tm.run("shade", [&] {
-                return launch_shade(L.sc, L.scenes, qin, wb.hits, wb.qshadow, wb.qmis, wb.ps, c->counts, cin, st);
+                return launch_shade(L.sc, L.scenes, qin, wb.hits, wb.qshadow, wb.qmis, wb.ps, c->counts, cin,
+                                    L.adj != nullptr, st);
             }));
             if (L.method != RD_METHOD_BXDF) {
                 HIP_TRY(tm.run("trace_any", [&] {
@@ -819,7 +867,9 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
                         return launch_trace_mis(L.sc, L.scenes, wb.qmis, wb.ps, c->counts, st);
                     }));
             }
-            HIP_TRY(tm.run("resolve", [&] { return launch_resolve(L.method, qin, wb.ps, c->counts, cin, st); }));
+            HIP_TRY(tm.run(L.adj ? "resolve_adjoint" : "resolve", [&] {
+                return launch_resolve(L.sc, L.method, qin, wb.ps, c->counts, cin, L.adj, st);
+            }));
         }
         HIP_TRY(tm.run("raygen", [&] {
             return launch_compact(L.sc, L.scenes, L.segs, L.nseg, L.npass, L.pool, wb.blk, qout, cout, wb.ps,
@@ -965,6 +1015,7 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
                 sg.nsamples = k;
                 sg.x0 = cr.x0; sg.y0 = cr.y0; sg.crop_w = cw; sg.npix = (int)np;
                 sg.width = reqs[i].scene->width;
+                sg.height = reqs[i].scene->height;
                 sg.seed = reqs[i].params.seed;
                 sg.sum_offset = (int)sum_off[i];
                 segs.push_back(sg);
@@ -979,7 +1030,7 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
             HIP_TRY(hipStreamSynchronize(st));
             std::memcpy(wb.host_segs, segs.data(), sizeof(BatchSeg) * segs.size());
             HIP_TRY(hipMemcpyAsync(wb.segs, wb.host_segs, sizeof(BatchSeg) * segs.size(), hipMemcpyHostToDevice, st));
-            PassLaunch L{rep, d_scenes, kinds[g].second, kinds[g].first == RD_INTEGRATOR_VOL_PATH, wb.segs,
+            PassLaunch L{rep, d_scenes, nullptr, kinds[g].second, kinds[g].first == RD_INTEGRATOR_VOL_PATH, wb.segs,
                          (int)segs.size(), (int)used, (int)std::min(pool, used)};
             rc = run_pass(c, tm, L, &max_bounce, cancel);
             if (rc) break;
@@ -1034,6 +1085,161 @@ int rd_render_batch_device(rd_ctx* c, int32_t n, const rd_request* reqs, int32_t
     if (!c || n < 0 || (n > 0 && !reqs)) return fail(RD_ERR_INVALID, "bad argument");
     if (n == 0) return RD_OK;
     return render_requests(c, n, reqs, false, wave_paths, cancel, progress);
+}
+
+// Path-replay backpropagation of an image loss to the scene's texture
+// parameters.  Every pass replays its paths twice with the primal's random
+// numbers: first the primal (per-sample radiance into the result buffer),
+// then the adjoint, whose resolve scatters dLoss/dtexel.
+int rd_adjoint_device(rd_ctx* c, rd_scene* s, const rd_render_params* p, const float* d_dl_dimage,
+                      float* const* d_tex_grads) {
+    if (!c || !s || !p || !d_dl_dimage || !d_tex_grads) return fail(RD_ERR_INVALID, "null argument");
+    if (p->spp <= 0) return fail(RD_ERR_INVALID, "spp must be positive");
+    if (s->integ.type != RD_INTEGRATOR_PATH) return fail(RD_ERR_UNSUPPORTED, "the adjoint covers the path integrator");
+    if (int rc = check_stack(s)) return rc;
+    Crop cr;
+    if (int rc = resolve_crop(s, p, &cr)) return rc;
+    CtxLock lock(c);
+    if (!lock.ok()) return fail(RD_ERR_BUSY, "context in use by another thread");
+    (void)hipSetDevice(c->device);
+    const int cw = cr.x1 - cr.x0;
+    const size_t npix = (size_t)cw * (cr.y1 - cr.y0);
+    const size_t total = npix * (size_t)p->spp;
+    const size_t pool = std::min(p->wave_paths > 0 ? (size_t)p->wave_paths : kDefaultPool, total);
+    const size_t pass_cap = std::max(std::min(kPassPaths, total), npix);
+    if (int rc = ensure_wave_buffers(c, pool, pass_cap, npix, false)) return rc;
+    if (int rc = ensure_adjoint_buffers(c, (size_t)s->num_textures)) return rc;
+    WaveBuffers& wb = c->wb;
+    hipStream_t st = c->stream;
+    // float64 accumulators for the requested textures
+    std::vector<double*> acc(std::max(s->num_textures, 1), nullptr);
+    std::vector<void*> acc_allocs;
+    auto free_acc = [&] { for (void* q : acc_allocs) (void)hipFree(q); acc_allocs.clear(); };
+    for (int t = 0; t < s->num_textures; t++) {
+        if (!d_tex_grads[t]) continue;
+        size_t bytes = s->tex_elems[t] * 3 * sizeof(double);
+        void* q = nullptr;
+        if (hipMalloc(&q, bytes) != hipSuccess) { free_acc(); return fail(RD_ERR_OOM, "hipMalloc gradient accumulator"); }
+        acc_allocs.push_back(q);
+        acc[t] = (double*)q;
+        if (hipMemsetAsync(q, 0, bytes, st) != hipSuccess) { free_acc(); return fail(RD_ERR_DEVICE, "clear accumulator"); }
+    }
+    if (hipMemcpyAsync(wb.grad_ptrs, acc.data(), sizeof(double*) * acc.size(), hipMemcpyHostToDevice, st) != hipSuccess) {
+        free_acc();
+        return fail(RD_ERR_DEVICE, "upload gradient table");
+    }
+    AdjArgs aa{d_dl_dimage, (double* const*)wb.grad_ptrs, 1.0f / (float)p->spp, 0};
+    Timer tm(c);
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    HIP_TRY(launch_zero_counts(c->counts, kCntSlots, st));
+    hipEvent_t ev0, ev1;
+    HIP_TRY(hipEventCreate(&ev0));
+    HIP_TRY(hipEventCreate(&ev1));
+    HIP_TRY(hipEventRecord(ev0, st));
+    const int per_pass = (int)std::max<size_t>(1, pass_cap / npix);
+    int passes = 0, max_bounce = 0, rc = RD_OK;
+    for (int sb = 0; sb < p->spp && !rc; sb += per_pass) {
+        BatchSeg sg{};
+        sg.req = 0;
+        sg.path_offset = 0;
+        sg.sample_base = sb;
+        sg.nsamples = std::min(per_pass, p->spp - sb);
+        sg.x0 = cr.x0; sg.y0 = cr.y0; sg.crop_w = cw; sg.npix = (int)npix;
+        sg.width = s->width;
+        sg.height = s->height;
+        sg.seed = p->seed;
+        HIP_TRY(hipStreamSynchronize(st));
+        *wb.host_segs = sg;
+        HIP_TRY(hipMemcpyAsync(wb.segs, wb.host_segs, sizeof(BatchSeg), hipMemcpyHostToDevice, st));
+        const int npass = (int)(npix * (size_t)sg.nsamples);
+        PassLaunch primal{s->dev, nullptr, nullptr, s->integ.method, false, wb.segs, 1, npass,
+                          (int)std::min(pool, (size_t)npass)};
+        rc = run_pass(c, tm, primal, &max_bounce, nullptr);
+        if (rc) break;
+        PassLaunch adjoint = primal;
+        adjoint.adj = &aa;
+        rc = run_pass(c, tm, adjoint, &max_bounce, nullptr);
+        passes++;
+    }
+    for (int t = 0; t < s->num_textures && !rc; t++)
+        if (acc[t] && launch_add_f64_to_f32(acc[t], d_tex_grads[t], (int64_t)s->tex_elems[t] * 3, st) != hipSuccess)
+            rc = fail(RD_ERR_DEVICE, "gradient conversion");
+    HIP_TRY(hipEventRecord(ev1, st));
+    HIP_TRY(hipMemcpyAsync(c->host_counts, c->counts, kCntSlots * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    tm.collect();
+    collect_stats(c, 2 * total, ms, passes, max_bounce);
+    free_acc();
+    return rc;
+}
+
+int rd_adjoint(rd_ctx* c, rd_scene* s, const rd_render_params* p, const float* dl_dimage, float* const* tex_grads) {
+    if (!c || !s || !p || !dl_dimage || !tex_grads) return fail(RD_ERR_INVALID, "null argument");
+    (void)hipSetDevice(c->device);
+    const size_t nimg = (size_t)s->width * s->height * 3;
+    std::vector<void*> bufs;
+    std::vector<float*> dgrads(std::max(s->num_textures, 1), nullptr);
+    int rc = RD_OK;
+    auto dalloc = [&](size_t bytes) -> void* {
+        void* q = nullptr;
+        if (hipMalloc(&q, bytes) != hipSuccess) return nullptr;
+        bufs.push_back(q);
+        return q;
+    };
+    float* ddl = (float*)dalloc(nimg * 4);
+    if (!ddl) rc = fail(RD_ERR_OOM, "hipMalloc dLoss/dImage");
+    if (!rc && hipMemcpy(ddl, dl_dimage, nimg * 4, hipMemcpyHostToDevice) != hipSuccess) rc = fail(RD_ERR_DEVICE, "copy");
+    for (int t = 0; t < s->num_textures && !rc; t++) {
+        if (!tex_grads[t]) continue;
+        size_t bytes = s->tex_elems[t] * 3 * sizeof(float);
+        dgrads[t] = (float*)dalloc(bytes);
+        if (!dgrads[t]) { rc = fail(RD_ERR_OOM, "hipMalloc gradient"); break; }
+        if (hipMemcpy(dgrads[t], tex_grads[t], bytes, hipMemcpyHostToDevice) != hipSuccess) rc = fail(RD_ERR_DEVICE, "copy");
+    }
+    if (!rc) rc = rd_adjoint_device(c, s, p, ddl, dgrads.data());
+    for (int t = 0; t < s->num_textures && !rc; t++) {
+        if (!tex_grads[t]) continue;
+        if (hipMemcpy(tex_grads[t], dgrads[t], s->tex_elems[t] * 3 * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(RD_ERR_DEVICE, "copy gradient");
+    }
+    for (void* q : bufs) (void)hipFree(q);
+    return rc;
+}
+
+// Replaces a texture's texels (w*h*3 floats, the layout of rd_texture_desc;
+// constants take 3 floats) — e.g. parameters updated by an optimiser step.
+static int set_texture(rd_ctx* c, rd_scene* s, int32_t tex, const float* data, hipMemcpyKind kind) {
+    if (!c || !s || !data || tex < 0 || tex >= s->num_textures) return fail(RD_ERR_INVALID, "bad argument");
+    for (int e : s->env_textures)
+        if (e == tex) return fail(RD_ERR_UNSUPPORTED, "environment-map textures are fixed (sampling tables)");
+    CtxLock lock(c);
+    if (!lock.ok()) return fail(RD_ERR_BUSY, "context in use by another thread");
+    (void)hipSetDevice(c->device);
+    const DevTexture& t = s->texs[tex];
+    if (t.w == 0) {
+        DevTexture nt = t;
+        nt.v[0] = data[0]; nt.v[1] = data[1]; nt.v[2] = data[2];
+        if (kind == hipMemcpyDeviceToDevice) {
+            float v[3];
+            HIP_TRY(hipMemcpy(v, data, sizeof(v), hipMemcpyDeviceToHost));
+            nt.v[0] = v[0]; nt.v[1] = v[1]; nt.v[2] = v[2];
+        }
+        s->texs[tex] = nt;
+        HIP_TRY(hipMemcpy((DevTexture*)s->dev.texs + tex, &nt, sizeof(DevTexture), hipMemcpyHostToDevice));
+        return RD_OK;
+    }
+    HIP_TRY(hipMemcpy((float*)s->dev.texels + t.offset, data, (size_t)t.w * t.h * 3 * sizeof(float), kind));
+    return RD_OK;
+}
+int rd_scene_set_texture(rd_ctx* c, rd_scene* s, int32_t tex, const float* data) {
+    return set_texture(c, s, tex, data, hipMemcpyHostToDevice);
+}
+int rd_scene_set_texture_device(rd_ctx* c, rd_scene* s, int32_t tex, const float* d_data) {
+    return set_texture(c, s, tex, d_data, hipMemcpyDeviceToDevice);
 }
 
 int rd_render_batch(rd_ctx* c, int32_t n, const rd_request* reqs, int32_t wave_paths, volatile int32_t* cancel,

@@ -31,6 +31,11 @@ struct PathState {
     float4* ray_o;      // next ray of the slot's path: {o.xyz, tmin}
     float4* ray_d;      //                              {d.xyz, tmax}
     float4* res;        // per pass path: final radiance (read in sample order by accumulate)
+    int* pix;           // image element of the path's pixel: x * height + y
+    // adjoint replay only: {R, material (int bits)}, {u, v, r_nee, r_mis}, {r_cont}
+    float4* adj0;
+    float4* adj1;
+    float4* adj2;
     uint32_t* rng_n;
     int* depth;
     uint8_t* vis;
@@ -74,11 +79,25 @@ struct BatchSeg {
     int path_offset;
     int sample_base, nsamples;
     int x0, y0, crop_w, npix;
-    int width;
+    int width, height;
     uint32_t seed;
     int sum_offset;
+};
+
+// Adjoint replay inputs: dLoss/dImage (column-major W*H*3, device), the
+// float64 gradient accumulator of every scene texture (device array of device
+// pointers, null = no gradient; layout as the texture, 3 values per texel:
+// many path terms of both signs meet in one texel, so they are summed in
+// float64), 1/spp.
+struct AdjArgs {
+    const float* dl;
+    double* const* grads;
+    float inv_spp;
     int _pad;
 };
+
+// out[i] += (float)acc[i] for i < n.
+hipError_t launch_add_f64_to_f32(const double* acc, float* out, int64_t n, hipStream_t s);
 
 struct Timing;  // per-stage HIP event timing (rd_api.cpp)
 
@@ -100,14 +119,17 @@ hipError_t launch_trace_ext(const DevScene& sc, const DevScene* scenes, RayQueue
 // `scenes` (device array, may be null): batched mode, the path's scene is
 // scenes[ps.req[pid]]; otherwise every path uses `sc`.
 hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
-                        RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in, hipStream_t s);
+                        RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in, bool adjoint,
+                        hipStream_t s);
 hipError_t launch_accumulate_batch(const BatchSeg* segs, int nseg, int max_npix, PathState ps, float4* sum,
                                    float4* sum2, int* counts, hipStream_t s);
 hipError_t launch_trace_shadow(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,
                                hipStream_t s);
 hipError_t launch_trace_mis(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,
                             hipStream_t s);
-hipError_t launch_resolve(int method, RayQueue qin, PathState ps, int* counts, int cnt_in, hipStream_t s);
+// adj: null for rendering; the adjoint replay's gradient inputs otherwise.
+hipError_t launch_resolve(const DevScene& sc, int method, RayQueue qin, PathState ps, int* counts, int cnt_in,
+                          const AdjArgs* adj, hipStream_t s);
 // vol_path stages (VolumetricPathTracer.Li and its Scene.Transmittance /
 // Scene.IntersectTr chains): shade_vol draws every random number of the
 // bounce and starts one Transmittance chain (light sample) and one

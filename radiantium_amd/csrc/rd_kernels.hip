@@ -414,6 +414,7 @@ __device__ __forceinline__ void start_path(const DevScene& sc, const BatchSeg& s
     ps.key[slot] = key;
     ps.gid[slot] = g;
     ps.req[slot] = sg.req;
+    ps.pix[slot] = x * sg.height + y;
     ps.depth[slot] = 0;
     if (ps.env) ps.env[slot] = sc.global_medium;   // vol_path: rayEnv = Scene.GlobalMedium
 }
@@ -617,7 +618,10 @@ hipError_t launch_trace_chain(const DevScene& sc, const DevScene* scenes, RayQue
 
 // One bounce of PathTracer.Mis / NextEventEstimation / OnlySampleBxdf for
 // every path in the extension queue (PathTracer.cs:43-270).
-template <bool kBatch>
+// kAdj: the adjoint replay also records, per material vertex, the albedo R,
+// the texture coordinates and the roughness ratio (plastic_rho_ratio) of its
+// light-sample, BSDF-sample and continuation terms for resolve's gradients.
+template <bool kBatch, bool kAdj>
 __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __restrict__ scenes, RayQueue qin,
                                                const float4* __restrict__ hits, RayQueue qshadow, RayQueue qmis,
                                                PathState ps, int* counts, int cnt_in) {
@@ -683,6 +687,10 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __r
                 nvert++;
                 MatCtx m = load_material(sc, pr.material, it.u, it.v);
                 f3 wo_l = to_local(it.sh, wo);
+                float rho = 0.0f, r_nee = 0.0f, r_mis = 0.0f, r_cont = 0.0f;
+                if constexpr (kAdj) {
+                    if (m.type == kMatRoughPlastic) rho = tex_sample(sc, sc.mats[pr.material].tex[1], it.u, it.v).x;
+                }
                 if (in.method == 0) {
                     if (pr.light >= 0) L = L + beta * inct_le(wo);
                 } else if (sc.num_lights > 0) {
@@ -706,6 +714,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __r
                                     fr = fr * abs_cos_theta(to_local(it.sh, ls.wi));
                                     float w = power_heuristic(ls.pdf, spdf);
                                     nee = fr * ls.li * w / ls.pdf;
+                                    if constexpr (kAdj) r_nee = plastic_rho_ratio(m, rho, wo_l, wi_l);
                                 }
                             }
                         }
@@ -728,6 +737,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __r
                                 st3(ps.mis_f, pid, fr2, w2);
                                 ps.mis_li[pid] = make_float4(0.f, 0.f, 0.f, sp2);
                                 ps.sel_light[pid] = l;
+                                if constexpr (kAdj) r_mis = plastic_rho_ratio(m, rho, wo_l, bs.wi);
                             }
                         }
                     } else {
@@ -741,6 +751,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __r
                                 f3 wi_l = to_local(it.sh, ls.wi);
                                 f3 fr = mat_fr(m, wo_l, wi_l);
                                 nee = fr * ls.li * abs_cos_theta(wi_l) / ls.pdf;
+                                if constexpr (kAdj) r_nee = plastic_rho_ratio(m, rho, wo_l, wi_l);
                             }
                         }
                     }
@@ -759,6 +770,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __r
                 BSample bs = mat_sample(m, wo_l, rng);
                 bool sspec = (bs.type & kSpecular) != 0;
                 if (bs.pdf > 0.0f) {
+                    if constexpr (kAdj) r_cont = plastic_rho_ratio(m, rho, wo_l, bs.wi);
                     beta = beta * (bs.fr * abs_cos_theta(bs.wi) / bs.pdf);
                     ed = to_world(it.sh, bs.wi);
                     eo = it.P; etmin = 0.0001f; etmax = kFloatMax;
@@ -771,6 +783,13 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __r
                     if (alive) { want_ext = true; depth++; }
                 }
                 if (sspec) nflags |= kFlagSpecular;
+                if constexpr (kAdj) {
+                    const bool has_params = m.type == kMatDiffuse || m.type == kMatRoughPlastic;
+                    nflags |= kFlagVertex;
+                    ps.adj0[pid] = make_float4(m.r.x, m.r.y, m.r.z, __int_as_float(has_params ? pr.material : -1));
+                    ps.adj1[pid] = make_float4(it.u, it.v, r_nee, r_mis);
+                    ps.adj2[pid] = make_float4(r_cont, 0.f, 0.f, 0.f);
+                }
             }
             st3(ps.L, pid, L);
             st3(ps.beta, pid, beta);
@@ -805,13 +824,17 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __r
 }
 
 hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
-                        RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in, hipStream_t s) {
+                        RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in, bool adjoint,
+                        hipStream_t s) {
     if (scenes)
-        hipLaunchKernelGGL(k_shade<true>, dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits, qshadow,
-                           qmis, ps, counts, cnt_in);
+        hipLaunchKernelGGL((k_shade<true, false>), dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
+                           qshadow, qmis, ps, counts, cnt_in);
+    else if (adjoint)
+        hipLaunchKernelGGL((k_shade<false, true>), dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
+                           qshadow, qmis, ps, counts, cnt_in);
     else
-        hipLaunchKernelGGL(k_shade<false>, dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits, qshadow,
-                           qmis, ps, counts, cnt_in);
+        hipLaunchKernelGGL((k_shade<false, false>), dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
+                           qshadow, qmis, ps, counts, cnt_in);
     return hipGetLastError();
 }
 
@@ -819,50 +842,101 @@ hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin
 // for Mis; radiance += coeff * EstimateLight / lightPdf (:81) for Nee.  A
 // path that ended this bounce then retires its radiance into the pass's
 // per-sample result (its slot is free for the next compaction).
-__global__ __launch_bounds__(256) void k_resolve(int method, RayQueue qin, PathState ps, int* counts, int cnt_in) {
+//
+// kAdj (path-replay backpropagation): the path's full radiance Ltot comes
+// from the primal replay; after this vertex's terms, Lr = Ltot - L is what the
+// rest of the path adds, all of it through the continuation's BSDF factor.
+// With delta = dLoss/dI(pixel) / spp and the vertex's terms C_nee, C_mis:
+//   dLoss/dR_c   = delta_c (C_nee_c + C_mis_c + Lr_c) / R_c
+//   dLoss/drho   = sum_c delta_c (C_nee_c r_nee + C_mis_c r_mis + Lr_c r_cont)
+// scattered onto the texels of the albedo / roughness lookups.
+template <bool kAdj>
+__global__ __launch_bounds__(256) void k_resolve(DevScene sc, int method, RayQueue qin, PathState ps, int* counts,
+                                                 int cnt_in, AdjArgs aa) {
     int n = counts[cnt_in];
-    for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
-        int k = base + threadIdx.x;
-        bool done = false;
-        int pid = 0;
-        if (k < n) {
-            pid = qin.pid[k];
-            int flags = ps.flags[pid];
-            done = (flags & kFlagDone) != 0;
-            if (flags & kFlagNeePending) {
-                float4 nb = ps.nee_beta[pid];
-                f3 beta = mk(nb.x, nb.y, nb.z);
-                float selpdf = nb.w;
-                bool vis = (flags & kFlagShadowIssued) && ps.vis[pid];
-                f3 L = ld3(ps.L, pid);
-                if (method == 2) {
-                    f3 le = mk(0.0f);
-                    if (vis) le = le + ld3(ps.nee_le, pid);
-                    if (flags & kFlagMisIssued) {
-                        float4 li4 = ps.mis_li[pid];
-                        f3 li = mk(li4.x, li4.y, li4.z);
-                        if (!is_black(li)) {
-                            float4 f4 = ps.mis_f[pid];
-                            le = le + mk(f4.x, f4.y, f4.z) * li * f4.w / li4.w;
-                        }
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+        int pid = qin.pid[k];
+        int flags = ps.flags[pid];
+        bool done = (flags & kFlagDone) != 0;
+        f3 L = ld3(ps.L, pid);
+        f3 c_nee = mk(0.0f), c_mis = mk(0.0f);
+        if (flags & kFlagNeePending) {
+            float4 nb = ps.nee_beta[pid];
+            f3 beta = mk(nb.x, nb.y, nb.z);
+            float selpdf = nb.w;
+            bool vis = (flags & kFlagShadowIssued) && ps.vis[pid];
+            if (method == 2) {
+                f3 le = mk(0.0f), le_mis = mk(0.0f);
+                if (vis) le = le + ld3(ps.nee_le, pid);
+                if (flags & kFlagMisIssued) {
+                    float4 li4 = ps.mis_li[pid];
+                    f3 li = mk(li4.x, li4.y, li4.z);
+                    if (!is_black(li)) {
+                        float4 f4 = ps.mis_f[pid];
+                        le_mis = mk(f4.x, f4.y, f4.z) * li * f4.w / li4.w;
+                        le = le + le_mis;
                     }
-                    f3 result = mk(0.0f) + le / selpdf;
-                    L = L + beta * result;
-                } else {
-                    f3 e = vis ? ld3(ps.nee_le, pid) : mk(0.0f);
-                    L = L + beta * e / selpdf;
                 }
-                st3(ps.L, pid, L);
-                if (done) ps.res[ps.gid[pid]] = make_float4(L.x, L.y, L.z, 0.f);
-            } else if (done) {
-                ps.res[ps.gid[pid]] = ps.L[pid];
+                f3 result = mk(0.0f) + le / selpdf;
+                L = L + beta * result;
+                if constexpr (kAdj) {
+                    if (vis) c_nee = beta * (ld3(ps.nee_le, pid) / selpdf);
+                    c_mis = beta * (le_mis / selpdf);
+                }
+            } else {
+                f3 e = vis ? ld3(ps.nee_le, pid) : mk(0.0f);
+                L = L + beta * e / selpdf;
+                if constexpr (kAdj) c_nee = beta * e / selpdf;
             }
+            if constexpr (!kAdj) st3(ps.L, pid, L);
+        }
+        if constexpr (kAdj) {
+            if (flags & kFlagNeePending) st3(ps.L, pid, L);
+            const float4 a0 = ps.adj0[pid];
+            const int mat = __float_as_int(a0.w);
+            const f3 ltot = ld3(ps.res, ps.gid[pid]);
+            if ((flags & kFlagVertex) && mat >= 0 && valid_color(ltot)) {
+                const f3 lr = ltot - L;
+                const float4 a1 = ps.adj1[pid];
+                const float r_cont = ps.adj2[pid].x;
+                const int pix = ps.pix[pid];
+                const f3 delta = mk(aa.dl[3 * pix], aa.dl[3 * pix + 1], aa.dl[3 * pix + 2]) * aa.inv_spp;
+                const f3 t = c_nee + c_mis + lr;
+                const f3 gr = mk(a0.x != 0 ? delta.x * t.x / a0.x : 0.0f, a0.y != 0 ? delta.y * t.y / a0.y : 0.0f,
+                                 a0.z != 0 ? delta.z * t.z / a0.z : 0.0f);
+                const DevMaterial& dm = sc.mats[mat];
+                double* g0 = aa.grads[dm.tex[0]];
+                if (g0) {
+                    TexFoot f = tex_footprint(sc, dm.tex[0], a1.x, a1.y);
+                    for (int i = 0; i < f.n; i++) {
+                        atomicAdd(g0 + 3 * f.idx[i], (double)(f.w[i] * gr.x));
+                        atomicAdd(g0 + 3 * f.idx[i] + 1, (double)(f.w[i] * gr.y));
+                        atomicAdd(g0 + 3 * f.idx[i] + 2, (double)(f.w[i] * gr.z));
+                    }
+                }
+                if (dm.type == kMatRoughPlastic) {
+                    double* g1 = aa.grads[dm.tex[1]];
+                    const float grho = dot(delta, c_nee * a1.z + c_mis * a1.w + lr * r_cont);
+                    if (g1 && grho != 0.0f) {
+                        TexFoot f = tex_footprint(sc, dm.tex[1], a1.x, a1.y);
+                        for (int i = 0; i < f.n; i++) atomicAdd(g1 + 3 * f.idx[i], (double)(f.w[i] * grho));
+                    }
+                }
+            }
+        } else {
+            if (done) ps.res[ps.gid[pid]] = make_float4(L.x, L.y, L.z, 0.f);
         }
     }
 }
 
-hipError_t launch_resolve(int method, RayQueue qin, PathState ps, int* counts, int cnt_in, hipStream_t s) {
-    hipLaunchKernelGGL(k_resolve, dim3(persistent_blocks()), dim3(256), 0, s, method, qin, ps, counts, cnt_in);
+hipError_t launch_resolve(const DevScene& sc, int method, RayQueue qin, PathState ps, int* counts, int cnt_in,
+                          const AdjArgs* adj, hipStream_t s) {
+    if (adj)
+        hipLaunchKernelGGL(k_resolve<true>, dim3(persistent_blocks()), dim3(256), 0, s, sc, method, qin, ps, counts,
+                           cnt_in, *adj);
+    else
+        hipLaunchKernelGGL(k_resolve<false>, dim3(persistent_blocks()), dim3(256), 0, s, sc, method, qin, ps, counts,
+                           cnt_in, AdjArgs{});
     return hipGetLastError();
 }
 
@@ -923,6 +997,18 @@ hipError_t launch_finalize(const WaveParams& wp, int height, int spp, const floa
     int blocks = (wp.npix + 255) / 256;
     if (blocks > persistent_blocks()) blocks = persistent_blocks();
     hipLaunchKernelGGL(k_finalize, dim3(blocks), dim3(256), 0, s, wp, height, spp, sum, sum2, out_rgb, out_sum2);
+    return hipGetLastError();
+}
+
+__global__ void k_add_f64_to_f32(const double* acc, float* out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        out[i] += (float)acc[i];
+}
+hipError_t launch_add_f64_to_f32(const double* acc, float* out, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > persistent_blocks()) blocks = persistent_blocks();
+    hipLaunchKernelGGL(k_add_f64_to_f32, dim3((unsigned)blocks), dim3(256), 0, s, acc, out, n);
     return hipGetLastError();
 }
 

@@ -1089,6 +1089,98 @@ __device__ __forceinline__ float light_pdf(const DevScene& sc, const DevLight& L
     return pdf;
 }
 
+// ---------------------------------------------------------------- parameter derivatives
+// For the adjoint (path-replay backpropagation, Vicini et al. 2021).  Diffuse
+// and RoughPlastic values are linear in the albedo texture R per channel, so a
+// term C carrying f(R) has dC_c/dR_c = C_c / R_c.  RoughPlastic's roughness rho
+// scales the distribution's alphas as (ax, ay) = s (a1, a2) with s = rho^2
+// (RoughnessToAlpha, Microfacet.cs:177-183); only the glossy lobe's D * G
+// depends on it, so dC_c/drho = C_c * (glossy / f) * (s dln(DG)/ds) * 2 / rho.
+// s * dln(D)/ds for wh (Microfacet.cs:46-59 Beckmann, :93-107 GGX).
+__device__ __forceinline__ float mf_sdlog_D(const Mf& m, f3 wh) {
+    float t2 = tan2_theta(wh);
+    if (isinf(t2)) return 0.0f;
+    float e = (cos2_phi(wh) / (m.ax * m.ax) + sin2_phi(wh) / (m.ay * m.ay)) * t2;
+    if (m.kind == 0) return 2 * e - 2;
+    return -2 * (1 - e) / (1 + e);
+}
+// s * dLambda/ds for w (Microfacet.cs:117-132).
+__device__ __forceinline__ float mf_sdlambda(const Mf& m, f3 w) {
+    float at = fabsf(tan_theta(w));
+    if (isinf(at)) return 0.0f;
+    float alpha = sqrtf(cos2_phi(w) * m.ax * m.ax + sin2_phi(w) * m.ay * m.ay);
+    if (m.kind == 0) {
+        float a = 1 / (alpha * at);
+        if (a >= 1.6f) return 0.0f;
+        float num = 1 - 1.259f * a + 0.396f * a * a, den = 3.535f * a + 2.181f * a * a;
+        float dnum = -1.259f + 0.792f * a, dden = 3.535f + 4.362f * a;
+        return -a * (dnum * den - num * dden) / (den * den);
+    }
+    float k = (alpha * at) * (alpha * at);
+    return k / (2 * sqrtf(1 + k));
+}
+// RoughPlastic split at (wo, wi) with f = R * (d + g): the diffuse and glossy
+// weights and s * dln(D G)/ds of the glossy lobe (zero where plastic_fr is).
+struct PlasticParts {
+    float d, g, sdlog;
+};
+__device__ __forceinline__ PlasticParts plastic_parts(const MatCtx& c, f3 wo, f3 wi) {
+    PlasticParts p{0.0f, 0.0f, 0.0f};
+    if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!c.two_sided) return p; }
+    if (!same_hemisphere(wo, wi)) return p;
+    p.d = c.kd / kPi;
+    float cto = abs_cos_theta(wo), cti = abs_cos_theta(wi);
+    f3 wh = wi + wo;
+    if (!(cti == 0 || cto == 0) && !(wh.x == 0 && wh.y == 0 && wh.z == 0)) {
+        wh = normalize(wh);
+        float F = fr_dielectric(dot(wi, faceforward(wh, mk(0, 0, 1))), c.eta_a, c.eta_b);
+        float D = mf_D(c.mf, wh), G = mf_G(c.mf, wo, wi);
+        p.g = c.ks * fabsf((D * G * F) / (4 * cti * cto));
+        p.sdlog = mf_sdlog_D(c.mf, wh) - G * (mf_sdlambda(c.mf, wo) + mf_sdlambda(c.mf, wi));
+    }
+    return p;
+}
+// dC/drho / C for a term whose BSDF factor is f(wo, wi) of a RoughPlastic with roughness rho.
+__device__ __forceinline__ float plastic_rho_ratio(const MatCtx& c, float rho, f3 wo, f3 wi) {
+    if (c.type != kMatRoughPlastic || !(rho > 0)) return 0.0f;
+    PlasticParts p = plastic_parts(c, wo, wi);
+    float f = p.d + p.g;
+    if (!(f > 0)) return 0.0f;
+    return (p.g / f) * p.sdlog * 2 / rho;
+}
+
+// The texels a tex_sample(u, v) reads and their weights: one for Nearest,
+// four for Linear (constant textures: element 0).  Element index is
+// x * h + y like the texel layout.
+struct TexFoot {
+    int64_t idx[4];
+    float w[4];
+    int n;
+};
+__device__ __forceinline__ TexFoot tex_footprint(const DevScene& sc, int id, float su, float sv) {
+    TexFoot f;
+    const DevTexture t = sc.texs[id];
+    if (t.w == 0) { f.n = 1; f.idx[0] = 0; f.w[0] = 1.0f; return f; }
+    float u, v;
+    if (t.wrap == 0) { u = nclamp(su, 0.0f, 1.0f); v = nclamp(sv, 0.0f, 1.0f); }
+    else { u = nclamp(su - floorf(su), 0.0f, 1.0f); v = nclamp(sv - floorf(sv), 0.0f, 1.0f); }
+    float fu = u * (t.w - 1), fv = v * (t.h - 1);
+    int pu = (int)fu, pv = (int)fv;
+    if (t.filter == 0) { f.n = 1; f.idx[0] = (int64_t)pu * t.h + pv; f.w[0] = 1.0f; return f; }
+    int dpu = (fu > pu + 0.5f) ? 1 : -1;
+    int dpv = (fv > pv + 0.5f) ? 1 : -1;
+    int apu = iclamp(pu + dpu, 0, t.w - 1);
+    int apv = iclamp(pv + dpv, 0, t.h - 1);
+    float du = nmin(fabsf(fu - pu - 0.5f), 1.0f);
+    float dv = nmin(fabsf(fv - pv - 0.5f), 1.0f);
+    f.n = 4;
+    f.idx[0] = (int64_t)pu * t.h + pv;   f.w[0] = (1 - du) * (1 - dv);
+    f.idx[1] = (int64_t)apu * t.h + pv;  f.w[1] = du * (1 - dv);
+    f.idx[2] = (int64_t)pu * t.h + apv;  f.w[2] = (1 - du) * dv;
+    f.idx[3] = (int64_t)apu * t.h + apv; f.w[3] = du * dv;
+    return f;
+}
+
 // ---------------------------------------------------------------- media
 // Homogeneous.Tr (Mediums/Homogeneous.cs:22-30): exp(sigma_t * (MinT - MaxT)).
 __device__ __forceinline__ f3 medium_tr(const DevMedium& m, float tmin, float tmax) {

@@ -13,6 +13,7 @@ enum : int {
     kFlagShadowIssued = 4,
     kFlagMisIssued = 8,
     kFlagDone = 16,        // the path ended this bounce (retired after resolve)
+    kFlagVertex = 32,      // adjoint replay: a material vertex recorded its derivative terms
 };
 
 // ---------------------------------------------------------------- wave helpers

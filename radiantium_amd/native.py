@@ -20,6 +20,7 @@ EXPORTS = [
     "rd_trace_closest_device", "rd_trace_any_device", "rd_render", "rd_render_device",
     "rd_get_render_stats", "rd_bvh_build_host", "rd_device_alloc", "rd_device_free", "rd_memcpy_h2d", "rd_memcpy_d2h",
     "rd_synchronize", "rd_kernel_timing", "rd_reset_kernel_timing", "rd_render_batch", "rd_render_batch_device",
+    "rd_adjoint", "rd_adjoint_device", "rd_scene_set_texture", "rd_scene_set_texture_device",
 ]
 
 
@@ -43,6 +44,10 @@ def lib():
         L.rd_create.argtypes = [C.c_int]
         L.rd_destroy.argtypes = [vp]
         L.rd_render_batch.argtypes = [vp, i32, vp, i32, vp, vp]
+        L.rd_adjoint.argtypes = [vp, vp, C.POINTER(abi.rd_render_params), vp, vp]
+        L.rd_adjoint_device.argtypes = [vp, vp, C.POINTER(abi.rd_render_params), vp, vp]
+        L.rd_scene_set_texture.argtypes = [vp, vp, i32, vp]
+        L.rd_scene_set_texture_device.argtypes = [vp, vp, i32, vp]
         L.rd_render_batch_device.argtypes = [vp, i32, vp, i32, vp, vp]
         L.rd_scene_create.restype = vp
         L.rd_scene_create.argtypes = [vp, C.POINTER(abi.rd_scene_desc)]
@@ -247,6 +252,47 @@ class Scene:
                               C.cast(cancel, C.c_void_p) if cancel is not None else None,
                               C.cast(progress, C.c_void_p) if progress is not None else None))
         return rgb, s2
+
+    def adjoint(self, spp, dl_dimage, textures, seed=0, crop=None, wave_paths=0):
+        """Path-replay adjoint (rd_adjoint): dLoss/dtexel for the listed texture
+        indices, given dLoss/dImage (host float32 [W*H*3]) of the render with
+        the same spp / seed / crop.  Returns {tex: float32 [n*3]}."""
+        p = abi.rd_render_params()
+        p.spp, p.seed, p.wave_paths = spp, seed, wave_paths
+        if crop is not None:
+            p.x0, p.y0, p.x1, p.y1 = crop
+        dl = np.ascontiguousarray(dl_dimage, np.float32)
+        ntex = len(self.desc.textures)
+        ptrs = (C.c_void_p * max(ntex, 1))()
+        out = {}
+        for t in textures:
+            tx = self.desc.textures[t]
+            n = tx.image.shape[0] * tx.image.shape[1] if tx.image is not None else 1
+            out[t] = np.zeros(n * 3, np.float32)
+            ptrs[t] = out[t].ctypes.data
+        check(lib().rd_adjoint(self.ctx.h, self.h, C.byref(p), _ptr(dl), ptrs))
+        return out
+
+    def adjoint_device(self, spp, d_dl, d_grads, seed=0, crop=None, wave_paths=0):
+        """rd_adjoint_device: d_grads is a list (one per scene texture) of device pointers or None."""
+        p = abi.rd_render_params()
+        p.spp, p.seed, p.wave_paths = spp, seed, wave_paths
+        if crop is not None:
+            p.x0, p.y0, p.x1, p.y1 = crop
+        ptrs = (C.c_void_p * max(len(d_grads), 1))(*[g or None for g in d_grads])
+        check(lib().rd_adjoint_device(self.ctx.h, self.h, C.byref(p), d_dl, ptrs))
+
+    def set_texture(self, tex, data):
+        """rd_scene_set_texture: replace texture `tex`'s texels ([w][h][3] or 3 floats)."""
+        a = np.ascontiguousarray(data, np.float32)
+        check(lib().rd_scene_set_texture(self.ctx.h, self.h, tex, _ptr(a)))
+        self.desc.textures[tex] = type(self.desc.textures[tex])(image=a.reshape(self.desc.textures[tex].image.shape),
+                                                                wrap=("Clamp", "Repeat")[self.desc.textures[tex].wrap],
+                                                                filter=("Nearest", "Linear")[self.desc.textures[tex].filter]) \
+            if self.desc.textures[tex].image is not None else self.desc.textures[tex]
+
+    def set_texture_device(self, tex, d_data):
+        check(lib().rd_scene_set_texture_device(self.ctx.h, self.h, tex, d_data))
 
     def render_device(self, spp, d_rgb, d_sum2=None, seed=0, crop=None, wave_paths=0):
         p = abi.rd_render_params()

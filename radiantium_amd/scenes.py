@@ -263,6 +263,69 @@ def batch_requests(seed=0, n=16, resolutions=(128, 256, 512, 1024), spps=(16, 64
     return out
 
 
+def diff_cornell(width=64, height=64, tex_res=512, seed=0, init=None, max_depth=8, method="Mis", rr=True,
+                 filter="Nearest", wall="rough_plastic"):
+    """Differentiable-rendering workload: the Cornell box of cornell_box()
+    with the left and right walls made rough_plastic (GGX) carrying tex_res^2
+    RGB albedo textures `r` and a roughness texture (GGX alpha ~ U[0.05, 0.8]
+    stored as roughness = sqrt(alpha), in every channel), drawn from `seed`;
+    init = (albedo, roughness) instead fills them uniformly.  Walls get UVs
+    spanning [0,1]^2.  rr=False sets min_depth = max_depth (no Russian
+    roulette); wall="diffuse" makes the walls Lambertian with the albedo
+    textures as kd.  Returns (SceneDesc, {"left_r", "left_rough", "right_r",
+    "right_rough"} -> texture index)."""
+    from .scene import Texture
+    rng = np.random.default_rng(seed)
+
+    def tex_pair():
+        if init is None:
+            alb = rng.uniform(0.05, 0.95, (tex_res, tex_res, 3))
+            rough = np.sqrt(rng.uniform(0.05, 0.8, (tex_res, tex_res)))
+        else:
+            alb = np.full((tex_res, tex_res, 3), init[0])
+            rough = np.full((tex_res, tex_res), init[1])
+        rough3 = np.repeat(rough[:, :, None], 3, axis=2)
+        return (Texture(image=alb.astype(np.float32), filter=filter),
+                Texture(image=rough3.astype(np.float32), filter=filter))
+
+    s = SceneDesc()
+    grey = s.add_material("diffuse", kd=(0.73, 0.73, 0.73))
+    light_mat = s.add_material("diffuse", kd=(0.0, 0.0, 0.0))
+    lr, lro = tex_pair()
+    rr_, rro = tex_pair()
+    if wall == "diffuse":
+        left = s.add_material("diffuse", kd=lr)
+        right = s.add_material("diffuse", kd=rr_)
+        s.add_texture(lro)
+        s.add_texture(rro)
+    else:
+        left = s.add_material("rough_plastic", r=lr, roughness=lro, dist="GGX")
+        right = s.add_material("rough_plastic", r=rr_, roughness=rro, dist="GGX")
+    L, H = -0.5, 0.5
+    uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32)
+    s.add_mesh(*_quad((L, 0, L), (L, 0, H), (H, 0, H), (H, 0, L)), material=grey)
+    s.add_mesh(*_quad((L, 1, L), (H, 1, L), (H, 1, H), (L, 1, H)), material=grey)
+    s.add_mesh(*_quad((L, 0, L), (H, 0, L), (H, 1, L), (L, 1, L)), material=grey)
+    pos, idx = _quad((L, 0, L), (L, 1, L), (L, 1, H), (L, 0, H))
+    s.add_mesh(pos, idx, material=left, uvs=uv)
+    pos, idx = _quad((H, 0, L), (H, 0, H), (H, 1, H), (H, 1, L))
+    s.add_mesh(pos, idx, material=right, uvs=uv)
+    s.add_mesh(*_box((0.3, 0.3, 0.3), (0.17, 0.0, 0.12), 17.0), material=grey)
+    s.add_mesh(*_box((0.3, 0.6, 0.3), (-0.17, 0.0, -0.15), -17.0), material=grey)
+    q = 0.125
+    y = 1.0 - 1e-3
+    s.add_mesh(*_quad((-q, y, -q), (q, y, -q), (q, y, q), (-q, y, q)), material=light_mat, le=(17.0, 12.0, 4.0))
+    s.set_camera(fov=40.0, width=width, height=height, origin=(0.0, 0.5, 1.85), target=(0.0, 0.5, 0.0),
+                 up=(0.0, 1.0, 0.0))
+    s.set_integrator(type="path", max_depth=max_depth, method=method, min_depth=3 if rr else max_depth)
+    s.bvh_max_prim = 1
+    mats = {m: s.materials[i] for i, m in ((left, "left"), (right, "right"))}
+    tex = {"left_r": mats["left"]["tex"][0], "right_r": mats["right"]["tex"][0]}
+    if wall != "diffuse":
+        tex.update(left_rough=mats["left"]["tex"][1], right_rough=mats["right"]["tex"][1])
+    return s, tex
+
+
 # ---------------------------------------------------------------- icosphere
 def icosphere(level):
     """Unit icosphere: 20*4^level faces, midpoint subdivision with shared
