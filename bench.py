@@ -18,6 +18,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -140,6 +142,91 @@ def batched_speedup(ctx, args, small=False):
                         + ("res {128,256}^2, spp {16,64}" if small else "res 128^2-1024^2, spp 16-256") + ", depth 64"}
 
 
+def adjoint_ratio(ctx, args):
+    """Differentiable-rendering workload: the Cornell box with the side walls
+    GGX rough plastic carrying 512x512 RGB albedo and roughness textures,
+    image 512x512.  Target = 256 spp render of seeded random textures; the
+    optimisation starts from uniform 0.5 textures; a step is a 64 spp primal
+    plus a 64 spp path-replay adjoint of the L2 loss, then an Adam update.
+    The ratio is timed on direct rd_render_device / rd_adjoint_device calls
+    (same spp and seed); the optimisation loop runs through the autograd
+    Function and reports its loss."""
+    import torch
+    from radiantium_amd import native, scenes
+    from radiantium_amd.autograd import render
+    R, T, spp = 512, 512, 64
+    tdesc, _ = scenes.diff_cornell(R, R, tex_res=T, seed=args.adj_seed)
+    tscene = native.Scene(ctx, tdesc)
+    target = torch.empty(R * R * 3, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    tscene.render_device(256, target.data_ptr(), seed=77)
+    ctx.synchronize()
+    tscene.close()
+    desc, tex = scenes.diff_cornell(R, R, tex_res=T, init=(0.5, 0.5))
+    scene = native.Scene(ctx, desc)
+    ids = sorted(tex.values())
+    img = torch.empty(R * R * 3, dtype=torch.float32, device="cuda")
+    grads = {t: torch.zeros(T * T * 3, dtype=torch.float32, device="cuda") for t in ids}
+    ptrs = [None] * len(desc.textures)
+    for t in ids:
+        ptrs[t] = grads[t].data_ptr()
+    scene.render_device(4, img.data_ptr(), seed=100)
+    ctx.synchronize()
+    dl = (2 * (img - target)).contiguous()
+    torch.cuda.synchronize()
+    scene.adjoint_device(4, dl.data_ptr(), ptrs, seed=100)
+    ctx.synchronize()
+    prim, adj, prim_dev, adj_dev, rays_p, rays_a = [], [], [], [], 0, 0
+    for i in range(args.adj_reps):
+        t0 = time.perf_counter()
+        scene.render_device(spp, img.data_ptr(), seed=i)
+        ctx.synchronize()
+        prim.append(time.perf_counter() - t0)
+        st = ctx.render_stats()
+        prim_dev.append(st["render_ms"])
+        rays_p += st["rays_closest"] + st["rays_any"]
+        dl = (2 * (img - target)).contiguous()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        scene.adjoint_device(spp, dl.data_ptr(), ptrs, seed=i)
+        ctx.synchronize()
+        adj.append(time.perf_counter() - t0)
+        st = ctx.render_stats()
+        adj_dev.append(st["render_ms"])
+        rays_a += st["rays_closest"] + st["rays_any"]
+    p_ms, a_ms = float(np.median(prim)) * 1000, float(np.median(adj)) * 1000
+    log(f"[bench] adjoint: primal {p_ms:.1f} ms, adjoint {a_ms:.1f} ms, ratio {a_ms / p_ms:.2f}")
+    params = {t: torch.full((T * T * 3,), 0.5, dtype=torch.float32, device="cuda", requires_grad=True) for t in ids}
+    rough = {tex[k] for k in tex if k.endswith("rough")}
+    opt = torch.optim.Adam(list(params.values()), lr=0.02)
+    tgt = target.view(R, R, 3)
+    losses, step_ms = [], []
+    for k in range(args.adj_steps):
+        t0 = time.perf_counter()
+        opt.zero_grad()
+        im = render(scene, params, spp, seed=1000 + k)
+        loss = ((im - tgt) ** 2).sum()
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            for t, x in params.items():
+                x.clamp_(0.05, 1.0) if t in rough else x.clamp_(0.0, 1.0)
+        losses.append(float(loss.detach()))
+        torch.cuda.synchronize()
+        step_ms.append((time.perf_counter() - t0) * 1000)
+        log(f"[bench] adjoint opt step {k}: loss {losses[-1]:.2f}, {step_ms[-1]:.0f} ms")
+    scene.close()
+    return {"ratio": round(a_ms / p_ms, 3), "primal_ms": round(p_ms, 1), "adjoint_ms": round(a_ms, 1),
+            "primal_device_ms": round(float(np.median(prim_dev)), 1),
+            "adjoint_device_ms": round(float(np.median(adj_dev)), 1),
+            "primal_mrays_s": round(rays_p / sum(prim) / 1e6, 1), "adjoint_mrays_s": round(rays_a / sum(adj) / 1e6, 1),
+            "opt_steps": args.adj_steps, "opt_step_ms": round(float(np.median(step_ms)), 1) if step_ms else None,
+            "loss_first": round(losses[0], 3) if losses else None, "loss_last": round(losses[-1], 3) if losses else None,
+            "workload": f"Cornell box, rough-plastic side walls with {T}^2 albedo + roughness textures, "
+                        f"{R}x{R}, {spp} spp primal + {spp} spp adjoint, max depth 8, target 256 spp "
+                        f"(texture seed {args.adj_seed}), start 0.5"}
+
+
 def copy_calibration():
     """A plain device copy of known size, profiled beside the path kernels so
     FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
@@ -168,6 +255,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-seed", type=int, default=0, help="seed of the batched workload (16 vol_path scenes)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-sequential measurement")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/primal measurement")
+    ap.add_argument("--adj-seed", type=int, default=9, help="texture seed of the adjoint target")
+    ap.add_argument("--adj-reps", type=int, default=3, help="timed primal/adjoint pairs")
+    ap.add_argument("--adj-steps", type=int, default=6, help="optimisation steps through the autograd Function")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (scale for PMC byte counters)")
     args = ap.parse_args()
@@ -213,7 +304,7 @@ def main():
     value = rays / elapsed / 1e6
 
     stages = {}
-    for name in ("raygen", "trace_closest", "shade", "trace_any", "trace_mis", "resolve", "accumulate", "finalize"):
+    for name in ("compact", "trace_closest", "shade", "trace_any", "trace_mis", "resolve", "accumulate", "finalize"):
         avg, cnt = ctx.kernel_timing(name)
         stages[name] = (avg, cnt)
         log(f"[bench] stage {name:14s} avg {avg * 1000:9.1f} us x {cnt:5d} = {avg * cnt:9.1f} ms")
@@ -241,6 +332,10 @@ def main():
         batched = batched_speedup(ctx, args)
         batched["small_requests"] = batched_speedup(ctx, args, small=True)
 
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_ratio(ctx, args)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(desc, args, value)
@@ -254,7 +349,7 @@ def main():
                                f"{args.max_depth}, bvh max_prim 4",
                    "width": args.width, "height": args.height, "spp": args.spp, "max_depth": args.max_depth,
                    "triangles": ntri, "parallelism": "1 GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "batched": batched,
+        "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "adjoint": adjoint,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

@@ -813,7 +813,7 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
     RayQueue qin = wb.qa, qout = wb.qb;
     int cin = kCntExtA, cout = kCntExtB;
     HIP_TRY(launch_init_pool(wb.ps, L.pool, c->counts, st));
-    HIP_TRY(tm.run("raygen", [&] {
+    HIP_TRY(tm.run("compact", [&] {
         return launch_compact(L.sc, L.scenes, L.segs, L.nseg, L.npass, L.pool, wb.blk, qin, cin, wb.ps, c->counts, st);
     }));
     int rc = RD_OK;
@@ -871,7 +871,7 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
                 return launch_resolve(L.sc, L.method, qin, wb.ps, c->counts, cin, L.adj, st);
             }));
         }
-        HIP_TRY(tm.run("raygen", [&] {
+        HIP_TRY(tm.run("compact", [&] {
             return launch_compact(L.sc, L.scenes, L.segs, L.nseg, L.npass, L.pool, wb.blk, qout, cout, wb.ps,
                                   c->counts, st);
         }));

@@ -36,6 +36,15 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} missing: build it with `python -m radiantium_amd.build`")
+        # torch ships its own libamdhip64 under the same soname
+        # (libamdhip64.so.7) as /opt/rocm's but opens it by another name, so a
+        # process that loads this library first ends up with two HIP runtimes
+        # and torch then sees no GPU.  Importing torch first makes this
+        # library bind to the runtime torch has already loaded.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp, i32, u64 = C.c_void_p, C.c_int32, C.c_uint64
         L.rd_abi_version.restype = C.c_int
