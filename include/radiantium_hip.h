@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define RD_ABI_VERSION 1
+#define RD_ABI_VERSION 2
 
 typedef enum rd_status {
     RD_OK = 0,
@@ -113,12 +113,15 @@ typedef struct rd_medium_desc {
     float scale;
 } rd_medium_desc;
 
-typedef enum rd_prim_kind { RD_PRIM_TRIANGLE = 0, RD_PRIM_SPHERE = 1 } rd_prim_kind;
+typedef enum rd_prim_kind { RD_PRIM_TRIANGLE = 0, RD_PRIM_SPHERE = 1, RD_PRIM_INSTANCE = 2 } rd_prim_kind;
 
-/* One GeometricPrimitive (Primitive.cs:44-78), in the builder's order
- * (entity order, then shapes, then model triangles; RendererBuilder.cs:940-999).
- * The position of a primitive in this array is the "primitive index" that
- * rd_trace_closest reports. */
+/* One GeometricPrimitive (Primitive.cs:44-78) or InstancedPrimitive
+ * (Primitive.cs:144-189), in the builder's order (entity order, then shapes,
+ * then model triangles, then instances; RendererBuilder.cs:940-999).  The
+ * position of a primitive in this array is the "primitive index" that
+ * rd_trace_closest reports.  For RD_PRIM_INSTANCE, `index` is an
+ * rd_scene_desc.instances entry and `light` must be -1 (InstancedPrimitive
+ * cannot carry a light). */
 typedef struct rd_prim_desc {
     int32_t kind;           /* rd_prim_kind                                   */
     int32_t index;          /* triangle or sphere index                       */
@@ -127,6 +130,26 @@ typedef struct rd_prim_desc {
     int32_t medium_inside;  /* -1: vacuum                                     */
     int32_t medium_outside; /* -1: vacuum                                     */
 } rd_prim_desc;
+
+/* One instanced mesh set (RendererBuilder's InstancedPrimitivesEntry,
+ * Config/RendererBuilder.cs:331-340, built at :895-921): triangles
+ * [first_triangle, first_triangle + num_triangles) of rd_scene_desc, already
+ * in the set's model space (its local position/rotation/scale applied, as
+ * TriangleMesh does), each a ShapeWrapperPrimitive (Primitive.cs:110-142)
+ * under the set's own BVH.  These triangles are not scene primitives. */
+typedef struct rd_instance_set_desc {
+    int32_t first_triangle;
+    int32_t num_triangles;
+    int32_t bvh_max_prim;   /* the set's accelerator "max_prim"; 0 = 1 (Bvh default) */
+    int32_t _pad;
+} rd_instance_set_desc;
+
+/* InstancedTransform (Primitive.cs:94-108): the entity's ModelToWorld. */
+typedef struct rd_instance_desc {
+    int32_t set;            /* rd_scene_desc.instance_sets index            */
+    int32_t _pad;
+    float to_world[16];     /* row-major M11..M44 (row vectors)             */
+} rd_instance_desc;
 
 typedef struct rd_camera_desc {
     float fov, near_z, far_z;     /* degrees; defaults 60, 0.001, 1000        */
@@ -173,6 +196,10 @@ typedef struct rd_scene_desc {
     int32_t bvh_max_prim;        /* accel "max_prim", default 1              */
     rd_camera_desc camera;
     rd_integrator_desc integrator;
+    int32_t num_instance_sets;
+    const rd_instance_set_desc* instance_sets;
+    int32_t num_instances;
+    const rd_instance_desc* instances;
 } rd_scene_desc;
 
 /* ---- rays for the traversal oracle hooks -------------------------------- */
@@ -261,6 +288,12 @@ int rd_bvh_build_host(const float* bmin, const float* bmax, int32_t n, int32_t m
 int rd_trace_closest(rd_ctx* ctx, rd_scene* scene, const rd_ray* rays, int32_t n, rd_hit* hits);
 /* Bvh.Intersect(ray) (Bvh.cs:126-172): occluded[i] = 1 if any hit. */
 int rd_trace_any(rd_ctx* ctx, rd_scene* scene, const rd_ray* rays, int32_t n, uint8_t* occluded);
+/* rd_trace_closest that also reports, for a hit on an InstancedPrimitive
+ * (Primitive.cs:165-187: prim = the instance's primitive index, t = world
+ * distance to the transformed hit point, u/v = barycentrics in the set), the
+ * hit triangle as an index into its instance set (0 .. num_triangles-1);
+ * sub[i] = -1 for other hits and misses. */
+int rd_trace_closest_sub(rd_ctx* ctx, rd_scene* scene, const rd_ray* rays, int32_t n, rd_hit* hits, int32_t* sub);
 /* Device-pointer variants: rays/hits already in HBM; stream may be NULL.
  * counters (device, may be NULL) receives {nodes visited, prims tested}
  * summed over the batch as two uint64. */

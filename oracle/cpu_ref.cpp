@@ -301,7 +301,24 @@ struct EnvLight {
     float radius = 0;
 };
 
+// An instanced mesh set (RendererBuilder's InstancedPrimitivesEntry,
+// Config/RendererBuilder.cs:895-921): triangles [first, first+count) in the
+// set's model space under the set's own Bvh.
+struct InstSet {
+    int first = 0, count = 0, max_prim = 1;
+    std::vector<Node> nodes;
+    std::vector<int> order;    // ordered position -> index within the set
+    int leaf_count = 0, max_depth = 0;
+};
+// InstancedPrimitive's transform (Primitive.cs:94-108, 144-161)
+struct Inst {
+    int set = 0;
+    M4 m2w, w2m;
+};
+
 struct cr_scene {
+    std::vector<InstSet> sets;
+    std::vector<Inst> insts;
     std::vector<V3> pos, nrm;
     std::vector<V2> uvs;
     std::vector<int32_t> idx;
@@ -450,17 +467,70 @@ static Inct sphere_inct(const cr_scene& s, int k, const Ray& r, const SurfPt& sp
     return it;
 }
 
+// A geometric primitive's test (GeometricPrimitive.Intersect, Primitive.cs:60-78).
 static inline bool prim_hit(const cr_scene& s, int p, const Ray& r, SurfPt* sp) {
     const rd_prim_desc& pd = s.prims[p];
     return pd.kind == RD_PRIM_TRIANGLE ? tri_hit(s, pd.index, r, sp) : sphere_hit(s, pd.index, r, sp);
 }
-static inline Inct prim_inct(const cr_scene& s, int p, const Ray& r, const SurfPt& sp) {
-    const rd_prim_desc& pd = s.prims[p];
-    return pd.kind == RD_PRIM_TRIANGLE ? tri_inct(s, pd.index, r, sp, p) : sphere_inct(s, pd.index, r, sp, p);
+
+// Ray3F.Transform (Ray3F.cs:27-34): origin and normalised direction through
+// m, MaxT = the distance to the image of ray.At(MaxT), MinT kept.
+static Ray transform_ray(const Ray& r, const M4& m) {
+    V3 o = xform_point(r.o, m);
+    V3 d = normalize(xform_vector(r.d, m));
+    V3 mx = xform_point(r.at(r.tmax), m);
+    return Ray(o, d, r.tmin, length(o - mx));
+}
+
+// BoundingBox3F.Transform (BoundingBox3F.cs:232-243): the eight corners in
+// the reference's order, through m.
+static BBox bbox_transform(const BBox& b, const M4& m) {
+    BBox r(xform_point(V3(b.mn.x, b.mn.y, b.mn.z), m));
+    r.expand(xform_point(V3(b.mx.x, b.mn.y, b.mn.z), m));
+    r.expand(xform_point(V3(b.mn.x, b.mx.y, b.mn.z), m));
+    r.expand(xform_point(V3(b.mn.x, b.mn.y, b.mx.z), m));
+    r.expand(xform_point(V3(b.mn.x, b.mx.y, b.mx.z), m));
+    r.expand(xform_point(V3(b.mx.x, b.mx.y, b.mn.z), m));
+    r.expand(xform_point(V3(b.mx.x, b.mn.y, b.mx.z), m));
+    r.expand(xform_point(V3(b.mx.x, b.mx.y, b.mx.z), m));
+    return r;
+}
+
+// One closest-hit candidate: world T, the scene primitive, the set triangle
+// (index within its set) for an instance, and the SurfacePoint (barycentrics;
+// t is the model-space t for an instance).
+struct Hit {
+    float T = 0;
+    int prim = -1, sub = -1;
+    SurfPt sp{0, 0, 0};
+};
+
+static inline Inct prim_inct(const cr_scene& s, const Hit& h, const Ray& r) {
+    const rd_prim_desc& pd = s.prims[h.prim];
+    if (pd.kind == RD_PRIM_INSTANCE) {
+        // InstancedPrimitive.Intersect (Primitive.cs:165-187): the set
+        // triangle's GetIntersection on the model ray, then P, N and Wr to world
+        const Inst& I = s.insts[pd.index];
+        Ray mr = transform_ray(r, I.w2m);
+        Inct m = tri_inct(s, s.sets[I.set].first + h.sub, mr, h.sp, h.prim);
+        Inct w;
+        w.P = xform_point(m.P, I.m2w);
+        w.sh = Frame(normalize(xform_vector(m.N(), I.m2w)));
+        w.wr = normalize(xform_vector(m.wr, I.m2w));
+        w.UV = m.UV;
+        w.T = h.T;
+        w.prim = h.prim;
+        return w;
+    }
+    return pd.kind == RD_PRIM_TRIANGLE ? tri_inct(s, pd.index, r, h.sp, h.prim) : sphere_inct(s, pd.index, r, h.sp, h.prim);
 }
 
 static BBox prim_bound(const cr_scene& s, int p) {
     const rd_prim_desc& pd = s.prims[p];
+    if (pd.kind == RD_PRIM_INSTANCE) {                     // InstancedPrimitive.WorldBound (Primitive.cs:160)
+        const Inst& I = s.insts[pd.index];
+        return bbox_transform(s.sets[I.set].nodes[0].b, I.m2w);
+    }
     if (pd.kind == RD_PRIM_TRIANGLE) {                     // Triangle ctor (Triangle.cs:88-91)
         BBox b(s.pa(pd.index));
         b.expand(s.pb(pd.index));
@@ -591,52 +661,72 @@ struct Builder {
     }
 };
 
-static void build_bvh(cr_scene& s) {
-    int np = (int)s.prims.size();
+// Bvh ctor over primitive bounds in primitive order.
+static void build_tree(const cr_scene& s, const std::vector<BBox>& bounds, int max_prim, std::vector<Node>* nodes,
+                       std::vector<int>* order, int* leaves, int* max_depth) {
+    int np = (int)bounds.size();
     if (np == 0) return;
-    Builder B(s, s.max_prim);
+    Builder B(s, max_prim);
     B.info.resize(np);
     for (int i = 0; i < np; i++) {
         B.info[i].num = i;
-        B.info[i].b = prim_bound(s, i);
+        B.info[i].b = bounds[i];
         B.info[i].c = 0.5f * B.info[i].b.mn + 0.5f * B.info[i].b.mx;   // BVHPrimitiveInfo :61-66
     }
     B.bn.reserve(2 * np);
     B.ordered.reserve(np);
     int maxd = 0;
     int root = B.build(0, np, 0, &maxd);
-    s.nodes.assign(B.total, Node());
+    nodes->assign(B.total, Node());
     int off = 0;
-    B.flatten(root, s.nodes, &off);
-    s.order = B.ordered;
-    s.leaf_count = B.leaves;
-    s.max_depth = maxd;
+    B.flatten(root, *nodes, &off);
+    *order = B.ordered;
+    *leaves = B.leaves;
+    *max_depth = maxd;
+}
+
+static void build_bvh(cr_scene& s) {
+    // every set's Bvh first (RendererBuilder.cs:895-921), then the scene's
+    for (InstSet& st : s.sets) {
+        std::vector<BBox> b(st.count);
+        for (int j = 0; j < st.count; j++) {                // Triangle ctor (Triangle.cs:88-91)
+            int t = st.first + j;
+            b[j] = BBox(s.pa(t));
+            b[j].expand(s.pb(t));
+            b[j].expand(s.pc(t));
+        }
+        build_tree(s, b, st.max_prim, &st.nodes, &st.order, &st.leaf_count, &st.max_depth);
+    }
+    std::vector<BBox> b(s.prims.size());
+    for (size_t i = 0; i < s.prims.size(); i++) b[i] = prim_bound(s, (int)i);
+    build_tree(s, b, s.max_prim, &s.nodes, &s.order, &s.leaf_count, &s.max_depth);
 }
 
 // ---------------------------------------------------------------- traversal
-// Bvh.Intersect(ray, out inct) (Bvh.cs:174-232).  The full Intersection is
-// built for the final hit only: candidate Intersections differ from it in
-// nothing the loop reads except T, which equals SurfacePoint.T.
-static bool bvh_closest(const cr_scene& s, Ray ray, int* prim_out, SurfPt* sp_out, uint64_t* cnt) {
-    if (s.nodes.empty()) return false;
+// Bvh.Intersect(ray, out inct) (Bvh.cs:174-232) over one tree; `test` is the
+// leaf primitive's Intersect and fills a candidate Hit (its T is what the
+// loop compares).  nv / nt count nodes visited and primitives tested over
+// every level.
+template <class Test>
+static bool dfs_closest(const std::vector<Node>& nodes, const std::vector<int>& order, Ray ray, Test test, Hit* out,
+                        uint64_t* nv, uint64_t* nt) {
+    if (nodes.empty()) return false;
     bool neg[3] = {ray.inv.x < 0, ray.inv.y < 0, ray.inv.z < 0};
     int stack[64];
     int sp = 0, cur = 0;
     float best = FLT_MAX;
     bool any = false;
-    uint64_t nv = 0, nt = 0;
     while (true) {
-        const Node& node = s.nodes[cur];
-        nv++;
+        const Node& node = nodes[cur];
+        (*nv)++;
         if (box_hit(node.b, ray)) {
             if (node.count > 0) {
                 for (int i = 0; i < node.count; ++i) {
-                    int p = s.order[node.offset + i];
-                    SurfPt pt;
-                    nt++;
-                    if (prim_hit(s, p, ray, &pt)) {
+                    Hit h;
+                    (*nt)++;
+                    if (test(order[node.offset + i], ray, &h)) {
                         any = true;
-                        if (pt.t < best) { best = pt.t; *prim_out = p; *sp_out = pt; ray.tmax = pt.t; }
+                        if (h.T < best) { best = h.T; *out = h; ray.tmax = h.T; }
                     }
                 }
                 if (sp == 0) break;
@@ -651,30 +741,26 @@ static bool bvh_closest(const cr_scene& s, Ray ray, int* prim_out, SurfPt* sp_ou
             cur = stack[--sp];
         }
     }
-    if (cnt) { cnt[0] += nv; cnt[1] += nt; }
     return any;
 }
 
-// Bvh.Intersect(ray) (Bvh.cs:126-172)
-static bool bvh_any(const cr_scene& s, const Ray& ray, uint64_t* cnt) {
-    if (s.nodes.empty()) return false;
+// Bvh.Intersect(ray) (Bvh.cs:126-172) over one tree.
+template <class Test>
+static bool dfs_any(const std::vector<Node>& nodes, const std::vector<int>& order, const Ray& ray, Test test,
+                    uint64_t* nv, uint64_t* nt) {
+    if (nodes.empty()) return false;
     bool neg[3] = {ray.inv.x < 0, ray.inv.y < 0, ray.inv.z < 0};
     int stack[64];
     int sp = 0, cur = 0;
-    uint64_t nv = 0, nt = 0;
-    bool hit = false;
     while (true) {
-        const Node& node = s.nodes[cur];
-        nv++;
+        const Node& node = nodes[cur];
+        (*nv)++;
         if (box_hit(node.b, ray)) {
             if (node.count > 0) {
                 for (int i = 0; i < node.count; ++i) {
-                    int p = s.order[node.offset + i];
-                    SurfPt pt;
-                    nt++;
-                    if (prim_hit(s, p, ray, &pt)) { hit = true; break; }
+                    (*nt)++;
+                    if (test(order[node.offset + i], ray)) return true;
                 }
-                if (hit) break;
                 if (sp == 0) break;
                 cur = stack[--sp];
             } else {
@@ -686,17 +772,75 @@ static bool bvh_any(const cr_scene& s, const Ray& ray, uint64_t* cnt) {
             cur = stack[--sp];
         }
     }
+    return false;
+}
+
+// Primitive.Intersect(ray, out inct) of scene primitive p.  An
+// InstancedPrimitive (Primitive.cs:165-187) runs its set's Bvh on the model
+// ray; its T is the world distance from the ray origin to the transformed
+// hit point (Triangle.GetIntersection's barycentric P, Triangle.cs:98-99).
+static bool prim_closest(const cr_scene& s, int p, const Ray& r, Hit* h, uint64_t* nv, uint64_t* nt) {
+    const rd_prim_desc& pd = s.prims[p];
+    if (pd.kind != RD_PRIM_INSTANCE) {
+        SurfPt sp;
+        if (!prim_hit(s, p, r, &sp)) return false;
+        h->T = sp.t; h->prim = p; h->sub = -1; h->sp = sp;
+        return true;
+    }
+    const Inst& I = s.insts[pd.index];
+    const InstSet& st = s.sets[I.set];
+    Ray mr = transform_ray(r, I.w2m);
+    Hit mh;
+    auto tri = [&](int j, const Ray& rr, Hit* o) {
+        SurfPt sp;
+        if (!tri_hit(s, st.first + j, rr, &sp)) return false;
+        o->T = sp.t; o->sub = j; o->sp = sp;
+        return true;
+    };
+    if (!dfs_closest(st.nodes, st.order, mr, tri, &mh, nv, nt)) return false;
+    int t = st.first + mh.sub;
+    V3 bary(1 - (mh.sp.u + mh.sp.v), mh.sp.u, mh.sp.v);
+    V3 pm = bary.x * s.pa(t) + bary.y * s.pb(t) + bary.z * s.pc(t);
+    V3 pw = xform_point(pm, I.m2w);
+    h->T = length(r.o - pw);            // Vector3.Distance(ray.O, p)
+    h->prim = p; h->sub = mh.sub; h->sp = mh.sp;
+    return true;
+}
+
+static bool prim_any(const cr_scene& s, int p, const Ray& r, uint64_t* nv, uint64_t* nt) {
+    const rd_prim_desc& pd = s.prims[p];
+    if (pd.kind != RD_PRIM_INSTANCE) {
+        SurfPt sp;
+        return prim_hit(s, p, r, &sp);
+    }
+    const Inst& I = s.insts[pd.index];
+    const InstSet& st = s.sets[I.set];
+    auto tri = [&](int j, const Ray& rr) { SurfPt sp; return tri_hit(s, st.first + j, rr, &sp); };
+    return dfs_any(st.nodes, st.order, transform_ray(r, I.w2m), tri, nv, nt);
+}
+
+static bool bvh_closest(const cr_scene& s, const Ray& ray, Hit* out, uint64_t* cnt) {
+    uint64_t nv = 0, nt = 0;
+    auto test = [&](int p, const Ray& r, Hit* h) { return prim_closest(s, p, r, h, &nv, &nt); };
+    bool any = dfs_closest(s.nodes, s.order, ray, test, out, &nv, &nt);
+    if (cnt) { cnt[0] += nv; cnt[1] += nt; }
+    return any;
+}
+
+static bool bvh_any(const cr_scene& s, const Ray& ray, uint64_t* cnt) {
+    uint64_t nv = 0, nt = 0;
+    auto test = [&](int p, const Ray& r) { return prim_any(s, p, r, &nv, &nt); };
+    bool hit = dfs_any(s.nodes, s.order, ray, test, &nv, &nt);
     if (cnt) { cnt[0] += nv; cnt[1] += nt; }
     return hit;
 }
 
 // Scene.Intersect(ray, out inct) (Scene.cs:32-35)
 static bool scene_intersect(const cr_scene& s, const Ray& r, Inct* it, uint64_t* st) {
-    int p = -1;
-    SurfPt sp;
+    Hit h;
     if (st) st[0]++;
-    if (!bvh_closest(s, r, &p, &sp, nullptr)) return false;
-    *it = prim_inct(s, p, r, sp);
+    if (!bvh_closest(s, r, &h, nullptr)) return false;
+    *it = prim_inct(s, h, r);
     return true;
 }
 static bool scene_occluded_ray(const cr_scene& s, const Ray& r, uint64_t* st) {
@@ -1591,9 +1735,10 @@ static float light_pdf_li(const cr_scene& s, int light, V3 refp, V3 wi) {
     const rd_light_desc& L = s.lights[light];
     if (L.type == RD_LIGHT_INFINITE) return env_pdf_dir(s.env[s.env_of_light[light]], wi);   // PdfLi :87-90
     Ray ray(refp, wi);
-    SurfPt sp;
-    if (!prim_hit(s, L.prim, ray, &sp)) return 0.0f;
-    Inct hit = prim_inct(s, L.prim, ray, sp);
+    Hit h;
+    if (!prim_hit(s, L.prim, ray, &h.sp)) return 0.0f;
+    h.T = h.sp.t; h.prim = L.prim;
+    Inct hit = prim_inct(s, h, ray);
     float pdf = length_sq(refp - hit.P) / (abs_dot(hit.N(), -wi) * s.prim_area[L.prim]);
     if (std::isinf(pdf)) pdf = 0.0f;
     return pdf;
@@ -2339,9 +2484,36 @@ cr_scene* cr_scene_create(const rd_scene_desc* d) {
             t.data.assign(td.data, td.data + (size_t)t.w * t.h * 3);
         }
     }
+    if (d->num_instance_sets < 0 || d->num_instances < 0) { set_err("negative count"); delete s; return nullptr; }
+    for (int i = 0; i < d->num_instance_sets; i++) {
+        const rd_instance_set_desc& sd = d->instance_sets[i];
+        if (sd.first_triangle < 0 || sd.num_triangles <= 0 || sd.num_triangles > d->num_triangles - sd.first_triangle) {
+            set_err("instance set range"); delete s; return nullptr;
+        }
+        InstSet st;
+        st.first = sd.first_triangle; st.count = sd.num_triangles; st.max_prim = sd.bvh_max_prim > 0 ? sd.bvh_max_prim : 1;
+        s->sets.push_back(st);
+    }
+    for (int i = 0; i < d->num_instances; i++) {
+        const rd_instance_desc& id = d->instances[i];
+        if (id.set < 0 || id.set >= d->num_instance_sets) { set_err("instance set index"); delete s; return nullptr; }
+        Inst in;
+        in.set = id.set;
+        for (int k = 0; k < 16; k++) in.m2w.m[k / 4][k % 4] = id.to_world[k];
+        // InstancedTransform ctor: Matrix4x4.Invert (Primitive.cs:99-106)
+        if (!m4_invert(in.m2w, &in.w2m)) { set_err("instance: invalid transform matrix"); delete s; return nullptr; }
+        s->insts.push_back(in);
+    }
     s->prim_area.resize(s->prims.size());
     for (size_t p = 0; p < s->prims.size(); p++) {
         const rd_prim_desc& pd = s->prims[p];
+        if (pd.kind == RD_PRIM_INSTANCE) {
+            if (pd.index < 0 || pd.index >= d->num_instances) { set_err("prim instance index"); delete s; return nullptr; }
+            if (pd.light != -1) { set_err("an instanced primitive cannot be a light"); delete s; return nullptr; }
+            s->prim_area[p] = 0.0f;
+            if (pd.material >= d->num_materials) { set_err("prim material/light index"); delete s; return nullptr; }
+            continue;
+        }
         if (pd.kind == RD_PRIM_TRIANGLE) {
             if (pd.index < 0 || pd.index >= d->num_triangles) { set_err("prim triangle index"); delete s; return nullptr; }
             int t = pd.index;
@@ -2446,13 +2618,18 @@ static inline Ray to_ray(const rd_ray& r) {
 }
 
 int cr_trace_closest(const cr_scene* s, const rd_ray* rays, int32_t n, rd_hit* hits, uint64_t* counters) {
+    return cr_trace_closest_sub(s, rays, n, hits, nullptr, counters);
+}
+
+int cr_trace_closest_sub(const cr_scene* s, const rd_ray* rays, int32_t n, rd_hit* hits, int32_t* sub,
+                         uint64_t* counters) {
     if (!s) return RD_ERR_INVALID;
     for (int i = 0; i < n; i++) {
         Ray r = to_ray(rays[i]);
-        int p = -1;
-        SurfPt sp{0, 0, 0};
-        if (bvh_closest(*s, r, &p, &sp, counters)) { hits[i].t = sp.t; hits[i].prim = p; hits[i].u = sp.u; hits[i].v = sp.v; }
+        Hit h;
+        if (bvh_closest(*s, r, &h, counters)) { hits[i].t = h.T; hits[i].prim = h.prim; hits[i].u = h.sp.u; hits[i].v = h.sp.v; }
         else { hits[i].t = 0; hits[i].prim = -1; hits[i].u = 0; hits[i].v = 0; }
+        if (sub) sub[i] = h.prim >= 0 ? h.sub : -1;
     }
     return RD_OK;
 }
@@ -2463,16 +2640,43 @@ int cr_trace_any(const cr_scene* s, const rd_ray* rays, int32_t n, uint8_t* occ,
     return RD_OK;
 }
 
+// Every primitive (and every triangle of every instance, on the model ray
+// with the original MaxT) tested; the smallest world T wins.
 int cr_trace_brute(const cr_scene* s, const rd_ray* rays, int32_t n, rd_hit* hits) {
+    return cr_trace_brute_sub(s, rays, n, hits, nullptr);
+}
+
+int cr_trace_brute_sub(const cr_scene* s, const rd_ray* rays, int32_t n, rd_hit* hits, int32_t* sub) {
     if (!s) return RD_ERR_INVALID;
     for (int i = 0; i < n; i++) {
         Ray r = to_ray(rays[i]);
         hits[i].prim = -1; hits[i].t = 0; hits[i].u = 0; hits[i].v = 0;
+        if (sub) sub[i] = -1;
         float best = FLT_MAX;
         for (int p = 0; p < (int)s->prims.size(); p++) {
-            SurfPt sp;
-            if (prim_hit(*s, p, r, &sp) && sp.t < best) {
-                best = sp.t; hits[i].t = sp.t; hits[i].prim = p; hits[i].u = sp.u; hits[i].v = sp.v;
+            const rd_prim_desc& pd = s->prims[p];
+            if (pd.kind != RD_PRIM_INSTANCE) {
+                SurfPt sp;
+                if (prim_hit(*s, p, r, &sp) && sp.t < best) {
+                    best = sp.t; hits[i].t = sp.t; hits[i].prim = p; hits[i].u = sp.u; hits[i].v = sp.v;
+                    if (sub) sub[i] = -1;
+                }
+                continue;
+            }
+            const Inst& I = s->insts[pd.index];
+            const InstSet& st = s->sets[I.set];
+            Ray mr = transform_ray(r, I.w2m);
+            for (int j = 0; j < st.count; j++) {
+                SurfPt sp;
+                int t = st.first + j;
+                if (!tri_hit(*s, t, mr, &sp)) continue;
+                V3 bary(1 - (sp.u + sp.v), sp.u, sp.v);
+                V3 pw = xform_point(bary.x * s->pa(t) + bary.y * s->pb(t) + bary.z * s->pc(t), I.m2w);
+                float T = length(r.o - pw);
+                if (T < best) {
+                    best = T; hits[i].t = T; hits[i].prim = p; hits[i].u = sp.u; hits[i].v = sp.v;
+                    if (sub) sub[i] = j;
+                }
             }
         }
     }

@@ -28,6 +28,11 @@ int cr_trace_any(const cr_scene* s, const rd_ray* rays, int32_t n, uint8_t* occl
 /* Brute force over all primitives in input order (the Naive aggregate,
  * Accelerators/Naive.cs): smallest t, first index on ties. */
 int cr_trace_brute(const cr_scene* s, const rd_ray* rays, int32_t n, rd_hit* hits);
+/* As cr_trace_closest / cr_trace_brute, plus sub[i] = the hit triangle's index
+ * within its instance set (-1 for other hits and misses). */
+int cr_trace_closest_sub(const cr_scene* s, const rd_ray* rays, int32_t n, rd_hit* hits, int32_t* sub,
+                         uint64_t* counters);
+int cr_trace_brute_sub(const cr_scene* s, const rd_ray* rays, int32_t n, rd_hit* hits, int32_t* sub);
 /* Camera.SampleRay for pixel sample positions (x,y) (already offset by U^2). */
 int cr_camera_rays(const cr_scene* s, const float* xy, int32_t n, rd_ray* rays);
 /* Renders like BlockBasedRenderer with `threads` workers over 32x32 blocks

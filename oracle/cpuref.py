@@ -36,6 +36,8 @@ def lib():
         for fn in ("cr_trace_closest", "cr_trace_any"):
             getattr(L, fn).argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
         L.cr_trace_brute.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
+        L.cr_trace_closest_sub.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.cr_trace_brute_sub.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
         L.cr_camera_rays.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
         L.cr_render.argtypes = [C.c_void_p, C.POINTER(abi.rd_render_params), C.c_int32, C.c_void_p,
                                 C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
@@ -226,6 +228,23 @@ class OracleScene:
         hits = np.zeros(rays.shape[0], abi.hit_dtype())
         lib().cr_trace_brute(self.h, _ptr(rays), rays.shape[0], _ptr(hits))
         return hits
+
+    def trace_closest_sub(self, rays):
+        """(hits, sub, counters): sub = the hit triangle's index within its
+        instance set, -1 for other hits."""
+        rays = np.ascontiguousarray(rays, abi.ray_dtype())
+        hits = np.zeros(rays.shape[0], abi.hit_dtype())
+        sub = np.full(rays.shape[0], -1, np.int32)
+        cnt = np.zeros(2, np.uint64)
+        lib().cr_trace_closest_sub(self.h, _ptr(rays), rays.shape[0], _ptr(hits), _ptr(sub), _ptr(cnt))
+        return hits, sub, cnt
+
+    def trace_brute_sub(self, rays):
+        rays = np.ascontiguousarray(rays, abi.ray_dtype())
+        hits = np.zeros(rays.shape[0], abi.hit_dtype())
+        sub = np.full(rays.shape[0], -1, np.int32)
+        lib().cr_trace_brute_sub(self.h, _ptr(rays), rays.shape[0], _ptr(hits), _ptr(sub))
+        return hits, sub
 
     def camera_rays(self, xy):
         xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)

@@ -5,7 +5,7 @@ checks sizes and offsets against the compiled library.
 """
 import ctypes as C
 
-ABI_VERSION = 1   # RD_ABI_VERSION
+ABI_VERSION = 2   # RD_ABI_VERSION
 RD_OK = 0
 RD_ERR_INVALID = 1
 RD_ERR_DEVICE = 2
@@ -39,6 +39,7 @@ LIGHT_AREA = 0
 LIGHT_INFINITE = 1
 PRIM_TRIANGLE = 0
 PRIM_SPHERE = 1
+PRIM_INSTANCE = 2
 INTEGRATOR_PATH = 0
 INTEGRATOR_VOL_PATH = 1
 METHOD_BXDF = 0
@@ -99,6 +100,15 @@ class rd_integrator_desc(C.Structure):
     ]
 
 
+class rd_instance_set_desc(C.Structure):
+    _fields_ = [("first_triangle", C.c_int32), ("num_triangles", C.c_int32), ("bvh_max_prim", C.c_int32),
+                ("_pad", C.c_int32)]
+
+
+class rd_instance_desc(C.Structure):
+    _fields_ = [("set", C.c_int32), ("_pad", C.c_int32), ("to_world", C.c_float * 16)]
+
+
 class rd_scene_desc(C.Structure):
     _fields_ = [
         ("num_vertices", C.c_int32), ("positions", C.POINTER(C.c_float)),
@@ -113,6 +123,8 @@ class rd_scene_desc(C.Structure):
         ("num_media", C.c_int32), ("media", C.POINTER(rd_medium_desc)),
         ("global_medium", C.c_int32), ("bvh_max_prim", C.c_int32),
         ("camera", rd_camera_desc), ("integrator", rd_integrator_desc),
+        ("num_instance_sets", C.c_int32), ("instance_sets", C.POINTER(rd_instance_set_desc)),
+        ("num_instances", C.c_int32), ("instances", C.POINTER(rd_instance_desc)),
     ]
 
 

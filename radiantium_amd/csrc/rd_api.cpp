@@ -119,6 +119,8 @@ struct rd_scene {
     std::vector<DevTexture> texs;       // host copy of the texture table
     std::vector<size_t> tex_elems;      // texels per texture (1 for constants)
     std::vector<int> env_textures;      // textures an environment light samples
+    std::vector<int> set_tri_base;      // DevPrim index of set i's first triangle, minus num_top_prims
+    int num_top_prims = 0;
     ~rd_scene() {
         for (void* p : allocs) (void)hipFree(p);
         if (ctx) ctx->live_scenes--;
@@ -250,10 +252,12 @@ int ensure_wave_buffers(rd_ctx* c, size_t pool, size_t results, size_t pixels, b
         ps.rng_n = (uint32_t*)alloc(cap * sizeof(uint32_t));
         ps.depth = (int*)alloc(cap * sizeof(int));
         ps.vis = (uint8_t*)alloc(cap);
+        ps.hsub = (int*)alloc(cap * sizeof(int));
         ok = ok && w.hits && ps.L && ps.beta && ps.nee_beta && ps.nee_le && ps.mis_f && ps.mis_li && ps.flags &&
              ps.sel_light && ps.key && ps.req && ps.gid && ps.pix && ps.alive && ps.ray_o && ps.ray_d && w.blk && ps.rng_n &&
-             ps.depth && ps.vis;
+             ps.depth && ps.vis && ps.hsub;
         ps.env = nullptr; ps.nee_li = ps.sh_to = ps.sh_tr = ps.mis_tr = nullptr;
+        ps.hsub2 = nullptr;
         if (ok && vol) {
             ok = mkq(w.qshadow2, kCntShadowB) && mkq(w.qmis2, kCntMisB);
             w.hits2 = (float4*)alloc(cap * sizeof(float4));
@@ -262,7 +266,8 @@ int ensure_wave_buffers(rd_ctx* c, size_t pool, size_t results, size_t pixels, b
             ps.sh_to = (float4*)alloc(cap * sizeof(float4));
             ps.sh_tr = (float4*)alloc(cap * sizeof(float4));
             ps.mis_tr = (float4*)alloc(cap * sizeof(float4));
-            ok = ok && w.hits2 && ps.env && ps.nee_li && ps.sh_to && ps.sh_tr && ps.mis_tr;
+            ps.hsub2 = (int*)alloc(cap * sizeof(int));
+            ok = ok && w.hits2 && ps.env && ps.nee_li && ps.sh_to && ps.sh_tr && ps.mis_tr && ps.hsub2;
         }
         ps.adj0 = ps.adj1 = ps.adj2 = nullptr;
         if (!ok) { w.release(); return fail(RD_ERR_OOM, "hipMalloc failed for the path pool"); }
@@ -401,11 +406,29 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     if (d->integrator.type != RD_INTEGRATOR_PATH && d->integrator.type != RD_INTEGRATOR_VOL_PATH)
         return bad("unknown integrator type");
     if (d->global_medium < -1 || d->global_medium >= d->num_media) return bad("global medium out of range");
+    if (d->num_instance_sets < 0 || d->num_instances < 0) return bad("negative count");
+    if (d->num_instance_sets > 0 && !d->instance_sets) return bad("instance sets missing");
+    if (d->num_instances > 0 && !d->instances) return bad("instances missing");
+    for (int i = 0; i < d->num_instance_sets; i++) {
+        const rd_instance_set_desc& sd = d->instance_sets[i];
+        if (sd.first_triangle < 0 || sd.num_triangles <= 0 || sd.num_triangles > d->num_triangles - sd.first_triangle)
+            return bad("instance set triangle range out of range");
+    }
+    for (int i = 0; i < d->num_instances; i++) {
+        const rd_instance_desc& id = d->instances[i];
+        if (id.set < 0 || id.set >= d->num_instance_sets) return bad("instance set index out of range");
+        float inv[16];
+        if (!invert4(id.to_world, inv)) return bad("instance: invalid transform matrix");   // InstancedTransform ctor
+    }
     for (int i = 0; i < d->num_prims; i++) {
         const rd_prim_desc& pd = d->prims[i];
         if (pd.medium_inside < -1 || pd.medium_inside >= d->num_media || pd.medium_outside < -1 ||
             pd.medium_outside >= d->num_media)
             return bad("primitive medium out of range");
+        if (pd.kind == RD_PRIM_INSTANCE) {
+            if (pd.index < 0 || pd.index >= d->num_instances) return bad("primitive instance index out of range");
+            if (pd.light != -1) return bad("an instanced primitive cannot be a light");
+        }
     }
     if (d->integrator.strategy != RD_STRATEGY_UNIFORM) return bad("light strategy All is not built for the GPU path yet");
     for (int t = 0; t < d->num_triangles; t++) {
@@ -445,6 +468,7 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
         }
         if (l.type != RD_LIGHT_AREA) return bad("unknown light type");
         if (l.prim < 0 || l.prim >= d->num_prims) return bad("light primitive out of range");
+        if (d->prims[l.prim].kind == RD_PRIM_INSTANCE) return bad("an instanced primitive cannot be a light");
     }
 
     if (c->closed) { g_err = "context destroyed"; return nullptr; }
@@ -455,9 +479,37 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     s->height = d->camera.height;
     auto t0 = std::chrono::steady_clock::now();
     const int np = d->num_prims;
-    std::vector<float> bmin(3 * (size_t)np), bmax(3 * (size_t)np);
-    std::vector<DevPrim> info(np);
     auto P = [&](int v, int k) { return d->positions[3 * (size_t)v + k]; };
+    // Triangle ctor: bounds of the three vertices (Triangle.cs:83-94)
+    auto tri_bounds = [&](int tri, float* lo3, float* hi3) {
+        const int* ix = d->indices + 3 * (size_t)tri;
+        for (int k = 0; k < 3; k++) {
+            float a = P(ix[0], k), b = P(ix[1], k), cc = P(ix[2], k);
+            float lo = a, hi = a;
+            lo = lo < b ? lo : b; hi = hi > b ? hi : b;
+            lo = lo < cc ? lo : cc; hi = hi > cc ? hi : cc;
+            lo3[k] = lo;
+            hi3[k] = hi;
+        }
+    };
+    // one BVH per instance set over its ShapeWrapperPrimitives (RendererBuilder.cs:895-921)
+    const int nsets = d->num_instance_sets;
+    std::vector<BvhResult> blas(nsets);
+    std::vector<int> set_tri_base(nsets + 1, 0);
+    for (int si = 0; si < nsets; si++) {
+        const rd_instance_set_desc& sd = d->instance_sets[si];
+        std::vector<float> lo(3 * (size_t)sd.num_triangles), hi(3 * (size_t)sd.num_triangles);
+        for (int j = 0; j < sd.num_triangles; j++) tri_bounds(sd.first_triangle + j, &lo[3 * j], &hi[3 * j]);
+        std::string berr;
+        if (!build_bvh(sd.num_triangles, lo.data(), hi.data(), sd.bvh_max_prim > 0 ? sd.bvh_max_prim : 1, &blas[si], &berr)) {
+            delete s;
+            g_err = berr;
+            return nullptr;
+        }
+        set_tri_base[si + 1] = set_tri_base[si] + sd.num_triangles;
+    }
+    std::vector<float> bmin(3 * (size_t)np), bmax(3 * (size_t)np);
+    std::vector<DevPrim> info((size_t)np + set_tri_base[nsets]);
     for (int p = 0; p < np; p++) {
         const rd_prim_desc& pd = d->prims[p];
         DevPrim& di = info[p];
@@ -470,17 +522,33 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
             f3 A = mk(P(ix[0], 0), P(ix[0], 1), P(ix[0], 2));
             f3 B = mk(P(ix[1], 0), P(ix[1], 1), P(ix[1], 2));
             f3 C = mk(P(ix[2], 0), P(ix[2], 1), P(ix[2], 2));
-            // Triangle ctor: bounds of the three vertices, area = |cross|/2 (Triangle.cs:83-94)
-            for (int k = 0; k < 3; k++) {
-                float a = comp(A, k), b = comp(B, k), cc = comp(C, k);
-                float lo = a, hi = a;
-                lo = lo < b ? lo : b; hi = hi > b ? hi : b;
-                lo = lo < cc ? lo : cc; hi = hi > cc ? hi : cc;
-                bmin[3 * p + k] = lo;
-                bmax[3 * p + k] = hi;
-            }
+            // area = |cross|/2 (Triangle.cs:83-94)
+            tri_bounds(pd.index, &bmin[3 * p], &bmax[3 * p]);
             di.area = 0.5f * length(cross(B - A, C - A));
             di.flags = d->tri_flags ? d->tri_flags[pd.index] : 0;
+        } else if (pd.kind == RD_PRIM_INSTANCE) {
+            // InstancedPrimitive.WorldBound = BoundingBox3F.Transform(set root, ModelToWorld)
+            // (Primitive.cs:154-161, BoundingBox3F.cs:232-243)
+            const rd_instance_desc& id = d->instances[pd.index];
+            const NodeRec& root = blas[id.set].nodes[0];
+            const float* M = id.to_world;
+            float lo[3], hi[3];
+            // corners in the reference's ExpendBy order: bit 0 = max x, bit 1 = max y, bit 2 = max z
+            static const int kCorner[8] = {0, 1, 2, 4, 6, 3, 5, 7};
+            for (int c = 0; c < 8; c++) {
+                int b = kCorner[c];
+                float x = (b & 1) ? root.mx[0] : root.mn[0];
+                float y = (b & 2) ? root.mx[1] : root.mn[1];
+                float z = (b & 4) ? root.mx[2] : root.mn[2];
+                float w[3];
+                for (int k = 0; k < 3; k++) w[k] = x * M[k] + y * M[4 + k] + z * M[8 + k] + M[12 + k];
+                for (int k = 0; k < 3; k++) {
+                    if (c == 0) { lo[k] = w[k]; hi[k] = w[k]; }
+                    else { lo[k] = lo[k] < w[k] ? lo[k] : w[k]; hi[k] = hi[k] > w[k] ? hi[k] : w[k]; }
+                }
+            }
+            for (int k = 0; k < 3; k++) { bmin[3 * p + k] = lo[k]; bmax[3 * p + k] = hi[k]; }
+            di.area = 0.0f;
         } else if (pd.kind == RD_PRIM_SPHERE) {
             if (pd.index < 0 || pd.index >= d->num_spheres) { delete s; return bad("primitive sphere index out of range"); }
             const float* S = d->spheres + 4 * (size_t)pd.index;
@@ -502,37 +570,83 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     s->nodes = bvh.nodes;
     s->order = bvh.order;
 
-    // node and ordered-primitive records
-    std::vector<float4> nodes(2 * bvh.nodes.size());
-    for (size_t i = 0; i < bvh.nodes.size(); i++) {
-        const NodeRec& r = bvh.nodes[i];
-        int packed = (int)r.count | ((int)r.axis << 16);
-        float offf, packf;
-        std::memcpy(&offf, &r.offset, 4);
-        std::memcpy(&packf, &packed, 4);
-        nodes[2 * i] = make_float4(r.mn[0], r.mn[1], r.mn[2], r.mx[0]);
-        nodes[2 * i + 1] = make_float4(r.mx[1], r.mx[2], offf, packf);
-    }
+    // node and ordered-primitive records: the scene tree, then every set's
+    // tree with its child / primitive offsets made absolute
+    auto ibits = [](int v) { float f; std::memcpy(&f, &v, 4); return f; };
+    std::vector<float4> nodes;
+    auto add_nodes = [&](const BvhResult& b, int node_base, int prim_base) {
+        for (const NodeRec& r : b.nodes) {
+            int packed = (int)r.count | ((int)r.axis << 16);
+            int off = r.count > 0 ? r.offset + prim_base : r.offset + node_base;
+            nodes.push_back(make_float4(r.mn[0], r.mn[1], r.mn[2], r.mx[0]));
+            nodes.push_back(make_float4(r.mx[1], r.mx[2], ibits(off), ibits(packed)));
+        }
+    };
+    add_nodes(bvh, 0, 0);
     std::vector<float4> prims(3 * (size_t)np);
+    auto put_tri = [&](size_t i, int tri, int orig) {
+        const int* ix = d->indices + 3 * (size_t)tri;
+        prims[3 * i] = make_float4(P(ix[0], 0), P(ix[0], 1), P(ix[0], 2), ibits(orig));
+        prims[3 * i + 1] = make_float4(P(ix[1], 0), P(ix[1], 1), P(ix[1], 2), ibits(kTri));
+        prims[3 * i + 2] = make_float4(P(ix[2], 0), P(ix[2], 1), P(ix[2], 2), 0.0f);
+    };
     for (int i = 0; i < np; i++) {
         int orig = bvh.order[i];
         const rd_prim_desc& pd = d->prims[orig];
-        float origf, kindf;
-        int kind = pd.kind;
-        std::memcpy(&origf, &orig, 4);
-        std::memcpy(&kindf, &kind, 4);
         if (pd.kind == RD_PRIM_TRIANGLE) {
-            const int* ix = d->indices + 3 * (size_t)pd.index;
-            prims[3 * i] = make_float4(P(ix[0], 0), P(ix[0], 1), P(ix[0], 2), origf);
-            prims[3 * i + 1] = make_float4(P(ix[1], 0), P(ix[1], 1), P(ix[1], 2), kindf);
-            prims[3 * i + 2] = make_float4(P(ix[2], 0), P(ix[2], 1), P(ix[2], 2), 0.0f);
+            put_tri(i, pd.index, orig);
+        } else if (pd.kind == RD_PRIM_INSTANCE) {
+            prims[3 * i] = make_float4(ibits(pd.index), 0.f, 0.f, ibits(orig));
+            prims[3 * i + 1] = make_float4(0.f, 0.f, 0.f, ibits(kInst));
+            prims[3 * i + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
         } else {
             const float* S = d->spheres + 4 * (size_t)pd.index;
-            prims[3 * i] = make_float4(S[0], S[1], S[2], origf);
-            prims[3 * i + 1] = make_float4(S[3], 0.f, 0.f, kindf);
+            prims[3 * i] = make_float4(S[0], S[1], S[2], ibits(orig));
+            prims[3 * i + 1] = make_float4(S[3], 0.f, 0.f, ibits(kSphere));
             prims[3 * i + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
     }
+    std::vector<int> set_root(nsets);
+    int blas_depth = 0;
+    size_t blas_nodes = 0;
+    for (int si = 0; si < nsets; si++) {
+        const rd_instance_set_desc& sd = d->instance_sets[si];
+        const BvhResult& b = blas[si];
+        set_root[si] = (int)(nodes.size() / 2);
+        int prim_base = (int)(prims.size() / 3);
+        add_nodes(b, set_root[si], prim_base);
+        prims.resize(prims.size() + 3 * (size_t)sd.num_triangles);
+        for (int k = 0; k < sd.num_triangles; k++) {
+            int j = b.order[k];   // index within the set
+            put_tri((size_t)prim_base + k, sd.first_triangle + j, np + set_tri_base[si] + j);
+        }
+        for (int j = 0; j < sd.num_triangles; j++) {
+            int tri = sd.first_triangle + j;
+            DevPrim& di = info[(size_t)np + set_tri_base[si] + j];
+            const int* ix = d->indices + 3 * (size_t)tri;
+            f3 A = mk(P(ix[0], 0), P(ix[0], 1), P(ix[0], 2));
+            f3 B = mk(P(ix[1], 0), P(ix[1], 1), P(ix[1], 2));
+            f3 C = mk(P(ix[2], 0), P(ix[2], 1), P(ix[2], 2));
+            di.kind = kTri; di.index = tri; di.material = -1; di.light = -1; di.med_in = -1; di.med_out = -1;
+            di.area = 0.5f * length(cross(B - A, C - A));
+            di.flags = d->tri_flags ? d->tri_flags[tri] : 0;
+        }
+        blas_depth = std::max(blas_depth, b.max_depth);
+        blas_nodes += b.nodes.size();
+    }
+    std::vector<DevInstance> insts(d->num_instances);
+    for (int i = 0; i < d->num_instances; i++) {
+        const rd_instance_desc& id = d->instances[i];
+        DevInstance& o = insts[i];
+        std::memset(&o, 0, sizeof(o));
+        float inv[16];
+        invert4(id.to_world, inv);   // InstancedTransform: Matrix4x4.Invert (Primitive.cs:99-106)
+        for (int r = 0; r < 4; r++)
+            for (int c = 0; c < 3; c++) { o.m2w[3 * r + c] = id.to_world[4 * r + c]; o.w2m[3 * r + c] = inv[4 * r + c]; }
+        o.root = set_root[id.set];
+    }
+    s->set_tri_base = set_tri_base;
+    s->num_top_prims = np;
     std::vector<float4> spheres(d->num_spheres);
     for (int i = 0; i < d->num_spheres; i++)
         spheres[i] = make_float4(d->spheres[4 * i], d->spheres[4 * i + 1], d->spheres[4 * i + 2], d->spheres[4 * i + 3]);
@@ -616,9 +730,11 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
 
     DevScene& ds = s->dev;
     if (!setup_camera(d->camera, &ds.cam)) { delete s; return bad("camera matrix not invertible"); }
-    ds.num_nodes = (int)bvh.nodes.size();
+    ds.num_nodes = (int)(nodes.size() / 2);
     ds.num_prims = np;
-    ds.stack_needed = bvh.max_depth;
+    // a set's traversal runs on top of the scene tree's stack entries
+    ds.stack_needed = bvh.max_depth + (nsets > 0 ? blas_depth : 0);
+    ds.num_insts = d->num_instances;
     ds.num_lights = d->num_lights;
     ds.global_medium = d->global_medium;
     int rc = RD_OK;
@@ -635,6 +751,7 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     if (!rc) rc = upload(s, texels.data(), texels.size(), &ds.texels);
     if (!rc) rc = upload(s, lights.data(), lights.size(), &ds.lights);
     if (!rc) rc = upload(s, media.data(), media.size(), &ds.media);
+    if (!rc) rc = upload(s, insts.data(), insts.size(), &ds.insts);
     for (size_t i = 0; i < envs.size() && !rc; i++) {
         const EnvTables& t = env_tables[i];
         rc = upload(s, t.cond_func.data(), t.cond_func.size(), &envs[i].cond_func);
@@ -660,7 +777,7 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     s->integ.strategy = I.strategy;
     s->integ.rr_threshold = I.rr_threshold;
 
-    s->info.num_nodes = ds.num_nodes;
+    s->info.num_nodes = (int)bvh.nodes.size();   // the scene tree (rd_scene_export_bvh)
     s->info.num_leaves = bvh.leaves;
     s->info.max_depth = bvh.max_depth;
     s->info.num_prims = np;
@@ -722,7 +839,7 @@ int rd_trace_closest_device(rd_ctx* c, rd_scene* s, const rd_ray* d_rays, int32_
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     Timer tm(c);
     HIP_TRY(tm.run("trace_rays_closest", [&] {
-        return launch_trace_rays(s->dev, (const float4*)d_rays, n, (float4*)d_hits, nullptr,
+        return launch_trace_rays(s->dev, (const float4*)d_rays, n, (float4*)d_hits, nullptr, nullptr,
                                  (unsigned long long*)d_counters, false, c->counts, st);
     }));
     if (!stream) {
@@ -739,8 +856,8 @@ int rd_trace_any_device(rd_ctx* c, rd_scene* s, const rd_ray* d_rays, int32_t n,
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     Timer tm(c);
     HIP_TRY(tm.run("trace_rays_any", [&] {
-        return launch_trace_rays(s->dev, (const float4*)d_rays, n, nullptr, d_occ, (unsigned long long*)d_counters,
-                                 true, c->counts, st);
+        return launch_trace_rays(s->dev, (const float4*)d_rays, n, nullptr, d_occ, nullptr,
+                                 (unsigned long long*)d_counters, true, c->counts, st);
     }));
     if (!stream) {
         HIP_TRY(hipStreamSynchronize(st));
@@ -764,6 +881,42 @@ int rd_trace_closest(rd_ctx* c, rd_scene* s, const rd_ray* rays, int32_t n, rd_h
     if (!rc && hipMemcpy(hits, dh, sizeof(rd_hit) * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(RD_ERR_DEVICE, "copy hits");
     (void)hipFree(dr);
     (void)hipFree(dh);
+    return rc;
+}
+
+int rd_trace_closest_sub(rd_ctx* c, rd_scene* s, const rd_ray* rays, int32_t n, rd_hit* hits, int32_t* sub) {
+    if (!c || !s || n < 0 || (n > 0 && (!rays || !hits || !sub))) return fail(RD_ERR_INVALID, "bad argument");
+    if (n == 0) return RD_OK;
+    if (int rc = check_stack(s)) return rc;
+    CtxLock lock(c);
+    if (!lock.ok()) return fail(RD_ERR_BUSY, "context in use by another thread");
+    (void)hipSetDevice(c->device);
+    void *dr = nullptr, *dh = nullptr, *ds = nullptr;
+    if (hipMalloc(&dr, sizeof(rd_ray) * (size_t)n) != hipSuccess) return fail(RD_ERR_OOM, "hipMalloc rays");
+    int rc = RD_OK;
+    if (hipMalloc(&dh, sizeof(rd_hit) * (size_t)n) != hipSuccess) rc = fail(RD_ERR_OOM, "hipMalloc hits");
+    if (!rc && hipMalloc(&ds, sizeof(int32_t) * (size_t)n) != hipSuccess) rc = fail(RD_ERR_OOM, "hipMalloc sub");
+    hipStream_t st = c->stream;
+    if (!rc && hipMemcpyAsync(dr, rays, sizeof(rd_ray) * (size_t)n, hipMemcpyHostToDevice, st) != hipSuccess)
+        rc = fail(RD_ERR_DEVICE, "copy rays");
+    if (!rc && hipMemsetAsync(ds, 0xff, sizeof(int32_t) * (size_t)n, st) != hipSuccess) rc = fail(RD_ERR_DEVICE, "clear sub");
+    if (!rc && launch_trace_rays(s->dev, (const float4*)dr, n, (float4*)dh, nullptr, (int*)ds, nullptr, false, c->counts,
+                                 st) != hipSuccess)
+        rc = fail(RD_ERR_DEVICE, "trace launch");
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(RD_ERR_DEVICE, "trace");
+    if (!rc && hipMemcpy(hits, dh, sizeof(rd_hit) * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(RD_ERR_DEVICE, "copy hits");
+    if (!rc && hipMemcpy(sub, ds, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(RD_ERR_DEVICE, "copy sub");
+    // DevPrim index of a set triangle -> its index within the set
+    for (int32_t i = 0; i < n && !rc; i++) {
+        int g = sub[i];
+        if (g < 0) { sub[i] = -1; continue; }
+        int k = g - s->num_top_prims;
+        auto it = std::upper_bound(s->set_tri_base.begin(), s->set_tri_base.end(), k);
+        sub[i] = k - *(it - 1);
+    }
+    (void)hipFree(dr);
+    (void)hipFree(dh);
+    (void)hipFree(ds);
     return rc;
 }
 
@@ -991,7 +1144,10 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
     for (size_t g = 0; g < groups.size() && !rc; g++) {
         const std::vector<int>& G = groups[g];
         DevScene rep = dscenes[G[0]];   // the traversal variant must cover the deepest tree of the group
-        for (int i : G) rep.stack_needed = std::max(rep.stack_needed, dscenes[i].stack_needed);
+        for (int i : G) {
+            rep.stack_needed = std::max(rep.stack_needed, dscenes[i].stack_needed);
+            rep.num_insts = std::max(rep.num_insts, dscenes[i].num_insts);   // any instanced scene: two-level kernels
+        }
         std::vector<int> next_sample(n, 0);
         size_t gi = 0;
         while (gi < G.size() && !rc) {

@@ -45,6 +45,10 @@ struct PathState {
     float4* sh_to;      // Transmittance chain: light point; w = current medium (int bits)
     float4* sh_tr;      // Transmittance chain: accumulated Tr
     float4* mis_tr;     // IntersectTr chain: accumulated Tr; w = current medium (int bits)
+    // instanced scenes: set triangle (DevPrim index) of the hit at each queue
+    // position, for the extension / Transmittance hits and the IntersectTr hits
+    int* hsub;
+    int* hsub2;
 };
 
 struct WaveParams {
@@ -102,7 +106,7 @@ hipError_t launch_add_f64_to_f32(const double* acc, float* out, int64_t n, hipSt
 struct Timing;  // per-stage HIP event timing (rd_api.cpp)
 
 // Traversal over caller-provided ray arrays (rd_ray / rd_hit layouts).
-hipError_t launch_trace_rays(const DevScene& sc, const float4* rays, int n, float4* hits, uint8_t* occ,
+hipError_t launch_trace_rays(const DevScene& sc, const float4* rays, int n, float4* hits, uint8_t* occ, int* sub,
                              unsigned long long* counters, bool any, int* counts, hipStream_t s);
 
 // Wavefront path tracing stages.  A pass is a list of segments (BatchSeg);

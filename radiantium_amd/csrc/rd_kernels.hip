@@ -73,6 +73,7 @@ struct TraceArgs {
     int cursor_slot;          // work cursor, zero on entry
     int zero[3];              // count slots block 0 clears for the next stage (-1: none)
     const DevScene* scenes;   // batched queue modes: the ray's scene is scenes[ps.req[pid]]
+    int* sub;                 // closest modes of instanced scenes: set triangle (DevPrim index) or -1
 };
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
@@ -189,6 +190,172 @@ __device__ __forceinline__ void trav_finish(const DevScene& sc, const TraceArgs&
     }
 }
 
+// ---------------------------------------------------------------- two-level traversal
+// Instanced scenes (Primitive.cs:144-189): a scene leaf's InstancedPrimitive
+// runs its set's Bvh.Intersect on the ray transformed into model space
+// (Ray3F.Transform: origin and normalised direction through WorldToModel,
+// MaxT = the model distance to the image of ray.At(MaxT), MinT kept) before
+// the leaf's next primitive, exactly as the reference's nested call does.  The
+// set's DFS runs on the same LDS stack above the scene entries (sp_base); the
+// rest of the scene leaf waits in li/le.  A set hit becomes a scene hit at
+// T = |O - M2W * P_model| (P_model from the barycentrics, Triangle.cs:98-99)
+// and competes with the other scene primitives on T.
+struct TravI {
+    Trav t;            // current ray (world, or model inside a set), stack, world best
+    f3 wo, wd;         // the world ray while inside a set
+    float wtmax;
+    int sub;           // set triangle of the world best (DevPrim index), -1
+    int sp_base;       // stack depth at set entry, -1 at scene level
+    int li, le;        // scene leaf: next primitive record, end
+    int inst, inst_orig;
+    float bt, bu, bv;  // best set hit (model t)
+    int bord;          // its primitive record, -1 none
+};
+
+__device__ __forceinline__ void travi_begin(TravI& s, float4 ra, float4 rb) {
+    trav_begin(s.t, ra, rb);
+    s.sub = -1; s.sp_base = -1; s.li = s.le = 0; s.inst = s.inst_orig = 0;
+    s.bt = s.bu = s.bv = 0.f; s.bord = -1;
+    s.wo = s.t.o; s.wd = s.t.d; s.wtmax = s.t.tmax;
+}
+
+__device__ __forceinline__ void set_ray(Trav& t, f3 o, f3 d, float tmax) {
+    t.o = o; t.d = d; t.tmax = tmax;
+    t.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    t.nx = t.inv.x < 0; t.ny = t.inv.y < 0; t.nz = t.inv.z < 0;
+}
+
+// InstancedPrimitive.Intersect: Ray3F.Transform (Ray3F.cs:27-34) and the set's root.
+__device__ __forceinline__ void enter_set(const DevScene& sc, TravI& s, int inst, int orig) {
+    Trav& t = s.t;
+    const DevInstance& I = sc.insts[inst];
+    s.wo = t.o; s.wd = t.d; s.wtmax = t.tmax;
+    f3 mo = xform_point12(I.w2m, t.o);
+    f3 md = normalize(xform_vector12(I.w2m, t.d));
+    f3 mx = xform_point12(I.w2m, t.o + t.tmax * t.d);
+    set_ray(t, mo, md, length(mo - mx));
+    s.sp_base = t.sp;
+    s.inst = inst; s.inst_orig = orig;
+    s.bord = -1;
+    t.cur = I.root;
+}
+
+// Back at scene level: the set's best hit (if any) as a scene hit
+// (Primitive.cs:175-182 and the T comparison of Bvh.cs:196-203).
+__device__ __forceinline__ void leave_set(const DevScene& sc, const float4* __restrict__ prims, TravI& s) {
+    Trav& t = s.t;
+    if (s.bord >= 0) {
+        const float4* p = prims + 3 * s.bord;
+        float4 p0 = p[0], p1 = p[1], p2 = p[2];
+        float bx = 1 - (s.bu + s.bv), by = s.bu, bz = s.bv;
+        f3 pm = bx * mk(p0.x, p0.y, p0.z) + by * mk(p1.x, p1.y, p1.z) + bz * mk(p2.x, p2.y, p2.z);
+        f3 pw = xform_point12(sc.insts[s.inst].m2w, pm);
+        float T = length(s.wo - pw);
+        if (T < (t.best.prim >= 0 ? t.best.t : kFloatMax)) {
+            t.best.t = T; t.best.u = s.bu; t.best.v = s.bv; t.best.prim = s.inst_orig;
+            s.sub = __float_as_int(p0.w);
+            s.wtmax = T;
+        }
+    }
+    set_ray(t, s.wo, s.wd, s.wtmax);
+    s.sp_base = -1;
+}
+
+// The rest of a scene leaf from li: plain primitives are tested on the world
+// ray, an instance enters its set (false: keep stepping).  At the leaf's end
+// the scene stack is popped; true when the ray is finished.
+template <bool kAny, int kBlock>
+__device__ __forceinline__ bool scene_leaf(const DevScene& sc, const float4* __restrict__ prims, TravI& s, int* stk,
+                                           uint32_t& nt) {
+    Trav& t = s.t;
+    while (s.li < s.le) {
+        const float4* p = prims + 3 * s.li;
+        s.li++;
+        float4 p0 = p[0], p1 = p[1];
+        nt++;
+        int kind = __float_as_int(p1.w);
+        if (kind == kInst) {
+            enter_set(sc, s, __float_as_int(p0.x), __float_as_int(p0.w));
+            return false;
+        }
+        float tu = 0.f, tv = 0.f, tt = 0.f;
+        bool h;
+        if (kind == kTri) {
+            float4 p2 = p[2];
+            h = tri_test(mk(p0.x, p0.y, p0.z), mk(p1.x, p1.y, p1.z), mk(p2.x, p2.y, p2.z), t.o, t.d, t.tmin, t.tmax,
+                         &tu, &tv, &tt);
+        } else {
+            h = sphere_test(make_float4(p0.x, p0.y, p0.z, p1.x), t.o, t.d, t.tmin, t.tmax, &tt);
+        }
+        if (h) {
+            if (kAny) { t.best.prim = __float_as_int(p0.w); return true; }
+            if (tt < (t.best.prim >= 0 ? t.best.t : kFloatMax)) {
+                t.best.t = tt; t.best.u = tu; t.best.v = tv; t.best.prim = __float_as_int(p0.w);
+                s.sub = -1;
+                t.tmax = tt;
+            }
+        }
+    }
+    if (t.sp == 0) return true;
+    t.cur = stk[(--t.sp) * kBlock];
+    return false;
+}
+
+// One node of the two-level DFS; true when the ray is finished.
+template <bool kAny, int kBlock>
+__device__ __forceinline__ bool trav_step_inst(const DevScene& sc, const float4* __restrict__ nodes,
+                                               const float4* __restrict__ prims, TravI& s, int* stk, uint32_t& nv,
+                                               uint32_t& nt) {
+    Trav& t = s.t;
+    float4 lo = nodes[2 * t.cur], hi = nodes[2 * t.cur + 1];
+    nv++;
+    bool pop;
+    if (box_hit(lo, hi, t.o, t.inv, t.nx, t.ny, t.nz, t.tmax)) {
+        int off = __float_as_int(hi.z);
+        int ca = __float_as_int(hi.w);
+        int cnt = ca & 0xffff;
+        if (cnt > 0) {
+            if (s.sp_base < 0) {
+                s.li = off;
+                s.le = off + cnt;
+                return scene_leaf<kAny, kBlock>(sc, prims, s, stk, nt);
+            }
+            // a set leaf: its triangles on the model ray, best by model t
+            for (int i = 0; i < cnt; i++) {
+                const float4* p = prims + 3 * (off + i);
+                float4 p0 = p[0], p1 = p[1], p2 = p[2];
+                nt++;
+                float tu = 0.f, tv = 0.f, tt = 0.f;
+                if (tri_test(mk(p0.x, p0.y, p0.z), mk(p1.x, p1.y, p1.z), mk(p2.x, p2.y, p2.z), t.o, t.d, t.tmin,
+                             t.tmax, &tu, &tv, &tt)) {
+                    if (kAny) { t.best.prim = s.inst_orig; return true; }
+                    if (tt < (s.bord >= 0 ? s.bt : kFloatMax)) {
+                        s.bt = tt; s.bu = tu; s.bv = tv; s.bord = off + i;
+                        t.tmax = tt;
+                    }
+                }
+            }
+            pop = true;
+        } else {
+            int axis = ca >> 16;
+            bool neg = axis == 0 ? t.nx : (axis == 1 ? t.ny : t.nz);
+            if (neg) { stk[(t.sp++) * kBlock] = t.cur + 1; t.cur = off; }
+            else { stk[(t.sp++) * kBlock] = off; t.cur = t.cur + 1; }
+            pop = false;
+        }
+    } else {
+        pop = true;
+    }
+    if (!pop) return false;
+    if (s.sp_base >= 0 && t.sp == s.sp_base) {
+        leave_set(sc, prims, s);
+        return scene_leaf<kAny, kBlock>(sc, prims, s, stk, nt);
+    }
+    if (t.sp == 0) return true;
+    t.cur = stk[(--t.sp) * kBlock];
+    return false;
+}
+
 template <int kMode>
 __device__ __forceinline__ void trav_counters(const TraceArgs& a, uint32_t nrays, uint32_t nv, uint32_t nt) {
     constexpr bool kAny = (kMode == kModeRaysAny || kMode == kModeShadow);
@@ -233,6 +400,34 @@ __global__ __launch_bounds__(kBlock) void k_trace_straight(DevScene sc, TraceArg
             while (!trav_step<kAny, kBlock>(sc.nodes, sc.prims, t, stk, nv, nt)) {}
             trav_finish<kMode>(sc, a, r, t);
         }
+        nrays++;
+    }
+    trav_counters<kMode>(a, nrays, nv, nt);
+}
+
+// Instanced scenes: straight schedule over the two-level DFS; closest modes
+// also write the set triangle of each hit to a.sub.
+template <int kMode, int kStack, int kBlock, bool kBatch>
+__global__ __launch_bounds__(kBlock) void k_trace_inst(DevScene sc, TraceArgs a) {
+    constexpr bool kAny = (kMode == kModeRaysAny || kMode == kModeShadow);
+    __shared__ int lds[kStack * kBlock];
+    int* stk = lds + threadIdx.x;
+    const int n = trav_count(a, kMode <= kModeRaysAny);
+    if (blockIdx.x == 0 && threadIdx.x < 3 && a.zero[threadIdx.x] >= 0) a.counts[a.zero[threadIdx.x]] = 0;
+    uint32_t nv = 0, nt = 0, nrays = 0;
+    for (int r = blockIdx.x * kBlock + threadIdx.x; r < n; r += gridDim.x * kBlock) {
+        float4 ra, rb;
+        trav_load<kMode>(a, r, &ra, &rb);
+        TravI s;
+        travi_begin(s, ra, rb);
+        const DevScene& s2 = kBatch ? a.scenes[a.ps.req[a.q.pid[r]]] : sc;
+        const float4* nodes = s2.nodes;
+        const float4* prims = s2.prims;
+        if (s2.num_nodes > 0)
+            while (!trav_step_inst<kAny, kBlock>(s2, nodes, prims, s, stk, nv, nt)) {}
+        if (s.sp_base >= 0) set_ray(s.t, s.wo, s.wd, s.wtmax);   // any-hit ended inside a set
+        trav_finish<kMode>(s2, a, r, s.t);
+        if (!kAny && a.sub) a.sub[r] = s.t.best.prim >= 0 ? s.sub : -1;
         nrays++;
     }
     trav_counters<kMode>(a, nrays, nv, nt);
@@ -301,6 +496,14 @@ int get_trace_schedule(int cls) { return (cls >= 0 && cls < 3) ? g_schedule[cls]
 // launched (64 = the reference's Bvh.StackSize).
 template <int kMode, int kStack, int kBlock>
 static void launch_trace_variant(const DevScene& sc, const TraceArgs& a, int blocks, int schedule, hipStream_t s) {
+    if (sc.num_insts > 0) {
+        if (kMode >= kModeExt && a.scenes)
+            hipLaunchKernelGGL((k_trace_inst<kMode, kStack, kBlock, kMode >= kModeExt>), dim3(blocks), dim3(kBlock), 0,
+                               s, sc, a);
+        else
+            hipLaunchKernelGGL((k_trace_inst<kMode, kStack, kBlock, false>), dim3(blocks), dim3(kBlock), 0, s, sc, a);
+        return;
+    }
     if constexpr (kMode >= kModeExt) {
         if (a.scenes) {   // batched: straight schedule, per-ray scene
             hipLaunchKernelGGL((k_trace_straight<kMode, kStack, kBlock, true>), dim3(blocks), dim3(kBlock), 0, s, sc, a);
@@ -347,12 +550,12 @@ __global__ void k_zero_slot(int* c, int slot) {
     if (threadIdx.x == 0) c[slot] = 0;
 }
 
-hipError_t launch_trace_rays(const DevScene& sc, const float4* rays, int n, float4* hits, uint8_t* occ,
+hipError_t launch_trace_rays(const DevScene& sc, const float4* rays, int n, float4* hits, uint8_t* occ, int* sub,
                              unsigned long long* counters, bool any, int* counts, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_zero_slot, dim3(1), dim3(64), 0, s, counts, kCntWork + 3);
     TraceArgs a = trace_args(counts);
-    a.rays = rays; a.hits = hits; a.occ = occ; a.counters = counters; a.n_fixed = n;
+    a.rays = rays; a.hits = hits; a.occ = occ; a.counters = counters; a.n_fixed = n; a.sub = sub;
     a.cursor_slot = kCntWork + 3;
     int blocks = (n + 255) / 256;
     return any ? launch_trace_mode<kModeRaysAny>(sc, a, blocks, s) : launch_trace_mode<kModeRaysClosest>(sc, a, blocks, s);
@@ -587,6 +790,7 @@ hipError_t launch_trace_ext(const DevScene& sc, const DevScene* scenes, RayQueue
     TraceArgs a = trace_args(counts);
     a.q = q; a.ps = ps; a.hits = hits; a.cnt_slot = cnt_slot; a.cursor_slot = kCntWork + 0; a.scenes = scenes;
     a.zero[0] = za; a.zero[1] = zb; a.zero[2] = zc;
+    a.sub = ps.hsub;
     return launch_trace_mode<kModeExt>(sc, a, 0, s);
 }
 
@@ -611,6 +815,7 @@ hipError_t launch_trace_chain(const DevScene& sc, const DevScene* scenes, RayQue
                               bool mis, int* counts, int cnt_slot, int zero_slot, hipStream_t s) {
     TraceArgs a = trace_args(counts);
     a.q = q; a.ps = ps; a.hits = hits; a.cnt_slot = cnt_slot; a.cursor_slot = kCntWork + (mis ? 2 : 1);
+    a.sub = mis ? ps.hsub2 : ps.hsub;
     a.scenes = scenes;
     a.zero[0] = zero_slot;
     return mis ? launch_trace_mode<kModeChainMis>(sc, a, 0, s) : launch_trace_mode<kModeChainShadow>(sc, a, 0, s);
@@ -657,7 +862,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __r
             Inct it;
             if (hit) {
                 pr = sc.prim_info[prim];
-                it = make_inct(sc, pr, ro, rd, h.z, h.w, h.x);
+                it = make_inct_any(sc, pr, pr.kind == kInst ? ps.hsub[k] : -1, ro, rd, h.z, h.w, h.x);
             }
             auto inct_le = [&](f3 w) -> f3 {   // Intersection.Le -> DiffuseAreaLight.L
                 if (pr.light < 0) return mk(0.0f);
@@ -687,6 +892,9 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __r
                 nvert++;
                 MatCtx m = load_material(sc, pr.material, it.u, it.v);
                 f3 wo_l = to_local(it.sh, wo);
+                // EstimateDirect takes wo = inct.Wr (PathTracer.cs:314), -ray.D
+                // except on an instance, where it is mapped through the transforms
+                f3 wo_d = to_local(it.sh, it.wr);
                 float rho = 0.0f, r_nee = 0.0f, r_mis = 0.0f, r_cont = 0.0f;
                 if constexpr (kAdj) {
                     if (m.type == kMatRoughPlastic) rho = tex_sample(sc, sc.mats[pr.material].tex[1], it.u, it.v).x;
@@ -703,8 +911,8 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __r
                         // EstimateDirect, light-sampling half (PathTracer.cs:317-350)
                         if (ls.pdf > 0.0f && !is_black(ls.li)) {
                             f3 wi_l = to_local(it.sh, ls.wi);
-                            f3 fr = mat_fr(m, wo_l, wi_l);
-                            float spdf = mat_pdf(m, wo_l, wi_l);
+                            f3 fr = mat_fr(m, wo_d, wi_l);
+                            float spdf = mat_pdf(m, wo_d, wi_l);
                             if (!is_black(fr)) {
                                 f3 vec = ls.p - it.P;
                                 float dis = length(vec);
@@ -714,12 +922,12 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __r
                                     fr = fr * abs_cos_theta(to_local(it.sh, ls.wi));
                                     float w = power_heuristic(ls.pdf, spdf);
                                     nee = fr * ls.li * w / ls.pdf;
-                                    if constexpr (kAdj) r_nee = plastic_rho_ratio(m, rho, wo_l, wi_l);
+                                    if constexpr (kAdj) r_nee = plastic_rho_ratio(m, rho, wo_d, wi_l);
                                 }
                             }
                         }
                         // EstimateDirect, BSDF-sampling half (PathTracer.cs:351-397)
-                        BSample bs = mat_sample(m, wo_l, rng);
+                        BSample bs = mat_sample(m, wo_d, rng);
                         f3 fr2 = bs.fr * abs_cos_theta(bs.wi);
                         float sp2 = bs.pdf;
                         if (!is_black(fr2) && sp2 > 0) {
@@ -737,7 +945,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __r
                                 st3(ps.mis_f, pid, fr2, w2);
                                 ps.mis_li[pid] = make_float4(0.f, 0.f, 0.f, sp2);
                                 ps.sel_light[pid] = l;
-                                if constexpr (kAdj) r_mis = plastic_rho_ratio(m, rho, wo_l, bs.wi);
+                                if constexpr (kAdj) r_mis = plastic_rho_ratio(m, rho, wo_d, bs.wi);
                             }
                         }
                     } else {

@@ -52,7 +52,19 @@ struct Inct {
     float u, v;   // texture coordinates (Intersection.UV)
     float T;
     Frame sh;
+    f3 wr;        // Intersection.Wr: -ray.D, or its image for an instance
 };
+
+// Vector3.Transform / TransformNormal with a 4x3 row-major matrix (rows 1-3
+// linear, row 4 translation; row vectors).
+__device__ __forceinline__ f3 xform_point12(const float* M, f3 v) {
+    return mk(v.x * M[0] + v.y * M[3] + v.z * M[6] + M[9], v.x * M[1] + v.y * M[4] + v.z * M[7] + M[10],
+              v.x * M[2] + v.y * M[5] + v.z * M[8] + M[11]);
+}
+__device__ __forceinline__ f3 xform_vector12(const float* M, f3 v) {
+    return mk(v.x * M[0] + v.y * M[3] + v.z * M[6], v.x * M[1] + v.y * M[4] + v.z * M[7],
+              v.x * M[2] + v.y * M[5] + v.z * M[8]);
+}
 
 // Triangle.GetIntersection (Shapes/Triangle.cs:96-120) / Sphere.GetIntersection
 // (Shapes/Sphere.cs:34-48) for a hit (bu, bv, t) on original primitive `prim`.
@@ -86,7 +98,35 @@ __device__ __forceinline__ Inct make_inct(const DevScene& sc, const DevPrim& pr,
         it.u = phi / (2 * kPi);
         it.v = theta / kPi;
     }
+    it.wr = -rd;
     return it;
+}
+
+// InstancedPrimitive.Intersect's hit record (Primitive.cs:170-183) for a hit
+// on set triangle `sub` (a DevPrim index) with world distance T: the
+// triangle's GetIntersection in model space (ray direction normalised into
+// model space, Ray3F.Transform), then P and the shading normal to world
+// space, a new frame from that normal, Wr = normalize(M2W * (-d_model)).
+__device__ __forceinline__ Inct make_inct_inst(const DevScene& sc, const DevPrim& pr, int sub, f3 ro, f3 rd, float bu,
+                                               float bv, float T) {
+    const DevInstance& I = sc.insts[pr.index];
+    f3 md = normalize(xform_vector12(I.w2m, rd));
+    f3 mo = xform_point12(I.w2m, ro);
+    Inct m = make_inct(sc, sc.prim_info[sub], mo, md, bu, bv, T);
+    Inct w;
+    w.P = xform_point12(I.m2w, m.P);
+    w.sh = make_frame(normalize(xform_vector12(I.m2w, m.sh.Z)));
+    w.wr = normalize(xform_vector12(I.m2w, m.wr));
+    w.u = m.u;
+    w.v = m.v;
+    w.T = T;
+    return w;
+}
+
+__device__ __forceinline__ Inct make_inct_any(const DevScene& sc, const DevPrim& pr, int sub, f3 ro, f3 rd, float bu,
+                                              float bv, float t) {
+    if (pr.kind == kInst) return make_inct_inst(sc, pr, sub, ro, rd, bu, bv, t);
+    return make_inct(sc, pr, ro, rd, bu, bv, t);
 }
 
 // Geometric/shading normal only (for Le checks at a hit).

@@ -8,15 +8,20 @@
 // Primitives are stored in BVH leaf order, 48 bytes each (three float4):
 //   triangle: {a.xyz, orig}, {b.xyz, kind=0}, {c.xyz, 0}
 //   sphere:   {center.xyz, orig}, {radius, 0, 0, kind=1}, {0,0,0,0}
+//   instance: {instance (int bits), 0, 0, orig}, {0, 0, 0, kind=2}, {0,0,0,0}
 // so a leaf's primitives are contiguous and each test reads 36 (triangle) or
 // 16 (sphere) payload bytes.
+// Instanced scenes keep every set's BVH in the same two arrays after the
+// top-level tree (child and primitive offsets absolute); a set triangle's
+// "orig" is its DevPrim index (after the scene primitives), and its vertices
+// are in the set's model space.
 #pragma once
 #include <stdint.h>
 #include "rd_math.h"
 
 namespace rd {
 
-enum : int { kTri = 0, kSphere = 1 };
+enum : int { kTri = 0, kSphere = 1, kInst = 2 };
 enum : int {
     kMatDiffuse = 0, kMatRoughMetal = 1, kMatRoughGlass = 2, kMatRoughPlastic = 3,
     kMatPerfectGlass = 4, kMatPerfectMirror = 5, kMatDisney = 6
@@ -72,13 +77,23 @@ struct DevMedium {
 
 // Per original primitive (rd_prim_desc order): shading record.
 struct DevPrim {
-    int kind;      // kTri / kSphere
-    int index;     // triangle or sphere index
+    int kind;      // kTri / kSphere / kInst
+    int index;     // triangle, sphere or instance index
     int material;  // -1: no surface
     int light;     // -1: not emissive
     int med_in, med_out;
     float area;
     int flags;     // triangles: bit0 has normal, bit1 has uv
+};
+
+// InstancedTransform (Primitive.cs:94-108): 4x3 row-major (rows 1-3 = the
+// linear part, row 4 = translation, row vectors), WorldToModel inverted on
+// the host; root = node index of the set's BVH root.
+struct DevInstance {
+    float w2m[12];
+    float m2w[12];
+    int root;
+    int _pad[3];
 };
 
 struct DevCamera {
@@ -114,6 +129,8 @@ struct DevScene {
     const DevMedium* media;
     const DevEnv* envs;    // Scene.InfiniteLights order
     int num_envs;
+    const DevInstance* insts;
+    int num_insts;
     int global_medium;
     DevIntegrator integ;   // the scene's integrator (a batch may mix parameters)
     DevCamera cam;

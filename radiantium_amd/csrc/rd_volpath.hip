@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void k_shade_vol(DevScene sc0, const DevScene*
             Inct it;
             if (hit) {
                 pr = sc.prim_info[prim];
-                it = make_inct(sc, pr, ro, rd, h.z, h.w, h.x);
+                it = make_inct_any(sc, pr, pr.kind == kInst ? ps.hsub[k] : -1, ro, rd, h.z, h.w, h.x);
             }
             // starts the Transmittance chain toward the light point `to`
             auto start_shadow = [&](f3 from, f3 to, int medium) {
@@ -157,6 +157,7 @@ __global__ __launch_bounds__(256) void k_shade_vol(DevScene sc0, const DevScene*
                     nvert++;
                     MatCtx m = load_material(sc, pr.material, it.u, it.v);
                     f3 wo_l = to_local(it.sh, wo);
+                    f3 wo_d = to_local(it.sh, it.wr);   // EstimateDirectSurface: wo = inct.Wr (:156)
                     if (sc.num_lights > 0) {
                         // EstimateDirectSurface (:151-219)
                         int l = rng.next_int(sc.num_lights);
@@ -165,8 +166,8 @@ __global__ __launch_bounds__(256) void k_shade_vol(DevScene sc0, const DevScene*
                         LSample ls = light_sample(sc, Lt, it.P, rng);
                         if (ls.pdf > 0.0f && !is_black(ls.li)) {
                             f3 wi_l = to_local(it.sh, ls.wi);
-                            f3 fr = mat_fr(m, wo_l, wi_l);
-                            float spdf = mat_pdf(m, wo_l, wi_l);
+                            f3 fr = mat_fr(m, wo_d, wi_l);
+                            float spdf = mat_pdf(m, wo_d, wi_l);
                             if (!is_black(fr)) {
                                 start_shadow(it.P, ls.p, env);
                                 nflags |= kFlagShadowIssued;
@@ -174,7 +175,7 @@ __global__ __launch_bounds__(256) void k_shade_vol(DevScene sc0, const DevScene*
                                 st3(ps.nee_li, pid, ls.li, ls.pdf);
                             }
                         }
-                        BSample bs = mat_sample(m, wo_l, rng);
+                        BSample bs = mat_sample(m, wo_d, rng);
                         f3 fr2 = bs.fr * abs_cos_theta(bs.wi);
                         float sp2 = bs.pdf;
                         if (!is_black(fr2) && sp2 > 0) {
@@ -303,7 +304,7 @@ __global__ __launch_bounds__(256) void k_chain_advance(DevScene sc0, const DevSc
                 } else {
                     if (env >= 0) tr = tr * medium_tr(sc.media[env], tmin, tmax);
                     if (any) {
-                        Inct it = make_inct(sc, pr, o, d, h.z, h.w, h.x);
+                        Inct it = make_inct_any(sc, pr, pr.kind == kInst ? ps.hsub[k] : -1, o, d, h.z, h.w, h.x);
                         env = get_medium(pr, it.sh.Z, d);
                         f3 to = mk(to4.x, to4.y, to4.z);
                         float dis = length(to - it.P);
@@ -326,7 +327,7 @@ __global__ __launch_bounds__(256) void k_chain_advance(DevScene sc0, const DevSc
                     ps.mis_li[pid] = make_float4(li.x, li.y, li.z, prev.w);
                 } else {
                     if (env >= 0) tr = tr * medium_tr(sc.media[env], tmin, length(o + h.x * d));
-                    Inct it = make_inct(sc, pr, o, d, h.z, h.w, h.x);
+                    Inct it = make_inct_any(sc, pr, pr.kind == kInst ? ps.hsub2[k] : -1, o, d, h.z, h.w, h.x);
                     if (pr.material >= 0) {
                         f3 li = mk(0.0f);
                         int sel = ps.sel_light[pid];

@@ -21,6 +21,7 @@ EXPORTS = [
     "rd_get_render_stats", "rd_bvh_build_host", "rd_device_alloc", "rd_device_free", "rd_memcpy_h2d", "rd_memcpy_d2h",
     "rd_synchronize", "rd_kernel_timing", "rd_reset_kernel_timing", "rd_render_batch", "rd_render_batch_device",
     "rd_adjoint", "rd_adjoint_device", "rd_scene_set_texture", "rd_scene_set_texture_device",
+    "rd_trace_closest_sub",
 ]
 
 
@@ -64,6 +65,7 @@ def lib():
         L.rd_scene_get_info.argtypes = [vp, C.POINTER(abi.rd_scene_info)]
         L.rd_scene_export_bvh.argtypes = [vp, vp, i32, vp, i32]
         L.rd_trace_closest.argtypes = [vp, vp, vp, i32, vp]
+        L.rd_trace_closest_sub.argtypes = [vp, vp, vp, i32, vp, vp]
         L.rd_trace_any.argtypes = [vp, vp, vp, i32, vp]
         L.rd_trace_closest_device.argtypes = [vp, vp, vp, i32, vp, vp, vp]
         L.rd_trace_any_device.argtypes = [vp, vp, vp, i32, vp, vp, vp]
@@ -235,6 +237,15 @@ class Scene:
         hits = np.zeros(rays.shape[0], abi.hit_dtype())
         check(lib().rd_trace_closest(self.ctx.h, self.h, _ptr(rays), rays.shape[0], _ptr(hits)))
         return hits
+
+    def trace_closest_sub(self, rays):
+        """rd_trace_closest_sub: (hits, sub) with sub = the hit triangle's index
+        within its instance set, -1 for hits not on an instance."""
+        rays = np.ascontiguousarray(rays, abi.ray_dtype())
+        hits = np.zeros(rays.shape[0], abi.hit_dtype())
+        sub = np.full(rays.shape[0], -1, np.int32)
+        check(lib().rd_trace_closest_sub(self.ctx.h, self.h, _ptr(rays), rays.shape[0], _ptr(hits), _ptr(sub)))
+        return hits, sub
 
     def trace_any(self, rays):
         rays = np.ascontiguousarray(rays, abi.ray_dtype())

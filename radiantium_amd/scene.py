@@ -78,6 +78,8 @@ class SceneDesc:
         self.integrator = dict(type="path", max_depth=-1, min_depth=3, rr_threshold=0.99,
                                method="Mis", strategy="Uniform")
         self._ntri = 0
+        self.instance_sets = []   # (first_triangle, num_triangles, bvh_max_prim)
+        self.instances = []       # (set, to_world 4x4)
         self._keep = []
 
     # ---- textures / materials -------------------------------------------
@@ -151,9 +153,25 @@ class SceneDesc:
         """A scene entity with one model: one primitive per triangle, and with
         `le` one DiffuseAreaLight per triangle (RendererBuilder.cs:976-992).
         Returns the index of its first primitive."""
+        first_prim = self.num_prims()
+        t0, nt = self._add_triangles(positions, indices, normals, uvs)
+        kinds = np.zeros(nt, np.int32)
+        tris = np.arange(t0, t0 + nt, dtype=np.int32)
+        if le is not None:
+            light0 = len(self.lights)
+            for i in range(nt):
+                self.lights.append(dict(type=abi.LIGHT_AREA, prim=first_prim + i, le=tuple(le), tex=-1, to_world=None))
+            lights = np.arange(light0, light0 + nt, dtype=np.int32)
+        else:
+            lights = np.full(nt, -1, np.int32)
+        block = np.stack([kinds, tris, np.full(nt, material, np.int32), lights,
+                          np.full(nt, medium_inside, np.int32), np.full(nt, medium_outside, np.int32)], axis=1)
+        self.prims.append(block)
+        return first_prim
+
+    def _add_triangles(self, positions, indices, normals, uvs):
         pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
         idx = np.ascontiguousarray(indices, dtype=np.int32).reshape(-1, 3)
-        nv = pos.shape[0]
         flags = 0
         self._positions.append(pos)
         if normals is not None:
@@ -170,24 +188,32 @@ class SceneDesc:
             self._uvs.append(None)
         self._indices.append(idx + self._nverts)
         self._tflags.append(np.full(idx.shape[0], flags, dtype=np.uint8))
-        self._nverts += nv
-        first_prim = self.num_prims()
+        self._nverts += pos.shape[0]
         t0 = self._ntri
-        nt = idx.shape[0]
-        self._ntri += nt
-        kinds = np.zeros(nt, np.int32)
-        tris = np.arange(t0, t0 + nt, dtype=np.int32)
-        if le is not None:
-            light0 = len(self.lights)
-            for i in range(nt):
-                self.lights.append(dict(type=abi.LIGHT_AREA, prim=first_prim + i, le=tuple(le), tex=-1, to_world=None))
-            lights = np.arange(light0, light0 + nt, dtype=np.int32)
-        else:
-            lights = np.full(nt, -1, np.int32)
-        block = np.stack([kinds, tris, np.full(nt, material, np.int32), lights,
-                          np.full(nt, medium_inside, np.int32), np.full(nt, medium_outside, np.int32)], axis=1)
-        self.prims.append(block)
-        return first_prim
+        self._ntri += idx.shape[0]
+        return t0, idx.shape[0]
+
+    def add_instance_set(self, positions, indices, normals=None, uvs=None, bvh_max_prim=1):
+        """An instanced mesh set (RendererBuilder's InstancedPrimitivesEntry,
+        Config/RendererBuilder.cs:895-921): triangles already in the set's
+        model space (its local transform applied, as TriangleMesh does) under
+        the set's own BVH ("accel" max_prim; the reference's default is 1).
+        The triangles are not scene primitives.  Returns the set index."""
+        t0, nt = self._add_triangles(positions, indices, normals, uvs)
+        self.instance_sets.append((t0, nt, int(bvh_max_prim)))
+        return len(self.instance_sets) - 1
+
+    def add_instance(self, set_index, to_world, material=-1, medium_inside=-1, medium_outside=-1):
+        """An entity instancing a set: one InstancedPrimitive (Primitive.cs:144-189)
+        with the entity's ModelToWorld (row-major 4x4, row vectors), material
+        and media.  Returns its primitive index."""
+        if not 0 <= set_index < len(self.instance_sets):
+            raise ValueError("instance set out of range")
+        k = len(self.instances)
+        self.instances.append((int(set_index), np.asarray(to_world, np.float32).reshape(4, 4)))
+        p = self.num_prims()
+        self.prims.append(np.array([[abi.PRIM_INSTANCE, k, material, -1, medium_inside, medium_outside]], np.int32))
+        return p
 
     def add_sphere(self, center, radius, material=-1, le=None, medium_inside=-1, medium_outside=-1):
         k = len(self.spheres)
@@ -330,6 +356,21 @@ class SceneDesc:
         d.media = C.cast(earr, C.POINTER(abi.rd_medium_desc))
         d.global_medium = self.global_medium
         d.bvh_max_prim = self.bvh_max_prim
+
+        sarr = (abi.rd_instance_set_desc * max(len(self.instance_sets), 1))()
+        for i, (t0, nt, mp) in enumerate(self.instance_sets):
+            sarr[i].first_triangle, sarr[i].num_triangles, sarr[i].bvh_max_prim = t0, nt, mp
+        keep.append(sarr)
+        d.num_instance_sets = len(self.instance_sets)
+        d.instance_sets = C.cast(sarr, C.POINTER(abi.rd_instance_set_desc))
+        iarr = (abi.rd_instance_desc * max(len(self.instances), 1))()
+        for i, (st, tw) in enumerate(self.instances):
+            iarr[i].set = st
+            for j in range(16):
+                iarr[i].to_world[j] = float(tw.reshape(16)[j])
+        keep.append(iarr)
+        d.num_instances = len(self.instances)
+        d.instances = C.cast(iarr, C.POINTER(abi.rd_instance_desc))
 
         c = self.camera
         cam = d.camera

@@ -326,6 +326,119 @@ def diff_cornell(width=64, height=64, tex_res=512, seed=0, init=None, max_depth=
     return s, tex
 
 
+# ---------------------------------------------------------------- instancing
+def superquadric(e1, e2, nu=64, nv=32, radius=(1.0, 1.0, 1.0)):
+    """Superquadric surface (x, y, z) = r * (c(v)^e1 c(u)^e2, s(v)^e1, c(v)^e1 s(u)^e2)
+    with signed powers c^e = sign(cos) |cos|^e, u in [0, 2 pi) (nu segments),
+    v in [-pi/2, pi/2] (nv rings, poles as single vertices).  Triangles wind
+    outward; UVs (u / 2 pi, v / pi + 1/2).  Returns (positions, indices, uvs)."""
+    def sp(x, e):
+        return np.sign(x) * np.abs(x) ** e
+    us = np.arange(nu) * (2 * np.pi / nu)
+    vs = -np.pi / 2 + np.arange(1, nv) * (np.pi / nv)
+    U, V = np.meshgrid(us, vs, indexing="xy")              # rings x segments
+    cv, sv, cu, su = sp(np.cos(V), e1), sp(np.sin(V), e1), sp(np.cos(U), e2), sp(np.sin(U), e2)
+    ring = np.stack([radius[0] * cv * cu, radius[1] * sv, radius[2] * cv * su], -1).reshape(-1, 3)
+    pos = np.concatenate([[[0.0, -radius[1], 0.0]], ring, [[0.0, radius[1], 0.0]]]).astype(np.float32)
+    uv_r = np.stack([U / (2 * np.pi), V / np.pi + 0.5], -1).reshape(-1, 2)
+    uvs = np.concatenate([[[0.5, 0.0]], uv_r, [[0.5, 1.0]]]).astype(np.float32)
+    nr = nv - 1
+    top = 1 + nr * nu
+    idx = []
+    for j in range(nu):
+        jn = (j + 1) % nu
+        idx.append((0, 1 + jn, 1 + j))                      # south cap
+        idx.append((top, 1 + (nr - 1) * nu + j, 1 + (nr - 1) * nu + jn))   # north cap
+    for r in range(nr - 1):
+        a0, a1 = 1 + r * nu, 1 + (r + 1) * nu
+        for j in range(nu):
+            jn = (j + 1) % nu
+            idx.append((a0 + j, a0 + jn, a1 + jn))
+            idx.append((a0 + j, a1 + jn, a1 + j))
+    idx = np.array(idx, np.int32)[:, [0, 2, 1]]
+    return pos, np.ascontiguousarray(idx), uvs
+
+
+def _rand_rotation(rng):
+    """Uniform random rotation (unit quaternion) as a row-vector 3x3 matrix."""
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                  [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                  [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+    return R.T
+
+
+def instance_matrix(rot, scale, translate):
+    """Entity ModelToWorld = Scale * Rotation * Translation in the reference's
+    row-vector convention (RendererBuilder.cs:925-938): rows 1-3 = scale *
+    rotation rows, row 4 = translation."""
+    M = np.eye(4)
+    M[:3, :3] = np.asarray(scale, np.float64).reshape(-1, 1) * np.asarray(rot, np.float64) \
+        if np.ndim(scale) else scale * np.asarray(rot, np.float64)
+    M[3, :3] = translate
+    return M.astype(np.float32)
+
+
+def instanced_scene(width=64, height=64, n_instances=48, n_sets=3, nu=24, nv=12, seed=0, max_depth=16,
+                    method="Mis", env=(64, 32), area_light=True, extent=2.5, set_max_prim=4, ground=True,
+                    nonuniform=False, integrator="path"):
+    """Instanced scene: `n_sets` distinct superquadric meshes (exponents drawn
+    from [0.5, 2.5]) as instance sets, `n_instances` InstancedPrimitives with
+    seeded random rotations, scales in [0.5, 2] (per axis when `nonuniform`)
+    and positions in a cube of half-size `extent`, materials cycling Disney /
+    rough plastic / rough metal; optional ground quad and area light at the
+    scene level, an HDR sky (log-normal texels plus a sun disk, rotated).  The
+    set BVHs use max_prim `set_max_prim`, the scene BVH max_prim 1."""
+    from .scene import Texture
+    rng = np.random.default_rng(seed)
+    s = SceneDesc()
+    mats = [s.add_material("disney", base_color=(0.8, 0.5, 0.3), roughness=0.35, metallic=0.2, clearcoat=0.4),
+            s.add_material("rough_plastic", r=(0.25, 0.45, 0.75), roughness=0.3, dist="GGX"),
+            s.add_material("rough_metal", roughness=0.25)]
+    sets = []
+    for k in range(n_sets):
+        e1, e2 = rng.uniform(0.5, 2.5, 2)
+        pos, idx, uvs = superquadric(e1, e2, nu, nv)
+        sets.append(s.add_instance_set(pos, idx, uvs=uvs, bvh_max_prim=set_max_prim))
+    if ground:
+        g = s.add_material("diffuse", kd=(0.45, 0.45, 0.45))
+        W = 4 * extent
+        s.add_mesh(*_quad((-W, -extent - 0.6, -W), (-W, -extent - 0.6, W), (W, -extent - 0.6, W),
+                          (W, -extent - 0.6, -W)), material=g)
+    for i in range(n_instances):
+        scale = rng.uniform(0.5, 2.0, 3) if nonuniform else rng.uniform(0.5, 2.0)
+        M = instance_matrix(_rand_rotation(rng), scale, rng.uniform(-extent, extent, 3))
+        s.add_instance(sets[i % n_sets], M, material=mats[i % 3])
+    if area_light:
+        black = s.add_material("diffuse", kd=(0.0, 0.0, 0.0))
+        q, y = 0.6 * extent, 2.5 * extent
+        s.add_mesh(*_quad((-q, y, -q), (q, y, -q), (q, y, q), (-q, y, q)), material=black, le=(8.0, 7.5, 7.0))
+    if env is not None:
+        sky = hdr_sky(env[0], env[1], seed=seed + 1)
+        c, sn = math.cos(math.radians(25.0)), math.sin(math.radians(25.0))
+        tw = np.array([[c, 0, -sn, 0], [0, 1, 0, 0], [sn, 0, c, 0], [0, 0, 0, 1]], np.float32)
+        s.add_infinite_light(Texture(image=sky, filter="Nearest"), to_world=tw)
+    d = 3.2 * extent
+    s.set_camera(fov=45.0, width=width, height=height, origin=(0.35 * d, 0.3 * d, d), target=(0.0, 0.0, 0.0),
+                 up=(0.0, 1.0, 0.0))
+    s.set_integrator(type=integrator, max_depth=max_depth, method=method)
+    s.bvh_max_prim = 1
+    return s
+
+
+def instanced_workload(width=1024, height=1024, n_instances=1024, n_sets=8, seed=0, max_depth=16,
+                       env=(2048, 1024), nu=224, nv=224):
+    """The instanced benchmark workload: 1024 instances of 8 superquadrics
+    (224 x 224 grid, 99,904 triangles each; exponents in [0.5, 2.5]) with
+    seeded random rotations and scales in [0.5, 2] spread through a cube,
+    Disney / plastic / rough-metal materials, under a 2048 x 1024 log-normal
+    (mu 0, sigma 1.5) HDR sky with a sun disk; 1024 x 1024, max depth 16."""
+    return instanced_scene(width, height, n_instances=n_instances, n_sets=n_sets, nu=nu, nv=nv, seed=seed,
+                           max_depth=max_depth, env=env, area_light=False, extent=9.0, ground=False)
+
+
 # ---------------------------------------------------------------- icosphere
 def icosphere(level):
     """Unit icosphere: 20*4^level faces, midpoint subdivision with shared
