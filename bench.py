@@ -227,6 +227,58 @@ def adjoint_ratio(ctx, args):
                         f"(texture seed {args.adj_seed}), start 0.5"}
 
 
+def instanced_throughput(ctx, args):
+    """Instanced workload: 1024 InstancedPrimitives of 8 superquadrics
+    (99,904 triangles each, exponents in [0.5, 2.5]; 102 M instanced
+    triangles) with random rotations and scales in [0.5, 2], Disney / plastic
+    / rough-metal materials, a 2048 x 1024 log-normal HDR sky with a sun disk
+    importance-sampled through Distribution2D; path Mis, 1024 x 1024,
+    128 spp, max depth 16.  One frame, timed like the main workload, and the
+    oracle on a centred crop of the same frame."""
+    from radiantium_amd import native, scenes
+    from oracle import cpuref
+    t0 = time.perf_counter()
+    desc = scenes.instanced_workload()
+    scene = native.Scene(ctx, desc)
+    info = scene.info()
+    log(f"[bench] instanced scene: {len(desc.instances)} instances, built and uploaded in "
+        f"{time.perf_counter() - t0:.1f}s")
+    W, H, spp = desc.camera["width"], desc.camera["height"], args.inst_spp
+    d_rgb = ctx.alloc(W * H * 12)
+    scene.render_device(1, d_rgb, seed=99)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    scene.render_device(spp, d_rgb, seed=1)
+    ctx.synchronize()
+    secs = time.perf_counter() - t0
+    st = ctx.render_stats()
+    ctx.free(d_rgb)
+    rays = st["rays_closest"] + st["rays_any"]
+    rate = rays / secs / 1e6
+    log(f"[bench] instanced: {secs * 1000:.0f} ms, {rays / 1e6:.0f} M rays, {rate:.1f} Mrays/s")
+    cpu = None
+    if not args.no_cpu_baseline:
+        if not os.path.exists(cpuref.LIB_PATH):
+            cpuref.build()
+        threads = max(1, min(16, os.cpu_count() or 1))
+        o = cpuref.OracleScene(desc)
+        cw = ch = 64
+        crop = ((W - cw) // 2, (H - ch) // 2, (W + cw) // 2, (H + ch) // 2)
+        _, _, cst, csecs = o.render(spp, seed=1, crop=crop, threads=threads, with_sum2=False)
+        crays = cst["rays_closest"] + cst["rays_any"]
+        cpu = {"value": round(crays / csecs / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"a centred {cw}x{ch} crop at {spp} spp: {crays} BVH queries in {csecs:.1f} s"}
+        log(f"[cpu] instanced crop: {cpu['value']} Mrays/s")
+    c = st["class_rays"]
+    return {"mrays_s": round(rate, 1), "ms": round(secs * 1000, 1), "rays": rays,
+            "nodes_per_ray": [round(n / max(r, 1), 2) for n, r in zip(st["class_nodes"], c)],
+            "prims_per_ray": [round(n / max(r, 1), 2) for n, r in zip(st["class_prims"], c)],
+            "stack_depth": info["max_depth"], "cpu_baseline": cpu,
+            "speedup_vs_cpu": round(rate / cpu["value"], 1) if cpu else None,
+            "workload": f"1024 instances of 8 superquadrics (99,904 tris each), 2048x1024 HDR sky + sun; "
+                        f"path Mis, {W}x{H}, {spp} spp, max depth 16"}
+
+
 def copy_calibration():
     """A plain device copy of known size, profiled beside the path kernels so
     FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
@@ -256,6 +308,8 @@ def main():
     ap.add_argument("--batch-seed", type=int, default=0, help="seed of the batched workload (16 vol_path scenes)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-sequential measurement")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/primal measurement")
+    ap.add_argument("--no-instanced", action="store_true", help="skip the instanced-workload measurement")
+    ap.add_argument("--inst-spp", type=int, default=128, help="spp of the instanced workload frame")
     ap.add_argument("--adj-seed", type=int, default=9, help="texture seed of the adjoint target")
     ap.add_argument("--adj-reps", type=int, default=3, help="timed primal/adjoint pairs")
     ap.add_argument("--adj-steps", type=int, default=6, help="optimisation steps through the autograd Function")
@@ -336,6 +390,10 @@ def main():
     if not args.no_adjoint:
         adjoint = adjoint_ratio(ctx, args)
 
+    instanced = None
+    if not args.no_instanced:
+        instanced = instanced_throughput(ctx, args)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(desc, args, value)
@@ -350,6 +408,7 @@ def main():
                    "width": args.width, "height": args.height, "spp": args.spp, "max_depth": args.max_depth,
                    "triangles": ntri, "parallelism": "1 GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "adjoint": adjoint,
+        "instanced": instanced,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
