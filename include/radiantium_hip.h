@@ -182,6 +182,9 @@ typedef struct rd_scene_desc {
     const uint8_t* tri_flags;    /* per triangle: bit0 has normal, bit1 has uv; NULL = all 0 */
     int32_t num_spheres;
     const float* spheres;        /* [num_spheres][4] center.xyz, radius      */
+    const float* sphere_w2m;     /* [num_spheres][9] linear part of the sphere's
+                                    WorldToModel (Sphere.cs:19-31), used for its
+                                    UV (Sphere.cs:40); NULL = identity       */
     int32_t num_prims;
     const rd_prim_desc* prims;
     int32_t num_materials;
@@ -282,6 +285,27 @@ int rd_scene_export_bvh(const rd_scene* scene, void* nodes32, int32_t max_nodes,
  * records (capacity max_nodes; 2n-1 always suffices), prim_order n indices. */
 int rd_bvh_build_host(const float* bmin, const float* bmax, int32_t n, int32_t max_prim, void* nodes32,
                       int32_t max_nodes, int32_t* prim_order, int32_t* num_nodes, int32_t* max_depth);
+
+/* ---- scene configs ------------------------------------------------------- */
+/* The reference's scene config (JSON, as Radiantium.Cli/Program.cs:25-140
+ * reads it) with its OBJ models and PNG images, built into a scene
+ * description the way RendererBuilder.Build builds the scene
+ * (Config/RendererBuilder.cs:867-1010).  search_dir: base of relative asset
+ * locations (NULL = the config's directory; RendererBuilder.GetPath).
+ * params (may be NULL) receives the renderer block: spp (default 256).
+ * Integrators other than "path" / "vol_path" and image formats other than
+ * PNG give RD_ERR_UNSUPPORTED. */
+typedef struct rd_json_scene rd_json_scene;
+int rd_json_load(const char* path, const char* search_dir, rd_json_scene** out, rd_render_params* params);
+const rd_scene_desc* rd_json_scene_desc(const rd_json_scene* scene);
+void rd_json_scene_free(rd_json_scene* scene);
+/* rd_json_load + rd_scene_create: the scene a C# WavefrontRenderer builds
+ * from the same config path the CLI receives (Program.cs:27).  NULL on
+ * failure (rd_last_error). */
+rd_scene* rd_scene_load_json(rd_ctx* ctx, const char* path, const char* search_dir, rd_render_params* params);
+/* Spectrum.TryGetMatel (Spectrum.cs:38-41): the RGB eta and k of a named
+ * conductor ("Cu", "Au", "Ag", ...), for rd_material_desc.eta / .k. */
+int rd_metal_eta_k(const char* name, float* eta, float* k);
 
 /* ---- traversal oracle hooks: Scene.Intersect / IsOccluded ---------------- */
 /* Bvh.Intersect(ray, out inct) (Bvh.cs:174-232) for n rays (host buffers). */

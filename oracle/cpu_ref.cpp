@@ -324,6 +324,7 @@ struct cr_scene {
     std::vector<int32_t> idx;
     std::vector<uint8_t> tflags;
     std::vector<float> sph;
+    std::vector<float> sph_w2m;         // 9 per sphere, empty = identity
     std::vector<rd_prim_desc> prims;
     std::vector<float> prim_area;
     std::vector<Tex> tex;
@@ -447,8 +448,9 @@ static inline bool sphere_hit(const cr_scene& s, int k, const Ray& r, SurfPt* sp
     return hit;
 }
 
-// Sphere.GetIntersection (Sphere.cs:34-48); sphere shapes carry an identity
-// WorldToModel here (the build bakes transforms into the centre).
+// Sphere.GetIntersection (Sphere.cs:34-48): the centre is already in world
+// space (the builder moves it, RendererBuilder.cs:49-54); WorldToModel only
+// rotates the normal for the UV.
 static Inct sphere_inct(const cr_scene& s, int k, const Ray& r, const SurfPt& sp, int prim) {
     const float* S = &s.sph[4 * k];
     V3 c(S[0], S[1], S[2]);
@@ -457,7 +459,12 @@ static Inct sphere_inct(const cr_scene& s, int k, const Ray& r, const SurfPt& sp
     V3 n = normalize(r.at(t) - c);
     it.P = n * S[3] + c;
     it.sh = Frame(n);
-    V3 local = n;
+    V3 local = n;                                            // TransformNormal(n, WorldToModel) (Sphere.cs:40)
+    if (!s.sph_w2m.empty()) {
+        const float* M = &s.sph_w2m[9 * k];
+        local = V3(n.x * M[0] + n.y * M[3] + n.z * M[6], n.x * M[1] + n.y * M[4] + n.z * M[7],
+                   n.x * M[2] + n.y * M[5] + n.z * M[8]);
+    }
     float theta = std::acos(-local.y);
     float phi = std::atan2(-local.z, local.x) + PI_F;
     it.UV = V2(phi / (2 * PI_F), theta / PI_F);
@@ -2466,6 +2473,7 @@ cr_scene* cr_scene_create(const rd_scene_desc* d) {
         }
     }
     if (d->num_spheres) s->sph.assign(d->spheres, d->spheres + 4 * (size_t)d->num_spheres);
+    if (d->num_spheres && d->sphere_w2m) s->sph_w2m.assign(d->sphere_w2m, d->sphere_w2m + 9 * (size_t)d->num_spheres);
     s->prims.assign(d->prims, d->prims + d->num_prims);
     s->mats.assign(d->materials, d->materials + d->num_materials);
     s->lights.assign(d->lights, d->lights + d->num_lights);

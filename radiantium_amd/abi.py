@@ -115,7 +115,7 @@ class rd_scene_desc(C.Structure):
         ("normals", C.POINTER(C.c_float)), ("uvs", C.POINTER(C.c_float)),
         ("num_triangles", C.c_int32), ("indices", C.POINTER(C.c_int32)),
         ("tri_flags", C.POINTER(C.c_uint8)),
-        ("num_spheres", C.c_int32), ("spheres", C.POINTER(C.c_float)),
+        ("num_spheres", C.c_int32), ("spheres", C.POINTER(C.c_float)), ("sphere_w2m", C.POINTER(C.c_float)),
         ("num_prims", C.c_int32), ("prims", C.POINTER(rd_prim_desc)),
         ("num_materials", C.c_int32), ("materials", C.POINTER(rd_material_desc)),
         ("num_textures", C.c_int32), ("textures", C.POINTER(rd_texture_desc)),

@@ -20,7 +20,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unus
           "-Wno-unused-variable"]
 DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-fhip-fp32-correctly-rounded-divide-sqrt",
           "-fno-gpu-flush-denormals-to-zero", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-HOST_SOURCES = ["rd_api.cpp", "rd_camera.cpp", "rd_bvh_build.cpp", "rd_envmap.cpp"]
+HOST_SOURCES = ["rd_api.cpp", "rd_camera.cpp", "rd_bvh_build.cpp", "rd_envmap.cpp", "rd_json.cpp"]
 DEVICE_SOURCES = ["rd_kernels.hip", "rd_volpath.hip"]
 
 
@@ -56,7 +56,7 @@ def build(verbose=False):
     if failed:
         raise RuntimeError("hipcc compilation failed")
     tmp = LIB + ".tmp"
-    subprocess.check_call([cc, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs)
+    subprocess.check_call([cc, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lz"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -26,6 +26,9 @@ static int fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
+namespace rd {
+void set_last_error(const std::string& msg) { g_err = msg; }
+}
 #define HIP_TRY(expr)                                                                          \
     do {                                                                                       \
         hipError_t e_ = (expr);                                                                \
@@ -746,6 +749,7 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     if (!rc && d->normals) rc = upload(s, d->normals, 3 * (size_t)d->num_vertices, &ds.nrm);
     if (!rc && d->uvs) rc = upload(s, d->uvs, 2 * (size_t)d->num_vertices, &ds.uv);
     if (!rc) rc = upload(s, spheres.data(), spheres.size(), &ds.spheres);
+    if (!rc && d->sphere_w2m && d->num_spheres > 0) rc = upload(s, d->sphere_w2m, 9 * (size_t)d->num_spheres, &ds.sphere_w2m);
     if (!rc) rc = upload(s, mats.data(), mats.size(), &ds.mats);
     if (!rc) rc = upload(s, texs.data(), texs.size(), &ds.texs);
     if (!rc) rc = upload(s, texels.data(), texels.size(), &ds.texels);

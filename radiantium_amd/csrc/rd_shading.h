@@ -93,8 +93,15 @@ __device__ __forceinline__ Inct make_inct(const DevScene& sc, const DevPrim& pr,
         f3 n = normalize((ro + t * rd) - c);
         it.P = n * s.w + c;
         it.sh = make_frame(n);
-        float theta = acosf(-n.y);
-        float phi = atan2f(-n.z, n.x) + kPi;
+        // local = TransformNormal(n, WorldToModel) (Sphere.cs:40)
+        f3 local = n;
+        if (sc.sphere_w2m) {
+            const float* M = sc.sphere_w2m + 9 * pr.index;
+            local = mk(n.x * M[0] + n.y * M[3] + n.z * M[6], n.x * M[1] + n.y * M[4] + n.z * M[7],
+                       n.x * M[2] + n.y * M[5] + n.z * M[8]);
+        }
+        float theta = acosf(-local.y);
+        float phi = atan2f(-local.z, local.x) + kPi;
         it.u = phi / (2 * kPi);
         it.v = theta / kPi;
     }

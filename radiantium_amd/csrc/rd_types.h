@@ -122,6 +122,7 @@ struct DevScene {
     const float* nrm;
     const float* uv;
     const float4* spheres;
+    const float* sphere_w2m;   // 9 per sphere (WorldToModel linear part), null = identity
     const DevMaterial* mats;
     const DevTexture* texs;
     const float* texels;

@@ -21,7 +21,8 @@ EXPORTS = [
     "rd_get_render_stats", "rd_bvh_build_host", "rd_device_alloc", "rd_device_free", "rd_memcpy_h2d", "rd_memcpy_d2h",
     "rd_synchronize", "rd_kernel_timing", "rd_reset_kernel_timing", "rd_render_batch", "rd_render_batch_device",
     "rd_adjoint", "rd_adjoint_device", "rd_scene_set_texture", "rd_scene_set_texture_device",
-    "rd_trace_closest_sub",
+    "rd_trace_closest_sub", "rd_json_load", "rd_json_scene_desc", "rd_json_scene_free", "rd_scene_load_json",
+    "rd_metal_eta_k",
 ]
 
 
@@ -81,6 +82,13 @@ def lib():
         L.rd_kernel_timing.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.rd_reset_kernel_timing.argtypes = [vp]
         L.rd_bvh_build_host.argtypes = [vp, vp, i32, i32, vp, i32, vp, vp, vp]
+        L.rd_json_load.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(vp), C.POINTER(abi.rd_render_params)]
+        L.rd_json_scene_desc.restype = C.POINTER(abi.rd_scene_desc)
+        L.rd_json_scene_desc.argtypes = [vp]
+        L.rd_json_scene_free.argtypes = [vp]
+        L.rd_scene_load_json.restype = vp
+        L.rd_scene_load_json.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(abi.rd_render_params)]
+        L.rd_metal_eta_k.argtypes = [C.c_char_p, vp, vp]
         _lib = L
     return _lib
 
@@ -108,6 +116,45 @@ def bvh_build_host(bmin, bmax, max_prim):
     check(lib().rd_bvh_build_host(_ptr(bmin), _ptr(bmax), n, max_prim, _ptr(nodes), cap, _ptr(order),
                                   C.byref(nn), C.byref(md)))
     return nodes[: nn.value * 32].reshape(-1, 32), order[:n], md.value
+
+
+def metal_eta_k(name):
+    """rd_metal_eta_k: (eta, k) RGB of a named conductor (Spectrum.NameToEtaAndK)."""
+    eta = np.zeros(3, np.float32)
+    k = np.zeros(3, np.float32)
+    check(lib().rd_metal_eta_k(name.encode(), _ptr(eta), _ptr(k)))
+    return tuple(float(v) for v in eta), tuple(float(v) for v in k)
+
+
+class JsonScene:
+    """A scene config of the reference (JSON + OBJ / PNG assets) read by
+    rd_json_load into an rd_scene_desc; usable wherever a SceneDesc is (its
+    to_c() is the loaded description).  `spp` is the config's renderer spp."""
+
+    def __init__(self, path, search_dir=None):
+        self.h = None
+        h = C.c_void_p()
+        params = abi.rd_render_params()
+        check(lib().rd_json_load(os.fsencode(path), os.fsencode(search_dir) if search_dir else None, C.byref(h),
+                                 C.byref(params)))
+        self.h = h.value
+        self.path = path
+        self.spp = params.spp
+        self._desc = lib().rd_json_scene_desc(self.h).contents
+        cam = self._desc.camera
+        self.camera = {"width": cam.width, "height": cam.height}
+        self.textures = []
+
+    def to_c(self):
+        return self._desc
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rd_json_scene_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Context:
@@ -211,6 +258,21 @@ class Scene:
             raise RadiantiumError(abi.RD_ERR_INVALID, lib().rd_last_error().decode())
         self.width = scene_desc.camera["width"]
         self.height = scene_desc.camera["height"]
+
+    @classmethod
+    def from_json(cls, ctx, path, search_dir=None):
+        """rd_scene_load_json: (scene, spp of the config's renderer block)."""
+        params = abi.rd_render_params()
+        h = lib().rd_scene_load_json(ctx.h, os.fsencode(path), os.fsencode(search_dir) if search_dir else None,
+                                     C.byref(params))
+        if not h:
+            raise RadiantiumError(abi.RD_ERR_INVALID, lib().rd_last_error().decode())
+        s = cls.__new__(cls)
+        s.ctx, s.desc, s._c, s.h = ctx, None, None, h
+        js = JsonScene(path, search_dir)      # the image size (the camera's screen)
+        s.width, s.height = js.camera["width"], js.camera["height"]
+        js.close()
+        return s, params.spp
 
     def close(self):
         if getattr(self, "h", None):

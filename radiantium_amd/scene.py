@@ -65,6 +65,7 @@ class SceneDesc:
         self._has_normals = False
         self._has_uvs = False
         self.spheres = []
+        self.sphere_w2m = []     # 3x3 per sphere
         self.prims = []          # (kind, index, material, light, med_in, med_out)
         self.materials = []      # dicts
         self.textures = []       # Texture
@@ -105,6 +106,9 @@ class SceneDesc:
             t[1] = self._tex(p.get("anisotropic"), 0.0)
             m["eta"] = tuple(p.get("eta", CU_ETA))
             m["k"] = tuple(p.get("k", CU_K))
+            if "metal_type" in p:   # a named conductor of Spectrum.NameToEtaAndK
+                from .native import metal_eta_k
+                m["eta"], m["k"] = metal_eta_k(p["metal_type"])
         elif kind == "rough_glass":
             m["dist"] = dist(p.get("dist", "GGX"))
             t[0] = self._tex(p.get("r"), 1.0)
@@ -215,8 +219,19 @@ class SceneDesc:
         self.prims.append(np.array([[abi.PRIM_INSTANCE, k, material, -1, medium_inside, medium_outside]], np.int32))
         return p
 
-    def add_sphere(self, center, radius, material=-1, le=None, medium_inside=-1, medium_outside=-1):
+    def add_sphere(self, center, radius, material=-1, le=None, medium_inside=-1, medium_outside=-1, to_world=None):
+        """A sphere shape (Shapes/Sphere.cs; the builder of RendererBuilder.cs:49-54):
+        `center` is moved by the entity's to_world (row-major 4x4, row
+        vectors), the radius is kept, and the inverse's linear part rotates
+        the hit normal into the sphere's frame for its UV."""
         k = len(self.spheres)
+        w2m = np.eye(3, dtype=np.float32)
+        if to_world is not None:
+            M = np.asarray(to_world, np.float32).reshape(4, 4)
+            c = np.asarray(center, np.float32)
+            center = tuple(float(v) for v in (c[0] * M[0, :3] + c[1] * M[1, :3] + c[2] * M[2, :3] + M[3, :3]))
+            w2m = np.linalg.inv(M.astype(np.float64))[:3, :3].astype(np.float32)
+        self.sphere_w2m.append(w2m)
         self.spheres.append(tuple(center) + (float(radius),))
         p = self.num_prims()
         light = -1
@@ -292,6 +307,8 @@ class SceneDesc:
         d.tri_flags = tfl.ctypes.data_as(C.POINTER(C.c_uint8))
         d.num_spheres = a["spheres"].shape[0]
         d.spheres = fptr(a["spheres"])
+        w2m = np.array(self.sphere_w2m, np.float32).reshape(-1, 9)
+        d.sphere_w2m = fptr(w2m) if (w2m.size and not np.array_equal(w2m, np.tile(np.eye(3, dtype=np.float32).reshape(9), (w2m.shape[0], 1)))) else fptr(None)
         prims = a["prims"]
         n = prims.shape[0]
         parr = (abi.rd_prim_desc * max(n, 1))()

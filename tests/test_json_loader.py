@@ -1,0 +1,424 @@
+"""The native scene-config loader (rd_json_load / rd_scene_load_json) against
+the reference's config rules: Radiantium.Cli/Program.cs:25-140 (what is read
+and in which order), Config/RendererBuilder.cs (defaults, entity transforms,
+per-entity materials / media / lights, instancing), Config/JsonConfigProvider.cs
+(number reads, transposed matrices), Radiantium.Core/WavefrontObjReader.cs
+(OBJ rules) and TextureUtility.LoadImageFromPath (image flip, scaling,
+sRGB -> linear).  Configs, OBJ files and PNG images are written by the tests.
+CPU only (no device needed to load)."""
+import ctypes as C
+import json
+import math
+import os
+import struct
+import zlib
+
+import numpy as np
+import pytest
+
+from radiantium_amd import abi, scenes
+from radiantium_amd.native import JsonScene, RadiantiumError
+from radiantium_amd.scene import SceneDesc
+
+REF_EXAMPLES = "/root/reference/Example"
+
+
+def _write(path, text):
+    with open(path, "w") as f:
+        f.write(text)
+
+
+def _cfg(tmp_path, scene, assets=(), integrator=None, camera=None, extra=None, name="scene.json"):
+    c = {"renderer": {"spp": 64}, "integrator": integrator or {"type": "path", "max_depth": 6},
+         "camera": camera or {"type": "perspective", "screen": [32, 24], "fov": 40, "origin": [0, 1, 4],
+                              "target": [0, 1, 0], "up": [0, 1, 0]},
+         "accel": {"type": "bvh"}, "assets": list(assets), "scene": scene}
+    c.update(extra or {})
+    p = tmp_path / name
+    _write(p, json.dumps(c, indent=1))
+    return str(p)
+
+
+def _arr(ptr, n, dtype=np.float32):
+    if not ptr or n == 0:
+        return np.zeros(0, dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype).copy()
+
+
+_alive = []
+
+
+def _geometry(js):
+    _alive.append(js)          # the description points into the loader's storage
+    d = js.to_c()
+    pos = _arr(d.positions, 3 * d.num_vertices).reshape(-1, 3)
+    idx = _arr(d.indices, 3 * d.num_triangles, np.int32).reshape(-1, 3)
+    prims = [(d.prims[i].kind, d.prims[i].index, d.prims[i].material, d.prims[i].light,
+              d.prims[i].medium_inside, d.prims[i].medium_outside) for i in range(d.num_prims)]
+    return d, pos, idx, prims
+
+
+QUAD_OBJ = """# quad
+o quad
+v 0 0 0
+v 1 0 0
+v 1 1 0
+v 0 1 0
+vt 0 0
+vt 1 0
+vt 1 1
+vt 0 1
+f 1/1 2/2 3/3
+f 1/1 3/3 4/4
+"""
+
+
+def test_obj_model_positions_order_and_defaults(tmp_path):
+    """A model entity: vertices deduplicated in face order, 1-based indices,
+    texcoords kept, the reference's material defaults (diffuse kd 0.5) and the
+    renderer spp read from the config."""
+    _write(tmp_path / "quad.obj", QUAD_OBJ)
+    p = _cfg(tmp_path, [{"material": {"type": "diffuse"}, "shape": {"type": "model", "name": "q"}}],
+             assets=[{"type": "model", "name": "q", "location": "quad.obj"}])
+    js = JsonScene(p)
+    d, pos, idx, prims = _geometry(js)
+    assert js.spp == 64
+    # face order 1,2,3 then 1,3,4: new vertices 1,2,3,4 in that order
+    assert np.array_equal(pos, np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], np.float32))
+    assert idx.tolist() == [[0, 1, 2], [0, 2, 3]]
+    uv = _arr(d.uvs, 2 * d.num_vertices).reshape(-1, 2)
+    assert uv.tolist() == [[0, 0], [1, 0], [1, 1], [0, 1]]
+    assert [pp[:4] for pp in prims] == [(0, 0, 0, -1), (0, 1, 0, -1)]
+    assert d.num_materials == 1 and d.materials[0].type == abi.MATERIAL_NAMES["diffuse"]
+    kd = d.textures[d.materials[0].tex[0]]
+    assert kd.width == 0 and list(kd.value) == [0.5, 0.5, 0.5]
+    assert d.integrator.type == abi.INTEGRATOR_PATH and d.integrator.max_depth == 6
+    assert d.integrator.min_depth == 3 and d.integrator.rr_threshold == np.float32(0.99)
+    assert d.camera.width == 32 and d.camera.height == 24 and d.camera.use_matrix == 0
+    assert d.bvh_max_prim == 1
+
+
+def test_obj_rules(tmp_path):
+    """WavefrontObjReader: a vertex used with two texcoords becomes two
+    vertices; a quad face (four corners) fails to parse and is dropped; an
+    unparsable 'v' line is (0,0,0); 'v//vn' carries a normal."""
+    _write(tmp_path / "m.obj", "\n".join([
+        "v 0 0 0", "v 1 0 0", "v 0 1 0", "v 1 1 0", "v bad 1 2",
+        "vt 0 0", "vt 1 1", "vn 0 0 1",
+        "f 1/1 2/1 3/1", "f 2/2 4/2 3/2", "f 1 2 4 3", "f 1//1 2//1 5//1", ""]))
+    p = _cfg(tmp_path, [{"material": {"type": "diffuse"}, "shape": {"type": "model", "name": "m"}}],
+             assets=[{"type": "model", "name": "m", "location": "m.obj"}])
+    with pytest.raises(RadiantiumError):
+        # faces with and without normals in one model: the reference indexes
+        # its arrays out of step; the loader refuses
+        JsonScene(p)
+    _write(tmp_path / "m.obj", "\n".join([
+        "v 0 0 0", "v 1 0 0", "v 0 1 0", "v 1 1 0", "v bad 1 2",
+        "vt 0 0", "vt 1 1",
+        "f 1/1 2/1 3/1", "f 2/2 4/2 3/2", "f 1 2 4 3", "f 1/1 2/1 5/1", ""]))
+    js = JsonScene(p)
+    d, pos, idx, prims = _geometry(js)
+    # (1,vt1) (2,vt1) (3,vt1) | (2,vt2) (4,vt2) (3,vt2) | quad dropped | (1,vt1) (2,vt1) (5,vt1)
+    assert pos.tolist() == [[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 0]]
+    assert idx.tolist() == [[0, 1, 2], [3, 4, 5], [0, 1, 6]]
+    assert len(prims) == 3
+
+
+def test_obj_generated_normals_follow_the_reference(tmp_path):
+    """will_gen_normal: per triangle the face normal nf is written to each
+    corner, averaged (normalize(nf + n)) with the value already stored at the
+    FIRST corner (WavefrontObjReader.cs:476-491 reads na, nb and nc all from
+    corner 0)."""
+    _write(tmp_path / "g.obj", "v 0 0 0\nv 1 0 0\nv 0 1 0\nv 0 0 1\nf 1 2 3\nf 1 4 2\n")
+    p = _cfg(tmp_path, [{"material": {"type": "diffuse"}, "shape": {"type": "model", "name": "g"}}],
+             assets=[{"type": "model", "name": "g", "location": "g.obj", "will_gen_normal": True}])
+    d, pos, idx, prims = _geometry(JsonScene(p))
+    n = _arr(d.normals, 3 * d.num_vertices).reshape(-1, 3)
+    f1 = np.array([0, 0, 1], np.float32)                       # cross((1,0,0),(0,1,0))
+    f2 = np.cross(np.array([0, 0, 1.0]), np.array([1, 0, 0.0]))  # (0,1,0)
+    f2 = (f2 / np.linalg.norm(f2)).astype(np.float32)
+    avg = (f2 + f1) / np.linalg.norm(f2 + f1)                  # first corner (vertex 0) had f1
+    # triangle 2 corners: v0 (had f1) -> avg, v3 (zero... but reads v0's) -> avg, v1 (reads v0's) -> avg
+    assert np.allclose(n[0], avg, atol=1e-6)
+    assert np.allclose(n[3], avg, atol=1e-6) and np.allclose(n[1], avg, atol=1e-6)
+    assert np.allclose(n[2], f1, atol=1e-6)
+    assert all(tf & 1 for tf in _arr(d.tri_flags, d.num_triangles, np.uint8))
+
+
+def test_entity_transforms(tmp_path):
+    """ReadTransform: position, axis-angle rotation (degrees), scale composed
+    as Scale * Rotation * Translation (row vectors); a child's ModelToWorld
+    is parent * local (RendererBuilder.cs:936), which with row vectors applies
+    the parent's transform first and the child's own after it; quaternion and
+    16-number (transposed) rotations give the same matrix."""
+    _write(tmp_path / "quad.obj", QUAD_OBJ)
+    asset = [{"type": "model", "name": "q", "location": "quad.obj"}]
+    ent = lambda rot: [{"material": {"type": "diffuse"}, "shape": {"type": "model", "name": "q"},
+                        "transform": {"position": [1, 2, 3], "rotation": rot, "scale": [2, 2, 2]},
+                        "children": [{"material": {"type": "diffuse"}, "shape": {"type": "model", "name": "q"},
+                                      "transform": {"position": [0, 0, 1]}}]}]
+    _, pa, _, _ = _geometry(JsonScene(_cfg(tmp_path, ent({"axis": [0, 1, 0], "angle": 90}), asset, name="a.json")))
+    s = math.sqrt(0.5)
+    _, pq, _, _ = _geometry(JsonScene(_cfg(tmp_path, ent([0, s, 0, s]), asset, name="q.json")))
+    # column-vector 90-degree rotation about y, as a config would write it (the loader transposes)
+    Rc = [0, 0, 1, 0, 0, 1, 0, 0, -1, 0, 0, 0, 0, 0, 0, 1]
+    _, pm, _, _ = _geometry(JsonScene(_cfg(tmp_path, ent(Rc), asset, name="m.json")))
+    # parent: p' = (2 p) R_y(90) + (1, 2, 3); R_y(90) in row vectors maps x -> -z, z -> x
+    quad = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], np.float64)
+    R = np.array([[0, 0, -1], [0, 1, 0], [1, 0, 0]], np.float64)
+    parent = 2 * quad @ R + [1, 2, 3]
+    child = (quad @ (2 * R) + [1, 2, 3]) + [0, 0, 1]
+    expect = np.concatenate([parent, child])
+    for got in (pa, pq, pm):
+        assert np.allclose(got, expect, atol=1e-5), got
+
+
+def test_spheres_media_lights(tmp_path):
+    """Spheres move their centre by the entity transform (radius kept) and
+    carry the inverse for their UV; every entity gets its own media, "is_global"
+    resolves to the camera's medium, "same" sets both sides; an area light per
+    shape; an entity with a light and neither material nor medium is an
+    infinite light placed at its position in Scene.Lights."""
+    hom = lambda s: {"type": "homogeneous", "sigma_a": [0.1, 0.1, 0.1], "sigma_s": [s, s, s], "g": 0.3}
+    cam = {"type": "perspective", "screen": [16, 16], "origin": [0, 0, 5], "target": [0, 0, 0], "medium": hom(0.5)}
+    scene = [
+        {"material": {"type": "diffuse", "kd": [0.2, 0.3, 0.4]}, "shape": {"type": "sphere", "radius": 0.5},
+         "light": {"type": "diffuse_area", "le": [3, 3, 3]}, "medium": {"outside": {"is_global": True}}},
+        {"light": {"type": "infinite", "le": [0.1, 0.2, 0.3]}},
+        {"shape": {"type": "sphere", "radius": 1.0, "center": [1, 0, 0]},
+         "transform": {"position": [0, 2, 0], "rotation": {"axis": [0, 0, 1], "angle": 90}},
+         "medium": {"same": hom(2.0)}},
+        {"material": {"type": "perfect_glass"}, "shape": {"type": "sphere", "radius": 0.3},
+         "medium": {"inside": hom(3.0), "outside": {"is_global": True}}},
+    ]
+    d, _, _, prims = _geometry(JsonScene(_cfg(tmp_path, scene, camera=cam, integrator={"type": "vol_path"})))
+    assert d.num_spheres == 3 and d.num_media == 3 and d.global_medium == 0
+    sph = _arr(d.spheres, 12).reshape(3, 4)
+    assert np.allclose(sph[1], [0, 3, 0, 1.0], atol=1e-6)      # (1,0,0) rotated 90 about z -> (0,1,0), + (0,2,0)
+    w2m = _arr(d.sphere_w2m, 27).reshape(3, 3, 3)
+    assert np.allclose(w2m[0], np.eye(3)) and np.allclose(w2m[1], [[0, -1, 0], [1, 0, 0], [0, 0, 1]], atol=1e-6)
+    assert prims[0][2:] == (0, 0, -1, 0)                       # material 0, light 0, outside = global
+    assert prims[1][2:] == (-1, -1, 1, 1)                      # no surface, "same" medium 1
+    assert prims[2][4:] == (2, 0)
+    assert d.num_lights == 2 and d.lights[0].type == abi.LIGHT_AREA and d.lights[0].prim == 0
+    assert list(d.lights[0].le) == [3, 3, 3]
+    assert d.lights[1].type == abi.LIGHT_INFINITE
+    assert list(d.textures[d.lights[1].tex].value) == [np.float32(0.1), np.float32(0.2), np.float32(0.3)]
+    assert d.media[1].sigma_s[0] == 2.0 and d.media[0].g == np.float32(0.3)
+    assert d.integrator.type == abi.INTEGRATOR_VOL_PATH
+
+
+def test_instanced_entries(tmp_path):
+    """"instanced" entries become instance sets (their local transform baked
+    into the triangles, their own accel max_prim); entities with
+    {"instanced": {"id": k}} become InstancedPrimitives with the entity's
+    ModelToWorld, material and media."""
+    _write(tmp_path / "quad.obj", QUAD_OBJ)
+    c = {"instanced": [{"shape": {"type": "model", "name": "q"}, "accel": {"type": "bvh", "max_prim": 4},
+                        "transform": {"position": [0, 0, 2]}}]}
+    scene = [{"material": {"type": "rough_metal"}, "instanced": {"id": 0},
+              "transform": {"position": [5, 0, 0]}},
+             {"material": {"type": "diffuse"}, "instanced": {"id": 0}, "transform": {"scale": [3, 3, 3]}}]
+    js = JsonScene(_cfg(tmp_path, scene, assets=[{"type": "model", "name": "q", "location": "quad.obj"}], extra=c))
+    d, pos, idx, prims = _geometry(js)
+    assert d.num_instance_sets == 1 and d.num_instances == 2
+    st = d.instance_sets[0]
+    assert (st.first_triangle, st.num_triangles, st.bvh_max_prim) == (0, 2, 4)
+    assert np.allclose(pos[:, 2], 2.0)                         # the entry's local transform is baked in
+    tw = [np.array(list(d.instances[k].to_world)).reshape(4, 4) for k in range(2)]
+    assert np.allclose(tw[0][3, :3], [5, 0, 0]) and np.allclose(np.diag(tw[1])[:3], 3)
+    assert [p[:3] for p in prims] == [(abi.PRIM_INSTANCE, 0, 0), (abi.PRIM_INSTANCE, 1, 1)]
+
+
+def _png(path, img, ctype, depth, filters=(0, 1, 2, 3, 4), palette=None):
+    """Writes a PNG (rows filtered by cycling through `filters`)."""
+    h, w = img.shape[:2]
+    ch = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[ctype]
+    bpp = max(1, ch * depth // 8)
+    raw = bytearray()
+    prev = bytes(((w * ch * depth + 7) // 8))
+    for y in range(h):
+        if depth == 16:
+            row = b"".join(struct.pack(">H", int(v)) for v in img[y].reshape(-1))
+        elif depth == 8:
+            row = bytes(int(v) for v in img[y].reshape(-1))
+        else:
+            bits = "".join(format(int(v), f"0{depth}b") for v in img[y].reshape(-1))
+            bits += "0" * (-len(bits) % 8)
+            row = bytes(int(bits[i:i + 8], 2) for i in range(0, len(bits), 8))
+        f = filters[y % len(filters)]
+        out = bytearray()
+        for x in range(len(row)):
+            a = row[x - bpp] if x >= bpp else 0
+            b = prev[x]
+            c = prev[x - bpp] if x >= bpp else 0
+            if f == 0: pr = 0
+            elif f == 1: pr = a
+            elif f == 2: pr = b
+            elif f == 3: pr = (a + b) // 2
+            else:
+                p_ = a + b - c
+                pa, pb, pc = abs(p_ - a), abs(p_ - b), abs(p_ - c)
+                pr = a if (pa <= pb and pa <= pc) else (b if pb <= pc else c)
+            out.append((row[x] - pr) & 0xFF)
+        raw += bytes([f]) + out
+        prev = row
+
+    def chunk(t, data):
+        return struct.pack(">I", len(data)) + t + data + struct.pack(">I", zlib.crc32(t + data) & 0xFFFFFFFF)
+    data = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, depth, ctype, 0, 0, 0))
+    if palette is not None:
+        data += chunk(b"PLTE", bytes(palette.reshape(-1).tolist()))
+    data += chunk(b"IDAT", zlib.compress(bytes(raw))) + chunk(b"IEND", b"")
+    with open(path, "wb") as f:
+        f.write(data)
+
+
+def _tex_of(tmp_path, fname, extra_asset=None, sampler=None):
+    asset = {"type": "image", "name": "im", "location": fname}
+    asset.update(extra_asset or {})
+    kd = {"type": "image", "name": "im"}
+    if sampler:
+        kd["sampler"] = sampler
+    js = JsonScene(_cfg(tmp_path, [{"material": {"type": "diffuse", "kd": kd}, "shape": {"type": "sphere"}}],
+                        assets=[asset], name=fname + ".json"))
+    _alive.append(js)
+    d = js.to_c()
+    t = d.textures[d.materials[0].tex[0]]
+    return t, _arr(t.data, 3 * t.width * t.height).reshape(t.width, t.height, 3)
+
+
+def _to_linear(v):
+    v = v.astype(np.float32)
+    return np.where(v <= np.float32(0.04045), v * np.float32(1 / 12.92),
+                    np.power((v + np.float32(0.055)) * np.float32(1 / 1.055), np.float32(2.4))).astype(np.float32)
+
+
+@pytest.mark.parametrize("ctype,depth", [(2, 8), (6, 8), (2, 16), (3, 8), (3, 4)])
+def test_png_images(tmp_path, ctype, depth):
+    """PNG decoding (every filter type): texel (x, y) of the texture is image
+    row H-1-y (flipped), samples / (2^depth - 1), sRGB -> linear by default,
+    RGB of RGBA, palette entries / 255."""
+    rng = np.random.default_rng(ctype * 100 + depth)
+    h, w = 7, 9
+    if ctype == 3:
+        pal = rng.integers(0, 256, (16, 3))
+        ind = rng.integers(0, 16 if depth >= 4 else 2 ** depth, (h, w))
+        _png(tmp_path / "i.png", ind, ctype, depth, palette=pal)
+        rgb = pal[ind] / np.float32(255.0)
+    else:
+        ch = {2: 3, 6: 4}[ctype]
+        img = rng.integers(0, 2 ** depth, (h, w, ch))
+        _png(tmp_path / "i.png", img, ctype, depth)
+        rgb = img[..., :3] / np.float32(2 ** depth - 1)
+    rgb = rgb.astype(np.float32)
+    t, tex = _tex_of(tmp_path, "i.png", sampler={"filter": "Linear", "wrap": "Repeat"})
+    assert (t.width, t.height, t.filter, t.wrap) == (w, h, 1, 1)
+    expect = _to_linear(rgb)[::-1].transpose(1, 0, 2)            # [x][y], row 0 of the texture = last image row
+    assert np.allclose(tex, expect, rtol=2e-6, atol=1e-9)
+    # is_cast_to_linear false and is_flip_y false: raw samples in image order
+    t2, tex2 = _tex_of(tmp_path, "i.png", {"is_cast_to_linear": False, "is_flip_y": False})
+    assert np.array_equal(tex2, rgb.transpose(1, 0, 2))
+
+
+def test_errors(tmp_path):
+    _write(tmp_path / "quad.obj", QUAD_OBJ)
+    with pytest.raises(RadiantiumError, match="UNSUPPORTED"):
+        JsonScene(_cfg(tmp_path, [], integrator={"type": "ao"}, name="ao.json"))
+    with pytest.raises(RadiantiumError, match="invalid location"):
+        JsonScene(_cfg(tmp_path, [], assets=[{"type": "model", "name": "x", "location": "missing.obj"}], name="m.json"))
+    with pytest.raises(RadiantiumError, match="UNSUPPORTED"):
+        _write(tmp_path / "sky.exr", "not an image")
+        JsonScene(_cfg(tmp_path, [], assets=[{"type": "image", "name": "s", "location": "sky.exr"}], name="e.json"))
+    _write(tmp_path / "broken.json", '{"camera": {"type": "perspective"}, "scene": [}')
+    with pytest.raises(RadiantiumError, match="JSON"):
+        JsonScene(str(tmp_path / "broken.json"))
+
+
+def test_loaded_scene_matches_the_python_description(tmp_path):
+    """The same Cornell-style scene written as a config (OBJ walls, a sphere,
+    an area-light quad) and built with SceneDesc: identical description."""
+    walls = "\n".join(["v -1 0 -1", "v 1 0 -1", "v 1 2 -1", "v -1 2 -1", "v -1 0 1", "v 1 0 1",
+                       "f 1 5 6", "f 1 6 2", "f 1 2 3", "f 1 3 4", ""])
+    light = "\n".join(["v -0.25 1.99 -0.25", "v 0.25 1.99 -0.25", "v 0.25 1.99 0.25", "v -0.25 1.99 0.25",
+                       "f 1 2 3", "f 1 3 4", ""])
+    _write(tmp_path / "walls.obj", walls)
+    _write(tmp_path / "light.obj", light)
+    scene = [{"material": {"type": "diffuse", "kd": [0.7, 0.7, 0.7]}, "shape": {"type": "model", "name": "w"}},
+             {"material": {"type": "rough_plastic", "r": [0.2, 0.4, 0.8], "roughness": 0.2, "dist": "GGX"},
+              "shape": {"type": "sphere", "center": [0.2, 0.5, 0.1], "radius": 0.4}},
+             {"material": {"type": "diffuse", "kd": 0}, "shape": {"type": "model", "name": "l"},
+              "light": {"type": "diffuse_area", "le": [17, 12, 4]}}]
+    js = JsonScene(_cfg(tmp_path, scene, assets=[{"type": "model", "name": "w", "location": "walls.obj"},
+                                                 {"type": "model", "name": "l", "location": "light.obj"}]))
+    d, pos, idx, prims = _geometry(js)
+    s = SceneDesc()
+    m0 = s.add_material("diffuse", kd=(0.7, 0.7, 0.7))
+    s.add_mesh(np.array([[-1, 0, -1], [1, 0, -1], [1, 2, -1], [-1, 2, -1], [-1, 0, 1], [1, 0, 1]], np.float32),
+               np.array([[0, 4, 5], [0, 5, 1], [0, 1, 2], [0, 2, 3]]), material=m0)
+    m1 = s.add_material("rough_plastic", r=(0.2, 0.4, 0.8), roughness=0.2, dist="GGX")
+    s.add_sphere((0.2, 0.5, 0.1), 0.4, material=m1)
+    m2 = s.add_material("diffuse", kd=0.0)
+    s.add_mesh(np.array([[-0.25, 1.99, -0.25], [0.25, 1.99, -0.25], [0.25, 1.99, 0.25], [-0.25, 1.99, 0.25]],
+                        np.float32), np.array([[0, 1, 2], [0, 2, 3]]), material=m2, le=(17, 12, 4))
+    a = s.arrays()
+    # OBJ vertices come in first-use order (ToModel), so compare triangle corners
+    assert np.array_equal(pos[idx], a["positions"][a["indices"]])
+    assert prims == [tuple(int(v) for v in r) for r in a["prims"]]
+    assert np.array_equal(_arr(d.spheres, 4), np.array(s.spheres[0], np.float32))
+    assert d.num_lights == len(s.lights) == 2
+    for i in range(d.num_materials):
+        got, want = d.materials[i], s.materials[i]
+        assert got.type == want["type"] and got.dist == want["dist"]
+        for k in (0, 1):
+            gt, wt = d.textures[got.tex[k]], s.textures[want["tex"][k]]
+            assert list(gt.value) == [np.float32(v) for v in wt.value]
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_EXAMPLES), reason="the reference's example configs are not present")
+def test_reference_example_configs():
+    """Every example config of the reference: the ones for the path / vol_path
+    integrators whose assets are all present load; the others fail with the
+    reason (integrator not on this path, missing or undecodable asset)."""
+    loaded, refused = [], []
+    for root, _, files in os.walk(REF_EXAMPLES):
+        for f in sorted(files):
+            if not f.endswith(".json"):
+                continue
+            p = os.path.join(root, f)
+            cfg = json.load(open(p))
+            integ = cfg.get("integrator", {}).get("type")
+            assets_ok = all(os.path.exists(os.path.join(root, a["location"])) for a in cfg.get("assets", []))
+            try:
+                js = JsonScene(p)
+                loaded.append((p, js.to_c().num_prims))
+            except RadiantiumError as e:
+                refused.append((p, str(e)))
+                assert integ not in ("path", "vol_path") or not assets_ok or "UNSUPPORTED" in str(e), (p, str(e))
+                continue
+            assert integ in ("path", "vol_path") and assets_ok, p
+    assert len(loaded) >= 10, (loaded, refused)
+    assert all(n > 0 for _, n in loaded)
+
+
+def test_metal_types(tmp_path):
+    """"metal_type": a name of Spectrum.NameToEtaAndK (Spectrum.cs:46-129), an
+    unknown name is an error; the object form reads "eta" / "k" from the
+    material itself (RendererBuilder.cs:141-147); the default is Cu."""
+    from radiantium_amd.native import metal_eta_k
+    au = metal_eta_k("Au")
+    assert np.allclose(au[0], (0.1428174, 0.3741433, 1.4394432)) and np.allclose(au[1], (3.9747169, 2.3806596, 1.5998082))
+    cu = metal_eta_k("Cu")
+    mats = [{"type": "rough_metal", "metal_type": "Au"}, {"type": "rough_metal"},
+            {"type": "rough_metal", "metal_type": {"x": 1}, "eta": [1, 2, 3], "k": [4, 5, 6]}]
+    scene = [{"material": m, "shape": {"type": "sphere"}} for m in mats]
+    d, _, _, _ = _geometry(JsonScene(_cfg(tmp_path, scene)))
+    assert np.allclose(list(d.materials[0].eta), au[0]) and np.allclose(list(d.materials[0].k), au[1])
+    assert np.allclose(list(d.materials[1].eta), cu[0]) and np.allclose(list(d.materials[1].k), cu[1])
+    assert list(d.materials[2].eta) == [1, 2, 3] and list(d.materials[2].k) == [4, 5, 6]
+    with pytest.raises(RadiantiumError, match="no metal type"):
+        JsonScene(_cfg(tmp_path, [{"material": {"type": "rough_metal", "metal_type": "Xx"},
+                                   "shape": {"type": "sphere"}}], name="bad.json"))
+    s = SceneDesc()
+    s.add_material("rough_metal", metal_type="Au")
+    assert s.materials[0]["eta"] == au[0]
