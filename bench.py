@@ -358,7 +358,8 @@ def main():
     value = rays / elapsed / 1e6
 
     stages = {}
-    for name in ("compact", "trace_closest", "shade", "trace_any", "trace_mis", "resolve", "accumulate", "finalize"):
+    for name in ("compact", "sort", "trace_closest", "shade", "trace_any", "trace_mis", "resolve", "accumulate",
+                 "finalize"):
         avg, cnt = ctx.kernel_timing(name)
         stages[name] = (avg, cnt)
         log(f"[bench] stage {name:14s} avg {avg * 1000:9.1f} us x {cnt:5d} = {avg * cnt:9.1f} ms")

@@ -21,7 +21,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unus
 DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-fhip-fp32-correctly-rounded-divide-sqrt",
           "-fno-gpu-flush-denormals-to-zero", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 HOST_SOURCES = ["rd_api.cpp", "rd_camera.cpp", "rd_bvh_build.cpp", "rd_envmap.cpp", "rd_json.cpp"]
-DEVICE_SOURCES = ["rd_kernels.hip", "rd_volpath.hip"]
+DEVICE_SOURCES = ["rd_kernels.hip", "rd_volpath.hip", "rd_sort.hip"]
 
 
 def hipcc():

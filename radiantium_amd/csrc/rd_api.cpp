@@ -68,6 +68,15 @@ struct WaveBuffers {
     std::vector<void*> adj_allocs;
     void** grad_ptrs = nullptr;     // adjoint: device table of gradient accumulators
     size_t grad_cap = 0;
+    // extension-queue coherence sort
+    size_t sort_cap = 0;
+    std::vector<void*> sort_allocs;
+    uint32_t* sort_keys = nullptr;
+    uint32_t* sort_keys_alt = nullptr;
+    int* sort_vals = nullptr;
+    int* sort_perm = nullptr;
+    void* sort_temp = nullptr;
+    size_t sort_temp_bytes = 0;
     static void free_list(std::vector<void*>& v) {
         for (void* p : v) (void)hipFree(p);
         v.clear();
@@ -79,6 +88,11 @@ struct WaveBuffers {
         free_list(scene_allocs);
         free_list(seg_allocs);
         free_list(adj_allocs);
+        free_list(sort_allocs);
+        sort_cap = 0;
+        sort_keys = sort_keys_alt = nullptr;
+        sort_vals = sort_perm = nullptr;
+        sort_temp = nullptr;
         adj_cap = grad_cap = 0;
         grad_ptrs = nullptr;
         if (host_segs) (void)hipHostFree(host_segs);
@@ -102,6 +116,11 @@ struct rd_ctx {
     rd_render_stats stats{};
     std::map<std::string, StageTiming> timing;
     bool timing_enabled = true;
+    // order extension rays by direction octant and origin before tracing
+    // them (bounces after the first; RD_RAY_SORT=0 turns it off)
+    bool ray_sort = true;
+    int ray_sort_bits = 15;       // key bits (RD_RAY_SORT_BITS): octant + Morton prefix
+    int ray_sort_min = 1 << 16;   // smaller queues are traced in queue order
     // scenes created on this context and not yet destroyed; rd_destroy with
     // live scenes releases the wavefront buffers and leaves the rest to the
     // last rd_scene_destroy, so destruction order does not matter
@@ -298,6 +317,32 @@ int ensure_wave_buffers(rd_ctx* c, size_t pool, size_t results, size_t pixels, b
 }
 
 // Adjoint replay: per-slot derivative records and a table of `ntex` gradient pointers.
+// Buffers of the extension-queue sort, sized for the pool.
+int ensure_sort_buffers(rd_ctx* c, size_t pool) {
+    WaveBuffers& w = c->wb;
+    if (w.sort_cap >= pool) return RD_OK;
+    WaveBuffers::free_list(w.sort_allocs);
+    w.sort_cap = 0;
+    auto alloc = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        w.sort_allocs.push_back(p);
+        return p;
+    };
+    w.sort_keys = (uint32_t*)alloc(pool * 4);
+    w.sort_keys_alt = (uint32_t*)alloc(pool * 4);
+    w.sort_vals = (int*)alloc(pool * 4);
+    w.sort_perm = (int*)alloc(pool * 4);
+    w.sort_temp_bytes = ray_sort_temp_bytes((int)pool);
+    w.sort_temp = alloc(std::max<size_t>(w.sort_temp_bytes, 16));
+    if (!w.sort_keys || !w.sort_keys_alt || !w.sort_vals || !w.sort_perm || !w.sort_temp) {
+        WaveBuffers::free_list(w.sort_allocs);
+        return fail(RD_ERR_OOM, "hipMalloc failed for the ray sort");
+    }
+    w.sort_cap = pool;
+    return RD_OK;
+}
+
 int ensure_adjoint_buffers(rd_ctx* c, size_t ntex) {
     WaveBuffers& w = c->wb;
     if (w.adj_cap < w.cap || w.grad_cap < ntex) {
@@ -364,6 +409,8 @@ rd_ctx* rd_create(int device) {
         if (sscanf(env, "%d,%d,%d", &v[0], &v[1], &v[2]) == 3)
             for (int k = 0; k < 3; k++) set_trace_schedule(k, v[k]);
     }
+    if (const char* env = getenv("RD_RAY_SORT")) c->ray_sort = atoi(env) != 0;
+    if (const char* env = getenv("RD_RAY_SORT_BITS")) c->ray_sort_bits = std::max(3, std::min(30, atoi(env)));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         g_err = "hipStreamCreate failed"; delete c; return nullptr;
     }
@@ -975,8 +1022,20 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
     }));
     int rc = RD_OK;
     for (int bounce = 0; !rc; bounce++) {
+        // the queue size is known on the host from the second bounce on
+        const int* perm = nullptr;
+        if (c->ray_sort && bounce > 0 && c->host_counts[cin] >= c->ray_sort_min) {
+            if (int e = ensure_sort_buffers(c, (size_t)L.pool)) return e;
+            HIP_TRY(tm.run("sort", [&] {
+                return launch_ray_sort(L.sc.nodes, qin, c->host_counts[cin], c->ray_sort_bits, wb.sort_keys,
+                                       wb.sort_keys_alt,
+                                       wb.sort_vals, wb.sort_perm, wb.sort_temp, wb.sort_temp_bytes, st);
+            }));
+            perm = wb.sort_perm;
+        }
         HIP_TRY(tm.run("trace_closest", [&] {
-            return launch_trace_ext(L.sc, L.scenes, qin, wb.ps, wb.hits, c->counts, cin, cout, kCntShadow, kCntMis, st);
+            return launch_trace_ext(L.sc, L.scenes, qin, wb.ps, wb.hits, c->counts, cin, cout, kCntShadow, kCntMis,
+                                    perm, st);
         }));
         if (L.vol) {
             // VolumetricPathTracer.Li: shade, then the Transmittance / IntersectTr

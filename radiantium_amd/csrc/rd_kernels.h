@@ -119,7 +119,15 @@ hipError_t launch_init_pool(PathState ps, int pool, int* counts, hipStream_t s);
 hipError_t launch_compact(const DevScene& sc, const DevScene* scenes, const BatchSeg* segs, int nseg, int npass,
                           int pool, int2* blk, RayQueue q, int cnt_q, PathState ps, int* counts, hipStream_t s);
 hipError_t launch_trace_ext(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, float4* hits,
-                            int* counts, int cnt_slot, int zero_a, int zero_b, int zero_c, hipStream_t s);
+                            int* counts, int cnt_slot, int za, int zb, int zc, const int* perm, hipStream_t s);
+// Coherence order of the first n rays of q (direction octant, then the
+// origin's Morton code in the root box of `nodes`, `bits` key bits in all,
+// 3..30): perm receives the queue
+// positions in traversal order.  keys / keys_alt / vals: n each; temp of
+// ray_sort_temp_bytes(>= n) bytes.
+size_t ray_sort_temp_bytes(int max_n);
+hipError_t launch_ray_sort(const float4* nodes, RayQueue q, int n, int bits, uint32_t* keys, uint32_t* keys_alt,
+                           int* vals, int* perm, void* temp, size_t temp_bytes, hipStream_t s);
 // `scenes` (device array, may be null): batched mode, the path's scene is
 // scenes[ps.req[pid]]; otherwise every path uses `sc`.
 hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,

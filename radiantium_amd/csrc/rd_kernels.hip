@@ -74,6 +74,7 @@ struct TraceArgs {
     int zero[3];              // count slots block 0 clears for the next stage (-1: none)
     const DevScene* scenes;   // batched queue modes: the ray's scene is scenes[ps.req[pid]]
     int* sub;                 // closest modes of instanced scenes: set triangle (DevPrim index) or -1
+    const int* perm;          // extension queue: the order to trace its rays in (null = queue order)
 };
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
@@ -385,7 +386,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_straight(DevScene sc, TraceArg
     const int n = trav_count(a, kMode <= kModeRaysAny);
     if (blockIdx.x == 0 && threadIdx.x < 3 && a.zero[threadIdx.x] >= 0) a.counts[a.zero[threadIdx.x]] = 0;
     uint32_t nv = 0, nt = 0, nrays = 0;
-    for (int r = blockIdx.x * kBlock + threadIdx.x; r < n; r += gridDim.x * kBlock) {
+    for (int r0 = blockIdx.x * kBlock + threadIdx.x; r0 < n; r0 += gridDim.x * kBlock) {
+        const int r = a.perm ? a.perm[r0] : r0;
         float4 ra, rb;
         trav_load<kMode>(a, r, &ra, &rb);
         Trav t;
@@ -415,7 +417,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_inst(DevScene sc, TraceArgs a)
     const int n = trav_count(a, kMode <= kModeRaysAny);
     if (blockIdx.x == 0 && threadIdx.x < 3 && a.zero[threadIdx.x] >= 0) a.counts[a.zero[threadIdx.x]] = 0;
     uint32_t nv = 0, nt = 0, nrays = 0;
-    for (int r = blockIdx.x * kBlock + threadIdx.x; r < n; r += gridDim.x * kBlock) {
+    for (int r0 = blockIdx.x * kBlock + threadIdx.x; r0 < n; r0 += gridDim.x * kBlock) {
+        const int r = a.perm ? a.perm[r0] : r0;
         float4 ra, rb;
         trav_load<kMode>(a, r, &ra, &rb);
         TravI s;
@@ -461,7 +464,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(DevScene sc, TraceArgs 
                 if (!active) {
                     int rr = base + __popcll(idle & lanemask_lt());
                     if (rr < n) {
-                        r = rr;
+                        r = a.perm ? a.perm[rr] : rr;
                         float4 ra, rb;
                         trav_load<kMode>(a, r, &ra, &rb);
                         trav_begin(t, ra, rb);
@@ -540,6 +543,7 @@ static hipError_t launch_trace_mode(const DevScene& sc, const TraceArgs& a, int 
 
 static TraceArgs trace_args(int* counts) {
     TraceArgs a{};
+    a.perm = nullptr;
     a.counts = counts;
     a.zero[0] = a.zero[1] = a.zero[2] = -1;
     a.scenes = nullptr;
@@ -786,8 +790,9 @@ hipError_t launch_init_pool(PathState ps, int pool, int* counts, hipStream_t s) 
 // Closest hit of the extension queue.  It also clears the queue counters and
 // work cursors the following shade / shadow / MIS stages start from.
 hipError_t launch_trace_ext(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, float4* hits,
-                            int* counts, int cnt_slot, int za, int zb, int zc, hipStream_t s) {
+                            int* counts, int cnt_slot, int za, int zb, int zc, const int* perm, hipStream_t s) {
     TraceArgs a = trace_args(counts);
+    a.perm = perm;
     a.q = q; a.ps = ps; a.hits = hits; a.cnt_slot = cnt_slot; a.cursor_slot = kCntWork + 0; a.scenes = scenes;
     a.zero[0] = za; a.zero[1] = zb; a.zero[2] = zc;
     a.sub = ps.hsub;
