@@ -75,6 +75,7 @@ struct WaveBuffers {
     uint32_t* sort_keys_alt = nullptr;
     int* sort_vals = nullptr;
     int* sort_perm = nullptr;
+    int* sort_rank = nullptr;       // queue position -> place in the traced order
     void* sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
     static void free_list(std::vector<void*>& v) {
@@ -91,7 +92,7 @@ struct WaveBuffers {
         free_list(sort_allocs);
         sort_cap = 0;
         sort_keys = sort_keys_alt = nullptr;
-        sort_vals = sort_perm = nullptr;
+        sort_vals = sort_perm = sort_rank = nullptr;
         sort_temp = nullptr;
         adj_cap = grad_cap = 0;
         grad_ptrs = nullptr;
@@ -121,6 +122,7 @@ struct rd_ctx {
     bool ray_sort = true;
     int ray_sort_bits = 15;       // key bits (RD_RAY_SORT_BITS): octant + Morton prefix
     int ray_sort_min = 1 << 16;   // smaller queues are traced in queue order
+    bool ray_sort_shadow = true;  // shadow / MIS queues in that order too (RD_RAY_SORT_SHADOW)
     // scenes created on this context and not yet destroyed; rd_destroy with
     // live scenes releases the wavefront buffers and leaves the rest to the
     // last rd_scene_destroy, so destruction order does not matter
@@ -333,9 +335,10 @@ int ensure_sort_buffers(rd_ctx* c, size_t pool) {
     w.sort_keys_alt = (uint32_t*)alloc(pool * 4);
     w.sort_vals = (int*)alloc(pool * 4);
     w.sort_perm = (int*)alloc(pool * 4);
+    w.sort_rank = (int*)alloc(pool * 4);
     w.sort_temp_bytes = ray_sort_temp_bytes((int)pool);
     w.sort_temp = alloc(std::max<size_t>(w.sort_temp_bytes, 16));
-    if (!w.sort_keys || !w.sort_keys_alt || !w.sort_vals || !w.sort_perm || !w.sort_temp) {
+    if (!w.sort_keys || !w.sort_keys_alt || !w.sort_vals || !w.sort_perm || !w.sort_rank || !w.sort_temp) {
         WaveBuffers::free_list(w.sort_allocs);
         return fail(RD_ERR_OOM, "hipMalloc failed for the ray sort");
     }
@@ -410,6 +413,7 @@ rd_ctx* rd_create(int device) {
             for (int k = 0; k < 3; k++) set_trace_schedule(k, v[k]);
     }
     if (const char* env = getenv("RD_RAY_SORT")) c->ray_sort = atoi(env) != 0;
+    if (const char* env = getenv("RD_RAY_SORT_SHADOW")) c->ray_sort_shadow = atoi(env) != 0;
     if (const char* env = getenv("RD_RAY_SORT_BITS")) c->ray_sort_bits = std::max(3, std::min(30, atoi(env)));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         g_err = "hipStreamCreate failed"; delete c; return nullptr;
@@ -1035,8 +1039,10 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
         }
         HIP_TRY(tm.run("trace_closest", [&] {
             return launch_trace_ext(L.sc, L.scenes, qin, wb.ps, wb.hits, c->counts, cin, cout, kCntShadow, kCntMis,
-                                    perm, st);
+                                    perm, wb.sort_rank, st);
         }));
+        // with a sorted extension queue the shadow / MIS rays follow its order
+        const int* rank = (perm && c->ray_sort_shadow) ? wb.sort_rank : nullptr;
         if (L.vol) {
             // VolumetricPathTracer.Li: shade, then the Transmittance / IntersectTr
             // chains step by step until both chain queues drain
@@ -1072,7 +1078,7 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
         } else {
             HIP_TRY(tm.run("shade", [&] {
                 return launch_shade(L.sc, L.scenes, qin, wb.hits, wb.qshadow, wb.qmis, wb.ps, c->counts, cin,
-                                    L.adj != nullptr, st);
+                                    L.adj != nullptr, rank, st);
             }));
             if (L.method != RD_METHOD_BXDF) {
                 HIP_TRY(tm.run("trace_any", [&] {

@@ -118,8 +118,11 @@ hipError_t launch_trace_rays(const DevScene& sc, const float4* rays, int n, floa
 hipError_t launch_init_pool(PathState ps, int pool, int* counts, hipStream_t s);
 hipError_t launch_compact(const DevScene& sc, const DevScene* scenes, const BatchSeg* segs, int nseg, int npass,
                           int pool, int2* blk, RayQueue q, int cnt_q, PathState ps, int* counts, hipStream_t s);
+// perm: trace order (null = queue order); rank (with perm) receives each
+// queue position's place in that order.
 hipError_t launch_trace_ext(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, float4* hits,
-                            int* counts, int cnt_slot, int za, int zb, int zc, const int* perm, hipStream_t s);
+                            int* counts, int cnt_slot, int za, int zb, int zc, const int* perm, int* rank,
+                            hipStream_t s);
 // Coherence order of the first n rays of q (direction octant, then the
 // origin's Morton code in the root box of `nodes`, `bits` key bits in all,
 // 3..30): perm receives the queue
@@ -130,9 +133,11 @@ hipError_t launch_ray_sort(const float4* nodes, RayQueue q, int n, int bits, uin
                            int* vals, int* perm, void* temp, size_t temp_bytes, hipStream_t s);
 // `scenes` (device array, may be null): batched mode, the path's scene is
 // scenes[ps.req[pid]]; otherwise every path uses `sc`.
+// rank (null = append): write each path's shadow / MIS ray at rank[queue
+// position], holes pid -1, both queue sizes = the extension queue's.
 hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
                         RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in, bool adjoint,
-                        hipStream_t s);
+                        const int* rank, hipStream_t s);
 hipError_t launch_accumulate_batch(const BatchSeg* segs, int nseg, int max_npix, PathState ps, float4* sum,
                                    float4* sum2, int* counts, hipStream_t s);
 hipError_t launch_trace_shadow(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,

@@ -75,6 +75,7 @@ struct TraceArgs {
     const DevScene* scenes;   // batched queue modes: the ray's scene is scenes[ps.req[pid]]
     int* sub;                 // closest modes of instanced scenes: set triangle (DevPrim index) or -1
     const int* perm;          // extension queue: the order to trace its rays in (null = queue order)
+    int* rank;                // with perm: rank[queue position] = position in that order
 };
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
@@ -388,6 +389,9 @@ __global__ __launch_bounds__(kBlock) void k_trace_straight(DevScene sc, TraceArg
     uint32_t nv = 0, nt = 0, nrays = 0;
     for (int r0 = blockIdx.x * kBlock + threadIdx.x; r0 < n; r0 += gridDim.x * kBlock) {
         const int r = a.perm ? a.perm[r0] : r0;
+        if (a.rank) a.rank[r] = r0;
+        // ranked shadow / MIS queues leave holes (pid -1) for paths without that ray
+        if ((kMode == kModeShadow || kMode == kModeMis) && a.q.pid[r] < 0) continue;
         float4 ra, rb;
         trav_load<kMode>(a, r, &ra, &rb);
         Trav t;
@@ -419,6 +423,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_inst(DevScene sc, TraceArgs a)
     uint32_t nv = 0, nt = 0, nrays = 0;
     for (int r0 = blockIdx.x * kBlock + threadIdx.x; r0 < n; r0 += gridDim.x * kBlock) {
         const int r = a.perm ? a.perm[r0] : r0;
+        if (a.rank) a.rank[r] = r0;
+        if ((kMode == kModeShadow || kMode == kModeMis) && a.q.pid[r] < 0) continue;
         float4 ra, rb;
         trav_load<kMode>(a, r, &ra, &rb);
         TravI s;
@@ -465,6 +471,9 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(DevScene sc, TraceArgs 
                     int rr = base + __popcll(idle & lanemask_lt());
                     if (rr < n) {
                         r = a.perm ? a.perm[rr] : rr;
+                        if (a.rank) a.rank[r] = rr;
+                    }
+                    if (rr < n && !((kMode == kModeShadow || kMode == kModeMis) && a.q.pid[r] < 0)) {
                         float4 ra, rb;
                         trav_load<kMode>(a, r, &ra, &rb);
                         trav_begin(t, ra, rb);
@@ -544,6 +553,7 @@ static hipError_t launch_trace_mode(const DevScene& sc, const TraceArgs& a, int 
 static TraceArgs trace_args(int* counts) {
     TraceArgs a{};
     a.perm = nullptr;
+    a.rank = nullptr;
     a.counts = counts;
     a.zero[0] = a.zero[1] = a.zero[2] = -1;
     a.scenes = nullptr;
@@ -790,9 +800,11 @@ hipError_t launch_init_pool(PathState ps, int pool, int* counts, hipStream_t s) 
 // Closest hit of the extension queue.  It also clears the queue counters and
 // work cursors the following shade / shadow / MIS stages start from.
 hipError_t launch_trace_ext(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, float4* hits,
-                            int* counts, int cnt_slot, int za, int zb, int zc, const int* perm, hipStream_t s) {
+                            int* counts, int cnt_slot, int za, int zb, int zc, const int* perm, int* rank,
+                            hipStream_t s) {
     TraceArgs a = trace_args(counts);
     a.perm = perm;
+    a.rank = perm ? rank : nullptr;
     a.q = q; a.ps = ps; a.hits = hits; a.cnt_slot = cnt_slot; a.cursor_slot = kCntWork + 0; a.scenes = scenes;
     a.zero[0] = za; a.zero[1] = zb; a.zero[2] = zc;
     a.sub = ps.hsub;
@@ -834,7 +846,7 @@ hipError_t launch_trace_chain(const DevScene& sc, const DevScene* scenes, RayQue
 template <bool kBatch, bool kAdj>
 __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __restrict__ scenes, RayQueue qin,
                                                const float4* __restrict__ hits, RayQueue qshadow, RayQueue qmis,
-                                               PathState ps, int* counts, int cnt_in) {
+                                               PathState ps, int* counts, int cnt_in, const int* __restrict__ rank) {
     int n = counts[cnt_in];
     // the traversal work cursors restart at zero for the next trace stages
     if (blockIdx.x == 0 && threadIdx.x < 3) counts[kCntWork + threadIdx.x] = 0;
@@ -1020,34 +1032,53 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __r
                 ps.ray_d[pid] = make_float4(ed.x, ed.y, ed.z, etmax);
             }
         }
-        int slot = wave_append(&counts[kCntShadow], want_shadow);
-        if (want_shadow) {
-            qshadow.pid[slot] = pid;
-            qshadow.org[slot] = make_float4(so.x, so.y, so.z, 0.0001f);
-            qshadow.dir[slot] = make_float4(sd.x, sd.y, sd.z, stmax);
-        }
-        slot = wave_append(&counts[kCntMis], want_mis);
-        if (want_mis) {
-            qmis.pid[slot] = pid;
-            qmis.org[slot] = make_float4(mo.x, mo.y, mo.z, 0.0001f);
-            qmis.dir[slot] = make_float4(md.x, md.y, md.z, kFloatMax);
+        if (rank) {
+            // ranked: each path's shadow / MIS ray sits at its rank in the
+            // coherence order of the extension queue, holes marked pid -1
+            if (k < n) {
+                const int r = rank[k];
+                qshadow.pid[r] = want_shadow ? pid : -1;
+                if (want_shadow) {
+                    qshadow.org[r] = make_float4(so.x, so.y, so.z, 0.0001f);
+                    qshadow.dir[r] = make_float4(sd.x, sd.y, sd.z, stmax);
+                }
+                qmis.pid[r] = want_mis ? pid : -1;
+                if (want_mis) {
+                    qmis.org[r] = make_float4(mo.x, mo.y, mo.z, 0.0001f);
+                    qmis.dir[r] = make_float4(md.x, md.y, md.z, kFloatMax);
+                }
+            }
+        } else {
+            int slot = wave_append(&counts[kCntShadow], want_shadow);
+            if (want_shadow) {
+                qshadow.pid[slot] = pid;
+                qshadow.org[slot] = make_float4(so.x, so.y, so.z, 0.0001f);
+                qshadow.dir[slot] = make_float4(sd.x, sd.y, sd.z, stmax);
+            }
+            slot = wave_append(&counts[kCntMis], want_mis);
+            if (want_mis) {
+                qmis.pid[slot] = pid;
+                qmis.org[slot] = make_float4(mo.x, mo.y, mo.z, 0.0001f);
+                qmis.dir[slot] = make_float4(md.x, md.y, md.z, kFloatMax);
+            }
         }
     }
+    if (rank && blockIdx.x == 0 && threadIdx.x == 0) { counts[kCntShadow] = n; counts[kCntMis] = n; }
     add_counter((unsigned long long*)(counts + kCntVertices), nvert);
 }
 
 hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
                         RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in, bool adjoint,
-                        hipStream_t s) {
+                        const int* rank, hipStream_t s) {
     if (scenes)
         hipLaunchKernelGGL((k_shade<true, false>), dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
-                           qshadow, qmis, ps, counts, cnt_in);
+                           qshadow, qmis, ps, counts, cnt_in, rank);
     else if (adjoint)
         hipLaunchKernelGGL((k_shade<false, true>), dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
-                           qshadow, qmis, ps, counts, cnt_in);
+                           qshadow, qmis, ps, counts, cnt_in, rank);
     else
         hipLaunchKernelGGL((k_shade<false, false>), dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
-                           qshadow, qmis, ps, counts, cnt_in);
+                           qshadow, qmis, ps, counts, cnt_in, rank);
     return hipGetLastError();
 }
 
