@@ -505,7 +505,9 @@ int get_trace_schedule(int cls) { return (cls >= 0 && cls < 3) ? g_schedule[cls]
 
 // LDS stack variants: the DFS stack never holds more entries than the
 // deepest leaf's depth, so the smallest variant that covers the tree is
-// launched (64 = the reference's Bvh.StackSize).
+// launched (64 = the reference's Bvh.StackSize).  LDS per wavefront
+// (entries x 64 x 4 B) sets the occupancy: 24 or 26 entries give six
+// wavefronts per SIMD, 32 give five.
 template <int kMode, int kStack, int kBlock>
 static void launch_trace_variant(const DevScene& sc, const TraceArgs& a, int blocks, int schedule, hipStream_t s) {
     if (sc.num_insts > 0) {
@@ -537,6 +539,10 @@ static hipError_t launch_trace_mode(const DevScene& sc, const TraceArgs& a, int 
     if (need <= 24) {
         int b = blocks_hint > 0 ? std::min(blocks_hint, pb) : pb;
         launch_trace_variant<kMode, 24, 256>(sc, a, b, sched, s);
+    } else if (need <= 26) {
+        // 26 entries x 256 lanes x 4 B: six blocks fill the 160 KB of LDS, six waves per SIMD
+        int b = blocks_hint > 0 ? std::min(blocks_hint, pb) : pb;
+        launch_trace_variant<kMode, 26, 256>(sc, a, b, sched, s);
     } else if (need <= 32) {
         int b = blocks_hint > 0 ? std::min(blocks_hint, pb) : pb;
         launch_trace_variant<kMode, 32, 256>(sc, a, b, sched, s);
