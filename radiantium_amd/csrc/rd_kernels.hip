@@ -643,7 +643,7 @@ __device__ __forceinline__ void start_path(const DevScene& sc, const BatchSeg& s
 }
 
 // ---- ordered compaction of the pool
-// Blocks of kCompactSlots slots (256 threads x 4 slots).  count: per block the
+// Blocks of kCompactSlots slots (256 threads x 4 runs of 256 slots).  count: per block the
 // slots whose path continues and the free slots; scan: block offsets of the
 // free slots and of the queue entries (continuing slots + free slots that get
 // a new path, i.e. whose pool-wide free rank is below the paths left);
@@ -675,10 +675,10 @@ __device__ __forceinline__ int block_excl_scan(int v, int* tmp, int* total) {
 
 __global__ __launch_bounds__(256) void k_pool_count(const int* __restrict__ alive, int pool, int2* blk) {
     __shared__ int tmp[4];
-    int s0 = blockIdx.x * kCompactSlots + threadIdx.x * 4;
+    const int base = blockIdx.x * kCompactSlots;
     int na = 0, nf = 0;
-    for (int i = 0; i < 4; i++) {
-        int slot = s0 + i;
+    for (int i = 0; i < kCompactSlots / 256; i++) {   // coalesced: slot base + i * 256 + thread
+        int slot = base + i * 256 + threadIdx.x;
         if (slot < pool) {
             if (alive[slot]) na++;
             else nf++;
@@ -733,45 +733,34 @@ __global__ __launch_bounds__(256) void k_pool_scatter(DevScene sc0, const DevSce
                                                       int pool, const int2* __restrict__ off, const int* counts,
                                                       RayQueue q) {
     __shared__ int tmp[4];
-    const int s0 = blockIdx.x * kCompactSlots + threadIdx.x * 4;
+    const int base = blockIdx.x * kCompactSlots;
     const int first = counts[kCntFree];
     const int left = max(npass - first, 0);
     const int2 o = off[blockIdx.x];
-    int st[4];
-    int nf = 0;
-    for (int i = 0; i < 4; i++) {
-        int slot = s0 + i;
-        st[i] = slot < pool ? ps.alive[slot] : -1;
-        if (st[i] == 0) nf++;
-    }
-    int tf;
-    const int fpre = block_excl_scan<256>(nf, tmp, &tf);
-    int frank = o.x + fpre, ns = 0;
-    for (int i = 0; i < 4; i++) {
-        if (st[i] == 1) ns++;
-        else if (st[i] == 0) { if (frank < left) ns++; frank++; }
-    }
-    int ts;
-    int pos = o.y + block_excl_scan<256>(ns, tmp, &ts);
-    frank = o.x + fpre;
-    for (int i = 0; i < 4; i++) {
-        const int slot = s0 + i;
-        if (st[i] == 1) {
+    int frank_base = o.x, pos_base = o.y;
+    // the block's slots in four runs of 256 (coalesced accesses); within and
+    // across runs, free ranks and queue positions follow slot order
+    for (int i = 0; i < kCompactSlots / 256; i++) {
+        const int slot = base + i * 256 + threadIdx.x;
+        const int st = slot < pool ? ps.alive[slot] : -1;
+        int tf;
+        const int frank = frank_base + block_excl_scan<256>(st == 0 ? 1 : 0, tmp, &tf);
+        const bool take = st == 1 || (st == 0 && frank < left);
+        int ts;
+        const int pos = pos_base + block_excl_scan<256>(take ? 1 : 0, tmp, &ts);
+        if (st == 1) {
             q.pid[pos] = slot;
             q.org[pos] = ps.ray_o[slot];
             q.dir[pos] = ps.ray_d[slot];
-            pos++;
-        } else if (st[i] == 0) {
-            if (frank < left) {
-                const int g = first + frank;
-                const BatchSeg sg = segs[find_seg(segs, nseg, g)];
-                if constexpr (kBatch) start_path(scenes[sg.req], sg, g, slot, pos, q, ps);
-                else start_path(sc0, sg, g, slot, pos, q, ps);
-                ps.alive[slot] = 1;
-                pos++;
-            }
-            frank++;
+        } else if (take) {
+            const int g = first + frank;
+            const BatchSeg sg = segs[find_seg(segs, nseg, g)];
+            if constexpr (kBatch) start_path(scenes[sg.req], sg, g, slot, pos, q, ps);
+            else start_path(sc0, sg, g, slot, pos, q, ps);
+            ps.alive[slot] = 1;
         }
+        frank_base += tf;
+        pos_base += ts;
     }
 }
 
@@ -850,7 +839,7 @@ hipError_t launch_trace_chain(const DevScene& sc, const DevScene* scenes, RayQue
 // the texture coordinates and the roughness ratio (plastic_rho_ratio) of its
 // light-sample, BSDF-sample and continuation terms for resolve's gradients.
 template <bool kBatch, bool kAdj>
-__global__ __launch_bounds__(256) void k_shade(DevScene sc0, const DevScene* __restrict__ scenes, RayQueue qin,
+__global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* __restrict__ scenes, RayQueue qin,
                                                const float4* __restrict__ hits, RayQueue qshadow, RayQueue qmis,
                                                PathState ps, int* counts, int cnt_in, const int* __restrict__ rank) {
     int n = counts[cnt_in];

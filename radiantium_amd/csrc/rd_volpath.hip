@@ -25,7 +25,7 @@
 namespace rd {
 
 template <bool kBatch>
-__global__ __launch_bounds__(256) void k_shade_vol(DevScene sc0, const DevScene* __restrict__ scenes, RayQueue qin,
+__global__ __launch_bounds__(256, 3) void k_shade_vol(DevScene sc0, const DevScene* __restrict__ scenes, RayQueue qin,
                                                    const float4* __restrict__ hits, RayQueue qshadow, RayQueue qmis,
                                                    PathState ps, int* counts, int cnt_in) {
     int n = counts[cnt_in];
