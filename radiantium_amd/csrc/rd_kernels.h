@@ -168,7 +168,8 @@ hipError_t launch_zero_counts(int* counts, int n, hipStream_t s);
 
 // Traversal schedule per query class (0 extension, 1 shadow, 2 MIS):
 // 0 = straight (grid-stride, one ray per lane to completion),
-// 1 = refill (idle lanes refilled from an atomic cursor).
+// 1 = refill (idle lanes refilled from an atomic cursor),
+// 2 = straight with 64-ray chunks per wavefront from an atomic cursor.
 void set_trace_schedule(int cls, int schedule);
 int get_trace_schedule(int cls);
 

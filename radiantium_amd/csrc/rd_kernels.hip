@@ -13,6 +13,9 @@
 //   resolve       radiance += coeff * (le_light + le_bsdf) / lightPdf
 // The random stream is keyed by (seed, pixel, sample) and a per-path draw
 // counter, so the kernels reproduce the CPU restatement's draws exactly.
+#include <mutex>
+#include <unordered_map>
+
 #include "rd_kernels.h"
 #include "rd_shading.h"
 #include "rd_wave.h"
@@ -76,6 +79,7 @@ struct TraceArgs {
     int* sub;                 // closest modes of instanced scenes: set triangle (DevPrim index) or -1
     const int* perm;          // extension queue: the order to trace its rays in (null = queue order)
     int* rank;                // with perm: rank[queue position] = position in that order
+    int dyn;                  // straight schedule: 0 grid-stride, 1 chunks from the work cursor
 };
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
@@ -377,8 +381,19 @@ __device__ __forceinline__ int trav_count(const TraceArgs& a, bool fixed) {
     return fixed ? a.n_fixed : a.counts[a.cnt_slot];
 }
 
-// Schedule 0 ("straight"): each lane runs its ray to completion, rays are
-// taken grid-stride.  kBatch: each ray traverses its own request's scene.
+// The next 64 consecutive ranks for the calling wavefront, from the work
+// cursor (wave-uniform): wavefronts that finish early take more rays, so the
+// launch ends evenly whatever the grid and the rays' costs.
+__device__ __forceinline__ int next_chunk(int* cursor) {
+    int base = 0;
+    if (__lane_id() == 0) base = atomicAdd(cursor, 64);
+    return __shfl(base, 0);
+}
+
+// Schedule 0 / 2 ("straight"): each lane runs its ray to completion; rays
+// are taken grid-stride (0) or in chunks of 64 consecutive rays per
+// wavefront from the work cursor (2, a.dyn).  kBatch: each ray traverses its
+// own request's scene.
 template <int kMode, int kStack, int kBlock, bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_trace_straight(DevScene sc, TraceArgs a) {
     constexpr bool kAny = (kMode == kModeRaysAny || kMode == kModeShadow);
@@ -386,12 +401,14 @@ __global__ __launch_bounds__(kBlock) void k_trace_straight(DevScene sc, TraceArg
     int* stk = lds + threadIdx.x;
     const int n = trav_count(a, kMode <= kModeRaysAny);
     if (blockIdx.x == 0 && threadIdx.x < 3 && a.zero[threadIdx.x] >= 0) a.counts[a.zero[threadIdx.x]] = 0;
+    int* cursor = a.counts + a.cursor_slot;
     uint32_t nv = 0, nt = 0, nrays = 0;
-    for (int r0 = blockIdx.x * kBlock + threadIdx.x; r0 < n; r0 += gridDim.x * kBlock) {
+    // one ray per lane; the chunk fetch stays outside it, where the whole wavefront is active
+    auto one = [&](int r0) {
         const int r = a.perm ? a.perm[r0] : r0;
         if (a.rank) a.rank[r] = r0;
         // ranked shadow / MIS queues leave holes (pid -1) for paths without that ray
-        if ((kMode == kModeShadow || kMode == kModeMis) && a.q.pid[r] < 0) continue;
+        if ((kMode == kModeShadow || kMode == kModeMis) && a.q.pid[r] < 0) return;
         float4 ra, rb;
         trav_load<kMode>(a, r, &ra, &rb);
         Trav t;
@@ -407,6 +424,12 @@ __global__ __launch_bounds__(kBlock) void k_trace_straight(DevScene sc, TraceArg
             trav_finish<kMode>(sc, a, r, t);
         }
         nrays++;
+    };
+    if (a.dyn) {
+        for (int c = next_chunk(cursor); c < n; c = next_chunk(cursor))
+            if (c + __lane_id() < n) one(c + __lane_id());
+    } else {
+        for (int r0 = blockIdx.x * kBlock + threadIdx.x; r0 < n; r0 += gridDim.x * kBlock) one(r0);
     }
     trav_counters<kMode>(a, nrays, nv, nt);
 }
@@ -420,11 +443,12 @@ __global__ __launch_bounds__(kBlock) void k_trace_inst(DevScene sc, TraceArgs a)
     int* stk = lds + threadIdx.x;
     const int n = trav_count(a, kMode <= kModeRaysAny);
     if (blockIdx.x == 0 && threadIdx.x < 3 && a.zero[threadIdx.x] >= 0) a.counts[a.zero[threadIdx.x]] = 0;
+    int* cursor = a.counts + a.cursor_slot;
     uint32_t nv = 0, nt = 0, nrays = 0;
-    for (int r0 = blockIdx.x * kBlock + threadIdx.x; r0 < n; r0 += gridDim.x * kBlock) {
+    auto one = [&](int r0) {
         const int r = a.perm ? a.perm[r0] : r0;
         if (a.rank) a.rank[r] = r0;
-        if ((kMode == kModeShadow || kMode == kModeMis) && a.q.pid[r] < 0) continue;
+        if ((kMode == kModeShadow || kMode == kModeMis) && a.q.pid[r] < 0) return;
         float4 ra, rb;
         trav_load<kMode>(a, r, &ra, &rb);
         TravI s;
@@ -438,6 +462,12 @@ __global__ __launch_bounds__(kBlock) void k_trace_inst(DevScene sc, TraceArgs a)
         trav_finish<kMode>(s2, a, r, s.t);
         if (!kAny && a.sub) a.sub[r] = s.t.best.prim >= 0 ? s.sub : -1;
         nrays++;
+    };
+    if (a.dyn) {
+        for (int c = next_chunk(cursor); c < n; c = next_chunk(cursor))
+            if (c + __lane_id() < n) one(c + __lane_id());
+    } else {
+        for (int r0 = blockIdx.x * kBlock + threadIdx.x; r0 < n; r0 += gridDim.x * kBlock) one(r0);
     }
     trav_counters<kMode>(a, nrays, nv, nt);
 }
@@ -495,11 +525,14 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(DevScene sc, TraceArgs 
     trav_counters<kMode>(a, nrays, nv, nt);
 }
 
-// Traversal schedule per query class (0 straight, 1 refill): extension,
-// shadow and MIS, chosen by the autotuner.
-static int g_schedule[3] = {0, 0, 0};
+// Traversal schedule per query class (0 straight grid-stride, 1 refill,
+// 2 straight with cursor chunks): extension, shadow and MIS.  Extension and
+// shadow rays take chunks (their costs vary widely between wavefronts); the
+// MIS queue is mostly holes, whose wavefronts would hit the one cursor too
+// often for chunks, and its few rays gain from refilling idle lanes.
+static int g_schedule[3] = {2, 2, 1};
 void set_trace_schedule(int cls, int schedule) {
-    if (cls >= 0 && cls < 3) g_schedule[cls] = schedule ? 1 : 0;
+    if (cls >= 0 && cls < 3) g_schedule[cls] = schedule >= 0 && schedule <= 2 ? schedule : 0;
 }
 int get_trace_schedule(int cls) { return (cls >= 0 && cls < 3) ? g_schedule[cls] : 0; }
 
@@ -508,34 +541,58 @@ int get_trace_schedule(int cls) { return (cls >= 0 && cls < 3) ? g_schedule[cls]
 // launched (64 = the reference's Bvh.StackSize).  LDS per wavefront
 // (entries x 64 x 4 B) sets the occupancy: 24 or 26 entries give six
 // wavefronts per SIMD, 32 give five.
+// Blocks of `kernel` that the device holds at once (registers and LDS), at
+// least one per CU: grids beyond it would run as a second, partial round.
+static int resident_blocks(const void* kernel, int block) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    return per_cu * g_num_cus;
+}
+
+static void launch_resident(void (*kernel)(DevScene, TraceArgs), int blocks, int block, hipStream_t s,
+                            const DevScene& sc, const TraceArgs& a) {
+    static std::mutex mu;
+    static std::unordered_map<const void*, int> cache;   // per kernel
+    const void* key = reinterpret_cast<const void*>(kernel);
+    int res;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache.find(key);
+        if (it == cache.end()) it = cache.emplace(key, resident_blocks(key, block)).first;
+        res = it->second;
+    }
+    hipLaunchKernelGGL(kernel, dim3(std::min(blocks, res)), dim3(block), 0, s, sc, a);
+}
+
 template <int kMode, int kStack, int kBlock>
 static void launch_trace_variant(const DevScene& sc, const TraceArgs& a, int blocks, int schedule, hipStream_t s) {
     if (sc.num_insts > 0) {
         if (kMode >= kModeExt && a.scenes)
-            hipLaunchKernelGGL((k_trace_inst<kMode, kStack, kBlock, kMode >= kModeExt>), dim3(blocks), dim3(kBlock), 0,
-                               s, sc, a);
+            launch_resident(k_trace_inst<kMode, kStack, kBlock, kMode >= kModeExt>, blocks, kBlock, s, sc, a);
         else
-            hipLaunchKernelGGL((k_trace_inst<kMode, kStack, kBlock, false>), dim3(blocks), dim3(kBlock), 0, s, sc, a);
+            launch_resident(k_trace_inst<kMode, kStack, kBlock, false>, blocks, kBlock, s, sc, a);
         return;
     }
     if constexpr (kMode >= kModeExt) {
         if (a.scenes) {   // batched: straight schedule, per-ray scene
-            hipLaunchKernelGGL((k_trace_straight<kMode, kStack, kBlock, true>), dim3(blocks), dim3(kBlock), 0, s, sc, a);
+            launch_resident(k_trace_straight<kMode, kStack, kBlock, true>, blocks, kBlock, s, sc, a);
             return;
         }
     }
-    if (schedule) hipLaunchKernelGGL((k_trace_refill<kMode, kStack, kBlock>), dim3(blocks), dim3(kBlock), 0, s, sc, a);
-    else hipLaunchKernelGGL((k_trace_straight<kMode, kStack, kBlock, false>), dim3(blocks), dim3(kBlock), 0, s, sc, a);
+    if (schedule == 1) launch_resident(k_trace_refill<kMode, kStack, kBlock>, blocks, kBlock, s, sc, a);
+    else launch_resident(k_trace_straight<kMode, kStack, kBlock, false>, blocks, kBlock, s, sc, a);
 }
 
 template <int kMode>
-static hipError_t launch_trace_mode(const DevScene& sc, const TraceArgs& a, int blocks_hint, hipStream_t s) {
+static hipError_t launch_trace_mode(const DevScene& sc, const TraceArgs& a0, int blocks_hint, hipStream_t s) {
     int need = sc.stack_needed;
     int pb = persistent_blocks();
     int cls = kMode == kModeExt ? 0
               : ((kMode == kModeShadow || kMode == kModeChainShadow) ? 1
                  : ((kMode == kModeMis || kMode == kModeChainMis) ? 2 : 0));
     int sched = kMode <= kModeRaysAny ? 0 : g_schedule[cls];
+    TraceArgs a = a0;
+    a.dyn = sched == 2 ? 1 : 0;
     if (need <= 24) {
         int b = blocks_hint > 0 ? std::min(blocks_hint, pb) : pb;
         launch_trace_variant<kMode, 24, 256>(sc, a, b, sched, s);
@@ -563,6 +620,7 @@ static TraceArgs trace_args(int* counts) {
     a.counts = counts;
     a.zero[0] = a.zero[1] = a.zero[2] = -1;
     a.scenes = nullptr;
+    a.dyn = 0;
     return a;
 }
 
