@@ -352,7 +352,8 @@ def main():
         rays += st["rays_closest"] + st["rays_any"]
         for k in agg:
             agg[k] = [a + b for a, b in zip(agg[k], st[k])]
-        log(f"[bench] step {i}: {st['render_ms']:.1f} ms, {(st['rays_closest'] + st['rays_any']) / 1e6:.1f} M rays")
+        log(f"[bench] step {i}: {st['render_ms']:.1f} ms, {(st['rays_closest'] + st['rays_any']) / 1e6:.1f} M rays "
+            f"(extension / MIS / shadow: {' / '.join(f'{r / 1e6:.1f}' for r in st['class_rays'])} M)")
     ctx.synchronize()
     elapsed = time.perf_counter() - t_start
     value = rays / elapsed / 1e6

@@ -341,6 +341,27 @@ int rd_render_device(rd_ctx* ctx, rd_scene* scene, const rd_render_params* param
                      volatile int32_t* progress);
 int rd_get_render_stats(const rd_ctx* ctx, rd_render_stats* stats);
 
+/* ---- tuning of a context's wavefront ------------------------------------
+ * No counterpart in the reference (its CPU renderer has none of these
+ * choices); for autotuners and benchmarks.  None of them changes a result:
+ * every ray's query and every path's random numbers are the same under any
+ * setting.  rd_set_tuning leaves fields that are -1 as they are. */
+typedef struct rd_tuning {
+    int32_t trace_schedule[3]; /* per query class [extension, shadow, MIS]:
+                                  0 grid-stride (one ray per lane to its end),
+                                  1 idle lanes refilled from a work cursor,
+                                  2 64-ray chunks per wavefront from it     */
+    int32_t ray_sort;          /* extension rays traced in coherence order
+                                  (direction octant, origin Morton code)    */
+    int32_t ray_sort_bits;     /* sort key bits, 3..30                      */
+    int32_t ray_sort_min;      /* smallest extension queue that is sorted   */
+    int32_t ray_sort_shadow;   /* shadow rays in the same order             */
+    int32_t reserved[8];       /* 0                                         */
+} rd_tuning;
+
+int rd_get_tuning(const rd_ctx* ctx, rd_tuning* out);
+int rd_set_tuning(rd_ctx* ctx, const rd_tuning* tuning);
+
 /* ---- batched rendering: many scenes / sample counts in shared waves ----- */
 /* One request: a scene of this context, its render parameters (spp, seed,
  * crop; wave_paths and flags are ignored) and its output buffers (layout as

@@ -122,7 +122,11 @@ struct rd_ctx {
     bool ray_sort = true;
     int ray_sort_bits = 15;       // key bits (RD_RAY_SORT_BITS): octant + Morton prefix
     int ray_sort_min = 1 << 16;   // smaller queues are traced in queue order
-    bool ray_sort_shadow = true;  // shadow / MIS queues in that order too (RD_RAY_SORT_SHADOW)
+    bool ray_sort_shadow = true;  // shadow queue in that order too (RD_RAY_SORT_SHADOW)
+    // traversal schedule per query class [extension, shadow, MIS]
+    // (RD_TRACE_SCHEDULE="e,s,m"): chunks for the long extension and shadow
+    // queries, whose costs vary widely between wavefronts
+    int trace_schedule[3] = {2, 2, 2};
     // scenes created on this context and not yet destroyed; rd_destroy with
     // live scenes releases the wavefront buffers and leaves the rest to the
     // last rd_scene_destroy, so destruction order does not matter
@@ -407,10 +411,11 @@ rd_ctx* rd_create(int device) {
     rd_ctx* c = new rd_ctx();
     c->device = device;
     set_device_grid(prop.multiProcessorCount);
-    if (const char* env = getenv("RD_TRACE_SCHEDULE")) {   // "ext,shadow,mis", e.g. "0,1,0"
+    if (const char* env = getenv("RD_TRACE_SCHEDULE")) {   // "ext,shadow,mis", e.g. "2,2,1"
         int v[3] = {0, 0, 0};
         if (sscanf(env, "%d,%d,%d", &v[0], &v[1], &v[2]) == 3)
-            for (int k = 0; k < 3; k++) set_trace_schedule(k, v[k]);
+            for (int k = 0; k < 3; k++)
+                if (v[k] >= 0 && v[k] < kNumSchedules) c->trace_schedule[k] = v[k];
     }
     if (const char* env = getenv("RD_RAY_SORT")) c->ray_sort = atoi(env) != 0;
     if (const char* env = getenv("RD_RAY_SORT_SHADOW")) c->ray_sort_shadow = atoi(env) != 0;
@@ -1039,7 +1044,7 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
         }
         HIP_TRY(tm.run("trace_closest", [&] {
             return launch_trace_ext(L.sc, L.scenes, qin, wb.ps, wb.hits, c->counts, cin, cout, kCntShadow, kCntMis,
-                                    perm, wb.sort_rank, st);
+                                    perm, wb.sort_rank, c->trace_schedule[0], st);
         }));
         // with a sorted extension queue the shadow / MIS rays follow its order
         const int* rank = (perm && c->ray_sort_shadow) ? wb.sort_rank : nullptr;
@@ -1054,10 +1059,12 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
             for (int step = 0;; step++) {
                 if (step >= kMaxChainSteps) return fail(RD_ERR_DEVICE, "medium chain did not terminate");
                 HIP_TRY(tm.run("trace_chain", [&] {
-                    return launch_trace_chain(L.sc, L.scenes, sa, wb.ps, wb.hits, false, c->counts, csa, csb, st);
+                    return launch_trace_chain(L.sc, L.scenes, sa, wb.ps, wb.hits, false, c->counts, csa, csb,
+                                              c->trace_schedule[1], st);
                 }));
                 HIP_TRY(tm.run("trace_chain", [&] {
-                    return launch_trace_chain(L.sc, L.scenes, ma, wb.ps, wb.hits2, true, c->counts, cma, cmb, st);
+                    return launch_trace_chain(L.sc, L.scenes, ma, wb.ps, wb.hits2, true, c->counts, cma, cmb,
+                                              c->trace_schedule[2], st);
                 }));
                 HIP_TRY(tm.run("chain_advance", [&] {
                     return launch_chain_advance(L.sc, L.scenes, sa, wb.hits, sb, false, wb.ps, c->counts, csa, csb, st);
@@ -1082,11 +1089,13 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
             }));
             if (L.method != RD_METHOD_BXDF) {
                 HIP_TRY(tm.run("trace_any", [&] {
-                    return launch_trace_shadow(L.sc, L.scenes, wb.qshadow, wb.ps, c->counts, st);
+                    return launch_trace_shadow(L.sc, L.scenes, wb.qshadow, wb.ps, c->counts, c->trace_schedule[1],
+                                               st);
                 }));
                 if (L.method == RD_METHOD_MIS)
                     HIP_TRY(tm.run("trace_mis", [&] {
-                        return launch_trace_mis(L.sc, L.scenes, wb.qmis, wb.ps, c->counts, st);
+                        return launch_trace_mis(L.sc, L.scenes, wb.qmis, wb.ps, c->counts, c->trace_schedule[2],
+                                                st);
                     }));
             }
             HIP_TRY(tm.run(L.adj ? "resolve_adjoint" : "resolve", [&] {
@@ -1532,6 +1541,36 @@ int rd_render(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* out_rgb,
 int rd_get_render_stats(const rd_ctx* c, rd_render_stats* st) {
     if (!c || !st) return fail(RD_ERR_INVALID, "null argument");
     *st = c->stats;
+    return RD_OK;
+}
+
+int rd_get_tuning(const rd_ctx* c, rd_tuning* t) {
+    if (!c || !t) return fail(RD_ERR_INVALID, "null argument");
+    *t = rd_tuning{};
+    for (int k = 0; k < 3; k++) t->trace_schedule[k] = c->trace_schedule[k];
+    t->ray_sort = c->ray_sort ? 1 : 0;
+    t->ray_sort_bits = c->ray_sort_bits;
+    t->ray_sort_min = c->ray_sort_min;
+    t->ray_sort_shadow = c->ray_sort_shadow ? 1 : 0;
+    return RD_OK;
+}
+
+int rd_set_tuning(rd_ctx* c, const rd_tuning* t) {
+    if (!c || !t) return fail(RD_ERR_INVALID, "null argument");
+    for (int k = 0; k < 3; k++)
+        if (t->trace_schedule[k] < -1 || t->trace_schedule[k] >= kNumSchedules)
+            return fail(RD_ERR_INVALID, "trace_schedule: 0, 1, 2 or -1");
+    if (t->ray_sort_bits != -1 && (t->ray_sort_bits < 3 || t->ray_sort_bits > 30))
+        return fail(RD_ERR_INVALID, "ray_sort_bits: 3..30 or -1");
+    if (t->ray_sort_min < -1) return fail(RD_ERR_INVALID, "ray_sort_min: >= 0 or -1");
+    CtxLock lock(c);
+    if (!lock.ok()) return fail(RD_ERR_BUSY, "context in use by another thread");
+    for (int k = 0; k < 3; k++)
+        if (t->trace_schedule[k] >= 0) c->trace_schedule[k] = t->trace_schedule[k];
+    if (t->ray_sort >= 0) c->ray_sort = t->ray_sort != 0;
+    if (t->ray_sort_bits >= 0) c->ray_sort_bits = t->ray_sort_bits;
+    if (t->ray_sort_min >= 0) c->ray_sort_min = t->ray_sort_min;
+    if (t->ray_sort_shadow >= 0) c->ray_sort_shadow = t->ray_sort_shadow != 0;
     return RD_OK;
 }
 

@@ -121,7 +121,7 @@ hipError_t launch_compact(const DevScene& sc, const DevScene* scenes, const Batc
 // perm: trace order (null = queue order); rank (with perm) receives each
 // queue position's place in that order.
 hipError_t launch_trace_ext(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, float4* hits,
-                            int* counts, int cnt_slot, int za, int zb, int zc, const int* perm, int* rank,
+                            int* counts, int cnt_slot, int za, int zb, int zc, const int* perm, int* rank, int sched,
                             hipStream_t s);
 // Coherence order of the first n rays of q (direction octant, then the
 // origin's Morton code in the root box of `nodes`, `bits` key bits in all,
@@ -133,17 +133,18 @@ hipError_t launch_ray_sort(const float4* nodes, RayQueue q, int n, int bits, uin
                            int* vals, int* perm, void* temp, size_t temp_bytes, hipStream_t s);
 // `scenes` (device array, may be null): batched mode, the path's scene is
 // scenes[ps.req[pid]]; otherwise every path uses `sc`.
-// rank (null = append): write each path's shadow / MIS ray at rank[queue
-// position], holes pid -1, both queue sizes = the extension queue's.
+// rank (null = append): write each path's shadow ray at rank[queue
+// position], holes pid -1, shadow queue size = the extension queue's; MIS
+// rays are appended either way.
 hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
                         RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in, bool adjoint,
                         const int* rank, hipStream_t s);
 hipError_t launch_accumulate_batch(const BatchSeg* segs, int nseg, int max_npix, PathState ps, float4* sum,
                                    float4* sum2, int* counts, hipStream_t s);
 hipError_t launch_trace_shadow(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,
-                               hipStream_t s);
+                               int sched, hipStream_t s);
 hipError_t launch_trace_mis(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,
-                            hipStream_t s);
+                            int sched, hipStream_t s);
 // adj: null for rendering; the adjoint replay's gradient inputs otherwise.
 hipError_t launch_resolve(const DevScene& sc, int method, RayQueue qin, PathState ps, int* counts, int cnt_in,
                           const AdjArgs* adj, hipStream_t s);
@@ -156,7 +157,7 @@ hipError_t launch_resolve(const DevScene& sc, int method, RayQueue qin, PathStat
 hipError_t launch_shade_vol(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
                             RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in, hipStream_t s);
 hipError_t launch_trace_chain(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, float4* hits,
-                              bool mis, int* counts, int cnt_slot, int zero_slot, hipStream_t s);
+                              bool mis, int* counts, int cnt_slot, int zero_slot, int sched, hipStream_t s);
 hipError_t launch_chain_advance(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
                                 RayQueue qout, bool mis, PathState ps, int* counts, int cnt_in, int cnt_out,
                                 hipStream_t s);
@@ -166,12 +167,12 @@ hipError_t launch_finalize(const WaveParams& wp, int height, int spp, const floa
 hipError_t launch_zero(float4* buf, int n, hipStream_t s);
 hipError_t launch_zero_counts(int* counts, int n, hipStream_t s);
 
-// Traversal schedule per query class (0 extension, 1 shadow, 2 MIS):
+// Traversal schedules (the `sched` argument of the queue launchers):
 // 0 = straight (grid-stride, one ray per lane to completion),
 // 1 = refill (idle lanes refilled from an atomic cursor),
 // 2 = straight with 64-ray chunks per wavefront from an atomic cursor.
-void set_trace_schedule(int cls, int schedule);
-int get_trace_schedule(int cls);
+// The batched and instanced kernels run straight (0 or 2).
+constexpr int kNumSchedules = 3;
 
 // Occupancy-sized persistent grid for the device.
 void set_device_grid(int num_cus);

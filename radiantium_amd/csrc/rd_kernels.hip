@@ -407,8 +407,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_straight(DevScene sc, TraceArg
     auto one = [&](int r0) {
         const int r = a.perm ? a.perm[r0] : r0;
         if (a.rank) a.rank[r] = r0;
-        // ranked shadow / MIS queues leave holes (pid -1) for paths without that ray
-        if ((kMode == kModeShadow || kMode == kModeMis) && a.q.pid[r] < 0) return;
+        // ranked shadow queues leave holes (pid -1) for paths without a shadow ray
+        if (kMode == kModeShadow && a.q.pid[r] < 0) return;
         float4 ra, rb;
         trav_load<kMode>(a, r, &ra, &rb);
         Trav t;
@@ -448,7 +448,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_inst(DevScene sc, TraceArgs a)
     auto one = [&](int r0) {
         const int r = a.perm ? a.perm[r0] : r0;
         if (a.rank) a.rank[r] = r0;
-        if ((kMode == kModeShadow || kMode == kModeMis) && a.q.pid[r] < 0) return;
+        if (kMode == kModeShadow && a.q.pid[r] < 0) return;
         float4 ra, rb;
         trav_load<kMode>(a, r, &ra, &rb);
         TravI s;
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(DevScene sc, TraceArgs 
                         r = a.perm ? a.perm[rr] : rr;
                         if (a.rank) a.rank[r] = rr;
                     }
-                    if (rr < n && !((kMode == kModeShadow || kMode == kModeMis) && a.q.pid[r] < 0)) {
+                    if (rr < n && !(kMode == kModeShadow && a.q.pid[r] < 0)) {
                         float4 ra, rb;
                         trav_load<kMode>(a, r, &ra, &rb);
                         trav_begin(t, ra, rb);
@@ -513,7 +513,10 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(DevScene sc, TraceArgs 
                 }
             }
         }
-        if (__ballot(active) == 0) break;
+        if (__ballot(active) == 0) {
+            if (drained) break;
+            continue;   // a chunk of holes only (ranked shadow queue): take the next one
+        }
         if (!active) continue;
         // a few steps between refill checks
         bool done = false;
@@ -525,16 +528,6 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(DevScene sc, TraceArgs 
     trav_counters<kMode>(a, nrays, nv, nt);
 }
 
-// Traversal schedule per query class (0 straight grid-stride, 1 refill,
-// 2 straight with cursor chunks): extension, shadow and MIS.  Extension and
-// shadow rays take chunks (their costs vary widely between wavefronts); the
-// MIS queue is mostly holes, whose wavefronts would hit the one cursor too
-// often for chunks, and its few rays gain from refilling idle lanes.
-static int g_schedule[3] = {2, 2, 1};
-void set_trace_schedule(int cls, int schedule) {
-    if (cls >= 0 && cls < 3) g_schedule[cls] = schedule >= 0 && schedule <= 2 ? schedule : 0;
-}
-int get_trace_schedule(int cls) { return (cls >= 0 && cls < 3) ? g_schedule[cls] : 0; }
 
 // LDS stack variants: the DFS stack never holds more entries than the
 // deepest leaf's depth, so the smallest variant that covers the tree is
@@ -584,13 +577,11 @@ static void launch_trace_variant(const DevScene& sc, const TraceArgs& a, int blo
 }
 
 template <int kMode>
-static hipError_t launch_trace_mode(const DevScene& sc, const TraceArgs& a0, int blocks_hint, hipStream_t s) {
+static hipError_t launch_trace_mode(const DevScene& sc, const TraceArgs& a0, int blocks_hint, int sched,
+                                    hipStream_t s) {
     int need = sc.stack_needed;
     int pb = persistent_blocks();
-    int cls = kMode == kModeExt ? 0
-              : ((kMode == kModeShadow || kMode == kModeChainShadow) ? 1
-                 : ((kMode == kModeMis || kMode == kModeChainMis) ? 2 : 0));
-    int sched = kMode <= kModeRaysAny ? 0 : g_schedule[cls];
+    if (kMode <= kModeRaysAny || sched < 0 || sched >= kNumSchedules) sched = 0;
     TraceArgs a = a0;
     a.dyn = sched == 2 ? 1 : 0;
     if (need <= 24) {
@@ -636,7 +627,8 @@ hipError_t launch_trace_rays(const DevScene& sc, const float4* rays, int n, floa
     a.rays = rays; a.hits = hits; a.occ = occ; a.counters = counters; a.n_fixed = n; a.sub = sub;
     a.cursor_slot = kCntWork + 3;
     int blocks = (n + 255) / 256;
-    return any ? launch_trace_mode<kModeRaysAny>(sc, a, blocks, s) : launch_trace_mode<kModeRaysClosest>(sc, a, blocks, s);
+    return any ? launch_trace_mode<kModeRaysAny>(sc, a, blocks, 0, s)
+               : launch_trace_mode<kModeRaysClosest>(sc, a, blocks, 0, s);
 }
 // ---------------------------------------------------------------- wavefront
 // Vector3.Transform / TransformNormal with a row-major 4x4 (row-vector convention).
@@ -853,7 +845,7 @@ hipError_t launch_init_pool(PathState ps, int pool, int* counts, hipStream_t s) 
 // Closest hit of the extension queue.  It also clears the queue counters and
 // work cursors the following shade / shadow / MIS stages start from.
 hipError_t launch_trace_ext(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, float4* hits,
-                            int* counts, int cnt_slot, int za, int zb, int zc, const int* perm, int* rank,
+                            int* counts, int cnt_slot, int za, int zb, int zc, const int* perm, int* rank, int sched,
                             hipStream_t s) {
     TraceArgs a = trace_args(counts);
     a.perm = perm;
@@ -861,34 +853,35 @@ hipError_t launch_trace_ext(const DevScene& sc, const DevScene* scenes, RayQueue
     a.q = q; a.ps = ps; a.hits = hits; a.cnt_slot = cnt_slot; a.cursor_slot = kCntWork + 0; a.scenes = scenes;
     a.zero[0] = za; a.zero[1] = zb; a.zero[2] = zc;
     a.sub = ps.hsub;
-    return launch_trace_mode<kModeExt>(sc, a, 0, s);
+    return launch_trace_mode<kModeExt>(sc, a, 0, sched, s);
 }
 
 // Any hit of the shadow queue: Scene.IsOccluded (Scene.cs:58-70).
 hipError_t launch_trace_shadow(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,
-                               hipStream_t s) {
+                               int sched, hipStream_t s) {
     TraceArgs a = trace_args(counts);
     a.q = q; a.ps = ps; a.cnt_slot = kCntShadow; a.cursor_slot = kCntWork + 1; a.scenes = scenes;
-    return launch_trace_mode<kModeShadow>(sc, a, 0, s);
+    return launch_trace_mode<kModeShadow>(sc, a, 0, sched, s);
 }
 
 // Closest hit of the BSDF-sampled MIS rays and the radiance they carry.
 hipError_t launch_trace_mis(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,
-                            hipStream_t s) {
+                            int sched, hipStream_t s) {
     TraceArgs a = trace_args(counts);
     a.q = q; a.ps = ps; a.cnt_slot = kCntMis; a.cursor_slot = kCntWork + 2; a.scenes = scenes;
-    return launch_trace_mode<kModeMis>(sc, a, 0, s);
+    return launch_trace_mode<kModeMis>(sc, a, 0, sched, s);
 }
 // One closest-hit step of the vol_path Transmittance (mis = false) or
 // IntersectTr (mis = true) chains; clears the next chain queue's counter.
 hipError_t launch_trace_chain(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, float4* hits,
-                              bool mis, int* counts, int cnt_slot, int zero_slot, hipStream_t s) {
+                              bool mis, int* counts, int cnt_slot, int zero_slot, int sched, hipStream_t s) {
     TraceArgs a = trace_args(counts);
     a.q = q; a.ps = ps; a.hits = hits; a.cnt_slot = cnt_slot; a.cursor_slot = kCntWork + (mis ? 2 : 1);
     a.sub = mis ? ps.hsub2 : ps.hsub;
     a.scenes = scenes;
     a.zero[0] = zero_slot;
-    return mis ? launch_trace_mode<kModeChainMis>(sc, a, 0, s) : launch_trace_mode<kModeChainShadow>(sc, a, 0, s);
+    return mis ? launch_trace_mode<kModeChainMis>(sc, a, 0, sched, s)
+               : launch_trace_mode<kModeChainShadow>(sc, a, 0, sched, s);
 }
 
 // One bounce of PathTracer.Mis / NextEventEstimation / OnlySampleBxdf for
@@ -1086,19 +1079,14 @@ __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* 
             }
         }
         if (rank) {
-            // ranked: each path's shadow / MIS ray sits at its rank in the
-            // coherence order of the extension queue, holes marked pid -1
+            // ranked: each path's shadow ray sits at its rank in the coherence
+            // order of the extension queue, holes marked pid -1
             if (k < n) {
                 const int r = rank[k];
                 qshadow.pid[r] = want_shadow ? pid : -1;
                 if (want_shadow) {
                     qshadow.org[r] = make_float4(so.x, so.y, so.z, 0.0001f);
                     qshadow.dir[r] = make_float4(sd.x, sd.y, sd.z, stmax);
-                }
-                qmis.pid[r] = want_mis ? pid : -1;
-                if (want_mis) {
-                    qmis.org[r] = make_float4(mo.x, mo.y, mo.z, 0.0001f);
-                    qmis.dir[r] = make_float4(md.x, md.y, md.z, kFloatMax);
                 }
             }
         } else {
@@ -1108,15 +1096,17 @@ __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* 
                 qshadow.org[slot] = make_float4(so.x, so.y, so.z, 0.0001f);
                 qshadow.dir[slot] = make_float4(sd.x, sd.y, sd.z, stmax);
             }
-            slot = wave_append(&counts[kCntMis], want_mis);
-            if (want_mis) {
-                qmis.pid[slot] = pid;
-                qmis.org[slot] = make_float4(mo.x, mo.y, mo.z, 0.0001f);
-                qmis.dir[slot] = make_float4(md.x, md.y, md.z, kFloatMax);
-            }
+        }
+        // MIS rays (only those that can reach the selected light) are few:
+        // appended, so their queue holds no holes
+        int slot = wave_append(&counts[kCntMis], want_mis);
+        if (want_mis) {
+            qmis.pid[slot] = pid;
+            qmis.org[slot] = make_float4(mo.x, mo.y, mo.z, 0.0001f);
+            qmis.dir[slot] = make_float4(md.x, md.y, md.z, kFloatMax);
         }
     }
-    if (rank && blockIdx.x == 0 && threadIdx.x == 0) { counts[kCntShadow] = n; counts[kCntMis] = n; }
+    if (rank && blockIdx.x == 0 && threadIdx.x == 0) counts[kCntShadow] = n;
     add_counter((unsigned long long*)(counts + kCntVertices), nvert);
 }
 

@@ -22,7 +22,7 @@ EXPORTS = [
     "rd_synchronize", "rd_kernel_timing", "rd_reset_kernel_timing", "rd_render_batch", "rd_render_batch_device",
     "rd_adjoint", "rd_adjoint_device", "rd_scene_set_texture", "rd_scene_set_texture_device",
     "rd_trace_closest_sub", "rd_json_load", "rd_json_scene_desc", "rd_json_scene_free", "rd_scene_load_json",
-    "rd_metal_eta_k",
+    "rd_metal_eta_k", "rd_get_tuning", "rd_set_tuning",
 ]
 
 
@@ -89,6 +89,8 @@ def lib():
         L.rd_scene_load_json.restype = vp
         L.rd_scene_load_json.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(abi.rd_render_params)]
         L.rd_metal_eta_k.argtypes = [C.c_char_p, vp, vp]
+        L.rd_get_tuning.argtypes = [vp, C.POINTER(abi.rd_tuning)]
+        L.rd_set_tuning.argtypes = [vp, C.POINTER(abi.rd_tuning)]
         _lib = L
     return _lib
 
@@ -235,6 +237,25 @@ class Context:
 
     def reset_kernel_timing(self):
         check(lib().rd_reset_kernel_timing(self.h))
+
+    def tuning(self):
+        """The context's wavefront tuning (rd_get_tuning) as a dict."""
+        t = abi.rd_tuning()
+        check(lib().rd_get_tuning(self.h, C.byref(t)))
+        return {"trace_schedule": list(t.trace_schedule), "ray_sort": t.ray_sort, "ray_sort_bits": t.ray_sort_bits,
+                "ray_sort_min": t.ray_sort_min, "ray_sort_shadow": t.ray_sort_shadow}
+
+    def set_tuning(self, trace_schedule=None, ray_sort=None, ray_sort_bits=None, ray_sort_min=None,
+                   ray_sort_shadow=None):
+        """rd_set_tuning; arguments left None keep their value."""
+        t = abi.rd_tuning()
+        sched = trace_schedule if trace_schedule is not None else (-1, -1, -1)
+        for i in range(3):
+            t.trace_schedule[i] = -1 if sched[i] is None else int(sched[i])
+        for f, v in (("ray_sort", ray_sort), ("ray_sort_bits", ray_sort_bits), ("ray_sort_min", ray_sort_min),
+                     ("ray_sort_shadow", ray_sort_shadow)):
+            setattr(t, f, -1 if v is None else int(v))
+        check(lib().rd_set_tuning(self.h, C.byref(t)))
 
     def render_stats(self):
         st = abi.rd_render_stats()

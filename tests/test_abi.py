@@ -24,6 +24,11 @@ def test_every_declared_symbol_is_exported():
     assert not missing, missing
     # the Python binding's list covers the header
     assert set(names) <= set(native.EXPORTS), set(names) - set(native.EXPORTS)
+    # every entry point that takes arguments has its ctypes signature declared
+    # (an undeclared one would pass 64-bit handles as 32-bit ints)
+    no_args = {"rd_abi_version", "rd_last_error"}
+    undeclared = [n for n in names if n not in no_args and getattr(lib, n).argtypes is None]
+    assert not undeclared, undeclared
 
 
 def test_abi_version_and_struct_sizes():
