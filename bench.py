@@ -29,8 +29,10 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per query: ray in (32) + hit out (16 closest, 4 any-hit)
 # + 32 per node visited + 36 per triangle tested
 RAY_IN = 32
-HIT_OUT = {"trace_closest": 16, "trace_mis": 16, "trace_any": 4}
-CLASS = {"trace_closest": 0, "trace_mis": 1, "trace_any": 2}
+# traversal stages -> (query class, hit-out bytes) of the rays they trace;
+# classes as in rd_render_stats: 0 extension, 1 MIS, 2 shadow
+TRACE_STAGES = {"trace_closest": [(0, 16)], "trace_mis": [(1, 16)], "trace_any": [(2, 4)],
+                "trace_secondary": [(1, 16), (2, 4)]}
 
 
 def log(*a):
@@ -353,33 +355,38 @@ def main():
         for k in agg:
             agg[k] = [a + b for a, b in zip(agg[k], st[k])]
         log(f"[bench] step {i}: {st['render_ms']:.1f} ms, {(st['rays_closest'] + st['rays_any']) / 1e6:.1f} M rays "
-            f"(extension / MIS / shadow: {' / '.join(f'{r / 1e6:.1f}' for r in st['class_rays'])} M)")
+            f"(extension / MIS / shadow: {' / '.join(f'{r / 1e6:.1f}' for r in st['class_rays'])} M, nodes per ray "
+            f"{' / '.join(f'{n / max(r, 1):.1f}' for n, r in zip(st['class_nodes'], st['class_rays']))})")
     ctx.synchronize()
     elapsed = time.perf_counter() - t_start
     value = rays / elapsed / 1e6
 
     stages = {}
-    for name in ("compact", "sort", "trace_closest", "shade", "trace_any", "trace_mis", "resolve", "accumulate",
-                 "finalize"):
+    for name in ("compact", "sort", "trace_closest", "shade", "trace_secondary", "trace_any", "trace_mis",
+                 "resolve", "accumulate", "finalize"):
         avg, cnt = ctx.kernel_timing(name)
+        if not cnt:
+            continue
         stages[name] = (avg, cnt)
-        log(f"[bench] stage {name:14s} avg {avg * 1000:9.1f} us x {cnt:5d} = {avg * cnt:9.1f} ms")
+        log(f"[bench] stage {name:15s} avg {avg * 1000:9.1f} us x {cnt:5d} = {avg * cnt:9.1f} ms")
     dom = max(stages, key=lambda k: stages[k][0] * stages[k][1])
     roofline = None
-    trace_dom = dom if dom in CLASS else max(CLASS, key=lambda k: stages[k][0] * stages[k][1])
-    c = CLASS[trace_dom]
+    traced = [k for k in TRACE_STAGES if k in stages]
+    trace_dom = dom if dom in TRACE_STAGES else max(traced, key=lambda k: stages[k][0] * stages[k][1])
     avg_ms, cnt = stages[trace_dom]
     if cnt:
-        total_bytes = (agg["class_rays"][c] * (RAY_IN + HIT_OUT[trace_dom]) + 32 * agg["class_nodes"][c]
-                       + 36 * agg["class_prims"][c])
+        cls = TRACE_STAGES[trace_dom]
+        total_bytes = sum(agg["class_rays"][c] * (RAY_IN + out) + 32 * agg["class_nodes"][c]
+                          + 36 * agg["class_prims"][c] for c, out in cls)
+        rays_c = sum(agg["class_rays"][c] for c, _ in cls)
         per_launch = total_bytes / cnt
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
         roofline = {"kernel": trace_dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": pmc_traffic(trace_dom, args),
                     "bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg_ms, 4),
-                    "nodes_per_ray": round(agg["class_nodes"][c] / max(agg["class_rays"][c], 1), 2),
-                    "prims_per_ray": round(agg["class_prims"][c] / max(agg["class_rays"][c], 1), 2)}
+                    "nodes_per_ray": round(sum(agg["class_nodes"][c] for c, _ in cls) / max(rays_c, 1), 2),
+                    "prims_per_ray": round(sum(agg["class_prims"][c] for c, _ in cls) / max(rays_c, 1), 2)}
     if dom != trace_dom and roofline is not None:
         roofline["note"] = f"longest stage is {dom}; roofline reported for the longest traversal kernel"
 

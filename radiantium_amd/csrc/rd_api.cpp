@@ -1087,16 +1087,16 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
                 return launch_shade(L.sc, L.scenes, qin, wb.hits, wb.qshadow, wb.qmis, wb.ps, c->counts, cin,
                                     L.adj != nullptr, rank, st);
             }));
-            if (L.method != RD_METHOD_BXDF) {
+            if (L.method == RD_METHOD_MIS) {
+                HIP_TRY(tm.run("trace_secondary", [&] {
+                    return launch_trace_secondary(L.sc, L.scenes, wb.qshadow, wb.qmis, wb.ps, c->counts,
+                                                  c->trace_schedule[1], c->trace_schedule[2], st);
+                }));
+            } else if (L.method == RD_METHOD_NEE) {
                 HIP_TRY(tm.run("trace_any", [&] {
                     return launch_trace_shadow(L.sc, L.scenes, wb.qshadow, wb.ps, c->counts, c->trace_schedule[1],
                                                st);
                 }));
-                if (L.method == RD_METHOD_MIS)
-                    HIP_TRY(tm.run("trace_mis", [&] {
-                        return launch_trace_mis(L.sc, L.scenes, wb.qmis, wb.ps, c->counts, c->trace_schedule[2],
-                                                st);
-                    }));
             }
             HIP_TRY(tm.run(L.adj ? "resolve_adjoint" : "resolve", [&] {
                 return launch_resolve(L.sc, L.method, qin, wb.ps, c->counts, cin, L.adj, st);

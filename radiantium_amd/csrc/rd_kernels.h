@@ -145,6 +145,9 @@ hipError_t launch_trace_shadow(const DevScene& sc, const DevScene* scenes, RayQu
                                int sched, hipStream_t s);
 hipError_t launch_trace_mis(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,
                             int sched, hipStream_t s);
+// Both of the above in one launch where the schedules allow it.
+hipError_t launch_trace_secondary(const DevScene& sc, const DevScene* scenes, RayQueue qshadow, RayQueue qmis,
+                                  PathState ps, int* counts, int sched_shadow, int sched_mis, hipStream_t s);
 // adj: null for rendering; the adjoint replay's gradient inputs otherwise.
 hipError_t launch_resolve(const DevScene& sc, int method, RayQueue qin, PathState ps, int* counts, int cnt_in,
                           const AdjArgs* adj, hipStream_t s);

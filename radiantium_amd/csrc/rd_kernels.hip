@@ -59,7 +59,8 @@ struct Hit {
 enum : int {
     kModeRaysClosest = 0, kModeRaysAny = 1, kModeExt = 2, kModeShadow = 3, kModeMis = 4,
     kModeChainShadow = 5,   // vol_path Transmittance segment: closest hit into hits[]
-    kModeChainMis = 6       // vol_path IntersectTr segment: closest hit into hits[]
+    kModeChainMis = 6,      // vol_path IntersectTr segment: closest hit into hits[]
+    kModeSecondary = 7      // MIS rays (q, closest) then shadow rays (q2, any hit) in one launch
 };
 constexpr int kRefill = 16;
 
@@ -80,6 +81,8 @@ struct TraceArgs {
     const int* perm;          // extension queue: the order to trace its rays in (null = queue order)
     int* rank;                // with perm: rank[queue position] = position in that order
     int dyn;                  // straight schedule: 0 grid-stride, 1 chunks from the work cursor
+    RayQueue q2;              // kModeSecondary: the shadow queue (q = the MIS queue)
+    int cnt_slot2;            // kModeSecondary: its count slot
 };
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
@@ -472,6 +475,52 @@ __global__ __launch_bounds__(kBlock) void k_trace_inst(DevScene sc, TraceArgs a)
     trav_counters<kMode>(a, nrays, nv, nt);
 }
 
+// MIS and shadow rays of one bounce in one launch: the concatenated queue
+// [MIS rays | shadow rays] is taken like a straight schedule, so the MIS
+// rays (few, often long) run beside the shadow rays instead of leaving the
+// chip idle behind their longest one.  Each ray's query and result are those
+// of the separate MIS (closest hit + Le) and shadow (any hit) launches.
+template <int kStack, int kBlock, bool kBatch>
+__global__ __launch_bounds__(kBlock) void k_trace_secondary(DevScene sc, TraceArgs a) {
+    __shared__ int lds[kStack * kBlock];
+    int* stk = lds + threadIdx.x;
+    const int n1 = a.counts[a.cnt_slot];
+    const int n = n1 + a.counts[a.cnt_slot2];
+    int* cursor = a.counts + a.cursor_slot;
+    uint32_t nv_m = 0, nt_m = 0, nr_m = 0, nv_s = 0, nt_s = 0, nr_s = 0;
+    auto one = [&](int r0) {
+        const bool mis = r0 < n1;
+        const int r = mis ? r0 : r0 - n1;
+        const RayQueue& q = mis ? a.q : a.q2;
+        const int pid = q.pid[r];
+        if (pid < 0) return;   // a hole of the ranked shadow queue
+        Trav t;
+        trav_begin(t, q.org[r], q.dir[r]);
+        const DevScene& s2 = kBatch ? a.scenes[a.ps.req[pid]] : sc;
+        const float4* nodes = s2.nodes;
+        const float4* prims = s2.prims;
+        if (mis) {
+            while (!trav_step<false, kBlock>(nodes, prims, t, stk, nv_m, nt_m)) {}
+            trav_finish<kModeMis>(s2, a, r, t);
+            nr_m++;
+        } else {
+            while (!trav_step<true, kBlock>(nodes, prims, t, stk, nv_s, nt_s)) {}
+            a.ps.vis[pid] = t.best.prim >= 0 ? 0 : 1;
+            nr_s++;
+        }
+    };
+    if (a.dyn) {
+        for (int c = next_chunk(cursor); c < n; c = next_chunk(cursor))
+            if (c + __lane_id() < n) one(c + __lane_id());
+    } else {
+        for (int r0 = blockIdx.x * kBlock + threadIdx.x; r0 < n; r0 += gridDim.x * kBlock) one(r0);
+    }
+    add_class_counters(a.counts, 1, nr_m, nv_m, nt_m);
+    add_counter((unsigned long long*)(a.counts + kCntRaysClosest), nr_m);
+    add_class_counters(a.counts, 2, nr_s, nv_s, nt_s);
+    add_counter((unsigned long long*)(a.counts + kCntRaysAny), nr_s);
+}
+
 // Schedule 1 ("refill"): lanes that finish are refilled from an atomic ray
 // cursor once at least kRefill lanes of the wavefront are idle.
 template <int kMode, int kStack, int kBlock>
@@ -564,6 +613,11 @@ static void launch_trace_variant(const DevScene& sc, const TraceArgs& a, int blo
             launch_resident(k_trace_inst<kMode, kStack, kBlock, kMode >= kModeExt>, blocks, kBlock, s, sc, a);
         else
             launch_resident(k_trace_inst<kMode, kStack, kBlock, false>, blocks, kBlock, s, sc, a);
+        return;
+    }
+    if constexpr (kMode == kModeSecondary) {
+        if (a.scenes) launch_resident(k_trace_secondary<kStack, kBlock, true>, blocks, kBlock, s, sc, a);
+        else launch_resident(k_trace_secondary<kStack, kBlock, false>, blocks, kBlock, s, sc, a);
         return;
     }
     if constexpr (kMode >= kModeExt) {
@@ -862,6 +916,22 @@ hipError_t launch_trace_shadow(const DevScene& sc, const DevScene* scenes, RayQu
     TraceArgs a = trace_args(counts);
     a.q = q; a.ps = ps; a.cnt_slot = kCntShadow; a.cursor_slot = kCntWork + 1; a.scenes = scenes;
     return launch_trace_mode<kModeShadow>(sc, a, 0, sched, s);
+}
+
+// The MIS rays (closest hit + Le) and the shadow rays (any hit) of a bounce:
+// one fused launch on plain scenes with a straight schedule, else the two
+// launches in turn.
+hipError_t launch_trace_secondary(const DevScene& sc, const DevScene* scenes, RayQueue qshadow, RayQueue qmis,
+                                  PathState ps, int* counts, int sched_shadow, int sched_mis, hipStream_t s) {
+    if (sc.num_insts > 0 || sched_shadow == 1 || sched_mis == 1) {
+        if (hipError_t e = launch_trace_mis(sc, scenes, qmis, ps, counts, sched_mis, s)) return e;
+        return launch_trace_shadow(sc, scenes, qshadow, ps, counts, sched_shadow, s);
+    }
+    TraceArgs a = trace_args(counts);
+    a.q = qmis; a.cnt_slot = kCntMis;
+    a.q2 = qshadow; a.cnt_slot2 = kCntShadow;
+    a.ps = ps; a.cursor_slot = kCntWork + 1; a.scenes = scenes;
+    return launch_trace_mode<kModeSecondary>(sc, a, 0, sched_shadow, s);
 }
 
 // Closest hit of the BSDF-sampled MIS rays and the radiance they carry.
