@@ -350,7 +350,9 @@ typedef struct rd_tuning {
     int32_t trace_schedule[3]; /* per query class [extension, shadow, MIS]:
                                   0 grid-stride (one ray per lane to its end),
                                   1 idle lanes refilled from a work cursor,
-                                  2 64-ray chunks per wavefront from it     */
+                                  2 64-ray chunks per wavefront from it,
+                                  3 256-ray chunks, four rays per lane traced
+                                    back to back                            */
     int32_t ray_sort;          /* extension rays traced in coherence order
                                   (direction octant, origin Morton code)    */
     int32_t ray_sort_bits;     /* sort key bits, 3..30                      */

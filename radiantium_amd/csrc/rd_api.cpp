@@ -1559,7 +1559,7 @@ int rd_set_tuning(rd_ctx* c, const rd_tuning* t) {
     if (!c || !t) return fail(RD_ERR_INVALID, "null argument");
     for (int k = 0; k < 3; k++)
         if (t->trace_schedule[k] < -1 || t->trace_schedule[k] >= kNumSchedules)
-            return fail(RD_ERR_INVALID, "trace_schedule: 0, 1, 2 or -1");
+            return fail(RD_ERR_INVALID, "trace_schedule: 0 to 3, or -1");
     if (t->ray_sort_bits != -1 && (t->ray_sort_bits < 3 || t->ray_sort_bits > 30))
         return fail(RD_ERR_INVALID, "ray_sort_bits: 3..30 or -1");
     if (t->ray_sort_min < -1) return fail(RD_ERR_INVALID, "ray_sort_min: >= 0 or -1");

@@ -173,9 +173,11 @@ hipError_t launch_zero_counts(int* counts, int n, hipStream_t s);
 // Traversal schedules (the `sched` argument of the queue launchers):
 // 0 = straight (grid-stride, one ray per lane to completion),
 // 1 = refill (idle lanes refilled from an atomic cursor),
-// 2 = straight with 64-ray chunks per wavefront from an atomic cursor.
-// The batched and instanced kernels run straight (0 or 2).
-constexpr int kNumSchedules = 3;
+// 2 = straight with 64-ray chunks per wavefront from an atomic cursor,
+// 3 = sequences: 4 x 64-ray chunks per wavefront, each lane tracing its four
+//     rays back to back.
+// The batched and instanced kernels run straight (0, or chunks for 2 and 3).
+constexpr int kNumSchedules = 4;
 
 // Occupancy-sized persistent grid for the device.
 void set_device_grid(int num_cus);

@@ -387,10 +387,18 @@ __device__ __forceinline__ int trav_count(const TraceArgs& a, bool fixed) {
 // The next 64 consecutive ranks for the calling wavefront, from the work
 // cursor (wave-uniform): wavefronts that finish early take more rays, so the
 // launch ends evenly whatever the grid and the rays' costs.
-__device__ __forceinline__ int next_chunk(int* cursor) {
+__device__ __forceinline__ int next_chunk(int* cursor, int size = 64) {
     int base = 0;
-    if (__lane_id() == 0) base = atomicAdd(cursor, 64);
+    if (__lane_id() == 0) base = atomicAdd(cursor, size);
     return __shfl(base, 0);
+}
+
+// Blocks per CU that the LDS stack allows (160 KB of LDS per CU), at most 8:
+// the register budget the traversal kernels are compiled for, so registers
+// never lower the occupancy below what the stack already sets (the
+// two-level kernel is left unbounded: at that budget it spills heavily).
+constexpr int lds_blocks(int stack, int block) {
+    return (160 * 1024) / (stack * block * 4) < 8 ? (160 * 1024) / (stack * block * 4) : 8;
 }
 
 // Schedule 0 / 2 ("straight"): each lane runs its ray to completion; rays
@@ -398,7 +406,7 @@ __device__ __forceinline__ int next_chunk(int* cursor) {
 // wavefront from the work cursor (2, a.dyn).  kBatch: each ray traverses its
 // own request's scene.
 template <int kMode, int kStack, int kBlock, bool kBatch>
-__global__ __launch_bounds__(kBlock) void k_trace_straight(DevScene sc, TraceArgs a) {
+__global__ __launch_bounds__(kBlock, lds_blocks(kStack, kBlock)) void k_trace_straight(DevScene sc, TraceArgs a) {
     constexpr bool kAny = (kMode == kModeRaysAny || kMode == kModeShadow);
     __shared__ int lds[kStack * kBlock];
     int* stk = lds + threadIdx.x;
@@ -475,13 +483,60 @@ __global__ __launch_bounds__(kBlock) void k_trace_inst(DevScene sc, TraceArgs a)
     trav_counters<kMode>(a, nrays, nv, nt);
 }
 
+// Schedule 3 ("sequences"): a wavefront takes kSeq x 64 consecutive rays
+// from the work cursor and lane l traces rays l, l + 64, ... of them one after
+// another, starting its next ray as soon as one ends, so a wavefront waits
+// for its slowest lane once per kSeq rays instead of once per ray.
+constexpr int kSeq = 4;
+
+template <int kMode, int kStack, int kBlock>
+__global__ __launch_bounds__(kBlock, lds_blocks(kStack, kBlock)) void k_trace_seq(DevScene sc, TraceArgs a) {
+    constexpr bool kAny = (kMode == kModeRaysAny || kMode == kModeShadow);
+    __shared__ int lds[kStack * kBlock];
+    int* stk = lds + threadIdx.x;
+    const int n = trav_count(a, kMode <= kModeRaysAny);
+    if (blockIdx.x == 0 && threadIdx.x < 3 && a.zero[threadIdx.x] >= 0) a.counts[a.zero[threadIdx.x]] = 0;
+    int* cursor = a.counts + a.cursor_slot;
+    uint32_t nv = 0, nt = 0, nrays = 0;
+    for (int c = next_chunk(cursor, 64 * kSeq); c < n; c = next_chunk(cursor, 64 * kSeq)) {
+        int j = 0, r = 0;
+        Trav t;
+        // the lane's next ray of the chunk that needs tracing; false when none is left
+        auto start = [&]() {
+            while (j < kSeq) {
+                const int r0 = c + __lane_id() + 64 * j;
+                j++;
+                if (r0 >= n) break;
+                r = a.perm ? a.perm[r0] : r0;
+                if (a.rank) a.rank[r] = r0;
+                if (kMode == kModeShadow && a.q.pid[r] < 0) continue;   // a hole of the ranked queue
+                float4 ra, rb;
+                trav_load<kMode>(a, r, &ra, &rb);
+                trav_begin(t, ra, rb);
+                return true;
+            }
+            j = kSeq;
+            return false;
+        };
+        bool active = start();
+        while (__ballot(active)) {
+            if (active && trav_step<kAny, kBlock>(sc.nodes, sc.prims, t, stk, nv, nt)) {
+                trav_finish<kMode>(sc, a, r, t);
+                nrays++;
+                active = start();
+            }
+        }
+    }
+    trav_counters<kMode>(a, nrays, nv, nt);
+}
+
 // MIS and shadow rays of one bounce in one launch: the concatenated queue
 // [MIS rays | shadow rays] is taken like a straight schedule, so the MIS
 // rays (few, often long) run beside the shadow rays instead of leaving the
 // chip idle behind their longest one.  Each ray's query and result are those
 // of the separate MIS (closest hit + Le) and shadow (any hit) launches.
 template <int kStack, int kBlock, bool kBatch>
-__global__ __launch_bounds__(kBlock) void k_trace_secondary(DevScene sc, TraceArgs a) {
+__global__ __launch_bounds__(kBlock, lds_blocks(kStack, kBlock)) void k_trace_secondary(DevScene sc, TraceArgs a) {
     __shared__ int lds[kStack * kBlock];
     int* stk = lds + threadIdx.x;
     const int n1 = a.counts[a.cnt_slot];
@@ -509,7 +564,42 @@ __global__ __launch_bounds__(kBlock) void k_trace_secondary(DevScene sc, TraceAr
             nr_s++;
         }
     };
-    if (a.dyn) {
+    if (a.dyn == 2 && !kBatch) {
+        // sequences of kSeq rays per lane, as k_trace_seq
+        for (int c = next_chunk(cursor, 64 * kSeq); c < n; c = next_chunk(cursor, 64 * kSeq)) {
+            int j = 0, r = 0, pid = 0;
+            bool mis = false;
+            Trav t;
+            auto start = [&]() {
+                while (j < kSeq) {
+                    const int r0 = c + __lane_id() + 64 * j;
+                    j++;
+                    if (r0 >= n) break;
+                    mis = r0 < n1;
+                    r = mis ? r0 : r0 - n1;
+                    const RayQueue& q = mis ? a.q : a.q2;
+                    pid = q.pid[r];
+                    if (pid < 0) continue;
+                    trav_begin(t, q.org[r], q.dir[r]);
+                    return true;
+                }
+                j = kSeq;
+                return false;
+            };
+            bool active = start();
+            while (__ballot(active)) {
+                if (active) {
+                    const bool done = mis ? trav_step<false, kBlock>(sc.nodes, sc.prims, t, stk, nv_m, nt_m)
+                                          : trav_step<true, kBlock>(sc.nodes, sc.prims, t, stk, nv_s, nt_s);
+                    if (done) {
+                        if (mis) { trav_finish<kModeMis>(sc, a, r, t); nr_m++; }
+                        else { a.ps.vis[pid] = t.best.prim >= 0 ? 0 : 1; nr_s++; }
+                        active = start();
+                    }
+                }
+            }
+        }
+    } else if (a.dyn) {
         for (int c = next_chunk(cursor); c < n; c = next_chunk(cursor))
             if (c + __lane_id() < n) one(c + __lane_id());
     } else {
@@ -524,7 +614,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_secondary(DevScene sc, TraceAr
 // Schedule 1 ("refill"): lanes that finish are refilled from an atomic ray
 // cursor once at least kRefill lanes of the wavefront are idle.
 template <int kMode, int kStack, int kBlock>
-__global__ __launch_bounds__(kBlock) void k_trace_refill(DevScene sc, TraceArgs a) {
+__global__ __launch_bounds__(kBlock, lds_blocks(kStack, kBlock)) void k_trace_refill(DevScene sc, TraceArgs a) {
     constexpr bool kAny = (kMode == kModeRaysAny || kMode == kModeShadow);
     __shared__ int lds[kStack * kBlock];
     int* stk = lds + threadIdx.x;
@@ -627,6 +717,7 @@ static void launch_trace_variant(const DevScene& sc, const TraceArgs& a, int blo
         }
     }
     if (schedule == 1) launch_resident(k_trace_refill<kMode, kStack, kBlock>, blocks, kBlock, s, sc, a);
+    else if (schedule == 3) launch_resident(k_trace_seq<kMode, kStack, kBlock>, blocks, kBlock, s, sc, a);
     else launch_resident(k_trace_straight<kMode, kStack, kBlock, false>, blocks, kBlock, s, sc, a);
 }
 
@@ -637,7 +728,9 @@ static hipError_t launch_trace_mode(const DevScene& sc, const TraceArgs& a0, int
     int pb = persistent_blocks();
     if (kMode <= kModeRaysAny || sched < 0 || sched >= kNumSchedules) sched = 0;
     TraceArgs a = a0;
-    a.dyn = sched == 2 ? 1 : 0;
+    // straight kernels: 2 = cursor chunks; 3 = sequences (batched and
+    // instanced scenes, which have no sequence kernel, take chunks)
+    a.dyn = sched == 2 ? 1 : (sched == 3 ? 2 : 0);
     if (need <= 24) {
         int b = blocks_hint > 0 ? std::min(blocks_hint, pb) : pb;
         launch_trace_variant<kMode, 24, 256>(sc, a, b, sched, s);

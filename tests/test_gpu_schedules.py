@@ -32,7 +32,7 @@ def test_schedules_bit_identical(ctx, which):
         ref, rst = _render(ctx, sc, spp, trace_schedule=(0, 0, 0), ray_sort=1, ray_sort_shadow=1)
         print(f"[schedules] {which}: class rays {rst['class_rays']}", flush=True)
         assert rst["paths"] > 2_000_000 and max(rst["class_rays"]) > 1_000_000
-        for sched in ((1, 1, 1), (2, 2, 2), (2, 2, 1), (0, 1, 2)):
+        for sched in ((1, 1, 1), (2, 2, 2), (2, 2, 1), (0, 1, 2), (3, 3, 3), (3, 2, 0)):
             img, st = _render(ctx, sc, spp, trace_schedule=sched)
             print(f"[schedules] {which}: {sched}", flush=True)
             assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), sched
@@ -53,7 +53,7 @@ def test_tuning_roundtrip_and_validation(ctx):
         ctx.set_tuning(trace_schedule=(1, None, 0), ray_sort_bits=12)
         t = ctx.tuning()
         assert t["trace_schedule"] == [1, base["trace_schedule"][1], 0] and t["ray_sort_bits"] == 12
-        for bad in (dict(trace_schedule=(3, 0, 0)), dict(ray_sort_bits=31), dict(ray_sort_bits=2),
+        for bad in (dict(trace_schedule=(4, 0, 0)), dict(ray_sort_bits=31), dict(ray_sort_bits=2),
                     dict(ray_sort_min=-2)):
             with pytest.raises(RadiantiumError):
                 ctx.set_tuning(**bad)
