@@ -227,7 +227,9 @@ typedef struct rd_render_params {
     int32_t spp;            /* renderer "spp", default 256                  */
     uint32_t seed;          /* counter-based RNG seed                       */
     int32_t x0, y0, x1, y1; /* pixel crop [x0,x1) x [y0,y1); all 0 = full   */
-    int32_t wave_paths;     /* paths in flight per wave; 0 = autotuned      */
+    int32_t wave_paths;     /* paths in flight per wave; 0 = as many as a
+                               quarter of the device memory holds (every
+                               path of the render where it fits)            */
     int32_t flags;          /* RD_RENDER_* bits                              */
 } rd_render_params;
 

@@ -1,5 +1,6 @@
-"""Traversal schedules and ray ordering only decide which lane traces which
-ray, so renders are bit-identical under every rd_tuning setting.  The frames
+"""Traversal schedules, ray ordering and the slot pool size only decide
+which lane traces which ray and how many paths are in flight, so renders are
+bit-identical under every rd_tuning setting and every wave_paths.  The frames
 here keep more rays in flight than the chip has lanes (about 393K), so the
 work-cursor schedules run through many chunks per wavefront, and the ranked
 shadow / MIS queues hold runs of holes longer than a wavefront."""
@@ -61,3 +62,24 @@ def test_tuning_roundtrip_and_validation(ctx):
     finally:
         ctx.set_tuning(**base)
     assert ctx.tuning() == base
+
+
+@pytest.mark.parametrize("which", ["icosphere_mis", "fog_vol"])
+def test_pool_size_bit_identical(ctx, which):
+    """Small pools (many waves of paths, new paths started in freed slots
+    every bounce) and the default pool (every path of the render at once)
+    give the same image and the same query counts."""
+    if which == "icosphere_mis":
+        desc, spp = scenes.displaced_icosphere_scene(level=5, width=512, height=288, max_depth=8), 4
+    else:
+        desc, spp = scenes.vol_fog(256, 256, max_depth=16), 8
+    sc = Scene(ctx, desc)
+    ref, _ = sc.render(spp, seed=9)
+    rst = ctx.render_stats()
+    for wp in (1 << 16, 100_000, 1 << 19):
+        img, _ = sc.render(spp, seed=9, wave_paths=wp)
+        st = ctx.render_stats()
+        print(f"[pool] {which}: wave_paths {wp}: {st['waves']} waves", flush=True)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), wp
+        for k in KEYS:
+            assert st[k] == rst[k], (wp, k, st[k], rst[k])
