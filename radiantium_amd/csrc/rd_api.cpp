@@ -276,7 +276,7 @@ int ensure_wave_buffers(rd_ctx* c, size_t pool, size_t results, size_t pixels, b
         ps.alive = (int*)alloc(cap * sizeof(int));
         ps.ray_o = (float4*)alloc(cap * sizeof(float4));
         ps.ray_d = (float4*)alloc(cap * sizeof(float4));
-        w.blk = (int2*)alloc(2 * ((cap + 1023) / 1024) * sizeof(int2));
+        w.blk = (int2*)alloc(compact_scratch_int2(cap) * sizeof(int2));
         ps.rng_n = (uint32_t*)alloc(cap * sizeof(uint32_t));
         ps.depth = (int*)alloc(cap * sizeof(int));
         ps.vis = (uint8_t*)alloc(cap);
@@ -1017,7 +1017,8 @@ struct PassLaunch {
 
 // The bounce loop of one pass: every bounce traces and shades the queue,
 // retires finished paths, then compacts the pool into the next queue (slot
-// order) while starting the pass's next paths in the free slots;
+// order) while starting the pass's next paths in the free slots (once all
+// have started, by filtering the last queue, which gives the same queue);
 // one sync per bounce reads the next queue size (for vol_path also the chain
 // queue sizes).  Ends when the queue is empty (the pass has run dry).
 int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatile int32_t* cancel) {
@@ -1030,6 +1031,7 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
         return launch_compact(L.sc, L.scenes, L.segs, L.nseg, L.npass, L.pool, wb.blk, qin, cin, wb.ps, c->counts, st);
     }));
     int rc = RD_OK;
+    bool all_started = false;   // known on the host after each bounce's compaction
     for (int bounce = 0; !rc; bounce++) {
         // the queue size is known on the host from the second bounce on
         const int* perm = nullptr;
@@ -1103,11 +1105,19 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
             }));
         }
         HIP_TRY(tm.run("compact", [&] {
+            // once every path of the pass has started, only the entries of
+            // this queue can continue: filter it instead of scanning the pool
+            if (all_started)
+                return launch_compact_queue(qin, c->host_counts[cin], L.npass, wb.blk, qout, cout, wb.ps, c->counts,
+                                            st);
             return launch_compact(L.sc, L.scenes, L.segs, L.nseg, L.npass, L.pool, wb.blk, qout, cout, wb.ps,
                                   c->counts, st);
         }));
         HIP_TRY(hipMemcpyAsync(c->host_counts + cout, c->counts + cout, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(c->host_counts + kCntNext, c->counts + kCntNext, sizeof(int), hipMemcpyDeviceToHost,
+                               st));
         HIP_TRY(hipStreamSynchronize(st));
+        all_started = c->host_counts[kCntNext] >= L.npass;
         if (bounce > *max_bounce) *max_bounce = bounce;
         if (c->host_counts[cout] == 0) break;
         if (cancel && *cancel) rc = fail(RD_ERR_CANCELLED, "render cancelled");

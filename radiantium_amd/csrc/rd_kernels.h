@@ -114,10 +114,19 @@ hipError_t launch_trace_rays(const DevScene& sc, const float4* rays, int n, floa
 // slot; compact builds the next extension queue in slot order from the slots
 // whose path continues and starts the pass's next paths in the free slots
 // (slot order, consecutive pass indices); resolve retires finished paths into
-// ps.res.  blk: 2 * ceil(pool / 1024) int2 of scratch.
+// ps.res.  blk: compact_scratch_int2(pool) int2 of scratch.
+// compact_queue: the next queue from the n entries of the previous one whose
+// path continues, same order; for bounces after every path of the pass has
+// started (it starts none).
 hipError_t launch_init_pool(PathState ps, int pool, int* counts, hipStream_t s);
 hipError_t launch_compact(const DevScene& sc, const DevScene* scenes, const BatchSeg* segs, int nseg, int npass,
                           int pool, int2* blk, RayQueue q, int cnt_q, PathState ps, int* counts, hipStream_t s);
+hipError_t launch_compact_queue(RayQueue qin, int n, int npass, int2* blk, RayQueue q, int cnt_q, PathState ps,
+                                int* counts, hipStream_t s);
+inline size_t compact_scratch_int2(size_t pool) {
+    const size_t nblk = (pool + 1023) / 1024;
+    return 2 * nblk + 2 * ((nblk + 1023) / 1024);
+}
 // perm: trace order (null = queue order); rank (with perm) receives each
 // queue position's place in that order.
 hipError_t launch_trace_ext(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, float4* hits,

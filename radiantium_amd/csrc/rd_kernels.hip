@@ -887,40 +887,96 @@ __global__ __launch_bounds__(256) void k_pool_count(const int* __restrict__ aliv
     if (threadIdx.x == 0) blk[blockIdx.x] = make_int2(ta, tf);
 }
 
-__global__ __launch_bounds__(1024) void k_pool_scan(const int2* __restrict__ blk, int nblk, int npass, int* counts,
-                                                    int cnt_q, int2* off) {
+// Block offsets in three steps, for any number of count blocks: tiles of
+// 1024 blocks are summed, one block scans the tile sums, and every tile then
+// scans its blocks from its tile prefix.  The paths started are the first
+// `left` free slots, so a tile starts min(max(left - free before it, 0), its
+// free slots) of them.
+__global__ __launch_bounds__(1024) void k_scan_tiles(const int2* __restrict__ blk, int nblk, int2* tsum) {
+    __shared__ int tmp[16];
+    const int b = blockIdx.x * 1024 + threadIdx.x;
+    const int2 c = b < nblk ? blk[b] : make_int2(0, 0);
+    int ta, tf;
+    block_excl_scan<1024>(c.x, tmp, &ta);
+    block_excl_scan<1024>(c.y, tmp, &tf);
+    if (threadIdx.x == 0) tsum[blockIdx.x] = make_int2(ta, tf);
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(const int2* __restrict__ tsum, int ntiles, int npass, int* counts,
+                                                   int cnt_q, int2* tpre) {
     __shared__ int tmp[16];
     const int next = counts[kCntNext];
     const int left = max(npass - next, 0);
-    const int per = (nblk + 1023) / 1024;
-    const int b0 = threadIdx.x * per;
-    int fsum = 0;
-    for (int i = 0; i < per; i++)
-        if (b0 + i < nblk) fsum += blk[b0 + i].y;
-    int ftot;
-    const int fpre = block_excl_scan<1024>(fsum, tmp, &ftot);
-    int ssum = 0, run = fpre;
-    for (int i = 0; i < per; i++) {
-        if (b0 + i >= nblk) break;
-        int2 c = blk[b0 + i];
-        ssum += c.x + min(max(left - run, 0), c.y);
-        run += c.y;
-    }
-    int stot;
-    const int spre = block_excl_scan<1024>(ssum, tmp, &stot);
-    run = fpre;
-    int srun = spre;
-    for (int i = 0; i < per; i++) {
-        if (b0 + i >= nblk) break;
-        int2 c = blk[b0 + i];
-        off[b0 + i] = make_int2(run, srun);
-        srun += c.x + min(max(left - run, 0), c.y);
-        run += c.y;
+    int frun = 0, qrun = 0;
+    for (int t0 = 0; t0 < ntiles; t0 += 1024) {
+        const int t = t0 + threadIdx.x;
+        const int2 c = t < ntiles ? tsum[t] : make_int2(0, 0);
+        int ftot;
+        const int fpre = frun + block_excl_scan<1024>(c.y, tmp, &ftot);
+        const int q = c.x + min(max(left - fpre, 0), c.y);
+        int qtot;
+        const int qpre = qrun + block_excl_scan<1024>(q, tmp, &qtot);
+        if (t < ntiles) tpre[t] = make_int2(fpre, qpre);
+        frun += ftot;
+        qrun += qtot;
     }
     if (threadIdx.x == 0) {
-        counts[cnt_q] = stot;
+        counts[cnt_q] = qrun;
         counts[kCntFree] = next;
-        counts[kCntNext] = next + min(left, ftot);
+        counts[kCntNext] = next + min(left, frun);
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_apply(const int2* __restrict__ blk, int nblk,
+                                                     const int2* __restrict__ tpre, int npass, const int* counts,
+                                                     int2* off) {
+    __shared__ int tmp[16];
+    const int left = max(npass - counts[kCntFree], 0);
+    const int b = blockIdx.x * 1024 + threadIdx.x;
+    const int2 c = b < nblk ? blk[b] : make_int2(0, 0);
+    const int2 tp = tpre[blockIdx.x];
+    int tf;
+    const int fpre = tp.x + block_excl_scan<1024>(c.y, tmp, &tf);
+    const int q = c.x + min(max(left - fpre, 0), c.y);
+    int tq;
+    const int qpre = tp.y + block_excl_scan<1024>(q, tmp, &tq);
+    if (b < nblk) off[b] = make_int2(fpre, qpre);
+}
+
+// Queue filter: once every path of the pass has started, the next queue is
+// the previous one's entries whose path continues, in the same order (slot
+// order, as the pool compaction would give).  Counts per 1024 entries.
+__global__ __launch_bounds__(256) void k_queue_count(const int* __restrict__ pid, int n,
+                                                     const int* __restrict__ alive, int2* blk) {
+    __shared__ int tmp[4];
+    const int base = blockIdx.x * kCompactSlots;
+    int na = 0;
+    for (int i = 0; i < kCompactSlots / 256; i++) {
+        const int k = base + i * 256 + threadIdx.x;
+        if (k < n && alive[pid[k]]) na++;
+    }
+    int ta;
+    block_excl_scan<256>(na, tmp, &ta);
+    if (threadIdx.x == 0) blk[blockIdx.x] = make_int2(ta, 0);
+}
+
+__global__ __launch_bounds__(256) void k_queue_scatter(RayQueue qin, int n, PathState ps,
+                                                       const int2* __restrict__ off, RayQueue q) {
+    __shared__ int tmp[4];
+    const int base = blockIdx.x * kCompactSlots;
+    int pos_base = off[blockIdx.x].y;
+    for (int i = 0; i < kCompactSlots / 256; i++) {
+        const int k = base + i * 256 + threadIdx.x;
+        const int slot = k < n ? qin.pid[k] : -1;
+        const bool take = slot >= 0 && ps.alive[slot] == 1;
+        int ts;
+        const int pos = pos_base + block_excl_scan<256>(take ? 1 : 0, tmp, &ts);
+        if (take) {
+            q.pid[pos] = slot;
+            q.org[pos] = ps.ray_o[slot];
+            q.dir[pos] = ps.ray_d[slot];
+        }
+        pos_base += ts;
     }
 }
 
@@ -961,12 +1017,36 @@ __global__ __launch_bounds__(256) void k_pool_scatter(DevScene sc0, const DevSce
     }
 }
 
+// blk: nblk block counts, then nblk offsets (off), then 2 x the tile count.
+static void scan_blocks(const int2* blk, int nblk, int npass, int* counts, int cnt_q, int2* off, hipStream_t s) {
+    const int ntiles = (nblk + 1023) / 1024;
+    int2* tsum = off + nblk;
+    int2* tpre = tsum + ntiles;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(1024), 0, s, blk, nblk, tsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, tsum, ntiles, npass, counts, cnt_q, tpre);
+    hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(1024), 0, s, blk, nblk, tpre, npass, counts, off);
+}
+
+hipError_t launch_compact_queue(RayQueue qin, int n, int npass, int2* blk, RayQueue q, int cnt_q, PathState ps,
+                                int* counts, hipStream_t s) {
+    if (n <= 0) {
+        hipLaunchKernelGGL(k_zero_slot, dim3(1), dim3(64), 0, s, counts, cnt_q);
+        return hipGetLastError();
+    }
+    const int nblk = (n + kCompactSlots - 1) / kCompactSlots;
+    int2* off = blk + nblk;
+    hipLaunchKernelGGL(k_queue_count, dim3(nblk), dim3(256), 0, s, qin.pid, n, ps.alive, blk);
+    scan_blocks(blk, nblk, npass, counts, cnt_q, off, s);
+    hipLaunchKernelGGL(k_queue_scatter, dim3(nblk), dim3(256), 0, s, qin, n, ps, off, q);
+    return hipGetLastError();
+}
+
 hipError_t launch_compact(const DevScene& sc, const DevScene* scenes, const BatchSeg* segs, int nseg, int npass,
                           int pool, int2* blk, RayQueue q, int cnt_q, PathState ps, int* counts, hipStream_t s) {
     const int nblk = (pool + kCompactSlots - 1) / kCompactSlots;
     int2* off = blk + nblk;
     hipLaunchKernelGGL(k_pool_count, dim3(nblk), dim3(256), 0, s, ps.alive, pool, blk);
-    hipLaunchKernelGGL(k_pool_scan, dim3(1), dim3(1024), 0, s, blk, nblk, npass, counts, cnt_q, off);
+    scan_blocks(blk, nblk, npass, counts, cnt_q, off, s);
     if (scenes)
         hipLaunchKernelGGL(k_pool_scatter<true>, dim3(nblk), dim3(256), 0, s, sc, scenes, segs, nseg, npass, ps, pool,
                            off, counts, q);
