@@ -362,8 +362,8 @@ def main():
     value = rays / elapsed / 1e6
 
     stages = {}
-    for name in ("compact", "sort", "trace_closest", "shade", "trace_secondary", "trace_any", "trace_mis",
-                 "resolve", "accumulate", "finalize"):
+    for name in ("compact", "sort", "trace_closest", "shade", "compact_shadow", "trace_secondary", "trace_any",
+                 "trace_mis", "resolve", "accumulate", "finalize"):
         avg, cnt = ctx.kernel_timing(name)
         if not cnt:
             continue

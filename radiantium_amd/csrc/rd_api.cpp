@@ -258,7 +258,10 @@ int ensure_wave_buffers(rd_ctx* c, size_t pool, size_t results, size_t pixels, b
             q.dir = (float4*)alloc(cap * sizeof(float4));
             return q.pid && q.org && q.dir;
         };
-        bool ok = mkq(w.qa, kCntExtA) && mkq(w.qb, kCntExtB) && mkq(w.qshadow, kCntShadow) && mkq(w.qmis, kCntMis);
+        // qshadow2: vol_path's second Transmittance chain queue; the path
+        // integrator's ranked shadow queue without its holes
+        bool ok = mkq(w.qa, kCntExtA) && mkq(w.qb, kCntExtB) && mkq(w.qshadow, kCntShadow) && mkq(w.qmis, kCntMis) &&
+                  mkq(w.qshadow2, kCntShadowB);
         w.hits = (float4*)alloc(cap * sizeof(float4));
         PathState& ps = w.ps;
         ps.L = (float4*)alloc(cap * sizeof(float4));
@@ -287,7 +290,7 @@ int ensure_wave_buffers(rd_ctx* c, size_t pool, size_t results, size_t pixels, b
         ps.env = nullptr; ps.nee_li = ps.sh_to = ps.sh_tr = ps.mis_tr = nullptr;
         ps.hsub2 = nullptr;
         if (ok && vol) {
-            ok = mkq(w.qshadow2, kCntShadowB) && mkq(w.qmis2, kCntMisB);
+            ok = mkq(w.qmis2, kCntMisB);
             w.hits2 = (float4*)alloc(cap * sizeof(float4));
             ps.env = (int*)alloc(cap * sizeof(int));
             ps.nee_li = (float4*)alloc(cap * sizeof(float4));
@@ -1089,15 +1092,26 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
                 return launch_shade(L.sc, L.scenes, qin, wb.hits, wb.qshadow, wb.qmis, wb.ps, c->counts, cin,
                                     L.adj != nullptr, rank, st);
             }));
+            // a ranked shadow queue holds a hole for every path without a
+            // shadow ray: trace it without them, in the same order
+            RayQueue qsh = wb.qshadow;
+            int csh = kCntShadow;
+            if (rank && L.method != RD_METHOD_BXDF) {
+                HIP_TRY(tm.run("compact_shadow", [&] {
+                    return launch_compact_holes(wb.qshadow, c->host_counts[cin], wb.blk, wb.qshadow2, kCntShadowB,
+                                                c->counts, st);
+                }));
+                qsh = wb.qshadow2;
+                csh = kCntShadowB;
+            }
             if (L.method == RD_METHOD_MIS) {
                 HIP_TRY(tm.run("trace_secondary", [&] {
-                    return launch_trace_secondary(L.sc, L.scenes, wb.qshadow, wb.qmis, wb.ps, c->counts,
+                    return launch_trace_secondary(L.sc, L.scenes, qsh, csh, wb.qmis, wb.ps, c->counts,
                                                   c->trace_schedule[1], c->trace_schedule[2], st);
                 }));
             } else if (L.method == RD_METHOD_NEE) {
                 HIP_TRY(tm.run("trace_any", [&] {
-                    return launch_trace_shadow(L.sc, L.scenes, wb.qshadow, wb.ps, c->counts, c->trace_schedule[1],
-                                               st);
+                    return launch_trace_shadow(L.sc, L.scenes, qsh, wb.ps, c->counts, csh, c->trace_schedule[1], st);
                 }));
             }
             HIP_TRY(tm.run(L.adj ? "resolve_adjoint" : "resolve", [&] {

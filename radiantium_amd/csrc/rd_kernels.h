@@ -151,12 +151,16 @@ hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin
 hipError_t launch_accumulate_batch(const BatchSeg* segs, int nseg, int max_npix, PathState ps, float4* sum,
                                    float4* sum2, int* counts, hipStream_t s);
 hipError_t launch_trace_shadow(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,
-                               int sched, hipStream_t s);
+                               int cnt_slot, int sched, hipStream_t s);
 hipError_t launch_trace_mis(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,
                             int sched, hipStream_t s);
 // Both of the above in one launch where the schedules allow it.
-hipError_t launch_trace_secondary(const DevScene& sc, const DevScene* scenes, RayQueue qshadow, RayQueue qmis,
-                                  PathState ps, int* counts, int sched_shadow, int sched_mis, hipStream_t s);
+hipError_t launch_trace_secondary(const DevScene& sc, const DevScene* scenes, RayQueue qshadow, int cnt_shadow,
+                                  RayQueue qmis, PathState ps, int* counts, int sched_shadow, int sched_mis,
+                                  hipStream_t s);
+// The n entries of a ranked queue that hold a ray, in order, into q (count at
+// cnt_q); blk as for launch_compact.
+hipError_t launch_compact_holes(RayQueue qin, int n, int2* blk, RayQueue q, int cnt_q, int* counts, hipStream_t s);
 // adj: null for rendering; the adjoint replay's gradient inputs otherwise.
 hipError_t launch_resolve(const DevScene& sc, int method, RayQueue qin, PathState ps, int* counts, int cnt_in,
                           const AdjArgs* adj, hipStream_t s);

@@ -903,7 +903,7 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(const int2* __restrict__ bl
 }
 
 __global__ __launch_bounds__(1024) void k_scan_top(const int2* __restrict__ tsum, int ntiles, int npass, int* counts,
-                                                   int cnt_q, int2* tpre) {
+                                                   int cnt_q, int2* tpre, int pass_state) {
     __shared__ int tmp[16];
     const int next = counts[kCntNext];
     const int left = max(npass - next, 0);
@@ -922,8 +922,10 @@ __global__ __launch_bounds__(1024) void k_scan_top(const int2* __restrict__ tsum
     }
     if (threadIdx.x == 0) {
         counts[cnt_q] = qrun;
-        counts[kCntFree] = next;
-        counts[kCntNext] = next + min(left, frun);
+        if (pass_state) {
+            counts[kCntFree] = next;
+            counts[kCntNext] = next + min(left, frun);
+        }
     }
 }
 
@@ -958,6 +960,39 @@ __global__ __launch_bounds__(256) void k_queue_count(const int* __restrict__ pid
     int ta;
     block_excl_scan<256>(na, tmp, &ta);
     if (threadIdx.x == 0) blk[blockIdx.x] = make_int2(ta, 0);
+}
+
+// Hole removal: the entries of a ranked queue that hold a ray (pid >= 0), in
+// the same order.  Counts per 1024 entries.
+__global__ __launch_bounds__(256) void k_hole_count(const int* __restrict__ pid, int n, int2* blk) {
+    __shared__ int tmp[4];
+    const int base = blockIdx.x * kCompactSlots;
+    int na = 0;
+    for (int i = 0; i < kCompactSlots / 256; i++) {
+        const int k = base + i * 256 + threadIdx.x;
+        if (k < n && pid[k] >= 0) na++;
+    }
+    int ta;
+    block_excl_scan<256>(na, tmp, &ta);
+    if (threadIdx.x == 0) blk[blockIdx.x] = make_int2(ta, 0);
+}
+
+__global__ __launch_bounds__(256) void k_hole_scatter(RayQueue qin, int n, const int2* __restrict__ off, RayQueue q) {
+    __shared__ int tmp[4];
+    const int base = blockIdx.x * kCompactSlots;
+    int pos_base = off[blockIdx.x].y;
+    for (int i = 0; i < kCompactSlots / 256; i++) {
+        const int k = base + i * 256 + threadIdx.x;
+        const int pid = k < n ? qin.pid[k] : -1;
+        int ts;
+        const int pos = pos_base + block_excl_scan<256>(pid >= 0 ? 1 : 0, tmp, &ts);
+        if (pid >= 0) {
+            q.pid[pos] = pid;
+            q.org[pos] = qin.org[k];
+            q.dir[pos] = qin.dir[k];
+        }
+        pos_base += ts;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_queue_scatter(RayQueue qin, int n, PathState ps,
@@ -1018,12 +1053,16 @@ __global__ __launch_bounds__(256) void k_pool_scatter(DevScene sc0, const DevSce
 }
 
 // blk: nblk block counts, then nblk offsets (off), then 2 x the tile count.
-static void scan_blocks(const int2* blk, int nblk, int npass, int* counts, int cnt_q, int2* off, hipStream_t s) {
+// pass_state: the compaction of the pass's pool or queue (it records the
+// paths it started); hole removal leaves those counters alone.
+static void scan_blocks(const int2* blk, int nblk, int npass, int* counts, int cnt_q, int2* off, bool pass_state,
+                        hipStream_t s) {
     const int ntiles = (nblk + 1023) / 1024;
     int2* tsum = off + nblk;
     int2* tpre = tsum + ntiles;
     hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(1024), 0, s, blk, nblk, tsum);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, tsum, ntiles, npass, counts, cnt_q, tpre);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, tsum, ntiles, npass, counts, cnt_q, tpre,
+                       pass_state ? 1 : 0);
     hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(1024), 0, s, blk, nblk, tpre, npass, counts, off);
 }
 
@@ -1036,8 +1075,21 @@ hipError_t launch_compact_queue(RayQueue qin, int n, int npass, int2* blk, RayQu
     const int nblk = (n + kCompactSlots - 1) / kCompactSlots;
     int2* off = blk + nblk;
     hipLaunchKernelGGL(k_queue_count, dim3(nblk), dim3(256), 0, s, qin.pid, n, ps.alive, blk);
-    scan_blocks(blk, nblk, npass, counts, cnt_q, off, s);
+    scan_blocks(blk, nblk, npass, counts, cnt_q, off, true, s);
     hipLaunchKernelGGL(k_queue_scatter, dim3(nblk), dim3(256), 0, s, qin, n, ps, off, q);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_holes(RayQueue qin, int n, int2* blk, RayQueue q, int cnt_q, int* counts, hipStream_t s) {
+    if (n <= 0) {
+        hipLaunchKernelGGL(k_zero_slot, dim3(1), dim3(64), 0, s, counts, cnt_q);
+        return hipGetLastError();
+    }
+    const int nblk = (n + kCompactSlots - 1) / kCompactSlots;
+    int2* off = blk + nblk;
+    hipLaunchKernelGGL(k_hole_count, dim3(nblk), dim3(256), 0, s, qin.pid, n, blk);
+    scan_blocks(blk, nblk, 0, counts, cnt_q, off, false, s);
+    hipLaunchKernelGGL(k_hole_scatter, dim3(nblk), dim3(256), 0, s, qin, n, off, q);
     return hipGetLastError();
 }
 
@@ -1046,7 +1098,7 @@ hipError_t launch_compact(const DevScene& sc, const DevScene* scenes, const Batc
     const int nblk = (pool + kCompactSlots - 1) / kCompactSlots;
     int2* off = blk + nblk;
     hipLaunchKernelGGL(k_pool_count, dim3(nblk), dim3(256), 0, s, ps.alive, pool, blk);
-    scan_blocks(blk, nblk, npass, counts, cnt_q, off, s);
+    scan_blocks(blk, nblk, npass, counts, cnt_q, off, true, s);
     if (scenes)
         hipLaunchKernelGGL(k_pool_scatter<true>, dim3(nblk), dim3(256), 0, s, sc, scenes, segs, nseg, npass, ps, pool,
                            off, counts, q);
@@ -1085,24 +1137,25 @@ hipError_t launch_trace_ext(const DevScene& sc, const DevScene* scenes, RayQueue
 
 // Any hit of the shadow queue: Scene.IsOccluded (Scene.cs:58-70).
 hipError_t launch_trace_shadow(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,
-                               int sched, hipStream_t s) {
+                               int cnt_slot, int sched, hipStream_t s) {
     TraceArgs a = trace_args(counts);
-    a.q = q; a.ps = ps; a.cnt_slot = kCntShadow; a.cursor_slot = kCntWork + 1; a.scenes = scenes;
+    a.q = q; a.ps = ps; a.cnt_slot = cnt_slot; a.cursor_slot = kCntWork + 1; a.scenes = scenes;
     return launch_trace_mode<kModeShadow>(sc, a, 0, sched, s);
 }
 
 // The MIS rays (closest hit + Le) and the shadow rays (any hit) of a bounce:
 // one fused launch on plain scenes with a straight schedule, else the two
 // launches in turn.
-hipError_t launch_trace_secondary(const DevScene& sc, const DevScene* scenes, RayQueue qshadow, RayQueue qmis,
-                                  PathState ps, int* counts, int sched_shadow, int sched_mis, hipStream_t s) {
+hipError_t launch_trace_secondary(const DevScene& sc, const DevScene* scenes, RayQueue qshadow, int cnt_shadow,
+                                  RayQueue qmis, PathState ps, int* counts, int sched_shadow, int sched_mis,
+                                  hipStream_t s) {
     if (sc.num_insts > 0 || sched_shadow == 1 || sched_mis == 1) {
         if (hipError_t e = launch_trace_mis(sc, scenes, qmis, ps, counts, sched_mis, s)) return e;
-        return launch_trace_shadow(sc, scenes, qshadow, ps, counts, sched_shadow, s);
+        return launch_trace_shadow(sc, scenes, qshadow, ps, counts, cnt_shadow, sched_shadow, s);
     }
     TraceArgs a = trace_args(counts);
     a.q = qmis; a.cnt_slot = kCntMis;
-    a.q2 = qshadow; a.cnt_slot2 = kCntShadow;
+    a.q2 = qshadow; a.cnt_slot2 = cnt_shadow;
     a.ps = ps; a.cursor_slot = kCntWork + 1; a.scenes = scenes;
     return launch_trace_mode<kModeSecondary>(sc, a, 0, sched_shadow, s);
 }
