@@ -51,6 +51,14 @@ struct Hit {
     int prim;   // original primitive index, -1 = miss
 };
 
+// Whole-record loads.  Left alone, the compiler sinks the parts of a node
+// (or primitive) record into the branches that use them, so one node step
+// becomes up to three dependent round trips (x/y bounds, then z bounds, then
+// the child offset) and five load instructions.  Passing the loaded values
+// through an empty asm statement makes every part live at that point: both
+// 16-byte loads issue together and are waited for once.
+__device__ __forceinline__ void whole(float4& a) { asm volatile("" : "+v"(a.x), "+v"(a.y), "+v"(a.z), "+v"(a.w)); }
+
 // ---------------------------------------------------------------- traversal kernels
 // One kernel template per (query class, LDS stack depth, schedule).  Every
 // ray walks the reference's depth-first order (Bvh.cs:126-232) one node per
@@ -119,6 +127,7 @@ template <bool kAny, int kBlock>
 __device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes, const float4* __restrict__ prims, Trav& t,
                                           int* stk, uint32_t& nv, uint32_t& nt) {
     float4 lo = nodes[2 * t.cur], hi = nodes[2 * t.cur + 1];
+    whole(lo); whole(hi);
     nv++;
     if (box_hit(lo, hi, t.o, t.inv, t.nx, t.ny, t.nz, t.tmax)) {
         int off = __float_as_int(hi.z);
@@ -127,12 +136,12 @@ __device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes, cons
         if (cnt > 0) {
             for (int i = 0; i < cnt; i++) {
                 const float4* p = prims + 3 * (off + i);
-                float4 p0 = p[0], p1 = p[1];
+                float4 p0 = p[0], p1 = p[1], p2 = p[2];   // a sphere's third float4 is padding
+                whole(p0); whole(p1); whole(p2);
                 nt++;
                 float tu = 0.f, tv = 0.f, tt = 0.f;
                 bool h;
                 if (__float_as_int(p1.w) == kTri) {
-                    float4 p2 = p[2];
                     h = tri_test(mk(p0.x, p0.y, p0.z), mk(p1.x, p1.y, p1.z), mk(p2.x, p2.y, p2.z), t.o, t.d, t.tmin,
                                  t.tmax, &tu, &tv, &tt);
                 } else {
