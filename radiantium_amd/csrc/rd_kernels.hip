@@ -289,7 +289,8 @@ __device__ __forceinline__ bool scene_leaf(const DevScene& sc, const float4* __r
     while (s.li < s.le) {
         const float4* p = prims + 3 * s.li;
         s.li++;
-        float4 p0 = p[0], p1 = p[1];
+        float4 p0 = p[0], p1 = p[1], p2 = p[2];   // padding for spheres and instances
+        whole(p0); whole(p1); whole(p2);
         nt++;
         int kind = __float_as_int(p1.w);
         if (kind == kInst) {
@@ -299,7 +300,6 @@ __device__ __forceinline__ bool scene_leaf(const DevScene& sc, const float4* __r
         float tu = 0.f, tv = 0.f, tt = 0.f;
         bool h;
         if (kind == kTri) {
-            float4 p2 = p[2];
             h = tri_test(mk(p0.x, p0.y, p0.z), mk(p1.x, p1.y, p1.z), mk(p2.x, p2.y, p2.z), t.o, t.d, t.tmin, t.tmax,
                          &tu, &tv, &tt);
         } else {
@@ -326,6 +326,7 @@ __device__ __forceinline__ bool trav_step_inst(const DevScene& sc, const float4*
                                                uint32_t& nt) {
     Trav& t = s.t;
     float4 lo = nodes[2 * t.cur], hi = nodes[2 * t.cur + 1];
+    whole(lo); whole(hi);
     nv++;
     bool pop;
     if (box_hit(lo, hi, t.o, t.inv, t.nx, t.ny, t.nz, t.tmax)) {
@@ -342,6 +343,7 @@ __device__ __forceinline__ bool trav_step_inst(const DevScene& sc, const float4*
             for (int i = 0; i < cnt; i++) {
                 const float4* p = prims + 3 * (off + i);
                 float4 p0 = p[0], p1 = p[1], p2 = p[2];
+                whole(p0); whole(p1); whole(p2);
                 nt++;
                 float tu = 0.f, tv = 0.f, tt = 0.f;
                 if (tri_test(mk(p0.x, p0.y, p0.z), mk(p1.x, p1.y, p1.z), mk(p2.x, p2.y, p2.z), t.o, t.d, t.tmin,
