@@ -30,20 +30,22 @@ int persistent_blocks() { return g_num_cus * 8; }
 // BoundingBox3F.Intersect (BoundingBox3F.cs:155-176) on a node stored as
 // lo = {min.xyz, max.x}, hi = {max.y, max.z, ...}.  The reference's factor
 // 1 + 2*Gamma(3) is exactly 1.0f and is omitted.
+// Branch-free: the early returns become flags (once false, the later values
+// no longer matter), so a wavefront's lanes never split over the test.
 __device__ __forceinline__ bool box_hit(float4 lo, float4 hi, f3 o, f3 inv, bool nx, bool ny, bool nz, float ray_tmax) {
     float tmin = ((nx ? lo.w : lo.x) - o.x) * inv.x;
     float tmax = ((nx ? lo.x : lo.w) - o.x) * inv.x;
     float tymin = ((ny ? hi.x : lo.y) - o.y) * inv.y;
     float tymax = ((ny ? lo.y : hi.x) - o.y) * inv.y;
-    if (tmin > tymax || tymin > tmax) return false;
-    if (tymin > tmin) tmin = tymin;
-    if (tymax < tmax) tmax = tymax;
+    bool ok = !(tmin > tymax) & !(tymin > tmax);
+    tmin = tymin > tmin ? tymin : tmin;
+    tmax = tymax < tmax ? tymax : tmax;
     float tzmin = ((nz ? hi.y : lo.z) - o.z) * inv.z;
     float tzmax = ((nz ? lo.z : hi.y) - o.z) * inv.z;
-    if (tmin > tzmax || tzmin > tmax) return false;
-    if (tzmin > tmin) tmin = tzmin;
-    if (tzmax < tmax) tmax = tzmax;
-    return (tmin < ray_tmax) && (tmax > 0);
+    ok = ok & !(tmin > tzmax) & !(tzmin > tmax);
+    tmin = tzmin > tmin ? tzmin : tmin;
+    tmax = tzmax < tmax ? tzmax : tmax;
+    return ok & (tmin < ray_tmax) & (tmax > 0);
 }
 
 struct Hit {
