@@ -1171,18 +1171,32 @@ int resolve_crop(const rd_scene* s, const rd_render_params* p, Crop* out) {
     return RD_OK;
 }
 
-// Paths of one pass (their per-sample results stay in HBM until the pass's
-// accumulation): 2^27 paths = 2 GiB of results.
-constexpr size_t kPassPaths = (size_t)1 << 27;
+// Paths of one pass (their per-sample results, 16 B each, stay in HBM until
+// the pass's accumulation): as many as an eighth of the free device memory
+// holds, a power of two in [kMinPass, kMaxPass] (2^29 paths = 8 GiB of
+// results).  A pass ends with the drain of its last bounces, so larger passes
+// have fewer of those.
+constexpr size_t kMinPass = (size_t)1 << 24;
+constexpr size_t kMaxPass = (size_t)1 << 29;
+
+size_t pass_limit() {
+    size_t free_b = 0, total_b = 0;
+    size_t pass = kMinPass;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+        while (pass * 2 <= free_b / 8 / 16 && pass * 2 <= kMaxPass) pass *= 2;
+    return pass;
+}
+
 // Slot pool when the caller leaves wave_paths at 0: as many paths in flight
 // as a quarter of the device memory (and half of what is free) holds at
 // kPoolSlotBytes per slot (the path-state, queue and sort arrays of a slot,
-// vol_path's included), a power of two in [kMinPool, kPassPaths], never more
+// vol_path's included), a power of two in [kMinPool, kMaxPool], never more
 // than the render's paths.  Larger pools mean fewer, longer bounce launches,
 // and every launch ends with a tail in which the chip drains while its
 // longest rays finish: on the 1.31M-triangle 1080p workload 2^22 slots give
 // 1323 Mrays/s, 2^27 (every path of the frame at once, 50 GB) 1824.
 constexpr size_t kMinPool = (size_t)1 << 22;
+constexpr size_t kMaxPool = (size_t)1 << 27;
 constexpr size_t kPoolSlotBytes = 512;
 
 size_t default_pool(size_t paths) {
@@ -1190,7 +1204,7 @@ size_t default_pool(size_t paths) {
     size_t pool = kMinPool;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
         const size_t slots = std::min(total_b / 4, free_b / 2) / kPoolSlotBytes;
-        while (pool * 2 <= slots && pool * 2 <= kPassPaths) pool *= 2;
+        while (pool * 2 <= slots && pool * 2 <= kMaxPool) pool *= 2;
     }
     return std::min(pool, std::max<size_t>(paths, 1));
 }
@@ -1231,7 +1245,7 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
     }
     const size_t pool = pool_paths > 0 ? std::min((size_t)pool_paths, std::max<size_t>(total_paths, 1))
                                        : default_pool(total_paths);
-    const size_t pass_cap = std::max(std::min(kPassPaths, total_paths), max_npix);
+    const size_t pass_cap = std::max(std::min(pass_limit(), total_paths), max_npix);
     if (pass_cap > (size_t)INT32_MAX / 2 || total_pix > (size_t)INT32_MAX / 2) return fail(RD_ERR_INVALID, "render too large");
     bool any_vol = false;
     for (auto& k : kinds) any_vol = any_vol || k.first == RD_INTEGRATOR_VOL_PATH;
@@ -1385,7 +1399,7 @@ int rd_adjoint_device(rd_ctx* c, rd_scene* s, const rd_render_params* p, const f
     const size_t total = npix * (size_t)p->spp;
     const size_t pool = p->wave_paths > 0 ? std::min((size_t)p->wave_paths, std::max<size_t>(total, 1))
                                           : default_pool(total);
-    const size_t pass_cap = std::max(std::min(kPassPaths, total), npix);
+    const size_t pass_cap = std::max(std::min(pass_limit(), total), npix);
     if (int rc = ensure_wave_buffers(c, pool, pass_cap, npix, false)) return rc;
     if (int rc = ensure_adjoint_buffers(c, (size_t)s->num_textures)) return rc;
     WaveBuffers& wb = c->wb;
