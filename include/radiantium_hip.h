@@ -366,6 +366,16 @@ typedef struct rd_tuning {
 int rd_get_tuning(const rd_ctx* ctx, rd_tuning* out);
 int rd_set_tuning(rd_ctx* ctx, const rd_tuning* tuning);
 
+/* Autotuner: renders `params` (typically the production render with a few
+ * spp) under candidate tunings, one knob at a time from the current setting
+ * (extension and shadow schedules, the coherence sort and its key bits),
+ * keeps each knob's fastest value by device time (best of two renders, a
+ * change must gain 1 %), leaves the result set on the context and copies it
+ * to `out` (optional).  No counterpart in the reference.  Outputs of later
+ * renders do not depend on it; rd_get_render_stats afterwards describes the
+ * last trial render. */
+int rd_autotune(rd_ctx* ctx, rd_scene* scene, const rd_render_params* params, rd_tuning* out);
+
 /* ---- batched rendering: many scenes / sample counts in shared waves ----- */
 /* One request: a scene of this context, its render parameters (spp, seed,
  * crop; wave_paths and flags are ignored) and its output buffers (layout as

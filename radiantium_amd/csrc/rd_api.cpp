@@ -1633,6 +1633,63 @@ int rd_set_tuning(rd_ctx* c, const rd_tuning* t) {
     return RD_OK;
 }
 
+int rd_autotune(rd_ctx* c, rd_scene* s, const rd_render_params* p, rd_tuning* out) {
+    if (!c || !s || !p) return fail(RD_ERR_INVALID, "null argument");
+    if (p->spp <= 0) return fail(RD_ERR_INVALID, "spp must be positive");
+    {
+        CtxLock lock(c);
+        if (!lock.ok()) return fail(RD_ERR_BUSY, "context in use by another thread");
+        (void)hipSetDevice(c->device);
+        float* d = nullptr;
+        if (hipMalloc(&d, (size_t)s->width * s->height * 3 * sizeof(float)) != hipSuccess)
+            return fail(RD_ERR_OOM, "hipMalloc output");
+        rd_request r{};
+        r.scene = s;
+        r.params = *p;
+        r.out_rgb = d;
+        int rc = RD_OK;
+        auto time_it = [&]() -> double {   // device ms, best of two renders
+            double best = 1e300;
+            for (int k = 0; k < 2 && !rc; k++) {
+                rc = render_requests(c, 1, &r, true, p->wave_paths, nullptr, nullptr);
+                best = std::min(best, (double)c->stats.render_ms);
+            }
+            return best;
+        };
+        // each knob in turn: its values are tried with the others fixed and
+        // the fastest kept (a change must gain 1 %)
+        struct Knob {
+            int* field;
+            std::vector<int> values;
+        };
+        int sort = c->ray_sort ? 1 : 0;
+        std::vector<Knob> knobs = {{&c->trace_schedule[0], {0, 2, 3}},
+                                   {&c->trace_schedule[1], {0, 2, 3}},
+                                   {&sort, {0, 1}},
+                                   {&c->ray_sort_bits, {12, 15, 18}}};
+        rc = render_requests(c, 1, &r, true, p->wave_paths, nullptr, nullptr);   // buffers, code objects
+        double best = rc ? 0.0 : time_it();
+        for (Knob& k : knobs) {
+            if (rc) break;
+            if (k.field == &c->ray_sort_bits && !c->ray_sort) continue;
+            const int keep = *k.field;
+            int chosen = keep;
+            for (int v : k.values) {
+                if (v == keep || rc) continue;
+                *k.field = v;
+                c->ray_sort = sort != 0;
+                const double t = time_it();
+                if (!rc && t < best * 0.99) { best = t; chosen = v; }
+            }
+            *k.field = chosen;
+            c->ray_sort = sort != 0;
+        }
+        (void)hipFree(d);
+        if (rc) return rc;
+    }
+    return out ? rd_get_tuning(c, out) : RD_OK;
+}
+
 void* rd_device_alloc(rd_ctx* c, uint64_t bytes) {
     if (!c) return nullptr;
     (void)hipSetDevice(c->device);

@@ -22,7 +22,7 @@ EXPORTS = [
     "rd_synchronize", "rd_kernel_timing", "rd_reset_kernel_timing", "rd_render_batch", "rd_render_batch_device",
     "rd_adjoint", "rd_adjoint_device", "rd_scene_set_texture", "rd_scene_set_texture_device",
     "rd_trace_closest_sub", "rd_json_load", "rd_json_scene_desc", "rd_json_scene_free", "rd_scene_load_json",
-    "rd_metal_eta_k", "rd_get_tuning", "rd_set_tuning",
+    "rd_metal_eta_k", "rd_get_tuning", "rd_set_tuning", "rd_autotune",
 ]
 
 
@@ -91,6 +91,7 @@ def lib():
         L.rd_metal_eta_k.argtypes = [C.c_char_p, vp, vp]
         L.rd_get_tuning.argtypes = [vp, C.POINTER(abi.rd_tuning)]
         L.rd_set_tuning.argtypes = [vp, C.POINTER(abi.rd_tuning)]
+        L.rd_autotune.argtypes = [vp, vp, C.POINTER(abi.rd_render_params), C.POINTER(abi.rd_tuning)]
         _lib = L
     return _lib
 
@@ -256,6 +257,15 @@ class Context:
                      ("ray_sort_shadow", ray_sort_shadow)):
             setattr(t, f, -1 if v is None else int(v))
         check(lib().rd_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, scene, spp=4, seed=0, wave_paths=0):
+        """rd_autotune on `scene` rendered at `spp`; returns the tuning it
+        leaves on the context (as tuning())."""
+        p = abi.rd_render_params()
+        p.spp, p.seed, p.wave_paths = spp, seed, wave_paths
+        t = abi.rd_tuning()
+        check(lib().rd_autotune(self.h, scene.h, C.byref(p), C.byref(t)))
+        return self.tuning()
 
     def render_stats(self):
         st = abi.rd_render_stats()

@@ -83,3 +83,21 @@ def test_pool_size_bit_identical(ctx, which):
         assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), wp
         for k in KEYS:
             assert st[k] == rst[k], (wp, k, st[k], rst[k])
+
+
+def test_autotune(ctx):
+    """rd_autotune leaves a valid tuning on the context; renders after it are
+    the same as before it."""
+    base = ctx.tuning()
+    desc = scenes.displaced_icosphere_scene(level=5, width=512, height=288, max_depth=8)
+    sc = Scene(ctx, desc)
+    try:
+        ref, _ = sc.render(4, seed=11)
+        t = ctx.autotune(sc, spp=2)
+        print(f"[autotune] {t}", flush=True)
+        assert all(v in (0, 1, 2, 3) for v in t["trace_schedule"])
+        assert t["ray_sort"] in (0, 1) and 3 <= t["ray_sort_bits"] <= 30
+        img, _ = sc.render(4, seed=11)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+    finally:
+        ctx.set_tuning(**base)
