@@ -360,7 +360,11 @@ typedef struct rd_tuning {
     int32_t ray_sort_bits;     /* sort key bits, 3..30                      */
     int32_t ray_sort_min;      /* smallest extension queue that is sorted   */
     int32_t ray_sort_shadow;   /* shadow rays in the same order             */
-    int32_t reserved[8];       /* 0                                         */
+    int32_t overlap;           /* a bounce's shadow / MIS traversal and
+                                  resolve on a second stream beside the next
+                                  bounce's compaction, sort and extension
+                                  traversal (once every path has started)   */
+    int32_t reserved[7];       /* 0                                         */
 } rd_tuning;
 
 int rd_get_tuning(const rd_ctx* ctx, rd_tuning* out);

@@ -64,6 +64,7 @@ struct WaveBuffers {
     BatchSeg* segs = nullptr;       // device segment table
     BatchSeg* host_segs = nullptr;  // pinned staging for it
     int2* blk = nullptr;            // pool compaction scratch
+    int2* blk2 = nullptr;           // shadow-queue compaction scratch (second stream)
     size_t adj_cap = 0;             // adjoint per-slot arrays
     std::vector<void*> adj_allocs;
     void** grad_ptrs = nullptr;     // adjoint: device table of gradient accumulators
@@ -110,6 +111,10 @@ struct WaveBuffers {
 struct rd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream: a bounce's shadow / MIS traversal and resolve run on it
+    // beside the next bounce's compaction, sort and extension traversal
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_shaded = nullptr, ev_resolved = nullptr;
     std::mutex mu;
     int* counts = nullptr;        // device counters (kCntSlots ints)
     int* host_counts = nullptr;   // pinned mirror
@@ -127,6 +132,7 @@ struct rd_ctx {
     // (RD_TRACE_SCHEDULE="e,s,m"): chunks for the long extension and shadow
     // queries, whose costs vary widely between wavefronts
     int trace_schedule[3] = {2, 2, 2};
+    bool overlap = true;   // RD_OVERLAP=0 turns the two-stream bounce overlap off
     // scenes created on this context and not yet destroyed; rd_destroy with
     // live scenes releases the wavefront buffers and leaves the rest to the
     // last rd_scene_destroy, so destruction order does not matter
@@ -200,13 +206,14 @@ struct Timer {
         return p;
     }
     template <class F>
-    hipError_t run(const char* stage, F&& launch) {
+    hipError_t run(const char* stage, F&& launch, hipStream_t s = nullptr) {
         if (!ctx->timing_enabled) return launch();
+        if (!s) s = ctx->stream;
         EventPair p = get();
         p.stage = stage;
-        (void)hipEventRecord(p.a, ctx->stream);
+        (void)hipEventRecord(p.a, s);
         hipError_t e = launch();
-        (void)hipEventRecord(p.b, ctx->stream);
+        (void)hipEventRecord(p.b, s);
         pairs.push_back(p);
         return e;
     }
@@ -280,12 +287,14 @@ int ensure_wave_buffers(rd_ctx* c, size_t pool, size_t results, size_t pixels, b
         ps.ray_o = (float4*)alloc(cap * sizeof(float4));
         ps.ray_d = (float4*)alloc(cap * sizeof(float4));
         w.blk = (int2*)alloc(compact_scratch_int2(cap) * sizeof(int2));
+        w.blk2 = (int2*)alloc(compact_scratch_int2(cap) * sizeof(int2));
         ps.rng_n = (uint32_t*)alloc(cap * sizeof(uint32_t));
         ps.depth = (int*)alloc(cap * sizeof(int));
         ps.vis = (uint8_t*)alloc(cap);
         ps.hsub = (int*)alloc(cap * sizeof(int));
         ok = ok && w.hits && ps.L && ps.beta && ps.nee_beta && ps.nee_le && ps.mis_f && ps.mis_li && ps.flags &&
-             ps.sel_light && ps.key && ps.req && ps.gid && ps.pix && ps.alive && ps.ray_o && ps.ray_d && w.blk && ps.rng_n &&
+             ps.sel_light && ps.key && ps.req && ps.gid && ps.pix && ps.alive && ps.ray_o && ps.ray_d && w.blk && w.blk2 &&
+             ps.rng_n &&
              ps.depth && ps.vis && ps.hsub;
         ps.env = nullptr; ps.nee_li = ps.sh_to = ps.sh_tr = ps.mis_tr = nullptr;
         ps.hsub2 = nullptr;
@@ -423,13 +432,26 @@ rd_ctx* rd_create(int device) {
     if (const char* env = getenv("RD_RAY_SORT")) c->ray_sort = atoi(env) != 0;
     if (const char* env = getenv("RD_RAY_SORT_SHADOW")) c->ray_sort_shadow = atoi(env) != 0;
     if (const char* env = getenv("RD_RAY_SORT_BITS")) c->ray_sort_bits = std::max(3, std::min(30, atoi(env)));
+    if (const char* env = getenv("RD_OVERLAP")) c->overlap = atoi(env) != 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         g_err = "hipStreamCreate failed"; delete c; return nullptr;
+    }
+    if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_shaded, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_resolved, hipEventDisableTiming) != hipSuccess) {
+        g_err = "hipStreamCreate failed";
+        if (c->stream2) (void)hipStreamDestroy(c->stream2);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return nullptr;
     }
     if (hipMalloc(&c->counts, kCntSlots * sizeof(int)) != hipSuccess ||
         hipHostMalloc(&c->host_counts, kCntSlots * sizeof(int), hipHostMallocDefault) != hipSuccess) {
         g_err = "counter allocation failed";
         if (c->counts) (void)hipFree(c->counts);
+        (void)hipEventDestroy(c->ev_shaded);
+        (void)hipEventDestroy(c->ev_resolved);
+        (void)hipStreamDestroy(c->stream2);
         (void)hipStreamDestroy(c->stream);
         delete c;
         return nullptr;
@@ -441,6 +463,9 @@ rd_ctx* rd_create(int device) {
 static void free_ctx(rd_ctx* c) {
     if (c->counts) (void)hipFree(c->counts);
     if (c->host_counts) (void)hipHostFree(c->host_counts);
+    (void)hipEventDestroy(c->ev_shaded);
+    (void)hipEventDestroy(c->ev_resolved);
+    (void)hipStreamDestroy(c->stream2);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1035,6 +1060,7 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
     }));
     int rc = RD_OK;
     bool all_started = false;   // known on the host after each bounce's compaction
+    bool resolve_pending = false;   // the last bounce's resolve is still queued on stream2
     for (int bounce = 0; !rc; bounce++) {
         // the queue size is known on the host from the second bounce on
         const int* perm = nullptr;
@@ -1047,9 +1073,14 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
             }));
             perm = wb.sort_perm;
         }
+        // the counters the next stages start from; after an overlapped bounce
+        // stream2 clears the shadow / MIS ones behind its resolve, which
+        // still reads this queue's old count slot
+        const int za = resolve_pending ? -1 : cout, zb = resolve_pending ? -1 : kCntShadow;
+        const int zc = resolve_pending ? -1 : kCntMis;
         HIP_TRY(tm.run("trace_closest", [&] {
-            return launch_trace_ext(L.sc, L.scenes, qin, wb.ps, wb.hits, c->counts, cin, cout, kCntShadow, kCntMis,
-                                    perm, wb.sort_rank, c->trace_schedule[0], st);
+            return launch_trace_ext(L.sc, L.scenes, qin, wb.ps, wb.hits, c->counts, cin, za, zb, zc, perm,
+                                    wb.sort_rank, c->trace_schedule[0], st);
         }));
         // with a sorted extension queue the shadow / MIS rays follow its order
         const int* rank = (perm && c->ray_sort_shadow) ? wb.sort_rank : nullptr;
@@ -1088,35 +1119,57 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
             }
             HIP_TRY(tm.run("resolve", [&] { return launch_resolve_vol(qin, wb.ps, c->counts, cin, st); }));
         } else {
+            // shade adds to L, which the last bounce's resolve also writes
+            if (resolve_pending) {
+                HIP_TRY(hipStreamWaitEvent(st, c->ev_resolved, 0));
+                resolve_pending = false;
+            }
             HIP_TRY(tm.run("shade", [&] {
                 return launch_shade(L.sc, L.scenes, qin, wb.hits, wb.qshadow, wb.qmis, wb.ps, c->counts, cin,
                                     L.adj != nullptr, rank, st);
             }));
+            // Once every path of the pass has started, the next compaction
+            // only filters this queue by `alive` (set by shade), so the
+            // shadow / MIS traversal and resolve of this bounce (which touch
+            // neither `alive` nor the queues the next bounce builds) run on
+            // stream2 beside the next bounce's compaction, sort and
+            // extension traversal, filling each other's drain tails.
+            const bool overlap = c->overlap && all_started && !L.adj;
+            hipStream_t s2 = overlap ? c->stream2 : st;
+            if (overlap) {
+                HIP_TRY(hipEventRecord(c->ev_shaded, st));
+                HIP_TRY(hipStreamWaitEvent(s2, c->ev_shaded, 0));
+            }
             // a ranked shadow queue holds a hole for every path without a
             // shadow ray: trace it without them, in the same order
             RayQueue qsh = wb.qshadow;
             int csh = kCntShadow;
             if (rank && L.method != RD_METHOD_BXDF) {
                 HIP_TRY(tm.run("compact_shadow", [&] {
-                    return launch_compact_holes(wb.qshadow, c->host_counts[cin], wb.blk, wb.qshadow2, kCntShadowB,
-                                                c->counts, st);
-                }));
+                    return launch_compact_holes(wb.qshadow, c->host_counts[cin], overlap ? wb.blk2 : wb.blk,
+                                                wb.qshadow2, kCntShadowB, c->counts, s2);
+                }, s2));
                 qsh = wb.qshadow2;
                 csh = kCntShadowB;
             }
             if (L.method == RD_METHOD_MIS) {
                 HIP_TRY(tm.run("trace_secondary", [&] {
                     return launch_trace_secondary(L.sc, L.scenes, qsh, csh, wb.qmis, wb.ps, c->counts,
-                                                  c->trace_schedule[1], c->trace_schedule[2], st);
-                }));
+                                                  c->trace_schedule[1], c->trace_schedule[2], s2);
+                }, s2));
             } else if (L.method == RD_METHOD_NEE) {
                 HIP_TRY(tm.run("trace_any", [&] {
-                    return launch_trace_shadow(L.sc, L.scenes, qsh, wb.ps, c->counts, csh, c->trace_schedule[1], st);
-                }));
+                    return launch_trace_shadow(L.sc, L.scenes, qsh, wb.ps, c->counts, csh, c->trace_schedule[1], s2);
+                }, s2));
             }
             HIP_TRY(tm.run(L.adj ? "resolve_adjoint" : "resolve", [&] {
-                return launch_resolve(L.sc, L.method, qin, wb.ps, c->counts, cin, L.adj, st);
-            }));
+                return launch_resolve(L.sc, L.method, qin, wb.ps, c->counts, cin, L.adj, s2);
+            }, s2));
+            if (overlap) {
+                HIP_TRY(launch_zero_slots(c->counts, kCntShadow, kCntMis, s2));
+                HIP_TRY(hipEventRecord(c->ev_resolved, s2));
+                resolve_pending = true;
+            }
         }
         HIP_TRY(tm.run("compact", [&] {
             // once every path of the pass has started, only the entries of
@@ -1138,6 +1191,7 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
         std::swap(qin, qout);
         std::swap(cin, cout);
     }
+    if (resolve_pending) HIP_TRY(hipStreamWaitEvent(st, c->ev_resolved, 0));
     return rc;
 }
 
@@ -1611,6 +1665,7 @@ int rd_get_tuning(const rd_ctx* c, rd_tuning* t) {
     t->ray_sort_bits = c->ray_sort_bits;
     t->ray_sort_min = c->ray_sort_min;
     t->ray_sort_shadow = c->ray_sort_shadow ? 1 : 0;
+    t->overlap = c->overlap ? 1 : 0;
     return RD_OK;
 }
 
@@ -1630,6 +1685,7 @@ int rd_set_tuning(rd_ctx* c, const rd_tuning* t) {
     if (t->ray_sort_bits >= 0) c->ray_sort_bits = t->ray_sort_bits;
     if (t->ray_sort_min >= 0) c->ray_sort_min = t->ray_sort_min;
     if (t->ray_sort_shadow >= 0) c->ray_sort_shadow = t->ray_sort_shadow != 0;
+    if (t->overlap >= 0) c->overlap = t->overlap != 0;
     return RD_OK;
 }
 
@@ -1662,11 +1718,12 @@ int rd_autotune(rd_ctx* c, rd_scene* s, const rd_render_params* p, rd_tuning* ou
             int* field;
             std::vector<int> values;
         };
-        int sort = c->ray_sort ? 1 : 0;
+        int sort = c->ray_sort ? 1 : 0, ovl = c->overlap ? 1 : 0;
         std::vector<Knob> knobs = {{&c->trace_schedule[0], {0, 2, 3}},
                                    {&c->trace_schedule[1], {0, 2, 3}},
                                    {&sort, {0, 1}},
-                                   {&c->ray_sort_bits, {12, 15, 18}}};
+                                   {&c->ray_sort_bits, {12, 15, 18}},
+                                   {&ovl, {0, 1}}};
         rc = render_requests(c, 1, &r, true, p->wave_paths, nullptr, nullptr);   // buffers, code objects
         double best = rc ? 0.0 : time_it();
         for (Knob& k : knobs) {
@@ -1678,11 +1735,13 @@ int rd_autotune(rd_ctx* c, rd_scene* s, const rd_render_params* p, rd_tuning* ou
                 if (v == keep || rc) continue;
                 *k.field = v;
                 c->ray_sort = sort != 0;
+                c->overlap = ovl != 0;
                 const double t = time_it();
                 if (!rc && t < best * 0.99) { best = t; chosen = v; }
             }
             *k.field = chosen;
             c->ray_sort = sort != 0;
+            c->overlap = ovl != 0;
         }
         (void)hipFree(d);
         if (rc) return rc;

@@ -182,6 +182,7 @@ hipError_t launch_finalize(const WaveParams& wp, int height, int spp, const floa
                            float* out_rgb, float* out_sum2, hipStream_t s);
 hipError_t launch_zero(float4* buf, int n, hipStream_t s);
 hipError_t launch_zero_counts(int* counts, int n, hipStream_t s);
+hipError_t launch_zero_slots(int* counts, int a, int b, hipStream_t s);
 
 // Traversal schedules (the `sched` argument of the queue launchers):
 // 0 = straight (grid-stride, one ray per lane to completion),

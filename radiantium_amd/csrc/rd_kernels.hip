@@ -779,6 +779,15 @@ __global__ void k_zero_slot(int* c, int slot) {
     if (threadIdx.x == 0) c[slot] = 0;
 }
 
+__global__ void k_zero_slots(int* c, int a, int b) {
+    if (threadIdx.x == 0) { c[a] = 0; c[b] = 0; }
+}
+
+hipError_t launch_zero_slots(int* counts, int a, int b, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_slots, dim3(1), dim3(64), 0, s, counts, a, b);
+    return hipGetLastError();
+}
+
 hipError_t launch_trace_rays(const DevScene& sc, const float4* rays, int n, float4* hits, uint8_t* occ, int* sub,
                              unsigned long long* counters, bool any, int* counts, hipStream_t s) {
     if (n <= 0) return hipSuccess;

@@ -30,7 +30,7 @@ def test_schedules_bit_identical(ctx, which):
     base = ctx.tuning()
     sc = Scene(ctx, desc)
     try:
-        ref, rst = _render(ctx, sc, spp, trace_schedule=(0, 0, 0), ray_sort=1, ray_sort_shadow=1)
+        ref, rst = _render(ctx, sc, spp, trace_schedule=(0, 0, 0), ray_sort=1, ray_sort_shadow=1, overlap=1)
         print(f"[schedules] {which}: class rays {rst['class_rays']}", flush=True)
         assert rst["paths"] > 2_000_000 and max(rst["class_rays"]) > 1_000_000
         for sched in ((1, 1, 1), (2, 2, 2), (2, 2, 1), (0, 1, 2), (3, 3, 3), (3, 2, 0)):
@@ -39,6 +39,12 @@ def test_schedules_bit_identical(ctx, which):
             assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), sched
             for k in KEYS:
                 assert st[k] == rst[k], (sched, k, st[k], rst[k])
+        # no overlap of a bounce's shadow / MIS stage with the next bounce
+        img, st = _render(ctx, sc, spp, trace_schedule=(2, 2, 2), overlap=0)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+        for k in KEYS:
+            assert st[k] == rst[k], ("no overlap", k)
+        ctx.set_tuning(overlap=1)
         # queue order instead of the coherence order: same rays, same image
         img, st = _render(ctx, sc, spp, trace_schedule=(2, 2, 2), ray_sort=0)
         assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
