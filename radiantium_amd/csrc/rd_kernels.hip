@@ -577,42 +577,7 @@ __global__ __launch_bounds__(kBlock, lds_blocks(kStack, kBlock)) void k_trace_se
             nr_s++;
         }
     };
-    if (a.dyn == 2 && !kBatch) {
-        // sequences of kSeq rays per lane, as k_trace_seq
-        for (int c = next_chunk(cursor, 64 * kSeq); c < n; c = next_chunk(cursor, 64 * kSeq)) {
-            int j = 0, r = 0, pid = 0;
-            bool mis = false;
-            Trav t;
-            auto start = [&]() {
-                while (j < kSeq) {
-                    const int r0 = c + __lane_id() + 64 * j;
-                    j++;
-                    if (r0 >= n) break;
-                    mis = r0 < n1;
-                    r = mis ? r0 : r0 - n1;
-                    const RayQueue& q = mis ? a.q : a.q2;
-                    pid = q.pid[r];
-                    if (pid < 0) continue;
-                    trav_begin(t, q.org[r], q.dir[r]);
-                    return true;
-                }
-                j = kSeq;
-                return false;
-            };
-            bool active = start();
-            while (__ballot(active)) {
-                if (active) {
-                    const bool done = mis ? trav_step<false, kBlock>(sc.nodes, sc.prims, t, stk, nv_m, nt_m)
-                                          : trav_step<true, kBlock>(sc.nodes, sc.prims, t, stk, nv_s, nt_s);
-                    if (done) {
-                        if (mis) { trav_finish<kModeMis>(sc, a, r, t); nr_m++; }
-                        else { a.ps.vis[pid] = t.best.prim >= 0 ? 0 : 1; nr_s++; }
-                        active = start();
-                    }
-                }
-            }
-        }
-    } else if (a.dyn) {
+    if (a.dyn) {
         for (int c = next_chunk(cursor); c < n; c = next_chunk(cursor))
             if (c + __lane_id() < n) one(c + __lane_id());
     } else {
@@ -1190,7 +1155,7 @@ hipError_t launch_trace_shadow(const DevScene& sc, const DevScene* scenes, RayQu
 hipError_t launch_trace_secondary(const DevScene& sc, const DevScene* scenes, RayQueue qshadow, int cnt_shadow,
                                   RayQueue qmis, PathState ps, int* counts, int sched_shadow, int sched_mis,
                                   hipStream_t s) {
-    if (sc.num_insts > 0 || sched_shadow == 1 || sched_mis == 1) {
+    if (sc.num_insts > 0 || sched_shadow == 1 || sched_mis == 1 || sched_shadow == 3 || sched_mis == 3) {
         if (hipError_t e = launch_trace_mis(sc, scenes, qmis, ps, counts, sched_mis, s)) return e;
         return launch_trace_shadow(sc, scenes, qshadow, ps, counts, cnt_shadow, sched_shadow, s);
     }
