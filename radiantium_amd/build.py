@@ -18,8 +18,11 @@ ARCH = os.environ.get("RD_OFFLOAD_ARCH", "gfx950")
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
           "-Wno-unused-variable"]
-DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-fhip-fp32-correctly-rounded-divide-sqrt",
+# hipcc compiles the .cpp host sources as HIP too: they get the same target
+# (without it hipcc adds a code object for its default architecture)
+TARGET = [f"--offload-arch={ARCH}", "-fhip-fp32-correctly-rounded-divide-sqrt",
           "-fno-gpu-flush-denormals-to-zero", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+DEVICE = ["-x", "hip"] + TARGET
 HOST_SOURCES = ["rd_api.cpp", "rd_camera.cpp", "rd_bvh_build.cpp", "rd_envmap.cpp", "rd_json.cpp"]
 DEVICE_SOURCES = ["rd_kernels.hip", "rd_volpath.hip", "rd_sort.hip"]
 
@@ -38,7 +41,7 @@ def build(verbose=False):
     jobs = []
     for src in HOST_SOURCES + DEVICE_SOURCES:
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-        cmd = [cc] + COMMON + (DEVICE if src in DEVICE_SOURCES else []) + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [cc] + COMMON + (DEVICE if src in DEVICE_SOURCES else TARGET) + ["-c", os.path.join(CSRC, src), "-o", obj]
         jobs.append((cmd, src))
         objs.append(obj)
     procs = [(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT), src) for cmd, src in jobs]

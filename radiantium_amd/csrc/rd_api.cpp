@@ -353,7 +353,11 @@ int ensure_sort_buffers(rd_ctx* c, size_t pool) {
     w.sort_perm = (int*)alloc(pool * 4);
     w.sort_rank = (int*)alloc(pool * 4);
     w.sort_temp_bytes = ray_sort_temp_bytes((int)pool);
-    w.sort_temp = alloc(std::max<size_t>(w.sort_temp_bytes, 16));
+    if (w.sort_temp_bytes == 0) {
+        WaveBuffers::free_list(w.sort_allocs);
+        return fail(RD_ERR_DEVICE, "ray sort workspace size query failed");
+    }
+    w.sort_temp = alloc(w.sort_temp_bytes);
     if (!w.sort_keys || !w.sort_keys_alt || !w.sort_vals || !w.sort_perm || !w.sort_rank || !w.sort_temp) {
         WaveBuffers::free_list(w.sort_allocs);
         return fail(RD_ERR_OOM, "hipMalloc failed for the ray sort");
@@ -695,7 +699,6 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     }
     std::vector<int> set_root(nsets);
     int blas_depth = 0;
-    size_t blas_nodes = 0;
     for (int si = 0; si < nsets; si++) {
         const rd_instance_set_desc& sd = d->instance_sets[si];
         const BvhResult& b = blas[si];
@@ -719,7 +722,6 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
             di.flags = d->tri_flags ? d->tri_flags[tri] : 0;
         }
         blas_depth = std::max(blas_depth, b.max_depth);
-        blas_nodes += b.nodes.size();
     }
     std::vector<DevInstance> insts(d->num_instances);
     for (int i = 0; i < d->num_instances; i++) {
@@ -1082,8 +1084,11 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
             return launch_trace_ext(L.sc, L.scenes, qin, wb.ps, wb.hits, c->counts, cin, za, zb, zc, perm,
                                     wb.sort_rank, c->trace_schedule[0], st);
         }));
-        // with a sorted extension queue the shadow / MIS rays follow its order
+        // the shadow rays follow the order the extension rays were traced in:
+        // the sorted order's ranks, or queue order (camera rays in pixel
+        // tiles at the first bounce) where there was no sort
         const int* rank = (perm && c->ray_sort_shadow) ? wb.sort_rank : nullptr;
+        const bool rank_identity = !perm && c->ray_sort_shadow && L.method != RD_METHOD_BXDF;
         if (L.vol) {
             // VolumetricPathTracer.Li: shade, then the Transmittance / IntersectTr
             // chains step by step until both chain queues drain
@@ -1126,7 +1131,7 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
             }
             HIP_TRY(tm.run("shade", [&] {
                 return launch_shade(L.sc, L.scenes, qin, wb.hits, wb.qshadow, wb.qmis, wb.ps, c->counts, cin,
-                                    L.adj != nullptr, rank, st);
+                                    L.adj != nullptr, rank, rank_identity, st);
             }));
             // Once every path of the pass has started, the next compaction
             // only filters this queue by `alive` (set by shade), so the
@@ -1144,9 +1149,11 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
             // shadow ray: trace it without them, in the same order
             RayQueue qsh = wb.qshadow;
             int csh = kCntShadow;
-            if (rank && L.method != RD_METHOD_BXDF) {
+            if ((rank || rank_identity) && L.method != RD_METHOD_BXDF) {
+                // the queue size is on the host from the second bounce on
+                const int nmax = bounce > 0 ? c->host_counts[cin] : L.pool;
                 HIP_TRY(tm.run("compact_shadow", [&] {
-                    return launch_compact_holes(wb.qshadow, c->host_counts[cin], overlap ? wb.blk2 : wb.blk,
+                    return launch_compact_holes(wb.qshadow, nmax, c->counts + cin, overlap ? wb.blk2 : wb.blk,
                                                 wb.qshadow2, kCntShadowB, c->counts, s2);
                 }, s2));
                 qsh = wb.qshadow2;

@@ -136,7 +136,7 @@ hipError_t launch_trace_ext(const DevScene& sc, const DevScene* scenes, RayQueue
 // origin's Morton code in the root box of `nodes`, `bits` key bits in all,
 // 3..30): perm receives the queue
 // positions in traversal order.  keys / keys_alt / vals: n each; temp of
-// ray_sort_temp_bytes(>= n) bytes.
+// ray_sort_temp_bytes(>= n) bytes (0: the size query failed).
 size_t ray_sort_temp_bytes(int max_n);
 hipError_t launch_ray_sort(const float4* nodes, RayQueue q, int n, int bits, uint32_t* keys, uint32_t* keys_alt,
                            int* vals, int* perm, void* temp, size_t temp_bytes, hipStream_t s);
@@ -145,9 +145,10 @@ hipError_t launch_ray_sort(const float4* nodes, RayQueue q, int n, int bits, uin
 // rank (null = append): write each path's shadow ray at rank[queue
 // position], holes pid -1, shadow queue size = the extension queue's; MIS
 // rays are appended either way.
+// rank_identity (rank null): ranked with every ray at its own queue position.
 hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
                         RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in, bool adjoint,
-                        const int* rank, hipStream_t s);
+                        const int* rank, bool rank_identity, hipStream_t s);
 hipError_t launch_accumulate_batch(const BatchSeg* segs, int nseg, int max_npix, PathState ps, float4* sum,
                                    float4* sum2, int* counts, hipStream_t s);
 hipError_t launch_trace_shadow(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,
@@ -159,8 +160,10 @@ hipError_t launch_trace_secondary(const DevScene& sc, const DevScene* scenes, Ra
                                   RayQueue qmis, PathState ps, int* counts, int sched_shadow, int sched_mis,
                                   hipStream_t s);
 // The n entries of a ranked queue that hold a ray, in order, into q (count at
-// cnt_q); blk as for launch_compact.
-hipError_t launch_compact_holes(RayQueue qin, int n, int2* blk, RayQueue q, int cnt_q, int* counts, hipStream_t s);
+// cnt_q); blk as for launch_compact.  dev_n (optional): the queue size on the
+// device, n then being an upper bound.
+hipError_t launch_compact_holes(RayQueue qin, int n, const int* dev_n, int2* blk, RayQueue q, int cnt_q, int* counts,
+                                hipStream_t s);
 // adj: null for rendering; the adjoint replay's gradient inputs otherwise.
 hipError_t launch_resolve(const DevScene& sc, int method, RayQueue qin, PathState ps, int* counts, int cnt_in,
                           const AdjArgs* adj, hipStream_t s);

@@ -970,8 +970,9 @@ __global__ __launch_bounds__(256) void k_queue_count(const int* __restrict__ pid
 
 // Hole removal: the entries of a ranked queue that hold a ray (pid >= 0), in
 // the same order.  Counts per 1024 entries.
-__global__ __launch_bounds__(256) void k_hole_count(const int* __restrict__ pid, int n, int2* blk) {
+__global__ __launch_bounds__(256) void k_hole_count(const int* __restrict__ pid, int n, const int* dev_n, int2* blk) {
     __shared__ int tmp[4];
+    if (dev_n) n = min(n, *dev_n);
     const int base = blockIdx.x * kCompactSlots;
     int na = 0;
     for (int i = 0; i < kCompactSlots / 256; i++) {
@@ -983,8 +984,10 @@ __global__ __launch_bounds__(256) void k_hole_count(const int* __restrict__ pid,
     if (threadIdx.x == 0) blk[blockIdx.x] = make_int2(ta, 0);
 }
 
-__global__ __launch_bounds__(256) void k_hole_scatter(RayQueue qin, int n, const int2* __restrict__ off, RayQueue q) {
+__global__ __launch_bounds__(256) void k_hole_scatter(RayQueue qin, int n, const int* dev_n, const int2* __restrict__ off,
+                                                      RayQueue q) {
     __shared__ int tmp[4];
+    if (dev_n) n = min(n, *dev_n);
     const int base = blockIdx.x * kCompactSlots;
     int pos_base = off[blockIdx.x].y;
     for (int i = 0; i < kCompactSlots / 256; i++) {
@@ -1086,16 +1089,17 @@ hipError_t launch_compact_queue(RayQueue qin, int n, int npass, int2* blk, RayQu
     return hipGetLastError();
 }
 
-hipError_t launch_compact_holes(RayQueue qin, int n, int2* blk, RayQueue q, int cnt_q, int* counts, hipStream_t s) {
+hipError_t launch_compact_holes(RayQueue qin, int n, const int* dev_n, int2* blk, RayQueue q, int cnt_q, int* counts,
+                                hipStream_t s) {
     if (n <= 0) {
         hipLaunchKernelGGL(k_zero_slot, dim3(1), dim3(64), 0, s, counts, cnt_q);
         return hipGetLastError();
     }
     const int nblk = (n + kCompactSlots - 1) / kCompactSlots;
     int2* off = blk + nblk;
-    hipLaunchKernelGGL(k_hole_count, dim3(nblk), dim3(256), 0, s, qin.pid, n, blk);
+    hipLaunchKernelGGL(k_hole_count, dim3(nblk), dim3(256), 0, s, qin.pid, n, dev_n, blk);
     scan_blocks(blk, nblk, 0, counts, cnt_q, off, false, s);
-    hipLaunchKernelGGL(k_hole_scatter, dim3(nblk), dim3(256), 0, s, qin, n, off, q);
+    hipLaunchKernelGGL(k_hole_scatter, dim3(nblk), dim3(256), 0, s, qin, n, dev_n, off, q);
     return hipGetLastError();
 }
 
@@ -1194,7 +1198,8 @@ hipError_t launch_trace_chain(const DevScene& sc, const DevScene* scenes, RayQue
 template <bool kBatch, bool kAdj>
 __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* __restrict__ scenes, RayQueue qin,
                                                const float4* __restrict__ hits, RayQueue qshadow, RayQueue qmis,
-                                               PathState ps, int* counts, int cnt_in, const int* __restrict__ rank) {
+                                               PathState ps, int* counts, int cnt_in, const int* __restrict__ rank,
+                                               bool rank_identity) {
     int n = counts[cnt_in];
     // the traversal work cursors restart at zero for the next trace stages
     if (blockIdx.x == 0 && threadIdx.x < 3) counts[kCntWork + threadIdx.x] = 0;
@@ -1380,11 +1385,12 @@ __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* 
                 ps.ray_d[pid] = make_float4(ed.x, ed.y, ed.z, etmax);
             }
         }
-        if (rank) {
+        if (rank || rank_identity) {
             // ranked: each path's shadow ray sits at its rank in the coherence
-            // order of the extension queue, holes marked pid -1
+            // order of the extension queue (its own position when that queue
+            // is traced in queue order), holes marked pid -1
             if (k < n) {
-                const int r = rank[k];
+                const int r = rank ? rank[k] : k;
                 qshadow.pid[r] = want_shadow ? pid : -1;
                 if (want_shadow) {
                     qshadow.org[r] = make_float4(so.x, so.y, so.z, 0.0001f);
@@ -1408,22 +1414,22 @@ __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* 
             qmis.dir[slot] = make_float4(md.x, md.y, md.z, kFloatMax);
         }
     }
-    if (rank && blockIdx.x == 0 && threadIdx.x == 0) counts[kCntShadow] = n;
+    if ((rank || rank_identity) && blockIdx.x == 0 && threadIdx.x == 0) counts[kCntShadow] = n;
     add_counter((unsigned long long*)(counts + kCntVertices), nvert);
 }
 
 hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
                         RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in, bool adjoint,
-                        const int* rank, hipStream_t s) {
+                        const int* rank, bool rank_identity, hipStream_t s) {
     if (scenes)
         hipLaunchKernelGGL((k_shade<true, false>), dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
-                           qshadow, qmis, ps, counts, cnt_in, rank);
+                           qshadow, qmis, ps, counts, cnt_in, rank, rank_identity);
     else if (adjoint)
         hipLaunchKernelGGL((k_shade<false, true>), dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
-                           qshadow, qmis, ps, counts, cnt_in, rank);
+                           qshadow, qmis, ps, counts, cnt_in, rank, rank_identity);
     else
         hipLaunchKernelGGL((k_shade<false, false>), dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
-                           qshadow, qmis, ps, counts, cnt_in, rank);
+                           qshadow, qmis, ps, counts, cnt_in, rank, rank_identity);
     return hipGetLastError();
 }
 

@@ -46,8 +46,9 @@ __global__ __launch_bounds__(256) void k_ray_keys(const float4* __restrict__ nod
 
 size_t ray_sort_temp_bytes(int max_n) {
     size_t bytes = 0;
-    rocprim::radix_sort_pairs((void*)nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr, (const int*)nullptr,
-                              (int*)nullptr, max_n, 0, 30, (hipStream_t)0);
+    if (rocprim::radix_sort_pairs((void*)nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                  (const int*)nullptr, (int*)nullptr, max_n, 0, 30, (hipStream_t)0) != hipSuccess)
+        return 0;
     return bytes;
 }
 
