@@ -129,9 +129,11 @@ struct rd_ctx {
     int ray_sort_min = 1 << 16;   // smaller queues are traced in queue order
     bool ray_sort_shadow = true;  // shadow queue in that order too (RD_RAY_SORT_SHADOW)
     // traversal schedule per query class [extension, shadow, MIS]
-    // (RD_TRACE_SCHEDULE="e,s,m"): chunks for the long extension and shadow
-    // queries, whose costs vary widely between wavefronts
-    int trace_schedule[3] = {2, 2, 2};
+    // (RD_TRACE_SCHEDULE="e,s,m"): sequences for the extension rays (in the
+    // tile-major path order a lane's four rays are four samples of one pixel
+    // at the first bounce, so the later three find their nodes in L1), chunks
+    // for the shadow and MIS rays (which the fused launch then traces together)
+    int trace_schedule[3] = {3, 2, 2};
     bool overlap = true;   // RD_OVERLAP=0 turns the two-stream bounce overlap off
     // scenes created on this context and not yet destroyed; rd_destroy with
     // live scenes releases the wavefront buffers and leaves the rest to the

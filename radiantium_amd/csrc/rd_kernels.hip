@@ -780,7 +780,7 @@ __device__ __forceinline__ f3 xform_vector(const float* M, f3 v) {
 // paths that finished are refilled with the next samples of the pass, so the
 // bounce kernels see a full queue until the pass runs dry.  Pass path g lives
 // in the segment whose [path_offset, path_offset + nsamples * npix) holds it
-// (sample-major within a segment, as a single render's sample loop).
+// (tile-major within a segment: seg_path below).
 __device__ __forceinline__ int find_seg(const BatchSeg* segs, int nseg, int p) {
     int lo = 0, hi = nseg - 1;
     while (lo < hi) {
@@ -809,14 +809,45 @@ __device__ __forceinline__ void tile_pixel(int q, int w, int h, int* px, int* py
     *py = tr * 8 + o / tw;
 }
 
+// Path order inside a segment: tile by tile (tile_pixel's order), and inside
+// a tile, sample by sample with the tile's pixels in order.  A wavefront's 64
+// consecutive paths are one 8x8 tile at one sample, as in a sample-by-sample
+// sweep, but all samples of a tile now run back to back: the chip's rays in
+// flight cover about a hundred tiles instead of a band of the image per
+// sample, so the BVH nodes and triangles under them are fetched into L2 once
+// rather than once per sample.  seg_path: segment-local path index ->
+// (pixel q in tile order, sample si); seg_local is its inverse.
+__device__ __forceinline__ void seg_path(int local, int w, int h, int ns, int* q, int* si) {
+    // a segment holds fewer than 2^31 paths, but a full tile row of a crop a
+    // few pixels tall can be counted past that
+    const long long row = 8LL * w * ns;         // paths per full tile row
+    const int tr = (int)(local / row);
+    const int th = min(8, h - tr * 8);
+    const int r = (int)(local - tr * row);
+    const int ntc = (w + 7) / 8;
+    const int j = (int)min((long long)r / (8LL * th * ns), (long long)(ntc - 1));
+    const int o = (int)(r - (long long)j * 8 * th * ns);
+    const int nt = min(8, w - j * 8) * th;      // pixels in the tile
+    *si = o / nt;
+    *q = tr * 8 * w + j * 8 * th + (o - *si * nt);
+}
+__device__ __forceinline__ int seg_local(int q, int si, int w, int h, int ns) {
+    const int tr = q / (8 * w);
+    const int th = min(8, h - tr * 8);
+    const int r = q - tr * 8 * w;
+    const int ntc = (w + 7) / 8;
+    const int j = min(r / (8 * th), ntc - 1);
+    const int o = r - j * 8 * th;               // pixel within the tile
+    const int nt = min(8, w - j * 8) * th;
+    return (q - o) * ns + si * nt + o;
+}
+
 // Pixel sample (x,y)+U^2 and PerspectiveCamera.SampleRay (Camera.cs:182-191),
 // BlockBasedRenderer.RenderTask (BlockBasedRenderer.cs:136-139), into slot `slot`.
 __device__ __forceinline__ void start_path(const DevScene& sc, const BatchSeg& sg, int g, int slot, int qi, RayQueue q,
                                            PathState ps) {
-    int local = g - sg.path_offset;
-    int si = local / sg.npix;
-    int qq = local - si * sg.npix;
-    int tx, ty;
+    int si, qq, tx, ty;
+    seg_path(g - sg.path_offset, sg.crop_w, sg.npix / sg.crop_w, sg.nsamples, &qq, &si);
     tile_pixel(qq, sg.crop_w, sg.npix / sg.crop_w, &tx, &ty);
     int x = sg.x0 + tx, y = sg.y0 + ty;
     const uint64_t key = path_key(sg.seed, (uint32_t)(y * sg.width + x), (uint32_t)(sg.sample_base + si));
@@ -1543,12 +1574,15 @@ __global__ __launch_bounds__(256) void k_accumulate_batch(const BatchSeg* segs, 
     uint64_t invalid = 0;
     for (int g = blockIdx.y; g < nseg; g += gridDim.y) {
         const BatchSeg sg = segs[g];
+        const int h = sg.npix / sg.crop_w;
         for (int q = blockIdx.x * 256 + threadIdx.x; q < sg.npix; q += gridDim.x * 256) {
             int o = sg.sum_offset + q;
             float4 s4 = sum[o], q4 = sum2[o];
             f3 s = mk(s4.x, s4.y, s4.z), s2 = mk(q4.x, q4.y, q4.z);
+            const int p0 = sg.path_offset + seg_local(q, 0, sg.crop_w, h, sg.nsamples);
+            const int step = seg_local(q, 1, sg.crop_w, h, sg.nsamples) - seg_local(q, 0, sg.crop_w, h, sg.nsamples);
             for (int j = 0; j < sg.nsamples; j++) {
-                f3 c = ld3(ps.res, sg.path_offset + j * sg.npix + q);
+                f3 c = ld3(ps.res, p0 + j * step);
                 if (!valid_color(c)) { c = mk(0.0f); invalid++; }
                 s = s + c;
                 s2 = s2 + c * c;
