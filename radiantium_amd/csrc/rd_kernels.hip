@@ -578,8 +578,12 @@ __global__ __launch_bounds__(kBlock, lds_blocks(kStack, kBlock)) void k_trace_se
         }
     };
     if (a.dyn) {
-        for (int c = next_chunk(cursor); c < n; c = next_chunk(cursor))
-            if (c + __lane_id() < n) one(c + __lane_id());
+        // 256-ray chunks traced as four rounds of 64: the rounds are the
+        // neighbouring rays of the shadow order (at the first bounce the
+        // same tile's next samples), whose nodes the previous round left in L1
+        for (int c = next_chunk(cursor, 256); c < n; c = next_chunk(cursor, 256))
+            for (int k = 0; k < 4; k++)
+                if (c + 64 * k + __lane_id() < n) one(c + 64 * k + __lane_id());
     } else {
         for (int r0 = blockIdx.x * kBlock + threadIdx.x; r0 < n; r0 += gridDim.x * kBlock) one(r0);
     }
