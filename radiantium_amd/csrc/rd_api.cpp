@@ -1062,8 +1062,15 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
     HIP_TRY(tm.run("compact", [&] {
         return launch_compact(L.sc, L.scenes, L.segs, L.nseg, L.npass, L.pool, wb.blk, qin, cin, wb.ps, c->counts, st);
     }));
+    // the first queue's size: the first bounce's queue filter and shadow-hole
+    // compaction are sized on the host
+    HIP_TRY(hipMemcpyAsync(c->host_counts + cin, c->counts + cin, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
     int rc = RD_OK;
-    bool all_started = false;   // known on the host after each bounce's compaction
+    // known on the host after each bounce's compaction; the first one starts
+    // a path in every slot (all are free), so it starts them all when the
+    // pass fits the pool
+    bool all_started = L.npass <= L.pool;
     bool resolve_pending = false;   // the last bounce's resolve is still queued on stream2
     for (int bounce = 0; !rc; bounce++) {
         // the queue size is known on the host from the second bounce on
@@ -1141,7 +1148,11 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
             // neither `alive` nor the queues the next bounce builds) run on
             // stream2 beside the next bounce's compaction, sort and
             // extension traversal, filling each other's drain tails.
-            const bool overlap = c->overlap && all_started && !L.adj;
+            // (not at the first bounce: its MIS + shadow launch is the
+            // frame's longest after the first extension trace, and beside the
+            // second bounce's extension trace both lose more to each other
+            // than the overlap fills)
+            const bool overlap = c->overlap && all_started && bounce > 0 && !L.adj;
             hipStream_t s2 = overlap ? c->stream2 : st;
             if (overlap) {
                 HIP_TRY(hipEventRecord(c->ev_shaded, st));
@@ -1152,10 +1163,8 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
             RayQueue qsh = wb.qshadow;
             int csh = kCntShadow;
             if ((rank || rank_identity) && L.method != RD_METHOD_BXDF) {
-                // the queue size is on the host from the second bounce on
-                const int nmax = bounce > 0 ? c->host_counts[cin] : L.pool;
                 HIP_TRY(tm.run("compact_shadow", [&] {
-                    return launch_compact_holes(wb.qshadow, nmax, c->counts + cin, overlap ? wb.blk2 : wb.blk,
+                    return launch_compact_holes(wb.qshadow, c->host_counts[cin], overlap ? wb.blk2 : wb.blk,
                                                 wb.qshadow2, kCntShadowB, c->counts, s2);
                 }, s2));
                 qsh = wb.qshadow2;

@@ -160,10 +160,8 @@ hipError_t launch_trace_secondary(const DevScene& sc, const DevScene* scenes, Ra
                                   RayQueue qmis, PathState ps, int* counts, int sched_shadow, int sched_mis,
                                   hipStream_t s);
 // The n entries of a ranked queue that hold a ray, in order, into q (count at
-// cnt_q); blk as for launch_compact.  dev_n (optional): the queue size on the
-// device, n then being an upper bound.
-hipError_t launch_compact_holes(RayQueue qin, int n, const int* dev_n, int2* blk, RayQueue q, int cnt_q, int* counts,
-                                hipStream_t s);
+// cnt_q); blk as for launch_compact.
+hipError_t launch_compact_holes(RayQueue qin, int n, int2* blk, RayQueue q, int cnt_q, int* counts, hipStream_t s);
 // adj: null for rendering; the adjoint replay's gradient inputs otherwise.
 hipError_t launch_resolve(const DevScene& sc, int method, RayQueue qin, PathState ps, int* counts, int cnt_in,
                           const AdjArgs* adj, hipStream_t s);

@@ -1005,9 +1005,8 @@ __global__ __launch_bounds__(256) void k_queue_count(const int* __restrict__ pid
 
 // Hole removal: the entries of a ranked queue that hold a ray (pid >= 0), in
 // the same order.  Counts per 1024 entries.
-__global__ __launch_bounds__(256) void k_hole_count(const int* __restrict__ pid, int n, const int* dev_n, int2* blk) {
+__global__ __launch_bounds__(256) void k_hole_count(const int* __restrict__ pid, int n, int2* blk) {
     __shared__ int tmp[4];
-    if (dev_n) n = min(n, *dev_n);
     const int base = blockIdx.x * kCompactSlots;
     int na = 0;
     for (int i = 0; i < kCompactSlots / 256; i++) {
@@ -1019,10 +1018,8 @@ __global__ __launch_bounds__(256) void k_hole_count(const int* __restrict__ pid,
     if (threadIdx.x == 0) blk[blockIdx.x] = make_int2(ta, 0);
 }
 
-__global__ __launch_bounds__(256) void k_hole_scatter(RayQueue qin, int n, const int* dev_n, const int2* __restrict__ off,
-                                                      RayQueue q) {
+__global__ __launch_bounds__(256) void k_hole_scatter(RayQueue qin, int n, const int2* __restrict__ off, RayQueue q) {
     __shared__ int tmp[4];
-    if (dev_n) n = min(n, *dev_n);
     const int base = blockIdx.x * kCompactSlots;
     int pos_base = off[blockIdx.x].y;
     for (int i = 0; i < kCompactSlots / 256; i++) {
@@ -1124,17 +1121,16 @@ hipError_t launch_compact_queue(RayQueue qin, int n, int npass, int2* blk, RayQu
     return hipGetLastError();
 }
 
-hipError_t launch_compact_holes(RayQueue qin, int n, const int* dev_n, int2* blk, RayQueue q, int cnt_q, int* counts,
-                                hipStream_t s) {
+hipError_t launch_compact_holes(RayQueue qin, int n, int2* blk, RayQueue q, int cnt_q, int* counts, hipStream_t s) {
     if (n <= 0) {
         hipLaunchKernelGGL(k_zero_slot, dim3(1), dim3(64), 0, s, counts, cnt_q);
         return hipGetLastError();
     }
     const int nblk = (n + kCompactSlots - 1) / kCompactSlots;
     int2* off = blk + nblk;
-    hipLaunchKernelGGL(k_hole_count, dim3(nblk), dim3(256), 0, s, qin.pid, n, dev_n, blk);
+    hipLaunchKernelGGL(k_hole_count, dim3(nblk), dim3(256), 0, s, qin.pid, n, blk);
     scan_blocks(blk, nblk, 0, counts, cnt_q, off, false, s);
-    hipLaunchKernelGGL(k_hole_scatter, dim3(nblk), dim3(256), 0, s, qin, n, dev_n, off, q);
+    hipLaunchKernelGGL(k_hole_scatter, dim3(nblk), dim3(256), 0, s, qin, n, off, q);
     return hipGetLastError();
 }
 
