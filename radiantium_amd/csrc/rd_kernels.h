@@ -191,7 +191,8 @@ hipError_t launch_zero_slots(int* counts, int a, int b, hipStream_t s);
 // 2 = straight with 64-ray chunks per wavefront from an atomic cursor,
 // 3 = sequences: 4 x 64-ray chunks per wavefront, each lane tracing its four
 //     rays back to back.
-// The batched and instanced kernels run straight (0, or chunks for 2 and 3).
+// Batched queues take sequences for 3 and chunks otherwise; instanced scenes
+// run straight (0, or chunks for 2 and 3).
 constexpr int kNumSchedules = 4;
 
 // Occupancy-sized persistent grid for the device.

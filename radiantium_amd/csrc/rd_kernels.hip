@@ -502,7 +502,9 @@ __global__ __launch_bounds__(kBlock) void k_trace_inst(DevScene sc, TraceArgs a)
 // for its slowest lane once per kSeq rays instead of once per ray.
 constexpr int kSeq = 4;
 
-template <int kMode, int kStack, int kBlock>
+// kBatch: each ray's scene is a.scenes[ps.req[pid]]; a lane keeps its
+// current ray's node and primitive arrays in registers.
+template <int kMode, int kStack, int kBlock, bool kBatch>
 __global__ __launch_bounds__(kBlock, lds_blocks(kStack, kBlock)) void k_trace_seq(DevScene sc, TraceArgs a) {
     constexpr bool kAny = (kMode == kModeRaysAny || kMode == kModeShadow);
     __shared__ int lds[kStack * kBlock];
@@ -512,7 +514,9 @@ __global__ __launch_bounds__(kBlock, lds_blocks(kStack, kBlock)) void k_trace_se
     int* cursor = a.counts + a.cursor_slot;
     uint32_t nv = 0, nt = 0, nrays = 0;
     for (int c = next_chunk(cursor, 64 * kSeq); c < n; c = next_chunk(cursor, 64 * kSeq)) {
-        int j = 0, r = 0;
+        int j = 0, r = 0, req = 0;
+        const float4* nodes = sc.nodes;
+        const float4* prims = sc.prims;
         Trav t;
         // the lane's next ray of the chunk that needs tracing; false when none is left
         auto start = [&]() {
@@ -526,6 +530,11 @@ __global__ __launch_bounds__(kBlock, lds_blocks(kStack, kBlock)) void k_trace_se
                 float4 ra, rb;
                 trav_load<kMode>(a, r, &ra, &rb);
                 trav_begin(t, ra, rb);
+                if constexpr (kBatch) {
+                    req = a.ps.req[a.q.pid[r]];
+                    nodes = a.scenes[req].nodes;
+                    prims = a.scenes[req].prims;
+                }
                 return true;
             }
             j = kSeq;
@@ -533,8 +542,9 @@ __global__ __launch_bounds__(kBlock, lds_blocks(kStack, kBlock)) void k_trace_se
         };
         bool active = start();
         while (__ballot(active)) {
-            if (active && trav_step<kAny, kBlock>(sc.nodes, sc.prims, t, stk, nv, nt)) {
-                trav_finish<kMode>(sc, a, r, t);
+            if (active && trav_step<kAny, kBlock>(nodes, prims, t, stk, nv, nt)) {
+                if constexpr (kBatch) trav_finish<kMode>(a.scenes[req], a, r, t);
+                else trav_finish<kMode>(sc, a, r, t);
                 nrays++;
                 active = start();
             }
@@ -693,13 +703,14 @@ static void launch_trace_variant(const DevScene& sc, const TraceArgs& a, int blo
         return;
     }
     if constexpr (kMode >= kModeExt) {
-        if (a.scenes) {   // batched: straight schedule, per-ray scene
-            launch_resident(k_trace_straight<kMode, kStack, kBlock, true>, blocks, kBlock, s, sc, a);
+        if (a.scenes) {   // batched: per-ray scene; sequences or the straight schedule
+            if (schedule == 3) launch_resident(k_trace_seq<kMode, kStack, kBlock, true>, blocks, kBlock, s, sc, a);
+            else launch_resident(k_trace_straight<kMode, kStack, kBlock, true>, blocks, kBlock, s, sc, a);
             return;
         }
     }
     if (schedule == 1) launch_resident(k_trace_refill<kMode, kStack, kBlock>, blocks, kBlock, s, sc, a);
-    else if (schedule == 3) launch_resident(k_trace_seq<kMode, kStack, kBlock>, blocks, kBlock, s, sc, a);
+    else if (schedule == 3) launch_resident(k_trace_seq<kMode, kStack, kBlock, false>, blocks, kBlock, s, sc, a);
     else launch_resident(k_trace_straight<kMode, kStack, kBlock, false>, blocks, kBlock, s, sc, a);
 }
 
@@ -710,8 +721,8 @@ static hipError_t launch_trace_mode(const DevScene& sc, const TraceArgs& a0, int
     int pb = persistent_blocks();
     if (kMode <= kModeRaysAny || sched < 0 || sched >= kNumSchedules) sched = 0;
     TraceArgs a = a0;
-    // straight kernels: 2 = cursor chunks; 3 = sequences (batched and
-    // instanced scenes, which have no sequence kernel, take chunks)
+    // straight kernels: 2 = cursor chunks; 3 = sequences (instanced
+    // scenes, which have no sequence kernel, take chunks)
     a.dyn = sched == 2 ? 1 : (sched == 3 ? 2 : 0);
     if (need <= 24) {
         int b = blocks_hint > 0 ? std::min(blocks_hint, pb) : pb;
