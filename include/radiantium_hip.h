@@ -359,7 +359,8 @@ typedef struct rd_tuning {
                                   (direction octant, origin Morton code)    */
     int32_t ray_sort_bits;     /* sort key bits, 3..30                      */
     int32_t ray_sort_min;      /* smallest extension queue that is sorted   */
-    int32_t ray_sort_shadow;   /* shadow rays in the same order             */
+    int32_t ray_sort_shadow;   /* 1: shadow rays in that sorted order; 0:
+                                  in the extension queue's own order        */
     int32_t overlap;           /* a bounce's shadow / MIS traversal and
                                   resolve on a second stream beside the next
                                   bounce's compaction, sort and extension

@@ -127,7 +127,10 @@ struct rd_ctx {
     bool ray_sort = true;
     int ray_sort_bits = 15;       // key bits (RD_RAY_SORT_BITS): octant + Morton prefix
     int ray_sort_min = 1 << 16;   // smaller queues are traced in queue order
-    bool ray_sort_shadow = true;  // shadow queue in that order too (RD_RAY_SORT_SHADOW)
+    // shadow queue in the sorted order too (RD_RAY_SORT_SHADOW=1), or in the
+    // extension queue's own order (0: on the workload 2 % faster MIS + shadow
+    // launches than the sorted order, whose key is the extension ray's)
+    bool ray_sort_shadow = false;
     // traversal schedule per query class [extension, shadow, MIS]
     // (RD_TRACE_SCHEDULE="e,s,m"): sequences for the extension rays (in the
     // tile-major path order a lane's four rays are four samples of one pixel
@@ -1093,11 +1096,12 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
             return launch_trace_ext(L.sc, L.scenes, qin, wb.ps, wb.hits, c->counts, cin, za, zb, zc, perm,
                                     wb.sort_rank, c->trace_schedule[0], st);
         }));
-        // the shadow rays follow the order the extension rays were traced in:
-        // the sorted order's ranks, or queue order (camera rays in pixel
-        // tiles at the first bounce) where there was no sort
+        // the shadow rays follow the order the extension rays were traced in
+        // (the sorted order's ranks; ray_sort_shadow 0: the queue order
+        // whatever the sort), or the queue order (tile-major path order at
+        // the first bounce) where there was no sort
         const int* rank = (perm && c->ray_sort_shadow) ? wb.sort_rank : nullptr;
-        const bool rank_identity = !perm && c->ray_sort_shadow && L.method != RD_METHOD_BXDF;
+        const bool rank_identity = !rank && L.method != RD_METHOD_BXDF;
         if (L.vol) {
             // VolumetricPathTracer.Li: shade, then the Transmittance / IntersectTr
             // chains step by step until both chain queues drain

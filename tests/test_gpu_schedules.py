@@ -50,6 +50,11 @@ def test_schedules_bit_identical(ctx, which):
         assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
         for k in KEYS:
             assert st[k] == rst[k], ("unsorted", k)
+        # sorted extension rays, shadow rays in the queue's own order
+        img, st = _render(ctx, sc, spp, trace_schedule=(3, 2, 2), ray_sort=1, ray_sort_shadow=0)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+        for k in KEYS:
+            assert st[k] == rst[k], ("shadow in queue order", k)
     finally:
         ctx.set_tuning(**base)
 
