@@ -373,7 +373,8 @@ int rd_set_tuning(rd_ctx* ctx, const rd_tuning* tuning);
 
 /* Autotuner: renders `params` (typically the production render with a few
  * spp) under candidate tunings, one knob at a time from the current setting
- * (extension and shadow schedules, the coherence sort and its key bits),
+ * (extension and shadow schedules, the coherence sort, its key bits and the
+ * shadow rays' order, the stream overlap),
  * keeps each knob's fastest value by device time (best of two renders, a
  * change must gain 1 %), leaves the result set on the context and copies it
  * to `out` (optional).  No counterpart in the reference.  Outputs of later

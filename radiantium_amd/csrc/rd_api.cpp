@@ -1740,29 +1740,32 @@ int rd_autotune(rd_ctx* c, rd_scene* s, const rd_render_params* p, rd_tuning* ou
             int* field;
             std::vector<int> values;
         };
-        int sort = c->ray_sort ? 1 : 0, ovl = c->overlap ? 1 : 0;
+        int sort = c->ray_sort ? 1 : 0, ovl = c->overlap ? 1 : 0, sort_sh = c->ray_sort_shadow ? 1 : 0;
         std::vector<Knob> knobs = {{&c->trace_schedule[0], {0, 2, 3}},
                                    {&c->trace_schedule[1], {0, 2, 3}},
                                    {&sort, {0, 1}},
                                    {&c->ray_sort_bits, {12, 15, 18}},
+                                   {&sort_sh, {0, 1}},
                                    {&ovl, {0, 1}}};
         rc = render_requests(c, 1, &r, true, p->wave_paths, nullptr, nullptr);   // buffers, code objects
         double best = rc ? 0.0 : time_it();
         for (Knob& k : knobs) {
             if (rc) break;
-            if (k.field == &c->ray_sort_bits && !c->ray_sort) continue;
+            if ((k.field == &c->ray_sort_bits || k.field == &sort_sh) && !c->ray_sort) continue;
             const int keep = *k.field;
             int chosen = keep;
             for (int v : k.values) {
                 if (v == keep || rc) continue;
                 *k.field = v;
                 c->ray_sort = sort != 0;
+                c->ray_sort_shadow = sort_sh != 0;
                 c->overlap = ovl != 0;
                 const double t = time_it();
                 if (!rc && t < best * 0.99) { best = t; chosen = v; }
             }
             *k.field = chosen;
             c->ray_sort = sort != 0;
+            c->ray_sort_shadow = sort_sh != 0;
             c->overlap = ovl != 0;
         }
         (void)hipFree(d);
