@@ -588,11 +588,11 @@ __global__ __launch_bounds__(kBlock, lds_blocks(kStack, kBlock)) void k_trace_se
         }
     };
     if (a.dyn) {
-        // 256-ray chunks traced as four rounds of 64: the rounds are the
+        // 128-ray chunks traced as two rounds of 64: the second round is the
         // neighbouring rays of the shadow order (at the first bounce the
-        // same tile's next samples), whose nodes the previous round left in L1
-        for (int c = next_chunk(cursor, 256); c < n; c = next_chunk(cursor, 256))
-            for (int k = 0; k < 4; k++)
+        // same tile's next sample), whose nodes the first round left in L1
+        for (int c = next_chunk(cursor, 128); c < n; c = next_chunk(cursor, 128))
+            for (int k = 0; k < 2; k++)
                 if (c + 64 * k + __lane_id() < n) one(c + 64 * k + __lane_id());
     } else {
         for (int r0 = blockIdx.x * kBlock + threadIdx.x; r0 < n; r0 += gridDim.x * kBlock) one(r0);
