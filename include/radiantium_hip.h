@@ -343,6 +343,28 @@ int rd_render_device(rd_ctx* ctx, rd_scene* scene, const rd_render_params* param
                      volatile int32_t* progress);
 int rd_get_render_stats(const rd_ctx* ctx, rd_render_stats* stats);
 
+/* rd_render_device without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The render's working memory (path pool, ray queues,
+ * per-sample results, per-pixel sums, segment table) is carved from a fixed
+ * work area of RD_NOALLOC_SCRATCH_BYTES that the library holds on every
+ * device it runs on until the process ends (looked up once per device by
+ * rd_create); the caller passes the scene and the device outputs as for
+ * rd_render_device.  Limit: the render's pixels (its crop) must fit with
+ * 48 B each (sums and one sample's results) beside a pool of 1024 paths,
+ * about 2.7 M pixels; larger renders give RD_ERR_INVALID before any launch.
+ * params->wave_paths > 0 caps the pool; the coherence sort and stage timing
+ * are off (neither changes a result), so the image equals rd_render_device's
+ * bit for bit.  Every context on a device shares that device's work area,
+ * which keeps the last call's contents (each call clears what it reads
+ * first): calls that overlap on one device, from several host threads or
+ * processes' contexts, race. */
+#define RD_NOALLOC_SCRATCH_BYTES (128u << 20)
+int rd_render_device_noalloc(rd_ctx* ctx, rd_scene* scene, const rd_render_params* params,
+                             float* d_out_rgb, float* d_out_sum2, volatile int32_t* cancel,
+                             volatile int32_t* progress);
+
 /* ---- tuning of a context's wavefront ------------------------------------
  * No counterpart in the reference (its CPU renderer has none of these
  * choices); for autotuners and benchmarks.  None of them changes a result:

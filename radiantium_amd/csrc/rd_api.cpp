@@ -18,6 +18,7 @@
 #include "rd_camera.h"
 #include "rd_envmap.h"
 #include "rd_kernels.h"
+#include "rd_scratch.h"
 
 using namespace rd;
 
@@ -119,6 +120,13 @@ struct rd_ctx {
     int* counts = nullptr;        // device counters (kCntSlots ints)
     int* host_counts = nullptr;   // pinned mirror
     WaveBuffers wb;
+    // rd_render_device_noalloc: this device's fixed work area (device
+    // address from hipGetSymbolAddress), the buffers carved from it for the
+    // current call, and the segment table's pinned staging
+    void* scratch = nullptr;
+    WaveBuffers wb_na;
+    BatchSeg* na_host_segs = nullptr;
+    hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;   // a render's device time
     rd_render_stats stats{};
     std::map<std::string, StageTiming> timing;
     bool timing_enabled = true;
@@ -241,6 +249,66 @@ struct Timer {
 // crosses; this bounds the host loop for pathological geometry.
 constexpr int kMaxChainSteps = 1 << 16;
 
+// Points the path pool of `w` (cap slots: path state, ray queues, hits,
+// compaction scratch, + the vol_path arrays) at memory from `alloc`
+// (bytes -> pointer, null when it has none left).  False if any is missing.
+template <class Alloc>
+bool assign_pool(rd_ctx* c, WaveBuffers& w, size_t cap, bool vol, Alloc&& alloc) {
+    auto mkq = [&](RayQueue& q, int slot) -> bool {
+        q.count = c->counts + slot;
+        q.pid = (int*)alloc(cap * sizeof(int));
+        q.org = (float4*)alloc(cap * sizeof(float4));
+        q.dir = (float4*)alloc(cap * sizeof(float4));
+        return q.pid && q.org && q.dir;
+    };
+    // qshadow2: vol_path's second Transmittance chain queue; the path
+    // integrator's ranked shadow queue without its holes
+    bool ok = mkq(w.qa, kCntExtA) && mkq(w.qb, kCntExtB) && mkq(w.qshadow, kCntShadow) && mkq(w.qmis, kCntMis) &&
+              mkq(w.qshadow2, kCntShadowB);
+    w.hits = (float4*)alloc(cap * sizeof(float4));
+    PathState& ps = w.ps;
+    ps.L = (float4*)alloc(cap * sizeof(float4));
+    ps.beta = (float4*)alloc(cap * sizeof(float4));
+    ps.nee_beta = (float4*)alloc(cap * sizeof(float4));
+    ps.nee_le = (float4*)alloc(cap * sizeof(float4));
+    ps.mis_f = (float4*)alloc(cap * sizeof(float4));
+    ps.mis_li = (float4*)alloc(cap * sizeof(float4));
+    ps.flags = (int*)alloc(cap * sizeof(int));
+    ps.sel_light = (int*)alloc(cap * sizeof(int));
+    ps.key = (uint64_t*)alloc(cap * sizeof(uint64_t));
+    ps.req = (int*)alloc(cap * sizeof(int));
+    ps.gid = (int*)alloc(cap * sizeof(int));
+    ps.pix = (int*)alloc(cap * sizeof(int));
+    ps.alive = (int*)alloc(cap * sizeof(int));
+    ps.ray_o = (float4*)alloc(cap * sizeof(float4));
+    ps.ray_d = (float4*)alloc(cap * sizeof(float4));
+    w.blk = (int2*)alloc(compact_scratch_int2(cap) * sizeof(int2));
+    w.blk2 = (int2*)alloc(compact_scratch_int2(cap) * sizeof(int2));
+    ps.rng_n = (uint32_t*)alloc(cap * sizeof(uint32_t));
+    ps.depth = (int*)alloc(cap * sizeof(int));
+    ps.vis = (uint8_t*)alloc(cap);
+    ps.hsub = (int*)alloc(cap * sizeof(int));
+    ok = ok && w.hits && ps.L && ps.beta && ps.nee_beta && ps.nee_le && ps.mis_f && ps.mis_li && ps.flags &&
+         ps.sel_light && ps.key && ps.req && ps.gid && ps.pix && ps.alive && ps.ray_o && ps.ray_d && w.blk && w.blk2 &&
+         ps.rng_n &&
+         ps.depth && ps.vis && ps.hsub;
+    ps.env = nullptr; ps.nee_li = ps.sh_to = ps.sh_tr = ps.mis_tr = nullptr;
+    ps.hsub2 = nullptr;
+    if (ok && vol) {
+        ok = mkq(w.qmis2, kCntMisB);
+        w.hits2 = (float4*)alloc(cap * sizeof(float4));
+        ps.env = (int*)alloc(cap * sizeof(int));
+        ps.nee_li = (float4*)alloc(cap * sizeof(float4));
+        ps.sh_to = (float4*)alloc(cap * sizeof(float4));
+        ps.sh_tr = (float4*)alloc(cap * sizeof(float4));
+        ps.mis_tr = (float4*)alloc(cap * sizeof(float4));
+        ps.hsub2 = (int*)alloc(cap * sizeof(int));
+        ok = ok && w.hits2 && ps.env && ps.nee_li && ps.sh_to && ps.sh_tr && ps.mis_tr && ps.hsub2;
+    }
+    ps.adj0 = ps.adj1 = ps.adj2 = nullptr;
+    return ok;
+}
+
 // Grows the render buffers to hold `pool` path slots (+ the vol_path arrays),
 // `results` per-sample results and `pixels` accumulation pixels.
 int ensure_wave_buffers(rd_ctx* c, size_t pool, size_t results, size_t pixels, bool vol) {
@@ -262,59 +330,7 @@ int ensure_wave_buffers(rd_ctx* c, size_t pool, size_t results, size_t pixels, b
         w.cap = 0;
         w.vol = false;
         size_t cap = std::max(pool, (size_t)1024);
-        auto alloc = [&](size_t bytes) { return alloc_in(w.pool_allocs, bytes); };
-        auto mkq = [&](RayQueue& q, int slot) -> bool {
-            q.count = c->counts + slot;
-            q.pid = (int*)alloc(cap * sizeof(int));
-            q.org = (float4*)alloc(cap * sizeof(float4));
-            q.dir = (float4*)alloc(cap * sizeof(float4));
-            return q.pid && q.org && q.dir;
-        };
-        // qshadow2: vol_path's second Transmittance chain queue; the path
-        // integrator's ranked shadow queue without its holes
-        bool ok = mkq(w.qa, kCntExtA) && mkq(w.qb, kCntExtB) && mkq(w.qshadow, kCntShadow) && mkq(w.qmis, kCntMis) &&
-                  mkq(w.qshadow2, kCntShadowB);
-        w.hits = (float4*)alloc(cap * sizeof(float4));
-        PathState& ps = w.ps;
-        ps.L = (float4*)alloc(cap * sizeof(float4));
-        ps.beta = (float4*)alloc(cap * sizeof(float4));
-        ps.nee_beta = (float4*)alloc(cap * sizeof(float4));
-        ps.nee_le = (float4*)alloc(cap * sizeof(float4));
-        ps.mis_f = (float4*)alloc(cap * sizeof(float4));
-        ps.mis_li = (float4*)alloc(cap * sizeof(float4));
-        ps.flags = (int*)alloc(cap * sizeof(int));
-        ps.sel_light = (int*)alloc(cap * sizeof(int));
-        ps.key = (uint64_t*)alloc(cap * sizeof(uint64_t));
-        ps.req = (int*)alloc(cap * sizeof(int));
-        ps.gid = (int*)alloc(cap * sizeof(int));
-        ps.pix = (int*)alloc(cap * sizeof(int));
-        ps.alive = (int*)alloc(cap * sizeof(int));
-        ps.ray_o = (float4*)alloc(cap * sizeof(float4));
-        ps.ray_d = (float4*)alloc(cap * sizeof(float4));
-        w.blk = (int2*)alloc(compact_scratch_int2(cap) * sizeof(int2));
-        w.blk2 = (int2*)alloc(compact_scratch_int2(cap) * sizeof(int2));
-        ps.rng_n = (uint32_t*)alloc(cap * sizeof(uint32_t));
-        ps.depth = (int*)alloc(cap * sizeof(int));
-        ps.vis = (uint8_t*)alloc(cap);
-        ps.hsub = (int*)alloc(cap * sizeof(int));
-        ok = ok && w.hits && ps.L && ps.beta && ps.nee_beta && ps.nee_le && ps.mis_f && ps.mis_li && ps.flags &&
-             ps.sel_light && ps.key && ps.req && ps.gid && ps.pix && ps.alive && ps.ray_o && ps.ray_d && w.blk && w.blk2 &&
-             ps.rng_n &&
-             ps.depth && ps.vis && ps.hsub;
-        ps.env = nullptr; ps.nee_li = ps.sh_to = ps.sh_tr = ps.mis_tr = nullptr;
-        ps.hsub2 = nullptr;
-        if (ok && vol) {
-            ok = mkq(w.qmis2, kCntMisB);
-            w.hits2 = (float4*)alloc(cap * sizeof(float4));
-            ps.env = (int*)alloc(cap * sizeof(int));
-            ps.nee_li = (float4*)alloc(cap * sizeof(float4));
-            ps.sh_to = (float4*)alloc(cap * sizeof(float4));
-            ps.sh_tr = (float4*)alloc(cap * sizeof(float4));
-            ps.mis_tr = (float4*)alloc(cap * sizeof(float4));
-            ps.hsub2 = (int*)alloc(cap * sizeof(int));
-            ok = ok && w.hits2 && ps.env && ps.nee_li && ps.sh_to && ps.sh_tr && ps.mis_tr && ps.hsub2;
-        }
-        ps.adj0 = ps.adj1 = ps.adj2 = nullptr;
+        bool ok = assign_pool(c, w, cap, vol, [&](size_t bytes) { return alloc_in(w.pool_allocs, bytes); });
         if (!ok) { w.release(); return fail(RD_ERR_OOM, "hipMalloc failed for the path pool"); }
         w.cap = cap;
         w.vol = vol;
@@ -338,6 +354,54 @@ int ensure_wave_buffers(rd_ctx* c, size_t pool, size_t results, size_t pixels, b
     }
     return RD_OK;
 }
+
+// rd_render_device_noalloc: the render's buffers carved from the context's
+// fixed work area, in 256-byte aligned pieces: segment table, per-pixel sums,
+// path pool, per-sample results.  With base == null nothing is assigned for
+// real and the carve only measures (Bump::used).
+struct Bump {
+    char* base;
+    size_t size;
+    size_t used = 0;
+    void* take(size_t bytes) {
+        const size_t at = (used + 255) & ~(size_t)255;
+        if (at > size || bytes > size - at) { used = size + 1; return nullptr; }
+        used = at + bytes;
+        return base ? (void*)(base + at) : (void*)(uintptr_t)(at + 256);   // measuring: any non-null
+    }
+};
+
+bool carve_scratch(rd_ctx* c, WaveBuffers& w, Bump& b, size_t pool, size_t results, size_t pixels, bool vol) {
+    w.segs = (BatchSeg*)b.take(sizeof(BatchSeg) * kMaxSegs);
+    w.sum = (float4*)b.take(pixels * sizeof(float4));
+    w.sum2 = (float4*)b.take(pixels * sizeof(float4));
+    bool ok = w.segs && w.sum && w.sum2 && assign_pool(c, w, pool, vol, [&](size_t bytes) { return b.take(bytes); });
+    w.ps.res = (float4*)b.take(results * sizeof(float4));
+    ok = ok && w.ps.res;
+    w.host_segs = c->na_host_segs;
+    w.cap = pool;
+    w.res_cap = results;
+    w.pix_cap = pixels;
+    w.vol = vol;
+    w.sort_cap = 0;
+    w.sort_keys = w.sort_keys_alt = nullptr;
+    w.sort_vals = w.sort_perm = w.sort_rank = nullptr;
+    w.sort_temp = nullptr;
+    w.scenes = nullptr;
+    return ok;
+}
+
+size_t scratch_need(rd_ctx* c, size_t pool, size_t results, size_t pixels, bool vol) {
+    WaveBuffers tmp;
+    Bump b{nullptr, (size_t)-1 / 2};
+    carve_scratch(c, tmp, b, pool, results, pixels, vol);
+    tmp.host_segs = nullptr;   // not owned
+    return b.used;
+}
+
+// Smallest pool the work area must hold beside the sums and one sample's results.
+constexpr size_t kNoAllocMinPool = 1024;
+constexpr size_t kNoAllocMaxPool = (size_t)1 << 17;
 
 // Adjoint replay: per-slot derivative records and a table of `ntex` gradient pointers.
 // Buffers of the extension-queue sort, sized for the pool.
@@ -422,6 +486,8 @@ extern "C" {
 int rd_abi_version(void) { return RD_ABI_VERSION; }
 const char* rd_last_error(void) { return g_err.c_str(); }
 
+static void free_ctx(rd_ctx* c);
+
 rd_ctx* rd_create(int device) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) { g_err = "no HIP device"; return nullptr; }
@@ -466,12 +532,27 @@ rd_ctx* rd_create(int device) {
         return nullptr;
     }
     (void)hipMemset(c->counts, 0, kCntSlots * sizeof(int));
+    // set-up for rd_render_device_noalloc, so that it allocates nothing: this
+    // device's work-area address, the segment table's pinned staging, the
+    // render-time events
+    hipError_t se = rd::scratch_address(device, &c->scratch);
+    if (se != hipSuccess ||
+        hipHostMalloc((void**)&c->na_host_segs, sizeof(BatchSeg) * kMaxSegs, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreate(&c->ev_t0) != hipSuccess || hipEventCreate(&c->ev_t1) != hipSuccess) {
+        g_err = se != hipSuccess ? std::string("work-area lookup failed: ") + hipGetErrorString(se)
+                                 : std::string("context set-up failed");
+        free_ctx(c);
+        return nullptr;
+    }
     return c;
 }
 
 static void free_ctx(rd_ctx* c) {
     if (c->counts) (void)hipFree(c->counts);
     if (c->host_counts) (void)hipHostFree(c->host_counts);
+    if (c->na_host_segs) (void)hipHostFree(c->na_host_segs);
+    if (c->ev_t0) (void)hipEventDestroy(c->ev_t0);
+    if (c->ev_t1) (void)hipEventDestroy(c->ev_t1);
     (void)hipEventDestroy(c->ev_shaded);
     (void)hipEventDestroy(c->ev_resolved);
     (void)hipStreamDestroy(c->stream2);
@@ -1048,6 +1129,8 @@ struct PassLaunch {
     int nseg;
     int npass;     // paths in the pass
     int pool;      // path slots in flight
+    WaveBuffers* wb = nullptr;   // the pass's buffers (null: the context's own)
+    bool sort_ok = true;         // the coherence sort may run (its buffers are allocated on demand)
 };
 
 // The bounce loop of one pass: every bounce traces and shades the queue,
@@ -1057,7 +1140,7 @@ struct PassLaunch {
 // one sync per bounce reads the next queue size (for vol_path also the chain
 // queue sizes).  Ends when the queue is empty (the pass has run dry).
 int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatile int32_t* cancel) {
-    WaveBuffers& wb = c->wb;
+    WaveBuffers& wb = L.wb ? *L.wb : c->wb;
     hipStream_t st = c->stream;
     RayQueue qin = wb.qa, qout = wb.qb;
     int cin = kCntExtA, cout = kCntExtB;
@@ -1078,7 +1161,7 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
     for (int bounce = 0; !rc; bounce++) {
         // the queue size is known on the host from the second bounce on
         const int* perm = nullptr;
-        if (c->ray_sort && bounce > 0 && c->host_counts[cin] >= c->ray_sort_min) {
+        if (L.sort_ok && c->ray_sort && bounce > 0 && c->host_counts[cin] >= c->ray_sort_min) {
             if (int e = ensure_sort_buffers(c, (size_t)L.pool)) return e;
             HIP_TRY(tm.run("sort", [&] {
                 return launch_ray_sort(L.sc.nodes, qin, c->host_counts[cin], c->ray_sort_bits, wb.sort_keys,
@@ -1292,7 +1375,7 @@ size_t default_pool(size_t paths) {
 // (a pass may hold several requests, a request may span several passes);
 // within a pass paths stream through the slot pool.
 int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, int32_t pool_paths,
-                    volatile int32_t* cancel, volatile int32_t* progress) {
+                    volatile int32_t* cancel, volatile int32_t* progress, bool noalloc = false) {
     std::vector<Crop> crops(n);
     for (int i = 0; i < n; i++) {
         const rd_request& r = reqs[i];
@@ -1319,19 +1402,47 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
         total_pix += np;
         total_paths += np * (size_t)reqs[i].params.spp;
     }
-    const size_t pool = pool_paths > 0 ? std::min((size_t)pool_paths, std::max<size_t>(total_paths, 1))
-                                       : default_pool(total_paths);
-    const size_t pass_cap = std::max(std::min(pass_limit(), total_paths), max_npix);
-    if (pass_cap > (size_t)INT32_MAX / 2 || total_pix > (size_t)INT32_MAX / 2) return fail(RD_ERR_INVALID, "render too large");
     bool any_vol = false;
     for (auto& k : kinds) any_vol = any_vol || k.first == RD_INTEGRATOR_VOL_PATH;
-    if (int rc = ensure_wave_buffers(c, pool, pass_cap, total_pix, any_vol)) return rc;
-    WaveBuffers& wb = c->wb;
+    size_t pool = 0, pass_cap = 0;
+    if (!noalloc) {
+        pool = pool_paths > 0 ? std::min((size_t)pool_paths, std::max<size_t>(total_paths, 1))
+                              : default_pool(total_paths);
+        pass_cap = std::max(std::min(pass_limit(), total_paths), max_npix);
+    } else {
+        // everything from the work area: the sums and one sample's results
+        // must fit beside the smallest pool; then the largest pool up to
+        // kNoAllocMaxPool (or the caller's cap), then results for the rest
+        if (!c->scratch) return fail(RD_ERR_DEVICE, "no work area on this device");
+        if (total_pix > kScratchBytes || scratch_need(c, kNoAllocMinPool, max_npix, total_pix, any_vol) > kScratchBytes)
+            return fail(RD_ERR_INVALID, "render of " + std::to_string(total_pix) + " pixels does not fit the " +
+                                            std::to_string(kScratchBytes >> 20) +
+                                            " MiB work area of rd_render_device_noalloc (48 B per pixel)");
+        size_t cap = kNoAllocMaxPool;
+        if (pool_paths > 0) cap = std::min(cap, (size_t)pool_paths);
+        cap = std::max(std::min(cap, total_paths), kNoAllocMinPool);
+        while (cap > kNoAllocMinPool && scratch_need(c, cap, max_npix, total_pix, any_vol) > kScratchBytes) cap /= 2;
+        pool = std::max(cap, kNoAllocMinPool);
+        const size_t fixed = scratch_need(c, pool, 0, total_pix, any_vol);
+        pass_cap = std::max(std::min((kScratchBytes - std::min(fixed, kScratchBytes)) / sizeof(float4), total_paths),
+                            max_npix);
+        while (pass_cap > max_npix && scratch_need(c, pool, pass_cap, total_pix, any_vol) > kScratchBytes) pass_cap--;
+    }
+    if (pass_cap > (size_t)INT32_MAX / 2 || total_pix > (size_t)INT32_MAX / 2) return fail(RD_ERR_INVALID, "render too large");
+    if (noalloc) {
+        Bump b{(char*)c->scratch, kScratchBytes};
+        if (!carve_scratch(c, c->wb_na, b, pool, pass_cap, total_pix, any_vol))
+            return fail(RD_ERR_INVALID, "render does not fit the work area of rd_render_device_noalloc");
+    } else if (int rc = ensure_wave_buffers(c, pool, pass_cap, total_pix, any_vol)) {
+        return rc;
+    }
+    WaveBuffers& wb = noalloc ? c->wb_na : c->wb;
     hipStream_t st = c->stream;
     std::vector<DevScene> dscenes(n);
     for (int i = 0; i < n; i++) dscenes[i] = reqs[i].scene->dev;
     const DevScene* d_scenes = nullptr;
     if (!single) {
+        if (noalloc) return fail(RD_ERR_INVALID, "batches allocate their scene table");
         if (int rc = ensure_batch_buffers(c, (size_t)n)) return rc;
         HIP_TRY(hipMemcpyAsync(wb.scenes, dscenes.data(), sizeof(DevScene) * n, hipMemcpyHostToDevice, st));
         d_scenes = wb.scenes;
@@ -1341,14 +1452,19 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
         size_t o = 0;
         for (int i = 0; i < n; i++) { sum_off[i] = o; o += (size_t)(crops[i].x1 - crops[i].x0) * (crops[i].y1 - crops[i].y0); }
     }
+    // stage timing creates events on demand: off without allocation
+    struct TimingOff {
+        rd_ctx* c;
+        bool was;
+        ~TimingOff() { c->timing_enabled = was; }
+    } timing_guard{c, c->timing_enabled};
+    if (noalloc) c->timing_enabled = false;
     Timer tm(c);
     std::memset(&c->stats, 0, sizeof(c->stats));
     HIP_TRY(launch_zero_counts(c->counts, kCntSlots, st));
     HIP_TRY(launch_zero(wb.sum, (int)total_pix, st));
     HIP_TRY(launch_zero(wb.sum2, (int)total_pix, st));
-    hipEvent_t ev0, ev1;
-    HIP_TRY(hipEventCreate(&ev0));
-    HIP_TRY(hipEventCreate(&ev1));
+    hipEvent_t ev0 = c->ev_t0, ev1 = c->ev_t1;
     HIP_TRY(hipEventRecord(ev0, st));
     int passes = 0, max_bounce = 0;
     int rc = RD_OK;
@@ -1399,7 +1515,7 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
             std::memcpy(wb.host_segs, segs.data(), sizeof(BatchSeg) * segs.size());
             HIP_TRY(hipMemcpyAsync(wb.segs, wb.host_segs, sizeof(BatchSeg) * segs.size(), hipMemcpyHostToDevice, st));
             PassLaunch L{rep, d_scenes, nullptr, kinds[g].second, kinds[g].first == RD_INTEGRATOR_VOL_PATH, wb.segs,
-                         (int)segs.size(), (int)used, (int)std::min(pool, used)};
+                         (int)segs.size(), (int)used, (int)std::min(pool, used), &wb, !noalloc};
             rc = run_pass(c, tm, L, &max_bounce, cancel);
             if (rc) break;
             HIP_TRY(tm.run("accumulate", [&] {
@@ -1426,8 +1542,6 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
     HIP_TRY(hipStreamSynchronize(st));
     float ms = 0;
     (void)hipEventElapsedTime(&ms, ev0, ev1);
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
     tm.collect();
     collect_stats(c, total_paths, ms, passes, max_bounce);
     return rc;
@@ -1444,6 +1558,19 @@ int rd_render_device(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d
     r.out_rgb = d_out_rgb;
     r.out_sum2 = d_out_sum2;
     return render_requests(c, 1, &r, true, p->wave_paths, cancel, progress);
+}
+
+// rd_render_device with every work buffer carved from the context's fixed
+// work area (no device allocation; see render_requests).
+int rd_render_device_noalloc(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d_out_rgb,
+                             float* d_out_sum2, volatile int32_t* cancel, volatile int32_t* progress) {
+    if (!c || !s || !p || !d_out_rgb) return fail(RD_ERR_INVALID, "null argument");
+    rd_request r{};
+    r.scene = s;
+    r.params = *p;
+    r.out_rgb = d_out_rgb;
+    r.out_sum2 = d_out_sum2;
+    return render_requests(c, 1, &r, true, p->wave_paths, cancel, progress, true);
 }
 
 // Batched rendering: the requests' paths share passes and the slot pool; see

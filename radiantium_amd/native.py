@@ -22,7 +22,7 @@ EXPORTS = [
     "rd_synchronize", "rd_kernel_timing", "rd_reset_kernel_timing", "rd_render_batch", "rd_render_batch_device",
     "rd_adjoint", "rd_adjoint_device", "rd_scene_set_texture", "rd_scene_set_texture_device",
     "rd_trace_closest_sub", "rd_json_load", "rd_json_scene_desc", "rd_json_scene_free", "rd_scene_load_json",
-    "rd_metal_eta_k", "rd_get_tuning", "rd_set_tuning", "rd_autotune",
+    "rd_metal_eta_k", "rd_get_tuning", "rd_set_tuning", "rd_autotune", "rd_render_device_noalloc",
 ]
 
 
@@ -72,6 +72,7 @@ def lib():
         L.rd_trace_any_device.argtypes = [vp, vp, vp, i32, vp, vp, vp]
         L.rd_render.argtypes = [vp, vp, C.POINTER(abi.rd_render_params), vp, vp, vp, vp]
         L.rd_render_device.argtypes = [vp, vp, C.POINTER(abi.rd_render_params), vp, vp, vp, vp]
+        L.rd_render_device_noalloc.argtypes = [vp, vp, C.POINTER(abi.rd_render_params), vp, vp, vp, vp]
         L.rd_get_render_stats.argtypes = [vp, C.POINTER(abi.rd_render_stats)]
         L.rd_device_alloc.restype = vp
         L.rd_device_alloc.argtypes = [vp, u64]
@@ -413,3 +414,12 @@ class Scene:
         if crop is not None:
             p.x0, p.y0, p.x1, p.y1 = crop
         check(lib().rd_render_device(self.ctx.h, self.h, C.byref(p), d_rgb, d_sum2, None, None))
+
+    def render_device_noalloc(self, spp, d_rgb, d_sum2=None, seed=0, crop=None, wave_paths=0):
+        """rd_render_device_noalloc: as render_device, with no device allocation
+        (work buffers from the library's fixed per-device work area)."""
+        p = abi.rd_render_params()
+        p.spp, p.seed, p.wave_paths = spp, seed, wave_paths
+        if crop is not None:
+            p.x0, p.y0, p.x1, p.y1 = crop
+        check(lib().rd_render_device_noalloc(self.ctx.h, self.h, C.byref(p), d_rgb, d_sum2, None, None))
