@@ -2923,6 +2923,27 @@ int cr_material_sample(const cr_scene* s, int32_t material, const float* wo, con
     return RD_OK;
 }
 
+// Batched forms of the two above for quadrature and sampling statistics:
+// n direction pairs (eval) or n samples with keys key0 + j (sample).
+int cr_material_eval_n(const cr_scene* s, int32_t material, int32_t n, const float* wo, const float* wi,
+                       const float* uv, float* fr_rgb, float* pdf) {
+    for (int32_t j = 0; j < n; j++) {
+        int rc = cr_material_eval(s, material, wo + 3 * j, wi + 3 * j, uv, fr_rgb + 3 * j, pdf + j);
+        if (rc) return rc;
+    }
+    return RD_OK;
+}
+
+int cr_material_sample_n(const cr_scene* s, int32_t material, int32_t n, const float* wo, const float* uv,
+                         uint64_t key0, float* wi, float* fr_rgb, float* pdf, int32_t* type) {
+    for (int32_t j = 0; j < n; j++) {
+        int rc = cr_material_sample(s, material, wo + 3 * j, uv, key0 + (uint64_t)j, wi + 3 * j, fr_rgb + 3 * j,
+                                    pdf + j, type + j);
+        if (rc) return rc;
+    }
+    return RD_OK;
+}
+
 float cr_fresnel_dielectric(float ci, float ei, float et) { return fresnel_dielectric(ci, ei, et); }
 
 void cr_fresnel_conductor(float ci, const float* eta, const float* k, float* out) {

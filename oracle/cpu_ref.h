@@ -51,6 +51,11 @@ int cr_material_eval(const cr_scene* s, int32_t material, const float* wo, const
                      const float* uv, float* fr_rgb, float* pdf);
 int cr_material_sample(const cr_scene* s, int32_t material, const float* wo, const float* uv,
                        uint64_t key, float* wi, float* fr_rgb, float* pdf, int32_t* type);
+/* The same over n direction pairs / n samples (keys key0 + j). */
+int cr_material_eval_n(const cr_scene* s, int32_t material, int32_t n, const float* wo, const float* wi,
+                       const float* uv, float* fr_rgb, float* pdf);
+int cr_material_sample_n(const cr_scene* s, int32_t material, int32_t n, const float* wo, const float* uv,
+                         uint64_t key0, float* wi, float* fr_rgb, float* pdf, int32_t* type);
 
 /* Distribution1D / Distribution2D probes on a caller's function values.
  * dist1d: out = {continuous sample, pdf, discrete pdf of the sampled

@@ -44,6 +44,10 @@ def lib():
         L.cr_sample_li.argtypes = [C.c_void_p, C.c_uint32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
         L.cr_material_eval.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_void_p]
+        L.cr_material_eval_n.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p]
+        L.cr_material_sample_n.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_uint64,
+                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.cr_material_sample.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_uint64,
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.cr_fresnel_dielectric.restype = C.c_float
@@ -326,6 +330,25 @@ class OracleScene:
         fr = np.zeros(3, np.float32); pdf = np.zeros(1, np.float32)
         lib().cr_material_eval(self.h, mat, _ptr(wo), _ptr(wi), _ptr(uv), _ptr(fr), _ptr(pdf))
         return fr, float(pdf[0])
+
+    def material_eval_n(self, mat, wo, wi, uv=(0.0, 0.0)):
+        """material_eval over arrays: wo, wi [n,3] (wo may be one direction) -> (fr [n,3], pdf [n])."""
+        wi = np.ascontiguousarray(wi, np.float32).reshape(-1, 3)
+        wo = np.ascontiguousarray(np.broadcast_to(np.asarray(wo, np.float32).reshape(-1, 3), wi.shape), np.float32)
+        uv = np.asarray(uv, np.float32)
+        fr = np.zeros((wi.shape[0], 3), np.float32); pdf = np.zeros(wi.shape[0], np.float32)
+        lib().cr_material_eval_n(self.h, mat, wi.shape[0], _ptr(wo), _ptr(wi), _ptr(uv), _ptr(fr), _ptr(pdf))
+        return fr, pdf
+
+    def material_sample_n(self, mat, wo, n, key0, uv=(0.0, 0.0)):
+        """n samples of one wo with keys key0 .. key0+n-1 -> (wi [n,3], fr [n,3], pdf [n], type [n])."""
+        wo = np.ascontiguousarray(np.broadcast_to(np.asarray(wo, np.float32).reshape(1, 3), (n, 3)), np.float32)
+        uv = np.asarray(uv, np.float32)
+        wi = np.zeros((n, 3), np.float32); fr = np.zeros((n, 3), np.float32)
+        pdf = np.zeros(n, np.float32); ty = np.zeros(n, np.int32)
+        lib().cr_material_sample_n(self.h, mat, n, _ptr(wo), _ptr(uv), C.c_uint64(key0), _ptr(wi), _ptr(fr),
+                                   _ptr(pdf), _ptr(ty))
+        return wi, fr, pdf, ty
 
     def material_sample(self, mat, wo, key, uv=(0.0, 0.0)):
         wo = np.asarray(wo, np.float32); uv = np.asarray(uv, np.float32)

@@ -97,13 +97,15 @@ def checker_texture(n=64, c0=(0.8, 0.8, 0.8), c1=(0.2, 0.3, 0.7), cells=8):
     return img.astype(np.float32)
 
 
-def material_box(width=64, height=64, max_depth=8, method="Mis"):
+def material_box(width=64, height=64, max_depth=8, method="Mis", delta=True):
     """Cornell box exercising every material of the registry and the image
     texture sampler: a textured rough_plastic left wall (Linear filter, Repeat
     wrap, UVs tiled twice), GGX plastic floor, diffuse green right wall,
     a Disney short box (sheen + clearcoat), a perfect_mirror tall box, a
     perfect_glass sphere, an anisotropic rough_metal sphere and a rough_glass
-    sphere, lit by the ceiling quad light and a small emissive sphere."""
+    sphere, lit by the ceiling quad light and a small emissive sphere.
+    delta=False makes the mirror box a rough_metal and the glass sphere a
+    rough_glass (no delta BSDF left)."""
     from .scene import Texture
     s = SceneDesc()
     grey = s.add_material("diffuse", kd=(0.73, 0.73, 0.73))
@@ -113,8 +115,12 @@ def material_box(width=64, height=64, max_depth=8, method="Mis"):
     plastic_floor = s.add_material("rough_plastic", r=(0.6, 0.5, 0.4), roughness=0.25, dist="GGX")
     disney = s.add_material("disney", base_color=(0.8, 0.3, 0.2), roughness=0.45, sheen=0.6, sheen_tint=0.5,
                             clearcoat=0.7, clearcoat_gloss=0.6, specular_tint=0.3, metallic=0.1)
-    mirror = s.add_material("perfect_mirror", r=(0.9, 0.9, 0.9))
-    glass = s.add_material("perfect_glass")
+    if delta:
+        mirror = s.add_material("perfect_mirror", r=(0.9, 0.9, 0.9))
+        glass = s.add_material("perfect_glass")
+    else:
+        mirror = s.add_material("rough_metal", roughness=0.3)
+        glass = s.add_material("rough_glass", roughness=0.35)
     metal = s.add_material("rough_metal", roughness=0.4, anisotropic=0.5)
     rglass = s.add_material("rough_glass", roughness=0.2)
     light_mat = s.add_material("diffuse", kd=(0.0, 0.0, 0.0))

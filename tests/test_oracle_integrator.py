@@ -128,3 +128,56 @@ def test_estimators_agree_on_cornell(oracle):
         zz = z(a, b)
         assert np.all(np.abs(zz) < 4.5), (a, b, np.abs(zz).max())
         assert abs(np.mean(zz)) < 0.5, (a, b, np.mean(zz))
+
+
+def test_estimators_agree_on_material_box(oracle):
+    """The three estimators agree on the material showcase without delta
+    BSDFs: textured rough plastic, GGX plastic, Disney (sheen, clearcoat),
+    anisotropic rough metal, rough glass, a sphere light beside the quad
+    light.  A mistake in any material's Pdf against its Sample, or in the
+    light pdfs, biases Mis's weights or Bxdf's estimate and fails the z-test."""
+    imgs = {}
+    for method, spp in (("Mis", 1024), ("Nee", 1024), ("Bxdf", 4096)):
+        o = oracle.OracleScene(scenes.material_box(24, 24, method=method, delta=False))
+        rgb, s2, _, _ = o.render(spp, seed=23, threads=0)
+        imgs[method] = (rgb.astype(np.float64), s2.astype(np.float64), spp)
+
+    def blk(a):
+        return a.reshape(6, 4, 6, 4, 3).sum(axis=(1, 3))
+
+    for a, b in (("Mis", "Nee"), ("Mis", "Bxdf"), ("Nee", "Bxdf")):
+        (ma, sa, na), (mb, sb, nb) = imgs[a], imgs[b]
+        va = np.maximum(sa / na - ma ** 2, 0) / na
+        vb = np.maximum(sb / nb - mb ** 2, 0) / nb
+        z = (blk(ma) - blk(mb)) / np.sqrt(blk(va) + blk(vb) + 1e-8)
+        assert np.all(np.abs(z) < 4.5), (a, b, np.abs(z).max())
+        assert abs(np.mean(z)) < 0.5, (a, b, np.mean(z))
+
+
+def test_mis_counts_light_twice_through_a_mirror(oracle):
+    """The reference's Mis (PathTracer.cs:192-411) takes a delta BSDF sample
+    in EstimateDirect with weight 1 when it hits the light, and the main loop
+    counts the light again at the next vertex because the path is specular:
+    seen in a perfect mirror, a light is twice as bright under Mis as under
+    Nee (which samples no BSDF in its light estimate).  Reproduced on purpose;
+    pixels that see only the light's reflection give exactly R Le and 2 R Le."""
+    R, Le = np.array([0.9, 0.8, 0.7]), np.array([1.0, 2.0, 3.0])
+    out = {}
+    for method in ("Nee", "Mis"):
+        s = SceneDesc()
+        mirror = s.add_material("perfect_mirror", r=tuple(R))
+        dark = s.add_material("diffuse", kd=(0.0, 0.0, 0.0))
+        s.add_mesh(*_quad((-4, 0, -4), (-4, 0, 4), (4, 0, 4), (4, 0, -4)), material=mirror)
+        # one triangle, so one light: the uniform light pick always picks it
+        tri = np.array([[-1.5, 2.0, -2.5], [1.5, 2.0, -2.5], [0.0, 2.0, 0.5]], np.float32)
+        s.add_mesh(tri, np.array([[0, 1, 2]], np.int32), material=dark, le=tuple(Le))
+        s.set_camera(fov=50.0, width=32, height=32, origin=(0.0, 1.0, 3.0), target=(0.0, 0.0, -1.0),
+                     up=(0.0, 1.0, 0.0))
+        s.set_integrator(type="path", max_depth=4, method=method)
+        rgb, _, _, _ = oracle.OracleScene(s).render(4, seed=5, threads=0)
+        out[method] = rgb.reshape(-1, 3).astype(np.float64)
+    full = np.all(np.abs(out["Nee"] - R * Le) < 1e-5, axis=1)
+    assert full.sum() > 20
+    np.testing.assert_allclose(out["Mis"][full], np.broadcast_to(2 * R * Le, (full.sum(), 3)), rtol=1e-5)
+    # elsewhere neither estimator sees anything but the reflection's edge
+    assert np.all(out["Mis"] <= 2 * R * Le + 1e-5)
