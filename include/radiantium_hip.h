@@ -342,6 +342,12 @@ int rd_render_device(rd_ctx* ctx, rd_scene* scene, const rd_render_params* param
                      float* d_out_rgb, float* d_out_sum2, volatile int32_t* cancel,
                      volatile int32_t* progress);
 int rd_get_render_stats(const rd_ctx* ctx, rd_render_stats* stats);
+/* rd_render's per-pixel mean without its final rounding to float: out_mean
+ * (host double[W*H*3], same layout) receives sum / spp in float64.  Every
+ * render keeps its per-pixel sums in float64 over float32 sample radiance
+ * (BlockBasedRenderer.cs:140-178 sums in float32); rd_render rounds the mean
+ * to float, this keeps it, e.g. for finite-difference gradient checks. */
+int rd_render_mean_f64(rd_ctx* ctx, rd_scene* scene, const rd_render_params* params, double* out_mean);
 
 /* rd_render_device without any device allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
@@ -352,8 +358,9 @@ int rd_get_render_stats(const rd_ctx* ctx, rd_render_stats* stats);
  * device it runs on until the process ends (looked up once per device by
  * rd_create); the caller passes the scene and the device outputs as for
  * rd_render_device.  Limit: the render's pixels (its crop) must fit with
- * 48 B each (sums and one sample's results) beside a pool of 1024 paths,
- * about 2.7 M pixels; larger renders give RD_ERR_INVALID before any launch.
+ * 64 B each (sums and one sample's results) beside a pool of 1024 paths,
+ * about 2.09 M pixels (1920x1080 fits); larger renders give RD_ERR_INVALID
+ * before any launch.
  * params->wave_paths > 0 caps the pool; the coherence sort and stage timing
  * are off (neither changes a result), so the image equals rd_render_device's
  * bit for bit.  Every context on a device shares that device's work area,

@@ -2728,16 +2728,22 @@ int cr_render(const cr_scene* s, const rd_render_params* p, int32_t threads, flo
             int ex = std::min(bx + bs, x1), ey = std::min(by + bs, y1);
             for (int x = bx; x < ex; x++) {
                 for (int y = by; y < ey; y++) {
-                    C3 sum(0.0f), sum2(0.0f);
+                    // the per-pixel sum and mean in double over float32
+                    // samples (the reference's ColorBuffer sums in float32,
+                    // BlockBasedRenderer.cs:140-178); the sum of squares float
+                    double sr = 0, sg = 0, sb = 0;
+                    C3 sum2(0.0f);
                     for (int t = 0; t < p->spp; t++) {
                         C3 c = pixel_sample(*s, p->seed, x, y, t, st);
                         if (!c.valid()) { c = C3(0.0f); st[4]++; }
-                        sum = sum + c;
+                        sr += (double)c.r;
+                        sg += (double)c.g;
+                        sb += (double)c.b;
                         sum2 = sum2 + c * c;
                     }
                     size_t o = ((size_t)x * H + y) * 3;
-                    C3 mean = sum / (float)p->spp;
-                    out_rgb[o] = mean.r; out_rgb[o + 1] = mean.g; out_rgb[o + 2] = mean.b;
+                    const double n = (double)p->spp;
+                    out_rgb[o] = (float)(sr / n); out_rgb[o + 1] = (float)(sg / n); out_rgb[o + 2] = (float)(sb / n);
                     if (out_sum2) { out_sum2[o] = sum2.r; out_sum2[o + 1] = sum2.g; out_sum2[o + 2] = sum2.b; }
                 }
             }

@@ -59,7 +59,7 @@ struct WaveBuffers {
     float4* hits = nullptr;
     float4* hits2 = nullptr;        // vol_path: IntersectTr chain hits
     PathState ps{};
-    float4* sum = nullptr;
+    double4* sum = nullptr;   // float64 per-pixel sums
     float4* sum2 = nullptr;
     DevScene* scenes = nullptr;
     BatchSeg* segs = nullptr;       // device segment table
@@ -347,7 +347,7 @@ int ensure_wave_buffers(rd_ctx* c, size_t pool, size_t results, size_t pixels, b
         WaveBuffers::free_list(w.pix_allocs);
         w.pix_cap = 0;
         size_t pcap = std::max(pixels, (size_t)1024);
-        w.sum = (float4*)alloc_in(w.pix_allocs, pcap * sizeof(float4));
+        w.sum = (double4*)alloc_in(w.pix_allocs, pcap * sizeof(double4));
         w.sum2 = (float4*)alloc_in(w.pix_allocs, pcap * sizeof(float4));
         if (!w.sum || !w.sum2) { w.release(); return fail(RD_ERR_OOM, "hipMalloc failed for accumulation"); }
         w.pix_cap = pcap;
@@ -373,7 +373,7 @@ struct Bump {
 
 bool carve_scratch(rd_ctx* c, WaveBuffers& w, Bump& b, size_t pool, size_t results, size_t pixels, bool vol) {
     w.segs = (BatchSeg*)b.take(sizeof(BatchSeg) * kMaxSegs);
-    w.sum = (float4*)b.take(pixels * sizeof(float4));
+    w.sum = (double4*)b.take(pixels * sizeof(double4));
     w.sum2 = (float4*)b.take(pixels * sizeof(float4));
     bool ok = w.segs && w.sum && w.sum2 && assign_pool(c, w, pool, vol, [&](size_t bytes) { return b.take(bytes); });
     w.ps.res = (float4*)b.take(results * sizeof(float4));
@@ -1375,11 +1375,12 @@ size_t default_pool(size_t paths) {
 // (a pass may hold several requests, a request may span several passes);
 // within a pass paths stream through the slot pool.
 int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, int32_t pool_paths,
-                    volatile int32_t* cancel, volatile int32_t* progress, bool noalloc = false) {
+                    volatile int32_t* cancel, volatile int32_t* progress, bool noalloc = false,
+                    double* out64 = nullptr) {
     std::vector<Crop> crops(n);
     for (int i = 0; i < n; i++) {
         const rd_request& r = reqs[i];
-        if (!r.scene || !r.out_rgb) return fail(RD_ERR_INVALID, "request without scene or output");
+        if (!r.scene || (!r.out_rgb && !(single && out64))) return fail(RD_ERR_INVALID, "request without scene or output");
         if (r.scene->ctx != c) return fail(RD_ERR_INVALID, "request scene belongs to another context");
         if (r.params.spp <= 0) return fail(RD_ERR_INVALID, "spp must be positive");
         if (int rc = check_stack(r.scene)) return rc;
@@ -1417,7 +1418,7 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
         if (total_pix > kScratchBytes || scratch_need(c, kNoAllocMinPool, max_npix, total_pix, any_vol) > kScratchBytes)
             return fail(RD_ERR_INVALID, "render of " + std::to_string(total_pix) + " pixels does not fit the " +
                                             std::to_string(kScratchBytes >> 20) +
-                                            " MiB work area of rd_render_device_noalloc (48 B per pixel)");
+                                            " MiB work area of rd_render_device_noalloc (64 B per pixel)");
         size_t cap = kNoAllocMaxPool;
         if (pool_paths > 0) cap = std::min(cap, (size_t)pool_paths);
         cap = std::max(std::min(cap, total_paths), kNoAllocMinPool);
@@ -1534,7 +1535,8 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
         wp.width = reqs[i].scene->width;
         HIP_TRY(tm.run("finalize", [&] {
             return launch_finalize(wp, reqs[i].scene->height, reqs[i].params.spp, wb.sum + sum_off[i],
-                                   wb.sum2 + sum_off[i], reqs[i].out_rgb, reqs[i].out_sum2, st);
+                                   wb.sum2 + sum_off[i], reqs[i].out_rgb, reqs[i].out_sum2, single ? out64 : nullptr,
+                                   st);
         }));
     }
     HIP_TRY(hipEventRecord(ev1, st));
@@ -1797,6 +1799,28 @@ int rd_render(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* out_rgb,
     if (!rc && d2 && hipMemcpy(out_sum2, d2, n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(RD_ERR_DEVICE, "copy output");
     (void)hipFree(d1);
     if (d2) (void)hipFree(d2);
+    return rc;
+}
+
+int rd_render_mean_f64(rd_ctx* c, rd_scene* s, const rd_render_params* p, double* out_mean) {
+    if (!c || !s || !p || !out_mean) return fail(RD_ERR_INVALID, "null argument");
+    CtxLock lock(c);
+    if (!lock.ok()) return fail(RD_ERR_BUSY, "context in use by another thread");
+    (void)hipSetDevice(c->device);
+    size_t n = (size_t)s->width * s->height * 3;
+    double* d = nullptr;
+    if (hipMalloc(&d, n * sizeof(double)) != hipSuccess) return fail(RD_ERR_OOM, "hipMalloc output");
+    int rc = RD_OK;
+    // pixels outside a crop keep the caller's values
+    if (hipMemcpy(d, out_mean, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(RD_ERR_DEVICE, "copy output");
+    rd_request r{};
+    r.scene = s;
+    r.params = *p;
+    if (!rc) rc = render_requests(c, 1, &r, true, p->wave_paths, nullptr, nullptr, false, d);
+    if (!rc && hipMemcpy(out_mean, d, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(RD_ERR_DEVICE, "copy output");
+    (void)hipFree(d);
     return rc;
 }
 

@@ -149,7 +149,8 @@ hipError_t launch_ray_sort(const float4* nodes, RayQueue q, int n, int bits, uin
 hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
                         RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in, bool adjoint,
                         const int* rank, bool rank_identity, hipStream_t s);
-hipError_t launch_accumulate_batch(const BatchSeg* segs, int nseg, int max_npix, PathState ps, float4* sum,
+// sum: float64 per-pixel sums {r, g, b, 0}; sum2: float32 sums of squares
+hipError_t launch_accumulate_batch(const BatchSeg* segs, int nseg, int max_npix, PathState ps, double4* sum,
                                    float4* sum2, int* counts, hipStream_t s);
 hipError_t launch_trace_shadow(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, int* counts,
                                int cnt_slot, int sched, hipStream_t s);
@@ -179,9 +180,11 @@ hipError_t launch_chain_advance(const DevScene& sc, const DevScene* scenes, RayQ
                                 RayQueue qout, bool mis, PathState ps, int* counts, int cnt_in, int cnt_out,
                                 hipStream_t s);
 hipError_t launch_resolve_vol(RayQueue qin, PathState ps, int* counts, int cnt_in, hipStream_t s);
-hipError_t launch_finalize(const WaveParams& wp, int height, int spp, const float4* sum, const float4* sum2,
-                           float* out_rgb, float* out_sum2, hipStream_t s);
+// The per-pixel mean into out_rgb (float32), or into out64 (float64) when it is not null.
+hipError_t launch_finalize(const WaveParams& wp, int height, int spp, const double4* sum, const float4* sum2,
+                           float* out_rgb, float* out_sum2, double* out64, hipStream_t s);
 hipError_t launch_zero(float4* buf, int n, hipStream_t s);
+hipError_t launch_zero(double4* buf, int n, hipStream_t s);
 hipError_t launch_zero_counts(int* counts, int n, hipStream_t s);
 hipError_t launch_zero_slots(int* counts, int a, int b, hipStream_t s);
 

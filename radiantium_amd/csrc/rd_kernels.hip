@@ -1579,8 +1579,11 @@ hipError_t launch_resolve(const DevScene& sc, int method, RayQueue qin, PathStat
 
 // Pass accumulation: for every segment of the pass, each of its pixels adds
 // the segment's samples in sample order to its request's sums
-// (BlockBasedRenderer.cs:140-146: invalid samples count as 0).
-__global__ __launch_bounds__(256) void k_accumulate_batch(const BatchSeg* segs, int nseg, PathState ps, float4* sum,
+// (BlockBasedRenderer.cs:140-146: invalid samples count as 0).  Sample
+// radiance is float32; the per-pixel sum is float64 (the reference sums in
+// float32: here only the final per-pixel mean is kept in float64, so a
+// pixel's mean does not round at the sum's ulp), the sum of squares float32.
+__global__ __launch_bounds__(256) void k_accumulate_batch(const BatchSeg* segs, int nseg, PathState ps, double4* sum,
                                                           float4* sum2, int* counts) {
     uint64_t invalid = 0;
     for (int g = blockIdx.y; g < nseg; g += gridDim.y) {
@@ -1588,24 +1591,28 @@ __global__ __launch_bounds__(256) void k_accumulate_batch(const BatchSeg* segs, 
         const int h = sg.npix / sg.crop_w;
         for (int q = blockIdx.x * 256 + threadIdx.x; q < sg.npix; q += gridDim.x * 256) {
             int o = sg.sum_offset + q;
-            float4 s4 = sum[o], q4 = sum2[o];
-            f3 s = mk(s4.x, s4.y, s4.z), s2 = mk(q4.x, q4.y, q4.z);
+            double4 s4 = sum[o];
+            float4 q4 = sum2[o];
+            double sx = s4.x, sy = s4.y, sz = s4.z;
+            f3 s2 = mk(q4.x, q4.y, q4.z);
             const int p0 = sg.path_offset + seg_local(q, 0, sg.crop_w, h, sg.nsamples);
             const int step = seg_local(q, 1, sg.crop_w, h, sg.nsamples) - seg_local(q, 0, sg.crop_w, h, sg.nsamples);
             for (int j = 0; j < sg.nsamples; j++) {
                 f3 c = ld3(ps.res, p0 + j * step);
                 if (!valid_color(c)) { c = mk(0.0f); invalid++; }
-                s = s + c;
+                sx += (double)c.x;
+                sy += (double)c.y;
+                sz += (double)c.z;
                 s2 = s2 + c * c;
             }
-            sum[o] = make_float4(s.x, s.y, s.z, 0.f);
+            sum[o] = make_double4(sx, sy, sz, 0.0);
             sum2[o] = make_float4(s2.x, s2.y, s2.z, 0.f);
         }
     }
     add_counter((unsigned long long*)(counts + kCntInvalid), invalid);
 }
 
-hipError_t launch_accumulate_batch(const BatchSeg* segs, int nseg, int max_npix, PathState ps, float4* sum,
+hipError_t launch_accumulate_batch(const BatchSeg* segs, int nseg, int max_npix, PathState ps, double4* sum,
                                    float4* sum2, int* counts, hipStream_t s) {
     int bx = (max_npix + 255) / 256;
     if (bx > 1024) bx = 1024;
@@ -1615,17 +1622,19 @@ hipError_t launch_accumulate_batch(const BatchSeg* segs, int nseg, int max_npix,
 }
 
 // sum / spp into the column-major ColorBuffer layout (x*H + y)*3
-// (ColorBuffer.cs:35, BlockBasedRenderer.cs:167-178).
-__global__ __launch_bounds__(256) void k_finalize(WaveParams wp, int height, int spp, const float4* sum,
-                                                  const float4* sum2, float* out, float* out2) {
+// (ColorBuffer.cs:35, BlockBasedRenderer.cs:167-178): the mean in float64,
+// delivered as float32 (out) or float64 (out64).
+template <class T>
+__global__ __launch_bounds__(256) void k_finalize(WaveParams wp, int height, int spp, const double4* sum,
+                                                  const float4* sum2, T* out, float* out2) {
     for (int q = blockIdx.x * 256 + threadIdx.x; q < wp.npix; q += gridDim.x * 256) {
         int tx, ty;
         tile_pixel(q, wp.crop_w, wp.crop_h, &tx, &ty);
         int x = wp.x0 + tx, y = wp.y0 + ty;
         size_t o = ((size_t)x * height + y) * 3;
-        float4 s = sum[q];
-        float fs = (float)spp;
-        out[o] = s.x / fs; out[o + 1] = s.y / fs; out[o + 2] = s.z / fs;
+        double4 s = sum[q];
+        double ds = (double)spp;
+        out[o] = (T)(s.x / ds); out[o + 1] = (T)(s.y / ds); out[o + 2] = (T)(s.z / ds);
         if (out2) {
             float4 s2 = sum2[q];
             out2[o] = s2.x; out2[o + 1] = s2.y; out2[o + 2] = s2.z;
@@ -1633,11 +1642,16 @@ __global__ __launch_bounds__(256) void k_finalize(WaveParams wp, int height, int
     }
 }
 
-hipError_t launch_finalize(const WaveParams& wp, int height, int spp, const float4* sum, const float4* sum2,
-                           float* out_rgb, float* out_sum2, hipStream_t s) {
+hipError_t launch_finalize(const WaveParams& wp, int height, int spp, const double4* sum, const float4* sum2,
+                           float* out_rgb, float* out_sum2, double* out64, hipStream_t s) {
     int blocks = (wp.npix + 255) / 256;
     if (blocks > persistent_blocks()) blocks = persistent_blocks();
-    hipLaunchKernelGGL(k_finalize, dim3(blocks), dim3(256), 0, s, wp, height, spp, sum, sum2, out_rgb, out_sum2);
+    if (out64)
+        hipLaunchKernelGGL(k_finalize<double>, dim3(blocks), dim3(256), 0, s, wp, height, spp, sum, sum2, out64,
+                           out_sum2);
+    else
+        hipLaunchKernelGGL(k_finalize<float>, dim3(blocks), dim3(256), 0, s, wp, height, spp, sum, sum2, out_rgb,
+                           out_sum2);
     return hipGetLastError();
 }
 
@@ -1653,6 +1667,16 @@ hipError_t launch_add_f64_to_f32(const double* acc, float* out, int64_t n, hipSt
     return hipGetLastError();
 }
 
+__global__ void k_zero_d4(double4* b, int n) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) b[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+}
+hipError_t launch_zero(double4* buf, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int blocks = (n + 255) / 256;
+    if (blocks > persistent_blocks()) blocks = persistent_blocks();
+    hipLaunchKernelGGL(k_zero_d4, dim3(blocks), dim3(256), 0, s, buf, n);
+    return hipGetLastError();
+}
 __global__ void k_zero4(float4* b, int n) {
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) b[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }

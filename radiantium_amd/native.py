@@ -23,6 +23,7 @@ EXPORTS = [
     "rd_adjoint", "rd_adjoint_device", "rd_scene_set_texture", "rd_scene_set_texture_device",
     "rd_trace_closest_sub", "rd_json_load", "rd_json_scene_desc", "rd_json_scene_free", "rd_scene_load_json",
     "rd_metal_eta_k", "rd_get_tuning", "rd_set_tuning", "rd_autotune", "rd_render_device_noalloc",
+    "rd_render_mean_f64",
 ]
 
 
@@ -73,6 +74,7 @@ def lib():
         L.rd_render.argtypes = [vp, vp, C.POINTER(abi.rd_render_params), vp, vp, vp, vp]
         L.rd_render_device.argtypes = [vp, vp, C.POINTER(abi.rd_render_params), vp, vp, vp, vp]
         L.rd_render_device_noalloc.argtypes = [vp, vp, C.POINTER(abi.rd_render_params), vp, vp, vp, vp]
+        L.rd_render_mean_f64.argtypes = [vp, vp, C.POINTER(abi.rd_render_params), vp]
         L.rd_get_render_stats.argtypes = [vp, C.POINTER(abi.rd_render_stats)]
         L.rd_device_alloc.restype = vp
         L.rd_device_alloc.argtypes = [vp, u64]
@@ -366,6 +368,16 @@ class Scene:
                               C.cast(cancel, C.c_void_p) if cancel is not None else None,
                               C.cast(progress, C.c_void_p) if progress is not None else None))
         return rgb, s2
+
+    def render_mean_f64(self, spp, seed=0, crop=None, wave_paths=0):
+        """rd_render_mean_f64: the per-pixel mean [W*H*3] column-major in float64."""
+        p = abi.rd_render_params()
+        p.spp, p.seed, p.wave_paths = spp, seed, wave_paths
+        if crop is not None:
+            p.x0, p.y0, p.x1, p.y1 = crop
+        out = np.zeros(self.width * self.height * 3, np.float64)
+        check(lib().rd_render_mean_f64(self.ctx.h, self.h, C.byref(p), _ptr(out)))
+        return out
 
     def adjoint(self, spp, dl_dimage, textures, seed=0, crop=None, wave_paths=0):
         """Path-replay adjoint (rd_adjoint): dLoss/dtexel for the listed texture
