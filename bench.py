@@ -39,19 +39,91 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_scene(args):
+def make_scene(args, camera_z=None):
     from radiantium_amd import scenes
     return scenes.displaced_icosphere_scene(level=args.level, width=args.width, height=args.height,
-                                            max_depth=args.max_depth, seed=0)
+                                            max_depth=args.max_depth, seed=0,
+                                            camera_z=args.camera_z if camera_z is None else camera_z)
 
 
-def cpu_baseline(desc, args, gpu_rate):
-    """Times the oracle on a centred crop of the same workload with the box's
-    host cores; the crop is sized from a short calibration to ~15 s."""
+def host_cores():
+    """Cores this process may run on (its CPU affinity; on the GPU box the
+    container's cgroup quota equals it)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def primary_coverage(scene, desc, level, nx=192, ny=108):
+    """Fraction of primary rays through an nx x ny grid of pixel centres that
+    first meet the displaced mesh, the ground, or nothing (pinhole rays with
+    the camera's horizontal fov; traced on the GPU).  The mesh's triangles
+    are the scene's first 20 * 4^level primitives, the ground's the next two."""
+    from radiantium_amd import abi
+    c = desc.camera
+    o = np.array(c["origin"], np.float64)
+    z = np.array(c["target"], np.float64) - o
+    z /= np.linalg.norm(z)
+    x = np.cross(np.array(c["up"], np.float64), z)
+    x /= np.linalg.norm(x)
+    y = np.cross(z, x)
+    tx = np.tan(np.radians(c["fov"]) / 2)
+    ty = tx * c["height"] / c["width"]
+    U, V = np.meshgrid(((np.arange(nx) + 0.5) / nx * 2 - 1) * tx, ((np.arange(ny) + 0.5) / ny * 2 - 1) * ty)
+    d = z + U.reshape(-1, 1) * x - V.reshape(-1, 1) * y
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros(d.shape[0], abi.ray_dtype())
+    rays["o"] = o.astype(np.float32)
+    rays["d"] = d.astype(np.float32)
+    rays["tmin"] = 1e-4
+    rays["tmax"] = 3.4e38
+    prim = scene.trace_closest(rays)["prim"]
+    nmesh = 20 * 4 ** level
+    return {"mesh": round(float(np.mean((prim >= 0) & (prim < nmesh))), 4),
+            "ground": round(float(np.mean((prim >= nmesh) & (prim < nmesh + 2))), 4),
+            "miss": round(float(np.mean(prim < 0)), 4)}
+
+
+def close_framing(ctx, args):
+    """The same workload with the camera at z = 4.6 instead of 7.0, where
+    about half of the primary rays meet the mesh (more nodes per ray)."""
+    from radiantium_amd import native
+    desc = make_scene(args, camera_z=4.6)
+    scene = native.Scene(ctx, desc)
+    cov = primary_coverage(scene, desc, args.level)
+    d_rgb = ctx.alloc(args.width * args.height * 12)
+    scene.render_device(args.spp, d_rgb, seed=1000)
+    ctx.synchronize()
+    rays, agg = 0, {"class_rays": [0, 0, 0], "class_nodes": [0, 0, 0]}
+    t0 = time.perf_counter()
+    for i in range(2):
+        scene.render_device(args.spp, d_rgb, seed=i)
+        st = ctx.render_stats()
+        rays += st["rays_closest"] + st["rays_any"]
+        for k in agg:
+            agg[k] = [a + b for a, b in zip(agg[k], st[k])]
+    ctx.synchronize()
+    secs = time.perf_counter() - t0
+    ctx.free(d_rgb)
+    scene.close()
+    rate = rays / secs / 1e6
+    log(f"[bench] close framing (camera z 4.6): {rate:.1f} Mrays/s, primary rays {cov}")
+    return {"mrays_s": round(rate, 1), "ms_per_frame": round(secs / 2 * 1000, 1), "camera_z": 4.6,
+            "primary_rays": cov,
+            "nodes_per_ray": [round(n / max(r, 1), 2) for n, r in zip(agg["class_nodes"], agg["class_rays"])]}
+
+
+def cpu_baseline(desc, args, gpu_rate, scene):
+    """Times the oracle on the same workload with every host core this
+    process may use: the whole frame when a short calibration says it fits
+    ~45 s, else a centred crop sized to ~15 s.  Then checks equal RMSE: the
+    GPU renders the same pixels at the same spp with an independent seed,
+    and both images are compared with a 16x-spp GPU render of them."""
     from oracle import cpuref
     if not os.path.exists(cpuref.LIB_PATH):
         cpuref.build()
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_cores()
     t0 = time.perf_counter()
     o = cpuref.OracleScene(desc)
     log(f"[cpu] oracle BVH build {time.perf_counter() - t0:.1f}s, threads={threads}")
@@ -72,28 +144,47 @@ def cpu_baseline(desc, args, gpu_rate):
     else:
         scale = (args.cpu_seconds / max(secs, 1e-3)) ** 0.5
         cw, ch = min(W, max(16, int(cw * scale) // 4 * 4)), min(H, max(16, int(ch * scale) // 4 * 4))
-    _, _, st, secs = o.render(args.spp, seed=1, crop=crop(cw, ch), threads=threads, with_sum2=False)
+    cr = crop(cw, ch)
+    cpu_img, _, st, secs = o.render(args.spp, seed=1, crop=cr, threads=threads, with_sum2=False)
     rays = st["rays_closest"] + st["rays_any"]
     rate = rays / secs / 1e6
     log(f"[cpu] crop {cw}x{ch} spp {args.spp}: {rays} rays in {secs:.2f}s = {rate:.3f} Mrays/s")
+    # equal RMSE: the same estimator at the same spp, against a converged image
+    gpu_img, _ = scene.render(args.spp, seed=2, crop=cr, with_sum2=False)
+    ref, _ = scene.render(args.spp * 16, seed=3, crop=cr, with_sum2=False)
+    x, y = np.meshgrid(np.arange(cr[0], cr[2]), np.arange(cr[1], cr[3]), indexing="ij")
+    idx = ((x * H + y) * 3).reshape(-1, 1) + np.arange(3)
+    rm = lambda a: float(np.sqrt(np.mean((a[idx].astype(np.float64) - ref[idx]) ** 2)))
+    rmse_cpu, rmse_gpu = rm(cpu_img), rm(gpu_img)
+    log(f"[cpu] RMSE against {args.spp * 16} spp: oracle {rmse_cpu:.5f}, GPU {rmse_gpu:.5f}")
     what = "the whole frame" if (cw, ch) == (W, H) else f"a centred {cw}x{ch} crop of the frame"
-    return {"value": round(rate, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{what} at {args.spp} spp, seed 1: {rays} BVH queries in {secs:.1f} s"}
+    return {"value": round(rate, 4), "unit": "Mrays/s", "cores": threads, "host_cpus": os.cpu_count(),
+            "kind": "port",
+            "sample": f"{what} at {args.spp} spp, seed 1: {rays} BVH queries in {secs:.1f} s",
+            "rmse": {"cpu": round(rmse_cpu, 6), "gpu": round(rmse_gpu, 6), "ratio": round(rmse_gpu / rmse_cpu, 4),
+                     "equal": bool(abs(rmse_gpu / rmse_cpu - 1) < 0.05),
+                     "against": f"the GPU render of the same pixels at {args.spp * 16} spp (seed 3); "
+                                f"CPU seed 1, GPU seed 2, same spp"}}
 
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "traffic_icosphere_l8_1080p_64spp.json")
 
 
-def pmc_traffic(stage, args):
-    """HBM bytes per launch of `stage` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE runs (scaled by a plain copy kernel's counters), when this run is
-    the profiled workload; None otherwise."""
-    if (args.width, args.height, args.spp, args.level, args.max_depth) != (1920, 1080, 64, 8, 12):
+def pmc_stage(stage, args):
+    """Per-frame counter figures of `stage` from the committed rocprofv3 runs
+    of this workload (one frame each): bytes fetched and written at the
+    L2 / fabric interface (FETCH_SIZE, WRITE_SIZE scaled by a plain copy
+    kernel's counters; the fabric serves them from the Infinity Cache or
+    HBM), the L2 hit rate and the occupancy, summed or weighted over every
+    launch of the stage in the frame.  None unless this run is the profiled
+    workload."""
+    if (args.width, args.height, args.spp, args.level, args.max_depth, args.camera_z) != (1920, 1080, 64, 8, 12, 7.0):
         return None
     try:
         with open(TRAFFIC_FILE) as f:
-            return json.load(f)["stages"][stage]["hbm_bytes_per_launch"]
+            st = json.load(f)["stages"][stage]
+        return st if "fabric_bytes_per_frame" in st else None
     except (OSError, KeyError, ValueError):
         return None
 
@@ -113,6 +204,7 @@ def batched_speedup(ctx, args, small=False):
         reqs = scenes.batch_requests(args.batch_seed)
     sc = [native.Scene(ctx, d) for d, _, _ in reqs]
     outs = [ctx.alloc(d.camera["width"] * d.camera["height"] * 12) for d, _, _ in reqs]
+    outs_seq = [ctx.alloc(d.camera["width"] * d.camera["height"] * 12) for d, _, _ in reqs]
     paths = sum(d.camera["width"] * d.camera["height"] * spp for d, spp, _ in reqs)
     # warm both paths once at low spp
     for s_, o in zip(sc, outs):
@@ -121,7 +213,7 @@ def batched_speedup(ctx, args, small=False):
     ctx.synchronize()
     t0 = time.perf_counter()
     rays_seq = 0
-    for s_, (d, spp, seed), o in zip(sc, reqs, outs):
+    for s_, (d, spp, seed), o in zip(sc, reqs, outs_seq):
         s_.render_device(spp, o, seed=seed)
         st = ctx.render_stats()
         rays_seq += st["rays_closest"] + st["rays_any"]
@@ -133,12 +225,19 @@ def batched_speedup(ctx, args, small=False):
     bat = time.perf_counter() - t0
     st = ctx.render_stats()
     rays_bat = st["rays_closest"] + st["rays_any"]
-    for o in outs:
+    same = True
+    for (d, _, _), a, b in zip(reqs, outs, outs_seq):
+        n = d.camera["width"] * d.camera["height"] * 3
+        ia = ctx.d2h(np.empty(n, np.float32), a)
+        ib = ctx.d2h(np.empty(n, np.float32), b)
+        same = same and bool(np.array_equal(ia.view(np.uint32), ib.view(np.uint32)))
+    for o in outs + outs_seq:
         ctx.free(o)
     log(f"[bench] batched: {len(reqs)} requests, {paths / 1e6:.0f} M paths; sequential {seq * 1000:.0f} ms, "
         f"batched {bat * 1000:.0f} ms")
     return {"speedup": round(seq / bat, 3), "sequential_ms": round(seq * 1000, 1), "batched_ms": round(bat * 1000, 1),
             "requests": len(reqs), "paths": paths, "rays": rays_bat, "rays_equal": rays_seq == rays_bat,
+            "images_bit_identical": same,
             "batched_mrays_s": round(rays_bat / bat / 1e6, 1),
             "workload": f"16 vol_path glass-medium Cornell boxes, seed {args.batch_seed}, "
                         + ("res {128,256}^2, spp {16,64}" if small else "res 128^2-1024^2, spp 16-256") + ", depth 64"}
@@ -262,7 +361,7 @@ def instanced_throughput(ctx, args):
     if not args.no_cpu_baseline:
         if not os.path.exists(cpuref.LIB_PATH):
             cpuref.build()
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = host_cores()
         o = cpuref.OracleScene(desc)
         cw = ch = 64
         crop = ((W - cw) // 2, (H - ch) // 2, (W + cw) // 2, (H + ch) // 2)
@@ -304,6 +403,8 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--max-depth", type=int, default=12)
+    ap.add_argument("--camera-z", type=float, default=7.0, help="camera distance of the main workload")
+    ap.add_argument("--no-close", action="store_true", help="skip the close-framing (camera z 4.6) frame")
     ap.add_argument("--wave-paths", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -315,6 +416,8 @@ def main():
     ap.add_argument("--adj-seed", type=int, default=9, help="texture seed of the adjoint target")
     ap.add_argument("--adj-reps", type=int, default=3, help="timed primal/adjoint pairs")
     ap.add_argument("--adj-steps", type=int, default=6, help="optimisation steps through the autograd Function")
+    ap.add_argument("--only-instanced", action="store_true",
+                    help="run the instanced-workload frame alone (for profiling); prints its record")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (scale for PMC byte counters)")
     args = ap.parse_args()
@@ -328,6 +431,10 @@ def main():
 
     if args.copy_calibration:
         copy_calibration()
+    if args.only_instanced:
+        args.no_cpu_baseline = True
+        print(json.dumps({"instanced": instanced_throughput(native.Context(0), args)}), flush=True)
+        return
     t0 = time.perf_counter()
     desc = make_scene(args)
     ntri = desc.arrays()["indices"].shape[0]
@@ -338,6 +445,8 @@ def main():
     info = scene.info()
     log(f"[bench] BVH {info['num_nodes']} nodes, depth {info['max_depth']}, build {info['build_ms']:.0f} ms, "
         f"upload total {time.perf_counter() - t0:.1f}s")
+    coverage = primary_coverage(scene, desc, args.level)
+    log(f"[bench] primary rays (camera z {args.camera_z}): {coverage}")
     npix = args.width * args.height
     d_rgb = ctx.alloc(npix * 12)
     for i in range(args.warmup):
@@ -381,12 +490,28 @@ def main():
         rays_c = sum(agg["class_rays"][c] for c, _ in cls)
         per_launch = total_bytes / cnt
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
+        launches_per_frame = cnt / args.steps
+        pmc = pmc_stage(trace_dom, args)
+        # the counters' per-frame bytes over this run's launches per frame: the
+        # same launch partition as the algorithmic bytes per launch
+        traffic = int(pmc["fabric_bytes_per_frame"] / launches_per_frame) if pmc else None
         roofline = {"kernel": trace_dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": pmc_traffic(trace_dom, args),
-                    "bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg_ms, 4),
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "bytes_per_launch": int(per_launch), "bytes_per_frame": int(total_bytes / args.steps),
+                    "launches_per_frame": round(launches_per_frame, 2), "avg_launch_ms": round(avg_ms, 4),
                     "nodes_per_ray": round(sum(agg["class_nodes"][c] for c, _ in cls) / max(rays_c, 1), 2),
-                    "prims_per_ray": round(sum(agg["class_prims"][c] for c, _ in cls) / max(rays_c, 1), 2)}
+                    "prims_per_ray": round(sum(agg["class_prims"][c] for c, _ in cls) / max(rays_c, 1), 2),
+                    "achieved_is": "algorithmic bytes (32 B ray + hit + 32 B per node + 36 B per triangle, "
+                                   "counted by the kernel) per launch / launch time"}
+        if pmc:
+            fabric_gbs = traffic / (avg_ms * 1e-3) / 1e9
+            roofline.update({
+                "fabric_achieved": round(fabric_gbs, 1), "fabric_frac": round(fabric_gbs / HBM_PEAK_GBS, 4),
+                "fabric_share_of_algorithmic": round(traffic / per_launch, 4),
+                "l2_hit_rate": pmc.get("l2_hit_rate"), "occupancy_pct": pmc.get("occupancy_pct"),
+                "limiter": "L1 / texture-address path: the bytes that reach the L2-fabric interface "
+                           "(Infinity Cache + HBM) are a fifth of the algorithmic bytes, and the kernel runs "
+                           "well below HBM bandwidth on them"})
     if dom != trace_dom and roofline is not None:
         roofline["note"] = f"longest stage is {dom}; roofline reported for the longest traversal kernel"
 
@@ -403,9 +528,13 @@ def main():
     if not args.no_instanced:
         instanced = instanced_throughput(ctx, args)
 
+    close = None
+    if not args.no_close:
+        close = close_framing(ctx, args)
+
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(desc, args, value)
+        cpu = cpu_baseline(desc, args, value, scene)
 
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
@@ -413,11 +542,14 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"displaced icosphere level {args.level} ({ntri} triangles) + rough-glass ground "
                                f"+ quad light; path Mis; {args.width}x{args.height}, {args.spp} spp, max depth "
-                               f"{args.max_depth}, bvh max_prim 4",
+                               f"{args.max_depth}, bvh max_prim 4; camera (0, 2.2, {args.camera_z:g}) -> (0, 0.9, 0), "
+                               f"fov 40: primary rays {coverage['mesh']:.0%} mesh, {coverage['ground']:.0%} ground, "
+                               f"{coverage['miss']:.0%} miss",
+                   "camera_z": args.camera_z, "primary_rays": coverage,
                    "width": args.width, "height": args.height, "spp": args.spp, "max_depth": args.max_depth,
                    "triangles": ntri, "parallelism": "1 GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "adjoint": adjoint,
-        "instanced": instanced,
+        "instanced": instanced, "close_framing": close,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

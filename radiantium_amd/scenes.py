@@ -522,7 +522,7 @@ def value_noise(p, seed, octaves=3, base_freq=4.0):
 
 
 def displaced_icosphere_scene(level=8, width=1920, height=1080, max_depth=12, seed=0, groups=64,
-                              with_light=True):
+                              with_light=True, camera_z=7.0):
     """1M-triangle workload: icosphere subdivided to `level` (level 8 =
     1,310,720 triangles), radially displaced by seeded 3-octave value noise
     of amplitude 0.05, over a ground quad.  The mesh is split into `groups`
@@ -532,7 +532,10 @@ def displaced_icosphere_scene(level=8, width=1920, height=1080, max_depth=12, se
     ground is a rough_glass with etaB = 1.5 (roughness: reference default
     0.3).  The workload names no emitter, so a 2 x 2 quad area light of
     radiance (8,8,8) faces down from y = 4 (one light per triangle, as the
-    reference builds emissive meshes).  BVH max_prim 4."""
+    reference builds emissive meshes).  BVH max_prim 4.  The camera (fov 40)
+    looks from (0, 2.2, camera_z) at (0, 0.9, 0): at camera_z = 7.0 about
+    22 % of the primary rays meet the mesh (50 % the ground, 28 % nothing),
+    at camera_z = 4.6 about half of them."""
     rng = np.random.default_rng(seed)
     v, f = icosphere(level)
     disp = value_noise(v, seed, octaves=3)
@@ -562,7 +565,7 @@ def displaced_icosphere_scene(level=8, width=1920, height=1080, max_depth=12, se
         lm = s.add_material("diffuse", kd=(0.0, 0.0, 0.0))
         y = 4.0
         s.add_mesh(*_quad((-1, y, -1), (1, y, -1), (1, y, 1), (-1, y, 1)), material=lm, le=(8.0, 8.0, 8.0))
-    s.set_camera(fov=40.0, width=width, height=height, origin=(0.0, 2.2, 7.0), target=(0.0, 0.9, 0.0),
+    s.set_camera(fov=40.0, width=width, height=height, origin=(0.0, 2.2, float(camera_z)), target=(0.0, 0.9, 0.0),
                  up=(0.0, 1.0, 0.0))
     s.set_integrator(type="path", max_depth=max_depth, method="Mis")
     s.bvh_max_prim = 4
