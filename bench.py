@@ -348,12 +348,20 @@ def instanced_throughput(ctx, args):
     d_rgb = ctx.alloc(W * H * 12)
     scene.render_device(1, d_rgb, seed=99)
     ctx.synchronize()
+    ctx.reset_kernel_timing()
     t0 = time.perf_counter()
     scene.render_device(spp, d_rgb, seed=1)
     ctx.synchronize()
     secs = time.perf_counter() - t0
     st = ctx.render_stats()
     ctx.free(d_rgb)
+    stages = {}
+    for name in ("compact", "sort", "trace_closest", "shade", "compact_shadow", "trace_secondary", "trace_any",
+                 "trace_mis", "resolve", "accumulate"):
+        avg, cnt = ctx.kernel_timing(name)
+        if cnt:
+            stages[name] = round(avg * cnt, 1)
+            log(f"[bench] instanced stage {name:15s} avg {avg * 1000:9.1f} us x {cnt:4d} = {avg * cnt:8.1f} ms")
     rays = st["rays_closest"] + st["rays_any"]
     rate = rays / secs / 1e6
     log(f"[bench] instanced: {secs * 1000:.0f} ms, {rays / 1e6:.0f} M rays, {rate:.1f} Mrays/s")
@@ -374,7 +382,7 @@ def instanced_throughput(ctx, args):
     return {"mrays_s": round(rate, 1), "ms": round(secs * 1000, 1), "rays": rays,
             "nodes_per_ray": [round(n / max(r, 1), 2) for n, r in zip(st["class_nodes"], c)],
             "prims_per_ray": [round(n / max(r, 1), 2) for n, r in zip(st["class_prims"], c)],
-            "stack_depth": info["max_depth"], "cpu_baseline": cpu,
+            "stack_depth": info["max_depth"], "stage_ms": stages, "cpu_baseline": cpu,
             "speedup_vs_cpu": round(rate / cpu["value"], 1) if cpu else None,
             "workload": f"1024 instances of 8 superquadrics (99,904 tris each), 2048x1024 HDR sky + sun; "
                         f"path Mis, {W}x{H}, {spp} spp, max depth 16"}

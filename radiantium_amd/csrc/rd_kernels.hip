@@ -406,12 +406,19 @@ __device__ __forceinline__ int next_chunk(int* cursor, int size = 64) {
     return __shfl(base, 0);
 }
 
-// Blocks per CU that the LDS stack allows (160 KB of LDS per CU), at most 8:
-// the register budget the traversal kernels are compiled for, so registers
-// never lower the occupancy below what the stack already sets (the
-// two-level kernel is left unbounded: at that budget it spills heavily).
+// Waves per SIMD that the LDS stack allows (160 KB of LDS per CU, four SIMDs;
+// a wavefront's stack takes stack x 64 x 4 B), at most 8: the second
+// argument of __launch_bounds__ on this target is the minimum waves per SIMD,
+// so the traversal kernels are compiled for the register budget of exactly
+// the occupancy the stack already sets, whatever their block size.
 constexpr int lds_blocks(int stack, int block) {
-    return (160 * 1024) / (stack * block * 4) < 8 ? (160 * 1024) / (stack * block * 4) : 8;
+    return (160 * 1024) / (stack * 64 * 4) / 4 < 8 ? (160 * 1024) / (stack * 64 * 4) / 4 : 8;
+}
+// The two-level kernels hold more state (the world ray, the scene leaf, the
+// set's best hit): their budget is capped at 96 (straight) or 128 (sequence)
+// VGPRs, above which shallow two-level trees would spill on every step.
+constexpr int inst_waves(int stack, int block, int cap) {
+    return lds_blocks(stack, block) < cap ? lds_blocks(stack, block) : cap;
 }
 
 // Schedule 0 / 2 ("straight"): each lane runs its ray to completion; rays
@@ -459,9 +466,10 @@ __global__ __launch_bounds__(kBlock, lds_blocks(kStack, kBlock)) void k_trace_st
 }
 
 // Instanced scenes: straight schedule over the two-level DFS; closest modes
-// also write the set triangle of each hit to a.sub.
+// also write the set triangle of each hit to a.sub.  Compiled for the
+// occupancy the LDS stack allows, as the one-level kernels are.
 template <int kMode, int kStack, int kBlock, bool kBatch>
-__global__ __launch_bounds__(kBlock) void k_trace_inst(DevScene sc, TraceArgs a) {
+__global__ __launch_bounds__(kBlock, inst_waves(kStack, kBlock, 5)) void k_trace_inst(DevScene sc, TraceArgs a) {
     constexpr bool kAny = (kMode == kModeRaysAny || kMode == kModeShadow);
     __shared__ int lds[kStack * kBlock];
     int* stk = lds + threadIdx.x;
@@ -558,8 +566,9 @@ __global__ __launch_bounds__(kBlock, lds_blocks(kStack, kBlock)) void k_trace_se
 // rays (few, often long) run beside the shadow rays instead of leaving the
 // chip idle behind their longest one.  Each ray's query and result are those
 // of the separate MIS (closest hit + Le) and shadow (any hit) launches.
-template <int kStack, int kBlock, bool kBatch>
-__global__ __launch_bounds__(kBlock, lds_blocks(kStack, kBlock)) void k_trace_secondary(DevScene sc, TraceArgs a) {
+template <int kStack, int kBlock, bool kBatch, bool kInst>
+__global__ __launch_bounds__(kBlock, kInst ? inst_waves(kStack, kBlock, 4) : lds_blocks(kStack, kBlock))
+void k_trace_secondary(DevScene sc, TraceArgs a) {
     __shared__ int lds[kStack * kBlock];
     int* stk = lds + threadIdx.x;
     const int n1 = a.counts[a.cnt_slot];
@@ -572,11 +581,27 @@ __global__ __launch_bounds__(kBlock, lds_blocks(kStack, kBlock)) void k_trace_se
         const RayQueue& q = mis ? a.q : a.q2;
         const int pid = q.pid[r];
         if (pid < 0) return;   // a hole of the ranked shadow queue
-        Trav t;
-        trav_begin(t, q.org[r], q.dir[r]);
         const DevScene& s2 = kBatch ? a.scenes[a.ps.req[pid]] : sc;
         const float4* nodes = s2.nodes;
         const float4* prims = s2.prims;
+        if constexpr (kInst) {   // the two-level DFS (Primitive.cs:144-189)
+            TravI ti;
+            travi_begin(ti, q.org[r], q.dir[r]);
+            if (mis) {
+                if (s2.num_nodes > 0)
+                    while (!trav_step_inst<false, kBlock>(s2, nodes, prims, ti, stk, nv_m, nt_m)) {}
+                trav_finish<kModeMis>(s2, a, r, ti.t);
+                nr_m++;
+            } else {
+                if (s2.num_nodes > 0)
+                    while (!trav_step_inst<true, kBlock>(s2, nodes, prims, ti, stk, nv_s, nt_s)) {}
+                a.ps.vis[pid] = ti.t.best.prim >= 0 ? 0 : 1;
+                nr_s++;
+            }
+            return;
+        }
+        Trav t;
+        trav_begin(t, q.org[r], q.dir[r]);
         if (mis) {
             while (!trav_step<false, kBlock>(nodes, prims, t, stk, nv_m, nt_m)) {}
             trav_finish<kModeMis>(s2, a, r, t);
@@ -690,16 +715,24 @@ static void launch_resident(void (*kernel)(DevScene, TraceArgs), int blocks, int
 
 template <int kMode, int kStack, int kBlock>
 static void launch_trace_variant(const DevScene& sc, const TraceArgs& a, int blocks, int schedule, hipStream_t s) {
+    if constexpr (kMode == kModeSecondary) {
+        if (sc.num_insts > 0) {
+            if (a.scenes) launch_resident(k_trace_secondary<kStack, kBlock, true, true>, blocks, kBlock, s, sc, a);
+            else launch_resident(k_trace_secondary<kStack, kBlock, false, true>, blocks, kBlock, s, sc, a);
+        } else {
+            if (a.scenes) launch_resident(k_trace_secondary<kStack, kBlock, true, false>, blocks, kBlock, s, sc, a);
+            else launch_resident(k_trace_secondary<kStack, kBlock, false, false>, blocks, kBlock, s, sc, a);
+        }
+        return;
+    }
+    // instanced scenes: the two-level kernel, sequences taken as chunks (a
+    // lane's rays back to back measured slower there: its lanes then diverge
+    // between scene and set steps)
     if (sc.num_insts > 0) {
         if (kMode >= kModeExt && a.scenes)
             launch_resident(k_trace_inst<kMode, kStack, kBlock, kMode >= kModeExt>, blocks, kBlock, s, sc, a);
         else
             launch_resident(k_trace_inst<kMode, kStack, kBlock, false>, blocks, kBlock, s, sc, a);
-        return;
-    }
-    if constexpr (kMode == kModeSecondary) {
-        if (a.scenes) launch_resident(k_trace_secondary<kStack, kBlock, true>, blocks, kBlock, s, sc, a);
-        else launch_resident(k_trace_secondary<kStack, kBlock, false>, blocks, kBlock, s, sc, a);
         return;
     }
     if constexpr (kMode >= kModeExt) {
@@ -721,8 +754,7 @@ static hipError_t launch_trace_mode(const DevScene& sc, const TraceArgs& a0, int
     int pb = persistent_blocks();
     if (kMode <= kModeRaysAny || sched < 0 || sched >= kNumSchedules) sched = 0;
     TraceArgs a = a0;
-    // straight kernels: 2 = cursor chunks; 3 = sequences (instanced
-    // scenes, which have no sequence kernel, take chunks)
+    // straight kernels: 2 = cursor chunks; 3 = sequences
     a.dyn = sched == 2 ? 1 : (sched == 3 ? 2 : 0);
     if (need <= 24) {
         int b = blocks_hint > 0 ? std::min(blocks_hint, pb) : pb;
@@ -734,6 +766,11 @@ static hipError_t launch_trace_mode(const DevScene& sc, const TraceArgs& a0, int
     } else if (need <= 32) {
         int b = blocks_hint > 0 ? std::min(blocks_hint, pb) : pb;
         launch_trace_variant<kMode, 32, 256>(sc, a, b, sched, s);
+    } else if (need <= 40) {
+        // 40 entries x 128 lanes x 4 B: eight blocks, four waves per SIMD (the
+        // two-level trees of a 14-deep scene over 26-deep instance sets)
+        int b = blocks_hint > 0 ? std::min(2 * blocks_hint, 2 * pb) : 2 * pb;
+        launch_trace_variant<kMode, 40, 128>(sc, a, b, sched, s);
     } else if (need <= 48) {
         int b = blocks_hint > 0 ? std::min(2 * blocks_hint, 2 * pb) : 2 * pb;
         launch_trace_variant<kMode, 48, 128>(sc, a, b, sched, s);
@@ -1196,12 +1233,12 @@ hipError_t launch_trace_shadow(const DevScene& sc, const DevScene* scenes, RayQu
 }
 
 // The MIS rays (closest hit + Le) and the shadow rays (any hit) of a bounce:
-// one fused launch on plain scenes with a straight schedule, else the two
-// launches in turn.
+// one fused launch (the one- or two-level traversal) with a straight
+// schedule, else the two launches in turn.
 hipError_t launch_trace_secondary(const DevScene& sc, const DevScene* scenes, RayQueue qshadow, int cnt_shadow,
                                   RayQueue qmis, PathState ps, int* counts, int sched_shadow, int sched_mis,
                                   hipStream_t s) {
-    if (sc.num_insts > 0 || sched_shadow == 1 || sched_mis == 1 || sched_shadow == 3 || sched_mis == 3) {
+    if (sched_shadow == 1 || sched_mis == 1 || sched_shadow == 3 || sched_mis == 3) {
         if (hipError_t e = launch_trace_mis(sc, scenes, qmis, ps, counts, sched_mis, s)) return e;
         return launch_trace_shadow(sc, scenes, qshadow, ps, counts, cnt_shadow, sched_shadow, s);
     }

@@ -112,3 +112,24 @@ def test_autotune(ctx):
         assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
     finally:
         ctx.set_tuning(**base)
+
+
+def test_instanced_schedules_bit_identical(ctx):
+    """The two-level traversal under every schedule (sequences, chunks,
+    grid-stride) gives the same image and query counts; more rays than the
+    chip has lanes."""
+    base = ctx.tuning()
+    desc = scenes.instanced_scene(448, 448, n_instances=96, n_sets=3, nu=64, nv=32, seed=2, max_depth=8)
+    sc = Scene(ctx, desc)
+    try:
+        ref, rst = _render(ctx, sc, 4, trace_schedule=(0, 0, 0))
+        print(f"[schedules] instanced: class rays {rst['class_rays']}", flush=True)
+        assert rst["paths"] > 700_000
+        for sched in ((3, 2, 2), (2, 2, 2), (3, 3, 3), (1, 1, 1), (2, 3, 3)):
+            img, st = _render(ctx, sc, 4, trace_schedule=sched)
+            print(f"[schedules] instanced: {sched}", flush=True)
+            assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), sched
+            for k in KEYS:
+                assert st[k] == rst[k], (sched, k, st[k], rst[k])
+    finally:
+        ctx.set_tuning(**base)
