@@ -394,7 +394,23 @@ typedef struct rd_tuning {
                                   resolve on a second stream beside the next
                                   bounce's compaction, sort and extension
                                   traversal (once every path has started)   */
-    int32_t reserved[7];       /* 0                                         */
+    int32_t device_loop;       /* the tail of a pass (every path started, at
+                                  most tail_rays rays in the queue) runs
+                                  device-driven: each bounce's kernels read
+                                  their queue sizes on the device and the
+                                  host checks for the end once per 8
+                                  bounces; 0 off (one host sync per bounce),
+                                  1 kernel launches, 2 two bounces captured
+                                  as one hipGraph and replayed (default;
+                                  rd_render_params.flags RD_RENDER_NO_GRAPH
+                                  makes it 1 for that render)              */
+    int32_t tail_rays;         /* queue size below which the tail starts    */
+    int32_t chain_lockstep;    /* vol_path chains: 1 one segment per launch
+                                  over the chain queues, the host looping
+                                  until both are empty; 0 (default) every
+                                  chain walked to its end by one lane in
+                                  one launch                               */
+    int32_t reserved[4];       /* 0                                         */
 } rd_tuning;
 
 int rd_get_tuning(const rd_ctx* ctx, rd_tuning* out);

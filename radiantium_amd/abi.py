@@ -163,7 +163,8 @@ class rd_tuning(C.Structure):
     _fields_ = [
         ("trace_schedule", C.c_int32 * 3), ("ray_sort", C.c_int32), ("ray_sort_bits", C.c_int32),
         ("ray_sort_min", C.c_int32), ("ray_sort_shadow", C.c_int32), ("overlap", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("device_loop", C.c_int32), ("tail_rays", C.c_int32), ("chain_lockstep", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 

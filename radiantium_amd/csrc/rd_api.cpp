@@ -11,6 +11,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/radiantium_hip.h"
@@ -146,6 +147,17 @@ struct rd_ctx {
     // for the shadow and MIS rays (which the fused launch then traces together)
     int trace_schedule[3] = {3, 2, 2};
     bool overlap = true;   // RD_OVERLAP=0 turns the two-stream bounce overlap off
+    // the tail of a pass device-driven (RD_DEVICE_LOOP: 0 off, 1 launches,
+    // 2 captured as a hipGraph), from queues of at most tail_rays rays
+    // (RD_TAIL_RAYS); vol_path chains in lockstep launches (RD_CHAIN_LOCKSTEP)
+    int device_loop = 2;
+    int tail_rays = 1 << 18;
+    bool chain_lockstep = false;
+    // the last tail graph instantiated, and the bytes of every value its
+    // launches were recorded with: a pass whose tail would record the same
+    // launches replays it without capturing again
+    hipGraphExec_t tail_exec = nullptr;
+    std::string tail_key;
     // scenes created on this context and not yet destroyed; rd_destroy with
     // live scenes releases the wavefront buffers and leaves the rest to the
     // last rd_scene_destroy, so destruction order does not matter
@@ -244,10 +256,6 @@ struct Timer {
         pairs.clear();
     }
 };
-
-// A Transmittance / IntersectTr chain takes one step per medium boundary it
-// crosses; this bounds the host loop for pathological geometry.
-constexpr int kMaxChainSteps = 1 << 16;
 
 // Points the path pool of `w` (cap slots: path state, ray queues, hits,
 // compaction scratch, + the vol_path arrays) at memory from `alloc`
@@ -508,6 +516,9 @@ rd_ctx* rd_create(int device) {
     if (const char* env = getenv("RD_RAY_SORT_SHADOW")) c->ray_sort_shadow = atoi(env) != 0;
     if (const char* env = getenv("RD_RAY_SORT_BITS")) c->ray_sort_bits = std::max(3, std::min(30, atoi(env)));
     if (const char* env = getenv("RD_OVERLAP")) c->overlap = atoi(env) != 0;
+    if (const char* env = getenv("RD_DEVICE_LOOP")) c->device_loop = std::max(0, std::min(2, atoi(env)));
+    if (const char* env = getenv("RD_TAIL_RAYS")) c->tail_rays = std::max(0, atoi(env));
+    if (const char* env = getenv("RD_CHAIN_LOCKSTEP")) c->chain_lockstep = atoi(env) != 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         g_err = "hipStreamCreate failed"; delete c; return nullptr;
     }
@@ -553,6 +564,7 @@ static void free_ctx(rd_ctx* c) {
     if (c->na_host_segs) (void)hipHostFree(c->na_host_segs);
     if (c->ev_t0) (void)hipEventDestroy(c->ev_t0);
     if (c->ev_t1) (void)hipEventDestroy(c->ev_t1);
+    if (c->tail_exec) (void)hipGraphExecDestroy(c->tail_exec);
     (void)hipEventDestroy(c->ev_shaded);
     (void)hipEventDestroy(c->ev_resolved);
     (void)hipStreamDestroy(c->stream2);
@@ -1131,7 +1143,151 @@ struct PassLaunch {
     int pool;      // path slots in flight
     WaveBuffers* wb = nullptr;   // the pass's buffers (null: the context's own)
     bool sort_ok = true;         // the coherence sort may run (its buffers are allocated on demand)
+    bool graph_ok = true;        // the device-driven tail may be captured as a hipGraph
 };
+
+// One bounce of the device-driven tail: the launches of run_pass's bounce
+// once every path has started (the same stages and streams, the shadow / MIS
+// traversal and resolve beside the next bounce's compaction and extension
+// traversal on stream2), without the sort, and with every kernel reading its
+// queue size on the device; the grids of the compactions are sized for
+// `bound` rays (once every path of the pass has started a queue only
+// shrinks).  *pending: this bounce's resolve is still queued on stream2.
+hipError_t tail_bounce(rd_ctx* c, const PassLaunch& L, WaveBuffers& wb, RayQueue qin, RayQueue qout, int cin, int cout,
+                       int bound, bool* pending) {
+    hipStream_t st = c->stream;
+    hipError_t e;
+#define TAIL_TRY(x) do { if ((e = (x)) != hipSuccess) return e; } while (0)
+    TAIL_TRY(launch_tail_tick(c->counts, cin, st));
+    const int za = *pending ? -1 : cout, zb = *pending ? -1 : kCntShadow, zc = *pending ? -1 : kCntMis;
+    TAIL_TRY(launch_trace_ext(L.sc, L.scenes, qin, wb.ps, wb.hits, c->counts, cin, za, zb, zc, nullptr, nullptr,
+                              c->trace_schedule[0], st));
+    if (L.vol) {
+        TAIL_TRY(launch_shade_vol(L.sc, L.scenes, qin, wb.hits, wb.qshadow, wb.qmis, wb.ps, c->counts, cin, st));
+        TAIL_TRY(launch_chain_full(L.sc, L.scenes, wb.qshadow, wb.ps, false, c->counts, kCntShadow, st));
+        TAIL_TRY(launch_chain_full(L.sc, L.scenes, wb.qmis, wb.ps, true, c->counts, kCntMis, st));
+        TAIL_TRY(launch_resolve_vol(qin, wb.ps, c->counts, cin, st));
+    } else {
+        if (*pending) {   // shade adds to L, which the last bounce's resolve also writes
+            TAIL_TRY(hipStreamWaitEvent(st, c->ev_resolved, 0));
+            *pending = false;
+        }
+        const bool ranked = L.method != RD_METHOD_BXDF;
+        TAIL_TRY(launch_shade(L.sc, L.scenes, qin, wb.hits, wb.qshadow, wb.qmis, wb.ps, c->counts, cin, false, nullptr,
+                              ranked, st));
+        const bool overlap = c->overlap;
+        hipStream_t s2 = overlap ? c->stream2 : st;
+        if (overlap) {
+            TAIL_TRY(hipEventRecord(c->ev_shaded, st));
+            TAIL_TRY(hipStreamWaitEvent(s2, c->ev_shaded, 0));
+        }
+        RayQueue qsh = wb.qshadow;
+        int csh = kCntShadow;
+        if (ranked) {
+            TAIL_TRY(launch_compact_holes(wb.qshadow, bound, overlap ? wb.blk2 : wb.blk, wb.qshadow2, kCntShadowB,
+                                          c->counts, s2, c->counts + cin));
+            qsh = wb.qshadow2;
+            csh = kCntShadowB;
+        }
+        if (L.method == RD_METHOD_MIS)
+            TAIL_TRY(launch_trace_secondary(L.sc, L.scenes, qsh, csh, wb.qmis, wb.ps, c->counts, c->trace_schedule[1],
+                                            c->trace_schedule[2], s2));
+        else if (L.method == RD_METHOD_NEE)
+            TAIL_TRY(launch_trace_shadow(L.sc, L.scenes, qsh, wb.ps, c->counts, csh, c->trace_schedule[1], s2));
+        TAIL_TRY(launch_resolve(L.sc, L.method, qin, wb.ps, c->counts, cin, nullptr, s2));
+        if (overlap) {
+            TAIL_TRY(launch_zero_slots(c->counts, kCntShadow, kCntMis, s2));
+            TAIL_TRY(hipEventRecord(c->ev_resolved, s2));
+            *pending = true;
+        }
+    }
+    TAIL_TRY(launch_compact_queue(qin, bound, L.npass, wb.blk, qout, cout, wb.ps, c->counts, st, c->counts + cin));
+#undef TAIL_TRY
+    return hipSuccess;
+}
+
+// The rest of a pass once every path has started and the queue holds at most
+// c->tail_rays rays, in groups of kTailBounces bounces (an even number, so
+// the queue roles qin -> qout -> qin repeat from group to group) queued
+// without a host synchronization; the host reads the queue size once per
+// group.  The first group runs as plain launches; the next ones replay it as
+// a hipGraph (device_loop 2: captured once and kept on the context for
+// passes that would record the same launches) or run as plain launches
+// again (1).  Bounces past the end of the pass run on empty queues and do
+// nothing.
+constexpr int kTailBounces = 8;
+constexpr int kMaxTailGroups = 1 << 16;
+
+extern "C++" {
+template <class T>
+static void key_add(std::string& k, const T& v) {
+    static_assert(std::is_trivially_copyable<T>::value, "key fields are plain values");
+    k.append(reinterpret_cast<const char*>(&v), sizeof(T));
+}
+}
+
+int run_tail(rd_ctx* c, Timer& tm, const PassLaunch& L, WaveBuffers& wb, RayQueue qin, RayQueue qout, int cin,
+             int cout, int bound, int first_bounce, int* max_bounce, volatile int32_t* cancel) {
+    hipStream_t st = c->stream;
+    HIP_TRY(launch_zero_slots(c->counts, kCntTail, kCntTail, st));
+    auto group = [&]() -> hipError_t {
+        bool pending = false;
+        for (int i = 0; i < kTailBounces; i += 2) {
+            hipError_t e = tail_bounce(c, L, wb, qin, qout, cin, cout, bound, &pending);
+            if (e == hipSuccess) e = tail_bounce(c, L, wb, qout, qin, cout, cin, bound, &pending);
+            if (e != hipSuccess) return e;
+        }
+        return pending ? hipStreamWaitEvent(st, c->ev_resolved, 0) : hipSuccess;
+    };
+    auto read_counts = [&]() -> hipError_t {
+        hipError_t e = hipMemcpyAsync(c->host_counts + cin, c->counts + cin, sizeof(int), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(c->host_counts + kCntTail, c->counts + kCntTail, sizeof(int), hipMemcpyDeviceToHost, st);
+        return e == hipSuccess ? hipStreamSynchronize(st) : e;
+    };
+    int rc = RD_OK;
+    hipError_t e = tm.run("tail", group);
+    if (e == hipSuccess) e = read_counts();
+    if (e != hipSuccess) rc = fail(RD_ERR_DEVICE, std::string("device-driven tail: ") + hipGetErrorString(e));
+    if (!rc && c->host_counts[cin] > 0 && c->device_loop == 2 && L.graph_ok) {
+        // every value the group's launches are recorded with
+        std::string key;
+        key_add(key, L.sc); key_add(key, L.scenes); key_add(key, L.method); key_add(key, L.vol);
+        key_add(key, L.npass); key_add(key, bound); key_add(key, wb.ps); key_add(key, wb.hits);
+        key_add(key, wb.qshadow); key_add(key, wb.qshadow2); key_add(key, wb.qmis); key_add(key, wb.blk);
+        key_add(key, wb.blk2); key_add(key, qin); key_add(key, qout); key_add(key, cin); key_add(key, cout);
+        key_add(key, c->trace_schedule); key_add(key, c->overlap); key_add(key, c->counts);
+        if (!c->tail_exec || key != c->tail_key) {
+            if (c->tail_exec) (void)hipGraphExecDestroy(c->tail_exec);
+            c->tail_exec = nullptr;
+            c->tail_key.clear();
+            hipGraph_t graph = nullptr;
+            e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                hipError_t e1 = group();
+                hipError_t e2 = hipStreamEndCapture(st, &graph);
+                e = e1 != hipSuccess ? e1 : e2;
+            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&c->tail_exec, graph, nullptr, nullptr, 0);
+            if (graph) (void)hipGraphDestroy(graph);
+            if (e == hipSuccess) c->tail_key = key;
+            else {
+                c->tail_exec = nullptr;
+                rc = fail(RD_ERR_DEVICE, std::string("tail graph capture: ") + hipGetErrorString(e));
+            }
+        }
+    }
+    for (int g = 1; !rc && c->host_counts[cin] > 0; g++) {
+        if (cancel && *cancel) { rc = fail(RD_ERR_CANCELLED, "render cancelled"); break; }
+        if (g >= kMaxTailGroups) { rc = fail(RD_ERR_DEVICE, "pass did not end"); break; }
+        const bool graph = c->device_loop == 2 && L.graph_ok;
+        e = tm.run("tail", [&] { return graph ? hipGraphLaunch(c->tail_exec, st) : group(); });
+        if (e == hipSuccess) e = read_counts();
+        if (e != hipSuccess) rc = fail(RD_ERR_DEVICE, std::string("device-driven tail: ") + hipGetErrorString(e));
+    }
+    *max_bounce = std::max(*max_bounce, first_bounce + c->host_counts[kCntTail] - 1);
+    return rc;
+}
 
 // The bounce loop of one pass: every bounce traces and shades the queue,
 // retires finished paths, then compacts the pool into the next queue (slot
@@ -1193,7 +1349,16 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
             }));
             RayQueue sa = wb.qshadow, sb = wb.qshadow2, ma = wb.qmis, mb = wb.qmis2;
             int csa = kCntShadow, csb = kCntShadowB, cma = kCntMis, cmb = kCntMisB;
-            for (int step = 0;; step++) {
+            if (!c->chain_lockstep) {
+                // every chain walked to its end by one lane: no host loop
+                HIP_TRY(tm.run("trace_chain", [&] {
+                    return launch_chain_full(L.sc, L.scenes, sa, wb.ps, false, c->counts, csa, st);
+                }));
+                HIP_TRY(tm.run("trace_chain", [&] {
+                    return launch_chain_full(L.sc, L.scenes, ma, wb.ps, true, c->counts, cma, st);
+                }));
+            }
+            for (int step = 0; c->chain_lockstep; step++) {
                 if (step >= kMaxChainSteps) return fail(RD_ERR_DEVICE, "medium chain did not terminate");
                 HIP_TRY(tm.run("trace_chain", [&] {
                     return launch_trace_chain(L.sc, L.scenes, sa, wb.ps, wb.hits, false, c->counts, csa, csb,
@@ -1295,6 +1460,14 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
         if (cancel && *cancel) rc = fail(RD_ERR_CANCELLED, "render cancelled");
         std::swap(qin, qout);
         std::swap(cin, cout);
+        if (!rc && c->device_loop > 0 && !L.adj && all_started && c->host_counts[cin] <= c->tail_rays) {
+            if (resolve_pending) {
+                HIP_TRY(hipStreamWaitEvent(st, c->ev_resolved, 0));
+                resolve_pending = false;
+            }
+            rc = run_tail(c, tm, L, wb, qin, qout, cin, cout, c->host_counts[cin], bounce + 1, max_bounce, cancel);
+            break;
+        }
     }
     if (resolve_pending) HIP_TRY(hipStreamWaitEvent(st, c->ev_resolved, 0));
     return rc;
@@ -1516,7 +1689,8 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
             std::memcpy(wb.host_segs, segs.data(), sizeof(BatchSeg) * segs.size());
             HIP_TRY(hipMemcpyAsync(wb.segs, wb.host_segs, sizeof(BatchSeg) * segs.size(), hipMemcpyHostToDevice, st));
             PassLaunch L{rep, d_scenes, nullptr, kinds[g].second, kinds[g].first == RD_INTEGRATOR_VOL_PATH, wb.segs,
-                         (int)segs.size(), (int)used, (int)std::min(pool, used), &wb, !noalloc};
+                         (int)segs.size(), (int)used, (int)std::min(pool, used), &wb, !noalloc,
+                         !(single && (reqs[0].params.flags & RD_RENDER_NO_GRAPH))};
             rc = run_pass(c, tm, L, &max_bounce, cancel);
             if (rc) break;
             HIP_TRY(tm.run("accumulate", [&] {
@@ -1542,6 +1716,7 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
     HIP_TRY(hipEventRecord(ev1, st));
     HIP_TRY(hipMemcpyAsync(c->host_counts, c->counts, kCntSlots * sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    if (!rc && c->host_counts[kCntChainStuck]) rc = fail(RD_ERR_DEVICE, "medium chain did not terminate");
     float ms = 0;
     (void)hipEventElapsedTime(&ms, ev0, ev1);
     tm.collect();
@@ -1839,6 +2014,9 @@ int rd_get_tuning(const rd_ctx* c, rd_tuning* t) {
     t->ray_sort_min = c->ray_sort_min;
     t->ray_sort_shadow = c->ray_sort_shadow ? 1 : 0;
     t->overlap = c->overlap ? 1 : 0;
+    t->device_loop = c->device_loop;
+    t->tail_rays = c->tail_rays;
+    t->chain_lockstep = c->chain_lockstep ? 1 : 0;
     return RD_OK;
 }
 
@@ -1850,6 +2028,8 @@ int rd_set_tuning(rd_ctx* c, const rd_tuning* t) {
     if (t->ray_sort_bits != -1 && (t->ray_sort_bits < 3 || t->ray_sort_bits > 30))
         return fail(RD_ERR_INVALID, "ray_sort_bits: 3..30 or -1");
     if (t->ray_sort_min < -1) return fail(RD_ERR_INVALID, "ray_sort_min: >= 0 or -1");
+    if (t->device_loop < -1 || t->device_loop > 2) return fail(RD_ERR_INVALID, "device_loop: 0, 1, 2 or -1");
+    if (t->tail_rays < -1) return fail(RD_ERR_INVALID, "tail_rays: >= 0 or -1");
     CtxLock lock(c);
     if (!lock.ok()) return fail(RD_ERR_BUSY, "context in use by another thread");
     for (int k = 0; k < 3; k++)
@@ -1859,6 +2039,9 @@ int rd_set_tuning(rd_ctx* c, const rd_tuning* t) {
     if (t->ray_sort_min >= 0) c->ray_sort_min = t->ray_sort_min;
     if (t->ray_sort_shadow >= 0) c->ray_sort_shadow = t->ray_sort_shadow != 0;
     if (t->overlap >= 0) c->overlap = t->overlap != 0;
+    if (t->device_loop >= 0) c->device_loop = t->device_loop;
+    if (t->tail_rays >= 0) c->tail_rays = t->tail_rays;
+    if (t->chain_lockstep >= 0) c->chain_lockstep = t->chain_lockstep != 0;
     return RD_OK;
 }
 

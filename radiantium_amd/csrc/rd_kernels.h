@@ -72,8 +72,14 @@ enum : int {
     kCntShadowB = 34, kCntMisB = 35,   // vol_path chain queues, second buffers
     kCntFree = 36,        // pass path index of the first path the last compaction started
     kCntNext = 37,        // next pass path to start
-    kCntSlots = 38
+    kCntTail = 38,        // bounces the device-driven tail processed
+    kCntChainStuck = 39,  // set when a medium chain reached kMaxChainSteps
+    kCntSlots = 40
 };
+
+// A Transmittance / IntersectTr chain takes one step per medium boundary it
+// crosses; this bounds it for pathological geometry (the render then fails).
+constexpr int kMaxChainSteps = 1 << 16;
 
 // One segment of a batched wave: samples [sample_base, sample_base + nsamples)
 // of every pixel of request `req`'s crop, stored at paths [path_offset, ...)
@@ -121,8 +127,10 @@ hipError_t launch_trace_rays(const DevScene& sc, const float4* rays, int n, floa
 hipError_t launch_init_pool(PathState ps, int pool, int* counts, hipStream_t s);
 hipError_t launch_compact(const DevScene& sc, const DevScene* scenes, const BatchSeg* segs, int nseg, int npass,
                           int pool, int2* blk, RayQueue q, int cnt_q, PathState ps, int* counts, hipStream_t s);
+// n_dev (optional): the queue's count on the device, at most n, which then
+// only sizes the grids (the device-driven tail does not read counts back).
 hipError_t launch_compact_queue(RayQueue qin, int n, int npass, int2* blk, RayQueue q, int cnt_q, PathState ps,
-                                int* counts, hipStream_t s);
+                                int* counts, hipStream_t s, const int* n_dev = nullptr);
 inline size_t compact_scratch_int2(size_t pool) {
     const size_t nblk = (pool + 1023) / 1024;
     return 2 * nblk + 2 * ((nblk + 1023) / 1024);
@@ -162,7 +170,10 @@ hipError_t launch_trace_secondary(const DevScene& sc, const DevScene* scenes, Ra
                                   hipStream_t s);
 // The n entries of a ranked queue that hold a ray, in order, into q (count at
 // cnt_q); blk as for launch_compact.
-hipError_t launch_compact_holes(RayQueue qin, int n, int2* blk, RayQueue q, int cnt_q, int* counts, hipStream_t s);
+hipError_t launch_compact_holes(RayQueue qin, int n, int2* blk, RayQueue q, int cnt_q, int* counts, hipStream_t s,
+                                const int* n_dev = nullptr);
+// Counts one more tail bounce (kCntTail) when the queue at cnt_in is not empty.
+hipError_t launch_tail_tick(int* counts, int cnt_in, hipStream_t s);
 // adj: null for rendering; the adjoint replay's gradient inputs otherwise.
 hipError_t launch_resolve(const DevScene& sc, int method, RayQueue qin, PathState ps, int* counts, int cnt_in,
                           const AdjArgs* adj, hipStream_t s);
@@ -180,6 +191,12 @@ hipError_t launch_chain_advance(const DevScene& sc, const DevScene* scenes, RayQ
                                 RayQueue qout, bool mis, PathState ps, int* counts, int cnt_in, int cnt_out,
                                 hipStream_t s);
 hipError_t launch_resolve_vol(RayQueue qin, PathState ps, int* counts, int cnt_in, hipStream_t s);
+// Every chain of the queue (count at cnt_slot) walked to its end by one lane:
+// its segments' closest-hit queries and chain steps, in the order the
+// lockstep stages above take them; the work cursor must be zero on entry
+// (shade_vol clears it).
+hipError_t launch_chain_full(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, bool mis,
+                             int* counts, int cnt_slot, hipStream_t s);
 // The per-pixel mean into out_rgb (float32), or into out64 (float64) when it is not null.
 hipError_t launch_finalize(const WaveParams& wp, int height, int spp, const double4* sum, const float4* sum2,
                            float* out_rgb, float* out_sum2, double* out64, hipStream_t s);

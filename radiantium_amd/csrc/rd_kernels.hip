@@ -16,6 +16,7 @@
 #include <mutex>
 #include <unordered_map>
 
+#include "rd_chain.h"
 #include "rd_kernels.h"
 #include "rd_shading.h"
 #include "rd_wave.h"
@@ -70,7 +71,9 @@ enum : int {
     kModeRaysClosest = 0, kModeRaysAny = 1, kModeExt = 2, kModeShadow = 3, kModeMis = 4,
     kModeChainShadow = 5,   // vol_path Transmittance segment: closest hit into hits[]
     kModeChainMis = 6,      // vol_path IntersectTr segment: closest hit into hits[]
-    kModeSecondary = 7      // MIS rays (q, closest) then shadow rays (q2, any hit) in one launch
+    kModeSecondary = 7,     // MIS rays (q, closest) then shadow rays (q2, any hit) in one launch
+    kModeChainFullShadow = 8,   // vol_path Transmittance chains, each walked to its end by one lane
+    kModeChainFullMis = 9       // vol_path IntersectTr chains, likewise
 };
 constexpr int kRefill = 16;
 
@@ -397,13 +400,26 @@ __device__ __forceinline__ int trav_count(const TraceArgs& a, bool fixed) {
     return fixed ? a.n_fixed : a.counts[a.cnt_slot];
 }
 
-// The next 64 consecutive ranks for the calling wavefront, from the work
-// cursor (wave-uniform): wavefronts that finish early take more rays, so the
-// launch ends evenly whatever the grid and the rays' costs.
+// Chunks of consecutive ranks for the cursor schedules (wave-uniform), taken
+// from the work cursor, so wavefronts that finish early take more rays and
+// the launch ends evenly whatever the grid and the rays' costs.  Atomics on
+// one address serialise (tens of ns each at the device scope), so a resident
+// grid over a small or empty queue would pay one per wavefront: only the
+// first ceil(n / size) wavefronts of the grid take part (together they take
+// every chunk), the others exit without touching the cursor.  (Measured on
+// the extension traversal of large queues: a static first chunk per
+// wavefront made it 14 % slower, a plain load of the cursor before each
+// atomic twice as slow.)
 __device__ __forceinline__ int next_chunk(int* cursor, int size = 64) {
     int base = 0;
     if (__lane_id() == 0) base = atomicAdd(cursor, size);
     return __shfl(base, 0);
+}
+
+// Whether the calling wavefront takes part in a cursor schedule over n rays.
+__device__ __forceinline__ bool chunk_wave(int n, int size = 64) {
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    return (int64_t)wave * size < n;
 }
 
 // Waves per SIMD that the LDS stack allows (160 KB of LDS per CU, four SIMDs;
@@ -457,6 +473,7 @@ __global__ __launch_bounds__(kBlock, lds_blocks(kStack, kBlock)) void k_trace_st
         nrays++;
     };
     if (a.dyn) {
+        if (!chunk_wave(n)) return;
         for (int c = next_chunk(cursor); c < n; c = next_chunk(cursor))
             if (c + __lane_id() < n) one(c + __lane_id());
     } else {
@@ -496,6 +513,7 @@ __global__ __launch_bounds__(kBlock, inst_waves(kStack, kBlock, 5)) void k_trace
         nrays++;
     };
     if (a.dyn) {
+        if (!chunk_wave(n)) return;
         for (int c = next_chunk(cursor); c < n; c = next_chunk(cursor))
             if (c + __lane_id() < n) one(c + __lane_id());
     } else {
@@ -521,6 +539,7 @@ __global__ __launch_bounds__(kBlock, lds_blocks(kStack, kBlock)) void k_trace_se
     if (blockIdx.x == 0 && threadIdx.x < 3 && a.zero[threadIdx.x] >= 0) a.counts[a.zero[threadIdx.x]] = 0;
     int* cursor = a.counts + a.cursor_slot;
     uint32_t nv = 0, nt = 0, nrays = 0;
+    if (!chunk_wave(n, 64 * kSeq)) return;
     for (int c = next_chunk(cursor, 64 * kSeq); c < n; c = next_chunk(cursor, 64 * kSeq)) {
         int j = 0, r = 0, req = 0;
         const float4* nodes = sc.nodes;
@@ -616,6 +635,7 @@ void k_trace_secondary(DevScene sc, TraceArgs a) {
         // 128-ray chunks traced as two rounds of 64: the second round is the
         // neighbouring rays of the shadow order (at the first bounce the
         // same tile's next sample), whose nodes the first round left in L1
+        if (!chunk_wave(n, 128)) return;
         for (int c = next_chunk(cursor, 128); c < n; c = next_chunk(cursor, 128))
             for (int k = 0; k < 2; k++)
                 if (c + 64 * k + __lane_id() < n) one(c + 64 * k + __lane_id());
@@ -626,6 +646,63 @@ void k_trace_secondary(DevScene sc, TraceArgs a) {
     add_counter((unsigned long long*)(a.counts + kCntRaysClosest), nr_m);
     add_class_counters(a.counts, 2, nr_s, nv_s, nt_s);
     add_counter((unsigned long long*)(a.counts + kCntRaysAny), nr_s);
+}
+
+// vol_path chains walked to the end per lane: a lane takes a chain ray of
+// the queue (64-ray chunks from the work cursor), traces its segment, applies
+// chain_step and, while the chain crosses medium boundaries, traces the next
+// segment itself.  Each segment is the same closest-hit query and the same
+// chain step as the lockstep kernels' (k_trace_* in a chain mode +
+// k_chain_advance), so results and query counts are theirs; no host loop
+// over the steps.
+template <bool kMis, int kStack, int kBlock, bool kBatch, bool kInst>
+__global__ __launch_bounds__(kBlock, kInst ? inst_waves(kStack, kBlock, 4) : lds_blocks(kStack, kBlock))
+void k_chain_full(DevScene sc, TraceArgs a) {
+    constexpr int kChainMode = kMis ? kModeChainMis : kModeChainShadow;
+    __shared__ int lds[kStack * kBlock];
+    int* stk = lds + threadIdx.x;
+    const int n = a.counts[a.cnt_slot];
+    int* cursor = a.counts + a.cursor_slot;
+    uint32_t nv = 0, nt = 0, nrays = 0;
+    auto one = [&](int r) {
+        const int pid = a.q.pid[r];
+        const DevScene& s2 = kBatch ? a.scenes[a.ps.req[pid]] : sc;
+        float4 ra = a.q.org[r], rb = a.q.dir[r];
+        for (int step = 0;; step++) {
+            if (step >= kMaxChainSteps) {   // the lockstep loop's bound: the render fails
+                atomicOr(a.counts + kCntChainStuck, 1);
+                break;
+            }
+            float4 h;
+            int hsub = -1;
+            if constexpr (kInst) {
+                TravI ti;
+                travi_begin(ti, ra, rb);
+                if (s2.num_nodes > 0)
+                    while (!trav_step_inst<false, kBlock>(s2, s2.nodes, s2.prims, ti, stk, nv, nt)) {}
+                const Trav& t = ti.t;
+                h = t.best.prim >= 0 ? make_float4(t.best.t, __int_as_float(t.best.prim), t.best.u, t.best.v)
+                                     : make_float4(0.f, __int_as_float(-1), 0.f, 0.f);
+                hsub = t.best.prim >= 0 ? ti.sub : -1;
+            } else {
+                Trav t;
+                trav_begin(t, ra, rb);
+                while (!trav_step<false, kBlock>(s2.nodes, s2.prims, t, stk, nv, nt)) {}
+                h = t.best.prim >= 0 ? make_float4(t.best.t, __int_as_float(t.best.prim), t.best.u, t.best.v)
+                                     : make_float4(0.f, __int_as_float(-1), 0.f, 0.f);
+            }
+            nrays++;
+            f3 no, nd;
+            float ntmin, ntmax;
+            if (!chain_step(s2, kMis, pid, ra, rb, h, hsub, a.ps, &no, &nd, &ntmin, &ntmax)) break;
+            ra = make_float4(no.x, no.y, no.z, ntmin);
+            rb = make_float4(nd.x, nd.y, nd.z, ntmax);
+        }
+    };
+    if (!chunk_wave(n)) return;
+    for (int c = next_chunk(cursor); c < n; c = next_chunk(cursor))
+        if (c + __lane_id() < n) one(c + __lane_id());
+    trav_counters<kChainMode>(a, nrays, nv, nt);
 }
 
 // Schedule 1 ("refill"): lanes that finish are refilled from an atomic ray
@@ -715,7 +792,17 @@ static void launch_resident(void (*kernel)(DevScene, TraceArgs), int blocks, int
 
 template <int kMode, int kStack, int kBlock>
 static void launch_trace_variant(const DevScene& sc, const TraceArgs& a, int blocks, int schedule, hipStream_t s) {
-    if constexpr (kMode == kModeSecondary) {
+    if constexpr (kMode == kModeChainFullShadow || kMode == kModeChainFullMis) {
+        constexpr bool kMis = kMode == kModeChainFullMis;
+        if (sc.num_insts > 0) {
+            if (a.scenes) launch_resident(k_chain_full<kMis, kStack, kBlock, true, true>, blocks, kBlock, s, sc, a);
+            else launch_resident(k_chain_full<kMis, kStack, kBlock, false, true>, blocks, kBlock, s, sc, a);
+        } else {
+            if (a.scenes) launch_resident(k_chain_full<kMis, kStack, kBlock, true, false>, blocks, kBlock, s, sc, a);
+            else launch_resident(k_chain_full<kMis, kStack, kBlock, false, false>, blocks, kBlock, s, sc, a);
+        }
+        return;
+    } else if constexpr (kMode == kModeSecondary) {
         if (sc.num_insts > 0) {
             if (a.scenes) launch_resident(k_trace_secondary<kStack, kBlock, true, true>, blocks, kBlock, s, sc, a);
             else launch_resident(k_trace_secondary<kStack, kBlock, false, true>, blocks, kBlock, s, sc, a);
@@ -724,7 +811,7 @@ static void launch_trace_variant(const DevScene& sc, const TraceArgs& a, int blo
             else launch_resident(k_trace_secondary<kStack, kBlock, false, false>, blocks, kBlock, s, sc, a);
         }
         return;
-    }
+    } else {
     // instanced scenes: the two-level kernel, sequences taken as chunks (a
     // lane's rays back to back measured slower there: its lanes then diverge
     // between scene and set steps)
@@ -745,6 +832,7 @@ static void launch_trace_variant(const DevScene& sc, const TraceArgs& a, int blo
     if (schedule == 1) launch_resident(k_trace_refill<kMode, kStack, kBlock>, blocks, kBlock, s, sc, a);
     else if (schedule == 3) launch_resident(k_trace_seq<kMode, kStack, kBlock, false>, blocks, kBlock, s, sc, a);
     else launch_resident(k_trace_straight<kMode, kStack, kBlock, false>, blocks, kBlock, s, sc, a);
+    }
 }
 
 template <int kMode>
@@ -1037,9 +1125,10 @@ __global__ __launch_bounds__(1024) void k_scan_apply(const int2* __restrict__ bl
 // Queue filter: once every path of the pass has started, the next queue is
 // the previous one's entries whose path continues, in the same order (slot
 // order, as the pool compaction would give).  Counts per 1024 entries.
-__global__ __launch_bounds__(256) void k_queue_count(const int* __restrict__ pid, int n,
+__global__ __launch_bounds__(256) void k_queue_count(const int* __restrict__ pid, int n, const int* n_dev,
                                                      const int* __restrict__ alive, int2* blk) {
     __shared__ int tmp[4];
+    if (n_dev) n = min(n, *n_dev);
     const int base = blockIdx.x * kCompactSlots;
     int na = 0;
     for (int i = 0; i < kCompactSlots / 256; i++) {
@@ -1053,8 +1142,9 @@ __global__ __launch_bounds__(256) void k_queue_count(const int* __restrict__ pid
 
 // Hole removal: the entries of a ranked queue that hold a ray (pid >= 0), in
 // the same order.  Counts per 1024 entries.
-__global__ __launch_bounds__(256) void k_hole_count(const int* __restrict__ pid, int n, int2* blk) {
+__global__ __launch_bounds__(256) void k_hole_count(const int* __restrict__ pid, int n, const int* n_dev, int2* blk) {
     __shared__ int tmp[4];
+    if (n_dev) n = min(n, *n_dev);
     const int base = blockIdx.x * kCompactSlots;
     int na = 0;
     for (int i = 0; i < kCompactSlots / 256; i++) {
@@ -1066,8 +1156,10 @@ __global__ __launch_bounds__(256) void k_hole_count(const int* __restrict__ pid,
     if (threadIdx.x == 0) blk[blockIdx.x] = make_int2(ta, 0);
 }
 
-__global__ __launch_bounds__(256) void k_hole_scatter(RayQueue qin, int n, const int2* __restrict__ off, RayQueue q) {
+__global__ __launch_bounds__(256) void k_hole_scatter(RayQueue qin, int n, const int* n_dev,
+                                                      const int2* __restrict__ off, RayQueue q) {
     __shared__ int tmp[4];
+    if (n_dev) n = min(n, *n_dev);
     const int base = blockIdx.x * kCompactSlots;
     int pos_base = off[blockIdx.x].y;
     for (int i = 0; i < kCompactSlots / 256; i++) {
@@ -1084,9 +1176,10 @@ __global__ __launch_bounds__(256) void k_hole_scatter(RayQueue qin, int n, const
     }
 }
 
-__global__ __launch_bounds__(256) void k_queue_scatter(RayQueue qin, int n, PathState ps,
+__global__ __launch_bounds__(256) void k_queue_scatter(RayQueue qin, int n, const int* n_dev, PathState ps,
                                                        const int2* __restrict__ off, RayQueue q) {
     __shared__ int tmp[4];
+    if (n_dev) n = min(n, *n_dev);
     const int base = blockIdx.x * kCompactSlots;
     int pos_base = off[blockIdx.x].y;
     for (int i = 0; i < kCompactSlots / 256; i++) {
@@ -1156,29 +1249,40 @@ static void scan_blocks(const int2* blk, int nblk, int npass, int* counts, int c
 }
 
 hipError_t launch_compact_queue(RayQueue qin, int n, int npass, int2* blk, RayQueue q, int cnt_q, PathState ps,
-                                int* counts, hipStream_t s) {
+                                int* counts, hipStream_t s, const int* n_dev) {
     if (n <= 0) {
         hipLaunchKernelGGL(k_zero_slot, dim3(1), dim3(64), 0, s, counts, cnt_q);
         return hipGetLastError();
     }
     const int nblk = (n + kCompactSlots - 1) / kCompactSlots;
     int2* off = blk + nblk;
-    hipLaunchKernelGGL(k_queue_count, dim3(nblk), dim3(256), 0, s, qin.pid, n, ps.alive, blk);
+    hipLaunchKernelGGL(k_queue_count, dim3(nblk), dim3(256), 0, s, qin.pid, n, n_dev, ps.alive, blk);
     scan_blocks(blk, nblk, npass, counts, cnt_q, off, true, s);
-    hipLaunchKernelGGL(k_queue_scatter, dim3(nblk), dim3(256), 0, s, qin, n, ps, off, q);
+    hipLaunchKernelGGL(k_queue_scatter, dim3(nblk), dim3(256), 0, s, qin, n, n_dev, ps, off, q);
     return hipGetLastError();
 }
 
-hipError_t launch_compact_holes(RayQueue qin, int n, int2* blk, RayQueue q, int cnt_q, int* counts, hipStream_t s) {
+hipError_t launch_compact_holes(RayQueue qin, int n, int2* blk, RayQueue q, int cnt_q, int* counts, hipStream_t s,
+                                const int* n_dev) {
     if (n <= 0) {
         hipLaunchKernelGGL(k_zero_slot, dim3(1), dim3(64), 0, s, counts, cnt_q);
         return hipGetLastError();
     }
     const int nblk = (n + kCompactSlots - 1) / kCompactSlots;
     int2* off = blk + nblk;
-    hipLaunchKernelGGL(k_hole_count, dim3(nblk), dim3(256), 0, s, qin.pid, n, blk);
+    hipLaunchKernelGGL(k_hole_count, dim3(nblk), dim3(256), 0, s, qin.pid, n, n_dev, blk);
     scan_blocks(blk, nblk, 0, counts, cnt_q, off, false, s);
-    hipLaunchKernelGGL(k_hole_scatter, dim3(nblk), dim3(256), 0, s, qin, n, off, q);
+    hipLaunchKernelGGL(k_hole_scatter, dim3(nblk), dim3(256), 0, s, qin, n, n_dev, off, q);
+    return hipGetLastError();
+}
+
+// The device-driven tail's bounce count: one more bounce when its queue is
+// not empty (read on the device, so the launch can sit in a graph).
+__global__ void k_tail_tick(int* counts, int cnt_in) {
+    if (threadIdx.x == 0 && counts[cnt_in] > 0) counts[kCntTail]++;
+}
+hipError_t launch_tail_tick(int* counts, int cnt_in, hipStream_t s) {
+    hipLaunchKernelGGL(k_tail_tick, dim3(1), dim3(64), 0, s, counts, cnt_in);
     return hipGetLastError();
 }
 
@@ -1256,6 +1360,17 @@ hipError_t launch_trace_mis(const DevScene& sc, const DevScene* scenes, RayQueue
     a.q = q; a.ps = ps; a.cnt_slot = kCntMis; a.cursor_slot = kCntWork + 2; a.scenes = scenes;
     return launch_trace_mode<kModeMis>(sc, a, 0, sched, s);
 }
+// vol_path Transmittance (mis = false) or IntersectTr (mis = true) chains of
+// the queue, each walked to its end (k_chain_full).
+hipError_t launch_chain_full(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, bool mis,
+                             int* counts, int cnt_slot, hipStream_t s) {
+    TraceArgs a = trace_args(counts);
+    a.q = q; a.ps = ps; a.cnt_slot = cnt_slot; a.cursor_slot = kCntWork + (mis ? 2 : 1);
+    a.scenes = scenes;
+    return mis ? launch_trace_mode<kModeChainFullMis>(sc, a, 0, 2, s)
+               : launch_trace_mode<kModeChainFullShadow>(sc, a, 0, 2, s);
+}
+
 // One closest-hit step of the vol_path Transmittance (mis = false) or
 // IntersectTr (mis = true) chains; clears the next chain queue's counter.
 hipError_t launch_trace_chain(const DevScene& sc, const DevScene* scenes, RayQueue q, PathState ps, float4* hits,

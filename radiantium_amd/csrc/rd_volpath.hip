@@ -18,6 +18,7 @@
 //                  either finishes the chain or appends its next segment.
 //                  Neither chain draws random numbers (Homogeneous.Tr does not).
 //   resolve_vol    radiance += coeff * (le_light + le_bsdf) / lightPdf.
+#include "rd_chain.h"
 #include "rd_kernels.h"
 #include "rd_shading.h"
 #include "rd_wave.h"
@@ -263,14 +264,8 @@ hipError_t launch_shade_vol(const DevScene& sc, const DevScene* scenes, RayQueue
     return hipGetLastError();
 }
 
-// One step of a chain after its closest-hit query.
-//   Transmittance (Scene.cs:79-101): a surface blocks (Tr = 0); a medium
-//   multiplies Tr over the whole segment [1e-4, dis - 1e-4]; a boundary hit
-//   continues from the hit point toward the light, in the medium behind it.
-//   IntersectTr (Scene.cs:108-130): a miss multiplies Tr to float.MaxValue and
-//   takes the infinite lights; a hit multiplies Tr up to |ray.At(t)| (the
-//   length of the hit's position vector, as written); a surface ends the chain
-//   with the selected light's Le; a boundary continues with MinT 0.001.
+// One step of a chain after its closest-hit query (chain_step), for every
+// chain ray of the queue; continuing chains append their next segment.
 template <bool kBatch>
 __global__ __launch_bounds__(256) void k_chain_advance(DevScene sc0, const DevScene* __restrict__ scenes, RayQueue qin,
                                                        const float4* __restrict__ hits, RayQueue qout, int mis,
@@ -286,68 +281,11 @@ __global__ __launch_bounds__(256) void k_chain_advance(DevScene sc0, const DevSc
         int pid = 0;
         if (k < n) {
             pid = qin.pid[k];
-            auto run = [&](const DevScene& sc) {
-            float4 a = qin.org[k], b = qin.dir[k];
-            f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
-            float tmin = a.w, tmax = b.w;
+            const DevScene& sc = kBatch ? scenes[ps.req[pid]] : sc0;
             float4 h = hits[k];
             int prim = __float_as_int(h.y);
-            bool any = prim >= 0;
-            DevPrim pr;
-            if (any) pr = sc.prim_info[prim];
-            if (!mis) {
-                float4 to4 = ps.sh_to[pid];
-                int env = __float_as_int(to4.w);
-                f3 tr = ld3(ps.sh_tr, pid);
-                if (any && pr.material >= 0) {
-                    tr = mk(0.0f);
-                } else {
-                    if (env >= 0) tr = tr * medium_tr(sc.media[env], tmin, tmax);
-                    if (any) {
-                        Inct it = make_inct_any(sc, pr, pr.kind == kInst ? ps.hsub[k] : -1, o, d, h.z, h.w, h.x);
-                        env = get_medium(pr, it.sh.Z, d);
-                        f3 to = mk(to4.x, to4.y, to4.z);
-                        float dis = length(to - it.P);
-                        if (!(dis < 0.0001f)) {
-                            want = true;
-                            no = it.P; nd = d; ntmin = 0.0001f; ntmax = dis - 0.0001f;
-                        }
-                    }
-                }
-                st3(ps.sh_tr, pid, tr);
-                ps.sh_to[pid] = make_float4(to4.x, to4.y, to4.z, __int_as_float(env));
-            } else {
-                float4 tr4 = ps.mis_tr[pid];
-                int env = __float_as_int(tr4.w);
-                f3 tr = mk(tr4.x, tr4.y, tr4.z);
-                if (!any) {
-                    if (env >= 0) tr = tr * medium_tr(sc.media[env], tmin, tmax);
-                    f3 li = eval_infinite(sc, d);
-                    float4 prev = ps.mis_li[pid];
-                    ps.mis_li[pid] = make_float4(li.x, li.y, li.z, prev.w);
-                } else {
-                    if (env >= 0) tr = tr * medium_tr(sc.media[env], tmin, length(o + h.x * d));
-                    Inct it = make_inct_any(sc, pr, pr.kind == kInst ? ps.hsub2[k] : -1, o, d, h.z, h.w, h.x);
-                    if (pr.material >= 0) {
-                        f3 li = mk(0.0f);
-                        int sel = ps.sel_light[pid];
-                        if (pr.light >= 0 && pr.light == sel) {
-                            const DevLight& Lt = sc.lights[sel];
-                            if (dot(it.sh.Z, -d) > 0) li = mk(Lt.le[0], Lt.le[1], Lt.le[2]);
-                        }
-                        float4 prev = ps.mis_li[pid];
-                        ps.mis_li[pid] = make_float4(li.x, li.y, li.z, prev.w);
-                    } else {
-                        env = get_medium(pr, it.sh.Z, d);
-                        want = true;
-                        no = it.P; nd = d; ntmin = 0.001f; ntmax = kFloatMax;
-                    }
-                }
-                ps.mis_tr[pid] = make_float4(tr.x, tr.y, tr.z, __int_as_float(env));
-            }
-            };
-            if constexpr (kBatch) run(scenes[ps.req[pid]]);
-            else run(sc0);
+            int hsub = (prim >= 0 && sc.prim_info[prim].kind == kInst) ? (mis ? ps.hsub2[k] : ps.hsub[k]) : -1;
+            want = chain_step(sc, mis != 0, pid, qin.org[k], qin.dir[k], h, hsub, ps, &no, &nd, &ntmin, &ntmax);
         }
         int slot = wave_append(&counts[cnt_out], want);
         if (want) {

@@ -247,17 +247,19 @@ class Context:
         t = abi.rd_tuning()
         check(lib().rd_get_tuning(self.h, C.byref(t)))
         return {"trace_schedule": list(t.trace_schedule), "ray_sort": t.ray_sort, "ray_sort_bits": t.ray_sort_bits,
-                "ray_sort_min": t.ray_sort_min, "ray_sort_shadow": t.ray_sort_shadow, "overlap": t.overlap}
+                "ray_sort_min": t.ray_sort_min, "ray_sort_shadow": t.ray_sort_shadow, "overlap": t.overlap,
+                "device_loop": t.device_loop, "tail_rays": t.tail_rays, "chain_lockstep": t.chain_lockstep}
 
     def set_tuning(self, trace_schedule=None, ray_sort=None, ray_sort_bits=None, ray_sort_min=None,
-                   ray_sort_shadow=None, overlap=None):
+                   ray_sort_shadow=None, overlap=None, device_loop=None, tail_rays=None, chain_lockstep=None):
         """rd_set_tuning; arguments left None keep their value."""
         t = abi.rd_tuning()
         sched = trace_schedule if trace_schedule is not None else (-1, -1, -1)
         for i in range(3):
             t.trace_schedule[i] = -1 if sched[i] is None else int(sched[i])
         for f, v in (("ray_sort", ray_sort), ("ray_sort_bits", ray_sort_bits), ("ray_sort_min", ray_sort_min),
-                     ("ray_sort_shadow", ray_sort_shadow), ("overlap", overlap)):
+                     ("ray_sort_shadow", ray_sort_shadow), ("overlap", overlap), ("device_loop", device_loop),
+                     ("tail_rays", tail_rays), ("chain_lockstep", chain_lockstep)):
             setattr(t, f, -1 if v is None else int(v))
         check(lib().rd_set_tuning(self.h, C.byref(t)))
 
