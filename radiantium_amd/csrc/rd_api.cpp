@@ -924,6 +924,8 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     ds.num_insts = d->num_instances;
     ds.num_lights = d->num_lights;
     ds.global_medium = d->global_medium;
+    ds.mat_mask = 0;
+    for (const DevMaterial& m : mats) ds.mat_mask |= mat_bit(m.type);
     int rc = RD_OK;
     if (!rc) rc = upload(s, nodes.data(), nodes.size(), &ds.nodes);
     if (!rc) rc = upload(s, prims.data(), prims.size(), &ds.prims);
@@ -1649,6 +1651,7 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
         for (int i : G) {
             rep.stack_needed = std::max(rep.stack_needed, dscenes[i].stack_needed);
             rep.num_insts = std::max(rep.num_insts, dscenes[i].num_insts);   // any instanced scene: two-level kernels
+            rep.mat_mask |= dscenes[i].mat_mask;   // the shading kernels must hold every scene's materials
         }
         std::vector<int> next_sample(n, 0);
         size_t gi = 0;

@@ -1389,7 +1389,7 @@ hipError_t launch_trace_chain(const DevScene& sc, const DevScene* scenes, RayQue
 // kAdj: the adjoint replay also records, per material vertex, the albedo R,
 // the texture coordinates and the roughness ratio (plastic_rho_ratio) of its
 // light-sample, BSDF-sample and continuation terms for resolve's gradients.
-template <bool kBatch, bool kAdj>
+template <bool kBatch, bool kAdj, unsigned kM>
 __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* __restrict__ scenes, RayQueue qin,
                                                const float4* __restrict__ hits, RayQueue qshadow, RayQueue qmis,
                                                PathState ps, int* counts, int cnt_in, const int* __restrict__ rank,
@@ -1454,7 +1454,7 @@ __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* 
                 go = false;
             } else if (go) {
                 nvert++;
-                MatCtx m = load_material(sc, pr.material, it.u, it.v);
+                MatCtx m = load_material<kM>(sc, pr.material, it.u, it.v);
                 f3 wo_l = to_local(it.sh, wo);
                 // EstimateDirect takes wo = inct.Wr (PathTracer.cs:314), -ray.D
                 // except on an instance, where it is mapped through the transforms
@@ -1475,8 +1475,8 @@ __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* 
                         // EstimateDirect, light-sampling half (PathTracer.cs:317-350)
                         if (ls.pdf > 0.0f && !is_black(ls.li)) {
                             f3 wi_l = to_local(it.sh, ls.wi);
-                            f3 fr = mat_fr(m, wo_d, wi_l);
-                            float spdf = mat_pdf(m, wo_d, wi_l);
+                            f3 fr = mat_fr<kM>(m, wo_d, wi_l);
+                            float spdf = mat_pdf<kM>(m, wo_d, wi_l);
                             if (!is_black(fr)) {
                                 f3 vec = ls.p - it.P;
                                 float dis = length(vec);
@@ -1491,7 +1491,7 @@ __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* 
                             }
                         }
                         // EstimateDirect, BSDF-sampling half (PathTracer.cs:351-397)
-                        BSample bs = mat_sample(m, wo_d, rng);
+                        BSample bs = mat_sample<kM>(m, wo_d, rng);
                         f3 fr2 = bs.fr * abs_cos_theta(bs.wi);
                         float sp2 = bs.pdf;
                         if (!is_black(fr2) && sp2 > 0) {
@@ -1521,7 +1521,7 @@ __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* 
                                 want_shadow = true;
                                 so = it.P; sd = normalize(vec); stmax = dis - 0.0001f;
                                 f3 wi_l = to_local(it.sh, ls.wi);
-                                f3 fr = mat_fr(m, wo_l, wi_l);
+                                f3 fr = mat_fr<kM>(m, wo_l, wi_l);
                                 nee = fr * ls.li * abs_cos_theta(wi_l) / ls.pdf;
                                 if constexpr (kAdj) r_nee = plastic_rho_ratio(m, rho, wo_l, wi_l);
                             }
@@ -1539,7 +1539,7 @@ __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* 
                 if (want_shadow) nflags |= kFlagShadowIssued;
                 if (want_mis) nflags |= kFlagMisIssued;
                 // continuation (PathTracer.cs:227-268)
-                BSample bs = mat_sample(m, wo_l, rng);
+                BSample bs = mat_sample<kM>(m, wo_l, rng);
                 bool sspec = (bs.type & kSpecular) != 0;
                 if (bs.pdf > 0.0f) {
                     if constexpr (kAdj) r_cont = plastic_rho_ratio(m, rho, wo_l, bs.wi);
@@ -1615,15 +1615,26 @@ __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* 
 hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
                         RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in, bool adjoint,
                         const int* rank, bool rank_identity, hipStream_t s) {
-    if (scenes)
-        hipLaunchKernelGGL((k_shade<true, false>), dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
-                           qshadow, qmis, ps, counts, cnt_in, rank, rank_identity);
-    else if (adjoint)
-        hipLaunchKernelGGL((k_shade<false, true>), dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
-                           qshadow, qmis, ps, counts, cnt_in, rank, rank_identity);
-    else
-        hipLaunchKernelGGL((k_shade<false, false>), dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
-                           qshadow, qmis, ps, counts, cnt_in, rank, rank_identity);
+    const dim3 g(persistent_blocks()), b(256);
+    if (adjoint) {
+        hipLaunchKernelGGL((k_shade<false, true, kMatsAll>), g, b, 0, s, sc, scenes, qin, hits, qshadow, qmis, ps,
+                           counts, cnt_in, rank, rank_identity);
+        return hipGetLastError();
+    }
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, g, b, 0, s, sc, scenes, qin, hits, qshadow, qmis, ps, counts, cnt_in, rank,
+                           rank_identity);
+    };
+    const unsigned set = mat_set(sc.mat_mask);
+    if (scenes) {
+        if (set == kMatsDelta) go(k_shade<true, false, kMatsDelta>);
+        else if (set == kMatsMicrofacet) go(k_shade<true, false, kMatsMicrofacet>);
+        else go(k_shade<true, false, kMatsAll>);
+    } else {
+        if (set == kMatsDelta) go(k_shade<false, false, kMatsDelta>);
+        else if (set == kMatsMicrofacet) go(k_shade<false, false, kMatsMicrofacet>);
+        else go(k_shade<false, false, kMatsAll>);
+    }
     return hipGetLastError();
 }
 

@@ -382,6 +382,10 @@ struct DisneyP {
     bool thin, sdist_black;
 };
 
+// Whether material kind k is compiled into a kernel for the set kM; the
+// cases of the other kinds compile to nothing (callers pass only kinds in kM).
+#define RD_MAT(k) ((kM & mat_bit(k)) != 0)
+
 struct MatCtx {
     int type, two_sided;
     Mf mf;
@@ -391,6 +395,8 @@ struct MatCtx {
     DisneyP dp;
 };
 
+// kM: the material kinds compiled in (the others load nothing)
+template <unsigned kM = kMatsAll>
 __device__ __forceinline__ MatCtx load_material(const DevScene& sc, int id, float su, float sv) {
     const DevMaterial& m = sc.mats[id];
     MatCtx c;
@@ -404,49 +410,58 @@ __device__ __forceinline__ MatCtx load_material(const DevScene& sc, int id, floa
     c.mf.kind = 0; c.mf.ax = c.mf.ay = 0.001f;
     switch (m.type) {
     case kMatDiffuse:
-        c.r = tex_sample(sc, m.tex[0], su, sv);
+        if constexpr (RD_MAT(kMatDiffuse)) c.r = tex_sample(sc, m.tex[0], su, sv);
         break;
     case kMatRoughMetal:   // RoughMetal.GetParam (RoughMetal.cs:28-31)
-        c.mf = make_mf(m.dist, tex_sample(sc, m.tex[0], su, sv).x, tex_sample(sc, m.tex[1], su, sv).x);
-        c.eta = mk(m.eta[0], m.eta[1], m.eta[2]);
-        c.k = mk(m.k[0], m.k[1], m.k[2]);
+        if constexpr (RD_MAT(kMatRoughMetal)) {
+            c.mf = make_mf(m.dist, tex_sample(sc, m.tex[0], su, sv).x, tex_sample(sc, m.tex[1], su, sv).x);
+            c.eta = mk(m.eta[0], m.eta[1], m.eta[2]);
+            c.k = mk(m.k[0], m.k[1], m.k[2]);
+        }
         break;
     case kMatRoughGlass:   // RoughGlass.GetParam (RoughGlass.cs:146-149)
-        c.mf = make_mf(m.dist, tex_sample(sc, m.tex[2], su, sv).x, tex_sample(sc, m.tex[3], su, sv).x);
-        c.r = tex_sample(sc, m.tex[0], su, sv);
-        c.t = tex_sample(sc, m.tex[1], su, sv);
+        if constexpr (RD_MAT(kMatRoughGlass)) {
+            c.mf = make_mf(m.dist, tex_sample(sc, m.tex[2], su, sv).x, tex_sample(sc, m.tex[3], su, sv).x);
+            c.r = tex_sample(sc, m.tex[0], su, sv);
+            c.t = tex_sample(sc, m.tex[1], su, sv);
+        }
         break;
     case kMatRoughPlastic: // RoughPlastic.SampleParams / SampleRoughness (RoughPlastic.cs:37-45)
-        c.mf = make_mf(m.dist, tex_sample(sc, m.tex[1], su, sv).x, tex_sample(sc, m.tex[2], su, sv).x);
-        c.r = tex_sample(sc, m.tex[0], su, sv);
+        if constexpr (RD_MAT(kMatRoughPlastic)) {
+            c.mf = make_mf(m.dist, tex_sample(sc, m.tex[1], su, sv).x, tex_sample(sc, m.tex[2], su, sv).x);
+            c.r = tex_sample(sc, m.tex[0], su, sv);
+        }
         break;
     case kMatPerfectGlass:
-        c.r = tex_sample(sc, m.tex[0], su, sv);
-        c.t = tex_sample(sc, m.tex[1], su, sv);
+        if constexpr (RD_MAT(kMatPerfectGlass)) {
+            c.r = tex_sample(sc, m.tex[0], su, sv);
+            c.t = tex_sample(sc, m.tex[1], su, sv);
+        }
         break;
     case kMatPerfectMirror:
-        c.r = tex_sample(sc, m.tex[0], su, sv);
+        if constexpr (RD_MAT(kMatPerfectMirror)) c.r = tex_sample(sc, m.tex[0], su, sv);
         break;
-    case kMatDisney: {
-        DisneyP& b = c.dp;
-        b.color = tex_sample(sc, m.tex[0], su, sv);
-        b.metallic = tex_sample(sc, m.tex[1], su, sv).x;
-        b.roughness = tex_sample(sc, m.tex[2], su, sv).x;
-        b.eta = tex_sample(sc, m.tex[3], su, sv).x;
-        b.spec_scale = tex_sample(sc, m.tex[4], su, sv).x;
-        b.spec_tint = tex_sample(sc, m.tex[5], su, sv).x;
-        b.aniso = tex_sample(sc, m.tex[6], su, sv).x;
-        b.sheen = tex_sample(sc, m.tex[7], su, sv).x;
-        b.sheen_tint = tex_sample(sc, m.tex[8], su, sv).x;
-        b.clearcoat = tex_sample(sc, m.tex[9], su, sv).x;
-        b.cc_gloss = tex_sample(sc, m.tex[10], su, sv).x;
-        b.sdist_black = is_black(tex_sample(sc, m.tex[11], su, sv));
-        b.transmission = tex_sample(sc, m.tex[12], su, sv).x;
-        b.trans_rough = tex_sample(sc, m.tex[13], su, sv).x;
-        b.flatness = tex_sample(sc, m.tex[14], su, sv).x;
-        b.thin = m.thin != 0;
+    case kMatDisney:
+        if constexpr (RD_MAT(kMatDisney)) {
+            DisneyP& b = c.dp;
+            b.color = tex_sample(sc, m.tex[0], su, sv);
+            b.metallic = tex_sample(sc, m.tex[1], su, sv).x;
+            b.roughness = tex_sample(sc, m.tex[2], su, sv).x;
+            b.eta = tex_sample(sc, m.tex[3], su, sv).x;
+            b.spec_scale = tex_sample(sc, m.tex[4], su, sv).x;
+            b.spec_tint = tex_sample(sc, m.tex[5], su, sv).x;
+            b.aniso = tex_sample(sc, m.tex[6], su, sv).x;
+            b.sheen = tex_sample(sc, m.tex[7], su, sv).x;
+            b.sheen_tint = tex_sample(sc, m.tex[8], su, sv).x;
+            b.clearcoat = tex_sample(sc, m.tex[9], su, sv).x;
+            b.cc_gloss = tex_sample(sc, m.tex[10], su, sv).x;
+            b.sdist_black = is_black(tex_sample(sc, m.tex[11], su, sv));
+            b.transmission = tex_sample(sc, m.tex[12], su, sv).x;
+            b.trans_rough = tex_sample(sc, m.tex[13], su, sv).x;
+            b.flatness = tex_sample(sc, m.tex[14], su, sv).x;
+            b.thin = m.thin != 0;
+        }
         break;
-    }
     default:
         break;
     }
@@ -678,87 +693,119 @@ __device__ __forceinline__ float disney_pdf(const DisneyP& b, f3 wo, f3 wi) {  /
 
 // Material.Fr (Diffuse.cs:21-27, RoughMetal.cs:33-58, RoughGlass.cs:151-195,
 // RoughPlastic.cs:65-74, FresnelSpecularBsdf.cs:30, SpecularReflectionBrdf.cs:17, Disney.cs:87-90)
+template <unsigned kM = kMatsAll>
 __device__ __forceinline__ f3 mat_fr(const MatCtx& c, f3 wo, f3 wi) {
     switch (c.type) {
     case kMatDiffuse:
-        if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!c.two_sided) return mk(0.0f); }
-        return same_hemisphere(wo, wi) ? c.r / kPi : mk(0.0f);
-    case kMatRoughMetal:
-        if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!c.two_sided) return mk(0.0f); }
-        return conductor_fr(c, wo, wi);
-    case kMatRoughGlass: {
-        bool is_reflect = same_hemisphere(wo, wi);
-        bool entering = cos_theta(wo) > 0;
-        float ei = entering ? c.eta_a : c.eta_b;
-        float et = entering ? c.eta_b : c.eta_a;
-        float eta = is_reflect ? 1 : et / ei;
-        f3 wh = normalize(wo + wi * eta);
-        if (cos_theta(wh) < 0) wh = -wh;
-        float sd = dot(wo, wh) + eta * dot(wi, wh);
-        float D = mf_D(c.mf, wh);
-        float G = mf_G(c.mf, wo, wi);
-        float F = fr_dielectric(dot(wo, wh), ei, et);
-        f3 bsdf;
-        if (is_reflect) {
-            bsdf = c.r * (D * G * F / (4.0f * cos_theta(wi) * cos_theta(wo)));
-        } else {
-            if (dot(wo, wh) * dot(wi, wh) > 0) return mk(0.0f);
-            float factor = 1 / eta;
-            bsdf = (1 - F) * c.t *
-                   fabsf(D * G * eta * eta * abs_dot(wi, wh) * abs_dot(wo, wh) * factor * factor /
-                         (cos_theta(wo) * cos_theta(wi) * sd * sd));
+        if constexpr (RD_MAT(kMatDiffuse)) {
+            if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!c.two_sided) return mk(0.0f); }
+            return same_hemisphere(wo, wi) ? c.r / kPi : mk(0.0f);
         }
-        return valid_color(bsdf) ? bsdf : mk(0.0f);
+        return mk(0.0f);
+    case kMatRoughMetal:
+        if constexpr (RD_MAT(kMatRoughMetal)) {
+            if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!c.two_sided) return mk(0.0f); }
+            return conductor_fr(c, wo, wi);
+        }
+        return mk(0.0f);
+    case kMatRoughGlass: {
+        if constexpr (RD_MAT(kMatRoughGlass)) {
+            bool is_reflect = same_hemisphere(wo, wi);
+            bool entering = cos_theta(wo) > 0;
+            float ei = entering ? c.eta_a : c.eta_b;
+            float et = entering ? c.eta_b : c.eta_a;
+            float eta = is_reflect ? 1 : et / ei;
+            f3 wh = normalize(wo + wi * eta);
+            if (cos_theta(wh) < 0) wh = -wh;
+            float sd = dot(wo, wh) + eta * dot(wi, wh);
+            float D = mf_D(c.mf, wh);
+            float G = mf_G(c.mf, wo, wi);
+            float F = fr_dielectric(dot(wo, wh), ei, et);
+            f3 bsdf;
+            if (is_reflect) {
+                bsdf = c.r * (D * G * F / (4.0f * cos_theta(wi) * cos_theta(wo)));
+            } else {
+                if (dot(wo, wh) * dot(wi, wh) > 0) return mk(0.0f);
+                float factor = 1 / eta;
+                bsdf = (1 - F) * c.t *
+                       fabsf(D * G * eta * eta * abs_dot(wi, wh) * abs_dot(wo, wh) * factor * factor /
+                             (cos_theta(wo) * cos_theta(wi) * sd * sd));
+            }
+            return valid_color(bsdf) ? bsdf : mk(0.0f);
+        }
+        return mk(0.0f);
     }
     case kMatRoughPlastic:
-        return plastic_fr(c, wo, wi);
+        if constexpr (RD_MAT(kMatRoughPlastic)) {
+            return plastic_fr(c, wo, wi);
+        }
+        return mk(0.0f);
     case kMatDisney:
-        return disney_fr(c.dp, wo, wi);
+        if constexpr (RD_MAT(kMatDisney)) {
+            return disney_fr(c.dp, wo, wi);
+        }
+        return mk(0.0f);
     default:   // perfect glass / mirror: delta lobes
         return mk(0.0f);
     }
 }
 
 // Material.Pdf
+template <unsigned kM = kMatsAll>
 __device__ __forceinline__ float mat_pdf(const MatCtx& c, f3 wo, f3 wi) {
     switch (c.type) {
     case kMatDiffuse:
-        if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!c.two_sided) return 0.0f; }
-        return same_hemisphere(wo, wi) ? abs_cos_theta(wi) / kPi : 0.0f;
+        if constexpr (RD_MAT(kMatDiffuse)) {
+            if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!c.two_sided) return 0.0f; }
+            return same_hemisphere(wo, wi) ? abs_cos_theta(wi) / kPi : 0.0f;
+        }
+        return 0.0f;
     case kMatRoughMetal: {
-        if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!c.two_sided) return 0.0f; }
-        return mf_reflect_pdf(c.mf, wo, wi);
+        if constexpr (RD_MAT(kMatRoughMetal)) {
+            if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!c.two_sided) return 0.0f; }
+            return mf_reflect_pdf(c.mf, wo, wi);
+        }
+        return 0.0f;
     }
     case kMatRoughGlass: {
-        bool is_reflect = same_hemisphere(wo, wi);
-        bool entering = cos_theta(wo) > 0;
-        float ei = entering ? c.eta_a : c.eta_b;
-        float et = entering ? c.eta_b : c.eta_a;
-        float pdf;
-        if (is_reflect) {
-            f3 wh = normalize(wo + wi);
-            if (cos_theta(wh) < 0) wh = -wh;
-            if (dot(wo, wh) * cos_theta(wo) <= 0 || dot(wi, wh) * cos_theta(wi) <= 0) return 0.0f;
-            float jac = 1.0f / (4.0f * abs_dot(wh, wi));
-            float F = fr_dielectric(dot(wo, wh), ei, et);
-            pdf = mf_pdf(c.mf, wo, wh) * jac * F;
-        } else {
-            float eta = et / ei;
-            f3 wh = normalize(wo + wi * eta);
-            if (cos_theta(wh) < 0) wh = -wh;
-            if (dot(wo, wh) * dot(wi, wh) > 0) return 0.0f;
-            if (dot(wo, wh) * cos_theta(wo) <= 0 || dot(wi, wh) * cos_theta(wi) <= 0) return 0.0f;
-            float sd = dot(wo, wh) + eta * dot(wi, wh);
-            float dwh = fabsf((eta * eta * dot(wi, wh)) / (sd * sd));
-            float F = fr_dielectric(dot(wo, wh), ei, et);
-            pdf = mf_pdf(c.mf, wo, wh) * dwh * (1 - F);
+        if constexpr (RD_MAT(kMatRoughGlass)) {
+            bool is_reflect = same_hemisphere(wo, wi);
+            bool entering = cos_theta(wo) > 0;
+            float ei = entering ? c.eta_a : c.eta_b;
+            float et = entering ? c.eta_b : c.eta_a;
+            float pdf;
+            if (is_reflect) {
+                f3 wh = normalize(wo + wi);
+                if (cos_theta(wh) < 0) wh = -wh;
+                if (dot(wo, wh) * cos_theta(wo) <= 0 || dot(wi, wh) * cos_theta(wi) <= 0) return 0.0f;
+                float jac = 1.0f / (4.0f * abs_dot(wh, wi));
+                float F = fr_dielectric(dot(wo, wh), ei, et);
+                pdf = mf_pdf(c.mf, wo, wh) * jac * F;
+            } else {
+                float eta = et / ei;
+                f3 wh = normalize(wo + wi * eta);
+                if (cos_theta(wh) < 0) wh = -wh;
+                if (dot(wo, wh) * dot(wi, wh) > 0) return 0.0f;
+                if (dot(wo, wh) * cos_theta(wo) <= 0 || dot(wi, wh) * cos_theta(wi) <= 0) return 0.0f;
+                float sd = dot(wo, wh) + eta * dot(wi, wh);
+                float dwh = fabsf((eta * eta * dot(wi, wh)) / (sd * sd));
+                float F = fr_dielectric(dot(wo, wh), ei, et);
+                pdf = mf_pdf(c.mf, wo, wh) * dwh * (1 - F);
+            }
+            return isinf(pdf) || isnan(pdf) ? 0.0f : pdf;
         }
-        return isinf(pdf) || isnan(pdf) ? 0.0f : pdf;
+        return 0.0f;
     }
     case kMatRoughPlastic:
-        return plastic_pdf(c, wo, wi);
+        if constexpr (RD_MAT(kMatRoughPlastic)) {
+            return plastic_pdf(c, wo, wi);
+        }
+        return 0.0f;
     case kMatDisney:
-        return disney_pdf(c.dp, wo, wi);
+        if constexpr (RD_MAT(kMatDisney)) {
+            return disney_pdf(c.dp, wo, wi);
+        }
+        return 0.0f;
     default:
         return 0.0f;
     }
@@ -875,114 +922,136 @@ __device__ __forceinline__ BSample disney_sample(const DisneyP& b, f3 wo, Rng& r
 
 // Material.Sample (Diffuse.cs:39-49, RoughMetal.cs:87-112, RoughGlass.cs:240-297,
 // RoughPlastic.cs:112-166, FresnelSpecularBsdf.cs:34-65, PerfectMirror.cs:29-39, Disney.cs:97-100)
+template <unsigned kM = kMatsAll>
 __device__ __forceinline__ BSample mat_sample(const MatCtx& c, f3 wo, Rng& rng) {
     BSample s;
     s.wi = mk(0.0f); s.fr = mk(0.0f); s.pdf = 0.0f; s.type = 0;
     switch (c.type) {
     case kMatDiffuse: {
-        if (cos_theta(wo) <= 0) { if (!c.two_sided) return s; }
-        f3 wi;
-        if (!cosine_dir(wo, rng, &wi)) return s;
-        s.wi = wi;
-        s.pdf = same_hemisphere(wo, wi) ? abs_cos_theta(wi) / kPi : 0.0f;
-        s.fr = c.r / kPi;
-        s.type = kReflection | kDiffuse;
+        if constexpr (RD_MAT(kMatDiffuse)) {
+            if (cos_theta(wo) <= 0) { if (!c.two_sided) return s; }
+            f3 wi;
+            if (!cosine_dir(wo, rng, &wi)) return s;
+            s.wi = wi;
+            s.pdf = same_hemisphere(wo, wi) ? abs_cos_theta(wi) / kPi : 0.0f;
+            s.fr = c.r / kPi;
+            s.type = kReflection | kDiffuse;
+            return s;
+        }
         return s;
     }
     case kMatRoughMetal: {
-        if (cos_theta(wo) <= 0) { if (!c.two_sided) return s; }
-        f3 wi, wh;
-        if (!mf_reflect_dir(c.mf, wo, rng, &wi, &wh)) return s;
-        s.pdf = mf_pdf(c.mf, wo, wh) / (4 * dot(wi, wh));
-        s.fr = conductor_fr(c, wo, wi);
-        s.wi = wi;
-        s.type = kReflection | kGlossy;
+        if constexpr (RD_MAT(kMatRoughMetal)) {
+            if (cos_theta(wo) <= 0) { if (!c.two_sided) return s; }
+            f3 wi, wh;
+            if (!mf_reflect_dir(c.mf, wo, rng, &wi, &wh)) return s;
+            s.pdf = mf_pdf(c.mf, wo, wh) / (4 * dot(wi, wh));
+            s.fr = conductor_fr(c, wo, wi);
+            s.wi = wi;
+            s.type = kReflection | kGlossy;
+            return s;
+        }
         return s;
     }
     case kMatRoughGlass: {
-        f3 wh = mf_sample_wh(c.mf, wo, rng);
-        float F = fr_dielectric(dot(wo, wh), c.eta_a, c.eta_b);
-        if (rng.next_float() < F) {
-            f3 wi = reflect(-wo, wh);
-            if (!same_hemisphere(wo, wi)) return s;
+        if constexpr (RD_MAT(kMatRoughGlass)) {
+            f3 wh = mf_sample_wh(c.mf, wo, rng);
+            float F = fr_dielectric(dot(wo, wh), c.eta_a, c.eta_b);
+            if (rng.next_float() < F) {
+                f3 wi = reflect(-wo, wh);
+                if (!same_hemisphere(wo, wi)) return s;
+                float D = mf_D(c.mf, wh);
+                float G = mf_G(c.mf, wo, wi);
+                f3 brdf = c.r * (D * G * F / (4.0f * cos_theta(wi) * cos_theta(wo)));
+                float jac = 1.0f / (4.0f * abs_dot(wh, wi));
+                float pdf = mf_pdf(c.mf, wo, wh) * jac * F;
+                if (isinf(pdf) || isnan(pdf) || !valid_color(brdf)) return s;
+                s.wi = wi; s.fr = brdf; s.pdf = pdf; s.type = kReflection | kGlossy;
+                return s;
+            }
+            bool entering = cos_theta(wo) > 0;
+            float ei = entering ? c.eta_a : c.eta_b;
+            float et = entering ? c.eta_b : c.eta_a;
+            float eta = et / ei;
+            f3 wi;
+            if (refract(wo, wh, ei / et, &wi)) {
+                if (dot(wo, wh) * dot(wi, wh) > 0) return s;
+            } else {
+                wi = reflect(-wo, wh);
+                eta = 1;
+            }
+            float sd = dot(wo, wh) + eta * dot(wi, wh);
+            float factor = 1 / eta;
             float D = mf_D(c.mf, wh);
             float G = mf_G(c.mf, wo, wi);
-            f3 brdf = c.r * (D * G * F / (4.0f * cos_theta(wi) * cos_theta(wo)));
-            float jac = 1.0f / (4.0f * abs_dot(wh, wi));
-            float pdf = mf_pdf(c.mf, wo, wh) * jac * F;
-            if (isinf(pdf) || isnan(pdf) || !valid_color(brdf)) return s;
-            s.wi = wi; s.fr = brdf; s.pdf = pdf; s.type = kReflection | kGlossy;
+            f3 btdf = (1 - F) * c.t *
+                      fabsf(D * G * eta * eta * abs_dot(wi, wh) * abs_dot(wo, wh) * factor * factor /
+                            (cos_theta(wo) * cos_theta(wi) * sd * sd));
+            float dwh = fabsf((eta * eta * dot(wi, wh)) / (sd * sd));
+            float pdf = mf_pdf(c.mf, wo, wh) * dwh * (1 - F);
+            if (isinf(pdf) || isnan(pdf) || !valid_color(btdf)) return s;
+            s.wi = wi; s.fr = btdf; s.pdf = pdf; s.type = kTransmission | kGlossy;
             return s;
         }
-        bool entering = cos_theta(wo) > 0;
-        float ei = entering ? c.eta_a : c.eta_b;
-        float et = entering ? c.eta_b : c.eta_a;
-        float eta = et / ei;
-        f3 wi;
-        if (refract(wo, wh, ei / et, &wi)) {
-            if (dot(wo, wh) * dot(wi, wh) > 0) return s;
-        } else {
-            wi = reflect(-wo, wh);
-            eta = 1;
-        }
-        float sd = dot(wo, wh) + eta * dot(wi, wh);
-        float factor = 1 / eta;
-        float D = mf_D(c.mf, wh);
-        float G = mf_G(c.mf, wo, wi);
-        f3 btdf = (1 - F) * c.t *
-                  fabsf(D * G * eta * eta * abs_dot(wi, wh) * abs_dot(wo, wh) * factor * factor /
-                        (cos_theta(wo) * cos_theta(wi) * sd * sd));
-        float dwh = fabsf((eta * eta * dot(wi, wh)) / (sd * sd));
-        float pdf = mf_pdf(c.mf, wo, wh) * dwh * (1 - F);
-        if (isinf(pdf) || isnan(pdf) || !valid_color(btdf)) return s;
-        s.wi = wi; s.fr = btdf; s.pdf = pdf; s.type = kTransmission | kGlossy;
         return s;
     }
     case kMatRoughPlastic: {
-        if (cos_theta(wo) <= 0) { if (!c.two_sided) return s; }
-        float p = c.ks / (c.kd + c.ks);
-        f3 gwi = mk(0.0f), gwh, dwi = mk(0.0f);
-        int gty = 0, dty = 0;
-        if (mf_reflect_dir(c.mf, wo, rng, &gwi, &gwh)) gty = kReflection | kGlossy;
-        else gwi = mk(0.0f);
-        if (cosine_dir(wo, rng, &dwi)) dty = kReflection | kDiffuse;
-        else dwi = mk(0.0f);
-        bool glossy = rng.next_float() < p;
-        f3 wi = glossy ? gwi : dwi;
-        s.wi = wi;
-        s.type = glossy ? gty : dty;
-        s.pdf = plastic_pdf(c, wo, wi);
-        s.fr = plastic_fr(c, wo, wi);
+        if constexpr (RD_MAT(kMatRoughPlastic)) {
+            if (cos_theta(wo) <= 0) { if (!c.two_sided) return s; }
+            float p = c.ks / (c.kd + c.ks);
+            f3 gwi = mk(0.0f), gwh, dwi = mk(0.0f);
+            int gty = 0, dty = 0;
+            if (mf_reflect_dir(c.mf, wo, rng, &gwi, &gwh)) gty = kReflection | kGlossy;
+            else gwi = mk(0.0f);
+            if (cosine_dir(wo, rng, &dwi)) dty = kReflection | kDiffuse;
+            else dwi = mk(0.0f);
+            bool glossy = rng.next_float() < p;
+            f3 wi = glossy ? gwi : dwi;
+            s.wi = wi;
+            s.type = glossy ? gty : dty;
+            s.pdf = plastic_pdf(c, wo, wi);
+            s.fr = plastic_fr(c, wo, wi);
+            return s;
+        }
         return s;
     }
     case kMatPerfectGlass: {
-        float f = fr_dielectric(cos_theta(wo), c.eta_a, c.eta_b);
-        if (rng.next_float() < f) {
-            if (cos_theta(wo) == 0) return s;
-            f3 wi = mk(-wo.x, -wo.y, wo.z);
-            s.wi = wi; s.fr = f * c.r / abs_cos_theta(wi); s.pdf = f; s.type = kReflection | kSpecular;
+        if constexpr (RD_MAT(kMatPerfectGlass)) {
+            float f = fr_dielectric(cos_theta(wo), c.eta_a, c.eta_b);
+            if (rng.next_float() < f) {
+                if (cos_theta(wo) == 0) return s;
+                f3 wi = mk(-wo.x, -wo.y, wo.z);
+                s.wi = wi; s.fr = f * c.r / abs_cos_theta(wi); s.pdf = f; s.type = kReflection | kSpecular;
+                return s;
+            }
+            bool entering = cos_theta(wo) > 0;
+            float ei = entering ? c.eta_a : c.eta_b, et = entering ? c.eta_b : c.eta_a;
+            f3 n = entering ? mk(0, 0, 1) : mk(0, 0, -1);
+            f3 wi;
+            if (!refract(wo, n, ei / et, &wi) || cos_theta(wi) == 0.0f) return s;
+            f3 ft = c.t * (1.0f - f);
+            ft = ft * ((ei * ei) / (et * et));
+            ft = ft / abs_cos_theta(wi);
+            s.wi = wi; s.fr = ft; s.pdf = 1 - f; s.type = kTransmission | kSpecular;
             return s;
         }
-        bool entering = cos_theta(wo) > 0;
-        float ei = entering ? c.eta_a : c.eta_b, et = entering ? c.eta_b : c.eta_a;
-        f3 n = entering ? mk(0, 0, 1) : mk(0, 0, -1);
-        f3 wi;
-        if (!refract(wo, n, ei / et, &wi) || cos_theta(wi) == 0.0f) return s;
-        f3 ft = c.t * (1.0f - f);
-        ft = ft * ((ei * ei) / (et * et));
-        ft = ft / abs_cos_theta(wi);
-        s.wi = wi; s.fr = ft; s.pdf = 1 - f; s.type = kTransmission | kSpecular;
         return s;
     }
     case kMatPerfectMirror: {
-        if (cos_theta(wo) <= 0) { if (!c.two_sided) return s; }
-        if (abs_cos_theta(wo) == 0) return s;
-        f3 wi = mk(-wo.x, -wo.y, wo.z);
-        s.wi = wi; s.fr = c.r / abs_cos_theta(wi); s.pdf = 1.0f; s.type = kReflection | kSpecular;
+        if constexpr (RD_MAT(kMatPerfectMirror)) {
+            if (cos_theta(wo) <= 0) { if (!c.two_sided) return s; }
+            if (abs_cos_theta(wo) == 0) return s;
+            f3 wi = mk(-wo.x, -wo.y, wo.z);
+            s.wi = wi; s.fr = c.r / abs_cos_theta(wi); s.pdf = 1.0f; s.type = kReflection | kSpecular;
+            return s;
+        }
         return s;
     }
     case kMatDisney:
-        return disney_sample(c.dp, wo, rng);
+        if constexpr (RD_MAT(kMatDisney)) {
+            return disney_sample(c.dp, wo, rng);
+        }
+        return s;
     default:
         return s;
     }

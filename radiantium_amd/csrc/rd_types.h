@@ -26,6 +26,18 @@ enum : int {
     kMatDiffuse = 0, kMatRoughMetal = 1, kMatRoughGlass = 2, kMatRoughPlastic = 3,
     kMatPerfectGlass = 4, kMatPerfectMirror = 5, kMatDisney = 6
 };
+// Sets of material kinds a shading kernel is compiled for (bit k: kind k).  A
+// scene's set is known on the host, and the kernels compiled for the smallest
+// set that holds it keep only those kinds' code: the same arithmetic for
+// every material they meet, fewer registers.
+constexpr unsigned mat_bit(int k) { return 1u << k; }
+constexpr unsigned kMatsAll = 0x7fu;
+constexpr unsigned kMatsDelta = mat_bit(kMatDiffuse) | mat_bit(kMatPerfectGlass) | mat_bit(kMatPerfectMirror);
+constexpr unsigned kMatsMicrofacet = kMatsDelta | mat_bit(kMatRoughMetal) | mat_bit(kMatRoughGlass);
+// the smallest compiled set holding `mask`
+constexpr unsigned mat_set(unsigned mask) {
+    return (mask & ~kMatsDelta) == 0 ? kMatsDelta : ((mask & ~kMatsMicrofacet) == 0 ? kMatsMicrofacet : kMatsAll);
+}
 // BxdfType flags (IBxdf.cs:7-15)
 enum : int { kReflection = 1, kTransmission = 2, kDiffuse = 4, kGlossy = 8, kSpecular = 16 };
 
@@ -135,6 +147,7 @@ struct DevScene {
     int global_medium;
     DevIntegrator integ;   // the scene's integrator (a batch may mix parameters)
     DevCamera cam;
+    unsigned mat_mask;     // material kinds the scene holds (bit k: kind k); read on the host
 };
 
 

@@ -25,7 +25,7 @@
 
 namespace rd {
 
-template <bool kBatch>
+template <bool kBatch, unsigned kM>
 __global__ __launch_bounds__(256, 3) void k_shade_vol(DevScene sc0, const DevScene* __restrict__ scenes, RayQueue qin,
                                                    const float4* __restrict__ hits, RayQueue qshadow, RayQueue qmis,
                                                    PathState ps, int* counts, int cnt_in) {
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(256, 3) void k_shade_vol(DevScene sc0, const DevSce
                     env = get_medium(pr, it.sh.Z, rd);
                 } else {
                     nvert++;
-                    MatCtx m = load_material(sc, pr.material, it.u, it.v);
+                    MatCtx m = load_material<kM>(sc, pr.material, it.u, it.v);
                     f3 wo_l = to_local(it.sh, wo);
                     f3 wo_d = to_local(it.sh, it.wr);   // EstimateDirectSurface: wo = inct.Wr (:156)
                     if (sc.num_lights > 0) {
@@ -167,8 +167,8 @@ __global__ __launch_bounds__(256, 3) void k_shade_vol(DevScene sc0, const DevSce
                         LSample ls = light_sample(sc, Lt, it.P, rng);
                         if (ls.pdf > 0.0f && !is_black(ls.li)) {
                             f3 wi_l = to_local(it.sh, ls.wi);
-                            f3 fr = mat_fr(m, wo_d, wi_l);
-                            float spdf = mat_pdf(m, wo_d, wi_l);
+                            f3 fr = mat_fr<kM>(m, wo_d, wi_l);
+                            float spdf = mat_pdf<kM>(m, wo_d, wi_l);
                             if (!is_black(fr)) {
                                 start_shadow(it.P, ls.p, env);
                                 nflags |= kFlagShadowIssued;
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(256, 3) void k_shade_vol(DevScene sc0, const DevSce
                                 st3(ps.nee_li, pid, ls.li, ls.pdf);
                             }
                         }
-                        BSample bs = mat_sample(m, wo_d, rng);
+                        BSample bs = mat_sample<kM>(m, wo_d, rng);
                         f3 fr2 = bs.fr * abs_cos_theta(bs.wi);
                         float sp2 = bs.pdf;
                         if (!is_black(fr2) && sp2 > 0) {
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(256, 3) void k_shade_vol(DevScene sc0, const DevSce
                         nflags |= kFlagNeePending;
                         st3(ps.nee_beta, pid, beta, selpdf);
                     }
-                    BSample bs = mat_sample(m, wo_l, rng);
+                    BSample bs = mat_sample<kM>(m, wo_l, rng);
                     spec = (bs.type & kSpecular) != 0;
                     if (bs.pdf > 0.0f) {
                         beta = beta * (bs.fr * abs_cos_theta(bs.wi) / bs.pdf);
@@ -255,12 +255,20 @@ __global__ __launch_bounds__(256, 3) void k_shade_vol(DevScene sc0, const DevSce
 
 hipError_t launch_shade_vol(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
                             RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in, hipStream_t s) {
-    if (scenes)
-        hipLaunchKernelGGL(k_shade_vol<true>, dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
-                           qshadow, qmis, ps, counts, cnt_in);
-    else
-        hipLaunchKernelGGL(k_shade_vol<false>, dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits,
-                           qshadow, qmis, ps, counts, cnt_in);
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(persistent_blocks()), dim3(256), 0, s, sc, scenes, qin, hits, qshadow, qmis,
+                           ps, counts, cnt_in);
+    };
+    const unsigned set = mat_set(sc.mat_mask);
+    if (scenes) {
+        if (set == kMatsDelta) go(k_shade_vol<true, kMatsDelta>);
+        else if (set == kMatsMicrofacet) go(k_shade_vol<true, kMatsMicrofacet>);
+        else go(k_shade_vol<true, kMatsAll>);
+    } else {
+        if (set == kMatsDelta) go(k_shade_vol<false, kMatsDelta>);
+        else if (set == kMatsMicrofacet) go(k_shade_vol<false, kMatsMicrofacet>);
+        else go(k_shade_vol<false, kMatsAll>);
+    }
     return hipGetLastError();
 }
 
