@@ -41,7 +41,8 @@ typedef enum rd_status {
     RD_ERR_CANCELLED = 4,   /* *cancel became non-zero during rd_render     */
     RD_ERR_STACK = 5,       /* traversal stack overflow (Bvh.StackSize=64)  */
     RD_ERR_UNSUPPORTED = 6, /* feature not built for this path              */
-    RD_ERR_BUSY = 7         /* context used from two threads at once        */
+    RD_ERR_BUSY = 7,        /* context used from two threads at once        */
+    RD_ERR_IO = 8           /* file could not be read or written            */
 } rd_status;
 
 /* ---- scene description (mirrors RendererBuilder.Build's products) ------ */
@@ -288,15 +289,49 @@ int rd_scene_export_bvh(const rd_scene* scene, void* nodes32, int32_t max_nodes,
 int rd_bvh_build_host(const float* bmin, const float* bmax, int32_t n, int32_t max_prim, void* nodes32,
                       int32_t max_nodes, int32_t* prim_order, int32_t* num_nodes, int32_t* max_depth);
 
+/* ---- image files --------------------------------------------------------- */
+/* A ColorBuffer (Radiantium.Core/ColorBuffer.cs:8-30): pixel (x, y) at
+ * data[(x * height + y) * channels]; y = 0 is the image's bottom row. */
+typedef struct rd_image {
+    int32_t width, height, channels;
+    float* data;   /* owned by the library: rd_image_free */
+} rd_image;
+/* TextureUtility.LoadImageFromPath(path, needChannel, isFlipY, isCastToLinear)
+ * (Radiantium.Core/TextureUtility.cs:118-203): PNG, OpenEXR (scanline;
+ * NONE / RLE / ZIPS / ZIP / PIZ), Radiance RGBE (.hdr) and PFM, values as
+ * Magick.NET-Q16-HDRI delivers them (OpenEXR through its half RGBA
+ * interface: four channels, alpha 1 when absent).  need_channel <= 0 keeps
+ * the file's channel count.  Errors: RD_ERR_IO, RD_ERR_INVALID (corrupt),
+ * RD_ERR_UNSUPPORTED (tiled / deep / multi-part OpenEXR, PXR24 / B44 / DWA
+ * compression, interlaced PNG); rd_image_last_error has the message. */
+int rd_image_load(const char* path, int need_channel, int flip_y, int cast_to_linear, rd_image* out);
+void rd_image_free(rd_image* img);
+const char* rd_image_last_error(void);
+/* ColorBuffer.SaveOpenExr (TextureUtility.cs:85-116): half RGBA, rows in
+ * ColorBuffer order (its `k = i`: row 0 first, so the file reads upside down
+ * next to SavePng's), each channel half(QuantumScale * (color * Quantum.Max))
+ * as Magick writes it, alpha 1.  compression: 0 NONE (Magick's default) or
+ * 3 ZIP.  data in ColorBuffer layout, channels >= 1 (ColorBuffer.GetRGB). */
+int rd_image_save_exr(const char* path, const float* data, int width, int height, int channels, int compression);
+/* ColorBuffer.SavePng (TextureUtility.cs:54-83): 8-bit RGBA (Png32), rows
+ * flipped, ScaleQuantumToChar(clamp(color * Quantum.Max)).  to_srgb applies
+ * Color3F.ToSrgb to every pixel first: what ResultOutput.Save's IsPngToSrgb
+ * branch (Radiantium.Offline/ResultOutput.cs:26-38) means to do; its loop, as
+ * written, converts only part of column 0 and leaves the rest black. */
+int rd_image_save_png(const char* path, const float* data, int width, int height, int channels, int to_srgb);
+/* No counterpart in the reference: PFM (float RGB, little-endian, rows from
+ * the bottom, so ColorBuffer row 0 first), for lossless float output. */
+int rd_image_save_pfm(const char* path, const float* data, int width, int height, int channels);
+
 /* ---- scene configs ------------------------------------------------------- */
 /* The reference's scene config (JSON, as Radiantium.Cli/Program.cs:25-140
- * reads it) with its OBJ models and PNG images, built into a scene
+ * reads it) with its OBJ models and images, built into a scene
  * description the way RendererBuilder.Build builds the scene
  * (Config/RendererBuilder.cs:867-1010).  search_dir: base of relative asset
  * locations (NULL = the config's directory; RendererBuilder.GetPath).
  * params (may be NULL) receives the renderer block: spp (default 256).
- * Integrators other than "path" / "vol_path" and image formats other than
- * PNG give RD_ERR_UNSUPPORTED. */
+ * Images load as rd_image_load loads them.  Integrators other than "path" /
+ * "vol_path" give RD_ERR_UNSUPPORTED. */
 typedef struct rd_json_scene rd_json_scene;
 int rd_json_load(const char* path, const char* search_dir, rd_json_scene** out, rd_render_params* params);
 const rd_scene_desc* rd_json_scene_desc(const rd_json_scene* scene);

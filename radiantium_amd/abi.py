@@ -14,10 +14,11 @@ RD_ERR_CANCELLED = 4
 RD_ERR_STACK = 5
 RD_ERR_UNSUPPORTED = 6
 RD_ERR_BUSY = 7
+RD_ERR_IO = 8
 
 STATUS_NAMES = {
     0: "RD_OK", 1: "RD_ERR_INVALID", 2: "RD_ERR_DEVICE", 3: "RD_ERR_OOM",
-    4: "RD_ERR_CANCELLED", 5: "RD_ERR_STACK", 6: "RD_ERR_UNSUPPORTED", 7: "RD_ERR_BUSY",
+    4: "RD_ERR_CANCELLED", 5: "RD_ERR_STACK", 6: "RD_ERR_UNSUPPORTED", 7: "RD_ERR_BUSY", 8: "RD_ERR_IO",
 }
 
 # material kinds (registry names of Config/RendererBuilder.cs:86-213)
@@ -148,6 +149,10 @@ class rd_request(C.Structure):
         ("scene", C.c_void_p), ("params", rd_render_params),
         ("out_rgb", C.c_void_p), ("out_sum2", C.c_void_p),
     ]
+
+
+class rd_image(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("channels", C.c_int32), ("data", C.POINTER(C.c_float))]
 
 
 class rd_render_stats(C.Structure):

@@ -17,11 +17,10 @@
 //     triangles only, v / v/vt / v//vn / v/vt/vn, 1-based, vertices deduplicated
 //     by (v, vn, vt) in face order, optional normal generation);
 //   - images as TextureUtility.LoadImageFromPath delivers them
-//     (Radiantium.Core/TextureUtility.cs:118-200): PNG decoded here (8/16-bit and
-//     palette / low-depth gray, not interlaced), samples / max, rows flipped
-//     unless is_flip_y is false, sRGB -> linear unless is_cast_to_linear is
-//     false (default: false for .exr/.hdr, true otherwise), sampled through
-//     ColorBuffer.GetRGB.  Other image formats are reported as unsupported.
+//     (Radiantium.Core/TextureUtility.cs:118-200): PNG, OpenEXR, Radiance and
+//     PFM decoded by rd_image.cpp, rows flipped unless is_flip_y is false,
+//     sRGB -> linear unless is_cast_to_linear is false (default: false for
+//     .exr/.hdr, true otherwise), sampled through ColorBuffer.GetRGB.
 #include <zlib.h>
 
 #include <cmath>
@@ -40,6 +39,7 @@
 
 #include "../../include/radiantium_hip.h"
 #include "rd_camera.h"
+#include "rd_image.h"
 #include "rd_metals.h"
 
 namespace rd {
@@ -603,114 +603,11 @@ struct ObjFile {
     }
 };
 
-// ---------------------------------------------------------------- PNG
+// ---------------------------------------------------------------- images
 struct Image {
     int w = 0, h = 0, channels = 0;
     std::vector<float> data;   // ColorBuffer: (x * h + y) * channels
 };
-
-uint32_t be32(const unsigned char* p) { return ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | p[3]; }
-
-// Decoded samples in row-major order, `ch` channels, values / (2^depth - 1).
-void decode_png(const std::string& bytes, const std::string& name, int* W, int* H, int* C, std::vector<float>* out) {
-    const unsigned char* d = (const unsigned char*)bytes.data();
-    size_t n = bytes.size();
-    static const unsigned char sig[8] = {137, 80, 78, 71, 13, 10, 26, 10};
-    if (n < 8 || memcmp(d, sig, 8) != 0) unsupported("image " + name + ": only PNG images are decoded natively");
-    size_t i = 8;
-    int w = 0, h = 0, depth = 0, ctype = 0, interlace = 0;
-    std::vector<unsigned char> idat, plte;
-    bool seen_hdr = false;
-    while (i + 8 <= n) {
-        uint32_t len = be32(d + i);
-        std::string type((const char*)d + i + 4, 4);
-        if (i + 12 + (size_t)len > n) bad("image " + name + ": truncated PNG");
-        const unsigned char* c = d + i + 8;
-        if (type == "IHDR") {
-            if (len < 13) bad("image " + name + ": bad IHDR");
-            w = (int)be32(c); h = (int)be32(c + 4); depth = c[8]; ctype = c[9]; interlace = c[12];
-            seen_hdr = true;
-        } else if (type == "PLTE") {
-            plte.assign(c, c + len);
-        } else if (type == "IDAT") {
-            idat.insert(idat.end(), c, c + len);
-        } else if (type == "IEND") {
-            break;
-        }
-        i += 12 + (size_t)len;
-    }
-    if (!seen_hdr || w <= 0 || h <= 0 || (size_t)w * h > (1u << 28)) bad("image " + name + ": bad PNG header");
-    if (interlace != 0) unsupported("image " + name + ": interlaced PNG");
-    int ch;
-    switch (ctype) {
-        case 0: ch = 1; break;
-        case 2: ch = 3; break;
-        case 3: ch = 1; break;   // palette index, expanded to RGB below
-        case 4: ch = 2; break;
-        case 6: ch = 4; break;
-        default: bad("image " + name + ": bad PNG colour type");
-    }
-    bool low = depth < 8;
-    if (!(depth == 8 || depth == 16 || (low && (ctype == 0 || ctype == 3) && (depth == 1 || depth == 2 || depth == 4))))
-        unsupported("image " + name + ": PNG bit depth " + std::to_string(depth));
-    size_t bpp_bits = (size_t)ch * depth;
-    size_t stride = ((size_t)w * bpp_bits + 7) / 8;
-    size_t bpp = std::max<size_t>(1, bpp_bits / 8);
-    std::vector<unsigned char> raw((stride + 1) * (size_t)h);
-    uLongf raw_len = raw.size();
-    if (uncompress(raw.data(), &raw_len, idat.data(), idat.size()) != Z_OK || raw_len != raw.size())
-        bad("image " + name + ": corrupt PNG data");
-    std::vector<unsigned char> img(stride * (size_t)h);
-    for (int y = 0; y < h; y++) {
-        const unsigned char* src = raw.data() + (size_t)y * (stride + 1);
-        unsigned char* cur = img.data() + (size_t)y * stride;
-        const unsigned char* prev = y > 0 ? cur - stride : nullptr;
-        int f = src[0];
-        src++;
-        for (size_t x = 0; x < stride; x++) {
-            int a = x >= bpp ? cur[x - bpp] : 0;
-            int b = prev ? prev[x] : 0;
-            int c = (prev && x >= bpp) ? prev[x - bpp] : 0;
-            int v = src[x];
-            switch (f) {
-                case 0: break;
-                case 1: v += a; break;
-                case 2: v += b; break;
-                case 3: v += (a + b) / 2; break;
-                case 4: {
-                    int p = a + b - c, pa = std::abs(p - a), pb = std::abs(p - b), pc = std::abs(p - c);
-                    v += (pa <= pb && pa <= pc) ? a : (pb <= pc ? b : c);
-                    break;
-                }
-                default: bad("image " + name + ": bad PNG filter");
-            }
-            cur[x] = (unsigned char)v;
-        }
-    }
-    int oc = ctype == 3 ? 3 : ch;
-    out->assign((size_t)w * h * oc, 0.0f);
-    const float maxv = (float)((1u << depth) - 1);
-    for (int y = 0; y < h; y++) {
-        const unsigned char* row = img.data() + (size_t)y * stride;
-        for (int x = 0; x < w; x++) {
-            auto sample = [&](int k) -> unsigned {
-                if (depth == 16) return ((unsigned)row[2 * ((size_t)x * ch + k)] << 8) | row[2 * ((size_t)x * ch + k) + 1];
-                if (depth == 8) return row[(size_t)x * ch + k];
-                size_t bit = (size_t)x * depth;
-                return (row[bit / 8] >> (8 - depth - (bit % 8))) & ((1u << depth) - 1);
-            };
-            float* o = out->data() + ((size_t)y * w + x) * oc;
-            if (ctype == 3) {
-                unsigned idx = sample(0);
-                if (3 * idx + 2 >= plte.size()) bad("image " + name + ": palette index out of range");
-                for (int k = 0; k < 3; k++) o[k] = (float)plte[3 * idx + k] / 255.0f;
-            } else {
-                for (int k = 0; k < ch; k++) o[k] = (float)sample(k) / maxv;
-            }
-        }
-    }
-    *W = w; *H = h; *C = oc;
-}
 
 bool ext_is_hdr(const std::string& loc) {   // TextureUtility.IsHdr: the extension contains "exr" or "hdr"
     size_t dot = loc.find_last_of('.');
@@ -721,9 +618,14 @@ bool ext_is_hdr(const std::string& loc) {   // TextureUtility.IsHdr: the extensi
 }
 
 Image load_image(const std::string& path, const std::string& name, int need_channel, bool flip_y, bool to_linear) {
-    int w, h, ch;
-    std::vector<float> px;
-    decode_png(read_file(path), name, &w, &h, &ch, &px);
+    rd::RawImage raw;
+    try {
+        raw = rd::decode_image(read_file(path), name);
+    } catch (const rd::ImageError& e) {
+        throw LoadError(e.code, e.what());
+    }
+    const int w = raw.w, h = raw.h, ch = raw.channels;
+    const std::vector<float>& px = raw.px;
     Image im;
     im.w = w; im.h = h;
     im.channels = need_channel > 0 ? need_channel : ch;

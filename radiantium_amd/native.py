@@ -23,7 +23,8 @@ EXPORTS = [
     "rd_adjoint", "rd_adjoint_device", "rd_scene_set_texture", "rd_scene_set_texture_device",
     "rd_trace_closest_sub", "rd_json_load", "rd_json_scene_desc", "rd_json_scene_free", "rd_scene_load_json",
     "rd_metal_eta_k", "rd_get_tuning", "rd_set_tuning", "rd_autotune", "rd_render_device_noalloc",
-    "rd_render_mean_f64",
+    "rd_render_mean_f64", "rd_image_load", "rd_image_free", "rd_image_last_error", "rd_image_save_exr",
+    "rd_image_save_png", "rd_image_save_pfm",
 ]
 
 
@@ -95,6 +96,12 @@ def lib():
         L.rd_get_tuning.argtypes = [vp, C.POINTER(abi.rd_tuning)]
         L.rd_set_tuning.argtypes = [vp, C.POINTER(abi.rd_tuning)]
         L.rd_autotune.argtypes = [vp, vp, C.POINTER(abi.rd_render_params), C.POINTER(abi.rd_tuning)]
+        L.rd_image_load.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(abi.rd_image)]
+        L.rd_image_free.argtypes = [C.POINTER(abi.rd_image)]
+        L.rd_image_last_error.restype = C.c_char_p
+        L.rd_image_save_exr.argtypes = [C.c_char_p, vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.rd_image_save_png.argtypes = [C.c_char_p, vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.rd_image_save_pfm.argtypes = [C.c_char_p, vp, C.c_int, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -107,6 +114,56 @@ def check(rc):
 
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def _image_check(rc):
+    if rc != 0:
+        raise RadiantiumError(rc, lib().rd_image_last_error().decode())
+
+
+def image_load(path, need_channel=-1, flip_y=True, cast_to_linear=None):
+    """TextureUtility.LoadImageFromPath (rd_image_load): a ColorBuffer as
+    float32 [width, height, channels], [x, y] with y = 0 the bottom row when
+    flip_y.  cast_to_linear None: the reference's default (false for .exr /
+    .hdr locations, true otherwise)."""
+    if cast_to_linear is None:
+        ext = os.path.splitext(str(path))[1]
+        cast_to_linear = not ("exr" in ext or "hdr" in ext)
+    im = abi.rd_image()
+    _image_check(lib().rd_image_load(os.fsencode(path), int(need_channel), int(bool(flip_y)), int(bool(cast_to_linear)),
+                                     C.byref(im)))
+    try:
+        n = im.width * im.height * im.channels
+        out = np.ctypeslib.as_array(im.data, shape=(n,)).copy()
+    finally:
+        lib().rd_image_free(C.byref(im))
+    return out.reshape(im.width, im.height, im.channels)
+
+
+def _colorbuffer(img):
+    a = np.ascontiguousarray(img, np.float32)
+    if a.ndim == 2:
+        a = a[:, :, None]
+    return a
+
+
+def image_save_exr(path, img, compression=0):
+    """ColorBuffer.SaveOpenExr (rd_image_save_exr); img float32 [width, height, channels]."""
+    a = _colorbuffer(img)
+    _image_check(lib().rd_image_save_exr(os.fsencode(path), _ptr(a), a.shape[0], a.shape[1], a.shape[2], compression))
+
+
+def image_save_png(path, img, to_srgb=False):
+    """ColorBuffer.SavePng (rd_image_save_png); to_srgb: Color3F.ToSrgb first."""
+    a = _colorbuffer(img)
+    _image_check(lib().rd_image_save_png(os.fsencode(path), _ptr(a), a.shape[0], a.shape[1], a.shape[2],
+                                         int(bool(to_srgb))))
+
+
+def image_save_pfm(path, img):
+    """rd_image_save_pfm: float RGB, ColorBuffer row 0 first (PFM's bottom row)."""
+    a = _colorbuffer(img)
+    _image_check(lib().rd_image_save_pfm(os.fsencode(path), _ptr(a), a.shape[0], a.shape[1], a.shape[2]))
 
 
 def bvh_build_host(bmin, bmax, max_prim):

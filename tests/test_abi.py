@@ -26,7 +26,7 @@ def test_every_declared_symbol_is_exported():
     assert set(names) <= set(native.EXPORTS), set(names) - set(native.EXPORTS)
     # every entry point that takes arguments has its ctypes signature declared
     # (an undeclared one would pass 64-bit handles as 32-bit ints)
-    no_args = {"rd_abi_version", "rd_last_error"}
+    no_args = {"rd_abi_version", "rd_last_error", "rd_image_last_error"}
     undeclared = [n for n in names if n not in no_args and getattr(lib, n).argtypes is None]
     assert not undeclared, undeclared
 
@@ -38,6 +38,7 @@ def test_abi_version_and_struct_sizes():
     assert C.sizeof(abi.rd_ray) == 32
     assert C.sizeof(abi.rd_hit) == 16
     assert C.sizeof(abi.rd_medium_desc) == 32
+    assert C.sizeof(abi.rd_image) == 24
     assert C.sizeof(abi.rd_prim_desc) == 24
     assert C.sizeof(abi.rd_render_params) == 32
     assert C.sizeof(abi.rd_texture_desc) == 40
