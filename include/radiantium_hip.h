@@ -445,7 +445,16 @@ typedef struct rd_tuning {
                                   until both are empty; 0 (default) every
                                   chain walked to its end by one lane in
                                   one launch                               */
-    int32_t reserved[4];       /* 0                                         */
+    int32_t pool_log2;         /* cap of the slot pool when wave_paths is 0:
+                                  at most 2^pool_log2 paths in flight
+                                  (10..27); 0 = the memory rule alone      */
+    int32_t tile;              /* edge of the pixel tiles paths are
+                                  numbered in (1..64, default 8)            */
+    int32_t stack_min;         /* traversal LDS stack variant at least this
+                                  deep (0, or 24..64): a deeper variant
+                                  than the tree needs, with 128-thread
+                                  blocks from 40 entries on                */
+    int32_t reserved[1];       /* 0                                         */
 } rd_tuning;
 
 int rd_get_tuning(const rd_ctx* ctx, rd_tuning* out);
@@ -454,13 +463,27 @@ int rd_set_tuning(rd_ctx* ctx, const rd_tuning* tuning);
 /* Autotuner: renders `params` (typically the production render with a few
  * spp) under candidate tunings, one knob at a time from the current setting
  * (extension and shadow schedules, the coherence sort, its key bits and the
- * shadow rays' order, the stream overlap),
- * keeps each knob's fastest value by device time (best of two renders, a
- * change must gain 1 %), leaves the result set on the context and copies it
- * to `out` (optional).  No counterpart in the reference.  Outputs of later
- * renders do not depend on it; rd_get_render_stats afterwards describes the
- * last trial render. */
+ * shadow rays' order, the stream overlap, the slot pool's size (queue
+ * capacity), the path tile, the traversal stack variant and with it the
+ * block size, the device-driven tail and its threshold), keeps each knob's
+ * fastest value by device time (best of two renders, a change must gain
+ * 1 %), leaves the result set on the context and copies it to `out`
+ * (optional).  The result is also kept in a process-wide cache under the
+ * device (architecture, compute units) and the scene class (integrator and
+ * method, instancing, log2 of the primitive count and of the paths per
+ * render, both in steps of two).  No counterpart in the reference.  Outputs
+ * of later renders do not depend on it; rd_get_render_stats afterwards
+ * describes the last trial render. */
 int rd_autotune(rd_ctx* ctx, rd_scene* scene, const rd_render_params* params, rd_tuning* out);
+/* The cached tuning of this device and scene class, set on the context when
+ * there is one (*found = 1), else rd_autotune (*found = 0). */
+int rd_autotune_cached(rd_ctx* ctx, rd_scene* scene, const rd_render_params* params, rd_tuning* out,
+                       int32_t* found);
+/* The cache as text, one line per device and scene class, so a server keeps
+ * its tunings across restarts; load merges into the cache. */
+int rd_tuning_cache_save(const char* path);
+int rd_tuning_cache_load(const char* path);
+void rd_tuning_cache_clear(void);
 
 /* ---- batched rendering: many scenes / sample counts in shared waves ----- */
 /* One request: a scene of this context, its render parameters (spp, seed,

@@ -158,6 +158,11 @@ struct rd_ctx {
     // launches replays it without capturing again
     hipGraphExec_t tail_exec = nullptr;
     std::string tail_key;
+    // slot pool cap (log2; 0: the memory rule), path tile edge, least LDS
+    // stack variant (RD_POOL_LOG2, RD_TILE, RD_STACK_MIN)
+    int pool_log2 = 0;
+    int tile = 8;
+    int stack_min = 0;
     // scenes created on this context and not yet destroyed; rd_destroy with
     // live scenes releases the wavefront buffers and leaves the rest to the
     // last rd_scene_destroy, so destruction order does not matter
@@ -519,6 +524,9 @@ rd_ctx* rd_create(int device) {
     if (const char* env = getenv("RD_DEVICE_LOOP")) c->device_loop = std::max(0, std::min(2, atoi(env)));
     if (const char* env = getenv("RD_TAIL_RAYS")) c->tail_rays = std::max(0, atoi(env));
     if (const char* env = getenv("RD_CHAIN_LOCKSTEP")) c->chain_lockstep = atoi(env) != 0;
+    if (const char* env = getenv("RD_POOL_LOG2")) { int v = atoi(env); c->pool_log2 = v == 0 ? 0 : std::max(10, std::min(27, v)); }
+    if (const char* env = getenv("RD_TILE")) c->tile = std::max(1, std::min(64, atoi(env)));
+    if (const char* env = getenv("RD_STACK_MIN")) c->stack_min = std::max(0, std::min(64, atoi(env)));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         g_err = "hipStreamCreate failed"; delete c; return nullptr;
     }
@@ -1533,13 +1541,14 @@ constexpr size_t kMinPool = (size_t)1 << 22;
 constexpr size_t kMaxPool = (size_t)1 << 27;
 constexpr size_t kPoolSlotBytes = 512;
 
-size_t default_pool(size_t paths) {
+size_t default_pool(size_t paths, int cap_log2) {
     size_t free_b = 0, total_b = 0;
     size_t pool = kMinPool;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
         const size_t slots = std::min(total_b / 4, free_b / 2) / kPoolSlotBytes;
         while (pool * 2 <= slots && pool * 2 <= kMaxPool) pool *= 2;
     }
+    if (cap_log2 > 0) pool = std::min(pool, (size_t)1 << cap_log2);   // rd_tuning.pool_log2
     return std::min(pool, std::max<size_t>(paths, 1));
 }
 
@@ -1583,7 +1592,7 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
     size_t pool = 0, pass_cap = 0;
     if (!noalloc) {
         pool = pool_paths > 0 ? std::min((size_t)pool_paths, std::max<size_t>(total_paths, 1))
-                              : default_pool(total_paths);
+                              : default_pool(total_paths, c->pool_log2);
         pass_cap = std::max(std::min(pass_limit(), total_paths), max_npix);
     } else {
         // everything from the work area: the sums and one sample's results
@@ -1653,6 +1662,7 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
             rep.num_insts = std::max(rep.num_insts, dscenes[i].num_insts);   // any instanced scene: two-level kernels
             rep.mat_mask |= dscenes[i].mat_mask;   // the shading kernels must hold every scene's materials
         }
+        rep.stack_needed = std::max(rep.stack_needed, c->stack_min);   // rd_tuning.stack_min
         std::vector<int> next_sample(n, 0);
         size_t gi = 0;
         while (gi < G.size() && !rc) {
@@ -1678,6 +1688,7 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
                 sg.width = reqs[i].scene->width;
                 sg.height = reqs[i].scene->height;
                 sg.seed = reqs[i].params.seed;
+                sg.tile = c->tile;
                 sg.sum_offset = (int)sum_off[i];
                 segs.push_back(sg);
                 used += np * (size_t)k;
@@ -1710,6 +1721,7 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
         wp.x0 = cr.x0; wp.y0 = cr.y0; wp.crop_w = cr.x1 - cr.x0; wp.crop_h = cr.y1 - cr.y0;
         wp.npix = wp.crop_w * wp.crop_h;
         wp.width = reqs[i].scene->width;
+        wp.tile = c->tile;
         HIP_TRY(tm.run("finalize", [&] {
             return launch_finalize(wp, reqs[i].scene->height, reqs[i].params.spp, wb.sum + sum_off[i],
                                    wb.sum2 + sum_off[i], reqs[i].out_rgb, reqs[i].out_sum2, single ? out64 : nullptr,
@@ -1781,7 +1793,7 @@ int rd_adjoint_device(rd_ctx* c, rd_scene* s, const rd_render_params* p, const f
     const size_t npix = (size_t)cw * (cr.y1 - cr.y0);
     const size_t total = npix * (size_t)p->spp;
     const size_t pool = p->wave_paths > 0 ? std::min((size_t)p->wave_paths, std::max<size_t>(total, 1))
-                                          : default_pool(total);
+                                          : default_pool(total, c->pool_log2);
     const size_t pass_cap = std::max(std::min(pass_limit(), total), npix);
     if (int rc = ensure_wave_buffers(c, pool, pass_cap, npix, false)) return rc;
     if (int rc = ensure_adjoint_buffers(c, (size_t)s->num_textures)) return rc;
@@ -1824,6 +1836,7 @@ int rd_adjoint_device(rd_ctx* c, rd_scene* s, const rd_render_params* p, const f
         sg.width = s->width;
         sg.height = s->height;
         sg.seed = p->seed;
+        sg.tile = c->tile;
         HIP_TRY(hipStreamSynchronize(st));
         *wb.host_segs = sg;
         HIP_TRY(hipMemcpyAsync(wb.segs, wb.host_segs, sizeof(BatchSeg), hipMemcpyHostToDevice, st));
@@ -2020,6 +2033,9 @@ int rd_get_tuning(const rd_ctx* c, rd_tuning* t) {
     t->device_loop = c->device_loop;
     t->tail_rays = c->tail_rays;
     t->chain_lockstep = c->chain_lockstep ? 1 : 0;
+    t->pool_log2 = c->pool_log2;
+    t->tile = c->tile;
+    t->stack_min = c->stack_min;
     return RD_OK;
 }
 
@@ -2033,6 +2049,10 @@ int rd_set_tuning(rd_ctx* c, const rd_tuning* t) {
     if (t->ray_sort_min < -1) return fail(RD_ERR_INVALID, "ray_sort_min: >= 0 or -1");
     if (t->device_loop < -1 || t->device_loop > 2) return fail(RD_ERR_INVALID, "device_loop: 0, 1, 2 or -1");
     if (t->tail_rays < -1) return fail(RD_ERR_INVALID, "tail_rays: >= 0 or -1");
+    if (t->pool_log2 != -1 && t->pool_log2 != 0 && (t->pool_log2 < 10 || t->pool_log2 > 27))
+        return fail(RD_ERR_INVALID, "pool_log2: 0, 10..27 or -1");
+    if (t->tile != -1 && (t->tile < 1 || t->tile > 64)) return fail(RD_ERR_INVALID, "tile: 1..64 or -1");
+    if (t->stack_min < -1 || t->stack_min > 64) return fail(RD_ERR_INVALID, "stack_min: 0..64 or -1");
     CtxLock lock(c);
     if (!lock.ok()) return fail(RD_ERR_BUSY, "context in use by another thread");
     for (int k = 0; k < 3; k++)
@@ -2045,8 +2065,68 @@ int rd_set_tuning(rd_ctx* c, const rd_tuning* t) {
     if (t->device_loop >= 0) c->device_loop = t->device_loop;
     if (t->tail_rays >= 0) c->tail_rays = t->tail_rays;
     if (t->chain_lockstep >= 0) c->chain_lockstep = t->chain_lockstep != 0;
+    if (t->pool_log2 >= 0) c->pool_log2 = t->pool_log2;
+    if (t->tile >= 0) c->tile = t->tile;
+    if (t->stack_min >= 0) c->stack_min = t->stack_min;
     return RD_OK;
 }
+
+namespace {
+
+// Tunings found by rd_autotune, per device and scene class, for the process.
+struct TuneCache {
+    std::mutex mu;
+    std::map<std::string, rd_tuning> m;
+};
+TuneCache& tune_cache() {
+    static TuneCache t;
+    return t;
+}
+
+int ilog2(uint64_t v) {
+    int l = 0;
+    while (v > 1) { v >>= 1; l++; }
+    return l;
+}
+
+// device: architecture and compute units; scene class: integrator and
+// method, instancing, log2 of the primitives and of the render's paths
+// (both rounded down to even)
+std::string tune_key(const rd_ctx* c, const rd_scene* s, const rd_render_params* p) {
+    hipDeviceProp_t prop{};
+    std::string dev = "unknown";
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess)
+        dev = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount);
+    Crop cr{0, 0, s->width, s->height};
+    (void)resolve_crop(s, p, &cr);
+    const uint64_t paths = (uint64_t)(cr.x1 - cr.x0) * (uint64_t)(cr.y1 - cr.y0) * (uint64_t)std::max(p->spp, 1);
+    const DevIntegrator& I = s->dev.integ;
+    char buf[160];
+    std::snprintf(buf, sizeof(buf), "%s|%s/%d|inst%d|prims2^%d|paths2^%d", dev.c_str(),
+                  I.type == RD_INTEGRATOR_VOL_PATH ? "vol_path" : "path", I.type == RD_INTEGRATOR_VOL_PATH ? 0 : I.method,
+                  s->dev.num_insts > 0 ? 1 : 0, ilog2((uint64_t)std::max(s->dev.num_prims, 1)) & ~1,
+                  ilog2(std::max<uint64_t>(paths, 1)) & ~1);
+    return buf;
+}
+
+const int kTuneFields = 15;
+void tuning_to_ints(const rd_tuning& t, int* v) {
+    const int32_t f[kTuneFields] = {t.trace_schedule[0], t.trace_schedule[1], t.trace_schedule[2], t.ray_sort,
+                                    t.ray_sort_bits,     t.ray_sort_min,      t.ray_sort_shadow,   t.overlap,
+                                    t.device_loop,       t.tail_rays,         t.chain_lockstep,    t.pool_log2,
+                                    t.tile,              t.stack_min,         0};
+    for (int k = 0; k < kTuneFields; k++) v[k] = f[k];
+}
+rd_tuning tuning_from_ints(const int* v) {
+    rd_tuning t{};
+    for (int k = 0; k < 3; k++) t.trace_schedule[k] = v[k];
+    t.ray_sort = v[3]; t.ray_sort_bits = v[4]; t.ray_sort_min = v[5]; t.ray_sort_shadow = v[6]; t.overlap = v[7];
+    t.device_loop = v[8]; t.tail_rays = v[9]; t.chain_lockstep = v[10]; t.pool_log2 = v[11]; t.tile = v[12];
+    t.stack_min = v[13];
+    return t;
+}
+
+}  // namespace
 
 int rd_autotune(rd_ctx* c, rd_scene* s, const rd_render_params* p, rd_tuning* out) {
     if (!c || !s || !p) return fail(RD_ERR_INVALID, "null argument");
@@ -2077,18 +2157,42 @@ int rd_autotune(rd_ctx* c, rd_scene* s, const rd_render_params* p, rd_tuning* ou
             int* field;
             std::vector<int> values;
         };
+        Crop cr{0, 0, s->width, s->height};
+        (void)resolve_crop(s, p, &cr);
+        const uint64_t paths = (uint64_t)(cr.x1 - cr.x0) * (cr.y1 - cr.y0) * (uint64_t)p->spp;
+        // slot pools below the render's paths (larger ones change nothing)
+        std::vector<int> pools = {0};
+        if (p->wave_paths <= 0)
+            for (int l : {22, 24, 26})
+                if (((uint64_t)1 << l) < paths) pools.push_back(l);
+        // the next deeper LDS stack variant (from 40 entries: 128-thread blocks)
+        std::vector<int> stacks = {0};
+        {
+            const int need = std::max(s->dev.stack_needed, 1);
+            for (int v : {24, 26, 32, 40, 48, 64})
+                if (v > need) { if (std::find(stacks.begin(), stacks.end(), v) == stacks.end()) stacks.push_back(v); break; }
+            if (need <= 32) stacks.push_back(40);   // the 128-thread family
+            std::sort(stacks.begin(), stacks.end());
+            stacks.erase(std::unique(stacks.begin(), stacks.end()), stacks.end());
+        }
         int sort = c->ray_sort ? 1 : 0, ovl = c->overlap ? 1 : 0, sort_sh = c->ray_sort_shadow ? 1 : 0;
         std::vector<Knob> knobs = {{&c->trace_schedule[0], {0, 2, 3}},
                                    {&c->trace_schedule[1], {0, 2, 3}},
                                    {&sort, {0, 1}},
                                    {&c->ray_sort_bits, {12, 15, 18}},
                                    {&sort_sh, {0, 1}},
-                                   {&ovl, {0, 1}}};
+                                   {&ovl, {0, 1}},
+                                   {&c->tile, {4, 8, 16}},
+                                   {&c->pool_log2, pools},
+                                   {&c->stack_min, stacks},
+                                   {&c->device_loop, {0, 2}},
+                                   {&c->tail_rays, {1 << 16, 1 << 18, 1 << 20}}};
         rc = render_requests(c, 1, &r, true, p->wave_paths, nullptr, nullptr);   // buffers, code objects
         double best = rc ? 0.0 : time_it();
         for (Knob& k : knobs) {
             if (rc) break;
             if ((k.field == &c->ray_sort_bits || k.field == &sort_sh) && !c->ray_sort) continue;
+            if (k.field == &c->tail_rays && c->device_loop == 0) continue;
             const int keep = *k.field;
             int chosen = keep;
             for (int v : k.values) {
@@ -2108,7 +2212,88 @@ int rd_autotune(rd_ctx* c, rd_scene* s, const rd_render_params* p, rd_tuning* ou
         (void)hipFree(d);
         if (rc) return rc;
     }
+    rd_tuning t{};
+    (void)rd_get_tuning(c, &t);
+    {
+        TuneCache& tc = tune_cache();
+        std::lock_guard<std::mutex> g(tc.mu);
+        tc.m[tune_key(c, s, p)] = t;
+    }
+    if (out) *out = t;
+    return RD_OK;
+}
+
+int rd_autotune_cached(rd_ctx* c, rd_scene* s, const rd_render_params* p, rd_tuning* out, int32_t* found) {
+    if (!c || !s || !p) return fail(RD_ERR_INVALID, "null argument");
+    if (found) *found = 0;
+    rd_tuning t{};
+    bool hit = false;
+    {
+        TuneCache& tc = tune_cache();
+        std::lock_guard<std::mutex> g(tc.mu);
+        auto it = tc.m.find(tune_key(c, s, p));
+        if (it != tc.m.end()) { t = it->second; hit = true; }
+    }
+    if (!hit) return rd_autotune(c, s, p, out);
+    if (int rc = rd_set_tuning(c, &t)) return rc;
+    if (found) *found = 1;
     return out ? rd_get_tuning(c, out) : RD_OK;
+}
+
+int rd_tuning_cache_save(const char* path) {
+    if (!path) return fail(RD_ERR_INVALID, "null path");
+    TuneCache& tc = tune_cache();
+    std::lock_guard<std::mutex> g(tc.mu);
+    std::FILE* f = std::fopen(path, "w");
+    if (!f) return fail(RD_ERR_IO, std::string("cannot open ") + path);
+    for (const auto& kv : tc.m) {
+        int v[kTuneFields];
+        tuning_to_ints(kv.second, v);
+        std::fprintf(f, "%s", kv.first.c_str());
+        for (int k = 0; k < kTuneFields; k++) std::fprintf(f, "\t%d", v[k]);
+        std::fprintf(f, "\n");
+    }
+    return std::fclose(f) == 0 ? RD_OK : fail(RD_ERR_IO, std::string("cannot write ") + path);
+}
+
+int rd_tuning_cache_load(const char* path) {
+    if (!path) return fail(RD_ERR_INVALID, "null path");
+    std::FILE* f = std::fopen(path, "r");
+    if (!f) return fail(RD_ERR_IO, std::string("cannot open ") + path);
+    std::map<std::string, rd_tuning> got;
+    char line[1024];
+    int rc = RD_OK;
+    while (std::fgets(line, sizeof(line), f)) {
+        std::string l(line);
+        while (!l.empty() && (l.back() == '\n' || l.back() == '\r')) l.pop_back();
+        if (l.empty()) continue;
+        const size_t tab = l.find('\t');
+        if (tab == std::string::npos) { rc = fail(RD_ERR_INVALID, "tuning cache: bad line"); break; }
+        int v[kTuneFields];
+        const char* q = l.c_str() + tab;
+        int k = 0;
+        for (; k < kTuneFields; k++) {
+            char* e = nullptr;
+            const long x = std::strtol(q, &e, 10);
+            if (e == q) break;
+            v[k] = (int)x;
+            q = e;
+        }
+        if (k != kTuneFields) { rc = fail(RD_ERR_INVALID, "tuning cache: bad line"); break; }
+        got[l.substr(0, tab)] = tuning_from_ints(v);
+    }
+    std::fclose(f);
+    if (rc) return rc;
+    TuneCache& tc = tune_cache();
+    std::lock_guard<std::mutex> g(tc.mu);
+    for (auto& kv : got) tc.m[kv.first] = kv.second;
+    return RD_OK;
+}
+
+void rd_tuning_cache_clear(void) {
+    TuneCache& tc = tune_cache();
+    std::lock_guard<std::mutex> g(tc.mu);
+    tc.m.clear();
 }
 
 void* rd_device_alloc(rd_ctx* c, uint64_t bytes) {

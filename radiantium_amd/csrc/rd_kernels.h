@@ -58,6 +58,7 @@ struct WaveParams {
     int nsamples;       // samples per pixel in this wave
     int width;          // full image width (pixel id = y * width + x)
     uint32_t seed;
+    int tile;           // path-order tile edge (the accumulation buffers' order)
 };
 
 // Device counters used by the bounce loop.
@@ -92,6 +93,7 @@ struct BatchSeg {
     int width, height;
     uint32_t seed;
     int sum_offset;
+    int tile;   // path-order tile edge in pixels (tile_pixel)
 };
 
 // Adjoint replay inputs: dLoss/dImage (column-major W*H*3, device), the

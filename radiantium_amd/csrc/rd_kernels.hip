@@ -931,22 +931,23 @@ __device__ __forceinline__ int find_seg(const BatchSeg* segs, int nseg, int p) {
     return lo;
 }
 
-// Pixel q (< w*h) of a crop in 8x8 tiles: tile rows of 8 pixel rows top to
-// bottom, tiles left to right, pixels row-major inside a tile (the last tile
-// row and column may be narrower).  Consecutive paths then start camera rays
-// over a square of the image, which share more BVH nodes per wavefront load
-// than a row's (about 28 % fewer distinct node lines on the 1080p workload).
-// The accumulation buffers are kept in the same order.
-__device__ __forceinline__ void tile_pixel(int q, int w, int h, int* px, int* py) {
-    const int tr = q / (8 * w);
-    const int th = min(8, h - tr * 8);
-    const int r = q - tr * 8 * w;
-    const int ntc = (w + 7) / 8;
-    const int j = min(r / (8 * th), ntc - 1);
-    const int o = r - j * 8 * th;
-    const int tw = min(8, w - j * 8);
-    *px = j * 8 + o % tw;
-    *py = tr * 8 + o / tw;
+// Pixel q (< w*h) of a crop in t x t tiles (t = 8 by default,
+// rd_tuning.tile): tile rows of t pixel rows top to bottom, tiles left to
+// right, pixels row-major inside a tile (the last tile row and column may be
+// narrower).  Consecutive paths then start camera rays over a square of the
+// image, which share more BVH nodes per wavefront load than a row's (about
+// 28 % fewer distinct node lines on the 1080p workload at t = 8).  The
+// accumulation buffers are kept in the same order.
+__device__ __forceinline__ void tile_pixel(int q, int w, int h, int t, int* px, int* py) {
+    const int tr = q / (t * w);
+    const int th = min(t, h - tr * t);
+    const int r = q - tr * t * w;
+    const int ntc = (w + t - 1) / t;
+    const int j = min(r / (t * th), ntc - 1);
+    const int o = r - j * t * th;
+    const int tw = min(t, w - j * t);
+    *px = j * t + o % tw;
+    *py = tr * t + o / tw;
 }
 
 // Path order inside a segment: tile by tile (tile_pixel's order), and inside
@@ -956,29 +957,31 @@ __device__ __forceinline__ void tile_pixel(int q, int w, int h, int* px, int* py
 // flight cover about a hundred tiles instead of a band of the image per
 // sample, so the BVH nodes and triangles under them are fetched into L2 once
 // rather than once per sample.  seg_path: segment-local path index ->
-// (pixel q in tile order, sample si); seg_local is its inverse.
-__device__ __forceinline__ void seg_path(int local, int w, int h, int ns, int* q, int* si) {
+// (pixel q in tile order, sample si); seg_local is its inverse.  The order is
+// a schedule only: a path's key is its pixel and sample, and every pixel's
+// samples are summed in sample order, so images do not depend on t.
+__device__ __forceinline__ void seg_path(int local, int w, int h, int ns, int t, int* q, int* si) {
     // a segment holds fewer than 2^31 paths, but a full tile row of a crop a
     // few pixels tall can be counted past that
-    const long long row = 8LL * w * ns;         // paths per full tile row
+    const long long row = (long long)t * w * ns;   // paths per full tile row
     const int tr = (int)(local / row);
-    const int th = min(8, h - tr * 8);
+    const int th = min(t, h - tr * t);
     const int r = (int)(local - tr * row);
-    const int ntc = (w + 7) / 8;
-    const int j = (int)min((long long)r / (8LL * th * ns), (long long)(ntc - 1));
-    const int o = (int)(r - (long long)j * 8 * th * ns);
-    const int nt = min(8, w - j * 8) * th;      // pixels in the tile
+    const int ntc = (w + t - 1) / t;
+    const int j = (int)min((long long)r / ((long long)t * th * ns), (long long)(ntc - 1));
+    const int o = (int)(r - (long long)j * t * th * ns);
+    const int nt = min(t, w - j * t) * th;         // pixels in the tile
     *si = o / nt;
-    *q = tr * 8 * w + j * 8 * th + (o - *si * nt);
+    *q = tr * t * w + j * t * th + (o - *si * nt);
 }
-__device__ __forceinline__ int seg_local(int q, int si, int w, int h, int ns) {
-    const int tr = q / (8 * w);
-    const int th = min(8, h - tr * 8);
-    const int r = q - tr * 8 * w;
-    const int ntc = (w + 7) / 8;
-    const int j = min(r / (8 * th), ntc - 1);
-    const int o = r - j * 8 * th;               // pixel within the tile
-    const int nt = min(8, w - j * 8) * th;
+__device__ __forceinline__ int seg_local(int q, int si, int w, int h, int ns, int t) {
+    const int tr = q / (t * w);
+    const int th = min(t, h - tr * t);
+    const int r = q - tr * t * w;
+    const int ntc = (w + t - 1) / t;
+    const int j = min(r / (t * th), ntc - 1);
+    const int o = r - j * t * th;                  // pixel within the tile
+    const int nt = min(t, w - j * t) * th;
     return (q - o) * ns + si * nt + o;
 }
 
@@ -987,8 +990,8 @@ __device__ __forceinline__ int seg_local(int q, int si, int w, int h, int ns) {
 __device__ __forceinline__ void start_path(const DevScene& sc, const BatchSeg& sg, int g, int slot, int qi, RayQueue q,
                                            PathState ps) {
     int si, qq, tx, ty;
-    seg_path(g - sg.path_offset, sg.crop_w, sg.npix / sg.crop_w, sg.nsamples, &qq, &si);
-    tile_pixel(qq, sg.crop_w, sg.npix / sg.crop_w, &tx, &ty);
+    seg_path(g - sg.path_offset, sg.crop_w, sg.npix / sg.crop_w, sg.nsamples, sg.tile, &qq, &si);
+    tile_pixel(qq, sg.crop_w, sg.npix / sg.crop_w, sg.tile, &tx, &ty);
     int x = sg.x0 + tx, y = sg.y0 + ty;
     const uint64_t key = path_key(sg.seed, (uint32_t)(y * sg.width + x), (uint32_t)(sg.sample_base + si));
     Rng rng{key, 0u};
@@ -1758,8 +1761,8 @@ __global__ __launch_bounds__(256) void k_accumulate_batch(const BatchSeg* segs, 
             float4 q4 = sum2[o];
             double sx = s4.x, sy = s4.y, sz = s4.z;
             f3 s2 = mk(q4.x, q4.y, q4.z);
-            const int p0 = sg.path_offset + seg_local(q, 0, sg.crop_w, h, sg.nsamples);
-            const int step = seg_local(q, 1, sg.crop_w, h, sg.nsamples) - seg_local(q, 0, sg.crop_w, h, sg.nsamples);
+            const int p0 = sg.path_offset + seg_local(q, 0, sg.crop_w, h, sg.nsamples, sg.tile);
+            const int step = seg_local(q, 1, sg.crop_w, h, sg.nsamples, sg.tile) - seg_local(q, 0, sg.crop_w, h, sg.nsamples, sg.tile);
             for (int j = 0; j < sg.nsamples; j++) {
                 f3 c = ld3(ps.res, p0 + j * step);
                 if (!valid_color(c)) { c = mk(0.0f); invalid++; }
@@ -1792,7 +1795,7 @@ __global__ __launch_bounds__(256) void k_finalize(WaveParams wp, int height, int
                                                   const float4* sum2, T* out, float* out2) {
     for (int q = blockIdx.x * 256 + threadIdx.x; q < wp.npix; q += gridDim.x * 256) {
         int tx, ty;
-        tile_pixel(q, wp.crop_w, wp.crop_h, &tx, &ty);
+        tile_pixel(q, wp.crop_w, wp.crop_h, wp.tile, &tx, &ty);
         int x = wp.x0 + tx, y = wp.y0 + ty;
         size_t o = ((size_t)x * height + y) * 3;
         double4 s = sum[q];

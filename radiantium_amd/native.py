@@ -24,7 +24,8 @@ EXPORTS = [
     "rd_trace_closest_sub", "rd_json_load", "rd_json_scene_desc", "rd_json_scene_free", "rd_scene_load_json",
     "rd_metal_eta_k", "rd_get_tuning", "rd_set_tuning", "rd_autotune", "rd_render_device_noalloc",
     "rd_render_mean_f64", "rd_image_load", "rd_image_free", "rd_image_last_error", "rd_image_save_exr",
-    "rd_image_save_png", "rd_image_save_pfm",
+    "rd_image_save_png", "rd_image_save_pfm", "rd_autotune_cached", "rd_tuning_cache_save", "rd_tuning_cache_load",
+    "rd_tuning_cache_clear",
 ]
 
 
@@ -96,6 +97,11 @@ def lib():
         L.rd_get_tuning.argtypes = [vp, C.POINTER(abi.rd_tuning)]
         L.rd_set_tuning.argtypes = [vp, C.POINTER(abi.rd_tuning)]
         L.rd_autotune.argtypes = [vp, vp, C.POINTER(abi.rd_render_params), C.POINTER(abi.rd_tuning)]
+        L.rd_autotune_cached.argtypes = [vp, vp, C.POINTER(abi.rd_render_params), C.POINTER(abi.rd_tuning),
+                                         C.POINTER(C.c_int32)]
+        L.rd_tuning_cache_save.argtypes = [C.c_char_p]
+        L.rd_tuning_cache_load.argtypes = [C.c_char_p]
+        L.rd_tuning_cache_clear.argtypes = []
         L.rd_image_load.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(abi.rd_image)]
         L.rd_image_free.argtypes = [C.POINTER(abi.rd_image)]
         L.rd_image_last_error.restype = C.c_char_p
@@ -164,6 +170,20 @@ def image_save_pfm(path, img):
     """rd_image_save_pfm: float RGB, ColorBuffer row 0 first (PFM's bottom row)."""
     a = _colorbuffer(img)
     _image_check(lib().rd_image_save_pfm(os.fsencode(path), _ptr(a), a.shape[0], a.shape[1], a.shape[2]))
+
+
+def tuning_cache_save(path):
+    """rd_tuning_cache_save: the process's autotuner results as text."""
+    check(lib().rd_tuning_cache_save(os.fsencode(path)))
+
+
+def tuning_cache_load(path):
+    """rd_tuning_cache_load: merge a saved cache into the process's."""
+    check(lib().rd_tuning_cache_load(os.fsencode(path)))
+
+
+def tuning_cache_clear():
+    lib().rd_tuning_cache_clear()
 
 
 def bvh_build_host(bmin, bmax, max_prim):
@@ -305,10 +325,12 @@ class Context:
         check(lib().rd_get_tuning(self.h, C.byref(t)))
         return {"trace_schedule": list(t.trace_schedule), "ray_sort": t.ray_sort, "ray_sort_bits": t.ray_sort_bits,
                 "ray_sort_min": t.ray_sort_min, "ray_sort_shadow": t.ray_sort_shadow, "overlap": t.overlap,
-                "device_loop": t.device_loop, "tail_rays": t.tail_rays, "chain_lockstep": t.chain_lockstep}
+                "device_loop": t.device_loop, "tail_rays": t.tail_rays, "chain_lockstep": t.chain_lockstep,
+                "pool_log2": t.pool_log2, "tile": t.tile, "stack_min": t.stack_min}
 
     def set_tuning(self, trace_schedule=None, ray_sort=None, ray_sort_bits=None, ray_sort_min=None,
-                   ray_sort_shadow=None, overlap=None, device_loop=None, tail_rays=None, chain_lockstep=None):
+                   ray_sort_shadow=None, overlap=None, device_loop=None, tail_rays=None, chain_lockstep=None,
+                   pool_log2=None, tile=None, stack_min=None):
         """rd_set_tuning; arguments left None keep their value."""
         t = abi.rd_tuning()
         sched = trace_schedule if trace_schedule is not None else (-1, -1, -1)
@@ -316,16 +338,22 @@ class Context:
             t.trace_schedule[i] = -1 if sched[i] is None else int(sched[i])
         for f, v in (("ray_sort", ray_sort), ("ray_sort_bits", ray_sort_bits), ("ray_sort_min", ray_sort_min),
                      ("ray_sort_shadow", ray_sort_shadow), ("overlap", overlap), ("device_loop", device_loop),
-                     ("tail_rays", tail_rays), ("chain_lockstep", chain_lockstep)):
+                     ("tail_rays", tail_rays), ("chain_lockstep", chain_lockstep), ("pool_log2", pool_log2),
+                     ("tile", tile), ("stack_min", stack_min)):
             setattr(t, f, -1 if v is None else int(v))
         check(lib().rd_set_tuning(self.h, C.byref(t)))
 
-    def autotune(self, scene, spp=4, seed=0, wave_paths=0):
-        """rd_autotune on `scene` rendered at `spp`; returns the tuning it
-        leaves on the context (as tuning())."""
+    def autotune(self, scene, spp=4, seed=0, wave_paths=0, cached=False):
+        """rd_autotune (cached: rd_autotune_cached) on `scene` rendered at
+        `spp`; returns the tuning it leaves on the context (as tuning()),
+        and with cached=True also whether it came from the cache."""
         p = abi.rd_render_params()
         p.spp, p.seed, p.wave_paths = spp, seed, wave_paths
         t = abi.rd_tuning()
+        if cached:
+            found = C.c_int32(0)
+            check(lib().rd_autotune_cached(self.h, scene.h, C.byref(p), C.byref(t), C.byref(found)))
+            return self.tuning(), bool(found.value)
         check(lib().rd_autotune(self.h, scene.h, C.byref(p), C.byref(t)))
         return self.tuning()
 
