@@ -323,6 +323,15 @@ int rd_image_save_png(const char* path, const float* data, int width, int height
  * the bottom, so ColorBuffer row 0 first), for lossless float output. */
 int rd_image_save_pfm(const char* path, const float* data, int width, int height, int channels);
 
+/* Bvh.RecursiveBuild + FlattenBVHTree (Bvh.cs:234-442) on the device, the
+ * tree rd_bvh_build_host builds, node for node: device bounds in (d_bmin,
+ * d_bmax: [n][3]), device records out (d_nodes32: capacity max_nodes >= 2n-1
+ * records of 32 bytes; d_prim_order: n indices).  Runs on the context's
+ * stream and returns when the tree is complete. */
+int rd_bvh_build_device(rd_ctx* ctx, const float* d_bmin, const float* d_bmax, int32_t n, int32_t max_prim,
+                        void* d_nodes32, int32_t max_nodes, int32_t* d_prim_order, int32_t* num_nodes,
+                        int32_t* max_depth);
+
 /* ---- scene configs ------------------------------------------------------- */
 /* The reference's scene config (JSON, as Radiantium.Cli/Program.cs:25-140
  * reads it) with its OBJ models and images, built into a scene
@@ -454,7 +463,11 @@ typedef struct rd_tuning {
                                   deep (0, or 24..64): a deeper variant
                                   than the tree needs, with 128-thread
                                   blocks from 40 entries on                */
-    int32_t reserved[1];       /* 0                                         */
+    int32_t bvh_build;         /* where rd_scene_create builds its trees:
+                                  0 the host, 1 the device for trees of 2^15
+                                  primitives or more (default), 2 the device
+                                  always; both give the same tree, node for
+                                  node                                      */
 } rd_tuning;
 
 int rd_get_tuning(const rd_ctx* ctx, rd_tuning* out);

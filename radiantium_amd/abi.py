@@ -169,8 +169,7 @@ class rd_tuning(C.Structure):
         ("trace_schedule", C.c_int32 * 3), ("ray_sort", C.c_int32), ("ray_sort_bits", C.c_int32),
         ("ray_sort_min", C.c_int32), ("ray_sort_shadow", C.c_int32), ("overlap", C.c_int32),
         ("device_loop", C.c_int32), ("tail_rays", C.c_int32), ("chain_lockstep", C.c_int32),
-        ("pool_log2", C.c_int32), ("tile", C.c_int32), ("stack_min", C.c_int32),
-        ("reserved", C.c_int32 * 1),
+        ("pool_log2", C.c_int32), ("tile", C.c_int32), ("stack_min", C.c_int32), ("bvh_build", C.c_int32),
     ]
 
 

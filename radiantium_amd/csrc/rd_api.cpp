@@ -163,6 +163,8 @@ struct rd_ctx {
     int pool_log2 = 0;
     int tile = 8;
     int stack_min = 0;
+    // scene trees: 0 host, 1 device from kDeviceBuildMin primitives, 2 device (RD_BVH_BUILD)
+    int bvh_build = 1;
     // scenes created on this context and not yet destroyed; rd_destroy with
     // live scenes releases the wavefront buffers and leaves the rest to the
     // last rd_scene_destroy, so destruction order does not matter
@@ -527,6 +529,7 @@ rd_ctx* rd_create(int device) {
     if (const char* env = getenv("RD_POOL_LOG2")) { int v = atoi(env); c->pool_log2 = v == 0 ? 0 : std::max(10, std::min(27, v)); }
     if (const char* env = getenv("RD_TILE")) c->tile = std::max(1, std::min(64, atoi(env)));
     if (const char* env = getenv("RD_STACK_MIN")) c->stack_min = std::max(0, std::min(64, atoi(env)));
+    if (const char* env = getenv("RD_BVH_BUILD")) c->bvh_build = std::max(0, std::min(2, atoi(env)));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         g_err = "hipStreamCreate failed"; delete c; return nullptr;
     }
@@ -588,6 +591,11 @@ void rd_destroy(rd_ctx* c) {
     c->closed = true;
     if (c->live_scenes == 0) free_ctx(c);
 }
+
+// Trees of at least this many primitives are built on the device when
+// rd_tuning.bvh_build is 1: below it the host builds faster than the
+// device's per-level launches and synchronizations.
+constexpr int kDeviceBuildMin = 1 << 15;
 
 rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     if (!c || !d) { g_err = "null argument"; return nullptr; }
@@ -689,6 +697,12 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
             hi3[k] = hi;
         }
     };
+    // the host builder or the same tree built on the device (rd_tuning.bvh_build)
+    auto tree = [&](int n, const float* lo, const float* hi, int mp, BvhResult* r, std::string* e) {
+        if (c->bvh_build == 2 || (c->bvh_build == 1 && n >= kDeviceBuildMin))
+            return gpu_build_bvh(n, lo, hi, mp, c->stream, r, e);
+        return build_bvh(n, lo, hi, mp, r, e);
+    };
     // one BVH per instance set over its ShapeWrapperPrimitives (RendererBuilder.cs:895-921)
     const int nsets = d->num_instance_sets;
     std::vector<BvhResult> blas(nsets);
@@ -698,7 +712,7 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
         std::vector<float> lo(3 * (size_t)sd.num_triangles), hi(3 * (size_t)sd.num_triangles);
         for (int j = 0; j < sd.num_triangles; j++) tri_bounds(sd.first_triangle + j, &lo[3 * j], &hi[3 * j]);
         std::string berr;
-        if (!build_bvh(sd.num_triangles, lo.data(), hi.data(), sd.bvh_max_prim > 0 ? sd.bvh_max_prim : 1, &blas[si], &berr)) {
+        if (!tree(sd.num_triangles, lo.data(), hi.data(), sd.bvh_max_prim > 0 ? sd.bvh_max_prim : 1, &blas[si], &berr)) {
             delete s;
             g_err = berr;
             return nullptr;
@@ -758,7 +772,7 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     }
     BvhResult bvh;
     std::string err;
-    if (!build_bvh(np, bmin.data(), bmax.data(), d->bvh_max_prim > 0 ? d->bvh_max_prim : 1, &bvh, &err)) {
+    if (!tree(np, bmin.data(), bmax.data(), d->bvh_max_prim > 0 ? d->bvh_max_prim : 1, &bvh, &err)) {
         delete s;
         g_err = err;
         return nullptr;
@@ -1027,6 +1041,26 @@ int rd_bvh_build_host(const float* bmin, const float* bmax, int32_t n, int32_t m
         std::memcpy(nodes32, r.nodes.data(), r.nodes.size() * sizeof(NodeRec));
     }
     if (prim_order) std::memcpy(prim_order, r.order.data(), r.order.size() * sizeof(int32_t));
+    return RD_OK;
+}
+
+int rd_bvh_build_device(rd_ctx* c, const float* d_bmin, const float* d_bmax, int32_t n, int32_t max_prim,
+                        void* d_nodes32, int32_t max_nodes, int32_t* d_prim_order, int32_t* num_nodes,
+                        int32_t* max_depth) {
+    if (!c || (n > 0 && (!d_bmin || !d_bmax || !d_nodes32 || !d_prim_order)) || !num_nodes || !max_depth)
+        return fail(RD_ERR_INVALID, "null argument");
+    if (n < 0) return fail(RD_ERR_INVALID, "negative primitive count");
+    if (n > 0 && (int64_t)max_nodes < 2 * (int64_t)n - 1) return fail(RD_ERR_INVALID, "max_nodes below 2n - 1");
+    CtxLock lock(c);
+    if (!lock.ok()) return fail(RD_ERR_BUSY, "context in use by another thread");
+    (void)hipSetDevice(c->device);
+    std::string err;
+    int depth = 0;
+    const int nn = gpu_build_bvh_device(n, d_bmin, d_bmax, max_prim, (NodeRec*)d_nodes32, d_prim_order, c->stream,
+                                        &depth, &err);
+    if (nn < 0) return fail(RD_ERR_INVALID, err);
+    *num_nodes = nn;
+    *max_depth = depth;
     return RD_OK;
 }
 
@@ -2036,6 +2070,7 @@ int rd_get_tuning(const rd_ctx* c, rd_tuning* t) {
     t->pool_log2 = c->pool_log2;
     t->tile = c->tile;
     t->stack_min = c->stack_min;
+    t->bvh_build = c->bvh_build;
     return RD_OK;
 }
 
@@ -2053,6 +2088,7 @@ int rd_set_tuning(rd_ctx* c, const rd_tuning* t) {
         return fail(RD_ERR_INVALID, "pool_log2: 0, 10..27 or -1");
     if (t->tile != -1 && (t->tile < 1 || t->tile > 64)) return fail(RD_ERR_INVALID, "tile: 1..64 or -1");
     if (t->stack_min < -1 || t->stack_min > 64) return fail(RD_ERR_INVALID, "stack_min: 0..64 or -1");
+    if (t->bvh_build < -1 || t->bvh_build > 2) return fail(RD_ERR_INVALID, "bvh_build: 0, 1, 2 or -1");
     CtxLock lock(c);
     if (!lock.ok()) return fail(RD_ERR_BUSY, "context in use by another thread");
     for (int k = 0; k < 3; k++)
@@ -2068,6 +2104,7 @@ int rd_set_tuning(rd_ctx* c, const rd_tuning* t) {
     if (t->pool_log2 >= 0) c->pool_log2 = t->pool_log2;
     if (t->tile >= 0) c->tile = t->tile;
     if (t->stack_min >= 0) c->stack_min = t->stack_min;
+    if (t->bvh_build >= 0) c->bvh_build = t->bvh_build;
     return RD_OK;
 }
 
@@ -2114,7 +2151,7 @@ void tuning_to_ints(const rd_tuning& t, int* v) {
     const int32_t f[kTuneFields] = {t.trace_schedule[0], t.trace_schedule[1], t.trace_schedule[2], t.ray_sort,
                                     t.ray_sort_bits,     t.ray_sort_min,      t.ray_sort_shadow,   t.overlap,
                                     t.device_loop,       t.tail_rays,         t.chain_lockstep,    t.pool_log2,
-                                    t.tile,              t.stack_min,         0};
+                                    t.tile,              t.stack_min,         t.bvh_build};
     for (int k = 0; k < kTuneFields; k++) v[k] = f[k];
 }
 rd_tuning tuning_from_ints(const int* v) {
@@ -2123,6 +2160,7 @@ rd_tuning tuning_from_ints(const int* v) {
     t.ray_sort = v[3]; t.ray_sort_bits = v[4]; t.ray_sort_min = v[5]; t.ray_sort_shadow = v[6]; t.overlap = v[7];
     t.device_loop = v[8]; t.tail_rays = v[9]; t.chain_lockstep = v[10]; t.pool_log2 = v[11]; t.tile = v[12];
     t.stack_min = v[13];
+    t.bvh_build = v[14];
     return t;
 }
 

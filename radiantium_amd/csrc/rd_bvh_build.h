@@ -1,5 +1,6 @@
 // rd_bvh_build.h — host SAH BVH builder (product side).
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
 #include <vector>
@@ -28,5 +29,13 @@ struct BvhResult {
 // by centroid order, equal-centroid ranges become one leaf) over n
 // primitive bounds (bmin/bmax: [n][3]).  Returns false with *err on failure.
 bool build_bvh(int n, const float* bmin, const float* bmax, int max_prims, BvhResult* out, std::string* err);
+
+// The same tree built on the device (rd_bvh_gpu.hip): host bounds in, host
+// records out, on stream s.
+bool gpu_build_bvh(int n, const float* bmin, const float* bmax, int max_prims, hipStream_t s, BvhResult* out,
+                   std::string* err);
+// Device bounds in, device records out; returns the node count or -1.
+int gpu_build_bvh_device(int n, const float* d_bmin, const float* d_bmax, int max_prims, NodeRec* d_nodes,
+                         int32_t* d_order, hipStream_t s, int* max_depth, std::string* err);
 
 }  // namespace rd

@@ -25,7 +25,7 @@ EXPORTS = [
     "rd_metal_eta_k", "rd_get_tuning", "rd_set_tuning", "rd_autotune", "rd_render_device_noalloc",
     "rd_render_mean_f64", "rd_image_load", "rd_image_free", "rd_image_last_error", "rd_image_save_exr",
     "rd_image_save_png", "rd_image_save_pfm", "rd_autotune_cached", "rd_tuning_cache_save", "rd_tuning_cache_load",
-    "rd_tuning_cache_clear",
+    "rd_tuning_cache_clear", "rd_bvh_build_device",
 ]
 
 
@@ -97,6 +97,7 @@ def lib():
         L.rd_get_tuning.argtypes = [vp, C.POINTER(abi.rd_tuning)]
         L.rd_set_tuning.argtypes = [vp, C.POINTER(abi.rd_tuning)]
         L.rd_autotune.argtypes = [vp, vp, C.POINTER(abi.rd_render_params), C.POINTER(abi.rd_tuning)]
+        L.rd_bvh_build_device.argtypes = [vp, vp, vp, i32, i32, vp, i32, vp, C.POINTER(i32), C.POINTER(i32)]
         L.rd_autotune_cached.argtypes = [vp, vp, C.POINTER(abi.rd_render_params), C.POINTER(abi.rd_tuning),
                                          C.POINTER(C.c_int32)]
         L.rd_tuning_cache_save.argtypes = [C.c_char_p]
@@ -170,6 +171,35 @@ def image_save_pfm(path, img):
     """rd_image_save_pfm: float RGB, ColorBuffer row 0 first (PFM's bottom row)."""
     a = _colorbuffer(img)
     _image_check(lib().rd_image_save_pfm(os.fsencode(path), _ptr(a), a.shape[0], a.shape[1], a.shape[2]))
+
+
+def bvh_build_device(ctx, bmin, bmax, max_prim):
+    """rd_bvh_build_device from host arrays (uploaded, built, read back):
+    (nodes [N,32] uint8, order, max_depth), as bvh_build_host."""
+    bmin = np.ascontiguousarray(bmin, np.float32)
+    bmax = np.ascontiguousarray(bmax, np.float32)
+    n = bmin.shape[0]
+    cap = max(2 * n - 1, 1)
+    d_lo, d_hi = ctx.alloc(max(bmin.nbytes, 4)), ctx.alloc(max(bmax.nbytes, 4))
+    d_nodes, d_order = ctx.alloc(cap * 32), ctx.alloc(max(n, 1) * 4)
+    try:
+        if n:
+            ctx.h2d(d_lo, bmin)
+            ctx.h2d(d_hi, bmax)
+        nn = C.c_int32(0)
+        md = C.c_int32(0)
+        check(lib().rd_bvh_build_device(ctx.h, d_lo, d_hi, n, max_prim, d_nodes, cap, d_order, C.byref(nn),
+                                        C.byref(md)))
+        nodes = np.zeros(max(nn.value, 1) * 32, np.uint8)
+        order = np.zeros(max(n, 1), np.int32)
+        if nn.value:
+            ctx.d2h(nodes[: nn.value * 32], d_nodes)
+        if n:
+            ctx.d2h(order[:n], d_order)
+        return nodes[: nn.value * 32].reshape(-1, 32), order[:n], md.value
+    finally:
+        for p in (d_lo, d_hi, d_nodes, d_order):
+            ctx.free(p)
 
 
 def tuning_cache_save(path):
@@ -326,11 +356,11 @@ class Context:
         return {"trace_schedule": list(t.trace_schedule), "ray_sort": t.ray_sort, "ray_sort_bits": t.ray_sort_bits,
                 "ray_sort_min": t.ray_sort_min, "ray_sort_shadow": t.ray_sort_shadow, "overlap": t.overlap,
                 "device_loop": t.device_loop, "tail_rays": t.tail_rays, "chain_lockstep": t.chain_lockstep,
-                "pool_log2": t.pool_log2, "tile": t.tile, "stack_min": t.stack_min}
+                "pool_log2": t.pool_log2, "tile": t.tile, "stack_min": t.stack_min, "bvh_build": t.bvh_build}
 
     def set_tuning(self, trace_schedule=None, ray_sort=None, ray_sort_bits=None, ray_sort_min=None,
                    ray_sort_shadow=None, overlap=None, device_loop=None, tail_rays=None, chain_lockstep=None,
-                   pool_log2=None, tile=None, stack_min=None):
+                   pool_log2=None, tile=None, stack_min=None, bvh_build=None):
         """rd_set_tuning; arguments left None keep their value."""
         t = abi.rd_tuning()
         sched = trace_schedule if trace_schedule is not None else (-1, -1, -1)
@@ -339,7 +369,7 @@ class Context:
         for f, v in (("ray_sort", ray_sort), ("ray_sort_bits", ray_sort_bits), ("ray_sort_min", ray_sort_min),
                      ("ray_sort_shadow", ray_sort_shadow), ("overlap", overlap), ("device_loop", device_loop),
                      ("tail_rays", tail_rays), ("chain_lockstep", chain_lockstep), ("pool_log2", pool_log2),
-                     ("tile", tile), ("stack_min", stack_min)):
+                     ("tile", tile), ("stack_min", stack_min), ("bvh_build", bvh_build)):
             setattr(t, f, -1 if v is None else int(v))
         check(lib().rd_set_tuning(self.h, C.byref(t)))
 
