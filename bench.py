@@ -500,9 +500,12 @@ def main():
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
         launches_per_frame = cnt / args.steps
         pmc = pmc_stage(trace_dom, args)
-        # the counters' per-frame bytes over this run's launches per frame: the
+        # the counters' bytes of the launches this run's stage timer counts (the
+        # frame's launches before its device-driven tail) over their number: the
         # same launch partition as the algorithmic bytes per launch
-        traffic = int(pmc["fabric_bytes_per_frame"] / launches_per_frame) if pmc else None
+        traffic = None
+        if pmc and pmc.get("timed_launches_per_frame"):
+            traffic = int(pmc["fabric_bytes_timed_launches"] / pmc["timed_launches_per_frame"])
         roofline = {"kernel": trace_dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "bytes_per_launch": int(per_launch), "bytes_per_frame": int(total_bytes / args.steps),
