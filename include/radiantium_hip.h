@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define RD_ABI_VERSION 2
+#define RD_ABI_VERSION 3
 
 typedef enum rd_status {
     RD_OK = 0,
@@ -160,7 +160,16 @@ typedef struct rd_camera_desc {
     float to_world[16];           /* row-major M11..M44 (after ReadMat4)      */
 } rd_camera_desc;
 
-typedef enum rd_integrator_type { RD_INTEGRATOR_PATH = 0, RD_INTEGRATOR_VOL_PATH = 1 } rd_integrator_type;
+/* "path" (Integrators/PathTracer.cs), "vol_path" (VolumetricPathTracer.cs),
+ * "adjoint_particle" (AdjointParticleIntegrator.cs: light tracing, rendered
+ * as AdjointParticleRenderer renders it), "bdpt" (BidirectionalPathTracer.cs
+ * under BidirectionalPathTracingRenderer). */
+typedef enum rd_integrator_type {
+    RD_INTEGRATOR_PATH = 0,
+    RD_INTEGRATOR_VOL_PATH = 1,
+    RD_INTEGRATOR_PARTICLE = 2,
+    RD_INTEGRATOR_BDPT = 3
+} rd_integrator_type;
 typedef enum rd_path_method { RD_METHOD_BXDF = 0, RD_METHOD_NEE = 1, RD_METHOD_MIS = 2 } rd_path_method;
 typedef enum rd_light_strategy { RD_STRATEGY_UNIFORM = 0, RD_STRATEGY_ALL = 1 } rd_light_strategy;
 
@@ -232,6 +241,10 @@ typedef struct rd_render_params {
                                quarter of the device memory holds (every
                                path of the render where it fits)            */
     int32_t flags;          /* RD_RENDER_* bits                              */
+    int64_t particles;      /* adjoint_particle scenes: the renderer's
+                               "particle_count" (AdjointParticleRenderer;
+                               0 = its default 16384); spp and the crop are
+                               not used by that renderer                    */
 } rd_render_params;
 
 #define RD_RENDER_NO_GRAPH 1 /* launch the bounce loop kernel by kernel     */
@@ -338,9 +351,10 @@ int rd_bvh_build_device(rd_ctx* ctx, const float* d_bmin, const float* d_bmax, i
  * description the way RendererBuilder.Build builds the scene
  * (Config/RendererBuilder.cs:867-1010).  search_dir: base of relative asset
  * locations (NULL = the config's directory; RendererBuilder.GetPath).
- * params (may be NULL) receives the renderer block: spp (default 256).
- * Images load as rd_image_load loads them.  Integrators other than "path" /
- * "vol_path" give RD_ERR_UNSUPPORTED. */
+ * params (may be NULL) receives the renderer block: spp (default 256) and
+ * particle_count (particles, default 16384).  Images load as rd_image_load
+ * loads them.  Integrators "path", "vol_path", "adjoint_particle" and "bdpt"
+ * load; others ("ao", "gbuffer") give RD_ERR_UNSUPPORTED. */
 typedef struct rd_json_scene rd_json_scene;
 int rd_json_load(const char* path, const char* search_dir, rd_json_scene** out, rd_render_params* params);
 const rd_scene_desc* rd_json_scene_desc(const rd_json_scene* scene);

@@ -9,6 +9,7 @@
 #include "cpu_ref.h"
 #include "cpu_ref_math.h"
 
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cstdio>
@@ -166,6 +167,8 @@ struct Camera {
     float near_z, far_z;
     float inv_res_x, inv_res_y;
     M4 clip_to_camera, camera_to_world;
+    M4 camera_to_clip, world_to_camera;
+    float min_x, min_y, max_x, max_y, normalization;   // image plane at z = 1 (Camera.cs:172-179)
     int W, H;
     bool init(const rd_camera_desc& c) {
         W = c.width; H = c.height;
@@ -200,9 +203,51 @@ struct Camera {
         M4 p = m4_mul(m4_mul(persp, trans), scale);
         if (!m4_invert(p, &clip_to_camera)) return false;
         camera_to_world = c2w;
+        camera_to_clip = p;
         M4 w2c;
         if (!m4_invert(c2w, &w2c)) return false;
+        world_to_camera = w2c;
+        V3 pmin = xform_point(V3(0, 0, 0), clip_to_camera);
+        V3 pmax = xform_point(V3(1, 1, 0), clip_to_camera);
+        pmin = pmin / pmin.z;
+        pmax = pmax / pmax.z;
+        min_x = pmax.x; min_y = pmax.y;
+        max_x = pmin.x; max_y = pmin.y;
+        float area = std::fabs((max_x - min_x) * (max_y - min_y));
+        normalization = 1 / area;
         return true;
+    }
+    // Importance (Camera.cs:193-204): d normalised, camera space
+    float importance(V3 d) const {
+        float ct = d.z;
+        if (ct <= 0) return 0;
+        float focus = 1 / ct;
+        float px = d.x * focus, py = d.y * focus;
+        if (px < min_x || px > max_x || py < min_y || py > max_y) return 0.0f;
+        return normalization * focus * focus * focus;
+    }
+    // SampleWi (Camera.cs:221-239); ok = false for the default (pdf 0) result
+    struct Wi { V3 pos, dir; float dist = 0, pdf = 0, we = 0, sx = 0, sy = 0; bool ok = false; };
+    Wi sample_wi(V3 pos) const {
+        Wi r;
+        V3 lt = xform_point(pos, world_to_camera);
+        if (lt.z < near_z || lt.z > far_z) return r;
+        V3 screen = xform_point(lt, camera_to_clip);
+        screen.x /= screen.z;
+        screen.y /= screen.z;
+        if (screen.x < 0 || screen.x > 1 || screen.y < 0 || screen.y > 1) return r;
+        r.sx = screen.x * (float)(W - 1);
+        r.sy = screen.y * (float)(H - 1);
+        float dist = length(lt);
+        float inv = 1 / dist;
+        V3 local_dir = lt * inv;
+        r.pos = xform_point(V3(0, 0, 0), camera_to_world);
+        r.dir = (r.pos - pos) / dist;
+        r.dist = dist;
+        r.pdf = 1;
+        r.we = importance(local_dir) * inv * inv;
+        r.ok = true;
+        return r;
     }
     // SampleRay (Camera.cs:182-191)
     Ray sample_ray(float sx, float sy) const {
@@ -1122,7 +1167,8 @@ static C3 disney_inner_brdf(const Disney& b, V3 wo, V3 wi) {
                               [&](float c) { return C3(fresnel_dielectric(c, 1, b.eta)); }, wo, wi);
 }
 // MicrofacetTransmissionBtdf<Gtr2> (Bxdf/MicrofacetTransmissionBtdf.cs:25-77), etaA = 1, etaB = Eta
-static C3 disney_btdf_fr(const Disney& b, V3 wo, V3 wi) {
+// imp: TransportMode.Importance (no 1/eta radiance scaling, MicrofacetTransmissionBtdf.cs:40)
+static C3 disney_btdf_fr(const Disney& b, V3 wo, V3 wi, bool imp = false) {
     if (same_hemisphere(wo, wi)) return C3();
     float cto = cos_theta(wo), cti = cos_theta(wi);
     if (cti == 0 || cto == 0) return C3();
@@ -1135,7 +1181,7 @@ static C3 disney_btdf_fr(const Disney& b, V3 wo, V3 wi) {
     if (dot(wo, wh) * dot(wi, wh) > 0) return C3();
     C3 f(fresnel_dielectric(dot(wo, wh), 1.0f, b.eta));
     float sd = dot(wo, wh) + eta * dot(wi, wh);
-    float factor = 1 / eta;
+    float factor = imp ? 1.0f : 1 / eta;
     float D = d.D(wh), G = d.G(wo, wi);
     C3 T = (1 - b.metallic) * b.transmission * csqrt(b.color);
     return (1 - f) * T * std::fabs(D * G * eta * eta * abs_dot(wi, wh) * abs_dot(wo, wh) * factor * factor /
@@ -1172,10 +1218,10 @@ static float disney_clearcoat_pdf(const Disney& b, V3 wo, V3 wi) {
     float D = gtr1(abs_cos_theta(wh), gloss);
     return D * abs_cos_theta(wh) / (4 * dot(wo, wh));
 }
-static C3 disney_fr(const Disney& b, V3 wo, V3 wi) {                                      // DisneyBsdf.FrImpl
+static C3 disney_fr(const Disney& b, V3 wo, V3 wi, bool imp = false) {                                      // DisneyBsdf.FrImpl
     if (!same_hemisphere(wo, wi)) {
         if (b.transmission == 0) return C3(0.0f);
-        return disney_btdf_fr(b, wo, wi);
+        return disney_btdf_fr(b, wo, wi, imp);
     }
     if (cos_theta(wo) < 0 && cos_theta(wi) < 0) return disney_inner_brdf(b, wo, wi);
     C3 diffuse(0.0f);
@@ -1272,7 +1318,9 @@ static Disney load_disney(const cr_scene& s, const rd_material_desc& m, V2 uv) {
     return b;
 }
 
-static C3 material_fr(const cr_scene& s, const rd_material_desc& m, V3 wo, V3 wi, V2 uv) {
+// imp: TransportMode.Importance (light paths: RoughGlass.cs:63, FresnelSpecularBsdf.cs:58,
+// MicrofacetTransmissionBtdf.cs:40 drop the radiance scaling of refraction)
+static C3 material_fr(const cr_scene& s, const rd_material_desc& m, V3 wo, V3 wi, V2 uv, bool imp = false) {
     switch (m.type) {
     case RD_MAT_ROUGH_PLASTIC: {
         MfDist d(m.dist, tex_at(s, m.tex[1], uv).r, tex_at(s, m.tex[2], uv).r);
@@ -1282,7 +1330,7 @@ static C3 material_fr(const cr_scene& s, const rd_material_desc& m, V3 wo, V3 wi
     case RD_MAT_PERFECT_MIRROR:    // SpecularReflectionBrdf.Fr (SpecularReflectionBrdf.cs:17-20)
         return C3(0.0f);
     case RD_MAT_DISNEY:
-        return disney_fr(load_disney(s, m, uv), wo, wi);
+        return disney_fr(load_disney(s, m, uv), wo, wi, imp);
     case RD_MAT_DIFFUSE: {                                           // Materials/Diffuse.cs:21-27
         if (cos_theta(wo) <= 0 || cos_theta(wi) <= 0) { if (!m.two_sided) return C3(0.0f); }
         C3 R = tex_at(s, m.tex[0], uv);                              // LambertianReflectionBrdf.Fr :16-19
@@ -1314,7 +1362,7 @@ static C3 material_fr(const cr_scene& s, const rd_material_desc& m, V3 wo, V3 wi
             bsdf = r * (D * G * F / (4.0f * cos_theta(wi) * cos_theta(wo)));
         } else {
             if (dot(wo, wh) * dot(wi, wh) > 0) return C3();
-            float factor = 1 / eta;   // TransportMode.Radiance
+            float factor = imp ? 1.0f : 1 / eta;   // RoughGlass.cs:63
             bsdf = (1 - F) * t *
                    std::fabs(D * G * eta * eta * abs_dot(wi, wh) * abs_dot(wo, wh) * factor * factor /
                              (cos_theta(wo) * cos_theta(wi) * sqrt_denom * sqrt_denom));
@@ -1404,14 +1452,14 @@ static bool mf_reflect_sample_dir(V3 wo, Rng& rng, SampleWh swh, V3* wi_out, V3*
     return true;
 }
 
-static BSample disney_sample(const Disney& b, V3 wo, Rng& rng) {                         // DisneyBsdf.Sample
+static BSample disney_sample(const Disney& b, V3 wo, Rng& rng, bool imp = false) {                         // DisneyBsdf.Sample
     BSample out;
     Gtr2 sd = spec_dist(b), td = trans_dist(b);
     auto wh_spec = [&](V3 w, Rng& r) { return sd.m.sample_wh(w, r); };
     auto wh_trans = [&](V3 w, Rng& r) { return td.m.sample_wh(w, r); };
     auto finish = [&](V3 wi, int type) {
         out.wi = wi;
-        out.fr = disney_fr(b, wo, wi);
+        out.fr = disney_fr(b, wo, wi, imp);
         out.pdf = disney_pdf(b, wo, wi);
         out.type = type;
         return out;
@@ -1425,7 +1473,7 @@ static BSample disney_sample(const Disney& b, V3 wo, Rng& rng) {                
         float ei = entering ? 1.0f : b.eta, et = entering ? b.eta : 1.0f;
         V3 wi;
         if (!refract(wo, wh, ei / et, &wi)) return false;
-        C3 f = disney_btdf_fr(b, wo, wi);
+        C3 f = disney_btdf_fr(b, wo, wi, imp);
         float p = disney_btdf_pdf(b, wo, wi);
         if (!f.valid() || std::isinf(p)) return false;
         if (p == 0) return false;
@@ -1484,7 +1532,7 @@ static BSample disney_sample(const Disney& b, V3 wo, Rng& rng) {                
     return finish(wi, T_REFLECTION | T_GLOSSY);
 }
 
-static BSample material_sample(const cr_scene& s, const rd_material_desc& m, V3 wo, V2 uv, Rng& rng) {
+static BSample material_sample(const cr_scene& s, const rd_material_desc& m, V3 wo, V2 uv, Rng& rng, bool imp = false) {
     BSample out;
     switch (m.type) {
     case RD_MAT_ROUGH_PLASTIC: {                                     // RoughPlastic.SampleImpl (:134-151)
@@ -1524,7 +1572,7 @@ static BSample material_sample(const cr_scene& s, const rd_material_desc& m, V3 
         V3 wi;
         if (!refract(wo, n, ei / et, &wi) || cos_theta(wi) == 0.0f) return out;
         C3 ft = T * (1.0f - f);
-        ft = ft * ((ei * ei) / (et * et));   // TransportMode.Radiance
+        if (!imp) ft = ft * ((ei * ei) / (et * et));   // TransportMode.Radiance only (:58-61)
         ft = ft / abs_cos_theta(wi);
         out.wi = wi; out.fr = ft; out.pdf = 1 - f; out.type = T_TRANSMISSION | T_SPECULAR;
         return out;
@@ -1538,7 +1586,7 @@ static BSample material_sample(const cr_scene& s, const rd_material_desc& m, V3 
         return out;
     }
     case RD_MAT_DISNEY:
-        return disney_sample(load_disney(s, m, uv), wo, rng);
+        return disney_sample(load_disney(s, m, uv), wo, rng, imp);
     case RD_MAT_DIFFUSE: {                                           // Diffuse.cs:39-49
         if (cos_theta(wo) <= 0) { if (!m.two_sided) return out; }
         C3 R = tex_at(s, m.tex[0], uv);
@@ -1598,7 +1646,7 @@ static BSample material_sample(const cr_scene& s, const rd_material_desc& m, V3 
             eta = 1;
         }
         float sd = dot(wo, wh) + eta * dot(wi, wh);
-        float factor = 1 / eta;
+        float factor = imp ? 1.0f : 1 / eta;   // RoughGlass.cs:161
         float D = d.D(wh);
         float G = d.G(wo, wi);
         C3 btdf = (1 - F) * t *
@@ -2443,6 +2491,125 @@ static C3 pixel_sample(const cr_scene& s, uint32_t seed, int x, int y, int t, ui
     return li(s, ray, rng, st);
 }
 
+// ---------------------------------------------------------------- light tracing
+// Light.SampleEmit: DiffuseAreaLight.SampleEmit (DiffuseAreaLight.cs:70-77) and
+// InfiniteAreaLight.SampleEmit (InfiniteAreaLight.cs:122-133).  Returns false
+// when the emission pdf is 0 (the environment map's SampleLightDir failing):
+// the reference then divides by that pdf; the particle ends here instead.
+struct Emit { V3 p, dir, n; C3 le; float pdf_pos = 0, pdf_dir = 0; };
+// Probability.SquareToConcentricDisk (Probability.cs:157-174)
+static inline void concentric_disk(float u0, float u1, float* dx, float* dy) {
+    float ox = 2.0f * u0 - 1, oy = 2.0f * u1 - 1;
+    if (ox == 0 && oy == 0) { *dx = 0; *dy = 0; return; }
+    float theta, r;
+    if (std::fabs(ox) > std::fabs(oy)) { r = ox; theta = (PI_F / 4) * (oy / ox); }
+    else { r = oy; theta = (PI_F / 2) - (PI_F / 4) * (ox / oy); }
+    *dx = r * std::cos(theta);
+    *dy = r * std::sin(theta);
+}
+static bool light_sample_emit(const cr_scene& s, int light, Rng& rng, Emit* e) {
+    const rd_light_desc& L = s.lights[light];
+    if (L.type == RD_LIGHT_INFINITE) {
+        const EnvLight& env = s.env[s.env_of_light[light]];
+        float u0 = rng.next_float(), u1 = rng.next_float();
+        V3 wdir;
+        float pdf;
+        V2 uv;
+        if (!env_sample_dir(env, u0, u1, &wdir, &pdf, &uv)) return false;
+        e->le = s.tex[env.tex].sample(uv);
+        V3 normal = -wdir;
+        Frame coord(normal);
+        float c0 = rng.next_float(), c1 = rng.next_float();
+        float cx, cy;
+        concentric_disk(c0, c1, &cx, &cy);
+        V3 disk = env.center + env.radius * (cx * coord.X + cy * coord.Y);
+        e->p = disk + wdir * env.radius;
+        e->dir = -wdir;
+        e->n = -wdir;
+        e->pdf_pos = 1 / (PI_F * sqr(env.radius));
+        e->pdf_dir = pdf;
+        return true;
+    }
+    V3 P, N;
+    float pdf_pos;
+    shape_sample(s, L.prim, rng, &P, &N, &pdf_pos);
+    float u0 = rng.next_float(), u1 = rng.next_float();
+    V3 lw = normalize(square_to_cosine_hemisphere(u0, u1));
+    float pdf_dir = cos_theta(lw) / PI_F;
+    V3 w = Frame(N).to_world(lw);
+    e->p = P;
+    e->dir = normalize(w);
+    e->n = N;
+    e->le = C3(L.le[0], L.le[1], L.le[2]);
+    e->pdf_pos = pdf_pos;
+    e->pdf_dir = pdf_dir;
+    return true;
+}
+
+// AdjointParticleIntegrator.EmitParticle (AdjointParticleIntegrator.cs:24-80):
+// splats into acc (float64, ColorBuffer layout) at the pixel the camera sees
+// each surface vertex in.
+static void emit_particle(const cr_scene& s, uint32_t seed, uint64_t index, double* acc, uint64_t* st) {
+    const rd_integrator_desc& I = s.integ;
+    Rng rng;
+    rng.key = particle_key(seed, index);
+    rng.n = 0;
+    if (st) st[2]++;
+    if (s.lights.empty()) return;                                   // Scene.SampleLight: pdf 0
+    int light = rng.next_int((int)s.lights.size());
+    float select_pdf = 1.0f / s.lights.size();
+    Emit e;
+    if (!light_sample_emit(s, light, rng, &e)) return;
+    if (e.le.black()) return;
+    C3 coeff = e.le * abs_dot(e.dir, e.n) / (select_pdf * e.pdf_pos * e.pdf_dir);
+    Ray ray(e.p, e.dir, 0.0001f);
+    const int H = s.cam.H;
+    for (int bounces = 1;; bounces++) {
+        if (I.max_depth != -1 && bounces >= I.max_depth) break;
+        if (bounces > I.min_depth) {
+            if (rng.next_float() > I.rr_threshold) break;
+            coeff = coeff / I.rr_threshold;
+        }
+        Inct it;
+        if (!scene_intersect(s, ray, &it, st)) break;
+        int mat = s.prims[it.prim].material;
+        if (mat < 0) {
+            ray = Ray(it.P, ray.d, ray.tmin);
+            bounces--;
+            continue;
+        }
+        if (st) st[3]++;
+        const rd_material_desc& m = s.mats[mat];
+        Camera::Wi cw = s.cam.sample_wi(it.P);
+        if (cw.pdf > 0 && cw.we != 0) {
+            if (!scene_is_occluded(s, cw.pos, it.P, st)) {
+                V3 wi_l = it.sh.to_local(cw.dir);
+                C3 f = material_fr(s, m, it.sh.to_local(it.wr), wi_l, it.UV, true);
+                float ac = abs_cos_theta(wi_l);
+                C3 color = coeff * f * C3(cw.we) * ac / cw.pdf;
+                int x = (int)std::floor(cw.sx), y = (int)std::floor(cw.sy);
+                double* px = acc + ((size_t)x * H + y) * 3;
+                px[0] += (double)color.r;
+                px[1] += (double)color.g;
+                px[2] += (double)color.b;
+            }
+        }
+        BSample bs = material_sample(s, m, it.sh.to_local(it.wr), it.UV, rng, true);
+        if (bs.pdf > 0.0f) coeff = coeff * (bs.fr * abs_cos_theta(bs.wi) / bs.pdf);
+        else break;
+        ray = spawn_ray(it, it.sh.to_world(bs.wi));
+    }
+}
+
+// AdjointParticleIntegrator.CameraTraceLight (:82-85) for the ray through
+// pixel corner (x, y) (AdjointParticleRenderer.AfterRender, :170-176).
+static C3 camera_trace_light(const cr_scene& s, int x, int y, uint64_t* st) {
+    Ray ray = s.cam.sample_ray((float)x, (float)y);
+    Inct it;
+    if (scene_intersect(s, ray, &it, st)) return inct_le(s, it, -ray.d);
+    return eval_infinite(s, ray);
+}
+
 // ---------------------------------------------------------------- C API
 extern "C" {
 
@@ -2563,7 +2730,7 @@ cr_scene* cr_scene_create(const rd_scene_desc* d) {
         int p = L.prim;
         if (p < 0 || p >= d->num_prims) { set_err("light prim index"); delete s; return nullptr; }
     }
-    if (d->integrator.type != RD_INTEGRATOR_PATH && d->integrator.type != RD_INTEGRATOR_VOL_PATH) {
+    if (d->integrator.type < RD_INTEGRATOR_PATH || d->integrator.type > RD_INTEGRATOR_BDPT) {
         set_err("unknown integrator type"); delete s; return nullptr;
     }
     for (int i = 0; i < d->num_media; i++) s->homo.push_back(make_homo(d->media[i]));
@@ -2760,6 +2927,77 @@ int cr_render(const cr_scene* s, const rd_render_params* p, int32_t threads, flo
     return RD_OK;
 }
 
+// AdjointParticleRenderer (Renderers/AdjointParticleRenderer.cs:127-178):
+// `particles` particles split over the workers, each worker summing its
+// splats in float64; the sums are added in worker order, rounded to float,
+// scaled by W*H / particles and the directly seen emission added.
+int cr_render_particles(const cr_scene* s, uint64_t particles, uint32_t seed, int32_t threads, float* out_rgb,
+                        uint64_t* stats, double* seconds) {
+    if (!s || !out_rgb || particles == 0) { set_err("bad particle render arguments"); return RD_ERR_INVALID; }
+    const int W = s->cam.W, H = s->cam.H;
+    const size_t n = (size_t)W * H * 3;
+    if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
+    std::vector<std::vector<double>> acc(threads, std::vector<double>(n, 0.0));
+    std::vector<std::array<uint64_t, 5>> sts(threads);
+    auto t0 = std::chrono::steady_clock::now();
+    const uint64_t chunk = (particles + threads - 1) / threads;
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; t++) {
+        pool.emplace_back([&, t]() {
+            uint64_t st[5] = {0, 0, 0, 0, 0};
+            uint64_t b = (uint64_t)t * chunk, e = std::min(particles, b + chunk);
+            for (uint64_t i = b; i < e; i++) emit_particle(*s, seed, i, acc[t].data(), st);
+            for (int k = 0; k < 5; k++) sts[t][k] = st[k];
+        });
+    }
+    for (auto& th : pool) th.join();
+    uint64_t st[5] = {0, 0, 0, 0, 0};
+    for (int t = 0; t < threads; t++) for (int k = 0; k < 5; k++) st[k] += sts[t][k];
+    const float v = ((float)W * (float)H) / (float)particles;
+    for (int x = 0; x < W; x++) {
+        for (int y = 0; y < H; y++) {
+            size_t o = ((size_t)x * H + y) * 3;
+            double sr = 0, sg = 0, sb = 0;
+            for (int t = 0; t < threads; t++) { sr += acc[t][o]; sg += acc[t][o + 1]; sb += acc[t][o + 2]; }
+            C3 c((float)sr, (float)sg, (float)sb);
+            c = c * v;
+            c = c + camera_trace_light(*s, x, y, st);
+            out_rgb[o] = c.r; out_rgb[o + 1] = c.g; out_rgb[o + 2] = c.b;
+        }
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (stats) for (int k = 0; k < 5; k++) stats[k] = st[k];
+    return RD_OK;
+}
+
+// Camera probes: SampleWi of a world point {ok, sx, sy, we, pdf, dist} and
+// Importance of a camera-space direction.
+int cr_camera_sample_wi(const cr_scene* s, const float* pos, float* out6) {
+    if (!s || !pos || !out6) return RD_ERR_INVALID;
+    Camera::Wi w = s->cam.sample_wi(V3(pos[0], pos[1], pos[2]));
+    out6[0] = w.ok ? 1.0f : 0.0f; out6[1] = w.sx; out6[2] = w.sy; out6[3] = w.we; out6[4] = w.pdf; out6[5] = w.dist;
+    return RD_OK;
+}
+float cr_camera_importance(const cr_scene* s, const float* d) {
+    return s ? s->cam.importance(V3(d[0], d[1], d[2])) : 0.0f;
+}
+// Light.SampleEmit probe for light `light` with RNG key `key`: out =
+// {p.xyz, dir.xyz, n.xyz, le.rgb, pdf_pos, pdf_dir}; returns 0 when the
+// emission pdf is 0.
+int cr_light_sample_emit(const cr_scene* s, int32_t light, uint64_t key, float* out14) {
+    if (!s || light < 0 || light >= (int)s->lights.size()) return -1;
+    Rng rng;
+    rng.key = key;
+    rng.n = 0;
+    Emit e;
+    if (!light_sample_emit(*s, light, rng, &e)) return 0;
+    const float v[14] = {e.p.x, e.p.y, e.p.z, e.dir.x, e.dir.y, e.dir.z, e.n.x, e.n.y, e.n.z,
+                         e.le.r, e.le.g, e.le.b, e.pdf_pos, e.pdf_dir};
+    for (int i = 0; i < 14; i++) out14[i] = v[i];
+    return 1;
+}
+
 int cr_adjoint(const cr_scene* s, const rd_render_params* p, int32_t threads, const float* dl_dimage,
                float* const* tex_grads) {
     if (!s || !p || !dl_dimage || !tex_grads || p->spp <= 0) return RD_ERR_INVALID;
@@ -2922,6 +3160,27 @@ int cr_material_sample(const cr_scene* s, int32_t material, const float* wo, con
     if (!s || material < 0 || material >= (int)s->mats.size()) return RD_ERR_INVALID;
     Rng rng; rng.key = key; rng.n = 0;
     BSample b = material_sample(*s, s->mats[material], V3(wo[0], wo[1], wo[2]), V2(uv[0], uv[1]), rng);
+    wi[0] = b.wi.x; wi[1] = b.wi.y; wi[2] = b.wi.z;
+    fr_rgb[0] = b.fr.r; fr_rgb[1] = b.fr.g; fr_rgb[2] = b.fr.b;
+    *pdf = b.pdf;
+    *type = b.type;
+    return RD_OK;
+}
+
+// The two probes above under TransportMode.Importance (imp = 1) or Radiance (0).
+int cr_material_eval_mode(const cr_scene* s, int32_t material, int32_t imp, const float* wo, const float* wi,
+                          const float* uv, float* fr_rgb) {
+    if (!s || material < 0 || material >= (int)s->mats.size()) return RD_ERR_INVALID;
+    C3 f = material_fr(*s, s->mats[material], V3(wo[0], wo[1], wo[2]), V3(wi[0], wi[1], wi[2]), V2(uv[0], uv[1]),
+                       imp != 0);
+    fr_rgb[0] = f.r; fr_rgb[1] = f.g; fr_rgb[2] = f.b;
+    return RD_OK;
+}
+int cr_material_sample_mode(const cr_scene* s, int32_t material, int32_t imp, const float* wo, const float* uv,
+                            uint64_t key, float* wi, float* fr_rgb, float* pdf, int32_t* type) {
+    if (!s || material < 0 || material >= (int)s->mats.size()) return RD_ERR_INVALID;
+    Rng rng; rng.key = key; rng.n = 0;
+    BSample b = material_sample(*s, s->mats[material], V3(wo[0], wo[1], wo[2]), V2(uv[0], uv[1]), rng, imp != 0);
     wi[0] = b.wi.x; wi[1] = b.wi.y; wi[2] = b.wi.z;
     fr_rgb[0] = b.fr.r; fr_rgb[1] = b.fr.g; fr_rgb[2] = b.fr.b;
     *pdf = b.pdf;

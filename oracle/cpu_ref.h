@@ -103,6 +103,25 @@ void cr_mf_sample(int32_t kind, float roughness, float aniso, const float* wo, u
 void cr_rng_floats(uint64_t key, int32_t n, float* out);
 uint64_t cr_path_key(uint32_t seed, uint32_t pixel, uint32_t sample);
 
+/* Light tracing (AdjointParticleIntegrator / AdjointParticleRenderer):
+ * `particles` particles with keys particle_key(seed, i); out_rgb W*H*3
+ * column-major; stats as cr_render ([4] unused). */
+int cr_render_particles(const cr_scene* s, uint64_t particles, uint32_t seed, int32_t threads, float* out_rgb,
+                        uint64_t* stats, double* seconds);
+/* Camera.SampleWi of a world point: out6 = {ok, screen x, screen y, We,
+ * pdf, distance}; Camera.Importance of a camera-space unit direction. */
+int cr_camera_sample_wi(const cr_scene* s, const float* pos, float* out6);
+float cr_camera_importance(const cr_scene* s, const float* d);
+/* Light.SampleEmit of light `light` drawn with RNG key `key`: out14 =
+ * {p, dir, n, le, pdf_pos, pdf_dir}; 1 on success, 0 when the pdf is 0. */
+int cr_light_sample_emit(const cr_scene* s, int32_t light, uint64_t key, float* out14);
+/* Material probes under a transport mode: imp = 1 TransportMode.Importance,
+ * 0 Radiance (cr_material_eval / cr_material_sample). */
+int cr_material_eval_mode(const cr_scene* s, int32_t material, int32_t imp, const float* wo, const float* wi,
+                          const float* uv, float* fr_rgb);
+int cr_material_sample_mode(const cr_scene* s, int32_t material, int32_t imp, const float* wo, const float* uv,
+                            uint64_t key, float* wi, float* fr_rgb, float* pdf, int32_t* type);
+
 #ifdef __cplusplus
 }
 #endif

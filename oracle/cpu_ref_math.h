@@ -175,6 +175,12 @@ inline uint64_t path_key(uint32_t seed, uint32_t pixel, uint32_t sample) {
     uint64_t k0 = ((uint64_t)seed << 32) | (uint64_t)sample;
     return mix64(k0 ^ mix64((uint64_t)pixel + 0x9E3779B97F4A7C15ull));
 }
+// Light-path key: a particle's (or a bidirectional sample's light subpath's)
+// 64-bit index over the pixel and sample words, the sample word's top bit
+// set so that no camera path (sample < 2^31) draws from the same stream.
+inline uint64_t particle_key(uint32_t seed, uint64_t index) {
+    return path_key(seed, (uint32_t)index, 0x80000000u | (uint32_t)(index >> 32));
+}
 struct Rng {
     uint64_t key;
     uint32_t n;

@@ -78,6 +78,16 @@ def lib():
         L.cr_rho_ratio.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.cr_path_key.restype = C.c_uint64
         L.cr_path_key.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+        L.cr_render_particles.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_int32, C.c_void_p, C.c_void_p,
+                                          C.c_void_p]
+        L.cr_camera_sample_wi.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.cr_camera_importance.restype = C.c_float
+        L.cr_camera_importance.argtypes = [C.c_void_p, C.c_void_p]
+        L.cr_light_sample_emit.argtypes = [C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p]
+        L.cr_material_eval_mode.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p]
+        L.cr_material_sample_mode.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_uint64,
+                                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -274,6 +284,50 @@ class OracleScene:
         stats = dict(rays_closest=int(st[0]), rays_any=int(st[1]), paths=int(st[2]), vertices=int(st[3]),
                      invalid=int(st[4]))
         return rgb, s2, stats, secs.value
+
+    def render_particles(self, particles, seed=0, threads=0):
+        """Light tracing (adjoint_particle): (rgb [W*H*3] column-major, stats dict, seconds)."""
+        rgb = np.zeros(self.width * self.height * 3, np.float32)
+        st = np.zeros(5, np.uint64)
+        secs = C.c_double(0)
+        rc = lib().cr_render_particles(self.h, C.c_uint64(particles), seed, threads, _ptr(rgb), _ptr(st),
+                                       C.byref(secs))
+        if rc != 0:
+            raise RuntimeError("oracle particle render: " + lib().cr_last_error().decode())
+        stats = dict(rays_closest=int(st[0]), rays_any=int(st[1]), paths=int(st[2]), vertices=int(st[3]))
+        return rgb, stats, secs.value
+
+    def camera_sample_wi(self, pos):
+        """Camera.SampleWi: None for the pdf-0 result, else (screen x, screen y, We, pdf, distance)."""
+        pos = np.asarray(pos, np.float32); out = np.zeros(6, np.float32)
+        lib().cr_camera_sample_wi(self.h, _ptr(pos), _ptr(out))
+        return None if out[0] == 0 else (float(out[1]), float(out[2]), float(out[3]), float(out[4]), float(out[5]))
+
+    def camera_importance(self, d):
+        d = np.asarray(d, np.float32)
+        return float(lib().cr_camera_importance(self.h, _ptr(d)))
+
+    def light_sample_emit(self, light, key):
+        """Light.SampleEmit: dict(p, dir, n, le, pdf_pos, pdf_dir) or None (pdf 0)."""
+        out = np.zeros(14, np.float32)
+        if lib().cr_light_sample_emit(self.h, light, C.c_uint64(key), _ptr(out)) != 1:
+            return None
+        return dict(p=out[0:3].copy(), dir=out[3:6].copy(), n=out[6:9].copy(), le=out[9:12].copy(),
+                    pdf_pos=float(out[12]), pdf_dir=float(out[13]))
+
+    def material_eval_mode(self, mat, wo, wi, importance, uv=(0.0, 0.0)):
+        wo = np.asarray(wo, np.float32); wi = np.asarray(wi, np.float32); uv = np.asarray(uv, np.float32)
+        fr = np.zeros(3, np.float32)
+        lib().cr_material_eval_mode(self.h, mat, 1 if importance else 0, _ptr(wo), _ptr(wi), _ptr(uv), _ptr(fr))
+        return fr
+
+    def material_sample_mode(self, mat, wo, key, importance, uv=(0.0, 0.0)):
+        wo = np.asarray(wo, np.float32); uv = np.asarray(uv, np.float32)
+        wi = np.zeros(3, np.float32); fr = np.zeros(3, np.float32); pdf = np.zeros(1, np.float32)
+        ty = np.zeros(1, np.int32)
+        lib().cr_material_sample_mode(self.h, mat, 1 if importance else 0, _ptr(wo), _ptr(uv), C.c_uint64(key),
+                                      _ptr(wi), _ptr(fr), _ptr(pdf), _ptr(ty))
+        return wi, fr, float(pdf[0]), int(ty[0])
 
     def sample_li(self, seed, x, y, sample):
         out = np.zeros(3, np.float32)

@@ -5,7 +5,7 @@ checks sizes and offsets against the compiled library.
 """
 import ctypes as C
 
-ABI_VERSION = 2   # RD_ABI_VERSION
+ABI_VERSION = 3   # RD_ABI_VERSION
 RD_OK = 0
 RD_ERR_INVALID = 1
 RD_ERR_DEVICE = 2
@@ -43,6 +43,8 @@ PRIM_SPHERE = 1
 PRIM_INSTANCE = 2
 INTEGRATOR_PATH = 0
 INTEGRATOR_VOL_PATH = 1
+INTEGRATOR_PARTICLE = 2
+INTEGRATOR_BDPT = 3
 METHOD_BXDF = 0
 METHOD_NEE = 1
 METHOD_MIS = 2
@@ -141,6 +143,7 @@ class rd_render_params(C.Structure):
     _fields_ = [
         ("spp", C.c_int32), ("seed", C.c_uint32), ("x0", C.c_int32), ("y0", C.c_int32),
         ("x1", C.c_int32), ("y1", C.c_int32), ("wave_paths", C.c_int32), ("flags", C.c_int32),
+        ("particles", C.c_int64),
     ]
 
 

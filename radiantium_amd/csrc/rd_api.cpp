@@ -175,6 +175,7 @@ struct rd_ctx {
 struct rd_scene {
     rd_ctx* ctx = nullptr;
     DevScene dev{};
+    LtCamera lt_cam{};                  // SampleWi / Importance state (light paths)
     DevIntegrator integ{};
     std::vector<void*> allocs;
     std::vector<NodeRec> nodes;
@@ -488,6 +489,19 @@ int ensure_batch_buffers(rd_ctx* c, size_t nscenes) {
     return RD_OK;
 }
 
+// Device buffers of one render that are freed when it returns.
+struct ScopedAllocs {
+    std::vector<void*> ptrs;
+    ~ScopedAllocs() { for (void* p : ptrs) (void)hipFree(p); }
+    template <class T>
+    T* get(size_t count) {
+        void* p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return nullptr;
+        ptrs.push_back(p);
+        return (T*)p;
+    }
+};
+
 int check_stack(const rd_scene* s) {
     if (s->dev.stack_needed > 64)
         return fail(RD_ERR_STACK, "BVH deeper than the 64-entry traversal stack (Bvh.StackSize)");
@@ -608,7 +622,7 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     if (d->num_spheres > 0 && !d->spheres) return bad("spheres without data");
     if (d->num_prims > 0 && !d->prims) return bad("prims missing");
     if (d->camera.width <= 0 || d->camera.height <= 0) return bad("camera screen must be positive");
-    if (d->integrator.type != RD_INTEGRATOR_PATH && d->integrator.type != RD_INTEGRATOR_VOL_PATH)
+    if (d->integrator.type < RD_INTEGRATOR_PATH || d->integrator.type > RD_INTEGRATOR_BDPT)
         return bad("unknown integrator type");
     if (d->global_medium < -1 || d->global_medium >= d->num_media) return bad("global medium out of range");
     if (d->num_instance_sets < 0 || d->num_instances < 0) return bad("negative count");
@@ -938,7 +952,7 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     }
 
     DevScene& ds = s->dev;
-    if (!setup_camera(d->camera, &ds.cam)) { delete s; return bad("camera matrix not invertible"); }
+    if (!setup_camera(d->camera, &ds.cam, &s->lt_cam)) { delete s; return bad("camera matrix not invertible"); }
     ds.num_nodes = (int)(nodes.size() / 2);
     ds.num_prims = np;
     // a set's traversal runs on top of the scene tree's stack entries
@@ -1601,6 +1615,10 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
         if (!r.scene || (!r.out_rgb && !(single && out64))) return fail(RD_ERR_INVALID, "request without scene or output");
         if (r.scene->ctx != c) return fail(RD_ERR_INVALID, "request scene belongs to another context");
         if (r.params.spp <= 0) return fail(RD_ERR_INVALID, "spp must be positive");
+        const int it = r.scene->dev.integ.type;
+        if (it != RD_INTEGRATOR_PATH && it != RD_INTEGRATOR_VOL_PATH)
+            return fail(RD_ERR_UNSUPPORTED, single ? "this entry point renders path and vol_path scenes"
+                                                   : "batched rendering covers path and vol_path scenes");
         if (int rc = check_stack(r.scene)) return rc;
         if (int rc = resolve_crop(r.scene, &r.params, &crops[i])) return rc;
     }
@@ -1773,11 +1791,117 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
     return rc;
 }
 
+// AdjointParticleRenderer (Renderers/AdjointParticleRenderer.cs:127-178) on
+// the device: waves of particles (particle i draws from
+// particle_key(seed, i), so the image does not depend on the wave size),
+// each wave bounced until no particle is left, the splats summed in a
+// float64 image; then RefRGB *= W*H / particles and the directly seen
+// emission of each pixel corner is added.
+int render_particles(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d_out, volatile int32_t* cancel,
+                     volatile int32_t* progress) {
+    if (int rc = check_stack(s)) return rc;
+    if (p->x0 || p->y0 || p->x1 || p->y1) {
+        Crop cr;
+        if (int rc = resolve_crop(s, p, &cr)) return rc;
+        if (cr.x0 != 0 || cr.y0 != 0 || cr.x1 != s->width || cr.y1 != s->height)
+            return fail(RD_ERR_INVALID, "the particle renderer covers the whole image (no crop)");
+    }
+    if (p->particles < 0) return fail(RD_ERR_INVALID, "particle count must not be negative");
+    const uint64_t total = p->particles > 0 ? (uint64_t)p->particles : 16384;   // "particle_count" default
+    const int W = s->width, H = s->height;
+    const size_t npix = (size_t)W * H;
+    const size_t wave_cap = (size_t)(p->wave_paths > 0 ? p->wave_paths : (1 << 22));
+    const size_t P = std::max<size_t>(1, std::min<size_t>(wave_cap, total));
+    const size_t qcap = std::max(P, npix);
+    hipStream_t st = c->stream;
+    ScopedAllocs A;
+    const size_t acc4 = (npix * 3 + 3) / 4;   // cleared as double4s
+    double* acc = A.get<double>(acc4 * 4);
+    RayQueue qa{nullptr, A.get<int>(qcap), A.get<float4>(qcap), A.get<float4>(qcap)};
+    RayQueue qb{nullptr, A.get<int>(P), A.get<float4>(P), A.get<float4>(P)};
+    RayQueue qs{nullptr, A.get<int>(P), A.get<float4>(P), A.get<float4>(P)};
+    float4* hits = A.get<float4>(qcap);
+    ParticleState ps{A.get<float4>(P), A.get<uint64_t>(P), A.get<uint32_t>(P), A.get<float4>(P), A.get<uint8_t>(P),
+                     s->dev.num_insts > 0 ? A.get<int>(qcap) : nullptr};
+    if (!acc || !qa.pid || !qa.org || !qa.dir || !qb.pid || !qb.org || !qb.dir || !qs.pid || !qs.org || !qs.dir ||
+        !hits || !ps.coeff || !ps.key || !ps.rng_n || !ps.splat || !ps.vis || (s->dev.num_insts > 0 && !ps.hsub))
+        return fail(RD_ERR_OOM, "hipMalloc failed for the particle render");
+    // the traversal kernels read queue counts, write hits / visibility and
+    // instance triangles through a PathState
+    PathState tps{};
+    tps.vis = ps.vis;
+    tps.hsub = ps.hsub;
+    const int kA = kCntExtA, kB = kCntExtB, kS = kCntShadow;
+    Timer tm(c);
+    hipEvent_t ev0 = c->ev_t0, ev1 = c->ev_t1;
+    HIP_TRY(hipEventRecord(ev0, st));
+    HIP_TRY(launch_zero_counts(c->counts, kCntSlots, st));
+    HIP_TRY(launch_zero(reinterpret_cast<double4*>(acc), (int)acc4, st));
+    int waves = 0, max_bounce = 0;
+    int rc = RD_OK;
+    for (uint64_t base = 0; base < total && !rc; base += P) {
+        const int n = (int)std::min<uint64_t>(P, total - base);
+        RayQueue qin = qa, qout = qb;
+        int cin = kA, cout = kB;
+        HIP_TRY(launch_zero_slots(c->counts, kA, kB, st));
+        HIP_TRY(tm.run("lt_emit", [&] { return launch_lt_emit(s->dev, p->seed, base, n, ps, qin, c->counts, cin, st); }));
+        for (int bounce = 0;; bounce++) {
+            HIP_TRY(hipMemcpyAsync(c->host_counts + cin, c->counts + cin, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            if (c->host_counts[cin] == 0) break;
+            max_bounce = std::max(max_bounce, bounce + 1);
+            qin.count = qout.count = nullptr;
+            HIP_TRY(tm.run("trace_closest", [&] {
+                return launch_trace_ext(s->dev, nullptr, qin, tps, hits, c->counts, cin, -1, -1, -1, nullptr, nullptr,
+                                        c->trace_schedule[0], st);
+            }));
+            HIP_TRY(launch_zero_slots(c->counts, cout, kS, st));
+            HIP_TRY(tm.run("shade", [&] {
+                return launch_lt_shade(s->dev, s->lt_cam, qin, cin, hits, ps, qout, cout, qs, kS, c->counts, st);
+            }));
+            HIP_TRY(tm.run("trace_any", [&] {
+                return launch_trace_shadow(s->dev, nullptr, qs, tps, c->counts, kS, c->trace_schedule[1], st);
+            }));
+            HIP_TRY(tm.run("splat", [&] { return launch_lt_splat(qs, kS, ps, acc, c->counts, st); }));
+            std::swap(qin, qout);
+            std::swap(cin, cout);
+            if (cancel && *cancel) { rc = fail(RD_ERR_CANCELLED, "render cancelled"); break; }
+        }
+        waves++;
+        if (progress) *progress = waves;
+        if (cancel && *cancel && !rc) rc = fail(RD_ERR_CANCELLED, "render cancelled");
+    }
+    if (!rc) {
+        // AfterRender: v = (float)ScreenX * ScreenY / particles, then CameraTraceLight
+        const float v = ((float)W * (float)H) / (float)total;
+        HIP_TRY(launch_lt_corner_rays(s->dev, qa, c->counts, kA, st));
+        HIP_TRY(tm.run("trace_closest", [&] {
+            return launch_trace_ext(s->dev, nullptr, qa, tps, hits, c->counts, kA, -1, -1, -1, nullptr, nullptr,
+                                    c->trace_schedule[0], st);
+        }));
+        HIP_TRY(tm.run("finalize", [&] { return launch_lt_final(s->dev, qa, hits, ps.hsub, acc, v, d_out, st); }));
+    }
+    HIP_TRY(hipEventRecord(ev1, st));
+    HIP_TRY(hipMemcpyAsync(c->host_counts, c->counts, kCntSlots * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    tm.collect();
+    collect_stats(c, total, ms, waves, max_bounce);
+    return rc;
+}
+
 }  // namespace
 
 int rd_render_device(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d_out_rgb, float* d_out_sum2,
                      volatile int32_t* cancel, volatile int32_t* progress) {
     if (!c || !s || !p || !d_out_rgb) return fail(RD_ERR_INVALID, "null argument");
+    if (s->ctx != c) return fail(RD_ERR_INVALID, "scene belongs to another context");
+    if (s->dev.integ.type == RD_INTEGRATOR_PARTICLE) {
+        if (d_out_sum2) return fail(RD_ERR_INVALID, "the particle renderer has no per-pixel sample sums");
+        (void)hipSetDevice(c->device);
+        return render_particles(c, s, p, d_out_rgb, cancel, progress);
+    }
     rd_request r{};
     r.scene = s;
     r.params = *p;

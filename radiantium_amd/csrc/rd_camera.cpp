@@ -56,7 +56,12 @@ bool invert4(const float* M, float* out) {
     return true;
 }
 
-bool setup_camera(const rd_camera_desc& c, DevCamera* out) {
+// Vector3.Transform(v, M) for a row-major 4x4: x*M11 + y*M21 + z*M31 + M41, ...
+static void xform_point4(const float* M, const float* v, float* o) {
+    for (int j = 0; j < 3; j++) o[j] = v[0] * M[j] + v[1] * M[4 + j] + v[2] * M[8 + j] + M[12 + j];
+}
+
+bool setup_camera(const rd_camera_desc& c, DevCamera* out, LtCamera* lt) {
     float c2w[16];
     if (c.use_matrix) {
         for (int q = 0; q < 16; q++) c2w[q] = c.to_world[q];
@@ -91,6 +96,27 @@ bool setup_camera(const rd_camera_desc& c, DevCamera* out) {
     float w2c[16];
     if (!invert4(c2w, w2c)) return false;
     for (int q = 0; q < 16; q++) out->camera_to_world[q] = c2w[q];
+    if (lt) {
+        for (int q = 0; q < 16; q++) {
+            lt->camera_to_clip[q] = p[q];
+            lt->world_to_camera[q] = w2c[q];
+            lt->camera_to_world[q] = c2w[q];
+        }
+        const float zero[3] = {0, 0, 0}, one[3] = {1, 1, 0};
+        float pmin[3], pmax[3];
+        xform_point4(out->clip_to_camera, zero, pmin);
+        xform_point4(out->clip_to_camera, one, pmax);
+        const float zmin = pmin[2], zmax = pmax[2];
+        for (int j = 0; j < 3; j++) { pmin[j] = pmin[j] / zmin; pmax[j] = pmax[j] / zmax; }
+        lt->min_x = pmax[0]; lt->min_y = pmax[1];
+        lt->max_x = pmin[0]; lt->max_y = pmin[1];
+        const float area = std::fabs((lt->max_x - lt->min_x) * (lt->max_y - lt->min_y));
+        lt->normalization = 1 / area;
+        lt->near_z = c.near_z;
+        lt->far_z = c.far_z;
+        lt->width = c.width;
+        lt->height = c.height;
+    }
     return true;
 }
 

@@ -899,9 +899,15 @@ struct Loader {
     }
 
     void build(rd_render_params* params) {
-        // renderer: the block renderer's parameters (RendererBuilder.cs:29-34, CreateRenderer)
+        // renderer: the one the integrator names (CreateRenderer, RendererBuilder.cs:514-526;
+        // block_based / bdpt read "spp", adjoint_particle "particle_count", :29-48)
         int spp = 256;
-        if (const JVal* r = root.get("renderer")) spp = read_i(r, "spp", 256);
+        int64_t particles = 16384;
+        const JVal* rnd = root.get("renderer");
+        if (rnd) {
+            spp = read_i(rnd, "spp", 256);
+            particles = read_i(rnd, "particle_count", 16384);
+        }
         // integrator
         rd_integrator_desc I{};
         I.type = RD_INTEGRATOR_PATH; I.max_depth = -1; I.min_depth = 3; I.rr_threshold = 0.99f;
@@ -909,10 +915,18 @@ struct Loader {
         if (const JVal* it = root.get("integrator")) {
             if (!it->get("type")) bad("integrator: no key type");
             std::string t = read_s(it, "type", "");
-            if (t != "path" && t != "vol_path") unsupported("integrator " + t + " is not on this path");
-            I.type = t == "path" ? RD_INTEGRATOR_PATH : RD_INTEGRATOR_VOL_PATH;
-            I.max_depth = read_i(it, "max_depth", -1);
-            I.min_depth = read_i(it, "min_depth", 3);
+            if (t != "path" && t != "vol_path" && t != "adjoint_particle" && t != "bdpt")
+                unsupported("integrator " + t + " is not on this path");
+            I.type = t == "path" ? RD_INTEGRATOR_PATH
+                                 : (t == "vol_path" ? RD_INTEGRATOR_VOL_PATH
+                                                    : (t == "bdpt" ? RD_INTEGRATOR_BDPT : RD_INTEGRATOR_PARTICLE));
+            // without a renderer block the builder makes a BlockBasedRenderer,
+            // which casts the integrator to MonteCarloIntegrator (:518-519)
+            if (!rnd && (I.type == RD_INTEGRATOR_PARTICLE || I.type == RD_INTEGRATOR_BDPT))
+                bad("integrator " + t + " needs a renderer block");
+            // bdpt: max_depth 7, min_depth 5 (:80-85); the others -1, 3 and rr 0.99
+            I.max_depth = read_i(it, "max_depth", t == "bdpt" ? 7 : -1);
+            I.min_depth = read_i(it, "min_depth", t == "bdpt" ? 5 : 3);
             I.rr_threshold = read_f(it, "rr_threshold", 0.99f);
             if (t == "path") {
                 std::string m = read_s(it, "method", "Mis");
@@ -1153,6 +1167,7 @@ struct Loader {
         if (params) {
             std::memset(params, 0, sizeof *params);
             params->spp = spp;
+            params->particles = particles;
         }
     }
 };

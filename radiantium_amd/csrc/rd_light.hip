@@ -1,0 +1,374 @@
+// rd_light.hip — gfx950 wavefront stages of light tracing: the reference's
+// AdjointParticleIntegrator.EmitParticle
+// (Radiantium.Offline/Integrators/AdjointParticleIntegrator.cs:24-80) under
+// AdjointParticleRenderer (Renderers/AdjointParticleRenderer.cs:127-178).
+//
+// One particle is a light path; a wave holds up to the pool's particles and
+// every bounce of the wave is
+//   trace_ext   Scene.Intersect of the live particles (closest hit, the path
+//               tracer's traversal kernels)
+//   lt_shade    the camera connection (Camera.SampleWi + the splat's colour,
+//               its IsOccluded ray appended to the shadow queue) and the
+//               importance-mode BSDF sample, then the next loop head (depth
+//               cap, Russian roulette) — every random draw of the bounce in
+//               the reference's order
+//   trace_any   the shadow rays (any hit)
+//   lt_splat    the splats of the unoccluded rays, added into a float64
+//               image with atomics (the reference adds under a lock, in
+//               whatever order its threads arrive)
+// and after the last wave one closest-hit query per pixel corner adds the
+// emission the camera sees directly (CameraTraceLight).
+#include "rd_light.h"
+#include "rd_shading.h"
+#include "rd_wave.h"
+
+namespace rd {
+
+namespace {
+
+// Vector3.Transform with a row-major 4x4 (row vectors, no w divide).
+__device__ __forceinline__ f3 xf_point(const float* M, f3 v) {
+    return mk(v.x * M[0] + v.y * M[4] + v.z * M[8] + M[12], v.x * M[1] + v.y * M[5] + v.z * M[9] + M[13],
+              v.x * M[2] + v.y * M[6] + v.z * M[10] + M[14]);
+}
+__device__ __forceinline__ f3 xf_vector(const float* M, f3 v) {
+    return mk(v.x * M[0] + v.y * M[4] + v.z * M[8], v.x * M[1] + v.y * M[5] + v.z * M[9],
+              v.x * M[2] + v.y * M[6] + v.z * M[10]);
+}
+
+// Camera.Importance (Camera.cs:193-204) of a camera-space unit direction.
+__device__ __forceinline__ float cam_importance(const LtCamera& c, f3 d) {
+    float ct = d.z;
+    if (ct <= 0) return 0;
+    float focus = 1 / ct;
+    float px = d.x * focus, py = d.y * focus;
+    if (px < c.min_x || px > c.max_x || py < c.min_y || py > c.max_y) return 0.0f;
+    return c.normalization * focus * focus * focus;
+}
+
+struct CamWi {
+    f3 pos, dir;
+    float pdf, we, sx, sy;
+};
+// PerspectiveCamera.SampleWi (Camera.cs:221-239): the camera position, the
+// direction to it, We = Importance / distance^2 and the screen position
+// (clip x, y divided by clip z, scaled by ScreenX - 1 / ScreenY - 1); pdf 0
+// outside [near, far] or the screen.
+__device__ __forceinline__ CamWi cam_sample_wi(const LtCamera& c, f3 pos) {
+    CamWi r;
+    r.pos = mk(0.0f); r.dir = mk(0.0f); r.pdf = 0.0f; r.we = 0.0f; r.sx = 0.0f; r.sy = 0.0f;
+    f3 lt = xf_point(c.world_to_camera, pos);
+    if (lt.z < c.near_z || lt.z > c.far_z) return r;
+    f3 screen = xf_point(c.camera_to_clip, lt);
+    screen.x /= screen.z;
+    screen.y /= screen.z;
+    if (screen.x < 0 || screen.x > 1 || screen.y < 0 || screen.y > 1) return r;
+    r.sx = screen.x * (float)(c.width - 1);
+    r.sy = screen.y * (float)(c.height - 1);
+    float dist = length(lt);
+    float inv = 1 / dist;
+    f3 local_dir = lt * inv;
+    r.pos = xf_point(c.camera_to_world, mk(0.0f));
+    r.dir = (r.pos - pos) / dist;
+    r.pdf = 1;
+    r.we = cam_importance(c, local_dir) * inv * inv;
+    return r;
+}
+
+// The loop head of EmitParticle (AdjointParticleIntegrator.cs:35-44): the
+// depth cap, then Russian roulette with the fixed threshold.
+__device__ __forceinline__ bool loop_head(const DevIntegrator& I, int bounces, f3& coeff, Rng& rng) {
+    if (I.max_depth != -1 && bounces >= I.max_depth) return false;
+    if (bounces > I.min_depth) {
+        if (rng.next_float() > I.rr_threshold) return false;
+        coeff = coeff / I.rr_threshold;
+    }
+    return true;
+}
+
+struct EmitS {
+    f3 p, dir, n, le;
+    float pdf_pos, pdf_dir;
+};
+// Light.SampleEmit: DiffuseAreaLight.SampleEmit (DiffuseAreaLight.cs:70-77),
+// InfiniteAreaLight.SampleEmit (InfiniteAreaLight.cs:122-133).  False when
+// the environment map's SampleLightDir gives pdf 0 (the reference would
+// divide by it; the particle ends instead).
+__device__ __forceinline__ bool sample_emit(const DevScene& sc, const DevLight& L, Rng& rng, EmitS* e) {
+    if (L.type == 1) {
+        const DevEnv& env = sc.envs[L.tex];
+        float u0 = rng.next_float(), u1 = rng.next_float();
+        f3 wdir;
+        float pdf, x, y;
+        if (!env_sample_dir(env, u0, u1, &wdir, &pdf, &x, &y)) return false;
+        e->le = tex_sample(sc, env.tex, x, y);
+        f3 normal = -wdir;
+        Frame coord = make_frame(normal);
+        float c0 = rng.next_float(), c1 = rng.next_float();
+        // Probability.SquareToConcentricDisk (Probability.cs:157-174)
+        float ox = 2.0f * c0 - 1, oy = 2.0f * c1 - 1;
+        float cx = 0.0f, cy = 0.0f;
+        if (!(ox == 0 && oy == 0)) {
+            float theta, r;
+            if (fabsf(ox) > fabsf(oy)) { r = ox; theta = (kPi / 4) * (oy / ox); }
+            else { r = oy; theta = (kPi / 2) - (kPi / 4) * (ox / oy); }
+            cx = r * cosf(theta);
+            cy = r * sinf(theta);
+        }
+        const f3 center = mk(env.center[0], env.center[1], env.center[2]);
+        f3 disk = center + env.radius * (cx * coord.X + cy * coord.Y);
+        e->p = disk + wdir * env.radius;
+        e->dir = -wdir;
+        e->n = -wdir;
+        e->pdf_pos = 1 / (kPi * sqr(env.radius));
+        e->pdf_dir = pdf;
+        return true;
+    }
+    float ux = rng.next_float(), uy = rng.next_float();
+    f3 P, N;
+    area_shape_sample(sc, L, ux, uy, &P, &N);
+    // Probability.SquareToCosineHemisphere (:94-100) via SquareToUniformDisk
+    float u0 = rng.next_float(), u1 = rng.next_float();
+    float radius = sqrtf(u0);
+    float angle = u1 * kPi * 2;
+    float x = radius * cosf(angle), y = radius * sinf(angle);
+    f3 lw = normalize(mk(x, y, sqrtf(1 - x * x - y * y)));
+    e->pdf_dir = cos_theta(lw) / kPi;
+    e->dir = normalize(to_world(make_frame(N), lw));
+    e->p = P;
+    e->n = N;
+    e->le = mk(L.le[0], L.le[1], L.le[2]);
+    e->pdf_pos = 1.0f / L.area;
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_lt_emit(DevScene sc, uint32_t seed, uint64_t base, int n, ParticleState ps,
+                                                 RayQueue q, int* counts, int cnt_q) {
+    if (blockIdx.x == 0 && threadIdx.x < 2) counts[kCntWork + threadIdx.x] = 0;   // traversal work cursors
+    for (int b0 = blockIdx.x * 256; b0 < n; b0 += gridDim.x * 256) {
+        const int i = b0 + threadIdx.x;
+        bool alive = false;
+        f3 coeff = mk(0.0f);
+        EmitS e;
+        e.p = mk(0.0f); e.dir = mk(0.0f);
+        Rng rng{particle_key(seed, base + (uint64_t)i), 0u};
+        if (i < n && sc.num_lights > 0) {
+            int l = rng.next_int(sc.num_lights);                  // Scene.SampleLight (Scene.cs:47-57)
+            float select_pdf = 1.0f / sc.num_lights;
+            if (sample_emit(sc, sc.lights[l], rng, &e) && !is_black(e.le)) {
+                coeff = e.le * abs_dot(e.dir, e.n) / (select_pdf * e.pdf_pos * e.pdf_dir);
+                alive = loop_head(sc.integ, 1, coeff, rng);
+            }
+        }
+        int pos = wave_append(counts + cnt_q, alive);
+        if (alive) {
+            q.pid[pos] = i;
+            q.org[pos] = make_float4(e.p.x, e.p.y, e.p.z, 0.0001f);
+            q.dir[pos] = make_float4(e.dir.x, e.dir.y, e.dir.z, kFloatMax);
+            ps.coeff[i] = make_float4(coeff.x, coeff.y, coeff.z, __int_as_float(1));
+            ps.key[i] = rng.key;
+            ps.rng_n[i] = rng.n;
+        }
+    }
+}
+
+template <unsigned kM>
+__global__ __launch_bounds__(256) void k_lt_shade(DevScene sc, LtCamera cam, RayQueue qin, int cnt_in,
+                                                  const float4* __restrict__ hits, ParticleState ps, RayQueue qout,
+                                                  int cnt_out, RayQueue qs, int cnt_s, int* counts) {
+    const int n = counts[cnt_in];
+    if (blockIdx.x == 0 && threadIdx.x < 2) counts[kCntWork + threadIdx.x] = 0;
+    uint64_t nvert = 0;
+    for (int b0 = blockIdx.x * 256; b0 < n; b0 += gridDim.x * 256) {
+        const int k = b0 + threadIdx.x;
+        bool want_ext = false, want_shadow = false;
+        f3 no = mk(0.f), nd = mk(0.f), so = mk(0.f), sd = mk(0.f), color = mk(0.f);
+        float ntmin = 0.f, stmax = 0.f;
+        int pid = 0, pixel = 0;
+        f3 coeff = mk(0.f);
+        int bounces = 0;
+        Rng rng{0, 0u};
+        if (k < n) {
+            pid = qin.pid[k];
+            const float4 h = hits[k];
+            const int prim = __float_as_int(h.y);
+            if (prim >= 0) {
+                const float4 ro = qin.org[k], rd = qin.dir[k];
+                const f3 o = mk(ro.x, ro.y, ro.z), d = mk(rd.x, rd.y, rd.z);
+                const float4 cf = ps.coeff[pid];
+                coeff = mk(cf.x, cf.y, cf.z);
+                bounces = __float_as_int(cf.w);
+                rng = Rng{ps.key[pid], ps.rng_n[pid]};
+                const DevPrim pr = sc.prim_info[prim];
+                const int sub = (pr.kind == kInst && ps.hsub) ? ps.hsub[k] : -1;
+                Inct it = make_inct_any(sc, pr, sub, o, d, h.z, h.w, h.x);
+                const int mat = pr.material;
+                if (mat < 0) {
+                    // a medium boundary: the same ray goes on from the hit,
+                    // `bounces--; continue;` (the loop head runs again)
+                    no = it.P; nd = d; ntmin = ro.w;
+                    want_ext = loop_head(sc.integ, bounces, coeff, rng);
+                } else {
+                    nvert++;
+                    MatCtx m = load_material<kM>(sc, mat, it.u, it.v);
+                    const f3 wo_l = to_local(it.sh, it.wr);
+                    CamWi cw = cam_sample_wi(cam, it.P);
+                    if (cw.pdf > 0 && cw.we != 0.0f) {
+                        // Scene.IsOccluded(camera, P) (Scene.cs:58-70)
+                        f3 vec = it.P - cw.pos;
+                        float dis = length(vec);
+                        if (!(dis < 0.001f)) {
+                            const f3 wi_l = to_local(it.sh, cw.dir);
+                            f3 f = mat_fr<kM>(m, wo_l, wi_l, true);
+                            float ac = abs_cos_theta(wi_l);
+                            color = coeff * f * mk(cw.we) * ac / cw.pdf;
+                            pixel = (int)floorf(cw.sx) * cam.height + (int)floorf(cw.sy);
+                            so = cw.pos; sd = normalize(vec); stmax = dis - 0.0001f;
+                            want_shadow = true;
+                        }
+                    }
+                    BSample bs = mat_sample<kM>(m, wo_l, rng, true);
+                    if (bs.pdf > 0.0f) {
+                        coeff = coeff * (bs.fr * abs_cos_theta(bs.wi) / bs.pdf);
+                        no = it.P; nd = to_world(it.sh, bs.wi); ntmin = 0.0001f;
+                        bounces++;
+                        want_ext = loop_head(sc.integ, bounces, coeff, rng);
+                    }
+                }
+            }
+        }
+        int sp = wave_append(counts + cnt_s, want_shadow);
+        if (want_shadow) {
+            qs.pid[sp] = sp;
+            qs.org[sp] = make_float4(so.x, so.y, so.z, 0.0001f);
+            qs.dir[sp] = make_float4(sd.x, sd.y, sd.z, stmax);
+            ps.splat[sp] = make_float4(color.x, color.y, color.z, __int_as_float(pixel));
+        }
+        int ep = wave_append(counts + cnt_out, want_ext);
+        if (want_ext) {
+            qout.pid[ep] = pid;
+            qout.org[ep] = make_float4(no.x, no.y, no.z, ntmin);
+            qout.dir[ep] = make_float4(nd.x, nd.y, nd.z, kFloatMax);
+            ps.coeff[pid] = make_float4(coeff.x, coeff.y, coeff.z, __int_as_float(bounces));
+            ps.rng_n[pid] = rng.n;
+        }
+    }
+    add_counter((unsigned long long*)(counts + kCntVertices), nvert);
+}
+
+__global__ __launch_bounds__(256) void k_lt_splat(RayQueue qs, int cnt_s, ParticleState ps, double* acc,
+                                                  const int* counts) {
+    const int n = counts[cnt_s];
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+        if (!ps.vis[k]) continue;
+        const float4 c = ps.splat[k];
+        double* px = acc + 3 * (size_t)__float_as_int(c.w);
+        atomicAdd(px + 0, (double)c.x);
+        atomicAdd(px + 1, (double)c.y);
+        atomicAdd(px + 2, (double)c.z);
+    }
+}
+
+// Camera.SampleRay (Camera.cs:182-191) at pixel corner (x, y).
+__global__ __launch_bounds__(256) void k_lt_corner_rays(DevScene sc, RayQueue q, int* counts, int cnt_q) {
+    const DevCamera& c = sc.cam;
+    const int n = c.width * c.height;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counts[cnt_q] = n;
+        counts[kCntWork + 0] = 0;
+    }
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const int x = i / c.height, y = i % c.height;
+        f3 ndc = mk((float)x * c.inv_res_x, (float)y * c.inv_res_y, 0.0f);
+        f3 nearp = xf_point(c.clip_to_camera, ndc);
+        f3 dir = normalize(nearp);
+        float inv_z = 1.0f / dir.z;
+        f3 o = xf_point(c.camera_to_world, mk(0.0f));
+        f3 d = normalize(xf_vector(c.camera_to_world, dir));
+        q.pid[i] = i;
+        q.org[i] = make_float4(o.x, o.y, o.z, c.near_z * inv_z);
+        q.dir[i] = make_float4(d.x, d.y, d.z, c.far_z * inv_z);
+    }
+}
+
+// AfterRender (AdjointParticleRenderer.cs:161-177): RefRGB *= v, then
+// += CameraTraceLight (AdjointParticleIntegrator.cs:82-85).
+__global__ __launch_bounds__(256) void k_lt_final(DevScene sc, RayQueue q, const float4* __restrict__ hits,
+                                                  const int* __restrict__ hsub, const double* __restrict__ acc,
+                                                  float v, float* out) {
+    const int n = sc.cam.width * sc.cam.height;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        f3 c = mk((float)acc[3 * (size_t)i], (float)acc[3 * (size_t)i + 1], (float)acc[3 * (size_t)i + 2]);
+        c = c * v;
+        const float4 h = hits[i];
+        const int prim = __float_as_int(h.y);
+        const float4 rd = q.dir[i];
+        const f3 d = mk(rd.x, rd.y, rd.z);
+        f3 le = mk(0.0f);
+        if (prim >= 0) {
+            const DevPrim pr = sc.prim_info[prim];
+            if (pr.kind != kInst && pr.light >= 0) {
+                const float4 ro = q.org[i];
+                Inct it = make_inct(sc, pr, mk(ro.x, ro.y, ro.z), d, h.z, h.w, h.x);
+                const DevLight& L = sc.lights[pr.light];
+                // DiffuseAreaLight.L (DiffuseAreaLight.cs:26-29)
+                le = dot(it.sh.Z, -d) > 0 ? mk(L.le[0], L.le[1], L.le[2]) : mk(0.0f);
+            }
+        } else {
+            le = eval_infinite(sc, d);
+        }
+        c = c + le;
+        out[3 * (size_t)i] = c.x;
+        out[3 * (size_t)i + 1] = c.y;
+        out[3 * (size_t)i + 2] = c.z;
+    }
+}
+
+int grid_for(int n) {
+    int b = (n + 255) / 256;
+    if (b < 1) b = 1;
+    if (b > persistent_blocks()) b = persistent_blocks();
+    return b;
+}
+
+}  // namespace
+
+hipError_t launch_lt_emit(const DevScene& sc, uint32_t seed, uint64_t base, int n, ParticleState ps, RayQueue q,
+                          int* counts, int cnt_q, hipStream_t s) {
+    hipLaunchKernelGGL(k_lt_emit, dim3(grid_for(n)), dim3(256), 0, s, sc, seed, base, n, ps, q, counts, cnt_q);
+    return hipGetLastError();
+}
+
+hipError_t launch_lt_shade(const DevScene& sc, const LtCamera& cam, RayQueue qin, int cnt_in, const float4* hits,
+                           ParticleState ps, RayQueue qout, int cnt_out, RayQueue qs, int cnt_s, int* counts,
+                           hipStream_t s) {
+    const dim3 g(persistent_blocks()), b(256);
+    const unsigned set = mat_set(sc.mat_mask);
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, g, b, 0, s, sc, cam, qin, cnt_in, hits, ps, qout, cnt_out, qs, cnt_s, counts);
+    };
+    if (set == kMatsDelta) go(k_lt_shade<kMatsDelta>);
+    else if (set == kMatsMicrofacet) go(k_lt_shade<kMatsMicrofacet>);
+    else go(k_lt_shade<kMatsAll>);
+    return hipGetLastError();
+}
+
+hipError_t launch_lt_splat(RayQueue qs, int cnt_s, ParticleState ps, double* acc, int* counts, hipStream_t s) {
+    hipLaunchKernelGGL(k_lt_splat, dim3(persistent_blocks()), dim3(256), 0, s, qs, cnt_s, ps, acc, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_lt_corner_rays(const DevScene& sc, RayQueue q, int* counts, int cnt_q, hipStream_t s) {
+    hipLaunchKernelGGL(k_lt_corner_rays, dim3(grid_for(sc.cam.width * sc.cam.height)), dim3(256), 0, s, sc, q, counts,
+                       cnt_q);
+    return hipGetLastError();
+}
+
+hipError_t launch_lt_final(const DevScene& sc, RayQueue q, const float4* hits, const int* hsub, const double* acc,
+                           float v, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_lt_final, dim3(grid_for(sc.cam.width * sc.cam.height)), dim3(256), 0, s, sc, q, hits, hsub,
+                       acc, v, out);
+    return hipGetLastError();
+}
+
+}  // namespace rd

@@ -561,8 +561,9 @@ __device__ __forceinline__ f3 gtr2_reflect_fr(const DisneyP& b, const Mf& m, boo
     float D = mf_D(m, wh), G = gtr2_G(m, wo, wi);
     return R * fabs3((D * G * F) / (4 * cti * cto));
 }
-// MicrofacetTransmissionBtdf<Gtr2> Fr / Pdf (MicrofacetTransmissionBtdf.cs:25-57), etaA = 1, etaB = Eta
-__device__ __forceinline__ f3 disney_btdf_fr(const DisneyP& b, f3 wo, f3 wi) {
+// MicrofacetTransmissionBtdf<Gtr2> Fr / Pdf (MicrofacetTransmissionBtdf.cs:25-57), etaA = 1, etaB = Eta;
+// imp: TransportMode.Importance (light paths), which drops the 1/eta radiance scaling (:40)
+__device__ __forceinline__ f3 disney_btdf_fr(const DisneyP& b, f3 wo, f3 wi, bool imp = false) {
     if (same_hemisphere(wo, wi)) return mk(0.0f);
     float cto = cos_theta(wo), cti = cos_theta(wi);
     if (cti == 0 || cto == 0) return mk(0.0f);
@@ -575,7 +576,7 @@ __device__ __forceinline__ f3 disney_btdf_fr(const DisneyP& b, f3 wo, f3 wi) {
     if (dot(wo, wh) * dot(wi, wh) > 0) return mk(0.0f);
     f3 F = mk(fr_dielectric(dot(wo, wh), 1.0f, b.eta));
     float sd = dot(wo, wh) + eta * dot(wi, wh);
-    float factor = 1 / eta;
+    float factor = imp ? 1.0f : 1 / eta;
     float D = mf_D(m, wh), G = gtr2_G(m, wo, wi);
     f3 T = (1 - b.metallic) * b.transmission * sqrt3(b.color);
     return (1 - F) * T * fabsf(D * G * eta * eta * abs_dot(wi, wh) * abs_dot(wo, wh) * factor * factor /
@@ -619,10 +620,10 @@ __device__ __forceinline__ float clearcoat_pdf(const DisneyP& b, f3 wo, f3 wi) {
     wh = normalize(wh);
     return gtr1(abs_cos_theta(wh), gloss) * abs_cos_theta(wh) / (4 * dot(wo, wh));
 }
-__device__ __forceinline__ f3 disney_fr(const DisneyP& b, f3 wo, f3 wi) {     // DisneyBsdf.FrImpl
+__device__ __forceinline__ f3 disney_fr(const DisneyP& b, f3 wo, f3 wi, bool imp = false) {     // DisneyBsdf.FrImpl
     if (!same_hemisphere(wo, wi)) {
         if (b.transmission == 0) return mk(0.0f);
-        return disney_btdf_fr(b, wo, wi);
+        return disney_btdf_fr(b, wo, wi, imp);
     }
     if (cos_theta(wo) < 0 && cos_theta(wi) < 0) return gtr2_reflect_fr(b, gtr2_dist(b, b.trans_rough), true, wo, wi);
     f3 diffuse = mk(0.0f);
@@ -692,9 +693,11 @@ __device__ __forceinline__ float disney_pdf(const DisneyP& b, f3 wo, f3 wi) {  /
 }
 
 // Material.Fr (Diffuse.cs:21-27, RoughMetal.cs:33-58, RoughGlass.cs:151-195,
-// RoughPlastic.cs:65-74, FresnelSpecularBsdf.cs:30, SpecularReflectionBrdf.cs:17, Disney.cs:87-90)
+// RoughPlastic.cs:65-74, FresnelSpecularBsdf.cs:30, SpecularReflectionBrdf.cs:17, Disney.cs:87-90).
+// imp: TransportMode.Importance (light paths): refraction without the radiance
+// scaling (RoughGlass.cs:63, MicrofacetTransmissionBtdf.cs:40).
 template <unsigned kM = kMatsAll>
-__device__ __forceinline__ f3 mat_fr(const MatCtx& c, f3 wo, f3 wi) {
+__device__ __forceinline__ f3 mat_fr(const MatCtx& c, f3 wo, f3 wi, bool imp = false) {
     switch (c.type) {
     case kMatDiffuse:
         if constexpr (RD_MAT(kMatDiffuse)) {
@@ -726,7 +729,7 @@ __device__ __forceinline__ f3 mat_fr(const MatCtx& c, f3 wo, f3 wi) {
                 bsdf = c.r * (D * G * F / (4.0f * cos_theta(wi) * cos_theta(wo)));
             } else {
                 if (dot(wo, wh) * dot(wi, wh) > 0) return mk(0.0f);
-                float factor = 1 / eta;
+                float factor = imp ? 1.0f : 1 / eta;
                 bsdf = (1 - F) * c.t *
                        fabsf(D * G * eta * eta * abs_dot(wi, wh) * abs_dot(wo, wh) * factor * factor /
                              (cos_theta(wo) * cos_theta(wi) * sd * sd));
@@ -742,7 +745,7 @@ __device__ __forceinline__ f3 mat_fr(const MatCtx& c, f3 wo, f3 wi) {
         return mk(0.0f);
     case kMatDisney:
         if constexpr (RD_MAT(kMatDisney)) {
-            return disney_fr(c.dp, wo, wi);
+            return disney_fr(c.dp, wo, wi, imp);
         }
         return mk(0.0f);
     default:   // perfect glass / mirror: delta lobes
@@ -842,7 +845,7 @@ __device__ __forceinline__ bool mf_reflect_dir(const Mf& m, f3 wo, Rng& rng, f3*
     return true;
 }
 
-__device__ __forceinline__ BSample disney_sample(const DisneyP& b, f3 wo, Rng& rng) {           // DisneyBsdf.Sample
+__device__ __forceinline__ BSample disney_sample(const DisneyP& b, f3 wo, Rng& rng, bool imp = false) {  // DisneyBsdf.Sample
     BSample s;
     s.wi = mk(0.0f); s.fr = mk(0.0f); s.pdf = 0.0f; s.type = 0;
     Mf td = gtr2_dist(b, b.trans_rough);
@@ -858,7 +861,7 @@ __device__ __forceinline__ BSample disney_sample(const DisneyP& b, f3 wo, Rng& r
         float ei = entering ? 1.0f : b.eta, et = entering ? b.eta : 1.0f;
         f3 w;
         if (!refract(wo, h, ei / et, &w)) return false;
-        f3 f = disney_btdf_fr(b, wo, w);
+        f3 f = disney_btdf_fr(b, wo, w, imp);
         float p = disney_btdf_pdf(b, wo, w);
         if (!valid_color(f) || isinf(p) || p == 0) return false;
         *out = w;
@@ -914,16 +917,17 @@ __device__ __forceinline__ BSample disney_sample(const DisneyP& b, f3 wo, Rng& r
     }
     if (!ok) return s;
     s.wi = wi;
-    s.fr = disney_fr(b, wo, wi);
+    s.fr = disney_fr(b, wo, wi, imp);
     s.pdf = disney_pdf(b, wo, wi);
     s.type = type;
     return s;
 }
 
 // Material.Sample (Diffuse.cs:39-49, RoughMetal.cs:87-112, RoughGlass.cs:240-297,
-// RoughPlastic.cs:112-166, FresnelSpecularBsdf.cs:34-65, PerfectMirror.cs:29-39, Disney.cs:97-100)
+// RoughPlastic.cs:112-166, FresnelSpecularBsdf.cs:34-65, PerfectMirror.cs:29-39, Disney.cs:97-100);
+// imp as for mat_fr (RoughGlass.cs:161, FresnelSpecularBsdf.cs:58-61).
 template <unsigned kM = kMatsAll>
-__device__ __forceinline__ BSample mat_sample(const MatCtx& c, f3 wo, Rng& rng) {
+__device__ __forceinline__ BSample mat_sample(const MatCtx& c, f3 wo, Rng& rng, bool imp = false) {
     BSample s;
     s.wi = mk(0.0f); s.fr = mk(0.0f); s.pdf = 0.0f; s.type = 0;
     switch (c.type) {
@@ -981,7 +985,7 @@ __device__ __forceinline__ BSample mat_sample(const MatCtx& c, f3 wo, Rng& rng) 
                 eta = 1;
             }
             float sd = dot(wo, wh) + eta * dot(wi, wh);
-            float factor = 1 / eta;
+            float factor = imp ? 1.0f : 1 / eta;
             float D = mf_D(c.mf, wh);
             float G = mf_G(c.mf, wo, wi);
             f3 btdf = (1 - F) * c.t *
@@ -1030,7 +1034,7 @@ __device__ __forceinline__ BSample mat_sample(const MatCtx& c, f3 wo, Rng& rng) 
             f3 wi;
             if (!refract(wo, n, ei / et, &wi) || cos_theta(wi) == 0.0f) return s;
             f3 ft = c.t * (1.0f - f);
-            ft = ft * ((ei * ei) / (et * et));
+            if (!imp) ft = ft * ((ei * ei) / (et * et));
             ft = ft / abs_cos_theta(wi);
             s.wi = wi; s.fr = ft; s.pdf = 1 - f; s.type = kTransmission | kSpecular;
             return s;
@@ -1049,7 +1053,7 @@ __device__ __forceinline__ BSample mat_sample(const MatCtx& c, f3 wo, Rng& rng) 
     }
     case kMatDisney:
         if constexpr (RD_MAT(kMatDisney)) {
-            return disney_sample(c.dp, wo, rng);
+            return disney_sample(c.dp, wo, rng, imp);
         }
         return s;
     default:
@@ -1124,27 +1128,39 @@ struct LSample {
 // Distribution2D picks a cell (marginal on u.Y, conditional on u.X); the
 // direction is the cell's corner, the pdf the cell's continuous density over
 // 2 pi^2 sin(theta); the shadow target lies 20 world radii out.
-__device__ __forceinline__ LSample env_sample_li(const DevScene& sc, const DevEnv& e, f3 refp, Rng& rng) {
-    LSample out;
-    out.p = mk(0.0f); out.wi = mk(0.0f); out.li = mk(0.0f); out.pdf = 0.0f;
-    float u0 = rng.next_float(), u1 = rng.next_float();
+// SampleLightDir (:92-109) from two uniforms: false when the pdf is 0, else
+// the world direction, its pdf and the cell corner (x, y) the map is read at.
+__device__ __forceinline__ bool env_sample_dir(const DevEnv& e, float u0, float u1, f3* wdir_out, float* pdf_out,
+                                               float* x_out, float* y_out) {
     int vj = dist_find(e.marg_cdf, e.h, u1);
     float pdf1 = e.marg_int > 0 ? e.marg_func[vj] / e.marg_int : 0;
     const float rint = e.marg_func[vj];
     int vi = dist_find(e.cond_cdf + (size_t)vj * (e.w + 1), e.w, u0);
     float pdf0 = rint > 0 ? e.cond_func[(size_t)vj * e.w + vi] / rint : 0;
     float pdf = pdf0 * pdf1;
-    if (pdf <= 0) return out;
+    if (pdf <= 0) return false;
     float y = vj / (float)e.h;
     float x = vi / (float)e.w;
     float theta = y * kPi;
     float phi = x * 2 * kPi;
     float sin_theta = sinf(theta);
-    if (sin_theta == 0) return out;
+    if (sin_theta == 0) return false;
     pdf /= (2 * kPi * kPi * sin_theta);
     float st = sinf(theta), ct = cosf(theta), sp = sinf(phi), cp = cosf(phi);   // MathExt.SphericalCoordinates
     f3 sph = mk(cp * st, sp * st, ct);
-    f3 wdir = normalize(xform_normal3(e.m2w, mk(sph.y, sph.z, -sph.x)));
+    *wdir_out = normalize(xform_normal3(e.m2w, mk(sph.y, sph.z, -sph.x)));
+    *pdf_out = pdf;
+    *x_out = x;
+    *y_out = y;
+    return true;
+}
+__device__ __forceinline__ LSample env_sample_li(const DevScene& sc, const DevEnv& e, f3 refp, Rng& rng) {
+    LSample out;
+    out.p = mk(0.0f); out.wi = mk(0.0f); out.li = mk(0.0f); out.pdf = 0.0f;
+    float u0 = rng.next_float(), u1 = rng.next_float();
+    f3 wdir;
+    float pdf, x, y;
+    if (!env_sample_dir(e, u0, u1, &wdir, &pdf, &x, &y)) return out;
     out.li = tex_sample(sc, e.tex, x, y);
     out.p = wdir * e.radius * 20.0f + mk(e.center[0], e.center[1], e.center[2]);
     out.wi = normalize(out.p - refp);
@@ -1154,10 +1170,9 @@ __device__ __forceinline__ LSample env_sample_li(const DevScene& sc, const DevEn
 
 // DiffuseAreaLight.SampleLi (Lights/DiffuseAreaLight.cs:48-68) with
 // Triangle.Sample (Triangle.cs:184-212) / Sphere.Sample (Sphere.cs:87-102).
-__device__ __forceinline__ LSample light_sample(const DevScene& sc, const DevLight& L, f3 refp, Rng& rng) {
-    if (L.type == 1) return env_sample_li(sc, sc.envs[L.tex], refp, rng);
+__device__ __forceinline__ void area_shape_sample(const DevScene& sc, const DevLight& L, float ux, float uy, f3* P_out,
+                                                  f3* N_out) {
     const DevPrim pr = sc.prim_info[L.prim];
-    float ux = rng.next_float(), uy = rng.next_float();
     f3 P, N;
     if (pr.kind == kTri) {
         int i0 = sc.tri_idx[3 * pr.index], i1 = sc.tri_idx[3 * pr.index + 1], i2 = sc.tri_idx[3 * pr.index + 2];
@@ -1176,6 +1191,14 @@ __device__ __forceinline__ LSample light_sample(const DevScene& sc, const DevLig
         N = normalize(v);
         P = N * s.w + mk(s.x, s.y, s.z);
     }
+    *P_out = P;
+    *N_out = N;
+}
+__device__ __forceinline__ LSample light_sample(const DevScene& sc, const DevLight& L, f3 refp, Rng& rng) {
+    if (L.type == 1) return env_sample_li(sc, sc.envs[L.tex], refp, rng);
+    float ux = rng.next_float(), uy = rng.next_float();
+    f3 P, N;
+    area_shape_sample(sc, L, ux, uy, &P, &N);
     float spdf = 1.0f / L.area;
     LSample out;
     f3 wi = P - refp;

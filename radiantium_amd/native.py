@@ -486,6 +486,17 @@ class Scene:
                               C.cast(progress, C.c_void_p) if progress is not None else None))
         return rgb, s2
 
+    def render_particles(self, particles, seed=0, wave_paths=0, cancel=None, progress=None):
+        """Light tracing of an adjoint_particle scene (rd_render with
+        params.particles): the image [W*H*3] column-major."""
+        p = abi.rd_render_params()
+        p.seed, p.wave_paths, p.particles = seed, wave_paths, particles
+        rgb = np.zeros(self.width * self.height * 3, np.float32)
+        check(lib().rd_render(self.ctx.h, self.h, C.byref(p), _ptr(rgb), None,
+                              C.cast(cancel, C.c_void_p) if cancel is not None else None,
+                              C.cast(progress, C.c_void_p) if progress is not None else None))
+        return rgb
+
     def render_mean_f64(self, spp, seed=0, crop=None, wave_paths=0):
         """rd_render_mean_f64: the per-pixel mean [W*H*3] column-major in float64."""
         p = abi.rd_render_params()

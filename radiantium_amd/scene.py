@@ -406,7 +406,8 @@ class SceneDesc:
             cam.up[j] = c["up"][j]
 
         it = self.integrator
-        d.integrator.type = {"path": abi.INTEGRATOR_PATH, "vol_path": abi.INTEGRATOR_VOL_PATH}[it["type"]]
+        d.integrator.type = {"path": abi.INTEGRATOR_PATH, "vol_path": abi.INTEGRATOR_VOL_PATH,
+                             "adjoint_particle": abi.INTEGRATOR_PARTICLE, "bdpt": abi.INTEGRATOR_BDPT}[it["type"]]
         d.integrator.max_depth = it["max_depth"]
         d.integrator.min_depth = it["min_depth"]
         d.integrator.rr_threshold = it["rr_threshold"]

@@ -40,7 +40,7 @@ def test_abi_version_and_struct_sizes():
     assert C.sizeof(abi.rd_medium_desc) == 32
     assert C.sizeof(abi.rd_image) == 24
     assert C.sizeof(abi.rd_prim_desc) == 24
-    assert C.sizeof(abi.rd_render_params) == 32
+    assert C.sizeof(abi.rd_render_params) == 40
     assert C.sizeof(abi.rd_texture_desc) == 40
     assert C.sizeof(abi.rd_material_desc) == 4 * 4 + 16 * 4 + 4 * 4 + 6 * 4
     assert C.sizeof(abi.rd_light_desc) == 4 * 2 + 12 + 4 + 64

@@ -375,11 +375,15 @@ def test_loaded_scene_matches_the_python_description(tmp_path):
             assert list(gt.value) == [np.float32(v) for v in wt.value]
 
 
+ON_PATH = ("path", "vol_path", "adjoint_particle", "bdpt")
+
+
 @pytest.mark.skipif(not os.path.isdir(REF_EXAMPLES), reason="the reference's example configs are not present")
 def test_reference_example_configs():
-    """Every example config of the reference: the ones for the path / vol_path
-    integrators whose assets are all present load; the others fail with the
-    reason (integrator not on this path, missing or undecodable asset)."""
+    """Every example config of the reference: the ones for the path,
+    vol_path, adjoint_particle and bdpt integrators whose assets are all
+    present load; the others fail with the reason (integrator not on this
+    path, missing or undecodable asset)."""
     loaded, refused = [], []
     for root, _, files in os.walk(REF_EXAMPLES):
         for f in sorted(files):
@@ -394,9 +398,9 @@ def test_reference_example_configs():
                 loaded.append((p, js.to_c().num_prims))
             except RadiantiumError as e:
                 refused.append((p, str(e)))
-                assert integ not in ("path", "vol_path") or not assets_ok or "UNSUPPORTED" in str(e), (p, str(e))
+                assert integ not in ON_PATH or not assets_ok or "UNSUPPORTED" in str(e), (p, str(e))
                 continue
-            assert integ in ("path", "vol_path") and assets_ok, p
+            assert integ in ON_PATH and assets_ok, p
     assert len(loaded) >= 10, (loaded, refused)
     assert all(n > 0 for _, n in loaded)
 
