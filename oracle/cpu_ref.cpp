@@ -2610,6 +2610,242 @@ static C3 camera_trace_light(const cr_scene& s, int x, int y, uint64_t* st) {
     return eval_infinite(s, ray);
 }
 
+// ---------------------------------------------------------------- bidirectional
+// BidirectionalPathTracer (Integrators/BidirectionalPathTracer.cs:11-435) and
+// BdptVertex / BdptPath (Renderers/BidirectionalPathTracingRenderer.cs:13-277).
+// The reference's MisWeight returns 1 for every strategy (its body is
+// commented out, :305-414), so the vertices' forward / reverse densities
+// never reach the image and are not kept here.
+enum { BV_CAMERA = 0, BV_LIGHT = 1, BV_SURFACE = 2, BV_MEDIUM = 3 };
+struct BVert {
+    int type = BV_CAMERA;
+    V3 p, n;            // n = 0 off surfaces (IsOnSurface => N != 0)
+    C3 coeff;
+    int env = -1;       // Msr.Medium: the medium a medium vertex sampled in;
+                        // the camera's / light's medium; -1 (null) on surfaces
+    Inct it;            // surface vertices
+    V3 wo;              // medium vertices: Msr.Wo
+    int light = -1;     // light vertices: the light (-1: the escaped-ray vertex, Light null)
+};
+static inline bool bv_on_surface(const BVert& v) { return !(v.n.x == 0 && v.n.y == 0 && v.n.z == 0); }
+static inline bool bv_is_light(const cr_scene& s, const BVert& v) {
+    return v.type == BV_LIGHT || (v.type == BV_SURFACE && s.prims[v.it.prim].light >= 0);
+}
+static inline bool bv_is_infinite(const cr_scene& s, const BVert& v) {
+    return v.type == BV_LIGHT && (v.light < 0 || s.lights[v.light].type == RD_LIGHT_INFINITE);
+}
+// BdptVertex.F (:94-106)
+static C3 bv_f(const cr_scene& s, const BVert& v, const BVert& next, bool imp) {
+    V3 wi = next.p - v.p;
+    if (length_sq(wi) == 0) return C3(0.0f);
+    wi = normalize(wi);
+    if (v.type == BV_SURFACE) {
+        const rd_material_desc& m = s.mats[s.prims[v.it.prim].material];
+        return material_fr(s, m, v.it.sh.to_local(v.it.wr), v.it.sh.to_local(wi), v.it.UV, imp);
+    }
+    if (v.type == BV_MEDIUM) return C3(hg_p(s.homo[v.env], v.wo, wi));
+    return C3(0.0f);
+}
+// BdptVertex.Le (:69-92)
+static C3 bv_le(const cr_scene& s, const BVert& v, const BVert& prev) {
+    if (!bv_is_light(s, v)) return C3(0.0f);
+    V3 w = prev.p - v.p;
+    if (length_sq(w) == 0) return C3(0.0f);
+    w = normalize(w);
+    if (bv_is_infinite(s, v)) {
+        C3 le(0.0f);
+        for (size_t i = 0; i < s.env.size(); i++) le = le + env_le(s, s.env[i], -w);
+        return le;
+    }
+    return inct_le(s, v.it, w);
+}
+
+// BidirectionalPathTracer.RandomWalk (:84-162): appends the walk's vertices
+// after path[0]; returns the walk count.  Rays after the first keep the
+// first ray's MinT, as written.
+static int bd_random_walk(const cr_scene& s, Ray ray, int env, Rng& rng, C3 coeff, std::vector<BVert>& path,
+                          int max_depth, bool imp, uint64_t* st) {
+    if (max_depth == 0) return 0;
+    int bounces = 1;
+    while (true) {
+        Inct it;
+        bool hit = scene_intersect(s, ray, &it, st);
+        MSample ms;
+        ms.medium = false;
+        if (env >= 0) {
+            Ray real(ray.o, ray.d, ray.tmin, hit ? it.T : FLT_MAX);
+            ms = medium_sample(s.homo[env], real, rng);
+            coeff = coeff * ms.tr;
+        }
+        if (coeff.black()) break;
+        if (ms.medium) {
+            BVert v;
+            v.type = BV_MEDIUM; v.p = ms.p; v.n = V3(0.0f); v.coeff = coeff; v.env = env; v.wo = ms.wo;
+            path.push_back(v);
+            if (++bounces >= max_depth) break;
+            PSample ps = hg_sample(s.homo[env], ms.wo, rng);
+            ray = Ray(ms.p, ps.wi, ray.tmin);
+        } else {
+            if (!hit) {
+                if (!imp) {
+                    BVert v;
+                    v.type = BV_LIGHT; v.light = -1; v.p = ray.at(10); v.n = -ray.d; v.coeff = coeff; v.env = env;
+                    path.push_back(v);
+                    ++bounces;
+                }
+                break;
+            }
+            int mat = s.prims[it.prim].material;
+            if (mat < 0) {
+                ray = Ray(it.P, ray.d, ray.tmin);
+                env = inct_medium(s, it, ray.d);
+                continue;
+            }
+            if (st) st[3]++;
+            BVert v;
+            v.type = BV_SURFACE; v.p = it.P; v.n = it.N(); v.coeff = coeff; v.env = -1; v.it = it;
+            path.push_back(v);
+            if (++bounces >= max_depth) break;
+            BSample bs = material_sample(s, s.mats[mat], it.sh.to_local(it.wr), it.UV, rng, imp);
+            if (bs.pdf > 0.0f) coeff = coeff * (bs.fr * abs_cos_theta(bs.wi) / bs.pdf);
+            else break;
+            V3 nd = it.sh.to_world(bs.wi);
+            ray = Ray(it.P, nd, ray.tmin);
+            env = inct_medium(s, it, nd);
+        }
+    }
+    return bounces - 1;
+}
+
+// The three random streams of bidirectional sample (pixel, sample): the
+// camera subpath (the sample position, then its walk), the light subpath,
+// and the light samples of the s = 1 strategies (three draws each, strategy
+// t at draws 3 (t - 2) ...).  Sample indices stay below 2^29.
+static inline uint64_t bd_light_key(uint32_t seed, uint32_t pixel, uint32_t sample) {
+    return path_key(seed, pixel, 0x40000000u | sample);
+}
+static inline uint64_t bd_nee_key(uint32_t seed, uint32_t pixel, uint32_t sample) {
+    return path_key(seed, pixel, 0x20000000u | sample);
+}
+
+// ConnectBdpt (:219-300) for strategy (s, t); MisWeight is 1.
+static C3 bd_connect(const cr_scene& s, const std::vector<BVert>& lp, int sn, const std::vector<BVert>& cp, int t,
+                     uint32_t seed, uint32_t pixel, uint32_t sample, uint64_t* st) {
+    C3 l(0.0f);
+    if (t > 1 && sn != 0 && cp[t - 1].type == BV_LIGHT) return C3(0.0f);
+    if (sn == 0) {
+        const BVert& pt = cp[t - 1];
+        if (bv_is_light(s, pt)) l = bv_le(s, pt, cp[t - 2]) * pt.coeff;
+    } else if (t == 1) {
+        const BVert& qs = lp[sn - 1];   // every vertex type past the light's own is connectible
+        Camera::Wi cw = s.cam.sample_wi(qs.p);
+        if (cw.pdf > 0 && cw.we != 0) {
+            BVert cam;
+            cam.type = BV_CAMERA; cam.p = cw.pos; cam.coeff = C3(cw.we) / cw.pdf;
+            l = qs.coeff * bv_f(s, qs, cam, true) * cam.coeff;
+            if (bv_on_surface(qs)) l = l * abs_dot(cw.dir, qs.n);
+            if (!l.black()) l = l * scene_transmittance(s, qs.p, cw.pos, qs.env, st);
+        }
+    } else if (sn == 1) {
+        const BVert& pt = cp[t - 1];
+        Rng rng;
+        rng.key = bd_nee_key(seed, pixel, sample);
+        rng.n = 3 * (uint32_t)(t - 2);
+        if (!s.lights.empty()) {
+            int light = rng.next_int((int)s.lights.size());
+            float select_pdf = 1.0f / s.lights.size();
+            LSample ls = light_sample_li(s, light, pt.p, rng);
+            if (ls.pdf > 0 && !ls.li.black()) {
+                BVert lv;
+                lv.type = BV_LIGHT; lv.light = light; lv.p = ls.p; lv.coeff = ls.li / (ls.pdf * select_pdf);
+                l = pt.coeff * bv_f(s, pt, lv, false) * lv.coeff;
+                if (bv_on_surface(pt)) l = l * abs_dot(ls.wi, pt.n);
+                if (!l.black()) l = l * scene_transmittance(s, pt.p, ls.p, pt.env, st);
+            }
+        }
+    } else {
+        const BVert& qs = lp[sn - 1];
+        const BVert& pt = cp[t - 1];
+        l = qs.coeff * bv_f(s, qs, pt, true) * bv_f(s, pt, qs, false) * pt.coeff;
+        if (!l.black()) {
+            // G (:311-320)
+            V3 d = qs.p - pt.p;
+            float g = 1 / length_sq(d);
+            d = d * std::sqrt(g);
+            if (bv_on_surface(qs)) g *= abs_dot(qs.n, d);
+            if (bv_on_surface(pt)) g *= abs_dot(pt.n, d);
+            C3 tr = scene_transmittance(s, qs.p, pt.p, qs.env, st);
+            l = l * (g * tr);
+        }
+    }
+    return l;
+}
+
+// One bidirectional sample (BidirectionalPathTracingRenderer.RenderTask
+// body, :396-407): the camera subpath, the light subpath, then every
+// strategy of Connect (:164-217) in its loop order.  All of them land on the
+// sample's own pixel (the reference's t = 1 splat takes floor(samplePoint)
+// whenever SampleWi set a screen position), so the sample's value is
+// (sum of the t = 1 strategies in s order) + (sum of the others in loop
+// order), each strategy that is not a valid colour counted as 0.
+// only_s / only_t >= 0: that strategy alone (a test probe of the estimator).
+static C3 bd_sample(const cr_scene& s, uint32_t seed, int x, int y, int sample, uint64_t* st, int only_s = -1,
+                    int only_t = -1) {
+    const rd_integrator_desc& I = s.integ;
+    const uint32_t pixel = (uint32_t)(y * s.cam.W + x);
+    Rng rng;
+    rng.key = path_key(seed, pixel, (uint32_t)sample);
+    rng.n = 0;
+    float ox = rng.next_float(), oy = rng.next_float();
+    if (st) st[2]++;
+    // GenerateCameraSubpath (:42-57)
+    std::vector<BVert> cp, lp;
+    cp.reserve(16); lp.reserve(16);
+    Ray ray = s.cam.sample_ray((float)x + ox, (float)y + oy);
+    BVert c0;
+    c0.type = BV_CAMERA; c0.p = ray.o; c0.n = V3(0.0f); c0.coeff = C3(1.0f); c0.env = s.global_medium;
+    cp.push_back(c0);
+    int n_cam = bd_random_walk(s, ray, s.global_medium, rng, C3(1.0f), cp, I.max_depth - 1, false, st) + 1;
+    // GenerateLightSubpath (:59-82)
+    int n_light = 0;
+    Rng lr;
+    lr.key = bd_light_key(seed, pixel, (uint32_t)sample);
+    lr.n = 0;
+    if (!s.lights.empty()) {
+        int light = lr.next_int((int)s.lights.size());
+        float select_pdf = 1.0f / s.lights.size();
+        Emit e;
+        if (light_sample_emit(s, light, lr, &e) && e.pdf_pos != 0 && e.pdf_dir != 0 && !e.le.black()) {
+            const rd_light_desc& L = s.lights[light];
+            int lenv = -1;                       // Light.GetMedium (DiffuseAreaLight.cs:88-91; infinite: null)
+            if (L.type == RD_LIGHT_AREA) {
+                const rd_prim_desc& pd = s.prims[L.prim];
+                lenv = dot(e.n, e.dir) > 0 ? pd.medium_outside : pd.medium_inside;
+            }
+            BVert l0;
+            l0.type = BV_LIGHT; l0.light = light; l0.p = e.p; l0.n = e.n; l0.coeff = e.le; l0.env = lenv;
+            lp.push_back(l0);
+            C3 coeff = e.le * abs_dot(e.dir, e.n) / (select_pdf * e.pdf_pos * e.pdf_dir);
+            Ray lray(e.p, e.dir, 0.0001f);
+            n_light = bd_random_walk(s, lray, lenv, lr, coeff, lp, I.max_depth - 1, true, st) + 1;
+        }
+    }
+    // Connect
+    C3 splat(0.0f), l(0.0f);
+    for (int t = 1; t < n_cam; t++) {
+        for (int sn = 0; sn < n_light; sn++) {
+            int depth = t + sn - 2;
+            if ((sn == 1 && t == 1) || depth < 0 || depth > I.max_depth) continue;
+            if (only_s >= 0 && (sn != only_s || t != only_t)) continue;
+            C3 c = bd_connect(s, lp, sn, cp, t, seed, pixel, (uint32_t)sample, st);
+            if (!c.valid()) c = C3(0.0f);
+            if (t != 1) l = l + c;
+            else splat = splat + c;
+        }
+    }
+    return splat + l;
+}
+
 // ---------------------------------------------------------------- C API
 extern "C" {
 
@@ -2968,6 +3204,75 @@ int cr_render_particles(const cr_scene* s, uint64_t particles, uint32_t seed, in
     auto t1 = std::chrono::steady_clock::now();
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
     if (stats) for (int k = 0; k < 5; k++) stats[k] = st[k];
+    return RD_OK;
+}
+
+// BidirectionalPathTracingRenderer (:380-452): per pixel of the crop the
+// mean of its spp bidirectional samples (float64 sums over float32 sample
+// values), 32x32 blocks over `threads` workers; out_sum2 the float32 sums of
+// squares.  stats as cr_render.
+static int render_bdpt(const cr_scene* s, const rd_render_params* p, int32_t threads, float* out_rgb, float* out_sum2,
+                       uint64_t* stats, double* seconds, int only_s, int only_t);
+int cr_render_bdpt(const cr_scene* s, const rd_render_params* p, int32_t threads, float* out_rgb, float* out_sum2,
+                   uint64_t* stats, double* seconds) {
+    return render_bdpt(s, p, threads, out_rgb, out_sum2, stats, seconds, -1, -1);
+}
+// One strategy (s, t) of the bidirectional estimator alone.
+int cr_render_bdpt_strategy(const cr_scene* s, const rd_render_params* p, int32_t threads, int32_t strat_s,
+                            int32_t strat_t, float* out_rgb) {
+    if (strat_s < 0 || strat_t < 1) return RD_ERR_INVALID;
+    return render_bdpt(s, p, threads, out_rgb, nullptr, nullptr, nullptr, strat_s, strat_t);
+}
+static int render_bdpt(const cr_scene* s, const rd_render_params* p, int32_t threads, float* out_rgb, float* out_sum2,
+                       uint64_t* stats, double* seconds, int only_s, int only_t) {
+    if (!s || !p || !out_rgb) return RD_ERR_INVALID;
+    int W = s->cam.W, H = s->cam.H;
+    int x0 = p->x0, y0 = p->y0, x1 = p->x1, y1 = p->y1;
+    if (x0 == 0 && y0 == 0 && x1 == 0 && y1 == 0) { x1 = W; y1 = H; }
+    if (x0 < 0 || y0 < 0 || x1 > W || y1 > H || x0 >= x1 || y0 >= y1 || p->spp <= 0 || p->spp >= (1 << 29)) {
+        set_err("bad crop/spp");
+        return RD_ERR_INVALID;
+    }
+    const int bs = 32;
+    int nbx = (x1 - x0 + bs - 1) / bs, nby = (y1 - y0 + bs - 1) / bs;
+    int nblocks = nbx * nby;
+    std::atomic<int> next(0);
+    std::mutex mu;
+    uint64_t tot[5] = {0, 0, 0, 0, 0};
+    if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
+    auto t0 = std::chrono::steady_clock::now();
+    auto worker = [&]() {
+        uint64_t st[5] = {0, 0, 0, 0, 0};
+        while (true) {
+            int b = next.fetch_add(1);
+            if (b >= nblocks) break;
+            int bx = x0 + (b % nbx) * bs, by = y0 + (b / nbx) * bs;
+            int ex = std::min(bx + bs, x1), ey = std::min(by + bs, y1);
+            for (int x = bx; x < ex; x++) {
+                for (int y = by; y < ey; y++) {
+                    double sr = 0, sg = 0, sb = 0;
+                    C3 sum2(0.0f);
+                    for (int t = 0; t < p->spp; t++) {
+                        C3 c = bd_sample(*s, p->seed, x, y, t, st, only_s, only_t);
+                        sr += (double)c.r; sg += (double)c.g; sb += (double)c.b;
+                        sum2 = sum2 + c * c;
+                    }
+                    size_t o = ((size_t)x * H + y) * 3;
+                    const double n = (double)p->spp;
+                    out_rgb[o] = (float)(sr / n); out_rgb[o + 1] = (float)(sg / n); out_rgb[o + 2] = (float)(sb / n);
+                    if (out_sum2) { out_sum2[o] = sum2.r; out_sum2[o + 1] = sum2.g; out_sum2[o + 2] = sum2.b; }
+                }
+            }
+        }
+        std::lock_guard<std::mutex> g(mu);
+        for (int i = 0; i < 5; i++) tot[i] += st[i];
+    };
+    std::vector<std::thread> pool;
+    for (int i = 0; i < threads; i++) pool.emplace_back(worker);
+    for (auto& th : pool) th.join();
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (stats) for (int i = 0; i < 5; i++) stats[i] = tot[i];
     return RD_OK;
 }
 

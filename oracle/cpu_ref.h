@@ -108,6 +108,16 @@ uint64_t cr_path_key(uint32_t seed, uint32_t pixel, uint32_t sample);
  * column-major; stats as cr_render ([4] unused). */
 int cr_render_particles(const cr_scene* s, uint64_t particles, uint32_t seed, int32_t threads, float* out_rgb,
                         uint64_t* stats, double* seconds);
+/* Bidirectional path tracing (BidirectionalPathTracer under
+ * BidirectionalPathTracingRenderer): per pixel of the crop the mean of spp
+ * samples; the camera subpath draws from path_key(seed, pixel, sample), the
+ * light subpath from path_key(seed, pixel, sample | 2^30), the s = 1 light
+ * samples from path_key(seed, pixel, sample | 2^29).  As cr_render. */
+int cr_render_bdpt(const cr_scene* s, const rd_render_params* params, int32_t threads, float* out_rgb,
+                   float* out_sum2, uint64_t* stats, double* seconds);
+/* The same with strategy (s, t) of Connect alone (every other one 0). */
+int cr_render_bdpt_strategy(const cr_scene* s, const rd_render_params* params, int32_t threads, int32_t strat_s,
+                            int32_t strat_t, float* out_rgb);
 /* Camera.SampleWi of a world point: out6 = {ok, screen x, screen y, We,
  * pdf, distance}; Camera.Importance of a camera-space unit direction. */
 int cr_camera_sample_wi(const cr_scene* s, const float* pos, float* out6);

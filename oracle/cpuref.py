@@ -78,6 +78,10 @@ def lib():
         L.cr_rho_ratio.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.cr_path_key.restype = C.c_uint64
         L.cr_path_key.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+        L.cr_render_bdpt.argtypes = [C.c_void_p, C.POINTER(abi.rd_render_params), C.c_int32, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p]
+        L.cr_render_bdpt_strategy.argtypes = [C.c_void_p, C.POINTER(abi.rd_render_params), C.c_int32, C.c_int32,
+                                              C.c_int32, C.c_void_p]
         L.cr_render_particles.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_int32, C.c_void_p, C.c_void_p,
                                           C.c_void_p]
         L.cr_camera_sample_wi.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
@@ -284,6 +288,34 @@ class OracleScene:
         stats = dict(rays_closest=int(st[0]), rays_any=int(st[1]), paths=int(st[2]), vertices=int(st[3]),
                      invalid=int(st[4]))
         return rgb, s2, stats, secs.value
+
+    def render_bdpt(self, spp, seed=0, crop=None, threads=0, with_sum2=True):
+        """Bidirectional path tracing: (mean rgb [W*H*3], sum2, stats dict, seconds)."""
+        p = abi.rd_render_params()
+        p.spp, p.seed = spp, seed
+        if crop is not None:
+            p.x0, p.y0, p.x1, p.y1 = crop
+        W, H = self.width, self.height
+        rgb = np.zeros(W * H * 3, np.float32)
+        s2 = np.zeros(W * H * 3, np.float32) if with_sum2 else None
+        st = np.zeros(5, np.uint64)
+        secs = C.c_double(0)
+        rc = lib().cr_render_bdpt(self.h, C.byref(p), threads, _ptr(rgb), _ptr(s2) if s2 is not None else None,
+                                  _ptr(st), C.byref(secs))
+        if rc != 0:
+            raise RuntimeError("oracle bdpt render: " + lib().cr_last_error().decode())
+        stats = dict(rays_closest=int(st[0]), rays_any=int(st[1]), paths=int(st[2]), vertices=int(st[3]))
+        return rgb, s2, stats, secs.value
+
+    def render_bdpt_strategy(self, spp, s, t, seed=0, threads=0):
+        """Strategy (s, t) of the bidirectional estimator alone: mean rgb [W*H*3]."""
+        p = abi.rd_render_params()
+        p.spp, p.seed = spp, seed
+        rgb = np.zeros(self.width * self.height * 3, np.float32)
+        rc = lib().cr_render_bdpt_strategy(self.h, C.byref(p), threads, s, t, _ptr(rgb))
+        if rc != 0:
+            raise RuntimeError("oracle bdpt strategy render failed")
+        return rgb
 
     def render_particles(self, particles, seed=0, threads=0):
         """Light tracing (adjoint_particle): (rgb [W*H*3] column-major, stats dict, seconds)."""

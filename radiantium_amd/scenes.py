@@ -52,17 +52,19 @@ def _box(size, center, angle_deg):
     return pos, np.array(idx, np.int32)
 
 
-def cornell_box(width=256, height=256, max_depth=8, method="Mis"):
+def cornell_box(width=256, height=256, max_depth=8, method="Mis", closed=False, light_kd=0.0):
     """Plain path integrator Cornell box: five unit wall quads (grey 0.73,
     red 0.65/0.05/0.05, green 0.12/0.45/0.15), boxes of size 0.3 and 0.6
     (0.3 x 0.6 x 0.3) rotated +-17 degrees about y, a 0.25 x 0.25 ceiling
     quad light of radiance (17,12,4); all Lambertian, one-sided.
-    Room: x in [-0.5,0.5], y in [0,1], z in [-0.5,0.5], open at +z."""
+    Room: x in [-0.5,0.5], y in [0,1], z in [-0.5,0.5], open at +z.
+    closed: a grey front wall at z = 0.5 and the camera inside the room (no
+    path leaves it); light_kd: the light's own diffuse albedo."""
     s = SceneDesc()
     grey = s.add_material("diffuse", kd=(0.73, 0.73, 0.73))
     red = s.add_material("diffuse", kd=(0.65, 0.05, 0.05))
     green = s.add_material("diffuse", kd=(0.12, 0.45, 0.15))
-    light_mat = s.add_material("diffuse", kd=(0.0, 0.0, 0.0))
+    light_mat = s.add_material("diffuse", kd=(light_kd, light_kd, light_kd))
     L, H = -0.5, 0.5
     # floor (normal +y)
     s.add_mesh(*_quad((L, 0, L), (L, 0, H), (H, 0, H), (H, 0, L)), material=grey)
@@ -81,8 +83,14 @@ def cornell_box(width=256, height=256, max_depth=8, method="Mis"):
     q = 0.125
     y = 1.0 - 1e-3
     s.add_mesh(*_quad((-q, y, -q), (q, y, -q), (q, y, q), (-q, y, q)), material=light_mat, le=(17.0, 12.0, 4.0))
-    s.set_camera(fov=40.0, width=width, height=height, origin=(0.0, 0.5, 1.85), target=(0.0, 0.5, 0.0),
-                 up=(0.0, 1.0, 0.0))
+    if closed:
+        # front wall z=+0.5 (normal -z), the camera just inside it
+        s.add_mesh(*_quad((L, 0, H), (L, 1, H), (H, 1, H), (H, 0, H)), material=grey)
+        s.set_camera(fov=70.0, width=width, height=height, origin=(0.0, 0.5, 0.45), target=(0.0, 0.45, 0.0),
+                     up=(0.0, 1.0, 0.0))
+    else:
+        s.set_camera(fov=40.0, width=width, height=height, origin=(0.0, 0.5, 1.85), target=(0.0, 0.5, 0.0),
+                     up=(0.0, 1.0, 0.0))
     s.set_integrator(type="path", max_depth=max_depth, method=method)
     s.bvh_max_prim = 1
     return s
