@@ -1791,6 +1791,135 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
     return rc;
 }
 
+// BidirectionalPathTracingRenderer (Renderers/BidirectionalPathTracingRenderer.cs:380-452)
+// on the device: passes of whole sample slices of the crop, each pass's
+// subpaths walked together, its strategies connected in parallel and their
+// visibility traced as Transmittance chains; per-pixel float64 sums in
+// sample order, then the mean.
+int render_bdpt(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d_out, float* d_out_sum2, double* out64,
+                volatile int32_t* cancel, volatile int32_t* progress) {
+    if (int rc = check_stack(s)) return rc;
+    if (p->spp <= 0 || p->spp >= (1 << 29)) return fail(RD_ERR_INVALID, "bdpt spp must be in [1, 2^29)");
+    Crop cr;
+    if (int rc = resolve_crop(s, p, &cr)) return rc;
+    const int crop_w = cr.x1 - cr.x0, crop_h = cr.y1 - cr.y0;
+    const int npix = crop_w * crop_h;
+    const int walk_cap = s->dev.integ.max_depth - 1;
+    const int vmax = walk_cap == 0 ? 1 : 1 + std::max(1, walk_cap - 1);
+    if (vmax > 256) return fail(RD_ERR_UNSUPPORTED, "bdpt max_depth above 257");
+    const int nstrat = (vmax - 1) * vmax;
+    // bytes per sample: vertices, walkers and their queues, strategy slots
+    // with their chain state, queues and hits
+    const size_t per_sample = 2 * (size_t)vmax * 68 + 2 * (36 * 2 + 32 + 20 + 8) + (size_t)nstrat * (48 + 72 + 20) + 16;
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    const size_t budget = std::min<size_t>(free_b / 4, (size_t)6 << 30);
+    size_t cap = std::max<size_t>(budget / per_sample, (size_t)npix);
+    if (p->wave_paths > 0) cap = std::max<size_t>((size_t)p->wave_paths, (size_t)npix);
+    const int ns_pass = (int)std::max<size_t>(1, std::min<size_t>(cap / npix, (size_t)p->spp));
+    const size_t Pmax = (size_t)npix * ns_pass;
+    if (Pmax > (size_t)INT32_MAX / 4 || Pmax * std::max(nstrat, 2) > (size_t)INT32_MAX)
+        return fail(RD_ERR_INVALID, "bdpt pass too large");
+    const size_t qw = 2 * Pmax, qc = std::max<size_t>(Pmax * nstrat, 1);
+    const size_t nv = 2 * (size_t)vmax * Pmax;
+    hipStream_t st = c->stream;
+    ScopedAllocs A;
+    BdptBuffers b{};
+    b.vmax = vmax; b.nstrat = nstrat; b.walk_cap = walk_cap;
+    b.v0 = A.get<float4>(nv); b.v1 = A.get<float4>(nv); b.v2 = A.get<float4>(nv); b.v3 = A.get<float4>(nv);
+    b.v4 = A.get<float>(nv);
+    b.nverts = A.get<int>(2 * Pmax);
+    b.wcoeff = A.get<float4>(qw); b.wenv = A.get<int>(qw); b.wkey = A.get<uint64_t>(qw); b.wn = A.get<uint32_t>(qw);
+    b.conn = A.get<float4>(qc); b.cto = A.get<float4>(qc); b.ctr = A.get<float4>(qc);
+    const size_t qmax = std::max(qw, qc);
+    RayQueue qa{nullptr, A.get<int>(qmax), A.get<float4>(qmax), A.get<float4>(qmax)};
+    RayQueue qb{nullptr, A.get<int>(qmax), A.get<float4>(qmax), A.get<float4>(qmax)};
+    float4* hits = A.get<float4>(qmax);
+    int* hsub = s->dev.num_insts > 0 ? A.get<int>(qmax) : nullptr;
+    float4* res = A.get<float4>(Pmax);
+    double4* sum = A.get<double4>(npix);
+    float4* sum2 = A.get<float4>(npix);
+    if (!b.v0 || !b.v1 || !b.v2 || !b.v3 || !b.v4 || !b.nverts || !b.wcoeff || !b.wenv || !b.wkey || !b.wn ||
+        !b.conn || !b.cto || !b.ctr || !qa.pid || !qa.org || !qa.dir || !qb.pid || !qb.org || !qb.dir || !hits ||
+        (s->dev.num_insts > 0 && !hsub) || !res || !sum || !sum2)
+        return fail(RD_ERR_OOM, "hipMalloc failed for the bidirectional render");
+    PathState tps{};
+    tps.hsub = hsub;
+    Timer tm(c);
+    hipEvent_t ev0 = c->ev_t0, ev1 = c->ev_t1;
+    HIP_TRY(hipEventRecord(ev0, st));
+    HIP_TRY(launch_zero_counts(c->counts, kCntSlots, st));
+    HIP_TRY(launch_zero(sum, npix, st));
+    HIP_TRY(launch_zero(reinterpret_cast<float4*>(sum2), npix, st));
+    BdptPass w{};
+    w.x0 = cr.x0; w.y0 = cr.y0; w.crop_w = crop_w; w.npix = npix; w.width = s->width; w.seed = p->seed;
+    int passes = 0, max_bounce = 0;
+    int rc = RD_OK;
+    // runs a queue loop: trace the queue at cin, then `stage` into the other
+    // queue, until a queue is empty
+    auto queue_loop = [&](const char* stage, int sa, int sb, auto&& launch_stage) -> int {
+        RayQueue qin = qa, qout = qb;
+        int cin = sa, cout = sb;
+        for (int step = 0;; step++) {
+            HIP_TRY(hipMemcpyAsync(c->host_counts + cin, c->counts + cin, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            if (c->host_counts[cin] == 0) break;
+            if (step >= kMaxChainSteps) return fail(RD_ERR_DEVICE, "bidirectional queue did not drain");
+            if (sa == kCntExtA) max_bounce = std::max(max_bounce, step + 1);
+            HIP_TRY(tm.run("trace_closest", [&] {
+                return launch_trace_ext(s->dev, nullptr, qin, tps, hits, c->counts, cin, -1, -1, -1, nullptr, nullptr,
+                                        0, st);
+            }));
+            HIP_TRY(launch_zero_slots(c->counts, cout, cout, st));
+            HIP_TRY(tm.run(stage, [&] { return launch_stage(qin, cin, qout, cout); }));
+            std::swap(qin, qout);
+            std::swap(cin, cout);
+        }
+        return RD_OK;
+    };
+    for (int base = 0; base < p->spp && !rc; base += ns_pass) {
+        const int ns = std::min(ns_pass, p->spp - base);
+        b.P = npix * ns;
+        w.sample_base = base;
+        HIP_TRY(launch_zero_slots(c->counts, kCntExtA, kCntExtB, st));
+        HIP_TRY(tm.run("bd_start", [&] { return launch_bd_start(s->dev, w, b, qa, c->counts, kCntExtA, st); }));
+        rc = queue_loop("bd_walk", kCntExtA, kCntExtB, [&](RayQueue qi, int ci, RayQueue qo, int co) {
+            return launch_bd_walk(s->dev, b, qi, ci, hits, hsub, qo, co, c->counts, st);
+        });
+        if (rc) break;
+        if (nstrat > 0) {
+            HIP_TRY(launch_zero_slots(c->counts, kCntShadow, kCntMis, st));
+            HIP_TRY(tm.run("bd_connect", [&] {
+                return launch_bd_connect(s->dev, s->lt_cam, w, b, qa, kCntShadow, c->counts, st);
+            }));
+            rc = queue_loop("bd_chain", kCntShadow, kCntMis, [&](RayQueue qi, int ci, RayQueue qo, int co) {
+                return launch_bd_chain(s->dev, b, qi, ci, hits, hsub, qo, co, c->counts, st);
+            });
+            if (rc) break;
+        } else {
+            HIP_TRY(launch_zero(b.conn, 1, st));
+        }
+        HIP_TRY(tm.run("bd_sum", [&] { return launch_bd_sum(b, res, st); }));
+        HIP_TRY(tm.run("accumulate", [&] { return launch_bd_accumulate(w, ns, res, sum, sum2, st); }));
+        passes++;
+        if (progress) *progress = passes;
+        if (cancel && *cancel) rc = fail(RD_ERR_CANCELLED, "render cancelled");
+    }
+    if (!rc) {
+        HIP_TRY(tm.run("finalize", [&] {
+            return launch_bd_finalize(w, s->height, p->spp, sum, sum2, d_out, d_out_sum2, out64, st);
+        }));
+    }
+    HIP_TRY(hipEventRecord(ev1, st));
+    HIP_TRY(hipMemcpyAsync(c->host_counts, c->counts, kCntSlots * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    tm.collect();
+    collect_stats(c, (uint64_t)npix * (uint64_t)p->spp, ms, passes, max_bounce);
+    return rc;
+}
+
 // AdjointParticleRenderer (Renderers/AdjointParticleRenderer.cs:127-178) on
 // the device: waves of particles (particle i draws from
 // particle_key(seed, i), so the image does not depend on the wave size),
@@ -1901,6 +2030,10 @@ int rd_render_device(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d
         if (d_out_sum2) return fail(RD_ERR_INVALID, "the particle renderer has no per-pixel sample sums");
         (void)hipSetDevice(c->device);
         return render_particles(c, s, p, d_out_rgb, cancel, progress);
+    }
+    if (s->dev.integ.type == RD_INTEGRATOR_BDPT) {
+        (void)hipSetDevice(c->device);
+        return render_bdpt(c, s, p, d_out_rgb, d_out_sum2, nullptr, cancel, progress);
     }
     rd_request r{};
     r.scene = s;
@@ -2166,7 +2299,8 @@ int rd_render_mean_f64(rd_ctx* c, rd_scene* s, const rd_render_params* p, double
     rd_request r{};
     r.scene = s;
     r.params = *p;
-    if (!rc) rc = render_requests(c, 1, &r, true, p->wave_paths, nullptr, nullptr, false, d);
+    if (!rc && s->dev.integ.type == RD_INTEGRATOR_BDPT) rc = render_bdpt(c, s, p, nullptr, nullptr, d, nullptr, nullptr);
+    else if (!rc) rc = render_requests(c, 1, &r, true, p->wave_paths, nullptr, nullptr, false, d);
     if (!rc && hipMemcpy(out_mean, d, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
         rc = fail(RD_ERR_DEVICE, "copy output");
     (void)hipFree(d);

@@ -52,4 +52,52 @@ hipError_t launch_lt_corner_rays(const DevScene& sc, RayQueue q, int* counts, in
 hipError_t launch_lt_final(const DevScene& sc, RayQueue q, const float4* hits, const int* hsub, const double* acc,
                            float v, float* out, hipStream_t s);
 
+// ---------------------------------------------------------------- bidirectional (rd_bdpt.hip)
+// One pass of bidirectional samples: every pixel of the crop (raster order)
+// times samples [sample_base, sample_base + P / npix).
+struct BdptPass {
+    int x0, y0, crop_w, npix;
+    int width;          // image width (pixel id = y * width + x)
+    int sample_base;
+    uint32_t seed;
+    int _pad;
+};
+
+// Per-pass device storage.  Vertex slot k of subpath `side` (0 camera,
+// 1 light) of sample i sits at (side * vmax + k) * P + i in five arrays:
+//   v0 {p, type}  v1 {n, medium}  v2 {coeff, aux}  v3 {w, u}  v4 v
+// (aux: a surface vertex's primitive, a light vertex's light or -1; w: a
+// surface vertex's Wr, a medium vertex's Wo).  Strategy slot kk = (t - 1) *
+// vmax + s of sample i at kk * P + i.
+struct BdptBuffers {
+    int P;              // samples in the pass
+    int vmax;           // vertex slots per subpath
+    int nstrat;         // (vmax - 1) * vmax strategy slots per sample
+    int walk_cap;       // RandomWalk's maxDepth: the integrator's max_depth - 1
+    float4 *v0, *v1, *v2, *v3;
+    float* v4;
+    int* nverts;        // [2][P]: nCamera, nLight
+    float4* wcoeff;     // [2P] walkers (camera i, light P + i): coeff, w = bounces
+    int* wenv;
+    uint64_t* wkey;
+    uint32_t* wn;
+    float4* conn;       // strategy value without visibility; w = the factor g the transmittance is scaled by
+    float4* cto;        // Transmittance chain: target, w = current medium
+    float4* ctr;        // Transmittance chain: accumulated Tr
+};
+
+hipError_t launch_bd_start(const DevScene& sc, const BdptPass& w, const BdptBuffers& b, RayQueue q, int* counts,
+                           int cnt_q, hipStream_t s);
+hipError_t launch_bd_walk(const DevScene& sc, const BdptBuffers& b, RayQueue qin, int cnt_in, const float4* hits,
+                          const int* hsub, RayQueue qout, int cnt_out, int* counts, hipStream_t s);
+hipError_t launch_bd_connect(const DevScene& sc, const LtCamera& cam, const BdptPass& w, const BdptBuffers& b,
+                             RayQueue qc, int cnt_c, int* counts, hipStream_t s);
+hipError_t launch_bd_chain(const DevScene& sc, const BdptBuffers& b, RayQueue qin, int cnt_in, const float4* hits,
+                           const int* hsub, RayQueue qout, int cnt_out, int* counts, hipStream_t s);
+hipError_t launch_bd_sum(const BdptBuffers& b, float4* res, hipStream_t s);
+hipError_t launch_bd_accumulate(const BdptPass& w, int nsamples, const float4* res, double4* sum, float4* sum2,
+                                hipStream_t s);
+hipError_t launch_bd_finalize(const BdptPass& w, int height, int spp, const double4* sum, const float4* sum2,
+                              float* out, float* out_sum2, double* out64, hipStream_t s);
+
 }  // namespace rd
