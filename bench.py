@@ -388,6 +388,93 @@ def instanced_throughput(ctx, args):
                         f"path Mis, {W}x{H}, {spp} spp, max depth 16"}
 
 
+def light_paths(ctx, args):
+    """The light-path integrators on the Cornell box of the plain-path
+    workload (config-0 geometry): light tracing (adjoint_particle: 2^24
+    particles, 512x512, the reference's default depth rule max_depth -1,
+    Russian roulette 0.99 after 3 bounces) and bidirectional path tracing
+    (bdpt: 256x256, 16 spp, the reference's default max_depth 7).  One timed
+    render each after a warmup, and the oracle on a bounded sample of the
+    same work (2^23 particles; the whole frame at 16 spp)."""
+    from radiantium_amd import native, scenes
+    from oracle import cpuref
+    out = {}
+    threads = host_cores()
+    if not args.no_cpu_baseline and not os.path.exists(cpuref.LIB_PATH):
+        cpuref.build()
+    # light tracing
+    desc = scenes.cornell_box(512, 512)
+    desc.integrator.update(type="adjoint_particle", max_depth=-1, min_depth=3)
+    scene = native.Scene(ctx, desc)
+    n = args.particles
+    d_rgb = ctx.alloc(512 * 512 * 12)
+    scene.render_particles_device(n // 16, d_rgb, seed=99)
+    ctx.synchronize()
+    ctx.reset_kernel_timing()
+    t0 = time.perf_counter()
+    scene.render_particles_device(n, d_rgb, seed=1)
+    ctx.synchronize()
+    secs = time.perf_counter() - t0
+    st = ctx.render_stats()
+    rays = st["rays_closest"] + st["rays_any"]
+    stages = {}
+    for name in ("lt_emit", "trace_closest", "shade", "trace_any", "splat", "finalize"):
+        avg, cnt = ctx.kernel_timing(name)
+        if cnt:
+            stages[name] = round(avg * cnt, 2)
+    lt = {"mrays_s": round(rays / secs / 1e6, 1), "ms": round(secs * 1000, 1), "rays": rays, "particles": n,
+          "mparticles_s": round(n / secs / 1e6, 1), "stage_ms": stages,
+          "workload": "Cornell box 512x512, adjoint_particle, max_depth -1, min_depth 3, rr 0.99"}
+    log(f"[bench] light tracing: {n} particles in {secs * 1000:.1f} ms, {lt['mrays_s']} Mrays/s, stages {stages}")
+    if not args.no_cpu_baseline:
+        cn = 1 << 23
+        _, cst, csecs = cpuref.OracleScene(desc).render_particles(cn, seed=1, threads=threads)
+        crays = cst["rays_closest"] + cst["rays_any"]
+        lt["cpu_baseline"] = {"value": round(crays / csecs / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+                              "kind": "port", "sample": f"{cn} particles: {crays} BVH queries in {csecs:.2f} s"}
+        lt["speedup_vs_cpu"] = round(lt["mrays_s"] / lt["cpu_baseline"]["value"], 1)
+        log(f"[cpu] light tracing: {lt['cpu_baseline']['value']} Mrays/s")
+    ctx.free(d_rgb)
+    scene.close()
+    out["light_tracing"] = lt
+    # bidirectional
+    desc = scenes.cornell_box(256, 256)
+    desc.integrator.update(type="bdpt", max_depth=7, min_depth=5)
+    scene = native.Scene(ctx, desc)
+    spp = 16
+    d_rgb = ctx.alloc(256 * 256 * 12)
+    scene.render_device(2, d_rgb, seed=99)
+    ctx.synchronize()
+    ctx.reset_kernel_timing()
+    t0 = time.perf_counter()
+    scene.render_device(spp, d_rgb, seed=1)
+    ctx.synchronize()
+    secs = time.perf_counter() - t0
+    st = ctx.render_stats()
+    rays = st["rays_closest"] + st["rays_any"]
+    stages = {}
+    for name in ("bd_start", "trace_closest", "bd_walk", "bd_connect", "bd_chain", "bd_sum", "accumulate"):
+        avg, cnt = ctx.kernel_timing(name)
+        if cnt:
+            stages[name] = round(avg * cnt, 2)
+    bd = {"mrays_s": round(rays / secs / 1e6, 1), "ms": round(secs * 1000, 1), "rays": rays,
+          "msamples_s": round(256 * 256 * spp / secs / 1e6, 2), "stage_ms": stages,
+          "workload": f"Cornell box 256x256, bdpt, {spp} spp, max_depth 7"}
+    log(f"[bench] bdpt: {secs * 1000:.1f} ms, {bd['mrays_s']} Mrays/s, stages {stages}")
+    if not args.no_cpu_baseline:
+        _, _, cst, csecs = cpuref.OracleScene(desc).render_bdpt(spp, seed=1, threads=threads, with_sum2=False)
+        crays = cst["rays_closest"] + cst["rays_any"]
+        bd["cpu_baseline"] = {"value": round(crays / csecs / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+                              "kind": "port",
+                              "sample": f"the whole frame at {spp} spp: {crays} BVH queries in {csecs:.2f} s"}
+        bd["speedup_vs_cpu"] = round(bd["mrays_s"] / bd["cpu_baseline"]["value"], 1)
+        log(f"[cpu] bdpt: {bd['cpu_baseline']['value']} Mrays/s")
+    ctx.free(d_rgb)
+    scene.close()
+    out["bdpt"] = bd
+    return out
+
+
 def copy_calibration():
     """A plain device copy of known size, profiled beside the path kernels so
     FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
@@ -424,6 +511,9 @@ def main():
     ap.add_argument("--adj-seed", type=int, default=9, help="texture seed of the adjoint target")
     ap.add_argument("--adj-reps", type=int, default=3, help="timed primal/adjoint pairs")
     ap.add_argument("--adj-steps", type=int, default=6, help="optimisation steps through the autograd Function")
+    ap.add_argument("--no-light", action="store_true", help="skip the light tracing / bdpt measurement")
+    ap.add_argument("--only-light", action="store_true", help="only the light tracing / bdpt measurement")
+    ap.add_argument("--particles", type=int, default=1 << 24, help="particles of the light tracing leg")
     ap.add_argument("--only-instanced", action="store_true",
                     help="run the instanced-workload frame alone (for profiling); prints its record")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -442,6 +532,9 @@ def main():
     if args.only_instanced:
         args.no_cpu_baseline = True
         print(json.dumps({"instanced": instanced_throughput(native.Context(0), args)}), flush=True)
+        return
+    if args.only_light:
+        print(json.dumps({"light_paths": light_paths(native.Context(0), args)}), flush=True)
         return
     t0 = time.perf_counter()
     desc = make_scene(args)
@@ -543,6 +636,10 @@ def main():
     if not args.no_close:
         close = close_framing(ctx, args)
 
+    lightp = None
+    if not args.no_light:
+        lightp = light_paths(ctx, args)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(desc, args, value, scene)
@@ -560,7 +657,7 @@ def main():
                    "width": args.width, "height": args.height, "spp": args.spp, "max_depth": args.max_depth,
                    "triangles": ntri, "parallelism": "1 GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "adjoint": adjoint,
-        "instanced": instanced, "close_framing": close,
+        "instanced": instanced, "close_framing": close, "light_paths": lightp,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

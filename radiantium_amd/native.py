@@ -555,6 +555,12 @@ class Scene:
             p.x0, p.y0, p.x1, p.y1 = crop
         check(lib().rd_render_device(self.ctx.h, self.h, C.byref(p), d_rgb, d_sum2, None, None))
 
+    def render_particles_device(self, particles, d_rgb, seed=0, wave_paths=0):
+        """Light tracing into a device image (rd_render_device with params.particles)."""
+        p = abi.rd_render_params()
+        p.seed, p.wave_paths, p.particles = seed, wave_paths, particles
+        check(lib().rd_render_device(self.ctx.h, self.h, C.byref(p), d_rgb, None, None, None))
+
     def render_device_noalloc(self, spp, d_rgb, d_sum2=None, seed=0, crop=None, wave_paths=0):
         """rd_render_device_noalloc: as render_device, with no device allocation
         (work buffers from the library's fixed per-device work area)."""
