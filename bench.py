@@ -475,6 +475,80 @@ def light_paths(ctx, args):
     return out
 
 
+def octree_leg(ctx, args):
+    """The same frame through the loose octree accelerator (the config's
+    "octree" accel, Accelerators/Octree.cs, its defaults max_depth 10,
+    max_count 10, out_bound 2) at fewer spp: octree build time, Mrays/s,
+    box and primitive tests per ray, the closest-hit traversal's
+    algorithmic bytes per launch over its launch time, and the oracle's
+    octree on a bounded crop.  The octree drops the primitives whose bound
+    lies flat on a cell face (here the ground and light quads, on the root's
+    faces), as the reference's does, so its frame is not the BVH frame."""
+    from radiantium_amd import native
+    from oracle import cpuref
+    spp = args.oct_spp
+    desc = make_scene(args)
+    desc.accel = dict(type="octree", max_depth=10, max_count=10, out_bound=2.0)
+    t0 = time.perf_counter()
+    scene = native.Scene(ctx, desc)
+    create_s = time.perf_counter() - t0
+    nodes, first, count, prims = scene.export_octree()
+    npr = desc.num_prims()
+    kept = int(np.unique(prims).size)
+    d_rgb = ctx.alloc(args.width * args.height * 12)
+    scene.render_device(1, d_rgb, seed=99)
+    ctx.synchronize()
+    ctx.reset_kernel_timing()
+    t0 = time.perf_counter()
+    scene.render_device(spp, d_rgb, seed=1)
+    ctx.synchronize()
+    secs = time.perf_counter() - t0
+    st = ctx.render_stats()
+    rays = st["rays_closest"] + st["rays_any"]
+    nodes_c, prims_c, rays_c = st["class_nodes"], st["class_prims"], st["class_rays"]
+    stages = {}
+    for name in ("trace_closest", "shade", "trace_any", "trace_mis", "resolve"):
+        avg, cnt = ctx.kernel_timing(name)
+        if cnt:
+            stages[name] = (avg, cnt)
+    oc = {"mrays_s": round(rays / secs / 1e6, 1), "ms": round(secs * 1000, 1), "rays": rays, "spp": spp,
+          "scene_create_s": round(create_s, 2), "nodes": int(nodes.shape[0]), "leaves": int(first.size),
+          "leaf_entries": int(prims.size), "primitives_kept": kept, "primitives": npr,
+          "box_tests_per_ray": round(sum(nodes_c) / max(sum(rays_c), 1), 2),
+          "prim_tests_per_ray": round(sum(prims_c) / max(sum(rays_c), 1), 2),
+          "stage_ms": {k: round(a * c, 2) for k, (a, c) in stages.items()},
+          "workload": f"the icosphere frame ({args.width}x{args.height}, max depth {args.max_depth}) at {spp} spp "
+                      f"through the octree accel (max_depth 10, max_count 10, out_bound 2)"}
+    if "trace_closest" in stages:
+        avg, cnt = stages["trace_closest"]
+        byts = rays_c[0] * (RAY_IN + 16) + 32 * nodes_c[0] + 36 * prims_c[0]
+        ach = byts / cnt / (avg * 1e-3) / 1e9
+        oc["roofline"] = {"kernel": "trace_closest (k_trace_oct)", "bound": "hbm", "achieved": round(ach, 1),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                          "bytes_per_launch": int(byts / cnt), "avg_launch_ms": round(avg, 4),
+                          "achieved_is": "32 B ray + 16 B hit + 32 B per box test + 36 B per triangle test, "
+                                         "counted by the kernel, per launch / launch time"}
+    log(f"[bench] octree: {secs * 1000:.1f} ms at {spp} spp, {oc['mrays_s']} Mrays/s, "
+        f"{oc['box_tests_per_ray']} boxes / {oc['prim_tests_per_ray']} triangles per ray, "
+        f"{kept}/{npr} primitives kept, create {create_s:.2f}s")
+    if not args.no_cpu_baseline:
+        if not os.path.exists(cpuref.LIB_PATH):
+            cpuref.build()
+        threads = host_cores()
+        o = cpuref.OracleScene(desc)
+        cspp = 4 * spp
+        _, _, cst, csecs = o.render(cspp, seed=1, threads=threads, with_sum2=False)
+        crays = cst["rays_closest"] + cst["rays_any"]
+        oc["cpu_baseline"] = {"value": round(crays / csecs / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+                              "kind": "port", "sample": f"the whole frame at {cspp} spp: {crays} octree "
+                                                        f"queries in {csecs:.2f} s"}
+        oc["speedup_vs_cpu"] = round(oc["mrays_s"] / oc["cpu_baseline"]["value"], 1)
+        log(f"[cpu] octree: {oc['cpu_baseline']['value']} Mrays/s")
+    ctx.free(d_rgb)
+    scene.close()
+    return oc
+
+
 def copy_calibration():
     """A plain device copy of known size, profiled beside the path kernels so
     FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
@@ -514,6 +588,9 @@ def main():
     ap.add_argument("--no-light", action="store_true", help="skip the light tracing / bdpt measurement")
     ap.add_argument("--only-light", action="store_true", help="only the light tracing / bdpt measurement")
     ap.add_argument("--particles", type=int, default=1 << 24, help="particles of the light tracing leg")
+    ap.add_argument("--no-octree", action="store_true", help="skip the octree-accelerator measurement")
+    ap.add_argument("--only-octree", action="store_true", help="only the octree-accelerator measurement")
+    ap.add_argument("--oct-spp", type=int, default=4, help="spp of the octree frame")
     ap.add_argument("--only-instanced", action="store_true",
                     help="run the instanced-workload frame alone (for profiling); prints its record")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -535,6 +612,9 @@ def main():
         return
     if args.only_light:
         print(json.dumps({"light_paths": light_paths(native.Context(0), args)}), flush=True)
+        return
+    if args.only_octree:
+        print(json.dumps({"octree": octree_leg(native.Context(0), args)}), flush=True)
         return
     t0 = time.perf_counter()
     desc = make_scene(args)
@@ -640,6 +720,10 @@ def main():
     if not args.no_light:
         lightp = light_paths(ctx, args)
 
+    octree = None
+    if not args.no_octree:
+        octree = octree_leg(ctx, args)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(desc, args, value, scene)
@@ -657,7 +741,7 @@ def main():
                    "width": args.width, "height": args.height, "spp": args.spp, "max_depth": args.max_depth,
                    "triangles": ntri, "parallelism": "1 GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "adjoint": adjoint,
-        "instanced": instanced, "close_framing": close, "light_paths": lightp,
+        "instanced": instanced, "close_framing": close, "light_paths": lightp, "octree": octree,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

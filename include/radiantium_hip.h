@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define RD_ABI_VERSION 3
+#define RD_ABI_VERSION 4
 
 typedef enum rd_status {
     RD_OK = 0,
@@ -213,7 +213,17 @@ typedef struct rd_scene_desc {
     const rd_instance_set_desc* instance_sets;
     int32_t num_instances;
     const rd_instance_desc* instances;
+    /* accelerator of the scene's primitives (the config's "accel",
+     * RendererBuilder.cs:86-87): RD_ACCEL_BVH (bvh_max_prim above) or
+     * RD_ACCEL_OCTREE (Accelerators/Octree.cs; not with instances) */
+    int32_t accel;
+    int32_t octree_max_depth;    /* octree "max_depth", default 10          */
+    int32_t octree_max_count;    /* octree "max_count", default 10          */
+    float octree_out_bound;      /* octree "out_bound", default 2           */
 } rd_scene_desc;
+
+#define RD_ACCEL_BVH 0
+#define RD_ACCEL_OCTREE 1
 
 /* ---- rays for the traversal oracle hooks -------------------------------- */
 
@@ -335,6 +345,22 @@ int rd_image_save_png(const char* path, const float* data, int width, int height
 /* No counterpart in the reference: PFM (float RGB, little-endian, rows from
  * the bottom, so ColorBuffer row 0 first), for lossless float output. */
 int rd_image_save_pfm(const char* path, const float* data, int width, int height, int channels);
+
+/* The loose octree of Accelerators/Octree.cs (ctor :48-93, RecursionBuild
+ * :197-281, FlattenNode :283-315) over primitive bounds, on the host; needs
+ * no device.  sizes3 receives {nodes, leaves, leaf entries}; with a null
+ * buffer (or one too small) nothing is copied and RD_OK / RD_ERR_INVALID
+ * tells, so a first call with nulls sizes the buffers.  nodes32: 32-byte
+ * LinearNode records (bound, NodeCount or LeafIndex, NextHead = -1 for
+ * leaves); leaf i holds leaf_prims[leaf_first[i] .. + leaf_count[i]). */
+int rd_octree_build_host(const float* bmin, const float* bmax, int32_t n, int32_t max_depth, int32_t max_count,
+                         float out_bound, void* nodes32, int32_t max_nodes, int32_t* leaf_first, int32_t* leaf_count,
+                         int32_t max_leaves, int32_t* leaf_prims, int32_t max_leaf_prims, int32_t* sizes3);
+/* The octree rd_scene_create built for an RD_ACCEL_OCTREE scene, in the
+ * same form (RD_ERR_INVALID for a BVH scene). */
+int rd_scene_export_octree(const rd_scene* scene, void* nodes32, int32_t max_nodes, int32_t* leaf_first,
+                           int32_t* leaf_count, int32_t max_leaves, int32_t* leaf_prims, int32_t max_leaf_prims,
+                           int32_t* sizes3);
 
 /* Bvh.RecursiveBuild + FlattenBVHTree (Bvh.cs:234-442) on the device, the
  * tree rd_bvh_build_host builds, node for node: device bounds in (d_bmin,

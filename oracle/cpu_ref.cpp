@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -386,6 +387,17 @@ struct cr_scene {
     std::vector<int> order;
     int leaf_count = 0;
     int max_depth = 0;
+    int accel = RD_ACCEL_BVH;           // the scene aggregate: Bvh or Octree
+    int oct_max_depth = 10, oct_max_count = 10;
+    float oct_out_bound = 2.0f;
+    struct OctLinear {                  // LinearNode (Octree.cs:28-35)
+        BBox b;
+        int count_or_leaf;              // NodeCount (interior) / LeafIndex (leaf)
+        int next_head;                  // -1: leaf
+    };
+    std::vector<OctLinear> oct;
+    std::vector<std::vector<int>> oct_leaves;
+    int oct_depth = 0;                  // _nowDepth
     Camera cam;
     rd_integrator_desc integ;
 
@@ -887,17 +899,240 @@ static bool bvh_any(const cr_scene& s, const Ray& ray, uint64_t* cnt) {
     return hit;
 }
 
+// ---------------------------------------------------------------- octree
+// Octree (Accelerators/Octree.cs), the config's "octree" accel: a sparse,
+// loose octree.  BoundingBox3F's strict tests (Core/BoundingBox3F.cs:47-73)
+// and GetCorner (:131-138) decide the build.
+static bool oct_contains(const BBox& b, V3 p) {
+    return p.x > b.mn.x && p.x < b.mx.x && p.y > b.mn.y && p.y < b.mx.y && p.z > b.mn.z && p.z < b.mx.z;
+}
+static bool oct_contains(const BBox& b, const BBox& o) {
+    return o.mn.x > b.mn.x && o.mx.x < b.mx.x && o.mn.y > b.mn.y && o.mx.y < b.mx.y && o.mn.z > b.mn.z &&
+           o.mx.z < b.mx.z;
+}
+static bool oct_overlaps(const BBox& b, const BBox& o) {
+    return o.mn.x < b.mx.x && o.mx.x > b.mn.x && o.mn.y < b.mx.y && o.mx.y > b.mn.y && o.mn.z < b.mx.z &&
+           o.mx.z > b.mn.z;
+}
+static V3 oct_corner(const BBox& b, int i) {
+    return V3((i & 1) == 0 ? b.mn.x : b.mx.x, (i & 2) == 0 ? b.mn.y : b.mx.y, (i & 4) == 0 ? b.mn.z : b.mx.z);
+}
+
+namespace {
+struct OctBuild {
+    // build-time Node (:12-17): a leaf holds primitives, an interior node 8
+    // child slots (null = no primitive went there)
+    struct N {
+        BBox box;
+        std::vector<int> prims;
+        std::unique_ptr<N> kids[8];
+        bool interior = false;
+    };
+    const std::vector<BBox>& bound;
+    int max_depth, max_count;
+    float out_bound;
+    int now_depth = 0, leaf_count = 0, node_count = 0;
+    int64_t entries = 0;   // child-list entries so far (a guard: copies multiply per level)
+    struct TooLarge {};
+
+    OctBuild(const std::vector<BBox>& b, int md, int mc, float ob) : bound(b), max_depth(md), max_count(mc), out_bound(ob) {}
+
+    // RecursionBuild (:227-309)
+    std::unique_ptr<N> build(const std::vector<int>& prim_index, const BBox& inner_bound, const BBox& out_bnd, int depth) {
+        const int cnt = (int)prim_index.size();
+        if (cnt == 0) return nullptr;
+        now_depth = std::max(now_depth, depth);
+        if (cnt <= max_count || depth >= max_depth) {
+            auto leaf = std::make_unique<N>();
+            leaf->box = out_bnd;
+            leaf->prims = prim_index;
+            leaf_count++;
+            node_count++;
+            return leaf;
+        }
+        const V3 center = inner_bound.center();
+        BBox inner[8], outer[8];
+        std::vector<int> child_data[8];
+        const float extend_len = (out_bound - 1) * 0.5f;
+        for (int i = 0; i < 8; i++) {
+            inner[i] = BBox(center);
+            inner[i].expand(oct_corner(inner_bound, i));
+            outer[i].mn = inner[i].mn - V3(extend_len);
+            outer[i].mx = inner[i].mx + V3(extend_len);
+        }
+        for (int i : prim_index) {
+            const BBox& ob = bound[i];
+            const V3 cp = ob.center();
+            bool inserted = false;
+            for (int j = 0; j < 8 && !inserted; j++) {
+                if (oct_contains(inner[j], cp) && oct_contains(outer[j], ob)) {
+                    child_data[j].push_back(i);
+                    inserted = true;
+                }
+            }
+            if (!inserted)
+                for (int j = 0; j < 8; j++)
+                    if (oct_overlaps(inner[j], ob)) child_data[j].push_back(i);
+        }
+        for (int j = 0; j < 8; j++) entries += (int64_t)child_data[j].size();
+        if (entries > ((int64_t)1 << 27)) throw TooLarge{};
+        for (int i = 0; i < 8; i++) {
+            BBox real;
+            for (int j : child_data[i]) real.unite(bound[j]);
+            outer[i] = real;
+        }
+        auto node = std::make_unique<N>();
+        node->box = out_bnd;
+        node->interior = true;
+        for (int i = 0; i < 8; i++) node->kids[i] = build(child_data[i], inner[i], outer[i], depth + 1);
+        node_count++;
+        return node;
+    }
+
+    // FlattenNode (:311-346): a node's non-null children get consecutive
+    // slots from `all`, then each child's subtree is flattened in turn
+    void flatten(const N& node, int this_index, int* all, cr_scene& s) {
+        s.oct[this_index].b = node.box;
+        if (!node.interior) {
+            s.oct_leaves.push_back(node.prims);
+            s.oct[this_index].count_or_leaf = (int)s.oct_leaves.size() - 1;
+            s.oct[this_index].next_head = -1;
+            return;
+        }
+        int not_null = 0;
+        for (const auto& k : node.kids) not_null += k ? 1 : 0;
+        const int head = *all;
+        s.oct[this_index].count_or_leaf = not_null;
+        s.oct[this_index].next_head = head;
+        *all += not_null;
+        int j = 0;
+        for (const auto& k : node.kids)
+            if (k) flatten(*k, head + j++, all, s);
+    }
+};
+}  // namespace
+
+// Octree ctor (:48-93) over the scene primitives' WorldBound
+static bool build_octree(cr_scene& s) {
+    std::vector<BBox> b(s.prims.size());
+    BBox root;
+    std::vector<int> idx(s.prims.size());
+    for (size_t i = 0; i < s.prims.size(); i++) {
+        b[i] = prim_bound(s, (int)i);
+        root.unite(b[i]);
+        idx[i] = (int)i;
+    }
+    OctBuild B(b, s.oct_max_depth, s.oct_max_count, s.oct_out_bound);
+    std::unique_ptr<OctBuild::N> r;
+    try {
+        r = B.build(idx, root, root, 0);
+    } catch (const OctBuild::TooLarge&) {
+        set_err("octree over 2^27 entries");
+        return false;
+    }
+    s.oct.clear();
+    s.oct_leaves.clear();
+    s.oct_depth = B.now_depth;
+    if (!r) return true;   // no primitive: an empty tree
+    s.oct.resize(B.node_count);
+    int all = 1;
+    B.flatten(*r, 0, &all, s);
+    if (all != B.node_count) { set_err("octree flatten count"); return false; }
+    return true;
+}
+
+// Octree.Intersect(ray, out inct) (:133-225): the root box first, then a
+// LIFO of the nodes whose box the ray met when their parent was expanded
+// (children pushed in slot order); a popped node is not tested again.  nv
+// counts box tests, nt primitive tests.
+static bool oct_closest(const cr_scene& s, Ray ray, Hit* out, uint64_t* nv, uint64_t* nt) {
+    if (s.oct.empty()) return false;
+    (*nv)++;
+    if (!box_hit(s.oct[0].b, ray)) return false;
+    int stack[256];   // Octree.StackSize
+    int sp = 0;
+    stack[sp++] = 0;
+    float best = FLT_MAX;
+    bool any = false;
+    while (sp > 0) {
+        const cr_scene::OctLinear& n = s.oct[stack[--sp]];
+        if (n.next_head == -1) {
+            for (int p : s.oct_leaves[n.count_or_leaf]) {
+                Hit h;
+                (*nt)++;
+                if (prim_closest(s, p, ray, &h, nv, nt)) {
+                    any = true;
+                    if (h.T < best) { best = h.T; *out = h; ray.tmax = h.T; }
+                }
+            }
+        } else {
+            for (int i = 0; i < n.count_or_leaf; i++) {
+                (*nv)++;
+                if (box_hit(s.oct[n.next_head + i].b, ray)) {
+                    if (sp >= 256) { set_err("octree stack overflow"); return false; }
+                    stack[sp++] = n.next_head + i;
+                }
+            }
+        }
+    }
+    return any;
+}
+
+// Octree.Intersect(ray) (:95-131)
+static bool oct_any(const cr_scene& s, const Ray& ray, uint64_t* nv, uint64_t* nt) {
+    if (s.oct.empty()) return false;
+    (*nv)++;
+    if (!box_hit(s.oct[0].b, ray)) return false;
+    int stack[256];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const cr_scene::OctLinear& n = s.oct[stack[--sp]];
+        if (n.next_head == -1) {
+            for (int p : s.oct_leaves[n.count_or_leaf]) {
+                (*nt)++;
+                if (prim_any(s, p, ray, nv, nt)) return true;
+            }
+        } else {
+            for (int i = 0; i < n.count_or_leaf; i++) {
+                (*nv)++;
+                if (box_hit(s.oct[n.next_head + i].b, ray)) {
+                    if (sp >= 256) { set_err("octree stack overflow"); return false; }
+                    stack[sp++] = n.next_head + i;
+                }
+            }
+        }
+    }
+    return false;
+}
+
+// The scene aggregate's closest / any query (Scene.cs:32-35, 58-70 call it).
+static bool agg_closest(const cr_scene& s, const Ray& ray, Hit* out, uint64_t* cnt) {
+    if (s.accel != RD_ACCEL_OCTREE) return bvh_closest(s, ray, out, cnt);
+    uint64_t nv = 0, nt = 0;
+    bool any = oct_closest(s, ray, out, &nv, &nt);
+    if (cnt) { cnt[0] += nv; cnt[1] += nt; }
+    return any;
+}
+static bool agg_any(const cr_scene& s, const Ray& ray, uint64_t* cnt) {
+    if (s.accel != RD_ACCEL_OCTREE) return bvh_any(s, ray, cnt);
+    uint64_t nv = 0, nt = 0;
+    bool hit = oct_any(s, ray, &nv, &nt);
+    if (cnt) { cnt[0] += nv; cnt[1] += nt; }
+    return hit;
+}
+
 // Scene.Intersect(ray, out inct) (Scene.cs:32-35)
 static bool scene_intersect(const cr_scene& s, const Ray& r, Inct* it, uint64_t* st) {
     Hit h;
     if (st) st[0]++;
-    if (!bvh_closest(s, r, &h, nullptr)) return false;
+    if (!agg_closest(s, r, &h, nullptr)) return false;
     *it = prim_inct(s, h, r);
     return true;
 }
 static bool scene_occluded_ray(const cr_scene& s, const Ray& r, uint64_t* st) {
     if (st) st[1]++;
-    return bvh_any(s, r, nullptr);
+    return agg_any(s, r, nullptr);
 }
 // Scene.IsOccluded (Scene.cs:58-70)
 static bool scene_is_occluded(const cr_scene& s, V3 from, V3 to, uint64_t* st) {
@@ -2883,6 +3118,11 @@ cr_scene* cr_scene_create(const rd_scene_desc* d) {
     if (d->num_media) s->media.assign(d->media, d->media + d->num_media);
     s->global_medium = d->global_medium;
     s->max_prim = d->bvh_max_prim > 0 ? d->bvh_max_prim : 1;
+    s->accel = d->accel;
+    if (s->accel != RD_ACCEL_BVH && s->accel != RD_ACCEL_OCTREE) { set_err("unknown accelerator"); delete s; return nullptr; }
+    s->oct_max_depth = d->octree_max_depth;
+    s->oct_max_count = d->octree_max_count;
+    s->oct_out_bound = d->octree_out_bound;
     s->integ = d->integrator;
     s->tex.resize(d->num_textures);
     for (int i = 0; i < d->num_textures; i++) {
@@ -2976,6 +3216,7 @@ cr_scene* cr_scene_create(const rd_scene_desc* d) {
             pd.medium_outside >= d->num_media) { set_err("prim medium index"); delete s; return nullptr; }
     if (!s->cam.init(d->camera)) { set_err("camera matrix not invertible"); delete s; return nullptr; }
     build_bvh(*s);
+    if (s->accel == RD_ACCEL_OCTREE && !build_octree(*s)) { delete s; return nullptr; }
     for (EnvLight& e : s->env) {
         // InfiniteAreaLight ctor (InfiniteAreaLight.cs:31-53): luminance * sin(theta)
         // of the map sampled at texel corners (x/W, y/H), rows of the 2D distribution
@@ -3028,6 +3269,73 @@ static inline Ray to_ray(const rd_ray& r) {
     return Ray(V3(r.o[0], r.o[1], r.o[2]), V3(r.d[0], r.d[1], r.d[2]), r.tmin, r.tmax);
 }
 
+// The octree as 32-byte LinearNode records plus the leaves' primitive lists.
+static void export_octree(const cr_scene& s, void* nodes32, int32_t* leaf_first, int32_t* leaf_count,
+                          int32_t* leaf_prims, int32_t* sizes3) {
+    int entries = 0;
+    for (const auto& l : s.oct_leaves) entries += (int)l.size();
+    if (sizes3) { sizes3[0] = (int32_t)s.oct.size(); sizes3[1] = (int32_t)s.oct_leaves.size(); sizes3[2] = entries; }
+    if (nodes32) {
+        unsigned char* out = (unsigned char*)nodes32;
+        for (size_t i = 0; i < s.oct.size(); i++) {
+            const cr_scene::OctLinear& n = s.oct[i];
+            float b[6] = {n.b.mn.x, n.b.mn.y, n.b.mn.z, n.b.mx.x, n.b.mx.y, n.b.mx.z};
+            memcpy(out + 32 * i, b, 24);
+            memcpy(out + 32 * i + 24, &n.count_or_leaf, 4);
+            memcpy(out + 32 * i + 28, &n.next_head, 4);
+        }
+    }
+    int e = 0;
+    for (size_t l = 0; l < s.oct_leaves.size(); l++) {
+        if (leaf_first) leaf_first[l] = e;
+        if (leaf_count) leaf_count[l] = (int32_t)s.oct_leaves[l].size();
+        for (int p : s.oct_leaves[l]) {
+            if (leaf_prims) leaf_prims[e] = p;
+            e++;
+        }
+    }
+}
+
+int cr_scene_export_octree(const cr_scene* s, void* nodes32, int32_t* leaf_first, int32_t* leaf_count,
+                           int32_t* leaf_prims, int32_t* sizes3) {
+    if (!s || s->accel != RD_ACCEL_OCTREE) return RD_ERR_INVALID;
+    export_octree(*s, nodes32, leaf_first, leaf_count, leaf_prims, sizes3);
+    return RD_OK;
+}
+
+int cr_octree_build(const float* bmin, const float* bmax, int32_t n, int32_t max_depth, int32_t max_count,
+                    float out_bound, void* nodes32, int32_t* leaf_first, int32_t* leaf_count, int32_t* leaf_prims,
+                    int32_t* sizes3, int32_t* now_depth) {
+    if (n < 0) return RD_ERR_INVALID;
+    cr_scene s;
+    s.oct_max_depth = max_depth; s.oct_max_count = max_count; s.oct_out_bound = out_bound;
+    std::vector<BBox> b(n);
+    BBox root;
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; i++) {
+        b[i].mn = V3(bmin[3 * i], bmin[3 * i + 1], bmin[3 * i + 2]);
+        b[i].mx = V3(bmax[3 * i], bmax[3 * i + 1], bmax[3 * i + 2]);
+        root.unite(b[i]);
+        idx[i] = i;
+    }
+    OctBuild B(b, max_depth, max_count, out_bound);
+    std::unique_ptr<OctBuild::N> r;
+    try {
+        r = B.build(idx, root, root, 0);
+    } catch (const OctBuild::TooLarge&) {
+        set_err("octree over 2^27 entries");
+        return RD_ERR_INVALID;
+    }
+    if (r) {
+        s.oct.resize(B.node_count);
+        int all = 1;
+        B.flatten(*r, 0, &all, s);
+    }
+    if (now_depth) *now_depth = B.now_depth;
+    export_octree(s, nodes32, leaf_first, leaf_count, leaf_prims, sizes3);
+    return RD_OK;
+}
+
 int cr_trace_closest(const cr_scene* s, const rd_ray* rays, int32_t n, rd_hit* hits, uint64_t* counters) {
     return cr_trace_closest_sub(s, rays, n, hits, nullptr, counters);
 }
@@ -3038,7 +3346,7 @@ int cr_trace_closest_sub(const cr_scene* s, const rd_ray* rays, int32_t n, rd_hi
     for (int i = 0; i < n; i++) {
         Ray r = to_ray(rays[i]);
         Hit h;
-        if (bvh_closest(*s, r, &h, counters)) { hits[i].t = h.T; hits[i].prim = h.prim; hits[i].u = h.sp.u; hits[i].v = h.sp.v; }
+        if (agg_closest(*s, r, &h, counters)) { hits[i].t = h.T; hits[i].prim = h.prim; hits[i].u = h.sp.u; hits[i].v = h.sp.v; }
         else { hits[i].t = 0; hits[i].prim = -1; hits[i].u = 0; hits[i].v = 0; }
         if (sub) sub[i] = h.prim >= 0 ? h.sub : -1;
     }
@@ -3047,7 +3355,7 @@ int cr_trace_closest_sub(const cr_scene* s, const rd_ray* rays, int32_t n, rd_hi
 
 int cr_trace_any(const cr_scene* s, const rd_ray* rays, int32_t n, uint8_t* occ, uint64_t* counters) {
     if (!s) return RD_ERR_INVALID;
-    for (int i = 0; i < n; i++) occ[i] = bvh_any(*s, to_ray(rays[i]), counters) ? 1 : 0;
+    for (int i = 0; i < n; i++) occ[i] = agg_any(*s, to_ray(rays[i]), counters) ? 1 : 0;
     return RD_OK;
 }
 

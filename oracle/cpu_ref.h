@@ -22,6 +22,18 @@ void cr_scene_destroy(cr_scene* s);
 int32_t cr_scene_num_nodes(const cr_scene* s);
 int32_t cr_scene_max_depth(const cr_scene* s);
 int cr_scene_export_bvh(const cr_scene* s, void* nodes32, int32_t* prim_order);
+/* The loose octree of an RD_ACCEL_OCTREE scene (Accelerators/Octree.cs):
+ * 32-byte LinearNode records (bound, NodeCount / LeafIndex, NextHead = -1
+ * for leaves) and leaf i's primitives leaf_prims[leaf_first[i] ..
+ * + leaf_count[i]); sizes3 = {nodes, leaves, leaf entries}; null buffers are
+ * skipped, so a first call with nulls sizes them. */
+int cr_scene_export_octree(const cr_scene* s, void* nodes32, int32_t* leaf_first, int32_t* leaf_count,
+                           int32_t* leaf_prims, int32_t* sizes3);
+/* The same tree over caller bounds (bmin / bmax: [n][3]); now_depth = the
+ * deepest level built (root 0). */
+int cr_octree_build(const float* bmin, const float* bmax, int32_t n, int32_t max_depth, int32_t max_count,
+                    float out_bound, void* nodes32, int32_t* leaf_first, int32_t* leaf_count, int32_t* leaf_prims,
+                    int32_t* sizes3, int32_t* now_depth);
 /* counters (may be NULL): [0] nodes visited (AABB tests), [1] primitives tested */
 int cr_trace_closest(const cr_scene* s, const rd_ray* rays, int32_t n, rd_hit* hits, uint64_t* counters);
 int cr_trace_any(const cr_scene* s, const rd_ray* rays, int32_t n, uint8_t* occluded, uint64_t* counters);

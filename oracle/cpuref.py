@@ -33,6 +33,9 @@ def lib():
         L.cr_scene_num_nodes.argtypes = [C.c_void_p]
         L.cr_scene_max_depth.argtypes = [C.c_void_p]
         L.cr_scene_export_bvh.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.cr_scene_export_octree.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+        L.cr_octree_build.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_float] + \
+            [C.c_void_p] * 6
         for fn in ("cr_trace_closest", "cr_trace_any"):
             getattr(L, fn).argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
         L.cr_trace_brute.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
@@ -194,6 +197,31 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def _octree_arrays(call):
+    sz = np.zeros(3, np.int32)
+    call(None, None, None, None, _ptr(sz))
+    nodes = np.zeros(max(sz[0], 1) * 32, np.uint8)
+    first = np.zeros(max(sz[1], 1), np.int32)
+    count = np.zeros(max(sz[1], 1), np.int32)
+    prims = np.zeros(max(sz[2], 1), np.int32)
+    call(_ptr(nodes), _ptr(first), _ptr(count), _ptr(prims), _ptr(sz))
+    return nodes[: sz[0] * 32].reshape(-1, 32), first[: sz[1]], count[: sz[1]], prims[: sz[2]]
+
+
+def octree_build(bmin, bmax, max_depth=10, max_count=10, out_bound=2.0):
+    """The oracle's Octree build over caller bounds: (nodes [N,32] uint8,
+    leaf_first, leaf_count, leaf_prims, now_depth)."""
+    bmin = np.ascontiguousarray(bmin, np.float32)
+    bmax = np.ascontiguousarray(bmax, np.float32)
+    depth = np.zeros(1, np.int32)
+    def call(a, b, c, d, e):
+        if lib().cr_octree_build(_ptr(bmin), _ptr(bmax), bmin.shape[0], max_depth, max_count, out_bound, a, b, c, d, e,
+                                 _ptr(depth)) != 0:
+            raise RuntimeError("oracle octree: " + lib().cr_last_error().decode())
+    out = _octree_arrays(call)
+    return out + (int(depth[0]),)
+
+
 class OracleScene:
     """The oracle's view of a SceneDesc (BVH built by the restated Bvh.cs)."""
 
@@ -226,6 +254,10 @@ class OracleScene:
         order = np.zeros(self.desc.num_prims(), np.int32)
         lib().cr_scene_export_bvh(self.h, _ptr(nodes), _ptr(order))
         return nodes.reshape(n, 32), order
+
+    def export_octree(self):
+        """(nodes [N,32] uint8, leaf_first, leaf_count, leaf_prims) of an octree scene."""
+        return _octree_arrays(lambda *a: lib().cr_scene_export_octree(self.h, *a))
 
     def trace_closest(self, rays, counters=False):
         rays = np.ascontiguousarray(rays, abi.ray_dtype())

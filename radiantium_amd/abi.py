@@ -5,7 +5,7 @@ checks sizes and offsets against the compiled library.
 """
 import ctypes as C
 
-ABI_VERSION = 3   # RD_ABI_VERSION
+ABI_VERSION = 4   # RD_ABI_VERSION
 RD_OK = 0
 RD_ERR_INVALID = 1
 RD_ERR_DEVICE = 2
@@ -41,6 +41,8 @@ LIGHT_INFINITE = 1
 PRIM_TRIANGLE = 0
 PRIM_SPHERE = 1
 PRIM_INSTANCE = 2
+ACCEL_BVH = 0
+ACCEL_OCTREE = 1
 INTEGRATOR_PATH = 0
 INTEGRATOR_VOL_PATH = 1
 INTEGRATOR_PARTICLE = 2
@@ -128,6 +130,8 @@ class rd_scene_desc(C.Structure):
         ("camera", rd_camera_desc), ("integrator", rd_integrator_desc),
         ("num_instance_sets", C.c_int32), ("instance_sets", C.POINTER(rd_instance_set_desc)),
         ("num_instances", C.c_int32), ("instances", C.POINTER(rd_instance_desc)),
+        ("accel", C.c_int32), ("octree_max_depth", C.c_int32), ("octree_max_count", C.c_int32),
+        ("octree_out_bound", C.c_float),
     ]
 
 

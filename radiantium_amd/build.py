@@ -23,7 +23,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unus
 TARGET = [f"--offload-arch={ARCH}", "-fhip-fp32-correctly-rounded-divide-sqrt",
           "-fno-gpu-flush-denormals-to-zero", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 DEVICE = ["-x", "hip"] + TARGET
-HOST_SOURCES = ["rd_api.cpp", "rd_camera.cpp", "rd_bvh_build.cpp", "rd_envmap.cpp", "rd_json.cpp", "rd_image.cpp"]
+HOST_SOURCES = ["rd_api.cpp", "rd_camera.cpp", "rd_bvh_build.cpp", "rd_envmap.cpp", "rd_json.cpp", "rd_image.cpp", "rd_octree.cpp"]
 DEVICE_SOURCES = ["rd_kernels.hip", "rd_volpath.hip", "rd_sort.hip", "rd_scratch.hip", "rd_bvh_gpu.hip", "rd_light.hip", "rd_bdpt.hip"]
 
 

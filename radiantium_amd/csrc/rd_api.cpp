@@ -17,6 +17,7 @@
 #include "../../include/radiantium_hip.h"
 #include "rd_bvh_build.h"
 #include "rd_camera.h"
+#include "rd_octree.h"
 #include "rd_envmap.h"
 #include "rd_kernels.h"
 #include "rd_scratch.h"
@@ -175,6 +176,7 @@ struct rd_ctx {
 struct rd_scene {
     rd_ctx* ctx = nullptr;
     DevScene dev{};
+    OctResult oct;                      // RD_ACCEL_OCTREE scenes: the tree as built
     LtCamera lt_cam{};                  // SampleWi / Importance state (light paths)
     DevIntegrator integ{};
     std::vector<void*> allocs;
@@ -503,6 +505,7 @@ struct ScopedAllocs {
 };
 
 int check_stack(const rd_scene* s) {
+    if (s->dev.octree) return s->dev.stack_needed > 256 ? fail(RD_ERR_STACK, "octree stack over 256 entries") : RD_OK;
     if (s->dev.stack_needed > 64)
         return fail(RD_ERR_STACK, "BVH deeper than the 64-entry traversal stack (Bvh.StackSize)");
     return RD_OK;
@@ -786,7 +789,33 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     }
     BvhResult bvh;
     std::string err;
-    if (!tree(np, bmin.data(), bmax.data(), d->bvh_max_prim > 0 ? d->bvh_max_prim : 1, &bvh, &err)) {
+    const bool use_octree = d->accel == RD_ACCEL_OCTREE;
+    if (d->accel != RD_ACCEL_BVH && !use_octree) { delete s; return bad("unknown accelerator"); }
+    if (use_octree && nsets > 0) { delete s; return bad("the octree accelerator holds no instanced primitives"); }
+    if (use_octree) {
+        if (!build_octree(np, bmin.data(), bmax.data(), d->octree_max_depth, d->octree_max_count,
+                          d->octree_out_bound, &s->oct, &err)) {
+            delete s;
+            g_err = err;
+            return nullptr;
+        }
+        // the traversal's LIFO at worst (every box hit) must fit Octree.StackSize
+        if (s->oct.stack_needed > 256) {
+            delete s;
+            g_err = "octree whose traversal can hold over 256 entries (Octree.StackSize)";
+            return nullptr;
+        }
+        // the rest of the creation reads the tree through bvh: its root box
+        // (the scene bound), leaf order and depth
+        for (const OctNode& o : s->oct.nodes) {
+            NodeRec r{};
+            for (int k = 0; k < 3; k++) { r.mn[k] = o.mn[k]; r.mx[k] = o.mx[k]; }
+            bvh.nodes.push_back(r);
+        }
+        bvh.order = s->oct.leaf_prims;
+        bvh.leaves = (int)s->oct.leaf_first.size();
+        bvh.max_depth = s->oct.max_depth;
+    } else if (!tree(np, bmin.data(), bmax.data(), d->bvh_max_prim > 0 ? d->bvh_max_prim : 1, &bvh, &err)) {
         delete s;
         g_err = err;
         return nullptr;
@@ -794,6 +823,7 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     auto t1 = std::chrono::steady_clock::now();
     s->nodes = bvh.nodes;
     s->order = bvh.order;
+    const int n_ordered = (int)bvh.order.size();   // the octree's leaves may repeat primitives
 
     // node and ordered-primitive records: the scene tree, then every set's
     // tree with its child / primitive offsets made absolute
@@ -807,15 +837,27 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
             nodes.push_back(make_float4(r.mx[1], r.mx[2], ibits(off), ibits(packed)));
         }
     };
-    add_nodes(bvh, 0, 0);
-    std::vector<float4> prims(3 * (size_t)np);
+    if (use_octree) {
+        // lo = {min, max.x}, hi = {max.y, max.z, offset, count}: a leaf's
+        // primitive records, an interior node's -NodeCount children
+        for (const OctNode& o : s->oct.nodes) {
+            const bool leaf = o.next_head < 0;
+            const int off = leaf ? s->oct.leaf_first[o.count_or_leaf] : o.next_head;
+            const int cnt = leaf ? s->oct.leaf_count[o.count_or_leaf] : -o.count_or_leaf;
+            nodes.push_back(make_float4(o.mn[0], o.mn[1], o.mn[2], o.mx[0]));
+            nodes.push_back(make_float4(o.mx[1], o.mx[2], ibits(off), ibits(cnt)));
+        }
+    } else {
+        add_nodes(bvh, 0, 0);
+    }
+    std::vector<float4> prims(3 * (size_t)n_ordered);
     auto put_tri = [&](size_t i, int tri, int orig) {
         const int* ix = d->indices + 3 * (size_t)tri;
         prims[3 * i] = make_float4(P(ix[0], 0), P(ix[0], 1), P(ix[0], 2), ibits(orig));
         prims[3 * i + 1] = make_float4(P(ix[1], 0), P(ix[1], 1), P(ix[1], 2), ibits(kTri));
         prims[3 * i + 2] = make_float4(P(ix[2], 0), P(ix[2], 1), P(ix[2], 2), 0.0f);
     };
-    for (int i = 0; i < np; i++) {
+    for (int i = 0; i < n_ordered; i++) {
         int orig = bvh.order[i];
         const rd_prim_desc& pd = d->prims[orig];
         if (pd.kind == RD_PRIM_TRIANGLE) {
@@ -923,7 +965,8 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
             invert4(l.to_world, inv);
             for (int r = 0; r < 3; r++)
                 for (int c = 0; c < 3; c++) { e.m2w[3 * r + c] = l.to_world[4 * r + c]; e.w2m[3 * r + c] = inv[4 * r + c]; }
-            const NodeRec& root = bvh.nodes[0];
+            NodeRec root{};
+            if (!bvh.nodes.empty()) root = bvh.nodes[0];
             f3 mn = mk(root.mn[0], root.mn[1], root.mn[2]), mx = mk(root.mx[0], root.mx[1], root.mx[2]);
             f3 c = (mx + mn) * 0.5f;
             e.center[0] = c.x; e.center[1] = c.y; e.center[2] = c.z;
@@ -956,7 +999,8 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     ds.num_nodes = (int)(nodes.size() / 2);
     ds.num_prims = np;
     // a set's traversal runs on top of the scene tree's stack entries
-    ds.stack_needed = bvh.max_depth + (nsets > 0 ? blas_depth : 0);
+    ds.stack_needed = use_octree ? s->oct.stack_needed : bvh.max_depth + (nsets > 0 ? blas_depth : 0);
+    ds.octree = use_octree ? 1 : 0;
     ds.num_insts = d->num_instances;
     ds.num_lights = d->num_lights;
     ds.global_medium = d->global_medium;
@@ -1040,6 +1084,45 @@ int rd_scene_export_bvh(const rd_scene* s, void* nodes32, int32_t max_nodes, int
         std::memcpy(prim_order, s->order.data(), s->order.size() * sizeof(int32_t));
     }
     return RD_OK;
+}
+
+// Copies an octree into the caller's buffers (sizes first; a null or short
+// buffer only sizes).
+static int copy_octree(const OctResult& o, void* nodes32, int32_t max_nodes, int32_t* leaf_first, int32_t* leaf_count,
+                       int32_t max_leaves, int32_t* leaf_prims, int32_t max_leaf_prims, int32_t* sizes3) {
+    if (sizes3) {
+        sizes3[0] = (int32_t)o.nodes.size();
+        sizes3[1] = (int32_t)o.leaf_first.size();
+        sizes3[2] = (int32_t)o.leaf_prims.size();
+    }
+    if (!nodes32 || !leaf_first || !leaf_count || !leaf_prims) return RD_OK;
+    if (max_nodes < (int32_t)o.nodes.size() || max_leaves < (int32_t)o.leaf_first.size() ||
+        max_leaf_prims < (int32_t)o.leaf_prims.size())
+        return fail(RD_ERR_INVALID, "octree buffers too small");
+    std::memcpy(nodes32, o.nodes.data(), o.nodes.size() * sizeof(OctNode));
+    std::memcpy(leaf_first, o.leaf_first.data(), o.leaf_first.size() * sizeof(int32_t));
+    std::memcpy(leaf_count, o.leaf_count.data(), o.leaf_count.size() * sizeof(int32_t));
+    std::memcpy(leaf_prims, o.leaf_prims.data(), o.leaf_prims.size() * sizeof(int32_t));
+    return RD_OK;
+}
+
+int rd_octree_build_host(const float* bmin, const float* bmax, int32_t n, int32_t max_depth, int32_t max_count,
+                         float out_bound, void* nodes32, int32_t max_nodes, int32_t* leaf_first, int32_t* leaf_count,
+                         int32_t max_leaves, int32_t* leaf_prims, int32_t max_leaf_prims, int32_t* sizes3) {
+    if (n < 0 || (n > 0 && (!bmin || !bmax))) return fail(RD_ERR_INVALID, "bad bounds");
+    OctResult o;
+    std::string err;
+    if (!build_octree(n, bmin, bmax, max_depth, max_count, out_bound, &o, &err)) return fail(RD_ERR_INVALID, err);
+    return copy_octree(o, nodes32, max_nodes, leaf_first, leaf_count, max_leaves, leaf_prims, max_leaf_prims, sizes3);
+}
+
+int rd_scene_export_octree(const rd_scene* s, void* nodes32, int32_t max_nodes, int32_t* leaf_first,
+                           int32_t* leaf_count, int32_t max_leaves, int32_t* leaf_prims, int32_t max_leaf_prims,
+                           int32_t* sizes3) {
+    if (!s) return fail(RD_ERR_INVALID, "null scene");
+    if (!s->dev.octree) return fail(RD_ERR_INVALID, "the scene's accelerator is the BVH");
+    return copy_octree(s->oct, nodes32, max_nodes, leaf_first, leaf_count, max_leaves, leaf_prims, max_leaf_prims,
+                       sizes3);
 }
 
 int rd_bvh_build_host(const float* bmin, const float* bmax, int32_t n, int32_t max_prim, void* nodes32,
@@ -1615,6 +1698,8 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
         if (!r.scene || (!r.out_rgb && !(single && out64))) return fail(RD_ERR_INVALID, "request without scene or output");
         if (r.scene->ctx != c) return fail(RD_ERR_INVALID, "request scene belongs to another context");
         if (r.params.spp <= 0) return fail(RD_ERR_INVALID, "spp must be positive");
+        if (!single && r.scene->dev.octree)
+            return fail(RD_ERR_UNSUPPORTED, "batched rendering covers BVH scenes (an octree scene renders alone)");
         const int it = r.scene->dev.integ.type;
         if (it != RD_INTEGRATOR_PATH && it != RD_INTEGRATOR_VOL_PATH)
             return fail(RD_ERR_UNSUPPORTED, single ? "this entry point renders path and vol_path scenes"

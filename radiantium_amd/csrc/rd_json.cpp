@@ -969,8 +969,10 @@ struct Loader {
             if (const JVal* tw = cam->get("to_world")) read_m4(tw, "to_world", m.m);
             std::memcpy(C.to_world, m.m, sizeof m.m);
         }
-        // scene accelerator: a BVH; an octree gives the same closest hits (ties aside)
+        // scene accelerator (RendererBuilder.cs:86-87): the SAH BVH or the loose octree
         int max_prim = 1;
+        int accel = RD_ACCEL_BVH, oct_depth = 10, oct_count = 10;
+        float oct_out = 2.0f;
         if (const JVal* ac = root.get("accel")) {
             if (!ac->get("type")) bad("accel: no key type");
             std::string t = read_s(ac, "type", "");
@@ -979,7 +981,10 @@ struct Loader {
                 std::string split = read_s(ac, "split", "SAH");
                 if (split != "SAH") unsupported("BVH split method " + split);
             } else if (t == "octree") {
-                B.notes.push_back("octree accelerator replaced by the SAH BVH (max_prim 1)");
+                accel = RD_ACCEL_OCTREE;
+                oct_depth = read_i(ac, "max_depth", 10);
+                oct_count = read_i(ac, "max_count", 10);
+                oct_out = read_f(ac, "out_bound", 2.0f);
             } else {
                 bad("unknown accelerator " + t);
             }
@@ -1158,6 +1163,14 @@ struct Loader {
         d.media = B.media.data();
         d.global_medium = global_medium;
         d.bvh_max_prim = max_prim;
+        d.accel = accel;
+        d.octree_max_depth = oct_depth;
+        d.octree_max_count = oct_count;
+        d.octree_out_bound = oct_out;
+        if (accel == RD_ACCEL_OCTREE && !B.sets.empty()) {
+            d.accel = RD_ACCEL_BVH;   // the device octree holds no instances
+            B.notes.push_back("octree accelerator of an instanced scene replaced by the SAH BVH (max_prim 1)");
+        }
         d.camera = C;
         d.integrator = I;
         d.num_instance_sets = (int)B.sets.size();

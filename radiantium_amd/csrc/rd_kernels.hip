@@ -835,9 +835,126 @@ static void launch_trace_variant(const DevScene& sc, const TraceArgs& a, int blo
     }
 }
 
+// ---------------------------------------------------------------- octree traversal
+// Octree.Intersect (Accelerators/Octree.cs:95-196): the root box, then a
+// LIFO stack of the nodes whose boxes the ray met when their parent was
+// popped (children tested and pushed in index order, popped in reverse; a
+// popped node is not tested again); a leaf tests its primitives in order (a
+// primitive may sit in several leaves): closest hit keeps the first strictly
+// closer hit and shrinks MaxT, any hit stops at the first.  Node records:
+// lo = {min, max.x}, hi = {max.y, max.z, offset, count}: a leaf has count > 0
+// primitive records from `offset`, an interior node -count children from
+// node `offset`.  The stack lives in private memory (kStack entries, at
+// least the tree's worst case: the host computes it from the flattened tree).
+template <bool kAny, int kStack>
+__device__ __forceinline__ void oct_trace(const float4* __restrict__ nodes, const float4* __restrict__ prims,
+                                          int num_nodes, Trav& t, uint32_t& nv, uint32_t& nt) {
+    if (num_nodes == 0) return;
+    nv++;
+    if (!box_hit(nodes[0], nodes[1], t.o, t.inv, t.nx, t.ny, t.nz, t.tmax)) return;
+    int stk[kStack];
+    int sp = 0;
+    stk[sp++] = 0;
+    while (sp > 0) {
+        const int idx = stk[--sp];
+        const float4 hi = nodes[2 * idx + 1];
+        const int off = __float_as_int(hi.z), cnt = __float_as_int(hi.w);
+        if (cnt > 0) {
+            for (int i = 0; i < cnt; i++) {
+                const float4* p = prims + 3 * (off + i);
+                float4 p0 = p[0], p1 = p[1], p2 = p[2];
+                whole(p0); whole(p1); whole(p2);
+                nt++;
+                float tu = 0.f, tv = 0.f, tt = 0.f;
+                bool h;
+                if (__float_as_int(p1.w) == kTri) {
+                    h = tri_test(mk(p0.x, p0.y, p0.z), mk(p1.x, p1.y, p1.z), mk(p2.x, p2.y, p2.z), t.o, t.d, t.tmin,
+                                 t.tmax, &tu, &tv, &tt);
+                } else {
+                    h = sphere_test(make_float4(p0.x, p0.y, p0.z, p1.x), t.o, t.d, t.tmin, t.tmax, &tt);
+                }
+                if (h) {
+                    if (kAny) { t.best.prim = __float_as_int(p0.w); return; }
+                    if (tt < (t.best.prim >= 0 ? t.best.t : kFloatMax)) {
+                        t.best.t = tt; t.best.u = tu; t.best.v = tv; t.best.prim = __float_as_int(p0.w);
+                        t.tmax = tt;
+                    }
+                }
+            }
+        } else {
+            for (int i = 0; i < -cnt; i++) {
+                const int ch = off + i;
+                nv++;
+                if (box_hit(nodes[2 * ch], nodes[2 * ch + 1], t.o, t.inv, t.nx, t.ny, t.nz, t.tmax) && sp < kStack)
+                    stk[sp++] = ch;
+            }
+        }
+    }
+}
+
+// Every query mode over an octree scene, one ray (or, for the full chain
+// modes, one chain) per lane, grid-stride.
+template <int kMode, int kStack>
+__global__ __launch_bounds__(256) void k_trace_oct(DevScene sc, TraceArgs a) {
+    constexpr bool kAny = (kMode == kModeRaysAny || kMode == kModeShadow);
+    constexpr bool kFull = (kMode == kModeChainFullShadow || kMode == kModeChainFullMis);
+    const int n = kFull ? a.counts[a.cnt_slot] : trav_count(a, kMode <= kModeRaysAny);
+    if (blockIdx.x == 0 && threadIdx.x < 3 && a.zero[threadIdx.x] >= 0) a.counts[a.zero[threadIdx.x]] = 0;
+    uint32_t nv = 0, nt = 0, nrays = 0;
+    for (int r0 = blockIdx.x * 256 + threadIdx.x; r0 < n; r0 += gridDim.x * 256) {
+        if constexpr (kFull) {
+            constexpr bool kMis = kMode == kModeChainFullMis;
+            const int pid = a.q.pid[r0];
+            float4 ra = a.q.org[r0], rb = a.q.dir[r0];
+            for (int step = 0;; step++) {
+                if (step >= kMaxChainSteps) {
+                    atomicOr(a.counts + kCntChainStuck, 1);
+                    break;
+                }
+                Trav t;
+                trav_begin(t, ra, rb);
+                oct_trace<false, kStack>(sc.nodes, sc.prims, sc.num_nodes, t, nv, nt);
+                const float4 h = t.best.prim >= 0 ? make_float4(t.best.t, __int_as_float(t.best.prim), t.best.u, t.best.v)
+                                                  : make_float4(0.f, __int_as_float(-1), 0.f, 0.f);
+                nrays++;
+                f3 no, nd;
+                float ntmin, ntmax;
+                if (!chain_step(sc, kMis, pid, ra, rb, h, -1, a.ps, &no, &nd, &ntmin, &ntmax)) break;
+                ra = make_float4(no.x, no.y, no.z, ntmin);
+                rb = make_float4(nd.x, nd.y, nd.z, ntmax);
+            }
+        } else {
+            const int r = a.perm ? a.perm[r0] : r0;
+            if (a.rank) a.rank[r] = r0;
+            if (kMode == kModeShadow && a.q.pid[r] < 0) continue;   // a hole of a ranked shadow queue
+            float4 ra, rb;
+            trav_load<kMode>(a, r, &ra, &rb);
+            Trav t;
+            trav_begin(t, ra, rb);
+            oct_trace<kAny, kStack>(sc.nodes, sc.prims, sc.num_nodes, t, nv, nt);
+            trav_finish<kMode>(sc, a, r, t);
+            nrays++;
+        }
+    }
+    if constexpr (kFull) trav_counters<kMode == kModeChainFullMis ? kModeChainMis : kModeChainShadow>(a, nrays, nv, nt);
+    else trav_counters<kMode>(a, nrays, nv, nt);
+}
+
 template <int kMode>
 static hipError_t launch_trace_mode(const DevScene& sc, const TraceArgs& a0, int blocks_hint, int sched,
                                     hipStream_t s) {
+    if (sc.octree) {
+        // octree scenes: one ray per lane; batches, instancing and the fused
+        // MIS + shadow launch are not built for them (the callers avoid them)
+        if constexpr (kMode == kModeSecondary) return hipErrorInvalidValue;
+        if (a0.scenes) return hipErrorInvalidValue;
+        const int pb = persistent_blocks();
+        const int b = blocks_hint > 0 ? std::min(blocks_hint, pb) : pb;
+        if (sc.stack_needed <= 32) hipLaunchKernelGGL((k_trace_oct<kMode, 32>), dim3(b), dim3(256), 0, s, sc, a0);
+        else if (sc.stack_needed <= 96) hipLaunchKernelGGL((k_trace_oct<kMode, 96>), dim3(b), dim3(256), 0, s, sc, a0);
+        else hipLaunchKernelGGL((k_trace_oct<kMode, 256>), dim3(b), dim3(256), 0, s, sc, a0);
+        return hipGetLastError();
+    }
     int need = sc.stack_needed;
     int pb = persistent_blocks();
     if (kMode <= kModeRaysAny || sched < 0 || sched >= kNumSchedules) sched = 0;
@@ -1345,7 +1462,7 @@ hipError_t launch_trace_shadow(const DevScene& sc, const DevScene* scenes, RayQu
 hipError_t launch_trace_secondary(const DevScene& sc, const DevScene* scenes, RayQueue qshadow, int cnt_shadow,
                                   RayQueue qmis, PathState ps, int* counts, int sched_shadow, int sched_mis,
                                   hipStream_t s) {
-    if (sched_shadow == 1 || sched_mis == 1 || sched_shadow == 3 || sched_mis == 3) {
+    if (sched_shadow == 1 || sched_mis == 1 || sched_shadow == 3 || sched_mis == 3 || sc.octree) {
         if (hipError_t e = launch_trace_mis(sc, scenes, qmis, ps, counts, sched_mis, s)) return e;
         return launch_trace_shadow(sc, scenes, qshadow, ps, counts, cnt_shadow, sched_shadow, s);
     }

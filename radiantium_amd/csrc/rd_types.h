@@ -148,6 +148,7 @@ struct DevScene {
     DevIntegrator integ;   // the scene's integrator (a batch may mix parameters)
     DevCamera cam;
     unsigned mat_mask;     // material kinds the scene holds (bit k: kind k); read on the host
+    int octree;            // 1: nodes / prims hold the loose octree of Accelerators/Octree.cs
 };
 
 

@@ -25,7 +25,7 @@ EXPORTS = [
     "rd_metal_eta_k", "rd_get_tuning", "rd_set_tuning", "rd_autotune", "rd_render_device_noalloc",
     "rd_render_mean_f64", "rd_image_load", "rd_image_free", "rd_image_last_error", "rd_image_save_exr",
     "rd_image_save_png", "rd_image_save_pfm", "rd_autotune_cached", "rd_tuning_cache_save", "rd_tuning_cache_load",
-    "rd_tuning_cache_clear", "rd_bvh_build_device",
+    "rd_tuning_cache_clear", "rd_bvh_build_device", "rd_octree_build_host", "rd_scene_export_octree",
 ]
 
 
@@ -87,6 +87,8 @@ def lib():
         L.rd_kernel_timing.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.rd_reset_kernel_timing.argtypes = [vp]
         L.rd_bvh_build_host.argtypes = [vp, vp, i32, i32, vp, i32, vp, vp, vp]
+        L.rd_octree_build_host.argtypes = [vp, vp, i32, i32, i32, C.c_float, vp, i32, vp, vp, i32, vp, i32, vp]
+        L.rd_scene_export_octree.argtypes = [vp, vp, i32, vp, vp, i32, vp, i32, vp]
         L.rd_json_load.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(vp), C.POINTER(abi.rd_render_params)]
         L.rd_json_scene_desc.restype = C.POINTER(abi.rd_scene_desc)
         L.rd_json_scene_desc.argtypes = [vp]
@@ -229,6 +231,29 @@ def bvh_build_host(bmin, bmax, max_prim):
     check(lib().rd_bvh_build_host(_ptr(bmin), _ptr(bmax), n, max_prim, _ptr(nodes), cap, _ptr(order),
                                   C.byref(nn), C.byref(md)))
     return nodes[: nn.value * 32].reshape(-1, 32), order[:n], md.value
+
+
+def _octree_arrays(call):
+    """Sizes, then fills, an octree export: (nodes [N,32] uint8, leaf_first,
+    leaf_count, leaf_prims)."""
+    sz = np.zeros(3, np.int32)
+    check(call(None, 0, None, None, 0, None, 0, _ptr(sz)))
+    nodes = np.zeros(max(sz[0], 1) * 32, np.uint8)
+    first = np.zeros(max(sz[1], 1), np.int32)
+    count = np.zeros(max(sz[1], 1), np.int32)
+    prims = np.zeros(max(sz[2], 1), np.int32)
+    check(call(_ptr(nodes), int(sz[0]), _ptr(first), _ptr(count), int(sz[1]), _ptr(prims), int(sz[2]), _ptr(sz)))
+    return nodes[: sz[0] * 32].reshape(-1, 32), first[: sz[1]], count[: sz[1]], prims[: sz[2]]
+
+
+def octree_build_host(bmin, bmax, max_depth=10, max_count=10, out_bound=2.0):
+    """Host loose-octree build (Accelerators/Octree.cs; no device needed):
+    (nodes [N,32] uint8 LinearNode records, leaf_first, leaf_count, leaf_prims)."""
+    bmin = np.ascontiguousarray(bmin, np.float32)
+    bmax = np.ascontiguousarray(bmax, np.float32)
+    n = bmin.shape[0]
+    return _octree_arrays(lambda *a: lib().rd_octree_build_host(_ptr(bmin), _ptr(bmax), n, max_depth, max_count,
+                                                                out_bound, *a))
 
 
 def metal_eta_k(name):
@@ -437,6 +462,10 @@ class Scene:
         i = abi.rd_scene_info()
         check(lib().rd_scene_get_info(self.h, C.byref(i)))
         return {f: getattr(i, f) for f, _ in abi.rd_scene_info._fields_}
+
+    def export_octree(self):
+        """The loose octree of an octree scene, as octree_build_host returns it."""
+        return _octree_arrays(lambda *a: lib().rd_scene_export_octree(self.h, *a))
 
     def export_bvh(self):
         inf = self.info()

@@ -73,6 +73,9 @@ class SceneDesc:
         self.media = []          # dicts
         self.global_medium = -1
         self.bvh_max_prim = 1
+        # accelerator: "bvh" or "octree" (Accelerators/Octree.cs) with its
+        # max_depth / max_count / out_bound (RendererBuilder.cs defaults)
+        self.accel = dict(type="bvh", max_depth=10, max_count=10, out_bound=2.0)
         self.camera = dict(fov=60.0, near=0.001, far=1000.0, width=1280, height=720,
                            origin=(0.0, 0.0, 3.0), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0),
                            to_world=None)
@@ -405,6 +408,10 @@ class SceneDesc:
             cam.target[j] = c["target"][j]
             cam.up[j] = c["up"][j]
 
+        d.accel = {"bvh": abi.ACCEL_BVH, "octree": abi.ACCEL_OCTREE}[self.accel["type"]]
+        d.octree_max_depth = self.accel["max_depth"]
+        d.octree_max_count = self.accel["max_count"]
+        d.octree_out_bound = self.accel["out_bound"]
         it = self.integrator
         d.integrator.type = {"path": abi.INTEGRATOR_PATH, "vol_path": abi.INTEGRATOR_VOL_PATH,
                              "adjoint_particle": abi.INTEGRATOR_PARTICLE, "bdpt": abi.INTEGRATOR_BDPT}[it["type"]]
