@@ -256,7 +256,7 @@ def adjoint_ratio(ctx, args):
     from radiantium_amd import native, scenes
     from radiantium_amd.autograd import render
     R, T, spp = 512, 512, 64
-    tdesc, _ = scenes.diff_cornell(R, R, tex_res=T, seed=args.adj_seed)
+    tdesc, ttex = scenes.diff_cornell(R, R, tex_res=T, seed=args.adj_seed)
     tscene = native.Scene(ctx, tdesc)
     target = torch.empty(R * R * 3, dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
@@ -299,8 +299,21 @@ def adjoint_ratio(ctx, args):
     log(f"[bench] adjoint: primal {p_ms:.1f} ms, adjoint {a_ms:.1f} ms, ratio {a_ms / p_ms:.2f}")
     params = {t: torch.full((T * T * 3,), 0.5, dtype=torch.float32, device="cuda", requires_grad=True) for t in ids}
     rough = {tex[k] for k in tex if k.endswith("rough")}
-    opt = torch.optim.Adam(list(params.values()), lr=0.02)
+    opt = torch.optim.Adam(list(params.values()), lr=0.03)
     tgt = target.view(R, R, 3)
+    # progress measures the render noise does not hide: the mean absolute
+    # texel error of each texture against the target's, and the loss of a
+    # fixed-seed render
+    want = {tex[k]: torch.tensor(tdesc.textures[ttex[k]].image.reshape(-1), device="cuda") for k in tex}
+
+    def texel_err():
+        return {k: round(float((params[tex[k]].detach() - want[tex[k]]).abs().mean()), 5) for k in sorted(tex)}
+
+    def eval_loss():
+        with torch.no_grad():
+            return round(float(((render(scene, params, spp, seed=424242) - tgt) ** 2).sum()), 3)
+
+    err0, eval0 = texel_err(), eval_loss()
     losses, step_ms = [], []
     for k in range(args.adj_steps):
         t0 = time.perf_counter()
@@ -316,6 +329,8 @@ def adjoint_ratio(ctx, args):
         torch.cuda.synchronize()
         step_ms.append((time.perf_counter() - t0) * 1000)
         log(f"[bench] adjoint opt step {k}: loss {losses[-1]:.2f}, {step_ms[-1]:.0f} ms")
+    err1, eval1 = texel_err(), eval_loss()
+    log(f"[bench] adjoint opt: texel error {err0} -> {err1}; fixed-seed loss {eval0} -> {eval1}")
     scene.close()
     return {"ratio": round(a_ms / p_ms, 3), "primal_ms": round(p_ms, 1), "adjoint_ms": round(a_ms, 1),
             "primal_device_ms": round(float(np.median(prim_dev)), 1),
@@ -323,6 +338,8 @@ def adjoint_ratio(ctx, args):
             "primal_mrays_s": round(rays_p / sum(prim) / 1e6, 1), "adjoint_mrays_s": round(rays_a / sum(adj) / 1e6, 1),
             "opt_steps": args.adj_steps, "opt_step_ms": round(float(np.median(step_ms)), 1) if step_ms else None,
             "loss_first": round(losses[0], 3) if losses else None, "loss_last": round(losses[-1], 3) if losses else None,
+            "texel_error_start": err0, "texel_error_end": err1,
+            "fixed_seed_loss_start": eval0, "fixed_seed_loss_end": eval1,
             "workload": f"Cornell box, rough-plastic side walls with {T}^2 albedo + roughness textures, "
                         f"{R}x{R}, {spp} spp primal + {spp} spp adjoint, max depth 8, target 256 spp "
                         f"(texture seed {args.adj_seed}), start 0.5"}
@@ -584,7 +601,7 @@ def main():
     ap.add_argument("--inst-spp", type=int, default=128, help="spp of the instanced workload frame")
     ap.add_argument("--adj-seed", type=int, default=9, help="texture seed of the adjoint target")
     ap.add_argument("--adj-reps", type=int, default=3, help="timed primal/adjoint pairs")
-    ap.add_argument("--adj-steps", type=int, default=6, help="optimisation steps through the autograd Function")
+    ap.add_argument("--adj-steps", type=int, default=16, help="optimisation steps through the autograd Function")
     ap.add_argument("--no-light", action="store_true", help="skip the light tracing / bdpt measurement")
     ap.add_argument("--only-light", action="store_true", help="only the light tracing / bdpt measurement")
     ap.add_argument("--particles", type=int, default=1 << 24, help="particles of the light tracing leg")
