@@ -55,7 +55,8 @@ typedef enum rd_material_type {
     RD_MAT_ROUGH_PLASTIC = 3,  /* "rough_plastic"  Materials/RoughPlastic.cs   */
     RD_MAT_PERFECT_GLASS = 4,  /* "perfect_glass"  Materials/PerfectGlass.cs   */
     RD_MAT_PERFECT_MIRROR = 5, /* "perfect_mirror" Materials/PerfectMirror.cs  */
-    RD_MAT_DISNEY = 6          /* "disney"         Materials/Disney.cs         */
+    RD_MAT_DISNEY = 6,         /* "disney"         Materials/Disney.cs         */
+    RD_MAT_SUBSURFACE = 7      /* "subsurface"     Materials/Subsurface.cs     */
 } rd_material_type;
 
 /* Texture slot meaning per material (index into rd_scene_desc.textures):
@@ -68,7 +69,13 @@ typedef enum rd_material_type {
  *   DISNEY         tex[0..14] = base_color metallic roughness eta specular
  *                  specular_tint anisotropic sheen sheen_tint clearcoat
  *                  clearcoat_gloss scatting_distance transmission
- *                  transmission_roughness flatness                (thin)   */
+ *                  transmission_roughness flatness                (thin)
+ *   SUBSURFACE     tex[0]=r tex[1]=t tex[2]=a tex[3]=scatting_distance
+ *                                                              (eta_a,eta_b)
+ * A non-thin Disney material with a non-black scatting_distance and a
+ * subsurface material with one sample the BSSRDF of Bssrdf.cs in the path
+ * integrator's Mis method (PathTracer.cs:244-259; the reference's branch is
+ * marked unfinished and is reproduced as written). */
 typedef struct rd_material_desc {
     int32_t type;       /* rd_material_type                                 */
     int32_t two_sided;  /* "is_two_side"                                    */

@@ -1277,7 +1277,8 @@ static bool refract(V3 wi, V3 n, float eta, V3* wt) {
 }
 
 // ---------------------------------------------------------------- materials
-enum { T_REFLECTION = 1, T_TRANSMISSION = 2, T_DIFFUSE = 4, T_GLOSSY = 8, T_SPECULAR = 16 };  // IBxdf.cs:7-15
+enum { T_REFLECTION = 1, T_TRANSMISSION = 2, T_DIFFUSE = 4, T_GLOSSY = 8, T_SPECULAR = 16,
+       T_SUBSURFACE = 32 };  // IBxdf.cs:7-15
 struct BSample { V3 wi; C3 fr; float pdf = 0; int type = 0; };
 
 static inline C3 tex_at(const cr_scene& s, int t, V2 uv) { return s.tex[t].sample(uv); }
@@ -1562,6 +1563,7 @@ static C3 material_fr(const cr_scene& s, const rd_material_desc& m, V3 wo, V3 wi
         return plastic_fr(d, tex_at(s, m.tex[0], uv), m.kd, m.ks, m.eta_a, m.eta_b, m.two_sided != 0, wo, wi);
     }
     case RD_MAT_PERFECT_GLASS:     // FresnelSpecularBsdf.Fr (FresnelSpecularBsdf.cs:30-33)
+    case RD_MAT_SUBSURFACE:        // Subsurface.Fr: the same FresnelSpecularBsdf (Subsurface.cs:31-36)
     case RD_MAT_PERFECT_MIRROR:    // SpecularReflectionBrdf.Fr (SpecularReflectionBrdf.cs:17-20)
         return C3(0.0f);
     case RD_MAT_DISNEY:
@@ -1616,6 +1618,7 @@ static float material_pdf(const cr_scene& s, const rd_material_desc& m, V3 wo, V
         return plastic_pdf(d, m.kd, m.ks, m.two_sided != 0, wo, wi);
     }
     case RD_MAT_PERFECT_GLASS:
+    case RD_MAT_SUBSURFACE:
     case RD_MAT_PERFECT_MIRROR:
         return 0.0f;
     case RD_MAT_DISNEY:
@@ -1737,8 +1740,18 @@ static BSample disney_sample(const Disney& b, V3 wo, Rng& rng, bool imp = false)
             if ((same_hemisphere(wo, wi) ? abs_cos_theta(wi) / PI_F : 0.0f) == 0) return BSample();
             return finish(wi, T_REFLECTION | T_DIFFUSE);
         }
-        // scatting_distance > 0 selects the BSSRDF branch, rejected at scene creation
-        return BSample();
+        // scatting_distance > 0: SpecularTransmissionBtdf(1, 1, Eta).Sample
+        // (SpecularTransmissionBtdf.cs:33-48), pdf scaled by the diffuse
+        // weight, flagged for the BSSRDF (DisneyBsdf.cs:357-364)
+        bool entering = cos_theta(wo) > 0;
+        float ei = entering ? 1.0f : b.eta, et = entering ? b.eta : 1.0f;
+        V3 n = entering ? V3(0, 0, 1) : V3(0, 0, -1);
+        if (!refract(wo, n, ei / et, &wi) || cos_theta(wi) == 0.0f) return BSample();
+        C3 ft = C3(1.0f) * (1.0f - fresnel_dielectric(cos_theta(wo), 1, b.eta));
+        if (!imp) ft = ft * ((ei * ei) / (et * et));
+        ft = ft / abs_cos_theta(wi);
+        out.wi = wi; out.fr = ft; out.pdf = 1.0f * dw; out.type = T_TRANSMISSION | T_SPECULAR | T_SUBSURFACE;
+        return out;
     }
     u -= dw;
     if (u < tw) {
@@ -1792,13 +1805,18 @@ static BSample material_sample(const cr_scene& s, const rd_material_desc& m, V3 
         out.type = ty;
         return out;
     }
+    case RD_MAT_SUBSURFACE:                                          // Subsurface.Sample (Subsurface.cs:45-55)
     case RD_MAT_PERFECT_GLASS: {                                     // FresnelSpecularBsdf.Sample (:34-65)
         C3 R = tex_at(s, m.tex[0], uv), T = tex_at(s, m.tex[1], uv);
         float f = fresnel_dielectric(cos_theta(wo), m.eta_a, m.eta_b);
+        // a subsurface material with a non-black scatting distance adds the
+        // SubsurfaceScatting flag to whatever FresnelSpecularBsdf sampled
+        const int sss = (m.type == RD_MAT_SUBSURFACE && !tex_at(s, m.tex[3], uv).black()) ? T_SUBSURFACE : 0;
         if (rng.next_float() < f) {
             if (cos_theta(wo) == 0) return out;
             V3 wi(-wo.x, -wo.y, wo.z);
-            out.wi = wi; out.fr = f * R / abs_cos_theta(wi); out.pdf = f; out.type = T_REFLECTION | T_SPECULAR;
+            out.wi = wi; out.fr = f * R / abs_cos_theta(wi); out.pdf = f;
+            out.type = T_REFLECTION | T_SPECULAR | sss;
             return out;
         }
         bool entering = cos_theta(wo) > 0;
@@ -1809,7 +1827,7 @@ static BSample material_sample(const cr_scene& s, const rd_material_desc& m, V3 
         C3 ft = T * (1.0f - f);
         if (!imp) ft = ft * ((ei * ei) / (et * et));   // TransportMode.Radiance only (:58-61)
         ft = ft / abs_cos_theta(wi);
-        out.wi = wi; out.fr = ft; out.pdf = 1 - f; out.type = T_TRANSMISSION | T_SPECULAR;
+        out.wi = wi; out.fr = ft; out.pdf = 1 - f; out.type = T_TRANSMISSION | T_SPECULAR | sss;
         return out;
     }
     case RD_MAT_PERFECT_MIRROR: {                                    // PerfectMirror.Sample + SpecularReflectionBrdf
@@ -2057,14 +2075,54 @@ static inline float power_heuristic(int nf, float fp, int ng, float gp) {
 static inline Ray spawn_ray(const Inct& it, V3 d) { return Ray(it.P, d, 0.0001f); }  // Intersection.cs:48-51
 
 // PathTracer.Mis.EstimateDirect (Integrators/PathTracer.cs:309-399)
-static C3 estimate_direct(const cr_scene& s, Rng& rng, int light, const Inct& it, uint64_t* st) {
+// BssrdfAdapter (Bssrdf.cs:203-243): the exit lobe at a BSSRDF's sampled
+// point, (1 - Fr(cos wi)) / (c pi) with c = 1 - 2 FresnelMoment(1 / eta),
+// times eta^2 in radiance mode; no hemisphere test in Fr, a cosine lobe on
+// wo's side for Pdf / Sample.
+static float fresnel_moment(float eta) {                       // BssrdfUtility.FresnelMoment (:247-267)
+    float eta2 = eta * eta, eta3 = eta2 * eta, eta4 = eta2 * eta2, eta5 = eta2 * eta3;
+    if (eta < 1)
+        return 0.45966f - 1.73965f * eta + 3.37668f * eta2 - 3.904945f * eta3 + 2.49277f * eta4 - 0.68441f * eta5;
+    return -4.61686f + 11.1136f * eta - 10.4646f * eta2 + 5.11455f * eta3 - 1.27198f * eta4 + 0.12746f * eta5;
+}
+static C3 adapter_fr(float eta, V3 wi) {
+    float ci = 1 - 2 * fresnel_moment(1 / eta);
+    float fr = fresnel_dielectric(cos_theta(wi), 1, eta);
+    C3 f = C3((1 - fr) / (ci * PI_F));
+    return f * (eta * eta);
+}
+static float adapter_pdf(V3 wo, V3 wi) { return same_hemisphere(wo, wi) ? abs_cos_theta(wi) / PI_F : 0.0f; }
+static BSample adapter_sample(float eta, V3 wo, Rng& rng) {
+    BSample out;
+    float ux = rng.next_float(), uy = rng.next_float();
+    V3 wi = normalize(square_to_cosine_hemisphere(ux, uy));
+    if (wo.z < 0) wi.z *= -1;
+    if (!same_hemisphere(wo, wi)) return out;
+    out.wi = wi;
+    out.pdf = adapter_pdf(wo, wi);
+    out.fr = adapter_fr(eta, wi);
+    out.type = T_REFLECTION | T_DIFFUSE;
+    return out;
+}
+
+// adapter_eta: the BSSRDF adapter's eta in place of the surface material
+// (EstimateDirect's bssrdfAdapter argument).
+static C3 estimate_direct(const cr_scene& s, Rng& rng, int light, const Inct& it, uint64_t* st,
+                          const float* adapter_eta = nullptr) {
     const rd_material_desc& m = s.mats[s.prims[it.prim].material];
     V3 wo = it.wr;
     C3 le(0.0f);
     LSample ls = light_sample_li(s, light, it.P, rng);
     if (ls.pdf > 0.0f && !ls.li.black()) {
-        C3 fr = material_fr(s, m, it.sh.to_local(wo), it.sh.to_local(ls.wi), it.UV);
-        float spdf = material_pdf(s, m, it.sh.to_local(wo), it.sh.to_local(ls.wi), it.UV);
+        C3 fr;
+        float spdf;
+        if (adapter_eta) {
+            fr = adapter_fr(*adapter_eta, it.sh.to_local(ls.wi));
+            spdf = adapter_pdf(it.sh.to_local(wo), it.sh.to_local(ls.wi));
+        } else {
+            fr = material_fr(s, m, it.sh.to_local(wo), it.sh.to_local(ls.wi), it.UV);
+            spdf = material_pdf(s, m, it.sh.to_local(wo), it.sh.to_local(ls.wi), it.UV);
+        }
         if (!fr.black()) {
             if (!scene_is_occluded(s, it.P, ls.p, st)) {
                 fr = fr * abs_cos_theta(it.sh.to_local(ls.wi));
@@ -2073,7 +2131,8 @@ static C3 estimate_direct(const cr_scene& s, Rng& rng, int light, const Inct& it
             }
         }
     }
-    BSample bs = material_sample(s, m, it.sh.to_local(wo), it.UV, rng);
+    BSample bs = adapter_eta ? adapter_sample(*adapter_eta, it.sh.to_local(wo), rng)
+                             : material_sample(s, m, it.sh.to_local(wo), it.UV, rng);
     C3 fr = bs.fr * abs_cos_theta(bs.wi);
     float spdf = bs.pdf;
     bool sampled_specular = (bs.type & T_SPECULAR) != 0;
@@ -2099,17 +2158,154 @@ static C3 estimate_direct(const cr_scene& s, Rng& rng, int light, const Inct& it
 }
 
 // PathTracer.Mis.SampleLightToEstimateDirect (PathTracer.cs:273-307)
-static C3 sample_light_direct(const cr_scene& s, Rng& rng, const Inct& it, uint64_t* st) {
+static C3 sample_light_direct(const cr_scene& s, Rng& rng, const Inct& it, uint64_t* st,
+                              const float* adapter_eta = nullptr) {
     C3 result;
     if (s.integ.strategy == RD_STRATEGY_UNIFORM) {
         if (s.lights.empty()) return result;                       // Scene.SampleLight (Scene.cs:47-57)
         int l = rng.next_int((int)s.lights.size());
         float lpdf = 1.0f / s.lights.size();
-        if (lpdf > 0.0f) result = result + estimate_direct(s, rng, l, it, st) / lpdf;
+        if (lpdf > 0.0f) result = result + estimate_direct(s, rng, l, it, st, adapter_eta) / lpdf;
     } else {
-        for (int l = 0; l < (int)s.lights.size(); l++) result = result + estimate_direct(s, rng, l, it, st);
+        for (int l = 0; l < (int)s.lights.size(); l++)
+            result = result + estimate_direct(s, rng, l, it, st, adapter_eta);
     }
     return result;
+}
+
+// ---------------------------------------------------------------- BSSRDF
+// BssrdfUtility.ScattingLUT (Bssrdf.cs:289-): 2048 radii r_i with
+// 1 - e^(-r)/4 - 3 e^(-r/3)/4 = i / 2047, the inverse CDF of the
+// normalised-diffusion radial pdf in units of d; solved here by bisection in
+// double.  i = 2047 has no root; the reference stores 111.68667351 there, a
+// value no accepted sample reaches (rMax is the 0.996 quantile).
+static const std::vector<float>& scatter_lut() {
+    static const std::vector<float> lut = [] {
+        const int n = 2048;
+        std::vector<float> t(n);
+        for (int i = 0; i < n - 1; i++) {
+            const double target = (double)i / (n - 1);
+            double lo = 0, hi = 64;
+            for (int it = 0; it < 200; it++) {
+                const double mid = 0.5 * (lo + hi);
+                const double f = 1 - 0.25 * std::exp(-mid) - 0.75 * std::exp(-mid / 3);
+                if (f < target) lo = mid;
+                else hi = mid;
+            }
+            t[i] = (float)(0.5 * (lo + hi));
+        }
+        t[n - 1] = 111.68667351f;
+        return t;
+    }();
+    return lut;
+}
+
+// NormalizedDiffusionRadialProfile (Bssrdf.cs:64-124): A and d per
+// channel, D = d / (100 (A - 0.33)^4 + 3.5).
+struct Profile {
+    C3 A, D;
+    Profile(C3 a, C3 d) : A(a) {
+        C3 t = a - C3(0.33f);
+        C3 sc = t * t * t * t * 100.0f + C3(3.5f);
+        D = C3(d.r / sc.r, d.g / sc.g, d.b / sc.b);
+    }
+    float dch(int c) const { return c == 0 ? D.r : (c == 1 ? D.g : D.b); }
+    float pdf_sr(int c, float distance) const {                 // PdfSr
+        float chd = dch(c);
+        distance /= chd;
+        return 0.25f * (std::exp(-distance) + std::exp(-distance / 3)) / chd;
+    }
+    C3 sr(float distance) const {                               // Sr
+        auto one = [&](float a, float d) {
+            return a * (std::exp(-distance / d) + std::exp(-distance / (3.0f * d))) / (8 * PI_F * d);
+        };
+        return C3(one(A.r, D.r), one(A.g, D.g), one(A.b, D.b));
+    }
+    float sample_sr(int c, float u) const {                     // SampleSr: the radius
+        const std::vector<float>& lut = scatter_lut();
+        float global = u * (float)(lut.size() - 1);
+        int low = std::min((int)global, (int)lut.size() - 2);
+        float local = global - low;
+        float r = lerpf(local, lut[low], lut[low + 1]);
+        return r * dch(c);
+    }
+};
+
+struct SssSample {
+    bool ok = false;
+    C3 S;
+    float pdf = 0;
+    Inct pi;
+};
+
+// SeparableBssrdf.SamplePi / PdfPi (Bssrdf.cs:139-200) at po, for the
+// material `mat` (Scene.Intersect's surfaces compared by material index).
+static SssSample sss_sample_pi(const cr_scene& s, const Inct& po, int mat, const Profile& prof, float eta, Rng& rng,
+                               uint64_t* st) {
+    SssSample out;
+    int channel = rng.next_int(3);
+    Frame proj = po.sh;
+    float u = rng.next_float();
+    if (u < 0.5f) {
+        proj = po.sh;
+    } else if (u < 0.75f) {
+        proj.X = po.sh.X; proj.Y = po.sh.Z; proj.Z = po.sh.Y;     // Coordinate(X, Z, Y)
+    } else {
+        proj.X = po.sh.Z; proj.Y = po.sh.Y; proj.Z = po.sh.X;     // Coordinate(Z, Y, X)
+    }
+    float r = prof.sample_sr(channel, rng.next_float());
+    float r_max = prof.sample_sr(channel, 0.996f);
+    if (r <= 0 || r > r_max) return out;
+    float phi = 2 * PI_F * rng.next_float();
+    float hl = std::sqrt(nmax(0, sqr(r_max) - sqr(r)));
+    float sp = std::sin(phi), cp = std::cos(phi);
+    V3 start(r * cp, r * sp, hl);
+    float len = 2 * hl;
+    Ray ray(po.P + proj.to_world(start), -proj.Z, 0.0001f, len);
+    std::vector<Inct> hits;
+    while (true) {
+        if (ray.tmin >= ray.tmax) break;
+        Inct it;
+        if (!scene_intersect(s, ray, &it, st)) break;
+        if (s.prims[it.prim].material == mat) hits.push_back(it);
+        float adv = it.T + 0.0001f;
+        ray = Ray(ray.at(adv), ray.d, ray.tmin, ray.tmax - adv);
+    }
+    if (hits.empty()) return out;
+    int sel = rng.next_int((int)hits.size());
+    const Inct& e = hits[sel];
+    // PdfPi
+    V3 ld = po.sh.to_local(po.P - e.P);
+    V3 ln = po.sh.to_local(e.sh.Z);
+    float rproj[3] = {std::sqrt(ld.y * ld.y + ld.z * ld.z), std::sqrt(ld.z * ld.z + ld.x * ld.x),
+                      std::sqrt(ld.x * ld.x + ld.y * ld.y)};
+    const float axis_pdf[3] = {0.25f, 0.25f, 0.5f};
+    float pdf = 0;
+    for (int axis = 0; axis < 3; axis++) {
+        float lna = std::fabs(axis == 0 ? ln.x : (axis == 1 ? ln.y : ln.z));
+        for (int c = 0; c < 3; c++) pdf += lna * prof.pdf_sr(c, rproj[axis]) * axis_pdf[axis];
+    }
+    pdf = pdf * (1.0f / 3.0f);
+    out.pdf = pdf / (2 * PI_F) / (float)hits.size();
+    float fro = 1 - fresnel_dielectric(cos_theta(po.sh.to_local(po.wr)), 1, eta);
+    out.S = prof.sr(length(po.P - e.P)) * fro;
+    out.pi = e;
+    out.ok = true;
+    return out;
+}
+
+// The BSSRDF of the material at po: a Disney material's profile is
+// (BaseColor * DiffuseWeight, ScattingDistance) with its Eta
+// (DisneyBsdf.GetBssrdf, DisneyBsdf.cs:514-518); a subsurface material's
+// (A, ScattingDistance) with EtaB / EtaA (Subsurface.cs:57-64).
+static Profile sss_profile(const cr_scene& s, const rd_material_desc& m, V2 uv, float* eta) {
+    if (m.type == RD_MAT_DISNEY) {
+        Disney b = load_disney(s, m, uv);
+        *eta = b.eta;
+        return Profile(b.color * (1 - b.metallic), b.sdist);
+    }
+    *eta = m.eta_b / m.eta_a;
+    return Profile(tex_at(s, m.tex[2], uv), tex_at(s, m.tex[3], uv));
 }
 
 // PathTracer.Mis (PathTracer.cs:192-270)
@@ -2140,6 +2336,25 @@ static C3 li_mis(const cr_scene& s, Ray ray, Rng& rng, uint64_t* st) {
         if (bs.pdf > 0.0f) coeff = coeff * (bs.fr * abs_cos_theta(bs.wi) / bs.pdf);
         else break;
         ray = spawn_ray(it, it.sh.to_world(bs.wi));
+        if ((bs.type & T_SUBSURFACE) && (bs.type & T_TRANSMISSION)) {
+            // the BSSRDF branch (PathTracer.cs:244-259): a point pi on the same
+            // material, its direct light through the adapter, the adapter's
+            // exit direction
+            float eta;
+            Profile prof = sss_profile(s, s.mats[mat], it.UV, &eta);
+            SssSample sss = sss_sample_pi(s, it, mat, prof, eta, rng, st);
+            if (!sss.ok || sss.pdf == 0 || sss.S.black()) break;
+            coeff = coeff * (sss.S / sss.pdf);
+            Inct pi = sss.pi;
+            pi.T = length(it.P - pi.P);
+            pi.prim = it.prim;                                  // Intersection(.., inct.Shape, ..)
+            radiance = radiance + coeff * sample_light_direct(s, rng, pi, st, &eta);
+            BSample ns = adapter_sample(eta, pi.sh.to_local(pi.wr), rng);
+            if (ns.pdf == 0 || ns.fr.black()) break;
+            coeff = coeff * (ns.fr * abs_cos_theta(ns.wi) / ns.pdf);
+            specular_path = (ns.type & T_SPECULAR) != 0;
+            ray = spawn_ray(pi, pi.sh.to_world(ns.wi));
+        }
         if (bounces > I.min_depth) {
             float q = nmin(coeff.max_elem(), I.rr_threshold);
             if (rng.next_float() > q) break;
@@ -3178,15 +3393,7 @@ cr_scene* cr_scene_create(const rd_scene_desc* d) {
     }
     for (int i = 0; i < d->num_materials; i++) {
         int ty = s->mats[i].type;
-        if (ty < RD_MAT_DIFFUSE || ty > RD_MAT_DISNEY) { set_err("unknown material type"); delete s; return nullptr; }
-        if (ty == RD_MAT_DISNEY) {
-            // scatting_distance > 0 takes the reference's unfinished BSSRDF branch
-            // (PathTracer.cs:244-259, "TODO"); it is outside this path
-            const Tex& sd = s->tex[s->mats[i].tex[11]];
-            bool black = sd.w == 0 ? sd.value.black() : true;
-            for (size_t k = 0; sd.w > 0 && k < sd.data.size(); k++) if (sd.data[k] != 0.0f) black = false;
-            if (!black && !s->mats[i].thin) { set_err("Disney scatting_distance > 0 (BSSRDF) is not on this path"); delete s; return nullptr; }
-        }
+        if (ty < RD_MAT_DIFFUSE || ty > RD_MAT_SUBSURFACE) { set_err("unknown material type"); delete s; return nullptr; }
     }
     s->env_of_light.assign(d->num_lights, -1);
     for (int i = 0; i < d->num_lights; i++) {
@@ -3823,6 +4030,59 @@ int cr_material_sample_n(const cr_scene* s, int32_t material, int32_t n, const f
 }
 
 float cr_fresnel_dielectric(float ci, float ei, float et) { return fresnel_dielectric(ci, ei, et); }
+
+// BSSRDF probes
+void cr_sss_lut(float* out) {
+    const std::vector<float>& l = scatter_lut();
+    for (size_t i = 0; i < l.size(); i++) out[i] = l[i];
+}
+
+float cr_fresnel_moment(float eta) { return fresnel_moment(eta); }
+
+void cr_sss_profile(const float* a, const float* d, int32_t n, const int32_t* channel, const float* u, float* out) {
+    Profile p(C3(a[0], a[1], a[2]), C3(d[0], d[1], d[2]));
+    for (int32_t j = 0; j < n; j++) {
+        float r = p.sample_sr(channel[j], u[j]);
+        C3 sr = p.sr(r);
+        float* o = out + 6 * j;
+        o[0] = r; o[1] = p.pdf_sr(channel[j], r); o[2] = sr.r; o[3] = sr.g; o[4] = sr.b; o[5] = p.dch(channel[j]);
+    }
+}
+
+void cr_sss_adapter(float eta, int32_t n, const float* wo, const float* wi, float* out) {
+    for (int32_t j = 0; j < n; j++) {
+        V3 o(wo[3 * j], wo[3 * j + 1], wo[3 * j + 2]), i(wi[3 * j], wi[3 * j + 1], wi[3 * j + 2]);
+        C3 f = adapter_fr(eta, i);
+        out[4 * j] = f.r; out[4 * j + 1] = f.g; out[4 * j + 2] = f.b; out[4 * j + 3] = adapter_pdf(o, i);
+    }
+}
+
+int cr_sss_sample_pi(const cr_scene* s, int32_t prim, const float* P, const float* N, const float* wr, const float* uv,
+                     uint64_t key0, int32_t n, float* out) {
+    if (!s || prim < 0 || prim >= (int)s->prims.size()) return RD_ERR_INVALID;
+    const int mat = s->prims[prim].material;
+    if (mat < 0) return RD_ERR_INVALID;
+    Inct po;
+    po.P = V3(P[0], P[1], P[2]);
+    po.UV = V2(uv[0], uv[1]);
+    po.T = 0;
+    po.prim = prim;
+    po.sh = Frame(normalize(V3(N[0], N[1], N[2])));
+    po.wr = V3(wr[0], wr[1], wr[2]);
+    float eta;
+    Profile prof = sss_profile(*s, s->mats[mat], po.UV, &eta);
+    for (int32_t j = 0; j < n; j++) {
+        Rng rng; rng.key = key0 + (uint64_t)j; rng.n = 0;
+        uint64_t st[5] = {0, 0, 0, 0, 0};
+        SssSample r = sss_sample_pi(*s, po, mat, prof, eta, rng, st);
+        float* o = out + 9 * j;
+        o[0] = r.ok ? 1.0f : 0.0f;
+        o[1] = r.S.r; o[2] = r.S.g; o[3] = r.S.b; o[4] = r.pdf;
+        o[5] = r.pi.P.x; o[6] = r.pi.P.y; o[7] = r.pi.P.z;
+        o[8] = (float)st[0];
+    }
+    return RD_OK;
+}
 
 void cr_fresnel_conductor(float ci, const float* eta, const float* k, float* out) {
     C3 f = fresnel_conductor(ci, C3(1.0f), C3(eta[0], eta[1], eta[2]), C3(k[0], k[1], k[2]));

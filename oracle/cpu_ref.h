@@ -107,6 +107,18 @@ float cr_rho_ratio(const cr_scene* s, int32_t material, const float* uv, const f
  * microfacet distribution (kind 0 Beckmann, 1 GGX): out = {D(wh), G(wo,wi),
  * Pdf(wo,wh)}; sample_wh draws with the given RNG key. */
 float cr_fresnel_dielectric(float cos_i, float eta_i, float eta_t);
+/* BSSRDF probes (Bssrdf.cs): the 2048-entry radius table; FresnelMoment;
+ * the normalised-diffusion profile for n (channel, u) pairs: out[6j..] =
+ * {SampleSr radius, PdfSr(channel, r), Sr(r) rgb, D[channel]}; the adapter
+ * for n (wo, wi): out[4j..] = {Fr rgb, Pdf}; SamplePi at a point po of
+ * primitive `prim` (shading normal N, Wr, UV) for n keys key0 + j:
+ * out[9j..] = {ok, S rgb, pdf, pi.P xyz, closest-hit queries}. */
+void cr_sss_lut(float* out2048);
+float cr_fresnel_moment(float eta);
+void cr_sss_profile(const float* a, const float* d, int32_t n, const int32_t* channel, const float* u, float* out);
+void cr_sss_adapter(float eta, int32_t n, const float* wo, const float* wi, float* out);
+int cr_sss_sample_pi(const cr_scene* s, int32_t prim, const float* P, const float* N, const float* wr, const float* uv,
+                     uint64_t key0, int32_t n, float* out);
 void cr_fresnel_conductor(float cos_i, const float* eta, const float* k, float* out_rgb);
 void cr_mf_eval(int32_t kind, float roughness, float aniso, const float* wh, const float* wo, const float* wi, float* out3);
 void cr_mf_sample(int32_t kind, float roughness, float aniso, const float* wo, uint64_t key, float* wh);

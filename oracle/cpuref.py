@@ -33,6 +33,13 @@ def lib():
         L.cr_scene_num_nodes.argtypes = [C.c_void_p]
         L.cr_scene_max_depth.argtypes = [C.c_void_p]
         L.cr_scene_export_bvh.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.cr_sss_lut.argtypes = [C.c_void_p]
+        L.cr_fresnel_moment.restype = C.c_float
+        L.cr_fresnel_moment.argtypes = [C.c_float]
+        L.cr_sss_profile.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.cr_sss_adapter.argtypes = [C.c_float, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.cr_sss_sample_pi.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_uint64, C.c_int32, C.c_void_p]
         L.cr_scene_export_octree.argtypes = [C.c_void_p] + [C.c_void_p] * 5
         L.cr_octree_build.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_float] + \
             [C.c_void_p] * 6
@@ -208,6 +215,34 @@ def _octree_arrays(call):
     return nodes[: sz[0] * 32].reshape(-1, 32), first[: sz[1]], count[: sz[1]], prims[: sz[2]]
 
 
+def sss_lut():
+    out = np.zeros(2048, np.float32)
+    lib().cr_sss_lut(_ptr(out))
+    return out
+
+
+def fresnel_moment(eta):
+    return float(lib().cr_fresnel_moment(eta))
+
+
+def sss_profile(a, d, channel, u):
+    """NormalizedDiffusionRadialProfile: rows {SampleSr radius, PdfSr, Sr rgb, D[channel]}."""
+    channel = np.ascontiguousarray(channel, np.int32)
+    u = np.ascontiguousarray(u, np.float32)
+    out = np.zeros((u.size, 6), np.float32)
+    lib().cr_sss_profile(_ptr(np.asarray(a, np.float32)), _ptr(np.asarray(d, np.float32)), u.size, _ptr(channel),
+                         _ptr(u), _ptr(out))
+    return out
+
+
+def sss_adapter(eta, wo, wi):
+    wo = np.ascontiguousarray(wo, np.float32).reshape(-1, 3)
+    wi = np.ascontiguousarray(wi, np.float32).reshape(-1, 3)
+    out = np.zeros((wo.shape[0], 4), np.float32)
+    lib().cr_sss_adapter(eta, wo.shape[0], _ptr(wo), _ptr(wi), _ptr(out))
+    return out
+
+
 def octree_build(bmin, bmax, max_depth=10, max_count=10, out_bound=2.0):
     """The oracle's Octree build over caller bounds: (nodes [N,32] uint8,
     leaf_first, leaf_count, leaf_prims, now_depth)."""
@@ -254,6 +289,16 @@ class OracleScene:
         order = np.zeros(self.desc.num_prims(), np.int32)
         lib().cr_scene_export_bvh(self.h, _ptr(nodes), _ptr(order))
         return nodes.reshape(n, 32), order
+
+    def sss_sample_pi(self, prim, P, N, wr, uv=(0.0, 0.0), key0=1, n=1):
+        """SeparableBssrdf.SamplePi at a point of primitive `prim`: rows {ok,
+        S rgb, pdf, pi xyz, closest-hit queries}."""
+        f = lambda v: np.ascontiguousarray(v, np.float32)
+        out = np.zeros((n, 9), np.float32)
+        rc = lib().cr_sss_sample_pi(self.h, prim, _ptr(f(P)), _ptr(f(N)), _ptr(f(wr)), _ptr(f(uv)), key0, n, _ptr(out))
+        if rc:
+            raise RuntimeError("oracle sss_sample_pi: bad argument")
+        return out
 
     def export_octree(self):
         """(nodes [N,32] uint8, leaf_first, leaf_count, leaf_prims) of an octree scene."""

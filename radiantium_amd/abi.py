@@ -29,11 +29,12 @@ MAT_ROUGH_PLASTIC = 3
 MAT_PERFECT_GLASS = 4
 MAT_PERFECT_MIRROR = 5
 MAT_DISNEY = 6
+MAT_SUBSURFACE = 7
 
 MATERIAL_NAMES = {
     "diffuse": MAT_DIFFUSE, "rough_metal": MAT_ROUGH_METAL, "rough_glass": MAT_ROUGH_GLASS,
     "rough_plastic": MAT_ROUGH_PLASTIC, "perfect_glass": MAT_PERFECT_GLASS,
-    "perfect_mirror": MAT_PERFECT_MIRROR, "disney": MAT_DISNEY,
+    "perfect_mirror": MAT_PERFECT_MIRROR, "disney": MAT_DISNEY, "subsurface": MAT_SUBSURFACE,
 }
 
 LIGHT_AREA = 0

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -82,6 +83,14 @@ struct WaveBuffers {
     int* sort_rank = nullptr;       // queue position -> place in the traced order
     void* sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
+    // BSSRDF queues and points (path Mis over subsurface materials)
+    size_t sss_cap = 0;
+    std::vector<void*> sss_allocs;
+    int* sss_q = nullptr;
+    int* sss_q2 = nullptr;
+    float4* sss0 = nullptr;
+    float4* sss1 = nullptr;
+    float4* sss2 = nullptr;
     static void free_list(std::vector<void*>& v) {
         for (void* p : v) (void)hipFree(p);
         v.clear();
@@ -94,6 +103,10 @@ struct WaveBuffers {
         free_list(seg_allocs);
         free_list(adj_allocs);
         free_list(sort_allocs);
+        free_list(sss_allocs);
+        sss_cap = 0;
+        sss_q = sss_q2 = nullptr;
+        sss0 = sss1 = sss2 = nullptr;
         sort_cap = 0;
         sort_keys = sort_keys_alt = nullptr;
         sort_vals = sort_perm = sort_rank = nullptr;
@@ -120,6 +133,7 @@ struct rd_ctx {
     hipEvent_t ev_shaded = nullptr, ev_resolved = nullptr;
     std::mutex mu;
     int* counts = nullptr;        // device counters (kCntSlots ints)
+    float* sss_lut = nullptr;     // the BSSRDF's 2048 radii, uploaded on first use
     int* host_counts = nullptr;   // pinned mirror
     WaveBuffers wb;
     // rd_render_device_noalloc: this device's fixed work area (device
@@ -177,6 +191,7 @@ struct rd_scene {
     rd_ctx* ctx = nullptr;
     DevScene dev{};
     OctResult oct;                      // RD_ACCEL_OCTREE scenes: the tree as built
+    bool sss = false;                   // a material may sample a BSSRDF (subsurface, Disney with scatting)
     LtCamera lt_cam{};                  // SampleWi / Importance state (light paths)
     DevIntegrator integ{};
     std::vector<void*> allocs;
@@ -324,6 +339,8 @@ bool assign_pool(rd_ctx* c, WaveBuffers& w, size_t cap, bool vol, Alloc&& alloc)
         ok = ok && w.hits2 && ps.env && ps.nee_li && ps.sh_to && ps.sh_tr && ps.mis_tr && ps.hsub2;
     }
     ps.adj0 = ps.adj1 = ps.adj2 = nullptr;
+    ps.sss_q = ps.sss_q2 = nullptr;
+    ps.sss0 = ps.sss1 = ps.sss2 = nullptr;
     return ok;
 }
 
@@ -370,6 +387,68 @@ int ensure_wave_buffers(rd_ctx* c, size_t pool, size_t results, size_t pixels, b
         if (!w.sum || !w.sum2) { w.release(); return fail(RD_ERR_OOM, "hipMalloc failed for accumulation"); }
         w.pix_cap = pcap;
     }
+    return RD_OK;
+}
+
+// BssrdfUtility.ScattingLUT (Bssrdf.cs): r_i with 1 - e^(-r)/4 - 3 e^(-r/3)/4
+// = i / 2047 for i < 2047, bisected in double to the float nearest the root;
+// the last entry (no root) is the reference's 111.68667351, reached by no
+// accepted sample.
+static std::vector<float> scatter_radii() {
+    std::vector<float> t(2048);
+    for (int i = 0; i < 2047; i++) {
+        const double target = (double)i / 2047.0;
+        double lo = 0, hi = 64;
+        for (int it = 0; it < 200; it++) {
+            const double mid = 0.5 * (lo + hi);
+            if (1 - 0.25 * std::exp(-mid) - 0.75 * std::exp(-mid / 3) < target) lo = mid;
+            else hi = mid;
+        }
+        t[i] = (float)(0.5 * (lo + hi));
+    }
+    t[2047] = 111.68667351f;
+    return t;
+}
+
+// The BSSRDF queues and points for `pool` slots (kept across renders) and the
+// radius table (uploaded once per context); points w.ps at them.
+int ensure_sss_buffers(rd_ctx* c, WaveBuffers& w, size_t pool) {
+    if (!c->sss_lut) {
+        const std::vector<float> lut = scatter_radii();
+        if (hipMalloc((void**)&c->sss_lut, lut.size() * sizeof(float)) != hipSuccess) {
+            c->sss_lut = nullptr;
+            return fail(RD_ERR_OOM, "hipMalloc failed for the BSSRDF table");
+        }
+        if (hipMemcpy(c->sss_lut, lut.data(), lut.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(RD_ERR_DEVICE, "BSSRDF table upload failed");
+    }
+    if (pool > w.sss_cap) {
+        WaveBuffers::free_list(w.sss_allocs);
+        w.sss_cap = 0;
+        auto get = [&](size_t bytes) -> void* {
+            void* p = nullptr;
+            if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+            w.sss_allocs.push_back(p);
+            return p;
+        };
+        w.sss_q = (int*)get(pool * sizeof(int));
+        w.sss_q2 = (int*)get(pool * sizeof(int));
+        w.sss0 = (float4*)get(pool * sizeof(float4));
+        w.sss1 = (float4*)get(pool * sizeof(float4));
+        w.sss2 = (float4*)get(pool * sizeof(float4));
+        if (!w.sss_q || !w.sss_q2 || !w.sss0 || !w.sss1 || !w.sss2) {
+            WaveBuffers::free_list(w.sss_allocs);
+            w.sss_q = w.sss_q2 = nullptr;
+            w.sss0 = w.sss1 = w.sss2 = nullptr;
+            return fail(RD_ERR_OOM, "hipMalloc failed for the BSSRDF queues");
+        }
+        w.sss_cap = pool;
+    }
+    w.ps.sss_q = w.sss_q;
+    w.ps.sss_q2 = w.sss_q2;
+    w.ps.sss0 = w.sss0;
+    w.ps.sss1 = w.sss1;
+    w.ps.sss2 = w.sss2;
     return RD_OK;
 }
 
@@ -593,6 +672,7 @@ static void free_ctx(rd_ctx* c) {
     if (c->ev_t0) (void)hipEventDestroy(c->ev_t0);
     if (c->ev_t1) (void)hipEventDestroy(c->ev_t1);
     if (c->tail_exec) (void)hipGraphExecDestroy(c->tail_exec);
+    if (c->sss_lut) (void)hipFree(c->sss_lut);
     (void)hipEventDestroy(c->ev_shaded);
     (void)hipEventDestroy(c->ev_resolved);
     (void)hipStreamDestroy(c->stream2);
@@ -662,22 +742,20 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
             if (v < 0 || v >= d->num_vertices) return bad("triangle index out of range");
         }
     }
+    bool has_sss = false;
     for (int i = 0; i < d->num_materials; i++) {
         int ty = d->materials[i].type;
-        if (ty < RD_MAT_DIFFUSE || ty > RD_MAT_DISNEY) return bad("unknown material type");
+        if (ty < RD_MAT_DIFFUSE || ty > RD_MAT_SUBSURFACE) return bad("unknown material type");
         if (d->num_textures == 0) return bad("materials need textures (constants count)");
         for (int k = 0; k < 16; k++) {
             int tx = d->materials[i].tex[k];
             if (tx < 0 || tx >= d->num_textures) return bad("material texture index out of range");
         }
-        if (ty == RD_MAT_DISNEY && !d->materials[i].thin) {
-            // scatting_distance > 0 selects the reference's unfinished BSSRDF
-            // branch (PathTracer.cs:244-259), which is not on this path
-            const rd_texture_desc& sd = d->textures[d->materials[i].tex[11]];
-            bool black = sd.width == 0 ? (sd.value[0] == 0 && sd.value[1] == 0 && sd.value[2] == 0) : true;
-            for (size_t k = 0; sd.width > 0 && k < (size_t)sd.width * sd.height * 3; k++)
-                if (sd.data[k] != 0.0f) black = false;
-            if (!black) return bad("Disney scatting_distance > 0 (BSSRDF) is not on this path");
+        if ((ty == RD_MAT_DISNEY && !d->materials[i].thin) || ty == RD_MAT_SUBSURFACE) {
+            // a scattering distance that is not black somewhere samples the
+            // BSSRDF (an image texture may be replaced later: it counts)
+            const rd_texture_desc& sd = d->textures[d->materials[i].tex[ty == RD_MAT_DISNEY ? 11 : 3]];
+            if (sd.width > 0 || sd.value[0] != 0 || sd.value[1] != 0 || sd.value[2] != 0) has_sss = true;
         }
     }
     for (int i = 0; i < d->num_lights; i++) {
@@ -696,6 +774,7 @@ rd_scene* rd_scene_create(rd_ctx* c, const rd_scene_desc* d) {
     if (c->closed) { g_err = "context destroyed"; return nullptr; }
     rd_scene* s = new rd_scene();
     s->ctx = c;
+    s->sss = has_sss;
     c->live_scenes++;
     s->width = d->camera.width;
     s->height = d->camera.height;
@@ -1285,7 +1364,25 @@ struct PassLaunch {
     WaveBuffers* wb = nullptr;   // the pass's buffers (null: the context's own)
     bool sort_ok = true;         // the coherence sort may run (its buffers are allocated on demand)
     bool graph_ok = true;        // the device-driven tail may be captured as a hipGraph
+    bool sss = false;            // path Mis over a BSSRDF: each bounce runs the BSSRDF stage
+    const float* sss_lut = nullptr;
 };
+
+// The BSSRDF stage of a bounce (PathTracer.cs:244-259), after the bounce's
+// resolve and before the compaction that reads `alive`: the probe for every
+// path k_shade queued, pi's direct light and continuation, their shadow / MIS
+// rays and resolve; then the two queues are emptied for the next bounce.
+hipError_t sss_phase(rd_ctx* c, const PassLaunch& L, WaveBuffers& wb, hipStream_t st) {
+    hipError_t e;
+    if ((e = launch_sss_probe(L.sc, wb.ps, c->counts, L.sss_lut, st))) return e;
+    if ((e = launch_sss_shade(L.sc, wb.qshadow, wb.qmis, wb.ps, c->counts, st))) return e;
+    if ((e = launch_trace_secondary(L.sc, L.scenes, wb.qshadow, kCntShadow, wb.qmis, wb.ps, c->counts,
+                                    c->trace_schedule[1], c->trace_schedule[2], st)))
+        return e;
+    RayQueue q2{nullptr, wb.ps.sss_q2, nullptr, nullptr};
+    if ((e = launch_resolve(L.sc, L.method, q2, wb.ps, c->counts, kCntSssB, nullptr, st))) return e;
+    return launch_zero_slots(c->counts, kCntSss, kCntSssB, st);
+}
 
 // One bounce of the device-driven tail: the launches of run_pass's bounce
 // once every path has started (the same stages and streams, the shadow / MIS
@@ -1316,7 +1413,7 @@ hipError_t tail_bounce(rd_ctx* c, const PassLaunch& L, WaveBuffers& wb, RayQueue
         const bool ranked = L.method != RD_METHOD_BXDF;
         TAIL_TRY(launch_shade(L.sc, L.scenes, qin, wb.hits, wb.qshadow, wb.qmis, wb.ps, c->counts, cin, false, nullptr,
                               ranked, st));
-        const bool overlap = c->overlap;
+        const bool overlap = c->overlap && !L.sss;
         hipStream_t s2 = overlap ? c->stream2 : st;
         if (overlap) {
             TAIL_TRY(hipEventRecord(c->ev_shaded, st));
@@ -1336,6 +1433,7 @@ hipError_t tail_bounce(rd_ctx* c, const PassLaunch& L, WaveBuffers& wb, RayQueue
         else if (L.method == RD_METHOD_NEE)
             TAIL_TRY(launch_trace_shadow(L.sc, L.scenes, qsh, wb.ps, c->counts, csh, c->trace_schedule[1], s2));
         TAIL_TRY(launch_resolve(L.sc, L.method, qin, wb.ps, c->counts, cin, nullptr, s2));
+        if (L.sss) TAIL_TRY(sss_phase(c, L, wb, st));
         if (overlap) {
             TAIL_TRY(launch_zero_slots(c->counts, kCntShadow, kCntMis, s2));
             TAIL_TRY(hipEventRecord(c->ev_resolved, s2));
@@ -1398,6 +1496,7 @@ int run_tail(rd_ctx* c, Timer& tm, const PassLaunch& L, WaveBuffers& wb, RayQueu
         key_add(key, wb.qshadow); key_add(key, wb.qshadow2); key_add(key, wb.qmis); key_add(key, wb.blk);
         key_add(key, wb.blk2); key_add(key, qin); key_add(key, qout); key_add(key, cin); key_add(key, cout);
         key_add(key, c->trace_schedule); key_add(key, c->overlap); key_add(key, c->counts);
+        key_add(key, L.sss); key_add(key, L.sss_lut);
         if (!c->tail_exec || key != c->tail_key) {
             if (c->tail_exec) (void)hipGraphExecDestroy(c->tail_exec);
             c->tail_exec = nullptr;
@@ -1545,7 +1644,7 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
             // frame's longest after the first extension trace, and beside the
             // second bounce's extension trace both lose more to each other
             // than the overlap fills)
-            const bool overlap = c->overlap && all_started && bounce > 0 && !L.adj;
+            const bool overlap = c->overlap && all_started && bounce > 0 && !L.adj && !L.sss;
             hipStream_t s2 = overlap ? c->stream2 : st;
             if (overlap) {
                 HIP_TRY(hipEventRecord(c->ev_shaded, st));
@@ -1576,6 +1675,7 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
             HIP_TRY(tm.run(L.adj ? "resolve_adjoint" : "resolve", [&] {
                 return launch_resolve(L.sc, L.method, qin, wb.ps, c->counts, cin, L.adj, s2);
             }, s2));
+            if (L.sss) HIP_TRY(tm.run("sss", [&] { return sss_phase(c, L, wb, st); }));
             if (overlap) {
                 HIP_TRY(launch_zero_slots(c->counts, kCntShadow, kCntMis, s2));
                 HIP_TRY(hipEventRecord(c->ev_resolved, s2));
@@ -1701,6 +1801,11 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
         if (!single && r.scene->dev.octree)
             return fail(RD_ERR_UNSUPPORTED, "batched rendering covers BVH scenes (an octree scene renders alone)");
         const int it = r.scene->dev.integ.type;
+        if (r.scene->sss && it == RD_INTEGRATOR_PATH && r.scene->dev.integ.method == RD_METHOD_MIS) {
+            if (!single)
+                return fail(RD_ERR_UNSUPPORTED, "batched rendering covers scenes without a BSSRDF (such a scene renders alone)");
+            if (noalloc) return fail(RD_ERR_UNSUPPORTED, "rd_render_device_noalloc covers scenes without a BSSRDF");
+        }
         if (it != RD_INTEGRATOR_PATH && it != RD_INTEGRATOR_VOL_PATH)
             return fail(RD_ERR_UNSUPPORTED, single ? "this entry point renders path and vol_path scenes"
                                                    : "batched rendering covers path and vol_path scenes");
@@ -1760,6 +1865,15 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
     }
     WaveBuffers& wb = noalloc ? c->wb_na : c->wb;
     hipStream_t st = c->stream;
+    // path Mis over a BSSRDF (one scene: batches and the work area refuse it above)
+    const bool need_sss = single && reqs[0].scene->sss && reqs[0].scene->dev.integ.type == RD_INTEGRATOR_PATH &&
+                          reqs[0].scene->dev.integ.method == RD_METHOD_MIS;
+    if (need_sss) {
+        if (int rc = ensure_sss_buffers(c, wb, pool)) return rc;
+    } else {
+        wb.ps.sss_q = wb.ps.sss_q2 = nullptr;
+        wb.ps.sss0 = wb.ps.sss1 = wb.ps.sss2 = nullptr;
+    }
     std::vector<DevScene> dscenes(n);
     for (int i = 0; i < n; i++) dscenes[i] = reqs[i].scene->dev;
     const DevScene* d_scenes = nullptr;
@@ -1842,6 +1956,8 @@ int render_requests(rd_ctx* c, int32_t n, const rd_request* reqs, bool single, i
             PassLaunch L{rep, d_scenes, nullptr, kinds[g].second, kinds[g].first == RD_INTEGRATOR_VOL_PATH, wb.segs,
                          (int)segs.size(), (int)used, (int)std::min(pool, used), &wb, !noalloc,
                          !(single && (reqs[0].params.flags & RD_RENDER_NO_GRAPH))};
+            L.sss = need_sss;
+            L.sss_lut = c->sss_lut;
             rc = run_pass(c, tm, L, &max_bounce, cancel);
             if (rc) break;
             HIP_TRY(tm.run("accumulate", [&] {
@@ -2159,6 +2275,8 @@ int rd_adjoint_device(rd_ctx* c, rd_scene* s, const rd_render_params* p, const f
     if (!c || !s || !p || !d_dl_dimage || !d_tex_grads) return fail(RD_ERR_INVALID, "null argument");
     if (p->spp <= 0) return fail(RD_ERR_INVALID, "spp must be positive");
     if (s->integ.type != RD_INTEGRATOR_PATH) return fail(RD_ERR_UNSUPPORTED, "the adjoint covers the path integrator");
+    if (s->sss && s->integ.method == RD_METHOD_MIS)
+        return fail(RD_ERR_UNSUPPORTED, "the adjoint covers scenes without a BSSRDF");
     if (int rc = check_stack(s)) return rc;
     Crop cr;
     if (int rc = resolve_crop(s, p, &cr)) return rc;
@@ -2174,6 +2292,8 @@ int rd_adjoint_device(rd_ctx* c, rd_scene* s, const rd_render_params* p, const f
     if (int rc = ensure_wave_buffers(c, pool, pass_cap, npix, false)) return rc;
     if (int rc = ensure_adjoint_buffers(c, (size_t)s->num_textures)) return rc;
     WaveBuffers& wb = c->wb;
+    wb.ps.sss_q = wb.ps.sss_q2 = nullptr;
+    wb.ps.sss0 = wb.ps.sss1 = wb.ps.sss2 = nullptr;
     hipStream_t st = c->stream;
     // float64 accumulators for the requested textures
     std::vector<double*> acc(std::max(s->num_textures, 1), nullptr);

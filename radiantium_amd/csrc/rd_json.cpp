@@ -796,6 +796,12 @@ struct Loader {
                 if (k == 12) d.thin = read_b(m, "is_thin", false);
                 d.tex[k] = read_tex(m, names[k], defs[k], defs[k], defs[k]);
             }
+        } else if (t == "subsurface") {   // RendererBuilder.cs:206-215
+            d.type = RD_MAT_SUBSURFACE;
+            d.tex[0] = read_tex(m, "r", 1, 1, 1);
+            d.tex[1] = read_tex(m, "t", 1, 1, 1);
+            d.tex[2] = read_tex(m, "a", 1, 1, 1);
+            d.tex[3] = read_tex(m, "scatting_distance", 0, 0, 0);
         } else {
             unsupported("material type " + t);
         }

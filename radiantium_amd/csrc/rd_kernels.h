@@ -49,6 +49,16 @@ struct PathState {
     // position, for the extension / Transmittance hits and the IntersectTr hits
     int* hsub;
     int* hsub2;
+    // path Mis over subsurface materials (PathTracer.cs:244-259): the paths
+    // whose sample entered a BSSRDF this bounce (kCntSss), those whose probe
+    // found a point (kCntSssB), and per path the point {P, material},
+    // {shading normal, u}, {Wr, v}: first po, then the sampled pi.  Null
+    // when the scene has no BSSRDF.
+    int* sss_q;
+    int* sss_q2;
+    float4* sss0;
+    float4* sss1;
+    float4* sss2;
 };
 
 struct WaveParams {
@@ -75,7 +85,8 @@ enum : int {
     kCntNext = 37,        // next pass path to start
     kCntTail = 38,        // bounces the device-driven tail processed
     kCntChainStuck = 39,  // set when a medium chain reached kMaxChainSteps
-    kCntSlots = 40
+    kCntSss = 40, kCntSssB = 41,   // BSSRDF queues
+    kCntSlots = 42
 };
 
 // A Transmittance / IntersectTr chain takes one step per medium boundary it
@@ -156,6 +167,12 @@ hipError_t launch_ray_sort(const float4* nodes, RayQueue q, int n, int bits, uin
 // position], holes pid -1, shadow queue size = the extension queue's; MIS
 // rays are appended either way.
 // rank_identity (rank null): ranked with every ray at its own queue position.
+// PathTracer.Mis's BSSRDF branch for the paths k_shade queued (kCntSss):
+// sample pi and weight the path (queue kCntSssB), then light pi through the
+// adapter and continue (lut: the 2048 radii of the normalised-diffusion CDF).
+hipError_t launch_sss_probe(const DevScene& sc, PathState ps, int* counts, const float* lut, hipStream_t s);
+hipError_t launch_sss_shade(const DevScene& sc, RayQueue qshadow, RayQueue qmis, PathState ps, int* counts,
+                            hipStream_t s);
 hipError_t launch_shade(const DevScene& sc, const DevScene* scenes, RayQueue qin, const float4* hits,
                         RayQueue qshadow, RayQueue qmis, PathState ps, int* counts, int cnt_in, bool adjoint,
                         const int* rank, bool rank_identity, hipStream_t s);

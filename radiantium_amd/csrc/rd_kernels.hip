@@ -1520,7 +1520,7 @@ __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* 
     uint64_t nvert = 0;
     for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
         int k = base + threadIdx.x;
-        bool want_ext = false, want_shadow = false, want_mis = false;
+        bool want_ext = false, want_shadow = false, want_mis = false, want_sss = false;
         f3 eo = mk(0.f), ed = mk(0.f), so = mk(0.f), sd = mk(0.f), mo = mk(0.f), md = mk(0.f);
         float etmin = 0.f, etmax = 0.f, stmax = 0.f;
         int pid = 0;
@@ -1667,7 +1667,19 @@ __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* 
                     ed = to_world(it.sh, bs.wi);
                     eo = it.P; etmin = 0.0001f; etmax = kFloatMax;
                     bool alive = true;
-                    if (depth > in.min_depth) {
+                    if constexpr (RD_MAT(kMatDisney) || RD_MAT(kMatSubsurface)) {
+                        // the sample entered a BSSRDF (PathTracer.cs:244): the
+                        // BSSRDF stage samples pi, lights it and continues the
+                        // path (its Russian roulette included)
+                        if (in.method == 2 && ps.sss_q && (bs.type & kSubsurface) && (bs.type & kTransmission)) {
+                            want_sss = true;
+                            alive = false;
+                            ps.sss0[pid] = make_float4(it.P.x, it.P.y, it.P.z, __int_as_float(pr.material));
+                            ps.sss1[pid] = make_float4(it.sh.Z.x, it.sh.Z.y, it.sh.Z.z, it.u);
+                            ps.sss2[pid] = make_float4(it.wr.x, it.wr.y, it.wr.z, it.v);
+                        }
+                    }
+                    if (!want_sss && depth > in.min_depth) {
                         float qq = nmin(max_elem(beta), in.rr_threshold);
                         if (rng.next_float() > qq) alive = false;
                         else beta = beta / qq;
@@ -1685,7 +1697,7 @@ __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* 
             }
             st3(ps.L, pid, L);
             st3(ps.beta, pid, beta);
-            if (!want_ext) nflags |= kFlagDone;
+            if (!want_ext && !want_sss) nflags |= kFlagDone;
             ps.flags[pid] = nflags;
             ps.rng_n[pid] = rng.n;
             ps.depth[pid] = depth;
@@ -1726,6 +1738,12 @@ __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* 
             qmis.pid[slot] = pid;
             qmis.org[slot] = make_float4(mo.x, mo.y, mo.z, 0.0001f);
             qmis.dir[slot] = make_float4(md.x, md.y, md.z, kFloatMax);
+        }
+        if constexpr (RD_MAT(kMatDisney) || RD_MAT(kMatSubsurface)) {
+            if (ps.sss_q) {
+                const int ss = wave_append(&counts[kCntSss], want_sss);
+                if (want_sss) ps.sss_q[ss] = pid;
+            }
         }
     }
     if ((rank || rank_identity) && blockIdx.x == 0 && threadIdx.x == 0) counts[kCntShadow] = n;
@@ -1970,6 +1988,364 @@ hipError_t launch_zero(float4* buf, int n, hipStream_t s) {
     hipLaunchKernelGGL(k_zero4, dim3(blocks), dim3(256), 0, s, buf, n);
     return hipGetLastError();
 }
+// ---------------------------------------------------------------- BSSRDF
+// PathTracer.Mis's subsurface branch (PathTracer.cs:244-259) for the paths
+// k_shade queued in ps.sss_q: k_sss_probe samples pi (SeparableBssrdf.
+// SamplePi / PdfPi, Bssrdf.cs:139-200, with the normalised-diffusion profile
+// of :64-124), k_sss_shade lights pi through the BssrdfAdapter
+// (Bssrdf.cs:203-243) and continues the path with the adapter's sample.
+
+// The BSSRDF's profile and eta at po: a Disney material's
+// (BaseColor * (1 - Metallic), ScattingDistance, Eta) (DisneyBsdf.GetBssrdf,
+// DisneyBsdf.cs:514-518), a subsurface material's (A, ScattingDistance,
+// EtaB / EtaA) (Subsurface.cs:57-64); D = d / (100 (A - 0.33)^4 + 3.5).
+struct SssProfile {
+    f3 A, D;
+    float eta;
+};
+__device__ __forceinline__ SssProfile sss_profile(const DevScene& sc, int mat, float u, float v) {
+    const DevMaterial& dm = sc.mats[mat];
+    SssProfile p;
+    f3 sd;
+    if (dm.type == kMatDisney) {
+        const f3 col = tex_sample(sc, dm.tex[0], u, v);
+        const float met = tex_sample(sc, dm.tex[1], u, v).x;
+        p.A = col * (1 - met);
+        sd = tex_sample(sc, dm.tex[11], u, v);
+        p.eta = tex_sample(sc, dm.tex[3], u, v).x;
+    } else {
+        p.A = tex_sample(sc, dm.tex[2], u, v);
+        sd = tex_sample(sc, dm.tex[3], u, v);
+        p.eta = dm.eta_b / dm.eta_a;
+    }
+    const f3 t = p.A - mk(0.33f);
+    const f3 s = t * t * t * t * 100.0f + mk(3.5f);
+    p.D = mk(sd.x / s.x, sd.y / s.y, sd.z / s.z);
+    return p;
+}
+__device__ __forceinline__ float sss_ch(f3 v, int c) { return c == 0 ? v.x : (c == 1 ? v.y : v.z); }
+// NormalizedDiffusionRadialProfile.PdfSr
+__device__ __forceinline__ float sss_pdf_sr(const SssProfile& p, int c, float distance) {
+    const float chd = sss_ch(p.D, c);
+    distance /= chd;
+    return 0.25f * (expf(-distance) + expf(-distance / 3)) / chd;
+}
+// NormalizedDiffusionRadialProfile.Sr
+__device__ __forceinline__ f3 sss_sr(const SssProfile& p, float distance) {
+    auto one = [&](float a, float d) { return a * (expf(-distance / d) + expf(-distance / (3.0f * d))) / (8 * kPi * d); };
+    return mk(one(p.A.x, p.D.x), one(p.A.y, p.D.y), one(p.A.z, p.D.z));
+}
+// NormalizedDiffusionRadialProfile.SampleSr: the radius from the 2048-entry table
+__device__ __forceinline__ float sss_sample_sr(const SssProfile& p, int c, float u, const float* __restrict__ lut) {
+    const float g = u * 2047.0f;
+    const int low = min((int)g, 2046);
+    const float local = g - low;
+    return lerpf(local, lut[low], lut[low + 1]) * sss_ch(p.D, c);
+}
+
+// BssrdfUtility.FresnelMoment and the adapter lobe
+__device__ __forceinline__ float fresnel_moment(float eta) {
+    const float eta2 = eta * eta, eta3 = eta2 * eta, eta4 = eta2 * eta2, eta5 = eta2 * eta3;
+    if (eta < 1)
+        return 0.45966f - 1.73965f * eta + 3.37668f * eta2 - 3.904945f * eta3 + 2.49277f * eta4 - 0.68441f * eta5;
+    return -4.61686f + 11.1136f * eta - 10.4646f * eta2 + 5.11455f * eta3 - 1.27198f * eta4 + 0.12746f * eta5;
+}
+__device__ __forceinline__ f3 adapter_fr(float eta, f3 wi) {
+    const float ci = 1 - 2 * fresnel_moment(1 / eta);
+    const float fr = fr_dielectric(cos_theta(wi), 1, eta);
+    const f3 f = mk((1 - fr) / (ci * kPi));
+    return f * (eta * eta);
+}
+__device__ __forceinline__ float adapter_pdf(f3 wo, f3 wi) {
+    return same_hemisphere(wo, wi) ? abs_cos_theta(wi) / kPi : 0.0f;
+}
+
+// One closest-hit query of the probe chain over the scene's accelerator:
+// kKind 0 the BVH, 1 the two-level tree, 2 the octree.  Returns the hit
+// {t, prim, u, v} (prim -1: a miss) and, for a set triangle, *sub.
+template <int kKind>
+__device__ __forceinline__ float4 sss_closest(const DevScene& sc, float4 ra, float4 rb, int* stk, int* sub,
+                                              uint32_t& nv, uint32_t& nt) {
+    *sub = -1;
+    const float4 miss = make_float4(0.f, __int_as_float(-1), 0.f, 0.f);
+    if (sc.num_nodes == 0) return miss;
+    if constexpr (kKind == 1) {
+        TravI ti;
+        travi_begin(ti, ra, rb);
+        while (!trav_step_inst<false, 128>(sc, sc.nodes, sc.prims, ti, stk, nv, nt)) {}
+        const Trav& t = ti.t;
+        if (t.best.prim < 0) return miss;
+        *sub = ti.sub;
+        return make_float4(t.best.t, __int_as_float(t.best.prim), t.best.u, t.best.v);
+    } else {
+        Trav t;
+        trav_begin(t, ra, rb);
+        if constexpr (kKind == 2) oct_trace<false, 256>(sc.nodes, sc.prims, sc.num_nodes, t, nv, nt);
+        else while (!trav_step<false, 128>(sc.nodes, sc.prims, t, stk, nv, nt)) {}
+        if (t.best.prim < 0) return miss;
+        return make_float4(t.best.t, __int_as_float(t.best.prim), t.best.u, t.best.v);
+    }
+}
+
+// SeparableBssrdf.SamplePi (Bssrdf.cs:139-183) for each path of ps.sss_q:
+// channel, projection axis, radius and angle from the path's stream, the
+// probe chain along the axis collecting the hits on po's material (walked
+// once to count them, then again to the one Next(count) picks: the same
+// deterministic queries), PdfPi (:185-200), S = Sr(|po - pi|) (1 - F).  A
+// path with a point takes beta *= S / pdf and goes to ps.sss_q2; the others
+// end here (their radiance is final).  Also clears the shadow / MIS queue
+// counters k_sss_shade appends to.
+template <int kKind>
+__global__ __launch_bounds__(128) void k_sss_probe(DevScene sc, PathState ps, int* counts,
+                                                   const float* __restrict__ lut) {
+    __shared__ int lds[64 * 128];
+    int* stk = lds + threadIdx.x;
+    const int n = counts[kCntSss];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counts[kCntShadow] = 0;
+        counts[kCntMis] = 0;
+    }
+    uint32_t nv = 0, nt = 0, nq = 0;
+    for (int base = blockIdx.x * 128; base < n; base += gridDim.x * 128) {
+        const int k = base + threadIdx.x;
+        bool ok = false;
+        int pid = -1;
+        if (k < n) {
+            pid = ps.sss_q[k];
+            const float4 a0 = ps.sss0[pid], a1 = ps.sss1[pid], a2 = ps.sss2[pid];
+            const f3 P = mk(a0.x, a0.y, a0.z);
+            const int mat = __float_as_int(a0.w);
+            const Frame fr = make_frame(mk(a1.x, a1.y, a1.z));
+            const f3 wr = mk(a2.x, a2.y, a2.z);
+            const SssProfile prof = sss_profile(sc, mat, a1.w, a2.w);
+            Rng rng{ps.key[pid], ps.rng_n[pid]};
+            const int ch = rng.next_int(3);
+            const float uax = rng.next_float();
+            Frame pj = fr;
+            if (uax < 0.5f) {
+            } else if (uax < 0.75f) {
+                pj.X = fr.X; pj.Y = fr.Z; pj.Z = fr.Y;   // Coordinate(X, Z, Y)
+            } else {
+                pj.X = fr.Z; pj.Y = fr.Y; pj.Z = fr.X;   // Coordinate(Z, Y, X)
+            }
+            const float r = sss_sample_sr(prof, ch, rng.next_float(), lut);
+            const float r_max = sss_sample_sr(prof, ch, 0.996f, lut);
+            if (!(r <= 0 || r > r_max)) {
+                const float phi = 2 * kPi * rng.next_float();
+                const float hl = sqrtf(nmax(0, sqr(r_max) - sqr(r)));
+                const float sp = sinf(phi), cp = cosf(phi);
+                const f3 o0 = P + to_world(pj, mk(r * cp, r * sp, hl));
+                const f3 d = -pj.Z;
+                const float len = 2 * hl;
+                // the chain: a query, the hit counted when on po's material,
+                // the origin moved 1e-4 past it and MaxT shortened to match
+                auto walk = [&](int stop, float4* hit, int* hsub, f3* horg, bool count_queries) -> int {
+                    f3 o = o0;
+                    float tmax = len;
+                    int found = 0;
+                    for (int step = 0; step < kMaxChainSteps; step++) {
+                        if (0.0001f >= tmax) return found;
+                        int sub;
+                        const float4 h = sss_closest<kKind>(sc, make_float4(o.x, o.y, o.z, 0.0001f),
+                                                            make_float4(d.x, d.y, d.z, tmax), stk, &sub, nv, nt);
+                        if (count_queries) nq++;
+                        const int prim = __float_as_int(h.y);
+                        if (prim < 0) return found;
+                        if (sc.prim_info[prim].material == mat) {
+                            if (found == stop) { *hit = h; *hsub = sub; *horg = o; return found + 1; }
+                            found++;
+                        }
+                        const float adv = h.x + 0.0001f;
+                        o = o + d * adv;
+                        tmax -= adv;
+                    }
+                    atomicOr(counts + kCntChainStuck, 1);
+                    return found;
+                };
+                float4 h = make_float4(0.f, 0.f, 0.f, 0.f);
+                int hsub = -1;
+                f3 horg = o0;
+                const int count = walk(-1, &h, &hsub, &horg, true);
+                if (count > 0) {
+                    const int sel = rng.next_int(count);
+                    walk(sel, &h, &hsub, &horg, false);
+                    const DevPrim pr = sc.prim_info[__float_as_int(h.y)];
+                    const Inct pi = make_inct_any(sc, pr, pr.kind == kInst ? hsub : -1, horg, d, h.z, h.w, h.x);
+                    // PdfPi
+                    const f3 ld = to_local(fr, P - pi.P);
+                    const f3 ln = to_local(fr, pi.sh.Z);
+                    const float rproj[3] = {sqrtf(ld.y * ld.y + ld.z * ld.z), sqrtf(ld.z * ld.z + ld.x * ld.x),
+                                            sqrtf(ld.x * ld.x + ld.y * ld.y)};
+                    const float axis_pdf[3] = {0.25f, 0.25f, 0.5f};
+                    float pdf = 0;
+                    for (int axis = 0; axis < 3; axis++) {
+                        const float lna = fabsf(axis == 0 ? ln.x : (axis == 1 ? ln.y : ln.z));
+                        for (int c = 0; c < 3; c++) pdf += lna * sss_pdf_sr(prof, c, rproj[axis]) * axis_pdf[axis];
+                    }
+                    pdf = pdf * (1.0f / 3.0f);
+                    pdf = pdf / (2 * kPi) / (float)count;
+                    const float fro = 1 - fr_dielectric(cos_theta(to_local(fr, wr)), 1, prof.eta);
+                    const f3 S = sss_sr(prof, length(P - pi.P)) * fro;
+                    if (!(pdf == 0 || is_black(S))) {
+                        ok = true;
+                        st3(ps.beta, pid, ld3(ps.beta, pid) * (S / pdf));
+                        ps.sss0[pid] = make_float4(pi.P.x, pi.P.y, pi.P.z, __int_as_float(mat));
+                        ps.sss1[pid] = make_float4(pi.sh.Z.x, pi.sh.Z.y, pi.sh.Z.z, pi.u);
+                        ps.sss2[pid] = make_float4(pi.wr.x, pi.wr.y, pi.wr.z, pi.v);
+                    }
+                }
+            }
+            ps.rng_n[pid] = rng.n;
+            if (!ok) {
+                // the path ends at po: its radiance (po's light included) is final
+                ps.flags[pid] = kFlagDone;
+                ps.alive[pid] = 0;
+                ps.res[ps.gid[pid]] = ps.L[pid];
+            }
+        }
+        const int slot = wave_append(&counts[kCntSssB], ok);
+        if (ok) ps.sss_q2[slot] = pid;
+    }
+    add_class_counters(counts, 0, nq, nv, nt);
+    add_counter((unsigned long long*)(counts + kCntRaysClosest), nq);
+}
+
+// Direct light at pi through the BssrdfAdapter and the adapter's continuation
+// (PathTracer.cs:252-258, EstimateDirect :309-389 with bssrdfAdapter): the
+// same light pick, light sample, shadow / MIS rays and Russian roulette as
+// k_shade's Mis vertex, with the adapter's Fr / Pdf / Sample at pi (its eta
+// sampled at pi's UV for a Disney material).  Resets the traversal work
+// cursors for the shadow / MIS launch that follows.
+__global__ __launch_bounds__(256) void k_sss_shade(DevScene sc, RayQueue qshadow, RayQueue qmis, PathState ps,
+                                                   int* counts) {
+    const int n = counts[kCntSssB];
+    if (blockIdx.x == 0 && threadIdx.x < 3) counts[kCntWork + threadIdx.x] = 0;
+    const DevIntegrator& in = sc.integ;
+    for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        const int k = base + threadIdx.x;
+        bool want_shadow = false, want_mis = false;
+        f3 so = mk(0.f), sd = mk(0.f), mo = mk(0.f), md = mk(0.f);
+        float stmax = 0.f;
+        int pid = 0;
+        if (k < n) {
+            pid = ps.sss_q2[k];
+            const float4 a0 = ps.sss0[pid], a1 = ps.sss1[pid], a2 = ps.sss2[pid];
+            const f3 P = mk(a0.x, a0.y, a0.z);
+            const DevMaterial& dm = sc.mats[__float_as_int(a0.w)];
+            const float eta = dm.type == kMatDisney ? tex_sample(sc, dm.tex[3], a1.w, a2.w).x : dm.eta_b / dm.eta_a;
+            const Frame fr = make_frame(mk(a1.x, a1.y, a1.z));
+            const f3 wo_d = to_local(fr, mk(a2.x, a2.y, a2.z));
+            Rng rng{ps.key[pid], ps.rng_n[pid]};
+            f3 beta = ld3(ps.beta, pid);
+            int depth = ps.depth[pid];
+            int nflags = 0;
+            if (sc.num_lights > 0) {
+                const int l = rng.next_int(sc.num_lights);
+                const float selpdf = 1.0f / (float)sc.num_lights;
+                const DevLight Lt = sc.lights[l];
+                const LSample ls = light_sample(sc, Lt, P, rng);
+                f3 nee = mk(0.0f);
+                if (ls.pdf > 0.0f && !is_black(ls.li)) {
+                    const f3 wi_l = to_local(fr, ls.wi);
+                    f3 f = adapter_fr(eta, wi_l);
+                    const float spdf = adapter_pdf(wo_d, wi_l);
+                    if (!is_black(f)) {
+                        const f3 vec = ls.p - P;
+                        const float dis = length(vec);
+                        if (!(dis < 0.001f)) {
+                            want_shadow = true;
+                            so = P; sd = normalize(vec); stmax = dis - 0.0001f;
+                            f = f * abs_cos_theta(to_local(fr, ls.wi));
+                            const float w = power_heuristic(ls.pdf, spdf);
+                            nee = f * ls.li * w / ls.pdf;
+                        }
+                    }
+                }
+                // the adapter's BSDF-sampling half
+                f3 wi2;
+                if (cosine_dir(wo_d, rng, &wi2)) {
+                    const f3 fr2 = adapter_fr(eta, wi2) * abs_cos_theta(wi2);
+                    const float sp2 = adapter_pdf(wo_d, wi2);
+                    if (!is_black(fr2) && sp2 > 0) {
+                        const f3 wi_w = to_world(fr, wi2);
+                        const float lp = light_pdf(sc, Lt, P, wi_w);
+                        if (lp != 0.0f) {
+                            want_mis = true;
+                            mo = P; md = wi_w;
+                            st3(ps.mis_f, pid, fr2, power_heuristic(sp2, lp));
+                            ps.mis_li[pid] = make_float4(0.f, 0.f, 0.f, sp2);
+                            ps.sel_light[pid] = l;
+                        }
+                    }
+                }
+                nflags |= kFlagNeePending;
+                st3(ps.nee_beta, pid, beta, selpdf);
+                st3(ps.nee_le, pid, nee);
+            } else {
+                nflags |= kFlagNeePending;
+                st3(ps.nee_beta, pid, beta, 1.0f);
+                st3(ps.nee_le, pid, mk(0.0f));
+            }
+            if (want_shadow) nflags |= kFlagShadowIssued;
+            if (want_mis) nflags |= kFlagMisIssued;
+            // the adapter's sample continues the path from pi
+            bool alive = false;
+            f3 wi;
+            if (cosine_dir(wo_d, rng, &wi)) {
+                const f3 f = adapter_fr(eta, wi);
+                const float pdf = adapter_pdf(wo_d, wi);
+                if (!(pdf == 0 || is_black(f))) {
+                    beta = beta * (f * abs_cos_theta(wi) / pdf);
+                    alive = true;
+                    if (depth > in.min_depth) {
+                        const float qq = nmin(max_elem(beta), in.rr_threshold);
+                        if (rng.next_float() > qq) alive = false;
+                        else beta = beta / qq;
+                    }
+                    if (alive) {
+                        depth++;
+                        const f3 ed = to_world(fr, wi);
+                        ps.ray_o[pid] = make_float4(P.x, P.y, P.z, 0.0001f);
+                        ps.ray_d[pid] = make_float4(ed.x, ed.y, ed.z, kFloatMax);
+                    }
+                }
+            }
+            if (!alive) nflags |= kFlagDone;
+            st3(ps.beta, pid, beta);
+            ps.flags[pid] = nflags;
+            ps.rng_n[pid] = rng.n;
+            ps.depth[pid] = depth;
+            ps.alive[pid] = alive ? 1 : 0;
+        }
+        int slot = wave_append(&counts[kCntShadow], want_shadow);
+        if (want_shadow) {
+            qshadow.pid[slot] = pid;
+            qshadow.org[slot] = make_float4(so.x, so.y, so.z, 0.0001f);
+            qshadow.dir[slot] = make_float4(sd.x, sd.y, sd.z, stmax);
+        }
+        slot = wave_append(&counts[kCntMis], want_mis);
+        if (want_mis) {
+            qmis.pid[slot] = pid;
+            qmis.org[slot] = make_float4(mo.x, mo.y, mo.z, 0.0001f);
+            qmis.dir[slot] = make_float4(md.x, md.y, md.z, kFloatMax);
+        }
+    }
+}
+
+hipError_t launch_sss_probe(const DevScene& sc, PathState ps, int* counts, const float* lut, hipStream_t s) {
+    const dim3 g(2 * persistent_blocks()), b(128);
+    if (sc.octree) hipLaunchKernelGGL(k_sss_probe<2>, g, b, 0, s, sc, ps, counts, lut);
+    else if (sc.num_insts > 0) hipLaunchKernelGGL(k_sss_probe<1>, g, b, 0, s, sc, ps, counts, lut);
+    else hipLaunchKernelGGL(k_sss_probe<0>, g, b, 0, s, sc, ps, counts, lut);
+    return hipGetLastError();
+}
+
+hipError_t launch_sss_shade(const DevScene& sc, RayQueue qshadow, RayQueue qmis, PathState ps, int* counts,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_sss_shade, dim3(persistent_blocks()), dim3(256), 0, s, sc, qshadow, qmis, ps, counts);
+    return hipGetLastError();
+}
+
 __global__ void k_zero_counts(int* c, int n) {
     int i = threadIdx.x;
     if (i < n) c[i] = 0;

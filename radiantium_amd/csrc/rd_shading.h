@@ -393,6 +393,7 @@ struct MatCtx {
     f3 eta, k;      // conductor
     float eta_a, eta_b, kd, ks;
     DisneyP dp;
+    bool sss;       // subsurface: a non-black scatting distance (Subsurface.cs:50-54)
 };
 
 // kM: the material kinds compiled in (the others load nothing)
@@ -408,6 +409,7 @@ __device__ __forceinline__ MatCtx load_material(const DevScene& sc, int id, floa
     c.ks = m.ks;
     c.r = mk(0.0f); c.t = mk(0.0f); c.eta = mk(0.0f); c.k = mk(0.0f);
     c.mf.kind = 0; c.mf.ax = c.mf.ay = 0.001f;
+    c.sss = false;
     switch (m.type) {
     case kMatDiffuse:
         if constexpr (RD_MAT(kMatDiffuse)) c.r = tex_sample(sc, m.tex[0], su, sv);
@@ -440,6 +442,13 @@ __device__ __forceinline__ MatCtx load_material(const DevScene& sc, int id, floa
         break;
     case kMatPerfectMirror:
         if constexpr (RD_MAT(kMatPerfectMirror)) c.r = tex_sample(sc, m.tex[0], su, sv);
+        break;
+    case kMatSubsurface:   // Subsurface.Sample's FresnelSpecularBsdf(R, T, EtaA, EtaB)
+        if constexpr (RD_MAT(kMatSubsurface)) {
+            c.r = tex_sample(sc, m.tex[0], su, sv);
+            c.t = tex_sample(sc, m.tex[1], su, sv);
+            c.sss = !is_black(tex_sample(sc, m.tex[3], su, sv));
+        }
         break;
     case kMatDisney:
         if constexpr (RD_MAT(kMatDisney)) {
@@ -882,10 +891,24 @@ __device__ __forceinline__ BSample disney_sample(const DisneyP& b, f3 wo, Rng& r
         disney_weights(b, &dw, &tw, &sw, &cw);
         float u = rng.next_float();
         if (u < dw) {
-            // scatting_distance > 0 (the BSSRDF branch) is rejected at scene creation
             if (b.thin || b.sdist_black) {
                 ok = cosine_dir(wo, rng, &wi) && ((same_hemisphere(wo, wi) ? abs_cos_theta(wi) / kPi : 0.0f) != 0);
                 type = kReflection | kDiffuse;
+            } else {
+                // the BSSRDF's entry: SpecularTransmissionBtdf(1, 1, Eta).Sample
+                // (SpecularTransmissionBtdf.cs:33-48), pdf times the diffuse
+                // weight, flagged (DisneyBsdf.cs:357-364); no Fr / Pdf call
+                const bool entering = cos_theta(wo) > 0;
+                const float ei = entering ? 1.0f : b.eta, et = entering ? b.eta : 1.0f;
+                const f3 n = entering ? mk(0, 0, 1) : mk(0, 0, -1);
+                if (!refract(wo, n, ei / et, &wi) || cos_theta(wi) == 0.0f) return s;
+                f3 ft = mk(1.0f) * (1.0f - fr_dielectric(cos_theta(wo), 1, b.eta));
+                if (!imp) ft = ft * ((ei * ei) / (et * et));
+                s.wi = wi;
+                s.fr = ft / abs_cos_theta(wi);
+                s.pdf = 1.0f * dw;
+                s.type = kTransmission | kSpecular | kSubsurface;
+                return s;
             }
         } else {
             u -= dw;
@@ -1019,13 +1042,15 @@ __device__ __forceinline__ BSample mat_sample(const MatCtx& c, f3 wo, Rng& rng, 
         }
         return s;
     }
+    case kMatSubsurface:   // Subsurface.Sample (Subsurface.cs:45-55): the glass's sample, flagged
     case kMatPerfectGlass: {
-        if constexpr (RD_MAT(kMatPerfectGlass)) {
+        if constexpr (RD_MAT(kMatPerfectGlass) || RD_MAT(kMatSubsurface)) {
+            const int sss = (c.type == kMatSubsurface && c.sss) ? kSubsurface : 0;
             float f = fr_dielectric(cos_theta(wo), c.eta_a, c.eta_b);
             if (rng.next_float() < f) {
                 if (cos_theta(wo) == 0) return s;
                 f3 wi = mk(-wo.x, -wo.y, wo.z);
-                s.wi = wi; s.fr = f * c.r / abs_cos_theta(wi); s.pdf = f; s.type = kReflection | kSpecular;
+                s.wi = wi; s.fr = f * c.r / abs_cos_theta(wi); s.pdf = f; s.type = kReflection | kSpecular | sss;
                 return s;
             }
             bool entering = cos_theta(wo) > 0;
@@ -1036,7 +1061,7 @@ __device__ __forceinline__ BSample mat_sample(const MatCtx& c, f3 wo, Rng& rng, 
             f3 ft = c.t * (1.0f - f);
             if (!imp) ft = ft * ((ei * ei) / (et * et));
             ft = ft / abs_cos_theta(wi);
-            s.wi = wi; s.fr = ft; s.pdf = 1 - f; s.type = kTransmission | kSpecular;
+            s.wi = wi; s.fr = ft; s.pdf = 1 - f; s.type = kTransmission | kSpecular | sss;
             return s;
         }
         return s;

@@ -24,14 +24,14 @@ namespace rd {
 enum : int { kTri = 0, kSphere = 1, kInst = 2 };
 enum : int {
     kMatDiffuse = 0, kMatRoughMetal = 1, kMatRoughGlass = 2, kMatRoughPlastic = 3,
-    kMatPerfectGlass = 4, kMatPerfectMirror = 5, kMatDisney = 6
+    kMatPerfectGlass = 4, kMatPerfectMirror = 5, kMatDisney = 6, kMatSubsurface = 7
 };
 // Sets of material kinds a shading kernel is compiled for (bit k: kind k).  A
 // scene's set is known on the host, and the kernels compiled for the smallest
 // set that holds it keep only those kinds' code: the same arithmetic for
 // every material they meet, fewer registers.
 constexpr unsigned mat_bit(int k) { return 1u << k; }
-constexpr unsigned kMatsAll = 0x7fu;
+constexpr unsigned kMatsAll = 0xffu;
 constexpr unsigned kMatsDelta = mat_bit(kMatDiffuse) | mat_bit(kMatPerfectGlass) | mat_bit(kMatPerfectMirror);
 constexpr unsigned kMatsMicrofacet = kMatsDelta | mat_bit(kMatRoughMetal) | mat_bit(kMatRoughGlass);
 // the smallest compiled set holding `mask`
@@ -39,7 +39,7 @@ constexpr unsigned mat_set(unsigned mask) {
     return (mask & ~kMatsDelta) == 0 ? kMatsDelta : ((mask & ~kMatsMicrofacet) == 0 ? kMatsMicrofacet : kMatsAll);
 }
 // BxdfType flags (IBxdf.cs:7-15)
-enum : int { kReflection = 1, kTransmission = 2, kDiffuse = 4, kGlossy = 8, kSpecular = 16 };
+enum : int { kReflection = 1, kTransmission = 2, kDiffuse = 4, kGlossy = 8, kSpecular = 16, kSubsurface = 32 };
 
 struct DevTexture {
     int w, h, wrap, filter;

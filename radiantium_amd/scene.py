@@ -136,6 +136,13 @@ class SceneDesc:
             m["eta_b"] = float(p.get("etaB", DEFAULT_ETA_B))
         elif kind == "perfect_mirror":
             t[0] = self._tex(p.get("r"), 1.0)
+        elif kind == "subsurface":   # RendererBuilder.cs:206-215
+            t[0] = self._tex(p.get("r"), 1.0)
+            t[1] = self._tex(p.get("t"), 1.0)
+            t[2] = self._tex(p.get("a"), 1.0)
+            t[3] = self._tex(p.get("scatting_distance"), 0.0)
+            m["eta_a"] = float(p.get("etaA", DEFAULT_ETA_A))
+            m["eta_b"] = float(p.get("etaB", DEFAULT_ETA_B))
         elif kind == "disney":
             names = [("base_color", 0.5), ("metallic", 0.0), ("roughness", 1.0), ("eta", 1.5),
                      ("specular", 1.0), ("specular_tint", 0.0), ("anisotropic", 0.0), ("sheen", 0.0),

@@ -215,6 +215,26 @@ def env_showcase(width=64, height=64, max_depth=8, method="Mis", map_w=64, map_h
     return s
 
 
+def sss_cornell(width=64, height=64, max_depth=8, method="Mis", disney=True, subsurface=True,
+                scatter=(0.05, 0.03, 0.02)):
+    """The Cornell box of cornell_box() with subsurface scattering: the tall
+    box a non-thin Disney material with a scattering distance (its BSSRDF
+    profile A = base colour * (1 - metallic)), and a sphere of the
+    "subsurface" material (FresnelSpecularBsdf at the surface, BSSRDF
+    profile A, scattering distance, eta 1.33)."""
+    s = cornell_box(width, height, max_depth=max_depth, method=method)
+    if disney:
+        skin = s.add_material("disney", base_color=(0.85, 0.55, 0.45), roughness=0.6, eta=1.4,
+                              scatting_distance=scatter)
+        # the tall box (the seventh mesh: twelve triangles) takes the Disney material
+        s.prims[6][:, 2] = skin
+    if subsurface:
+        jade = s.add_material("subsurface", a=(0.4, 0.8, 0.5), scatting_distance=tuple(2 * v for v in scatter),
+                              etaA=1.0, etaB=1.33)
+        s.add_sphere((0.2, 0.42, 0.15), 0.12, material=jade)
+    return s
+
+
 def vol_cornell(width=64, height=64, max_depth=64, sigma_a=0.05, sigma_s=2.0, g=0.5, radius=0.22):
     """Volumetric Cornell box: the Cornell box of cornell_box() with a
     perfect_glass sphere whose inside holds a homogeneous medium (grey
