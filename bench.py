@@ -566,6 +566,54 @@ def octree_leg(ctx, args):
     return oc
 
 
+def bssrdf_leg(ctx, args):
+    """Subsurface scattering through the path integrator's Mis method (the
+    BSSRDF branch of PathTracer.cs:244-259): the Cornell box with a Disney
+    tall box with a scattering distance and a sphere of the "subsurface"
+    material, 512x512, 64 spp, max depth 8.  One timed frame after a warmup,
+    the BSSRDF stage's share of it, and the oracle on the whole frame at 4 spp."""
+    from radiantium_amd import native, scenes
+    from oracle import cpuref
+    W = H = 512
+    spp = 64
+    desc = scenes.sss_cornell(W, H)
+    scene = native.Scene(ctx, desc)
+    d_rgb = ctx.alloc(W * H * 12)
+    scene.render_device(4, d_rgb, seed=99)
+    ctx.synchronize()
+    ctx.reset_kernel_timing()
+    t0 = time.perf_counter()
+    scene.render_device(spp, d_rgb, seed=1)
+    ctx.synchronize()
+    secs = time.perf_counter() - t0
+    st = ctx.render_stats()
+    rays = st["rays_closest"] + st["rays_any"]
+    stages = {}
+    for name in ("trace_closest", "shade", "trace_secondary", "resolve", "sss", "tail"):
+        avg, cnt = ctx.kernel_timing(name)
+        if cnt:
+            stages[name] = round(avg * cnt, 2)
+    out = {"mrays_s": round(rays / secs / 1e6, 1), "ms": round(secs * 1000, 1), "rays": rays,
+           "msamples_s": round(W * H * spp / secs / 1e6, 1), "stage_ms": stages,
+           "workload": f"Cornell box with a Disney subsurface box and a subsurface sphere, {W}x{H}, {spp} spp, "
+                       f"path Mis, max depth 8"}
+    log(f"[bench] bssrdf: {secs * 1000:.1f} ms, {out['mrays_s']} Mrays/s, stages {stages}")
+    if not args.no_cpu_baseline:
+        if not os.path.exists(cpuref.LIB_PATH):
+            cpuref.build()
+        threads = host_cores()
+        _, _, cst, csecs = cpuref.OracleScene(desc).render(4, seed=1, threads=threads, with_sum2=False)
+        crays = cst["rays_closest"] + cst["rays_any"]
+        out["cpu_baseline"] = {"value": round(crays / csecs / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+                               "kind": "port", "sample": f"the whole frame at 4 spp: {crays} BVH queries in "
+                                                         f"{csecs:.2f} s"}
+        out["speedup_vs_cpu"] = round(out["mrays_s"] / out["cpu_baseline"]["value"], 1)
+        log(f"[cpu] bssrdf: {out['cpu_baseline']['value']} Mrays/s")
+    ctx.free(d_rgb)
+    scene.close()
+    return out
+
+
 def copy_calibration():
     """A plain device copy of known size, profiled beside the path kernels so
     FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
@@ -608,6 +656,8 @@ def main():
     ap.add_argument("--no-octree", action="store_true", help="skip the octree-accelerator measurement")
     ap.add_argument("--only-octree", action="store_true", help="only the octree-accelerator measurement")
     ap.add_argument("--oct-spp", type=int, default=4, help="spp of the octree frame")
+    ap.add_argument("--no-bssrdf", action="store_true", help="skip the subsurface-scattering measurement")
+    ap.add_argument("--only-bssrdf", action="store_true", help="only the subsurface-scattering measurement")
     ap.add_argument("--only-instanced", action="store_true",
                     help="run the instanced-workload frame alone (for profiling); prints its record")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -632,6 +682,9 @@ def main():
         return
     if args.only_octree:
         print(json.dumps({"octree": octree_leg(native.Context(0), args)}), flush=True)
+        return
+    if args.only_bssrdf:
+        print(json.dumps({"bssrdf": bssrdf_leg(native.Context(0), args)}), flush=True)
         return
     t0 = time.perf_counter()
     desc = make_scene(args)
@@ -741,6 +794,10 @@ def main():
     if not args.no_octree:
         octree = octree_leg(ctx, args)
 
+    bssrdf = None
+    if not args.no_bssrdf:
+        bssrdf = bssrdf_leg(ctx, args)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(desc, args, value, scene)
@@ -759,6 +816,7 @@ def main():
                    "triangles": ntri, "parallelism": "1 GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "adjoint": adjoint,
         "instanced": instanced, "close_framing": close, "light_paths": lightp, "octree": octree,
+        "bssrdf": bssrdf,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

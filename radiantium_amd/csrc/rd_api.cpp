@@ -2140,7 +2140,17 @@ int render_particles(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d
     const uint64_t total = p->particles > 0 ? (uint64_t)p->particles : 16384;   // "particle_count" default
     const int W = s->width, H = s->height;
     const size_t npix = (size_t)W * H;
-    const size_t wave_cap = (size_t)(p->wave_paths > 0 ? p->wave_paths : (1 << 22));
+    // particles in flight: the caller's wave_paths, else as many as a quarter
+    // of the free device memory holds (about 180 bytes a particle: three ray
+    // queues, hits, the particle state), at most 2^25; fewer waves mean fewer
+    // of the long Russian-roulette tails, each of which runs nearly empty
+    size_t wave_cap = (size_t)p->wave_paths;
+    if (p->wave_paths <= 0) {
+        size_t free_b = 0, total_b = 0;
+        wave_cap = (size_t)1 << 22;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+            wave_cap = std::max(wave_cap, std::min((size_t)1 << 25, free_b / 4 / 192));
+    }
     const size_t P = std::max<size_t>(1, std::min<size_t>(wave_cap, total));
     const size_t qcap = std::max(P, npix);
     hipStream_t st = c->stream;
@@ -2175,28 +2185,43 @@ int render_particles(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d
         int cin = kA, cout = kB;
         HIP_TRY(launch_zero_slots(c->counts, kA, kB, st));
         HIP_TRY(tm.run("lt_emit", [&] { return launch_lt_emit(s->dev, p->seed, base, n, ps, qin, c->counts, cin, st); }));
-        for (int bounce = 0;; bounce++) {
+        // every kernel reads its queue size on the device, so once the queue
+        // is small the bounces are queued kLtGroup at a time with one host
+        // read per group (bounces past the end run on empty queues); a tick
+        // counts the bounces that had particles
+        constexpr int kLtGroup = 8;
+        constexpr int kLtGroupBelow = 1 << 18;
+        HIP_TRY(launch_zero_slots(c->counts, kCntTail, kCntTail, st));
+        for (int bounce = 0;;) {
             HIP_TRY(hipMemcpyAsync(c->host_counts + cin, c->counts + cin, sizeof(int), hipMemcpyDeviceToHost, st));
             HIP_TRY(hipStreamSynchronize(st));
             if (c->host_counts[cin] == 0) break;
-            max_bounce = std::max(max_bounce, bounce + 1);
-            qin.count = qout.count = nullptr;
-            HIP_TRY(tm.run("trace_closest", [&] {
-                return launch_trace_ext(s->dev, nullptr, qin, tps, hits, c->counts, cin, -1, -1, -1, nullptr, nullptr,
-                                        c->trace_schedule[0], st);
-            }));
-            HIP_TRY(launch_zero_slots(c->counts, cout, kS, st));
-            HIP_TRY(tm.run("shade", [&] {
-                return launch_lt_shade(s->dev, s->lt_cam, qin, cin, hits, ps, qout, cout, qs, kS, c->counts, st);
-            }));
-            HIP_TRY(tm.run("trace_any", [&] {
-                return launch_trace_shadow(s->dev, nullptr, qs, tps, c->counts, kS, c->trace_schedule[1], st);
-            }));
-            HIP_TRY(tm.run("splat", [&] { return launch_lt_splat(qs, kS, ps, acc, c->counts, st); }));
-            std::swap(qin, qout);
-            std::swap(cin, cout);
+            const int group = c->host_counts[cin] <= kLtGroupBelow ? kLtGroup : 1;
+            for (int g = 0; g < group; g++, bounce++) {
+                qin.count = qout.count = nullptr;
+                HIP_TRY(launch_tail_tick(c->counts, cin, st));
+                HIP_TRY(tm.run("trace_closest", [&] {
+                    return launch_trace_ext(s->dev, nullptr, qin, tps, hits, c->counts, cin, -1, -1, -1, nullptr,
+                                            nullptr, c->trace_schedule[0], st);
+                }));
+                HIP_TRY(launch_zero_slots(c->counts, cout, kS, st));
+                HIP_TRY(tm.run("shade", [&] {
+                    return launch_lt_shade(s->dev, s->lt_cam, qin, cin, hits, ps, qout, cout, qs, kS, c->counts, st);
+                }));
+                HIP_TRY(tm.run("trace_any", [&] {
+                    return launch_trace_shadow(s->dev, nullptr, qs, tps, c->counts, kS, c->trace_schedule[1], st);
+                }));
+                HIP_TRY(tm.run("splat", [&] { return launch_lt_splat(qs, kS, ps, acc, c->counts, st); }));
+                std::swap(qin, qout);
+                std::swap(cin, cout);
+            }
             if (cancel && *cancel) { rc = fail(RD_ERR_CANCELLED, "render cancelled"); break; }
+            if (bounce > (1 << 20)) { rc = fail(RD_ERR_DEVICE, "particle wave did not end"); break; }
         }
+        HIP_TRY(hipMemcpyAsync(c->host_counts + kCntTail, c->counts + kCntTail, sizeof(int), hipMemcpyDeviceToHost,
+                               st));
+        HIP_TRY(hipStreamSynchronize(st));
+        max_bounce = std::max(max_bounce, c->host_counts[kCntTail]);
         waves++;
         if (progress) *progress = waves;
         if (cancel && *cancel && !rc) rc = fail(RD_ERR_CANCELLED, "render cancelled");
