@@ -302,13 +302,16 @@ rd_ctx* rd_create(int device);
 void rd_destroy(rd_ctx* ctx);
 
 /* ---- scene: replaces RendererBuilder.Build + new Bvh(...) --------------- */
-/* Builds the SAH BVH (Accelerators/Bvh.cs:89-124) on the host and uploads the
- * flattened scene to HBM.  NULL on failure (see rd_last_error). */
+/* Builds the scene's accelerator (the SAH BVH of Accelerators/Bvh.cs:89-124,
+ * or the loose octree of Accelerators/Octree.cs:48-93 when desc->accel says
+ * so) and uploads the flattened scene to HBM.  NULL on failure (see
+ * rd_last_error). */
 rd_scene* rd_scene_create(rd_ctx* ctx, const rd_scene_desc* desc);
 void rd_scene_destroy(rd_scene* scene);
 int rd_scene_get_info(const rd_scene* scene, rd_scene_info* info);
 /* Copies the flattened BVH: nodes as 32-byte LinearBVHNode records
- * (Bvh.cs:18-26) and the BVH-ordered primitive indices. */
+ * (Bvh.cs:18-26) and the BVH-ordered primitive indices (RD_ERR_INVALID for an
+ * octree scene: rd_scene_export_octree). */
 int rd_scene_export_bvh(const rd_scene* scene, void* nodes32, int32_t max_nodes,
                         int32_t* prim_order, int32_t max_prims);
 

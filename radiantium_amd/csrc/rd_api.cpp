@@ -1154,6 +1154,7 @@ int rd_scene_get_info(const rd_scene* s, rd_scene_info* info) {
 
 int rd_scene_export_bvh(const rd_scene* s, void* nodes32, int32_t max_nodes, int32_t* prim_order, int32_t max_prims) {
     if (!s) return fail(RD_ERR_INVALID, "null scene");
+    if (s->dev.octree) return fail(RD_ERR_INVALID, "the scene's accelerator is the octree (rd_scene_export_octree)");
     if (nodes32) {
         if (max_nodes < (int32_t)s->nodes.size()) return fail(RD_ERR_INVALID, "node buffer too small");
         std::memcpy(nodes32, s->nodes.data(), s->nodes.size() * sizeof(NodeRec));
