@@ -2162,10 +2162,12 @@ int render_particles(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d
     RayQueue qb{nullptr, A.get<int>(P), A.get<float4>(P), A.get<float4>(P)};
     RayQueue qs{nullptr, A.get<int>(P), A.get<float4>(P), A.get<float4>(P)};
     float4* hits = A.get<float4>(qcap);
-    ParticleState ps{A.get<float4>(P), A.get<uint64_t>(P), A.get<uint32_t>(P), A.get<float4>(P), A.get<uint8_t>(P),
+    ParticleState ps{{A.get<float4>(P), A.get<float4>(P)}, {A.get<uint32_t>(P), A.get<uint32_t>(P)}, A.get<float4>(P),
+                     A.get<uint8_t>(P),
                      s->dev.num_insts > 0 ? A.get<int>(qcap) : nullptr};
     if (!acc || !qa.pid || !qa.org || !qa.dir || !qb.pid || !qb.org || !qb.dir || !qs.pid || !qs.org || !qs.dir ||
-        !hits || !ps.coeff || !ps.key || !ps.rng_n || !ps.splat || !ps.vis || (s->dev.num_insts > 0 && !ps.hsub))
+        !hits || !ps.st[0] || !ps.st[1] || !ps.rng_n[0] || !ps.rng_n[1] || !ps.splat || !ps.vis ||
+        (s->dev.num_insts > 0 && !ps.hsub))
         return fail(RD_ERR_OOM, "hipMalloc failed for the particle render");
     // the traversal kernels read queue counts, write hits / visibility and
     // instance triangles through a PathState
@@ -2186,6 +2188,7 @@ int render_particles(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d
         int cin = kA, cout = kB;
         HIP_TRY(launch_zero_slots(c->counts, kA, kB, st));
         HIP_TRY(tm.run("lt_emit", [&] { return launch_lt_emit(s->dev, p->seed, base, n, ps, qin, c->counts, cin, st); }));
+        int side = 0;   // the state buffer beside qin
         // every kernel reads its queue size on the device, so once the queue
         // is small the bounces are queued kLtGroup at a time with one host
         // read per group (bounces past the end run on empty queues); a tick
@@ -2201,20 +2204,25 @@ int render_particles(rd_ctx* c, rd_scene* s, const rd_render_params* p, float* d
             for (int g = 0; g < group; g++, bounce++) {
                 qin.count = qout.count = nullptr;
                 HIP_TRY(launch_tail_tick(c->counts, cin, st));
+                // grid-stride traversal (schedule 0): a particle's rays are
+                // cheap, and the work-cursor schedules' atomics cost more than
+                // they balance (2^24 particles in the Cornell box: 130 -> 106 ms)
                 HIP_TRY(tm.run("trace_closest", [&] {
                     return launch_trace_ext(s->dev, nullptr, qin, tps, hits, c->counts, cin, -1, -1, -1, nullptr,
-                                            nullptr, c->trace_schedule[0], st);
+                                            nullptr, 0, st);
                 }));
                 HIP_TRY(launch_zero_slots(c->counts, cout, kS, st));
                 HIP_TRY(tm.run("shade", [&] {
-                    return launch_lt_shade(s->dev, s->lt_cam, qin, cin, hits, ps, qout, cout, qs, kS, c->counts, st);
+                    return launch_lt_shade(s->dev, s->lt_cam, p->seed, base, side, qin, cin, hits, ps, qout, cout, qs,
+                                           kS, c->counts, st);
                 }));
                 HIP_TRY(tm.run("trace_any", [&] {
-                    return launch_trace_shadow(s->dev, nullptr, qs, tps, c->counts, kS, c->trace_schedule[1], st);
+                    return launch_trace_shadow(s->dev, nullptr, qs, tps, c->counts, kS, 0, st);
                 }));
                 HIP_TRY(tm.run("splat", [&] { return launch_lt_splat(qs, kS, ps, acc, c->counts, st); }));
                 std::swap(qin, qout);
                 std::swap(cin, cout);
+                side = 1 - side;
             }
             if (cancel && *cancel) { rc = fail(RD_ERR_CANCELLED, "render cancelled"); break; }
             if (bounce > (1 << 20)) { rc = fail(RD_ERR_DEVICE, "particle wave did not end"); break; }

@@ -23,10 +23,13 @@ struct LtCamera {
 };
 
 // Particle state of one wave (capacity = particles per wave).
+// A live particle's state travels with its extension-queue entry (two
+// buffers, one beside each of the ping-pong queues), so every stage reads
+// and writes it in queue order; its random stream's key is
+// particle_key(seed, wave base + the entry's pid), recomputed.
 struct ParticleState {
-    float4* coeff;      // throughput; w = the loop's bounce count (int bits)
-    uint64_t* key;      // particle_key(seed, particle index)
-    uint32_t* rng_n;    // draws taken so far
+    float4* st[2];      // per queue entry: throughput; w = the loop's bounce count (int bits)
+    uint32_t* rng_n[2]; // per queue entry: draws taken so far
     float4* splat;      // per shadow-queue entry: colour; w = image element x * H + y (int bits)
     uint8_t* vis;       // per shadow-queue entry: 1 when the camera sees the vertex
     int* hsub;          // instanced scenes: set triangle of each extension hit
@@ -39,7 +42,9 @@ hipError_t launch_lt_emit(const DevScene& sc, uint32_t seed, uint64_t base, int 
 // One bounce of every particle in qin (closest hits in hits): the camera
 // connection (a shadow ray into qs with its splat) and the importance-mode
 // BSDF sample (the next ray into qout).
-hipError_t launch_lt_shade(const DevScene& sc, const LtCamera& cam, RayQueue qin, int cnt_in, const float4* hits,
+// side: the state buffer beside qin (qout's is the other one).
+hipError_t launch_lt_shade(const DevScene& sc, const LtCamera& cam, uint32_t seed, uint64_t base, int side,
+                           RayQueue qin, int cnt_in, const float4* hits,
                            ParticleState ps, RayQueue qout, int cnt_out, RayQueue qs, int cnt_s, int* counts,
                            hipStream_t s);
 // Adds the splat of every unoccluded shadow-queue entry into acc (float64,

@@ -42,6 +42,7 @@ __device__ __forceinline__ bool loop_head(const DevIntegrator& I, int bounces, f
 
 __global__ __launch_bounds__(256) void k_lt_emit(DevScene sc, uint32_t seed, uint64_t base, int n, ParticleState ps,
                                                  RayQueue q, int* counts, int cnt_q) {
+    __shared__ int sh[256 / 64 + 1];
     if (blockIdx.x == 0 && threadIdx.x < 2) counts[kCntWork + threadIdx.x] = 0;   // traversal work cursors
     for (int b0 = blockIdx.x * 256; b0 < n; b0 += gridDim.x * 256) {
         const int i = b0 + threadIdx.x;
@@ -58,22 +59,23 @@ __global__ __launch_bounds__(256) void k_lt_emit(DevScene sc, uint32_t seed, uin
                 alive = loop_head(sc.integ, 1, coeff, rng);
             }
         }
-        int pos = wave_append(counts + cnt_q, alive);
+        int pos = block_append<256>(counts + cnt_q, alive, sh);
         if (alive) {
             q.pid[pos] = i;
             q.org[pos] = make_float4(e.p.x, e.p.y, e.p.z, 0.0001f);
             q.dir[pos] = make_float4(e.dir.x, e.dir.y, e.dir.z, kFloatMax);
-            ps.coeff[i] = make_float4(coeff.x, coeff.y, coeff.z, __int_as_float(1));
-            ps.key[i] = rng.key;
-            ps.rng_n[i] = rng.n;
+            ps.st[0][pos] = make_float4(coeff.x, coeff.y, coeff.z, __int_as_float(1));
+            ps.rng_n[0][pos] = rng.n;
         }
     }
 }
 
 template <unsigned kM>
-__global__ __launch_bounds__(256) void k_lt_shade(DevScene sc, LtCamera cam, RayQueue qin, int cnt_in,
+__global__ __launch_bounds__(256) void k_lt_shade(DevScene sc, LtCamera cam, uint32_t seed, uint64_t base, int side,
+                                                  RayQueue qin, int cnt_in,
                                                   const float4* __restrict__ hits, ParticleState ps, RayQueue qout,
                                                   int cnt_out, RayQueue qs, int cnt_s, int* counts) {
+    __shared__ int sh[256 / 64 + 1];
     const int n = counts[cnt_in];
     if (blockIdx.x == 0 && threadIdx.x < 2) counts[kCntWork + threadIdx.x] = 0;
     uint64_t nvert = 0;
@@ -93,10 +95,10 @@ __global__ __launch_bounds__(256) void k_lt_shade(DevScene sc, LtCamera cam, Ray
             if (prim >= 0) {
                 const float4 ro = qin.org[k], rd = qin.dir[k];
                 const f3 o = mk(ro.x, ro.y, ro.z), d = mk(rd.x, rd.y, rd.z);
-                const float4 cf = ps.coeff[pid];
+                const float4 cf = ps.st[side][k];
                 coeff = mk(cf.x, cf.y, cf.z);
                 bounces = __float_as_int(cf.w);
-                rng = Rng{ps.key[pid], ps.rng_n[pid]};
+                rng = Rng{particle_key(seed, base + (uint64_t)pid), ps.rng_n[side][k]};
                 const DevPrim pr = sc.prim_info[prim];
                 const int sub = (pr.kind == kInst && ps.hsub) ? ps.hsub[k] : -1;
                 Inct it = make_inct_any(sc, pr, sub, o, d, h.z, h.w, h.x);
@@ -135,20 +137,23 @@ __global__ __launch_bounds__(256) void k_lt_shade(DevScene sc, LtCamera cam, Ray
                 }
             }
         }
-        int sp = wave_append(counts + cnt_s, want_shadow);
+        // one atomic per block and queue: same-address atomics serialise,
+        // and particles of a cheap scene are shaded faster than they can be
+        // handed out one wavefront at a time
+        int sp = block_append<256>(counts + cnt_s, want_shadow, sh);
         if (want_shadow) {
             qs.pid[sp] = sp;
             qs.org[sp] = make_float4(so.x, so.y, so.z, 0.0001f);
             qs.dir[sp] = make_float4(sd.x, sd.y, sd.z, stmax);
             ps.splat[sp] = make_float4(color.x, color.y, color.z, __int_as_float(pixel));
         }
-        int ep = wave_append(counts + cnt_out, want_ext);
+        int ep = block_append<256>(counts + cnt_out, want_ext, sh);
         if (want_ext) {
             qout.pid[ep] = pid;
             qout.org[ep] = make_float4(no.x, no.y, no.z, ntmin);
             qout.dir[ep] = make_float4(nd.x, nd.y, nd.z, kFloatMax);
-            ps.coeff[pid] = make_float4(coeff.x, coeff.y, coeff.z, __int_as_float(bounces));
-            ps.rng_n[pid] = rng.n;
+            ps.st[1 - side][ep] = make_float4(coeff.x, coeff.y, coeff.z, __int_as_float(bounces));
+            ps.rng_n[1 - side][ep] = rng.n;
         }
     }
     add_counter((unsigned long long*)(counts + kCntVertices), nvert);
@@ -234,13 +239,15 @@ hipError_t launch_lt_emit(const DevScene& sc, uint32_t seed, uint64_t base, int 
     return hipGetLastError();
 }
 
-hipError_t launch_lt_shade(const DevScene& sc, const LtCamera& cam, RayQueue qin, int cnt_in, const float4* hits,
+hipError_t launch_lt_shade(const DevScene& sc, const LtCamera& cam, uint32_t seed, uint64_t base, int side,
+                           RayQueue qin, int cnt_in, const float4* hits,
                            ParticleState ps, RayQueue qout, int cnt_out, RayQueue qs, int cnt_s, int* counts,
                            hipStream_t s) {
     const dim3 g(persistent_blocks()), b(256);
     const unsigned set = mat_set(sc.mat_mask);
     auto go = [&](auto kernel) {
-        hipLaunchKernelGGL(kernel, g, b, 0, s, sc, cam, qin, cnt_in, hits, ps, qout, cnt_out, qs, cnt_s, counts);
+        hipLaunchKernelGGL(kernel, g, b, 0, s, sc, cam, seed, base, side, qin, cnt_in, hits, ps, qout, cnt_out, qs,
+                           cnt_s, counts);
     };
     if (set == kMatsDelta) go(k_lt_shade<kMatsDelta>);
     else if (set == kMatsMicrofacet) go(k_lt_shade<kMatsMicrofacet>);

@@ -31,6 +31,33 @@ __device__ __forceinline__ int wave_append(int* counter, bool pred) {
     return pred ? base + __popcll(below) : -1;
 }
 
+// As wave_append, with one atomic per block of kBlock threads (kBlock / 64
+// wavefronts): slots are handed out wave by wave in block order.  Every
+// thread of the block must reach it (block-uniform control flow); `sh` is
+// kBlock / 64 + 1 ints of shared memory, reused only after the call returns.
+template <int kBlock>
+__device__ __forceinline__ int block_append(int* counter, bool pred, int* sh) {
+    constexpr int kWaves = kBlock / 64;
+    const uint64_t mask = __ballot(pred);
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    if (lane == 0) sh[wave] = __popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int total = 0;
+        for (int w = 0; w < kWaves; w++) {
+            const int c = sh[w];
+            sh[w] = total;   // the wave's offset within the block's slots
+            total += c;
+        }
+        sh[kWaves] = total ? atomicAdd(counter, total) : 0;
+    }
+    __syncthreads();
+    const uint64_t below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+    const int slot = sh[kWaves] + sh[wave] + __popcll(below);
+    __syncthreads();
+    return pred ? slot : -1;
+}
+
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;
