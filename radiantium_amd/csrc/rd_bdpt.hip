@@ -85,6 +85,7 @@ __device__ __forceinline__ void sample_pixel(const BdptPass& w, int i, int* x, i
 
 __global__ __launch_bounds__(256) void k_bd_cam_start(DevScene sc, BdptPass w, BdptBuffers b, RayQueue q, int* counts,
                                                       int cnt_q) {
+    __shared__ int sh[256 / 64 + 1];   // block_append
     if (blockIdx.x == 0 && threadIdx.x == 0) counts[kCntWork + 0] = 0;
     for (int b0 = blockIdx.x * 256; b0 < b.P; b0 += gridDim.x * 256) {
         const int i = b0 + threadIdx.x;
@@ -108,7 +109,7 @@ __global__ __launch_bounds__(256) void k_bd_cam_start(DevScene sc, BdptPass w, B
             go = b.walk_cap != 0;
             if (!go) b.nverts[i] = 1;
         }
-        int pos = wave_append(counts + cnt_q, go);
+        int pos = block_append<256>(counts + cnt_q, go, sh);
         if (go) {
             q.pid[pos] = i;
             q.org[pos] = make_float4(o.x, o.y, o.z, tmin);
@@ -119,6 +120,7 @@ __global__ __launch_bounds__(256) void k_bd_cam_start(DevScene sc, BdptPass w, B
 
 __global__ __launch_bounds__(256) void k_bd_light_start(DevScene sc, BdptPass w, BdptBuffers b, RayQueue q,
                                                         int* counts, int cnt_q) {
+    __shared__ int sh[256 / 64 + 1];   // block_append
     for (int b0 = blockIdx.x * 256; b0 < b.P; b0 += gridDim.x * 256) {
         const int i = b0 + threadIdx.x;
         bool go = false;
@@ -154,7 +156,7 @@ __global__ __launch_bounds__(256) void k_bd_light_start(DevScene sc, BdptPass w,
             }
             if (!go) b.nverts[b.P + i] = n;
         }
-        int pos = wave_append(counts + cnt_q, go);
+        int pos = block_append<256>(counts + cnt_q, go, sh);
         if (go) {
             q.pid[pos] = b.P + i;
             q.org[pos] = make_float4(e.p.x, e.p.y, e.p.z, 0.0001f);
@@ -168,6 +170,7 @@ template <unsigned kM>
 __global__ __launch_bounds__(256) void k_bd_walk(DevScene sc, BdptBuffers b, RayQueue qin, int cnt_in,
                                                  const float4* __restrict__ hits, const int* __restrict__ hsub,
                                                  RayQueue qout, int cnt_out, int* counts) {
+    __shared__ int sh[256 / 64 + 1];   // block_append
     const int n = counts[cnt_in];
     if (blockIdx.x == 0 && threadIdx.x == 0) counts[kCntWork + 0] = 0;
     uint64_t nvert = 0;
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(256) void k_bd_walk(DevScene sc, BdptBuffers b, Ray
                 b.nverts[wid] = bounces;
             }
         }
-        int pos = wave_append(counts + cnt_out, go);
+        int pos = block_append<256>(counts + cnt_out, go, sh);
         if (go) {
             qout.pid[pos] = wid;
             qout.org[pos] = make_float4(no.x, no.y, no.z, ntmin);
@@ -315,6 +318,7 @@ __device__ __forceinline__ f3 vertex_le(const DevScene& sc, const BVert& v, f3 p
 template <unsigned kM>
 __global__ __launch_bounds__(256) void k_bd_connect(DevScene sc, LtCamera cam, BdptPass w, BdptBuffers b, RayQueue qc,
                                                     int cnt_c, int* counts) {
+    __shared__ int sh[256 / 64 + 1];   // block_append
     if (blockIdx.x == 0 && threadIdx.x == 0) counts[kCntWork + 0] = 0;
     const int V = b.vmax;
     const long long total = (long long)b.P * b.nstrat;
@@ -394,7 +398,7 @@ __global__ __launch_bounds__(256) void k_bd_connect(DevScene sc, LtCamera cam, B
             }
             b.conn[idx] = make_float4(l.x, l.y, l.z, g);
         }
-        int pos = wave_append(counts + cnt_c, chain);
+        int pos = block_append<256>(counts + cnt_c, chain, sh);
         if (chain) {
             const f3 vec = to - from;
             const float dis = length(vec);
@@ -413,6 +417,7 @@ __global__ __launch_bounds__(256) void k_bd_connect(DevScene sc, LtCamera cam, B
 __global__ __launch_bounds__(256) void k_bd_chain(DevScene sc, BdptBuffers b, RayQueue qin, int cnt_in,
                                                   const float4* __restrict__ hits, const int* __restrict__ hsub,
                                                   RayQueue qout, int cnt_out, int* counts) {
+    __shared__ int sh[256 / 64 + 1];   // block_append
     const int n = counts[cnt_in];
     if (blockIdx.x == 0 && threadIdx.x == 0) counts[kCntWork + 0] = 0;
     for (int b0 = blockIdx.x * 256; b0 < n; b0 += gridDim.x * 256) {
@@ -459,7 +464,7 @@ __global__ __launch_bounds__(256) void k_bd_chain(DevScene sc, BdptBuffers b, Ra
                 b.conn[id] = make_float4(l.x, l.y, l.z, c.w);
             }
         }
-        int pos = wave_append(counts + cnt_out, go);
+        int pos = block_append<256>(counts + cnt_out, go, sh);
         if (go) {
             qout.pid[pos] = id;
             qout.org[pos] = make_float4(no.x, no.y, no.z, 0.0001f);

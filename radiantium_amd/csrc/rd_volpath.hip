@@ -29,6 +29,7 @@ template <bool kBatch, unsigned kM>
 __global__ __launch_bounds__(256, 3) void k_shade_vol(DevScene sc0, const DevScene* __restrict__ scenes, RayQueue qin,
                                                    const float4* __restrict__ hits, RayQueue qshadow, RayQueue qmis,
                                                    PathState ps, int* counts, int cnt_in) {
+    __shared__ int sh[256 / 64 + 1];   // block_append
     int n = counts[cnt_in];
     if (blockIdx.x == 0 && threadIdx.x < 3) counts[kCntWork + threadIdx.x] = 0;
     uint64_t nvert = 0;
@@ -237,13 +238,13 @@ __global__ __launch_bounds__(256, 3) void k_shade_vol(DevScene sc0, const DevSce
                 ps.ray_d[pid] = make_float4(ed.x, ed.y, ed.z, etmax);
             }
         }
-        int slot = wave_append(&counts[kCntShadow], want_shadow);
+        int slot = block_append<256>(&counts[kCntShadow], want_shadow, sh);
         if (want_shadow) {
             qshadow.pid[slot] = pid;
             qshadow.org[slot] = make_float4(so.x, so.y, so.z, 0.0001f);
             qshadow.dir[slot] = make_float4(sd.x, sd.y, sd.z, stmax);
         }
-        slot = wave_append(&counts[kCntMis], want_mis);
+        slot = block_append<256>(&counts[kCntMis], want_mis, sh);
         if (want_mis) {
             qmis.pid[slot] = pid;
             qmis.org[slot] = make_float4(mo.x, mo.y, mo.z, mtmin);
@@ -278,6 +279,7 @@ template <bool kBatch>
 __global__ __launch_bounds__(256) void k_chain_advance(DevScene sc0, const DevScene* __restrict__ scenes, RayQueue qin,
                                                        const float4* __restrict__ hits, RayQueue qout, int mis,
                                                        PathState ps, int* counts, int cnt_in, int cnt_out) {
+    __shared__ int sh[256 / 64 + 1];   // block_append
     int n = counts[cnt_in];
     // the traversal work cursor of this chain restarts at zero for its next step
     if (blockIdx.x == 0 && threadIdx.x == 0) counts[kCntWork + (mis ? 2 : 1)] = 0;
@@ -295,7 +297,7 @@ __global__ __launch_bounds__(256) void k_chain_advance(DevScene sc0, const DevSc
             int hsub = (prim >= 0 && sc.prim_info[prim].kind == kInst) ? (mis ? ps.hsub2[k] : ps.hsub[k]) : -1;
             want = chain_step(sc, mis != 0, pid, qin.org[k], qin.dir[k], h, hsub, ps, &no, &nd, &ntmin, &ntmax);
         }
-        int slot = wave_append(&counts[cnt_out], want);
+        int slot = block_append<256>(&counts[cnt_out], want, sh);
         if (want) {
             qout.pid[slot] = pid;
             qout.org[slot] = make_float4(no.x, no.y, no.z, ntmin);
