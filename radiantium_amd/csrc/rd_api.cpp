@@ -1385,6 +1385,9 @@ hipError_t sss_phase(rd_ctx* c, const PassLaunch& L, WaveBuffers& wb, hipStream_
     return launch_zero_slots(c->counts, kCntSss, kCntSssB, st);
 }
 
+// Scenes with fewer BVH nodes than this (a 128 KB tree) trace unsorted.
+constexpr int kSortMinNodes = 1 << 12;
+
 // One bounce of the device-driven tail: the launches of run_pass's bounce
 // once every path has started (the same stages and streams, the shadow / MIS
 // traversal and resolve beside the next bounce's compaction and extension
@@ -1558,7 +1561,11 @@ int run_pass(rd_ctx* c, Timer& tm, const PassLaunch& L, int* max_bounce, volatil
     for (int bounce = 0; !rc; bounce++) {
         // the queue size is known on the host from the second bounce on
         const int* perm = nullptr;
-        if (L.sort_ok && c->ray_sort && bounce > 0 && c->host_counts[cin] >= c->ray_sort_min) {
+        // the coherence sort pays where traversal is memory-bound; a tree this
+        // small stays in the caches whatever the ray order (the 16-request
+        // vol_path batch's largest request: 610 ms sorted, 571 ms unsorted)
+        if (L.sort_ok && c->ray_sort && bounce > 0 && c->host_counts[cin] >= c->ray_sort_min &&
+            L.sc.num_nodes >= kSortMinNodes) {
             if (int e = ensure_sort_buffers(c, (size_t)L.pool)) return e;
             HIP_TRY(tm.run("sort", [&] {
                 return launch_ray_sort(L.sc.nodes, qin, c->host_counts[cin], c->ray_sort_bits, wb.sort_keys,
