@@ -699,9 +699,17 @@ void k_chain_full(DevScene sc, TraceArgs a) {
             rb = make_float4(nd.x, nd.y, nd.z, ntmax);
         }
     };
-    if (!chunk_wave(n)) return;
-    for (int c = next_chunk(cursor); c < n; c = next_chunk(cursor))
-        if (c + __lane_id() < n) one(c + __lane_id());
+    // a large queue: 256 chains per cursor atomic, walked as four
+    // wavefront-wide rounds (a chain of the small media scenes is a few cheap
+    // queries, and one atomic per 64 chains cost more than the balance it
+    // bought); a queue that would not give every wavefront such a chunk keeps
+    // 64-chain chunks, so more wavefronts share it
+    const int waves = gridDim.x * (kBlock / 64);
+    const int rounds = n >= 256 * waves ? 4 : 1;
+    if (!chunk_wave(n, 64 * rounds)) return;
+    for (int c = next_chunk(cursor, 64 * rounds); c < n; c = next_chunk(cursor, 64 * rounds))
+        for (int k = 0; k < rounds; k++)
+            if (c + 64 * k + __lane_id() < n) one(c + 64 * k + __lane_id());
     trav_counters<kChainMode>(a, nrays, nv, nt);
 }
 
