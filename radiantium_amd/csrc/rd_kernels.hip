@@ -1522,6 +1522,7 @@ __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* 
                                                const float4* __restrict__ hits, RayQueue qshadow, RayQueue qmis,
                                                PathState ps, int* counts, int cnt_in, const int* __restrict__ rank,
                                                bool rank_identity) {
+    __shared__ int sh_app[256 / 64 + 1];   // block_append
     int n = counts[cnt_in];
     // the traversal work cursors restart at zero for the next trace stages
     if (blockIdx.x == 0 && threadIdx.x < 3) counts[kCntWork + threadIdx.x] = 0;
@@ -1732,7 +1733,7 @@ __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* 
                 }
             }
         } else {
-            int slot = wave_append(&counts[kCntShadow], want_shadow);
+            int slot = block_append<256>(&counts[kCntShadow], want_shadow, sh_app);
             if (want_shadow) {
                 qshadow.pid[slot] = pid;
                 qshadow.org[slot] = make_float4(so.x, so.y, so.z, 0.0001f);
@@ -1741,7 +1742,7 @@ __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* 
         }
         // MIS rays (only those that can reach the selected light) are few:
         // appended, so their queue holds no holes
-        int slot = wave_append(&counts[kCntMis], want_mis);
+        int slot = block_append<256>(&counts[kCntMis], want_mis, sh_app);
         if (want_mis) {
             qmis.pid[slot] = pid;
             qmis.org[slot] = make_float4(mo.x, mo.y, mo.z, 0.0001f);
@@ -1749,7 +1750,7 @@ __global__ __launch_bounds__(256, 3) void k_shade(DevScene sc0, const DevScene* 
         }
         if constexpr (RD_MAT(kMatDisney) || RD_MAT(kMatSubsurface)) {
             if (ps.sss_q) {
-                const int ss = wave_append(&counts[kCntSss], want_sss);
+                const int ss = block_append<256>(&counts[kCntSss], want_sss, sh_app);
                 if (want_sss) ps.sss_q[ss] = pid;
             }
         }
