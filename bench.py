@@ -571,7 +571,7 @@ def bssrdf_leg(ctx, args):
     BSSRDF branch of PathTracer.cs:244-259): the Cornell box with a Disney
     tall box with a scattering distance and a sphere of the "subsurface"
     material, 512x512, 64 spp, max depth 8.  One timed frame after a warmup,
-    the BSSRDF stage's share of it, and the oracle on the whole frame at 4 spp."""
+    the BSSRDF stage's share of it, and the oracle on the whole frame at 16 spp."""
     from radiantium_amd import native, scenes
     from oracle import cpuref
     W = H = 512
@@ -602,10 +602,10 @@ def bssrdf_leg(ctx, args):
         if not os.path.exists(cpuref.LIB_PATH):
             cpuref.build()
         threads = host_cores()
-        _, _, cst, csecs = cpuref.OracleScene(desc).render(4, seed=1, threads=threads, with_sum2=False)
+        _, _, cst, csecs = cpuref.OracleScene(desc).render(16, seed=1, threads=threads, with_sum2=False)
         crays = cst["rays_closest"] + cst["rays_any"]
         out["cpu_baseline"] = {"value": round(crays / csecs / 1e6, 3), "unit": "Mrays/s", "cores": threads,
-                               "kind": "port", "sample": f"the whole frame at 4 spp: {crays} BVH queries in "
+                               "kind": "port", "sample": f"the whole frame at 16 spp: {crays} BVH queries in "
                                                          f"{csecs:.2f} s"}
         out["speedup_vs_cpu"] = round(out["mrays_s"] / out["cpu_baseline"]["value"], 1)
         log(f"[cpu] bssrdf: {out['cpu_baseline']['value']} Mrays/s")
