@@ -226,6 +226,7 @@ __device__ __forceinline__ void trav_finish(const DevScene& sc, const TraceArgs&
 struct TravI {
     Trav t;            // current ray (world, or model inside a set), stack, world best
     f3 wo, wd;         // the world ray while inside a set
+    f3 winv;           // and its reciprocal direction
     float wtmax;
     int sub;           // set triangle of the world best (DevPrim index), -1
     int sp_base;       // stack depth at set entry, -1 at scene level
@@ -240,6 +241,7 @@ __device__ __forceinline__ void travi_begin(TravI& s, float4 ra, float4 rb) {
     s.sub = -1; s.sp_base = -1; s.li = s.le = 0; s.inst = s.inst_orig = 0;
     s.bt = s.bu = s.bv = 0.f; s.bord = -1;
     s.wo = s.t.o; s.wd = s.t.d; s.wtmax = s.t.tmax;
+    s.winv = s.t.inv;
 }
 
 __device__ __forceinline__ void set_ray(Trav& t, f3 o, f3 d, float tmax) {
@@ -248,11 +250,20 @@ __device__ __forceinline__ void set_ray(Trav& t, f3 o, f3 d, float tmax) {
     t.nx = t.inv.x < 0; t.ny = t.inv.y < 0; t.nz = t.inv.z < 0;
 }
 
+// The world ray back after a set, with the reciprocal direction kept at
+// entry: the values set_ray would compute again, without its divisions.
+__device__ __forceinline__ void restore_world(TravI& s) {
+    Trav& t = s.t;
+    t.o = s.wo; t.d = s.wd; t.tmax = s.wtmax;
+    t.inv = s.winv;
+    t.nx = t.inv.x < 0; t.ny = t.inv.y < 0; t.nz = t.inv.z < 0;
+}
+
 // InstancedPrimitive.Intersect: Ray3F.Transform (Ray3F.cs:27-34) and the set's root.
 __device__ __forceinline__ void enter_set(const DevScene& sc, TravI& s, int inst, int orig) {
     Trav& t = s.t;
     const DevInstance& I = sc.insts[inst];
-    s.wo = t.o; s.wd = t.d; s.wtmax = t.tmax;
+    s.wo = t.o; s.wd = t.d; s.wtmax = t.tmax; s.winv = t.inv;
     f3 mo = xform_point12(I.w2m, t.o);
     f3 md = normalize(xform_vector12(I.w2m, t.d));
     f3 mx = xform_point12(I.w2m, t.o + t.tmax * t.d);
@@ -280,7 +291,7 @@ __device__ __forceinline__ void leave_set(const DevScene& sc, const float4* __re
             s.wtmax = T;
         }
     }
-    set_ray(t, s.wo, s.wd, s.wtmax);
+    restore_world(s);
     s.sp_base = -1;
 }
 
@@ -507,7 +518,7 @@ __global__ __launch_bounds__(kBlock, inst_waves(kStack, kBlock, 5)) void k_trace
         const float4* prims = s2.prims;
         if (s2.num_nodes > 0)
             while (!trav_step_inst<kAny, kBlock>(s2, nodes, prims, s, stk, nv, nt)) {}
-        if (s.sp_base >= 0) set_ray(s.t, s.wo, s.wd, s.wtmax);   // any-hit ended inside a set
+        if (s.sp_base >= 0) restore_world(s);   // any-hit ended inside a set
         trav_finish<kMode>(s2, a, r, s.t);
         if (!kAny && a.sub) a.sub[r] = s.t.best.prim >= 0 ? s.sub : -1;
         nrays++;
