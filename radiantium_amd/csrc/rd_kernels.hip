@@ -2158,7 +2158,13 @@ __global__ __launch_bounds__(128) void k_sss_probe(DevScene sc, PathState ps, in
                 const f3 d = -pj.Z;
                 const float len = 2 * hl;
                 // the chain: a query, the hit counted when on po's material,
-                // the origin moved 1e-4 past it and MaxT shortened to match
+                // the origin moved 1e-4 past it and MaxT shortened to match.
+                // The counting walk keeps its first two hits, so the select
+                // walk (the same queries again, up to the chosen hit) runs
+                // only for a later one.
+                float4 c_h[2];
+                int c_sub[2];
+                f3 c_org[2];
                 auto walk = [&](int stop, float4* hit, int* hsub, f3* horg, bool count_queries) -> int {
                     f3 o = o0;
                     float tmax = len;
@@ -2173,6 +2179,8 @@ __global__ __launch_bounds__(128) void k_sss_probe(DevScene sc, PathState ps, in
                         if (prim < 0) return found;
                         if (sc.prim_info[prim].material == mat) {
                             if (found == stop) { *hit = h; *hsub = sub; *horg = o; return found + 1; }
+                            if (stop < 0 && found == 0) { c_h[0] = h; c_sub[0] = sub; c_org[0] = o; }
+                            if (stop < 0 && found == 1) { c_h[1] = h; c_sub[1] = sub; c_org[1] = o; }
                             found++;
                         }
                         const float adv = h.x + 0.0001f;
@@ -2188,7 +2196,9 @@ __global__ __launch_bounds__(128) void k_sss_probe(DevScene sc, PathState ps, in
                 const int count = walk(-1, &h, &hsub, &horg, true);
                 if (count > 0) {
                     const int sel = rng.next_int(count);
-                    walk(sel, &h, &hsub, &horg, false);
+                    if (sel == 0) { h = c_h[0]; hsub = c_sub[0]; horg = c_org[0]; }
+                    else if (sel == 1) { h = c_h[1]; hsub = c_sub[1]; horg = c_org[1]; }
+                    else walk(sel, &h, &hsub, &horg, false);
                     const DevPrim pr = sc.prim_info[__float_as_int(h.y)];
                     const Inct pi = make_inct_any(sc, pr, pr.kind == kInst ? hsub : -1, horg, d, h.z, h.w, h.x);
                     // PdfPi
